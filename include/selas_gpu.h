@@ -1,0 +1,173 @@
+/*
+ * selas_gpu.h -- C ABI of libselas_gpu, the MI355X (gfx950) ray-cast and render path for Selas.
+ *
+ * Every entry point returns an int status: 0 on success, a negative SGPU_E* code on failure.
+ * Nothing throws across this boundary. sgpu_last_error() gives the message of the last failure
+ * on the calling host thread.
+ *
+ * Device buffers passed to the query functions are device pointers (allocated with
+ * sgpu_malloc, or by any HIP allocator in the same process). "stream" arguments are hipStream_t
+ * values passed as void*; NULL selects the null stream.
+ *
+ * The entry points replace the interfaces Selas programs against (file:line into the Selas
+ * source tree):
+ *   - the Embree 3.2 scene side: rtcNewDevice / rtcNewScene / rtcNewGeometry(TRIANGLE) /
+ *     rtcSetSharedGeometryBuffer / rtcCommitScene, as used in
+ *     Source/Core/SceneLib/ModelResource.cpp:236-298,447-472 and
+ *     Source/Core/SceneLib/SceneResource.cpp:218-249,286-312;
+ *   - the Embree 3.2 query side: rtcIntersect1/rtcIntersect8 and rtcOccluded1/rtcOccluded8,
+ *     as called by RayPick / OcclusionRay (Source/Applications/Selas/Source/PathTracer.cpp:71-139)
+ *     and the batch tracers (Source/Applications/Selas/Source/DeferredPathTracer.cpp:151-280).
+ */
+#ifndef SELAS_GPU_H
+#define SELAS_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------------------- */
+#define SGPU_OK 0
+#define SGPU_E_INVALID (-22)   /* bad argument (EINVAL) */
+#define SGPU_E_NOMEM (-12)     /* device or host allocation failed (ENOMEM) */
+#define SGPU_E_STATE (-16)     /* object in the wrong state, e.g. query before commit (EBUSY) */
+#define SGPU_E_HIP (-5)        /* a HIP runtime call failed (EIO) */
+#define SGPU_E_RANGE (-34)     /* a size exceeds a documented limit (ERANGE) */
+
+/* Embree's RTC_INVALID_GEOMETRY_ID: primID/geomID/instID of a miss. */
+#define SGPU_INVALID_ID 0xFFFFFFFFu
+
+typedef struct sgpu_context_s* sgpu_context;
+typedef struct sgpu_scene_s* sgpu_scene;
+
+/* Rays in structure-of-arrays form (device pointers, n entries each). Replaces RTCRay /
+ * RTCRay8 (embree3/rtcore_ray.h:26-40): org, dir, tnear, tfar. A hit is accepted when
+ * tnear < t <= tfar. */
+typedef struct sgpu_rays_soa {
+    const float* org_x;
+    const float* org_y;
+    const float* org_z;
+    const float* dir_x;
+    const float* dir_y;
+    const float* dir_z;
+    const float* tnear;
+    const float* tfar;
+} sgpu_rays_soa;
+
+/* Closest-hit results in structure-of-arrays form (device pointers). Replaces RTCHit
+ * (embree3/rtcore_ray.h:43-55). Any pointer may be NULL to skip that output.
+ *   t       hit distance (Embree writes it to ray.tfar); tfar is written on a miss
+ *   u, v    barycentric weights of vertices 1 and 2
+ *   prim_id primitive index within its geometry, SGPU_INVALID_ID on a miss
+ *   geom_id geometry id returned by sgpu_scene_add_triangles, SGPU_INVALID_ID on a miss
+ *   ng_*    unnormalised geometric normal (v1-v0)x(v2-v0)
+ * Passing rays.tfar's storage as hits.t reproduces Embree's in-place tfar update. */
+typedef struct sgpu_hits_soa {
+    float* t;
+    float* u;
+    float* v;
+    uint32_t* prim_id;
+    uint32_t* geom_id;
+    float* ng_x;
+    float* ng_y;
+    float* ng_z;
+} sgpu_hits_soa;
+
+/* Per-query traversal counters (filled by sgpu_intersect_stats). */
+typedef struct sgpu_trace_stats {
+    uint64_t rays;
+    uint64_t nodes_visited;  /* internal nodes whose child boxes were tested */
+    uint64_t tris_tested;    /* triangle tests */
+    uint64_t bytes_node;     /* size of one node record */
+    uint64_t bytes_tri;      /* size of one triangle record */
+} sgpu_trace_stats;
+
+/* ---- errors ----------------------------------------------------------------------------- */
+const char* sgpu_last_error(void);
+
+/* ---- context (replaces rtcNewDevice / rtcReleaseDevice, Source/Applications/Selas/Source/main.cpp:99) */
+int sgpu_context_create(int device, sgpu_context* out);
+int sgpu_context_destroy(sgpu_context ctx);
+
+/* ---- device memory helpers (plain hipMalloc/hipMemcpy wrappers for hosts without HIP headers) */
+int sgpu_malloc(sgpu_context ctx, size_t bytes, void** dptr);
+int sgpu_free(sgpu_context ctx, void* dptr);
+int sgpu_memcpy_htod(void* dst, const void* src, size_t bytes, void* stream);
+int sgpu_memcpy_dtoh(void* dst, const void* src, size_t bytes, void* stream);
+int sgpu_memset(void* dst, int value, size_t bytes, void* stream);
+int sgpu_stream_create(void** stream);
+int sgpu_stream_destroy(void* stream);
+int sgpu_stream_synchronize(void* stream);
+int sgpu_device_synchronize(void);
+
+/* HIP events for timing on a given stream. */
+int sgpu_event_create(void** ev);
+int sgpu_event_destroy(void* ev);
+int sgpu_event_record(void* ev, void* stream);
+int sgpu_event_elapsed_ms(void* start, void* stop, float* ms);
+
+/* ---- scenes --------------------------------------------------------------------------------
+ * sgpu_scene_create replaces rtcNewScene (SceneResource.cpp:228 / ModelResource.cpp:447).
+ * sgpu_scene_add_triangles replaces rtcNewGeometry(RTC_GEOMETRY_TYPE_TRIANGLE) +
+ * rtcSetSharedGeometryBuffer(VERTEX float3, INDEX uint3) + rtcAttachGeometry
+ * (ModelResource.cpp:260-298). Unlike Embree's shared buffers, the host arrays are copied; the
+ * caller may free them on return. An optional column-major 4x4 transform (Embree's
+ * RTC_FORMAT_FLOAT4X4_COLUMN_MAJOR, SubsceneResource.cpp:72) places the mesh in world space;
+ * the transformed vertices are what the BVH is built over (a flattened instance).
+ * sgpu_scene_commit replaces rtcCommitScene (ModelResource.cpp:468): the device builds the BVH. */
+int sgpu_scene_create(sgpu_context ctx, sgpu_scene* out);
+int sgpu_scene_add_triangles(sgpu_scene scene, const float* positions, uint32_t n_verts,
+                             const uint32_t* indices, uint32_t n_tris,
+                             const float* colmajor4x4, uint32_t* geom_id_out);
+int sgpu_scene_commit(sgpu_scene scene, void* stream);
+int sgpu_scene_release(sgpu_scene scene);
+/* Number of triangles, BVH nodes and maximum leaf depth of a committed scene. */
+int sgpu_scene_info(sgpu_scene scene, uint32_t* n_tris, uint32_t* n_nodes, uint32_t* max_depth);
+/* Synchronous health check after queries: SGPU_E_RANGE if any traversal on this scene ran out of
+ * stack (the stack is sized from the tree depth, so this is never expected), else 0. */
+int sgpu_scene_check(sgpu_scene scene);
+
+/* ---- queries --------------------------------------------------------------------------------
+ * sgpu_intersect replaces rtcIntersect1/8 (PathTracer.cpp:95-139, DeferredPathTracer.cpp:151-231):
+ * closest hit of each ray in (tnear, tfar]. Ties in t go to the smaller (geomID, primID).
+ * sgpu_occluded replaces rtcOccluded1/8 (PathTracer.cpp:71-92, DeferredPathTracer.cpp:234-280):
+ * occluded[i] = 1 when any triangle is hit in (tnear, tfar], else 0 (Embree signals the same
+ * by tfar = -inf). */
+int sgpu_intersect(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits,
+                   uint32_t n, void* stream);
+int sgpu_occluded(sgpu_scene scene, const sgpu_rays_soa* rays, uint32_t* occluded, uint32_t n,
+                  void* stream);
+/* Same query as sgpu_intersect with per-ray node/triangle counters, summed into *stats.
+ * Synchronous. Used to state the algorithmic bytes per ray. */
+int sgpu_intersect_stats(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits,
+                         uint32_t n, int any_hit, sgpu_trace_stats* stats);
+
+/* ---- procedural inputs (host side, deterministic from a seed) ------------------------------
+ * Gaussian-cluster triangle soup: n_clusters x tris_per_cluster triangles. Cluster centres are
+ * uniform in [0,1]^3, triangle centroids normal around them with sigma, two edge lengths
+ * log-uniform in [edge_min, edge_max], random orientation. Writes 9 floats per triangle
+ * (v0,v1,v2) to positions_out (host). */
+int sgpu_gen_cluster_soup(uint64_t seed, uint32_t n_clusters, uint32_t tris_per_cluster, float sigma,
+                          float edge_min, float edge_max, float* positions_out);
+/* Primary camera rays of a width x height pinhole camera (Selas InitializeRayCastCamera +
+ * JitteredCameraRay with a 4x4 correlated multi-jitter pattern, Camera.cpp:63-87,89-112),
+ * samples 0..spp-1 of each pixel, pixel-major. Writes 8 floats per ray into the SoA arrays
+ * (host pointers, n = width*height*spp). */
+int sgpu_gen_camera_rays(const float* cam_pos, const float* cam_look_at, const float* cam_up,
+                         float fov_degrees, uint32_t width, uint32_t height, uint32_t spp,
+                         float* ox, float* oy, float* oz, float* dx, float* dy, float* dz,
+                         float* tnear, float* tfar);
+/* Cosine-distributed rays leaving uniformly chosen points of uniformly chosen triangles of a
+ * soup (9 floats per triangle), on a random side, offset like Selas's OffsetRayOrigin. */
+int sgpu_gen_bounce_rays(uint64_t seed, const float* tri_positions, uint32_t n_tris, uint32_t n_rays,
+                         float* ox, float* oy, float* oz, float* dx, float* dy, float* dz,
+                         float* tnear, float* tfar);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SELAS_GPU_H */

@@ -1,0 +1,125 @@
+"""ORACLE -- test infrastructure only.
+
+ctypes binding of oracle/liboracle.so, the CPU restatement of Selas's hot path used as the
+checker by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg. The product
+(selas_amd / libselas_gpu) never imports this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+class _Rays(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("ox", "oy", "oz", "dx", "dy", "dz", "tnear", "tfar")]
+
+
+class _Hits(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("t", "prim", "u", "v")]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    U32 = C.c_uint32
+    lib.orc_closest_brute_f32.argtypes = [P, U32, C.POINTER(_Rays), U32, C.POINTER(_Hits), C.c_int]
+    lib.orc_occluded_brute_f32.argtypes = [P, U32, C.POINTER(_Rays), U32, P, C.c_int]
+    lib.orc_closest_brute_f64.argtypes = [P, U32, C.POINTER(_Rays), U32, P, P, P, P, C.c_int]
+    lib.orc_bvh_build.argtypes = [P, U32]
+    lib.orc_bvh_build.restype = P
+    lib.orc_bvh_free.argtypes = [P]
+    lib.orc_bvh_closest.argtypes = [P, C.POINTER(_Rays), U32, C.POINTER(_Hits), C.c_int]
+    lib.orc_bvh_occluded.argtypes = [P, C.POINTER(_Rays), U32, P, C.c_int]
+    lib.orc_bvh_node_count.argtypes = [P]
+    lib.orc_bvh_node_count.restype = U32
+    lib.orc_tri_test.argtypes = [P, P, P, P, P, C.c_float, P, P, P]
+    lib.orc_tri_test.restype = C.c_int
+    return lib
+
+
+lib = _load()
+
+RAY_KEYS = ("org_x", "org_y", "org_z", "dir_x", "dir_y", "dir_z", "tnear", "tfar")
+
+
+def _rays(r: dict):
+    arrs = [np.ascontiguousarray(r[k], np.float32) for k in RAY_KEYS]
+    return _Rays(*[a.ctypes.data for a in arrs]), arrs, int(arrs[0].shape[0])
+
+
+def _threads(threads):
+    return threads if threads else (os.cpu_count() or 1)
+
+
+def closest_brute_f32(tri9, rays, threads=0):
+    tri = np.ascontiguousarray(tri9, np.float32)
+    rs, keep, n = _rays(rays)
+    out = {"t": np.empty(n, np.float32), "prim": np.empty(n, np.uint32), "u": np.empty(n, np.float32),
+           "v": np.empty(n, np.float32)}
+    h = _Hits(*[out[k].ctypes.data for k in ("t", "prim", "u", "v")])
+    lib.orc_closest_brute_f32(tri.ctypes.data, tri.shape[0], C.byref(rs), n, C.byref(h), _threads(threads))
+    return out
+
+
+def occluded_brute_f32(tri9, rays, threads=0):
+    tri = np.ascontiguousarray(tri9, np.float32)
+    rs, keep, n = _rays(rays)
+    occ = np.empty(n, np.uint32)
+    lib.orc_occluded_brute_f32(tri.ctypes.data, tri.shape[0], C.byref(rs), n, occ.ctypes.data, _threads(threads))
+    return occ
+
+
+def closest_brute_f64(tri9, rays, threads=0):
+    tri = np.ascontiguousarray(tri9, np.float32)
+    rs, keep, n = _rays(rays)
+    prim = np.empty(n, np.uint32)
+    t1, t2, margin = (np.empty(n, np.float64) for _ in range(3))
+    lib.orc_closest_brute_f64(tri.ctypes.data, tri.shape[0], C.byref(rs), n, prim.ctypes.data, t1.ctypes.data,
+                              t2.ctypes.data, margin.ctypes.data, _threads(threads))
+    return {"prim": prim, "t": t1, "t2": t2, "margin": margin}
+
+
+class CpuBvh:
+    """The oracle's own SAH tree over a triangle soup (shares nothing with the device LBVH)."""
+
+    def __init__(self, tri9):
+        self._tri = np.ascontiguousarray(tri9, np.float32)
+        self.handle = lib.orc_bvh_build(self._tri.ctypes.data, self._tri.shape[0])
+
+    def closest(self, rays, threads=0):
+        rs, keep, n = _rays(rays)
+        out = {"t": np.empty(n, np.float32), "prim": np.empty(n, np.uint32), "u": np.empty(n, np.float32),
+               "v": np.empty(n, np.float32)}
+        h = _Hits(*[out[k].ctypes.data for k in ("t", "prim", "u", "v")])
+        lib.orc_bvh_closest(self.handle, C.byref(rs), n, C.byref(h), _threads(threads))
+        return out
+
+    def occluded(self, rays, threads=0):
+        rs, keep, n = _rays(rays)
+        occ = np.empty(n, np.uint32)
+        lib.orc_bvh_occluded(self.handle, C.byref(rs), n, occ.ctypes.data, _threads(threads))
+        return occ
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib.orc_bvh_free(self.handle)
+            self.handle = None
+
+
+def tri_test(v0, e1, e2, o, d, tnear):
+    a = [np.ascontiguousarray(x, np.float32) for x in (v0, e1, e2, o, d)]
+    t, u, v = C.c_float(), C.c_float(), C.c_float()
+    hit = lib.orc_tri_test(*[x.ctypes.data for x in a], C.c_float(tnear), C.byref(t), C.byref(u), C.byref(v))
+    return bool(hit), t.value, u.value, v.value

@@ -1,0 +1,322 @@
+// C ABI of libselas_gpu: contexts, device memory helpers, scenes and ray queries.
+// Declarations and the Selas/Embree interfaces each entry point replaces are in include/selas_gpu.h.
+#include "sgpu_internal.h"
+
+#include <cstring>
+#include <new>
+
+namespace {
+thread_local std::string g_last_error;
+
+hipStream_t as_stream(void* s) { return (hipStream_t)s; }
+
+// Column-major 4x4 (Embree RTC_FORMAT_FLOAT4X4_COLUMN_MAJOR): element (row r, col c) at m[c*4+r].
+// A point p maps to M * (p, 1).
+void xform_point(const float* m, const float* p, float* out)
+{
+    for (int r = 0; r < 3; ++r) out[r] = m[0 * 4 + r] * p[0] + m[1 * 4 + r] * p[1] + m[2 * 4 + r] * p[2] + m[3 * 4 + r];
+}
+}  // namespace
+
+void sgpu_set_error(const std::string& msg) { g_last_error = msg; }
+
+int sgpu_fail(int code, const std::string& msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+extern "C" {
+
+const char* sgpu_last_error(void) { return g_last_error.c_str(); }
+
+int sgpu_context_create(int device, sgpu_context* out)
+{
+    if (!out) return sgpu_fail(SGPU_E_INVALID, "sgpu_context_create: out is NULL");
+    *out = nullptr;
+    int count = 0;
+    SGPU_HIP_TRY(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return sgpu_fail(SGPU_E_INVALID, "sgpu_context_create: no such device");
+    SGPU_HIP_TRY(hipSetDevice(device));
+    sgpu_context ctx = new (std::nothrow) sgpu_context_s();
+    if (!ctx) return sgpu_fail(SGPU_E_NOMEM, "sgpu_context_create: out of host memory");
+    ctx->device = device;
+    hipError_t e = hipGetDeviceProperties(&ctx->props, device);
+    if (e != hipSuccess) {
+        delete ctx;
+        return sgpu_fail(SGPU_E_HIP, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e));
+    }
+    *out = ctx;
+    return SGPU_OK;
+}
+
+int sgpu_context_destroy(sgpu_context ctx)
+{
+    delete ctx;
+    return SGPU_OK;
+}
+
+int sgpu_malloc(sgpu_context ctx, size_t bytes, void** dptr)
+{
+    if (!ctx || !dptr) return sgpu_fail(SGPU_E_INVALID, "sgpu_malloc: NULL argument");
+    *dptr = nullptr;
+    if (bytes == 0) bytes = 1;
+    hipError_t e = hipMalloc(dptr, bytes);
+    if (e != hipSuccess) return sgpu_fail(SGPU_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return SGPU_OK;
+}
+
+int sgpu_free(sgpu_context ctx, void* dptr)
+{
+    (void)ctx;
+    if (dptr) SGPU_HIP_TRY(hipFree(dptr));
+    return SGPU_OK;
+}
+
+int sgpu_memcpy_htod(void* dst, const void* src, size_t bytes, void* stream)
+{
+    if (bytes == 0) return SGPU_OK;
+    if (!dst || !src) return sgpu_fail(SGPU_E_INVALID, "sgpu_memcpy_htod: NULL pointer");
+    SGPU_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, as_stream(stream)));
+    SGPU_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
+    return SGPU_OK;
+}
+
+int sgpu_memcpy_dtoh(void* dst, const void* src, size_t bytes, void* stream)
+{
+    if (bytes == 0) return SGPU_OK;
+    if (!dst || !src) return sgpu_fail(SGPU_E_INVALID, "sgpu_memcpy_dtoh: NULL pointer");
+    SGPU_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, as_stream(stream)));
+    SGPU_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
+    return SGPU_OK;
+}
+
+int sgpu_memset(void* dst, int value, size_t bytes, void* stream)
+{
+    if (bytes == 0) return SGPU_OK;
+    if (!dst) return sgpu_fail(SGPU_E_INVALID, "sgpu_memset: NULL pointer");
+    SGPU_HIP_TRY(hipMemsetAsync(dst, value, bytes, as_stream(stream)));
+    return SGPU_OK;
+}
+
+int sgpu_stream_create(void** stream)
+{
+    if (!stream) return sgpu_fail(SGPU_E_INVALID, "sgpu_stream_create: NULL argument");
+    hipStream_t s;
+    SGPU_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = (void*)s;
+    return SGPU_OK;
+}
+
+int sgpu_stream_destroy(void* stream)
+{
+    if (stream) SGPU_HIP_TRY(hipStreamDestroy(as_stream(stream)));
+    return SGPU_OK;
+}
+
+int sgpu_stream_synchronize(void* stream)
+{
+    SGPU_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
+    return SGPU_OK;
+}
+
+int sgpu_device_synchronize(void)
+{
+    SGPU_HIP_TRY(hipDeviceSynchronize());
+    return SGPU_OK;
+}
+
+int sgpu_event_create(void** ev)
+{
+    if (!ev) return sgpu_fail(SGPU_E_INVALID, "sgpu_event_create: NULL argument");
+    hipEvent_t e;
+    SGPU_HIP_TRY(hipEventCreate(&e));
+    *ev = (void*)e;
+    return SGPU_OK;
+}
+
+int sgpu_event_destroy(void* ev)
+{
+    if (ev) SGPU_HIP_TRY(hipEventDestroy((hipEvent_t)ev));
+    return SGPU_OK;
+}
+
+int sgpu_event_record(void* ev, void* stream)
+{
+    if (!ev) return sgpu_fail(SGPU_E_INVALID, "sgpu_event_record: NULL event");
+    SGPU_HIP_TRY(hipEventRecord((hipEvent_t)ev, as_stream(stream)));
+    return SGPU_OK;
+}
+
+int sgpu_event_elapsed_ms(void* start, void* stop, float* ms)
+{
+    if (!start || !stop || !ms) return sgpu_fail(SGPU_E_INVALID, "sgpu_event_elapsed_ms: NULL argument");
+    SGPU_HIP_TRY(hipEventSynchronize((hipEvent_t)stop));
+    SGPU_HIP_TRY(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return SGPU_OK;
+}
+
+// ---- scenes --------------------------------------------------------------------------------------
+
+int sgpu_scene_create(sgpu_context ctx, sgpu_scene* out)
+{
+    if (!ctx || !out) return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_create: NULL argument");
+    sgpu_scene s = new (std::nothrow) sgpu_scene_s();
+    if (!s) return sgpu_fail(SGPU_E_NOMEM, "sgpu_scene_create: out of host memory");
+    s->ctx = ctx;
+    *out = s;
+    return SGPU_OK;
+}
+
+int sgpu_scene_add_triangles(sgpu_scene scene, const float* positions, uint32_t n_verts, const uint32_t* indices,
+                             uint32_t n_tris, const float* colmajor4x4, uint32_t* geom_id_out)
+{
+    if (!scene) return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_add_triangles: NULL scene");
+    if (n_tris > 0 && (!positions || !indices))
+        return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_add_triangles: NULL positions or indices");
+    std::lock_guard<std::mutex> lock(scene->mu);
+    if (scene->committed) return sgpu_fail(SGPU_E_STATE, "sgpu_scene_add_triangles: scene already committed");
+    const uint64_t total = (uint64_t)scene->tri_geom.size() + n_tris;
+    if (total > (1ull << 30)) return sgpu_fail(SGPU_E_RANGE, "sgpu_scene_add_triangles: more than 2^30 triangles");
+    for (uint64_t t = 0; t < (uint64_t)n_tris * 3; ++t)
+        if (indices[t] >= n_verts)
+            return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_add_triangles: index out of range");
+    const uint32_t geom = (uint32_t)scene->geom_first.size();
+    scene->geom_first.push_back((uint32_t)scene->tri_geom.size());
+    const size_t base = scene->tri_pos.size();
+    try {
+        scene->tri_pos.resize(base + (size_t)n_tris * 9);
+        scene->tri_geom.resize(scene->tri_geom.size() + n_tris, geom);
+    } catch (...) {
+        return sgpu_fail(SGPU_E_NOMEM, "sgpu_scene_add_triangles: out of host memory");
+    }
+    float* dst = scene->tri_pos.data() + base;
+    for (uint32_t t = 0; t < n_tris; ++t) {
+        for (int k = 0; k < 3; ++k) {
+            const float* p = positions + (size_t)indices[(size_t)t * 3 + k] * 3;
+            float* o = dst + (size_t)t * 9 + k * 3;
+            if (colmajor4x4) {
+                xform_point(colmajor4x4, p, o);
+            } else {
+                o[0] = p[0];
+                o[1] = p[1];
+                o[2] = p[2];
+            }
+        }
+    }
+    if (geom_id_out) *geom_id_out = geom;
+    return SGPU_OK;
+}
+
+int sgpu_scene_commit(sgpu_scene scene, void* stream)
+{
+    if (!scene) return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_commit: NULL scene");
+    std::lock_guard<std::mutex> lock(scene->mu);
+    if (scene->committed) return sgpu_fail(SGPU_E_STATE, "sgpu_scene_commit: scene already committed");
+    SGPU_HIP_TRY(hipSetDevice(scene->ctx->device));
+    SGPU_HIP_TRY(hipMalloc((void**)&scene->d_overflow, sizeof(uint32_t)));
+    SGPU_HIP_TRY(hipMemset(scene->d_overflow, 0, sizeof(uint32_t)));
+    int rc = sgpu_bvh_build(scene, as_stream(stream));
+    if (rc != SGPU_OK) return rc;
+    scene->committed = true;
+    std::vector<float>().swap(scene->tri_pos);
+    return SGPU_OK;
+}
+
+int sgpu_scene_release(sgpu_scene scene)
+{
+    if (!scene) return SGPU_OK;
+    if (scene->d_nodes) (void)hipFree(scene->d_nodes);
+    if (scene->d_tris) (void)hipFree(scene->d_tris);
+    if (scene->d_overflow) (void)hipFree(scene->d_overflow);
+    delete scene;
+    return SGPU_OK;
+}
+
+int sgpu_scene_info(sgpu_scene scene, uint32_t* n_tris, uint32_t* n_nodes, uint32_t* max_depth)
+{
+    if (!scene) return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_info: NULL scene");
+    if (!scene->committed) return sgpu_fail(SGPU_E_STATE, "sgpu_scene_info: scene not committed");
+    if (n_tris) *n_tris = scene->n_tris;
+    if (n_nodes) *n_nodes = scene->n_nodes;
+    if (max_depth) *max_depth = scene->max_depth;
+    return SGPU_OK;
+}
+
+int sgpu_scene_check(sgpu_scene scene)
+{
+    if (!scene) return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_check: NULL scene");
+    if (!scene->committed) return sgpu_fail(SGPU_E_STATE, "sgpu_scene_check: scene not committed");
+    uint32_t flag = 0;
+    SGPU_HIP_TRY(hipMemcpy(&flag, scene->d_overflow, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (flag) return sgpu_fail(SGPU_E_RANGE, "traversal stack overflow: results of some query are incomplete");
+    return SGPU_OK;
+}
+
+// ---- queries -------------------------------------------------------------------------------------
+
+static int check_query(sgpu_scene scene, const sgpu_rays_soa* rays, uint32_t n, const char* who)
+{
+    if (!scene || !rays) return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": NULL argument");
+    if (!scene->committed) return sgpu_fail(SGPU_E_STATE, std::string(who) + ": scene not committed");
+    if (n > 0 && (!rays->org_x || !rays->org_y || !rays->org_z || !rays->dir_x || !rays->dir_y || !rays->dir_z ||
+                  !rays->tnear || !rays->tfar))
+        return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": NULL ray array");
+    return SGPU_OK;
+}
+
+int sgpu_intersect(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits, uint32_t n, void* stream)
+{
+    int rc = check_query(scene, rays, n, "sgpu_intersect");
+    if (rc) return rc;
+    if (!hits) return sgpu_fail(SGPU_E_INVALID, "sgpu_intersect: NULL hits");
+    if ((hits->ng_x || hits->ng_y || hits->ng_z) && !(hits->ng_x && hits->ng_y && hits->ng_z))
+        return sgpu_fail(SGPU_E_INVALID, "sgpu_intersect: ng_x/ng_y/ng_z must be all set or all NULL");
+    return sgpu_trace_launch(scene, rays, hits, nullptr, n, false, nullptr, as_stream(stream));
+}
+
+int sgpu_occluded(sgpu_scene scene, const sgpu_rays_soa* rays, uint32_t* occluded, uint32_t n, void* stream)
+{
+    int rc = check_query(scene, rays, n, "sgpu_occluded");
+    if (rc) return rc;
+    if (n > 0 && !occluded) return sgpu_fail(SGPU_E_INVALID, "sgpu_occluded: NULL output");
+    return sgpu_trace_launch(scene, rays, nullptr, occluded, n, true, nullptr, as_stream(stream));
+}
+
+int sgpu_intersect_stats(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits, uint32_t n,
+                         int any_hit, sgpu_trace_stats* stats)
+{
+    int rc = check_query(scene, rays, n, "sgpu_intersect_stats");
+    if (rc) return rc;
+    if (!stats) return sgpu_fail(SGPU_E_INVALID, "sgpu_intersect_stats: NULL stats");
+    uint64_t* d_stats = nullptr;
+    uint32_t* d_occ = nullptr;
+    SGPU_HIP_TRY(hipMalloc((void**)&d_stats, 2 * sizeof(uint64_t)));
+    SGPU_HIP_TRY(hipMemset(d_stats, 0, 2 * sizeof(uint64_t)));
+    if (any_hit) {
+        hipError_t e = hipMalloc((void**)&d_occ, (size_t)(n ? n : 1) * sizeof(uint32_t));
+        if (e != hipSuccess) {
+            (void)hipFree(d_stats);
+            return sgpu_fail(SGPU_E_NOMEM, "sgpu_intersect_stats: hipMalloc failed");
+        }
+    }
+    sgpu_hits_soa none = {};
+    rc = sgpu_trace_launch(scene, rays, any_hit ? &none : (hits ? hits : &none), d_occ, n, any_hit != 0, d_stats,
+                           nullptr);
+    uint64_t h[2] = {0, 0};
+    if (rc == SGPU_OK) {
+        hipError_t e = hipMemcpy(h, d_stats, sizeof(h), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = sgpu_fail(SGPU_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    }
+    (void)hipFree(d_stats);
+    if (d_occ) (void)hipFree(d_occ);
+    if (rc) return rc;
+    stats->rays = n;
+    stats->nodes_visited = h[0];
+    stats->tris_tested = h[1];
+    stats->bytes_node = sizeof(BvhNode);
+    stats->bytes_tri = sizeof(TriRecord);
+    return SGPU_OK;
+}
+
+}  // extern "C"

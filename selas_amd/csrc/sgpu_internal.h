@@ -1,0 +1,80 @@
+// Internal declarations shared by the translation units of libselas_gpu.
+#pragma once
+
+#include "../../include/selas_gpu.h"
+#include "sgpu_math.h"
+
+#include <hip/hip_runtime.h>
+#include <mutex>
+#include <string>
+#include <vector>
+
+// ---- device data layout of a committed scene ---------------------------------------------------
+//
+// BVH node: a binary node that stores the boxes of both children, laid out per axis so that
+// the two slab tests of one visit use three float4 loads plus one int4 load (64 B):
+//   x = (c0.lo.x, c0.hi.x, c1.lo.x, c1.hi.x), y = (...), z = (...), child = (c0, c1, 0, 0).
+// A child index >= 0 names an internal node; a negative index ~j names the triangle record j
+// (leaves hold one triangle).
+struct BvhNode {
+    float4 x;
+    float4 y;
+    float4 z;
+    int4 child;
+};
+static_assert(sizeof(BvhNode) == 64, "BvhNode must be 64 bytes");
+
+// Triangle record, 48 B, in BVH leaf order:
+//   a = (v0.x, v0.y, v0.z, bits(global triangle index)),
+//   b = (e1.x, e1.y, e1.z, bits(geomID)),    e1 = v1 - v0
+//   c = (e2.x, e2.y, e2.z, bits(primID)),    e2 = v2 - v0
+// The global triangle index orders triangles by (geomID, primID); it is the tie-break key.
+struct TriRecord {
+    float4 a;
+    float4 b;
+    float4 c;
+};
+static_assert(sizeof(TriRecord) == 48, "TriRecord must be 48 bytes");
+
+// Slots of the LDS part of the per-lane traversal stack; deeper entries spill to global memory.
+#define SGPU_STACK_LDS 16
+#define SGPU_TRACE_BLOCK 256
+
+struct sgpu_context_s {
+    int device;
+    hipDeviceProp_t props;
+};
+
+struct sgpu_scene_s {
+    sgpu_context ctx;
+    std::mutex mu;
+    // host copy of the world-space triangles (9 floats each) until commit, in global index order
+    std::vector<float> tri_pos;
+    std::vector<uint32_t> tri_geom;   // geomID per triangle
+    std::vector<uint32_t> geom_first; // first global index of each geometry
+    bool committed = false;
+    uint32_t n_tris = 0;
+    uint32_t n_nodes = 0;
+    uint32_t max_depth = 0;
+    BvhNode* d_nodes = nullptr;
+    TriRecord* d_tris = nullptr;
+    // set to non-zero by a traversal whose stack would exceed the tree depth (never expected)
+    uint32_t* d_overflow = nullptr;
+};
+
+// ---- error plumbing ------------------------------------------------------------------------------
+void sgpu_set_error(const std::string& msg);
+int sgpu_fail(int code, const std::string& msg);
+#define SGPU_HIP_TRY(expr)                                                                       \
+    do {                                                                                         \
+        hipError_t _e = (expr);                                                                  \
+        if (_e != hipSuccess)                                                                    \
+            return sgpu_fail(SGPU_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));   \
+    } while (0)
+
+// ---- internal entry points between translation units --------------------------------------------
+// Builds d_nodes/d_tris for the scene's triangles (sgpu_bvh.hip).
+int sgpu_bvh_build(sgpu_scene scene, hipStream_t stream);
+// Launches the traversal kernels (sgpu_trace.hip).
+int sgpu_trace_launch(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits,
+                      uint32_t* occluded, uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream);

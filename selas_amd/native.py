@@ -1,0 +1,306 @@
+"""ctypes binding of libselas_gpu (the C ABI declared in include/selas_gpu.h).
+
+This module is the Python host side used by tests and benchmarks. It mirrors the names of the
+C entry points one to one; the HIP code lives only in the shared library. Loading fails loudly
+when the library has not been built: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libselas_gpu.so")
+
+SGPU_OK = 0
+SGPU_E_INVALID = -22
+SGPU_E_NOMEM = -12
+SGPU_E_STATE = -16
+SGPU_E_HIP = -5
+SGPU_E_RANGE = -34
+INVALID_ID = 0xFFFFFFFF
+
+
+class SgpuError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libselas_gpu error {code}: {msg}")
+        self.code = code
+
+
+class RaysSoA(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("org_x", "org_y", "org_z", "dir_x", "dir_y", "dir_z", "tnear", "tfar")]
+
+
+class HitsSoA(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("t", "u", "v", "prim_id", "geom_id", "ng_x", "ng_y", "ng_z")]
+
+
+class TraceStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("rays", "nodes_visited", "tris_tested", "bytes_node", "bytes_tri")]
+
+
+_P = C.c_void_p
+_U32 = C.c_uint32
+_U64 = C.c_uint64
+_F = C.c_float
+_FP = C.POINTER(C.c_float)
+_U32P = C.POINTER(C.c_uint32)
+
+# name -> (argtypes); every function returns int
+_SIGNATURES = {
+    "sgpu_context_create": [C.c_int, C.POINTER(_P)],
+    "sgpu_context_destroy": [_P],
+    "sgpu_malloc": [_P, C.c_size_t, C.POINTER(_P)],
+    "sgpu_free": [_P, _P],
+    "sgpu_memcpy_htod": [_P, _P, C.c_size_t, _P],
+    "sgpu_memcpy_dtoh": [_P, _P, C.c_size_t, _P],
+    "sgpu_memset": [_P, C.c_int, C.c_size_t, _P],
+    "sgpu_stream_create": [C.POINTER(_P)],
+    "sgpu_stream_destroy": [_P],
+    "sgpu_stream_synchronize": [_P],
+    "sgpu_device_synchronize": [],
+    "sgpu_event_create": [C.POINTER(_P)],
+    "sgpu_event_destroy": [_P],
+    "sgpu_event_record": [_P, _P],
+    "sgpu_event_elapsed_ms": [_P, _P, _FP],
+    "sgpu_scene_create": [_P, C.POINTER(_P)],
+    "sgpu_scene_add_triangles": [_P, _FP, _U32, _U32P, _U32, _FP, _U32P],
+    "sgpu_scene_commit": [_P, _P],
+    "sgpu_scene_release": [_P],
+    "sgpu_scene_info": [_P, _U32P, _U32P, _U32P],
+    "sgpu_scene_check": [_P],
+    "sgpu_intersect": [_P, C.POINTER(RaysSoA), C.POINTER(HitsSoA), _U32, _P],
+    "sgpu_occluded": [_P, C.POINTER(RaysSoA), _P, _U32, _P],
+    "sgpu_intersect_stats": [_P, C.POINTER(RaysSoA), C.POINTER(HitsSoA), _U32, C.c_int, C.POINTER(TraceStats)],
+    "sgpu_gen_cluster_soup": [_U64, _U32, _U32, _F, _F, _F, _FP],
+    "sgpu_gen_camera_rays": [_FP, _FP, _FP, _F, _U32, _U32, _U32] + [_FP] * 8,
+    "sgpu_gen_bounce_rays": [_U64, _FP, _U32, _U32] + [_FP] * 8,
+}
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `make -C selas_amd/csrc` (or __graft_entry__.build()); "
+            "there is no CPU fallback for the HIP path")
+    lib = C.CDLL(LIB_PATH)
+    lib.sgpu_last_error.restype = C.c_char_p
+    lib.sgpu_last_error.argtypes = []
+    for name, args in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = C.c_int
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def declared_symbols():
+    """Every entry point this binding declares (the C header's full list)."""
+    return ["sgpu_last_error"] + list(_SIGNATURES)
+
+
+def check(rc: int):
+    if rc != SGPU_OK:
+        raise SgpuError(rc, lib.sgpu_last_error().decode(errors="replace"))
+
+
+def fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_FP)
+
+
+def uptr(a: np.ndarray):
+    assert a.dtype == np.uint32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_U32P)
+
+
+# ---- procedural inputs (host only; no GPU needed) ---------------------------------------------
+
+def gen_cluster_soup(seed: int, n_clusters: int, tris_per_cluster: int, sigma=0.02, edge_min=1e-4, edge_max=1e-2):
+    out = np.empty((n_clusters * tris_per_cluster, 9), np.float32)
+    check(lib.sgpu_gen_cluster_soup(seed, n_clusters, tris_per_cluster, sigma, edge_min, edge_max, fptr(out)))
+    return out
+
+
+RAY_FIELDS = ("org_x", "org_y", "org_z", "dir_x", "dir_y", "dir_z", "tnear", "tfar")
+
+
+def _ray_arrays(n):
+    return {k: np.empty(n, np.float32) for k in RAY_FIELDS}
+
+
+def gen_camera_rays(pos, look_at, up, fov_degrees, width, height, spp):
+    n = width * height * spp
+    r = _ray_arrays(n)
+    p = np.asarray(pos, np.float32)
+    la = np.asarray(look_at, np.float32)
+    u = np.asarray(up, np.float32)
+    check(lib.sgpu_gen_camera_rays(fptr(p), fptr(la), fptr(u), fov_degrees, width, height, spp,
+                                   *[fptr(r[k]) for k in RAY_FIELDS]))
+    return r
+
+
+def gen_bounce_rays(seed, tri9: np.ndarray, n_rays):
+    r = _ray_arrays(n_rays)
+    t = np.ascontiguousarray(tri9, np.float32)
+    check(lib.sgpu_gen_bounce_rays(seed, fptr(t), t.shape[0], n_rays, *[fptr(r[k]) for k in RAY_FIELDS]))
+    return r
+
+
+# ---- device objects -------------------------------------------------------------------------------
+
+class DeviceBuffer:
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = nbytes
+        p = _P()
+        check(lib.sgpu_malloc(ctx.handle, nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(lib.sgpu_memcpy_htod(self.ptr, a.ctypes.data, a.nbytes, None))
+
+    def download(self, dtype, count):
+        out = np.empty(count, dtype)
+        assert out.nbytes <= self.nbytes
+        check(lib.sgpu_memcpy_dtoh(out.ctypes.data, self.ptr, out.nbytes, None))
+        return out
+
+    def free(self):
+        if self.ptr:
+            check(lib.sgpu_free(self.ctx.handle, self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        h = _P()
+        check(lib.sgpu_context_create(device, C.byref(h)))
+        self.handle = h.value
+
+    def array(self, a: np.ndarray) -> DeviceBuffer:
+        a = np.ascontiguousarray(a)
+        b = DeviceBuffer(self, max(a.nbytes, 4))
+        b.upload(a)
+        return b
+
+    def empty(self, nbytes: int) -> DeviceBuffer:
+        return DeviceBuffer(self, max(nbytes, 4))
+
+    def close(self):
+        if self.handle:
+            check(lib.sgpu_context_destroy(self.handle))
+            self.handle = None
+
+
+class DeviceRays:
+    """SoA rays resident in device memory."""
+
+    def __init__(self, ctx: Context, rays: dict):
+        self.n = int(rays["org_x"].shape[0])
+        self.bufs = {k: ctx.array(np.ascontiguousarray(rays[k], np.float32)) for k in RAY_FIELDS}
+        self.soa = RaysSoA(*[self.bufs[k].ptr for k in RAY_FIELDS])
+
+    def set_field(self, name, values: np.ndarray):
+        self.bufs[name].upload(np.ascontiguousarray(values, np.float32))
+
+
+class DeviceHits:
+    FIELDS = (("t", np.float32), ("u", np.float32), ("v", np.float32), ("prim_id", np.uint32),
+              ("geom_id", np.uint32), ("ng_x", np.float32), ("ng_y", np.float32), ("ng_z", np.float32))
+
+    def __init__(self, ctx: Context, n: int, with_ng=True):
+        self.n = n
+        self.bufs = {}
+        ptrs = []
+        for name, dt in self.FIELDS:
+            if name.startswith("ng_") and not with_ng:
+                ptrs.append(None)
+                continue
+            b = ctx.empty(4 * n)
+            self.bufs[name] = (b, dt)
+            ptrs.append(b.ptr)
+        self.soa = HitsSoA(*ptrs)
+
+    def to_host(self):
+        return {k: b.download(dt, self.n) for k, (b, dt) in self.bufs.items()}
+
+
+class Scene:
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        h = _P()
+        check(lib.sgpu_scene_create(ctx.handle, C.byref(h)))
+        self.handle = h.value
+
+    def add_triangles(self, positions: np.ndarray, indices: np.ndarray, transform_colmajor=None) -> int:
+        pos = np.ascontiguousarray(positions, np.float32).reshape(-1)
+        idx = np.ascontiguousarray(indices, np.uint32).reshape(-1)
+        gid = C.c_uint32()
+        xf = None if transform_colmajor is None else fptr(np.ascontiguousarray(transform_colmajor, np.float32))
+        check(lib.sgpu_scene_add_triangles(self.handle, fptr(pos), pos.size // 3, uptr(idx), idx.size // 3, xf,
+                                           C.byref(gid)))
+        return gid.value
+
+    def add_soup(self, tri9: np.ndarray) -> int:
+        tri9 = np.ascontiguousarray(tri9, np.float32).reshape(-1, 9)
+        n = tri9.shape[0]
+        return self.add_triangles(tri9.reshape(-1, 3), np.arange(3 * n, dtype=np.uint32))
+
+    def commit(self, stream=None):
+        check(lib.sgpu_scene_commit(self.handle, stream))
+
+    def info(self):
+        a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(lib.sgpu_scene_info(self.handle, C.byref(a), C.byref(b), C.byref(c)))
+        return {"n_tris": a.value, "n_nodes": b.value, "max_depth": c.value}
+
+    def check(self):
+        check(lib.sgpu_scene_check(self.handle))
+
+    def intersect(self, rays: DeviceRays, hits: DeviceHits, stream=None):
+        check(lib.sgpu_intersect(self.handle, C.byref(rays.soa), C.byref(hits.soa), rays.n, stream))
+
+    def occluded(self, rays: DeviceRays, out: DeviceBuffer, stream=None):
+        check(lib.sgpu_occluded(self.handle, C.byref(rays.soa), out.ptr, rays.n, stream))
+
+    def intersect_stats(self, rays: DeviceRays, any_hit=False) -> dict:
+        st = TraceStats()
+        check(lib.sgpu_intersect_stats(self.handle, C.byref(rays.soa), None, rays.n, int(any_hit), C.byref(st)))
+        return {k: getattr(st, k) for k, _ in TraceStats._fields_}
+
+    def release(self):
+        if self.handle:
+            check(lib.sgpu_scene_release(self.handle))
+            self.handle = None
+
+
+def synchronize():
+    check(lib.sgpu_device_synchronize())
+
+
+class Event:
+    def __init__(self):
+        h = _P()
+        check(lib.sgpu_event_create(C.byref(h)))
+        self.handle = h.value
+
+    def record(self, stream=None):
+        check(lib.sgpu_event_record(self.handle, stream))
+
+    def elapsed_ms(self, later: "Event") -> float:
+        ms = C.c_float()
+        check(lib.sgpu_event_elapsed_ms(self.handle, later.handle, C.byref(ms)))
+        return ms.value
