@@ -78,7 +78,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-rays", type=int, default=4194304, help="rays in the CPU baseline sample")
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the triangles (1.0 = the workload)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run sgpu_device_copy of BYTES (HBM counter calibration) and exit")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        from selas_amd import native as N
+        ctx = N.Context(0)
+        a = ctx.empty(args.copy_calibration)
+        b = ctx.empty(args.copy_calibration)
+        for _ in range(3):
+            N.check(N.lib.sgpu_device_copy(b.ptr, a.ptr, args.copy_calibration, None))
+        N.synchronize()
+        print(json.dumps({"copy_calibration_bytes": args.copy_calibration, "launches": 3}))
+        return
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the ray-cast "

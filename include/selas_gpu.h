@@ -103,6 +103,10 @@ int sgpu_stream_destroy(void* stream);
 int sgpu_stream_synchronize(void* stream);
 int sgpu_device_synchronize(void);
 
+/* Device-to-device copy by a plain streaming kernel (16 B per lane, grid-stride). Besides being a
+ * copy, it is the calibration kernel for HBM byte counters: its traffic is exactly 2 x bytes. */
+int sgpu_device_copy(void* dst, const void* src, size_t bytes, void* stream);
+
 /* HIP events for timing on a given stream. */
 int sgpu_event_create(void** ev);
 int sgpu_event_destroy(void* ev);

@@ -10,6 +10,17 @@ thread_local std::string g_last_error;
 
 hipStream_t as_stream(void* s) { return (hipStream_t)s; }
 
+__global__ __launch_bounds__(256) void k_copy16(uint4* __restrict dst, const uint4* __restrict src, size_t n16)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += (size_t)gridDim.x * 256) dst[i] = src[i];
+}
+
+__global__ void k_copy1(uint8_t* __restrict dst, const uint8_t* __restrict src, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 // Column-major 4x4 (Embree RTC_FORMAT_FLOAT4X4_COLUMN_MAJOR): element (row r, col c) at m[c*4+r].
 // A point p maps to M * (p, 1).
 void xform_point(const float* m, const float* p, float* out)
@@ -96,6 +107,26 @@ int sgpu_memset(void* dst, int value, size_t bytes, void* stream)
     if (bytes == 0) return SGPU_OK;
     if (!dst) return sgpu_fail(SGPU_E_INVALID, "sgpu_memset: NULL pointer");
     SGPU_HIP_TRY(hipMemsetAsync(dst, value, bytes, as_stream(stream)));
+    return SGPU_OK;
+}
+
+int sgpu_device_copy(void* dst, const void* src, size_t bytes, void* stream)
+{
+    if (bytes == 0) return SGPU_OK;
+    if (!dst || !src) return sgpu_fail(SGPU_E_INVALID, "sgpu_device_copy: NULL pointer");
+    if ((((uintptr_t)dst | (uintptr_t)src | bytes) & 15) == 0) {
+        size_t n16 = bytes / 16;
+        size_t blocks = (n16 + 255) / 256;
+        if (blocks > 8192) blocks = 8192;
+        hipLaunchKernelGGL(k_copy16, dim3((uint32_t)blocks), dim3(256), 0, as_stream(stream), (uint4*)dst,
+                           (const uint4*)src, n16);
+    } else {
+        size_t blocks = (bytes + 255) / 256;
+        if (blocks > 8192) blocks = 8192;
+        hipLaunchKernelGGL(k_copy1, dim3((uint32_t)blocks), dim3(256), 0, as_stream(stream), (uint8_t*)dst,
+                           (const uint8_t*)src, bytes);
+    }
+    SGPU_HIP_TRY(hipGetLastError());
     return SGPU_OK;
 }
 

@@ -57,6 +57,7 @@ _SIGNATURES = {
     "sgpu_memcpy_htod": [_P, _P, C.c_size_t, _P],
     "sgpu_memcpy_dtoh": [_P, _P, C.c_size_t, _P],
     "sgpu_memset": [_P, C.c_int, C.c_size_t, _P],
+    "sgpu_device_copy": [_P, _P, C.c_size_t, _P],
     "sgpu_stream_create": [C.POINTER(_P)],
     "sgpu_stream_destroy": [_P],
     "sgpu_stream_synchronize": [_P],
