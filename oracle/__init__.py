@@ -123,3 +123,92 @@ def tri_test(v0, e1, e2, o, d, tnear):
     t, u, v = C.c_float(), C.c_float(), C.c_float()
     hit = lib.orc_tri_test(*[x.ctypes.data for x in a], C.c_float(tnear), C.byref(t), C.byref(u), C.byref(v))
     return bool(hit), t.value, u.value, v.value
+
+
+# ---- shading (orc_shade.c) -------------------------------------------------------------------------
+
+def _f(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+lib.orc_bsdf_eval_batch.argtypes = [C.c_void_p] * 4 + [C.c_uint32, C.c_void_p]
+lib.orc_bsdf_sample_batch.argtypes = [C.c_void_p] * 4 + [C.c_uint32, C.c_uint32, C.c_void_p]
+lib.orc_ggx.argtypes = [C.c_void_p] * 3 + [C.c_float] * 4 + [C.c_void_p]
+lib.orc_fresnel_dielectric.argtypes = [C.c_float] * 3
+lib.orc_fresnel_dielectric.restype = C.c_float
+lib.orc_schlick_weight.argtypes = [C.c_float]
+lib.orc_schlick_weight.restype = C.c_float
+lib.orc_schlick_r0.argtypes = [C.c_float]
+lib.orc_schlick_r0.restype = C.c_float
+lib.orc_schlick.argtypes = [C.c_float] * 2
+lib.orc_schlick.restype = C.c_float
+lib.orc_cmj.argtypes = [C.c_int32] * 4 + [C.c_void_p]
+lib.orc_rect_light.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_void_p]
+lib.orc_ray_quad.argtypes = [C.c_void_p]
+lib.orc_ray_quad.restype = C.c_int
+lib.orc_concentric_disc.argtypes = [C.c_float, C.c_float, C.c_void_p]
+lib.orc_make_frame.argtypes = [C.c_void_p] * 3
+
+
+class _Camera(C.Structure):
+    _fields_ = [(k, C.c_float * 3) for k in ("x", "y", "z", "pos", "forward")] + [
+        ("width", C.c_uint32), ("height", C.c_uint32), ("vipd", C.c_float)]
+
+
+lib.orc_camera_init.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(_Camera)]
+lib.orc_camera_cmj_dir.argtypes = [C.POINTER(_Camera)] + [C.c_int32] * 6 + [C.c_void_p]
+
+
+def bsdf_eval(blocks, n, v, l):
+    blocks, n, v, l = _f(blocks), _f(n), _f(v), _f(l)
+    out = np.zeros((blocks.shape[0], 5), np.float32)
+    lib.orc_bsdf_eval_batch(blocks.ctypes.data, n.ctypes.data, v.ctypes.data, l.ctypes.data, blocks.shape[0],
+                            out.ctypes.data)
+    return out
+
+
+def bsdf_sample(blocks, n, v, uniforms):
+    blocks, n, v, uniforms = _f(blocks), _f(n), _f(v), _f(uniforms)
+    out = np.zeros((blocks.shape[0], 14), np.float32)
+    lib.orc_bsdf_sample_batch(blocks.ctypes.data, n.ctypes.data, v.ctypes.data, uniforms.ctypes.data,
+                              uniforms.shape[1], blocks.shape[0], out.ctypes.data)
+    return out
+
+
+def ggx(wo, wi, wm, ax, ay, u1, u2):
+    out = np.zeros(8, np.float32)
+    lib.orc_ggx(_f(wo).ctypes.data, _f(wi).ctypes.data, _f(wm).ctypes.data, ax, ay, u1, u2, out.ctypes.data)
+    return out
+
+
+def cmj(s, m, n, p):
+    out = np.zeros(2, np.float32)
+    lib.orc_cmj(int(s), int(m), int(n), int(p), out.ctypes.data)
+    return out
+
+
+def camera_rays(cam10, width, height, px):
+    c = _Camera()
+    cam = _f(cam10)
+    lib.orc_camera_init(cam.ctypes.data, width, height, C.byref(c))
+    out = np.zeros((px.shape[0], 3), np.float32)
+    for i, q in enumerate(px):
+        lib.orc_camera_cmj_dir(C.byref(c), *[int(x) for x in q], out[i].ctypes.data)
+    basis = np.array(list(c.x) + list(c.y) + list(c.z) + [c.vipd], np.float32)
+    return out, basis
+
+
+def rect_light(q12, u, v):
+    out = np.zeros(5, np.float32)
+    lib.orc_rect_light(_f(q12).ctypes.data, u, v, out.ctypes.data)
+    return out
+
+
+def ray_quad(q18):
+    return lib.orc_ray_quad(_f(q18).ctypes.data)
+
+
+def concentric_disc(r0, r1):
+    out = np.zeros(2, np.float32)
+    lib.orc_concentric_disc(r0, r1, out.ctypes.data)
+    return out

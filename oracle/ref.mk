@@ -1,0 +1,42 @@
+# Compiles Selas's own source files, where they lie under /root/reference, together with
+# ref_harness.cpp into _ref/libselas_ref.so (test infrastructure; outputs only into _ref/).
+#
+# Flags: Selas has no Linux build target; -DIsOsx_=1 selects its POSIX platform macros
+# (Source/Core/SystemLib/BasicTypes.h:19-26). MemoryAllocation.cpp calls posix_memalign without
+# including <stdlib.h>, so that one file is compiled with -include stdlib.h (a system header).
+# Nothing is stubbed: every symbol resolves from these files (linked with --no-undefined).
+REF ?= /root/reference
+CORE := $(REF)/Source/Core
+APP := $(REF)/Source/Applications/Selas/Source
+OUT := _ref
+CXX ?= g++
+CXXFLAGS := -std=c++14 -O2 -fPIC -ffp-contract=off -DIsOsx_=1 -w \
+            -I$(CORE) -I$(APP) -I$(REF)/Middleware -I$(REF)/Middleware/embree-latest/include
+
+SRCS := $(CORE)/Shading/Disney.cpp $(CORE)/Shading/Ggx.cpp $(CORE)/Shading/Fresnel.cpp \
+        $(CORE)/MathLib/FloatFuncs.cpp $(CORE)/MathLib/FloatStructs.cpp $(CORE)/MathLib/Trigonometric.cpp \
+        $(CORE)/MathLib/Sampler.cpp $(CORE)/MathLib/Random.cpp $(CORE)/MathLib/CorrelatedMultiJitter.cpp \
+        $(CORE)/MathLib/GeometryIntersection.cpp $(CORE)/GeometryLib/Camera.cpp \
+        $(CORE)/GeometryLib/RectangulerLightSampler.cpp $(CORE)/GeometryLib/Disc.cpp \
+        $(CORE)/GeometryLib/AxisAlignedBox.cpp $(CORE)/SystemLib/Memory.cpp $(CORE)/SystemLib/JsAssert.cpp \
+        $(CORE)/IoLib/Serializer.cpp $(APP)/VCMHashGrid.cpp
+OBJS := $(patsubst $(REF)/%.cpp,$(OUT)/obj/%.o,$(SRCS)) $(OUT)/obj/MemoryAllocation.o $(OUT)/obj/ref_harness.o
+
+all: $(OUT)/libselas_ref.so
+
+$(OUT)/obj/%.o: $(REF)/%.cpp
+	@mkdir -p $(dir $@)
+	$(CXX) $(CXXFLAGS) -c $< -o $@
+
+$(OUT)/obj/MemoryAllocation.o: $(CORE)/SystemLib/MemoryAllocation.cpp
+	@mkdir -p $(dir $@)
+	$(CXX) $(CXXFLAGS) -include stdlib.h -c $< -o $@
+
+$(OUT)/obj/ref_harness.o: ref_harness.cpp
+	@mkdir -p $(dir $@)
+	$(CXX) $(CXXFLAGS) -c $< -o $@
+
+$(OUT)/libselas_ref.so: $(OBJS)
+	$(CXX) -shared -o $@ $(OBJS) -Wl,--no-undefined
+
+.PHONY: all

@@ -1,0 +1,258 @@
+"""Writes the golden vectors in tests/golden/*.npz from Selas's own compiled C++.
+
+Run in a container where the Selas source tree exists (it is compiled by `make -C oracle ref`
+into oracle/_ref/libselas_ref.so; see oracle/ref.mk and oracle/ref_harness.cpp):
+
+    make -C oracle ref && python tests/golden/make_golden.py
+
+Inputs are drawn from a seeded numpy generator; every output is computed by the reference's own
+functions (EvaluateDisney, SampleDisney, the GGX/Fresnel helpers, CorrelatedMultiJitter,
+InitializeRayCastCamera + JitteredCameraRay, the spherical-rectangle sampler, RayQuad,
+SampleConcentricDisc, BuildHashGrid + SearchHashGrid). The fixtures are data only.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+LIB = os.path.join(ROOT, "oracle", "_ref", "libselas_ref.so")
+MATERIALS = "/root/reference/Content/Materials"
+
+F = C.POINTER(C.c_float)
+
+
+def fp(a):
+    return a.ctypes.data_as(F)
+
+
+def unit(rng, n):
+    v = rng.normal(size=(n, 3))
+    return (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)
+
+
+def material_params(rng, n):
+    """Surface blocks (22 floats, layout in oracle/ref_harness.cpp): random Disney parameters with
+    corner values mixed in, plus the reference's own material files."""
+    p = np.zeros((n, 22), np.float32)
+    p[:, 0:3] = rng.uniform(0.02, 1.0, (n, 3))
+    p[:, 3:6] = rng.uniform(0.1, 1.0, (n, 3))
+
+    def mix(col, lo, hi, corners):
+        vals = rng.uniform(lo, hi, n)
+        pick = rng.random(n)
+        for k, c in enumerate(corners):
+            sel = (pick >= k / (len(corners) + 1)) & (pick < (k + 1) / (len(corners) + 1))
+            vals[sel] = c
+        p[:, col] = vals
+
+    mix(6, 0, 1, [0.0])          # sheen
+    mix(7, 0, 1, [0.0, 1.0])     # sheenTint
+    mix(8, 0, 1, [0.0])          # clearcoat
+    mix(9, 0, 1, [1.0])          # clearcoatGloss
+    mix(10, 0, 1, [0.0, 1.0])    # metallic
+    mix(11, 0, 1, [0.0, 1.0])    # specTrans
+    mix(12, 0, 0.5, [0.0])       # diffTrans (already x 0.5, as CalculateSurfaceParams stores it)
+    mix(13, 0, 1, [0.0])         # flatness
+    mix(14, 0, 1, [0.0])         # anisotropic
+    mix(16, 0, 1, [0.0])         # specularTint
+    mix(17, 0.02, 1, [0.3, 1.0, 0.005])  # roughness (0.005 < 0.01 exercises the Dirac flag)
+    p[:, 18] = rng.uniform(0.05, 2.0, n)           # scatterDistance
+    p[:, 19] = rng.uniform(1.1, 2.2, n)            # ior
+    front = rng.random(n) < 0.7
+    p[:, 15] = np.where(front, 1.0 / p[:, 19], p[:, 19])  # relativeIOR
+    p[:, 20] = (rng.random(n) < 0.5).astype(np.float32)     # shader: 0 thin, 1 solid
+    p[:, 21] = 0.0
+    # the reference's material files (Content/Materials/*.json) as the first rows
+    keys = ["sheen", "sheenTint", "clearcoat", "clearcoatGloss", "metallic", "specTrans", "diffTrans", "flatness",
+            "anisotropic", None, "specularTint", "roughness", None, "ior"]
+    if os.path.isdir(MATERIALS):
+        for i, name in enumerate(sorted(os.listdir(MATERIALS))):
+            m = json.load(open(os.path.join(MATERIALS, name)))
+            if isinstance(m.get("baseColor"), list):
+                p[i, 0:3] = m["baseColor"]
+            for k, key in enumerate(keys):
+                if key and key in m:
+                    p[i, 6 + k] = m[key] * (0.5 if key == "diffTrans" else 1.0)
+            p[i, 15] = 1.0 / p[i, 19]
+            p[i, 20] = 0.0 if m.get("shaderName", "DisneySolid") == "DisneyThin" else 1.0
+    return p
+
+
+def main():
+    lib = C.CDLL(LIB)
+    rng = np.random.default_rng(20261202)
+    out = {}
+
+    # ---- Disney eval -------------------------------------------------------------------------
+    n = 4000
+    params = material_params(rng, n)
+    nrm = unit(rng, n)
+    v = unit(rng, n)
+    l = unit(rng, n)
+    # bias v and l towards the normal's hemisphere so that most cases exercise the lobes
+    flip = (np.sum(v * nrm, 1) < 0) & (rng.random(n) < 0.8)
+    v[flip] *= -1
+    flip = (np.sum(l * nrm, 1) < 0) & (rng.random(n) < 0.6)
+    l[flip] *= -1
+    thin = (params[:, 20] == 0).astype(np.int32)
+    res = np.zeros((n, 5), np.float32)
+    for i in range(n):
+        lib.ref_disney_eval(fp(params[i]), fp(nrm[i]), fp(v[i]), fp(l[i]), C.c_int(int(thin[i])), fp(res[i]))
+    out["disney_eval"] = dict(params=params, n=nrm, v=v, l=l, thin=thin, out=res)
+
+    # ---- Disney sample -----------------------------------------------------------------------
+    n = 4000
+    params = material_params(rng, n)
+    nrm = unit(rng, n)
+    v = unit(rng, n)
+    flip = (np.sum(v * nrm, 1) < 0) & (rng.random(n) < 0.8)
+    v[flip] *= -1
+    thin = (params[:, 20] == 0).astype(np.int32)
+    seeds = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+    uni = np.zeros((n, 8), np.float32)
+    res = np.zeros((n, 14), np.float32)
+    for i in range(n):
+        lib.ref_disney_sample(fp(params[i]), fp(nrm[i]), fp(v[i]), C.c_int(int(thin[i])), C.c_uint32(int(seeds[i])),
+                              fp(uni[i]), fp(res[i]))
+    out["disney_sample"] = dict(params=params, n=nrm, v=v, thin=thin, seeds=seeds, uniforms=uni, out=res)
+
+    # ---- GGX ---------------------------------------------------------------------------------
+    n = 2000
+    wo = unit(rng, n)
+    wo[:, 1] = np.abs(wo[:, 1])
+    wi = unit(rng, n)
+    wm = unit(rng, n)
+    wm[:, 1] = np.abs(wm[:, 1])
+    ax = rng.uniform(0.001, 1.0, n).astype(np.float32)
+    ay = rng.uniform(0.001, 1.0, n).astype(np.float32)
+    u = rng.random((n, 2)).astype(np.float32)
+    res = np.zeros((n, 8), np.float32)
+    for i in range(n):
+        lib.ref_ggx(fp(wo[i]), fp(wi[i]), fp(wm[i]), C.c_float(ax[i]), C.c_float(ay[i]), C.c_float(u[i, 0]),
+                    C.c_float(u[i, 1]), fp(res[i]))
+    out["ggx"] = dict(wo=wo, wi=wi, wm=wm, ax=ax, ay=ay, u=u, out=res)
+
+    # ---- Fresnel -----------------------------------------------------------------------------
+    n = 2000
+    cosi = rng.uniform(-1.05, 1.05, n).astype(np.float32)
+    ni = np.where(rng.random(n) < 0.5, 1.0, rng.uniform(1.0, 2.5, n)).astype(np.float32)
+    nt = rng.uniform(1.0, 2.5, n).astype(np.float32)
+    eta = rng.uniform(0.3, 3.0, n).astype(np.float32)
+    r0 = rng.random(n).astype(np.float32)
+    res = np.zeros((n, 4), np.float32)
+    for i in range(n):
+        lib.ref_fresnel(C.c_float(cosi[i]), C.c_float(ni[i]), C.c_float(nt[i]), C.c_float(eta[i]), C.c_float(r0[i]),
+                        fp(res[i]))
+    out["fresnel"] = dict(cosi=cosi, ni=ni, nt=nt, eta=eta, r0=r0, out=res)
+
+    # ---- CMJ (bit-exact integer hashing) -------------------------------------------------------
+    rows = []
+    for m, nn in [(4, 4), (2, 2), (3, 5), (8, 8), (1, 1), (16, 1)]:
+        for p in list(rng.integers(0, 2**31, 40)) + [0, 1, 2**31 - 1, 1048575]:
+            for s in range(m * nn):
+                rows.append((s, m, nn, int(p)))
+    cmj_in = np.array(rows, np.int32)
+    res = np.zeros((len(rows), 2), np.float32)
+    for i, (s, m, nn, p) in enumerate(rows):
+        lib.ref_cmj(C.c_int32(s), C.c_int32(m), C.c_int32(nn), C.c_int32(p), fp(res[i]))
+    out["cmj"] = dict(args=cmj_in, out=res)
+
+    # ---- camera rays -----------------------------------------------------------------------------
+    cams = np.array([[0.5, 0.5, -1.2, 0.5, 0.5, 0.5, 0, 1, 0, 50.0],
+                     [3.0, 2.0, 5.0, 0.0, 0.5, 0.0, 0, 1, 0, 35.0],
+                     [-1.0, 4.0, 2.0, 0.2, 0.0, -0.3, 0.1, 1, 0, 70.0]], np.float32)
+    sizes = [(1024, 1024), (256, 256), (512, 300)]
+    cam_rows = []
+    for c, (w, h) in zip(cams, sizes):
+        k = 600
+        px = np.stack([rng.integers(0, w, k), rng.integers(0, h, k), rng.integers(0, 16, k), np.full(k, 4),
+                       np.full(k, 4), rng.integers(0, w * h, k)], 1).astype(np.int32)
+        rays = np.zeros((k, 6), np.float32)
+        basis = np.zeros(10, np.float32)
+        lib.ref_camera_rays(fp(c), C.c_uint32(w), C.c_uint32(h), px.ctypes.data_as(C.POINTER(C.c_int32)),
+                            C.c_uint32(k), fp(rays), fp(basis))
+        cam_rows.append((c, (w, h), px, rays, basis))
+    out["camera"] = dict(cams=cams, sizes=np.array(sizes, np.int32), px=np.stack([r[2] for r in cam_rows]),
+                         rays=np.stack([r[3] for r in cam_rows]), basis=np.stack([r[4] for r in cam_rows]))
+
+    # ---- spherical rectangle sampling ------------------------------------------------------------
+    n = 2000
+    q = np.zeros((n, 12), np.float32)
+    q[:, 0:3] = rng.uniform(-1, 1, (n, 3))
+    ex = unit(rng, n) * rng.uniform(0.1, 2.0, (n, 1))
+    tmp = unit(rng, n)
+    ey = np.cross(ex, tmp)
+    ey = ey / np.linalg.norm(ey, axis=1, keepdims=True) * rng.uniform(0.1, 2.0, (n, 1))
+    q[:, 3:6] = ex
+    q[:, 6:9] = ey
+    q[:, 9:12] = rng.uniform(-3, 3, (n, 3))
+    uv = rng.random((n, 2)).astype(np.float32)
+    res = np.zeros((n, 5), np.float32)
+    for i in range(n):
+        lib.ref_rect_light(fp(q[i]), C.c_float(uv[i, 0]), C.c_float(uv[i, 1]), fp(res[i]))
+    out["rect_light"] = dict(q=q, uv=uv, out=res)
+
+    # ---- RayQuad -----------------------------------------------------------------------------------
+    n = 3000
+    qq = np.zeros((n, 18), np.float32)
+    cen = rng.uniform(-1, 1, (n, 3))
+    ex = unit(rng, n) * rng.uniform(0.2, 1.5, (n, 1))
+    ez = np.cross(ex, unit(rng, n))
+    ez = ez / np.linalg.norm(ez, axis=1, keepdims=True) * rng.uniform(0.2, 1.5, (n, 1))
+    qq[:, 6:9] = cen - 0.5 * ex - 0.5 * ez
+    qq[:, 9:12] = cen + 0.5 * ex - 0.5 * ez
+    qq[:, 12:15] = cen - 0.5 * ex + 0.5 * ez
+    qq[:, 15:18] = cen + 0.5 * ex + 0.5 * ez
+    o = rng.uniform(-3, 3, (n, 3))
+    target = cen + (rng.random((n, 1)) * 1.6 - 0.8) * ex + (rng.random((n, 1)) * 1.6 - 0.8) * ez
+    d = target - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    qq[:, 0:3] = o
+    qq[:, 3:6] = d
+    hits = np.array([lib.ref_ray_quad(fp(qq[i])) for i in range(n)], np.int32)
+    out["ray_quad"] = dict(q=qq, hit=hits)
+
+    # ---- concentric disc ----------------------------------------------------------------------------
+    n = 2000
+    r01 = rng.random((n, 2)).astype(np.float32)
+    r01[:4] = [[0.5, 0.5], [0.0, 0.0], [1.0, 0.5], [0.5, 1.0]]
+    res = np.zeros((n, 2), np.float32)
+    for i in range(n):
+        lib.ref_concentric_disc(C.c_float(r01[i, 0]), C.c_float(r01[i, 1]), fp(res[i]))
+    out["disc"] = dict(r=r01, out=res)
+
+    # ---- photon hash grid -----------------------------------------------------------------------------
+    cases = []
+    for (npts, cells, radius) in [(6000, 4096, 0.02), (3000, 1000, 0.05), (500, 64, 0.1)]:
+        centres = rng.random((8, 3))
+        pts = (centres[rng.integers(0, 8, npts)] + rng.normal(0, 0.05, (npts, 3))).astype(np.float32)
+        queries = np.concatenate([pts[rng.integers(0, npts, 600)] + rng.normal(0, radius, (600, 3)),
+                                  rng.uniform(-0.2, 1.2, (200, 3))]).astype(np.float32)
+        maxf = 512
+        found = np.zeros((len(queries), maxf), np.uint32)
+        fc = np.zeros(len(queries), np.uint32)
+        ends = np.zeros(cells, np.uint32)
+        order = np.zeros(npts, np.uint32)
+        box = np.zeros(6, np.float32)
+        lib.ref_hash_grid(fp(pts), C.c_uint32(npts), C.c_uint32(cells), C.c_float(radius), fp(queries),
+                          C.c_uint32(len(queries)), C.c_uint32(maxf), found.ctypes.data_as(C.POINTER(C.c_uint32)),
+                          fc.ctypes.data_as(C.POINTER(C.c_uint32)), ends.ctypes.data_as(C.POINTER(C.c_uint32)),
+                          order.ctypes.data_as(C.POINTER(C.c_uint32)), fp(box))
+        assert fc.max() <= maxf
+        cases.append(dict(points=pts, cells=np.uint32(cells), radius=np.float32(radius), queries=queries,
+                          found=found, found_count=fc, cell_ends=ends, cell_indices=order, box=box))
+    for k, c in enumerate(cases):
+        out[f"hash_grid_{k}"] = c
+
+    for name, d in out.items():
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **d)
+        print("wrote", name)
+
+
+if __name__ == "__main__":
+    main()
