@@ -149,6 +149,28 @@ int sgpu_occluded(sgpu_scene scene, const sgpu_rays_soa* rays, uint32_t* occlude
 int sgpu_intersect_stats(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits,
                          uint32_t n, int any_hit, sgpu_trace_stats* stats);
 
+/* ---- shading ----------------------------------------------------------------------------------
+ * Batched replacements of EvaluateBsdf / SampleBsdfFunction (Source/Core/Shading/SurfaceScattering.cpp:18-60,
+ * dispatching to EvaluateDisney / SampleDisney, Disney.cpp:525,617). All pointers are device pointers.
+ * A surface is 22 floats (the material part of Selas's SurfaceParameters, SurfaceParameters.h:18-49):
+ *   [0..2] baseColor (linear), [3..5] transmittanceColor, [6] sheen, [7] sheenTint, [8] clearcoat,
+ *   [9] clearcoatGloss, [10] metallic, [11] specTrans, [12] diffTrans (already halved, as
+ *   CalculateSurfaceParams stores it), [13] flatness, [14] anisotropic, [15] relativeIOR,
+ *   [16] specularTint, [17] roughness, [18] scatterDistance, [19] ior,
+ *   [20] shader (0 = DisneyThin, 1 = DisneySolid, 2 = Lambert), [21] material flags.
+ * normals: 3 floats per point, the unit shading normal; the tangent frame is built from it as
+ * CalculateSurfaceParams does (MakeOrthogonalCoordinateSystem, y-up tangent space).
+ * v, l: unit world-space directions (v toward the viewer, l toward the light), 3 floats each.
+ * sgpu_bsdf_eval out (5 per point): f*|cos theta_l| rgb, forward pdf, reverse pdf.
+ * sgpu_bsdf_sample: uniforms_per_sample uniforms in [0,1) per point, consumed in the order Selas's
+ * CSampler would deliver them; out (14 per point): success, event flags (1 scatter, 2 transmission,
+ * 4 Dirac), reflectance rgb, wi xyz, forward pdf, reverse pdf, medium (0 vacuum, 1 isotropic),
+ * extinction rgb. */
+int sgpu_bsdf_eval(const float* surfaces, const float* normals, const float* v, const float* l,
+                   uint32_t count, float* out, void* stream);
+int sgpu_bsdf_sample(const float* surfaces, const float* normals, const float* v, const float* uniforms,
+                     uint32_t uniforms_per_sample, uint32_t count, float* out, void* stream);
+
 /* ---- procedural inputs (host side, deterministic from a seed) ------------------------------
  * Gaussian-cluster triangle soup: n_clusters x tris_per_cluster triangles. Cluster centres are
  * uniform in [0,1]^3, triangle centroids normal around them with sigma, two edge lengths

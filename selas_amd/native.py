@@ -75,6 +75,8 @@ _SIGNATURES = {
     "sgpu_intersect": [_P, C.POINTER(RaysSoA), C.POINTER(HitsSoA), _U32, _P],
     "sgpu_occluded": [_P, C.POINTER(RaysSoA), _P, _U32, _P],
     "sgpu_intersect_stats": [_P, C.POINTER(RaysSoA), C.POINTER(HitsSoA), _U32, C.c_int, C.POINTER(TraceStats)],
+    "sgpu_bsdf_eval": [_P, _P, _P, _P, _U32, _P, _P],
+    "sgpu_bsdf_sample": [_P, _P, _P, _P, _U32, _U32, _P, _P],
     "sgpu_gen_cluster_soup": [_U64, _U32, _U32, _F, _F, _F, _FP],
     "sgpu_gen_camera_rays": [_FP, _FP, _FP, _F, _U32, _U32, _U32] + [_FP] * 8,
     "sgpu_gen_bounce_rays": [_U64, _FP, _U32, _U32] + [_FP] * 8,
