@@ -171,6 +171,58 @@ int sgpu_bsdf_eval(const float* surfaces, const float* normals, const float* v, 
 int sgpu_bsdf_sample(const float* surfaces, const float* normals, const float* v, const float* uniforms,
                      uint32_t uniforms_per_sample, uint32_t count, float* out, void* stream);
 
+/* ---- scene shading data ------------------------------------------------------------------------
+ * A light, with the fields of Selas's SceneLight (Source/Core/SceneLib/SceneResource.h:32-40).
+ * type 0: quad light centred at `position`, edges `x` and `z` (full lengths), emitting `radiance`
+ * into the half-space `direction` points to; it is not intersectable geometry, as in Selas.
+ * type 1: point light at `position` with intensity `radiance` (an addition; Selas has none). */
+typedef struct sgpu_light {
+    uint32_t type;
+    float position[3];
+    float direction[3];
+    float x[3];
+    float z[3];
+    float radiance[3];
+} sgpu_light;
+
+/* Materials of a scene (host arrays, copied): 22 floats per material with the surface-block layout
+ * above, holding the material's own values (MaterialResourceData, Source/Core/SceneLib/ModelResource.h:68-88):
+ * the renderer applies CalculateSurfaceParams's transforms (baseColor^2.2, saturated metallic and
+ * specTrans, diffTrans x 0.5, relative IOR per hit); slot [15] is unused and slot [21] holds the
+ * material flags (1 = transparent, 16 = per-face texture). geom_material[g] is the material index
+ * of geometry g (one entry per geometry of the scene). */
+int sgpu_scene_set_materials(sgpu_scene scene, const float* materials, uint32_t n_materials,
+                             const uint32_t* geom_material, uint32_t n_geoms);
+/* Lights of a scene (host array, copied; one light set holding all of them) and the constant
+ * background radiance (SceneResourceData::backgroundIntensity, SceneResource.h:53). */
+int sgpu_scene_set_lights(sgpu_scene scene, const sgpu_light* lights, uint32_t n_lights, const float* background_rgb);
+
+/* ---- integrators -------------------------------------------------------------------------------- */
+typedef struct sgpu_camera {
+    float position[3];
+    float look_at[3];
+    float up[3];
+    float fov_degrees; /* horizontal field of view (Selas CameraSettings::fovDegrees) */
+    uint32_t width;
+    uint32_t height;
+} sgpu_camera;
+
+#define SGPU_PT_NEE_COSINE_ONCE 1u /* drop the extra Dot(normal, l) of Selas's quad-light estimator */
+
+typedef struct sgpu_pt_job {
+    sgpu_camera camera;
+    uint32_t spp;             /* paths per pixel (PathsPerPixel_) */
+    uint32_t max_path_length; /* bounces (MaxBounceCount_); Russian roulette after 8 */
+    uint64_t seed;            /* PCG32 seed; path i uses stream i */
+    uint32_t flags;           /* SGPU_PT_* */
+} sgpu_pt_job;
+
+/* Unidirectional path tracing: replaces PathTracer::GenerateImage / EvaluatePath
+ * (Source/Applications/Selas/Source/PathTracer.cpp:142-348). framebuffer is a device array of
+ * 2 x height x width x 3 floats: layer 0 = first-bounce reflectance, layer 1 = radiance, both
+ * averaged over spp. Synchronous on `stream`. */
+int sgpu_render_pt(sgpu_scene scene, const sgpu_pt_job* job, float* framebuffer, void* stream);
+
 /* ---- procedural inputs (host side, deterministic from a seed) ------------------------------
  * Gaussian-cluster triangle soup: n_clusters x tris_per_cluster triangles. Cluster centres are
  * uniform in [0,1]^3, triangle centroids normal around them with sigma, two edge lengths

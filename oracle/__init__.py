@@ -212,3 +212,66 @@ def concentric_disc(r0, r1):
     out = np.zeros(2, np.float32)
     lib.orc_concentric_disc(r0, r1, out.ctypes.data)
     return out
+
+
+# ---- integrators (orc_pt.c) --------------------------------------------------------------------------
+
+class _Scene(C.Structure):
+    _fields_ = [("bvh", C.c_void_p), ("tri9", C.c_void_p), ("tri_geom", C.c_void_p), ("materials", C.c_void_p),
+                ("geom_material", C.c_void_p), ("n_geoms", C.c_uint32), ("lights", C.c_void_p),
+                ("n_lights", C.c_uint32), ("background", C.c_float * 3)]
+
+
+class _PtJob(C.Structure):
+    _fields_ = [("cam10", C.c_float * 10), ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
+                ("max_path_length", C.c_uint32), ("seed", C.c_uint64), ("flags", C.c_uint32)]
+
+
+lib.orc_render_pt.argtypes = [C.POINTER(_Scene), C.POINTER(_PtJob), C.c_void_p, C.c_int]
+
+
+class RenderScene:
+    """The oracle's view of a workloads.*_scene dict (flattened triangles + the oracle SAH tree)."""
+
+    def __init__(self, sc: dict):
+        self.tri9 = np.ascontiguousarray(np.concatenate(sc["geoms"]), np.float32)
+        self.tri_geom = np.concatenate([np.full(g.shape[0], i, np.uint32) for i, g in enumerate(sc["geoms"])])
+        self.materials = np.ascontiguousarray(sc["materials"], np.float32)
+        self.geom_material = np.ascontiguousarray(sc["geom_material"], np.uint32)
+        self.lights = np.ascontiguousarray(sc["lights"])
+        self.bvh = CpuBvh(self.tri9)
+        self.s = _Scene(self.bvh.handle, self.tri9.ctypes.data, self.tri_geom.ctypes.data, self.materials.ctypes.data,
+                        self.geom_material.ctypes.data, self.geom_material.shape[0],
+                        self.lights.ctypes.data if self.lights.size else None, self.lights.shape[0],
+                        (C.c_float * 3)(*[float(x) for x in sc["background"]]))
+
+
+def render_pt(rs: RenderScene, cam: dict, width, height, spp, max_path_length, seed, flags=0, threads=0):
+    cam10 = list(cam["position"]) + list(cam["look_at"]) + list(cam["up"]) + [cam["fov"]]
+    job = _PtJob((C.c_float * 10)(*cam10), width, height, spp, max_path_length, seed, flags)
+    fb = np.zeros((2, height, width, 3), np.float32)
+    lib.orc_render_pt(C.byref(rs.s), C.byref(job), fb.ctypes.data, _threads(threads))
+    return fb
+
+
+class _Uniforms(C.Structure):
+    _fields_ = [("values", C.c_void_p), ("count", C.c_uint32), ("pos", C.c_uint32), ("pcg_state", C.c_void_p),
+                ("pcg_inc", C.c_uint64)]
+
+
+lib.orc_nee.argtypes = [C.POINTER(_Uniforms), C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+
+
+def nee(lights, position, normal, uniforms, cosine_once=False):
+    """NextEventEstimation restated (orc_nee): one light sample per row of `uniforms` (3 used).
+    Returns (k, 8): direction, radiance, distance, pdfW."""
+    lights = np.ascontiguousarray(lights)
+    pos = _f(position)
+    nrm = _f(normal)
+    u = _f(uniforms)
+    out = np.zeros((u.shape[0], 8), np.float32)
+    for i in range(u.shape[0]):
+        un = _Uniforms(u[i].ctypes.data, u.shape[1], 0, None, 0)
+        lib.orc_nee(C.byref(un), lights.ctypes.data, lights.shape[0], pos.ctypes.data, nrm.ctypes.data,
+                    int(cosine_once), out[i].ctypes.data)
+    return out

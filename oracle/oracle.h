@@ -37,6 +37,12 @@ void orc_bvh_free(orc_bvh* b);
 void orc_bvh_closest(const orc_bvh* b, const orc_rays* rays, uint32_t n, const orc_hits* hits, int threads);
 void orc_bvh_occluded(const orc_bvh* b, const orc_rays* rays, uint32_t n, uint32_t* occluded, int threads);
 uint32_t orc_bvh_node_count(const orc_bvh* b);
+/* single-ray forms for the integrator restatements; stack must hold orc_bvh_stack_size entries */
+uint32_t orc_bvh_stack_size(const orc_bvh* b);
+uint32_t orc_bvh_closest1(const orc_bvh* b, uint32_t* stack, const float* o, const float* d, float tnear, float tfar,
+                          float* t, float* u, float* v);
+int orc_bvh_occluded1(const orc_bvh* b, uint32_t* stack, const float* o, const float* d, float tnear, float tfar);
+void orc_parallel_for(void (*fn)(void* ctx, uint32_t begin, uint32_t end), void* ctx, uint32_t n, int threads);
 
 /* ---- shading (orc_shade.c) ---------------------------------------------------------------------- */
 typedef struct {
@@ -52,11 +58,15 @@ typedef struct {
     uint32_t flags;
 } orc_surface;
 
-/* A stream of uniforms consumed in order (a restatement of CSampler::UniformFloat's role). */
+/* A stream of uniforms consumed in order (the role of CSampler::UniformFloat): an explicit array,
+ * or a PCG32 generator when pcg_state is set (O'Neill 2014, pcg32_random_r; uniform = top 24 bits
+ * x 2^-24). */
 typedef struct {
     const float* values;
     uint32_t count;
     uint32_t pos;
+    uint64_t* pcg_state;
+    uint64_t pcg_inc;
 } orc_uniforms;
 
 typedef struct {
@@ -66,6 +76,8 @@ typedef struct {
 } orc_camera;
 
 float orc_uniform(orc_uniforms* u);
+uint32_t orc_pcg32_next(uint64_t* state, uint64_t inc);
+void orc_pcg32_seed(uint64_t initstate, uint64_t initseq, uint64_t* state, uint64_t* inc);
 void orc_make_frame(const float* n, float* t_out, float* b_out);
 void orc_surface_from_block(const float* block22, const float* n, orc_surface* s);
 float orc_schlick_weight(float u);
@@ -91,6 +103,40 @@ void orc_bsdf_eval_batch(const float* blocks, const float* n, const float* v, co
                          float* out);
 void orc_bsdf_sample_batch(const float* blocks, const float* n, const float* v, const float* uniforms,
                            uint32_t n_uniforms, uint32_t count, float* out);
+
+/* ---- integrators (orc_pt.c, orc_vcm.c) ------------------------------------------------------- */
+typedef struct {
+    uint32_t type; /* 0 quad, 1 point; same layout as sgpu_light */
+    float position[3], direction[3], x[3], z[3], radiance[3];
+} orc_light;
+
+typedef struct {
+    const orc_bvh* bvh;
+    const float* tri9;          /* world-space triangles, 9 floats each (global index order) */
+    const uint32_t* tri_geom;   /* geometry of each triangle */
+    const float* materials;     /* 22 floats per material (raw MaterialResourceData values) */
+    const uint32_t* geom_material;
+    uint32_t n_geoms;
+    const orc_light* lights;
+    uint32_t n_lights;
+    float background[3];
+} orc_scene;
+
+typedef struct {
+    float cam10[10]; /* position, look_at, up, fov_degrees */
+    uint32_t width, height, spp, max_path_length;
+    uint64_t seed;
+    uint32_t flags; /* 1: quad-light estimator without the extra Dot(normal, l) */
+} orc_pt_job;
+
+void orc_surface_from_hit(const float* o, const float* d, float t, const float* ng, const float* mat,
+                          const float* texel, orc_surface* s, float* position, float* error);
+void orc_offset_origin(const orc_surface* s, const float* pos, float error, const float* dir, float bias, float* out,
+                       float* signed_bias);
+void orc_nee(orc_uniforms* u, const orc_light* lights, uint32_t n_lights, const float* position, const float* normal,
+             int cosine_once, float* out8);
+/* fb: 2 x height x width x 3 floats (layer 0 first-bounce reflectance, layer 1 radiance) */
+void orc_render_pt(const orc_scene* sc, const orc_pt_job* job, float* fb, int threads);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,290 @@
+/*
+ * ORACLE -- test infrastructure only (see orc_trace.c for the rules). Compiled with
+ * -ffp-contract=off.
+ *
+ * CPU restatement of Selas's unidirectional path tracer, PathTracer::EvaluatePath and
+ * GenerateImage (Source/Applications/Selas/Source/PathTracer.cpp:95-348), with its helpers:
+ *   RayPick / OcclusionRay        PathTracer.cpp:71-139 (rays through the oracle's SAH tree)
+ *   CalculateSurfaceParams        Source/Core/Shading/SurfaceParameters.cpp:144-247 (flat-shaded
+ *                                 triangles: n = normalize(Ng), no textures)
+ *   OffsetRayOrigin               SurfaceParameters.cpp:403-408
+ *   NextEventEstimation           Source/Core/Shading/AreaLighting.cpp:353-374
+ *   SampleRectangleLightSolidAngle AreaLighting.cpp:113-153
+ *   EvaluateBsdf / SampleBsdfFunction via orc_shade.c
+ * Deviations, all documented: the sampler is a PCG32 stream per path (stream = path index,
+ * initstate = seed ^ (path * 0x9E3779B97F4A7C15)) instead of one mt19937 per thread; samples of a
+ * pixel are summed in sample order; media are vacuum (no config has a participating medium);
+ * point lights are an addition. With flags & 1 the quad-light estimator drops the extra
+ * Dot(normal, l) that Selas multiplies into the light sample (AreaLighting.cpp:140).
+ * Also the timed CPU path of the C1 benchmark ("port" baseline).
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define FLOAT_MAX 3.402823466e+38F
+
+static inline ov3 v3(float x, float y, float z)
+{
+    ov3 r = {x, y, z};
+    return r;
+}
+static inline ov3 add(ov3 a, ov3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline ov3 sub(ov3 a, ov3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline ov3 mul(ov3 a, ov3 b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline ov3 scl(ov3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+static inline ov3 neg(ov3 a) { return v3(-a.x, -a.y, -a.z); }
+static inline float dot(ov3 a, ov3 b) { return (a.x * b.x) + (a.y * b.y) + (a.z * b.z); }
+static inline ov3 cross(ov3 a, ov3 b)
+{
+    return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline ov3 normalize(ov3 a)
+{
+    float inv = 1.f / sqrtf(dot(a, a));
+    return v3(a.x * inv, a.y * inv, a.z * inv);
+}
+static inline float absf(float x) { return (x < 0) ? -x : x; }
+static inline float maxf(float a, float b) { return (a > b) ? a : b; }
+static inline float saturate(float x) { return x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x); }
+static inline ov3 ld3(const float* p) { return v3(p[0], p[1], p[2]); }
+
+/* CalculateSurfaceParams for a flat-shaded triangle hit; `texel` (or NULL) replaces the
+ * material's baseColor before the 2.2 power. */
+void orc_surface_from_hit(const float* o, const float* d, float t, const float* ng, const float* mat,
+                          const float* texel, orc_surface* s, float* position, float* error)
+{
+    ov3 p = v3(o[0] + t * d[0], o[1] + t * d[1], o[2] + t * d[2]);
+    const float kErr = 32.0f * 1.19209e-07f;
+    *error = kErr * maxf(maxf(absf(p.x), absf(p.y)), maxf(absf(p.z), t));
+    position[0] = p.x;
+    position[1] = p.y;
+    position[2] = p.z;
+    ov3 n = normalize(v3(ng[0], ng[1], ng[2]));
+    float nn[3] = {n.x, n.y, n.z};
+    float block[22];
+    memcpy(block, mat, sizeof(block));
+    const float* bc = texel ? texel : mat;
+    block[0] = powf(bc[0], 2.2f);
+    block[1] = powf(bc[1], 2.2f);
+    block[2] = powf(bc[2], 2.2f);
+    block[10] = saturate(mat[10]);
+    block[11] = saturate(mat[11]);
+    block[12] = mat[12] * 0.5f;
+    ov3 view = neg(v3(d[0], d[1], d[2]));
+    uint32_t flags = (uint32_t)mat[21];
+    block[15] = ((flags & 1u) && dot(view, n) < 0.0f) ? mat[19] : 1.0f / mat[19];
+    orc_surface_from_block(block, nn, s);
+}
+
+/* OffsetRayOrigin (SurfaceParameters.cpp:403-416) */
+void orc_offset_origin(const orc_surface* s, const float* pos, float error, const float* dir, float bias, float* out,
+                       float* signed_bias)
+{
+    ov3 n = s->n;
+    float off = dot(v3(dir[0], dir[1], dir[2]), n) < 0.0f ? -1.0f : 1.0f;
+    float sb = off * error * bias;
+    if (signed_bias) *signed_bias = sb;
+    out[0] = pos[0] + sb * n.x;
+    out[1] = pos[1] + sb * n.y;
+    out[2] = pos[2] + sb * n.z;
+}
+
+/* NextEventEstimation + SampleRectangleLightSolidAngle; out = dir[3], radiance[3], distance, pdfW */
+void orc_nee(orc_uniforms* u, const orc_light* lights, uint32_t n_lights, const float* position, const float* normal,
+             int cosine_once, float* out)
+{
+    memset(out, 0, 8 * sizeof(float));
+    out[7] = 1.0f;
+    if (n_lights == 0) return;
+    float per_light = 1.0f / n_lights;
+    float p0 = orc_uniform(u);
+    uint32_t idx = (uint32_t)(p0 * n_lights);
+    if (idx >= n_lights) idx = n_lights - 1;
+    const orc_light* L = &lights[idx];
+    ov3 pos = ld3(position), nrm = ld3(normal);
+    if (L->type == 1) {
+        ov3 ul = sub(ld3(L->position), pos);
+        float dist_sq = dot(ul, ul);
+        float dist = sqrtf(dist_sq);
+        ov3 l = scl(ul, 1.0f / dist);
+        ov3 rad = scl(ld3(L->radiance), 1.0f / dist_sq);
+        out[0] = l.x;
+        out[1] = l.y;
+        out[2] = l.z;
+        out[3] = rad.x;
+        out[4] = rad.y;
+        out[5] = rad.z;
+        out[6] = dist;
+        out[7] = 1.0f * per_light;
+        return;
+    }
+    ov3 ex = ld3(L->x), ez = ld3(L->z);
+    ov3 s = sub(sub(ld3(L->position), scl(ex, 0.5f)), scl(ez, 0.5f));
+    float q[12] = {s.x, s.y, s.z, ex.x, ex.y, ex.z, ez.x, ez.y, ez.z, pos.x, pos.y, pos.z};
+    float uu = orc_uniform(u);
+    float vv = orc_uniform(u);
+    float r[5];
+    orc_rect_light(q, uu, vv, r);
+    float pdf = 1.0f / r[0];
+    ov3 lp = v3(r[2], r[3], r[4]);
+    ov3 ul = sub(lp, pos);
+    float dist_sq = dot(ul, ul);
+    float dist = sqrtf(dist_sq);
+    ov3 l = scl(ul, 1.0f / dist);
+    float am = dot(ld3(L->direction), neg(l)) / dist_sq;
+    if (am > 0) {
+        ov3 rad = cosine_once ? scl(ld3(L->radiance), am) : scl(scl(ld3(L->radiance), am), dot(nrm, l));
+        out[0] = l.x;
+        out[1] = l.y;
+        out[2] = l.z;
+        out[3] = rad.x;
+        out[4] = rad.y;
+        out[5] = rad.z;
+        out[6] = dist;
+        out[7] = pdf * am * per_light;
+    } else {
+        out[7] = 1.0f * per_light;
+    }
+}
+
+typedef struct {
+    const orc_scene* sc;
+    const orc_pt_job* job;
+    orc_camera cam;
+    float* fb;
+    uint32_t npix;
+} pt_ctx;
+
+static void bsdf_eval_any(const orc_surface* s, const float* v, const float* l, float* out5)
+{
+    if (s->shader == 2)
+        orc_lambert_eval(s, v, l, out5);
+    else
+        orc_disney_eval(s, v, l, s->shader == 0, out5);
+}
+
+static int bsdf_sample_any(orc_uniforms* u, const orc_surface* s, const float* v, float* out14)
+{
+    memset(out14, 0, 14 * sizeof(float));
+    if (s->shader == 2) return orc_lambert_sample(u, s, v, out14);
+    return orc_disney_sample(u, s, v, s->shader == 0, out14);
+}
+
+/* EvaluatePath for one (pixel, sample); writes ld[6] = layer0 rgb, layer1 rgb */
+static void evaluate_path(const pt_ctx* c, uint32_t* stack, uint32_t x, uint32_t y, uint64_t path, float* ld)
+{
+    const orc_scene* sc = c->sc;
+    const orc_pt_job* job = c->job;
+    uint64_t st, inc;
+    orc_pcg32_seed(job->seed ^ (path * 0x9E3779B97F4A7C15ull), path, &st, &inc);
+    orc_uniforms u = {NULL, 0, 0, &st, inc};
+    ov3 l0 = v3(0, 0, 0), l1 = v3(0, 0, 0), thr = v3(1, 1, 1);
+    float vx = (float)x + orc_uniform(&u);
+    float vy = (float)y + orc_uniform(&u);
+    float o[3], d[3];
+    o[0] = c->cam.pos.x;
+    o[1] = c->cam.pos.y;
+    o[2] = c->cam.pos.z;
+    orc_camera_dir(&c->cam, vx, vy, d);
+    uint32_t bounce = 0;
+    while (bounce < job->max_path_length) {
+        float t, hu, hv;
+        uint32_t prim = orc_bvh_closest1(sc->bvh, stack, o, d, 0.0f, FLOAT_MAX, &t, &hu, &hv);
+        if (prim == 0xFFFFFFFFu) {
+            l1 = add(l1, mul(ld3(sc->background), thr));
+            break;
+        }
+        uint32_t geom = sc->tri_geom[prim];
+        const float* tp = sc->tri9 + (size_t)prim * 9;
+        float e1[3] = {tp[3] - tp[0], tp[4] - tp[1], tp[5] - tp[2]};
+        float e2[3] = {tp[6] - tp[0], tp[7] - tp[1], tp[8] - tp[2]};
+        ov3 ngv = cross(ld3(e1), ld3(e2));
+        float ng[3] = {ngv.x, ngv.y, ngv.z};
+        const float* mat = sc->materials + 22 * (size_t)sc->geom_material[geom];
+        orc_surface s;
+        float pos[3], err;
+        orc_surface_from_hit(o, d, t, ng, mat, NULL, &s, pos, &err);
+        float v[3] = {-d[0], -d[1], -d[2]};
+        float nrm[3] = {s.n.x, s.n.y, s.n.z};
+        float ls[8];
+        orc_nee(&u, sc->lights, sc->n_lights, pos, nrm, (job->flags & 1u) != 0, ls);
+        if (dot(ld3(ls + 3), v3(1, 1, 1)) > 0) {
+            float f[5];
+            bsdf_eval_any(&s, v, ls, f);
+            if (dot(ld3(f), v3(1, 1, 1)) > 0) {
+                float so[3];
+                orc_offset_origin(&s, pos, err, ls, 0.1f, so, NULL);
+                if (!orc_bvh_occluded1(sc->bvh, stack, so, ls, err, ls[6])) {
+                    ov3 smp = scl(mul(ld3(f), ld3(ls + 3)), 1.0f / ls[7]);
+                    l1 = add(l1, mul(smp, thr));
+                }
+            }
+        }
+        float bs[14];
+        if (!bsdf_sample_any(&u, &s, v, bs)) break;
+        thr = mul(scl(thr, 1.0f), ld3(bs + 2));
+        float no[3];
+        orc_offset_origin(&s, pos, err, bs + 5, 1.0f, no, NULL);
+        o[0] = no[0];
+        o[1] = no[1];
+        o[2] = no[2];
+        d[0] = bs[5];
+        d[1] = bs[6];
+        d[2] = bs[7];
+        if (bounce == 0) l0 = ld3(bs + 2);
+        ++bounce;
+        if (bounce > 8 && bounce < job->max_path_length) {
+            float cp = maxf(maxf(thr.x, thr.y), thr.z);
+            if (orc_uniform(&u) > cp) break;
+            thr = scl(thr, 1.0f / cp);
+        }
+    }
+    ld[0] = l0.x;
+    ld[1] = l0.y;
+    ld[2] = l0.z;
+    ld[3] = l1.x;
+    ld[4] = l1.y;
+    ld[5] = l1.z;
+}
+
+static void pt_range(void* cv, uint32_t begin, uint32_t end)
+{
+    const pt_ctx* c = (const pt_ctx*)cv;
+    uint32_t* stack = (uint32_t*)malloc(sizeof(uint32_t) * orc_bvh_stack_size(c->sc->bvh));
+    const uint32_t W = c->job->width, spp = c->job->spp;
+    for (uint32_t pix = begin; pix < end; ++pix) {
+        uint32_t x = pix % W, y = pix / W;
+        ov3 a0 = v3(0, 0, 0), a1 = v3(0, 0, 0);
+        for (uint32_t s = 0; s < spp; ++s) {
+            float ld[6];
+            evaluate_path(c, stack, x, y, (uint64_t)pix * spp + s, ld);
+            a0 = add(a0, ld3(ld));
+            a1 = add(a1, ld3(ld + 3));
+        }
+        float inv = 1.0f / (float)spp;
+        float* f0 = c->fb + 3 * (size_t)pix;
+        float* f1 = c->fb + 3 * ((size_t)c->npix + pix);
+        /* zero + sum, then FrameBuffer_Scale */
+        f0[0] = (0.0f + a0.x) * inv;
+        f0[1] = (0.0f + a0.y) * inv;
+        f0[2] = (0.0f + a0.z) * inv;
+        f1[0] = (0.0f + a1.x) * inv;
+        f1[1] = (0.0f + a1.y) * inv;
+        f1[2] = (0.0f + a1.z) * inv;
+    }
+    free(stack);
+}
+
+void orc_render_pt(const orc_scene* sc, const orc_pt_job* job, float* fb, int threads)
+{
+    pt_ctx c;
+    c.sc = sc;
+    c.job = job;
+    orc_camera_init(job->cam10, job->width, job->height, &c.cam);
+    c.fb = fb;
+    c.npix = job->width * job->height;
+    orc_parallel_for(pt_range, &c, c.npix, threads);
+}

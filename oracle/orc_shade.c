@@ -716,8 +716,27 @@ int orc_lambert_sample(orc_uniforms* u, const orc_surface* s, const float* vv, f
     return 1;
 }
 
+uint32_t orc_pcg32_next(uint64_t* state, uint64_t inc)
+{
+    uint64_t old = *state;
+    *state = old * 6364136223846793005ULL + inc;
+    uint32_t xorshifted = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xorshifted >> rot) | (xorshifted << ((32u - rot) & 31u));
+}
+
+void orc_pcg32_seed(uint64_t initstate, uint64_t initseq, uint64_t* state, uint64_t* inc)
+{
+    *state = 0u;
+    *inc = (initseq << 1u) | 1u;
+    orc_pcg32_next(state, *inc);
+    *state += initstate;
+    orc_pcg32_next(state, *inc);
+}
+
 float orc_uniform(orc_uniforms* u)
 {
+    if (u->pcg_state) return (float)(orc_pcg32_next(u->pcg_state, u->pcg_inc) >> 8) * (1.0f / 16777216.0f);
     if (u->pos < u->count) return u->values[u->pos++];
     u->pos++;
     return 0.5f;
@@ -943,7 +962,7 @@ void orc_bsdf_sample_batch(const float* blocks, const float* n, const float* v, 
     for (uint32_t i = 0; i < count; ++i) {
         orc_surface s;
         orc_surface_from_block(blocks + 22 * (size_t)i, n + 3 * (size_t)i, &s);
-        orc_uniforms u = {uniforms + (size_t)n_uniforms * i, n_uniforms, 0};
+        orc_uniforms u = {uniforms + (size_t)n_uniforms * i, n_uniforms, 0, NULL, 0};
         if (s.shader == 2)
             orc_lambert_sample(&u, &s, v + 3 * (size_t)i, out + 14 * (size_t)i);
         else

@@ -491,74 +491,95 @@ typedef struct {
     int any;
 } trav_ctx;
 
+/* One ray through the tree. `stack` holds at least depth + 4 entries. Closest hit (any == 0):
+ * returns the hit triangle's global index or 0xFFFFFFFF, with t/u/v; any hit: returns 1 / 0. */
+static uint32_t trav_one(const orc_bvh* b, uint32_t* stack, const float* o, const float* d, float tnear, float tfar,
+                         int any, float* t_out, float* u_out, float* v_out)
+{
+    float inv[3];
+    for (int k = 0; k < 3; ++k) {
+        float dk = fabsf(d[k]) < 1e-20f ? copysignf(1e-20f, d[k]) : d[k];
+        inv[k] = 1.0f / dk;
+    }
+    float best_t = tfar, bu = 0.0f, bv = 0.0f;
+    uint32_t best = 0xFFFFFFFFu;
+    int occ = 0;
+    int sp = 0;
+    if (b->ntri > 0) stack[sp++] = 0;
+    while (sp > 0 && !occ) {
+        const orc_node* n = &b->nodes[stack[--sp]];
+        float tm;
+        if (!box_hit(n, o, inv, tnear, best_t, &tm)) continue;
+        if (n->count > 0) {
+            for (uint32_t q = n->left_or_first; q < n->left_or_first + n->count; ++q) {
+                uint32_t k = b->prims[q];
+                float t, u, v;
+                if (!orc_tri_test(b->recs[k].v0, b->recs[k].e1, b->recs[k].e2, o, d, tnear, &t, &u, &v)) continue;
+                if (any) {
+                    if (t <= tfar) {
+                        occ = 1;
+                        break;
+                    }
+                } else if (t < best_t || (t == best_t && k < best)) {
+                    best_t = t;
+                    best = k;
+                    bu = u;
+                    bv = v;
+                }
+            }
+        } else {
+            uint32_t l = n->left_or_first;
+            float tl, tr;
+            int hl = box_hit(&b->nodes[l], o, inv, tnear, best_t, &tl);
+            int hr = box_hit(&b->nodes[l + 1], o, inv, tnear, best_t, &tr);
+            if (hl && hr) {
+                if (tl <= tr) {
+                    stack[sp++] = l + 1;
+                    stack[sp++] = l;
+                } else {
+                    stack[sp++] = l;
+                    stack[sp++] = l + 1;
+                }
+            } else if (hl) {
+                stack[sp++] = l;
+            } else if (hr) {
+                stack[sp++] = l + 1;
+            }
+        }
+    }
+    if (any) return (uint32_t)occ;
+    *t_out = best_t;
+    *u_out = best == 0xFFFFFFFFu ? 0.0f : bu;
+    *v_out = best == 0xFFFFFFFFu ? 0.0f : bv;
+    return best;
+}
+
+uint32_t orc_bvh_stack_size(const orc_bvh* b) { return b->depth + 4; }
+
+uint32_t orc_bvh_closest1(const orc_bvh* b, uint32_t* stack, const float* o, const float* d, float tnear, float tfar,
+                          float* t, float* u, float* v)
+{
+    return trav_one(b, stack, o, d, tnear, tfar, 0, t, u, v);
+}
+
+int orc_bvh_occluded1(const orc_bvh* b, uint32_t* stack, const float* o, const float* d, float tnear, float tfar)
+{
+    return (int)trav_one(b, stack, o, d, tnear, tfar, 1, NULL, NULL, NULL);
+}
+
 static void trav_range(void* cv, uint32_t begin, uint32_t end)
 {
     trav_ctx* c = (trav_ctx*)cv;
     const orc_bvh* b = c->b;
-    /* depth-first with both children pushed: at most one pending sibling per level, plus two */
     uint32_t* stack = (uint32_t*)malloc(sizeof(uint32_t) * (b->depth + 4));
     for (uint32_t i = begin; i < end; ++i) {
         float o[3] = {c->rays.ox[i], c->rays.oy[i], c->rays.oz[i]};
         float d[3] = {c->rays.dx[i], c->rays.dy[i], c->rays.dz[i]};
-        float inv[3];
-        for (int k = 0; k < 3; ++k) {
-            float dk = fabsf(d[k]) < 1e-20f ? copysignf(1e-20f, d[k]) : d[k];
-            inv[k] = 1.0f / dk;
-        }
         float tnear = c->rays.tnear[i], tfar = c->rays.tfar[i];
-        float best_t = tfar, bu = 0.0f, bv = 0.0f;
-        uint32_t best = 0xFFFFFFFFu;
-        int occ = 0;
-        int sp = 0;
-        if (b->ntri > 0) stack[sp++] = 0;
-        while (sp > 0 && !occ) {
-            const orc_node* n = &b->nodes[stack[--sp]];
-            float tm;
-            if (!box_hit(n, o, inv, tnear, best_t, &tm)) continue;
-            if (n->count > 0) {
-                for (uint32_t q = n->left_or_first; q < n->left_or_first + n->count; ++q) {
-                    uint32_t k = b->prims[q];
-                    float t, u, v;
-                    if (!orc_tri_test(b->recs[k].v0, b->recs[k].e1, b->recs[k].e2, o, d, tnear, &t, &u, &v)) continue;
-                    if (c->any) {
-                        if (t <= tfar) {
-                            occ = 1;
-                            break;
-                        }
-                    } else if (t < best_t || (t == best_t && k < best)) {
-                        best_t = t;
-                        best = k;
-                        bu = u;
-                        bv = v;
-                    }
-                }
-            } else {
-                uint32_t l = n->left_or_first;
-                float tl, tr;
-                int hl = box_hit(&b->nodes[l], o, inv, tnear, best_t, &tl);
-                int hr = box_hit(&b->nodes[l + 1], o, inv, tnear, best_t, &tr);
-                if (hl && hr) {
-                    if (tl <= tr) {
-                        stack[sp++] = l + 1;
-                        stack[sp++] = l;
-                    } else {
-                        stack[sp++] = l;
-                        stack[sp++] = l + 1;
-                    }
-                } else if (hl) {
-                    stack[sp++] = l;
-                } else if (hr) {
-                    stack[sp++] = l + 1;
-                }
-            }
-        }
         if (c->any) {
-            c->occluded[i] = (uint32_t)occ;
+            c->occluded[i] = trav_one(b, stack, o, d, tnear, tfar, 1, NULL, NULL, NULL);
         } else {
-            c->hits.t[i] = best_t;
-            c->hits.prim[i] = best;
-            c->hits.u[i] = best == 0xFFFFFFFFu ? 0.0f : bu;
-            c->hits.v[i] = best == 0xFFFFFFFFu ? 0.0f : bv;
+            c->hits.prim[i] = trav_one(b, stack, o, d, tnear, tfar, 0, &c->hits.t[i], &c->hits.u[i], &c->hits.v[i]);
         }
     }
     free(stack);

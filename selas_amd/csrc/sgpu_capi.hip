@@ -260,8 +260,89 @@ int sgpu_scene_release(sgpu_scene scene)
     if (scene->d_nodes) (void)hipFree(scene->d_nodes);
     if (scene->d_tris) (void)hipFree(scene->d_tris);
     if (scene->d_overflow) (void)hipFree(scene->d_overflow);
+    if (scene->d_materials) (void)hipFree(scene->d_materials);
+    if (scene->d_geom_material) (void)hipFree(scene->d_geom_material);
+    if (scene->d_lights) (void)hipFree(scene->d_lights);
     delete scene;
     return SGPU_OK;
+}
+
+int sgpu_scene_set_materials(sgpu_scene scene, const float* materials, uint32_t n_materials,
+                             const uint32_t* geom_material, uint32_t n_geoms)
+{
+    if (!scene || !materials || !geom_material || n_materials == 0 || n_geoms == 0)
+        return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_materials: NULL or empty argument");
+    for (uint32_t g = 0; g < n_geoms; ++g)
+        if (geom_material[g] >= n_materials)
+            return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_materials: material index out of range");
+    for (uint32_t m = 0; m < n_materials; ++m) {
+        float sh = materials[22 * (size_t)m + 20];
+        if (!(sh == 0.0f || sh == 1.0f || sh == 2.0f))
+            return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_materials: shader must be 0, 1 or 2");
+    }
+    std::lock_guard<std::mutex> lock(scene->mu);
+    SGPU_HIP_TRY(hipSetDevice(scene->ctx->device));
+    float* dm = nullptr;
+    uint32_t* dg = nullptr;
+    SGPU_HIP_TRY(hipMalloc((void**)&dm, 22 * sizeof(float) * (size_t)n_materials));
+    hipError_t e = hipMalloc((void**)&dg, sizeof(uint32_t) * (size_t)n_geoms);
+    if (e != hipSuccess) {
+        (void)hipFree(dm);
+        return sgpu_fail(SGPU_E_NOMEM, "sgpu_scene_set_materials: hipMalloc failed");
+    }
+    e = hipMemcpy(dm, materials, 22 * sizeof(float) * (size_t)n_materials, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dg, geom_material, sizeof(uint32_t) * (size_t)n_geoms, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(dm);
+        (void)hipFree(dg);
+        return sgpu_fail(SGPU_E_HIP, std::string("sgpu_scene_set_materials: ") + hipGetErrorString(e));
+    }
+    if (scene->d_materials) (void)hipFree(scene->d_materials);
+    if (scene->d_geom_material) (void)hipFree(scene->d_geom_material);
+    scene->d_materials = dm;
+    scene->n_materials = n_materials;
+    scene->d_geom_material = dg;
+    scene->n_geom_material = n_geoms;
+    return SGPU_OK;
+}
+
+int sgpu_scene_set_lights(sgpu_scene scene, const sgpu_light* lights, uint32_t n_lights, const float* background_rgb)
+{
+    if (!scene || (n_lights > 0 && !lights)) return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_lights: NULL argument");
+    for (uint32_t i = 0; i < n_lights; ++i)
+        if (lights[i].type > 1) return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_lights: light type must be 0 or 1");
+    std::lock_guard<std::mutex> lock(scene->mu);
+    SGPU_HIP_TRY(hipSetDevice(scene->ctx->device));
+    void* dl = nullptr;
+    if (n_lights > 0) {
+        SGPU_HIP_TRY(hipMalloc(&dl, sizeof(sgpu_light) * (size_t)n_lights));
+        hipError_t e = hipMemcpy(dl, lights, sizeof(sgpu_light) * (size_t)n_lights, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(dl);
+            return sgpu_fail(SGPU_E_HIP, std::string("sgpu_scene_set_lights: ") + hipGetErrorString(e));
+        }
+    }
+    if (scene->d_lights) (void)hipFree(scene->d_lights);
+    scene->d_lights = dl;
+    scene->n_lights = n_lights;
+    for (int k = 0; k < 3; ++k) scene->background[k] = background_rgb ? background_rgb[k] : 0.0f;
+    return SGPU_OK;
+}
+
+int sgpu_render_pt(sgpu_scene scene, const sgpu_pt_job* job, float* framebuffer, void* stream)
+{
+    if (!scene || !job || !framebuffer) return sgpu_fail(SGPU_E_INVALID, "sgpu_render_pt: NULL argument");
+    if (!scene->committed) return sgpu_fail(SGPU_E_STATE, "sgpu_render_pt: scene not committed");
+    if (!scene->d_materials || scene->n_geom_material < scene->geom_first.size())
+        return sgpu_fail(SGPU_E_STATE, "sgpu_render_pt: materials not set for every geometry");
+    if (job->camera.width == 0 || job->camera.height == 0 || job->spp == 0 || job->max_path_length == 0)
+        return sgpu_fail(SGPU_E_INVALID, "sgpu_render_pt: zero width, height, spp or path length");
+    if ((uint64_t)job->camera.width * job->camera.height > (1ull << 26))
+        return sgpu_fail(SGPU_E_RANGE, "sgpu_render_pt: more than 2^26 pixels");
+    if (job->spp > (1u << 20)) return sgpu_fail(SGPU_E_RANGE, "sgpu_render_pt: spp above 2^20");
+    if (job->max_path_length > 1024) return sgpu_fail(SGPU_E_RANGE, "sgpu_render_pt: path length above 1024");
+    SGPU_HIP_TRY(hipSetDevice(scene->ctx->device));
+    return sgpu_pt_render_impl(scene, job, framebuffer, as_stream(stream));
 }
 
 int sgpu_scene_info(sgpu_scene scene, uint32_t* n_tris, uint32_t* n_nodes, uint32_t* max_depth)

@@ -60,6 +60,14 @@ struct sgpu_scene_s {
     TriRecord* d_tris = nullptr;
     // set to non-zero by a traversal whose stack would exceed the tree depth (never expected)
     uint32_t* d_overflow = nullptr;
+    // shading data (sgpu_scene_set_materials / sgpu_scene_set_lights)
+    float* d_materials = nullptr;      // 22 floats per material
+    uint32_t n_materials = 0;
+    uint32_t* d_geom_material = nullptr;  // material index per geometry
+    uint32_t n_geom_material = 0;
+    void* d_lights = nullptr;          // sgpu_light records
+    uint32_t n_lights = 0;
+    float background[3] = {0.0f, 0.0f, 0.0f};
 };
 
 // ---- error plumbing ------------------------------------------------------------------------------
@@ -78,3 +86,5 @@ int sgpu_bvh_build(sgpu_scene scene, hipStream_t stream);
 // Launches the traversal kernels (sgpu_trace.hip).
 int sgpu_trace_launch(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits,
                       uint32_t* occluded, uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream);
+// Wavefront path tracer (sgpu_pt.hip); arguments validated by sgpu_render_pt.
+int sgpu_pt_render_impl(sgpu_scene scene, const sgpu_pt_job* job, float* framebuffer, hipStream_t stream);

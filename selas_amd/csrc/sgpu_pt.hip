@@ -1,0 +1,384 @@
+// Wavefront unidirectional path tracer: the device form of PathTracer::GenerateImage /
+// EvaluatePath (Source/Applications/Selas/Source/PathTracer.cpp:142-348).
+//
+// One path per (pixel, sample). Each bounce is a sequence of launches on one stream:
+//   sgpu_intersect over the active queue -> k_pt_shade (surface, NEE, BSDF sample, append the
+//   shadow ray and the continuation ray with ballot/prefix compaction) -> sgpu_occluded over the
+//   shadow queue -> k_pt_resolve (add unoccluded light contributions).
+// Per path the arithmetic and the order of random numbers follow EvaluatePath: camera jitter
+// (2), then per bounce NEE (light pick + 2), BSDF sample (1 + 2..3), Russian roulette after 8
+// bounces (1). Random numbers come from a PCG32 stream per path (stream = path index), so a
+// path's result does not depend on scheduling. Samples of a pixel are summed in sample order
+// and the frame is scaled by 1/spp, as FrameBuffer_Scale does (PathTracer.cpp:346).
+// Layers: 0 = first-bounce reflectance, 1 = radiance (PathTracer.cpp:191,224,241).
+#include "sgpu_camera.h"
+#include "sgpu_internal.h"
+#include "sgpu_light.h"
+#include "sgpu_surface.h"
+
+#include <algorithm>
+
+namespace {
+
+using namespace sgpu;
+
+constexpr int kBlock = 256;
+constexpr float kFloatMax = 3.402823466e+38F;
+
+// Append with one atomic per wave: ballot the predicate, count the lanes below, let the first
+// active lane reserve the wave's slots. Returns the slot for lanes with pred, else ~0.
+__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter)
+{
+    const uint64_t mask = __ballot(pred);
+    if (mask == 0) return 0xFFFFFFFFu;
+    const uint32_t lane = __lane_id();
+    const uint32_t below = __popcll(mask & ((1ull << lane) - 1ull));
+    const uint32_t leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader, 64);
+    return pred ? base + below : 0xFFFFFFFFu;
+}
+
+struct PathState {
+    float* thr;    // 3 per path
+    float* ld0;    // 3 per path, layer 0
+    float* ld1;    // 3 per path, layer 1
+    uint64_t* rng; // PCG32 state per path (increment = 2 * path_id + 1)
+    uint32_t* flags;  // bit 0: every scatter so far was a Dirac event
+};
+
+struct RayBuf {
+    float *ox, *oy, *oz, *dx, *dy, *dz, *tnear, *tfar;
+};
+
+struct HitBuf {
+    float *t, *u, *v, *ngx, *ngy, *ngz;
+    uint32_t *prim, *geom;
+};
+
+struct ShadowBuf {
+    RayBuf ray;
+    uint32_t* path;
+    float* c;  // 3 per shadow ray
+    uint32_t* occluded;
+};
+
+__device__ __forceinline__ void put_ray(const RayBuf& r, uint32_t i, f3 o, f3 d, float tnear, float tfar)
+{
+    r.ox[i] = o.x;
+    r.oy[i] = o.y;
+    r.oz[i] = o.z;
+    r.dx[i] = d.x;
+    r.dy[i] = d.y;
+    r.dz[i] = d.z;
+    r.tnear[i] = tnear;
+    r.tfar[i] = tfar;
+}
+
+__device__ __forceinline__ Pcg32 load_rng(const PathState& ps, uint64_t path_global, uint32_t p)
+{
+    Pcg32 r;
+    r.state = ps.rng[p];
+    r.inc = (path_global << 1u) | 1u;
+    return r;
+}
+
+__device__ __forceinline__ void add3(float* a, uint32_t p, f3 v)
+{
+    a[3 * p] += v.x;
+    a[3 * p + 1] += v.y;
+    a[3 * p + 2] += v.z;
+}
+
+__device__ __forceinline__ f3 get3(const float* a, uint32_t p) { return mk3(a[3 * p], a[3 * p + 1], a[3 * p + 2]); }
+__device__ __forceinline__ void set3(float* a, uint32_t p, f3 v)
+{
+    a[3 * p] = v.x;
+    a[3 * p + 1] = v.y;
+    a[3 * p + 2] = v.z;
+}
+
+__global__ __launch_bounds__(kBlock) void k_pt_generate(CameraParams cam, uint32_t spp, uint64_t seed,
+                                                        uint64_t path_base, uint32_t n_paths, PathState ps,
+                                                        RayBuf rays, uint32_t* __restrict queue)
+{
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n_paths) return;
+    uint64_t path = path_base + i;
+    uint32_t pixel = (uint32_t)(path / spp);
+    uint32_t x = pixel % cam.width, y = pixel / cam.width;
+    PcgUniforms u{pcg32_seed(seed ^ (path * 0x9E3779B97F4A7C15ull), path)};
+    // JitteredCameraRay (Camera.cpp:63-75)
+    float vx = (float)x + u.next();
+    float vy = (float)y + u.next();
+    f3 d = camera_image_to_world_dir(cam, vx, vy);
+    put_ray(rays, i, cam.position, d, 0.0f, kFloatMax);
+    queue[i] = i;
+    set3(ps.thr, i, mk3(1.0f, 1.0f, 1.0f));
+    set3(ps.ld0, i, mk3(0, 0, 0));
+    set3(ps.ld1, i, mk3(0, 0, 0));
+    ps.rng[i] = u.r.state;
+    ps.flags[i] = 1u;
+}
+
+struct ShadeParams {
+    const float* materials;
+    const uint32_t* geom_material;
+    uint32_t n_geoms;
+    const Light* lights;
+    uint32_t n_lights;
+    f3 background;
+    uint32_t bounce;  // bounces completed before this one
+    uint32_t max_len;
+    uint32_t flags;
+    uint64_t path_base;
+};
+
+__global__ __launch_bounds__(kBlock) void k_pt_shade(ShadeParams sp, uint32_t count, const uint32_t* __restrict queue,
+                                                     RayBuf rays, HitBuf hits, PathState ps, RayBuf next_rays,
+                                                     uint32_t* __restrict next_queue, ShadowBuf shadow,
+                                                     uint32_t* __restrict counters)
+{
+    uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    bool active = j < count;
+    bool cont = false, emit = false;
+    uint32_t p = 0;
+    f3 next_o = mk3(0, 0, 0), next_d = mk3(0, 0, 0), sh_o = mk3(0, 0, 0), sh_d = mk3(0, 0, 0), c = mk3(0, 0, 0);
+    float sh_tnear = 0.0f, sh_tfar = 0.0f;
+    if (active) {
+        p = queue[j];
+        f3 o = mk3(rays.ox[j], rays.oy[j], rays.oz[j]);
+        f3 d = mk3(rays.dx[j], rays.dy[j], rays.dz[j]);
+        f3 thr = get3(ps.thr, p);
+        uint32_t geom = hits.geom[j];
+        if (geom == SGPU_INVALID_ID || geom >= sp.n_geoms) {
+            // miss: background (EvaluateBackground and EvaluateBackgroundMiss agree without an IBL)
+            add3(ps.ld1, p, sp.background * thr);
+        } else {
+            PcgUniforms u{load_rng(ps, sp.path_base + p, p)};
+            const float* mat = sp.materials + 22 * (size_t)sp.geom_material[geom];
+            Surface s;
+            surface_from_hit(o, d, hits.t[j], mk3(hits.ngx[j], hits.ngy[j], hits.ngz[j]), mat, nullptr, s);
+            f3 v = -d;
+            LightSample ls;
+            next_event_estimation(u, sp.lights, sp.n_lights, s.position, s.n, (sp.flags & SGPU_PT_NEE_COSINE_ONCE) != 0,
+                                  ls);
+            if (dot3(ls.radiance, mk3(1, 1, 1)) > 0) {
+                float fp, rp;
+                f3 f = bsdf_eval(s, v, ls.direction, fp, rp);
+                if (dot3(f, mk3(1, 1, 1)) > 0) {
+                    // OcclusionRay (PathTracer.cpp:71-92): origin biased by 0.1 error, tnear = error
+                    emit = true;
+                    sh_o = offset_ray_origin(s, ls.direction, 0.1f);
+                    sh_d = ls.direction;
+                    sh_tnear = s.error;
+                    sh_tfar = ls.distance;
+                    c = (f * ls.radiance * (1.0f / ls.pdf_w)) * thr;
+                }
+            }
+            BsdfSample bs;
+            if (bsdf_sample(u, s, v, bs)) {
+                uint32_t fl = ps.flags[p];
+                if (!(bs.flags & kDirac)) fl &= ~1u;
+                ps.flags[p] = fl;
+                thr = (1.0f * thr) * bs.reflectance;
+                next_o = offset_ray_origin(s, bs.wi, 1.0f);
+                next_d = bs.wi;
+                if (sp.bounce == 0) set3(ps.ld0, p, bs.reflectance);
+                uint32_t bounce = sp.bounce + 1;
+                cont = bounce < sp.max_len;
+                if (cont && bounce > 8) {
+                    float cp = max_(max_(thr.x, thr.y), thr.z);
+                    if (u.next() > cp)
+                        cont = false;
+                    else
+                        thr = thr * (1.0f / cp);
+                }
+                set3(ps.thr, p, thr);
+            }
+            ps.rng[p] = u.r.state;
+        }
+    }
+    uint32_t ks = wave_append(emit, &counters[1]);
+    if (emit) {
+        put_ray(shadow.ray, ks, sh_o, sh_d, sh_tnear, sh_tfar);
+        shadow.path[ks] = p;
+        set3(shadow.c, ks, c);
+    }
+    uint32_t kn = wave_append(cont, &counters[0]);
+    if (cont) {
+        put_ray(next_rays, kn, next_o, next_d, 0.0f, kFloatMax);
+        next_queue[kn] = p;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_pt_resolve(uint32_t count, ShadowBuf shadow, PathState ps)
+{
+    uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= count) return;
+    if (shadow.occluded[k]) return;
+    add3(ps.ld1, shadow.path[k], get3(shadow.c, k));
+}
+
+// Sum each pixel's samples in sample order into the frame (both layers).
+__global__ __launch_bounds__(kBlock) void k_pt_accumulate(uint32_t spp, uint32_t pixel_base, uint32_t n_pixels,
+                                                          uint32_t n_frame_pixels, PathState ps, float* __restrict fb)
+{
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n_pixels) return;
+    f3 a0 = mk3(0, 0, 0), a1 = mk3(0, 0, 0);
+    for (uint32_t s = 0; s < spp; ++s) {
+        uint32_t p = i * spp + s;
+        a0 += get3(ps.ld0, p);
+        a1 += get3(ps.ld1, p);
+    }
+    uint32_t pix = pixel_base + i;
+    add3(fb, pix, a0);
+    add3(fb + 3 * (size_t)n_frame_pixels, pix, a1);
+}
+
+__global__ __launch_bounds__(kBlock) void k_scale(float* __restrict fb, size_t n, float s)
+{
+    size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) fb[i] = fb[i] * s;
+}
+
+struct Arena {
+    std::vector<void*> ptrs;
+    hipStream_t stream;
+    hipError_t err = hipSuccess;
+    template <typename T>
+    T* get(size_t count)
+    {
+        void* p = nullptr;
+        if (err == hipSuccess) err = hipMallocAsync(&p, count * sizeof(T) + 64, stream);
+        if (err == hipSuccess) ptrs.push_back(p);
+        return (T*)p;
+    }
+    RayBuf rays(size_t n)
+    {
+        RayBuf r;
+        r.ox = get<float>(n);
+        r.oy = get<float>(n);
+        r.oz = get<float>(n);
+        r.dx = get<float>(n);
+        r.dy = get<float>(n);
+        r.dz = get<float>(n);
+        r.tnear = get<float>(n);
+        r.tfar = get<float>(n);
+        return r;
+    }
+    ~Arena()
+    {
+        for (void* p : ptrs) (void)hipFreeAsync(p, stream);
+    }
+};
+
+sgpu_rays_soa as_soa(const RayBuf& r) { return sgpu_rays_soa{r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, r.tnear, r.tfar}; }
+
+}  // namespace
+
+int sgpu_pt_render_impl(sgpu_scene scene, const sgpu_pt_job* job, float* fb, hipStream_t stream)
+{
+    const uint32_t W = job->camera.width, H = job->camera.height, spp = job->spp;
+    const uint64_t n_pixels = (uint64_t)W * H;
+    const uint32_t max_chunk_paths = 1u << 22;
+    uint32_t chunk_pixels = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(n_pixels, max_chunk_paths / spp));
+    const uint32_t cap = chunk_pixels * spp;
+
+    CameraParams cam = camera_init(mk3(job->camera.position[0], job->camera.position[1], job->camera.position[2]),
+                                   mk3(job->camera.look_at[0], job->camera.look_at[1], job->camera.look_at[2]),
+                                   mk3(job->camera.up[0], job->camera.up[1], job->camera.up[2]),
+                                   job->camera.fov_degrees, W, H);
+
+    Arena A;
+    A.stream = stream;
+    PathState ps;
+    ps.thr = A.get<float>(3 * (size_t)cap);
+    ps.ld0 = A.get<float>(3 * (size_t)cap);
+    ps.ld1 = A.get<float>(3 * (size_t)cap);
+    ps.rng = A.get<uint64_t>(cap);
+    ps.flags = A.get<uint32_t>(cap);
+    RayBuf rays[2] = {A.rays(cap), A.rays(cap)};
+    uint32_t* queue[2] = {A.get<uint32_t>(cap), A.get<uint32_t>(cap)};
+    HitBuf hb;
+    hb.t = A.get<float>(cap);
+    hb.u = A.get<float>(cap);
+    hb.v = A.get<float>(cap);
+    hb.ngx = A.get<float>(cap);
+    hb.ngy = A.get<float>(cap);
+    hb.ngz = A.get<float>(cap);
+    hb.prim = A.get<uint32_t>(cap);
+    hb.geom = A.get<uint32_t>(cap);
+    ShadowBuf sb;
+    sb.ray = A.rays(cap);
+    sb.path = A.get<uint32_t>(cap);
+    sb.c = A.get<float>(3 * (size_t)cap);
+    sb.occluded = A.get<uint32_t>(cap);
+    uint32_t* counters = A.get<uint32_t>(4);
+    uint32_t* h_counters = nullptr;
+    if (A.err != hipSuccess) return sgpu_fail(SGPU_E_NOMEM, "sgpu_render_pt: device allocation failed");
+    SGPU_HIP_TRY(hipHostMalloc((void**)&h_counters, 4 * sizeof(uint32_t), hipHostMallocDefault));
+    struct HostFree {
+        uint32_t* p;
+        ~HostFree() { (void)hipHostFree(p); }
+    } hf{h_counters};
+
+    SGPU_HIP_TRY(hipMemsetAsync(fb, 0, 2 * 3 * n_pixels * sizeof(float), stream));
+
+    sgpu_hits_soa hsoa{hb.t, hb.u, hb.v, hb.prim, hb.geom, hb.ngx, hb.ngy, hb.ngz};
+    ShadeParams sp;
+    sp.materials = scene->d_materials;
+    sp.geom_material = scene->d_geom_material;
+    sp.n_geoms = scene->n_geom_material;
+    sp.lights = (const Light*)scene->d_lights;
+    sp.n_lights = scene->n_lights;
+    sp.background = mk3(scene->background[0], scene->background[1], scene->background[2]);
+    sp.max_len = job->max_path_length;
+    sp.flags = job->flags;
+
+    for (uint64_t pix0 = 0; pix0 < n_pixels; pix0 += chunk_pixels) {
+        const uint32_t npix = (uint32_t)std::min<uint64_t>(chunk_pixels, n_pixels - pix0);
+        const uint32_t n_paths = npix * spp;
+        const uint64_t path_base = pix0 * spp;
+        hipLaunchKernelGGL(k_pt_generate, dim3((n_paths + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, cam, spp,
+                           job->seed, path_base, n_paths, ps, rays[0], queue[0]);
+        SGPU_HIP_TRY(hipGetLastError());
+        uint32_t count = n_paths;
+        int cur = 0;
+        sp.path_base = path_base;
+        for (uint32_t bounce = 0; bounce < job->max_path_length && count > 0; ++bounce) {
+            sgpu_rays_soa rs = as_soa(rays[cur]);
+            int rc = sgpu_trace_launch(scene, &rs, &hsoa, nullptr, count, false, nullptr, stream);
+            if (rc) return rc;
+            SGPU_HIP_TRY(hipMemsetAsync(counters, 0, 4 * sizeof(uint32_t), stream));
+            sp.bounce = bounce;
+            hipLaunchKernelGGL(k_pt_shade, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, sp, count,
+                               queue[cur], rays[cur], hb, ps, rays[1 - cur], queue[1 - cur], sb, counters);
+            SGPU_HIP_TRY(hipGetLastError());
+            SGPU_HIP_TRY(hipMemcpyAsync(h_counters, counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+            SGPU_HIP_TRY(hipStreamSynchronize(stream));
+            const uint32_t n_next = h_counters[0], n_shadow = h_counters[1];
+            if (n_shadow > 0) {
+                sgpu_rays_soa ss = as_soa(sb.ray);
+                rc = sgpu_trace_launch(scene, &ss, nullptr, sb.occluded, n_shadow, true, nullptr, stream);
+                if (rc) return rc;
+                hipLaunchKernelGGL(k_pt_resolve, dim3((n_shadow + kBlock - 1) / kBlock), dim3(kBlock), 0, stream,
+                                   n_shadow, sb, ps);
+                SGPU_HIP_TRY(hipGetLastError());
+            }
+            count = n_next;
+            cur = 1 - cur;
+        }
+        hipLaunchKernelGGL(k_pt_accumulate, dim3((npix + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, spp,
+                           (uint32_t)pix0, npix, (uint32_t)n_pixels, ps, fb);
+        SGPU_HIP_TRY(hipGetLastError());
+    }
+    const size_t nf = 2 * 3 * n_pixels;
+    hipLaunchKernelGGL(k_scale, dim3((uint32_t)((nf + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream, fb, nf,
+                       1.0f / (float)spp);
+    SGPU_HIP_TRY(hipGetLastError());
+    SGPU_HIP_TRY(hipStreamSynchronize(stream));
+    return SGPU_OK;
+}

@@ -75,6 +75,9 @@ _SIGNATURES = {
     "sgpu_intersect": [_P, C.POINTER(RaysSoA), C.POINTER(HitsSoA), _U32, _P],
     "sgpu_occluded": [_P, C.POINTER(RaysSoA), _P, _U32, _P],
     "sgpu_intersect_stats": [_P, C.POINTER(RaysSoA), C.POINTER(HitsSoA), _U32, C.c_int, C.POINTER(TraceStats)],
+    "sgpu_scene_set_materials": [_P, _FP, _U32, _U32P, _U32],
+    "sgpu_scene_set_lights": [_P, _P, _U32, _FP],
+    "sgpu_render_pt": [_P, _P, _P, _P],
     "sgpu_bsdf_eval": [_P, _P, _P, _P, _U32, _P, _P],
     "sgpu_bsdf_sample": [_P, _P, _P, _P, _U32, _U32, _P, _P],
     "sgpu_gen_cluster_soup": [_U64, _U32, _U32, _F, _F, _F, _FP],
@@ -307,3 +310,49 @@ class Event:
         ms = C.c_float()
         check(lib.sgpu_event_elapsed_ms(self.handle, later.handle, C.byref(ms)))
         return ms.value
+
+
+# ---- integrators ------------------------------------------------------------------------------------
+
+class Camera(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("look_at", C.c_float * 3), ("up", C.c_float * 3),
+                ("fov_degrees", C.c_float), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+    @classmethod
+    def make(cls, cam: dict, width: int, height: int):
+        return cls((C.c_float * 3)(*cam["position"]), (C.c_float * 3)(*cam["look_at"]),
+                   (C.c_float * 3)(*cam["up"]), cam["fov"], width, height)
+
+
+class PtJob(C.Structure):
+    _fields_ = [("camera", Camera), ("spp", C.c_uint32), ("max_path_length", C.c_uint32), ("seed", C.c_uint64),
+                ("flags", C.c_uint32)]
+
+
+PT_NEE_COSINE_ONCE = 1
+
+
+def build_render_scene(ctx: Context, sc: dict) -> Scene:
+    """A committed device scene with materials and lights from a workloads.*_scene dict."""
+    scene = Scene(ctx)
+    for g in sc["geoms"]:
+        scene.add_soup(g)
+    scene.commit()
+    mats = np.ascontiguousarray(sc["materials"], np.float32)
+    gm = np.ascontiguousarray(sc["geom_material"], np.uint32)
+    check(lib.sgpu_scene_set_materials(scene.handle, fptr(mats), mats.shape[0], uptr(gm), gm.shape[0]))
+    lights = np.ascontiguousarray(sc["lights"])
+    bg = np.ascontiguousarray(sc["background"], np.float32)
+    check(lib.sgpu_scene_set_lights(scene.handle, lights.ctypes.data if lights.size else None, lights.shape[0],
+                                    fptr(bg)))
+    return scene
+
+
+def render_pt(scene: Scene, cam: dict, width, height, spp, max_path_length, seed, flags=0, stream=None):
+    ctx = scene.ctx
+    fb = ctx.empty(2 * 3 * width * height * 4)
+    job = PtJob(Camera.make(cam, width, height), spp, max_path_length, seed, flags)
+    check(lib.sgpu_render_pt(scene.handle, C.byref(job), fb.ptr, stream))
+    out = fb.download(np.float32, 2 * 3 * width * height).reshape(2, height, width, 3)
+    fb.free()
+    return out

@@ -39,6 +39,113 @@ def traversal_rays(tri9: np.ndarray, width: int = 1024, height: int = 1024, spp:
     return {k: np.concatenate([cam[k], bounce[k]]) for k in N.RAY_FIELDS}
 
 
+def icosphere(subdivisions: int, centre, radius):
+    """Icosahedron subdivided `subdivisions` times (20 * 4^n faces), vertices on the sphere,
+    faces wound so that (v1 - v0) x (v2 - v0) points outward. Returns (n, 9) float32."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(v, np.float64) / np.linalg.norm(v) for v in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdivisions):
+        cache = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    v = np.array(verts)
+    f = np.array(faces)
+    tri = v[f]  # (n, 3, 3)
+    ng = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    flip = np.sum(ng * tri.mean(axis=1), axis=1) < 0
+    tri[flip] = tri[flip][:, [0, 2, 1]]
+    tri = tri * radius + np.asarray(centre)
+    return tri.reshape(-1, 9).astype(np.float32)
+
+
+def material(shader, base_color, metallic=0.3, roughness=0.5, ior=1.5, **kw):
+    """A 22-float material record (raw MaterialResourceData values; defaults are Selas's
+    ImportMaterial defaults, Source/Core/BuildCommon/ImportMaterial.cpp:36-51)."""
+    m = np.zeros(22, np.float32)
+    m[0:3] = base_color
+    m[3:6] = kw.get("transmittance_color", (1.0, 1.0, 1.0))
+    m[6] = kw.get("sheen", 0.0)
+    m[7] = kw.get("sheen_tint", 0.0)
+    m[8] = kw.get("clearcoat", 0.0)
+    m[9] = kw.get("clearcoat_gloss", 0.0)
+    m[10] = metallic
+    m[11] = kw.get("spec_trans", 0.0)
+    m[12] = kw.get("diff_trans", 0.0)
+    m[13] = kw.get("flatness", 0.0)
+    m[14] = kw.get("anisotropic", 0.0)
+    m[16] = kw.get("specular_tint", 0.0)
+    m[17] = roughness
+    m[18] = kw.get("scatter_distance", 0.0)
+    m[19] = ior
+    m[20] = shader
+    m[21] = kw.get("flags", 0)
+    return m
+
+
+LIGHT_DTYPE = np.dtype([("type", np.uint32), ("position", np.float32, 3), ("direction", np.float32, 3),
+                        ("x", np.float32, 3), ("z", np.float32, 3), ("radiance", np.float32, 3)])
+
+
+def room_scene(seed: int = 1):
+    """Closed-room path tracing scene: the unit box [0,1]^3 as 12 axis-aligned quads (each of the
+    6 faces split in two halves; 24 triangles, normals inward), Lambert walls with albedo uniform in
+    [0.2, 0.8] per channel per quad, an icosphere subdivided 5 times (20,480 triangles, centre
+    (0.5, 0.3, 0.55), radius 0.25) with the Disney BSDF (solid, metallic 1, roughness 0.3, base
+    colour 0.8), one 0.3 x 0.3 quad light of radiance 15 facing down just below the ceiling (not
+    geometry, as in Selas), black background. Camera inside the room at z = 0.02 looking at +z."""
+    rng = np.random.default_rng(seed)
+    quads = []
+    for axis in range(3):
+        for side in (0.0, 1.0):
+            for half in (0, 1):
+                u_ax, v_ax = [a for a in range(3) if a != axis]
+                lo = np.zeros(3)
+                lo[axis] = side
+                lo[u_ax] = 0.5 * half
+                eu = np.zeros(3)
+                eu[u_ax] = 0.5
+                ev = np.zeros(3)
+                ev[v_ax] = 1.0
+                p0, p1, p2, p3 = lo, lo + eu, lo + eu + ev, lo + ev
+                tris = np.array([[p0, p1, p2], [p0, p2, p3]])
+                inward = np.zeros(3)
+                inward[axis] = 1.0 if side == 0.0 else -1.0
+                ng = np.cross(tris[:, 1] - tris[:, 0], tris[:, 2] - tris[:, 0])
+                flip = np.sum(ng * inward, axis=1) < 0
+                tris[flip] = tris[flip][:, [0, 2, 1]]
+                quads.append(tris.reshape(-1, 9).astype(np.float32))
+    sphere = icosphere(5, (0.5, 0.3, 0.55), 0.25)
+    geoms = quads + [sphere]
+    mats = [material(2, rng.uniform(0.2, 0.8, 3)) for _ in quads] + [material(1, (0.8, 0.8, 0.8), 1.0, 0.3)]
+    light = np.zeros(1, LIGHT_DTYPE)
+    light[0] = (0, (0.5, 0.999, 0.5), (0.0, -1.0, 0.0), (0.3, 0.0, 0.0), (0.0, 0.0, 0.3), (15.0, 15.0, 15.0))
+    return {
+        "geoms": geoms,
+        "materials": np.stack(mats),
+        "geom_material": np.arange(len(geoms), dtype=np.uint32),
+        "lights": light,
+        "background": np.zeros(3, np.float32),
+        "camera": {"position": (0.5, 0.5, 0.02), "look_at": (0.5, 0.45, 1.0), "up": (0.0, 1.0, 0.0), "fov": 70.0},
+    }
+
+
 def anyhit_tfar(t_closest: np.ndarray, prim: np.ndarray, seed: int = 5):
     """tfar for the any-hit pass: 0.5 * t_closest on a seeded half of the hit rays (expected
     unoccluded), +inf elsewhere (expected occluded iff the ray has a closest hit)."""
