@@ -223,6 +223,32 @@ typedef struct sgpu_pt_job {
  * averaged over spp. Synchronous on `stream`. */
 int sgpu_render_pt(sgpu_scene scene, const sgpu_pt_job* job, float* framebuffer, void* stream);
 
+typedef struct sgpu_vcm_job {
+    sgpu_camera camera;
+    uint32_t iterations;      /* VCM iterations: one light path and one camera path per pixel each */
+    uint32_t iteration_base;  /* global index of the first iteration (radius schedule and streams) */
+    uint32_t max_path_length; /* maxPathLength, 1..31 */
+    float base_radius;        /* merge radius of iteration 0 (vmSearchRadius) */
+    float radius_alpha;       /* radius shrink exponent (vmRadiusAlpha) */
+    uint32_t reserved;        /* must be 0 */
+    uint64_t seed;            /* PCG32 seed; path p of iteration i, pass s uses stream (i << 33 | p << 1 | s) */
+} sgpu_vcm_job;
+
+typedef struct sgpu_vcm_stats {
+    uint64_t rays;           /* closest-hit and shadow rays traced */
+    uint64_t light_vertices; /* light vertices stored, summed over iterations */
+    uint64_t samples;        /* camera paths = width x height x iterations */
+} sgpu_vcm_stats;
+
+/* Vertex connection and merging: replaces VCM::GenerateImage and its helpers
+ * (Source/Applications/Selas/Source/VCM.h:18, VCM.cpp:669-760, VCMCommon.cpp:37-113,
+ * VCMHashGrid.cpp:15-157) for quad and point lights (sgpu_scene_set_lights). framebuffer is a
+ * device array of height x width x 3 floats, the radiance averaged over the iterations. The merge
+ * radius of global iteration k is base_radius / (k + 1)^((1 - radius_alpha) / 2). stats may be
+ * NULL. Synchronous on `stream`. */
+int sgpu_render_vcm(sgpu_scene scene, const sgpu_vcm_job* job, float* framebuffer, sgpu_vcm_stats* stats,
+                    void* stream);
+
 /* ---- procedural inputs (host side, deterministic from a seed) ------------------------------
  * Gaussian-cluster triangle soup: n_clusters x tris_per_cluster triangles. Cluster centres are
  * uniform in [0,1]^3, triangle centroids normal around them with sigma, two edge lengths

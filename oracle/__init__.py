@@ -254,6 +254,63 @@ def render_pt(rs: RenderScene, cam: dict, width, height, spp, max_path_length, s
     return fb
 
 
+class _VcmJob(C.Structure):
+    _fields_ = [("cam10", C.c_float * 10), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("iterations", C.c_uint32), ("iteration_base", C.c_uint32), ("max_path_length", C.c_uint32),
+                ("base_radius", C.c_float), ("radius_alpha", C.c_float), ("seed", C.c_uint64)]
+
+
+class _HashGrid(C.Structure):
+    _fields_ = [("lo", C.c_float * 3), ("hi", C.c_float * 3), ("radius", C.c_float), ("radius_sq", C.c_float),
+                ("inv_cell", C.c_float), ("cell_count", C.c_uint32), ("count", C.c_uint32),
+                ("cell_ends", C.c_void_p), ("indices", C.c_void_p)]
+
+
+lib.orc_render_vcm.argtypes = [C.POINTER(_Scene), C.POINTER(_VcmJob), C.c_void_p, C.c_void_p, C.c_int]
+lib.orc_vcm_constants.argtypes = [C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_void_p]
+lib.orc_hash_grid_build.argtypes = [C.POINTER(_HashGrid), C.c_void_p, C.c_uint32, C.c_uint32, C.c_float]
+lib.orc_hash_grid_free.argtypes = [C.POINTER(_HashGrid)]
+lib.orc_hash_grid_query.argtypes = [C.POINTER(_HashGrid), C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]
+lib.orc_hash_grid_query.restype = C.c_uint32
+
+
+def render_vcm(rs: RenderScene, cam: dict, width, height, iterations, max_path_length, base_radius,
+               radius_alpha=0.75, seed=1, iteration_base=0, threads=0):
+    cam10 = list(cam["position"]) + list(cam["look_at"]) + list(cam["up"]) + [cam["fov"]]
+    job = _VcmJob((C.c_float * 10)(*cam10), width, height, iterations, iteration_base, max_path_length,
+                  base_radius, radius_alpha, seed)
+    fb = np.zeros((height, width, 3), np.float32)
+    stats = np.zeros(2, np.float32)
+    lib.orc_render_vcm(C.byref(rs.s), C.byref(job), fb.ctypes.data, stats.ctypes.data, _threads(threads))
+    return fb, {"light_vertices_per_iteration": float(stats[0]), "rays": float(stats[1])}
+
+
+def vcm_constants(vm_count, vc_count, base_radius, alpha, iteration):
+    out = np.zeros(6, np.float32)
+    lib.orc_vcm_constants(vm_count, vc_count, base_radius, alpha, iteration, out.ctypes.data)
+    return {"radius": out[0], "radius_sq": out[1], "vm_normalization": out[2], "vm_weight": out[3],
+            "vc_weight": out[4]}
+
+
+def hash_grid(points, cell_count, radius, queries, max_found=512):
+    """BuildHashGrid + SearchHashGrid restated: cell ends, point order, box and per-query finds."""
+    pts = _f(points)
+    g = _HashGrid()
+    lib.orc_hash_grid_build(C.byref(g), pts.ctypes.data, pts.shape[0], cell_count, radius)
+    ends = np.ctypeslib.as_array(C.cast(g.cell_ends, C.POINTER(C.c_uint32)), (cell_count,)).copy()
+    order = np.ctypeslib.as_array(C.cast(g.indices, C.POINTER(C.c_uint32)), (max(1, pts.shape[0]),)).copy()
+    box = np.array(list(g.lo) + list(g.hi), np.float32)
+    q = _f(queries)
+    found = np.zeros((q.shape[0], max_found), np.uint32)
+    counts = np.zeros(q.shape[0], np.uint32)
+    for i in range(q.shape[0]):
+        counts[i] = lib.orc_hash_grid_query(C.byref(g), pts.ctypes.data, q[i].ctypes.data, found[i].ctypes.data,
+                                            max_found)
+    lib.orc_hash_grid_free(C.byref(g))
+    return {"cell_ends": ends, "cell_indices": order[:pts.shape[0]], "box": box, "found": found,
+            "found_count": counts}
+
+
 class _Uniforms(C.Structure):
     _fields_ = [("values", C.c_void_p), ("count", C.c_uint32), ("pos", C.c_uint32), ("pcg_state", C.c_void_p),
                 ("pcg_inc", C.c_uint64)]

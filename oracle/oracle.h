@@ -138,6 +138,36 @@ void orc_nee(orc_uniforms* u, const orc_light* lights, uint32_t n_lights, const 
 /* fb: 2 x height x width x 3 floats (layer 0 first-bounce reflectance, layer 1 radiance) */
 void orc_render_pt(const orc_scene* sc, const orc_pt_job* job, float* fb, int threads);
 
+/* VCM (orc_vcm.c) */
+typedef struct {
+    float lo[3], hi[3];
+    float radius, radius_sq, inv_cell;
+    uint32_t cell_count, count;
+    uint32_t* cell_ends; /* exclusive end of each cell's range */
+    uint32_t* indices;   /* point index per sorted slot */
+} orc_hash_grid;
+
+typedef struct {
+    float cam10[10];
+    uint32_t width, height;
+    uint32_t iterations;     /* VCM iterations (one light path and one camera path per pixel each) */
+    uint32_t iteration_base; /* global index of the first iteration (radius schedule, streams) */
+    uint32_t max_path_length;
+    float base_radius, radius_alpha;
+    uint64_t seed;
+} orc_vcm_job;
+
+void orc_vcm_constants(uint32_t vm_count, uint32_t vc_count, float base_radius, float alpha, float iteration,
+                       float* out6);
+void orc_hash_grid_build(orc_hash_grid* g, const float* points, uint32_t count, uint32_t cell_count, float radius);
+void orc_hash_grid_free(orc_hash_grid* g);
+uint32_t orc_hash_grid_query(const orc_hash_grid* g, const float* points, const float* q, uint32_t* found,
+                             uint32_t max_found);
+/* fb_out: height x width x 3 floats (radiance averaged over iterations); stats_out (2 floats, may be
+ * NULL): light vertices per iteration, rays traced. Paths of one pass run on `threads` threads; the
+ * result does not depend on the thread count. */
+void orc_render_vcm(const orc_scene* sc, const orc_vcm_job* job, float* fb_out, float* stats_out, int threads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,756 @@
+/*
+ * ORACLE -- test infrastructure only (see orc_trace.c for the rules). Compiled with
+ * -ffp-contract=off.
+ *
+ * CPU restatement of Selas's vertex connection and merging integrator
+ * (Source/Applications/Selas/Source/VCM.cpp, whose code is commented out and refers to symbols that
+ * no longer exist, so it is restated rather than run), with its live helpers:
+ *   CalculateIterationConstants / SearchRadius   VCMCommon.cpp:91-113
+ *   GenerateCameraSample                         VCMCommon.cpp:66-88
+ *   BuildHashGrid / SearchHashGrid               VCMHashGrid.cpp:15-157 (restated exactly:
+ *                                                pinned bit-for-bit by golden vectors)
+ *   EvaluateLightPaths / EvaluateCameraPaths     VCM.cpp:502-666
+ *   ConnectLightPathToCamera                     VCM.cpp:190-242
+ *   ConnectCameraPathToLight                     VCM.cpp:245-290
+ *   ConnectPathVertices                          VCM.cpp:293-359
+ *   MergeVertices                                VCM.cpp:362-409
+ *   SampleBsdfScattering                         VCM.cpp:412-446
+ * The MIS recurrences (dVCM, dVC, dVM) are those of the reference, which follow Georgiev et al.,
+ * "Implementing Vertex Connection and Merging" (2012). Where the commented code cannot be run as
+ * written, or is inconsistent with that report, the report's form is used (each is listed in the
+ * repository's design notes): WorldToImage is defined by inverting ImageToWorldDirection; the
+ * virtual image plane distance is W / (2 tan(hfov/2)); the light-tracing image-to-surface factor
+ * includes |cos| / d^2; BSDF values are Selas's (f |cos l|), so connections and merges remove the
+ * cosines the report does not have; throughput is divided by the Russian-roulette probability;
+ * lights are quad and point lights (Selas's VCM emits only from the IBL) and, as in Selas's path
+ * tracer, they are not intersectable, so the technique "camera path hits the light" does not exist
+ * and its MIS terms are zero (dVC = dVM = 0 at the light, no light weight in NEE) -- exactly the
+ * treatment the report gives delta lights; NEE samples quad lights by area (a point-independent
+ * pdf, as the MIS weights require). Random numbers: one PCG32 stream per (iteration, path, pass).
+ * Also the timed CPU path of the VCM benchmark ("port" baseline).
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+static const float kPi = 3.141592653589793f;
+
+static inline ov3 v3(float x, float y, float z)
+{
+    ov3 r = {x, y, z};
+    return r;
+}
+static inline ov3 add(ov3 a, ov3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline ov3 sub(ov3 a, ov3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline ov3 mul(ov3 a, ov3 b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline ov3 scl(ov3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+static inline ov3 neg(ov3 a) { return v3(-a.x, -a.y, -a.z); }
+static inline float dot(ov3 a, ov3 b) { return (a.x * b.x) + (a.y * b.y) + (a.z * b.z); }
+static inline ov3 cross(ov3 a, ov3 b)
+{
+    return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline float length(ov3 a) { return sqrtf(dot(a, a)); }
+static inline ov3 normalize(ov3 a)
+{
+    float inv = 1.f / length(a);
+    return v3(a.x * inv, a.y * inv, a.z * inv);
+}
+static inline float absf(float x) { return (x < 0) ? -x : x; }
+static inline float maxf(float a, float b) { return (a > b) ? a : b; }
+static inline float saturate(float x) { return x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x); }
+static inline ov3 ld3(const float* p) { return v3(p[0], p[1], p[2]); }
+static inline int is_zero(ov3 a) { return a.x == 0 && a.y == 0 && a.z == 0; }
+
+/* ---- iteration constants (VCMCommon.cpp:91-113) ------------------------------------------------ */
+
+void orc_vcm_constants(uint32_t vm_count, uint32_t vc_count, float base_radius, float alpha, float iteration,
+                       float* out6)
+{
+    float r = base_radius / powf(iteration, 0.5f * (1.0f - alpha));
+    float r2 = r * r;
+    out6[0] = r;
+    out6[1] = r2;
+    out6[2] = 1.0f / (kPi * r2 * vm_count);       /* vmNormalization */
+    out6[3] = kPi * r2 * vm_count / vc_count;      /* vmWeight */
+    out6[4] = vc_count / (kPi * r2 * vm_count);    /* vcWeight */
+    out6[5] = 0.0f;
+}
+
+/* ---- hash grid (VCMHashGrid.cpp, restated exactly) ------------------------------------------- */
+
+static uint32_t cell_hash(int32_t x, int32_t y, int32_t z, uint32_t cell_count)
+{
+    uint32_t ux = (uint32_t)x, uy = (uint32_t)y, uz = (uint32_t)z;
+    return (uint32_t)(int)(((ux * 73856093u) ^ (uy * 19349663u) ^ (uz * 83492791u)) % cell_count);
+}
+
+static uint32_t cell_of(const orc_hash_grid* g, const float* p)
+{
+    int32_t x = (int32_t)floorf(g->inv_cell * (p[0] - g->lo[0]));
+    int32_t y = (int32_t)floorf(g->inv_cell * (p[1] - g->lo[1]));
+    int32_t z = (int32_t)floorf(g->inv_cell * (p[2] - g->lo[2]));
+    return cell_hash(x, y, z, g->cell_count);
+}
+
+void orc_hash_grid_build(orc_hash_grid* g, const float* points, uint32_t count, uint32_t cell_count, float radius)
+{
+    g->radius = radius;
+    g->radius_sq = radius * radius;
+    float cell = 2.0f * radius;
+    g->inv_cell = 1.0f / cell;
+    g->cell_count = cell_count;
+    g->count = count;
+    g->cell_ends = (uint32_t*)calloc(cell_count, sizeof(uint32_t));
+    g->indices = (uint32_t*)malloc(sizeof(uint32_t) * (count ? count : 1));
+    for (int k = 0; k < 3; ++k) {
+        g->lo[k] = 3.402823466e+38F;
+        g->hi[k] = -3.402823466e+38F;
+    }
+    for (uint32_t i = 0; i < count; ++i)
+        for (int k = 0; k < 3; ++k) {
+            float v = points[3 * i + k];
+            /* IncludePosition (AxisAlignedBox.cpp:32): Min / Max */
+            g->lo[k] = v < g->lo[k] ? v : g->lo[k];
+            g->hi[k] = v > g->hi[k] ? v : g->hi[k];
+        }
+    for (uint32_t i = 0; i < count; ++i) ++g->cell_ends[cell_of(g, points + 3 * i)];
+    uint32_t sum = 0;
+    for (uint32_t c = 0; c < cell_count; ++c) {
+        uint32_t n = g->cell_ends[c];
+        g->cell_ends[c] = sum;
+        sum += n;
+    }
+    for (uint32_t i = 0; i < count; ++i) {
+        uint32_t c = cell_of(g, points + 3 * i);
+        g->indices[g->cell_ends[c]++] = i;
+    }
+}
+
+void orc_hash_grid_free(orc_hash_grid* g)
+{
+    free(g->cell_ends);
+    free(g->indices);
+    g->cell_ends = NULL;
+    g->indices = NULL;
+}
+
+/* Visits the points within radius of q in the reference's order (8 cells, duplicates kept when two
+ * of the cells hash to the same bucket). Returns the number found; writes up to max_found indices. */
+uint32_t orc_hash_grid_query(const orc_hash_grid* g, const float* points, const float* q, uint32_t* found,
+                             uint32_t max_found)
+{
+    float dmin[3], dmax[3];
+    for (int k = 0; k < 3; ++k) {
+        dmin[k] = q[k] - g->lo[k];
+        dmax[k] = g->hi[k] - q[k];
+    }
+    if (dmin[0] < 0.0f || dmin[1] < 0.0f || dmin[2] < 0.0f) return 0;
+    if (dmax[0] < 0.0f || dmax[1] < 0.0f || dmax[2] < 0.0f) return 0;
+    int32_t mn[3];
+    for (int k = 0; k < 3; ++k) {
+        float cp = g->inv_cell * dmin[k];
+        float cf = floorf(cp);
+        float fr = cp - cf;
+        mn[k] = (int32_t)cf;
+        if (fr < 0.5f) --mn[k];
+    }
+    uint32_t n = 0;
+    for (int z = 0; z <= 1; ++z)
+        for (int y = 0; y <= 1; ++y)
+            for (int x = 0; x <= 1; ++x) {
+                uint32_t c = cell_hash(mn[0] + x, mn[1] + y, mn[2] + z, g->cell_count);
+                uint32_t b = c == 0 ? 0 : g->cell_ends[c - 1], e = g->cell_ends[c];
+                for (uint32_t s = b; s < e; ++s) {
+                    uint32_t pi = g->indices[s];
+                    const float* p = points + 3 * (size_t)pi;
+                    ov3 d = sub(ld3(q), ld3(p));
+                    if (dot(d, d) <= g->radius_sq) {
+                        if (n < max_found) found[n] = pi;
+                        ++n;
+                    }
+                }
+            }
+    return n;
+}
+
+/* ---- camera projection ---------------------------------------------------------------------- */
+
+/* WorldToImage: the inverse of ImageToWorldDirection (Camera.cpp:56-61); pixel coordinates as
+ * floats (x right, y down), returns 0 when the point is behind the camera. */
+static int world_to_image(const orc_camera* c, ov3 p, float* px, float* py)
+{
+    ov3 d = sub(p, c->pos);
+    float zc = dot(d, c->z);
+    if (!(zc > 0.0f)) return 0;
+    float cx = -dot(d, c->x) / (dot(c->x, c->x) * zc);
+    float cy = dot(d, c->y) / (dot(c->y, c->y) * zc);
+    *px = (cx + 1.0f) * 0.5f * (float)c->width;
+    *py = (1.0f - cy) * 0.5f * (float)c->height;
+    return 1;
+}
+
+/* ---- light emission and direct sampling ------------------------------------------------------- */
+
+typedef struct {
+    ov3 position, direction, throughput;
+    float dvcm, dvc, dvm;
+} emit_state;
+
+/* Light subpath start (GenerateLightSample, VCMCommon.cpp:37-63, for quad and point lights).
+ * Uses 5 uniforms: light pick, position (2, quad only), direction (2). */
+static void emit_light(orc_uniforms* u, const orc_light* lights, uint32_t n_lights, float vc_weight, emit_state* s)
+{
+    float p_pick = 1.0f / n_lights;
+    uint32_t idx = (uint32_t)(orc_uniform(u) * n_lights);
+    if (idx >= n_lights) idx = n_lights - 1;
+    const orc_light* L = &lights[idx];
+    float emission_pdf_w, direct_pdf_a;
+    ov3 radiance;
+    if (L->type == 1) {
+        float r0 = orc_uniform(u), r1 = orc_uniform(u);
+        /* uniform sphere (CSampler::UniformSphere, Sampler.cpp) */
+        float uu = 2.0f * r1 - 1.0f;
+        float norm = sqrtf(maxf(0.0f, 1.0f - uu * uu));
+        float theta = (kPi * 2.0f) * r0;
+        s->direction = v3(norm * cosf(theta), uu, norm * sinf(theta));
+        s->position = ld3(L->position);
+        emission_pdf_w = (1.0f / (4.0f * kPi)) * p_pick;
+        direct_pdf_a = 1.0f * p_pick;
+        radiance = ld3(L->radiance);
+    } else {
+        ov3 ex = ld3(L->x), ez = ld3(L->z);
+        float a = orc_uniform(u), b = orc_uniform(u);
+        s->position = add(add(sub(sub(ld3(L->position), scl(ex, 0.5f)), scl(ez, 0.5f)), scl(ex, a)), scl(ez, b));
+        float r0 = orc_uniform(u), r1 = orc_uniform(u);
+        float r = sqrtf(r0);
+        float th = (kPi * 2.0f) * r1;
+        ov3 local = v3(r * cosf(th), sqrtf(maxf(0.0f, 1 - r0)), r * sinf(th));
+        float t[3], bb[3];
+        ov3 n = ld3(L->direction);
+        float nn[3] = {n.x, n.y, n.z};
+        orc_make_frame(nn, t, bb);
+        s->direction = normalize(add(add(scl(ld3(t), local.x), scl(n, local.y)), scl(ld3(bb), local.z)));
+        float cos_l = local.y;
+        float area = length(ex) * length(ez);
+        float pdf_a = 1.0f / area;
+        emission_pdf_w = pdf_a * (cos_l / kPi) * p_pick;
+        direct_pdf_a = pdf_a * p_pick;
+        radiance = scl(ld3(L->radiance), cos_l);
+    }
+    (void)vc_weight;
+    s->throughput = scl(radiance, 1.0f / emission_pdf_w);
+    s->dvcm = direct_pdf_a / emission_pdf_w;
+    s->dvc = 0.0f; /* lights are not intersectable: no "hit the light" technique */
+    s->dvm = 0.0f;
+}
+
+/* Direct light sample for NEE at x (ConnectCameraPathToLight's light part). Uses 3 uniforms.
+ * out: dir, radiance, distance, directPdfW (incl. pick), emissionPdfW (incl. pick), cosAtLight */
+static int direct_light(orc_uniforms* u, const orc_light* lights, uint32_t n_lights, ov3 x, ov3* dir, ov3* radiance,
+                        float* dist, float* direct_pdf_w, float* emission_pdf_w, float* cos_at_light)
+{
+    float p_pick = 1.0f / n_lights;
+    uint32_t idx = (uint32_t)(orc_uniform(u) * n_lights);
+    if (idx >= n_lights) idx = n_lights - 1;
+    const orc_light* L = &lights[idx];
+    float a = orc_uniform(u), b = orc_uniform(u);
+    if (L->type == 1) {
+        ov3 d = sub(ld3(L->position), x);
+        float d2 = dot(d, d);
+        *dist = sqrtf(d2);
+        *dir = scl(d, 1.0f / *dist);
+        *radiance = ld3(L->radiance);
+        *direct_pdf_w = d2 * p_pick;
+        *emission_pdf_w = (1.0f / (4.0f * kPi)) * p_pick;
+        *cos_at_light = 1.0f;
+        return 1;
+    }
+    ov3 ex = ld3(L->x), ez = ld3(L->z);
+    ov3 y = add(add(sub(sub(ld3(L->position), scl(ex, 0.5f)), scl(ez, 0.5f)), scl(ex, a)), scl(ez, b));
+    ov3 d = sub(y, x);
+    float d2 = dot(d, d);
+    *dist = sqrtf(d2);
+    *dir = scl(d, 1.0f / *dist);
+    float cl = dot(ld3(L->direction), neg(*dir));
+    if (!(cl > 0.0f)) return 0;
+    float pdf_a = 1.0f / (length(ex) * length(ez));
+    *radiance = ld3(L->radiance);
+    *direct_pdf_w = pdf_a * d2 / cl * p_pick;
+    *emission_pdf_w = pdf_a * (cl / kPi) * p_pick;
+    *cos_at_light = cl;
+    return 1;
+}
+
+/* ---- per-vertex surfaces ------------------------------------------------------------------------ */
+
+typedef struct {
+    ov3 pos, n, wfix; /* wfix: direction toward the previous vertex of the subpath */
+    ov3 throughput;
+    float dvcm, dvc, dvm;
+    uint32_t path_length;
+    uint32_t geom;
+    float error;
+} vcm_vertex;
+
+typedef struct {
+    const orc_scene* sc;
+    const orc_vcm_job* job;
+    orc_camera cam;
+    float vm_weight, vc_weight, vm_norm, radius;
+    uint32_t* stack;
+} vcm_ctx;
+
+static int hit_surface(const vcm_ctx* c, const float* o, const float* d, float tnear, orc_surface* s, float* pos,
+                       float* err, uint32_t* geom_out)
+{
+    float t, hu, hv;
+    uint32_t prim = orc_bvh_closest1(c->sc->bvh, c->stack, o, d, tnear, 3.402823466e+38F, &t, &hu, &hv);
+    if (prim == 0xFFFFFFFFu) return 0;
+    uint32_t geom = c->sc->tri_geom[prim];
+    const float* tp = c->sc->tri9 + (size_t)prim * 9;
+    float e1[3] = {tp[3] - tp[0], tp[4] - tp[1], tp[5] - tp[2]};
+    float e2[3] = {tp[6] - tp[0], tp[7] - tp[1], tp[8] - tp[2]};
+    ov3 ngv = cross(ld3(e1), ld3(e2));
+    float ng[3] = {ngv.x, ngv.y, ngv.z};
+    const float* mat = c->sc->materials + 22 * (size_t)c->sc->geom_material[geom];
+    orc_surface_from_hit(o, d, t, ng, mat, NULL, s, pos, err);
+    *geom_out = geom;
+    return 1;
+}
+
+static void surface_of_vertex(const vcm_ctx* c, const vcm_vertex* v, orc_surface* s)
+{
+    /* the stored vertex keeps what CalculateSurfaceParams derives: position, normal, material */
+    const float* mat = c->sc->materials + 22 * (size_t)c->sc->geom_material[v->geom];
+    float block[22];
+    memcpy(block, mat, sizeof(block));
+    block[0] = powf(mat[0], 2.2f);
+    block[1] = powf(mat[1], 2.2f);
+    block[2] = powf(mat[2], 2.2f);
+    block[10] = saturate(mat[10]);
+    block[11] = saturate(mat[11]);
+    block[12] = mat[12] * 0.5f;
+    uint32_t flags = (uint32_t)mat[21];
+    block[15] = ((flags & 1u) && dot(v->wfix, v->n) < 0.0f) ? mat[19] : 1.0f / mat[19];
+    float nn[3] = {v->n.x, v->n.y, v->n.z};
+    orc_surface_from_block(block, nn, s);
+}
+
+static void eval_bsdf(const orc_surface* s, ov3 v, ov3 l, ov3* f, float* fwd, float* rev)
+{
+    float vv[3] = {v.x, v.y, v.z}, ll[3] = {l.x, l.y, l.z}, out[5];
+    if (s->shader == 2)
+        orc_lambert_eval(s, vv, ll, out);
+    else
+        orc_disney_eval(s, vv, ll, s->shader == 0, out);
+    *f = ld3(out);
+    *fwd = out[3];
+    *rev = out[4];
+}
+
+static float cont_prob(const orc_surface* s)
+{
+    return saturate(maxf(maxf(s->base_color.x, s->base_color.y), s->base_color.z));
+}
+
+/* SampleBsdfScattering (VCM.cpp:412-446) with the RR-compensated throughput. Returns 0 to stop. */
+static int scatter(orc_uniforms* u, const vcm_ctx* c, const orc_surface* s, ov3 pos, ov3* dir, ov3* thr, float* dvcm,
+                   float* dvc, float* dvm, uint32_t* path_length)
+{
+    float out[14];
+    memset(out, 0, sizeof(out));
+    float v[3] = {-dir->x, -dir->y, -dir->z};
+    int ok = s->shader == 2 ? orc_lambert_sample(u, s, v, out) : orc_disney_sample(u, s, v, s->shader == 0, out);
+    if (!ok) return 0;
+    ov3 refl = ld3(out + 2);
+    if (is_zero(refl)) return 0;
+    float cp = cont_prob(s);
+    if (orc_uniform(u) > cp) return 0;
+    float fwd = out[8] * cp, rev = out[9] * cp;
+    ov3 wi = ld3(out + 5);
+    float cos_b = absf(dot(wi, s->n));
+    (void)pos;
+    *thr = scl(mul(*thr, refl), 1.0f / cp);
+    float k = cos_b / fwd;
+    float nvc = k * (*dvc * rev + *dvcm + c->vm_weight);
+    float nvm = k * (*dvm * rev + *dvcm * c->vc_weight + 1.0f);
+    *dvc = nvc;
+    *dvm = nvm;
+    *dvcm = 1.0f / fwd;
+    *dir = wi;
+    ++*path_length;
+    return 1;
+}
+
+/* ---- the integrator ------------------------------------------------------------------------------ */
+
+static uint64_t stream_id(uint32_t iteration, uint32_t path, uint32_t pass)
+{
+    return ((uint64_t)iteration << 33) | ((uint64_t)path << 1) | pass;
+}
+
+/* Per-iteration work shared by the threads. Each light path writes its vertices and camera splats
+ * into slots of its own; they are then gathered in path order, so the lists (and the sums) are
+ * those of the sequential loop of EvaluateLightPaths whatever the thread count. */
+typedef struct {
+    vcm_ctx c;
+    float vipd;
+    uint32_t W, H, N, L, mlv, iteration;
+    vcm_vertex* slot_verts; /* N x mlv */
+    uint32_t* slot_nverts;
+    ov3* splat_c;           /* N x mlv */
+    uint32_t* splat_pix;
+    uint32_t* slot_nsplat;
+    uint32_t* path_rays;    /* rays traced by path i in the current pass */
+    const vcm_vertex* verts;
+    const uint32_t* path_ends;
+    const float* vpos;
+    uint32_t nv;
+    const orc_hash_grid* grid;
+    float* iter_fb;
+} vcm_iter;
+
+static void light_paths(void* ctx, uint32_t b, uint32_t e)
+{
+    vcm_iter* I = (vcm_iter*)ctx;
+    vcm_ctx c = I->c;
+    const orc_scene* sc = c.sc;
+    const uint32_t W = I->W, H = I->H, N = I->N, L = I->L;
+    c.stack = (uint32_t*)malloc(sizeof(uint32_t) * orc_bvh_stack_size(sc->bvh));
+    for (uint32_t i = b; i < e; ++i) {
+        uint32_t nvert = 0, nsplat = 0, nrays = 0;
+        vcm_vertex* mine = I->slot_verts + (size_t)i * I->mlv;
+        uint64_t st, inc, sid = stream_id(I->iteration, i, 0);
+        orc_pcg32_seed(c.job->seed ^ (sid * 0x9E3779B97F4A7C15ull), sid, &st, &inc);
+        orc_uniforms u = {NULL, 0, 0, &st, inc};
+        if (sc->n_lights > 0) {
+            emit_state es;
+            emit_light(&u, sc->lights, sc->n_lights, c.vc_weight, &es);
+            ov3 pos = es.position, dir = es.direction, thr = es.throughput;
+            float dvcm = es.dvcm, dvc = es.dvc, dvm = es.dvm;
+            uint32_t plen = 1;
+            while (plen + 2 < L) {
+                float o[3] = {pos.x, pos.y, pos.z}, d[3] = {dir.x, dir.y, dir.z};
+                orc_surface s;
+                float hp[3], err;
+                uint32_t geom;
+                ++nrays;
+                if (!hit_surface(&c, o, d, 0.00001f, &s, hp, &err, &geom)) break;
+                ov3 x = ld3(hp);
+                ov3 view = neg(dir);
+                float d2 = dot(sub(pos, x), sub(pos, x));
+                float cos_v = absf(dot(s.n, view));
+                /* finite lights: the first segment is converted to area measure too */
+                dvcm *= d2;
+                dvcm *= (1.0f / cos_v);
+                dvc *= (1.0f / cos_v);
+                dvm *= (1.0f / cos_v);
+                vcm_vertex* vx = &mine[nvert++];
+                vx->pos = x;
+                vx->n = s.n;
+                vx->wfix = view;
+                vx->throughput = thr;
+                vx->dvcm = dvcm;
+                vx->dvc = dvc;
+                vx->dvm = dvm;
+                vx->path_length = plen;
+                vx->geom = geom;
+                vx->error = err;
+                /* ConnectLightPathToCamera (VCM.cpp:190-242) */
+                ov3 to_pos = sub(x, c.cam.pos);
+                float px, py;
+                if (dot(c.cam.forward, to_pos) > 0.0f && world_to_image(&c.cam, x, &px, &py) && px >= 0.0f &&
+                    py >= 0.0f && px < (float)W && py < (float)H) {
+                    float dist = length(to_pos);
+                    ov3 to_cam = scl(to_pos, -1.0f / dist);
+                    float cos_cam = dot(c.cam.forward, neg(to_cam));
+                    ov3 f;
+                    float fwd, rev;
+                    eval_bsdf(&s, view, to_cam, &f, &fwd, &rev);
+                    if (!is_zero(f)) {
+                        float cp = cont_prob(&s);
+                        rev *= cp;
+                        float img_dist = I->vipd / cos_cam;
+                        float img_to_solid = img_dist * img_dist / cos_cam;
+                        float cos_to_cam = absf(dot(s.n, to_cam));
+                        float img_to_surface = img_to_solid * cos_to_cam / (dist * dist);
+                        float w_light = (img_to_surface / (float)N) * (c.vm_weight + dvcm + dvc * rev);
+                        float mis = 1.0f / (w_light + 1.0f);
+                        /* f carries |cos|: contribution = mis thr f img_to_solid / (N d^2) */
+                        ov3 contrib = scl(mul(thr, f), mis * img_to_solid / ((float)N * dist * dist));
+                        int ix = (int)floorf(px), iy = (int)floorf(py);
+                        if (!is_zero(contrib) && ix >= 0 && iy >= 0 && ix < (int)W && iy < (int)H) {
+                            float so[3];
+                            float tc[3] = {to_cam.x, to_cam.y, to_cam.z};
+                            orc_offset_origin(&s, hp, err, tc, 0.1f, so, NULL);
+                            ++nrays;
+                            if (!orc_bvh_occluded1(sc->bvh, c.stack, so, tc, err, dist)) {
+                                I->splat_c[(size_t)i * I->mlv + nsplat] = contrib;
+                                I->splat_pix[(size_t)i * I->mlv + nsplat] = (uint32_t)iy * W + (uint32_t)ix;
+                                ++nsplat;
+                            }
+                        }
+                    }
+                }
+                if (!scatter(&u, &c, &s, x, &dir, &thr, &dvcm, &dvc, &dvm, &plen)) break;
+                pos = x;
+            }
+        }
+        I->slot_nverts[i] = nvert;
+        I->slot_nsplat[i] = nsplat;
+        I->path_rays[i] = nrays;
+    }
+    free(c.stack);
+}
+
+static void camera_paths(void* ctx, uint32_t b, uint32_t e)
+{
+    vcm_iter* I = (vcm_iter*)ctx;
+    vcm_ctx c = I->c;
+    const orc_scene* sc = c.sc;
+    const uint32_t W = I->W, N = I->N, L = I->L;
+    c.stack = (uint32_t*)malloc(sizeof(uint32_t) * orc_bvh_stack_size(sc->bvh));
+    uint32_t* found = (uint32_t*)malloc(sizeof(uint32_t) * 65536);
+    for (uint32_t i = b; i < e; ++i) {
+        uint32_t nrays = 0;
+        uint64_t st, inc, sid = stream_id(I->iteration, i, 1);
+        orc_pcg32_seed(c.job->seed ^ (sid * 0x9E3779B97F4A7C15ull), sid, &st, &inc);
+        orc_uniforms u = {NULL, 0, 0, &st, inc};
+        uint32_t px = i % W, py = i / W;
+        /* GenerateCameraSample (VCMCommon.cpp:66-88) */
+        float jx = (float)px + orc_uniform(&u), jy = (float)py + orc_uniform(&u);
+        float dd[3];
+        orc_camera_dir(&c.cam, jx, jy, dd);
+        ov3 dir = ld3(dd), pos = c.cam.pos, thr = v3(1, 1, 1), color = v3(0, 0, 0);
+        float cos_cam = dot(c.cam.forward, dir);
+        float img_dist = I->vipd / cos_cam;
+        float cam_pdf_w = img_dist * img_dist / cos_cam;
+        float dvcm = (float)N * (1.0f / cam_pdf_w), dvc = 0.0f, dvm = 0.0f;
+        uint32_t plen = 1;
+        while (plen < L) {
+            float o[3] = {pos.x, pos.y, pos.z}, d[3] = {dir.x, dir.y, dir.z};
+            orc_surface s;
+            float hp[3], err;
+            uint32_t geom;
+            ++nrays;
+            if (!hit_surface(&c, o, d, 0.00001f, &s, hp, &err, &geom)) {
+                if (plen == 1) color = add(color, mul(thr, ld3(sc->background)));
+                break;
+            }
+            ov3 x = ld3(hp);
+            ov3 view = neg(dir);
+            float d2 = dot(sub(pos, x), sub(pos, x));
+            float cos_v = absf(dot(s.n, view));
+            dvcm *= d2;
+            dvcm /= cos_v;
+            dvc /= cos_v;
+            dvm /= cos_v;
+            /* NEE (ConnectCameraPathToLight, VCM.cpp:245-290) */
+            if (plen + 1 < L && sc->n_lights > 0) {
+                ov3 ldir, rad;
+                float ldist, dpdf, epdf, cl;
+                if (direct_light(&u, sc->lights, sc->n_lights, x, &ldir, &rad, &ldist, &dpdf, &epdf, &cl)) {
+                    ov3 f;
+                    float fwd, rev;
+                    eval_bsdf(&s, view, ldir, &f, &fwd, &rev);
+                    if (!is_zero(f)) {
+                        float cp = cont_prob(&s);
+                        rev *= cp;
+                        float cos_s = absf(dot(s.n, ldir));
+                        float w_cam = (epdf * cos_s / (dpdf * cl)) * (c.vm_weight + dvcm + dvc * rev);
+                        float mis = 1.0f / (1.0f + w_cam);
+                        ov3 contrib = scl(mul(f, rad), mis / dpdf);
+                        if (!is_zero(contrib)) {
+                            float so[3], ld[3] = {ldir.x, ldir.y, ldir.z};
+                            orc_offset_origin(&s, hp, err, ld, 0.1f, so, NULL);
+                            ++nrays;
+                            if (!orc_bvh_occluded1(sc->bvh, c.stack, so, ld, err, ldist))
+                                color = add(color, mul(thr, contrib));
+                        }
+                    }
+                }
+            }
+            /* VC to the vertices of light path i (vcCount = 1, VCM.cpp:619-636) */
+            {
+                uint32_t vb = i == 0 ? 0 : I->path_ends[i - 1], ve = I->path_ends[i];
+                for (uint32_t k = vb; k < ve; ++k) {
+                    const vcm_vertex* lv = &I->verts[k];
+                    if (lv->path_length + 1 + plen > L) break;
+                    ov3 dv = sub(lv->pos, x);
+                    float dist2 = dot(dv, dv);
+                    float dist = sqrtf(dist2);
+                    ov3 cdir = scl(dv, 1.0f / dist);
+                    ov3 fc;
+                    float cfwd, crev;
+                    eval_bsdf(&s, view, cdir, &fc, &cfwd, &crev);
+                    if (is_zero(fc)) continue;
+                    orc_surface ls;
+                    surface_of_vertex(&c, lv, &ls);
+                    ov3 fl;
+                    float lfwd, lrev;
+                    eval_bsdf(&ls, lv->wfix, neg(cdir), &fl, &lfwd, &lrev);
+                    if (is_zero(fl)) continue;
+                    float cos_c = absf(dot(cdir, s.n));
+                    float cos_l = absf(dot(neg(cdir), lv->n));
+                    float ccp = cont_prob(&s), lcp = cont_prob(&ls);
+                    cfwd *= ccp;
+                    crev *= ccp;
+                    lfwd *= lcp;
+                    lrev *= lcp;
+                    float c_pdf_a = cfwd * cos_l / dist2;
+                    float l_pdf_a = lfwd * cos_c / dist2;
+                    float w_light = c_pdf_a * (c.vm_weight + lv->dvcm + lv->dvc * lrev);
+                    float w_cam = l_pdf_a * (c.vm_weight + dvcm + dvc * crev);
+                    float mis = 1.0f / (w_light + 1.0f + w_cam);
+                    ov3 contrib = scl(mul(fc, fl), mis / dist2);
+                    if (is_zero(contrib)) continue;
+                    float sb;
+                    float so[3], cd[3] = {cdir.x, cdir.y, cdir.z};
+                    orc_offset_origin(&s, hp, err, cd, 0.1f, so, &sb);
+                    ++nrays;
+                    if (!orc_bvh_occluded1(sc->bvh, c.stack, so, cd, err, dist - 16.0f * absf(sb)))
+                        color = add(color, mul(mul(thr, lv->throughput), contrib));
+                }
+            }
+            /* VM (SearchHashGrid + MergeVertices, VCM.cpp:362-409, 638-655) */
+            if (I->nv > 0) {
+                uint32_t nf = orc_hash_grid_query(I->grid, I->vpos, hp, found, 65536);
+                if (nf > 65536) nf = 65536;
+                ov3 acc_vm = v3(0, 0, 0);
+                float ccp = cont_prob(&s);
+                for (uint32_t q = 0; q < nf; ++q) {
+                    const vcm_vertex* lv = &I->verts[found[q]];
+                    if (plen + lv->path_length > L) continue;
+                    orc_surface ls;
+                    surface_of_vertex(&c, lv, &ls);
+                    /* the reference's transparency rule (VCM.cpp:381-383) */
+                    if ((s.flags & 1u) != (ls.flags & 1u)) continue;
+                    ov3 f;
+                    float fwd, rev;
+                    eval_bsdf(&s, view, lv->wfix, &f, &fwd, &rev);
+                    if (is_zero(f)) continue;
+                    float cos_l = absf(dot(s.n, lv->wfix));
+                    fwd *= ccp;
+                    rev *= cont_prob(&ls);
+                    float w_light = lv->dvcm * c.vc_weight + lv->dvm * fwd;
+                    float w_cam = dvcm * c.vc_weight + dvm * rev;
+                    float mis = 1.0f / (w_light + 1.0f + w_cam);
+                    /* density estimate uses f without the cosine Selas's BSDF value carries */
+                    acc_vm = add(acc_vm, scl(mul(f, lv->throughput), mis / cos_l));
+                }
+                color = add(color, scl(mul(thr, acc_vm), c.vm_norm));
+            }
+            if (!scatter(&u, &c, &s, x, &dir, &thr, &dvcm, &dvc, &dvm, &plen)) break;
+            pos = x;
+        }
+        float* p = I->iter_fb + 3 * (size_t)i;
+        p[0] += color.x;
+        p[1] += color.y;
+        p[2] += color.z;
+        I->path_rays[i] = nrays;
+    }
+    free(found);
+    free(c.stack);
+}
+
+void orc_render_vcm(const orc_scene* sc, const orc_vcm_job* job, float* fb_out, float* stats_out, int threads)
+{
+    const uint32_t W = job->width, H = job->height, N = W * H, L = job->max_path_length;
+    const uint32_t mlv = L > 2 ? L - 2 : 1;
+    vcm_iter I;
+    memset(&I, 0, sizeof(I));
+    I.c.sc = sc;
+    I.c.job = job;
+    orc_camera_init(job->cam10, W, H, &I.c.cam);
+    /* virtual image plane distance: pixels per unit length at distance 1 (see the file comment) */
+    float hfov = job->cam10[9] * (kPi / 180.f);
+    I.vipd = (float)W / (2.0f * tanf(hfov * 0.5f));
+    I.W = W;
+    I.H = H;
+    I.N = N;
+    I.L = L;
+    I.mlv = mlv;
+    I.slot_verts = (vcm_vertex*)malloc(sizeof(vcm_vertex) * (size_t)N * mlv);
+    I.slot_nverts = (uint32_t*)malloc(sizeof(uint32_t) * N);
+    I.splat_c = (ov3*)malloc(sizeof(ov3) * (size_t)N * mlv);
+    I.splat_pix = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)N * mlv);
+    I.slot_nsplat = (uint32_t*)malloc(sizeof(uint32_t) * N);
+    I.path_rays = (uint32_t*)malloc(sizeof(uint32_t) * N);
+    vcm_vertex* verts = (vcm_vertex*)malloc(sizeof(vcm_vertex) * (size_t)N * mlv);
+    uint32_t* path_ends = (uint32_t*)malloc(sizeof(uint32_t) * N);
+    float* vpos = (float*)malloc(sizeof(float) * 3 * (size_t)N * mlv);
+    float* acc = (float*)calloc(3 * (size_t)N, sizeof(float));
+    float* iter_fb = (float*)malloc(sizeof(float) * 3 * (size_t)N);
+    double n_light_verts = 0, n_rays = 0;
+
+    for (uint32_t it = 0; it < job->iterations; ++it) {
+        float consts[6];
+        orc_vcm_constants(N, 1, job->base_radius, job->radius_alpha, (float)(job->iteration_base + it + 1), consts);
+        I.c.radius = consts[0];
+        I.c.vm_norm = consts[2];
+        I.c.vm_weight = consts[3];
+        I.c.vc_weight = consts[4];
+        I.iteration = job->iteration_base + it;
+        memset(iter_fb, 0, sizeof(float) * 3 * (size_t)N);
+
+        /* light subpaths (EvaluateLightPaths, VCM.cpp:502-561), then gathered in path order */
+        orc_parallel_for(light_paths, &I, N, threads);
+        uint32_t nv = 0;
+        for (uint32_t i = 0; i < N; ++i) {
+            for (uint32_t k = 0; k < I.slot_nverts[i]; ++k) {
+                const vcm_vertex* vx = &I.slot_verts[(size_t)i * mlv + k];
+                verts[nv] = *vx;
+                vpos[3 * nv] = vx->pos.x;
+                vpos[3 * nv + 1] = vx->pos.y;
+                vpos[3 * nv + 2] = vx->pos.z;
+                ++nv;
+            }
+            path_ends[i] = nv;
+            for (uint32_t k = 0; k < I.slot_nsplat[i]; ++k) {
+                float* p = iter_fb + 3 * (size_t)I.splat_pix[(size_t)i * mlv + k];
+                ov3 cc = I.splat_c[(size_t)i * mlv + k];
+                p[0] += cc.x;
+                p[1] += cc.y;
+                p[2] += cc.z;
+            }
+            n_rays += I.path_rays[i];
+        }
+        n_light_verts += nv;
+
+        /* photon grid (BuildHashGrid, VCMHashGrid.cpp:48) */
+        orc_hash_grid grid;
+        orc_hash_grid_build(&grid, vpos, nv, N, I.c.radius);
+
+        /* camera subpaths (EvaluateCameraPaths, VCM.cpp:576-666): path i adds to pixel i only */
+        I.verts = verts;
+        I.path_ends = path_ends;
+        I.vpos = vpos;
+        I.nv = nv;
+        I.grid = &grid;
+        I.iter_fb = iter_fb;
+        orc_parallel_for(camera_paths, &I, N, threads);
+        for (uint32_t i = 0; i < N; ++i) n_rays += I.path_rays[i];
+        orc_hash_grid_free(&grid);
+        for (size_t k = 0; k < 3 * (size_t)N; ++k) acc[k] += iter_fb[k];
+    }
+    float inv = 1.0f / (float)job->iterations;
+    for (size_t k = 0; k < 3 * (size_t)N; ++k) fb_out[k] = acc[k] * inv;
+    if (stats_out) {
+        stats_out[0] = (float)(n_light_verts / job->iterations);
+        stats_out[1] = (float)n_rays;
+    }
+    free(I.slot_verts);
+    free(I.slot_nverts);
+    free(I.splat_c);
+    free(I.splat_pix);
+    free(I.slot_nsplat);
+    free(I.path_rays);
+    free(verts);
+    free(path_ends);
+    free(vpos);
+    free(acc);
+    free(iter_fb);
+}

@@ -88,3 +88,5 @@ int sgpu_trace_launch(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hi
                       uint32_t* occluded, uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream);
 // Wavefront path tracer (sgpu_pt.hip); arguments validated by sgpu_render_pt.
 int sgpu_pt_render_impl(sgpu_scene scene, const sgpu_pt_job* job, float* framebuffer, hipStream_t stream);
+int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* framebuffer, sgpu_vcm_stats* stats,
+                         hipStream_t stream);

@@ -24,13 +24,13 @@ SGPU_HD float hit_error(f3 p, float t)
     return kErr * max_(max_(absf_(p.x), absf_(p.y)), max_(absf_(p.z), t));
 }
 
-// Fills everything but the base colour's texture lookup: `base_color` is the material value
-// raised to 2.2 unless `texel` (already filtered, before the 2.2 power) is given.
-SGPU_HD void surface_from_hit(f3 o, f3 d, float t, f3 ng, const float* __restrict mat, const f3* texel, Surface& s)
+// Shading frame and material of a surface point with shading normal `n` seen from `view` (the
+// unit direction toward the previous path vertex): everything CalculateSurfaceParams derives from
+// the material. `base_color` is the material value raised to 2.2 unless `texel` (already filtered,
+// before the 2.2 power) is given.
+SGPU_HD void surface_material(const float* __restrict mat, f3 n, f3 view, const f3* texel, Surface& s)
 {
-    s.position = mk3(o.x + t * d.x, o.y + t * d.y, o.z + t * d.z);
-    s.error = hit_error(s.position, t);
-    s.n = normalize3(ng);
+    s.n = n;
     make_frame(s.n, s.t, s.b);
     f3 bc = texel ? *texel : mk3(mat[0], mat[1], mat[2]);
     s.base_color = mk3(powf(bc.x, 2.2f), powf(bc.y, 2.2f), powf(bc.z, 2.2f));
@@ -50,8 +50,15 @@ SGPU_HD void surface_from_hit(f3 o, f3 d, float t, f3 ng, const float* __restric
     s.ior = mat[19];
     s.shader = (uint32_t)mat[20];
     s.flags = (uint32_t)mat[21];
-    f3 view = -d;
     s.relative_ior = ((s.flags & kMatTransparent) && dot3(view, s.n) < 0.0f) ? s.ior : 1.0f / s.ior;
+}
+
+// Surface of the hit at distance t along (o, d) with geometric normal ng.
+SGPU_HD void surface_from_hit(f3 o, f3 d, float t, f3 ng, const float* __restrict mat, const f3* texel, Surface& s)
+{
+    s.position = mk3(o.x + t * d.x, o.y + t * d.y, o.z + t * d.z);
+    s.error = hit_error(s.position, t);
+    surface_material(mat, normalize3(ng), -d, texel, s);
 }
 
 }  // namespace sgpu

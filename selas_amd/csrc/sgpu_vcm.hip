@@ -1,0 +1,1083 @@
+// Wavefront vertex connection and merging: the device form of Selas's VCM integrator
+// (Source/Applications/Selas/Source/VCM.cpp: VertexConnectionAndMerging :669, EvaluateLightPaths
+// :502, EvaluateCameraPaths :576, ConnectLightPathToCamera :190, ConnectCameraPathToLight :245,
+// ConnectPathVertices :293, MergeVertices :362, SampleBsdfScattering :412; VCMCommon.cpp:37-113;
+// VCMHashGrid.cpp:15-157), following Georgiev et al., "Implementing Vertex Connection and
+// Merging" (2012) where the commented reference cannot run as written. The integrator semantics
+// (and every deviation from the commented code) are those of the CPU restatement the tests
+// compare against; see the repository's design notes for the list.
+//
+// One iteration (W x H light paths, then W x H camera paths) is a fixed sequence of launches:
+//   light pass   k_light_gen, then per bounce: sgpu_intersect -> k_light_shade (store the vertex
+//                in its path's slot, queue the camera connection and the continuation ray) ->
+//                sgpu_occluded -> k_splat (atomic adds into the iteration frame)
+//   vertices     exclusive scan of per-path counts -> k_compact: path-major vertex array, the
+//                order in which Selas appends them (so the photon grid is built over the same list)
+//   photon grid  k_grid_bounds (box), k_grid_keys (hashed cell per vertex), rocPRIM radix sort of
+//                (cell, vertex) pairs (stable: equal cells keep vertex order, as the counting sort
+//                of BuildHashGrid does), k_grid_count + inclusive scan -> cell range ends
+//   camera pass  k_cam_gen, then per bounce: sgpu_intersect -> k_cam_shade (NEE and one vertex
+//                connection per light vertex of the same path index queued as shadow rays into
+//                fixed per-path slots; vertex merging over the 2x2x2 grid cells) -> sgpu_occluded
+//                -> k_cam_gather (adds the unoccluded slots of each path in slot order)
+//   k_accumulate adds the iteration's frame to the running sum.
+#include "sgpu_camera.h"
+#include "sgpu_internal.h"
+#include "sgpu_light.h"
+#include "sgpu_surface.h"
+
+#include <cstring>  // rocprim/iterator/texture_cache_iterator.hpp uses memset without including it
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include <algorithm>
+#include <cmath>
+
+namespace {
+
+using namespace sgpu;
+
+constexpr int kBlock = 256;
+constexpr float kFloatMax = 3.402823466e+38F;
+constexpr float kRayEps = 0.00001f;  // RayPick's tnear in the VCM code (VCM.cpp:137)
+constexpr uint32_t kMaxMerge = 65536;  // merged vertices per query, as the CPU range search caps them
+
+__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter)
+{
+    const uint64_t mask = __ballot(pred);
+    if (mask == 0) return 0xFFFFFFFFu;
+    const uint32_t lane = __lane_id();
+    const uint32_t below = __popcll(mask & ((1ull << lane) - 1ull));
+    const uint32_t leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader, 64);
+    return pred ? base + below : 0xFFFFFFFFu;
+}
+
+struct RayBuf {
+    float *ox, *oy, *oz, *dx, *dy, *dz, *tnear, *tfar;
+};
+struct HitBuf {
+    float *t, *u, *v, *ngx, *ngy, *ngz;
+    uint32_t *prim, *geom;
+};
+
+__device__ __forceinline__ void put_ray(const RayBuf& r, uint32_t i, f3 o, f3 d, float tnear, float tfar)
+{
+    r.ox[i] = o.x;
+    r.oy[i] = o.y;
+    r.oz[i] = o.z;
+    r.dx[i] = d.x;
+    r.dy[i] = d.y;
+    r.dz[i] = d.z;
+    r.tnear[i] = tnear;
+    r.tfar[i] = tfar;
+}
+
+// Light vertex, 64 B (VCMVertex, VCMCommon.h:36-46, with the surface data a connection needs in
+// place of the hit record: position, normal, incoming direction, material).
+struct LightVertex {
+    float pos[3];
+    float thr[3];
+    float n[3];
+    float wfix[3];  // direction toward the previous vertex of the light subpath
+    float dvcm, dvc, dvm;
+    uint32_t geom_plen;  // geometry << 5 | path length (path length < 32, as in PathState)
+};
+static_assert(sizeof(LightVertex) == 64, "LightVertex must be 64 bytes");
+
+// Subpath state (PathState, VCMCommon.h:48-60), SoA over paths.
+struct PathState {
+    float* pos;  // 3
+    float* dir;  // 3
+    float* thr;  // 3
+    float* dvcm;
+    float* dvc;
+    float* dvm;
+    uint32_t* plen;
+    uint64_t* rng;
+    float* color;  // 3, camera paths only
+};
+
+struct VcmParams {
+    CameraParams cam;
+    float vipd;  // W / (2 tan(hfov / 2))
+    uint32_t W, H, N, L, mlv;
+    float vm_weight, vc_weight, vm_norm, radius_sq, inv_cell;
+    uint64_t seed;
+    uint32_t iteration;
+    const float* materials;
+    const uint32_t* geom_material;
+    uint32_t n_geoms;
+    const Light* lights;
+    uint32_t n_lights;
+    f3 background;
+};
+
+__device__ __forceinline__ f3 g3(const float* a, uint32_t i) { return mk3(a[3 * i], a[3 * i + 1], a[3 * i + 2]); }
+__device__ __forceinline__ void s3(float* a, uint32_t i, f3 v)
+{
+    a[3 * i] = v.x;
+    a[3 * i + 1] = v.y;
+    a[3 * i + 2] = v.z;
+}
+
+__device__ __forceinline__ uint64_t stream_id(uint32_t iteration, uint32_t path, uint32_t pass)
+{
+    return ((uint64_t)iteration << 33) | ((uint64_t)path << 1) | pass;
+}
+
+__device__ __forceinline__ Pcg32 rng_init(uint64_t seed, uint64_t sid)
+{
+    return pcg32_seed(seed ^ (sid * 0x9E3779B97F4A7C15ull), sid);
+}
+
+__device__ __forceinline__ Pcg32 rng_load(const PathState& ps, uint32_t p, uint32_t iteration, uint32_t pass)
+{
+    Pcg32 r;
+    r.state = ps.rng[p];
+    r.inc = (stream_id(iteration, p, pass) << 1u) | 1u;
+    return r;
+}
+
+__device__ __forceinline__ float cont_prob(const Surface& s) { return saturate_(max_(max_(s.base_color.x, s.base_color.y), s.base_color.z)); }
+
+// Surface of a stored light vertex (the material transforms of CalculateSurfaceParams).
+__device__ __forceinline__ void vertex_surface(const VcmParams& P, const LightVertex& v, Surface& s)
+{
+    const float* mat = P.materials + 22 * (size_t)P.geom_material[v.geom_plen >> 5];
+    s.position = mk3(v.pos[0], v.pos[1], v.pos[2]);
+    s.error = 0.0f;
+    surface_material(mat, mk3(v.n[0], v.n[1], v.n[2]), mk3(v.wfix[0], v.wfix[1], v.wfix[2]), nullptr, s);
+}
+
+// SampleBsdfScattering (VCM.cpp:412-446) with the Russian-roulette-compensated throughput.
+template <class U>
+__device__ __forceinline__ bool scatter(U& u, const VcmParams& P, const Surface& s, f3& dir, f3& thr, float& dvcm,
+                                        float& dvc, float& dvm, uint32_t& plen)
+{
+    BsdfSample bs;
+    if (!bsdf_sample(u, s, -dir, bs)) return false;
+    if (is_zero3(bs.reflectance)) return false;
+    float cp = cont_prob(s);
+    if (u.next() > cp) return false;
+    float fwd = bs.fwd_pdf * cp, rev = bs.rev_pdf * cp;
+    float cos_b = absf_(dot3(bs.wi, s.n));
+    thr = (thr * bs.reflectance) * (1.0f / cp);
+    float k = cos_b / fwd;
+    float nvc = k * (dvc * rev + dvcm + P.vm_weight);
+    float nvm = k * (dvm * rev + dvcm * P.vc_weight + 1.0f);
+    dvc = nvc;
+    dvm = nvm;
+    dvcm = 1.0f / fwd;
+    dir = bs.wi;
+    ++plen;
+    return true;
+}
+
+// ---- light pass ----------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_light_gen(VcmParams P, PathState ps, RayBuf rays, uint32_t* __restrict queue,
+                                                      uint32_t* __restrict counters)
+{
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    bool live = false;
+    f3 o = mk3(0, 0, 0), d = mk3(0, 0, 0);
+    if (i < P.N) {
+        PcgUniforms u{rng_init(P.seed, stream_id(P.iteration, i, 0))};
+        ps.plen[i] = 1;
+        if (P.n_lights > 0) {
+            float p_pick = 1.0f / P.n_lights;
+            uint32_t idx = (uint32_t)(u.next() * P.n_lights);
+            if (idx >= P.n_lights) idx = P.n_lights - 1;
+            const Light L = P.lights[idx];
+            float emission_pdf_w, direct_pdf_a;
+            f3 radiance;
+            if (L.type == 1) {
+                float r0 = u.next(), r1 = u.next();
+                float uu = 2.0f * r1 - 1.0f;
+                float norm = sqrtf(max_(0.0f, 1.0f - uu * uu));
+                float theta = (kPi * 2.0f) * r0;
+                d = mk3(norm * cosf(theta), uu, norm * sinf(theta));
+                o = ld3(L.position);
+                emission_pdf_w = (1.0f / (4.0f * kPi)) * p_pick;
+                direct_pdf_a = 1.0f * p_pick;
+                radiance = ld3(L.radiance);
+            } else {
+                f3 ex = ld3(L.x), ez = ld3(L.z);
+                float a = u.next(), b = u.next();
+                o = ld3(L.position) - ex * 0.5f - ez * 0.5f + ex * a + ez * b;
+                float r0 = u.next(), r1 = u.next();
+                float r = sqrtf(r0);
+                float th = (kPi * 2.0f) * r1;
+                f3 local = mk3(r * cosf(th), sqrtf(max_(0.0f, 1 - r0)), r * sinf(th));
+                f3 n = ld3(L.direction), t, bb;
+                make_frame(n, t, bb);
+                d = normalize3(t * local.x + n * local.y + bb * local.z);
+                float cos_l = local.y;
+                float area = length3(ex) * length3(ez);
+                float pdf_a = 1.0f / area;
+                emission_pdf_w = pdf_a * (cos_l / kPi) * p_pick;
+                direct_pdf_a = pdf_a * p_pick;
+                radiance = ld3(L.radiance) * cos_l;
+            }
+            s3(ps.pos, i, o);
+            s3(ps.dir, i, d);
+            s3(ps.thr, i, radiance * (1.0f / emission_pdf_w));
+            ps.dvcm[i] = direct_pdf_a / emission_pdf_w;
+            ps.dvc[i] = 0.0f;
+            ps.dvm[i] = 0.0f;
+            live = 1 + 2 < P.L;
+        }
+        ps.rng[i] = u.r.state;
+    }
+    uint32_t k = wave_append(live, &counters[0]);
+    if (live) {
+        put_ray(rays, k, o, d, kRayEps, kFloatMax);
+        queue[k] = i;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_light_shade(VcmParams P, uint32_t count, const uint32_t* __restrict queue,
+                                                        RayBuf rays, HitBuf hits, PathState ps, LightVertex* __restrict verts,
+                                                        uint32_t* __restrict vcount, RayBuf next_rays,
+                                                        uint32_t* __restrict next_queue, RayBuf shadow,
+                                                        float* __restrict splat_c, uint32_t* __restrict splat_pix,
+                                                        uint32_t* __restrict counters)
+{
+    uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    bool cont = false, emit = false;
+    uint32_t p = 0, pix = 0;
+    f3 next_o = mk3(0, 0, 0), next_d = mk3(0, 0, 0), sh_o = mk3(0, 0, 0), sh_d = mk3(0, 0, 0), c = mk3(0, 0, 0);
+    float sh_tnear = 0.0f, sh_tfar = 0.0f;
+    if (j < count) {
+        p = queue[j];
+        uint32_t geom = hits.geom[j];
+        if (geom != SGPU_INVALID_ID && geom < P.n_geoms) {
+            Pcg32 r = rng_load(ps, p, P.iteration, 0);
+            PcgUniforms u{r};
+            f3 o = mk3(rays.ox[j], rays.oy[j], rays.oz[j]);
+            f3 d = mk3(rays.dx[j], rays.dy[j], rays.dz[j]);
+            const float* mat = P.materials + 22 * (size_t)P.geom_material[geom];
+            Surface s;
+            surface_from_hit(o, d, hits.t[j], mk3(hits.ngx[j], hits.ngy[j], hits.ngz[j]), mat, nullptr, s);
+            f3 x = s.position;
+            f3 view = -d;
+            f3 pos = g3(ps.pos, p);
+            f3 thr = g3(ps.thr, p);
+            float dvcm = ps.dvcm[p], dvc = ps.dvc[p], dvm = ps.dvm[p];
+            uint32_t plen = ps.plen[p];
+            f3 dd = pos - x;
+            float d2 = dot3(dd, dd);
+            float cos_v = absf_(dot3(s.n, view));
+            dvcm *= d2;
+            dvcm *= (1.0f / cos_v);
+            dvc *= (1.0f / cos_v);
+            dvm *= (1.0f / cos_v);
+            // store the vertex in the path's slot
+            uint32_t k = vcount[p];
+            if (k < P.mlv) {
+                LightVertex lv;
+                lv.pos[0] = x.x;
+                lv.pos[1] = x.y;
+                lv.pos[2] = x.z;
+                lv.thr[0] = thr.x;
+                lv.thr[1] = thr.y;
+                lv.thr[2] = thr.z;
+                lv.n[0] = s.n.x;
+                lv.n[1] = s.n.y;
+                lv.n[2] = s.n.z;
+                lv.wfix[0] = view.x;
+                lv.wfix[1] = view.y;
+                lv.wfix[2] = view.z;
+                lv.dvcm = dvcm;
+                lv.dvc = dvc;
+                lv.dvm = dvm;
+                lv.geom_plen = (geom << 5) | (plen & 31u);
+                verts[(size_t)p * P.mlv + k] = lv;
+                vcount[p] = k + 1;
+            }
+            // ConnectLightPathToCamera (VCM.cpp:190-242)
+            const CameraParams& cam = P.cam;
+            f3 to_pos = x - cam.position;
+            if (dot3(cam.forward, to_pos) > 0.0f) {
+                float zc = dot3(to_pos, cam.cam_z);
+                if (zc > 0.0f) {
+                    float cx = -dot3(to_pos, cam.cam_x) / (dot3(cam.cam_x, cam.cam_x) * zc);
+                    float cy = dot3(to_pos, cam.cam_y) / (dot3(cam.cam_y, cam.cam_y) * zc);
+                    float px = (cx + 1.0f) * 0.5f * (float)P.W;
+                    float py = (1.0f - cy) * 0.5f * (float)P.H;
+                    if (px >= 0.0f && py >= 0.0f && px < (float)P.W && py < (float)P.H) {
+                        float dist = length3(to_pos);
+                        f3 to_cam = to_pos * (-1.0f / dist);
+                        float cos_cam = dot3(cam.forward, -to_cam);
+                        float fwd, rev;
+                        f3 f = bsdf_eval(s, view, to_cam, fwd, rev);
+                        if (!is_zero3(f)) {
+                            float cp = cont_prob(s);
+                            rev *= cp;
+                            float img_dist = P.vipd / cos_cam;
+                            float img_to_solid = img_dist * img_dist / cos_cam;
+                            float cos_to_cam = absf_(dot3(s.n, to_cam));
+                            float img_to_surface = img_to_solid * cos_to_cam / (dist * dist);
+                            float w_light = (img_to_surface / (float)P.N) * (P.vm_weight + dvcm + dvc * rev);
+                            float mis = 1.0f / (w_light + 1.0f);
+                            c = (thr * f) * (mis * img_to_solid / ((float)P.N * dist * dist));
+                            if (!is_zero3(c)) {
+                                int ix = (int)floorf(px), iy = (int)floorf(py);
+                                if (ix >= 0 && iy >= 0 && ix < (int)P.W && iy < (int)P.H) {
+                                    emit = true;
+                                    pix = (uint32_t)iy * P.W + (uint32_t)ix;
+                                    sh_o = offset_ray_origin(s, to_cam, 0.1f);
+                                    sh_d = to_cam;
+                                    sh_tnear = s.error;
+                                    sh_tfar = dist;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+            f3 dir = d;
+            if (scatter(u, P, s, dir, thr, dvcm, dvc, dvm, plen)) {
+                cont = plen + 2 < P.L;
+                next_o = x;
+                next_d = dir;
+                s3(ps.pos, p, x);
+                s3(ps.dir, p, dir);
+                s3(ps.thr, p, thr);
+                ps.dvcm[p] = dvcm;
+                ps.dvc[p] = dvc;
+                ps.dvm[p] = dvm;
+                ps.plen[p] = plen;
+            }
+            ps.rng[p] = u.r.state;
+        }
+    }
+    uint32_t ks = wave_append(emit, &counters[1]);
+    if (emit) {
+        put_ray(shadow, ks, sh_o, sh_d, sh_tnear, sh_tfar);
+        s3(splat_c, ks, c);
+        splat_pix[ks] = pix;
+    }
+    uint32_t kn = wave_append(cont, &counters[0]);
+    if (cont) {
+        put_ray(next_rays, kn, next_o, next_d, kRayEps, kFloatMax);
+        next_queue[kn] = p;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_splat(uint32_t count, const uint32_t* __restrict occluded,
+                                                  const float* __restrict splat_c, const uint32_t* __restrict splat_pix,
+                                                  float* __restrict frame)
+{
+    uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= count || occluded[k]) return;
+    uint32_t pix = splat_pix[k];
+    atomicAdd(&frame[3 * pix], splat_c[3 * k]);
+    atomicAdd(&frame[3 * pix + 1], splat_c[3 * k + 1]);
+    atomicAdd(&frame[3 * pix + 2], splat_c[3 * k + 2]);
+}
+
+// path-major compaction of the light vertices (the order of Selas's lightVertices array)
+__global__ __launch_bounds__(kBlock) void k_compact(uint32_t N, uint32_t mlv, const LightVertex* __restrict verts,
+                                                    const uint32_t* __restrict vcount, const uint32_t* __restrict offset,
+                                                    LightVertex* __restrict cverts, float* __restrict cpos)
+{
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    uint32_t n = vcount[i], base = offset[i];
+    for (uint32_t k = 0; k < n; ++k) {
+        LightVertex v = verts[(size_t)i * mlv + k];
+        cverts[base + k] = v;
+        cpos[3 * (base + k)] = v.pos[0];
+        cpos[3 * (base + k) + 1] = v.pos[1];
+        cpos[3 * (base + k) + 2] = v.pos[2];
+    }
+}
+
+// ---- photon grid (BuildHashGrid, VCMHashGrid.cpp:48-97) ---------------------------------------------
+
+__device__ __forceinline__ uint32_t f2ord(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(uint32_t u) { return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u); }
+
+// box[0..2] = ord(min), box[3..5] = ~ord(max) (both reduced with atomicMin)
+__global__ __launch_bounds__(kBlock) void k_grid_bounds(const float* __restrict cpos, const uint32_t* __restrict nv_ptr,
+                                                        uint32_t* __restrict box)
+{
+    __shared__ uint32_t red[6][kBlock / 64];
+    uint32_t nv = *nv_ptr;
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t v[6];
+    for (int k = 0; k < 6; ++k) v[k] = 0xFFFFFFFFu;
+    if (i < nv)
+        for (int k = 0; k < 3; ++k) {
+            float x = cpos[3 * i + k];
+            v[k] = f2ord(x);
+            v[3 + k] = ~f2ord(x);
+        }
+    for (int k = 0; k < 6; ++k) {
+        uint32_t x = v[k];
+        for (int off = 32; off > 0; off >>= 1) x = min(x, (uint32_t)__shfl_xor((int)x, off, 64));
+        v[k] = x;
+    }
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 6; ++k) red[k][threadIdx.x >> 6] = v[k];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        uint32_t x = red[threadIdx.x][0];
+        for (int w = 1; w < kBlock / 64; ++w) x = min(x, red[threadIdx.x][w]);
+        atomicMin(&box[threadIdx.x], x);
+    }
+}
+
+__device__ __forceinline__ uint32_t cell_hash(int32_t x, int32_t y, int32_t z, uint32_t cell_count)
+{
+    uint32_t ux = (uint32_t)x, uy = (uint32_t)y, uz = (uint32_t)z;
+    return ((ux * 73856093u) ^ (uy * 19349663u) ^ (uz * 83492791u)) % cell_count;
+}
+
+__global__ __launch_bounds__(kBlock) void k_grid_keys(const float* __restrict cpos, const uint32_t* __restrict nv_ptr,
+                                                      const uint32_t* __restrict box, float inv_cell, uint32_t cell_count,
+                                                      uint32_t* __restrict keys, uint32_t* __restrict vals,
+                                                      uint32_t* __restrict cell_counts)
+{
+    uint32_t nv = *nv_ptr;
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nv) return;
+    float lo[3] = {ord2f(box[0]), ord2f(box[1]), ord2f(box[2])};
+    int32_t c[3];
+    for (int k = 0; k < 3; ++k) c[k] = (int32_t)floorf(inv_cell * (cpos[3 * i + k] - lo[k]));
+    uint32_t cell = cell_hash(c[0], c[1], c[2], cell_count);
+    keys[i] = cell;
+    vals[i] = i;
+    atomicAdd(&cell_counts[cell], 1u);
+}
+
+// ---- camera pass ----------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_cam_gen(VcmParams P, PathState ps, RayBuf rays, uint32_t* __restrict queue)
+{
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= P.N) return;
+    PcgUniforms u{rng_init(P.seed, stream_id(P.iteration, i, 1))};
+    uint32_t x = i % P.W, y = i / P.W;
+    float jx = (float)x + u.next(), jy = (float)y + u.next();
+    f3 d = camera_image_to_world_dir(P.cam, jx, jy);
+    float cos_cam = dot3(P.cam.forward, d);
+    float img_dist = P.vipd / cos_cam;
+    float cam_pdf_w = img_dist * img_dist / cos_cam;
+    s3(ps.pos, i, P.cam.position);
+    s3(ps.dir, i, d);
+    s3(ps.thr, i, mk3(1, 1, 1));
+    s3(ps.color, i, mk3(0, 0, 0));
+    ps.dvcm[i] = (float)P.N * (1.0f / cam_pdf_w);
+    ps.dvc[i] = 0.0f;
+    ps.dvm[i] = 0.0f;
+    ps.plen[i] = 1;
+    ps.rng[i] = u.r.state;
+    put_ray(rays, i, P.cam.position, d, kRayEps, kFloatMax);
+    queue[i] = i;
+}
+
+struct GridView {
+    const uint32_t* box;
+    const uint32_t* cell_ends;
+    const uint32_t* indices;
+    const LightVertex* cverts;
+    const uint32_t* path_end;  // inclusive end of each light path's vertices
+    uint32_t cell_count;
+    const uint32_t* nv_ptr;
+};
+
+struct CamSlots {
+    float* c;          // (mlv + 2) x 3 per path: NEE, VC 0..mlv-1, VM
+    uint32_t* valid;   // (mlv + 2) per path: 0 none, 1 unconditional, 2 pending shadow ray, 3 occluded
+    uint32_t* ray_slot;  // per queued shadow ray: path * (mlv + 2) + slot
+};
+
+__global__ __launch_bounds__(kBlock) void k_cam_shade(VcmParams P, uint32_t count, const uint32_t* __restrict queue,
+                                                      RayBuf rays, HitBuf hits, PathState ps, GridView G, CamSlots S,
+                                                      RayBuf next_rays, uint32_t* __restrict next_queue, RayBuf shadow,
+                                                      uint32_t* __restrict counters)
+{
+    const uint32_t nslots = P.mlv + 2;
+    uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    bool cont = false;
+    uint32_t p = 0;
+    f3 next_o = mk3(0, 0, 0), next_d = mk3(0, 0, 0);
+    // shadow rays of this lane, emitted below with wave-level appends (one per slot index)
+    bool active = j < count;
+    Surface s;
+    f3 x = mk3(0, 0, 0), view = mk3(0, 0, 0);
+    bool hit = false;
+    if (active) {
+        p = queue[j];
+        for (uint32_t k = 0; k < nslots; ++k) S.valid[(size_t)p * nslots + k] = 0;
+        uint32_t geom = hits.geom[j];
+        f3 d = mk3(rays.dx[j], rays.dy[j], rays.dz[j]);
+        if (geom == SGPU_INVALID_ID || geom >= P.n_geoms) {
+            if (ps.plen[p] == 1) {
+                f3 col = g3(ps.color, p) + g3(ps.thr, p) * P.background;
+                s3(ps.color, p, col);
+            }
+        } else {
+            hit = true;
+            f3 o = mk3(rays.ox[j], rays.oy[j], rays.oz[j]);
+            const float* mat = P.materials + 22 * (size_t)P.geom_material[geom];
+            surface_from_hit(o, d, hits.t[j], mk3(hits.ngx[j], hits.ngy[j], hits.ngz[j]), mat, nullptr, s);
+            x = s.position;
+            view = -d;
+        }
+    }
+    PcgUniforms u{Pcg32{0, 1}};
+    f3 thr = mk3(0, 0, 0);
+    float dvcm = 0, dvc = 0, dvm = 0;
+    uint32_t plen = 0;
+    if (hit) {
+        u.r = rng_load(ps, p, P.iteration, 1);
+        f3 pos = g3(ps.pos, p);
+        thr = g3(ps.thr, p);
+        dvcm = ps.dvcm[p];
+        dvc = ps.dvc[p];
+        dvm = ps.dvm[p];
+        plen = ps.plen[p];
+        f3 dd = pos - x;
+        float d2 = dot3(dd, dd);
+        float cos_v = absf_(dot3(s.n, view));
+        dvcm *= d2;
+        dvcm /= cos_v;
+        dvc /= cos_v;
+        dvm /= cos_v;
+    }
+    // ---- NEE (slot 0) ----
+    {
+        bool emit = false;
+        f3 so = mk3(0, 0, 0), ld = mk3(0, 0, 0);
+        float tn = 0, tf = 0;
+        if (hit && plen + 1 < P.L && P.n_lights > 0) {
+            float p_pick = 1.0f / P.n_lights;
+            uint32_t idx = (uint32_t)(u.next() * P.n_lights);
+            if (idx >= P.n_lights) idx = P.n_lights - 1;
+            const Light L = P.lights[idx];
+            float a = u.next(), b = u.next();
+            bool ok = true;
+            f3 rad;
+            float dist, dpdf, epdf, cl;
+            if (L.type == 1) {
+                f3 dv = ld3(L.position) - x;
+                float dd2 = dot3(dv, dv);
+                dist = sqrtf(dd2);
+                ld = dv * (1.0f / dist);
+                rad = ld3(L.radiance);
+                dpdf = dd2 * p_pick;
+                epdf = (1.0f / (4.0f * kPi)) * p_pick;
+                cl = 1.0f;
+            } else {
+                f3 ex = ld3(L.x), ez = ld3(L.z);
+                f3 y = ld3(L.position) - ex * 0.5f - ez * 0.5f + ex * a + ez * b;
+                f3 dv = y - x;
+                float dd2 = dot3(dv, dv);
+                dist = sqrtf(dd2);
+                ld = dv * (1.0f / dist);
+                cl = dot3(ld3(L.direction), -ld);
+                ok = cl > 0.0f;
+                float pdf_a = 1.0f / (length3(ex) * length3(ez));
+                rad = ld3(L.radiance);
+                dpdf = pdf_a * dd2 / cl * p_pick;
+                epdf = pdf_a * (cl / kPi) * p_pick;
+            }
+            if (ok) {
+                float fwd, rev;
+                f3 f = bsdf_eval(s, view, ld, fwd, rev);
+                if (!is_zero3(f)) {
+                    float cp = cont_prob(s);
+                    rev *= cp;
+                    float cos_s = absf_(dot3(s.n, ld));
+                    float w_cam = (epdf * cos_s / (dpdf * cl)) * (P.vm_weight + dvcm + dvc * rev);
+                    float mis = 1.0f / (1.0f + w_cam);
+                    f3 c = (f * rad) * (mis / dpdf);
+                    if (!is_zero3(c)) {
+                        emit = true;
+                        s3(S.c, p * nslots + 0, thr * c);
+                        so = offset_ray_origin(s, ld, 0.1f);
+                        tn = s.error;
+                        tf = dist;
+                    }
+                }
+            }
+        }
+        uint32_t ks = wave_append(emit, &counters[1]);
+        if (emit) {
+            put_ray(shadow, ks, so, ld, tn, tf);
+            S.ray_slot[ks] = p * nslots + 0;
+            S.valid[(size_t)p * nslots + 0] = 2;
+        }
+    }
+    // ---- vertex connections to light path p (slots 1..mlv) ----
+    uint32_t vb = 0, ve = 0;
+    if (hit) {
+        vb = p == 0 ? 0 : G.path_end[p - 1];
+        ve = G.path_end[p];
+    }
+    bool vc_live = hit;
+    for (uint32_t k = 0; k < P.mlv; ++k) {
+        bool emit = false;
+        f3 so = mk3(0, 0, 0), cd = mk3(0, 0, 0);
+        float tn = 0, tf = 0;
+        const uint32_t vi = vb + k;
+        if (vc_live && vi < ve) {
+            const LightVertex lv = G.cverts[vi];
+            uint32_t lplen = lv.geom_plen & 31u;
+            if (lplen + 1 + plen <= P.L) {
+                f3 lpos = mk3(lv.pos[0], lv.pos[1], lv.pos[2]);
+                f3 dv = lpos - x;
+                float dist2 = dot3(dv, dv);
+                float dist = sqrtf(dist2);
+                cd = dv * (1.0f / dist);
+                float cfwd, crev;
+                f3 fc = bsdf_eval(s, view, cd, cfwd, crev);
+                if (!is_zero3(fc)) {
+                    Surface ls;
+                    vertex_surface(P, lv, ls);
+                    float lfwd, lrev;
+                    f3 lw = mk3(lv.wfix[0], lv.wfix[1], lv.wfix[2]);
+                    f3 fl = bsdf_eval(ls, lw, -cd, lfwd, lrev);
+                    if (!is_zero3(fl)) {
+                        float cos_c = absf_(dot3(cd, s.n));
+                        float cos_l = absf_(dot3(-cd, ls.n));
+                        float ccp = cont_prob(s), lcp = cont_prob(ls);
+                        cfwd *= ccp;
+                        crev *= ccp;
+                        lfwd *= lcp;
+                        lrev *= lcp;
+                        float c_pdf_a = cfwd * cos_l / dist2;
+                        float l_pdf_a = lfwd * cos_c / dist2;
+                        float w_light = c_pdf_a * (P.vm_weight + lv.dvcm + lv.dvc * lrev);
+                        float w_cam = l_pdf_a * (P.vm_weight + dvcm + dvc * crev);
+                        float mis = 1.0f / (w_light + 1.0f + w_cam);
+                        f3 c = (fc * fl) * (mis / dist2);
+                        if (!is_zero3(c)) {
+                            f3 lthr = mk3(lv.thr[0], lv.thr[1], lv.thr[2]);
+                            float sb;
+                            so = offset_ray_origin(s, cd, 0.1f, &sb);
+                            tn = s.error;
+                            tf = dist - 16.0f * absf_(sb);
+                            emit = true;
+                            s3(S.c, p * nslots + 1 + k, (thr * lthr) * c);
+                        }
+                    }
+                }
+            } else {
+                vc_live = false;  // the reference stops at the first vertex that is too long (VCM.cpp:631)
+            }
+        }
+        uint32_t ks = wave_append(emit, &counters[1]);
+        if (emit) {
+            put_ray(shadow, ks, so, cd, tn, tf);
+            S.ray_slot[ks] = p * nslots + 1 + k;
+            S.valid[(size_t)p * nslots + 1 + k] = 2;
+        }
+    }
+    // ---- vertex merging (slot mlv + 1) ----
+    if (hit && *G.nv_ptr > 0) {
+        float lo[3] = {ord2f(G.box[0]), ord2f(G.box[1]), ord2f(G.box[2])};
+        float hi[3] = {ord2f(~G.box[3]), ord2f(~G.box[4]), ord2f(~G.box[5])};
+        float q[3] = {x.x, x.y, x.z};
+        bool inside = true;
+        float dmin[3];
+        for (int k = 0; k < 3; ++k) {
+            dmin[k] = q[k] - lo[k];
+            float dmax = hi[k] - q[k];
+            if (dmin[k] < 0.0f || dmax < 0.0f) inside = false;
+        }
+        if (inside) {
+            int32_t mn[3];
+            for (int k = 0; k < 3; ++k) {
+                float cp = P.inv_cell * dmin[k];
+                float cf = floorf(cp);
+                float fr = cp - cf;
+                mn[k] = (int32_t)cf;
+                if (fr < 0.5f) --mn[k];
+            }
+            f3 acc = mk3(0, 0, 0);
+            float ccp = cont_prob(s);
+            uint32_t found = 0;
+            for (int z = 0; z <= 1; ++z)
+                for (int y = 0; y <= 1; ++y)
+                    for (int xx = 0; xx <= 1; ++xx) {
+                        uint32_t cell = cell_hash(mn[0] + xx, mn[1] + y, mn[2] + z, G.cell_count);
+                        uint32_t b = cell == 0 ? 0 : G.cell_ends[cell - 1], e = G.cell_ends[cell];
+                        for (uint32_t t = b; t < e && found < kMaxMerge; ++t) {
+                            const LightVertex lv = G.cverts[G.indices[t]];
+                            f3 lpos = mk3(lv.pos[0], lv.pos[1], lv.pos[2]);
+                            f3 dv = x - lpos;
+                            if (!(dot3(dv, dv) <= P.radius_sq)) continue;
+                            ++found;
+                            uint32_t lplen = lv.geom_plen & 31u;
+                            if (plen + lplen > P.L) continue;
+                            Surface ls;
+                            vertex_surface(P, lv, ls);
+                            if ((s.flags & kMatTransparent) != (ls.flags & kMatTransparent)) continue;
+                            f3 lw = mk3(lv.wfix[0], lv.wfix[1], lv.wfix[2]);
+                            float fwd, rev;
+                            f3 f = bsdf_eval(s, view, lw, fwd, rev);
+                            if (is_zero3(f)) continue;
+                            float cos_l = absf_(dot3(s.n, lw));
+                            fwd *= ccp;
+                            rev *= cont_prob(ls);
+                            float w_light = lv.dvcm * P.vc_weight + lv.dvm * fwd;
+                            float w_cam = dvcm * P.vc_weight + dvm * rev;
+                            float mis = 1.0f / (w_light + 1.0f + w_cam);
+                            acc += (f * mk3(lv.thr[0], lv.thr[1], lv.thr[2])) * (mis / cos_l);
+                        }
+                    }
+            s3(S.c, p * nslots + 1 + P.mlv, (thr * acc) * P.vm_norm);
+            S.valid[(size_t)p * nslots + 1 + P.mlv] = 1;
+        }
+    }
+    // ---- continuation ----
+    if (hit) {
+        f3 dir = -view;
+        if (scatter(u, P, s, dir, thr, dvcm, dvc, dvm, plen)) {
+            cont = plen < P.L;
+            next_o = x;
+            next_d = dir;
+            s3(ps.pos, p, x);
+            s3(ps.dir, p, dir);
+            s3(ps.thr, p, thr);
+            ps.dvcm[p] = dvcm;
+            ps.dvc[p] = dvc;
+            ps.dvm[p] = dvm;
+            ps.plen[p] = plen;
+        }
+        ps.rng[p] = u.r.state;
+    }
+    uint32_t kn = wave_append(cont, &counters[0]);
+    if (cont) {
+        put_ray(next_rays, kn, next_o, next_d, kRayEps, kFloatMax);
+        next_queue[kn] = p;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cam_resolve(uint32_t count, const uint32_t* __restrict occluded, CamSlots S)
+{
+    uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= count) return;
+    S.valid[S.ray_slot[k]] = occluded[k] ? 3u : 1u;
+}
+
+// color += the path's valid slots, in slot order (NEE, connections, merging), as the reference adds them
+__global__ __launch_bounds__(kBlock) void k_cam_gather(uint32_t count, const uint32_t* __restrict queue, uint32_t nslots,
+                                                       CamSlots S, PathState ps)
+{
+    uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= count) return;
+    uint32_t p = queue[j];
+    f3 col = g3(ps.color, p);
+    for (uint32_t k = 0; k < nslots; ++k)
+        if (S.valid[(size_t)p * nslots + k] == 1u) col += g3(S.c, p * nslots + k);
+    s3(ps.color, p, col);
+}
+
+__global__ __launch_bounds__(kBlock) void k_frame_add(uint32_t N, const float* __restrict color, float* __restrict frame,
+                                                      float* __restrict acc)
+{
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    for (int k = 0; k < 3; ++k) {
+        float v = frame[3 * i + k] + color[3 * i + k];
+        acc[3 * i + k] += v;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scale(float* __restrict fb, size_t n, float s)
+{
+    size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) fb[i] = fb[i] * s;
+}
+
+struct Arena {
+    std::vector<void*> ptrs;
+    hipStream_t stream;
+    hipError_t err = hipSuccess;
+    template <typename T>
+    T* get(size_t count)
+    {
+        void* p = nullptr;
+        if (err == hipSuccess) err = hipMallocAsync(&p, count * sizeof(T) + 64, stream);
+        if (err == hipSuccess) ptrs.push_back(p);
+        return (T*)p;
+    }
+    RayBuf rays(size_t n)
+    {
+        RayBuf r;
+        r.ox = get<float>(n);
+        r.oy = get<float>(n);
+        r.oz = get<float>(n);
+        r.dx = get<float>(n);
+        r.dy = get<float>(n);
+        r.dz = get<float>(n);
+        r.tnear = get<float>(n);
+        r.tfar = get<float>(n);
+        return r;
+    }
+    ~Arena()
+    {
+        for (void* p : ptrs) (void)hipFreeAsync(p, stream);
+    }
+};
+
+sgpu_rays_soa as_soa(const RayBuf& r) { return sgpu_rays_soa{r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, r.tnear, r.tfar}; }
+
+inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, sgpu_vcm_stats* stats,
+                         hipStream_t stream)
+{
+    const uint32_t W = job->camera.width, H = job->camera.height, N = W * H, L = job->max_path_length;
+    const uint32_t mlv = L > 2 ? L - 2 : 1;
+    const uint32_t nslots = mlv + 2;
+    const float pi = 3.141592653589793f;
+
+    VcmParams P;
+    P.cam = camera_init(mk3(job->camera.position[0], job->camera.position[1], job->camera.position[2]),
+                        mk3(job->camera.look_at[0], job->camera.look_at[1], job->camera.look_at[2]),
+                        mk3(job->camera.up[0], job->camera.up[1], job->camera.up[2]), job->camera.fov_degrees, W, H);
+    float hfov = job->camera.fov_degrees * (pi / 180.f);
+    P.vipd = (float)W / (2.0f * tanf(hfov * 0.5f));
+    P.W = W;
+    P.H = H;
+    P.N = N;
+    P.L = L;
+    P.mlv = mlv;
+    P.seed = job->seed;
+    P.materials = scene->d_materials;
+    P.geom_material = scene->d_geom_material;
+    P.n_geoms = scene->n_geom_material;
+    P.lights = (const Light*)scene->d_lights;
+    P.n_lights = scene->n_lights;
+    P.background = mk3(scene->background[0], scene->background[1], scene->background[2]);
+
+    Arena A;
+    A.stream = stream;
+    PathState ps;
+    ps.pos = A.get<float>(3 * (size_t)N);
+    ps.dir = A.get<float>(3 * (size_t)N);
+    ps.thr = A.get<float>(3 * (size_t)N);
+    ps.dvcm = A.get<float>(N);
+    ps.dvc = A.get<float>(N);
+    ps.dvm = A.get<float>(N);
+    ps.plen = A.get<uint32_t>(N);
+    ps.rng = A.get<uint64_t>(N);
+    ps.color = A.get<float>(3 * (size_t)N);
+    RayBuf rays[2] = {A.rays(N), A.rays(N)};
+    uint32_t* queue[2] = {A.get<uint32_t>(N), A.get<uint32_t>(N)};
+    HitBuf hb;
+    hb.t = A.get<float>(N);
+    hb.u = A.get<float>(N);
+    hb.v = A.get<float>(N);
+    hb.ngx = A.get<float>(N);
+    hb.ngy = A.get<float>(N);
+    hb.ngz = A.get<float>(N);
+    hb.prim = A.get<uint32_t>(N);
+    hb.geom = A.get<uint32_t>(N);
+    const size_t max_shadow = (size_t)N * (mlv + 1);
+    RayBuf shadow = A.rays(max_shadow);
+    uint32_t* occl = A.get<uint32_t>(max_shadow);
+    float* splat_c = A.get<float>(3 * (size_t)N);
+    uint32_t* splat_pix = A.get<uint32_t>(N);
+    LightVertex* verts = A.get<LightVertex>((size_t)N * mlv);
+    LightVertex* cverts = A.get<LightVertex>((size_t)N * mlv);
+    float* cpos = A.get<float>(3 * (size_t)N * mlv);
+    uint32_t* vcount = A.get<uint32_t>(N + 1);
+    uint32_t* voffset = A.get<uint32_t>(N + 1);
+    uint32_t* path_end = A.get<uint32_t>(N);
+    uint32_t* keys = A.get<uint32_t>((size_t)N * mlv);
+    uint32_t* keys2 = A.get<uint32_t>((size_t)N * mlv);
+    uint32_t* vals = A.get<uint32_t>((size_t)N * mlv);
+    uint32_t* vals2 = A.get<uint32_t>((size_t)N * mlv);
+    uint32_t* cell_counts = A.get<uint32_t>(N);
+    uint32_t* cell_ends = A.get<uint32_t>(N);
+    uint32_t* box = A.get<uint32_t>(8);
+    uint32_t* counters = A.get<uint32_t>(4);
+    uint32_t* nv_dev = A.get<uint32_t>(1);
+    float* frame = A.get<float>(3 * (size_t)N);
+    CamSlots S;
+    S.c = A.get<float>(3 * (size_t)N * nslots);
+    S.valid = A.get<uint32_t>((size_t)N * nslots);
+    S.ray_slot = A.get<uint32_t>(max_shadow);
+    if (A.err != hipSuccess) return sgpu_fail(SGPU_E_NOMEM, "sgpu_render_vcm: device allocation failed");
+
+    // rocPRIM scratch (radix sort over at most N * mlv keys, scans over N)
+    size_t sort_bytes = 0, scan_bytes = 0, scan2_bytes = 0;
+    // cell keys are below N: sort only the bits that can be set
+    unsigned key_bits = 1;
+    while (key_bits < 32 && (N - 1) >> key_bits) ++key_bits;
+    SGPU_HIP_TRY(rocprim::radix_sort_pairs(nullptr, sort_bytes, keys, keys2, vals, vals2, (size_t)N * mlv, 0, key_bits,
+                                           stream));
+    SGPU_HIP_TRY(rocprim::exclusive_scan(nullptr, scan_bytes, vcount, voffset, 0u, (size_t)N + 1,
+                                         rocprim::plus<uint32_t>(), stream));
+    SGPU_HIP_TRY(rocprim::inclusive_scan(nullptr, scan2_bytes, cell_counts, cell_ends, (size_t)N,
+                                         rocprim::plus<uint32_t>(), stream));
+    void* tmp = A.get<uint8_t>(std::max(sort_bytes, std::max(scan_bytes, scan2_bytes)));
+    size_t tmp_bytes = std::max(sort_bytes, std::max(scan_bytes, scan2_bytes));
+    uint32_t* h = nullptr;
+    if (A.err != hipSuccess) return sgpu_fail(SGPU_E_NOMEM, "sgpu_render_vcm: device allocation failed");
+    SGPU_HIP_TRY(hipHostMalloc((void**)&h, 8 * sizeof(uint32_t), hipHostMallocDefault));
+    struct HostFree {
+        uint32_t* p;
+        ~HostFree() { (void)hipHostFree(p); }
+    } hf{h};
+
+    SGPU_HIP_TRY(hipMemsetAsync(fb, 0, 3 * (size_t)N * sizeof(float), stream));
+    sgpu_hits_soa hsoa{hb.t, hb.u, hb.v, hb.prim, hb.geom, hb.ngx, hb.ngy, hb.ngz};
+    uint64_t n_rays = 0, n_light_verts = 0;
+
+    for (uint32_t it = 0; it < job->iterations; ++it) {
+        const uint32_t gi = job->iteration_base + it;
+        // CalculateIterationConstants (VCMCommon.cpp:97-113), vmCount = W*H, vcCount = 1
+        float iteration_index = (float)gi + 1.0f;
+        float r = job->base_radius / powf(iteration_index, 0.5f * (1.0f - job->radius_alpha));
+        float r2 = r * r;
+        P.radius_sq = r2;
+        P.vm_norm = 1.0f / (pi * r2 * N);
+        P.vm_weight = pi * r2 * N / 1;
+        P.vc_weight = 1 / (pi * r2 * N);
+        P.inv_cell = 1.0f / (2.0f * r);
+        P.iteration = gi;
+
+        SGPU_HIP_TRY(hipMemsetAsync(frame, 0, 3 * (size_t)N * sizeof(float), stream));
+        SGPU_HIP_TRY(hipMemsetAsync(vcount, 0, ((size_t)N + 1) * sizeof(uint32_t), stream));
+        SGPU_HIP_TRY(hipMemsetAsync(counters, 0, 4 * sizeof(uint32_t), stream));
+
+        // ---- light pass ----
+        hipLaunchKernelGGL(k_light_gen, dim3(blocks_for(N)), dim3(kBlock), 0, stream, P, ps, rays[0], queue[0], counters);
+        SGPU_HIP_TRY(hipGetLastError());
+        SGPU_HIP_TRY(hipMemcpyAsync(h, counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        SGPU_HIP_TRY(hipStreamSynchronize(stream));
+        uint32_t count = h[0];
+        int cur = 0;
+        while (count > 0) {
+            sgpu_rays_soa rs = as_soa(rays[cur]);
+            int rc = sgpu_trace_launch(scene, &rs, &hsoa, nullptr, count, false, nullptr, stream);
+            if (rc) return rc;
+            n_rays += count;
+            SGPU_HIP_TRY(hipMemsetAsync(counters, 0, 4 * sizeof(uint32_t), stream));
+            hipLaunchKernelGGL(k_light_shade, dim3(blocks_for(count)), dim3(kBlock), 0, stream, P, count, queue[cur],
+                               rays[cur], hb, ps, verts, vcount, rays[1 - cur], queue[1 - cur], shadow, splat_c,
+                               splat_pix, counters);
+            SGPU_HIP_TRY(hipGetLastError());
+            SGPU_HIP_TRY(hipMemcpyAsync(h, counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+            SGPU_HIP_TRY(hipStreamSynchronize(stream));
+            uint32_t n_next = h[0], n_sh = h[1];
+            if (n_sh > 0) {
+                sgpu_rays_soa ss = as_soa(shadow);
+                rc = sgpu_trace_launch(scene, &ss, nullptr, occl, n_sh, true, nullptr, stream);
+                if (rc) return rc;
+                n_rays += n_sh;
+                hipLaunchKernelGGL(k_splat, dim3(blocks_for(n_sh)), dim3(kBlock), 0, stream, n_sh, occl, splat_c,
+                                   splat_pix, frame);
+                SGPU_HIP_TRY(hipGetLastError());
+            }
+            count = n_next;
+            cur = 1 - cur;
+        }
+
+        // ---- light vertices, path-major ----
+        SGPU_HIP_TRY(rocprim::exclusive_scan(tmp, tmp_bytes, vcount, voffset, 0u, (size_t)N + 1,
+                                             rocprim::plus<uint32_t>(), stream));
+        hipLaunchKernelGGL(k_compact, dim3(blocks_for(N)), dim3(kBlock), 0, stream, N, mlv, verts, vcount, voffset,
+                           cverts, cpos);
+        SGPU_HIP_TRY(hipGetLastError());
+        // path_end[i] = offset[i + 1] (inclusive end); the vertex count is offset[N]
+        SGPU_HIP_TRY(hipMemcpyAsync(path_end, voffset + 1, (size_t)N * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
+        SGPU_HIP_TRY(hipMemcpyAsync(nv_dev, voffset + N, sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
+        SGPU_HIP_TRY(hipMemcpyAsync(h, voffset + N, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        SGPU_HIP_TRY(hipStreamSynchronize(stream));
+        const uint32_t nv = h[0];
+        n_light_verts += nv;
+
+        // ---- photon grid ----
+        SGPU_HIP_TRY(hipMemsetAsync(box, 0xFF, 8 * sizeof(uint32_t), stream));
+        SGPU_HIP_TRY(hipMemsetAsync(cell_counts, 0, (size_t)N * sizeof(uint32_t), stream));
+        if (nv > 0) {
+            hipLaunchKernelGGL(k_grid_bounds, dim3(blocks_for(nv)), dim3(kBlock), 0, stream, cpos, nv_dev, box);
+            SGPU_HIP_TRY(hipGetLastError());
+            hipLaunchKernelGGL(k_grid_keys, dim3(blocks_for(nv)), dim3(kBlock), 0, stream, cpos, nv_dev, box, P.inv_cell,
+                               N, keys, vals, cell_counts);
+            SGPU_HIP_TRY(hipGetLastError());
+            size_t need = 0;
+            SGPU_HIP_TRY(rocprim::radix_sort_pairs(nullptr, need, keys, keys2, vals, vals2, (size_t)nv, 0, key_bits,
+                                                   stream));
+            if (need > tmp_bytes) {
+                tmp = A.get<uint8_t>(need);
+                tmp_bytes = need;
+                if (A.err != hipSuccess) return sgpu_fail(SGPU_E_NOMEM, "sgpu_render_vcm: device allocation failed");
+            }
+            SGPU_HIP_TRY(
+                rocprim::radix_sort_pairs(tmp, tmp_bytes, keys, keys2, vals, vals2, (size_t)nv, 0, key_bits, stream));
+        }
+        SGPU_HIP_TRY(rocprim::inclusive_scan(tmp, tmp_bytes, cell_counts, cell_ends, (size_t)N,
+                                             rocprim::plus<uint32_t>(), stream));
+        GridView G;
+        G.box = box;
+        G.cell_ends = cell_ends;
+        G.indices = vals2;
+        G.cverts = cverts;
+        G.path_end = path_end;
+        G.cell_count = N;
+        G.nv_ptr = nv_dev;
+
+        // ---- camera pass ----
+        hipLaunchKernelGGL(k_cam_gen, dim3(blocks_for(N)), dim3(kBlock), 0, stream, P, ps, rays[0], queue[0]);
+        SGPU_HIP_TRY(hipGetLastError());
+        count = L > 1 ? N : 0;  // a camera path of length 1 ends at the lens (VCM.cpp:585)
+        cur = 0;
+        while (count > 0) {
+            sgpu_rays_soa rs = as_soa(rays[cur]);
+            int rc = sgpu_trace_launch(scene, &rs, &hsoa, nullptr, count, false, nullptr, stream);
+            if (rc) return rc;
+            n_rays += count;
+            SGPU_HIP_TRY(hipMemsetAsync(counters, 0, 4 * sizeof(uint32_t), stream));
+            hipLaunchKernelGGL(k_cam_shade, dim3(blocks_for(count)), dim3(kBlock), 0, stream, P, count, queue[cur],
+                               rays[cur], hb, ps, G, S, rays[1 - cur], queue[1 - cur], shadow, counters);
+            SGPU_HIP_TRY(hipGetLastError());
+            SGPU_HIP_TRY(hipMemcpyAsync(h, counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+            SGPU_HIP_TRY(hipStreamSynchronize(stream));
+            uint32_t n_next = h[0], n_sh = h[1];
+            if (n_sh > 0) {
+                sgpu_rays_soa ss = as_soa(shadow);
+                rc = sgpu_trace_launch(scene, &ss, nullptr, occl, n_sh, true, nullptr, stream);
+                if (rc) return rc;
+                n_rays += n_sh;
+                hipLaunchKernelGGL(k_cam_resolve, dim3(blocks_for(n_sh)), dim3(kBlock), 0, stream, n_sh, occl, S);
+                SGPU_HIP_TRY(hipGetLastError());
+            }
+            hipLaunchKernelGGL(k_cam_gather, dim3(blocks_for(count)), dim3(kBlock), 0, stream, count, queue[cur],
+                               nslots, S, ps);
+            SGPU_HIP_TRY(hipGetLastError());
+            count = n_next;
+            cur = 1 - cur;
+        }
+        hipLaunchKernelGGL(k_frame_add, dim3(blocks_for(N)), dim3(kBlock), 0, stream, N, ps.color, frame, fb);
+        SGPU_HIP_TRY(hipGetLastError());
+    }
+    const size_t nf = 3 * (size_t)N;
+    hipLaunchKernelGGL(k_scale, dim3(blocks_for(nf)), dim3(kBlock), 0, stream, fb, nf, 1.0f / (float)job->iterations);
+    SGPU_HIP_TRY(hipGetLastError());
+    SGPU_HIP_TRY(hipStreamSynchronize(stream));
+    if (stats) {
+        stats->rays = n_rays;
+        stats->light_vertices = n_light_verts;
+        stats->samples = (uint64_t)N * job->iterations;
+    }
+    return SGPU_OK;
+}

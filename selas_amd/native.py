@@ -78,6 +78,7 @@ _SIGNATURES = {
     "sgpu_scene_set_materials": [_P, _FP, _U32, _U32P, _U32],
     "sgpu_scene_set_lights": [_P, _P, _U32, _FP],
     "sgpu_render_pt": [_P, _P, _P, _P],
+    "sgpu_render_vcm": [_P, _P, _P, _P, _P],
     "sgpu_bsdf_eval": [_P, _P, _P, _P, _U32, _P, _P],
     "sgpu_bsdf_sample": [_P, _P, _P, _P, _U32, _U32, _P, _P],
     "sgpu_gen_cluster_soup": [_U64, _U32, _U32, _F, _F, _F, _FP],
@@ -346,6 +347,34 @@ def build_render_scene(ctx: Context, sc: dict) -> Scene:
     check(lib.sgpu_scene_set_lights(scene.handle, lights.ctypes.data if lights.size else None, lights.shape[0],
                                     fptr(bg)))
     return scene
+
+
+class VcmJob(C.Structure):
+    _fields_ = [("camera", Camera), ("iterations", C.c_uint32), ("iteration_base", C.c_uint32),
+                ("max_path_length", C.c_uint32), ("base_radius", C.c_float), ("radius_alpha", C.c_float),
+                ("reserved", C.c_uint32), ("seed", C.c_uint64)]
+
+
+class VcmStats(C.Structure):
+    _fields_ = [("rays", C.c_uint64), ("light_vertices", C.c_uint64), ("samples", C.c_uint64)]
+
+
+def render_vcm(scene: Scene, cam: dict, width, height, iterations, max_path_length, base_radius, radius_alpha, seed,
+               iteration_base=0, stream=None, return_stats=False):
+    """VCM image (height x width x 3, radiance averaged over iterations) through sgpu_render_vcm."""
+    ctx = scene.ctx
+    fb = ctx.empty(3 * width * height * 4)
+    job = VcmJob(Camera.make(cam, width, height), iterations, iteration_base, max_path_length, base_radius,
+                 radius_alpha, 0, seed)
+    st = VcmStats()
+    try:
+        check(lib.sgpu_render_vcm(scene.handle, C.byref(job), fb.ptr, C.byref(st), stream))
+        out = fb.download(np.float32, 3 * width * height).reshape(height, width, 3)
+    finally:
+        fb.free()
+    if return_stats:
+        return out, {"rays": st.rays, "light_vertices": st.light_vertices, "samples": st.samples}
+    return out
 
 
 def render_pt(scene: Scene, cam: dict, width, height, spp, max_path_length, seed, flags=0, stream=None):

@@ -146,6 +146,85 @@ def room_scene(seed: int = 1):
     }
 
 
+def value_noise_2d(rng, lattice, x, z):
+    """Smooth value noise on [0,1]^2: random lattice values, smoothstep-weighted bilinear blend."""
+    g = rng.random((lattice + 1, lattice + 1))
+    fx, fz = x * lattice, z * lattice
+    ix = np.clip(np.floor(fx).astype(int), 0, lattice - 1)
+    iz = np.clip(np.floor(fz).astype(int), 0, lattice - 1)
+    tx, tz = fx - ix, fz - iz
+    sx, sz = tx * tx * (3 - 2 * tx), tz * tz * (3 - 2 * tz)
+    a = g[ix, iz] * (1 - sx) + g[ix + 1, iz] * sx
+    b = g[ix, iz + 1] * (1 - sx) + g[ix + 1, iz + 1] * sx
+    return a * (1 - sz) + b * sz
+
+
+def quad_grid_mesh(nu: int, nv: int, width: float, depth: float, heights: np.ndarray):
+    """A displaced grid of nu x nv quads; each quad (v0, v1, v2, v3) is split as Embree splits
+    quads: triangles (v0, v1, v3) and (v2, v3, v1). Returns (2 nu nv, 9) float32, quad-major."""
+    xs = np.linspace(0.0, width, nu + 1)
+    zs = np.linspace(0.0, depth, nv + 1)
+    X, Z = np.meshgrid(xs, zs, indexing="ij")
+    P = np.stack([X, heights, Z], axis=-1)  # (nu+1, nv+1, 3)
+    i, j = np.meshgrid(np.arange(nu), np.arange(nv), indexing="ij")
+    i, j = i.T.reshape(-1), j.T.reshape(-1)  # quad index = j * nu + i
+    v0, v1, v2, v3 = P[i, j], P[i + 1, j], P[i + 1, j + 1], P[i, j + 1]
+    ta = np.stack([v0, v1, v3], 1)
+    tb = np.stack([v2, v3, v1], 1)
+    tris = np.stack([ta, tb], 1).reshape(-1, 9)
+    return tris
+
+
+def random_rotation(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def vcm_scene(seed: int = 3, n_meshes: int = 64, nu: int = 128, nv: int = 64):
+    """VCM workload scene: n_meshes displaced-grid meshes of nu x nv quads (2 nu nv triangles each;
+    64 x 16,384 = 1,048,576 triangles), placed by seeded rotation, scale (1.5..3) and translation in
+    a 10^3 box; per-mesh Disney parameters: roughness uniform in [0.05, 1], metallic 0 or 1 (p 0.5),
+    specTrans 1 with probability 0.2 (else 0), base colour uniform in [0.2, 0.9]^3; one 4 x 4 quad
+    light (radiance 20) above the box facing down and one point light (intensity 60); merge radius
+    0.3 % of the scene diagonal."""
+    rng = np.random.default_rng(seed)
+    geoms, mats = [], []
+    for m in range(n_meshes):
+        xs = np.linspace(0, 1, nu + 1)
+        zs = np.linspace(0, 0.5, nv + 1)
+        X, Z = np.meshgrid(xs, zs, indexing="ij")
+        H = 0.12 * value_noise_2d(rng, 8, X, Z * 2)
+        local = quad_grid_mesh(nu, nv, 1.0, 0.5, H).reshape(-1, 3)
+        R = random_rotation(rng)
+        s = rng.uniform(1.5, 3.0)
+        t = rng.uniform(0.0, 10.0, 3)
+        world = (local @ R.T) * s + t
+        geoms.append(world.reshape(-1, 9).astype(np.float32))
+        metallic = 1.0 if rng.random() < 0.5 else 0.0
+        spec_trans = 1.0 if rng.random() < 0.2 else 0.0
+        mats.append(material(1, rng.uniform(0.2, 0.9, 3), metallic, rng.uniform(0.05, 1.0), 1.5, spec_trans=spec_trans))
+    lights = np.zeros(2, LIGHT_DTYPE)
+    lights[0] = (0, (5.0, 11.0, 5.0), (0.0, -1.0, 0.0), (4.0, 0.0, 0.0), (0.0, 0.0, 4.0), (20.0, 20.0, 20.0))
+    lights[1] = (1, (2.0, 8.0, 8.0), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (60.0, 60.0, 60.0))
+    allp = np.concatenate(geoms).reshape(-1, 3)
+    diag = float(np.linalg.norm(allp.max(0) - allp.min(0)))
+    return {
+        "geoms": geoms,
+        "materials": np.stack(mats),
+        "geom_material": np.arange(n_meshes, dtype=np.uint32),
+        "lights": lights,
+        "background": np.zeros(3, np.float32),
+        "camera": {"position": (5.0, 6.0, -14.0), "look_at": (5.0, 5.0, 5.0), "up": (0.0, 1.0, 0.0), "fov": 55.0},
+        "merge_radius": 0.003 * diag,
+        "radius_alpha": 0.75,
+        "max_path_length": 10,
+    }
+
+
 def anyhit_tfar(t_closest: np.ndarray, prim: np.ndarray, seed: int = 5):
     """tfar for the any-hit pass: 0.5 * t_closest on a seeded half of the hit rays (expected
     unoccluded), +inf elsewhere (expected occluded iff the ray has a closest hit)."""

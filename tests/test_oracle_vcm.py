@@ -1,0 +1,169 @@
+"""The VCM oracle (oracle/orc_vcm.c) pinned on the CPU.
+
+* BuildHashGrid / SearchHashGrid: bit-exact against vectors recorded from the reference's own
+  VCMHashGrid.cpp (tests/golden/make_golden.py): cell range ends, point order, box, and the list of
+  points found per query in the reference's visiting order.
+* CalculateIterationConstants: the closed forms of VCMCommon.cpp:91-113 (the file cannot be linked
+  here: AreaLighting.cpp, which it needs, calls an undefined SampleIBlPdf).
+* The integrator: the commented VCM.cpp cannot run, so it is pinned by what the mathematics fixes:
+  - a floor lit by a point and a quad light has a closed-form image (tests/scenes.py); the NEE,
+    light-tracing and merging estimators and their MIS weights must add up to it;
+  - with Lambert materials VCM and the path tracer estimate the same integral, so their image
+    means agree to the Monte Carlo error; between two facing planes with a large merge radius a
+    dropped term of the MIS recurrences moves the mean by 2-20 %, four times the tolerance or more
+    (checked by planting each such mistake in a copy of the oracle).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import scenes
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.mark.parametrize("case", [0, 1, 2])
+def test_hash_grid_matches_reference(case):
+    g = np.load(os.path.join(GOLD, f"hash_grid_{case}.npz"))
+    out = O.hash_grid(g["points"], int(g["cells"]), float(g["radius"]), g["queries"], max_found=512)
+    assert np.array_equal(out["cell_ends"], g["cell_ends"])
+    assert np.array_equal(out["cell_indices"], g["cell_indices"])
+    assert np.array_equal(out["box"], g["box"])
+    assert np.array_equal(out["found_count"], g["found_count"])
+    for i in range(g["queries"].shape[0]):
+        n = int(g["found_count"][i])
+        assert np.array_equal(out["found"][i, :n], g["found"][i, :n]), i
+    assert g["found_count"].sum() > 0
+
+
+def test_hash_grid_finds_every_point_in_radius():
+    """Brute-force check of the range search on points well inside the box (the reference's box
+    is not padded by the radius, so queries near its faces may miss points: a documented defect)."""
+    rng = np.random.default_rng(3)
+    pts = rng.random((4000, 3)).astype(np.float32)
+    q = (0.1 + 0.8 * rng.random((300, 3))).astype(np.float32)
+    r = 0.04
+    out = O.hash_grid(pts, 4000, r, q, max_found=4000)
+    for i in range(q.shape[0]):
+        d2 = np.sum((pts - q[i]) ** 2, axis=1, dtype=np.float32)
+        want = set(np.nonzero(d2 <= np.float32(r * r))[0].tolist())
+        got = out["found"][i, :out["found_count"][i]].tolist()
+        assert set(got) == want
+        assert len(got) >= len(want)  # duplicates only when two of the 8 cells share a bucket
+
+
+def test_iteration_constants():
+    n, r0, alpha = 512 * 512, 0.07, 0.75
+    for k in (1, 2, 10, 1000):
+        c = O.vcm_constants(n, 1, r0, alpha, k)
+        r = r0 / k ** (0.5 * (1 - alpha))
+        assert c["radius"] == pytest.approx(r, rel=1e-6)
+        assert c["radius_sq"] == pytest.approx(r * r, rel=1e-6)
+        assert c["vm_weight"] == pytest.approx(np.pi * r * r * n, rel=1e-6)
+        assert c["vc_weight"] * c["vm_weight"] == pytest.approx(1.0, rel=1e-6)
+        assert c["vm_normalization"] == pytest.approx(1.0 / (np.pi * r * r * n), rel=1e-6)
+    assert O.vcm_constants(n, 1, r0, 1.0, 77)["radius"] == pytest.approx(r0, rel=1e-7)
+
+
+@pytest.fixture(scope="module")
+def plane():
+    sc = scenes.plane_scene()
+    return sc, O.RenderScene(sc)
+
+
+def test_vcm_plane_closed_form(plane):
+    sc, rs = plane
+    w, h = 64, 48
+    ref = scenes.plane_image(sc, w, h)
+    # merge radius 0.03: merging carries about half the weight of the direct paths here, while its
+    # blur of the smooth irradiance stays near 0.4 %
+    fb, st = O.render_vcm(rs, sc["camera"], w, h, 64, 10, 0.03, 0.75, 3)
+    assert np.isfinite(fb).all()
+    ratio = fb.mean((0, 1)) / ref.mean((0, 1))
+    assert np.all(np.abs(ratio - 1) < 0.01), ratio
+    blk = lambda a: a.reshape(6, 8, 8, 8, 3).mean((1, 3))
+    br = blk(fb) / blk(ref)
+    assert np.all(np.abs(br - 1) < 0.07), (br.min(), br.max())
+    assert st["light_vertices_per_iteration"] > 0.3 * w * h
+
+
+def test_pt_plane_closed_form(plane):
+    """The path tracer with the corrected quad-light estimator reproduces the same closed form;
+    Selas's own estimator (extra Dot(normal, l)) darkens the quad light's share."""
+    sc, rs = plane
+    w, h = 64, 48
+    ref = scenes.plane_image(sc, w, h)
+    pt = O.render_pt(rs, sc["camera"], w, h, 64, 10, 3, 1)[1]
+    assert np.all(np.abs(pt.mean((0, 1)) / ref.mean((0, 1)) - 1) < 0.01)
+    sel = O.render_pt(rs, sc["camera"], w, h, 64, 10, 3, 0)[1]
+    assert np.all(sel.mean((0, 1)) / ref.mean((0, 1)) < 0.97)
+
+
+def _lambert_room():
+    sc = scenes.dim_room()
+    sc["materials"][-1, 20] = 2  # the sphere as Lambert too
+    return sc
+
+
+def test_vcm_matches_pt_on_lambert_room():
+    sc = _lambert_room()
+    rs = O.RenderScene(sc)
+    w, h = 64, 48
+    v = np.mean([O.render_vcm(rs, sc["camera"], w, h, 128, 10, 0.0013, 0.75, 100 + s)[0].mean() for s in range(4)])
+    p = np.mean([O.render_pt(rs, sc["camera"], w, h, 128, 10, 200 + s, 1)[1].mean() for s in range(4)])
+    assert abs(v / p - 1) < 0.002, v / p
+
+
+def test_vcm_matches_pt_between_two_planes():
+    """Multi-bounce paths with merging carrying real weight (radius 0.05): a dropped term of the
+    dVC / dVM recurrences or of a connection's MIS weight moves the image mean by 2-20 %."""
+    sc = scenes.sandwich_scene(0.85)
+    rs = O.RenderScene(sc)
+    v = np.mean([O.render_vcm(rs, sc["camera"], 64, 48, 128, 30, 0.05, 0.75, 100 + s)[0].mean() for s in range(4)])
+    p = np.mean([O.render_pt(rs, sc["camera"], 32, 24, 2048, 30, 200 + s, 1)[1].mean() for s in range(4)])
+    assert abs(v / p - 1) < 0.01, v / p
+
+
+def test_vcm_disney_room_close_to_pt():
+    """With the Disney sphere the reference's sampled-lobe pdfs (SampleDisney) and evaluated pdfs
+    (EvaluateDisney) differ, so the MIS weights built from both do not sum to one exactly; the
+    image stays within 2 % of the path tracer's."""
+    sc = scenes.dim_room()
+    rs = O.RenderScene(sc)
+    w, h = 64, 48
+    v = O.render_vcm(rs, sc["camera"], w, h, 128, 10, 0.0013, 0.75, 100)[0].mean()
+    p = O.render_pt(rs, sc["camera"], w, h, 128, 10, 200, 1)[1].mean()
+    assert abs(v / p - 1) < 0.02, v / p
+
+
+def test_vcm_independent_of_thread_count():
+    sc = scenes.dim_room()
+    rs = O.RenderScene(sc)
+    a = O.render_vcm(rs, sc["camera"], 40, 24, 2, 10, 0.005, 0.75, 9, threads=1)[0]
+    b = O.render_vcm(rs, sc["camera"], 40, 24, 2, 10, 0.005, 0.75, 9, threads=7)[0]
+    assert np.array_equal(a, b)
+
+
+def test_vcm_iteration_base_splits_a_render():
+    """Iterations are independent given their global index: rendering [0,4) equals the average of
+    [0,2) and [2,4) (the split used for progressive and batched rendering)."""
+    sc = scenes.dim_room()
+    rs = O.RenderScene(sc)
+    args = (sc["camera"], 32, 24)
+    full = O.render_vcm(rs, *args, 4, 10, 0.005, 0.75, 4)[0]
+    a = O.render_vcm(rs, *args, 2, 10, 0.005, 0.75, 4, iteration_base=0)[0]
+    b = O.render_vcm(rs, *args, 2, 10, 0.005, 0.75, 4, iteration_base=2)[0]
+    np.testing.assert_allclose(full, 0.5 * (a + b), rtol=2e-6, atol=1e-9)
+    c = O.render_vcm(rs, *args, 2, 10, 0.005, 0.75, 5, iteration_base=2)[0]
+    assert not np.array_equal(b, c)
+
+
+def test_vcm_without_lights_is_black():
+    sc = scenes.dim_room()
+    sc["lights"] = sc["lights"][:0]
+    rs = O.RenderScene(sc)
+    fb, st = O.render_vcm(rs, sc["camera"], 16, 16, 2, 10, 0.005, 0.75, 1)
+    assert not fb.any()
+    assert st["light_vertices_per_iteration"] == 0
