@@ -159,6 +159,15 @@ lib.orc_camera_init.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(_C
 lib.orc_camera_cmj_dir.argtypes = [C.POINTER(_Camera)] + [C.c_int32] * 6 + [C.c_void_p]
 
 
+lib.orc_set_libm.argtypes = [C.c_int]
+
+
+def set_libm_single(single: bool):
+    """Evaluate cos/sin/acos/pow with the single-precision libm, as the reference's code does
+    (True), or in double rounded to float, as the device code does (False, the default)."""
+    lib.orc_set_libm(1 if single else 0)
+
+
 def bsdf_eval(blocks, n, v, l):
     blocks, n, v, l = _f(blocks), _f(n), _f(v), _f(l)
     out = np.zeros((blocks.shape[0], 5), np.float32)

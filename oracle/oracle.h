@@ -75,6 +75,9 @@ typedef struct {
     float vipd; /* virtualImagePlaneDistance */
 } orc_camera;
 
+/* Transcendentals: 0 = double evaluated, rounded to float (as the device code); 1 = the
+ * single-precision libm functions (as Selas's code). See orc_fn.h. */
+void orc_set_libm(int single);
 float orc_uniform(orc_uniforms* u);
 uint32_t orc_pcg32_next(uint64_t* state, uint64_t inc);
 void orc_pcg32_seed(uint64_t initstate, uint64_t initseq, uint64_t* state, uint64_t* inc);

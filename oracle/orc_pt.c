@@ -19,6 +19,7 @@
  * Also the timed CPU path of the C1 benchmark ("port" baseline).
  */
 #include "oracle.h"
+#include "orc_fn.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -67,9 +68,9 @@ void orc_surface_from_hit(const float* o, const float* d, float t, const float* 
     float block[22];
     memcpy(block, mat, sizeof(block));
     const float* bc = texel ? texel : mat;
-    block[0] = powf(bc[0], 2.2f);
-    block[1] = powf(bc[1], 2.2f);
-    block[2] = powf(bc[2], 2.2f);
+    block[0] = pow_f(bc[0], 2.2f);
+    block[1] = pow_f(bc[1], 2.2f);
+    block[2] = pow_f(bc[2], 2.2f);
     block[10] = saturate(mat[10]);
     block[11] = saturate(mat[11]);
     block[12] = mat[12] * 0.5f;

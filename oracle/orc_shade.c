@@ -23,10 +23,15 @@
  * (FloatFuncs.h:147-151), so CalculateExtinction returns s * scatterDistance.
  */
 #include "oracle.h"
+#include "orc_fn.h"
 
 #include <math.h>
 #include <stdint.h>
 #include <string.h>
+
+int orc_libm_single = 0;
+
+void orc_set_libm(int single) { orc_libm_single = single ? 1 : 0; }
 
 static const float kPi = 3.141592653589793f;
 
@@ -167,7 +172,7 @@ float orc_schlick_weight(float u)
 static inline float schlick_scalar(float r0, float c) { return lerp(1.0f, orc_schlick_weight(c), r0); }
 static inline ov3 schlick3(ov3 r0, float c)
 {
-    float e = powf(1.0f - c, 5.0f);
+    float e = pow_f(1.0f - c, 5.0f);
     return add(r0, scl(sub(v3(1.0f, 1.0f, 1.0f), r0), e));
 }
 float orc_schlick_r0(float eta) { return square(eta - 1.0f) / square(eta + 1.0f); }
@@ -225,8 +230,8 @@ static ov3 sample_vndf(ov3 wo, float ax, float ay, float u1, float u2)
     float a = 1.0f / (1.0f + v.y);
     float r = sqrtf(u1);
     float phi = (u2 < a) ? (u2 / a) * kPi : kPi + (u2 - a) / (1.0f - a) * kPi;
-    float p1 = r * cosf(phi);
-    float p2 = r * sinf(phi) * ((u2 < a) ? 1.0f : v.y);
+    float p1 = r * cos_f(phi);
+    float p2 = r * sin_f(phi) * ((u2 < a) ? 1.0f : v.y);
     ov3 n = add(add(scl(t1, p1), scl(t2, p2)), scl(v, sqrtf(maxf(0.0f, 1.0f - p1 * p1 - p2 * p2))));
     return normalize(v3(ax * n.x, n.y, ay * n.z));
 }
@@ -506,10 +511,10 @@ static int sample_clearcoat(orc_uniforms* u, const orc_surface* s, ov3 v, bsdf_s
     ov3 wo = normalize(to_local(s, v));
     float a = 0.25f, a2 = a * a;
     float r0 = orc_uniform(u), r1 = orc_uniform(u);
-    float ct = sqrtf(maxf(0, (1.0f - powf(a2, 1.0f - r0)) / (1.0f - a2)));
+    float ct = sqrtf(maxf(0, (1.0f - pow_f(a2, 1.0f - r0)) / (1.0f - a2)));
     float st = sqrtf(maxf(0, 1.0f - ct * ct));
     float phi = (kPi * 2.0f) * r1;
-    ov3 wm = v3(st * cosf(phi), ct, st * sinf(phi));
+    ov3 wm = v3(st * cos_f(phi), ct, st * sin_f(phi));
     if (dot(wm, wo) < 0.0f) wm = neg(wm);
     ov3 wi = reflect(wm, wo);
     if (dot(wi, wo) < 0.0f) return 0;
@@ -594,7 +599,7 @@ static int sample_diffuse(orc_uniforms* u, const orc_surface* s, ov3 v, int thin
     float r0 = orc_uniform(u), r1 = orc_uniform(u);
     float r = sqrtf(r0);
     float theta = (kPi * 2.0f) * r1;
-    ov3 wi = scl(v3(r * cosf(theta), sqrtf(maxf(0.0f, 1 - r0)), r * sinf(theta)), sign);
+    ov3 wi = scl(v3(r * cos_f(theta), sqrtf(maxf(0.0f, 1 - r0)), r * sin_f(theta)), sign);
     ov3 wm = normalize(add(wi, wo));
     float nl = cos_theta(wi);
     if (nl == 0.0f) {
@@ -700,7 +705,7 @@ int orc_lambert_sample(orc_uniforms* u, const orc_surface* s, const float* vv, f
     float r0 = orc_uniform(u), r1 = orc_uniform(u);
     float r = sqrtf(r0);
     float theta = (kPi * 2.0f) * r1;
-    ov3 wi = v3(r * cosf(theta), sqrtf(maxf(0.0f, 1 - r0)), r * sinf(theta));
+    ov3 wi = v3(r * cos_f(theta), sqrtf(maxf(0.0f, 1 - r0)), r * sin_f(theta));
     ov3 w = normalize(to_world(s, wi));
     memset(out, 0, 14 * sizeof(float));
     out[0] = 1.0f;
@@ -856,12 +861,12 @@ void orc_rect_light(const float* q, float u, float v, float* out)
     ov3 v00 = v3(x0, y0, z0), v01 = v3(x0, y1, z0), v10 = v3(x1, y0, z0), v11 = v3(x1, y1, z0);
     ov3 n0 = normalize(cross(v00, v10)), n1 = normalize(cross(v10, v11));
     ov3 n2 = normalize(cross(v11, v01)), n3 = normalize(cross(v01, v00));
-    float g0 = acosf(-dot(n0, n1)), g1 = acosf(-dot(n1, n2)), g2 = acosf(-dot(n2, n3)), g3 = acosf(-dot(n3, n0));
+    float g0 = acos_f(-dot(n0, n1)), g1 = acos_f(-dot(n1, n2)), g2 = acos_f(-dot(n2, n3)), g3 = acos_f(-dot(n3, n0));
     float b0 = n0.z, b1 = n2.z, b0sq = b0 * b0;
     float k = 2 * kPi - g2 - g3;
     float S = g0 + g1 - k;
     float au = u * S + k;
-    float fu = (cosf(au) * b0 - b1) / sinf(au);
+    float fu = (cos_f(au) * b0 - b1) / sin_f(au);
     float cu = 1 / sqrtf(fu * fu + b0sq) * (fu > 0 ? +1 : -1);
     cu = clampf(cu, -1.0f, 1.0f);
     float xu = -(cu * z0) / sqrtf(1 - cu * cu);
@@ -937,8 +942,8 @@ void orc_concentric_disc(float r0, float r1, float* out)
                 phi = 0;
         }
     }
-    out[0] = r * cosf(phi);
-    out[1] = r * sinf(phi);
+    out[0] = r * cos_f(phi);
+    out[1] = r * sin_f(phi);
 }
 
 /* ---- batch forms (one call per array; used by the tests) ------------------------------------- */

@@ -30,6 +30,7 @@
  * Also the timed CPU path of the VCM benchmark ("port" baseline).
  */
 #include "oracle.h"
+#include "orc_fn.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -69,7 +70,7 @@ static inline int is_zero(ov3 a) { return a.x == 0 && a.y == 0 && a.z == 0; }
 void orc_vcm_constants(uint32_t vm_count, uint32_t vc_count, float base_radius, float alpha, float iteration,
                        float* out6)
 {
-    float r = base_radius / powf(iteration, 0.5f * (1.0f - alpha));
+    float r = base_radius / pow_f(iteration, 0.5f * (1.0f - alpha));
     float r2 = r * r;
     out6[0] = r;
     out6[1] = r2;
@@ -215,7 +216,7 @@ static void emit_light(orc_uniforms* u, const orc_light* lights, uint32_t n_ligh
         float uu = 2.0f * r1 - 1.0f;
         float norm = sqrtf(maxf(0.0f, 1.0f - uu * uu));
         float theta = (kPi * 2.0f) * r0;
-        s->direction = v3(norm * cosf(theta), uu, norm * sinf(theta));
+        s->direction = v3(norm * cos_f(theta), uu, norm * sin_f(theta));
         s->position = ld3(L->position);
         emission_pdf_w = (1.0f / (4.0f * kPi)) * p_pick;
         direct_pdf_a = 1.0f * p_pick;
@@ -227,7 +228,7 @@ static void emit_light(orc_uniforms* u, const orc_light* lights, uint32_t n_ligh
         float r0 = orc_uniform(u), r1 = orc_uniform(u);
         float r = sqrtf(r0);
         float th = (kPi * 2.0f) * r1;
-        ov3 local = v3(r * cosf(th), sqrtf(maxf(0.0f, 1 - r0)), r * sinf(th));
+        ov3 local = v3(r * cos_f(th), sqrtf(maxf(0.0f, 1 - r0)), r * sin_f(th));
         float t[3], bb[3];
         ov3 n = ld3(L->direction);
         float nn[3] = {n.x, n.y, n.z};
@@ -327,9 +328,9 @@ static void surface_of_vertex(const vcm_ctx* c, const vcm_vertex* v, orc_surface
     const float* mat = c->sc->materials + 22 * (size_t)c->sc->geom_material[v->geom];
     float block[22];
     memcpy(block, mat, sizeof(block));
-    block[0] = powf(mat[0], 2.2f);
-    block[1] = powf(mat[1], 2.2f);
-    block[2] = powf(mat[2], 2.2f);
+    block[0] = pow_f(mat[0], 2.2f);
+    block[1] = pow_f(mat[1], 2.2f);
+    block[2] = pow_f(mat[2], 2.2f);
     block[10] = saturate(mat[10]);
     block[11] = saturate(mat[11]);
     block[12] = mat[12] * 0.5f;

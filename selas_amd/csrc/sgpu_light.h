@@ -50,7 +50,7 @@ SGPU_HD void rect_init(f3 s, f3 ex, f3 ey, f3 o, RectSampler& r)
        v11 = mk3(r.x1, r.y1, r.z0);
     f3 n0 = normalize3(cross3(v00, v10)), n1 = normalize3(cross3(v10, v11));
     f3 n2 = normalize3(cross3(v11, v01)), n3 = normalize3(cross3(v01, v00));
-    float g0 = acosf(-dot3(n0, n1)), g1 = acosf(-dot3(n1, n2)), g2 = acosf(-dot3(n2, n3)), g3 = acosf(-dot3(n3, n0));
+    float g0 = acos_f(-dot3(n0, n1)), g1 = acos_f(-dot3(n1, n2)), g2 = acos_f(-dot3(n2, n3)), g3 = acos_f(-dot3(n3, n0));
     r.b0 = n0.z;
     r.b1 = n2.z;
     r.b0sq = r.b0 * r.b0;
@@ -61,7 +61,7 @@ SGPU_HD void rect_init(f3 s, f3 ex, f3 ey, f3 o, RectSampler& r)
 SGPU_HD f3 rect_sample(const RectSampler& r, float u, float v)
 {
     float au = u * r.S + r.k;
-    float fu = (cosf(au) * r.b0 - r.b1) / sinf(au);
+    float fu = (cos_f(au) * r.b0 - r.b1) / sin_f(au);
     float cu = 1 / sqrtf(fu * fu + r.b0sq) * (fu > 0 ? +1 : -1);
     cu = clamp_(cu, -1.0f, 1.0f);
     float xu = -(cu * r.z0) / sqrtf(1 - cu * cu);

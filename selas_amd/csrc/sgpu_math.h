@@ -80,5 +80,16 @@ SGPU_HD Pcg32 pcg32_seed(uint64_t initstate, uint64_t initseq)
     return r;
 }
 
+// Single-precision transcendentals of the sampling and shading code, evaluated in double and
+// rounded once to float. The double results of the device library and of the host's libm are both
+// within one double ulp of the exact value, so their float roundings agree except when the exact
+// value lies within ~2^-29 (relative) of a float rounding boundary: host and device then make the
+// same sampling decisions and hit the same triangles, path for path. The float values are within
+// one float ulp of the single-precision libm functions Selas calls (std::cos, std::pow, ...).
+SGPU_HD float cos_f(float x) { return (float)cos((double)x); }
+SGPU_HD float sin_f(float x) { return (float)sin((double)x); }
+SGPU_HD float acos_f(float x) { return (float)acos((double)x); }
+SGPU_HD float pow_f(float x, float y) { return (float)pow((double)x, (double)y); }
+
 // Uniform float in [0, 1): the top 24 bits scaled by 2^-24 (exact, never returns 1.0).
 SGPU_HD float pcg32_float(Pcg32& r) { return (float)(pcg32_next(r) >> 8) * (1.0f / 16777216.0f); }

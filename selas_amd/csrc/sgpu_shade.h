@@ -131,7 +131,7 @@ SGPU_HD float schlick_weight(float u)
 SGPU_HD float schlick_scalar(float r0, float c) { return lerp_(1.0f, schlick_weight(c), r0); }
 SGPU_HD f3 schlick3(f3 r0, float c)
 {
-    float e = powf(1.0f - c, 5.0f);
+    float e = pow_f(1.0f - c, 5.0f);
     return r0 + (mk3(1.0f, 1.0f, 1.0f) - r0) * e;
 }
 SGPU_HD float schlick_r0(float eta) { return sq(eta - 1.0f) / sq(eta + 1.0f); }
@@ -184,8 +184,8 @@ SGPU_HD f3 sample_vndf(f3 wo, float ax, float ay, float u1, float u2)
     float a = 1.0f / (1.0f + v.y);
     float r = sqrtf(u1);
     float phi = (u2 < a) ? (u2 / a) * kPi : kPi + (u2 - a) / (1.0f - a) * kPi;
-    float p1 = r * cosf(phi);
-    float p2 = r * sinf(phi) * ((u2 < a) ? 1.0f : v.y);
+    float p1 = r * cos_f(phi);
+    float p2 = r * sin_f(phi) * ((u2 < a) ? 1.0f : v.y);
     f3 n = p1 * t1 + p2 * t2 + sqrtf(max_(0.0f, 1.0f - p1 * p1 - p2 * p2)) * v;
     return normalize3(mk3(ax * n.x, n.y, ay * n.z));
 }
@@ -421,10 +421,10 @@ SGPU_HD bool sample_clearcoat(U& u, const Surface& s, f3 v, BsdfSample& o)
     const float a = 0.25f, a2 = a * a;
     float r0 = u.next();
     float r1 = u.next();
-    float ct = sqrtf(max_(0, (1.0f - powf(a2, 1.0f - r0)) / (1.0f - a2)));
+    float ct = sqrtf(max_(0, (1.0f - pow_f(a2, 1.0f - r0)) / (1.0f - a2)));
     float st = sqrtf(max_(0, 1.0f - ct * ct));
     float phi = (kPi * 2.0f) * r1;
-    f3 wm = mk3(st * cosf(phi), ct, st * sinf(phi));
+    f3 wm = mk3(st * cos_f(phi), ct, st * sin_f(phi));
     if (dot3(wm, wo) < 0.0f) wm = -wm;
     f3 wi = reflect_(wm, wo);
     if (dot3(wi, wo) < 0.0f) return false;
@@ -513,7 +513,7 @@ SGPU_HD bool sample_diffuse(U& u, const Surface& s, f3 v, bool thin, BsdfSample&
     float r1 = u.next();
     float r = sqrtf(r0);
     float theta = (kPi * 2.0f) * r1;
-    f3 wi = sg * mk3(r * cosf(theta), sqrtf(max_(0.0f, 1 - r0)), r * sinf(theta));
+    f3 wi = sg * mk3(r * cos_f(theta), sqrtf(max_(0.0f, 1 - r0)), r * sin_f(theta));
     f3 wm = normalize3(wi + wo);
     float nl = cos_theta(wi);
     if (nl == 0.0f) {
@@ -602,7 +602,7 @@ SGPU_HD bool lambert_sample(U& u, const Surface& s, f3 v, BsdfSample& o)
     float r1 = u.next();
     float r = sqrtf(r0);
     float theta = (kPi * 2.0f) * r1;
-    f3 wi = mk3(r * cosf(theta), sqrtf(max_(0.0f, 1 - r0)), r * sinf(theta));
+    f3 wi = mk3(r * cos_f(theta), sqrtf(max_(0.0f, 1 - r0)), r * sin_f(theta));
     o.flags = kScatter;
     o.reflectance = s.base_color;
     o.wi = normalize3(to_world(s, wi));

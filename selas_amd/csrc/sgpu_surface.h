@@ -33,7 +33,7 @@ SGPU_HD void surface_material(const float* __restrict mat, f3 n, f3 view, const 
     s.n = n;
     make_frame(s.n, s.t, s.b);
     f3 bc = texel ? *texel : mk3(mat[0], mat[1], mat[2]);
-    s.base_color = mk3(powf(bc.x, 2.2f), powf(bc.y, 2.2f), powf(bc.z, 2.2f));
+    s.base_color = mk3(pow_f(bc.x, 2.2f), pow_f(bc.y, 2.2f), pow_f(bc.z, 2.2f));
     s.transmittance_color = mk3(mat[3], mat[4], mat[5]);
     s.sheen = mat[6];
     s.sheen_tint = mat[7];

@@ -199,7 +199,7 @@ __global__ __launch_bounds__(kBlock) void k_light_gen(VcmParams P, PathState ps,
                 float uu = 2.0f * r1 - 1.0f;
                 float norm = sqrtf(max_(0.0f, 1.0f - uu * uu));
                 float theta = (kPi * 2.0f) * r0;
-                d = mk3(norm * cosf(theta), uu, norm * sinf(theta));
+                d = mk3(norm * cos_f(theta), uu, norm * sin_f(theta));
                 o = ld3(L.position);
                 emission_pdf_w = (1.0f / (4.0f * kPi)) * p_pick;
                 direct_pdf_a = 1.0f * p_pick;
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(kBlock) void k_light_gen(VcmParams P, PathState ps,
                 float r0 = u.next(), r1 = u.next();
                 float r = sqrtf(r0);
                 float th = (kPi * 2.0f) * r1;
-                f3 local = mk3(r * cosf(th), sqrtf(max_(0.0f, 1 - r0)), r * sinf(th));
+                f3 local = mk3(r * cos_f(th), sqrtf(max_(0.0f, 1 - r0)), r * sin_f(th));
                 f3 n = ld3(L.direction), t, bb;
                 make_frame(n, t, bb);
                 d = normalize3(t * local.x + n * local.y + bb * local.z);
@@ -945,7 +945,7 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
         const uint32_t gi = job->iteration_base + it;
         // CalculateIterationConstants (VCMCommon.cpp:97-113), vmCount = W*H, vcCount = 1
         float iteration_index = (float)gi + 1.0f;
-        float r = job->base_radius / powf(iteration_index, 0.5f * (1.0f - job->radius_alpha));
+        float r = job->base_radius / pow_f(iteration_index, 0.5f * (1.0f - job->radius_alpha));
         float r2 = r * r;
         P.radius_sq = r2;
         P.vm_norm = 1.0f / (pi * r2 * N);

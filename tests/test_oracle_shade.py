@@ -3,7 +3,8 @@ compiled functions (tests/golden/make_golden.py), and pins what no golden vector
 the light estimators) with closed forms.
 
 Tolerance: the restatement and the reference run the same FP32 operations in the same order with
-the same libm, so values agree to 2 ulp (a looser bound only guards against a libm rebuild).
+the same libm (the oracle's single-precision libm mode, set for this module), so values agree to
+2 ulp (a looser bound only guards against a libm rebuild).
 """
 import os
 
@@ -17,6 +18,13 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 def golden(name):
     return dict(np.load(os.path.join(GOLDEN, name + ".npz")))
+
+
+@pytest.fixture(autouse=True, scope="module")
+def reference_libm():
+    O.set_libm_single(True)
+    yield
+    O.set_libm_single(False)
 
 
 def ulp_close(a, b, ulps=2, atol=1e-30):
@@ -199,3 +207,21 @@ def test_disney_sample_pdf_consistent_with_eval(thin):
     # the diffuse-lobe part of the pdf matches (specular part differs by the double division)
     assert (e[:, 3] > 0).all()
     assert np.all(np.isfinite(s[ok, 2:5]))
+
+
+def test_libm_modes_agree():
+    """The oracle's default mode (transcendentals in double, rounded to float: the device's way)
+    takes the same sampling decisions as the reference's single-precision libm on the recorded
+    inputs, and its values differ only by the propagation of one-ulp differences."""
+    g = golden("disney_sample")
+    blocks = g["params"].copy()
+    blocks[:, 20] = np.where(g["thin"] == 1, 0.0, 1.0)
+    single = O.bsdf_sample(blocks, g["n"], g["v"], g["uniforms"])
+    O.set_libm_single(False)
+    try:
+        double = O.bsdf_sample(blocks, g["n"], g["v"], g["uniforms"])
+    finally:
+        O.set_libm_single(True)
+    assert np.array_equal(single[:, :2], double[:, :2])
+    scale = np.maximum(np.abs(single[:, 2:]).max(axis=1, keepdims=True), 1e-30)
+    assert (np.abs(single[:, 2:] - double[:, 2:]) <= 1e-5 * scale).all()
