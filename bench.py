@@ -1,14 +1,19 @@
-"""Benchmark of the ray-cast hot path on one MI355X.
+"""Benchmarks of the rendering hot path on one MI355X. Prints one JSON line.
 
-Workload (selas_amd/workloads.py): 2,097,152 triangles in 256 Gaussian clusters and 16,777,216
-rays (half primary camera rays, half cosine bounce rays). One step = one closest-hit pass
-(sgpu_intersect) plus one any-hit pass (sgpu_occluded) over all rays, inputs resident in HBM.
-value = rays traced per second over both passes, in Mrays/s.
-
-Prints one JSON line. The roofline object describes the dominant kernel (closest hit):
+--workload vcm (default): vertex connection and merging on the VCM workload scene
+(selas_amd/workloads.py vcm_scene: 64 displaced-grid meshes, 1,048,576 triangles, one quad and one
+point light), 512 x 512, max path length 10, merge radius 0.3 % of the scene diagonal shrinking
+with alpha 0.75. One step = one VCM iteration (a light path and a camera path per pixel); the timed
+region is one sgpu_render_vcm call of `steps` iterations (32 = the workload's job), with the scene
+and the BVH resident in HBM. value = camera samples per second; Mrays/s counts every closest-hit
+and shadow ray. The roofline object describes the closest-hit traversal kernel inside the render:
 algorithmic bytes per ray = 48 B of compulsory ray/hit traffic + nodes visited x 64 B + triangles
-tested x 48 B (counted by the instrumented traversal on all rays), divided by its average duration
-measured with HIP events on the stream it runs on.
+tested x 48 B (counted over all closest-hit rays of an untimed iteration), divided by the kernel's
+device time summed from HIP events recorded around each launch on the render's stream.
+
+--workload traversal: the BVH traversal microbenchmark (2,097,152 triangles in 256 Gaussian
+clusters, 16,777,216 rays, half primary and half bounce rays); one step = one closest-hit pass plus
+one any-hit pass over all rays; value in Mrays/s.
 """
 from __future__ import annotations
 
@@ -31,17 +36,17 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def load_traffic():
-    """HBM bytes per launch of the closest-hit kernel measured with rocprofv3 --pmc on this
-    workload (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), if recorded."""
-    p = os.path.join(ROOT, "profiles", "traffic_bvh_intersect_2Mtri_16Mray.json")
+def load_traffic(name="traffic_bvh_intersect_2Mtri_16Mray.json"):
+    """HBM bytes per launch of the closest-hit kernel measured with rocprofv3 --pmc on a workload
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), if recorded."""
+    p = os.path.join(ROOT, "profiles", name)
     if os.path.exists(p):
         with open(p) as f:
             return json.load(f).get("hbm_bytes_per_launch")
     return None
 
 
-def cpu_baseline(t9, rays, sample_rays, threads):
+def cpu_baseline_traversal(t9, rays, sample_rays, threads):
     import oracle as O
 
     log(f"cpu baseline: building the oracle SAH tree over {t9.shape[0]} triangles")
@@ -70,11 +75,140 @@ def cpu_baseline(t9, rays, sample_rays, threads):
     }
 
 
+def cpu_baseline_vcm(sc, width, iterations, threads):
+    import oracle as O
+
+    log(f"cpu baseline: building the oracle SAH tree over {sum(g.shape[0] for g in sc['geoms'])} triangles")
+    rs = O.RenderScene(sc)
+    log(f"cpu baseline: VCM {width}x{width}, {iterations} iterations on {threads} threads")
+    t0 = time.perf_counter()
+    fb, st = O.render_vcm(rs, sc["camera"], width, width, iterations, sc["max_path_length"], sc["merge_radius"],
+                          sc["radius_alpha"], 1, threads=threads)
+    secs = time.perf_counter() - t0
+    return {
+        "value": round(width * width * iterations / secs, 1),
+        "unit": "samples/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{iterations} VCM iterations at {width}x{width} on the same scene ({secs:.2f} s, "
+                  f"{st['rays'] / secs / 1e6:.2f} Mrays/s), oracle restatement (scalar C, oracle SAH tree, not Embree)",
+        "image_mean": round(float(fb.mean()), 5),
+    }
+
+
+def bench_vcm(args):
+    from selas_amd import native as N
+    from selas_amd import workloads as W
+
+    steps = 32 if args.steps is None else args.steps
+    w = args.width
+    log("building VCM workload scene")
+    sc = W.vcm_scene()
+    n_tris = sum(g.shape[0] for g in sc["geoms"])
+    ctx = N.Context(0)
+    tb = time.perf_counter()
+    scene = N.build_render_scene(ctx, sc)
+    build_ms = (time.perf_counter() - tb) * 1e3
+    info = scene.info()
+    log(f"scene: {info}, upload + build {build_ms:.1f} ms")
+    L, r, alpha = sc["max_path_length"], sc["merge_radius"], sc["radius_alpha"]
+    cam = sc["camera"]
+    fb = ctx.empty(3 * w * w * 4)
+
+    # traversal counts over the closest-hit rays of one untimed iteration
+    _, cnt = N.render_vcm(scene, cam, w, w, 1, L, r, alpha, 1, iteration_base=0, return_stats=True,
+                          flags=N.VCM_COUNT_TRAVERSAL, framebuffer=fb)
+    nodes_per_ray = cnt["nodes_visited"] / cnt["closest_rays"]
+    tris_per_ray = cnt["tris_tested"] / cnt["closest_rays"]
+    bytes_per_ray = 48 + nodes_per_ray * 64 + tris_per_ray * 48
+    log(f"closest hit in VCM: {nodes_per_ray:.2f} nodes/ray, {tris_per_ray:.2f} tris/ray, {bytes_per_ray:.1f} B/ray")
+    if args.warmup:
+        N.render_vcm(scene, cam, w, w, args.warmup, L, r, alpha, 1, framebuffer=fb)
+    N.synchronize()
+
+    e0, e1 = N.Event(), N.Event()
+    t0 = time.perf_counter()
+    e0.record()
+    _, st = N.render_vcm(scene, cam, w, w, steps, L, r, alpha, 1, return_stats=True, flags=N.VCM_KERNEL_TIMES,
+                         framebuffer=fb)
+    e1.record()
+    N.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_ms(e1)
+    scene.check()
+    img = fb.download(np.float32, 3 * w * w)
+    fb.free()
+    secs = ms / 1e3
+    samples = w * w * steps
+    value = samples / secs
+    closest_ms = st["closest_ms"]
+    achieved = st["closest_rays"] * bytes_per_ray / (closest_ms / 1e3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_vcm(sc, w, args.cpu_iterations, min(16, os.cpu_count() or 1))
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms / steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"vcm_{w}x{w}_{steps}it_L{L}_1Mtri",
+            "triangles": int(n_tris),
+            "width": w,
+            "height": w,
+            "iterations": steps,
+            "max_path_length": L,
+            "merge_radius": round(float(r), 6),
+            "radius_alpha": alpha,
+            "lights": "one quad light + one point light",
+            "textures": "none (materials use constant base colours)",
+        },
+        "mrays_s": round(st["rays"] / secs / 1e6, 2),
+        "rays_per_sample": round(st["rays"] / samples, 3),
+        "closest_rays": st["closest_rays"],
+        "shadow_rays": st["shadow_rays"],
+        "light_vertices_per_iteration": round(st["light_vertices"] / steps, 1),
+        "phase_ms": {k: round(st[k], 2) for k in ("closest_ms", "shadow_ms", "light_ms", "camera_ms", "grid_ms")},
+        "bvh_build_ms": round(build_ms, 1),
+        "wall_s_timed_region": round(wall, 3),
+        "image_mean": round(float(img.mean()), 5),
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": load_traffic("traffic_bvh_intersect_vcm_1Mtri.json"),
+            "kernel": "k_trace<false,false> (closest hit, all launches of the render)",
+            "launches": st["closest_launches"],
+            "avg_launch_ms": round(closest_ms / max(1, st["closest_launches"]), 4),
+            "bytes_per_ray": round(bytes_per_ray, 1),
+            "nodes_per_ray": round(nodes_per_ray, 3),
+            "tris_per_ray": round(tris_per_ray, 3),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=("vcm", "traversal"), default="vcm")
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (vcm: 32 iterations, traversal: 10)")
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--width", type=int, default=512, help="vcm: image width and height")
+    ap.add_argument("--cpu-iterations", type=int, default=2, help="vcm: iterations in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-rays", type=int, default=4194304, help="rays in the CPU baseline sample")
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the triangles (1.0 = the workload)")
@@ -94,10 +228,20 @@ def main():
         return
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the ray-cast "
-              "workload shards by ray ranges with a replicated BVH, which is not measured here.", file=sys.stderr)
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}). VCM would shard "
+              "by iterations (each GPU its own light vertices and photon grid, one sum of the frame "
+              "buffers at the end) and ray casts by ray ranges with a replicated BVH; neither is "
+              "measured here.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "vcm":
+        bench_vcm(args)
+    else:
+        if args.steps is None:
+            args.steps = 10
+        bench_traversal(args)
 
+
+def bench_traversal(args):
     from selas_amd import native as N
     from selas_amd import workloads as W
 
@@ -159,7 +303,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(t9, rays, args.cpu_rays, min(16, os.cpu_count() or 1))
+        cpu = cpu_baseline_traversal(t9, rays, args.cpu_rays, min(16, os.cpu_count() or 1))
 
     out = {
         "metric": METRIC,

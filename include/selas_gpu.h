@@ -223,6 +223,9 @@ typedef struct sgpu_pt_job {
  * averaged over spp. Synchronous on `stream`. */
 int sgpu_render_pt(sgpu_scene scene, const sgpu_pt_job* job, float* framebuffer, void* stream);
 
+#define SGPU_VCM_KERNEL_TIMES 1u    /* time each phase with events (sgpu_vcm_stats.*_ms) */
+#define SGPU_VCM_COUNT_TRAVERSAL 2u /* count nodes and triangles visited by closest-hit rays (slower) */
+
 typedef struct sgpu_vcm_job {
     sgpu_camera camera;
     uint32_t iterations;      /* VCM iterations: one light path and one camera path per pixel each */
@@ -230,14 +233,25 @@ typedef struct sgpu_vcm_job {
     uint32_t max_path_length; /* maxPathLength, 1..31 */
     float base_radius;        /* merge radius of iteration 0 (vmSearchRadius) */
     float radius_alpha;       /* radius shrink exponent (vmRadiusAlpha) */
-    uint32_t reserved;        /* must be 0 */
+    uint32_t flags;           /* SGPU_VCM_* */
     uint64_t seed;            /* PCG32 seed; path p of iteration i, pass s uses stream (i << 33 | p << 1 | s) */
 } sgpu_vcm_job;
 
 typedef struct sgpu_vcm_stats {
-    uint64_t rays;           /* closest-hit and shadow rays traced */
-    uint64_t light_vertices; /* light vertices stored, summed over iterations */
-    uint64_t samples;        /* camera paths = width x height x iterations */
+    uint64_t rays;             /* closest-hit and shadow rays traced */
+    uint64_t closest_rays;     /* closest-hit rays (light and camera subpath segments) */
+    uint64_t shadow_rays;      /* occlusion rays (NEE, vertex connections, light-tracing splats) */
+    uint64_t closest_launches; /* closest-hit traversal launches */
+    uint64_t light_vertices;   /* light vertices stored, summed over iterations */
+    uint64_t samples;          /* camera paths = width x height x iterations */
+    uint64_t nodes_visited;    /* SGPU_VCM_COUNT_TRAVERSAL: over all closest-hit rays */
+    uint64_t tris_tested;      /* SGPU_VCM_COUNT_TRAVERSAL: over all closest-hit rays */
+    float closest_ms;          /* SGPU_VCM_KERNEL_TIMES: device time of the closest-hit traversals */
+    float shadow_ms;           /* ... of the occlusion traversals */
+    float light_ms;            /* ... of the light-subpath kernels (emission, shading, splats) */
+    float camera_ms;           /* ... of the camera-subpath kernels (NEE, connections, merging) */
+    float grid_ms;             /* ... of the vertex compaction and photon grid build */
+    float reserved_ms;
 } sgpu_vcm_stats;
 
 /* Vertex connection and merging: replaces VCM::GenerateImage and its helpers

@@ -355,7 +355,8 @@ int sgpu_render_vcm(sgpu_scene scene, const sgpu_vcm_job* job, float* framebuffe
         return sgpu_fail(SGPU_E_RANGE, "sgpu_render_vcm: more than 2^27 geometries");
     if (job->camera.width == 0 || job->camera.height == 0 || job->iterations == 0 || job->max_path_length == 0)
         return sgpu_fail(SGPU_E_INVALID, "sgpu_render_vcm: zero width, height, iterations or path length");
-    if (job->reserved != 0) return sgpu_fail(SGPU_E_INVALID, "sgpu_render_vcm: reserved field is not 0");
+    if (job->flags & ~(SGPU_VCM_KERNEL_TIMES | SGPU_VCM_COUNT_TRAVERSAL))
+        return sgpu_fail(SGPU_E_INVALID, "sgpu_render_vcm: unknown flags");
     if (!(job->base_radius > 0.0f) || !(job->base_radius < 3.0e38f))
         return sgpu_fail(SGPU_E_INVALID, "sgpu_render_vcm: base_radius must be positive and finite");
     if (!(job->radius_alpha >= 0.0f && job->radius_alpha <= 1.0f))

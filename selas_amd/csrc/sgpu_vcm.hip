@@ -31,6 +31,8 @@
 #include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
+#include <cstring>
+#include <vector>
 #include <cmath>
 
 namespace {
@@ -833,6 +835,55 @@ struct Arena {
     }
 };
 
+// Device time per phase, from event pairs recorded around each phase's launches on the render's
+// stream (only when asked for: SGPU_VCM_KERNEL_TIMES). collect() runs after a stream sync.
+enum { kPhaseClosest, kPhaseShadow, kPhaseLight, kPhaseCamera, kPhaseGrid, kPhases };
+struct PhaseTimer {
+    bool on;
+    hipStream_t stream;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> phase;
+    size_t used = 0;
+    double ms[kPhases] = {};
+    PhaseTimer(bool enabled, hipStream_t s) : on(enabled), stream(s) {}
+    hipError_t begin(int ph)
+    {
+        if (!on) return hipSuccess;
+        while (ev.size() < 2 * (used + 1)) {
+            hipEvent_t e;
+            hipError_t err = hipEventCreate(&e);
+            if (err != hipSuccess) return err;
+            ev.push_back(e);
+        }
+        if (phase.size() < used + 1) phase.resize(used + 1);
+        phase[used] = ph;
+        return hipEventRecord(ev[2 * used], stream);
+    }
+    hipError_t end()
+    {
+        if (!on) return hipSuccess;
+        hipError_t err = hipEventRecord(ev[2 * used + 1], stream);
+        ++used;
+        return err;
+    }
+    hipError_t collect()
+    {
+        if (!on) return hipSuccess;
+        hipError_t err = hipStreamSynchronize(stream);
+        for (size_t i = 0; i < used && err == hipSuccess; ++i) {
+            float t = 0.0f;
+            err = hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]);
+            ms[phase[i]] += t;
+        }
+        used = 0;
+        return err;
+    }
+    ~PhaseTimer()
+    {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+};
+
 sgpu_rays_soa as_soa(const RayBuf& r) { return sgpu_rays_soa{r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, r.tnear, r.tfar}; }
 
 inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
@@ -939,7 +990,14 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
 
     SGPU_HIP_TRY(hipMemsetAsync(fb, 0, 3 * (size_t)N * sizeof(float), stream));
     sgpu_hits_soa hsoa{hb.t, hb.u, hb.v, hb.prim, hb.geom, hb.ngx, hb.ngy, hb.ngz};
-    uint64_t n_rays = 0, n_light_verts = 0;
+    uint64_t n_closest = 0, n_shadow = 0, n_light_verts = 0, n_launches = 0;
+    uint64_t* d_trav = nullptr;  // nodes visited, triangles tested by the closest-hit rays
+    if (job->flags & SGPU_VCM_COUNT_TRAVERSAL) {
+        d_trav = A.get<uint64_t>(2);
+        if (A.err != hipSuccess) return sgpu_fail(SGPU_E_NOMEM, "sgpu_render_vcm: device allocation failed");
+        SGPU_HIP_TRY(hipMemsetAsync(d_trav, 0, 2 * sizeof(uint64_t), stream));
+    }
+    PhaseTimer T(job->flags & SGPU_VCM_KERNEL_TIMES, stream);
 
     for (uint32_t it = 0; it < job->iterations; ++it) {
         const uint32_t gi = job->iteration_base + it;
@@ -959,39 +1017,51 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
         SGPU_HIP_TRY(hipMemsetAsync(counters, 0, 4 * sizeof(uint32_t), stream));
 
         // ---- light pass ----
+        SGPU_HIP_TRY(T.begin(kPhaseLight));
         hipLaunchKernelGGL(k_light_gen, dim3(blocks_for(N)), dim3(kBlock), 0, stream, P, ps, rays[0], queue[0], counters);
         SGPU_HIP_TRY(hipGetLastError());
+        SGPU_HIP_TRY(T.end());
         SGPU_HIP_TRY(hipMemcpyAsync(h, counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
         SGPU_HIP_TRY(hipStreamSynchronize(stream));
         uint32_t count = h[0];
         int cur = 0;
         while (count > 0) {
             sgpu_rays_soa rs = as_soa(rays[cur]);
-            int rc = sgpu_trace_launch(scene, &rs, &hsoa, nullptr, count, false, nullptr, stream);
+            SGPU_HIP_TRY(T.begin(kPhaseClosest));
+            int rc = sgpu_trace_launch(scene, &rs, &hsoa, nullptr, count, false, d_trav, stream);
             if (rc) return rc;
-            n_rays += count;
+            SGPU_HIP_TRY(T.end());
+            n_closest += count;
+            ++n_launches;
             SGPU_HIP_TRY(hipMemsetAsync(counters, 0, 4 * sizeof(uint32_t), stream));
+            SGPU_HIP_TRY(T.begin(kPhaseLight));
             hipLaunchKernelGGL(k_light_shade, dim3(blocks_for(count)), dim3(kBlock), 0, stream, P, count, queue[cur],
                                rays[cur], hb, ps, verts, vcount, rays[1 - cur], queue[1 - cur], shadow, splat_c,
                                splat_pix, counters);
             SGPU_HIP_TRY(hipGetLastError());
+            SGPU_HIP_TRY(T.end());
             SGPU_HIP_TRY(hipMemcpyAsync(h, counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
             SGPU_HIP_TRY(hipStreamSynchronize(stream));
             uint32_t n_next = h[0], n_sh = h[1];
             if (n_sh > 0) {
                 sgpu_rays_soa ss = as_soa(shadow);
+                SGPU_HIP_TRY(T.begin(kPhaseShadow));
                 rc = sgpu_trace_launch(scene, &ss, nullptr, occl, n_sh, true, nullptr, stream);
                 if (rc) return rc;
-                n_rays += n_sh;
+                SGPU_HIP_TRY(T.end());
+                n_shadow += n_sh;
+                SGPU_HIP_TRY(T.begin(kPhaseLight));
                 hipLaunchKernelGGL(k_splat, dim3(blocks_for(n_sh)), dim3(kBlock), 0, stream, n_sh, occl, splat_c,
                                    splat_pix, frame);
                 SGPU_HIP_TRY(hipGetLastError());
+                SGPU_HIP_TRY(T.end());
             }
             count = n_next;
             cur = 1 - cur;
         }
 
         // ---- light vertices, path-major ----
+        SGPU_HIP_TRY(T.begin(kPhaseGrid));
         SGPU_HIP_TRY(rocprim::exclusive_scan(tmp, tmp_bytes, vcount, voffset, 0u, (size_t)N + 1,
                                              rocprim::plus<uint32_t>(), stream));
         hipLaunchKernelGGL(k_compact, dim3(blocks_for(N)), dim3(kBlock), 0, stream, N, mlv, verts, vcount, voffset,
@@ -1000,12 +1070,14 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
         // path_end[i] = offset[i + 1] (inclusive end); the vertex count is offset[N]
         SGPU_HIP_TRY(hipMemcpyAsync(path_end, voffset + 1, (size_t)N * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
         SGPU_HIP_TRY(hipMemcpyAsync(nv_dev, voffset + N, sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
+        SGPU_HIP_TRY(T.end());
         SGPU_HIP_TRY(hipMemcpyAsync(h, voffset + N, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
         SGPU_HIP_TRY(hipStreamSynchronize(stream));
         const uint32_t nv = h[0];
         n_light_verts += nv;
 
         // ---- photon grid ----
+        SGPU_HIP_TRY(T.begin(kPhaseGrid));
         SGPU_HIP_TRY(hipMemsetAsync(box, 0xFF, 8 * sizeof(uint32_t), stream));
         SGPU_HIP_TRY(hipMemsetAsync(cell_counts, 0, (size_t)N * sizeof(uint32_t), stream));
         if (nv > 0) {
@@ -1027,6 +1099,7 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
         }
         SGPU_HIP_TRY(rocprim::inclusive_scan(tmp, tmp_bytes, cell_counts, cell_ends, (size_t)N,
                                              rocprim::plus<uint32_t>(), stream));
+        SGPU_HIP_TRY(T.end());
         GridView G;
         G.box = box;
         G.cell_ends = cell_ends;
@@ -1037,47 +1110,78 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
         G.nv_ptr = nv_dev;
 
         // ---- camera pass ----
+        SGPU_HIP_TRY(T.begin(kPhaseCamera));
         hipLaunchKernelGGL(k_cam_gen, dim3(blocks_for(N)), dim3(kBlock), 0, stream, P, ps, rays[0], queue[0]);
         SGPU_HIP_TRY(hipGetLastError());
+        SGPU_HIP_TRY(T.end());
         count = L > 1 ? N : 0;  // a camera path of length 1 ends at the lens (VCM.cpp:585)
         cur = 0;
         while (count > 0) {
             sgpu_rays_soa rs = as_soa(rays[cur]);
-            int rc = sgpu_trace_launch(scene, &rs, &hsoa, nullptr, count, false, nullptr, stream);
+            SGPU_HIP_TRY(T.begin(kPhaseClosest));
+            int rc = sgpu_trace_launch(scene, &rs, &hsoa, nullptr, count, false, d_trav, stream);
             if (rc) return rc;
-            n_rays += count;
+            SGPU_HIP_TRY(T.end());
+            n_closest += count;
+            ++n_launches;
             SGPU_HIP_TRY(hipMemsetAsync(counters, 0, 4 * sizeof(uint32_t), stream));
+            SGPU_HIP_TRY(T.begin(kPhaseCamera));
             hipLaunchKernelGGL(k_cam_shade, dim3(blocks_for(count)), dim3(kBlock), 0, stream, P, count, queue[cur],
                                rays[cur], hb, ps, G, S, rays[1 - cur], queue[1 - cur], shadow, counters);
             SGPU_HIP_TRY(hipGetLastError());
+            SGPU_HIP_TRY(T.end());
             SGPU_HIP_TRY(hipMemcpyAsync(h, counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
             SGPU_HIP_TRY(hipStreamSynchronize(stream));
             uint32_t n_next = h[0], n_sh = h[1];
             if (n_sh > 0) {
                 sgpu_rays_soa ss = as_soa(shadow);
+                SGPU_HIP_TRY(T.begin(kPhaseShadow));
                 rc = sgpu_trace_launch(scene, &ss, nullptr, occl, n_sh, true, nullptr, stream);
                 if (rc) return rc;
-                n_rays += n_sh;
+                SGPU_HIP_TRY(T.end());
+                n_shadow += n_sh;
+                SGPU_HIP_TRY(T.begin(kPhaseCamera));
                 hipLaunchKernelGGL(k_cam_resolve, dim3(blocks_for(n_sh)), dim3(kBlock), 0, stream, n_sh, occl, S);
                 SGPU_HIP_TRY(hipGetLastError());
+                SGPU_HIP_TRY(T.end());
             }
+            SGPU_HIP_TRY(T.begin(kPhaseCamera));
             hipLaunchKernelGGL(k_cam_gather, dim3(blocks_for(count)), dim3(kBlock), 0, stream, count, queue[cur],
                                nslots, S, ps);
             SGPU_HIP_TRY(hipGetLastError());
+            SGPU_HIP_TRY(T.end());
             count = n_next;
             cur = 1 - cur;
         }
+        SGPU_HIP_TRY(T.begin(kPhaseCamera));
         hipLaunchKernelGGL(k_frame_add, dim3(blocks_for(N)), dim3(kBlock), 0, stream, N, ps.color, frame, fb);
         SGPU_HIP_TRY(hipGetLastError());
+        SGPU_HIP_TRY(T.end());
+        SGPU_HIP_TRY(T.collect());
     }
     const size_t nf = 3 * (size_t)N;
     hipLaunchKernelGGL(k_scale, dim3(blocks_for(nf)), dim3(kBlock), 0, stream, fb, nf, 1.0f / (float)job->iterations);
     SGPU_HIP_TRY(hipGetLastError());
     SGPU_HIP_TRY(hipStreamSynchronize(stream));
     if (stats) {
-        stats->rays = n_rays;
+        std::memset(stats, 0, sizeof(*stats));
+        stats->rays = n_closest + n_shadow;
+        stats->closest_rays = n_closest;
+        stats->shadow_rays = n_shadow;
+        stats->closest_launches = n_launches;
         stats->light_vertices = n_light_verts;
         stats->samples = (uint64_t)N * job->iterations;
+        if (d_trav) {
+            uint64_t tr[2] = {0, 0};
+            SGPU_HIP_TRY(hipMemcpy(tr, d_trav, sizeof(tr), hipMemcpyDeviceToHost));
+            stats->nodes_visited = tr[0];
+            stats->tris_tested = tr[1];
+        }
+        stats->closest_ms = (float)T.ms[kPhaseClosest];
+        stats->shadow_ms = (float)T.ms[kPhaseShadow];
+        stats->light_ms = (float)T.ms[kPhaseLight];
+        stats->camera_ms = (float)T.ms[kPhaseCamera];
+        stats->grid_ms = (float)T.ms[kPhaseGrid];
     }
     return SGPU_OK;
 }

@@ -349,31 +349,41 @@ def build_render_scene(ctx: Context, sc: dict) -> Scene:
     return scene
 
 
+VCM_KERNEL_TIMES = 1
+VCM_COUNT_TRAVERSAL = 2
+
+
 class VcmJob(C.Structure):
     _fields_ = [("camera", Camera), ("iterations", C.c_uint32), ("iteration_base", C.c_uint32),
                 ("max_path_length", C.c_uint32), ("base_radius", C.c_float), ("radius_alpha", C.c_float),
-                ("reserved", C.c_uint32), ("seed", C.c_uint64)]
+                ("flags", C.c_uint32), ("seed", C.c_uint64)]
 
 
 class VcmStats(C.Structure):
-    _fields_ = [("rays", C.c_uint64), ("light_vertices", C.c_uint64), ("samples", C.c_uint64)]
+    _fields_ = [(k, C.c_uint64) for k in ("rays", "closest_rays", "shadow_rays", "closest_launches",
+                                          "light_vertices", "samples", "nodes_visited", "tris_tested")] + \
+               [(k, C.c_float) for k in ("closest_ms", "shadow_ms", "light_ms", "camera_ms", "grid_ms",
+                                         "reserved_ms")]
 
 
 def render_vcm(scene: Scene, cam: dict, width, height, iterations, max_path_length, base_radius, radius_alpha, seed,
-               iteration_base=0, stream=None, return_stats=False):
-    """VCM image (height x width x 3, radiance averaged over iterations) through sgpu_render_vcm."""
+               iteration_base=0, stream=None, return_stats=False, flags=0, framebuffer=None):
+    """VCM image (height x width x 3, radiance averaged over iterations) through sgpu_render_vcm.
+    With `framebuffer` (a device buffer of 12 x width x height bytes) the image stays on the device
+    and None is returned in its place."""
     ctx = scene.ctx
-    fb = ctx.empty(3 * width * height * 4)
+    fb = framebuffer if framebuffer is not None else ctx.empty(3 * width * height * 4)
     job = VcmJob(Camera.make(cam, width, height), iterations, iteration_base, max_path_length, base_radius,
-                 radius_alpha, 0, seed)
+                 radius_alpha, flags, seed)
     st = VcmStats()
     try:
         check(lib.sgpu_render_vcm(scene.handle, C.byref(job), fb.ptr, C.byref(st), stream))
-        out = fb.download(np.float32, 3 * width * height).reshape(height, width, 3)
+        out = None if framebuffer is not None else fb.download(np.float32, 3 * width * height).reshape(height, width, 3)
     finally:
-        fb.free()
+        if framebuffer is None:
+            fb.free()
     if return_stats:
-        return out, {"rays": st.rays, "light_vertices": st.light_vertices, "samples": st.samples}
+        return out, {k: getattr(st, k) for k, _ in VcmStats._fields_ if k != "reserved_ms"}
     return out
 
 

@@ -184,32 +184,55 @@ def random_rotation(rng):
                      [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
 
 
+def _frame_from_normal(n, twist):
+    """Rotation taking +y to the unit vector n, preceded by a rotation of `twist` radians about y."""
+    n = np.asarray(n, np.float64)
+    a = np.array([1.0, 0.0, 0.0]) if abs(n[0]) < 0.9 else np.array([0.0, 0.0, 1.0])
+    t = np.cross(a, n)
+    t /= np.linalg.norm(t)
+    b = np.cross(t, n)
+    M = np.stack([t, n, b], axis=1)  # columns: images of x, y, z
+    c, s_ = np.cos(twist), np.sin(twist)
+    Ry = np.array([[c, 0, s_], [0, 1, 0], [-s_, 0, c]])
+    return M @ Ry
+
+
 def vcm_scene(seed: int = 3, n_meshes: int = 64, nu: int = 128, nv: int = 64):
     """VCM workload scene: n_meshes displaced-grid meshes of nu x nv quads (2 nu nv triangles each;
-    64 x 16,384 = 1,048,576 triangles), placed by seeded rotation, scale (1.5..3) and translation in
-    a 10^3 box; per-mesh Disney parameters: roughness uniform in [0.05, 1], metallic 0 or 1 (p 0.5),
-    specTrans 1 with probability 0.2 (else 0), base colour uniform in [0.2, 0.9]^3; one 4 x 4 quad
-    light (radiance 20) above the box facing down and one point light (intensity 60); merge radius
-    0.3 % of the scene diagonal."""
+    64 x 16,384 = 1,048,576 triangles) placed by a seeded RNG as panels of an enclosure: each mesh
+    is a unit-by-half grid with value-noise displacement, scaled by 4..8, centred at a random
+    direction and distance 3.5..5 from the centre of the 10^3 box and turned so that its front
+    face (the side its geometric normal points to) faces the centre, with a random twist. Per mesh,
+    Disney parameters: roughness uniform in [0.05, 1], metallic 0 or 1 (p 0.5), specTrans 1 with
+    probability 0.2 (else 0), base colour uniform in [0.2, 0.9]^3. Lights inside the enclosure:
+    one 1.5 x 1.5 quad light (radiance 20) facing down and one point light (intensity 15). The
+    camera is inside too, so every surface it sees faces it (Selas's surfaces are one-sided for
+    opaque materials: no normal flip in CalculateSurfaceParams). Merge radius 0.3 % of the scene
+    diagonal."""
     rng = np.random.default_rng(seed)
     geoms, mats = [], []
+    centre = np.array([5.0, 5.0, 5.0])
     for m in range(n_meshes):
         xs = np.linspace(0, 1, nu + 1)
         zs = np.linspace(0, 0.5, nv + 1)
         X, Z = np.meshgrid(xs, zs, indexing="ij")
         H = 0.12 * value_noise_2d(rng, 8, X, Z * 2)
-        local = quad_grid_mesh(nu, nv, 1.0, 0.5, H).reshape(-1, 3)
-        R = random_rotation(rng)
-        s = rng.uniform(1.5, 3.0)
-        t = rng.uniform(0.0, 10.0, 3)
+        local = quad_grid_mesh(nu, nv, 1.0, 0.5, H).reshape(-1, 3, 3)
+        local = local[:, [0, 2, 1]].reshape(-1, 3)  # wind so the geometric normal is +y
+        local = local - np.array([0.5, 0.0, 0.25])
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        R = _frame_from_normal(-d, rng.uniform(0, 2 * np.pi))
+        s = rng.uniform(4.0, 8.0)
+        t = centre + d * rng.uniform(3.5, 5.0)
         world = (local @ R.T) * s + t
         geoms.append(world.reshape(-1, 9).astype(np.float32))
         metallic = 1.0 if rng.random() < 0.5 else 0.0
         spec_trans = 1.0 if rng.random() < 0.2 else 0.0
         mats.append(material(1, rng.uniform(0.2, 0.9, 3), metallic, rng.uniform(0.05, 1.0), 1.5, spec_trans=spec_trans))
     lights = np.zeros(2, LIGHT_DTYPE)
-    lights[0] = (0, (5.0, 11.0, 5.0), (0.0, -1.0, 0.0), (4.0, 0.0, 0.0), (0.0, 0.0, 4.0), (20.0, 20.0, 20.0))
-    lights[1] = (1, (2.0, 8.0, 8.0), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (60.0, 60.0, 60.0))
+    lights[0] = (0, (5.0, 7.5, 5.0), (0.0, -1.0, 0.0), (1.5, 0.0, 0.0), (0.0, 0.0, 1.5), (20.0, 20.0, 20.0))
+    lights[1] = (1, (4.0, 4.0, 6.0), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (15.0, 15.0, 15.0))
     allp = np.concatenate(geoms).reshape(-1, 3)
     diag = float(np.linalg.norm(allp.max(0) - allp.min(0)))
     return {
@@ -218,7 +241,7 @@ def vcm_scene(seed: int = 3, n_meshes: int = 64, nu: int = 128, nv: int = 64):
         "geom_material": np.arange(n_meshes, dtype=np.uint32),
         "lights": lights,
         "background": np.zeros(3, np.float32),
-        "camera": {"position": (5.0, 6.0, -14.0), "look_at": (5.0, 5.0, 5.0), "up": (0.0, 1.0, 0.0), "fov": 55.0},
+        "camera": {"position": (5.0, 5.0, 3.0), "look_at": (5.0, 4.6, 8.0), "up": (0.0, 1.0, 0.0), "fov": 70.0},
         "merge_radius": 0.003 * diag,
         "radius_alpha": 0.75,
         "max_path_length": 10,

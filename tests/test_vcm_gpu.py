@@ -2,9 +2,10 @@
 restatement, through the C ABI.
 
 Both sides draw each path's random numbers from the same PCG32 stream (one per iteration, path and
-pass) and use bit-identical ray casts and photon grids, so images agree pixel by pixel up to the
-device libm (sinf/cosf/powf differ from glibc by a few ulp; a rare path can take another branch)
-and the order of the atomic adds of light-tracing splats. Bars, written here:
+pass), evaluate cos/sin/acos/pow in double rounded to float, and use bit-identical ray casts and
+photon grids, so paths take the same branches and images agree pixel by pixel up to the order of
+the atomic adds of light-tracing splats (and a path whose double-evaluated function lands within
+2^-29 of a float rounding boundary on one side only). Bars, written here:
   * per pixel: |gpu - cpu| <= 1e-3 |cpu| + 1e-5 for at least 99 % of the pixel values;
   * whole image: RMSE(gpu - cpu) <= 2e-3 x mean radiance;
   * closed form: the floor lit by a point and a quad light (tests/scenes.py) within 1 % in mean;
@@ -114,6 +115,19 @@ def test_vcm_iteration_base_splits_a_render(room):
     assert frac >= 0.999 and rel_rmse <= 1e-4
 
 
+def test_vcm_stats_and_timing_flags(room):
+    """Phase timing and traversal counting change nothing in the image."""
+    sc, scene, _ = room
+    args = (sc["camera"], 48, 32, 2, 10, 0.005, 0.75, 6)
+    plain, s0 = N.render_vcm(scene, *args, return_stats=True)
+    timed, s1 = N.render_vcm(scene, *args, return_stats=True, flags=N.VCM_KERNEL_TIMES | N.VCM_COUNT_TRAVERSAL)
+    assert np.array_equal(plain, timed) or _close(timed, plain)[0] >= 0.999
+    assert s0["closest_ms"] == 0 and s0["nodes_visited"] == 0
+    assert s1["closest_ms"] > 0 and s1["camera_ms"] > 0 and s1["grid_ms"] > 0
+    assert s1["nodes_visited"] >= s1["closest_rays"] > 0 and s1["tris_tested"] > 0
+    assert s1["rays"] == s1["closest_rays"] + s1["shadow_rays"] == s0["rays"]
+
+
 def test_vcm_short_paths_and_no_lights(room, gpu_ctx):
     sc, scene, rs = room
     for L in (1, 2, 3):
@@ -137,10 +151,10 @@ def test_vcm_errors(room, gpu_ctx):
     try:
         def job(**kw):
             a = dict(iterations=1, iteration_base=0, max_path_length=10, base_radius=0.01, radius_alpha=0.75,
-                     reserved=0, seed=1)
+                     flags=0, seed=1)
             a.update(kw)
             return N.VcmJob(N.Camera.make(sc["camera"], kw.pop("w", 16), 16), a["iterations"], a["iteration_base"],
-                            a["max_path_length"], a["base_radius"], a["radius_alpha"], a["reserved"], a["seed"])
+                            a["max_path_length"], a["base_radius"], a["radius_alpha"], a["flags"], a["seed"])
 
         def call(j):
             return N.lib.sgpu_render_vcm(scene.handle, N.C.byref(j), fb.ptr, None, None)
@@ -151,7 +165,7 @@ def test_vcm_errors(room, gpu_ctx):
         assert call(job(base_radius=0.0)) == N.SGPU_E_INVALID
         assert call(job(base_radius=float("nan"))) == N.SGPU_E_INVALID
         assert call(job(radius_alpha=1.5)) == N.SGPU_E_INVALID
-        assert call(job(reserved=1)) == N.SGPU_E_INVALID
+        assert call(job(flags=4)) == N.SGPU_E_INVALID
         assert call(job(w=0)) == N.SGPU_E_INVALID
         assert N.lib.sgpu_render_vcm(scene.handle, None, fb.ptr, None, None) == N.SGPU_E_INVALID
         bare = N.Scene(gpu_ctx)
