@@ -178,7 +178,8 @@ def bench_vcm(args):
         "closest_rays": st["closest_rays"],
         "shadow_rays": st["shadow_rays"],
         "light_vertices_per_iteration": round(st["light_vertices"] / steps, 1),
-        "phase_ms": {k: round(st[k], 2) for k in ("closest_ms", "shadow_ms", "light_ms", "camera_ms", "grid_ms")},
+        "phase_ms": {k: round(st[k], 2) for k in ("closest_ms", "shadow_ms", "light_ms", "camera_ms", "merge_ms",
+                                                  "grid_ms")},
         "bvh_build_ms": round(build_ms, 1),
         "wall_s_timed_region": round(wall, 3),
         "image_mean": round(float(img.mean()), 5),

@@ -249,9 +249,9 @@ typedef struct sgpu_vcm_stats {
     float closest_ms;          /* SGPU_VCM_KERNEL_TIMES: device time of the closest-hit traversals */
     float shadow_ms;           /* ... of the occlusion traversals */
     float light_ms;            /* ... of the light-subpath kernels (emission, shading, splats) */
-    float camera_ms;           /* ... of the camera-subpath kernels (NEE, connections, merging) */
+    float camera_ms;           /* ... of the camera-subpath kernels (NEE, connections, continuation) */
     float grid_ms;             /* ... of the vertex compaction and photon grid build */
-    float reserved_ms;
+    float merge_ms;            /* ... of the vertex merging (photon range searches) */
 } sgpu_vcm_stats;
 
 /* Vertex connection and merging: replaces VCM::GenerateImage and its helpers

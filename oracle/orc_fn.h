@@ -21,5 +21,13 @@ static inline float cos_f(float x) { return orc_libm_single ? cosf(x) : (float)c
 static inline float sin_f(float x) { return orc_libm_single ? sinf(x) : (float)sin((double)x); }
 static inline float acos_f(float x) { return orc_libm_single ? acosf(x) : (float)acos((double)x); }
 static inline float pow_f(float x, float y) { return orc_libm_single ? powf(x, y) : (float)pow((double)x, (double)y); }
+/* x^5 (Schlick's weight): powf in mode 1, else double multiplications (the device's way) */
+static inline float pow5_f(float x)
+{
+    if (orc_libm_single) return powf(x, 5.0f);
+    double d = (double)x;
+    double d2 = d * d;
+    return (float)(d2 * d2 * d);
+}
 
 #endif

@@ -172,7 +172,7 @@ float orc_schlick_weight(float u)
 static inline float schlick_scalar(float r0, float c) { return lerp(1.0f, orc_schlick_weight(c), r0); }
 static inline ov3 schlick3(ov3 r0, float c)
 {
-    float e = pow_f(1.0f - c, 5.0f);
+    float e = pow5_f(1.0f - c);
     return add(r0, scl(sub(v3(1.0f, 1.0f, 1.0f), r0), e));
 }
 float orc_schlick_r0(float eta) { return square(eta - 1.0f) / square(eta + 1.0f); }

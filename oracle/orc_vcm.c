@@ -138,18 +138,19 @@ void orc_hash_grid_free(orc_hash_grid* g)
     g->indices = NULL;
 }
 
-/* Visits the points within radius of q in the reference's order (8 cells, duplicates kept when two
- * of the cells hash to the same bucket). Returns the number found; writes up to max_found indices. */
-uint32_t orc_hash_grid_query(const orc_hash_grid* g, const float* points, const float* q, uint32_t* found,
-                             uint32_t max_found)
+/* SearchHashGrid: calls fn(ctx, index) for each point within radius of q, in the reference's order
+ * (8 cells, x fastest; a point is visited twice when two of the cells hash to the same bucket). */
+typedef void (*grid_visit_fn)(void* ctx, uint32_t index);
+
+static void hash_grid_visit(const orc_hash_grid* g, const float* points, const float* q, grid_visit_fn fn, void* ctx)
 {
     float dmin[3], dmax[3];
     for (int k = 0; k < 3; ++k) {
         dmin[k] = q[k] - g->lo[k];
         dmax[k] = g->hi[k] - q[k];
     }
-    if (dmin[0] < 0.0f || dmin[1] < 0.0f || dmin[2] < 0.0f) return 0;
-    if (dmax[0] < 0.0f || dmax[1] < 0.0f || dmax[2] < 0.0f) return 0;
+    if (dmin[0] < 0.0f || dmin[1] < 0.0f || dmin[2] < 0.0f) return;
+    if (dmax[0] < 0.0f || dmax[1] < 0.0f || dmax[2] < 0.0f) return;
     int32_t mn[3];
     for (int k = 0; k < 3; ++k) {
         float cp = g->inv_cell * dmin[k];
@@ -158,7 +159,6 @@ uint32_t orc_hash_grid_query(const orc_hash_grid* g, const float* points, const 
         mn[k] = (int32_t)cf;
         if (fr < 0.5f) --mn[k];
     }
-    uint32_t n = 0;
     for (int z = 0; z <= 1; ++z)
         for (int y = 0; y <= 1; ++y)
             for (int x = 0; x <= 1; ++x) {
@@ -168,13 +168,31 @@ uint32_t orc_hash_grid_query(const orc_hash_grid* g, const float* points, const 
                     uint32_t pi = g->indices[s];
                     const float* p = points + 3 * (size_t)pi;
                     ov3 d = sub(ld3(q), ld3(p));
-                    if (dot(d, d) <= g->radius_sq) {
-                        if (n < max_found) found[n] = pi;
-                        ++n;
-                    }
+                    if (dot(d, d) <= g->radius_sq) fn(ctx, pi);
                 }
             }
-    return n;
+}
+
+typedef struct {
+    uint32_t* found;
+    uint32_t max_found, n;
+} found_list;
+
+static void collect_found(void* ctx, uint32_t index)
+{
+    found_list* f = (found_list*)ctx;
+    if (f->n < f->max_found) f->found[f->n] = index;
+    ++f->n;
+}
+
+/* The points within radius of q in the reference's order. Returns the number found; writes up to
+ * max_found indices. */
+uint32_t orc_hash_grid_query(const orc_hash_grid* g, const float* points, const float* q, uint32_t* found,
+                             uint32_t max_found)
+{
+    found_list f = {found, max_found, 0};
+    hash_grid_visit(g, points, q, collect_found, &f);
+    return f.n;
 }
 
 /* ---- camera projection ---------------------------------------------------------------------- */
@@ -507,6 +525,41 @@ static void light_paths(void* ctx, uint32_t b, uint32_t e)
     free(c.stack);
 }
 
+/* MergeVertices (VCM.cpp:362-409) for one light vertex found by the range search. */
+typedef struct {
+    const vcm_ctx* c;
+    const vcm_vertex* verts;
+    const orc_surface* s;
+    ov3 view;
+    float dvcm, dvm;
+    uint32_t plen;
+    float ccp;
+    ov3 acc;
+} merge_ctx;
+
+static void merge_vertex(void* ctx, uint32_t index)
+{
+    merge_ctx* m = (merge_ctx*)ctx;
+    const vcm_vertex* lv = &m->verts[index];
+    if (m->plen + lv->path_length > m->c->job->max_path_length) return;
+    orc_surface ls;
+    surface_of_vertex(m->c, lv, &ls);
+    /* the reference's transparency rule (VCM.cpp:381-383) */
+    if ((m->s->flags & 1u) != (ls.flags & 1u)) return;
+    ov3 f;
+    float fwd, rev;
+    eval_bsdf(m->s, m->view, lv->wfix, &f, &fwd, &rev);
+    if (is_zero(f)) return;
+    float cos_l = absf(dot(m->s->n, lv->wfix));
+    fwd *= m->ccp;
+    rev *= cont_prob(&ls);
+    float w_light = lv->dvcm * m->c->vc_weight + lv->dvm * fwd;
+    float w_cam = m->dvcm * m->c->vc_weight + m->dvm * rev;
+    float mis = 1.0f / (w_light + 1.0f + w_cam);
+    /* density estimate uses f without the cosine Selas's BSDF value carries */
+    m->acc = add(m->acc, scl(mul(f, lv->throughput), mis / cos_l));
+}
+
 static void camera_paths(void* ctx, uint32_t b, uint32_t e)
 {
     vcm_iter* I = (vcm_iter*)ctx;
@@ -514,7 +567,6 @@ static void camera_paths(void* ctx, uint32_t b, uint32_t e)
     const orc_scene* sc = c.sc;
     const uint32_t W = I->W, N = I->N, L = I->L;
     c.stack = (uint32_t*)malloc(sizeof(uint32_t) * orc_bvh_stack_size(sc->bvh));
-    uint32_t* found = (uint32_t*)malloc(sizeof(uint32_t) * 65536);
     for (uint32_t i = b; i < e; ++i) {
         uint32_t nrays = 0;
         uint64_t st, inc, sid = stream_id(I->iteration, i, 1);
@@ -618,31 +670,9 @@ static void camera_paths(void* ctx, uint32_t b, uint32_t e)
             }
             /* VM (SearchHashGrid + MergeVertices, VCM.cpp:362-409, 638-655) */
             if (I->nv > 0) {
-                uint32_t nf = orc_hash_grid_query(I->grid, I->vpos, hp, found, 65536);
-                if (nf > 65536) nf = 65536;
-                ov3 acc_vm = v3(0, 0, 0);
-                float ccp = cont_prob(&s);
-                for (uint32_t q = 0; q < nf; ++q) {
-                    const vcm_vertex* lv = &I->verts[found[q]];
-                    if (plen + lv->path_length > L) continue;
-                    orc_surface ls;
-                    surface_of_vertex(&c, lv, &ls);
-                    /* the reference's transparency rule (VCM.cpp:381-383) */
-                    if ((s.flags & 1u) != (ls.flags & 1u)) continue;
-                    ov3 f;
-                    float fwd, rev;
-                    eval_bsdf(&s, view, lv->wfix, &f, &fwd, &rev);
-                    if (is_zero(f)) continue;
-                    float cos_l = absf(dot(s.n, lv->wfix));
-                    fwd *= ccp;
-                    rev *= cont_prob(&ls);
-                    float w_light = lv->dvcm * c.vc_weight + lv->dvm * fwd;
-                    float w_cam = dvcm * c.vc_weight + dvm * rev;
-                    float mis = 1.0f / (w_light + 1.0f + w_cam);
-                    /* density estimate uses f without the cosine Selas's BSDF value carries */
-                    acc_vm = add(acc_vm, scl(mul(f, lv->throughput), mis / cos_l));
-                }
-                color = add(color, scl(mul(thr, acc_vm), c.vm_norm));
+                merge_ctx m = {&c, I->verts, &s, view, dvcm, dvm, plen, cont_prob(&s), v3(0, 0, 0)};
+                hash_grid_visit(I->grid, I->vpos, hp, merge_vertex, &m);
+                color = add(color, scl(mul(thr, m.acc), c.vm_norm));
             }
             if (!scatter(&u, &c, &s, x, &dir, &thr, &dvcm, &dvc, &dvm, &plen)) break;
             pos = x;
@@ -653,7 +683,6 @@ static void camera_paths(void* ctx, uint32_t b, uint32_t e)
         p[2] += color.z;
         I->path_rays[i] = nrays;
     }
-    free(found);
     free(c.stack);
 }
 

@@ -1,9 +1,11 @@
 // C ABI of libselas_gpu: contexts, device memory helpers, scenes and ray queries.
 // Declarations and the Selas/Embree interfaces each entry point replaces are in include/selas_gpu.h.
 #include "sgpu_internal.h"
+#include "sgpu_surface.h"
 
 #include <cstring>
 #include <new>
+#include <vector>
 
 namespace {
 thread_local std::string g_last_error;
@@ -290,7 +292,9 @@ int sgpu_scene_set_materials(sgpu_scene scene, const float* materials, uint32_t 
         (void)hipFree(dm);
         return sgpu_fail(SGPU_E_NOMEM, "sgpu_scene_set_materials: hipMalloc failed");
     }
-    e = hipMemcpy(dm, materials, 22 * sizeof(float) * (size_t)n_materials, hipMemcpyHostToDevice);
+    std::vector<float> shading(22 * (size_t)n_materials);
+    for (uint32_t m = 0; m < n_materials; ++m) sgpu::shading_record(materials + 22 * (size_t)m, &shading[22 * (size_t)m]);
+    e = hipMemcpy(dm, shading.data(), 22 * sizeof(float) * (size_t)n_materials, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(dg, geom_material, sizeof(uint32_t) * (size_t)n_geoms, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         (void)hipFree(dm);

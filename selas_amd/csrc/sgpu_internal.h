@@ -61,7 +61,7 @@ struct sgpu_scene_s {
     // set to non-zero by a traversal whose stack would exceed the tree depth (never expected)
     uint32_t* d_overflow = nullptr;
     // shading data (sgpu_scene_set_materials / sgpu_scene_set_lights)
-    float* d_materials = nullptr;      // 22 floats per material
+    float* d_materials = nullptr;      // shading record per material (22 floats, sgpu::shading_record)
     uint32_t n_materials = 0;
     uint32_t* d_geom_material = nullptr;  // material index per geometry
     uint32_t n_geom_material = 0;

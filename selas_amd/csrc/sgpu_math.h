@@ -90,6 +90,13 @@ SGPU_HD float cos_f(float x) { return (float)cos((double)x); }
 SGPU_HD float sin_f(float x) { return (float)sin((double)x); }
 SGPU_HD float acos_f(float x) { return (float)acos((double)x); }
 SGPU_HD float pow_f(float x, float y) { return (float)pow((double)x, (double)y); }
+// x^5 (Schlick's weight), by double multiplications: the same bits on host and device.
+SGPU_HD float pow5_f(float x)
+{
+    double d = (double)x;
+    double d2 = d * d;
+    return (float)(d2 * d2 * d);
+}
 
 // Uniform float in [0, 1): the top 24 bits scaled by 2^-24 (exact, never returns 1.0).
 SGPU_HD float pcg32_float(Pcg32& r) { return (float)(pcg32_next(r) >> 8) * (1.0f / 16777216.0f); }

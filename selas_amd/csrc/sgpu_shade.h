@@ -131,7 +131,7 @@ SGPU_HD float schlick_weight(float u)
 SGPU_HD float schlick_scalar(float r0, float c) { return lerp_(1.0f, schlick_weight(c), r0); }
 SGPU_HD f3 schlick3(f3 r0, float c)
 {
-    float e = pow_f(1.0f - c, 5.0f);
+    float e = pow5_f(1.0f - c);
     return r0 + (mk3(1.0f, 1.0f, 1.0f) - r0) * e;
 }
 SGPU_HD float schlick_r0(float eta) { return sq(eta - 1.0f) / sq(eta + 1.0f); }

@@ -13,7 +13,9 @@ namespace sgpu {
 
 // Material record: 22 floats with the layout of the surface block in include/selas_gpu.h, holding
 // the material's own values (before CalculateSurfaceParams's transforms); [15] is unused (the
-// relative IOR is computed per hit) and [21] holds the material flags.
+// relative IOR is computed per hit) and [21] holds the material flags. The device keeps a shading
+// record per material instead (shading_record): the same layout with CalculateSurfaceParams's
+// per-material transforms already applied and [15] holding the continuation probability.
 constexpr uint32_t kMatTransparent = 1u << 0;  // eTransparent (ModelResource.h:38)
 constexpr uint32_t kMatUsesPtex = 1u << 4;     // eUsesPtex (ModelResource.h:42)
 
@@ -24,24 +26,38 @@ SGPU_HD float hit_error(f3 p, float t)
     return kErr * max_(max_(absf_(p.x), absf_(p.y)), max_(absf_(p.z), t));
 }
 
+// Shading record of a raw material record: baseColor^2.2, saturated metallic and specTrans,
+// halved diffTrans (SurfaceParameters.cpp:197-229), and [15] = the Russian-roulette continuation
+// probability of the untextured material (saturate(max(baseColor^2.2))).
+SGPU_HD void shading_record(const float* __restrict raw, float* __restrict out)
+{
+    for (int k = 0; k < 22; ++k) out[k] = raw[k];
+    out[0] = pow_f(raw[0], 2.2f);
+    out[1] = pow_f(raw[1], 2.2f);
+    out[2] = pow_f(raw[2], 2.2f);
+    out[10] = saturate_(raw[10]);
+    out[11] = saturate_(raw[11]);
+    out[12] = raw[12] * 0.5f;
+    out[15] = saturate_(maxc3(mk3(out[0], out[1], out[2])));
+}
+
 // Shading frame and material of a surface point with shading normal `n` seen from `view` (the
-// unit direction toward the previous path vertex): everything CalculateSurfaceParams derives from
-// the material. `base_color` is the material value raised to 2.2 unless `texel` (already filtered,
-// before the 2.2 power) is given.
+// unit direction toward the previous path vertex), from the material's shading record. The base
+// colour is the record's unless `texel` (already filtered, before the 2.2 power) is given.
 SGPU_HD void surface_material(const float* __restrict mat, f3 n, f3 view, const f3* texel, Surface& s)
 {
     s.n = n;
     make_frame(s.n, s.t, s.b);
-    f3 bc = texel ? *texel : mk3(mat[0], mat[1], mat[2]);
-    s.base_color = mk3(pow_f(bc.x, 2.2f), pow_f(bc.y, 2.2f), pow_f(bc.z, 2.2f));
+    s.base_color = texel ? mk3(pow_f(texel->x, 2.2f), pow_f(texel->y, 2.2f), pow_f(texel->z, 2.2f))
+                         : mk3(mat[0], mat[1], mat[2]);
     s.transmittance_color = mk3(mat[3], mat[4], mat[5]);
     s.sheen = mat[6];
     s.sheen_tint = mat[7];
     s.clearcoat = mat[8];
     s.clearcoat_gloss = mat[9];
-    s.metallic = saturate_(mat[10]);
-    s.spec_trans = saturate_(mat[11]);
-    s.diff_trans = mat[12] * 0.5f;
+    s.metallic = mat[10];
+    s.spec_trans = mat[11];
+    s.diff_trans = mat[12];
     s.flatness = mat[13];
     s.anisotropic = mat[14];
     s.specular_tint = mat[16];

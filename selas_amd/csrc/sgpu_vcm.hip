@@ -42,7 +42,6 @@ using namespace sgpu;
 constexpr int kBlock = 256;
 constexpr float kFloatMax = 3.402823466e+38F;
 constexpr float kRayEps = 0.00001f;  // RayPick's tnear in the VCM code (VCM.cpp:137)
-constexpr uint32_t kMaxMerge = 65536;  // merged vertices per query, as the CPU range search caps them
 
 __device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter)
 {
@@ -487,6 +486,18 @@ __global__ __launch_bounds__(kBlock) void k_cam_gen(VcmParams P, PathState ps, R
     queue[i] = i;
 }
 
+// Camera vertex handed from k_cam_shade to k_cam_merge (64 B); geom_plen = 0xFFFFFFFF: no query.
+struct CamVertex {
+    float pos[3];
+    float n[3];
+    float view[3];
+    float thr[3];
+    float dvcm, dvm;
+    uint32_t geom_plen;  // geometry << 5 | path length
+    uint32_t path;
+};
+static_assert(sizeof(CamVertex) == 64, "CamVertex must be 64 bytes");
+
 struct GridView {
     const uint32_t* box;
     const uint32_t* cell_ends;
@@ -505,7 +516,8 @@ struct CamSlots {
 
 __global__ __launch_bounds__(kBlock) void k_cam_shade(VcmParams P, uint32_t count, const uint32_t* __restrict queue,
                                                       RayBuf rays, HitBuf hits, PathState ps, GridView G, CamSlots S,
-                                                      RayBuf next_rays, uint32_t* __restrict next_queue, RayBuf shadow,
+                                                      CamVertex* __restrict cam_vm, RayBuf next_rays,
+                                                      uint32_t* __restrict next_queue, RayBuf shadow,
                                                       uint32_t* __restrict counters)
 {
     const uint32_t nslots = P.mlv + 2;
@@ -518,6 +530,7 @@ __global__ __launch_bounds__(kBlock) void k_cam_shade(VcmParams P, uint32_t coun
     Surface s;
     f3 x = mk3(0, 0, 0), view = mk3(0, 0, 0);
     bool hit = false;
+    uint32_t geom_hit = 0;
     if (active) {
         p = queue[j];
         for (uint32_t k = 0; k < nslots; ++k) S.valid[(size_t)p * nslots + k] = 0;
@@ -530,6 +543,7 @@ __global__ __launch_bounds__(kBlock) void k_cam_shade(VcmParams P, uint32_t coun
             }
         } else {
             hit = true;
+            geom_hit = geom;
             f3 o = mk3(rays.ox[j], rays.oy[j], rays.oz[j]);
             const float* mat = P.materials + 22 * (size_t)P.geom_material[geom];
             surface_from_hit(o, d, hits.t[j], mk3(hits.ngx[j], hits.ngy[j], hits.ngz[j]), mat, nullptr, s);
@@ -686,62 +700,29 @@ __global__ __launch_bounds__(kBlock) void k_cam_shade(VcmParams P, uint32_t coun
             S.valid[(size_t)p * nslots + 1 + k] = 2;
         }
     }
-    // ---- vertex merging (slot mlv + 1) ----
-    if (hit && *G.nv_ptr > 0) {
-        float lo[3] = {ord2f(G.box[0]), ord2f(G.box[1]), ord2f(G.box[2])};
-        float hi[3] = {ord2f(~G.box[3]), ord2f(~G.box[4]), ord2f(~G.box[5])};
-        float q[3] = {x.x, x.y, x.z};
-        bool inside = true;
-        float dmin[3];
-        for (int k = 0; k < 3; ++k) {
-            dmin[k] = q[k] - lo[k];
-            float dmax = hi[k] - q[k];
-            if (dmin[k] < 0.0f || dmax < 0.0f) inside = false;
+    // ---- vertex merging (slot mlv + 1): handed to k_cam_merge ----
+    if (active) {
+        CamVertex v;
+        v.geom_plen = 0xFFFFFFFFu;
+        if (hit) {
+            v.pos[0] = x.x;
+            v.pos[1] = x.y;
+            v.pos[2] = x.z;
+            v.n[0] = s.n.x;
+            v.n[1] = s.n.y;
+            v.n[2] = s.n.z;
+            v.view[0] = view.x;
+            v.view[1] = view.y;
+            v.view[2] = view.z;
+            v.thr[0] = thr.x;
+            v.thr[1] = thr.y;
+            v.thr[2] = thr.z;
+            v.dvcm = dvcm;
+            v.dvm = dvm;
+            v.geom_plen = (geom_hit << 5) | (plen & 31u);
+            v.path = p;
         }
-        if (inside) {
-            int32_t mn[3];
-            for (int k = 0; k < 3; ++k) {
-                float cp = P.inv_cell * dmin[k];
-                float cf = floorf(cp);
-                float fr = cp - cf;
-                mn[k] = (int32_t)cf;
-                if (fr < 0.5f) --mn[k];
-            }
-            f3 acc = mk3(0, 0, 0);
-            float ccp = cont_prob(s);
-            uint32_t found = 0;
-            for (int z = 0; z <= 1; ++z)
-                for (int y = 0; y <= 1; ++y)
-                    for (int xx = 0; xx <= 1; ++xx) {
-                        uint32_t cell = cell_hash(mn[0] + xx, mn[1] + y, mn[2] + z, G.cell_count);
-                        uint32_t b = cell == 0 ? 0 : G.cell_ends[cell - 1], e = G.cell_ends[cell];
-                        for (uint32_t t = b; t < e && found < kMaxMerge; ++t) {
-                            const LightVertex lv = G.cverts[G.indices[t]];
-                            f3 lpos = mk3(lv.pos[0], lv.pos[1], lv.pos[2]);
-                            f3 dv = x - lpos;
-                            if (!(dot3(dv, dv) <= P.radius_sq)) continue;
-                            ++found;
-                            uint32_t lplen = lv.geom_plen & 31u;
-                            if (plen + lplen > P.L) continue;
-                            Surface ls;
-                            vertex_surface(P, lv, ls);
-                            if ((s.flags & kMatTransparent) != (ls.flags & kMatTransparent)) continue;
-                            f3 lw = mk3(lv.wfix[0], lv.wfix[1], lv.wfix[2]);
-                            float fwd, rev;
-                            f3 f = bsdf_eval(s, view, lw, fwd, rev);
-                            if (is_zero3(f)) continue;
-                            float cos_l = absf_(dot3(s.n, lw));
-                            fwd *= ccp;
-                            rev *= cont_prob(ls);
-                            float w_light = lv.dvcm * P.vc_weight + lv.dvm * fwd;
-                            float w_cam = dvcm * P.vc_weight + dvm * rev;
-                            float mis = 1.0f / (w_light + 1.0f + w_cam);
-                            acc += (f * mk3(lv.thr[0], lv.thr[1], lv.thr[2])) * (mis / cos_l);
-                        }
-                    }
-            s3(S.c, p * nslots + 1 + P.mlv, (thr * acc) * P.vm_norm);
-            S.valid[(size_t)p * nslots + 1 + P.mlv] = 1;
-        }
+        cam_vm[j] = v;
     }
     // ---- continuation ----
     if (hit) {
@@ -764,6 +745,89 @@ __global__ __launch_bounds__(kBlock) void k_cam_shade(VcmParams P, uint32_t coun
     if (cont) {
         put_ray(next_rays, kn, next_o, next_d, kRayEps, kFloatMax);
         next_queue[kn] = p;
+    }
+}
+
+// One merge query per camera vertex, eight lanes per query: lane k of the group scans grid cell k of
+// the 2 x 2 x 2 block SearchHashGrid visits (x fastest, then y, then z, VCMHashGrid.cpp:131-152)
+// and the group adds its eight partial sums with a fixed butterfly ((c0+c1)+(c2+c3))+((c4+c5)+
+// (c6+c7)). Positions are tested against the radius from the packed position array before the
+// 64-byte vertex is read. MergeVertices (VCM.cpp:362-409) per vertex found.
+__global__ __launch_bounds__(kBlock) void k_cam_merge(VcmParams P, uint32_t count, const CamVertex* __restrict cam_vm,
+                                                      GridView G, const float* __restrict cpos, CamSlots S)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t j = t >> 3, cell_k = t & 7u;
+    const uint32_t nslots = P.mlv + 2;
+    bool inside = false;
+    f3 acc = mk3(0, 0, 0);
+    CamVertex v;
+    v.geom_plen = 0xFFFFFFFFu;
+    if (j < count) v = cam_vm[j];
+    if (v.geom_plen != 0xFFFFFFFFu && *G.nv_ptr > 0) {
+        float lo[3] = {ord2f(G.box[0]), ord2f(G.box[1]), ord2f(G.box[2])};
+        float hi[3] = {ord2f(~G.box[3]), ord2f(~G.box[4]), ord2f(~G.box[5])};
+        inside = true;
+        float dmin[3];
+        for (int k = 0; k < 3; ++k) {
+            dmin[k] = v.pos[k] - lo[k];
+            float dmax = hi[k] - v.pos[k];
+            if (dmin[k] < 0.0f || dmax < 0.0f) inside = false;
+        }
+        if (inside) {
+            int32_t mn[3];
+            for (int k = 0; k < 3; ++k) {
+                float cp = P.inv_cell * dmin[k];
+                float cf = floorf(cp);
+                float fr = cp - cf;
+                mn[k] = (int32_t)cf;
+                if (fr < 0.5f) --mn[k];
+            }
+            const uint32_t cell = cell_hash(mn[0] + (int32_t)(cell_k & 1u), mn[1] + (int32_t)((cell_k >> 1) & 1u),
+                                            mn[2] + (int32_t)(cell_k >> 2), G.cell_count);
+            const uint32_t b = cell == 0 ? 0 : G.cell_ends[cell - 1], e = G.cell_ends[cell];
+            const f3 x = mk3(v.pos[0], v.pos[1], v.pos[2]);
+            const f3 view = mk3(v.view[0], v.view[1], v.view[2]);
+            const uint32_t plen = v.geom_plen & 31u;
+            Surface s;
+            s.position = x;
+            s.error = 0.0f;
+            surface_material(P.materials + 22 * (size_t)P.geom_material[v.geom_plen >> 5], mk3(v.n[0], v.n[1], v.n[2]),
+                             view, nullptr, s);
+            const float ccp = cont_prob(s);
+            for (uint32_t q = b; q < e; ++q) {
+                const uint32_t vi = G.indices[q];
+                f3 dv = x - mk3(cpos[3 * vi], cpos[3 * vi + 1], cpos[3 * vi + 2]);
+                if (!(dot3(dv, dv) <= P.radius_sq)) continue;
+                const LightVertex& lv = G.cverts[vi];
+                const uint32_t lgp = lv.geom_plen;
+                if (plen + (lgp & 31u) > P.L) continue;
+                const float* lmat = P.materials + 22 * (size_t)P.geom_material[lgp >> 5];
+                // the reference's transparency rule (VCM.cpp:381-383)
+                if ((s.flags & kMatTransparent) != ((uint32_t)lmat[21] & kMatTransparent)) continue;
+                f3 lw = mk3(lv.wfix[0], lv.wfix[1], lv.wfix[2]);
+                float fwd, rev;
+                f3 f = bsdf_eval(s, view, lw, fwd, rev);
+                if (is_zero3(f)) continue;
+                float cos_l = absf_(dot3(s.n, lw));
+                fwd *= ccp;
+                rev *= lmat[15];  // continuation probability of the light vertex's material
+                float w_light = lv.dvcm * P.vc_weight + lv.dvm * fwd;
+                float w_cam = v.dvcm * P.vc_weight + v.dvm * rev;
+                float mis = 1.0f / (w_light + 1.0f + w_cam);
+                acc += (f * mk3(lv.thr[0], lv.thr[1], lv.thr[2])) * (mis / cos_l);
+            }
+        }
+    }
+    for (int off = 1; off < 8; off <<= 1) {
+        acc.x += __shfl_xor(acc.x, off, 64);
+        acc.y += __shfl_xor(acc.y, off, 64);
+        acc.z += __shfl_xor(acc.z, off, 64);
+    }
+    if (inside && cell_k == 0) {
+        const uint32_t p = v.path;
+        s3(S.c, p * nslots + 1 + P.mlv, (mk3(v.thr[0], v.thr[1], v.thr[2]) * acc) * P.vm_norm);
+        S.valid[(size_t)p * nslots + 1 + P.mlv] = 1;
     }
 }
 
@@ -837,7 +901,7 @@ struct Arena {
 
 // Device time per phase, from event pairs recorded around each phase's launches on the render's
 // stream (only when asked for: SGPU_VCM_KERNEL_TIMES). collect() runs after a stream sync.
-enum { kPhaseClosest, kPhaseShadow, kPhaseLight, kPhaseCamera, kPhaseGrid, kPhases };
+enum { kPhaseClosest, kPhaseShadow, kPhaseLight, kPhaseCamera, kPhaseGrid, kPhaseMerge, kPhases };
 struct PhaseTimer {
     bool on;
     hipStream_t stream;
@@ -961,6 +1025,7 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
     uint32_t* counters = A.get<uint32_t>(4);
     uint32_t* nv_dev = A.get<uint32_t>(1);
     float* frame = A.get<float>(3 * (size_t)N);
+    CamVertex* cam_vm = A.get<CamVertex>(N);
     CamSlots S;
     S.c = A.get<float>(3 * (size_t)N * nslots);
     S.valid = A.get<uint32_t>((size_t)N * nslots);
@@ -1127,7 +1192,12 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
             SGPU_HIP_TRY(hipMemsetAsync(counters, 0, 4 * sizeof(uint32_t), stream));
             SGPU_HIP_TRY(T.begin(kPhaseCamera));
             hipLaunchKernelGGL(k_cam_shade, dim3(blocks_for(count)), dim3(kBlock), 0, stream, P, count, queue[cur],
-                               rays[cur], hb, ps, G, S, rays[1 - cur], queue[1 - cur], shadow, counters);
+                               rays[cur], hb, ps, G, S, cam_vm, rays[1 - cur], queue[1 - cur], shadow, counters);
+            SGPU_HIP_TRY(hipGetLastError());
+            SGPU_HIP_TRY(T.end());
+            SGPU_HIP_TRY(T.begin(kPhaseMerge));
+            hipLaunchKernelGGL(k_cam_merge, dim3(blocks_for(8ull * count)), dim3(kBlock), 0, stream, P, count, cam_vm, G,
+                               cpos, S);
             SGPU_HIP_TRY(hipGetLastError());
             SGPU_HIP_TRY(T.end());
             SGPU_HIP_TRY(hipMemcpyAsync(h, counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
@@ -1182,6 +1252,7 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
         stats->light_ms = (float)T.ms[kPhaseLight];
         stats->camera_ms = (float)T.ms[kPhaseCamera];
         stats->grid_ms = (float)T.ms[kPhaseGrid];
+        stats->merge_ms = (float)T.ms[kPhaseMerge];
     }
     return SGPU_OK;
 }

@@ -363,7 +363,7 @@ class VcmStats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("rays", "closest_rays", "shadow_rays", "closest_launches",
                                           "light_vertices", "samples", "nodes_visited", "tris_tested")] + \
                [(k, C.c_float) for k in ("closest_ms", "shadow_ms", "light_ms", "camera_ms", "grid_ms",
-                                         "reserved_ms")]
+                                         "merge_ms")]
 
 
 def render_vcm(scene: Scene, cam: dict, width, height, iterations, max_path_length, base_radius, radius_alpha, seed,
@@ -383,7 +383,7 @@ def render_vcm(scene: Scene, cam: dict, width, height, iterations, max_path_leng
         if framebuffer is None:
             fb.free()
     if return_stats:
-        return out, {k: getattr(st, k) for k, _ in VcmStats._fields_ if k != "reserved_ms"}
+        return out, {k: getattr(st, k) for k, _ in VcmStats._fields_}
     return out
 
 
