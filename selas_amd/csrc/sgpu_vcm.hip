@@ -7,20 +7,30 @@
 // (and every deviation from the commented code) are those of the CPU restatement the tests
 // compare against; see the repository's design notes for the list.
 //
-// One iteration (W x H light paths, then W x H camera paths) is a fixed sequence of launches:
-//   light pass   k_light_gen, then per bounce: sgpu_intersect -> k_light_shade (store the vertex
-//                in its path's slot, queue the camera connection and the continuation ray) ->
-//                sgpu_occluded -> k_splat (atomic adds into the iteration frame)
-//   vertices     exclusive scan of per-path counts -> k_compact: path-major vertex array, the
-//                order in which Selas appends them (so the photon grid is built over the same list)
-//   photon grid  k_grid_bounds (box), k_grid_keys (hashed cell per vertex), rocPRIM radix sort of
-//                (cell, vertex) pairs (stable: equal cells keep vertex order, as the counting sort
-//                of BuildHashGrid does), k_grid_count + inclusive scan -> cell range ends
-//   camera pass  k_cam_gen, then per bounce: sgpu_intersect -> k_cam_shade (NEE and one vertex
-//                connection per light vertex of the same path index queued as shadow rays into
-//                fixed per-path slots; vertex merging over the 2x2x2 grid cells) -> sgpu_occluded
-//                -> k_cam_gather (adds the unoccluded slots of each path in slot order)
-//   k_accumulate adds the iteration's frame to the running sum.
+// Work is organised in slices: a slice is one VCM iteration of one render job (W x H light paths,
+// then W x H camera paths, a photon grid of W x H cells). A wavefront runs the paths of many
+// slices together -- several iterations of one job, or iterations of many jobs -- so that every
+// launch is large enough to fill the chip; each path carries its slice index, and every
+// slice-dependent quantity (camera, merge radius, MIS weights, random streams, grid, image) is
+// looked up through it, so a slice's result does not depend on what it is batched with.
+//
+// One wavefront is a fixed sequence of launches:
+//   light pass   k_light_gen, then per bounce: closest-hit traversal -> k_light_shade (store the
+//                vertex in its path's slot, queue the camera connection and the continuation ray)
+//                -> occlusion traversal -> k_splat (atomic adds into the slice's frame)
+//   vertices     exclusive scan of per-path counts -> k_compact: the vertex array in path order,
+//                the order in which Selas appends them (so each slice's photon grid is built over
+//                the same list), slice-contiguous
+//   photon grid  k_grid_bounds (box per slice), k_grid_keys (slice's cell base + hashed cell),
+//                rocPRIM radix sort of (cell, vertex) pairs (stable: equal cells keep vertex order,
+//                as the counting sort of BuildHashGrid does), inclusive scan of cell counts ->
+//                cell range ends
+//   camera pass  k_cam_gen, then per bounce: closest-hit traversal -> k_cam_shade (NEE and one
+//                vertex connection per light vertex of the same path index, queued as shadow rays
+//                into fixed per-path slots; hands the vertex to the merge kernel) -> k_cam_merge
+//                (range search in the slice's grid) -> occlusion traversal -> k_cam_resolve ->
+//                k_cam_gather (adds the unoccluded slots of each path in slot order)
+//   k_frame_add  adds each slice's frame to its job's running sum, slices in iteration order.
 #include "sgpu_camera.h"
 #include "sgpu_internal.h"
 #include "sgpu_light.h"
@@ -31,9 +41,8 @@
 #include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
-#include <cstring>
-#include <vector>
 #include <cmath>
+#include <vector>
 
 namespace {
 
@@ -42,6 +51,7 @@ using namespace sgpu;
 constexpr int kBlock = 256;
 constexpr float kFloatMax = 3.402823466e+38F;
 constexpr float kRayEps = 0.00001f;  // RayPick's tnear in the VCM code (VCM.cpp:137)
+constexpr uint32_t kNoQuery = 0xFFFFFFFFu;
 
 __device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter)
 {
@@ -88,7 +98,19 @@ struct LightVertex {
 };
 static_assert(sizeof(LightVertex) == 64, "LightVertex must be 64 bytes");
 
-// Subpath state (PathState, VCMCommon.h:48-60), SoA over paths.
+// Camera vertex handed from k_cam_shade to k_cam_merge (64 B); geom_plen = kNoQuery: no query.
+struct CamVertex {
+    float pos[3];
+    float n[3];
+    float view[3];
+    float thr[3];
+    float dvcm, dvm;
+    uint32_t geom_plen;  // geometry << 5 | path length
+    uint32_t path;
+};
+static_assert(sizeof(CamVertex) == 64, "CamVertex must be 64 bytes");
+
+// Subpath state (PathState, VCMCommon.h:48-60), SoA over the paths of a wavefront.
 struct PathState {
     float* pos;  // 3
     float* dir;  // 3
@@ -97,23 +119,36 @@ struct PathState {
     float* dvc;
     float* dvm;
     uint32_t* plen;
+    uint32_t* slice;
     uint64_t* rng;
     float* color;  // 3, camera paths only
 };
 
-struct VcmParams {
+// One VCM iteration of one job (see the file comment).
+struct Slice {
     CameraParams cam;
     float vipd;  // W / (2 tan(hfov / 2))
     uint32_t W, H, N, L, mlv;
-    float vm_weight, vc_weight, vm_norm, radius_sq, inv_cell;
+    uint32_t path_base;  // first wavefront path of the slice; also its first grid cell
+    uint32_t iteration;  // global iteration index (radius schedule and random streams)
+    uint32_t job;        // index of the job whose image the slice adds to
     uint64_t seed;
-    uint32_t iteration;
-    const float* materials;
+    float vm_weight, vc_weight, vm_norm, radius_sq, inv_cell;
+    const float* materials;  // shading records (sgpu::shading_record)
     const uint32_t* geom_material;
     uint32_t n_geoms;
     const Light* lights;
     uint32_t n_lights;
     f3 background;
+};
+
+// Wavefront-wide arguments.
+struct Wave {
+    const Slice* slices;
+    uint32_t n_slices;
+    uint32_t total;   // paths in the wavefront (sum of the slices' N)
+    uint32_t mlv;     // vertex slots per path (largest slice mlv)
+    uint32_t nslots;  // camera contribution slots per path (mlv + 2)
 };
 
 __device__ __forceinline__ f3 g3(const float* a, uint32_t i) { return mk3(a[3 * i], a[3 * i + 1], a[3 * i + 2]); }
@@ -134,20 +169,34 @@ __device__ __forceinline__ Pcg32 rng_init(uint64_t seed, uint64_t sid)
     return pcg32_seed(seed ^ (sid * 0x9E3779B97F4A7C15ull), sid);
 }
 
-__device__ __forceinline__ Pcg32 rng_load(const PathState& ps, uint32_t p, uint32_t iteration, uint32_t pass)
+__device__ __forceinline__ Pcg32 rng_load(const PathState& ps, uint32_t g, const Slice& sl, uint32_t pass)
 {
     Pcg32 r;
-    r.state = ps.rng[p];
-    r.inc = (stream_id(iteration, p, pass) << 1u) | 1u;
+    r.state = ps.rng[g];
+    r.inc = (stream_id(sl.iteration, g - sl.path_base, pass) << 1u) | 1u;
     return r;
+}
+
+// slice of wavefront path g (slices are sorted by path_base)
+__device__ __forceinline__ uint32_t find_slice(const Wave& w, uint32_t g)
+{
+    uint32_t lo = 0, hi = w.n_slices - 1;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) >> 1;
+        if (w.slices[mid].path_base <= g)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
 }
 
 __device__ __forceinline__ float cont_prob(const Surface& s) { return saturate_(max_(max_(s.base_color.x, s.base_color.y), s.base_color.z)); }
 
 // Surface of a stored light vertex (the material transforms of CalculateSurfaceParams).
-__device__ __forceinline__ void vertex_surface(const VcmParams& P, const LightVertex& v, Surface& s)
+__device__ __forceinline__ void vertex_surface(const Slice& sl, const LightVertex& v, Surface& s)
 {
-    const float* mat = P.materials + 22 * (size_t)P.geom_material[v.geom_plen >> 5];
+    const float* mat = sl.materials + 22 * (size_t)sl.geom_material[v.geom_plen >> 5];
     s.position = mk3(v.pos[0], v.pos[1], v.pos[2]);
     s.error = 0.0f;
     surface_material(mat, mk3(v.n[0], v.n[1], v.n[2]), mk3(v.wfix[0], v.wfix[1], v.wfix[2]), nullptr, s);
@@ -155,7 +204,7 @@ __device__ __forceinline__ void vertex_surface(const VcmParams& P, const LightVe
 
 // SampleBsdfScattering (VCM.cpp:412-446) with the Russian-roulette-compensated throughput.
 template <class U>
-__device__ __forceinline__ bool scatter(U& u, const VcmParams& P, const Surface& s, f3& dir, f3& thr, float& dvcm,
+__device__ __forceinline__ bool scatter(U& u, const Slice& sl, const Surface& s, f3& dir, f3& thr, float& dvcm,
                                         float& dvc, float& dvm, uint32_t& plen)
 {
     BsdfSample bs;
@@ -167,8 +216,8 @@ __device__ __forceinline__ bool scatter(U& u, const VcmParams& P, const Surface&
     float cos_b = absf_(dot3(bs.wi, s.n));
     thr = (thr * bs.reflectance) * (1.0f / cp);
     float k = cos_b / fwd;
-    float nvc = k * (dvc * rev + dvcm + P.vm_weight);
-    float nvm = k * (dvm * rev + dvcm * P.vc_weight + 1.0f);
+    float nvc = k * (dvc * rev + dvcm + sl.vm_weight);
+    float nvm = k * (dvm * rev + dvcm * sl.vc_weight + 1.0f);
     dvc = nvc;
     dvm = nvm;
     dvcm = 1.0f / fwd;
@@ -179,20 +228,23 @@ __device__ __forceinline__ bool scatter(U& u, const VcmParams& P, const Surface&
 
 // ---- light pass ----------------------------------------------------------------------------------
 
-__global__ __launch_bounds__(kBlock) void k_light_gen(VcmParams P, PathState ps, RayBuf rays, uint32_t* __restrict queue,
+__global__ __launch_bounds__(kBlock) void k_light_gen(Wave w, PathState ps, RayBuf rays, uint32_t* __restrict queue,
                                                       uint32_t* __restrict counters)
 {
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t g = blockIdx.x * kBlock + threadIdx.x;
     bool live = false;
     f3 o = mk3(0, 0, 0), d = mk3(0, 0, 0);
-    if (i < P.N) {
-        PcgUniforms u{rng_init(P.seed, stream_id(P.iteration, i, 0))};
-        ps.plen[i] = 1;
-        if (P.n_lights > 0) {
-            float p_pick = 1.0f / P.n_lights;
-            uint32_t idx = (uint32_t)(u.next() * P.n_lights);
-            if (idx >= P.n_lights) idx = P.n_lights - 1;
-            const Light L = P.lights[idx];
+    if (g < w.total) {
+        const uint32_t si = find_slice(w, g);
+        const Slice& sl = w.slices[si];
+        ps.slice[g] = si;
+        PcgUniforms u{rng_init(sl.seed, stream_id(sl.iteration, g - sl.path_base, 0))};
+        ps.plen[g] = 1;
+        if (sl.n_lights > 0) {
+            float p_pick = 1.0f / sl.n_lights;
+            uint32_t idx = (uint32_t)(u.next() * sl.n_lights);
+            if (idx >= sl.n_lights) idx = sl.n_lights - 1;
+            const Light L = sl.lights[idx];
             float emission_pdf_w, direct_pdf_a;
             f3 radiance;
             if (L.type == 1) {
@@ -223,24 +275,24 @@ __global__ __launch_bounds__(kBlock) void k_light_gen(VcmParams P, PathState ps,
                 direct_pdf_a = pdf_a * p_pick;
                 radiance = ld3(L.radiance) * cos_l;
             }
-            s3(ps.pos, i, o);
-            s3(ps.dir, i, d);
-            s3(ps.thr, i, radiance * (1.0f / emission_pdf_w));
-            ps.dvcm[i] = direct_pdf_a / emission_pdf_w;
-            ps.dvc[i] = 0.0f;
-            ps.dvm[i] = 0.0f;
-            live = 1 + 2 < P.L;
+            s3(ps.pos, g, o);
+            s3(ps.dir, g, d);
+            s3(ps.thr, g, radiance * (1.0f / emission_pdf_w));
+            ps.dvcm[g] = direct_pdf_a / emission_pdf_w;
+            ps.dvc[g] = 0.0f;
+            ps.dvm[g] = 0.0f;
+            live = 1 + 2 < sl.L;
         }
-        ps.rng[i] = u.r.state;
+        ps.rng[g] = u.r.state;
     }
     uint32_t k = wave_append(live, &counters[0]);
     if (live) {
         put_ray(rays, k, o, d, kRayEps, kFloatMax);
-        queue[k] = i;
+        queue[k] = g;
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_light_shade(VcmParams P, uint32_t count, const uint32_t* __restrict queue,
+__global__ __launch_bounds__(kBlock) void k_light_shade(Wave w, uint32_t count, const uint32_t* __restrict queue,
                                                         RayBuf rays, HitBuf hits, PathState ps, LightVertex* __restrict verts,
                                                         uint32_t* __restrict vcount, RayBuf next_rays,
                                                         uint32_t* __restrict next_queue, RayBuf shadow,
@@ -254,13 +306,13 @@ __global__ __launch_bounds__(kBlock) void k_light_shade(VcmParams P, uint32_t co
     float sh_tnear = 0.0f, sh_tfar = 0.0f;
     if (j < count) {
         p = queue[j];
+        const Slice& sl = w.slices[ps.slice[p]];
         uint32_t geom = hits.geom[j];
-        if (geom != SGPU_INVALID_ID && geom < P.n_geoms) {
-            Pcg32 r = rng_load(ps, p, P.iteration, 0);
-            PcgUniforms u{r};
+        if (geom != SGPU_INVALID_ID && geom < sl.n_geoms) {
+            PcgUniforms u{rng_load(ps, p, sl, 0)};
             f3 o = mk3(rays.ox[j], rays.oy[j], rays.oz[j]);
             f3 d = mk3(rays.dx[j], rays.dy[j], rays.dz[j]);
-            const float* mat = P.materials + 22 * (size_t)P.geom_material[geom];
+            const float* mat = sl.materials + 22 * (size_t)sl.geom_material[geom];
             Surface s;
             surface_from_hit(o, d, hits.t[j], mk3(hits.ngx[j], hits.ngy[j], hits.ngz[j]), mat, nullptr, s);
             f3 x = s.position;
@@ -278,7 +330,7 @@ __global__ __launch_bounds__(kBlock) void k_light_shade(VcmParams P, uint32_t co
             dvm *= (1.0f / cos_v);
             // store the vertex in the path's slot
             uint32_t k = vcount[p];
-            if (k < P.mlv) {
+            if (k < sl.mlv) {
                 LightVertex lv;
                 lv.pos[0] = x.x;
                 lv.pos[1] = x.y;
@@ -296,20 +348,20 @@ __global__ __launch_bounds__(kBlock) void k_light_shade(VcmParams P, uint32_t co
                 lv.dvc = dvc;
                 lv.dvm = dvm;
                 lv.geom_plen = (geom << 5) | (plen & 31u);
-                verts[(size_t)p * P.mlv + k] = lv;
+                verts[(size_t)p * w.mlv + k] = lv;
                 vcount[p] = k + 1;
             }
             // ConnectLightPathToCamera (VCM.cpp:190-242)
-            const CameraParams& cam = P.cam;
+            const CameraParams& cam = sl.cam;
             f3 to_pos = x - cam.position;
             if (dot3(cam.forward, to_pos) > 0.0f) {
                 float zc = dot3(to_pos, cam.cam_z);
                 if (zc > 0.0f) {
                     float cx = -dot3(to_pos, cam.cam_x) / (dot3(cam.cam_x, cam.cam_x) * zc);
                     float cy = dot3(to_pos, cam.cam_y) / (dot3(cam.cam_y, cam.cam_y) * zc);
-                    float px = (cx + 1.0f) * 0.5f * (float)P.W;
-                    float py = (1.0f - cy) * 0.5f * (float)P.H;
-                    if (px >= 0.0f && py >= 0.0f && px < (float)P.W && py < (float)P.H) {
+                    float px = (cx + 1.0f) * 0.5f * (float)sl.W;
+                    float py = (1.0f - cy) * 0.5f * (float)sl.H;
+                    if (px >= 0.0f && py >= 0.0f && px < (float)sl.W && py < (float)sl.H) {
                         float dist = length3(to_pos);
                         f3 to_cam = to_pos * (-1.0f / dist);
                         float cos_cam = dot3(cam.forward, -to_cam);
@@ -318,18 +370,18 @@ __global__ __launch_bounds__(kBlock) void k_light_shade(VcmParams P, uint32_t co
                         if (!is_zero3(f)) {
                             float cp = cont_prob(s);
                             rev *= cp;
-                            float img_dist = P.vipd / cos_cam;
+                            float img_dist = sl.vipd / cos_cam;
                             float img_to_solid = img_dist * img_dist / cos_cam;
                             float cos_to_cam = absf_(dot3(s.n, to_cam));
                             float img_to_surface = img_to_solid * cos_to_cam / (dist * dist);
-                            float w_light = (img_to_surface / (float)P.N) * (P.vm_weight + dvcm + dvc * rev);
+                            float w_light = (img_to_surface / (float)sl.N) * (sl.vm_weight + dvcm + dvc * rev);
                             float mis = 1.0f / (w_light + 1.0f);
-                            c = (thr * f) * (mis * img_to_solid / ((float)P.N * dist * dist));
+                            c = (thr * f) * (mis * img_to_solid / ((float)sl.N * dist * dist));
                             if (!is_zero3(c)) {
                                 int ix = (int)floorf(px), iy = (int)floorf(py);
-                                if (ix >= 0 && iy >= 0 && ix < (int)P.W && iy < (int)P.H) {
+                                if (ix >= 0 && iy >= 0 && ix < (int)sl.W && iy < (int)sl.H) {
                                     emit = true;
-                                    pix = (uint32_t)iy * P.W + (uint32_t)ix;
+                                    pix = sl.path_base + (uint32_t)iy * sl.W + (uint32_t)ix;
                                     sh_o = offset_ray_origin(s, to_cam, 0.1f);
                                     sh_d = to_cam;
                                     sh_tnear = s.error;
@@ -341,8 +393,8 @@ __global__ __launch_bounds__(kBlock) void k_light_shade(VcmParams P, uint32_t co
                 }
             }
             f3 dir = d;
-            if (scatter(u, P, s, dir, thr, dvcm, dvc, dvm, plen)) {
-                cont = plen + 2 < P.L;
+            if (scatter(u, sl, s, dir, thr, dvcm, dvc, dvm, plen)) {
+                cont = plen + 2 < sl.L;
                 next_o = x;
                 next_d = dir;
                 s3(ps.pos, p, x);
@@ -381,20 +433,22 @@ __global__ __launch_bounds__(kBlock) void k_splat(uint32_t count, const uint32_t
     atomicAdd(&frame[3 * pix + 2], splat_c[3 * k + 2]);
 }
 
-// path-major compaction of the light vertices (the order of Selas's lightVertices array)
-__global__ __launch_bounds__(kBlock) void k_compact(uint32_t N, uint32_t mlv, const LightVertex* __restrict verts,
+// path-ordered compaction of the light vertices (the order of Selas's lightVertices array)
+__global__ __launch_bounds__(kBlock) void k_compact(Wave w, const LightVertex* __restrict verts,
                                                     const uint32_t* __restrict vcount, const uint32_t* __restrict offset,
-                                                    LightVertex* __restrict cverts, float* __restrict cpos)
+                                                    const uint32_t* __restrict path_slice, LightVertex* __restrict cverts,
+                                                    float* __restrict cpos, uint32_t* __restrict cslice)
 {
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= N) return;
-    uint32_t n = vcount[i], base = offset[i];
+    uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= w.total) return;
+    uint32_t n = vcount[g], base = offset[g], si = path_slice[g];
     for (uint32_t k = 0; k < n; ++k) {
-        LightVertex v = verts[(size_t)i * mlv + k];
+        LightVertex v = verts[(size_t)g * w.mlv + k];
         cverts[base + k] = v;
         cpos[3 * (base + k)] = v.pos[0];
         cpos[3 * (base + k) + 1] = v.pos[1];
         cpos[3 * (base + k) + 2] = v.pos[2];
+        cslice[base + k] = si;
     }
 }
 
@@ -407,33 +461,39 @@ __device__ __forceinline__ uint32_t f2ord(float f)
 }
 __device__ __forceinline__ float ord2f(uint32_t u) { return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u); }
 
-// box[0..2] = ord(min), box[3..5] = ~ord(max) (both reduced with atomicMin)
-__global__ __launch_bounds__(kBlock) void k_grid_bounds(const float* __restrict cpos, const uint32_t* __restrict nv_ptr,
-                                                        uint32_t* __restrict box)
+// Per slice s: box[8 s + 0..2] = ord(min), box[8 s + 3..5] = ~ord(max), both reduced with atomicMin
+// (a wave whose vertices all belong to one slice reduces first and issues one atomic per value).
+__global__ __launch_bounds__(kBlock) void k_grid_bounds(const float* __restrict cpos, const uint32_t* __restrict cslice,
+                                                        const uint32_t* __restrict nv_ptr, uint32_t* __restrict box)
 {
-    __shared__ uint32_t red[6][kBlock / 64];
-    uint32_t nv = *nv_ptr;
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t nv = *nv_ptr;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const bool valid = i < nv;
+    uint32_t si = valid ? cslice[i] : 0xFFFFFFFFu;
     uint32_t v[6];
     for (int k = 0; k < 6; ++k) v[k] = 0xFFFFFFFFu;
-    if (i < nv)
+    if (valid)
         for (int k = 0; k < 3; ++k) {
             float x = cpos[3 * i + k];
             v[k] = f2ord(x);
             v[3 + k] = ~f2ord(x);
         }
-    for (int k = 0; k < 6; ++k) {
-        uint32_t x = v[k];
-        for (int off = 32; off > 0; off >>= 1) x = min(x, (uint32_t)__shfl_xor((int)x, off, 64));
-        v[k] = x;
-    }
-    if ((threadIdx.x & 63) == 0)
-        for (int k = 0; k < 6; ++k) red[k][threadIdx.x >> 6] = v[k];
-    __syncthreads();
-    if (threadIdx.x < 6) {
-        uint32_t x = red[threadIdx.x][0];
-        for (int w = 1; w < kBlock / 64; ++w) x = min(x, red[threadIdx.x][w]);
-        atomicMin(&box[threadIdx.x], x);
+    // slice of the wave's first valid lane; uniform when every valid lane has it
+    const uint64_t vmask = __ballot(valid);
+    if (vmask == 0) return;
+    const uint32_t leader = __ffsll((unsigned long long)vmask) - 1;
+    const uint32_t s0 = __shfl(si, leader, 64);
+    const bool uniform = __ballot(valid && si != s0) == 0;
+    if (uniform) {
+        for (int k = 0; k < 6; ++k) {
+            uint32_t x = v[k];
+            for (int off = 32; off > 0; off >>= 1) x = min(x, (uint32_t)__shfl_xor((int)x, off, 64));
+            v[k] = x;
+        }
+        if (__lane_id() == leader)
+            for (int k = 0; k < 6; ++k) atomicMin(&box[8 * s0 + k], v[k]);
+    } else if (valid) {
+        for (int k = 0; k < 6; ++k) atomicMin(&box[8 * si + k], v[k]);
     }
 }
 
@@ -443,18 +503,21 @@ __device__ __forceinline__ uint32_t cell_hash(int32_t x, int32_t y, int32_t z, u
     return ((ux * 73856093u) ^ (uy * 19349663u) ^ (uz * 83492791u)) % cell_count;
 }
 
-__global__ __launch_bounds__(kBlock) void k_grid_keys(const float* __restrict cpos, const uint32_t* __restrict nv_ptr,
-                                                      const uint32_t* __restrict box, float inv_cell, uint32_t cell_count,
-                                                      uint32_t* __restrict keys, uint32_t* __restrict vals,
-                                                      uint32_t* __restrict cell_counts)
+// key = the slice's first cell + hashed cell (a slice of N paths owns cells [path_base, path_base + N))
+__global__ __launch_bounds__(kBlock) void k_grid_keys(Wave w, const float* __restrict cpos,
+                                                      const uint32_t* __restrict cslice, const uint32_t* __restrict nv_ptr,
+                                                      const uint32_t* __restrict box, uint32_t* __restrict keys,
+                                                      uint32_t* __restrict vals, uint32_t* __restrict cell_counts)
 {
     uint32_t nv = *nv_ptr;
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= nv) return;
-    float lo[3] = {ord2f(box[0]), ord2f(box[1]), ord2f(box[2])};
+    const uint32_t si = cslice[i];
+    const Slice& sl = w.slices[si];
+    float lo[3] = {ord2f(box[8 * si]), ord2f(box[8 * si + 1]), ord2f(box[8 * si + 2])};
     int32_t c[3];
-    for (int k = 0; k < 3; ++k) c[k] = (int32_t)floorf(inv_cell * (cpos[3 * i + k] - lo[k]));
-    uint32_t cell = cell_hash(c[0], c[1], c[2], cell_count);
+    for (int k = 0; k < 3; ++k) c[k] = (int32_t)floorf(sl.inv_cell * (cpos[3 * i + k] - lo[k]));
+    uint32_t cell = sl.path_base + cell_hash(c[0], c[1], c[2], sl.N);
     keys[i] = cell;
     vals[i] = i;
     atomicAdd(&cell_counts[cell], 1u);
@@ -462,68 +525,59 @@ __global__ __launch_bounds__(kBlock) void k_grid_keys(const float* __restrict cp
 
 // ---- camera pass ----------------------------------------------------------------------------------
 
-__global__ __launch_bounds__(kBlock) void k_cam_gen(VcmParams P, PathState ps, RayBuf rays, uint32_t* __restrict queue)
+__global__ __launch_bounds__(kBlock) void k_cam_gen(Wave w, PathState ps, RayBuf rays, uint32_t* __restrict queue)
 {
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= P.N) return;
-    PcgUniforms u{rng_init(P.seed, stream_id(P.iteration, i, 1))};
-    uint32_t x = i % P.W, y = i / P.W;
+    uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= w.total) return;
+    const uint32_t si = find_slice(w, g);
+    const Slice& sl = w.slices[si];
+    const uint32_t i = g - sl.path_base;
+    PcgUniforms u{rng_init(sl.seed, stream_id(sl.iteration, i, 1))};
+    uint32_t x = i % sl.W, y = i / sl.W;
     float jx = (float)x + u.next(), jy = (float)y + u.next();
-    f3 d = camera_image_to_world_dir(P.cam, jx, jy);
-    float cos_cam = dot3(P.cam.forward, d);
-    float img_dist = P.vipd / cos_cam;
+    f3 d = camera_image_to_world_dir(sl.cam, jx, jy);
+    float cos_cam = dot3(sl.cam.forward, d);
+    float img_dist = sl.vipd / cos_cam;
     float cam_pdf_w = img_dist * img_dist / cos_cam;
-    s3(ps.pos, i, P.cam.position);
-    s3(ps.dir, i, d);
-    s3(ps.thr, i, mk3(1, 1, 1));
-    s3(ps.color, i, mk3(0, 0, 0));
-    ps.dvcm[i] = (float)P.N * (1.0f / cam_pdf_w);
-    ps.dvc[i] = 0.0f;
-    ps.dvm[i] = 0.0f;
-    ps.plen[i] = 1;
-    ps.rng[i] = u.r.state;
-    put_ray(rays, i, P.cam.position, d, kRayEps, kFloatMax);
-    queue[i] = i;
+    ps.slice[g] = si;
+    s3(ps.pos, g, sl.cam.position);
+    s3(ps.dir, g, d);
+    s3(ps.thr, g, mk3(1, 1, 1));
+    s3(ps.color, g, mk3(0, 0, 0));
+    ps.dvcm[g] = (float)sl.N * (1.0f / cam_pdf_w);
+    ps.dvc[g] = 0.0f;
+    ps.dvm[g] = 0.0f;
+    ps.plen[g] = 1;
+    ps.rng[g] = u.r.state;
+    put_ray(rays, g, sl.cam.position, d, kRayEps, kFloatMax);
+    queue[g] = g;
 }
 
-// Camera vertex handed from k_cam_shade to k_cam_merge (64 B); geom_plen = 0xFFFFFFFF: no query.
-struct CamVertex {
-    float pos[3];
-    float n[3];
-    float view[3];
-    float thr[3];
-    float dvcm, dvm;
-    uint32_t geom_plen;  // geometry << 5 | path length
-    uint32_t path;
-};
-static_assert(sizeof(CamVertex) == 64, "CamVertex must be 64 bytes");
-
 struct GridView {
-    const uint32_t* box;
-    const uint32_t* cell_ends;
-    const uint32_t* indices;
+    const uint32_t* box;        // 8 per slice
+    const uint32_t* cell_ends;  // inclusive ends over all cells of the wavefront
+    const uint32_t* indices;    // compacted vertex per sorted slot
     const LightVertex* cverts;
-    const uint32_t* path_end;  // inclusive end of each light path's vertices
-    uint32_t cell_count;
+    const uint32_t* path_end;   // inclusive end of each light path's vertices
     const uint32_t* nv_ptr;
 };
 
 struct CamSlots {
-    float* c;          // (mlv + 2) x 3 per path: NEE, VC 0..mlv-1, VM
-    uint32_t* valid;   // (mlv + 2) per path: 0 none, 1 unconditional, 2 pending shadow ray, 3 occluded
-    uint32_t* ray_slot;  // per queued shadow ray: path * (mlv + 2) + slot
+    float* c;            // nslots x 3 per path: NEE, VC 0..mlv-1, VM
+    uint32_t* valid;     // nslots per path: 0 none, 1 unconditional, 2 pending shadow ray, 3 occluded
+    uint32_t* ray_slot;  // per queued shadow ray: path * nslots + slot
 };
 
-__global__ __launch_bounds__(kBlock) void k_cam_shade(VcmParams P, uint32_t count, const uint32_t* __restrict queue,
+__global__ __launch_bounds__(kBlock) void k_cam_shade(Wave w, uint32_t count, const uint32_t* __restrict queue,
                                                       RayBuf rays, HitBuf hits, PathState ps, GridView G, CamSlots S,
                                                       CamVertex* __restrict cam_vm, RayBuf next_rays,
                                                       uint32_t* __restrict next_queue, RayBuf shadow,
                                                       uint32_t* __restrict counters)
 {
-    const uint32_t nslots = P.mlv + 2;
+    const uint32_t nslots = w.nslots;
     uint32_t j = blockIdx.x * kBlock + threadIdx.x;
     bool cont = false;
-    uint32_t p = 0;
+    uint32_t p = 0, si = 0;
     f3 next_o = mk3(0, 0, 0), next_d = mk3(0, 0, 0);
     // shadow rays of this lane, emitted below with wave-level appends (one per slot index)
     bool active = j < count;
@@ -533,30 +587,33 @@ __global__ __launch_bounds__(kBlock) void k_cam_shade(VcmParams P, uint32_t coun
     uint32_t geom_hit = 0;
     if (active) {
         p = queue[j];
+        si = ps.slice[p];
         for (uint32_t k = 0; k < nslots; ++k) S.valid[(size_t)p * nslots + k] = 0;
+        const Slice& sl = w.slices[si];
         uint32_t geom = hits.geom[j];
         f3 d = mk3(rays.dx[j], rays.dy[j], rays.dz[j]);
-        if (geom == SGPU_INVALID_ID || geom >= P.n_geoms) {
+        if (geom == SGPU_INVALID_ID || geom >= sl.n_geoms) {
             if (ps.plen[p] == 1) {
-                f3 col = g3(ps.color, p) + g3(ps.thr, p) * P.background;
+                f3 col = g3(ps.color, p) + g3(ps.thr, p) * sl.background;
                 s3(ps.color, p, col);
             }
         } else {
             hit = true;
             geom_hit = geom;
             f3 o = mk3(rays.ox[j], rays.oy[j], rays.oz[j]);
-            const float* mat = P.materials + 22 * (size_t)P.geom_material[geom];
+            const float* mat = sl.materials + 22 * (size_t)sl.geom_material[geom];
             surface_from_hit(o, d, hits.t[j], mk3(hits.ngx[j], hits.ngy[j], hits.ngz[j]), mat, nullptr, s);
             x = s.position;
             view = -d;
         }
     }
+    const Slice& sl = w.slices[si];
     PcgUniforms u{Pcg32{0, 1}};
     f3 thr = mk3(0, 0, 0);
     float dvcm = 0, dvc = 0, dvm = 0;
     uint32_t plen = 0;
     if (hit) {
-        u.r = rng_load(ps, p, P.iteration, 1);
+        u.r = rng_load(ps, p, sl, 1);
         f3 pos = g3(ps.pos, p);
         thr = g3(ps.thr, p);
         dvcm = ps.dvcm[p];
@@ -576,11 +633,11 @@ __global__ __launch_bounds__(kBlock) void k_cam_shade(VcmParams P, uint32_t coun
         bool emit = false;
         f3 so = mk3(0, 0, 0), ld = mk3(0, 0, 0);
         float tn = 0, tf = 0;
-        if (hit && plen + 1 < P.L && P.n_lights > 0) {
-            float p_pick = 1.0f / P.n_lights;
-            uint32_t idx = (uint32_t)(u.next() * P.n_lights);
-            if (idx >= P.n_lights) idx = P.n_lights - 1;
-            const Light L = P.lights[idx];
+        if (hit && plen + 1 < sl.L && sl.n_lights > 0) {
+            float p_pick = 1.0f / sl.n_lights;
+            uint32_t idx = (uint32_t)(u.next() * sl.n_lights);
+            if (idx >= sl.n_lights) idx = sl.n_lights - 1;
+            const Light L = sl.lights[idx];
             float a = u.next(), b = u.next();
             bool ok = true;
             f3 rad;
@@ -615,7 +672,7 @@ __global__ __launch_bounds__(kBlock) void k_cam_shade(VcmParams P, uint32_t coun
                     float cp = cont_prob(s);
                     rev *= cp;
                     float cos_s = absf_(dot3(s.n, ld));
-                    float w_cam = (epdf * cos_s / (dpdf * cl)) * (P.vm_weight + dvcm + dvc * rev);
+                    float w_cam = (epdf * cos_s / (dpdf * cl)) * (sl.vm_weight + dvcm + dvc * rev);
                     float mis = 1.0f / (1.0f + w_cam);
                     f3 c = (f * rad) * (mis / dpdf);
                     if (!is_zero3(c)) {
@@ -642,7 +699,7 @@ __global__ __launch_bounds__(kBlock) void k_cam_shade(VcmParams P, uint32_t coun
         ve = G.path_end[p];
     }
     bool vc_live = hit;
-    for (uint32_t k = 0; k < P.mlv; ++k) {
+    for (uint32_t k = 0; k < w.mlv; ++k) {
         bool emit = false;
         f3 so = mk3(0, 0, 0), cd = mk3(0, 0, 0);
         float tn = 0, tf = 0;
@@ -650,7 +707,7 @@ __global__ __launch_bounds__(kBlock) void k_cam_shade(VcmParams P, uint32_t coun
         if (vc_live && vi < ve) {
             const LightVertex lv = G.cverts[vi];
             uint32_t lplen = lv.geom_plen & 31u;
-            if (lplen + 1 + plen <= P.L) {
+            if (lplen + 1 + plen <= sl.L) {
                 f3 lpos = mk3(lv.pos[0], lv.pos[1], lv.pos[2]);
                 f3 dv = lpos - x;
                 float dist2 = dot3(dv, dv);
@@ -660,7 +717,7 @@ __global__ __launch_bounds__(kBlock) void k_cam_shade(VcmParams P, uint32_t coun
                 f3 fc = bsdf_eval(s, view, cd, cfwd, crev);
                 if (!is_zero3(fc)) {
                     Surface ls;
-                    vertex_surface(P, lv, ls);
+                    vertex_surface(sl, lv, ls);
                     float lfwd, lrev;
                     f3 lw = mk3(lv.wfix[0], lv.wfix[1], lv.wfix[2]);
                     f3 fl = bsdf_eval(ls, lw, -cd, lfwd, lrev);
@@ -674,8 +731,8 @@ __global__ __launch_bounds__(kBlock) void k_cam_shade(VcmParams P, uint32_t coun
                         lrev *= lcp;
                         float c_pdf_a = cfwd * cos_l / dist2;
                         float l_pdf_a = lfwd * cos_c / dist2;
-                        float w_light = c_pdf_a * (P.vm_weight + lv.dvcm + lv.dvc * lrev);
-                        float w_cam = l_pdf_a * (P.vm_weight + dvcm + dvc * crev);
+                        float w_light = c_pdf_a * (sl.vm_weight + lv.dvcm + lv.dvc * lrev);
+                        float w_cam = l_pdf_a * (sl.vm_weight + dvcm + dvc * crev);
                         float mis = 1.0f / (w_light + 1.0f + w_cam);
                         f3 c = (fc * fl) * (mis / dist2);
                         if (!is_zero3(c)) {
@@ -700,10 +757,10 @@ __global__ __launch_bounds__(kBlock) void k_cam_shade(VcmParams P, uint32_t coun
             S.valid[(size_t)p * nslots + 1 + k] = 2;
         }
     }
-    // ---- vertex merging (slot mlv + 1): handed to k_cam_merge ----
+    // ---- vertex merging (slot nslots - 1): handed to k_cam_merge ----
     if (active) {
         CamVertex v;
-        v.geom_plen = 0xFFFFFFFFu;
+        v.geom_plen = kNoQuery;
         if (hit) {
             v.pos[0] = x.x;
             v.pos[1] = x.y;
@@ -727,8 +784,8 @@ __global__ __launch_bounds__(kBlock) void k_cam_shade(VcmParams P, uint32_t coun
     // ---- continuation ----
     if (hit) {
         f3 dir = -view;
-        if (scatter(u, P, s, dir, thr, dvcm, dvc, dvm, plen)) {
-            cont = plen < P.L;
+        if (scatter(u, sl, s, dir, thr, dvcm, dvc, dvm, plen)) {
+            cont = plen < sl.L;
             next_o = x;
             next_d = dir;
             s3(ps.pos, p, x);
@@ -753,20 +810,23 @@ __global__ __launch_bounds__(kBlock) void k_cam_shade(VcmParams P, uint32_t coun
 // and the group adds its eight partial sums with a fixed butterfly ((c0+c1)+(c2+c3))+((c4+c5)+
 // (c6+c7)). Positions are tested against the radius from the packed position array before the
 // 64-byte vertex is read. MergeVertices (VCM.cpp:362-409) per vertex found.
-__global__ __launch_bounds__(kBlock) void k_cam_merge(VcmParams P, uint32_t count, const CamVertex* __restrict cam_vm,
-                                                      GridView G, const float* __restrict cpos, CamSlots S)
+__global__ __launch_bounds__(kBlock) void k_cam_merge(Wave w, uint32_t count, const CamVertex* __restrict cam_vm,
+                                                      const uint32_t* __restrict path_slice, GridView G,
+                                                      const float* __restrict cpos, CamSlots S)
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t j = t >> 3, cell_k = t & 7u;
-    const uint32_t nslots = P.mlv + 2;
     bool inside = false;
     f3 acc = mk3(0, 0, 0);
     CamVertex v;
-    v.geom_plen = 0xFFFFFFFFu;
+    v.geom_plen = kNoQuery;
     if (j < count) v = cam_vm[j];
-    if (v.geom_plen != 0xFFFFFFFFu && *G.nv_ptr > 0) {
-        float lo[3] = {ord2f(G.box[0]), ord2f(G.box[1]), ord2f(G.box[2])};
-        float hi[3] = {ord2f(~G.box[3]), ord2f(~G.box[4]), ord2f(~G.box[5])};
+    if (v.geom_plen != kNoQuery && *G.nv_ptr > 0) {
+        const uint32_t si = path_slice[v.path];
+        const Slice& sl = w.slices[si];
+        const uint32_t* box = G.box + 8 * si;
+        float lo[3] = {ord2f(box[0]), ord2f(box[1]), ord2f(box[2])};
+        float hi[3] = {ord2f(~box[3]), ord2f(~box[4]), ord2f(~box[5])};
         inside = true;
         float dmin[3];
         for (int k = 0; k < 3; ++k) {
@@ -777,14 +837,15 @@ __global__ __launch_bounds__(kBlock) void k_cam_merge(VcmParams P, uint32_t coun
         if (inside) {
             int32_t mn[3];
             for (int k = 0; k < 3; ++k) {
-                float cp = P.inv_cell * dmin[k];
+                float cp = sl.inv_cell * dmin[k];
                 float cf = floorf(cp);
                 float fr = cp - cf;
                 mn[k] = (int32_t)cf;
                 if (fr < 0.5f) --mn[k];
             }
-            const uint32_t cell = cell_hash(mn[0] + (int32_t)(cell_k & 1u), mn[1] + (int32_t)((cell_k >> 1) & 1u),
-                                            mn[2] + (int32_t)(cell_k >> 2), G.cell_count);
+            const uint32_t cell =
+                sl.path_base + cell_hash(mn[0] + (int32_t)(cell_k & 1u), mn[1] + (int32_t)((cell_k >> 1) & 1u),
+                                         mn[2] + (int32_t)(cell_k >> 2), sl.N);
             const uint32_t b = cell == 0 ? 0 : G.cell_ends[cell - 1], e = G.cell_ends[cell];
             const f3 x = mk3(v.pos[0], v.pos[1], v.pos[2]);
             const f3 view = mk3(v.view[0], v.view[1], v.view[2]);
@@ -792,17 +853,17 @@ __global__ __launch_bounds__(kBlock) void k_cam_merge(VcmParams P, uint32_t coun
             Surface s;
             s.position = x;
             s.error = 0.0f;
-            surface_material(P.materials + 22 * (size_t)P.geom_material[v.geom_plen >> 5], mk3(v.n[0], v.n[1], v.n[2]),
-                             view, nullptr, s);
+            surface_material(sl.materials + 22 * (size_t)sl.geom_material[v.geom_plen >> 5],
+                             mk3(v.n[0], v.n[1], v.n[2]), view, nullptr, s);
             const float ccp = cont_prob(s);
             for (uint32_t q = b; q < e; ++q) {
                 const uint32_t vi = G.indices[q];
                 f3 dv = x - mk3(cpos[3 * vi], cpos[3 * vi + 1], cpos[3 * vi + 2]);
-                if (!(dot3(dv, dv) <= P.radius_sq)) continue;
+                if (!(dot3(dv, dv) <= sl.radius_sq)) continue;
                 const LightVertex& lv = G.cverts[vi];
                 const uint32_t lgp = lv.geom_plen;
-                if (plen + (lgp & 31u) > P.L) continue;
-                const float* lmat = P.materials + 22 * (size_t)P.geom_material[lgp >> 5];
+                if (plen + (lgp & 31u) > sl.L) continue;
+                const float* lmat = sl.materials + 22 * (size_t)sl.geom_material[lgp >> 5];
                 // the reference's transparency rule (VCM.cpp:381-383)
                 if ((s.flags & kMatTransparent) != ((uint32_t)lmat[21] & kMatTransparent)) continue;
                 f3 lw = mk3(lv.wfix[0], lv.wfix[1], lv.wfix[2]);
@@ -812,8 +873,8 @@ __global__ __launch_bounds__(kBlock) void k_cam_merge(VcmParams P, uint32_t coun
                 float cos_l = absf_(dot3(s.n, lw));
                 fwd *= ccp;
                 rev *= lmat[15];  // continuation probability of the light vertex's material
-                float w_light = lv.dvcm * P.vc_weight + lv.dvm * fwd;
-                float w_cam = v.dvcm * P.vc_weight + v.dvm * rev;
+                float w_light = lv.dvcm * sl.vc_weight + lv.dvm * fwd;
+                float w_cam = v.dvcm * sl.vc_weight + v.dvm * rev;
                 float mis = 1.0f / (w_light + 1.0f + w_cam);
                 acc += (f * mk3(lv.thr[0], lv.thr[1], lv.thr[2])) * (mis / cos_l);
             }
@@ -826,8 +887,9 @@ __global__ __launch_bounds__(kBlock) void k_cam_merge(VcmParams P, uint32_t coun
     }
     if (inside && cell_k == 0) {
         const uint32_t p = v.path;
-        s3(S.c, p * nslots + 1 + P.mlv, (mk3(v.thr[0], v.thr[1], v.thr[2]) * acc) * P.vm_norm);
-        S.valid[(size_t)p * nslots + 1 + P.mlv] = 1;
+        const Slice& sl = w.slices[path_slice[p]];
+        s3(S.c, p * w.nslots + w.nslots - 1, (mk3(v.thr[0], v.thr[1], v.thr[2]) * acc) * sl.vm_norm);
+        S.valid[(size_t)p * w.nslots + w.nslots - 1] = 1;
     }
 }
 
@@ -851,15 +913,23 @@ __global__ __launch_bounds__(kBlock) void k_cam_gather(uint32_t count, const uin
     s3(ps.color, p, col);
 }
 
-__global__ __launch_bounds__(kBlock) void k_frame_add(uint32_t N, const float* __restrict color, float* __restrict frame,
+// acc (a job's running sum, n pixels) += frame + color of each of the job's slices first..first+count-1
+__global__ __launch_bounds__(kBlock) void k_frame_add(Wave w, uint32_t first, uint32_t count, uint32_t n,
+                                                      const float* __restrict color, const float* __restrict frame,
                                                       float* __restrict acc)
 {
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= N) return;
-    for (int k = 0; k < 3; ++k) {
-        float v = frame[3 * i + k] + color[3 * i + k];
-        acc[3 * i + k] += v;
+    if (i >= n) return;
+    float a0 = acc[3 * i], a1 = acc[3 * i + 1], a2 = acc[3 * i + 2];
+    for (uint32_t s = first; s < first + count; ++s) {
+        const size_t g = (size_t)w.slices[s].path_base + i;
+        a0 += frame[3 * g] + color[3 * g];
+        a1 += frame[3 * g + 1] + color[3 * g + 1];
+        a2 += frame[3 * g + 2] + color[3 * g + 2];
     }
+    acc[3 * i] = a0;
+    acc[3 * i + 1] = a1;
+    acc[3 * i + 2] = a2;
 }
 
 __global__ __launch_bounds__(kBlock) void k_scale(float* __restrict fb, size_t n, float s)
@@ -952,6 +1022,9 @@ sgpu_rays_soa as_soa(const RayBuf& r) { return sgpu_rays_soa{r.ox, r.oy, r.oz, r
 
 inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
 
+// Paths per wavefront the render aims for: enough lanes to fill the chip in every bounce's launch.
+constexpr uint64_t kTargetPaths = 1u << 20;
+
 }  // namespace
 
 int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, sgpu_vcm_stats* stats,
@@ -962,96 +1035,112 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
     const uint32_t nslots = mlv + 2;
     const float pi = 3.141592653589793f;
 
-    VcmParams P;
-    P.cam = camera_init(mk3(job->camera.position[0], job->camera.position[1], job->camera.position[2]),
-                        mk3(job->camera.look_at[0], job->camera.look_at[1], job->camera.look_at[2]),
-                        mk3(job->camera.up[0], job->camera.up[1], job->camera.up[2]), job->camera.fov_degrees, W, H);
+    // slices per wavefront: iterations of this job run together up to ~kTargetPaths paths
+    const uint32_t per_wave =
+        (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(job->iterations, kTargetPaths / std::max<uint32_t>(N, 1)));
+    const uint64_t total_max = (uint64_t)per_wave * N;
+    if (total_max >= (1ull << 31)) return sgpu_fail(SGPU_E_RANGE, "sgpu_render_vcm: wavefront too large");
+    const uint32_t TM = (uint32_t)total_max;
+
+    Slice base;
+    std::memset(&base, 0, sizeof(base));
+    base.cam = camera_init(mk3(job->camera.position[0], job->camera.position[1], job->camera.position[2]),
+                           mk3(job->camera.look_at[0], job->camera.look_at[1], job->camera.look_at[2]),
+                           mk3(job->camera.up[0], job->camera.up[1], job->camera.up[2]), job->camera.fov_degrees, W, H);
     float hfov = job->camera.fov_degrees * (pi / 180.f);
-    P.vipd = (float)W / (2.0f * tanf(hfov * 0.5f));
-    P.W = W;
-    P.H = H;
-    P.N = N;
-    P.L = L;
-    P.mlv = mlv;
-    P.seed = job->seed;
-    P.materials = scene->d_materials;
-    P.geom_material = scene->d_geom_material;
-    P.n_geoms = scene->n_geom_material;
-    P.lights = (const Light*)scene->d_lights;
-    P.n_lights = scene->n_lights;
-    P.background = mk3(scene->background[0], scene->background[1], scene->background[2]);
+    base.vipd = (float)W / (2.0f * tanf(hfov * 0.5f));
+    base.W = W;
+    base.H = H;
+    base.N = N;
+    base.L = L;
+    base.mlv = mlv;
+    base.seed = job->seed;
+    base.job = 0;
+    base.materials = scene->d_materials;
+    base.geom_material = scene->d_geom_material;
+    base.n_geoms = scene->n_geom_material;
+    base.lights = (const Light*)scene->d_lights;
+    base.n_lights = scene->n_lights;
+    base.background = mk3(scene->background[0], scene->background[1], scene->background[2]);
 
     Arena A;
     A.stream = stream;
     PathState ps;
-    ps.pos = A.get<float>(3 * (size_t)N);
-    ps.dir = A.get<float>(3 * (size_t)N);
-    ps.thr = A.get<float>(3 * (size_t)N);
-    ps.dvcm = A.get<float>(N);
-    ps.dvc = A.get<float>(N);
-    ps.dvm = A.get<float>(N);
-    ps.plen = A.get<uint32_t>(N);
-    ps.rng = A.get<uint64_t>(N);
-    ps.color = A.get<float>(3 * (size_t)N);
-    RayBuf rays[2] = {A.rays(N), A.rays(N)};
-    uint32_t* queue[2] = {A.get<uint32_t>(N), A.get<uint32_t>(N)};
+    ps.pos = A.get<float>(3 * (size_t)TM);
+    ps.dir = A.get<float>(3 * (size_t)TM);
+    ps.thr = A.get<float>(3 * (size_t)TM);
+    ps.dvcm = A.get<float>(TM);
+    ps.dvc = A.get<float>(TM);
+    ps.dvm = A.get<float>(TM);
+    ps.plen = A.get<uint32_t>(TM);
+    ps.slice = A.get<uint32_t>(TM);
+    ps.rng = A.get<uint64_t>(TM);
+    ps.color = A.get<float>(3 * (size_t)TM);
+    RayBuf rays[2] = {A.rays(TM), A.rays(TM)};
+    uint32_t* queue[2] = {A.get<uint32_t>(TM), A.get<uint32_t>(TM)};
     HitBuf hb;
-    hb.t = A.get<float>(N);
-    hb.u = A.get<float>(N);
-    hb.v = A.get<float>(N);
-    hb.ngx = A.get<float>(N);
-    hb.ngy = A.get<float>(N);
-    hb.ngz = A.get<float>(N);
-    hb.prim = A.get<uint32_t>(N);
-    hb.geom = A.get<uint32_t>(N);
-    const size_t max_shadow = (size_t)N * (mlv + 1);
+    hb.t = A.get<float>(TM);
+    hb.u = A.get<float>(TM);
+    hb.v = A.get<float>(TM);
+    hb.ngx = A.get<float>(TM);
+    hb.ngy = A.get<float>(TM);
+    hb.ngz = A.get<float>(TM);
+    hb.prim = A.get<uint32_t>(TM);
+    hb.geom = A.get<uint32_t>(TM);
+    const size_t max_shadow = (size_t)TM * (mlv + 1);
+    const size_t max_verts = (size_t)TM * mlv;
     RayBuf shadow = A.rays(max_shadow);
     uint32_t* occl = A.get<uint32_t>(max_shadow);
-    float* splat_c = A.get<float>(3 * (size_t)N);
-    uint32_t* splat_pix = A.get<uint32_t>(N);
-    LightVertex* verts = A.get<LightVertex>((size_t)N * mlv);
-    LightVertex* cverts = A.get<LightVertex>((size_t)N * mlv);
-    float* cpos = A.get<float>(3 * (size_t)N * mlv);
-    uint32_t* vcount = A.get<uint32_t>(N + 1);
-    uint32_t* voffset = A.get<uint32_t>(N + 1);
-    uint32_t* path_end = A.get<uint32_t>(N);
-    uint32_t* keys = A.get<uint32_t>((size_t)N * mlv);
-    uint32_t* keys2 = A.get<uint32_t>((size_t)N * mlv);
-    uint32_t* vals = A.get<uint32_t>((size_t)N * mlv);
-    uint32_t* vals2 = A.get<uint32_t>((size_t)N * mlv);
-    uint32_t* cell_counts = A.get<uint32_t>(N);
-    uint32_t* cell_ends = A.get<uint32_t>(N);
-    uint32_t* box = A.get<uint32_t>(8);
+    float* splat_c = A.get<float>(3 * (size_t)TM);
+    uint32_t* splat_pix = A.get<uint32_t>(TM);
+    LightVertex* verts = A.get<LightVertex>(max_verts);
+    LightVertex* cverts = A.get<LightVertex>(max_verts);
+    float* cpos = A.get<float>(3 * max_verts);
+    uint32_t* cslice = A.get<uint32_t>(max_verts);
+    uint32_t* vcount = A.get<uint32_t>((size_t)TM + 1);
+    uint32_t* voffset = A.get<uint32_t>((size_t)TM + 1);
+    uint32_t* path_end = A.get<uint32_t>(TM);
+    uint32_t* keys = A.get<uint32_t>(max_verts);
+    uint32_t* keys2 = A.get<uint32_t>(max_verts);
+    uint32_t* vals = A.get<uint32_t>(max_verts);
+    uint32_t* vals2 = A.get<uint32_t>(max_verts);
+    uint32_t* cell_counts = A.get<uint32_t>(TM);
+    uint32_t* cell_ends = A.get<uint32_t>(TM);
+    uint32_t* box = A.get<uint32_t>(8 * (size_t)per_wave);
     uint32_t* counters = A.get<uint32_t>(4);
     uint32_t* nv_dev = A.get<uint32_t>(1);
-    float* frame = A.get<float>(3 * (size_t)N);
-    CamVertex* cam_vm = A.get<CamVertex>(N);
+    float* frame = A.get<float>(3 * (size_t)TM);
+    Slice* d_slices = A.get<Slice>(per_wave);
+    CamVertex* cam_vm = A.get<CamVertex>(TM);
     CamSlots S;
-    S.c = A.get<float>(3 * (size_t)N * nslots);
-    S.valid = A.get<uint32_t>((size_t)N * nslots);
+    S.c = A.get<float>(3 * (size_t)TM * nslots);
+    S.valid = A.get<uint32_t>((size_t)TM * nslots);
     S.ray_slot = A.get<uint32_t>(max_shadow);
     if (A.err != hipSuccess) return sgpu_fail(SGPU_E_NOMEM, "sgpu_render_vcm: device allocation failed");
 
-    // rocPRIM scratch (radix sort over at most N * mlv keys, scans over N)
-    size_t sort_bytes = 0, scan_bytes = 0, scan2_bytes = 0;
-    // cell keys are below N: sort only the bits that can be set
+    // cell keys are below the wavefront's path count: sort only the bits that can be set
     unsigned key_bits = 1;
-    while (key_bits < 32 && (N - 1) >> key_bits) ++key_bits;
-    SGPU_HIP_TRY(rocprim::radix_sort_pairs(nullptr, sort_bytes, keys, keys2, vals, vals2, (size_t)N * mlv, 0, key_bits,
+    while (key_bits < 32 && (TM - 1) >> key_bits) ++key_bits;
+    size_t sort_bytes = 0, scan_bytes = 0, scan2_bytes = 0;
+    SGPU_HIP_TRY(rocprim::radix_sort_pairs(nullptr, sort_bytes, keys, keys2, vals, vals2, max_verts, 0, key_bits,
                                            stream));
-    SGPU_HIP_TRY(rocprim::exclusive_scan(nullptr, scan_bytes, vcount, voffset, 0u, (size_t)N + 1,
+    SGPU_HIP_TRY(rocprim::exclusive_scan(nullptr, scan_bytes, vcount, voffset, 0u, (size_t)TM + 1,
                                          rocprim::plus<uint32_t>(), stream));
-    SGPU_HIP_TRY(rocprim::inclusive_scan(nullptr, scan2_bytes, cell_counts, cell_ends, (size_t)N,
+    SGPU_HIP_TRY(rocprim::inclusive_scan(nullptr, scan2_bytes, cell_counts, cell_ends, (size_t)TM,
                                          rocprim::plus<uint32_t>(), stream));
-    void* tmp = A.get<uint8_t>(std::max(sort_bytes, std::max(scan_bytes, scan2_bytes)));
     size_t tmp_bytes = std::max(sort_bytes, std::max(scan_bytes, scan2_bytes));
-    uint32_t* h = nullptr;
+    void* tmp = A.get<uint8_t>(tmp_bytes);
     if (A.err != hipSuccess) return sgpu_fail(SGPU_E_NOMEM, "sgpu_render_vcm: device allocation failed");
-    SGPU_HIP_TRY(hipHostMalloc((void**)&h, 8 * sizeof(uint32_t), hipHostMallocDefault));
+    // pinned host staging: counters (8 words) and the slice table
+    uint8_t* hbuf = nullptr;
+    const size_t hbytes = 64 + sizeof(Slice) * (size_t)per_wave;
+    SGPU_HIP_TRY(hipHostMalloc((void**)&hbuf, hbytes, hipHostMallocDefault));
     struct HostFree {
-        uint32_t* p;
+        void* p;
         ~HostFree() { (void)hipHostFree(p); }
-    } hf{h};
+    } hf{hbuf};
+    uint32_t* h = (uint32_t*)hbuf;
+    Slice* h_slices = (Slice*)(hbuf + 64);
 
     SGPU_HIP_TRY(hipMemsetAsync(fb, 0, 3 * (size_t)N * sizeof(float), stream));
     sgpu_hits_soa hsoa{hb.t, hb.u, hb.v, hb.prim, hb.geom, hb.ngx, hb.ngy, hb.ngz};
@@ -1064,26 +1153,43 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
     }
     PhaseTimer T(job->flags & SGPU_VCM_KERNEL_TIMES, stream);
 
-    for (uint32_t it = 0; it < job->iterations; ++it) {
-        const uint32_t gi = job->iteration_base + it;
-        // CalculateIterationConstants (VCMCommon.cpp:97-113), vmCount = W*H, vcCount = 1
-        float iteration_index = (float)gi + 1.0f;
-        float r = job->base_radius / pow_f(iteration_index, 0.5f * (1.0f - job->radius_alpha));
-        float r2 = r * r;
-        P.radius_sq = r2;
-        P.vm_norm = 1.0f / (pi * r2 * N);
-        P.vm_weight = pi * r2 * N / 1;
-        P.vc_weight = 1 / (pi * r2 * N);
-        P.inv_cell = 1.0f / (2.0f * r);
-        P.iteration = gi;
+    for (uint32_t it0 = 0; it0 < job->iterations; it0 += per_wave) {
+        const uint32_t ns = std::min(per_wave, job->iterations - it0);
+        const uint32_t total = ns * N;
+        // the previous wavefront has finished (its last step synchronised the stream): the staged
+        // slice table may be rewritten
+        for (uint32_t k = 0; k < ns; ++k) {
+            Slice sl = base;
+            const uint32_t gi = job->iteration_base + it0 + k;
+            // CalculateIterationConstants (VCMCommon.cpp:97-113), vmCount = W*H, vcCount = 1
+            float iteration_index = (float)gi + 1.0f;
+            float r = job->base_radius / pow_f(iteration_index, 0.5f * (1.0f - job->radius_alpha));
+            float r2 = r * r;
+            sl.radius_sq = r2;
+            sl.vm_norm = 1.0f / (pi * r2 * N);
+            sl.vm_weight = pi * r2 * N / 1;
+            sl.vc_weight = 1 / (pi * r2 * N);
+            sl.inv_cell = 1.0f / (2.0f * r);
+            sl.iteration = gi;
+            sl.path_base = k * N;
+            h_slices[k] = sl;
+        }
+        SGPU_HIP_TRY(hipMemcpyAsync(d_slices, h_slices, sizeof(Slice) * ns, hipMemcpyHostToDevice, stream));
+        Wave wv;
+        wv.slices = d_slices;
+        wv.n_slices = ns;
+        wv.total = total;
+        wv.mlv = mlv;
+        wv.nslots = nslots;
 
-        SGPU_HIP_TRY(hipMemsetAsync(frame, 0, 3 * (size_t)N * sizeof(float), stream));
-        SGPU_HIP_TRY(hipMemsetAsync(vcount, 0, ((size_t)N + 1) * sizeof(uint32_t), stream));
+        SGPU_HIP_TRY(hipMemsetAsync(frame, 0, 3 * (size_t)total * sizeof(float), stream));
+        SGPU_HIP_TRY(hipMemsetAsync(vcount, 0, ((size_t)total + 1) * sizeof(uint32_t), stream));
         SGPU_HIP_TRY(hipMemsetAsync(counters, 0, 4 * sizeof(uint32_t), stream));
 
         // ---- light pass ----
         SGPU_HIP_TRY(T.begin(kPhaseLight));
-        hipLaunchKernelGGL(k_light_gen, dim3(blocks_for(N)), dim3(kBlock), 0, stream, P, ps, rays[0], queue[0], counters);
+        hipLaunchKernelGGL(k_light_gen, dim3(blocks_for(total)), dim3(kBlock), 0, stream, wv, ps, rays[0], queue[0],
+                           counters);
         SGPU_HIP_TRY(hipGetLastError());
         SGPU_HIP_TRY(T.end());
         SGPU_HIP_TRY(hipMemcpyAsync(h, counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
@@ -1100,7 +1206,7 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
             ++n_launches;
             SGPU_HIP_TRY(hipMemsetAsync(counters, 0, 4 * sizeof(uint32_t), stream));
             SGPU_HIP_TRY(T.begin(kPhaseLight));
-            hipLaunchKernelGGL(k_light_shade, dim3(blocks_for(count)), dim3(kBlock), 0, stream, P, count, queue[cur],
+            hipLaunchKernelGGL(k_light_shade, dim3(blocks_for(count)), dim3(kBlock), 0, stream, wv, count, queue[cur],
                                rays[cur], hb, ps, verts, vcount, rays[1 - cur], queue[1 - cur], shadow, splat_c,
                                splat_pix, counters);
             SGPU_HIP_TRY(hipGetLastError());
@@ -1125,31 +1231,32 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
             cur = 1 - cur;
         }
 
-        // ---- light vertices, path-major ----
+        // ---- light vertices, in path order ----
         SGPU_HIP_TRY(T.begin(kPhaseGrid));
-        SGPU_HIP_TRY(rocprim::exclusive_scan(tmp, tmp_bytes, vcount, voffset, 0u, (size_t)N + 1,
+        SGPU_HIP_TRY(rocprim::exclusive_scan(tmp, tmp_bytes, vcount, voffset, 0u, (size_t)total + 1,
                                              rocprim::plus<uint32_t>(), stream));
-        hipLaunchKernelGGL(k_compact, dim3(blocks_for(N)), dim3(kBlock), 0, stream, N, mlv, verts, vcount, voffset,
-                           cverts, cpos);
+        hipLaunchKernelGGL(k_compact, dim3(blocks_for(total)), dim3(kBlock), 0, stream, wv, verts, vcount, voffset,
+                           ps.slice, cverts, cpos, cslice);
         SGPU_HIP_TRY(hipGetLastError());
-        // path_end[i] = offset[i + 1] (inclusive end); the vertex count is offset[N]
-        SGPU_HIP_TRY(hipMemcpyAsync(path_end, voffset + 1, (size_t)N * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
-        SGPU_HIP_TRY(hipMemcpyAsync(nv_dev, voffset + N, sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
+        // path_end[g] = offset[g + 1] (inclusive end); the vertex count is offset[total]
+        SGPU_HIP_TRY(
+            hipMemcpyAsync(path_end, voffset + 1, (size_t)total * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
+        SGPU_HIP_TRY(hipMemcpyAsync(nv_dev, voffset + total, sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
         SGPU_HIP_TRY(T.end());
-        SGPU_HIP_TRY(hipMemcpyAsync(h, voffset + N, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        SGPU_HIP_TRY(hipMemcpyAsync(h, voffset + total, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
         SGPU_HIP_TRY(hipStreamSynchronize(stream));
         const uint32_t nv = h[0];
         n_light_verts += nv;
 
-        // ---- photon grid ----
+        // ---- photon grids ----
         SGPU_HIP_TRY(T.begin(kPhaseGrid));
-        SGPU_HIP_TRY(hipMemsetAsync(box, 0xFF, 8 * sizeof(uint32_t), stream));
-        SGPU_HIP_TRY(hipMemsetAsync(cell_counts, 0, (size_t)N * sizeof(uint32_t), stream));
+        SGPU_HIP_TRY(hipMemsetAsync(box, 0xFF, 8 * sizeof(uint32_t) * ns, stream));
+        SGPU_HIP_TRY(hipMemsetAsync(cell_counts, 0, (size_t)total * sizeof(uint32_t), stream));
         if (nv > 0) {
-            hipLaunchKernelGGL(k_grid_bounds, dim3(blocks_for(nv)), dim3(kBlock), 0, stream, cpos, nv_dev, box);
+            hipLaunchKernelGGL(k_grid_bounds, dim3(blocks_for(nv)), dim3(kBlock), 0, stream, cpos, cslice, nv_dev, box);
             SGPU_HIP_TRY(hipGetLastError());
-            hipLaunchKernelGGL(k_grid_keys, dim3(blocks_for(nv)), dim3(kBlock), 0, stream, cpos, nv_dev, box, P.inv_cell,
-                               N, keys, vals, cell_counts);
+            hipLaunchKernelGGL(k_grid_keys, dim3(blocks_for(nv)), dim3(kBlock), 0, stream, wv, cpos, cslice, nv_dev,
+                               box, keys, vals, cell_counts);
             SGPU_HIP_TRY(hipGetLastError());
             size_t need = 0;
             SGPU_HIP_TRY(rocprim::radix_sort_pairs(nullptr, need, keys, keys2, vals, vals2, (size_t)nv, 0, key_bits,
@@ -1162,7 +1269,7 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
             SGPU_HIP_TRY(
                 rocprim::radix_sort_pairs(tmp, tmp_bytes, keys, keys2, vals, vals2, (size_t)nv, 0, key_bits, stream));
         }
-        SGPU_HIP_TRY(rocprim::inclusive_scan(tmp, tmp_bytes, cell_counts, cell_ends, (size_t)N,
+        SGPU_HIP_TRY(rocprim::inclusive_scan(tmp, tmp_bytes, cell_counts, cell_ends, (size_t)total,
                                              rocprim::plus<uint32_t>(), stream));
         SGPU_HIP_TRY(T.end());
         GridView G;
@@ -1171,15 +1278,14 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
         G.indices = vals2;
         G.cverts = cverts;
         G.path_end = path_end;
-        G.cell_count = N;
         G.nv_ptr = nv_dev;
 
         // ---- camera pass ----
         SGPU_HIP_TRY(T.begin(kPhaseCamera));
-        hipLaunchKernelGGL(k_cam_gen, dim3(blocks_for(N)), dim3(kBlock), 0, stream, P, ps, rays[0], queue[0]);
+        hipLaunchKernelGGL(k_cam_gen, dim3(blocks_for(total)), dim3(kBlock), 0, stream, wv, ps, rays[0], queue[0]);
         SGPU_HIP_TRY(hipGetLastError());
         SGPU_HIP_TRY(T.end());
-        count = L > 1 ? N : 0;  // a camera path of length 1 ends at the lens (VCM.cpp:585)
+        count = L > 1 ? total : 0;  // a camera path of length 1 ends at the lens (VCM.cpp:585)
         cur = 0;
         while (count > 0) {
             sgpu_rays_soa rs = as_soa(rays[cur]);
@@ -1191,13 +1297,13 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
             ++n_launches;
             SGPU_HIP_TRY(hipMemsetAsync(counters, 0, 4 * sizeof(uint32_t), stream));
             SGPU_HIP_TRY(T.begin(kPhaseCamera));
-            hipLaunchKernelGGL(k_cam_shade, dim3(blocks_for(count)), dim3(kBlock), 0, stream, P, count, queue[cur],
+            hipLaunchKernelGGL(k_cam_shade, dim3(blocks_for(count)), dim3(kBlock), 0, stream, wv, count, queue[cur],
                                rays[cur], hb, ps, G, S, cam_vm, rays[1 - cur], queue[1 - cur], shadow, counters);
             SGPU_HIP_TRY(hipGetLastError());
             SGPU_HIP_TRY(T.end());
             SGPU_HIP_TRY(T.begin(kPhaseMerge));
-            hipLaunchKernelGGL(k_cam_merge, dim3(blocks_for(8ull * count)), dim3(kBlock), 0, stream, P, count, cam_vm, G,
-                               cpos, S);
+            hipLaunchKernelGGL(k_cam_merge, dim3(blocks_for(8ull * count)), dim3(kBlock), 0, stream, wv, count, cam_vm,
+                               ps.slice, G, cpos, S);
             SGPU_HIP_TRY(hipGetLastError());
             SGPU_HIP_TRY(T.end());
             SGPU_HIP_TRY(hipMemcpyAsync(h, counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
@@ -1224,10 +1330,12 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
             cur = 1 - cur;
         }
         SGPU_HIP_TRY(T.begin(kPhaseCamera));
-        hipLaunchKernelGGL(k_frame_add, dim3(blocks_for(N)), dim3(kBlock), 0, stream, N, ps.color, frame, fb);
+        hipLaunchKernelGGL(k_frame_add, dim3(blocks_for(N)), dim3(kBlock), 0, stream, wv, 0u, ns, N, ps.color, frame,
+                           fb);
         SGPU_HIP_TRY(hipGetLastError());
         SGPU_HIP_TRY(T.end());
         SGPU_HIP_TRY(T.collect());
+        SGPU_HIP_TRY(hipStreamSynchronize(stream));
     }
     const size_t nf = 3 * (size_t)N;
     hipLaunchKernelGGL(k_scale, dim3(blocks_for(nf)), dim3(kBlock), 0, stream, fb, nf, 1.0f / (float)job->iterations);
