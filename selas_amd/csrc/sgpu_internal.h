@@ -83,9 +83,13 @@ int sgpu_fail(int code, const std::string& msg);
 // ---- internal entry points between translation units --------------------------------------------
 // Builds d_nodes/d_tris for the scene's triangles (sgpu_bvh.hip).
 int sgpu_bvh_build(sgpu_scene scene, hipStream_t stream);
-// Launches the traversal kernels (sgpu_trace.hip).
+// Launches the traversal kernels (sgpu_trace.hip). `spill` (may be NULL: then the call allocates
+// its own, stream-ordered) holds the spilled part of the per-lane stacks: at least
+// sgpu_trace_spill_words(scene) 32-bit words, not shared with a concurrent launch.
 int sgpu_trace_launch(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits,
-                      uint32_t* occluded, uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream);
+                      uint32_t* occluded, uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream,
+                      uint32_t* spill = nullptr);
+size_t sgpu_trace_spill_words(sgpu_scene scene);
 // Wavefront path tracer (sgpu_pt.hip); arguments validated by sgpu_render_pt.
 int sgpu_pt_render_impl(sgpu_scene scene, const sgpu_pt_job* job, float* framebuffer, hipStream_t stream);
 int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* framebuffer, sgpu_vcm_stats* stats,

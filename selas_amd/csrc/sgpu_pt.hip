@@ -316,6 +316,7 @@ int sgpu_pt_render_impl(sgpu_scene scene, const sgpu_pt_job* job, float* fb, hip
     sb.path = A.get<uint32_t>(cap);
     sb.c = A.get<float>(3 * (size_t)cap);
     sb.occluded = A.get<uint32_t>(cap);
+    uint32_t* spill = A.get<uint32_t>(sgpu_trace_spill_words(scene));
     uint32_t* counters = A.get<uint32_t>(4);
     uint32_t* h_counters = nullptr;
     if (A.err != hipSuccess) return sgpu_fail(SGPU_E_NOMEM, "sgpu_render_pt: device allocation failed");
@@ -350,7 +351,7 @@ int sgpu_pt_render_impl(sgpu_scene scene, const sgpu_pt_job* job, float* fb, hip
         sp.path_base = path_base;
         for (uint32_t bounce = 0; bounce < job->max_path_length && count > 0; ++bounce) {
             sgpu_rays_soa rs = as_soa(rays[cur]);
-            int rc = sgpu_trace_launch(scene, &rs, &hsoa, nullptr, count, false, nullptr, stream);
+            int rc = sgpu_trace_launch(scene, &rs, &hsoa, nullptr, count, false, nullptr, stream, spill);
             if (rc) return rc;
             SGPU_HIP_TRY(hipMemsetAsync(counters, 0, 4 * sizeof(uint32_t), stream));
             sp.bounce = bounce;
@@ -362,7 +363,7 @@ int sgpu_pt_render_impl(sgpu_scene scene, const sgpu_pt_job* job, float* fb, hip
             const uint32_t n_next = h_counters[0], n_shadow = h_counters[1];
             if (n_shadow > 0) {
                 sgpu_rays_soa ss = as_soa(sb.ray);
-                rc = sgpu_trace_launch(scene, &ss, nullptr, sb.occluded, n_shadow, true, nullptr, stream);
+                rc = sgpu_trace_launch(scene, &ss, nullptr, sb.occluded, n_shadow, true, nullptr, stream, spill);
                 if (rc) return rc;
                 hipLaunchKernelGGL(k_pt_resolve, dim3((n_shadow + kBlock - 1) / kBlock), dim3(kBlock), 0, stream,
                                    n_shadow, sb, ps);

@@ -189,22 +189,29 @@ __global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace(const BvhNode* __res
 
 }  // namespace
 
+size_t sgpu_trace_spill_words(sgpu_scene scene)
+{
+    const uint32_t cus = (uint32_t)scene->ctx->props.multiProcessorCount;
+    const uint32_t threads = cus * 8 * SGPU_TRACE_BLOCK;
+    const uint32_t cap = scene->max_depth + 1;
+    const uint32_t spill_depth = cap > SGPU_STACK_LDS ? cap - SGPU_STACK_LDS : 1;
+    return (size_t)spill_depth * threads;
+}
+
 int sgpu_trace_launch(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits, uint32_t* occluded,
-                      uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream)
+                      uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream, uint32_t* spill_buf)
 {
     if (n == 0) return SGPU_OK;
     const uint32_t cus = (uint32_t)scene->ctx->props.multiProcessorCount;
     uint32_t blocks = (n + SGPU_TRACE_BLOCK - 1) / SGPU_TRACE_BLOCK;
     const uint32_t max_blocks = cus * 8;
     if (blocks > max_blocks) blocks = max_blocks;
-    const uint32_t threads = blocks * SGPU_TRACE_BLOCK;
     const uint32_t cap = scene->max_depth + 1;
-    const uint32_t spill_depth = cap > SGPU_STACK_LDS ? cap - SGPU_STACK_LDS : 1;
 
-    // stream-ordered scratch for the spilled part of the stacks, private to this call
-    uint32_t* spill = nullptr;
-    const size_t spill_bytes = (size_t)spill_depth * threads * sizeof(uint32_t);
-    SGPU_HIP_TRY(hipMallocAsync((void**)&spill, spill_bytes, stream));
+    // stream-ordered scratch for the spilled part of the stacks, private to this call, unless the
+    // caller lends a buffer of sgpu_trace_spill_words (sized for the largest grid)
+    uint32_t* spill = spill_buf;
+    if (!spill) SGPU_HIP_TRY(hipMallocAsync((void**)&spill, sgpu_trace_spill_words(scene) * sizeof(uint32_t), stream));
     uint32_t* overflow = scene->d_overflow;
 
     sgpu_hits_soa h = {};
@@ -229,6 +236,6 @@ int sgpu_trace_launch(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hi
                                overflow);
     }
     SGPU_HIP_TRY(hipGetLastError());
-    SGPU_HIP_TRY(hipFreeAsync(spill, stream));
+    if (!spill_buf) SGPU_HIP_TRY(hipFreeAsync(spill, stream));
     return SGPU_OK;
 }

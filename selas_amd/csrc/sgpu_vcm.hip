@@ -461,13 +461,18 @@ __device__ __forceinline__ uint32_t f2ord(float f)
 }
 __device__ __forceinline__ float ord2f(uint32_t u) { return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u); }
 
-// Per slice s: box[8 s + 0..2] = ord(min), box[8 s + 3..5] = ~ord(max), both reduced with atomicMin
-// (a wave whose vertices all belong to one slice reduces first and issues one atomic per value).
+// Per slice s: box[8 s + 0..2] = ord(min), box[8 s + 3..5] = ~ord(max), both reduced with atomicMin.
+// Vertices are slice-contiguous, so a block almost always holds one slice: it then reduces through
+// its waves and LDS and issues one atomic per value; a block that straddles slices falls back to
+// one atomic per value per wave (or per lane, in a wave that straddles them).
 __global__ __launch_bounds__(kBlock) void k_grid_bounds(const float* __restrict cpos, const uint32_t* __restrict cslice,
                                                         const uint32_t* __restrict nv_ptr, uint32_t* __restrict box)
 {
+    __shared__ uint32_t red[6][kBlock / 64];
+    __shared__ uint32_t wslice[kBlock / 64];
     const uint32_t nv = *nv_ptr;
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t wave = threadIdx.x >> 6;
     const bool valid = i < nv;
     uint32_t si = valid ? cslice[i] : 0xFFFFFFFFu;
     uint32_t v[6];
@@ -478,10 +483,9 @@ __global__ __launch_bounds__(kBlock) void k_grid_bounds(const float* __restrict 
             v[k] = f2ord(x);
             v[3 + k] = ~f2ord(x);
         }
-    // slice of the wave's first valid lane; uniform when every valid lane has it
+    // slice of the wave's first valid lane; the wave is uniform when every valid lane has it
     const uint64_t vmask = __ballot(valid);
-    if (vmask == 0) return;
-    const uint32_t leader = __ffsll((unsigned long long)vmask) - 1;
+    const uint32_t leader = vmask ? (uint32_t)(__ffsll((unsigned long long)vmask) - 1) : 0u;
     const uint32_t s0 = __shfl(si, leader, 64);
     const bool uniform = __ballot(valid && si != s0) == 0;
     if (uniform) {
@@ -490,10 +494,38 @@ __global__ __launch_bounds__(kBlock) void k_grid_bounds(const float* __restrict 
             for (int off = 32; off > 0; off >>= 1) x = min(x, (uint32_t)__shfl_xor((int)x, off, 64));
             v[k] = x;
         }
-        if (__lane_id() == leader)
-            for (int k = 0; k < 6; ++k) atomicMin(&box[8 * s0 + k], v[k]);
     } else if (valid) {
         for (int k = 0; k < 6; ++k) atomicMin(&box[8 * si + k], v[k]);
+    }
+    if (__lane_id() == 0) {
+        // 0xFFFFFFFF: wave without valid lanes; 0xFFFFFFFE: wave already flushed per lane
+        wslice[wave] = vmask == 0 ? 0xFFFFFFFFu : (uniform ? s0 : 0xFFFFFFFEu);
+        for (int k = 0; k < 6; ++k) red[k][wave] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        uint32_t bs = 0xFFFFFFFFu;
+        bool one = true;
+        for (int q = 0; q < kBlock / 64; ++q) {
+            const uint32_t ws = wslice[q];
+            if (ws == 0xFFFFFFFFu || ws == 0xFFFFFFFEu) continue;
+            if (bs == 0xFFFFFFFFu) bs = ws;
+            else if (ws != bs) one = false;
+        }
+        if (one) {
+            if (bs != 0xFFFFFFFFu) {
+                uint32_t x = 0xFFFFFFFFu;
+                for (int q = 0; q < kBlock / 64; ++q)
+                    if (wslice[q] == bs) x = min(x, red[k][q]);
+                atomicMin(&box[8 * bs + k], x);
+            }
+        } else {
+            for (int q = 0; q < kBlock / 64; ++q) {
+                const uint32_t ws = wslice[q];
+                if (ws != 0xFFFFFFFFu && ws != 0xFFFFFFFEu) atomicMin(&box[8 * ws + k], red[k][q]);
+            }
+        }
     }
 }
 
@@ -521,6 +553,20 @@ __global__ __launch_bounds__(kBlock) void k_grid_keys(Wave w, const float* __res
     keys[i] = cell;
     vals[i] = i;
     atomicAdd(&cell_counts[cell], 1u);
+}
+
+// Photons in grid order: slot q of the sorted (cell, vertex) list gets its vertex and position
+// copied next to each other, so a cell's range is read as contiguous memory by the range search.
+__global__ __launch_bounds__(kBlock) void k_grid_gather(const uint32_t* __restrict nv_ptr,
+                                                        const uint32_t* __restrict sorted_idx,
+                                                        const LightVertex* __restrict cverts, LightVertex* __restrict sverts,
+                                                        float4* __restrict spos)
+{
+    uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= *nv_ptr) return;
+    const LightVertex v = cverts[sorted_idx[q]];
+    sverts[q] = v;
+    spos[q] = make_float4(v.pos[0], v.pos[1], v.pos[2], 0.0f);
 }
 
 // ---- camera pass ----------------------------------------------------------------------------------
@@ -556,8 +602,9 @@ __global__ __launch_bounds__(kBlock) void k_cam_gen(Wave w, PathState ps, RayBuf
 struct GridView {
     const uint32_t* box;        // 8 per slice
     const uint32_t* cell_ends;  // inclusive ends over all cells of the wavefront
-    const uint32_t* indices;    // compacted vertex per sorted slot
-    const LightVertex* cverts;
+    const LightVertex* sverts;  // vertices in grid order (cell ranges are contiguous)
+    const float4* spos;         // their positions
+    const LightVertex* cverts;  // vertices in path order
     const uint32_t* path_end;   // inclusive end of each light path's vertices
     const uint32_t* nv_ptr;
 };
@@ -808,11 +855,10 @@ __global__ __launch_bounds__(kBlock) void k_cam_shade(Wave w, uint32_t count, co
 // One merge query per camera vertex, eight lanes per query: lane k of the group scans grid cell k of
 // the 2 x 2 x 2 block SearchHashGrid visits (x fastest, then y, then z, VCMHashGrid.cpp:131-152)
 // and the group adds its eight partial sums with a fixed butterfly ((c0+c1)+(c2+c3))+((c4+c5)+
-// (c6+c7)). Positions are tested against the radius from the packed position array before the
-// 64-byte vertex is read. MergeVertices (VCM.cpp:362-409) per vertex found.
+// (c6+c7)). A cell's vertices are contiguous (grid order); positions are tested against the radius
+// before the 64-byte vertex is read. MergeVertices (VCM.cpp:362-409) per vertex found.
 __global__ __launch_bounds__(kBlock) void k_cam_merge(Wave w, uint32_t count, const CamVertex* __restrict cam_vm,
-                                                      const uint32_t* __restrict path_slice, GridView G,
-                                                      const float* __restrict cpos, CamSlots S)
+                                                      const uint32_t* __restrict path_slice, GridView G, CamSlots S)
 {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t j = t >> 3, cell_k = t & 7u;
@@ -857,10 +903,10 @@ __global__ __launch_bounds__(kBlock) void k_cam_merge(Wave w, uint32_t count, co
                              mk3(v.n[0], v.n[1], v.n[2]), view, nullptr, s);
             const float ccp = cont_prob(s);
             for (uint32_t q = b; q < e; ++q) {
-                const uint32_t vi = G.indices[q];
-                f3 dv = x - mk3(cpos[3 * vi], cpos[3 * vi + 1], cpos[3 * vi + 2]);
+                const float4 lp = G.spos[q];
+                f3 dv = x - mk3(lp.x, lp.y, lp.z);
                 if (!(dot3(dv, dv) <= sl.radius_sq)) continue;
-                const LightVertex& lv = G.cverts[vi];
+                const LightVertex& lv = G.sverts[q];
                 const uint32_t lgp = lv.geom_plen;
                 if (plen + (lgp & 31u) > sl.L) continue;
                 const float* lmat = sl.materials + 22 * (size_t)sl.geom_material[lgp >> 5];
@@ -1095,6 +1141,8 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
     uint32_t* splat_pix = A.get<uint32_t>(TM);
     LightVertex* verts = A.get<LightVertex>(max_verts);
     LightVertex* cverts = A.get<LightVertex>(max_verts);
+    LightVertex* sverts = A.get<LightVertex>(max_verts);
+    float4* spos = A.get<float4>(max_verts);
     float* cpos = A.get<float>(3 * max_verts);
     uint32_t* cslice = A.get<uint32_t>(max_verts);
     uint32_t* vcount = A.get<uint32_t>((size_t)TM + 1);
@@ -1112,6 +1160,7 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
     float* frame = A.get<float>(3 * (size_t)TM);
     Slice* d_slices = A.get<Slice>(per_wave);
     CamVertex* cam_vm = A.get<CamVertex>(TM);
+    uint32_t* spill = A.get<uint32_t>(sgpu_trace_spill_words(scene));
     CamSlots S;
     S.c = A.get<float>(3 * (size_t)TM * nslots);
     S.valid = A.get<uint32_t>((size_t)TM * nslots);
@@ -1199,7 +1248,7 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
         while (count > 0) {
             sgpu_rays_soa rs = as_soa(rays[cur]);
             SGPU_HIP_TRY(T.begin(kPhaseClosest));
-            int rc = sgpu_trace_launch(scene, &rs, &hsoa, nullptr, count, false, d_trav, stream);
+            int rc = sgpu_trace_launch(scene, &rs, &hsoa, nullptr, count, false, d_trav, stream, spill);
             if (rc) return rc;
             SGPU_HIP_TRY(T.end());
             n_closest += count;
@@ -1217,7 +1266,7 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
             if (n_sh > 0) {
                 sgpu_rays_soa ss = as_soa(shadow);
                 SGPU_HIP_TRY(T.begin(kPhaseShadow));
-                rc = sgpu_trace_launch(scene, &ss, nullptr, occl, n_sh, true, nullptr, stream);
+                rc = sgpu_trace_launch(scene, &ss, nullptr, occl, n_sh, true, nullptr, stream, spill);
                 if (rc) return rc;
                 SGPU_HIP_TRY(T.end());
                 n_shadow += n_sh;
@@ -1268,6 +1317,9 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
             }
             SGPU_HIP_TRY(
                 rocprim::radix_sort_pairs(tmp, tmp_bytes, keys, keys2, vals, vals2, (size_t)nv, 0, key_bits, stream));
+            hipLaunchKernelGGL(k_grid_gather, dim3(blocks_for(nv)), dim3(kBlock), 0, stream, nv_dev, vals2, cverts, sverts,
+                               spos);
+            SGPU_HIP_TRY(hipGetLastError());
         }
         SGPU_HIP_TRY(rocprim::inclusive_scan(tmp, tmp_bytes, cell_counts, cell_ends, (size_t)total,
                                              rocprim::plus<uint32_t>(), stream));
@@ -1275,7 +1327,8 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
         GridView G;
         G.box = box;
         G.cell_ends = cell_ends;
-        G.indices = vals2;
+        G.sverts = sverts;
+        G.spos = spos;
         G.cverts = cverts;
         G.path_end = path_end;
         G.nv_ptr = nv_dev;
@@ -1290,7 +1343,7 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
         while (count > 0) {
             sgpu_rays_soa rs = as_soa(rays[cur]);
             SGPU_HIP_TRY(T.begin(kPhaseClosest));
-            int rc = sgpu_trace_launch(scene, &rs, &hsoa, nullptr, count, false, d_trav, stream);
+            int rc = sgpu_trace_launch(scene, &rs, &hsoa, nullptr, count, false, d_trav, stream, spill);
             if (rc) return rc;
             SGPU_HIP_TRY(T.end());
             n_closest += count;
@@ -1303,7 +1356,7 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
             SGPU_HIP_TRY(T.end());
             SGPU_HIP_TRY(T.begin(kPhaseMerge));
             hipLaunchKernelGGL(k_cam_merge, dim3(blocks_for(8ull * count)), dim3(kBlock), 0, stream, wv, count, cam_vm,
-                               ps.slice, G, cpos, S);
+                               ps.slice, G, S);
             SGPU_HIP_TRY(hipGetLastError());
             SGPU_HIP_TRY(T.end());
             SGPU_HIP_TRY(hipMemcpyAsync(h, counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
@@ -1312,7 +1365,7 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
             if (n_sh > 0) {
                 sgpu_rays_soa ss = as_soa(shadow);
                 SGPU_HIP_TRY(T.begin(kPhaseShadow));
-                rc = sgpu_trace_launch(scene, &ss, nullptr, occl, n_sh, true, nullptr, stream);
+                rc = sgpu_trace_launch(scene, &ss, nullptr, occl, n_sh, true, nullptr, stream, spill);
                 if (rc) return rc;
                 SGPU_HIP_TRY(T.end());
                 n_shadow += n_sh;
