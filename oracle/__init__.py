@@ -341,3 +341,32 @@ def nee(lights, position, normal, uniforms, cosine_once=False):
         lib.orc_nee(C.byref(un), lights.ctypes.data, lights.shape[0], pos.ctypes.data, nrm.ctypes.data,
                     int(cosine_once), out[i].ctypes.data)
     return out
+
+
+class _Ptex(C.Structure):
+    _fields_ = [("res_log2", C.c_int), ("nfaces", C.c_uint32), ("texels", C.c_void_p), ("adjfaces", C.c_void_p),
+                ("adjedges", C.c_void_p)]
+
+
+lib.orc_ptex_eval_batch.argtypes = [C.POINTER(_Ptex), C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+
+
+class PtexTexture:
+    """Per-face texture data for the oracle's filter: texels (nfaces, res, res, 3) uint8,
+    adjfaces / adjedges (nfaces, 4)."""
+
+    def __init__(self, texels, res_log2, adjfaces, adjedges):
+        self.texels = np.ascontiguousarray(texels, np.uint8)
+        self.adjfaces = np.ascontiguousarray(adjfaces, np.int32)
+        self.adjedges = np.ascontiguousarray(adjedges, np.int32)
+        self.s = _Ptex(int(res_log2), self.texels.shape[0], self.texels.ctypes.data, self.adjfaces.ctypes.data,
+                       self.adjedges.ctypes.data)
+
+    def eval(self, faces, u, v):
+        faces = np.ascontiguousarray(faces, np.int32)
+        u = np.ascontiguousarray(u, np.float32)
+        v = np.ascontiguousarray(v, np.float32)
+        out = np.zeros((faces.shape[0], 3), np.float32)
+        lib.orc_ptex_eval_batch(C.byref(self.s), faces.shape[0], faces.ctypes.data, u.ctypes.data, v.ctypes.data,
+                                out.ctypes.data)
+        return out

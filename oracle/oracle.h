@@ -171,6 +171,19 @@ uint32_t orc_hash_grid_query(const orc_hash_grid* g, const float* points, const 
  * result does not depend on the thread count. */
 void orc_render_vcm(const orc_scene* sc, const orc_vcm_job* job, float* fb_out, float* stats_out, int threads);
 
+/* ---- per-face textures (orc_ptex.c) ------------------------------------------------------------- */
+typedef struct {
+    int res_log2;           /* every face has (1 << res_log2)^2 texels */
+    uint32_t nfaces;
+    const uint8_t* texels;  /* nfaces x res x res x 3 (row = v texel) */
+    const int32_t* adjfaces; /* 4 per face: bottom, right, top, left; -1 = border */
+    const int32_t* adjedges; /* 4 per face: the neighbour's edge id of the shared edge */
+} orc_ptex;
+
+void orc_ptex_eval(const orc_ptex* tx, int face, float u, float v, float* out3);
+void orc_ptex_eval_batch(const orc_ptex* tx, uint32_t n, const int32_t* faces, const float* u, const float* v,
+                         float* out);
+
 #ifdef __cplusplus
 }
 #endif

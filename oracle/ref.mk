@@ -11,7 +11,8 @@ APP := $(REF)/Source/Applications/Selas/Source
 OUT := _ref
 CXX ?= g++
 CXXFLAGS := -std=c++14 -O2 -fPIC -ffp-contract=off -DIsOsx_=1 -w \
-            -I$(CORE) -I$(APP) -I$(REF)/Middleware -I$(REF)/Middleware/embree-latest/include
+            -I$(CORE) -I$(APP) -I$(REF)/Middleware -I$(REF)/Middleware/embree-latest/include \
+            -I$(REF)/Middleware/ptex/Include
 
 SRCS := $(CORE)/Shading/Disney.cpp $(CORE)/Shading/Ggx.cpp $(CORE)/Shading/Fresnel.cpp \
         $(CORE)/MathLib/FloatFuncs.cpp $(CORE)/MathLib/FloatStructs.cpp $(CORE)/MathLib/Trigonometric.cpp \
@@ -20,6 +21,12 @@ SRCS := $(CORE)/Shading/Disney.cpp $(CORE)/Shading/Ggx.cpp $(CORE)/Shading/Fresn
         $(CORE)/GeometryLib/RectangulerLightSampler.cpp $(CORE)/GeometryLib/Disc.cpp \
         $(CORE)/GeometryLib/AxisAlignedBox.cpp $(CORE)/SystemLib/Memory.cpp $(CORE)/SystemLib/JsAssert.cpp \
         $(CORE)/IoLib/Serializer.cpp $(APP)/VCMHashGrid.cpp
+# the vendored Ptex 2.x reader and filters (PtexFilter::getFilter, PtexTexture::open); zlib is the
+# system's
+PTEX := $(REF)/Middleware/ptex/Include
+SRCS += $(PTEX)/PtexReader.cpp $(PTEX)/PtexCache.cpp $(PTEX)/PtexFilters.cpp $(PTEX)/PtexHalf.cpp \
+        $(PTEX)/PtexSeparableFilter.cpp $(PTEX)/PtexSeparableKernel.cpp $(PTEX)/PtexTriangleFilter.cpp \
+        $(PTEX)/PtexTriangleKernel.cpp $(PTEX)/PtexUtils.cpp
 OBJS := $(patsubst $(REF)/%.cpp,$(OUT)/obj/%.o,$(SRCS)) $(OUT)/obj/MemoryAllocation.o $(OUT)/obj/ref_harness.o
 
 all: $(OUT)/libselas_ref.so
@@ -37,6 +44,6 @@ $(OUT)/obj/ref_harness.o: ref_harness.cpp
 	$(CXX) $(CXXFLAGS) -c $< -o $@
 
 $(OUT)/libselas_ref.so: $(OBJS)
-	$(CXX) -shared -o $@ $(OBJS) -Wl,--no-undefined
+	$(CXX) -shared -o $@ $(OBJS) -lz -Wl,--no-undefined
 
 .PHONY: all

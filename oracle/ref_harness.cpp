@@ -4,6 +4,7 @@
 // that tests/golden/make_golden.py can record their outputs as fixtures. It restates nothing; every
 // value it returns is computed by the reference's own compiled code.
 #include "GeometryLib/Camera.h"
+#include "Ptexture.h"
 #include "GeometryLib/CoordinateSystem.h"
 #include "GeometryLib/Disc.h"
 #include "GeometryLib/RectangulerLightSampler.h"
@@ -267,6 +268,21 @@ void ref_hash_grid(const float* points, uint32_t count, uint32_t cell_count, flo
     }
     ShutdownHashGrid(&grid);
     verts.Shutdown();
+}
+
+// PtexFilter{f_bspline}.eval as CalculateSurfaceParams calls it (SurfaceParameters.cpp:196-203):
+// 3 channels from channel 0, zero filter footprint. Returns -1 when the file cannot be opened.
+int ref_ptex_eval(const char* path, uint32_t n, const int32_t* faces, const float* u, const float* v, float* out)
+{
+    Ptex::String err;
+    PtexTexture* tex = PtexTexture::open(path, err);
+    if (!tex) return -1;
+    PtexFilter::Options opts(PtexFilter::f_bspline);
+    PtexFilter* filter = PtexFilter::getFilter(tex, opts);
+    for (uint32_t i = 0; i < n; ++i) filter->eval(out + 3 * (size_t)i, 0, 3, faces[i], u[i], v[i], 0, 0, 0, 0);
+    filter->release();
+    tex->release();
+    return 0;
 }
 
 }  // extern "C"

@@ -8,7 +8,8 @@ into oracle/_ref/libselas_ref.so; see oracle/ref.mk and oracle/ref_harness.cpp):
 Inputs are drawn from a seeded numpy generator; every output is computed by the reference's own
 functions (EvaluateDisney, SampleDisney, the GGX/Fresnel helpers, CorrelatedMultiJitter,
 InitializeRayCastCamera + JitteredCameraRay, the spherical-rectangle sampler, RayQuad,
-SampleConcentricDisc, BuildHashGrid + SearchHashGrid). The fixtures are data only.
+SampleConcentricDisc, BuildHashGrid + SearchHashGrid, PtexFilter f_bspline over Ptex files written
+by tests/golden/ptx.py). The fixtures are data only. `--only ptex` rewrites just the Ptex cases.
 """
 from __future__ import annotations
 
@@ -249,10 +250,58 @@ def main():
     for k, c in enumerate(cases):
         out[f"hash_grid_{k}"] = c
 
+    write(out)
+
+
+def write(out):
     for name, d in out.items():
         np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **d)
         print("wrote", name)
 
 
+def ptex_cases():
+    """PtexFilter{f_bspline}.eval (the call of SurfaceParameters.cpp:196-203) on three quad meshes:
+    a 5 x 4 grid at 16 x 16 texels per face (borders, valence-4 interior corners), a closed cube
+    at 8 x 8 (rotated neighbours, valence-3 corners) and five quads around a valence-5 vertex at
+    4 x 4. Sample points mix uniform (u, v) with texel centres and edges, face edges and corners."""
+    import ptx
+
+    lib = C.CDLL(LIB)
+    lib.ref_ptex_eval.argtypes = [C.c_char_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.ref_ptex_eval.restype = C.c_int
+    rng = np.random.default_rng(42)
+    out = {}
+    meshes = [("grid", 4, ptx.grid_adjacency(5, 4), 3000),
+              ("cube", 3, ptx.quad_mesh_adjacency(ptx.cube_quads()), 2000),
+              ("star", 2, ptx.quad_mesh_adjacency(ptx.star_quads(5)), 1000)]
+    for name, rl, (af, ae), n in meshes:
+        nf, res = af.shape[0], 1 << rl
+        texels = rng.integers(0, 256, (nf, res, res, 3), dtype=np.uint8)
+        path = os.path.join(HERE, f"ptex_{name}.ptx")
+        ptx.write_ptx(path, rl, texels, af, ae)
+        faces = rng.integers(0, nf, n).astype(np.int32)
+        special = np.array([0.0, 1.0, 0.5, 0.5 / res, 1 - 0.5 / res, 1.0 / res, 1 - 1.0 / res, 1e-4, 1 - 1e-4],
+                           np.float32)
+        u = rng.random(n).astype(np.float32)
+        v = rng.random(n).astype(np.float32)
+        pick = rng.random((n, 2))
+        u[pick[:, 0] < 0.3] = special[rng.integers(0, special.size, (pick[:, 0] < 0.3).sum())]
+        v[pick[:, 1] < 0.3] = special[rng.integers(0, special.size, (pick[:, 1] < 0.3).sum())]
+        res_out = np.zeros((n, 3), np.float32)
+        rc = lib.ref_ptex_eval(path.encode(), n, faces.ctypes.data, u.ctypes.data, v.ctypes.data,
+                               res_out.ctypes.data)
+        assert rc == 0
+        out[f"ptex_{name}"] = dict(res_log2=np.int32(rl), texels=texels, adjfaces=af, adjedges=ae, faces=faces, u=u,
+                                   v=v, out=res_out)
+    return out
+
+
 if __name__ == "__main__":
-    main()
+    import sys
+
+    sys.path.insert(0, HERE)
+    if "--only" in sys.argv and sys.argv[sys.argv.index("--only") + 1] == "ptex":
+        write(ptex_cases())
+    else:
+        main()
+        write(ptex_cases())

@@ -1,0 +1,109 @@
+"""Minimal writer of Ptex files (.ptx) for test fixtures: quad meshes, 8-bit RGB, one resolution
+for all faces, base level only, every face zip-encoded.
+
+The layout is the one the Ptex reader parses (Middleware/ptex/Include/PtexIO.h:44-99 and
+PtexReader.cpp:155-200, 595-700): header, extended header, zipped face-info block, zipped
+per-face constant values, level-info block, level data (zipped face-data headers, then one zipped
+block per face holding the texels plane by plane), then an empty meta-data block and the 8-byte
+compatibility barrier. Written from that format description; the reference ships no writer.
+
+Also builds the adjacency of the test meshes (Ptex edge ids: 0 bottom v=0, 1 right u=1, 2 top v=1,
+3 left u=0; an edge's adjacent edge is the neighbour's edge id of the shared edge).
+"""
+from __future__ import annotations
+
+import struct
+import zlib
+
+import numpy as np
+
+MAGIC = ord("P") | (ord("t") << 8) | (ord("e") << 16) | (ord("x") << 24)
+MT_QUAD = 1
+DT_UINT8 = 0
+ENC_ZIPPED = 1
+
+
+def write_ptx(path, res_log2, texels, adjfaces, adjedges):
+    """texels: (nfaces, res, res, 3) uint8, row index = v texel, column = u texel.
+    adjfaces: (nfaces, 4) int (-1 = border); adjedges: (nfaces, 4) int in 0..3."""
+    texels = np.ascontiguousarray(texels, np.uint8)
+    nfaces, res = texels.shape[0], 1 << res_log2
+    assert texels.shape == (nfaces, res, res, 3)
+    # face info: res (ulog2, vlog2), adjedges (2 bits per edge), flags, adjfaces[4]
+    fi = b"".join(
+        struct.pack("<bbBB4i", res_log2, res_log2,
+                    sum((int(adjedges[f][e]) & 3) << (2 * e) for e in range(4)), 0,
+                    *[int(a) for a in adjfaces[f]])
+        for f in range(nfaces))
+    fi_z = zlib.compress(fi)
+    const = np.round(texels.reshape(nfaces, -1, 3).mean(axis=1)).astype(np.uint8)
+    const_z = zlib.compress(const.tobytes())
+    blocks = [zlib.compress(np.ascontiguousarray(texels[f].transpose(2, 0, 1)).tobytes()) for f in range(nfaces)]
+    fdh = struct.pack("<%dI" % nfaces, *[(len(b) & 0x3FFFFFFF) | (ENC_ZIPPED << 30) for b in blocks])
+    fdh_z = zlib.compress(fdh)
+    level = fdh_z + b"".join(blocks)
+    levelinfo = struct.pack("<QII", len(level), len(fdh_z), nfaces)
+    ext = struct.pack("<HHHHIIQQQ", 0, 0, 0, 0, 0, 0, 0, 0, 0)  # clamp borders, no edge filter, no edits
+    header = struct.pack("<IIIIiHHIIIIIIQII", MAGIC, 1, MT_QUAD, DT_UINT8, -1, 3, 1, nfaces, len(ext), len(fi_z),
+                         len(const_z), len(levelinfo), 4, len(level), 0, 0)
+    assert len(header) == 64 and len(ext) == 40
+    with open(path, "wb") as f:
+        f.write(header + ext + fi_z + const_z + levelinfo + level + struct.pack("<Q", 0))
+
+
+def grid_adjacency(nu, nv):
+    """Quad grid, face (i, j) = j * nu + i with u along i and v along j (the face order and uv
+    orientation of quad_grid_mesh in selas_amd/workloads.py)."""
+    n = nu * nv
+    af = -np.ones((n, 4), np.int32)
+    ae = np.zeros((n, 4), np.int32)
+    for j in range(nv):
+        for i in range(nu):
+            f = j * nu + i
+            if j > 0:
+                af[f, 0], ae[f, 0] = f - nu, 2
+            if i < nu - 1:
+                af[f, 1], ae[f, 1] = f + 1, 3
+            if j < nv - 1:
+                af[f, 2], ae[f, 2] = f + nu, 0
+            if i > 0:
+                af[f, 3], ae[f, 3] = f - 1, 1
+    return af, ae
+
+
+def quad_mesh_adjacency(quads):
+    """Adjacency of a quad mesh given as (n, 4) vertex indices, counter-clockwise (v0 v1 v2 v3 with
+    u along v0->v1 and v along v0->v3): edge e runs from corner e to corner e+1."""
+    quads = np.asarray(quads)
+    n = quads.shape[0]
+    edges = {}
+    for f in range(n):
+        for e in range(4):
+            a, b = int(quads[f, e]), int(quads[f, (e + 1) % 4])
+            edges[(a, b)] = (f, e)
+    af = -np.ones((n, 4), np.int32)
+    ae = np.zeros((n, 4), np.int32)
+    for f in range(n):
+        for e in range(4):
+            a, b = int(quads[f, e]), int(quads[f, (e + 1) % 4])
+            if (b, a) in edges:
+                af[f, e], ae[f, e] = edges[(b, a)]
+    return af, ae
+
+
+def cube_quads():
+    """A cube as 6 quads with outward orientation and assorted rotations between neighbours (every
+    corner has valence 3)."""
+    quads = np.array([[0, 3, 2, 1], [4, 5, 6, 7], [0, 1, 5, 4], [2, 3, 7, 6], [1, 2, 6, 5], [0, 4, 7, 3]])
+    return quads
+
+
+def star_quads(k=5):
+    """k quads around a centre vertex (valence k) with a border: centre 0, ring vertices 1..2k."""
+    quads = []
+    for i in range(k):
+        a = 1 + 2 * i
+        b = 1 + 2 * i + 1
+        c = 1 + (2 * i + 2) % (2 * k)
+        quads.append([0, a, b, c])
+    return np.array(quads)
