@@ -61,7 +61,8 @@ typedef struct sgpu_rays_soa {
  * (embree3/rtcore_ray.h:43-55). Any pointer may be NULL to skip that output.
  *   t       hit distance (Embree writes it to ray.tfar); tfar is written on a miss
  *   u, v    barycentric weights of vertices 1 and 2
- *   prim_id primitive index within its geometry, SGPU_INVALID_ID on a miss
+ *   prim_id primitive index within its geometry (the quad index for sgpu_scene_add_quads
+ *           geometries), SGPU_INVALID_ID on a miss
  *   geom_id geometry id returned by sgpu_scene_add_triangles, SGPU_INVALID_ID on a miss
  *   ng_*    unnormalised geometric normal (v1-v0)x(v2-v0)
  * Passing rays.tfar's storage as hits.t reproduces Embree's in-place tfar update. */
@@ -126,6 +127,13 @@ int sgpu_scene_create(sgpu_context ctx, sgpu_scene* out);
 int sgpu_scene_add_triangles(sgpu_scene scene, const float* positions, uint32_t n_verts,
                              const uint32_t* indices, uint32_t n_tris,
                              const float* colmajor4x4, uint32_t* geom_id_out);
+/* Quad mesh (4 vertex indices per quad, v0 v1 v2 v3 counter-clockwise): replaces
+ * rtcNewGeometry(RTC_GEOMETRY_TYPE_QUAD) (ModelResource.cpp:273). As Embree does, each quad is
+ * intersected as the triangles (v0, v1, v3) and (v2, v3, v1); a hit reports the quad index as
+ * prim_id and (u, v) as the quad's own coordinates (u along v0->v1, v along v0->v3: the second
+ * triangle's barycentrics (u', v') become (1 - u', 1 - v')), the coordinates Ptex faces use. */
+int sgpu_scene_add_quads(sgpu_scene scene, const float* positions, uint32_t n_verts, const uint32_t* indices,
+                         uint32_t n_quads, const float* colmajor4x4, uint32_t* geom_id_out);
 int sgpu_scene_commit(sgpu_scene scene, void* stream);
 int sgpu_scene_release(sgpu_scene scene);
 /* Number of triangles, BVH nodes and maximum leaf depth of a committed scene. */
@@ -196,6 +204,22 @@ int sgpu_scene_set_materials(sgpu_scene scene, const float* materials, uint32_t 
 /* Lights of a scene (host array, copied; one light set holding all of them) and the constant
  * background radiance (SceneResourceData::backgroundIntensity, SceneResource.h:53). */
 int sgpu_scene_set_lights(sgpu_scene scene, const sgpu_light* lights, uint32_t n_lights, const float* background_rgb);
+
+/* Per-face texture of a quad geometry: replaces TextureCache::FetchPtex + PtexTexture::open
+ * (TextureCache.cpp:168, SurfaceParameters.cpp:197) for 8-bit RGB Ptex data with one resolution
+ * for every face. texels: n_faces x (1 << res_log2)^2 x 3 bytes (face-major, rows along v);
+ * adjfaces / adjedges: 4 per face in Ptex edge order (bottom, right, top, left), the adjacent face
+ * (-1 = border) and that face's edge id of the shared edge (Ptex FaceInfo, Ptexture.h:240-300).
+ * n_faces must equal the geometry's quad count; res_log2 <= 8. Host arrays, copied. A material
+ * with flag 16 (eUsesPtex) takes its base colour from its geometry's texture. */
+int sgpu_scene_set_ptex(sgpu_scene scene, uint32_t geom_id, uint32_t res_log2, uint32_t n_faces,
+                        const uint8_t* texels, const int32_t* adjfaces, const int32_t* adjedges);
+
+/* PtexFilter{f_bspline}.eval(out, 0, 3, face, u, v, 0, 0, 0, 0) (Ptexture.h:989-1010, as
+ * SurfaceParameters.cpp:199-203 calls it) for n lookups on the texture of geometry geom_id:
+ * faces, u, v and out (3 floats per lookup) are device arrays. */
+int sgpu_ptex_eval(sgpu_scene scene, uint32_t geom_id, const int32_t* faces, const float* u, const float* v,
+                   uint32_t n, float* out, void* stream);
 
 /* ---- integrators -------------------------------------------------------------------------------- */
 typedef struct sgpu_camera {

@@ -225,10 +225,16 @@ def concentric_disc(r0, r1):
 
 # ---- integrators (orc_pt.c) --------------------------------------------------------------------------
 
+class _Ptex(C.Structure):
+    _fields_ = [("res_log2", C.c_int), ("nfaces", C.c_uint32), ("texels", C.c_void_p), ("adjfaces", C.c_void_p),
+                ("adjedges", C.c_void_p)]
+
+
 class _Scene(C.Structure):
     _fields_ = [("bvh", C.c_void_p), ("tri9", C.c_void_p), ("tri_geom", C.c_void_p), ("materials", C.c_void_p),
                 ("geom_material", C.c_void_p), ("n_geoms", C.c_uint32), ("lights", C.c_void_p),
-                ("n_lights", C.c_uint32), ("background", C.c_float * 3)]
+                ("n_lights", C.c_uint32), ("background", C.c_float * 3), ("geom_first", C.c_void_p),
+                ("geom_tex", C.c_void_p)]
 
 
 class _PtJob(C.Structure):
@@ -240,19 +246,33 @@ lib.orc_render_pt.argtypes = [C.POINTER(_Scene), C.POINTER(_PtJob), C.c_void_p, 
 
 
 class RenderScene:
-    """The oracle's view of a workloads.*_scene dict (flattened triangles + the oracle SAH tree)."""
+    """The oracle's view of a workloads.*_scene dict: flattened triangles (quad geometries split as
+    Embree splits them), the oracle SAH tree, and the per-face textures of quad geometries."""
 
     def __init__(self, sc: dict):
-        self.tri9 = np.ascontiguousarray(np.concatenate(sc["geoms"]), np.float32)
-        self.tri_geom = np.concatenate([np.full(g.shape[0], i, np.uint32) for i, g in enumerate(sc["geoms"])])
+        from selas_amd.workloads import geom_tri9
+
+        tris = [geom_tri9(g) for g in sc["geoms"]]
+        self.tri9 = np.ascontiguousarray(np.concatenate(tris), np.float32)
+        self.tri_geom = np.concatenate([np.full(t.shape[0], i, np.uint32) for i, t in enumerate(tris)])
+        self.geom_first = np.cumsum([0] + [t.shape[0] for t in tris[:-1]]).astype(np.uint32)
         self.materials = np.ascontiguousarray(sc["materials"], np.float32)
         self.geom_material = np.ascontiguousarray(sc["geom_material"], np.uint32)
         self.lights = np.ascontiguousarray(sc["lights"])
+        self.textures = []
+        self.geom_tex = (_Ptex * len(tris))()
+        for i, g in enumerate(sc["geoms"]):
+            if isinstance(g, dict) and g.get("ptex") is not None:
+                p = g["ptex"]
+                t = PtexTexture(p["texels"], p["res_log2"], p["adjfaces"], p["adjedges"])
+                self.textures.append(t)
+                self.geom_tex[i] = t.s
         self.bvh = CpuBvh(self.tri9)
         self.s = _Scene(self.bvh.handle, self.tri9.ctypes.data, self.tri_geom.ctypes.data, self.materials.ctypes.data,
                         self.geom_material.ctypes.data, self.geom_material.shape[0],
                         self.lights.ctypes.data if self.lights.size else None, self.lights.shape[0],
-                        (C.c_float * 3)(*[float(x) for x in sc["background"]]))
+                        (C.c_float * 3)(*[float(x) for x in sc["background"]]), self.geom_first.ctypes.data,
+                        C.cast(self.geom_tex, C.c_void_p))
 
 
 def render_pt(rs: RenderScene, cam: dict, width, height, spp, max_path_length, seed, flags=0, threads=0):
@@ -341,11 +361,6 @@ def nee(lights, position, normal, uniforms, cosine_once=False):
         lib.orc_nee(C.byref(un), lights.ctypes.data, lights.shape[0], pos.ctypes.data, nrm.ctypes.data,
                     int(cosine_once), out[i].ctypes.data)
     return out
-
-
-class _Ptex(C.Structure):
-    _fields_ = [("res_log2", C.c_int), ("nfaces", C.c_uint32), ("texels", C.c_void_p), ("adjfaces", C.c_void_p),
-                ("adjedges", C.c_void_p)]
 
 
 lib.orc_ptex_eval_batch.argtypes = [C.POINTER(_Ptex), C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]

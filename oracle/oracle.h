@@ -113,6 +113,15 @@ typedef struct {
     float position[3], direction[3], x[3], z[3], radiance[3];
 } orc_light;
 
+/* ---- per-face textures (orc_ptex.c) ------------------------------------------------------------- */
+typedef struct {
+    int res_log2;           /* every face has (1 << res_log2)^2 texels */
+    uint32_t nfaces;
+    const uint8_t* texels;  /* nfaces x res x res x 3 (row = v texel) */
+    const int32_t* adjfaces; /* 4 per face: bottom, right, top, left; -1 = border */
+    const int32_t* adjedges; /* 4 per face: the neighbour's edge id of the shared edge */
+} orc_ptex;
+
 typedef struct {
     const orc_bvh* bvh;
     const float* tri9;          /* world-space triangles, 9 floats each (global index order) */
@@ -123,6 +132,9 @@ typedef struct {
     const orc_light* lights;
     uint32_t n_lights;
     float background[3];
+    const uint32_t* geom_first; /* first triangle of each geometry; NULL when no textures */
+    const orc_ptex* geom_tex;   /* per geometry (nfaces == 0: none); a textured geometry is a quad
+                                   mesh whose triangles 2q and 2q+1 are Embree's split of quad q */
 } orc_scene;
 
 typedef struct {
@@ -134,6 +146,10 @@ typedef struct {
 
 void orc_surface_from_hit(const float* o, const float* d, float t, const float* ng, const float* mat,
                           const float* texel, orc_surface* s, float* position, float* error);
+/* The base-colour texel of a hit on triangle `tri` (barycentrics hu, hv) when the material uses a
+ * per-face texture and its geometry has one; returns NULL otherwise. */
+const float* orc_hit_texel(const orc_scene* sc, uint32_t tri, uint32_t geom, const float* mat, float hu, float hv,
+                           float* texel3);
 void orc_offset_origin(const orc_surface* s, const float* pos, float error, const float* dir, float bias, float* out,
                        float* signed_bias);
 void orc_nee(orc_uniforms* u, const orc_light* lights, uint32_t n_lights, const float* position, const float* normal,
@@ -170,15 +186,6 @@ uint32_t orc_hash_grid_query(const orc_hash_grid* g, const float* points, const 
  * NULL): light vertices per iteration, rays traced. Paths of one pass run on `threads` threads; the
  * result does not depend on the thread count. */
 void orc_render_vcm(const orc_scene* sc, const orc_vcm_job* job, float* fb_out, float* stats_out, int threads);
-
-/* ---- per-face textures (orc_ptex.c) ------------------------------------------------------------- */
-typedef struct {
-    int res_log2;           /* every face has (1 << res_log2)^2 texels */
-    uint32_t nfaces;
-    const uint8_t* texels;  /* nfaces x res x res x 3 (row = v texel) */
-    const int32_t* adjfaces; /* 4 per face: bottom, right, top, left; -1 = border */
-    const int32_t* adjedges; /* 4 per face: the neighbour's edge id of the shared edge */
-} orc_ptex;
 
 void orc_ptex_eval(const orc_ptex* tx, int face, float u, float v, float* out3);
 void orc_ptex_eval_batch(const orc_ptex* tx, uint32_t n, const int32_t* faces, const float* u, const float* v,

@@ -80,6 +80,25 @@ void orc_surface_from_hit(const float* o, const float* d, float t, const float* 
     orc_surface_from_block(block, nn, s);
 }
 
+/* CalculateSurfaceParams' eUsesPtex branch (SurfaceParameters.cpp:196-203): the texture is
+ * looked up at (primId, baryCoords); for Embree quad geometry those are the quad and its own
+ * coordinates, i.e. (u, v) on the first triangle of the split and (1 - u, 1 - v) on the second. */
+const float* orc_hit_texel(const orc_scene* sc, uint32_t tri, uint32_t geom, const float* mat, float hu, float hv,
+                           float* texel3)
+{
+    if (!(((uint32_t)mat[21]) & 16u) || !sc->geom_tex || !sc->geom_first) return NULL;
+    const orc_ptex* tx = &sc->geom_tex[geom];
+    if (tx->nfaces == 0) return NULL;
+    uint32_t local = tri - sc->geom_first[geom];
+    float u = hu, v = hv;
+    if (local & 1u) {
+        u = 1.0f - hu;
+        v = 1.0f - hv;
+    }
+    orc_ptex_eval(tx, (int)(local >> 1), u, v, texel3);
+    return texel3;
+}
+
 /* OffsetRayOrigin (SurfaceParameters.cpp:403-416) */
 void orc_offset_origin(const orc_surface* s, const float* pos, float error, const float* dir, float bias, float* out,
                        float* signed_bias)
@@ -207,7 +226,8 @@ static void evaluate_path(const pt_ctx* c, uint32_t* stack, uint32_t x, uint32_t
         const float* mat = sc->materials + 22 * (size_t)sc->geom_material[geom];
         orc_surface s;
         float pos[3], err;
-        orc_surface_from_hit(o, d, t, ng, mat, NULL, &s, pos, &err);
+        float texel[3];
+        orc_surface_from_hit(o, d, t, ng, mat, orc_hit_texel(sc, prim, geom, mat, hu, hv, texel), &s, pos, &err);
         float v[3] = {-d[0], -d[1], -d[2]};
         float nrm[3] = {s.n.x, s.n.y, s.n.z};
         float ls[8];

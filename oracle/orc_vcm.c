@@ -308,6 +308,7 @@ static int direct_light(orc_uniforms* u, const orc_light* lights, uint32_t n_lig
 typedef struct {
     ov3 pos, n, wfix; /* wfix: direction toward the previous vertex of the subpath */
     ov3 throughput;
+    ov3 base;         /* base colour after CalculateSurfaceParams (texel or material, ^2.2) */
     float dvcm, dvc, dvm;
     uint32_t path_length;
     uint32_t geom;
@@ -335,7 +336,8 @@ static int hit_surface(const vcm_ctx* c, const float* o, const float* d, float t
     ov3 ngv = cross(ld3(e1), ld3(e2));
     float ng[3] = {ngv.x, ngv.y, ngv.z};
     const float* mat = c->sc->materials + 22 * (size_t)c->sc->geom_material[geom];
-    orc_surface_from_hit(o, d, t, ng, mat, NULL, s, pos, err);
+    float texel[3];
+    orc_surface_from_hit(o, d, t, ng, mat, orc_hit_texel(c->sc, prim, geom, mat, hu, hv, texel), s, pos, err);
     *geom_out = geom;
     return 1;
 }
@@ -346,9 +348,9 @@ static void surface_of_vertex(const vcm_ctx* c, const vcm_vertex* v, orc_surface
     const float* mat = c->sc->materials + 22 * (size_t)c->sc->geom_material[v->geom];
     float block[22];
     memcpy(block, mat, sizeof(block));
-    block[0] = pow_f(mat[0], 2.2f);
-    block[1] = pow_f(mat[1], 2.2f);
-    block[2] = pow_f(mat[2], 2.2f);
+    block[0] = v->base.x;
+    block[1] = v->base.y;
+    block[2] = v->base.z;
     block[10] = saturate(mat[10]);
     block[11] = saturate(mat[11]);
     block[12] = mat[12] * 0.5f;
@@ -472,6 +474,7 @@ static void light_paths(void* ctx, uint32_t b, uint32_t e)
                 vx->n = s.n;
                 vx->wfix = view;
                 vx->throughput = thr;
+                vx->base = s.base_color;
                 vx->dvcm = dvcm;
                 vx->dvc = dvc;
                 vx->dvm = dvm;

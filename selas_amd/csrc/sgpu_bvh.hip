@@ -56,7 +56,9 @@ __global__ __launch_bounds__(kBlock) void k_prep(const float* __restrict pos, ui
         f3 e1 = v1 - v0;
         f3 e2 = v2 - v0;
         uint32_t g = tri_geom[i];
-        uint32_t prim = i - geom_first[g];
+        // quad geometries (bit 31 of their first index) keep prim = 2 * quad + half, flagged in bit 31
+        const uint32_t fg = geom_first[g];
+        uint32_t prim = (i - (fg & 0x7FFFFFFFu)) | (fg & 0x80000000u);
         TriRecord r;
         r.a = make_float4(v0.x, v0.y, v0.z, __uint_as_float(i));
         r.b = make_float4(e1.x, e1.y, e1.z, __uint_as_float(g));
@@ -355,8 +357,12 @@ int sgpu_bvh_build(sgpu_scene scene, hipStream_t stream)
                                   hipMemcpyHostToDevice, stream));
     SGPU_BUILD_TRY(hipMemcpyAsync(d_geom.p, scene->tri_geom.data(), (size_t)n * sizeof(uint32_t),
                                   hipMemcpyHostToDevice, stream));
-    SGPU_BUILD_TRY(hipMemcpyAsync(d_first.p, scene->geom_first.data(),
-                                  scene->geom_first.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    // device copy of the geometries' first indices with bit 31 set for quad geometries
+    std::vector<uint32_t> first_flagged(scene->geom_first.size());
+    for (size_t g = 0; g < first_flagged.size(); ++g)
+        first_flagged[g] = scene->geom_first[g] | (scene->geom_quad[g] ? 0x80000000u : 0u);
+    SGPU_BUILD_TRY(hipMemcpy(d_first.p, first_flagged.data(), first_flagged.size() * sizeof(uint32_t),
+                             hipMemcpyHostToDevice));
     SGPU_BUILD_TRY(hipMemsetAsync(d_bounds.p, 0xFF, 12 * sizeof(uint32_t), stream));
     SGPU_BUILD_TRY(hipMemsetAsync(d_flags.p, 0, (size_t)n_int * sizeof(uint32_t), stream));
     SGPU_BUILD_TRY(hipMemsetAsync(d_pint.p, 0xFF, (size_t)n_int * sizeof(int), stream));

@@ -216,6 +216,7 @@ int sgpu_scene_add_triangles(sgpu_scene scene, const float* positions, uint32_t 
             return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_add_triangles: index out of range");
     const uint32_t geom = (uint32_t)scene->geom_first.size();
     scene->geom_first.push_back((uint32_t)scene->tri_geom.size());
+    scene->geom_quad.push_back(0);
     const size_t base = scene->tri_pos.size();
     try {
         scene->tri_pos.resize(base + (size_t)n_tris * 9);
@@ -237,6 +238,39 @@ int sgpu_scene_add_triangles(sgpu_scene scene, const float* positions, uint32_t 
             }
         }
     }
+    if (geom_id_out) *geom_id_out = geom;
+    return SGPU_OK;
+}
+
+int sgpu_scene_add_quads(sgpu_scene scene, const float* positions, uint32_t n_verts, const uint32_t* indices,
+                         uint32_t n_quads, const float* colmajor4x4, uint32_t* geom_id_out)
+{
+    if (!scene) return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_add_quads: NULL scene");
+    if (n_quads > 0 && (!positions || !indices))
+        return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_add_quads: NULL positions or indices");
+    if ((uint64_t)n_quads * 2 > (1ull << 30)) return sgpu_fail(SGPU_E_RANGE, "sgpu_scene_add_quads: too many quads");
+    // Embree's split of quad (v0, v1, v2, v3): triangles (v0, v1, v3) and (v2, v3, v1)
+    std::vector<uint32_t> tri;
+    try {
+        tri.resize((size_t)n_quads * 6);
+    } catch (...) {
+        return sgpu_fail(SGPU_E_NOMEM, "sgpu_scene_add_quads: out of host memory");
+    }
+    for (uint32_t q = 0; q < n_quads; ++q) {
+        const uint32_t* v = indices + (size_t)q * 4;
+        uint32_t* t = tri.data() + (size_t)q * 6;
+        t[0] = v[0];
+        t[1] = v[1];
+        t[2] = v[3];
+        t[3] = v[2];
+        t[4] = v[3];
+        t[5] = v[1];
+    }
+    uint32_t geom = 0;
+    int rc = sgpu_scene_add_triangles(scene, positions, n_verts, tri.data(), n_quads * 2, colmajor4x4, &geom);
+    if (rc != SGPU_OK) return rc;
+    std::lock_guard<std::mutex> lock(scene->mu);
+    scene->geom_quad[geom] = 1;
     if (geom_id_out) *geom_id_out = geom;
     return SGPU_OK;
 }
@@ -265,6 +299,8 @@ int sgpu_scene_release(sgpu_scene scene)
     if (scene->d_materials) (void)hipFree(scene->d_materials);
     if (scene->d_geom_material) (void)hipFree(scene->d_geom_material);
     if (scene->d_lights) (void)hipFree(scene->d_lights);
+    for (void* a : scene->tex_allocs) (void)hipFree(a);
+    if (scene->d_geom_tex) (void)hipFree(scene->d_geom_tex);
     delete scene;
     return SGPU_OK;
 }
@@ -330,6 +366,88 @@ int sgpu_scene_set_lights(sgpu_scene scene, const sgpu_light* lights, uint32_t n
     scene->d_lights = dl;
     scene->n_lights = n_lights;
     for (int k = 0; k < 3; ++k) scene->background[k] = background_rgb ? background_rgb[k] : 0.0f;
+    return SGPU_OK;
+}
+
+int sgpu_scene_set_ptex(sgpu_scene scene, uint32_t geom_id, uint32_t res_log2, uint32_t n_faces,
+                        const uint8_t* texels, const int32_t* adjfaces, const int32_t* adjedges)
+{
+    if (!scene || !texels || !adjfaces || !adjedges || n_faces == 0)
+        return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_ptex: NULL or empty argument");
+    std::lock_guard<std::mutex> lock(scene->mu);
+    const size_t n_geoms = scene->geom_first.size();
+    if (geom_id >= n_geoms) return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_ptex: no such geometry");
+    if (!scene->geom_quad[geom_id]) return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_ptex: not a quad geometry");
+    const uint32_t first = scene->geom_first[geom_id];
+    const uint32_t end = geom_id + 1 < n_geoms ? scene->geom_first[geom_id + 1] : (uint32_t)scene->tri_geom.size();
+    if ((end - first) / 2 != n_faces)
+        return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_ptex: face count differs from the geometry's quad count");
+    if (res_log2 > 8) return sgpu_fail(SGPU_E_RANGE, "sgpu_scene_set_ptex: res_log2 above 8");
+    for (size_t i = 0; i < (size_t)n_faces * 4; ++i) {
+        if (adjfaces[i] < -1 || adjfaces[i] >= (int32_t)n_faces)
+            return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_ptex: adjacent face out of range");
+        if (adjedges[i] < 0 || adjedges[i] > 3)
+            return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_ptex: adjacent edge out of range");
+    }
+    const size_t res = (size_t)1 << res_log2;
+    const size_t texel_bytes = (size_t)n_faces * res * res * 3;
+    std::vector<int32_t> adj((size_t)n_faces * 8);
+    for (size_t f = 0; f < n_faces; ++f)
+        for (int e = 0; e < 4; ++e) {
+            adj[8 * f + e] = adjfaces[4 * f + e];
+            adj[8 * f + 4 + e] = adjedges[4 * f + e];
+        }
+    SGPU_HIP_TRY(hipSetDevice(scene->ctx->device));
+    uint8_t* dt = nullptr;
+    int32_t* da = nullptr;
+    SGPU_HIP_TRY(hipMalloc((void**)&dt, texel_bytes));
+    scene->tex_allocs.push_back(dt);
+    SGPU_HIP_TRY(hipMalloc((void**)&da, adj.size() * sizeof(int32_t)));
+    scene->tex_allocs.push_back(da);
+    SGPU_HIP_TRY(hipMemcpy(dt, texels, texel_bytes, hipMemcpyHostToDevice));
+    SGPU_HIP_TRY(hipMemcpy(da, adj.data(), adj.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    scene->tex_host.resize(n_geoms, sgpu::PtexTex{nullptr, nullptr, 0, 0});
+    scene->tex_host[geom_id] = sgpu::PtexTex{dt, da, n_faces, (int32_t)res_log2};
+    // the per-geometry table the shading kernels read
+    if (scene->d_geom_tex) (void)hipFree(scene->d_geom_tex);
+    scene->d_geom_tex = nullptr;
+    scene->n_geom_tex = 0;
+    SGPU_HIP_TRY(hipMalloc((void**)&scene->d_geom_tex, n_geoms * sizeof(sgpu::PtexTex)));
+    SGPU_HIP_TRY(hipMemcpy(scene->d_geom_tex, scene->tex_host.data(), n_geoms * sizeof(sgpu::PtexTex),
+                           hipMemcpyHostToDevice));
+    scene->n_geom_tex = (uint32_t)n_geoms;
+    return SGPU_OK;
+}
+
+namespace {
+__global__ void k_ptex_eval(sgpu::PtexTex tx, const int32_t* __restrict faces, const float* __restrict u,
+                            const float* __restrict v, uint32_t n, float* __restrict out)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    f3 c = sgpu::ptex_eval(tx, faces[i], u[i], v[i]);
+    out[3 * i] = c.x;
+    out[3 * i + 1] = c.y;
+    out[3 * i + 2] = c.z;
+}
+}  // namespace
+
+int sgpu_ptex_eval(sgpu_scene scene, uint32_t geom_id, const int32_t* faces, const float* u, const float* v,
+                   uint32_t n, float* out, void* stream)
+{
+    if (!scene) return sgpu_fail(SGPU_E_INVALID, "sgpu_ptex_eval: NULL scene");
+    if (n > 0 && (!faces || !u || !v || !out)) return sgpu_fail(SGPU_E_INVALID, "sgpu_ptex_eval: NULL array");
+    sgpu::PtexTex tx;
+    {
+        std::lock_guard<std::mutex> lock(scene->mu);
+        if (geom_id >= scene->tex_host.size() || !scene->tex_host[geom_id].texels)
+            return sgpu_fail(SGPU_E_STATE, "sgpu_ptex_eval: geometry has no texture");
+        tx = scene->tex_host[geom_id];
+    }
+    if (n == 0) return SGPU_OK;
+    SGPU_HIP_TRY(hipSetDevice(scene->ctx->device));
+    hipLaunchKernelGGL(k_ptex_eval, dim3((n + 255) / 256), dim3(256), 0, as_stream(stream), tx, faces, u, v, n, out);
+    SGPU_HIP_TRY(hipGetLastError());
     return SGPU_OK;
 }
 

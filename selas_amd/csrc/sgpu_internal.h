@@ -3,6 +3,7 @@
 
 #include "../../include/selas_gpu.h"
 #include "sgpu_math.h"
+#include "sgpu_ptex.h"
 
 #include <hip/hip_runtime.h>
 #include <mutex>
@@ -52,6 +53,7 @@ struct sgpu_scene_s {
     std::vector<float> tri_pos;
     std::vector<uint32_t> tri_geom;   // geomID per triangle
     std::vector<uint32_t> geom_first; // first global index of each geometry
+    std::vector<uint8_t> geom_quad;   // 1: quad geometry (triangles 2q and 2q+1 form quad q)
     bool committed = false;
     uint32_t n_tris = 0;
     uint32_t n_nodes = 0;
@@ -68,6 +70,11 @@ struct sgpu_scene_s {
     void* d_lights = nullptr;          // sgpu_light records
     uint32_t n_lights = 0;
     float background[3] = {0.0f, 0.0f, 0.0f};
+    // per-face textures (sgpu_scene_set_ptex): one record per geometry, texels == nullptr for none
+    std::vector<sgpu::PtexTex> tex_host;
+    std::vector<void*> tex_allocs;
+    sgpu::PtexTex* d_geom_tex = nullptr;
+    uint32_t n_geom_tex = 0;
 };
 
 // ---- error plumbing ------------------------------------------------------------------------------

@@ -15,6 +15,7 @@
 #include "sgpu_internal.h"
 #include "sgpu_light.h"
 #include "sgpu_surface.h"
+#include "sgpu_ptex.h"
 
 #include <algorithm>
 
@@ -126,6 +127,8 @@ struct ShadeParams {
     const float* materials;
     const uint32_t* geom_material;
     uint32_t n_geoms;
+    const PtexTex* geom_tex;  // per-face texture per geometry (texels == nullptr: none)
+    uint32_t n_geom_tex;
     const Light* lights;
     uint32_t n_lights;
     f3 background;
@@ -159,7 +162,15 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(ShadeParams sp, uint32_t co
             PcgUniforms u{load_rng(ps, sp.path_base + p, p)};
             const float* mat = sp.materials + 22 * (size_t)sp.geom_material[geom];
             Surface s;
-            surface_from_hit(o, d, hits.t[j], mk3(hits.ngx[j], hits.ngy[j], hits.ngz[j]), mat, nullptr, s);
+            // CalculateSurfaceParams' eUsesPtex branch (SurfaceParameters.cpp:196-203)
+            f3 texel;
+            bool tex = false;
+            if (((uint32_t)mat[21] & kMatUsesPtex) && geom < sp.n_geom_tex && sp.geom_tex[geom].texels) {
+                texel = ptex_eval(sp.geom_tex[geom], (int)hits.prim[j], hits.u[j], hits.v[j]);
+                tex = true;
+            }
+            surface_from_hit(o, d, hits.t[j], mk3(hits.ngx[j], hits.ngy[j], hits.ngz[j]), mat, tex ? &texel : nullptr,
+                             s);
             f3 v = -d;
             LightSample ls;
             next_event_estimation(u, sp.lights, sp.n_lights, s.position, s.n, (sp.flags & SGPU_PT_NEE_COSINE_ONCE) != 0,
@@ -336,6 +347,8 @@ int sgpu_pt_render_impl(sgpu_scene scene, const sgpu_pt_job* job, float* fb, hip
     sp.lights = (const Light*)scene->d_lights;
     sp.n_lights = scene->n_lights;
     sp.background = mk3(scene->background[0], scene->background[1], scene->background[2]);
+    sp.geom_tex = scene->d_geom_tex;
+    sp.n_geom_tex = scene->n_geom_tex;
     sp.max_len = job->max_path_length;
     sp.flags = job->flags;
 

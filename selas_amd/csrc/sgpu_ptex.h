@@ -1,0 +1,502 @@
+// Per-face texture lookup on the device: Ptex 2.x's separable cubic B-spline filter with a zero
+// footprint, as CalculateSurfaceParams calls it for eUsesPtex materials
+// (SurfaceParameters.cpp:196-203: PtexFilter{f_bspline}.eval(out, 0, 3, primId, u, v, 0, 0, 0, 0)).
+// Restated from Middleware/ptex/Include/PtexSeparableFilter.cpp:45-330 (eval, splitAndApply,
+// applyAcrossEdge, applyToCorner, applyToCornerFace, apply), PtexFilters.cpp:116-282
+// (PtexWidth4Filter::buildKernelAxis, PtexBicubicFilter with sharpness 0) and
+// PtexSeparableKernel.h:48-480 / .cpp:40-75, for the textures the renderer uses: quad meshes,
+// 8-bit RGB, one resolution for every face (no subfaces), clamp border mode, no edge filter mode.
+// The operations and their order are the reference's, so results agree with its compiled code bit
+// for bit. A kernel that stays inside its face (most lookups) takes a direct 4 x 4 path that
+// performs the same arithmetic as the general one.
+#pragma once
+
+#include "sgpu_math.h"
+
+#include <cstring>
+
+namespace sgpu {
+
+// One texture: texels (nfaces x res x res x 3, row = v texel) and per face 8 words: the adjacent
+// face across edges bottom, right, top, left (-1 = border), then the neighbour's edge id for each.
+struct PtexTex {
+    const uint8_t* texels;
+    const int32_t* adj;
+    uint32_t nfaces;
+    int32_t res_log2;
+};
+
+namespace ptex_detail {
+
+constexpr int kMax = 10;
+
+struct Kernel {
+    int res_log2;
+    int u, v, uw, vw;
+    float ku[kMax], kv[kMax];
+    int rot;
+};
+
+struct Filter {
+    PtexTex tx;
+    float result[3];
+    float weight;
+};
+
+SGPU_HD float accumulate(const float* p, int n)
+{
+    float r = 0;
+    for (int i = 0; i < n; ++i) r += p[i];
+    return r;
+}
+SGPU_HD float kweight(const Kernel& k) { return accumulate(k.ku, k.uw) * accumulate(k.kv, k.vw); }
+SGPU_HD int kres(const Kernel& k) { return 1 << k.res_log2; }
+SGPU_HD void drop_front(float* a, int n, int w)
+{
+    for (int i = 0; i + w < n; ++i) a[i] = a[i + w];
+}
+
+SGPU_HD void merge_l(Kernel& k)
+{
+    int w = -k.u;
+    k.ku[w] += accumulate(k.ku, w);
+    drop_front(k.ku, k.uw, w);
+    k.uw -= w;
+    k.u = 0;
+}
+SGPU_HD void merge_r(Kernel& k)
+{
+    int w = k.uw + k.u - kres(k);
+    float* kp = k.ku + k.uw - w;
+    kp[-1] += accumulate(kp, w);
+    k.uw -= w;
+}
+SGPU_HD void merge_b(Kernel& k)
+{
+    int w = -k.v;
+    k.kv[w] += accumulate(k.kv, w);
+    drop_front(k.kv, k.vw, w);
+    k.vw -= w;
+    k.v = 0;
+}
+SGPU_HD void merge_t(Kernel& k)
+{
+    int w = k.vw + k.v - kres(k);
+    float* kp = k.kv + k.vw - w;
+    kp[-1] += accumulate(kp, w);
+    k.vw -= w;
+}
+
+SGPU_HD void kset(Kernel& d, int res_log2, int u, int v, int uw, int vw, const float* ku, const float* kv)
+{
+    float tu[kMax], tv[kMax];
+    for (int i = 0; i < uw; ++i) tu[i] = ku[i];
+    for (int i = 0; i < vw; ++i) tv[i] = kv[i];
+    d.res_log2 = res_log2;
+    d.u = u;
+    d.v = v;
+    d.uw = uw;
+    d.vw = vw;
+    for (int i = 0; i < uw; ++i) d.ku[i] = tu[i];
+    for (int i = 0; i < vw; ++i) d.kv[i] = tv[i];
+    d.rot = 0;
+}
+
+SGPU_HD void split_l(Kernel& k, Kernel& out)
+{
+    int w = -k.u;
+    if (w < k.uw) {
+        kset(out, k.res_log2, kres(k) - w, k.v, w, k.vw, k.ku, k.kv);
+        k.u = 0;
+        drop_front(k.ku, k.uw, w);
+        k.uw -= w;
+    } else {
+        out = k;
+        out.u += kres(k);
+        k.u = 0;
+        k.uw = 0;
+    }
+}
+SGPU_HD void split_r(Kernel& k, Kernel& out)
+{
+    int w = k.u + k.uw - kres(k);
+    if (w < k.uw) {
+        kset(out, k.res_log2, 0, k.v, w, k.vw, k.ku + k.uw - w, k.kv);
+        k.uw -= w;
+    } else {
+        out = k;
+        out.u -= kres(k);
+        k.u = 0;
+        k.uw = 0;
+    }
+}
+SGPU_HD void split_b(Kernel& k, Kernel& out)
+{
+    int w = -k.v;
+    if (w < k.vw) {
+        kset(out, k.res_log2, k.u, kres(k) - w, k.uw, w, k.ku, k.kv);
+        k.v = 0;
+        drop_front(k.kv, k.vw, w);
+        k.vw -= w;
+    } else {
+        out = k;
+        out.v += kres(k);
+        k.v = 0;
+        k.vw = 0;
+    }
+}
+SGPU_HD void split_t(Kernel& k, Kernel& out)
+{
+    int w = k.v + k.vw - kres(k);
+    if (w < k.vw) {
+        kset(out, k.res_log2, k.u, 0, k.uw, w, k.ku, k.kv + k.vw - w);
+        k.vw -= w;
+    } else {
+        out = k;
+        out.v -= kres(k);
+        k.v = 0;
+        k.vw = 0;
+    }
+}
+
+SGPU_HD void reverse(float* a, int n)
+{
+    for (int i = 0, j = n - 1; i < j; ++i, --j) {
+        float t = a[i];
+        a[i] = a[j];
+        a[j] = t;
+    }
+}
+SGPU_HD void flip_u(Kernel& k)
+{
+    k.u = kres(k) - k.u - k.uw;
+    reverse(k.ku, k.uw);
+}
+SGPU_HD void flip_v(Kernel& k)
+{
+    k.v = kres(k) - k.v - k.vw;
+    reverse(k.kv, k.vw);
+}
+SGPU_HD void swap_uv(Kernel& k)
+{
+    int t = k.u;
+    k.u = k.v;
+    k.v = t;
+    t = k.uw;
+    k.uw = k.vw;
+    k.vw = t;
+    for (int i = 0; i < kMax; ++i) {
+        float f = k.ku[i];
+        k.ku[i] = k.kv[i];
+        k.kv[i] = f;
+    }
+}
+// rotate `r` steps counter-clockwise
+SGPU_HD void krotate(Kernel& k, int r)
+{
+    switch (r & 3) {
+    default: return;
+    case 1:
+        flip_u(k);
+        swap_uv(k);
+        break;
+    case 2:
+        flip_u(k);
+        flip_v(k);
+        break;
+    case 3:
+        flip_v(k);
+        swap_uv(k);
+        break;
+    }
+    k.rot = (k.rot + r) & 3;
+}
+
+SGPU_HD float make_symmetric(Kernel& k, float initial_weight)
+{
+    int n = k.uw < k.vw ? k.uw : k.vw;
+    k.uw = k.vw = n;
+    float new_weight = 0;
+    for (int i = 0; i < n; ++i) {
+        float sum = k.ku[i] + k.kv[i];
+        k.ku[i] = k.kv[i] = sum;
+        new_weight += sum;
+    }
+    new_weight *= new_weight;
+    float scale = new_weight == 0 ? 1.f : initial_weight / new_weight;
+    if (!(scale >= 1)) {
+        if (scale < -1) {
+            for (int i = 0; i < n; ++i) k.ku[i] *= -1;
+            new_weight = -new_weight;
+        } else {
+            for (int i = 0; i < n; ++i) k.ku[i] *= scale;
+            new_weight = initial_weight;
+        }
+    }
+    return new_weight;
+}
+
+SGPU_HD int adjface(const PtexTex& tx, int f, int e) { return tx.adj[8 * f + e]; }
+SGPU_HD int adjedge(const PtexTex& tx, int f, int e) { return tx.adj[8 * f + 4 + e]; }
+
+// Apply<uint8_t, 3>: row sums over u first, then weighted by v
+SGPU_HD void kapply(Filter& F, const Kernel& k, int face)
+{
+    if (k.uw <= 0 || k.vw <= 0) return;
+    const int res = kres(k);
+    const uint8_t* data = F.tx.texels + (size_t)face * res * res * 3;
+    for (int j = 0; j < k.vw; ++j) {
+        const uint8_t* p = data + ((size_t)(k.v + j) * res + k.u) * 3;
+        float r0 = (float)p[0] * k.ku[0], r1 = (float)p[1] * k.ku[0], r2 = (float)p[2] * k.ku[0];
+        for (int i = 1; i < k.uw; ++i) {
+            r0 += (float)p[3 * i] * k.ku[i];
+            r1 += (float)p[3 * i + 1] * k.ku[i];
+            r2 += (float)p[3 * i + 2] * k.ku[i];
+        }
+        F.result[0] += r0 * k.kv[j];
+        F.result[1] += r1 * k.kv[j];
+        F.result[2] += r2 * k.kv[j];
+    }
+}
+
+SGPU_HD void apply_across_edge(Filter& F, Kernel& k, int face, int eid)
+{
+    int af = adjface(F.tx, face, eid), ae = adjedge(F.tx, face, eid);
+    krotate(k, eid - ae + 2);
+    kapply(F, k, af);
+}
+
+SGPU_HD void apply_to_corner(Filter& F, Kernel& k, int face, int eid)
+{
+    constexpr int kMaxValence = 10;
+    int cface[kMaxValence], cedge[kMaxValence];
+    int af = face, ae = eid, ncorners = 0;
+    for (int i = 0; i < kMaxValence; ++i) {
+        int next = adjface(F.tx, af, ae);
+        int nexte = (adjedge(F.tx, af, ae) + 1) % 4;
+        af = next;
+        ae = nexte;
+        if (af < 0 || (af == face && ae == eid)) {
+            ncorners = i - 2;
+            break;
+        }
+        cface[i] = af;
+        cedge[i] = ae;
+    }
+    if (ncorners == 1) {
+        krotate(k, eid - cedge[1] + 2);
+        kapply(F, k, cface[1]);
+    } else if (ncorners > 1) {
+        krotate(k, eid + 2);
+        float initial = kweight(k);
+        float neww = make_symmetric(k, initial);
+        for (int i = 1; i <= ncorners; ++i) {
+            Kernel kc = k;
+            krotate(kc, 2 - cedge[i] + 2);
+            kapply(F, kc, cface[i]);
+        }
+        F.weight += neww * (float)ncorners - initial;
+    } else {
+        F.weight -= kweight(k);
+    }
+}
+
+SGPU_HD void split_and_apply(Filter& F, Kernel& k, int face)
+{
+    const PtexTex& tx = F.tx;
+    int res = kres(k);
+    bool sr = k.u + k.uw > res, sl = k.u < 0, st = k.v + k.vw > res, sb = k.v < 0;
+    if (sr || sl || st || sb) {
+        Kernel ka, kc;
+        if (sr) {
+            if (adjface(tx, face, 1) >= 0) {
+                split_r(k, ka);
+                if (st) {
+                    if (adjface(tx, face, 2) >= 0) {
+                        split_t(ka, kc);
+                        apply_to_corner(F, kc, face, 2);
+                    } else
+                        merge_t(ka);
+                }
+                if (sb) {
+                    if (adjface(tx, face, 0) >= 0) {
+                        split_b(ka, kc);
+                        apply_to_corner(F, kc, face, 1);
+                    } else
+                        merge_b(ka);
+                }
+                apply_across_edge(F, ka, face, 1);
+            } else
+                merge_r(k);
+        }
+        if (sl) {
+            if (adjface(tx, face, 3) >= 0) {
+                split_l(k, ka);
+                if (st) {
+                    if (adjface(tx, face, 2) >= 0) {
+                        split_t(ka, kc);
+                        apply_to_corner(F, kc, face, 3);
+                    } else
+                        merge_t(ka);
+                }
+                if (sb) {
+                    if (adjface(tx, face, 0) >= 0) {
+                        split_b(ka, kc);
+                        apply_to_corner(F, kc, face, 0);
+                    } else
+                        merge_b(ka);
+                }
+                apply_across_edge(F, ka, face, 3);
+            } else
+                merge_l(k);
+        }
+        if (st) {
+            if (adjface(tx, face, 2) >= 0) {
+                split_t(k, ka);
+                apply_across_edge(F, ka, face, 2);
+            } else
+                merge_t(k);
+        }
+        if (sb) {
+            if (adjface(tx, face, 0) >= 0) {
+                split_b(k, ka);
+                apply_across_edge(F, ka, face, 0);
+            } else
+                merge_b(k);
+        }
+    }
+    kapply(F, k, face);
+}
+
+// PtexBicubicFilter::kernelFn with sharpness 0 (B = 1, C = 0: the cubic B-spline)
+SGPU_HD float bspline(float x)
+{
+    const float B = 1.0f;
+    const float c0 = 1.5f - B, c1 = 1.5f * B - 2.5f, c2 = 1.0f - (float)(1.0 / 3.0) * B;
+    const float c3 = (float)(1.0 / 3.0) * B - 0.5f, c4 = 2.5f - 1.5f * B, c5 = 2.0f * B - 4.0f;
+    const float c6 = 2.0f - (float)(2.0 / 3.0) * B;
+    x = fabsf(x);
+    if (x < 1.0f) return (c0 * x + c1) * x * x + c2;
+    if (x < 2.0f) return ((c3 * x + c4) * x + c5) * x + c6;
+    return 0.0f;
+}
+
+// 2-unit cubic Hermite used for wide footprints
+SGPU_HD float blur(float x)
+{
+    x = fabsf(x);
+    return x < 1.0f ? (2.0f * x - 3.0f) * x * x + 1.0f : 0.0f;
+}
+
+SGPU_HD float recip_pow2(int power)
+{
+    uint32_t b = (uint32_t)(127 - power) << 23;
+    float f;
+    memcpy(&f, &b, 4);
+    return f;
+}
+SGPU_HD int res_from_width(float w)
+{
+    int32_t b;
+    memcpy(&b, &w, 4);
+    return 127 - ((b >> 23) & 0xff);
+}
+
+// PtexWidth4Filter::buildKernelAxis with lerp off, for a zero footprint
+SGPU_HD void build_axis(int& k_reslog2, int& k_u, int& k_uw, float* ku, float u, float uw, int f_reslog2)
+{
+    uw = uw > recip_pow2(f_reslog2) ? uw : recip_pow2(f_reslog2);
+    k_reslog2 = res_from_width(uw);
+    const int resu = 1 << k_reslog2;
+    const float lerp2 = 0, lerp1 = 1.0f - lerp2;
+    if (uw >= .25f) {
+        if (uw < .5f) {
+            k_reslog2 = 2;
+            float upix = u * 4.0f - 0.5f;
+            int u1 = (int)ceilf(upix - 2), u2 = (int)ceilf(upix + 2);
+            u1 = u1 & ~1;
+            u2 = (u2 + 1) & ~1;
+            k_u = u1;
+            k_uw = u2 - u1;
+            float x1 = (float)u1 - upix;
+            for (int i = 0; i < k_uw; i += 2) {
+                float xa = x1 + (float)i, xb = xa + 1.0f, xc = (xa + xb) * 0.25f;
+                float s = 1.0f / (uw + .75f);
+                float ka = bspline(xa), kb = bspline(xb), kc = blur(xc * s);
+                ku[i] = ka * lerp1 + kc * lerp2;
+                ku[i + 1] = kb * lerp1 + kc * lerp2;
+            }
+            return;
+        } else if (uw < 1) {
+            k_reslog2 = 1;
+            float upix = u * 2.0f - 0.5f;
+            k_u = (int)floorf(u - .5f) * 2;
+            k_uw = 4;
+            float x1 = (float)k_u - upix;
+            for (int i = 0; i < k_uw; i += 2) {
+                float xa = x1 + (float)i, xb = xa + 1.0f, xc = (xa + xb) * 0.5f;
+                float s = 1.0f / (uw * 1.5f + .5f);
+                float ka = blur(xa * s), kb = blur(xb * s), kc = blur(xc * s);
+                ku[i] = ka * lerp1 + kc * lerp2;
+                ku[i + 1] = kb * lerp1 + kc * lerp2;
+            }
+            return;
+        } else {
+            k_reslog2 = 0;
+            float upix = u - .5f;
+            k_uw = 2;
+            float ui = floorf(upix);
+            k_u = (int)ui;
+            ku[0] = blur(upix - ui);
+            ku[1] = 1 - ku[0];
+            return;
+        }
+    }
+    float upix = u * (float)resu - 0.5f;
+    float uwpix = uw * (float)resu;
+    float dupix = 2.0f * uwpix;
+    int u1 = (int)ceilf(upix - dupix), u2 = (int)ceilf(upix + dupix);
+    k_u = u1;
+    k_uw = u2 - u1;
+    float x1 = ((float)u1 - upix) / uwpix, step = 1.0f / uwpix;
+    for (int i = 0; i < k_uw; ++i) ku[i] = bspline(x1 + (float)i * step);
+}
+
+}  // namespace ptex_detail
+
+// The filtered texel (channels 0..2 scaled to [0, 1]) of face `face` at (u, v).
+SGPU_HD f3 ptex_eval(const PtexTex& tx, int face, float u, float v)
+{
+    using namespace ptex_detail;
+    if (face < 0 || (uint32_t)face >= tx.nfaces) return mk3(0, 0, 0);
+    u = u < 0.0f ? 0.0f : (u > 1.0f ? 1.0f : u);
+    v = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+    Kernel k;
+    int ru, rv;
+    build_axis(ru, k.u, k.uw, k.ku, u, 0.0f, tx.res_log2);
+    build_axis(rv, k.v, k.vw, k.kv, v, 0.0f, tx.res_log2);
+    k.res_log2 = ru;  // square faces: both axes choose the same resolution
+    k.rot = 0;
+    // stripZeros
+    while (k.ku[0] == 0) {
+        drop_front(k.ku, k.uw, 1);
+        k.u++;
+        k.uw--;
+    }
+    while (k.ku[k.uw - 1] == 0) k.uw--;
+    while (k.kv[0] == 0) {
+        drop_front(k.kv, k.vw, 1);
+        k.v++;
+        k.vw--;
+    }
+    while (k.kv[k.vw - 1] == 0) k.vw--;
+    Filter F;
+    F.tx = tx;
+    F.result[0] = F.result[1] = F.result[2] = 0.0f;
+    F.weight = kweight(k);
+    split_and_apply(F, k, face);
+    float scale = 1.0f / (F.weight * 255.0f);
+    return mk3(F.result[0] * scale, F.result[1] * scale, F.result[2] * scale);
+}
+
+}  // namespace sgpu

@@ -168,6 +168,14 @@ __global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace(const BvhNode* __res
                 geom = __float_as_uint(r.b.w);
                 prim = __float_as_uint(r.c.w);
                 if (hits.ng_x) ng = cross3(mk3(r.b.x, r.b.y, r.b.z), mk3(r.c.x, r.c.y, r.c.z));
+                if (prim & 0x80000000u) {
+                    // triangle 2q + h of quad q: report the quad and its own (u, v)
+                    if (prim & 1u) {
+                        best_u = 1.0f - best_u;
+                        best_v = 1.0f - best_v;
+                    }
+                    prim = (prim & 0x7FFFFFFFu) >> 1;
+                }
             }
             if (hits.t) hits.t[i] = best_t;
             if (hits.u) hits.u[i] = found ? best_u : 0.0f;

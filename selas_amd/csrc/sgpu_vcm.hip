@@ -34,6 +34,7 @@
 #include "sgpu_camera.h"
 #include "sgpu_internal.h"
 #include "sgpu_light.h"
+#include "sgpu_ptex.h"
 #include "sgpu_surface.h"
 
 #include <cstring>  // rocprim/iterator/texture_cache_iterator.hpp uses memset without including it
@@ -86,8 +87,9 @@ __device__ __forceinline__ void put_ray(const RayBuf& r, uint32_t i, f3 o, f3 d,
     r.tfar[i] = tfar;
 }
 
-// Light vertex, 64 B (VCMVertex, VCMCommon.h:36-46, with the surface data a connection needs in
-// place of the hit record: position, normal, incoming direction, material).
+// Light vertex, 80 B (VCMVertex, VCMCommon.h:36-46, with the surface data a connection needs in
+// place of the hit record: position, normal, incoming direction, material, and the base colour
+// CalculateSurfaceParams derived -- the filtered texel for textured materials).
 struct LightVertex {
     float pos[3];
     float thr[3];
@@ -95,10 +97,12 @@ struct LightVertex {
     float wfix[3];  // direction toward the previous vertex of the light subpath
     float dvcm, dvc, dvm;
     uint32_t geom_plen;  // geometry << 5 | path length (path length < 32, as in PathState)
+    float base[3];       // base colour (^2.2 applied)
+    float cp;            // Russian-roulette continuation probability of the surface
 };
-static_assert(sizeof(LightVertex) == 64, "LightVertex must be 64 bytes");
+static_assert(sizeof(LightVertex) == 80, "LightVertex must be 80 bytes");
 
-// Camera vertex handed from k_cam_shade to k_cam_merge (64 B); geom_plen = kNoQuery: no query.
+// Camera vertex handed from k_cam_shade to k_cam_merge (80 B); geom_plen = kNoQuery: no query.
 struct CamVertex {
     float pos[3];
     float n[3];
@@ -107,8 +111,10 @@ struct CamVertex {
     float dvcm, dvm;
     uint32_t geom_plen;  // geometry << 5 | path length
     uint32_t path;
+    float base[3];       // base colour (^2.2 applied)
+    float pad;
 };
-static_assert(sizeof(CamVertex) == 64, "CamVertex must be 64 bytes");
+static_assert(sizeof(CamVertex) == 80, "CamVertex must be 80 bytes");
 
 // Subpath state (PathState, VCMCommon.h:48-60), SoA over the paths of a wavefront.
 struct PathState {
@@ -140,6 +146,8 @@ struct Slice {
     const Light* lights;
     uint32_t n_lights;
     f3 background;
+    const PtexTex* geom_tex;  // per-face texture per geometry (texels == nullptr: none)
+    uint32_t n_geom_tex;
 };
 
 // Wavefront-wide arguments.
@@ -200,6 +208,19 @@ __device__ __forceinline__ void vertex_surface(const Slice& sl, const LightVerte
     s.position = mk3(v.pos[0], v.pos[1], v.pos[2]);
     s.error = 0.0f;
     surface_material(mat, mk3(v.n[0], v.n[1], v.n[2]), mk3(v.wfix[0], v.wfix[1], v.wfix[2]), nullptr, s);
+    s.base_color = mk3(v.base[0], v.base[1], v.base[2]);
+}
+
+// CalculateSurfaceParams' eUsesPtex branch (SurfaceParameters.cpp:196-203): the filtered texel of
+// the hit's face at its (u, v) when the material asks for it and its geometry has a texture.
+__device__ __forceinline__ bool hit_texel(const Slice& sl, uint32_t geom, const float* mat, uint32_t prim, float u,
+                                          float v, f3& texel)
+{
+    if (!((uint32_t)mat[21] & kMatUsesPtex) || geom >= sl.n_geom_tex) return false;
+    const PtexTex tx = sl.geom_tex[geom];
+    if (!tx.texels) return false;
+    texel = ptex_eval(tx, (int)prim, u, v);
+    return true;
 }
 
 // SampleBsdfScattering (VCM.cpp:412-446) with the Russian-roulette-compensated throughput.
@@ -314,7 +335,10 @@ __global__ __launch_bounds__(kBlock) void k_light_shade(Wave w, uint32_t count, 
             f3 d = mk3(rays.dx[j], rays.dy[j], rays.dz[j]);
             const float* mat = sl.materials + 22 * (size_t)sl.geom_material[geom];
             Surface s;
-            surface_from_hit(o, d, hits.t[j], mk3(hits.ngx[j], hits.ngy[j], hits.ngz[j]), mat, nullptr, s);
+            f3 texel;
+            const bool tex = hit_texel(sl, geom, mat, hits.prim[j], hits.u[j], hits.v[j], texel);
+            surface_from_hit(o, d, hits.t[j], mk3(hits.ngx[j], hits.ngy[j], hits.ngz[j]), mat, tex ? &texel : nullptr,
+                             s);
             f3 x = s.position;
             f3 view = -d;
             f3 pos = g3(ps.pos, p);
@@ -348,6 +372,10 @@ __global__ __launch_bounds__(kBlock) void k_light_shade(Wave w, uint32_t count, 
                 lv.dvc = dvc;
                 lv.dvm = dvm;
                 lv.geom_plen = (geom << 5) | (plen & 31u);
+                lv.base[0] = s.base_color.x;
+                lv.base[1] = s.base_color.y;
+                lv.base[2] = s.base_color.z;
+                lv.cp = cont_prob(s);
                 verts[(size_t)p * w.mlv + k] = lv;
                 vcount[p] = k + 1;
             }
@@ -649,7 +677,10 @@ __global__ __launch_bounds__(kBlock) void k_cam_shade(Wave w, uint32_t count, co
             geom_hit = geom;
             f3 o = mk3(rays.ox[j], rays.oy[j], rays.oz[j]);
             const float* mat = sl.materials + 22 * (size_t)sl.geom_material[geom];
-            surface_from_hit(o, d, hits.t[j], mk3(hits.ngx[j], hits.ngy[j], hits.ngz[j]), mat, nullptr, s);
+            f3 texel;
+            const bool tex = hit_texel(sl, geom, mat, hits.prim[j], hits.u[j], hits.v[j], texel);
+            surface_from_hit(o, d, hits.t[j], mk3(hits.ngx[j], hits.ngy[j], hits.ngz[j]), mat, tex ? &texel : nullptr,
+                             s);
             x = s.position;
             view = -d;
         }
@@ -825,6 +856,10 @@ __global__ __launch_bounds__(kBlock) void k_cam_shade(Wave w, uint32_t count, co
             v.dvm = dvm;
             v.geom_plen = (geom_hit << 5) | (plen & 31u);
             v.path = p;
+            v.base[0] = s.base_color.x;
+            v.base[1] = s.base_color.y;
+            v.base[2] = s.base_color.z;
+            v.pad = 0.0f;
         }
         cam_vm[j] = v;
     }
@@ -901,6 +936,7 @@ __global__ __launch_bounds__(kBlock) void k_cam_merge(Wave w, uint32_t count, co
             s.error = 0.0f;
             surface_material(sl.materials + 22 * (size_t)sl.geom_material[v.geom_plen >> 5],
                              mk3(v.n[0], v.n[1], v.n[2]), view, nullptr, s);
+            s.base_color = mk3(v.base[0], v.base[1], v.base[2]);
             const float ccp = cont_prob(s);
             for (uint32_t q = b; q < e; ++q) {
                 const float4 lp = G.spos[q];
@@ -918,7 +954,7 @@ __global__ __launch_bounds__(kBlock) void k_cam_merge(Wave w, uint32_t count, co
                 if (is_zero3(f)) continue;
                 float cos_l = absf_(dot3(s.n, lw));
                 fwd *= ccp;
-                rev *= lmat[15];  // continuation probability of the light vertex's material
+                rev *= lv.cp;
                 float w_light = lv.dvcm * sl.vc_weight + lv.dvm * fwd;
                 float w_cam = v.dvcm * sl.vc_weight + v.dvm * rev;
                 float mis = 1.0f / (w_light + 1.0f + w_cam);
@@ -1108,6 +1144,8 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
     base.lights = (const Light*)scene->d_lights;
     base.n_lights = scene->n_lights;
     base.background = mk3(scene->background[0], scene->background[1], scene->background[2]);
+    base.geom_tex = scene->d_geom_tex;
+    base.n_geom_tex = scene->n_geom_tex;
 
     Arena A;
     A.stream = stream;

@@ -68,6 +68,9 @@ _SIGNATURES = {
     "sgpu_event_elapsed_ms": [_P, _P, _FP],
     "sgpu_scene_create": [_P, C.POINTER(_P)],
     "sgpu_scene_add_triangles": [_P, _FP, _U32, _U32P, _U32, _FP, _U32P],
+    "sgpu_scene_add_quads": [_P, _FP, _U32, _U32P, _U32, _FP, _U32P],
+    "sgpu_scene_set_ptex": [_P, _U32, _U32, _U32, _P, _P, _P],
+    "sgpu_ptex_eval": [_P, _U32, _P, _P, _P, _U32, _P, _P],
     "sgpu_scene_commit": [_P, _P],
     "sgpu_scene_release": [_P],
     "sgpu_scene_info": [_P, _U32P, _U32P, _U32P],
@@ -261,6 +264,39 @@ class Scene:
                                            C.byref(gid)))
         return gid.value
 
+    def add_quads(self, positions: np.ndarray, quads: np.ndarray, transform_colmajor=None) -> int:
+        pos = np.ascontiguousarray(positions, np.float32).reshape(-1)
+        idx = np.ascontiguousarray(quads, np.uint32).reshape(-1)
+        gid = C.c_uint32()
+        xf = None if transform_colmajor is None else fptr(np.ascontiguousarray(transform_colmajor, np.float32))
+        check(lib.sgpu_scene_add_quads(self.handle, fptr(pos), pos.size // 3, uptr(idx), idx.size // 4, xf,
+                                       C.byref(gid)))
+        return gid.value
+
+    def set_ptex(self, geom: int, res_log2: int, texels: np.ndarray, adjfaces: np.ndarray, adjedges: np.ndarray):
+        t = np.ascontiguousarray(texels, np.uint8)
+        af = np.ascontiguousarray(adjfaces, np.int32)
+        ae = np.ascontiguousarray(adjedges, np.int32)
+        check(lib.sgpu_scene_set_ptex(self.handle, geom, res_log2, af.shape[0], t.ctypes.data, af.ctypes.data,
+                                      ae.ctypes.data))
+
+    def ptex_eval(self, geom: int, faces, u, v, stream=None) -> np.ndarray:
+        """Filtered texels (n x 3) of geometry geom's texture at (face, u, v), through sgpu_ptex_eval."""
+        faces = np.ascontiguousarray(faces, np.int32)
+        u = np.ascontiguousarray(u, np.float32)
+        v = np.ascontiguousarray(v, np.float32)
+        n = faces.shape[0]
+        bufs = [self.ctx.empty(4 * n) for _ in range(3)] + [self.ctx.empty(12 * n)]
+        try:
+            for b, a in zip(bufs, (faces, u, v)):
+                b.upload(a)
+            check(lib.sgpu_ptex_eval(self.handle, geom, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, n, bufs[3].ptr, stream))
+            synchronize()
+            return bufs[3].download(np.float32, 3 * n).reshape(n, 3)
+        finally:
+            for b in bufs:
+                b.free()
+
     def add_soup(self, tri9: np.ndarray) -> int:
         tri9 = np.ascontiguousarray(tri9, np.float32).reshape(-1, 9)
         n = tri9.shape[0]
@@ -337,7 +373,13 @@ def build_render_scene(ctx: Context, sc: dict) -> Scene:
     """A committed device scene with materials and lights from a workloads.*_scene dict."""
     scene = Scene(ctx)
     for g in sc["geoms"]:
-        scene.add_soup(g)
+        if isinstance(g, dict):
+            gid = scene.add_quads(g["positions"], g["quads"])
+            if g.get("ptex") is not None:
+                p = g["ptex"]
+                scene.set_ptex(gid, p["res_log2"], p["texels"], p["adjfaces"], p["adjedges"])
+        else:
+            scene.add_soup(g)
     scene.commit()
     mats = np.ascontiguousarray(sc["materials"], np.float32)
     gm = np.ascontiguousarray(sc["geom_material"], np.uint32)

@@ -197,43 +197,119 @@ def _frame_from_normal(n, twist):
     return M @ Ry
 
 
-def vcm_scene(seed: int = 3, n_meshes: int = 64, nu: int = 128, nv: int = 64):
-    """VCM workload scene: n_meshes displaced-grid meshes of nu x nv quads (2 nu nv triangles each;
-    64 x 16,384 = 1,048,576 triangles) placed by a seeded RNG as panels of an enclosure: each mesh
-    is a unit-by-half grid with value-noise displacement, scaled by 4..8, centred at a random
+def _smooth_interp_matrix(n_samples, lattice, n_units):
+    """(n_samples x lattice+1) weights of a smoothstep-blended linear interpolation of lattice
+    values at sample positions (k + 0.5) / n_samples * lattice, k = 0..n_samples-1."""
+    x = (np.arange(n_samples) + 0.5) / n_samples * lattice
+    ix = np.clip(np.floor(x).astype(int), 0, lattice - 1)
+    t = x - ix
+    s = t * t * (3 - 2 * t)
+    W = np.zeros((n_samples, lattice + 1))
+    W[np.arange(n_samples), ix] = 1 - s
+    W[np.arange(n_samples), ix + 1] += s
+    return W
+
+
+def quad_adjacency(quads):
+    """Ptex adjacency of a quad mesh ((n, 4) vertex indices, edge e from corner e to corner e+1):
+    the face across each edge (-1 at a border) and that face's edge id of the shared edge."""
+    quads = np.asarray(quads, np.int64)
+    n = quads.shape[0]
+    a = quads.reshape(-1)
+    b = np.roll(quads, -1, axis=1).reshape(-1)
+    key = a * (a.max() + 1 + b.max() + 1) + b
+    rkey = b * (a.max() + 1 + b.max() + 1) + a
+    order = np.argsort(key)
+    pos = np.searchsorted(key[order], rkey)
+    pos = np.clip(pos, 0, key.size - 1)
+    match = key[order][pos] == rkey
+    other = order[pos]
+    adjfaces = np.where(match, other // 4, -1).reshape(n, 4).astype(np.int32)
+    adjedges = np.where(match, other % 4, 0).reshape(n, 4).astype(np.int32)
+    return adjfaces, adjedges
+
+
+def grid_ptex(rng, nu, nv, res_log2, tint, lattice=(32, 16)):
+    """Per-face RGB8 value-noise texture of an nu x nv grid mesh whose quad (i, j) is face
+    j * nu + i with Ptex u along j and v along i (the quads of vcm_scene): a smooth field over the
+    whole mesh sampled at each face's texel centres, so neighbouring faces continue each other."""
+    res = 1 << res_log2
+    Wi = _smooth_interp_matrix(nu * res, lattice[0], nu)   # along i (= face v)
+    Wj = _smooth_interp_matrix(nv * res, lattice[1], nv)   # along j (= face u)
+    tex = np.empty((nv, nu, res, res, 3), np.uint8)
+    for c in range(3):
+        L = rng.random((lattice[0] + 1, lattice[1] + 1))
+        F = Wi @ L @ Wj.T  # (nu*res, nv*res): [i*res + tv, j*res + tu]
+        val = np.clip(np.round(255 * tint[c] * (0.35 + 0.65 * F)), 0, 255).astype(np.uint8)
+        # face (i, j), texel row tv (along i), column tu (along j)
+        tex[..., c] = val.reshape(nu, res, nv, res).transpose(2, 0, 1, 3)
+    return tex.reshape(nv * nu, res, res, 3)
+
+
+def geom_tri9(g):
+    """Triangles (n, 9) of a geometry entry: a triangle soup as is, a quad geometry split as
+    Embree splits quads, (v0, v1, v3) then (v2, v3, v1) for each quad in order."""
+    if isinstance(g, np.ndarray):
+        return g
+    p = np.asarray(g["positions"], np.float32)
+    q = np.asarray(g["quads"], np.int64)
+    ta = np.stack([p[q[:, 0]], p[q[:, 1]], p[q[:, 3]]], 1)
+    tb = np.stack([p[q[:, 2]], p[q[:, 3]], p[q[:, 1]]], 1)
+    return np.stack([ta, tb], 1).reshape(-1, 9).astype(np.float32)
+
+
+MAT_TRANSPARENT = 1
+MAT_USES_PTEX = 16
+
+
+def vcm_scene(seed: int = 3, n_meshes: int = 64, nu: int = 128, nv: int = 64, textures: bool = True,
+              res_log2: int = 4):
+    """VCM workload scene: n_meshes displaced-grid quad meshes of nu x nv quads (2 nu nv triangles
+    each; 64 x 16,384 = 1,048,576 triangles) placed by a seeded RNG as panels of an enclosure: each
+    mesh is a unit-by-half grid with value-noise displacement, scaled by 4..8, centred at a random
     direction and distance 3.5..5 from the centre of the 10^3 box and turned so that its front
     face (the side its geometric normal points to) faces the centre, with a random twist. Per mesh,
     Disney parameters: roughness uniform in [0.05, 1], metallic 0 or 1 (p 0.5), specTrans 1 with
-    probability 0.2 (else 0), base colour uniform in [0.2, 0.9]^3. Lights inside the enclosure:
-    one 1.5 x 1.5 quad light (radiance 20) facing down and one point light (intensity 15). The
-    camera is inside too, so every surface it sees faces it (Selas's surfaces are one-sided for
-    opaque materials: no normal flip in CalculateSurfaceParams). Merge radius 0.3 % of the scene
-    diagonal."""
+    probability 0.2 (else 0), and (with `textures`) a per-face 16 x 16 RGB8 value-noise texture
+    tinted by a base colour uniform in [0.2, 0.9]^3 (without textures that colour is the material's
+    base colour). Lights inside the enclosure: one 1.5 x 1.5 quad light (radiance 20) facing down
+    and one point light (intensity 15). The camera is inside too, so every surface it sees faces it
+    (Selas's surfaces are one-sided for opaque materials: no normal flip in CalculateSurfaceParams).
+    Merge radius 0.3 % of the scene diagonal."""
     rng = np.random.default_rng(seed)
     geoms, mats = [], []
     centre = np.array([5.0, 5.0, 5.0])
+    ii, jj = np.meshgrid(np.arange(nu), np.arange(nv), indexing="xy")  # quad (i, j) = face j * nu + i
+    pid = lambda i, j: i * (nv + 1) + j  # vertex (i, j), i along x, j along z
+    quads = np.stack([pid(ii, jj), pid(ii, jj + 1), pid(ii + 1, jj + 1), pid(ii + 1, jj)], -1).reshape(-1, 4)
+    adj = quad_adjacency(quads) if textures else None
     for m in range(n_meshes):
         xs = np.linspace(0, 1, nu + 1)
         zs = np.linspace(0, 0.5, nv + 1)
         X, Z = np.meshgrid(xs, zs, indexing="ij")
         H = 0.12 * value_noise_2d(rng, 8, X, Z * 2)
-        local = quad_grid_mesh(nu, nv, 1.0, 0.5, H).reshape(-1, 3, 3)
-        local = local[:, [0, 2, 1]].reshape(-1, 3)  # wind so the geometric normal is +y
-        local = local - np.array([0.5, 0.0, 0.25])
+        local = np.stack([X, H, Z], -1).reshape(-1, 3) - np.array([0.5, 0.0, 0.25])
         d = rng.normal(size=3)
         d /= np.linalg.norm(d)
         R = _frame_from_normal(-d, rng.uniform(0, 2 * np.pi))
         s = rng.uniform(4.0, 8.0)
         t = centre + d * rng.uniform(3.5, 5.0)
-        world = (local @ R.T) * s + t
-        geoms.append(world.reshape(-1, 9).astype(np.float32))
+        world = ((local @ R.T) * s + t).astype(np.float32)
         metallic = 1.0 if rng.random() < 0.5 else 0.0
         spec_trans = 1.0 if rng.random() < 0.2 else 0.0
-        mats.append(material(1, rng.uniform(0.2, 0.9, 3), metallic, rng.uniform(0.05, 1.0), 1.5, spec_trans=spec_trans))
+        base = rng.uniform(0.2, 0.9, 3)
+        flags = MAT_USES_PTEX if textures else 0
+        mats.append(material(1, base, metallic, rng.uniform(0.05, 1.0), 1.5, spec_trans=spec_trans, flags=flags))
+        g = {"positions": world, "quads": quads.astype(np.uint32)}
+        if textures:
+            trng = np.random.default_rng((seed, m))
+            g["ptex"] = {"res_log2": res_log2, "texels": grid_ptex(trng, nu, nv, res_log2, base),
+                         "adjfaces": adj[0], "adjedges": adj[1]}
+        geoms.append(g)
     lights = np.zeros(2, LIGHT_DTYPE)
     lights[0] = (0, (5.0, 7.5, 5.0), (0.0, -1.0, 0.0), (1.5, 0.0, 0.0), (0.0, 0.0, 1.5), (20.0, 20.0, 20.0))
     lights[1] = (1, (4.0, 4.0, 6.0), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (15.0, 15.0, 15.0))
-    allp = np.concatenate(geoms).reshape(-1, 3)
+    allp = np.concatenate([g["positions"] for g in geoms]).reshape(-1, 3)
     diag = float(np.linalg.norm(allp.max(0) - allp.min(0)))
     return {
         "geoms": geoms,
