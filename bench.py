@@ -1,8 +1,9 @@
 """Benchmarks of the rendering hot path on one MI355X. Prints one JSON line.
 
 --workload vcm (default): vertex connection and merging on the VCM workload scene
-(selas_amd/workloads.py vcm_scene: 64 displaced-grid meshes, 1,048,576 triangles, one quad and one
-point light), 512 x 512, max path length 10, merge radius 0.3 % of the scene diagonal shrinking
+(selas_amd/workloads.py vcm_scene: 64 displaced-grid quad meshes, 1,048,576 triangles, per-face
+16 x 16 RGB8 textures, one quad and one point light), 512 x 512, max path length 10, merge radius
+0.3 % of the scene diagonal shrinking
 with alpha 0.75. One step = one VCM iteration (a light path and a camera path per pixel); the timed
 region is one sgpu_render_vcm call of `steps` iterations (32 = the workload's job), with the scene
 and the BVH resident in HBM. value = camera samples per second; Mrays/s counts every closest-hit
@@ -78,7 +79,7 @@ def cpu_baseline_traversal(t9, rays, sample_rays, threads):
 def cpu_baseline_vcm(sc, width, iterations, threads):
     import oracle as O
 
-    log(f"cpu baseline: building the oracle SAH tree over {sum(g.shape[0] for g in sc['geoms'])} triangles")
+    log("cpu baseline: building the oracle SAH tree")
     rs = O.RenderScene(sc)
     log(f"cpu baseline: VCM {width}x{width}, {iterations} iterations on {threads} threads")
     t0 = time.perf_counter()
@@ -104,7 +105,7 @@ def bench_vcm(args):
     w = args.width
     log("building VCM workload scene")
     sc = W.vcm_scene()
-    n_tris = sum(g.shape[0] for g in sc["geoms"])
+    n_tris = sum(W.geom_tri9(g).shape[0] for g in sc["geoms"])
     ctx = N.Context(0)
     tb = time.perf_counter()
     scene = N.build_render_scene(ctx, sc)
@@ -171,7 +172,7 @@ def bench_vcm(args):
             "merge_radius": round(float(r), 6),
             "radius_alpha": alpha,
             "lights": "one quad light + one point light",
-            "textures": "none (materials use constant base colours)",
+            "textures": "per-face 16x16 RGB8 value noise (Ptex f_bspline), 403 MB",
         },
         "mrays_s": round(st["rays"] / secs / 1e6, 2),
         "rays_per_sample": round(st["rays"] / samples, 3),
