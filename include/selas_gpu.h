@@ -249,6 +249,8 @@ int sgpu_render_pt(sgpu_scene scene, const sgpu_pt_job* job, float* framebuffer,
 
 #define SGPU_VCM_KERNEL_TIMES 1u    /* time each phase with events (sgpu_vcm_stats.*_ms) */
 #define SGPU_VCM_COUNT_TRAVERSAL 2u /* count nodes and triangles visited by closest-hit rays (slower) */
+#define SGPU_VCM_ONE_SLICE_PER_WAVEFRONT 4u /* run each iteration of each job as its own wavefront (the
+                                               un-batched schedule, for comparisons) */
 
 typedef struct sgpu_vcm_job {
     sgpu_camera camera;
@@ -268,6 +270,7 @@ typedef struct sgpu_vcm_stats {
     uint64_t closest_launches; /* closest-hit traversal launches */
     uint64_t light_vertices;   /* light vertices stored, summed over iterations */
     uint64_t samples;          /* camera paths = width x height x iterations */
+    uint64_t wavefronts;       /* wavefronts the iterations were packed into */
     uint64_t nodes_visited;    /* SGPU_VCM_COUNT_TRAVERSAL: over all closest-hit rays */
     uint64_t tris_tested;      /* SGPU_VCM_COUNT_TRAVERSAL: over all closest-hit rays */
     float closest_ms;          /* SGPU_VCM_KERNEL_TIMES: device time of the closest-hit traversals */
@@ -286,6 +289,15 @@ typedef struct sgpu_vcm_stats {
  * NULL. Synchronous on `stream`. */
 int sgpu_render_vcm(sgpu_scene scene, const sgpu_vcm_job* job, float* framebuffer, sgpu_vcm_stats* stats,
                     void* stream);
+
+/* Batched form: n_jobs render jobs of any size, each on its own scene (scenes may repeat), run as
+ * one schedule. A job's iterations are slices; slices of all jobs are packed in job order into
+ * wavefronts of about 2^20 paths, so small jobs share launches and large ones fill the chip; rays
+ * of different scenes share traversal launches. Each framebuffers[j] receives exactly what
+ * sgpu_render_vcm(scenes[j], &jobs[j], ...) writes. flags: SGPU_VCM_* for the whole batch (the jobs'
+ * own flags must be 0); stats (may be NULL) sums the batch. Synchronous on `stream`. */
+int sgpu_render_vcm_batch(const sgpu_scene* scenes, const sgpu_vcm_job* jobs, float* const* framebuffers,
+                          uint32_t n_jobs, uint32_t flags, sgpu_vcm_stats* stats, void* stream);
 
 /* ---- procedural inputs (host side, deterministic from a seed) ------------------------------
  * Gaussian-cluster triangle soup: n_clusters x tris_per_cluster triangles. Cluster centres are

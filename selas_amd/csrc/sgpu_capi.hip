@@ -467,29 +467,57 @@ int sgpu_render_pt(sgpu_scene scene, const sgpu_pt_job* job, float* framebuffer,
     return sgpu_pt_render_impl(scene, job, framebuffer, as_stream(stream));
 }
 
+static int check_vcm_job(sgpu_scene scene, const sgpu_vcm_job* job, const void* framebuffer, const char* who)
+{
+    if (!scene || !job || !framebuffer) return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": NULL argument");
+    if (!scene->committed) return sgpu_fail(SGPU_E_STATE, std::string(who) + ": scene not committed");
+    if (!scene->d_materials || scene->n_geom_material < scene->geom_first.size())
+        return sgpu_fail(SGPU_E_STATE, std::string(who) + ": materials not set for every geometry");
+    if (scene->n_geom_material >= (1u << 27))
+        return sgpu_fail(SGPU_E_RANGE, std::string(who) + ": more than 2^27 geometries");
+    if (job->camera.width == 0 || job->camera.height == 0 || job->iterations == 0 || job->max_path_length == 0)
+        return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": zero width, height, iterations or path length");
+    if (job->flags & ~(SGPU_VCM_KERNEL_TIMES | SGPU_VCM_COUNT_TRAVERSAL | SGPU_VCM_ONE_SLICE_PER_WAVEFRONT))
+        return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": unknown flags");
+    if (!(job->base_radius > 0.0f) || !(job->base_radius < 3.0e38f))
+        return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": base_radius must be positive and finite");
+    if (!(job->radius_alpha >= 0.0f && job->radius_alpha <= 1.0f))
+        return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": radius_alpha must lie in [0, 1]");
+    if ((uint64_t)job->camera.width * job->camera.height > (1ull << 24))
+        return sgpu_fail(SGPU_E_RANGE, std::string(who) + ": more than 2^24 pixels");
+    if (job->max_path_length > 31) return sgpu_fail(SGPU_E_RANGE, std::string(who) + ": path length above 31");
+    if ((uint64_t)job->iteration_base + job->iterations > (1ull << 30))
+        return sgpu_fail(SGPU_E_RANGE, std::string(who) + ": iteration index above 2^30");
+    return SGPU_OK;
+}
+
 int sgpu_render_vcm(sgpu_scene scene, const sgpu_vcm_job* job, float* framebuffer, sgpu_vcm_stats* stats, void* stream)
 {
-    if (!scene || !job || !framebuffer) return sgpu_fail(SGPU_E_INVALID, "sgpu_render_vcm: NULL argument");
-    if (!scene->committed) return sgpu_fail(SGPU_E_STATE, "sgpu_render_vcm: scene not committed");
-    if (!scene->d_materials || scene->n_geom_material < scene->geom_first.size())
-        return sgpu_fail(SGPU_E_STATE, "sgpu_render_vcm: materials not set for every geometry");
-    if (scene->n_geom_material >= (1u << 27))
-        return sgpu_fail(SGPU_E_RANGE, "sgpu_render_vcm: more than 2^27 geometries");
-    if (job->camera.width == 0 || job->camera.height == 0 || job->iterations == 0 || job->max_path_length == 0)
-        return sgpu_fail(SGPU_E_INVALID, "sgpu_render_vcm: zero width, height, iterations or path length");
-    if (job->flags & ~(SGPU_VCM_KERNEL_TIMES | SGPU_VCM_COUNT_TRAVERSAL))
-        return sgpu_fail(SGPU_E_INVALID, "sgpu_render_vcm: unknown flags");
-    if (!(job->base_radius > 0.0f) || !(job->base_radius < 3.0e38f))
-        return sgpu_fail(SGPU_E_INVALID, "sgpu_render_vcm: base_radius must be positive and finite");
-    if (!(job->radius_alpha >= 0.0f && job->radius_alpha <= 1.0f))
-        return sgpu_fail(SGPU_E_INVALID, "sgpu_render_vcm: radius_alpha must lie in [0, 1]");
-    if ((uint64_t)job->camera.width * job->camera.height > (1ull << 24))
-        return sgpu_fail(SGPU_E_RANGE, "sgpu_render_vcm: more than 2^24 pixels");
-    if (job->max_path_length > 31) return sgpu_fail(SGPU_E_RANGE, "sgpu_render_vcm: path length above 31");
-    if ((uint64_t)job->iteration_base + job->iterations > (1ull << 30))
-        return sgpu_fail(SGPU_E_RANGE, "sgpu_render_vcm: iteration index above 2^30");
+    int rc = check_vcm_job(scene, job, framebuffer, "sgpu_render_vcm");
+    if (rc) return rc;
     SGPU_HIP_TRY(hipSetDevice(scene->ctx->device));
     return sgpu_vcm_render_impl(scene, job, framebuffer, stats, as_stream(stream));
+}
+
+int sgpu_render_vcm_batch(const sgpu_scene* scenes, const sgpu_vcm_job* jobs, float* const* framebuffers,
+                          uint32_t n_jobs, uint32_t flags, sgpu_vcm_stats* stats, void* stream)
+{
+    if (!scenes || !jobs || !framebuffers || n_jobs == 0)
+        return sgpu_fail(SGPU_E_INVALID, "sgpu_render_vcm_batch: NULL or empty argument");
+    if (flags & ~(SGPU_VCM_KERNEL_TIMES | SGPU_VCM_COUNT_TRAVERSAL | SGPU_VCM_ONE_SLICE_PER_WAVEFRONT))
+        return sgpu_fail(SGPU_E_INVALID, "sgpu_render_vcm_batch: unknown flags");
+    for (uint32_t j = 0; j < n_jobs; ++j) {
+        int rc = check_vcm_job(scenes[j], &jobs[j], framebuffers[j], "sgpu_render_vcm_batch");
+        if (rc) return rc;
+        if (jobs[j].flags != 0) return sgpu_fail(SGPU_E_INVALID, "sgpu_render_vcm_batch: job flags must be 0");
+        if (scenes[j]->ctx != scenes[0]->ctx)
+            return sgpu_fail(SGPU_E_INVALID, "sgpu_render_vcm_batch: scenes of different contexts");
+        for (uint32_t k = 0; k < j; ++k)
+            if (framebuffers[k] == framebuffers[j])
+                return sgpu_fail(SGPU_E_INVALID, "sgpu_render_vcm_batch: two jobs share a framebuffer");
+    }
+    SGPU_HIP_TRY(hipSetDevice(scenes[0]->ctx->device));
+    return sgpu_vcm_render_batch_impl(scenes, jobs, framebuffers, n_jobs, flags, stats, as_stream(stream));
 }
 
 int sgpu_scene_info(sgpu_scene scene, uint32_t* n_tris, uint32_t* n_nodes, uint32_t* max_depth)

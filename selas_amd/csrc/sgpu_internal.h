@@ -97,7 +97,23 @@ int sgpu_trace_launch(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hi
                       uint32_t* occluded, uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream,
                       uint32_t* spill = nullptr);
 size_t sgpu_trace_spill_words(sgpu_scene scene);
+size_t sgpu_trace_spill_words(sgpu_context ctx, uint32_t max_depth);
+// A scene's tree as a multi-scene traversal launch reads it (one table entry per scene of a batch).
+struct TraceScene {
+    const BvhNode* nodes;
+    const TriRecord* tris;
+    uint32_t n_tris;
+    uint32_t pad;
+};
+// One launch over rays of several scenes: ray i traverses table[ray_scene[i]] (device arrays);
+// max_depth is the largest tree depth in the table.
+int sgpu_trace_launch_multi(sgpu_context ctx, const TraceScene* table, const uint32_t* ray_scene, uint32_t max_depth,
+                            uint32_t* overflow, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits,
+                            uint32_t* occluded, uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream,
+                            uint32_t* spill = nullptr);
 // Wavefront path tracer (sgpu_pt.hip); arguments validated by sgpu_render_pt.
 int sgpu_pt_render_impl(sgpu_scene scene, const sgpu_pt_job* job, float* framebuffer, hipStream_t stream);
 int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* framebuffer, sgpu_vcm_stats* stats,
                          hipStream_t stream);
+int sgpu_vcm_render_batch_impl(const sgpu_scene* scenes, const sgpu_vcm_job* jobs, float* const* framebuffers,
+                               uint32_t n_jobs, uint32_t flags, sgpu_vcm_stats* stats, hipStream_t stream);

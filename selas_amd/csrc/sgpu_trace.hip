@@ -52,9 +52,11 @@ __device__ __forceinline__ float safe_rcp(float x)
     return __builtin_amdgcn_rcpf(y);
 }
 
-template <bool ANY, bool STATS>
+template <bool ANY, bool STATS, bool MULTI>
 __global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace(const BvhNode* __restrict nodes,
                                                             const TriRecord* __restrict tris, uint32_t n_tris,
+                                                            const TraceScene* __restrict table,
+                                                            const uint32_t* __restrict ray_scene,
                                                             sgpu_rays_soa rays, sgpu_hits_soa hits,
                                                             uint32_t* __restrict occluded, uint32_t n,
                                                             uint32_t* __restrict spill, uint32_t stack_cap,
@@ -74,6 +76,16 @@ __global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace(const BvhNode* __res
         float tfar = rays.tfar[i];
         f3 inv = mk3(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
         f3 oinv = mk3(o.x * inv.x, o.y * inv.y, o.z * inv.z);
+        // the ray's scene: the launch's one, or its own entry of the table in a multi-scene launch
+        const BvhNode* __restrict rn = nodes;
+        const TriRecord* __restrict rt = tris;
+        uint32_t rnt = n_tris;
+        if (MULTI) {
+            const TraceScene ts = table[ray_scene[i]];
+            rn = ts.nodes;
+            rt = ts.tris;
+            rnt = ts.n_tris;
+        }
 
         float best_t = tfar;
         uint32_t best_key = 0xFFFFFFFFu;
@@ -82,11 +94,11 @@ __global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace(const BvhNode* __res
         bool hit_any = false;
 
         uint32_t sp = 0;
-        int node = n_tris > 0 ? 0 : 0x7FFFFFFF;
-        bool done = n_tris == 0;
+        int node = rnt > 0 ? 0 : 0x7FFFFFFF;
+        bool done = rnt == 0;
         while (!done) {
             if (node >= 0) {
-                const BvhNode nd = nodes[node];
+                const BvhNode nd = rn[node];
                 if (STATS) ++cnt_nodes;
                 float l0x = fmaf(nd.x.x, inv.x, -oinv.x), h0x = fmaf(nd.x.y, inv.x, -oinv.x);
                 float l1x = fmaf(nd.x.z, inv.x, -oinv.x), h1x = fmaf(nd.x.w, inv.x, -oinv.x);
@@ -128,7 +140,7 @@ __global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace(const BvhNode* __res
                 }
             } else {
                 const uint32_t j = (uint32_t)(~node);
-                const TriRecord r = tris[j];
+                const TriRecord r = rt[j];
                 if (STATS) ++cnt_tris;
                 float t, u, v;
                 if (tri_test(r, o, d, tnear, t, u, v)) {
@@ -164,7 +176,7 @@ __global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace(const BvhNode* __res
             uint32_t prim = SGPU_INVALID_ID, geom = SGPU_INVALID_ID;
             f3 ng = mk3(0.0f, 0.0f, 0.0f);
             if (found) {
-                const TriRecord r = tris[best_tri];
+                const TriRecord r = rt[best_tri];
                 geom = __float_as_uint(r.b.w);
                 prim = __float_as_uint(r.c.w);
                 if (hits.ng_x) ng = cross3(mk3(r.b.x, r.b.y, r.b.z), mk3(r.c.x, r.c.y, r.c.z));
@@ -197,53 +209,91 @@ __global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace(const BvhNode* __res
 
 }  // namespace
 
-size_t sgpu_trace_spill_words(sgpu_scene scene)
+
+namespace {
+
+template <bool ANY, bool STATS>
+void launch_trace(bool multi, uint32_t blocks, hipStream_t stream, const BvhNode* nodes, const TriRecord* tris,
+                  uint32_t n_tris, const TraceScene* table, const uint32_t* ray_scene, const sgpu_rays_soa& rays,
+                  const sgpu_hits_soa& h, uint32_t* occluded, uint32_t n, uint32_t* spill, uint32_t cap,
+                  unsigned long long* st, uint32_t* overflow)
 {
-    const uint32_t cus = (uint32_t)scene->ctx->props.multiProcessorCount;
-    const uint32_t threads = cus * 8 * SGPU_TRACE_BLOCK;
-    const uint32_t cap = scene->max_depth + 1;
-    const uint32_t spill_depth = cap > SGPU_STACK_LDS ? cap - SGPU_STACK_LDS : 1;
-    return (size_t)spill_depth * threads;
+    if (multi)
+        hipLaunchKernelGGL((k_trace<ANY, STATS, true>), dim3(blocks), dim3(SGPU_TRACE_BLOCK), 0, stream, nodes, tris,
+                           n_tris, table, ray_scene, rays, h, occluded, n, spill, cap, st, overflow);
+    else
+        hipLaunchKernelGGL((k_trace<ANY, STATS, false>), dim3(blocks), dim3(SGPU_TRACE_BLOCK), 0, stream, nodes,
+                           tris, n_tris, table, ray_scene, rays, h, occluded, n, spill, cap, st, overflow);
 }
 
-int sgpu_trace_launch(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits, uint32_t* occluded,
-                      uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream, uint32_t* spill_buf)
+// One traversal launch over n rays: of `scene`, or (table != nullptr) of the scenes of a batch,
+// ray i in table[ray_scene[i]]; max_depth bounds every scene's tree depth.
+int trace_launch(sgpu_context ctx, const BvhNode* nodes, const TriRecord* tris, uint32_t n_tris, uint32_t max_depth,
+                 uint32_t* overflow, const TraceScene* table, const uint32_t* ray_scene, const sgpu_rays_soa* rays,
+                 const sgpu_hits_soa* hits, uint32_t* occluded, uint32_t n, bool any_hit, uint64_t* d_stats,
+                 hipStream_t stream, uint32_t* spill_buf)
 {
     if (n == 0) return SGPU_OK;
-    const uint32_t cus = (uint32_t)scene->ctx->props.multiProcessorCount;
+    const uint32_t cus = (uint32_t)ctx->props.multiProcessorCount;
     uint32_t blocks = (n + SGPU_TRACE_BLOCK - 1) / SGPU_TRACE_BLOCK;
     const uint32_t max_blocks = cus * 8;
     if (blocks > max_blocks) blocks = max_blocks;
-    const uint32_t cap = scene->max_depth + 1;
+    const uint32_t cap = max_depth + 1;
 
     // stream-ordered scratch for the spilled part of the stacks, private to this call, unless the
     // caller lends a buffer of sgpu_trace_spill_words (sized for the largest grid)
     uint32_t* spill = spill_buf;
-    if (!spill) SGPU_HIP_TRY(hipMallocAsync((void**)&spill, sgpu_trace_spill_words(scene) * sizeof(uint32_t), stream));
-    uint32_t* overflow = scene->d_overflow;
-
+    if (!spill) SGPU_HIP_TRY(hipMallocAsync((void**)&spill, sgpu_trace_spill_words(ctx, max_depth) * sizeof(uint32_t),
+                                            stream));
     sgpu_hits_soa h = {};
     if (hits) h = *hits;
     unsigned long long* st = (unsigned long long*)d_stats;
+    const bool multi = table != nullptr;
     if (any_hit) {
         if (st)
-            hipLaunchKernelGGL((k_trace<true, true>), dim3(blocks), dim3(SGPU_TRACE_BLOCK), 0, stream, scene->d_nodes,
-                               scene->d_tris, scene->n_tris, *rays, h, occluded, n, spill, cap, st, overflow);
+            launch_trace<true, true>(multi, blocks, stream, nodes, tris, n_tris, table, ray_scene, *rays, h, occluded,
+                                     n, spill, cap, st, overflow);
         else
-            hipLaunchKernelGGL((k_trace<true, false>), dim3(blocks), dim3(SGPU_TRACE_BLOCK), 0, stream,
-                               scene->d_nodes, scene->d_tris, scene->n_tris, *rays, h, occluded, n, spill, cap, st,
-                               overflow);
+            launch_trace<true, false>(multi, blocks, stream, nodes, tris, n_tris, table, ray_scene, *rays, h, occluded,
+                                      n, spill, cap, st, overflow);
     } else {
         if (st)
-            hipLaunchKernelGGL((k_trace<false, true>), dim3(blocks), dim3(SGPU_TRACE_BLOCK), 0, stream,
-                               scene->d_nodes, scene->d_tris, scene->n_tris, *rays, h, occluded, n, spill, cap, st,
-                               overflow);
+            launch_trace<false, true>(multi, blocks, stream, nodes, tris, n_tris, table, ray_scene, *rays, h, occluded,
+                                      n, spill, cap, st, overflow);
         else
-            hipLaunchKernelGGL((k_trace<false, false>), dim3(blocks), dim3(SGPU_TRACE_BLOCK), 0, stream,
-                               scene->d_nodes, scene->d_tris, scene->n_tris, *rays, h, occluded, n, spill, cap, st,
-                               overflow);
+            launch_trace<false, false>(multi, blocks, stream, nodes, tris, n_tris, table, ray_scene, *rays, h,
+                                       occluded, n, spill, cap, st, overflow);
     }
     SGPU_HIP_TRY(hipGetLastError());
     if (!spill_buf) SGPU_HIP_TRY(hipFreeAsync(spill, stream));
     return SGPU_OK;
+}
+
+}  // namespace
+
+size_t sgpu_trace_spill_words(sgpu_context ctx, uint32_t max_depth)
+{
+    const uint32_t cus = (uint32_t)ctx->props.multiProcessorCount;
+    const uint32_t threads = cus * 8 * SGPU_TRACE_BLOCK;
+    const uint32_t cap = max_depth + 1;
+    const uint32_t spill_depth = cap > SGPU_STACK_LDS ? cap - SGPU_STACK_LDS : 1;
+    return (size_t)spill_depth * threads;
+}
+
+size_t sgpu_trace_spill_words(sgpu_scene scene) { return sgpu_trace_spill_words(scene->ctx, scene->max_depth); }
+
+int sgpu_trace_launch(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits, uint32_t* occluded,
+                      uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream, uint32_t* spill_buf)
+{
+    return trace_launch(scene->ctx, scene->d_nodes, scene->d_tris, scene->n_tris, scene->max_depth, scene->d_overflow,
+                        nullptr, nullptr, rays, hits, occluded, n, any_hit, d_stats, stream, spill_buf);
+}
+
+int sgpu_trace_launch_multi(sgpu_context ctx, const TraceScene* table, const uint32_t* ray_scene, uint32_t max_depth,
+                            uint32_t* overflow, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits,
+                            uint32_t* occluded, uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream,
+                            uint32_t* spill_buf)
+{
+    return trace_launch(ctx, nullptr, nullptr, 0, max_depth, overflow, table, ray_scene, rays, hits, occluded, n,
+                        any_hit, d_stats, stream, spill_buf);
 }

@@ -69,14 +69,17 @@ __device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter)
 
 struct RayBuf {
     float *ox, *oy, *oz, *dx, *dy, *dz, *tnear, *tfar;
+    uint32_t* scene;  // per ray, the scene it traverses (multi-scene wavefronts only, else nullptr)
 };
 struct HitBuf {
     float *t, *u, *v, *ngx, *ngy, *ngz;
     uint32_t *prim, *geom;
 };
 
-__device__ __forceinline__ void put_ray(const RayBuf& r, uint32_t i, f3 o, f3 d, float tnear, float tfar)
+__device__ __forceinline__ void put_ray(const RayBuf& r, uint32_t i, f3 o, f3 d, float tnear, float tfar,
+                                        uint32_t scene)
 {
+    if (r.scene) r.scene[i] = scene;
     r.ox[i] = o.x;
     r.oy[i] = o.y;
     r.oz[i] = o.z;
@@ -138,6 +141,7 @@ struct Slice {
     uint32_t path_base;  // first wavefront path of the slice; also its first grid cell
     uint32_t iteration;  // global iteration index (radius schedule and random streams)
     uint32_t job;        // index of the job whose image the slice adds to
+    uint32_t scene;      // index of the job's scene in the wavefront's traversal table
     uint64_t seed;
     float vm_weight, vc_weight, vm_norm, radius_sq, inv_cell;
     const float* materials;  // shading records (sgpu::shading_record)
@@ -254,10 +258,12 @@ __global__ __launch_bounds__(kBlock) void k_light_gen(Wave w, PathState ps, RayB
 {
     uint32_t g = blockIdx.x * kBlock + threadIdx.x;
     bool live = false;
+    uint32_t scn = 0;
     f3 o = mk3(0, 0, 0), d = mk3(0, 0, 0);
     if (g < w.total) {
         const uint32_t si = find_slice(w, g);
         const Slice& sl = w.slices[si];
+        scn = sl.scene;
         ps.slice[g] = si;
         PcgUniforms u{rng_init(sl.seed, stream_id(sl.iteration, g - sl.path_base, 0))};
         ps.plen[g] = 1;
@@ -308,7 +314,7 @@ __global__ __launch_bounds__(kBlock) void k_light_gen(Wave w, PathState ps, RayB
     }
     uint32_t k = wave_append(live, &counters[0]);
     if (live) {
-        put_ray(rays, k, o, d, kRayEps, kFloatMax);
+        put_ray(rays, k, o, d, kRayEps, kFloatMax, scn);
         queue[k] = g;
     }
 }
@@ -322,12 +328,13 @@ __global__ __launch_bounds__(kBlock) void k_light_shade(Wave w, uint32_t count, 
 {
     uint32_t j = blockIdx.x * kBlock + threadIdx.x;
     bool cont = false, emit = false;
-    uint32_t p = 0, pix = 0;
+    uint32_t p = 0, pix = 0, scn = 0;
     f3 next_o = mk3(0, 0, 0), next_d = mk3(0, 0, 0), sh_o = mk3(0, 0, 0), sh_d = mk3(0, 0, 0), c = mk3(0, 0, 0);
     float sh_tnear = 0.0f, sh_tfar = 0.0f;
     if (j < count) {
         p = queue[j];
         const Slice& sl = w.slices[ps.slice[p]];
+        scn = sl.scene;
         uint32_t geom = hits.geom[j];
         if (geom != SGPU_INVALID_ID && geom < sl.n_geoms) {
             PcgUniforms u{rng_load(ps, p, sl, 0)};
@@ -438,13 +445,13 @@ __global__ __launch_bounds__(kBlock) void k_light_shade(Wave w, uint32_t count, 
     }
     uint32_t ks = wave_append(emit, &counters[1]);
     if (emit) {
-        put_ray(shadow, ks, sh_o, sh_d, sh_tnear, sh_tfar);
+        put_ray(shadow, ks, sh_o, sh_d, sh_tnear, sh_tfar, scn);
         s3(splat_c, ks, c);
         splat_pix[ks] = pix;
     }
     uint32_t kn = wave_append(cont, &counters[0]);
     if (cont) {
-        put_ray(next_rays, kn, next_o, next_d, kRayEps, kFloatMax);
+        put_ray(next_rays, kn, next_o, next_d, kRayEps, kFloatMax, scn);
         next_queue[kn] = p;
     }
 }
@@ -599,32 +606,43 @@ __global__ __launch_bounds__(kBlock) void k_grid_gather(const uint32_t* __restri
 
 // ---- camera pass ----------------------------------------------------------------------------------
 
-__global__ __launch_bounds__(kBlock) void k_cam_gen(Wave w, PathState ps, RayBuf rays, uint32_t* __restrict queue)
+__global__ __launch_bounds__(kBlock) void k_cam_gen(Wave w, PathState ps, RayBuf rays, uint32_t* __restrict queue,
+                                                    uint32_t* __restrict counters)
 {
     uint32_t g = blockIdx.x * kBlock + threadIdx.x;
-    if (g >= w.total) return;
-    const uint32_t si = find_slice(w, g);
-    const Slice& sl = w.slices[si];
-    const uint32_t i = g - sl.path_base;
-    PcgUniforms u{rng_init(sl.seed, stream_id(sl.iteration, i, 1))};
-    uint32_t x = i % sl.W, y = i / sl.W;
-    float jx = (float)x + u.next(), jy = (float)y + u.next();
-    f3 d = camera_image_to_world_dir(sl.cam, jx, jy);
-    float cos_cam = dot3(sl.cam.forward, d);
-    float img_dist = sl.vipd / cos_cam;
-    float cam_pdf_w = img_dist * img_dist / cos_cam;
-    ps.slice[g] = si;
-    s3(ps.pos, g, sl.cam.position);
-    s3(ps.dir, g, d);
-    s3(ps.thr, g, mk3(1, 1, 1));
-    s3(ps.color, g, mk3(0, 0, 0));
-    ps.dvcm[g] = (float)sl.N * (1.0f / cam_pdf_w);
-    ps.dvc[g] = 0.0f;
-    ps.dvm[g] = 0.0f;
-    ps.plen[g] = 1;
-    ps.rng[g] = u.r.state;
-    put_ray(rays, g, sl.cam.position, d, kRayEps, kFloatMax);
-    queue[g] = g;
+    bool live = false;
+    uint32_t scn = 0;
+    f3 o = mk3(0, 0, 0), d = mk3(0, 0, 0);
+    if (g < w.total) {
+        const uint32_t si = find_slice(w, g);
+        const Slice& sl = w.slices[si];
+        scn = sl.scene;
+        const uint32_t i = g - sl.path_base;
+        PcgUniforms u{rng_init(sl.seed, stream_id(sl.iteration, i, 1))};
+        uint32_t x = i % sl.W, y = i / sl.W;
+        float jx = (float)x + u.next(), jy = (float)y + u.next();
+        d = camera_image_to_world_dir(sl.cam, jx, jy);
+        o = sl.cam.position;
+        float cos_cam = dot3(sl.cam.forward, d);
+        float img_dist = sl.vipd / cos_cam;
+        float cam_pdf_w = img_dist * img_dist / cos_cam;
+        ps.slice[g] = si;
+        s3(ps.pos, g, o);
+        s3(ps.dir, g, d);
+        s3(ps.thr, g, mk3(1, 1, 1));
+        s3(ps.color, g, mk3(0, 0, 0));
+        ps.dvcm[g] = (float)sl.N * (1.0f / cam_pdf_w);
+        ps.dvc[g] = 0.0f;
+        ps.dvm[g] = 0.0f;
+        ps.plen[g] = 1;
+        ps.rng[g] = u.r.state;
+        live = sl.L > 1;  // a camera path of length 1 ends at the lens (VCM.cpp:585)
+    }
+    uint32_t k = wave_append(live, &counters[0]);
+    if (live) {
+        put_ray(rays, k, o, d, kRayEps, kFloatMax, scn);
+        queue[k] = g;
+    }
 }
 
 struct GridView {
@@ -765,7 +783,7 @@ __global__ __launch_bounds__(kBlock) void k_cam_shade(Wave w, uint32_t count, co
         }
         uint32_t ks = wave_append(emit, &counters[1]);
         if (emit) {
-            put_ray(shadow, ks, so, ld, tn, tf);
+            put_ray(shadow, ks, so, ld, tn, tf, sl.scene);
             S.ray_slot[ks] = p * nslots + 0;
             S.valid[(size_t)p * nslots + 0] = 2;
         }
@@ -830,7 +848,7 @@ __global__ __launch_bounds__(kBlock) void k_cam_shade(Wave w, uint32_t count, co
         }
         uint32_t ks = wave_append(emit, &counters[1]);
         if (emit) {
-            put_ray(shadow, ks, so, cd, tn, tf);
+            put_ray(shadow, ks, so, cd, tn, tf, sl.scene);
             S.ray_slot[ks] = p * nslots + 1 + k;
             S.valid[(size_t)p * nslots + 1 + k] = 2;
         }
@@ -882,7 +900,7 @@ __global__ __launch_bounds__(kBlock) void k_cam_shade(Wave w, uint32_t count, co
     }
     uint32_t kn = wave_append(cont, &counters[0]);
     if (cont) {
-        put_ray(next_rays, kn, next_o, next_d, kRayEps, kFloatMax);
+        put_ray(next_rays, kn, next_o, next_d, kRayEps, kFloatMax, sl.scene);
         next_queue[kn] = p;
     }
 }
@@ -1032,9 +1050,10 @@ struct Arena {
         if (err == hipSuccess) ptrs.push_back(p);
         return (T*)p;
     }
-    RayBuf rays(size_t n)
+    RayBuf rays(size_t n, bool with_scene)
     {
         RayBuf r;
+        r.scene = with_scene ? get<uint32_t>(n) : nullptr;
         r.ox = get<float>(n);
         r.oy = get<float>(n);
         r.oz = get<float>(n);
@@ -1107,45 +1126,102 @@ inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kB
 // Paths per wavefront the render aims for: enough lanes to fill the chip in every bounce's launch.
 constexpr uint64_t kTargetPaths = 1u << 20;
 
+// One job of a render call, validated by the C ABI.
+struct JobIn {
+    sgpu_scene scene;
+    const sgpu_vcm_job* job;
+    float* fb;
+};
+
+struct Wavefront {
+    uint32_t first_slice, n_slices, total;
+};
+
 }  // namespace
 
-int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, sgpu_vcm_stats* stats,
-                         hipStream_t stream)
+int sgpu_vcm_render_batch_impl(const sgpu_scene* scenes, const sgpu_vcm_job* jobs, float* const* framebuffers,
+                               uint32_t n_jobs, uint32_t flags, sgpu_vcm_stats* stats, hipStream_t stream)
 {
-    const uint32_t W = job->camera.width, H = job->camera.height, N = W * H, L = job->max_path_length;
-    const uint32_t mlv = L > 2 ? L - 2 : 1;
-    const uint32_t nslots = mlv + 2;
     const float pi = 3.141592653589793f;
+    const bool kernel_times = flags & SGPU_VCM_KERNEL_TIMES;
+    const bool count_traversal = flags & SGPU_VCM_COUNT_TRAVERSAL;
+    const bool one_slice = flags & SGPU_VCM_ONE_SLICE_PER_WAVEFRONT;
+    sgpu_context ctx = scenes[0]->ctx;
 
-    // slices per wavefront: iterations of this job run together up to ~kTargetPaths paths
-    const uint32_t per_wave =
-        (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(job->iterations, kTargetPaths / std::max<uint32_t>(N, 1)));
-    const uint64_t total_max = (uint64_t)per_wave * N;
-    if (total_max >= (1ull << 31)) return sgpu_fail(SGPU_E_RANGE, "sgpu_render_vcm: wavefront too large");
-    const uint32_t TM = (uint32_t)total_max;
+    // ---- scenes of the batch: one traversal table entry each ----
+    std::vector<sgpu_scene> uniq;
+    std::vector<uint32_t> job_scene(n_jobs);
+    uint32_t max_depth = 0;
+    for (uint32_t j = 0; j < n_jobs; ++j) {
+        auto it = std::find(uniq.begin(), uniq.end(), scenes[j]);
+        job_scene[j] = (uint32_t)(it - uniq.begin());
+        if (it == uniq.end()) uniq.push_back(scenes[j]);
+        max_depth = std::max(max_depth, scenes[j]->max_depth);
+    }
+    const bool multi = uniq.size() > 1;
 
-    Slice base;
-    std::memset(&base, 0, sizeof(base));
-    base.cam = camera_init(mk3(job->camera.position[0], job->camera.position[1], job->camera.position[2]),
-                           mk3(job->camera.look_at[0], job->camera.look_at[1], job->camera.look_at[2]),
-                           mk3(job->camera.up[0], job->camera.up[1], job->camera.up[2]), job->camera.fov_degrees, W, H);
-    float hfov = job->camera.fov_degrees * (pi / 180.f);
-    base.vipd = (float)W / (2.0f * tanf(hfov * 0.5f));
-    base.W = W;
-    base.H = H;
-    base.N = N;
-    base.L = L;
-    base.mlv = mlv;
-    base.seed = job->seed;
-    base.job = 0;
-    base.materials = scene->d_materials;
-    base.geom_material = scene->d_geom_material;
-    base.n_geoms = scene->n_geom_material;
-    base.lights = (const Light*)scene->d_lights;
-    base.n_lights = scene->n_lights;
-    base.background = mk3(scene->background[0], scene->background[1], scene->background[2]);
-    base.geom_tex = scene->d_geom_tex;
-    base.n_geom_tex = scene->n_geom_tex;
+    // ---- per-job slice template and the slice schedule ----
+    std::vector<Slice> job_base(n_jobs);
+    uint32_t mlv = 1;
+    uint64_t total_iterations = 0;
+    for (uint32_t j = 0; j < n_jobs; ++j) {
+        const sgpu_vcm_job* jb = &jobs[j];
+        const sgpu_scene sc = scenes[j];
+        const uint32_t W = jb->camera.width, H = jb->camera.height, L = jb->max_path_length;
+        Slice b;
+        std::memset(&b, 0, sizeof(b));
+        b.cam = camera_init(mk3(jb->camera.position[0], jb->camera.position[1], jb->camera.position[2]),
+                            mk3(jb->camera.look_at[0], jb->camera.look_at[1], jb->camera.look_at[2]),
+                            mk3(jb->camera.up[0], jb->camera.up[1], jb->camera.up[2]), jb->camera.fov_degrees, W, H);
+        float hfov = jb->camera.fov_degrees * (pi / 180.f);
+        b.vipd = (float)W / (2.0f * tanf(hfov * 0.5f));
+        b.W = W;
+        b.H = H;
+        b.N = W * H;
+        b.L = L;
+        b.mlv = L > 2 ? L - 2 : 1;
+        b.seed = jb->seed;
+        b.job = j;
+        b.scene = job_scene[j];
+        b.materials = sc->d_materials;
+        b.geom_material = sc->d_geom_material;
+        b.n_geoms = sc->n_geom_material;
+        b.lights = (const Light*)sc->d_lights;
+        b.n_lights = sc->n_lights;
+        b.background = mk3(sc->background[0], sc->background[1], sc->background[2]);
+        b.geom_tex = sc->d_geom_tex;
+        b.n_geom_tex = sc->n_geom_tex;
+        job_base[j] = b;
+        mlv = std::max(mlv, b.mlv);
+        total_iterations += jb->iterations;
+    }
+    // slices in job order, iterations in order; wavefronts take consecutive slices up to
+    // kTargetPaths paths (a larger slice takes one alone)
+    std::vector<std::pair<uint32_t, uint32_t>> sched;  // (job, iteration offset)
+    sched.reserve(total_iterations);
+    for (uint32_t j = 0; j < n_jobs; ++j)
+        for (uint32_t it = 0; it < jobs[j].iterations; ++it) sched.push_back({j, it});
+    std::vector<Wavefront> waves;
+    uint64_t TM64 = 0;
+    uint32_t max_slices = 1;
+    for (size_t k = 0; k < sched.size();) {
+        Wavefront wf{(uint32_t)k, 0, 0};
+        uint64_t total = 0;
+        while (k < sched.size()) {
+            const uint64_t n = job_base[sched[k].first].N;
+            if (wf.n_slices > 0 && (one_slice || total + n > kTargetPaths)) break;
+            total += n;
+            ++wf.n_slices;
+            ++k;
+        }
+        if (total >= (1ull << 31)) return sgpu_fail(SGPU_E_RANGE, "sgpu_render_vcm: wavefront too large");
+        wf.total = (uint32_t)total;
+        waves.push_back(wf);
+        TM64 = std::max<uint64_t>(TM64, total);
+        max_slices = std::max(max_slices, wf.n_slices);
+    }
+    const uint32_t TM = (uint32_t)TM64;
+    const uint32_t nslots = mlv + 2;
 
     Arena A;
     A.stream = stream;
@@ -1160,7 +1236,7 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
     ps.slice = A.get<uint32_t>(TM);
     ps.rng = A.get<uint64_t>(TM);
     ps.color = A.get<float>(3 * (size_t)TM);
-    RayBuf rays[2] = {A.rays(TM), A.rays(TM)};
+    RayBuf rays[2] = {A.rays(TM, multi), A.rays(TM, multi)};
     uint32_t* queue[2] = {A.get<uint32_t>(TM), A.get<uint32_t>(TM)};
     HitBuf hb;
     hb.t = A.get<float>(TM);
@@ -1173,7 +1249,7 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
     hb.geom = A.get<uint32_t>(TM);
     const size_t max_shadow = (size_t)TM * (mlv + 1);
     const size_t max_verts = (size_t)TM * mlv;
-    RayBuf shadow = A.rays(max_shadow);
+    RayBuf shadow = A.rays(max_shadow, multi);
     uint32_t* occl = A.get<uint32_t>(max_shadow);
     float* splat_c = A.get<float>(3 * (size_t)TM);
     uint32_t* splat_pix = A.get<uint32_t>(TM);
@@ -1192,13 +1268,14 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
     uint32_t* vals2 = A.get<uint32_t>(max_verts);
     uint32_t* cell_counts = A.get<uint32_t>(TM);
     uint32_t* cell_ends = A.get<uint32_t>(TM);
-    uint32_t* box = A.get<uint32_t>(8 * (size_t)per_wave);
+    uint32_t* box = A.get<uint32_t>(8 * (size_t)max_slices);
     uint32_t* counters = A.get<uint32_t>(4);
     uint32_t* nv_dev = A.get<uint32_t>(1);
     float* frame = A.get<float>(3 * (size_t)TM);
-    Slice* d_slices = A.get<Slice>(per_wave);
+    Slice* d_slices = A.get<Slice>(max_slices);
     CamVertex* cam_vm = A.get<CamVertex>(TM);
-    uint32_t* spill = A.get<uint32_t>(sgpu_trace_spill_words(scene));
+    uint32_t* spill = A.get<uint32_t>(sgpu_trace_spill_words(ctx, max_depth));
+    TraceScene* d_table = multi ? A.get<TraceScene>(uniq.size()) : nullptr;
     CamSlots S;
     S.c = A.get<float>(3 * (size_t)TM * nslots);
     S.valid = A.get<uint32_t>((size_t)TM * nslots);
@@ -1218,9 +1295,9 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
     size_t tmp_bytes = std::max(sort_bytes, std::max(scan_bytes, scan2_bytes));
     void* tmp = A.get<uint8_t>(tmp_bytes);
     if (A.err != hipSuccess) return sgpu_fail(SGPU_E_NOMEM, "sgpu_render_vcm: device allocation failed");
-    // pinned host staging: counters (8 words) and the slice table
+    // pinned host staging: counters (16 words), the slice table and the scene table
     uint8_t* hbuf = nullptr;
-    const size_t hbytes = 64 + sizeof(Slice) * (size_t)per_wave;
+    const size_t hbytes = 64 + sizeof(Slice) * (size_t)max_slices + sizeof(TraceScene) * uniq.size();
     SGPU_HIP_TRY(hipHostMalloc((void**)&hbuf, hbytes, hipHostMallocDefault));
     struct HostFree {
         void* p;
@@ -1228,37 +1305,54 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
     } hf{hbuf};
     uint32_t* h = (uint32_t*)hbuf;
     Slice* h_slices = (Slice*)(hbuf + 64);
+    TraceScene* h_table = (TraceScene*)(hbuf + 64 + sizeof(Slice) * (size_t)max_slices);
+    if (multi) {
+        for (size_t k = 0; k < uniq.size(); ++k) h_table[k] = TraceScene{uniq[k]->d_nodes, uniq[k]->d_tris, uniq[k]->n_tris, 0};
+        SGPU_HIP_TRY(hipMemcpyAsync(d_table, h_table, sizeof(TraceScene) * uniq.size(), hipMemcpyHostToDevice, stream));
+    }
+    auto trace = [&](const RayBuf& r, const sgpu_hits_soa* hits, uint32_t* occ, uint32_t n, bool any,
+                     uint64_t* d_st) -> int {
+        sgpu_rays_soa rs = as_soa(r);
+        if (multi)
+            return sgpu_trace_launch_multi(ctx, d_table, r.scene, max_depth, uniq[0]->d_overflow, &rs, hits, occ, n, any,
+                                           d_st, stream, spill);
+        return sgpu_trace_launch(uniq[0], &rs, hits, occ, n, any, d_st, stream, spill);
+    };
 
-    SGPU_HIP_TRY(hipMemsetAsync(fb, 0, 3 * (size_t)N * sizeof(float), stream));
+    for (uint32_t j = 0; j < n_jobs; ++j)
+        SGPU_HIP_TRY(hipMemsetAsync(framebuffers[j], 0, 3 * (size_t)job_base[j].N * sizeof(float), stream));
     sgpu_hits_soa hsoa{hb.t, hb.u, hb.v, hb.prim, hb.geom, hb.ngx, hb.ngy, hb.ngz};
-    uint64_t n_closest = 0, n_shadow = 0, n_light_verts = 0, n_launches = 0;
+    uint64_t n_closest = 0, n_shadow = 0, n_light_verts = 0, n_launches = 0, n_samples = 0;
     uint64_t* d_trav = nullptr;  // nodes visited, triangles tested by the closest-hit rays
-    if (job->flags & SGPU_VCM_COUNT_TRAVERSAL) {
+    if (count_traversal) {
         d_trav = A.get<uint64_t>(2);
         if (A.err != hipSuccess) return sgpu_fail(SGPU_E_NOMEM, "sgpu_render_vcm: device allocation failed");
         SGPU_HIP_TRY(hipMemsetAsync(d_trav, 0, 2 * sizeof(uint64_t), stream));
     }
-    PhaseTimer T(job->flags & SGPU_VCM_KERNEL_TIMES, stream);
+    PhaseTimer T(kernel_times, stream);
 
-    for (uint32_t it0 = 0; it0 < job->iterations; it0 += per_wave) {
-        const uint32_t ns = std::min(per_wave, job->iterations - it0);
-        const uint32_t total = ns * N;
+    for (const Wavefront& wf : waves) {
+        const uint32_t ns = wf.n_slices, total = wf.total;
         // the previous wavefront has finished (its last step synchronised the stream): the staged
         // slice table may be rewritten
+        uint32_t path_base = 0;
         for (uint32_t k = 0; k < ns; ++k) {
-            Slice sl = base;
-            const uint32_t gi = job->iteration_base + it0 + k;
+            const auto& js = sched[wf.first_slice + k];
+            const sgpu_vcm_job* jb = &jobs[js.first];
+            Slice sl = job_base[js.first];
+            const uint32_t gi = jb->iteration_base + js.second;
             // CalculateIterationConstants (VCMCommon.cpp:97-113), vmCount = W*H, vcCount = 1
             float iteration_index = (float)gi + 1.0f;
-            float r = job->base_radius / pow_f(iteration_index, 0.5f * (1.0f - job->radius_alpha));
+            float r = jb->base_radius / pow_f(iteration_index, 0.5f * (1.0f - jb->radius_alpha));
             float r2 = r * r;
             sl.radius_sq = r2;
-            sl.vm_norm = 1.0f / (pi * r2 * N);
-            sl.vm_weight = pi * r2 * N / 1;
-            sl.vc_weight = 1 / (pi * r2 * N);
+            sl.vm_norm = 1.0f / (pi * r2 * sl.N);
+            sl.vm_weight = pi * r2 * sl.N / 1;
+            sl.vc_weight = 1 / (pi * r2 * sl.N);
             sl.inv_cell = 1.0f / (2.0f * r);
             sl.iteration = gi;
-            sl.path_base = k * N;
+            sl.path_base = path_base;
+            path_base += sl.N;
             h_slices[k] = sl;
         }
         SGPU_HIP_TRY(hipMemcpyAsync(d_slices, h_slices, sizeof(Slice) * ns, hipMemcpyHostToDevice, stream));
@@ -1268,6 +1362,7 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
         wv.total = total;
         wv.mlv = mlv;
         wv.nslots = nslots;
+        n_samples += total;
 
         SGPU_HIP_TRY(hipMemsetAsync(frame, 0, 3 * (size_t)total * sizeof(float), stream));
         SGPU_HIP_TRY(hipMemsetAsync(vcount, 0, ((size_t)total + 1) * sizeof(uint32_t), stream));
@@ -1284,9 +1379,8 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
         uint32_t count = h[0];
         int cur = 0;
         while (count > 0) {
-            sgpu_rays_soa rs = as_soa(rays[cur]);
             SGPU_HIP_TRY(T.begin(kPhaseClosest));
-            int rc = sgpu_trace_launch(scene, &rs, &hsoa, nullptr, count, false, d_trav, stream, spill);
+            int rc = trace(rays[cur], &hsoa, nullptr, count, false, d_trav);
             if (rc) return rc;
             SGPU_HIP_TRY(T.end());
             n_closest += count;
@@ -1302,9 +1396,8 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
             SGPU_HIP_TRY(hipStreamSynchronize(stream));
             uint32_t n_next = h[0], n_sh = h[1];
             if (n_sh > 0) {
-                sgpu_rays_soa ss = as_soa(shadow);
                 SGPU_HIP_TRY(T.begin(kPhaseShadow));
-                rc = sgpu_trace_launch(scene, &ss, nullptr, occl, n_sh, true, nullptr, stream, spill);
+                rc = trace(shadow, nullptr, occl, n_sh, true, nullptr);
                 if (rc) return rc;
                 SGPU_HIP_TRY(T.end());
                 n_shadow += n_sh;
@@ -1372,16 +1465,19 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
         G.nv_ptr = nv_dev;
 
         // ---- camera pass ----
+        SGPU_HIP_TRY(hipMemsetAsync(counters, 0, 4 * sizeof(uint32_t), stream));
         SGPU_HIP_TRY(T.begin(kPhaseCamera));
-        hipLaunchKernelGGL(k_cam_gen, dim3(blocks_for(total)), dim3(kBlock), 0, stream, wv, ps, rays[0], queue[0]);
+        hipLaunchKernelGGL(k_cam_gen, dim3(blocks_for(total)), dim3(kBlock), 0, stream, wv, ps, rays[0], queue[0],
+                           counters);
         SGPU_HIP_TRY(hipGetLastError());
         SGPU_HIP_TRY(T.end());
-        count = L > 1 ? total : 0;  // a camera path of length 1 ends at the lens (VCM.cpp:585)
+        SGPU_HIP_TRY(hipMemcpyAsync(h, counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        SGPU_HIP_TRY(hipStreamSynchronize(stream));
+        count = h[0];
         cur = 0;
         while (count > 0) {
-            sgpu_rays_soa rs = as_soa(rays[cur]);
             SGPU_HIP_TRY(T.begin(kPhaseClosest));
-            int rc = sgpu_trace_launch(scene, &rs, &hsoa, nullptr, count, false, d_trav, stream, spill);
+            int rc = trace(rays[cur], &hsoa, nullptr, count, false, d_trav);
             if (rc) return rc;
             SGPU_HIP_TRY(T.end());
             n_closest += count;
@@ -1401,9 +1497,8 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
             SGPU_HIP_TRY(hipStreamSynchronize(stream));
             uint32_t n_next = h[0], n_sh = h[1];
             if (n_sh > 0) {
-                sgpu_rays_soa ss = as_soa(shadow);
                 SGPU_HIP_TRY(T.begin(kPhaseShadow));
-                rc = sgpu_trace_launch(scene, &ss, nullptr, occl, n_sh, true, nullptr, stream, spill);
+                rc = trace(shadow, nullptr, occl, n_sh, true, nullptr);
                 if (rc) return rc;
                 SGPU_HIP_TRY(T.end());
                 n_shadow += n_sh;
@@ -1420,17 +1515,27 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
             count = n_next;
             cur = 1 - cur;
         }
+        // each job's slices of this wavefront (consecutive, in iteration order) into its sum
         SGPU_HIP_TRY(T.begin(kPhaseCamera));
-        hipLaunchKernelGGL(k_frame_add, dim3(blocks_for(N)), dim3(kBlock), 0, stream, wv, 0u, ns, N, ps.color, frame,
-                           fb);
-        SGPU_HIP_TRY(hipGetLastError());
+        for (uint32_t k = 0; k < ns;) {
+            const uint32_t jb = sched[wf.first_slice + k].first;
+            uint32_t e = k + 1;
+            while (e < ns && sched[wf.first_slice + e].first == jb) ++e;
+            hipLaunchKernelGGL(k_frame_add, dim3(blocks_for(job_base[jb].N)), dim3(kBlock), 0, stream, wv, k, e - k,
+                               job_base[jb].N, ps.color, frame, framebuffers[jb]);
+            SGPU_HIP_TRY(hipGetLastError());
+            k = e;
+        }
         SGPU_HIP_TRY(T.end());
         SGPU_HIP_TRY(T.collect());
         SGPU_HIP_TRY(hipStreamSynchronize(stream));
     }
-    const size_t nf = 3 * (size_t)N;
-    hipLaunchKernelGGL(k_scale, dim3(blocks_for(nf)), dim3(kBlock), 0, stream, fb, nf, 1.0f / (float)job->iterations);
-    SGPU_HIP_TRY(hipGetLastError());
+    for (uint32_t j = 0; j < n_jobs; ++j) {
+        const size_t nf = 3 * (size_t)job_base[j].N;
+        hipLaunchKernelGGL(k_scale, dim3(blocks_for(nf)), dim3(kBlock), 0, stream, framebuffers[j], nf,
+                           1.0f / (float)jobs[j].iterations);
+        SGPU_HIP_TRY(hipGetLastError());
+    }
     SGPU_HIP_TRY(hipStreamSynchronize(stream));
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
@@ -1439,7 +1544,8 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
         stats->shadow_rays = n_shadow;
         stats->closest_launches = n_launches;
         stats->light_vertices = n_light_verts;
-        stats->samples = (uint64_t)N * job->iterations;
+        stats->samples = n_samples;
+        stats->wavefronts = waves.size();
         if (d_trav) {
             uint64_t tr[2] = {0, 0};
             SGPU_HIP_TRY(hipMemcpy(tr, d_trav, sizeof(tr), hipMemcpyDeviceToHost));
@@ -1454,4 +1560,11 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, s
         stats->merge_ms = (float)T.ms[kPhaseMerge];
     }
     return SGPU_OK;
+}
+
+int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, sgpu_vcm_stats* stats,
+                         hipStream_t stream)
+{
+    float* fbs[1] = {fb};
+    return sgpu_vcm_render_batch_impl(&scene, job, fbs, 1, job->flags, stats, stream);
 }

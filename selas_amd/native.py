@@ -82,6 +82,7 @@ _SIGNATURES = {
     "sgpu_scene_set_lights": [_P, _P, _U32, _FP],
     "sgpu_render_pt": [_P, _P, _P, _P],
     "sgpu_render_vcm": [_P, _P, _P, _P, _P],
+    "sgpu_render_vcm_batch": [_P, _P, _P, _U32, _U32, _P, _P],
     "sgpu_bsdf_eval": [_P, _P, _P, _P, _U32, _P, _P],
     "sgpu_bsdf_sample": [_P, _P, _P, _P, _U32, _U32, _P, _P],
     "sgpu_gen_cluster_soup": [_U64, _U32, _U32, _F, _F, _F, _FP],
@@ -393,6 +394,7 @@ def build_render_scene(ctx: Context, sc: dict) -> Scene:
 
 VCM_KERNEL_TIMES = 1
 VCM_COUNT_TRAVERSAL = 2
+VCM_ONE_SLICE_PER_WAVEFRONT = 4
 
 
 class VcmJob(C.Structure):
@@ -403,7 +405,8 @@ class VcmJob(C.Structure):
 
 class VcmStats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("rays", "closest_rays", "shadow_rays", "closest_launches",
-                                          "light_vertices", "samples", "nodes_visited", "tris_tested")] + \
+                                          "light_vertices", "samples", "wavefronts", "nodes_visited",
+                                          "tris_tested")] + \
                [(k, C.c_float) for k in ("closest_ms", "shadow_ms", "light_ms", "camera_ms", "grid_ms",
                                          "merge_ms")]
 
@@ -424,6 +427,39 @@ def render_vcm(scene: Scene, cam: dict, width, height, iterations, max_path_leng
     finally:
         if framebuffer is None:
             fb.free()
+    if return_stats:
+        return out, {k: getattr(st, k) for k, _ in VcmStats._fields_}
+    return out
+
+
+def render_vcm_batch(jobs, flags=0, stream=None, return_stats=False, keep_on_device=False):
+    """Renders a list of jobs with sgpu_render_vcm_batch. Each job is a dict: scene (Scene), camera
+    (dict), width, height, iterations, max_path_length, base_radius, radius_alpha, seed and optionally
+    iteration_base. Returns the images (height x width x 3 each) or, with keep_on_device, the device
+    framebuffers."""
+    n = len(jobs)
+    ctx = jobs[0]["scene"].ctx
+    arr_j = (VcmJob * n)()
+    scenes = (_P * n)()
+    fbs = [ctx.empty(3 * j["width"] * j["height"] * 4) for j in jobs]
+    fb_ptrs = (_P * n)(*[b.ptr for b in fbs])
+    for k, j in enumerate(jobs):
+        scenes[k] = j["scene"].handle
+        arr_j[k] = VcmJob(Camera.make(j["camera"], j["width"], j["height"]), j["iterations"],
+                          j.get("iteration_base", 0), j["max_path_length"], j["base_radius"], j["radius_alpha"], 0,
+                          j["seed"])
+    st = VcmStats()
+    try:
+        check(lib.sgpu_render_vcm_batch(scenes, arr_j, fb_ptrs, n, flags, C.byref(st), stream))
+        if keep_on_device:
+            out = fbs
+        else:
+            out = [b.download(np.float32, 3 * j["width"] * j["height"]).reshape(j["height"], j["width"], 3)
+                   for b, j in zip(fbs, jobs)]
+    finally:
+        if not keep_on_device:
+            for b in fbs:
+                b.free()
     if return_stats:
         return out, {k: getattr(st, k) for k, _ in VcmStats._fields_}
     return out

@@ -165,7 +165,7 @@ def test_vcm_errors(room, gpu_ctx):
         assert call(job(base_radius=0.0)) == N.SGPU_E_INVALID
         assert call(job(base_radius=float("nan"))) == N.SGPU_E_INVALID
         assert call(job(radius_alpha=1.5)) == N.SGPU_E_INVALID
-        assert call(job(flags=4)) == N.SGPU_E_INVALID
+        assert call(job(flags=8)) == N.SGPU_E_INVALID
         assert call(job(w=0)) == N.SGPU_E_INVALID
         assert N.lib.sgpu_render_vcm(scene.handle, None, fb.ptr, None, None) == N.SGPU_E_INVALID
         bare = N.Scene(gpu_ctx)
