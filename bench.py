@@ -203,9 +203,235 @@ def bench_vcm(args):
     print(json.dumps(out), flush=True)
 
 
+def _build_scene(N, W, ctx, seed, tris):
+    """A sized VCM workload scene on the device; the host arrays are dropped once it is built."""
+    spec = W.sized_vcm_scene(seed, tris)
+    n = sum(W.geom_tri9(g).shape[0] for g in spec["geoms"])
+    scene = N.build_render_scene(ctx, spec)
+    return scene, spec["camera"], float(spec["merge_radius"]), float(spec["radius_alpha"]), n
+
+
+def _frac_close(a, b):
+    return float((np.abs(a - b) <= 1e-3 * np.abs(b) + 1e-5).mean())
+
+
+def bench_batch(args):
+    """64 uneven VCM jobs, each on a scene of its own, rendered as one batch (sgpu_render_vcm_batch)
+    and, for comparison, one job at a time with one wavefront per iteration (the plain GPU
+    schedule) and one job at a time with its iterations packed (sgpu_render_vcm)."""
+    from selas_amd import native as N
+    from selas_amd import workloads as W
+
+    mix = W.job_mix(args.batch_seed, args.batch_jobs)
+    ctx = N.Context(0)
+    t0 = time.perf_counter()
+    jobs, tris = [], 0
+    for k, m in enumerate(mix):
+        scene, cam, radius, alpha, n = _build_scene(N, W, ctx, 10000 + k, m["scene_tris"])
+        tris += n
+        jobs.append({"scene": scene, "camera": cam, "width": m["width"], "height": m["height"],
+                     "iterations": m["iterations"], "max_path_length": m["max_path_length"], "base_radius": radius,
+                     "radius_alpha": alpha, "seed": m["seed"]})
+    log(f"batch: {len(jobs)} scenes, {tris} triangles, built in {time.perf_counter() - t0:.1f} s")
+    samples = sum(j["width"] * j["height"] * j["iterations"] for j in jobs)
+    fb_batch = [ctx.empty(12 * j["width"] * j["height"]) for j in jobs]
+    fb_plain = [ctx.empty(12 * j["width"] * j["height"]) for j in jobs]
+    # warm-up (untimed): one iteration of the first jobs
+    nw = min(len(jobs), 8)
+    N.render_vcm_batch([dict(j, iterations=1) for j in jobs[:nw]], framebuffers=fb_batch[:nw])
+    N.synchronize()
+
+    e0, e1 = N.Event(), N.Event()
+    e0.record()
+    st = N.render_vcm_batch(jobs, framebuffers=fb_batch, return_stats=True)[1]
+    e1.record()
+    N.synchronize()
+    batch_ms = e0.elapsed_ms(e1)
+    log(f"batch: {samples} samples in {batch_ms:.1f} ms, {st['wavefronts']} wavefronts")
+
+    def one_at_a_time(flags, fbs):
+        a, b = N.Event(), N.Event()
+        a.record()
+        rays = 0
+        for j, fb in zip(jobs, fbs):
+            _, s1 = N.render_vcm(j["scene"], j["camera"], j["width"], j["height"], j["iterations"],
+                                 j["max_path_length"], j["base_radius"], j["radius_alpha"], j["seed"],
+                                 return_stats=True, flags=flags, framebuffer=fb)
+            rays += s1["rays"]
+        b.record()
+        N.synchronize()
+        return a.elapsed_ms(b), rays
+
+    plain_ms, plain_rays = one_at_a_time(N.VCM_ONE_SLICE_PER_WAVEFRONT, fb_plain)
+    log(f"plain: {plain_ms:.1f} ms")
+    solo_ms = None
+    if not args.no_solo:
+        solo_ms, _ = one_at_a_time(0, fb_plain)
+        log(f"one job at a time, iterations packed: {solo_ms:.1f} ms")
+    close = [_frac_close(a.download(np.float32, 3 * j["width"] * j["height"]),
+                         b.download(np.float32, 3 * j["width"] * j["height"]))
+             for a, b, j in zip(fb_batch, fb_plain, jobs)]
+    for b in fb_batch + fb_plain:
+        b.free()
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_batch(mix, args.cpu_iterations, min(16, os.cpu_count() or 1))
+    value = samples / (batch_ms / 1e3)
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": args.warmup,
+        "ms_per_step": round(batch_ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"vcm_batch_{len(jobs)}jobs",
+            "jobs": len(jobs),
+            "resolution": "square, side log-uniform in [64, 1024]",
+            "iterations": "uniform in 1..64",
+            "scenes": "one per job, 64 displaced-grid quad meshes, triangles log-uniform in 10K..2M, textured",
+            "triangles_total": int(tris),
+            "samples": int(samples),
+            "max_path_length": 10,
+        },
+        "mrays_s": round(st["rays"] / (batch_ms / 1e3) / 1e6, 2),
+        "wavefronts": st["wavefronts"],
+        "plain_gpu": {"samples_per_s": round(samples / (plain_ms / 1e3), 1), "ms": round(plain_ms, 1),
+                      "mrays_s": round(plain_rays / (plain_ms / 1e3) / 1e6, 2),
+                      "schedule": "one job at a time, one wavefront per iteration"},
+        "vs_plain_gpu": round(plain_ms / batch_ms, 3),
+        "solo_gpu": None if solo_ms is None else {
+            "samples_per_s": round(samples / (solo_ms / 1e3), 1), "ms": round(solo_ms, 1),
+            "schedule": "one job at a time, its iterations packed into wavefronts (sgpu_render_vcm)"},
+        "same_images_min_frac": round(min(close), 5),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_batch(mix, iterations, threads):
+    """The oracle on one job of the mix (the median scene size), `iterations` iterations."""
+    import oracle as O
+    from selas_amd import workloads as W
+
+    k = int(np.argsort([m["scene_tris"] for m in mix])[len(mix) // 2])
+    m = mix[k]
+    spec = W.sized_vcm_scene(10000 + k, m["scene_tris"])
+    rs = O.RenderScene(spec)
+    w, h = m["width"], m["height"]
+    t0 = time.perf_counter()
+    O.render_vcm(rs, spec["camera"], w, h, iterations, m["max_path_length"], spec["merge_radius"],
+                 spec["radius_alpha"], m["seed"], threads=threads)
+    secs = time.perf_counter() - t0
+    return {"value": round(w * h * iterations / secs, 1), "unit": "samples/s", "cores": threads, "kind": "port",
+            "sample": f"job {k} of the mix ({w}x{h}, {iterations} iterations, {secs:.2f} s), oracle "
+                      f"restatement (scalar C, oracle SAH tree, not Embree)"}
+
+
+def serve_sweep(N, W, ctx, rates, n_jobs, seed, n_scenes=16, plain=True, budget=200 << 30):
+    """Poisson request streams at each rate, served by the request loop over resident scenes,
+    and by the plain one-job-at-a-time GPU schedule on the same requests."""
+    from selas_amd import serving as S
+
+    sizes = W.scene_sizes(seed, n_scenes)
+    t0 = time.perf_counter()
+    scenes = {}
+    for sid in range(n_scenes):
+        sc, cam, radius, alpha, _ = _build_scene(N, W, ctx, seed * 1000 + sid, sizes[sid])
+        scenes[sid] = (sc, cam, radius, alpha)
+    log(f"serve: {n_scenes} resident scenes ({int(sizes.sum())} triangles) built in {time.perf_counter() - t0:.1f} s")
+    def no_build(sid):
+        raise RuntimeError(f"scene {sid} is not resident")
+
+    served = S.GpuEngine(ctx, no_build, scenes=scenes)
+    base = S.GpuEngine(ctx, no_build, scenes=scenes, plain=True) if plain else None
+    out = []
+    try:
+        # warm-up: one small request through each engine
+        warm = [S.Request(0, 0.0, 0, 64, 64, 1, 1)]
+        S.RequestLoop(served, budget).run(warm)
+        if base:
+            S.RequestLoop(base, budget, max_slots=1).run([S.Request(0, 0.0, 0, 64, 64, 1, 1)])
+        for rate in rates:
+            mix = W.job_mix(seed * 7919 + int(rate * 10), n_jobs, n_scenes=n_scenes)
+            reqs = S.poisson_requests(seed, rate, mix)
+            loop = S.RequestLoop(served, budget, max_slots=64)
+            loop.run(reqs)
+            res = {"rate": rate, "served": S.summarize(reqs, loop.makespan)}
+            res["served"].update(steps=loop.steps, peak_in_flight=loop.peak_in_flight,
+                                 peak_hbm_bytes=int(loop.peak_used + served.fixed_bytes))
+            if base:
+                reqs_b = S.poisson_requests(seed, rate, mix)
+                lb = S.RequestLoop(base, budget, max_slots=1)
+                lb.run(reqs_b)
+                res["plain"] = S.summarize(reqs_b, lb.makespan)
+            log(f"serve rate {rate}/s: " + json.dumps({k: v for k, v in res.items() if k != "rate"}))
+            out.append(res)
+    finally:
+        served.close()
+        if base:
+            base.close()
+        for sc, *_ in scenes.values():
+            sc.release()
+    return out
+
+
+def _round(d):
+    return {k: (round(v, 2) if isinstance(v, float) else v) for k, v in d.items()}
+
+
+def bench_serve(args):
+    from selas_amd import native as N
+    from selas_amd import workloads as W
+
+    rates = [float(x) for x in args.rates.split(",")]
+    ctx = N.Context(0)
+    res = serve_sweep(N, W, ctx, rates, args.serve_jobs, args.batch_seed, plain=not args.no_plain)
+    top = max(res, key=lambda r: r["served"]["samples_per_s"])
+    out = {
+        "metric": METRIC,
+        "value": round(top["served"]["samples_per_s"], 1),
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": len(rates),
+        "warmup": 1,
+        "ms_per_step": round(1e3 * sum(r["served"]["makespan_s"] for r in res) / len(res), 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"vcm_serve_poisson_{len(rates)}rates_{args.serve_jobs}jobs",
+            "rates_per_s": rates,
+            "jobs_per_rate": args.serve_jobs,
+            "scenes": "16 resident, 64 displaced-grid quad meshes each, triangles log-uniform in 10K..2M, textured",
+            "resolution": "square, side log-uniform in [64, 1024]",
+            "iterations": "uniform in 1..64",
+            "admission": "arrival order, 64 slots, HBM budget 200 GiB (scenes + framebuffers)",
+            "schedule": "continuous batching, wavefronts of 2^20 paths, least remaining work first",
+        },
+        "sweep": [{"rate": r["rate"], "served": _round(r["served"]),
+                   "plain": _round(r["plain"]) if "plain" in r else None} for r in res],
+        "p50_ms": round(top["served"]["p50_ms"], 1),
+        "p99_ms": round(top["served"]["p99_ms"], 1),
+        "vs_plain_gpu": (round(top["served"]["samples_per_s"] / top["plain"]["samples_per_s"], 3)
+                         if "plain" in top else None),
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=("vcm", "traversal"), default="vcm")
+    ap.add_argument("--workload", choices=("vcm", "traversal", "batch", "serve"), default="vcm")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (vcm: 32 iterations, traversal: 10)")
     ap.add_argument("--warmup", type=int, default=2)
@@ -213,6 +439,12 @@ def main():
     ap.add_argument("--cpu-iterations", type=int, default=2, help="vcm: iterations in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-rays", type=int, default=4194304, help="rays in the CPU baseline sample")
+    ap.add_argument("--batch-jobs", type=int, default=64, help="batch: jobs in the mix; serve: -")
+    ap.add_argument("--batch-seed", type=int, default=7, help="batch/serve: seed of the job mix and scenes")
+    ap.add_argument("--no-solo", action="store_true", help="batch: skip the packed one-job-at-a-time leg")
+    ap.add_argument("--rates", default="1,2,5,10,20,50,100,200", help="serve: Poisson arrival rates (jobs/s)")
+    ap.add_argument("--serve-jobs", type=int, default=24, help="serve: requests per rate")
+    ap.add_argument("--no-plain", action="store_true", help="serve: skip the plain one-job-at-a-time runs")
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the triangles (1.0 = the workload)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run sgpu_device_copy of BYTES (HBM counter calibration) and exit")
@@ -237,6 +469,10 @@ def main():
         sys.exit(2)
     if args.workload == "vcm":
         bench_vcm(args)
+    elif args.workload == "batch":
+        bench_batch(args)
+    elif args.workload == "serve":
+        bench_serve(args)
     else:
         if args.steps is None:
             args.steps = 10

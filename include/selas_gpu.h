@@ -138,6 +138,9 @@ int sgpu_scene_commit(sgpu_scene scene, void* stream);
 int sgpu_scene_release(sgpu_scene scene);
 /* Number of triangles, BVH nodes and maximum leaf depth of a committed scene. */
 int sgpu_scene_info(sgpu_scene scene, uint32_t* n_tris, uint32_t* n_nodes, uint32_t* max_depth);
+/* Device memory the scene holds (tree, triangles, shading records, lights, textures), in bytes:
+ * what keeping it resident costs. No counterpart in Selas (its scenes live in host memory). */
+int sgpu_scene_device_bytes(sgpu_scene scene, uint64_t* bytes);
 /* Synchronous health check after queries: SGPU_E_RANGE if any traversal on this scene ran out of
  * stack (the stack is sized from the tree depth, so this is never expected), else 0. */
 int sgpu_scene_check(sgpu_scene scene);
@@ -251,6 +254,9 @@ int sgpu_render_pt(sgpu_scene scene, const sgpu_pt_job* job, float* framebuffer,
 #define SGPU_VCM_COUNT_TRAVERSAL 2u /* count nodes and triangles visited by closest-hit rays (slower) */
 #define SGPU_VCM_ONE_SLICE_PER_WAVEFRONT 4u /* run each iteration of each job as its own wavefront (the
                                                un-batched schedule, for comparisons) */
+#define SGPU_VCM_SHORTEST_FIRST 8u /* servers only: fill each wavefront with the admitted jobs of least
+                                      remaining work (paths x iterations left) first, ties in admission
+                                      order, instead of in admission order */
 
 typedef struct sgpu_vcm_job {
     sgpu_camera camera;
@@ -298,6 +304,39 @@ int sgpu_render_vcm(sgpu_scene scene, const sgpu_vcm_job* job, float* framebuffe
  * own flags must be 0); stats (may be NULL) sums the batch. Synchronous on `stream`. */
 int sgpu_render_vcm_batch(const sgpu_scene* scenes, const sgpu_vcm_job* jobs, float* const* framebuffers,
                           uint32_t n_jobs, uint32_t flags, sgpu_vcm_stats* stats, void* stream);
+
+/* A persistent VCM executor for continuous batching (request loops): wavefront buffers are sized
+ * once; jobs are admitted at any time (sgpu_vcm_server_submit) and every sgpu_vcm_server_step runs
+ * one wavefront over the next iterations of the admitted jobs in admission order (the oldest job's
+ * iterations first, then the next job's, up to wavefront_paths paths; with SGPU_VCM_SHORTEST_FIRST
+ * the jobs of least remaining work first), retiring the jobs whose last iteration ran. A job's framebuffer (device, height x width x 3 floats, owned by the caller and
+ * untouched by anything else until the job is retired) then holds exactly what sgpu_render_vcm
+ * writes for it. Scenes must stay alive while jobs on them are admitted. Calls on one server must
+ * not overlap; submit and step on the same stream, or order the streams. This has no counterpart
+ * in Selas (its renders are one job at a time); sgpu_render_vcm_batch is built on it. */
+typedef struct sgpu_vcm_server_s* sgpu_vcm_server;
+
+typedef struct sgpu_vcm_server_config {
+    uint64_t wavefront_paths;  /* paths per wavefront; also the largest admissible image (W x H) */
+    uint32_t max_path_length;  /* largest path length of the admitted jobs, 1..31 */
+    uint32_t flags;            /* SGPU_VCM_* */
+} sgpu_vcm_server_config;
+
+int sgpu_vcm_server_create(sgpu_context ctx, const sgpu_vcm_server_config* config, void* stream,
+                           sgpu_vcm_server* out);
+/* Admits a job (its flags must be 0); *job_id_out (may be NULL) receives its id. */
+int sgpu_vcm_server_submit(sgpu_vcm_server server, sgpu_scene scene, const sgpu_vcm_job* job, float* framebuffer,
+                           uint64_t* job_id_out, void* stream);
+/* Runs one wavefront (nothing when no job is admitted); writes up to max_done ids of the jobs it
+ * retired to done_ids, their count to *n_done and the number still admitted to *n_active (any of
+ * the three may be NULL). Synchronous on `stream`. */
+int sgpu_vcm_server_step(sgpu_vcm_server server, void* stream, uint64_t* done_ids, uint32_t max_done,
+                         uint32_t* n_done, uint32_t* n_active);
+/* Totals since creation. */
+int sgpu_vcm_server_stats(sgpu_vcm_server server, sgpu_vcm_stats* stats);
+/* Device bytes a server with this configuration allocates (admission planning). */
+int sgpu_vcm_server_footprint(sgpu_context ctx, const sgpu_vcm_server_config* config, uint64_t* bytes);
+int sgpu_vcm_server_destroy(sgpu_vcm_server server, void* stream);
 
 /* ---- procedural inputs (host side, deterministic from a seed) ------------------------------
  * Gaussian-cluster triangle soup: n_clusters x tris_per_cluster triangles. Cluster centres are

@@ -520,6 +520,67 @@ int sgpu_render_vcm_batch(const sgpu_scene* scenes, const sgpu_vcm_job* jobs, fl
     return sgpu_vcm_render_batch_impl(scenes, jobs, framebuffers, n_jobs, flags, stats, as_stream(stream));
 }
 
+static int check_server_config(const sgpu_vcm_server_config* cfg, const char* who)
+{
+    if (!cfg) return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": NULL config");
+    if (cfg->wavefront_paths == 0 || cfg->wavefront_paths > (1ull << 26))
+        return sgpu_fail(SGPU_E_RANGE, std::string(who) + ": wavefront_paths must lie in 1..2^26");
+    if (cfg->max_path_length == 0 || cfg->max_path_length > 31)
+        return sgpu_fail(SGPU_E_RANGE, std::string(who) + ": max_path_length must lie in 1..31");
+    if (cfg->flags & ~(SGPU_VCM_KERNEL_TIMES | SGPU_VCM_COUNT_TRAVERSAL | SGPU_VCM_ONE_SLICE_PER_WAVEFRONT |
+                       SGPU_VCM_SHORTEST_FIRST))
+        return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": unknown flags");
+    return SGPU_OK;
+}
+
+int sgpu_vcm_server_create(sgpu_context ctx, const sgpu_vcm_server_config* config, void* stream,
+                           sgpu_vcm_server* out)
+{
+    if (!ctx || !out) return sgpu_fail(SGPU_E_INVALID, "sgpu_vcm_server_create: NULL argument");
+    int rc = check_server_config(config, "sgpu_vcm_server_create");
+    if (rc) return rc;
+    SGPU_HIP_TRY(hipSetDevice(ctx->device));
+    return sgpu_vcm_server_create_impl(ctx, config, as_stream(stream), out);
+}
+
+int sgpu_vcm_server_submit(sgpu_vcm_server server, sgpu_scene scene, const sgpu_vcm_job* job, float* framebuffer,
+                           uint64_t* job_id_out, void* stream)
+{
+    if (!server) return sgpu_fail(SGPU_E_INVALID, "sgpu_vcm_server_submit: NULL server");
+    int rc = check_vcm_job(scene, job, framebuffer, "sgpu_vcm_server_submit");
+    if (rc) return rc;
+    if (job->flags != 0) return sgpu_fail(SGPU_E_INVALID, "sgpu_vcm_server_submit: job flags must be 0");
+    SGPU_HIP_TRY(hipSetDevice(scene->ctx->device));
+    return sgpu_vcm_server_submit_impl(server, scene, job, framebuffer, job_id_out, as_stream(stream));
+}
+
+int sgpu_vcm_server_step(sgpu_vcm_server server, void* stream, uint64_t* done_ids, uint32_t max_done,
+                         uint32_t* n_done, uint32_t* n_active)
+{
+    if (!server) return sgpu_fail(SGPU_E_INVALID, "sgpu_vcm_server_step: NULL server");
+    return sgpu_vcm_server_step_impl(server, as_stream(stream), done_ids, max_done, n_done, n_active);
+}
+
+int sgpu_vcm_server_stats(sgpu_vcm_server server, sgpu_vcm_stats* stats)
+{
+    if (!server || !stats) return sgpu_fail(SGPU_E_INVALID, "sgpu_vcm_server_stats: NULL argument");
+    return sgpu_vcm_server_stats_impl(server, stats);
+}
+
+int sgpu_vcm_server_footprint(sgpu_context ctx, const sgpu_vcm_server_config* config, uint64_t* bytes)
+{
+    if (!ctx || !bytes) return sgpu_fail(SGPU_E_INVALID, "sgpu_vcm_server_footprint: NULL argument");
+    int rc = check_server_config(config, "sgpu_vcm_server_footprint");
+    if (rc) return rc;
+    *bytes = sgpu_vcm_server_bytes(ctx, config);
+    return SGPU_OK;
+}
+
+int sgpu_vcm_server_destroy(sgpu_vcm_server server, void* stream)
+{
+    return sgpu_vcm_server_destroy_impl(server, as_stream(stream));
+}
+
 int sgpu_scene_info(sgpu_scene scene, uint32_t* n_tris, uint32_t* n_nodes, uint32_t* max_depth)
 {
     if (!scene) return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_info: NULL scene");
@@ -527,6 +588,20 @@ int sgpu_scene_info(sgpu_scene scene, uint32_t* n_tris, uint32_t* n_nodes, uint3
     if (n_tris) *n_tris = scene->n_tris;
     if (n_nodes) *n_nodes = scene->n_nodes;
     if (max_depth) *max_depth = scene->max_depth;
+    return SGPU_OK;
+}
+
+int sgpu_scene_device_bytes(sgpu_scene scene, uint64_t* bytes)
+{
+    if (!scene || !bytes) return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_device_bytes: NULL argument");
+    if (!scene->committed) return sgpu_fail(SGPU_E_STATE, "sgpu_scene_device_bytes: scene not committed");
+    uint64_t b = (uint64_t)scene->n_nodes * sizeof(BvhNode) + (uint64_t)scene->n_tris * sizeof(TriRecord) +
+                 sizeof(uint32_t) + (uint64_t)scene->n_materials * 22 * sizeof(float) +
+                 (uint64_t)scene->n_geom_material * sizeof(uint32_t) + (uint64_t)scene->n_lights * sizeof(sgpu_light) +
+                 (uint64_t)scene->n_geom_tex * sizeof(sgpu::PtexTex);
+    for (const sgpu::PtexTex& t : scene->tex_host)
+        if (t.texels) b += (uint64_t)t.nfaces * (3ull << (2 * t.res_log2)) + (uint64_t)t.nfaces * 8 * sizeof(int32_t);
+    *bytes = b;
     return SGPU_OK;
 }
 

@@ -117,3 +117,12 @@ int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* frame
                          hipStream_t stream);
 int sgpu_vcm_render_batch_impl(const sgpu_scene* scenes, const sgpu_vcm_job* jobs, float* const* framebuffers,
                                uint32_t n_jobs, uint32_t flags, sgpu_vcm_stats* stats, hipStream_t stream);
+int sgpu_vcm_server_create_impl(sgpu_context ctx, const sgpu_vcm_server_config* cfg, hipStream_t stream,
+                                sgpu_vcm_server* out);
+int sgpu_vcm_server_destroy_impl(sgpu_vcm_server sv, hipStream_t stream);
+int sgpu_vcm_server_submit_impl(sgpu_vcm_server sv, sgpu_scene scene, const sgpu_vcm_job* job, float* fb,
+                                uint64_t* id_out, hipStream_t stream);
+int sgpu_vcm_server_step_impl(sgpu_vcm_server sv, hipStream_t stream, uint64_t* done_ids, uint32_t max_done,
+                              uint32_t* n_done, uint32_t* n_active);
+int sgpu_vcm_server_stats_impl(sgpu_vcm_server sv, sgpu_vcm_stats* st);
+uint64_t sgpu_vcm_server_bytes(sgpu_context ctx, const sgpu_vcm_server_config* cfg);

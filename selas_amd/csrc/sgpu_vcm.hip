@@ -1038,15 +1038,21 @@ __global__ __launch_bounds__(kBlock) void k_scale(float* __restrict fb, size_t n
     if (i < n) fb[i] = fb[i] * s;
 }
 
+// Stream-ordered device allocations freed together; a dry-run arena only adds up their sizes.
 struct Arena {
     std::vector<void*> ptrs;
-    hipStream_t stream;
+    hipStream_t stream = nullptr;
     hipError_t err = hipSuccess;
+    bool dry = false;
+    uint64_t bytes = 0;
     template <typename T>
     T* get(size_t count)
     {
+        const size_t n = count * sizeof(T) + 64;
+        bytes += n;
+        if (dry) return reinterpret_cast<T*>((uintptr_t)256);
         void* p = nullptr;
-        if (err == hipSuccess) err = hipMallocAsync(&p, count * sizeof(T) + 64, stream);
+        if (err == hipSuccess) err = hipMallocAsync(&p, n, stream);
         if (err == hipSuccess) ptrs.push_back(p);
         return (T*)p;
     }
@@ -1123,109 +1129,74 @@ sgpu_rays_soa as_soa(const RayBuf& r) { return sgpu_rays_soa{r.ox, r.oy, r.oz, r
 
 inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
 
-// Paths per wavefront the render aims for: enough lanes to fill the chip in every bounce's launch.
-constexpr uint64_t kTargetPaths = 1u << 20;
-
-// One job of a render call, validated by the C ABI.
-struct JobIn {
+// One admitted job of a server.
+struct ServerJob {
+    uint64_t id;
     sgpu_scene scene;
-    const sgpu_vcm_job* job;
+    sgpu_vcm_job job;
     float* fb;
-};
-
-struct Wavefront {
-    uint32_t first_slice, n_slices, total;
+    Slice base;      // everything but the per-iteration values
+    uint32_t next;   // next iteration offset to schedule
 };
 
 }  // namespace
 
-int sgpu_vcm_render_batch_impl(const sgpu_scene* scenes, const sgpu_vcm_job* jobs, float* const* framebuffers,
-                               uint32_t n_jobs, uint32_t flags, sgpu_vcm_stats* stats, hipStream_t stream)
-{
-    const float pi = 3.141592653589793f;
-    const bool kernel_times = flags & SGPU_VCM_KERNEL_TIMES;
-    const bool count_traversal = flags & SGPU_VCM_COUNT_TRAVERSAL;
-    const bool one_slice = flags & SGPU_VCM_ONE_SLICE_PER_WAVEFRONT;
-    sgpu_context ctx = scenes[0]->ctx;
-
-    // ---- scenes of the batch: one traversal table entry each ----
-    std::vector<sgpu_scene> uniq;
-    std::vector<uint32_t> job_scene(n_jobs);
-    uint32_t max_depth = 0;
-    for (uint32_t j = 0; j < n_jobs; ++j) {
-        auto it = std::find(uniq.begin(), uniq.end(), scenes[j]);
-        job_scene[j] = (uint32_t)(it - uniq.begin());
-        if (it == uniq.end()) uniq.push_back(scenes[j]);
-        max_depth = std::max(max_depth, scenes[j]->max_depth);
-    }
-    const bool multi = uniq.size() > 1;
-
-    // ---- per-job slice template and the slice schedule ----
-    std::vector<Slice> job_base(n_jobs);
-    uint32_t mlv = 1;
-    uint64_t total_iterations = 0;
-    for (uint32_t j = 0; j < n_jobs; ++j) {
-        const sgpu_vcm_job* jb = &jobs[j];
-        const sgpu_scene sc = scenes[j];
-        const uint32_t W = jb->camera.width, H = jb->camera.height, L = jb->max_path_length;
-        Slice b;
-        std::memset(&b, 0, sizeof(b));
-        b.cam = camera_init(mk3(jb->camera.position[0], jb->camera.position[1], jb->camera.position[2]),
-                            mk3(jb->camera.look_at[0], jb->camera.look_at[1], jb->camera.look_at[2]),
-                            mk3(jb->camera.up[0], jb->camera.up[1], jb->camera.up[2]), jb->camera.fov_degrees, W, H);
-        float hfov = jb->camera.fov_degrees * (pi / 180.f);
-        b.vipd = (float)W / (2.0f * tanf(hfov * 0.5f));
-        b.W = W;
-        b.H = H;
-        b.N = W * H;
-        b.L = L;
-        b.mlv = L > 2 ? L - 2 : 1;
-        b.seed = jb->seed;
-        b.job = j;
-        b.scene = job_scene[j];
-        b.materials = sc->d_materials;
-        b.geom_material = sc->d_geom_material;
-        b.n_geoms = sc->n_geom_material;
-        b.lights = (const Light*)sc->d_lights;
-        b.n_lights = sc->n_lights;
-        b.background = mk3(sc->background[0], sc->background[1], sc->background[2]);
-        b.geom_tex = sc->d_geom_tex;
-        b.n_geom_tex = sc->n_geom_tex;
-        job_base[j] = b;
-        mlv = std::max(mlv, b.mlv);
-        total_iterations += jb->iterations;
-    }
-    // slices in job order, iterations in order; wavefronts take consecutive slices up to
-    // kTargetPaths paths (a larger slice takes one alone)
-    std::vector<std::pair<uint32_t, uint32_t>> sched;  // (job, iteration offset)
-    sched.reserve(total_iterations);
-    for (uint32_t j = 0; j < n_jobs; ++j)
-        for (uint32_t it = 0; it < jobs[j].iterations; ++it) sched.push_back({j, it});
-    std::vector<Wavefront> waves;
-    uint64_t TM64 = 0;
-    uint32_t max_slices = 1;
-    for (size_t k = 0; k < sched.size();) {
-        Wavefront wf{(uint32_t)k, 0, 0};
-        uint64_t total = 0;
-        while (k < sched.size()) {
-            const uint64_t n = job_base[sched[k].first].N;
-            if (wf.n_slices > 0 && (one_slice || total + n > kTargetPaths)) break;
-            total += n;
-            ++wf.n_slices;
-            ++k;
-        }
-        if (total >= (1ull << 31)) return sgpu_fail(SGPU_E_RANGE, "sgpu_render_vcm: wavefront too large");
-        wf.total = (uint32_t)total;
-        waves.push_back(wf);
-        TM64 = std::max<uint64_t>(TM64, total);
-        max_slices = std::max(max_slices, wf.n_slices);
-    }
-    const uint32_t TM = (uint32_t)TM64;
-    const uint32_t nslots = mlv + 2;
-
+// A persistent VCM executor: wavefront buffers sized once, jobs admitted at any time and advanced
+// wavefront by wavefront (see the file comment for what a slice and a wavefront are).
+struct sgpu_vcm_server_s {
+    sgpu_context ctx = nullptr;
+    sgpu_vcm_server_config cfg;
+    hipStream_t alloc_stream = nullptr;
+    uint32_t TM = 0, mlv = 0, nslots = 0, max_slices = 0;
+    unsigned key_bits = 1;
     Arena A;
-    A.stream = stream;
+    // wavefront buffers
     PathState ps;
+    RayBuf rays[2], shadow;
+    uint32_t* queue[2] = {nullptr, nullptr};
+    HitBuf hb;
+    uint32_t *occl = nullptr, *splat_pix = nullptr, *vcount = nullptr, *voffset = nullptr, *path_end = nullptr;
+    float *splat_c = nullptr, *cpos = nullptr, *frame = nullptr;
+    LightVertex *verts = nullptr, *cverts = nullptr, *sverts = nullptr;
+    float4* spos = nullptr;
+    uint32_t *cslice = nullptr, *keys = nullptr, *keys2 = nullptr, *vals = nullptr, *vals2 = nullptr;
+    uint32_t *cell_counts = nullptr, *cell_ends = nullptr, *box = nullptr, *counters = nullptr, *nv_dev = nullptr;
+    Slice* d_slices = nullptr;
+    CamVertex* cam_vm = nullptr;
+    uint32_t* spill = nullptr;
+    uint32_t spill_depth_cap = 0;
+    CamSlots S;
+    void* tmp = nullptr;
+    size_t tmp_bytes = 0;
+    uint64_t* d_trav = nullptr;
+    uint8_t* hbuf = nullptr;  // pinned: counters, slice table, scene table
+    uint32_t* h = nullptr;
+    Slice* h_slices = nullptr;
+    TraceScene* h_table = nullptr;
+    // scenes seen by the server (traversal table), jobs in admission order
+    std::vector<sgpu_scene> scenes;
+    TraceScene* d_table = nullptr;
+    uint32_t table_cap = 0;
+    bool table_dirty = false;
+    uint32_t max_depth = 0;
+    std::vector<ServerJob> active;
+    uint64_t next_id = 1;
+    // totals
+    uint64_t n_closest = 0, n_shadow = 0, n_light_verts = 0, n_launches = 0, n_samples = 0, n_waves = 0;
+    uint64_t trav_nodes = 0, trav_tris = 0;
+    double ms[kPhases] = {};
+};
+
+namespace {
+
+constexpr uint32_t kMaxSlicesPerWave = 4096;
+constexpr uint32_t kMaxSceneDepth = 64;  // stack spill is sized for trees up to this depth
+
+int server_alloc(sgpu_vcm_server sv)
+{
+    Arena& A = sv->A;
+    const uint32_t TM = sv->TM, mlv = sv->mlv, nslots = sv->nslots;
+    PathState& ps = sv->ps;
     ps.pos = A.get<float>(3 * (size_t)TM);
     ps.dir = A.get<float>(3 * (size_t)TM);
     ps.thr = A.get<float>(3 * (size_t)TM);
@@ -1236,9 +1207,11 @@ int sgpu_vcm_render_batch_impl(const sgpu_scene* scenes, const sgpu_vcm_job* job
     ps.slice = A.get<uint32_t>(TM);
     ps.rng = A.get<uint64_t>(TM);
     ps.color = A.get<float>(3 * (size_t)TM);
-    RayBuf rays[2] = {A.rays(TM, multi), A.rays(TM, multi)};
-    uint32_t* queue[2] = {A.get<uint32_t>(TM), A.get<uint32_t>(TM)};
-    HitBuf hb;
+    sv->rays[0] = A.rays(TM, true);
+    sv->rays[1] = A.rays(TM, true);
+    sv->queue[0] = A.get<uint32_t>(TM);
+    sv->queue[1] = A.get<uint32_t>(TM);
+    HitBuf& hb = sv->hb;
     hb.t = A.get<float>(TM);
     hb.u = A.get<float>(TM);
     hb.v = A.get<float>(TM);
@@ -1249,101 +1222,223 @@ int sgpu_vcm_render_batch_impl(const sgpu_scene* scenes, const sgpu_vcm_job* job
     hb.geom = A.get<uint32_t>(TM);
     const size_t max_shadow = (size_t)TM * (mlv + 1);
     const size_t max_verts = (size_t)TM * mlv;
-    RayBuf shadow = A.rays(max_shadow, multi);
-    uint32_t* occl = A.get<uint32_t>(max_shadow);
-    float* splat_c = A.get<float>(3 * (size_t)TM);
-    uint32_t* splat_pix = A.get<uint32_t>(TM);
-    LightVertex* verts = A.get<LightVertex>(max_verts);
-    LightVertex* cverts = A.get<LightVertex>(max_verts);
-    LightVertex* sverts = A.get<LightVertex>(max_verts);
-    float4* spos = A.get<float4>(max_verts);
-    float* cpos = A.get<float>(3 * max_verts);
-    uint32_t* cslice = A.get<uint32_t>(max_verts);
-    uint32_t* vcount = A.get<uint32_t>((size_t)TM + 1);
-    uint32_t* voffset = A.get<uint32_t>((size_t)TM + 1);
-    uint32_t* path_end = A.get<uint32_t>(TM);
-    uint32_t* keys = A.get<uint32_t>(max_verts);
-    uint32_t* keys2 = A.get<uint32_t>(max_verts);
-    uint32_t* vals = A.get<uint32_t>(max_verts);
-    uint32_t* vals2 = A.get<uint32_t>(max_verts);
-    uint32_t* cell_counts = A.get<uint32_t>(TM);
-    uint32_t* cell_ends = A.get<uint32_t>(TM);
-    uint32_t* box = A.get<uint32_t>(8 * (size_t)max_slices);
-    uint32_t* counters = A.get<uint32_t>(4);
-    uint32_t* nv_dev = A.get<uint32_t>(1);
-    float* frame = A.get<float>(3 * (size_t)TM);
-    Slice* d_slices = A.get<Slice>(max_slices);
-    CamVertex* cam_vm = A.get<CamVertex>(TM);
-    uint32_t* spill = A.get<uint32_t>(sgpu_trace_spill_words(ctx, max_depth));
-    TraceScene* d_table = multi ? A.get<TraceScene>(uniq.size()) : nullptr;
-    CamSlots S;
-    S.c = A.get<float>(3 * (size_t)TM * nslots);
-    S.valid = A.get<uint32_t>((size_t)TM * nslots);
-    S.ray_slot = A.get<uint32_t>(max_shadow);
-    if (A.err != hipSuccess) return sgpu_fail(SGPU_E_NOMEM, "sgpu_render_vcm: device allocation failed");
-
+    sv->shadow = A.rays(max_shadow, true);
+    sv->occl = A.get<uint32_t>(max_shadow);
+    sv->splat_c = A.get<float>(3 * (size_t)TM);
+    sv->splat_pix = A.get<uint32_t>(TM);
+    sv->verts = A.get<LightVertex>(max_verts);
+    sv->cverts = A.get<LightVertex>(max_verts);
+    sv->sverts = A.get<LightVertex>(max_verts);
+    sv->spos = A.get<float4>(max_verts);
+    sv->cpos = A.get<float>(3 * max_verts);
+    sv->cslice = A.get<uint32_t>(max_verts);
+    sv->vcount = A.get<uint32_t>((size_t)TM + 1);
+    sv->voffset = A.get<uint32_t>((size_t)TM + 1);
+    sv->path_end = A.get<uint32_t>(TM);
+    sv->keys = A.get<uint32_t>(max_verts);
+    sv->keys2 = A.get<uint32_t>(max_verts);
+    sv->vals = A.get<uint32_t>(max_verts);
+    sv->vals2 = A.get<uint32_t>(max_verts);
+    sv->cell_counts = A.get<uint32_t>(TM);
+    sv->cell_ends = A.get<uint32_t>(TM);
+    sv->box = A.get<uint32_t>(8 * (size_t)sv->max_slices);
+    sv->counters = A.get<uint32_t>(4);
+    sv->nv_dev = A.get<uint32_t>(1);
+    sv->frame = A.get<float>(3 * (size_t)TM);
+    sv->d_slices = A.get<Slice>(sv->max_slices);
+    sv->cam_vm = A.get<CamVertex>(TM);
+    sv->d_trav = A.get<uint64_t>(2);
+    sv->S.c = A.get<float>(3 * (size_t)TM * nslots);
+    sv->S.valid = A.get<uint32_t>((size_t)TM * nslots);
+    sv->S.ray_slot = A.get<uint32_t>(max_shadow);
+    if (A.err != hipSuccess) return sgpu_fail(SGPU_E_NOMEM, "sgpu_vcm_server: device allocation failed");
     // cell keys are below the wavefront's path count: sort only the bits that can be set
-    unsigned key_bits = 1;
-    while (key_bits < 32 && (TM - 1) >> key_bits) ++key_bits;
+    sv->key_bits = 1;
+    while (sv->key_bits < 32 && (TM - 1) >> sv->key_bits) ++sv->key_bits;
     size_t sort_bytes = 0, scan_bytes = 0, scan2_bytes = 0;
-    SGPU_HIP_TRY(rocprim::radix_sort_pairs(nullptr, sort_bytes, keys, keys2, vals, vals2, max_verts, 0, key_bits,
-                                           stream));
-    SGPU_HIP_TRY(rocprim::exclusive_scan(nullptr, scan_bytes, vcount, voffset, 0u, (size_t)TM + 1,
-                                         rocprim::plus<uint32_t>(), stream));
-    SGPU_HIP_TRY(rocprim::inclusive_scan(nullptr, scan2_bytes, cell_counts, cell_ends, (size_t)TM,
-                                         rocprim::plus<uint32_t>(), stream));
-    size_t tmp_bytes = std::max(sort_bytes, std::max(scan_bytes, scan2_bytes));
-    void* tmp = A.get<uint8_t>(tmp_bytes);
-    if (A.err != hipSuccess) return sgpu_fail(SGPU_E_NOMEM, "sgpu_render_vcm: device allocation failed");
-    // pinned host staging: counters (16 words), the slice table and the scene table
-    uint8_t* hbuf = nullptr;
-    const size_t hbytes = 64 + sizeof(Slice) * (size_t)max_slices + sizeof(TraceScene) * uniq.size();
-    SGPU_HIP_TRY(hipHostMalloc((void**)&hbuf, hbytes, hipHostMallocDefault));
-    struct HostFree {
-        void* p;
-        ~HostFree() { (void)hipHostFree(p); }
-    } hf{hbuf};
-    uint32_t* h = (uint32_t*)hbuf;
-    Slice* h_slices = (Slice*)(hbuf + 64);
-    TraceScene* h_table = (TraceScene*)(hbuf + 64 + sizeof(Slice) * (size_t)max_slices);
-    if (multi) {
-        for (size_t k = 0; k < uniq.size(); ++k) h_table[k] = TraceScene{uniq[k]->d_nodes, uniq[k]->d_tris, uniq[k]->n_tris, 0};
-        SGPU_HIP_TRY(hipMemcpyAsync(d_table, h_table, sizeof(TraceScene) * uniq.size(), hipMemcpyHostToDevice, stream));
+    SGPU_HIP_TRY(rocprim::radix_sort_pairs(nullptr, sort_bytes, sv->keys, sv->keys2, sv->vals, sv->vals2, max_verts, 0,
+                                           sv->key_bits, A.stream));
+    SGPU_HIP_TRY(rocprim::exclusive_scan(nullptr, scan_bytes, sv->vcount, sv->voffset, 0u, (size_t)TM + 1,
+                                         rocprim::plus<uint32_t>(), A.stream));
+    SGPU_HIP_TRY(rocprim::inclusive_scan(nullptr, scan2_bytes, sv->cell_counts, sv->cell_ends, (size_t)TM,
+                                         rocprim::plus<uint32_t>(), A.stream));
+    sv->tmp_bytes = std::max(sort_bytes, std::max(scan_bytes, scan2_bytes));
+    sv->tmp = A.get<uint8_t>(sv->tmp_bytes);
+    sv->spill_depth_cap = kMaxSceneDepth;
+    sv->spill = A.get<uint32_t>(sgpu_trace_spill_words(sv->ctx, kMaxSceneDepth));
+    if (A.err != hipSuccess) return sgpu_fail(SGPU_E_NOMEM, "sgpu_vcm_server: device allocation failed");
+    if (A.dry) return SGPU_OK;
+    const size_t hbytes = 64 + sizeof(Slice) * (size_t)sv->max_slices;
+    SGPU_HIP_TRY(hipHostMalloc((void**)&sv->hbuf, hbytes, hipHostMallocDefault));
+    sv->h = (uint32_t*)sv->hbuf;
+    sv->h_slices = (Slice*)(sv->hbuf + 64);
+    return SGPU_OK;
+}
+
+// Scene table entry of `scene` (added on first use).
+uint32_t server_scene(sgpu_vcm_server sv, sgpu_scene scene)
+{
+    for (size_t k = 0; k < sv->scenes.size(); ++k)
+        if (sv->scenes[k] == scene) return (uint32_t)k;
+    sv->scenes.push_back(scene);
+    sv->table_dirty = true;
+    sv->max_depth = std::max(sv->max_depth, scene->max_depth);
+    return (uint32_t)(sv->scenes.size() - 1);
+}
+
+int server_upload_table(sgpu_vcm_server sv, hipStream_t stream)
+{
+    if (!sv->table_dirty) return SGPU_OK;
+    SGPU_HIP_TRY(hipStreamSynchronize(stream));
+    const uint32_t n = (uint32_t)sv->scenes.size();
+    if (n > sv->table_cap) {
+        if (sv->d_table) (void)hipFree(sv->d_table);
+        if (sv->h_table) (void)hipHostFree(sv->h_table);
+        sv->d_table = nullptr;
+        sv->h_table = nullptr;
+        sv->table_cap = std::max<uint32_t>(16, 2 * n);
+        SGPU_HIP_TRY(hipMalloc((void**)&sv->d_table, sizeof(TraceScene) * sv->table_cap));
+        SGPU_HIP_TRY(hipHostMalloc((void**)&sv->h_table, sizeof(TraceScene) * sv->table_cap, hipHostMallocDefault));
     }
-    auto trace = [&](const RayBuf& r, const sgpu_hits_soa* hits, uint32_t* occ, uint32_t n, bool any,
-                     uint64_t* d_st) -> int {
-        sgpu_rays_soa rs = as_soa(r);
-        if (multi)
-            return sgpu_trace_launch_multi(ctx, d_table, r.scene, max_depth, uniq[0]->d_overflow, &rs, hits, occ, n, any,
-                                           d_st, stream, spill);
-        return sgpu_trace_launch(uniq[0], &rs, hits, occ, n, any, d_st, stream, spill);
+    for (uint32_t k = 0; k < n; ++k)
+        sv->h_table[k] = TraceScene{sv->scenes[k]->d_nodes, sv->scenes[k]->d_tris, sv->scenes[k]->n_tris, 0};
+    SGPU_HIP_TRY(hipMemcpyAsync(sv->d_table, sv->h_table, sizeof(TraceScene) * n, hipMemcpyHostToDevice, stream));
+    SGPU_HIP_TRY(hipStreamSynchronize(stream));
+    sv->table_dirty = false;
+    return SGPU_OK;
+}
+
+}  // namespace
+
+int sgpu_vcm_server_create_impl(sgpu_context ctx, const sgpu_vcm_server_config* cfg, hipStream_t stream,
+                                sgpu_vcm_server* out)
+{
+    sgpu_vcm_server sv = new (std::nothrow) sgpu_vcm_server_s;
+    if (!sv) return sgpu_fail(SGPU_E_NOMEM, "sgpu_vcm_server_create: out of host memory");
+    sv->ctx = ctx;
+    sv->cfg = *cfg;
+    sv->A.stream = stream;
+    sv->TM = (uint32_t)cfg->wavefront_paths;
+    sv->mlv = cfg->max_path_length > 2 ? cfg->max_path_length - 2 : 1;
+    sv->nslots = sv->mlv + 2;
+    sv->max_slices = kMaxSlicesPerWave;
+    int rc = server_alloc(sv);
+    if (rc) {
+        sgpu_vcm_server_destroy_impl(sv, stream);
+        return rc;
+    }
+    *out = sv;
+    return SGPU_OK;
+}
+
+uint64_t sgpu_vcm_server_bytes(sgpu_context ctx, const sgpu_vcm_server_config* cfg)
+{
+    sgpu_vcm_server_s sv;
+    sv.ctx = ctx;
+    sv.cfg = *cfg;
+    sv.A.dry = true;
+    sv.TM = (uint32_t)cfg->wavefront_paths;
+    sv.mlv = cfg->max_path_length > 2 ? cfg->max_path_length - 2 : 1;
+    sv.nslots = sv.mlv + 2;
+    sv.max_slices = kMaxSlicesPerWave;
+    server_alloc(&sv);
+    return sv.A.bytes;
+}
+
+int sgpu_vcm_server_destroy_impl(sgpu_vcm_server sv, hipStream_t stream)
+{
+    if (!sv) return SGPU_OK;
+    sv->A.stream = stream;
+    if (sv->hbuf) (void)hipHostFree(sv->hbuf);
+    if (sv->h_table) (void)hipHostFree(sv->h_table);
+    if (sv->d_table) (void)hipFree(sv->d_table);
+    delete sv;  // the arena frees its buffers, stream-ordered
+    return SGPU_OK;
+}
+
+int sgpu_vcm_server_submit_impl(sgpu_vcm_server sv, sgpu_scene sc, const sgpu_vcm_job* jb, float* fb,
+                                uint64_t* id_out, hipStream_t stream)
+{
+    const float pi = 3.141592653589793f;
+    const uint32_t W = jb->camera.width, H = jb->camera.height, L = jb->max_path_length;
+    if ((uint64_t)W * H > sv->TM) return sgpu_fail(SGPU_E_RANGE, "sgpu_vcm_server_submit: image larger than a wavefront");
+    if (L > sv->cfg.max_path_length)
+        return sgpu_fail(SGPU_E_RANGE, "sgpu_vcm_server_submit: path length above the server's maximum");
+    if (sc->max_depth > kMaxSceneDepth) return sgpu_fail(SGPU_E_RANGE, "sgpu_vcm_server_submit: tree too deep");
+    ServerJob j;
+    j.id = sv->next_id++;
+    j.scene = sc;
+    j.job = *jb;
+    j.fb = fb;
+    j.next = 0;
+    Slice& b = j.base;
+    std::memset(&b, 0, sizeof(b));
+    b.cam = camera_init(mk3(jb->camera.position[0], jb->camera.position[1], jb->camera.position[2]),
+                        mk3(jb->camera.look_at[0], jb->camera.look_at[1], jb->camera.look_at[2]),
+                        mk3(jb->camera.up[0], jb->camera.up[1], jb->camera.up[2]), jb->camera.fov_degrees, W, H);
+    float hfov = jb->camera.fov_degrees * (pi / 180.f);
+    b.vipd = (float)W / (2.0f * tanf(hfov * 0.5f));
+    b.W = W;
+    b.H = H;
+    b.N = W * H;
+    b.L = L;
+    b.mlv = L > 2 ? L - 2 : 1;
+    b.seed = jb->seed;
+    b.scene = server_scene(sv, sc);
+    b.materials = sc->d_materials;
+    b.geom_material = sc->d_geom_material;
+    b.n_geoms = sc->n_geom_material;
+    b.lights = (const Light*)sc->d_lights;
+    b.n_lights = sc->n_lights;
+    b.background = mk3(sc->background[0], sc->background[1], sc->background[2]);
+    b.geom_tex = sc->d_geom_tex;
+    b.n_geom_tex = sc->n_geom_tex;
+    SGPU_HIP_TRY(hipMemsetAsync(fb, 0, 3 * (size_t)b.N * sizeof(float), stream));
+    sv->active.push_back(j);
+    if (id_out) *id_out = j.id;
+    return SGPU_OK;
+}
+
+int sgpu_vcm_server_step_impl(sgpu_vcm_server sv, hipStream_t stream, uint64_t* done_ids, uint32_t max_done,
+                              uint32_t* n_done, uint32_t* n_active)
+{
+    const float pi = 3.141592653589793f;
+    if (n_done) *n_done = 0;
+    if (sv->active.empty()) {
+        if (n_active) *n_active = 0;
+        return SGPU_OK;
+    }
+    const uint32_t flags = sv->cfg.flags;
+    const bool one_slice = flags & SGPU_VCM_ONE_SLICE_PER_WAVEFRONT;
+    int rc = server_upload_table(sv, stream);
+    if (rc) return rc;
+
+    // ---- the wavefront: consecutive iterations of the jobs in admission order ----
+    struct Run {
+        uint32_t job, first, count;  // job index in `active`, first slice, slices
     };
-
-    for (uint32_t j = 0; j < n_jobs; ++j)
-        SGPU_HIP_TRY(hipMemsetAsync(framebuffers[j], 0, 3 * (size_t)job_base[j].N * sizeof(float), stream));
-    sgpu_hits_soa hsoa{hb.t, hb.u, hb.v, hb.prim, hb.geom, hb.ngx, hb.ngy, hb.ngz};
-    uint64_t n_closest = 0, n_shadow = 0, n_light_verts = 0, n_launches = 0, n_samples = 0;
-    uint64_t* d_trav = nullptr;  // nodes visited, triangles tested by the closest-hit rays
-    if (count_traversal) {
-        d_trav = A.get<uint64_t>(2);
-        if (A.err != hipSuccess) return sgpu_fail(SGPU_E_NOMEM, "sgpu_render_vcm: device allocation failed");
-        SGPU_HIP_TRY(hipMemsetAsync(d_trav, 0, 2 * sizeof(uint64_t), stream));
+    std::vector<Run> runs;
+    std::vector<uint32_t> order(sv->active.size());
+    for (uint32_t k = 0; k < order.size(); ++k) order[k] = k;
+    if (flags & SGPU_VCM_SHORTEST_FIRST) {
+        auto left = [&](uint32_t k) {
+            const ServerJob& j = sv->active[k];
+            return (uint64_t)j.base.N * (j.job.iterations - j.next);
+        };
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return left(a) < left(b); });
     }
-    PhaseTimer T(kernel_times, stream);
-
-    for (const Wavefront& wf : waves) {
-        const uint32_t ns = wf.n_slices, total = wf.total;
-        // the previous wavefront has finished (its last step synchronised the stream): the staged
-        // slice table may be rewritten
-        uint32_t path_base = 0;
-        for (uint32_t k = 0; k < ns; ++k) {
-            const auto& js = sched[wf.first_slice + k];
-            const sgpu_vcm_job* jb = &jobs[js.first];
-            Slice sl = job_base[js.first];
-            const uint32_t gi = jb->iteration_base + js.second;
+    uint32_t ns = 0, total = 0;
+    for (uint32_t oi = 0; oi < order.size() && ns < sv->max_slices; ++oi) {
+        const uint32_t ji = order[oi];
+        ServerJob& j = sv->active[ji];
+        Run run{ji, ns, 0};
+        while (j.next + run.count < j.job.iterations && ns < sv->max_slices) {
+            if (ns > 0 && (one_slice || (uint64_t)total + j.base.N > sv->TM)) break;
+            Slice sl = j.base;
+            const uint32_t gi = j.job.iteration_base + j.next + run.count;
             // CalculateIterationConstants (VCMCommon.cpp:97-113), vmCount = W*H, vcCount = 1
             float iteration_index = (float)gi + 1.0f;
-            float r = jb->base_radius / pow_f(iteration_index, 0.5f * (1.0f - jb->radius_alpha));
+            float r = j.job.base_radius / pow_f(iteration_index, 0.5f * (1.0f - j.job.radius_alpha));
             float r2 = r * r;
             sl.radius_sq = r2;
             sl.vm_norm = 1.0f / (pi * r2 * sl.N);
@@ -1351,220 +1446,302 @@ int sgpu_vcm_render_batch_impl(const sgpu_scene* scenes, const sgpu_vcm_job* job
             sl.vc_weight = 1 / (pi * r2 * sl.N);
             sl.inv_cell = 1.0f / (2.0f * r);
             sl.iteration = gi;
-            sl.path_base = path_base;
-            path_base += sl.N;
-            h_slices[k] = sl;
+            sl.path_base = total;
+            sv->h_slices[ns] = sl;
+            total += sl.N;
+            ++ns;
+            ++run.count;
         }
-        SGPU_HIP_TRY(hipMemcpyAsync(d_slices, h_slices, sizeof(Slice) * ns, hipMemcpyHostToDevice, stream));
-        Wave wv;
-        wv.slices = d_slices;
-        wv.n_slices = ns;
-        wv.total = total;
-        wv.mlv = mlv;
-        wv.nslots = nslots;
-        n_samples += total;
-
-        SGPU_HIP_TRY(hipMemsetAsync(frame, 0, 3 * (size_t)total * sizeof(float), stream));
-        SGPU_HIP_TRY(hipMemsetAsync(vcount, 0, ((size_t)total + 1) * sizeof(uint32_t), stream));
-        SGPU_HIP_TRY(hipMemsetAsync(counters, 0, 4 * sizeof(uint32_t), stream));
-
-        // ---- light pass ----
-        SGPU_HIP_TRY(T.begin(kPhaseLight));
-        hipLaunchKernelGGL(k_light_gen, dim3(blocks_for(total)), dim3(kBlock), 0, stream, wv, ps, rays[0], queue[0],
-                           counters);
-        SGPU_HIP_TRY(hipGetLastError());
-        SGPU_HIP_TRY(T.end());
-        SGPU_HIP_TRY(hipMemcpyAsync(h, counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-        SGPU_HIP_TRY(hipStreamSynchronize(stream));
-        uint32_t count = h[0];
-        int cur = 0;
-        while (count > 0) {
-            SGPU_HIP_TRY(T.begin(kPhaseClosest));
-            int rc = trace(rays[cur], &hsoa, nullptr, count, false, d_trav);
-            if (rc) return rc;
-            SGPU_HIP_TRY(T.end());
-            n_closest += count;
-            ++n_launches;
-            SGPU_HIP_TRY(hipMemsetAsync(counters, 0, 4 * sizeof(uint32_t), stream));
-            SGPU_HIP_TRY(T.begin(kPhaseLight));
-            hipLaunchKernelGGL(k_light_shade, dim3(blocks_for(count)), dim3(kBlock), 0, stream, wv, count, queue[cur],
-                               rays[cur], hb, ps, verts, vcount, rays[1 - cur], queue[1 - cur], shadow, splat_c,
-                               splat_pix, counters);
-            SGPU_HIP_TRY(hipGetLastError());
-            SGPU_HIP_TRY(T.end());
-            SGPU_HIP_TRY(hipMemcpyAsync(h, counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-            SGPU_HIP_TRY(hipStreamSynchronize(stream));
-            uint32_t n_next = h[0], n_sh = h[1];
-            if (n_sh > 0) {
-                SGPU_HIP_TRY(T.begin(kPhaseShadow));
-                rc = trace(shadow, nullptr, occl, n_sh, true, nullptr);
-                if (rc) return rc;
-                SGPU_HIP_TRY(T.end());
-                n_shadow += n_sh;
-                SGPU_HIP_TRY(T.begin(kPhaseLight));
-                hipLaunchKernelGGL(k_splat, dim3(blocks_for(n_sh)), dim3(kBlock), 0, stream, n_sh, occl, splat_c,
-                                   splat_pix, frame);
-                SGPU_HIP_TRY(hipGetLastError());
-                SGPU_HIP_TRY(T.end());
-            }
-            count = n_next;
-            cur = 1 - cur;
-        }
-
-        // ---- light vertices, in path order ----
-        SGPU_HIP_TRY(T.begin(kPhaseGrid));
-        SGPU_HIP_TRY(rocprim::exclusive_scan(tmp, tmp_bytes, vcount, voffset, 0u, (size_t)total + 1,
-                                             rocprim::plus<uint32_t>(), stream));
-        hipLaunchKernelGGL(k_compact, dim3(blocks_for(total)), dim3(kBlock), 0, stream, wv, verts, vcount, voffset,
-                           ps.slice, cverts, cpos, cslice);
-        SGPU_HIP_TRY(hipGetLastError());
-        // path_end[g] = offset[g + 1] (inclusive end); the vertex count is offset[total]
-        SGPU_HIP_TRY(
-            hipMemcpyAsync(path_end, voffset + 1, (size_t)total * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
-        SGPU_HIP_TRY(hipMemcpyAsync(nv_dev, voffset + total, sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
-        SGPU_HIP_TRY(T.end());
-        SGPU_HIP_TRY(hipMemcpyAsync(h, voffset + total, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-        SGPU_HIP_TRY(hipStreamSynchronize(stream));
-        const uint32_t nv = h[0];
-        n_light_verts += nv;
-
-        // ---- photon grids ----
-        SGPU_HIP_TRY(T.begin(kPhaseGrid));
-        SGPU_HIP_TRY(hipMemsetAsync(box, 0xFF, 8 * sizeof(uint32_t) * ns, stream));
-        SGPU_HIP_TRY(hipMemsetAsync(cell_counts, 0, (size_t)total * sizeof(uint32_t), stream));
-        if (nv > 0) {
-            hipLaunchKernelGGL(k_grid_bounds, dim3(blocks_for(nv)), dim3(kBlock), 0, stream, cpos, cslice, nv_dev, box);
-            SGPU_HIP_TRY(hipGetLastError());
-            hipLaunchKernelGGL(k_grid_keys, dim3(blocks_for(nv)), dim3(kBlock), 0, stream, wv, cpos, cslice, nv_dev,
-                               box, keys, vals, cell_counts);
-            SGPU_HIP_TRY(hipGetLastError());
-            size_t need = 0;
-            SGPU_HIP_TRY(rocprim::radix_sort_pairs(nullptr, need, keys, keys2, vals, vals2, (size_t)nv, 0, key_bits,
-                                                   stream));
-            if (need > tmp_bytes) {
-                tmp = A.get<uint8_t>(need);
-                tmp_bytes = need;
-                if (A.err != hipSuccess) return sgpu_fail(SGPU_E_NOMEM, "sgpu_render_vcm: device allocation failed");
-            }
-            SGPU_HIP_TRY(
-                rocprim::radix_sort_pairs(tmp, tmp_bytes, keys, keys2, vals, vals2, (size_t)nv, 0, key_bits, stream));
-            hipLaunchKernelGGL(k_grid_gather, dim3(blocks_for(nv)), dim3(kBlock), 0, stream, nv_dev, vals2, cverts, sverts,
-                               spos);
-            SGPU_HIP_TRY(hipGetLastError());
-        }
-        SGPU_HIP_TRY(rocprim::inclusive_scan(tmp, tmp_bytes, cell_counts, cell_ends, (size_t)total,
-                                             rocprim::plus<uint32_t>(), stream));
-        SGPU_HIP_TRY(T.end());
-        GridView G;
-        G.box = box;
-        G.cell_ends = cell_ends;
-        G.sverts = sverts;
-        G.spos = spos;
-        G.cverts = cverts;
-        G.path_end = path_end;
-        G.nv_ptr = nv_dev;
-
-        // ---- camera pass ----
-        SGPU_HIP_TRY(hipMemsetAsync(counters, 0, 4 * sizeof(uint32_t), stream));
-        SGPU_HIP_TRY(T.begin(kPhaseCamera));
-        hipLaunchKernelGGL(k_cam_gen, dim3(blocks_for(total)), dim3(kBlock), 0, stream, wv, ps, rays[0], queue[0],
-                           counters);
-        SGPU_HIP_TRY(hipGetLastError());
-        SGPU_HIP_TRY(T.end());
-        SGPU_HIP_TRY(hipMemcpyAsync(h, counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-        SGPU_HIP_TRY(hipStreamSynchronize(stream));
-        count = h[0];
-        cur = 0;
-        while (count > 0) {
-            SGPU_HIP_TRY(T.begin(kPhaseClosest));
-            int rc = trace(rays[cur], &hsoa, nullptr, count, false, d_trav);
-            if (rc) return rc;
-            SGPU_HIP_TRY(T.end());
-            n_closest += count;
-            ++n_launches;
-            SGPU_HIP_TRY(hipMemsetAsync(counters, 0, 4 * sizeof(uint32_t), stream));
-            SGPU_HIP_TRY(T.begin(kPhaseCamera));
-            hipLaunchKernelGGL(k_cam_shade, dim3(blocks_for(count)), dim3(kBlock), 0, stream, wv, count, queue[cur],
-                               rays[cur], hb, ps, G, S, cam_vm, rays[1 - cur], queue[1 - cur], shadow, counters);
-            SGPU_HIP_TRY(hipGetLastError());
-            SGPU_HIP_TRY(T.end());
-            SGPU_HIP_TRY(T.begin(kPhaseMerge));
-            hipLaunchKernelGGL(k_cam_merge, dim3(blocks_for(8ull * count)), dim3(kBlock), 0, stream, wv, count, cam_vm,
-                               ps.slice, G, S);
-            SGPU_HIP_TRY(hipGetLastError());
-            SGPU_HIP_TRY(T.end());
-            SGPU_HIP_TRY(hipMemcpyAsync(h, counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-            SGPU_HIP_TRY(hipStreamSynchronize(stream));
-            uint32_t n_next = h[0], n_sh = h[1];
-            if (n_sh > 0) {
-                SGPU_HIP_TRY(T.begin(kPhaseShadow));
-                rc = trace(shadow, nullptr, occl, n_sh, true, nullptr);
-                if (rc) return rc;
-                SGPU_HIP_TRY(T.end());
-                n_shadow += n_sh;
-                SGPU_HIP_TRY(T.begin(kPhaseCamera));
-                hipLaunchKernelGGL(k_cam_resolve, dim3(blocks_for(n_sh)), dim3(kBlock), 0, stream, n_sh, occl, S);
-                SGPU_HIP_TRY(hipGetLastError());
-                SGPU_HIP_TRY(T.end());
-            }
-            SGPU_HIP_TRY(T.begin(kPhaseCamera));
-            hipLaunchKernelGGL(k_cam_gather, dim3(blocks_for(count)), dim3(kBlock), 0, stream, count, queue[cur],
-                               nslots, S, ps);
-            SGPU_HIP_TRY(hipGetLastError());
-            SGPU_HIP_TRY(T.end());
-            count = n_next;
-            cur = 1 - cur;
-        }
-        // each job's slices of this wavefront (consecutive, in iteration order) into its sum
-        SGPU_HIP_TRY(T.begin(kPhaseCamera));
-        for (uint32_t k = 0; k < ns;) {
-            const uint32_t jb = sched[wf.first_slice + k].first;
-            uint32_t e = k + 1;
-            while (e < ns && sched[wf.first_slice + e].first == jb) ++e;
-            hipLaunchKernelGGL(k_frame_add, dim3(blocks_for(job_base[jb].N)), dim3(kBlock), 0, stream, wv, k, e - k,
-                               job_base[jb].N, ps.color, frame, framebuffers[jb]);
-            SGPU_HIP_TRY(hipGetLastError());
-            k = e;
-        }
-        SGPU_HIP_TRY(T.end());
-        SGPU_HIP_TRY(T.collect());
-        SGPU_HIP_TRY(hipStreamSynchronize(stream));
+        if (run.count) runs.push_back(run);
+        if (one_slice && ns > 0) break;
+        if (run.count == 0 || j.next + run.count < j.job.iterations) break;  // the wavefront is full
     }
-    for (uint32_t j = 0; j < n_jobs; ++j) {
-        const size_t nf = 3 * (size_t)job_base[j].N;
-        hipLaunchKernelGGL(k_scale, dim3(blocks_for(nf)), dim3(kBlock), 0, stream, framebuffers[j], nf,
-                           1.0f / (float)jobs[j].iterations);
-        SGPU_HIP_TRY(hipGetLastError());
-    }
+    bool single = true;
+    for (uint32_t k = 1; k < ns; ++k) single &= sv->h_slices[k].scene == sv->h_slices[0].scene;
+    const uint32_t scene0 = sv->h_slices[0].scene;
+    sgpu_scene first_scene = sv->scenes[scene0];
+
+    const bool kernel_times = flags & SGPU_VCM_KERNEL_TIMES;
+    uint64_t* d_trav = (flags & SGPU_VCM_COUNT_TRAVERSAL) ? sv->d_trav : nullptr;
+    PhaseTimer T(kernel_times, stream);
+    auto trace = [&](const RayBuf& r, const sgpu_hits_soa* hits, uint32_t* occ, uint32_t n, bool any,
+                     uint64_t* d_st) -> int {
+        sgpu_rays_soa rs = as_soa(r);
+        if (!single)
+            return sgpu_trace_launch_multi(sv->ctx, sv->d_table, r.scene, sv->max_depth, first_scene->d_overflow, &rs,
+                                           hits, occ, n, any, d_st, stream, sv->spill);
+        return sgpu_trace_launch(first_scene, &rs, hits, occ, n, any, d_st, stream, sv->spill);
+    };
+    // per-ray scene indices are only written and read in multi-scene wavefronts
+    RayBuf rays[2] = {sv->rays[0], sv->rays[1]}, shadow = sv->shadow;
+    if (single) rays[0].scene = rays[1].scene = shadow.scene = nullptr;
+    PathState& ps = sv->ps;
+    HitBuf& hb = sv->hb;
+    CamSlots& S = sv->S;
+    uint32_t* h = sv->h;
+    uint32_t** queue = sv->queue;
+
+    SGPU_HIP_TRY(hipMemcpyAsync(sv->d_slices, sv->h_slices, sizeof(Slice) * ns, hipMemcpyHostToDevice, stream));
+    if (d_trav) SGPU_HIP_TRY(hipMemsetAsync(d_trav, 0, 2 * sizeof(uint64_t), stream));
+    Wave wv;
+    wv.slices = sv->d_slices;
+    wv.n_slices = ns;
+    wv.total = total;
+    wv.mlv = sv->mlv;
+    wv.nslots = sv->nslots;
+    sv->n_samples += total;
+    ++sv->n_waves;
+    sgpu_hits_soa hsoa{hb.t, hb.u, hb.v, hb.prim, hb.geom, hb.ngx, hb.ngy, hb.ngz};
+
+    SGPU_HIP_TRY(hipMemsetAsync(sv->frame, 0, 3 * (size_t)total * sizeof(float), stream));
+    SGPU_HIP_TRY(hipMemsetAsync(sv->vcount, 0, ((size_t)total + 1) * sizeof(uint32_t), stream));
+    SGPU_HIP_TRY(hipMemsetAsync(sv->counters, 0, 4 * sizeof(uint32_t), stream));
+
+    // ---- light pass ----
+    SGPU_HIP_TRY(T.begin(kPhaseLight));
+    hipLaunchKernelGGL(k_light_gen, dim3(blocks_for(total)), dim3(kBlock), 0, stream, wv, ps, rays[0], queue[0],
+                       sv->counters);
+    SGPU_HIP_TRY(hipGetLastError());
+    SGPU_HIP_TRY(T.end());
+    SGPU_HIP_TRY(hipMemcpyAsync(h, sv->counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     SGPU_HIP_TRY(hipStreamSynchronize(stream));
-    if (stats) {
-        std::memset(stats, 0, sizeof(*stats));
-        stats->rays = n_closest + n_shadow;
-        stats->closest_rays = n_closest;
-        stats->shadow_rays = n_shadow;
-        stats->closest_launches = n_launches;
-        stats->light_vertices = n_light_verts;
-        stats->samples = n_samples;
-        stats->wavefronts = waves.size();
-        if (d_trav) {
-            uint64_t tr[2] = {0, 0};
-            SGPU_HIP_TRY(hipMemcpy(tr, d_trav, sizeof(tr), hipMemcpyDeviceToHost));
-            stats->nodes_visited = tr[0];
-            stats->tris_tested = tr[1];
+    uint32_t count = h[0];
+    int cur = 0;
+    while (count > 0) {
+        SGPU_HIP_TRY(T.begin(kPhaseClosest));
+        rc = trace(rays[cur], &hsoa, nullptr, count, false, d_trav);
+        if (rc) return rc;
+        SGPU_HIP_TRY(T.end());
+        sv->n_closest += count;
+        ++sv->n_launches;
+        SGPU_HIP_TRY(hipMemsetAsync(sv->counters, 0, 4 * sizeof(uint32_t), stream));
+        SGPU_HIP_TRY(T.begin(kPhaseLight));
+        hipLaunchKernelGGL(k_light_shade, dim3(blocks_for(count)), dim3(kBlock), 0, stream, wv, count, queue[cur],
+                           rays[cur], hb, ps, sv->verts, sv->vcount, rays[1 - cur], queue[1 - cur], shadow, sv->splat_c,
+                           sv->splat_pix, sv->counters);
+        SGPU_HIP_TRY(hipGetLastError());
+        SGPU_HIP_TRY(T.end());
+        SGPU_HIP_TRY(hipMemcpyAsync(h, sv->counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        SGPU_HIP_TRY(hipStreamSynchronize(stream));
+        uint32_t n_next = h[0], n_sh = h[1];
+        if (n_sh > 0) {
+            SGPU_HIP_TRY(T.begin(kPhaseShadow));
+            rc = trace(shadow, nullptr, sv->occl, n_sh, true, nullptr);
+            if (rc) return rc;
+            SGPU_HIP_TRY(T.end());
+            sv->n_shadow += n_sh;
+            SGPU_HIP_TRY(T.begin(kPhaseLight));
+            hipLaunchKernelGGL(k_splat, dim3(blocks_for(n_sh)), dim3(kBlock), 0, stream, n_sh, sv->occl, sv->splat_c,
+                               sv->splat_pix, sv->frame);
+            SGPU_HIP_TRY(hipGetLastError());
+            SGPU_HIP_TRY(T.end());
         }
-        stats->closest_ms = (float)T.ms[kPhaseClosest];
-        stats->shadow_ms = (float)T.ms[kPhaseShadow];
-        stats->light_ms = (float)T.ms[kPhaseLight];
-        stats->camera_ms = (float)T.ms[kPhaseCamera];
-        stats->grid_ms = (float)T.ms[kPhaseGrid];
-        stats->merge_ms = (float)T.ms[kPhaseMerge];
+        count = n_next;
+        cur = 1 - cur;
     }
+
+    // ---- light vertices, in path order ----
+    SGPU_HIP_TRY(T.begin(kPhaseGrid));
+    SGPU_HIP_TRY(rocprim::exclusive_scan(sv->tmp, sv->tmp_bytes, sv->vcount, sv->voffset, 0u, (size_t)total + 1,
+                                         rocprim::plus<uint32_t>(), stream));
+    hipLaunchKernelGGL(k_compact, dim3(blocks_for(total)), dim3(kBlock), 0, stream, wv, sv->verts, sv->vcount,
+                       sv->voffset, ps.slice, sv->cverts, sv->cpos, sv->cslice);
+    SGPU_HIP_TRY(hipGetLastError());
+    // path_end[g] = offset[g + 1] (inclusive end); the vertex count is offset[total]
+    SGPU_HIP_TRY(hipMemcpyAsync(sv->path_end, sv->voffset + 1, (size_t)total * sizeof(uint32_t),
+                                hipMemcpyDeviceToDevice, stream));
+    SGPU_HIP_TRY(hipMemcpyAsync(sv->nv_dev, sv->voffset + total, sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
+    SGPU_HIP_TRY(T.end());
+    SGPU_HIP_TRY(hipMemcpyAsync(h, sv->voffset + total, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    SGPU_HIP_TRY(hipStreamSynchronize(stream));
+    const uint32_t nv = h[0];
+    sv->n_light_verts += nv;
+
+    // ---- photon grids ----
+    SGPU_HIP_TRY(T.begin(kPhaseGrid));
+    SGPU_HIP_TRY(hipMemsetAsync(sv->box, 0xFF, 8 * sizeof(uint32_t) * ns, stream));
+    SGPU_HIP_TRY(hipMemsetAsync(sv->cell_counts, 0, (size_t)total * sizeof(uint32_t), stream));
+    if (nv > 0) {
+        hipLaunchKernelGGL(k_grid_bounds, dim3(blocks_for(nv)), dim3(kBlock), 0, stream, sv->cpos, sv->cslice,
+                           sv->nv_dev, sv->box);
+        SGPU_HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_grid_keys, dim3(blocks_for(nv)), dim3(kBlock), 0, stream, wv, sv->cpos, sv->cslice,
+                           sv->nv_dev, sv->box, sv->keys, sv->vals, sv->cell_counts);
+        SGPU_HIP_TRY(hipGetLastError());
+        size_t need = 0;
+        SGPU_HIP_TRY(rocprim::radix_sort_pairs(nullptr, need, sv->keys, sv->keys2, sv->vals, sv->vals2, (size_t)nv, 0,
+                                               sv->key_bits, stream));
+        if (need > sv->tmp_bytes) return sgpu_fail(SGPU_E_NOMEM, "sgpu_vcm_server: sort scratch too small");
+        SGPU_HIP_TRY(rocprim::radix_sort_pairs(sv->tmp, sv->tmp_bytes, sv->keys, sv->keys2, sv->vals, sv->vals2,
+                                               (size_t)nv, 0, sv->key_bits, stream));
+        hipLaunchKernelGGL(k_grid_gather, dim3(blocks_for(nv)), dim3(kBlock), 0, stream, sv->nv_dev, sv->vals2,
+                           sv->cverts, sv->sverts, sv->spos);
+        SGPU_HIP_TRY(hipGetLastError());
+    }
+    SGPU_HIP_TRY(rocprim::inclusive_scan(sv->tmp, sv->tmp_bytes, sv->cell_counts, sv->cell_ends, (size_t)total,
+                                         rocprim::plus<uint32_t>(), stream));
+    SGPU_HIP_TRY(T.end());
+    GridView G;
+    G.box = sv->box;
+    G.cell_ends = sv->cell_ends;
+    G.sverts = sv->sverts;
+    G.spos = sv->spos;
+    G.cverts = sv->cverts;
+    G.path_end = sv->path_end;
+    G.nv_ptr = sv->nv_dev;
+
+    // ---- camera pass ----
+    SGPU_HIP_TRY(hipMemsetAsync(sv->counters, 0, 4 * sizeof(uint32_t), stream));
+    SGPU_HIP_TRY(T.begin(kPhaseCamera));
+    hipLaunchKernelGGL(k_cam_gen, dim3(blocks_for(total)), dim3(kBlock), 0, stream, wv, ps, rays[0], queue[0],
+                       sv->counters);
+    SGPU_HIP_TRY(hipGetLastError());
+    SGPU_HIP_TRY(T.end());
+    SGPU_HIP_TRY(hipMemcpyAsync(h, sv->counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    SGPU_HIP_TRY(hipStreamSynchronize(stream));
+    count = h[0];
+    cur = 0;
+    while (count > 0) {
+        SGPU_HIP_TRY(T.begin(kPhaseClosest));
+        rc = trace(rays[cur], &hsoa, nullptr, count, false, d_trav);
+        if (rc) return rc;
+        SGPU_HIP_TRY(T.end());
+        sv->n_closest += count;
+        ++sv->n_launches;
+        SGPU_HIP_TRY(hipMemsetAsync(sv->counters, 0, 4 * sizeof(uint32_t), stream));
+        SGPU_HIP_TRY(T.begin(kPhaseCamera));
+        hipLaunchKernelGGL(k_cam_shade, dim3(blocks_for(count)), dim3(kBlock), 0, stream, wv, count, queue[cur],
+                           rays[cur], hb, ps, G, S, sv->cam_vm, rays[1 - cur], queue[1 - cur], shadow, sv->counters);
+        SGPU_HIP_TRY(hipGetLastError());
+        SGPU_HIP_TRY(T.end());
+        SGPU_HIP_TRY(T.begin(kPhaseMerge));
+        hipLaunchKernelGGL(k_cam_merge, dim3(blocks_for(8ull * count)), dim3(kBlock), 0, stream, wv, count, sv->cam_vm,
+                           ps.slice, G, S);
+        SGPU_HIP_TRY(hipGetLastError());
+        SGPU_HIP_TRY(T.end());
+        SGPU_HIP_TRY(hipMemcpyAsync(h, sv->counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        SGPU_HIP_TRY(hipStreamSynchronize(stream));
+        uint32_t n_next = h[0], n_sh = h[1];
+        if (n_sh > 0) {
+            SGPU_HIP_TRY(T.begin(kPhaseShadow));
+            rc = trace(shadow, nullptr, sv->occl, n_sh, true, nullptr);
+            if (rc) return rc;
+            SGPU_HIP_TRY(T.end());
+            sv->n_shadow += n_sh;
+            SGPU_HIP_TRY(T.begin(kPhaseCamera));
+            hipLaunchKernelGGL(k_cam_resolve, dim3(blocks_for(n_sh)), dim3(kBlock), 0, stream, n_sh, sv->occl, S);
+            SGPU_HIP_TRY(hipGetLastError());
+            SGPU_HIP_TRY(T.end());
+        }
+        SGPU_HIP_TRY(T.begin(kPhaseCamera));
+        hipLaunchKernelGGL(k_cam_gather, dim3(blocks_for(count)), dim3(kBlock), 0, stream, count, queue[cur],
+                           sv->nslots, S, ps);
+        SGPU_HIP_TRY(hipGetLastError());
+        SGPU_HIP_TRY(T.end());
+        count = n_next;
+        cur = 1 - cur;
+    }
+    // each job's slices of this wavefront (consecutive, in iteration order) into its sum; a job
+    // whose last iteration ran is scaled by 1 / iterations and retired
+    SGPU_HIP_TRY(T.begin(kPhaseCamera));
+    std::vector<uint32_t> finished;
+    for (const Run& run : runs) {
+        ServerJob& j = sv->active[run.job];
+        hipLaunchKernelGGL(k_frame_add, dim3(blocks_for(j.base.N)), dim3(kBlock), 0, stream, wv, run.first, run.count,
+                           j.base.N, ps.color, sv->frame, j.fb);
+        SGPU_HIP_TRY(hipGetLastError());
+        j.next += run.count;
+        if (j.next == j.job.iterations) {
+            const size_t nf = 3 * (size_t)j.base.N;
+            hipLaunchKernelGGL(k_scale, dim3(blocks_for(nf)), dim3(kBlock), 0, stream, j.fb, nf,
+                               1.0f / (float)j.job.iterations);
+            SGPU_HIP_TRY(hipGetLastError());
+            finished.push_back(run.job);
+        }
+    }
+    SGPU_HIP_TRY(T.end());
+    SGPU_HIP_TRY(T.collect());
+    SGPU_HIP_TRY(hipStreamSynchronize(stream));
+    for (int k = kPhaseClosest; k < kPhases; ++k) sv->ms[k] += T.ms[k];
+    if (d_trav) {
+        uint64_t tr[2] = {0, 0};
+        SGPU_HIP_TRY(hipMemcpy(tr, d_trav, sizeof(tr), hipMemcpyDeviceToHost));
+        sv->trav_nodes += tr[0];
+        sv->trav_tris += tr[1];
+    }
+    std::sort(finished.begin(), finished.end());  // erase from the back
+    uint32_t nd = 0;
+    for (size_t k = finished.size(); k-- > 0;) {
+        const uint32_t ji = finished[k];
+        if (done_ids && nd < max_done) done_ids[nd] = sv->active[ji].id;
+        ++nd;
+        sv->active.erase(sv->active.begin() + ji);
+    }
+    if (n_done) *n_done = nd;
+    if (n_active) *n_active = (uint32_t)sv->active.size();
     return SGPU_OK;
+}
+
+int sgpu_vcm_server_stats_impl(sgpu_vcm_server sv, sgpu_vcm_stats* st)
+{
+    std::memset(st, 0, sizeof(*st));
+    st->rays = sv->n_closest + sv->n_shadow;
+    st->closest_rays = sv->n_closest;
+    st->shadow_rays = sv->n_shadow;
+    st->closest_launches = sv->n_launches;
+    st->light_vertices = sv->n_light_verts;
+    st->samples = sv->n_samples;
+    st->wavefronts = sv->n_waves;
+    st->nodes_visited = sv->trav_nodes;
+    st->tris_tested = sv->trav_tris;
+    st->closest_ms = (float)sv->ms[kPhaseClosest];
+    st->shadow_ms = (float)sv->ms[kPhaseShadow];
+    st->light_ms = (float)sv->ms[kPhaseLight];
+    st->camera_ms = (float)sv->ms[kPhaseCamera];
+    st->grid_ms = (float)sv->ms[kPhaseGrid];
+    st->merge_ms = (float)sv->ms[kPhaseMerge];
+    return SGPU_OK;
+}
+
+// Paths per wavefront the one-shot renders aim for: enough lanes to fill the chip in every
+// bounce's launch.
+constexpr uint64_t kTargetPaths = 1u << 20;
+
+int sgpu_vcm_render_batch_impl(const sgpu_scene* scenes, const sgpu_vcm_job* jobs, float* const* framebuffers,
+                               uint32_t n_jobs, uint32_t flags, sgpu_vcm_stats* stats, hipStream_t stream)
+{
+    sgpu_vcm_server_config cfg;
+    std::memset(&cfg, 0, sizeof(cfg));
+    uint64_t largest = 1;
+    uint32_t L = 1;
+    for (uint32_t j = 0; j < n_jobs; ++j) {
+        largest = std::max<uint64_t>(largest, (uint64_t)jobs[j].camera.width * jobs[j].camera.height);
+        L = std::max(L, jobs[j].max_path_length);
+    }
+    // wavefront capacity: the target, unless a slice is larger; never more than the batch needs
+    uint64_t needed = 0;
+    for (uint32_t j = 0; j < n_jobs; ++j)
+        needed += (uint64_t)jobs[j].camera.width * jobs[j].camera.height * jobs[j].iterations;
+    cfg.wavefront_paths = std::max<uint64_t>(largest, std::min<uint64_t>(kTargetPaths, needed));
+    cfg.max_path_length = L;
+    cfg.flags = flags;
+    sgpu_vcm_server sv = nullptr;
+    int rc = sgpu_vcm_server_create_impl(scenes[0]->ctx, &cfg, stream, &sv);
+    if (rc) return rc;
+    for (uint32_t j = 0; j < n_jobs && rc == SGPU_OK; ++j)
+        rc = sgpu_vcm_server_submit_impl(sv, scenes[j], &jobs[j], framebuffers[j], nullptr, stream);
+    uint32_t n_active = n_jobs;
+    while (rc == SGPU_OK && n_active > 0) rc = sgpu_vcm_server_step_impl(sv, stream, nullptr, 0, nullptr, &n_active);
+    if (rc == SGPU_OK && stats) sgpu_vcm_server_stats_impl(sv, stats);
+    sgpu_vcm_server_destroy_impl(sv, stream);
+    return rc;
 }
 
 int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, sgpu_vcm_stats* stats,
                          hipStream_t stream)
 {
     float* fbs[1] = {fb};
-    return sgpu_vcm_render_batch_impl(&scene, job, fbs, 1, job->flags, stats, stream);
+    sgpu_vcm_job j = *job;
+    j.flags = 0;
+    return sgpu_vcm_render_batch_impl(&scene, &j, fbs, 1, job->flags, stats, stream);
 }

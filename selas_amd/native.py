@@ -74,6 +74,7 @@ _SIGNATURES = {
     "sgpu_scene_commit": [_P, _P],
     "sgpu_scene_release": [_P],
     "sgpu_scene_info": [_P, _U32P, _U32P, _U32P],
+    "sgpu_scene_device_bytes": [_P, C.POINTER(C.c_uint64)],
     "sgpu_scene_check": [_P],
     "sgpu_intersect": [_P, C.POINTER(RaysSoA), C.POINTER(HitsSoA), _U32, _P],
     "sgpu_occluded": [_P, C.POINTER(RaysSoA), _P, _U32, _P],
@@ -83,6 +84,12 @@ _SIGNATURES = {
     "sgpu_render_pt": [_P, _P, _P, _P],
     "sgpu_render_vcm": [_P, _P, _P, _P, _P],
     "sgpu_render_vcm_batch": [_P, _P, _P, _U32, _U32, _P, _P],
+    "sgpu_vcm_server_create": [_P, _P, _P, _P],
+    "sgpu_vcm_server_submit": [_P, _P, _P, _P, _P, _P],
+    "sgpu_vcm_server_step": [_P, _P, _P, _U32, _P, _P],
+    "sgpu_vcm_server_stats": [_P, _P],
+    "sgpu_vcm_server_footprint": [_P, _P, _P],
+    "sgpu_vcm_server_destroy": [_P, _P],
     "sgpu_bsdf_eval": [_P, _P, _P, _P, _U32, _P, _P],
     "sgpu_bsdf_sample": [_P, _P, _P, _P, _U32, _U32, _P, _P],
     "sgpu_gen_cluster_soup": [_U64, _U32, _U32, _F, _F, _F, _FP],
@@ -311,6 +318,11 @@ class Scene:
         check(lib.sgpu_scene_info(self.handle, C.byref(a), C.byref(b), C.byref(c)))
         return {"n_tris": a.value, "n_nodes": b.value, "max_depth": c.value}
 
+    def device_bytes(self) -> int:
+        b = C.c_uint64()
+        check(lib.sgpu_scene_device_bytes(self.handle, C.byref(b)))
+        return b.value
+
     def check(self):
         check(lib.sgpu_scene_check(self.handle))
 
@@ -395,6 +407,7 @@ def build_render_scene(ctx: Context, sc: dict) -> Scene:
 VCM_KERNEL_TIMES = 1
 VCM_COUNT_TRAVERSAL = 2
 VCM_ONE_SLICE_PER_WAVEFRONT = 4
+VCM_SHORTEST_FIRST = 8  # servers only
 
 
 class VcmJob(C.Structure):
@@ -432,16 +445,73 @@ def render_vcm(scene: Scene, cam: dict, width, height, iterations, max_path_leng
     return out
 
 
-def render_vcm_batch(jobs, flags=0, stream=None, return_stats=False, keep_on_device=False):
+class VcmServerConfig(C.Structure):
+    _fields_ = [("wavefront_paths", C.c_uint64), ("max_path_length", C.c_uint32), ("flags", C.c_uint32)]
+
+
+def vcm_job_struct(j: dict) -> VcmJob:
+    return VcmJob(Camera.make(j["camera"], j["width"], j["height"]), j["iterations"], j.get("iteration_base", 0),
+                  j["max_path_length"], j["base_radius"], j["radius_alpha"], 0, j["seed"])
+
+
+class VcmServer:
+    """A persistent VCM executor (sgpu_vcm_server_*): submit jobs at any time, advance one wavefront
+    per step; a job's framebuffer (DeviceBuffer, H x W x 3 floats) is final once its id is retired."""
+
+    def __init__(self, ctx: Context, wavefront_paths=1 << 20, max_path_length=10, flags=0, stream=None):
+        self.ctx = ctx
+        self.stream = stream
+        self.cfg = VcmServerConfig(wavefront_paths, max_path_length, flags)
+        h = _P()
+        check(lib.sgpu_vcm_server_create(ctx.handle, C.byref(self.cfg), stream, C.byref(h)))
+        self.handle = h.value
+        self._done = (C.c_uint64 * 4096)()
+
+    @staticmethod
+    def footprint(ctx: Context, wavefront_paths=1 << 20, max_path_length=10, flags=0) -> int:
+        cfg = VcmServerConfig(wavefront_paths, max_path_length, flags)
+        b = C.c_uint64()
+        check(lib.sgpu_vcm_server_footprint(ctx.handle, C.byref(cfg), C.byref(b)))
+        return b.value
+
+    def submit(self, scene: Scene, job: dict, framebuffer: DeviceBuffer) -> int:
+        js = vcm_job_struct(job)
+        jid = C.c_uint64()
+        check(lib.sgpu_vcm_server_submit(self.handle, scene.handle, C.byref(js), framebuffer.ptr, C.byref(jid),
+                                         self.stream))
+        return jid.value
+
+    def step(self):
+        """Runs one wavefront; returns (ids retired, jobs still admitted)."""
+        nd, na = C.c_uint32(), C.c_uint32()
+        check(lib.sgpu_vcm_server_step(self.handle, self.stream, self._done, 4096, C.byref(nd), C.byref(na)))
+        return [self._done[k] for k in range(min(nd.value, 4096))], na.value
+
+    def stats(self) -> dict:
+        st = VcmStats()
+        check(lib.sgpu_vcm_server_stats(self.handle, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in VcmStats._fields_}
+
+    def close(self):
+        if self.handle:
+            check(lib.sgpu_vcm_server_destroy(self.handle, self.stream))
+            self.handle = None
+
+
+def render_vcm_batch(jobs, flags=0, stream=None, return_stats=False, keep_on_device=False, framebuffers=None):
     """Renders a list of jobs with sgpu_render_vcm_batch. Each job is a dict: scene (Scene), camera
     (dict), width, height, iterations, max_path_length, base_radius, radius_alpha, seed and optionally
     iteration_base. Returns the images (height x width x 3 each) or, with keep_on_device, the device
-    framebuffers."""
+    framebuffers; with `framebuffers` (DeviceBuffers, one per job) the images stay in those and
+    nothing is returned but the stats."""
     n = len(jobs)
     ctx = jobs[0]["scene"].ctx
     arr_j = (VcmJob * n)()
     scenes = (_P * n)()
-    fbs = [ctx.empty(3 * j["width"] * j["height"] * 4) for j in jobs]
+    if framebuffers is not None:
+        assert len(framebuffers) == n
+        keep_on_device = True
+    fbs = framebuffers if framebuffers is not None else [ctx.empty(3 * j["width"] * j["height"] * 4) for j in jobs]
     fb_ptrs = (_P * n)(*[b.ptr for b in fbs])
     for k, j in enumerate(jobs):
         scenes[k] = j["scene"].handle
@@ -452,7 +522,7 @@ def render_vcm_batch(jobs, flags=0, stream=None, return_stats=False, keep_on_dev
     try:
         check(lib.sgpu_render_vcm_batch(scenes, arr_j, fb_ptrs, n, flags, C.byref(st), stream))
         if keep_on_device:
-            out = fbs
+            out = None if framebuffers is not None else fbs
         else:
             out = [b.download(np.float32, 3 * j["width"] * j["height"]).reshape(j["height"], j["width"], 3)
                    for b, j in zip(fbs, jobs)]

@@ -14,6 +14,8 @@ BVH traversal microbenchmark (the 2M-triangle / 16M-ray workload):
 """
 from __future__ import annotations
 
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 from . import native as N
@@ -232,18 +234,25 @@ def quad_adjacency(quads):
 def grid_ptex(rng, nu, nv, res_log2, tint, lattice=(32, 16)):
     """Per-face RGB8 value-noise texture of an nu x nv grid mesh whose quad (i, j) is face
     j * nu + i with Ptex u along j and v along i (the quads of vcm_scene): a smooth field over the
-    whole mesh sampled at each face's texel centres, so neighbouring faces continue each other."""
+    whole mesh sampled at each face's texel centres, so neighbouring faces continue each other.
+    Channel c is round(255 tint_c (0.35 + 0.65 F_c)) with F_c = Wi L_c Wj^T in [0, 1]; since the
+    rows of Wi and Wj sum to 1 the affine map is applied to the lattice values L_c."""
     res = 1 << res_log2
-    Wi = _smooth_interp_matrix(nu * res, lattice[0], nu)   # along i (= face v)
-    Wj = _smooth_interp_matrix(nv * res, lattice[1], nv)   # along j (= face u)
-    tex = np.empty((nv, nu, res, res, 3), np.uint8)
+    Wi = _smooth_interp_matrix(nu * res, lattice[0], nu)  # along i (= face v)
+    Wj = _smooth_interp_matrix(nv * res, lattice[1], nv)  # along j (= face u)
+    nb = lattice[1] + 1
+    # T[(i, tv), (c, b)] = (Wi L_c)[i*res + tv, b]
+    T = np.concatenate([Wi @ (255 * tint[c] * (0.35 + 0.65 * rng.random((lattice[0] + 1, nb))))
+                        for c in range(3)], axis=1).astype(np.float32)
+    # block-diagonal right factor: Bm[(c, b), (j, tu, c)] = Wj[j*res + tu, b]
+    Bm = np.zeros((3, nb, nv, res, 3), np.float32)
+    Wj3 = Wj.reshape(nv, res, nb).transpose(2, 0, 1)
     for c in range(3):
-        L = rng.random((lattice[0] + 1, lattice[1] + 1))
-        F = Wi @ L @ Wj.T  # (nu*res, nv*res): [i*res + tv, j*res + tu]
-        val = np.clip(np.round(255 * tint[c] * (0.35 + 0.65 * F)), 0, 255).astype(np.uint8)
-        # face (i, j), texel row tv (along i), column tu (along j)
-        tex[..., c] = val.reshape(nu, res, nv, res).transpose(2, 0, 1, 3)
-    return tex.reshape(nv * nu, res, res, 3)
+        Bm[c, :, :, :, c] = Wj3
+    out = T @ Bm.reshape(3 * nb, nv * res * 3)  # [(i, tv), (j, tu, c)]
+    np.rint(out, out=out)  # in [0, 255] since tint <= 1
+    tex = out.astype(np.uint8).reshape(nu * res, nv, res * 3).transpose(1, 0, 2)  # [j, (i, tv), (tu, c)]
+    return np.ascontiguousarray(tex).reshape(nv * nu, res, res, 3)
 
 
 def geom_tri9(g):
@@ -277,7 +286,7 @@ def vcm_scene(seed: int = 3, n_meshes: int = 64, nu: int = 128, nv: int = 64, te
     (Selas's surfaces are one-sided for opaque materials: no normal flip in CalculateSurfaceParams).
     Merge radius 0.3 % of the scene diagonal."""
     rng = np.random.default_rng(seed)
-    geoms, mats = [], []
+    geoms, mats, tints = [], [], []
     centre = np.array([5.0, 5.0, 5.0])
     ii, jj = np.meshgrid(np.arange(nu), np.arange(nv), indexing="xy")  # quad (i, j) = face j * nu + i
     pid = lambda i, j: i * (nv + 1) + j  # vertex (i, j), i along x, j along z
@@ -302,10 +311,15 @@ def vcm_scene(seed: int = 3, n_meshes: int = 64, nu: int = 128, nv: int = 64, te
         mats.append(material(1, base, metallic, rng.uniform(0.05, 1.0), 1.5, spec_trans=spec_trans, flags=flags))
         g = {"positions": world, "quads": quads.astype(np.uint32)}
         if textures:
-            trng = np.random.default_rng((seed, m))
-            g["ptex"] = {"res_log2": res_log2, "texels": grid_ptex(trng, nu, nv, res_log2, base),
-                         "adjfaces": adj[0], "adjedges": adj[1]}
+            g["ptex"] = {"res_log2": res_log2, "texels": None, "adjfaces": adj[0], "adjedges": adj[1]}
+            tints.append(base)
         geoms.append(g)
+    if textures:  # each mesh's texture has its own stream (seed, m): synthesised in parallel
+        with ThreadPoolExecutor(min(8, n_meshes)) as ex:
+            texels = ex.map(lambda m: grid_ptex(np.random.default_rng((seed, m)), nu, nv, res_log2, tints[m]),
+                            range(n_meshes))
+            for g, t in zip(geoms, texels):
+                g["ptex"]["texels"] = t
     lights = np.zeros(2, LIGHT_DTYPE)
     lights[0] = (0, (5.0, 7.5, 5.0), (0.0, -1.0, 0.0), (1.5, 0.0, 0.0), (0.0, 0.0, 1.5), (20.0, 20.0, 20.0))
     lights[1] = (1, (4.0, 4.0, 6.0), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (15.0, 15.0, 15.0))
@@ -322,6 +336,40 @@ def vcm_scene(seed: int = 3, n_meshes: int = 64, nu: int = 128, nv: int = 64, te
         "radius_alpha": 0.75,
         "max_path_length": 10,
     }
+
+
+def sized_vcm_scene(seed: int, n_tris: float, textures: bool = True):
+    """vcm_scene with 64 meshes of nu x nv quads, nu = 2 nv, sized so that 128 nu nv is about
+    n_tris (9,216 triangles at 10K, 2,120,448 at 2M)."""
+    nv = max(2, int(round(np.sqrt(n_tris / 256.0))))
+    return vcm_scene(seed=seed, n_meshes=64, nu=2 * nv, nv=nv, textures=textures)
+
+
+def job_mix(seed: int, n_jobs: int, n_scenes: int = 0):
+    """Uneven VCM render jobs: square images with side log-uniform in [64, 1024] (pixel count
+    log-uniform in 64^2..1024^2), iterations uniform in 1..64, path length 10, per-job seeds. With
+    n_scenes == 0 every job has a scene of its own (seed, triangle count log-uniform in 10K..2M);
+    otherwise each job picks one of n_scenes shared scenes uniformly (scene ids 0..n_scenes-1, whose
+    sizes scene_sizes gives)."""
+    rng = np.random.default_rng(seed)
+    side = np.rint(np.exp(rng.uniform(np.log(64), np.log(1024), n_jobs))).astype(int)
+    its = rng.integers(1, 65, n_jobs)
+    seeds = rng.integers(0, 1 << 40, n_jobs)
+    if n_scenes:
+        scene = rng.integers(0, n_scenes, n_jobs)
+        tris = np.zeros(n_jobs)
+    else:
+        scene = np.arange(n_jobs)
+        tris = np.exp(rng.uniform(np.log(1e4), np.log(2e6), n_jobs))
+    return [{"width": int(side[k]), "height": int(side[k]), "iterations": int(its[k]), "max_path_length": 10,
+             "seed": int(seeds[k]), "scene": int(scene[k]), "scene_tris": float(tris[k])} for k in range(n_jobs)]
+
+
+def scene_sizes(seed: int, n_scenes: int):
+    """Triangle counts (log-uniform in 10K..2M) of n_scenes shared scenes; scene k is
+    sized_vcm_scene(seed * 1000 + k, size[k])."""
+    rng = np.random.default_rng((seed, n_scenes))
+    return np.exp(rng.uniform(np.log(1e4), np.log(2e6), n_scenes))
 
 
 def anyhit_tfar(t_closest: np.ndarray, prim: np.ndarray, seed: int = 5):
