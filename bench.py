@@ -12,6 +12,17 @@ algorithmic bytes per ray = 48 B of compulsory ray/hit traffic + nodes visited x
 tested x 48 B (counted over all closest-hit rays of an untimed iteration), divided by the kernel's
 device time summed from HIP events recorded around each launch on the render's stream.
 
+The default run adds two legs to the same line (skip them with --no-batch / --no-serve):
+"batched": 64 uneven jobs (square images with side log-uniform in [64, 1024], 1..64 iterations,
+each on a scene of its own of 10K..2M triangles), rendered as one batch and, for comparison, one job
+at a time with one wavefront per iteration (the plain GPU schedule); "serving": a Poisson stream of
+such requests over 16 resident scenes at --serve-rate requests/s through the request loop
+(selas_amd/serving.py), p50/p99 latency from arrival to the image on the host, beside the plain
+schedule on the same stream. The top-level p50_ms/p99_ms and vs_plain_gpu come from these legs.
+
+--workload batch / serve: the batched leg alone (with --solo, also one job at a time with its
+iterations packed), and the serving leg swept over --rates.
+
 --workload traversal: the BVH traversal microbenchmark (2,097,152 triangles in 256 Gaussian
 clusters, 16,777,216 rays, half primary and half bounce rays); one step = one closest-hit pass plus
 one any-hit pass over all rays; value in Mrays/s.
@@ -148,6 +159,14 @@ def bench_vcm(args):
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline_vcm(sc, w, args.cpu_iterations, min(16, os.cpu_count() or 1))
+    scene.release()
+    del sc
+    batch = serving = None
+    if not args.no_batch:
+        batch = batch_summary(run_batch(N, W, ctx, args.batch_jobs, args.batch_seed, args.solo))
+    if not args.no_serve:
+        serving = serving_summary(serve_sweep(N, W, ctx, [args.serve_rate], args.serve_jobs or 24,
+                                              args.batch_seed)[0])
 
     out = {
         "metric": METRIC,
@@ -191,7 +210,7 @@ def bench_vcm(args):
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": load_traffic("traffic_bvh_intersect_vcm_1Mtri.json"),
-            "kernel": "k_trace<false,false> (closest hit, all launches of the render)",
+            "kernel": "k_trace<false,false,false> (closest hit, all launches of the render)",
             "launches": st["closest_launches"],
             "avg_launch_ms": round(closest_ms / max(1, st["closest_launches"]), 4),
             "bytes_per_ray": round(bytes_per_ray, 1),
@@ -199,7 +218,14 @@ def bench_vcm(args):
             "tris_per_ray": round(tris_per_ray, 3),
         },
         "cpu_baseline": cpu,
+        "batched": batch,
+        "serving": serving,
     }
+    if serving:
+        out["p50_ms"] = serving["served"]["p50_ms"]
+        out["p99_ms"] = serving["served"]["p99_ms"]
+    if batch:
+        out["vs_plain_gpu"] = batch["vs_plain_gpu"]
     print(json.dumps(out), flush=True)
 
 
@@ -215,15 +241,11 @@ def _frac_close(a, b):
     return float((np.abs(a - b) <= 1e-3 * np.abs(b) + 1e-5).mean())
 
 
-def bench_batch(args):
-    """64 uneven VCM jobs, each on a scene of its own, rendered as one batch (sgpu_render_vcm_batch)
+def run_batch(N, W, ctx, n_jobs, seed, solo):
+    """n_jobs uneven VCM jobs, each on a scene of its own, rendered as one batch (sgpu_render_vcm_batch)
     and, for comparison, one job at a time with one wavefront per iteration (the plain GPU
-    schedule) and one job at a time with its iterations packed (sgpu_render_vcm)."""
-    from selas_amd import native as N
-    from selas_amd import workloads as W
-
-    mix = W.job_mix(args.batch_seed, args.batch_jobs)
-    ctx = N.Context(0)
+    schedule) and, with `solo`, one job at a time with its iterations packed (sgpu_render_vcm)."""
+    mix = W.job_mix(seed, n_jobs)
     t0 = time.perf_counter()
     jobs, tris = [], 0
     for k, m in enumerate(mix):
@@ -265,7 +287,7 @@ def bench_batch(args):
     plain_ms, plain_rays = one_at_a_time(N.VCM_ONE_SLICE_PER_WAVEFRONT, fb_plain)
     log(f"plain: {plain_ms:.1f} ms")
     solo_ms = None
-    if not args.no_solo:
+    if solo:
         solo_ms, _ = one_at_a_time(0, fb_plain)
         log(f"one job at a time, iterations packed: {solo_ms:.1f} ms")
     close = [_frac_close(a.download(np.float32, 3 * j["width"] * j["height"]),
@@ -273,44 +295,79 @@ def bench_batch(args):
              for a, b, j in zip(fb_batch, fb_plain, jobs)]
     for b in fb_batch + fb_plain:
         b.free()
+    for j in jobs:
+        j["scene"].release()
+    return {
+        "mix": mix,
+        "jobs": len(jobs),
+        "triangles_total": int(tris),
+        "samples": int(samples),
+        "samples_per_s": samples / (batch_ms / 1e3),
+        "ms": batch_ms,
+        "mrays_s": st["rays"] / (batch_ms / 1e3) / 1e6,
+        "wavefronts": st["wavefronts"],
+        "plain_ms": plain_ms,
+        "plain_samples_per_s": samples / (plain_ms / 1e3),
+        "plain_mrays_s": plain_rays / (plain_ms / 1e3) / 1e6,
+        "solo_ms": solo_ms,
+        "same_images_min_frac": min(close),
+    }
 
+
+def batch_summary(b):
+    return {
+        "jobs": b["jobs"],
+        "samples": b["samples"],
+        "triangles_total": b["triangles_total"],
+        "samples_per_s": round(b["samples_per_s"], 1),
+        "mrays_s": round(b["mrays_s"], 2),
+        "ms": round(b["ms"], 1),
+        "wavefronts": b["wavefronts"],
+        "plain_gpu": {"samples_per_s": round(b["plain_samples_per_s"], 1), "ms": round(b["plain_ms"], 1),
+                      "mrays_s": round(b["plain_mrays_s"], 2),
+                      "schedule": "one job at a time, one wavefront per iteration"},
+        "vs_plain_gpu": round(b["plain_ms"] / b["ms"], 3),
+        "solo_gpu": None if b["solo_ms"] is None else {
+            "samples_per_s": round(b["samples"] / (b["solo_ms"] / 1e3), 1), "ms": round(b["solo_ms"], 1),
+            "schedule": "one job at a time, its iterations packed into wavefronts (sgpu_render_vcm)"},
+        "same_images_min_frac": round(b["same_images_min_frac"], 5),
+    }
+
+
+def bench_batch(args):
+    from selas_amd import native as N
+    from selas_amd import workloads as W
+
+    ctx = N.Context(0)
+    b = run_batch(N, W, ctx, args.batch_jobs, args.batch_seed, args.solo)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline_batch(mix, args.cpu_iterations, min(16, os.cpu_count() or 1))
-    value = samples / (batch_ms / 1e3)
+        cpu = cpu_baseline_batch(b["mix"], args.cpu_iterations, min(16, os.cpu_count() or 1))
+    summ = batch_summary(b)
     out = {
         "metric": METRIC,
-        "value": round(value, 1),
+        "value": round(b["samples_per_s"], 1),
         "unit": "samples/s",
         "n_gpus": 1,
         "steps": 1,
         "warmup": args.warmup,
-        "ms_per_step": round(batch_ms, 3),
+        "ms_per_step": round(b["ms"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"vcm_batch_{len(jobs)}jobs",
-            "jobs": len(jobs),
+            "workload": f"vcm_batch_{b['jobs']}jobs",
+            "jobs": b["jobs"],
             "resolution": "square, side log-uniform in [64, 1024]",
             "iterations": "uniform in 1..64",
             "scenes": "one per job, 64 displaced-grid quad meshes, triangles log-uniform in 10K..2M, textured",
-            "triangles_total": int(tris),
-            "samples": int(samples),
+            "triangles_total": b["triangles_total"],
+            "samples": b["samples"],
             "max_path_length": 10,
         },
-        "mrays_s": round(st["rays"] / (batch_ms / 1e3) / 1e6, 2),
-        "wavefronts": st["wavefronts"],
-        "plain_gpu": {"samples_per_s": round(samples / (plain_ms / 1e3), 1), "ms": round(plain_ms, 1),
-                      "mrays_s": round(plain_rays / (plain_ms / 1e3) / 1e6, 2),
-                      "schedule": "one job at a time, one wavefront per iteration"},
-        "vs_plain_gpu": round(plain_ms / batch_ms, 3),
-        "solo_gpu": None if solo_ms is None else {
-            "samples_per_s": round(samples / (solo_ms / 1e3), 1), "ms": round(solo_ms, 1),
-            "schedule": "one job at a time, its iterations packed into wavefronts (sgpu_render_vcm)"},
-        "same_images_min_frac": round(min(close), 5),
+        **{k: v for k, v in summ.items() if k not in ("jobs", "samples", "triangles_total")},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -360,7 +417,8 @@ def serve_sweep(N, W, ctx, rates, n_jobs, seed, n_scenes=16, plain=True, budget=
         if base:
             S.RequestLoop(base, budget, max_slots=1).run([S.Request(0, 0.0, 0, 64, 64, 1, 1)])
         for rate in rates:
-            mix = W.job_mix(seed * 7919 + int(rate * 10), n_jobs, n_scenes=n_scenes)
+            n = n_jobs(rate) if callable(n_jobs) else n_jobs
+            mix = W.job_mix(seed * 7919 + int(rate * 10), n, n_scenes=n_scenes)
             reqs = S.poisson_requests(seed, rate, mix)
             loop = S.RequestLoop(served, budget, max_slots=64)
             loop.run(reqs)
@@ -383,6 +441,14 @@ def serve_sweep(N, W, ctx, rates, n_jobs, seed, n_scenes=16, plain=True, budget=
     return out
 
 
+def serving_summary(r):
+    out = {"rate_per_s": r["rate"], "served": _round(r["served"])}
+    if "plain" in r:
+        out["plain_gpu"] = _round(r["plain"])
+        out["vs_plain_gpu"] = round(r["served"]["samples_per_s"] / r["plain"]["samples_per_s"], 3)
+    return out
+
+
 def _round(d):
     return {k: (round(v, 2) if isinstance(v, float) else v) for k, v in d.items()}
 
@@ -393,7 +459,9 @@ def bench_serve(args):
 
     rates = [float(x) for x in args.rates.split(",")]
     ctx = N.Context(0)
-    res = serve_sweep(N, W, ctx, rates, args.serve_jobs, args.batch_seed, plain=not args.no_plain)
+    # requests per rate: about 8 s of arrivals, 16..64
+    n_jobs = args.serve_jobs or (lambda r: int(min(64, max(16, round(8 * r)))))
+    res = serve_sweep(N, W, ctx, rates, n_jobs, args.batch_seed, plain=not args.no_plain)
     top = max(res, key=lambda r: r["served"]["samples_per_s"])
     out = {
         "metric": METRIC,
@@ -409,9 +477,9 @@ def bench_serve(args):
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"vcm_serve_poisson_{len(rates)}rates_{args.serve_jobs}jobs",
+            "workload": f"vcm_serve_poisson_{len(rates)}rates",
             "rates_per_s": rates,
-            "jobs_per_rate": args.serve_jobs,
+            "jobs_per_rate": [r["served"]["jobs"] + r["served"]["rejected"] for r in res],
             "scenes": "16 resident, 64 displaced-grid quad meshes each, triangles log-uniform in 10K..2M, textured",
             "resolution": "square, side log-uniform in [64, 1024]",
             "iterations": "uniform in 1..64",
@@ -441,10 +509,14 @@ def main():
     ap.add_argument("--cpu-rays", type=int, default=4194304, help="rays in the CPU baseline sample")
     ap.add_argument("--batch-jobs", type=int, default=64, help="batch: jobs in the mix; serve: -")
     ap.add_argument("--batch-seed", type=int, default=7, help="batch/serve: seed of the job mix and scenes")
-    ap.add_argument("--no-solo", action="store_true", help="batch: skip the packed one-job-at-a-time leg")
+    ap.add_argument("--solo", action="store_true", help="batch: also time the packed one-job-at-a-time schedule")
     ap.add_argument("--rates", default="1,2,5,10,20,50,100,200", help="serve: Poisson arrival rates (jobs/s)")
-    ap.add_argument("--serve-jobs", type=int, default=24, help="serve: requests per rate")
+    ap.add_argument("--serve-rate", type=float, default=10.0, help="vcm: arrival rate of the serving leg")
+    ap.add_argument("--serve-jobs", type=int, default=None,
+                    help="requests per rate (default: 24 in the serving leg; 8 s of arrivals, 16..64, in serve)")
     ap.add_argument("--no-plain", action="store_true", help="serve: skip the plain one-job-at-a-time runs")
+    ap.add_argument("--no-batch", action="store_true", help="vcm: skip the batched leg")
+    ap.add_argument("--no-serve", action="store_true", help="vcm: skip the serving leg")
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the triangles (1.0 = the workload)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run sgpu_device_copy of BYTES (HBM counter calibration) and exit")
@@ -575,7 +647,7 @@ def bench_traversal(args):
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": load_traffic(),
-            "kernel": "k_trace<false,false> (closest hit)",
+            "kernel": "k_trace<false,false,false> (closest hit)",
             "bytes_per_ray": round(bytes_per_ray, 1),
             "nodes_per_ray": round(nodes_per_ray, 3),
             "tris_per_ray": round(tris_per_ray, 3),
