@@ -134,6 +134,7 @@ def bench_vcm(args):
     tris_per_ray = cnt["tris_tested"] / cnt["closest_rays"]
     bytes_per_ray = 48 + nodes_per_ray * 64 + tris_per_ray * 48
     log(f"closest hit in VCM: {nodes_per_ray:.2f} nodes/ray, {tris_per_ray:.2f} tris/ray, {bytes_per_ray:.1f} B/ray")
+    tuning = apply_tuning(args, ctx, lambda c: N.render_vcm(scene, cam, w, w, steps, L, r, alpha, 1, framebuffer=fb))
     if args.warmup:
         N.render_vcm(scene, cam, w, w, args.warmup, L, r, alpha, 1, framebuffer=fb)
     N.synchronize()
@@ -166,7 +167,7 @@ def bench_vcm(args):
         batch = batch_summary(run_batch(N, W, ctx, args.batch_jobs, args.batch_seed, args.solo))
     if not args.no_serve:
         serving = serving_summary(serve_sweep(N, W, ctx, [args.serve_rate], args.serve_jobs or 24,
-                                              args.batch_seed)[0])
+                                              args.batch_seed, wavefront_paths=args.serve_paths)[0])
 
     out = {
         "metric": METRIC,
@@ -218,6 +219,7 @@ def bench_vcm(args):
             "tris_per_ray": round(tris_per_ray, 3),
         },
         "cpu_baseline": cpu,
+        "tuning": tuning,
         "batched": batch,
         "serving": serving,
     }
@@ -227,6 +229,23 @@ def bench_vcm(args):
     if batch:
         out["vs_plain_gpu"] = batch["vs_plain_gpu"]
     print(json.dumps(out), flush=True)
+
+
+def apply_tuning(args, ctx, probe):
+    """Launch shape for the timed runs: given on the command line, the defaults (--no-tune), or
+    found by the autotuner (selas_amd/autotune.py) with `probe` (the timed workload, run once)."""
+    from selas_amd import autotune as A
+
+    if args.wavefront_paths or args.trace_blocks:
+        ctx.set_tuning(vcm_wavefront_paths=args.wavefront_paths, trace_blocks_per_cu=args.trace_blocks)
+        return dict(ctx.tuning(), source="command line")
+    if args.no_tune:
+        return dict(ctx.tuning(), source="defaults")
+    t0 = time.perf_counter()
+    r = A.tune(ctx, probe, log=log)
+    return dict(r["best"], source="autotuned", tuned_ms=round(r["seconds"] * 1e3, 1),
+                default_ms=None if r["default_seconds"] is None else round(r["default_seconds"] * 1e3, 1),
+                trials=len(r["trials"]), search_s=round(time.perf_counter() - t0, 1))
 
 
 def _build_scene(N, W, ctx, seed, tris):
@@ -241,7 +260,7 @@ def _frac_close(a, b):
     return float((np.abs(a - b) <= 1e-3 * np.abs(b) + 1e-5).mean())
 
 
-def run_batch(N, W, ctx, n_jobs, seed, solo):
+def run_batch(N, W, ctx, n_jobs, seed, solo, tune_args=None):
     """n_jobs uneven VCM jobs, each on a scene of its own, rendered as one batch (sgpu_render_vcm_batch)
     and, for comparison, one job at a time with one wavefront per iteration (the plain GPU
     schedule) and, with `solo`, one job at a time with its iterations packed (sgpu_render_vcm)."""
@@ -256,6 +275,14 @@ def run_batch(N, W, ctx, n_jobs, seed, solo):
                      "radius_alpha": alpha, "seed": m["seed"]})
     log(f"batch: {len(jobs)} scenes, {tris} triangles, built in {time.perf_counter() - t0:.1f} s")
     samples = sum(j["width"] * j["height"] * j["iterations"] for j in jobs)
+    tuning = None
+    if tune_args is not None:
+        # probe: the first 16 jobs with at most 8 iterations each
+        sub = [dict(j, iterations=min(j["iterations"], 8)) for j in jobs[:16]]
+        sub_fb = [ctx.empty(12 * j["width"] * j["height"]) for j in sub]
+        tuning = apply_tuning(tune_args, ctx, lambda c: N.render_vcm_batch(sub, framebuffers=sub_fb))
+        for b in sub_fb:
+            b.free()
     fb_batch = [ctx.empty(12 * j["width"] * j["height"]) for j in jobs]
     fb_plain = [ctx.empty(12 * j["width"] * j["height"]) for j in jobs]
     # warm-up (untimed): one iteration of the first jobs
@@ -311,6 +338,7 @@ def run_batch(N, W, ctx, n_jobs, seed, solo):
         "plain_mrays_s": plain_rays / (plain_ms / 1e3) / 1e6,
         "solo_ms": solo_ms,
         "same_images_min_frac": min(close),
+        "tuning": tuning,
     }
 
 
@@ -339,7 +367,7 @@ def bench_batch(args):
     from selas_amd import workloads as W
 
     ctx = N.Context(0)
-    b = run_batch(N, W, ctx, args.batch_jobs, args.batch_seed, args.solo)
+    b = run_batch(N, W, ctx, args.batch_jobs, args.batch_seed, args.solo, tune_args=args)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline_batch(b["mix"], args.cpu_iterations, min(16, os.cpu_count() or 1))
@@ -368,6 +396,7 @@ def bench_batch(args):
             "max_path_length": 10,
         },
         **{k: v for k, v in summ.items() if k not in ("jobs", "samples", "triangles_total")},
+        "tuning": b["tuning"],
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -392,7 +421,7 @@ def cpu_baseline_batch(mix, iterations, threads):
                       f"restatement (scalar C, oracle SAH tree, not Embree)"}
 
 
-def serve_sweep(N, W, ctx, rates, n_jobs, seed, n_scenes=16, plain=True, budget=200 << 30):
+def serve_sweep(N, W, ctx, rates, n_jobs, seed, n_scenes=16, plain=True, budget=200 << 30, wavefront_paths=1 << 20):
     """Poisson request streams at each rate, served by the request loop over resident scenes,
     and by the plain one-job-at-a-time GPU schedule on the same requests."""
     from selas_amd import serving as S
@@ -407,7 +436,7 @@ def serve_sweep(N, W, ctx, rates, n_jobs, seed, n_scenes=16, plain=True, budget=
     def no_build(sid):
         raise RuntimeError(f"scene {sid} is not resident")
 
-    served = S.GpuEngine(ctx, no_build, scenes=scenes)
+    served = S.GpuEngine(ctx, no_build, scenes=scenes, wavefront_paths=wavefront_paths)
     base = S.GpuEngine(ctx, no_build, scenes=scenes, plain=True) if plain else None
     out = []
     try:
@@ -461,7 +490,10 @@ def bench_serve(args):
     ctx = N.Context(0)
     # requests per rate: about 8 s of arrivals, 16..64
     n_jobs = args.serve_jobs or (lambda r: int(min(64, max(16, round(8 * r)))))
-    res = serve_sweep(N, W, ctx, rates, n_jobs, args.batch_seed, plain=not args.no_plain)
+    if args.trace_blocks:
+        ctx.set_tuning(trace_blocks_per_cu=args.trace_blocks)
+    res = serve_sweep(N, W, ctx, rates, n_jobs, args.batch_seed, plain=not args.no_plain,
+                      wavefront_paths=args.serve_paths)
     top = max(res, key=lambda r: r["served"]["samples_per_s"])
     out = {
         "metric": METRIC,
@@ -484,7 +516,7 @@ def bench_serve(args):
             "resolution": "square, side log-uniform in [64, 1024]",
             "iterations": "uniform in 1..64",
             "admission": "arrival order, 64 slots, HBM budget 200 GiB (scenes + framebuffers)",
-            "schedule": "continuous batching, wavefronts of 2^20 paths, least remaining work first",
+            "schedule": f"continuous batching, wavefronts of {args.serve_paths} paths, least remaining work first",
         },
         "sweep": [{"rate": r["rate"], "served": _round(r["served"]),
                    "plain": _round(r["plain"]) if "plain" in r else None} for r in res],
@@ -515,8 +547,12 @@ def main():
     ap.add_argument("--serve-jobs", type=int, default=None,
                     help="requests per rate (default: 24 in the serving leg; 8 s of arrivals, 16..64, in serve)")
     ap.add_argument("--no-plain", action="store_true", help="serve: skip the plain one-job-at-a-time runs")
+    ap.add_argument("--serve-paths", type=int, default=1 << 20, help="serving: paths per wavefront of the server")
     ap.add_argument("--no-batch", action="store_true", help="vcm: skip the batched leg")
     ap.add_argument("--no-serve", action="store_true", help="vcm: skip the serving leg")
+    ap.add_argument("--no-tune", action="store_true", help="use the default launch shape instead of autotuning")
+    ap.add_argument("--wavefront-paths", type=int, default=None, help="VCM paths per wavefront (skips autotuning)")
+    ap.add_argument("--trace-blocks", type=int, default=None, help="traversal blocks per CU (skips autotuning)")
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the triangles (1.0 = the workload)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run sgpu_device_copy of BYTES (HBM counter calibration) and exit")

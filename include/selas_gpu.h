@@ -89,9 +89,34 @@ typedef struct sgpu_trace_stats {
 /* ---- errors ----------------------------------------------------------------------------- */
 const char* sgpu_last_error(void);
 
-/* ---- context (replaces rtcNewDevice / rtcReleaseDevice, Source/Applications/Selas/Source/main.cpp:99) */
+/* ---- context (replaces rtcNewDevice / rtcReleaseDevice, Source/Applications/Selas/Source/main.cpp:99)
+ * A context keeps device memory between calls: the stream-ordered pool of its device holds freed
+ * memory for reuse, and the VCM wavefront buffers of the last sgpu_render_vcm(_batch) call stay
+ * allocated until a call needs larger ones or the context is destroyed. */
 int sgpu_context_create(int device, sgpu_context* out);
 int sgpu_context_destroy(sgpu_context ctx);
+
+/* Launch-shape parameters of a context (no counterpart in Selas; an autotuner sets them). Not to be
+ * changed while renders on the context are running. */
+typedef struct sgpu_tuning {
+    uint64_t vcm_wavefront_paths; /* paths per wavefront of sgpu_render_vcm(_batch): 2^12..2^24
+                                     (default 2^20); a larger image than this gets one slice each */
+    uint32_t trace_blocks_per_cu; /* traversal grid: at most this many 256-lane blocks per CU, the
+                                     rest of the rays taken grid-stride: 1..32 (default 8) */
+    uint32_t reserved;            /* 0 */
+} sgpu_tuning;
+int sgpu_set_tuning(sgpu_context ctx, const sgpu_tuning* tuning);
+int sgpu_get_tuning(sgpu_context ctx, sgpu_tuning* tuning);
+
+/* The context's device: name, compute units, device memory (total and free now). */
+typedef struct sgpu_device_info {
+    char name[128];
+    uint32_t compute_units;
+    uint32_t reserved;
+    uint64_t total_bytes;
+    uint64_t free_bytes;
+} sgpu_device_info;
+int sgpu_get_device_info(sgpu_context ctx, sgpu_device_info* info);
 
 /* ---- device memory helpers (plain hipMalloc/hipMemcpy wrappers for hosts without HIP headers) */
 int sgpu_malloc(sgpu_context ctx, size_t bytes, void** dptr);

@@ -59,13 +59,59 @@ int sgpu_context_create(int device, sgpu_context* out)
         delete ctx;
         return sgpu_fail(SGPU_E_HIP, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e));
     }
+    // stream-ordered allocations freed by one call stay mapped for the next (wavefront buffers are
+    // tens of GB at the tuned sizes; mapping them anew on every call costs more than the render)
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
+        uint64_t keep = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
     *out = ctx;
     return SGPU_OK;
 }
 
 int sgpu_context_destroy(sgpu_context ctx)
 {
+    if (!ctx) return SGPU_OK;
+    sgpu_vcm_release_cached(ctx);
     delete ctx;
+    return SGPU_OK;
+}
+
+int sgpu_set_tuning(sgpu_context ctx, const sgpu_tuning* t)
+{
+    if (!ctx || !t) return sgpu_fail(SGPU_E_INVALID, "sgpu_set_tuning: NULL argument");
+    if (t->vcm_wavefront_paths < (1u << 12) || t->vcm_wavefront_paths > (1u << 24))
+        return sgpu_fail(SGPU_E_RANGE, "sgpu_set_tuning: vcm_wavefront_paths outside 2^12..2^24");
+    if (t->trace_blocks_per_cu < 1 || t->trace_blocks_per_cu > kMaxTraceBlocksPerCu)
+        return sgpu_fail(SGPU_E_RANGE, "sgpu_set_tuning: trace_blocks_per_cu outside 1..32");
+    if (t->reserved != 0) return sgpu_fail(SGPU_E_INVALID, "sgpu_set_tuning: reserved must be 0");
+    ctx->vcm_wavefront_paths = t->vcm_wavefront_paths;
+    ctx->trace_blocks_per_cu = t->trace_blocks_per_cu;
+    return SGPU_OK;
+}
+
+int sgpu_get_tuning(sgpu_context ctx, sgpu_tuning* t)
+{
+    if (!ctx || !t) return sgpu_fail(SGPU_E_INVALID, "sgpu_get_tuning: NULL argument");
+    t->vcm_wavefront_paths = ctx->vcm_wavefront_paths;
+    t->trace_blocks_per_cu = ctx->trace_blocks_per_cu;
+    t->reserved = 0;
+    return SGPU_OK;
+}
+
+int sgpu_get_device_info(sgpu_context ctx, sgpu_device_info* info)
+{
+    if (!ctx || !info) return sgpu_fail(SGPU_E_INVALID, "sgpu_get_device_info: NULL argument");
+    std::memset(info, 0, sizeof(*info));
+    // some runtimes leave the marketing name empty: then the architecture name (gfx950:...)
+    std::strncpy(info->name, ctx->props.name[0] ? ctx->props.name : ctx->props.gcnArchName, sizeof(info->name) - 1);
+    info->compute_units = (uint32_t)ctx->props.multiProcessorCount;
+    SGPU_HIP_TRY(hipSetDevice(ctx->device));
+    size_t free_b = 0, total_b = 0;
+    SGPU_HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    info->total_bytes = total_b;
+    info->free_bytes = free_b;
     return SGPU_OK;
 }
 

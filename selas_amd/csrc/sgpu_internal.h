@@ -41,9 +41,18 @@ static_assert(sizeof(TriRecord) == 48, "TriRecord must be 48 bytes");
 #define SGPU_STACK_LDS 16
 #define SGPU_TRACE_BLOCK 256
 
+// The traversal grid is at most kMaxTraceBlocksPerCu blocks per CU (sgpu_tuning bound); spill
+// buffers are sized for it, so any tuning within the bound fits any buffer.
+constexpr uint32_t kMaxTraceBlocksPerCu = 32;
+
 struct sgpu_context_s {
     int device;
     hipDeviceProp_t props;
+    uint64_t vcm_wavefront_paths = 1u << 20;
+    uint32_t trace_blocks_per_cu = 8;
+    // the VCM server of the last sgpu_render_vcm(_batch) call, reused while large enough
+    std::mutex vcm_mu;
+    sgpu_vcm_server vcm_cache = nullptr;
 };
 
 struct sgpu_scene_s {
@@ -126,3 +135,5 @@ int sgpu_vcm_server_step_impl(sgpu_vcm_server sv, hipStream_t stream, uint64_t* 
                               uint32_t* n_done, uint32_t* n_active);
 int sgpu_vcm_server_stats_impl(sgpu_vcm_server sv, sgpu_vcm_stats* st);
 uint64_t sgpu_vcm_server_bytes(sgpu_context ctx, const sgpu_vcm_server_config* cfg);
+// Frees the server a context keeps between sgpu_render_vcm(_batch) calls (at context destruction).
+void sgpu_vcm_release_cached(sgpu_context ctx);

@@ -15,6 +15,8 @@
 // Closest hit: smallest t, ties to the smaller global triangle index (= smaller (geomID, primID)).
 #include "sgpu_internal.h"
 
+#include <algorithm>
+
 namespace {
 
 __device__ __forceinline__ bool tri_test(const TriRecord& r, const f3& o, const f3& d, float tnear, float& t,
@@ -236,7 +238,7 @@ int trace_launch(sgpu_context ctx, const BvhNode* nodes, const TriRecord* tris, 
     if (n == 0) return SGPU_OK;
     const uint32_t cus = (uint32_t)ctx->props.multiProcessorCount;
     uint32_t blocks = (n + SGPU_TRACE_BLOCK - 1) / SGPU_TRACE_BLOCK;
-    const uint32_t max_blocks = cus * 8;
+    const uint32_t max_blocks = cus * std::min(ctx->trace_blocks_per_cu, kMaxTraceBlocksPerCu);
     if (blocks > max_blocks) blocks = max_blocks;
     const uint32_t cap = max_depth + 1;
 
@@ -274,7 +276,7 @@ int trace_launch(sgpu_context ctx, const BvhNode* nodes, const TriRecord* tris, 
 size_t sgpu_trace_spill_words(sgpu_context ctx, uint32_t max_depth)
 {
     const uint32_t cus = (uint32_t)ctx->props.multiProcessorCount;
-    const uint32_t threads = cus * 8 * SGPU_TRACE_BLOCK;
+    const uint32_t threads = cus * kMaxTraceBlocksPerCu * SGPU_TRACE_BLOCK;
     const uint32_t cap = max_depth + 1;
     const uint32_t spill_depth = cap > SGPU_STACK_LDS ? cap - SGPU_STACK_LDS : 1;
     return (size_t)spill_depth * threads;

@@ -910,30 +910,51 @@ __global__ __launch_bounds__(kBlock) void k_cam_shade(Wave w, uint32_t count, co
 // and the group adds its eight partial sums with a fixed butterfly ((c0+c1)+(c2+c3))+((c4+c5)+
 // (c6+c7)). A cell's vertices are contiguous (grid order); positions are tested against the radius
 // before the 64-byte vertex is read. MergeVertices (VCM.cpp:362-409) per vertex found.
+// Vertex merging (SearchHashGrid + MergeVertices, VCMHashGrid.cpp:107-157, VCM.cpp:362-409), one
+// query per half-wave (32 lanes), the queries of a bounce taken grid-stride. The lanes of a half
+// scan the photons of their query's 2x2x2 cells as one list (the cells' ranges concatenated in the
+// reference's visiting order, strided over the lanes), append those within the radius to the half's
+// LDS list (ballot and prefix count, list order = scan order) and evaluate the camera BSDF for up to
+// 32 listed photons at a time, one per lane, both halves in the same pass: an evaluation pass then
+// serves two queries, and the acceptance test does not make whole waves wait on every BSDF
+// evaluation. Each lane sums its photons in list order; a fixed butterfly over the half's lanes
+// gives the query's sum.
+constexpr uint32_t kMergeWaves = kBlock / 64;
+
 __global__ __launch_bounds__(kBlock) void k_cam_merge(Wave w, uint32_t count, const CamVertex* __restrict cam_vm,
                                                       const uint32_t* __restrict path_slice, GridView G, CamSlots S)
 {
-    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t j = t >> 3, cell_k = t & 7u;
-    bool inside = false;
-    f3 acc = mk3(0, 0, 0);
-    CamVertex v;
-    v.geom_plen = kNoQuery;
-    if (j < count) v = cam_vm[j];
-    if (v.geom_plen != kNoQuery && *G.nv_ptr > 0) {
-        const uint32_t si = path_slice[v.path];
-        const Slice& sl = w.slices[si];
-        const uint32_t* box = G.box + 8 * si;
-        float lo[3] = {ord2f(box[0]), ord2f(box[1]), ord2f(box[2])};
-        float hi[3] = {ord2f(~box[3]), ord2f(~box[4]), ord2f(~box[5])};
-        inside = true;
-        float dmin[3];
-        for (int k = 0; k < 3; ++k) {
-            dmin[k] = v.pos[k] - lo[k];
-            float dmax = hi[k] - v.pos[k];
-            if (dmin[k] < 0.0f || dmax < 0.0f) inside = false;
+    __shared__ uint32_t list_lds[kMergeWaves][2][64];
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    const uint32_t half = lane >> 5, hl = lane & 31u;  // half-wave and lane within it
+    uint32_t* list = list_lds[wid][half];
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const unsigned long long half_bits = half ? 0xFFFFFFFF00000000ull : 0x00000000FFFFFFFFull;
+    if (*G.nv_ptr == 0) return;
+    const uint32_t stride = 2 * gridDim.x * kMergeWaves;
+    for (uint32_t j0 = 2 * (blockIdx.x * kMergeWaves + wid); j0 < count; j0 += stride) {
+        // ---- this half's query (none past the end, or without a query, or outside the photon box) ----
+        const uint32_t j = j0 + half;
+        CamVertex v;
+        v.geom_plen = kNoQuery;
+        if (j < count) v = cam_vm[j];
+        bool active = v.geom_plen != kNoQuery;
+        uint32_t si = 0;
+        float dmin[3] = {0.0f, 0.0f, 0.0f};
+        if (active) {
+            si = path_slice[v.path];
+            const uint32_t* box = G.box + 8 * si;
+            for (int k = 0; k < 3; ++k) {
+                dmin[k] = v.pos[k] - ord2f(box[k]);
+                const float dmax = ord2f(~box[3 + k]) - v.pos[k];
+                if (dmin[k] < 0.0f || dmax < 0.0f) active = false;
+            }
         }
-        if (inside) {
+        if (__ballot(active) == 0) continue;
+        const Slice& sl = w.slices[si];
+        // lanes 0..7 of the half: cell k = (x, y, z) bits of k, its photon range, offset in the scan list
+        uint32_t cb = 0, cn = 0;
+        if (active && hl < 8) {
             int32_t mn[3];
             for (int k = 0; k < 3; ++k) {
                 float cp = sl.inv_cell * dmin[k];
@@ -942,54 +963,95 @@ __global__ __launch_bounds__(kBlock) void k_cam_merge(Wave w, uint32_t count, co
                 mn[k] = (int32_t)cf;
                 if (fr < 0.5f) --mn[k];
             }
-            const uint32_t cell =
-                sl.path_base + cell_hash(mn[0] + (int32_t)(cell_k & 1u), mn[1] + (int32_t)((cell_k >> 1) & 1u),
-                                         mn[2] + (int32_t)(cell_k >> 2), sl.N);
-            const uint32_t b = cell == 0 ? 0 : G.cell_ends[cell - 1], e = G.cell_ends[cell];
-            const f3 x = mk3(v.pos[0], v.pos[1], v.pos[2]);
-            const f3 view = mk3(v.view[0], v.view[1], v.view[2]);
-            const uint32_t plen = v.geom_plen & 31u;
-            Surface s;
-            s.position = x;
-            s.error = 0.0f;
+            const uint32_t cell = sl.path_base + cell_hash(mn[0] + (int32_t)(hl & 1u), mn[1] + (int32_t)((hl >> 1) & 1u),
+                                                           mn[2] + (int32_t)(hl >> 2), sl.N);
+            cb = cell == 0 ? 0 : G.cell_ends[cell - 1];
+            cn = G.cell_ends[cell] - cb;
+        }
+        uint32_t incl = cn;
+        for (uint32_t d = 1; d < 8; d <<= 1) {
+            const uint32_t y = (uint32_t)__shfl_up((int)incl, d, 32);
+            if (hl >= d) incl += y;
+        }
+        const uint32_t excl = incl - cn;
+        uint32_t ends[8];
+        for (int k = 0; k < 8; ++k) ends[k] = (uint32_t)__shfl((int)incl, k, 32);
+        const uint32_t total = ends[7];  // 0 for a half without a query
+
+        const f3 x = mk3(v.pos[0], v.pos[1], v.pos[2]);
+        const f3 view = mk3(v.view[0], v.view[1], v.view[2]);
+        const uint32_t plen = v.geom_plen & 31u;
+        Surface s;
+        s.position = x;
+        s.error = 0.0f;
+        float ccp = 0.0f;
+        if (active) {
             surface_material(sl.materials + 22 * (size_t)sl.geom_material[v.geom_plen >> 5],
                              mk3(v.n[0], v.n[1], v.n[2]), view, nullptr, s);
             s.base_color = mk3(v.base[0], v.base[1], v.base[2]);
-            const float ccp = cont_prob(s);
-            for (uint32_t q = b; q < e; ++q) {
+            ccp = cont_prob(s);
+        }
+        f3 acc = mk3(0, 0, 0);
+        auto merge = [&](uint32_t q) {
+            const LightVertex& lv = G.sverts[q];
+            const uint32_t lgp = lv.geom_plen;
+            if (plen + (lgp & 31u) > sl.L) return;
+            const float* lmat = sl.materials + 22 * (size_t)sl.geom_material[lgp >> 5];
+            // the reference's transparency rule (VCM.cpp:381-383)
+            if ((s.flags & kMatTransparent) != ((uint32_t)lmat[21] & kMatTransparent)) return;
+            f3 lw = mk3(lv.wfix[0], lv.wfix[1], lv.wfix[2]);
+            float fwd, rev;
+            f3 f = bsdf_eval(s, view, lw, fwd, rev);
+            if (is_zero3(f)) return;
+            float cos_l = absf_(dot3(s.n, lw));
+            fwd *= ccp;
+            rev *= lv.cp;
+            float w_light = lv.dvcm * sl.vc_weight + lv.dvm * fwd;
+            float w_cam = v.dvcm * sl.vc_weight + v.dvm * rev;
+            float mis = 1.0f / (w_light + 1.0f + w_cam);
+            acc += (f * mk3(lv.thr[0], lv.thr[1], lv.thr[2])) * (mis / cos_l);
+        };
+        uint32_t listed = 0;  // same in every lane of the half
+        for (uint32_t c0 = 0; __ballot(c0 < total) != 0; c0 += 32) {
+            const uint32_t i = c0 + hl;
+            // every lane takes part in the shuffles (lanes past the end look at their last photon)
+            const uint32_t ic = i < total ? i : (total > 0 ? total - 1 : 0);
+            uint32_t k = 0;
+            for (int m = 0; m < 7; ++m) k += ic >= ends[m] ? 1u : 0u;
+            const uint32_t q = (uint32_t)__shfl((int)cb, (int)k, 32) + (ic - (uint32_t)__shfl((int)excl, (int)k, 32));
+            bool near = false;
+            if (i < total) {
                 const float4 lp = G.spos[q];
-                f3 dv = x - mk3(lp.x, lp.y, lp.z);
-                if (!(dot3(dv, dv) <= sl.radius_sq)) continue;
-                const LightVertex& lv = G.sverts[q];
-                const uint32_t lgp = lv.geom_plen;
-                if (plen + (lgp & 31u) > sl.L) continue;
-                const float* lmat = sl.materials + 22 * (size_t)sl.geom_material[lgp >> 5];
-                // the reference's transparency rule (VCM.cpp:381-383)
-                if ((s.flags & kMatTransparent) != ((uint32_t)lmat[21] & kMatTransparent)) continue;
-                f3 lw = mk3(lv.wfix[0], lv.wfix[1], lv.wfix[2]);
-                float fwd, rev;
-                f3 f = bsdf_eval(s, view, lw, fwd, rev);
-                if (is_zero3(f)) continue;
-                float cos_l = absf_(dot3(s.n, lw));
-                fwd *= ccp;
-                rev *= lv.cp;
-                float w_light = lv.dvcm * sl.vc_weight + lv.dvm * fwd;
-                float w_cam = v.dvcm * sl.vc_weight + v.dvm * rev;
-                float mis = 1.0f / (w_light + 1.0f + w_cam);
-                acc += (f * mk3(lv.thr[0], lv.thr[1], lv.thr[2])) * (mis / cos_l);
+                const f3 dv = x - mk3(lp.x, lp.y, lp.z);
+                near = dot3(dv, dv) <= sl.radius_sq;
+            }
+            const unsigned long long m = __ballot(near) & half_bits;
+            if (near) list[listed + (uint32_t)__popcll(m & below)] = q;
+            listed += (uint32_t)__popcll(m);
+            __builtin_amdgcn_wave_barrier();
+            if (__ballot(listed >= 32) != 0) {
+                // both halves evaluate up to 32 listed photons and keep the rest
+                const uint32_t mine = hl < listed ? list[hl] : 0u;
+                const uint32_t rest = hl + 32 < listed ? list[hl + 32] : 0u;
+                __builtin_amdgcn_wave_barrier();
+                if (hl + 32 < listed) list[hl] = rest;
+                __builtin_amdgcn_wave_barrier();
+                if (hl < listed) merge(mine);
+                listed = listed > 32 ? listed - 32 : 0;
             }
         }
-    }
-    for (int off = 1; off < 8; off <<= 1) {
-        acc.x += __shfl_xor(acc.x, off, 64);
-        acc.y += __shfl_xor(acc.y, off, 64);
-        acc.z += __shfl_xor(acc.z, off, 64);
-    }
-    if (inside && cell_k == 0) {
-        const uint32_t p = v.path;
-        const Slice& sl = w.slices[path_slice[p]];
-        s3(S.c, p * w.nslots + w.nslots - 1, (mk3(v.thr[0], v.thr[1], v.thr[2]) * acc) * sl.vm_norm);
-        S.valid[(size_t)p * w.nslots + w.nslots - 1] = 1;
+        if (hl < listed) merge(list[hl]);
+        __builtin_amdgcn_wave_barrier();
+        for (int off = 1; off < 32; off <<= 1) {
+            acc.x += __shfl_xor(acc.x, off, 32);
+            acc.y += __shfl_xor(acc.y, off, 32);
+            acc.z += __shfl_xor(acc.z, off, 32);
+        }
+        if (active && hl == 0) {
+            const uint32_t p = v.path;
+            s3(S.c, p * w.nslots + w.nslots - 1, (mk3(v.thr[0], v.thr[1], v.thr[2]) * acc) * sl.vm_norm);
+            S.valid[(size_t)p * w.nslots + w.nslots - 1] = 1;
+        }
     }
 }
 
@@ -1148,6 +1210,7 @@ struct sgpu_vcm_server_s {
     sgpu_vcm_server_config cfg;
     hipStream_t alloc_stream = nullptr;
     uint32_t TM = 0, mlv = 0, nslots = 0, max_slices = 0;
+    uint32_t cap = 0;  // paths per wavefront in use (<= TM: a cached server may serve smaller requests)
     unsigned key_bits = 1;
     Arena A;
     // wavefront buffers
@@ -1310,6 +1373,21 @@ int server_upload_table(sgpu_vcm_server sv, hipStream_t stream)
 
 }  // namespace
 
+// A server made ready for a new set of jobs: no jobs, no scenes, zero totals, `cfg`'s wavefront
+// size (at most the allocated one) and flags.
+static void server_reset(sgpu_vcm_server sv, const sgpu_vcm_server_config& cfg)
+{
+    sv->active.clear();
+    sv->scenes.clear();
+    sv->table_dirty = true;
+    sv->max_depth = 0;
+    sv->cfg.flags = cfg.flags;
+    sv->cap = (uint32_t)std::min<uint64_t>(cfg.wavefront_paths, sv->TM);
+    sv->n_closest = sv->n_shadow = sv->n_light_verts = sv->n_launches = sv->n_samples = sv->n_waves = 0;
+    sv->trav_nodes = sv->trav_tris = 0;
+    for (int k = 0; k < kPhases; ++k) sv->ms[k] = 0.0;
+}
+
 int sgpu_vcm_server_create_impl(sgpu_context ctx, const sgpu_vcm_server_config* cfg, hipStream_t stream,
                                 sgpu_vcm_server* out)
 {
@@ -1319,6 +1397,7 @@ int sgpu_vcm_server_create_impl(sgpu_context ctx, const sgpu_vcm_server_config* 
     sv->cfg = *cfg;
     sv->A.stream = stream;
     sv->TM = (uint32_t)cfg->wavefront_paths;
+    sv->cap = sv->TM;
     sv->mlv = cfg->max_path_length > 2 ? cfg->max_path_length - 2 : 1;
     sv->nslots = sv->mlv + 2;
     sv->max_slices = kMaxSlicesPerWave;
@@ -1361,7 +1440,7 @@ int sgpu_vcm_server_submit_impl(sgpu_vcm_server sv, sgpu_scene sc, const sgpu_vc
 {
     const float pi = 3.141592653589793f;
     const uint32_t W = jb->camera.width, H = jb->camera.height, L = jb->max_path_length;
-    if ((uint64_t)W * H > sv->TM) return sgpu_fail(SGPU_E_RANGE, "sgpu_vcm_server_submit: image larger than a wavefront");
+    if ((uint64_t)W * H > sv->cap) return sgpu_fail(SGPU_E_RANGE, "sgpu_vcm_server_submit: image larger than a wavefront");
     if (L > sv->cfg.max_path_length)
         return sgpu_fail(SGPU_E_RANGE, "sgpu_vcm_server_submit: path length above the server's maximum");
     if (sc->max_depth > kMaxSceneDepth) return sgpu_fail(SGPU_E_RANGE, "sgpu_vcm_server_submit: tree too deep");
@@ -1433,7 +1512,7 @@ int sgpu_vcm_server_step_impl(sgpu_vcm_server sv, hipStream_t stream, uint64_t* 
         ServerJob& j = sv->active[ji];
         Run run{ji, ns, 0};
         while (j.next + run.count < j.job.iterations && ns < sv->max_slices) {
-            if (ns > 0 && (one_slice || (uint64_t)total + j.base.N > sv->TM)) break;
+            if (ns > 0 && (one_slice || (uint64_t)total + j.base.N > sv->cap)) break;
             Slice sl = j.base;
             const uint32_t gi = j.job.iteration_base + j.next + run.count;
             // CalculateIterationConstants (VCMCommon.cpp:97-113), vmCount = W*H, vcCount = 1
@@ -1615,8 +1694,10 @@ int sgpu_vcm_server_step_impl(sgpu_vcm_server sv, hipStream_t stream, uint64_t* 
         SGPU_HIP_TRY(hipGetLastError());
         SGPU_HIP_TRY(T.end());
         SGPU_HIP_TRY(T.begin(kPhaseMerge));
-        hipLaunchKernelGGL(k_cam_merge, dim3(blocks_for(8ull * count)), dim3(kBlock), 0, stream, wv, count, sv->cam_vm,
-                           ps.slice, G, S);
+        const uint32_t merge_blocks = std::min<uint32_t>((count + 2 * kMergeWaves - 1) / (2 * kMergeWaves),
+                                                         16u * (uint32_t)sv->ctx->props.multiProcessorCount);
+        hipLaunchKernelGGL(k_cam_merge, dim3(merge_blocks), dim3(kBlock), 0, stream, wv, count, sv->cam_vm, ps.slice,
+                           G, S);
         SGPU_HIP_TRY(hipGetLastError());
         SGPU_HIP_TRY(T.end());
         SGPU_HIP_TRY(hipMemcpyAsync(h, sv->counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
@@ -1705,7 +1786,6 @@ int sgpu_vcm_server_stats_impl(sgpu_vcm_server sv, sgpu_vcm_stats* st)
 
 // Paths per wavefront the one-shot renders aim for: enough lanes to fill the chip in every
 // bounce's launch.
-constexpr uint64_t kTargetPaths = 1u << 20;
 
 int sgpu_vcm_render_batch_impl(const sgpu_scene* scenes, const sgpu_vcm_job* jobs, float* const* framebuffers,
                                uint32_t n_jobs, uint32_t flags, sgpu_vcm_stats* stats, hipStream_t stream)
@@ -1722,19 +1802,49 @@ int sgpu_vcm_render_batch_impl(const sgpu_scene* scenes, const sgpu_vcm_job* job
     uint64_t needed = 0;
     for (uint32_t j = 0; j < n_jobs; ++j)
         needed += (uint64_t)jobs[j].camera.width * jobs[j].camera.height * jobs[j].iterations;
-    cfg.wavefront_paths = std::max<uint64_t>(largest, std::min<uint64_t>(kTargetPaths, needed));
+    cfg.wavefront_paths = std::max<uint64_t>(largest, std::min<uint64_t>(scenes[0]->ctx->vcm_wavefront_paths, needed));
     cfg.max_path_length = L;
     cfg.flags = flags;
+    // the context keeps the last server it built and reuses it while it is large enough (its
+    // buffers are then already mapped); a call that finds it in use builds a private one
+    sgpu_context ctx = scenes[0]->ctx;
+    std::unique_lock<std::mutex> lk(ctx->vcm_mu, std::try_to_lock);
     sgpu_vcm_server sv = nullptr;
-    int rc = sgpu_vcm_server_create_impl(scenes[0]->ctx, &cfg, stream, &sv);
-    if (rc) return rc;
+    int rc = SGPU_OK;
+    if (lk.owns_lock() && ctx->vcm_cache && ctx->vcm_cache->TM >= cfg.wavefront_paths &&
+        ctx->vcm_cache->cfg.max_path_length >= cfg.max_path_length) {
+        sv = ctx->vcm_cache;
+        server_reset(sv, cfg);
+    } else {
+        if (lk.owns_lock() && ctx->vcm_cache) {
+            sgpu_vcm_server_destroy_impl(ctx->vcm_cache, stream);
+            ctx->vcm_cache = nullptr;
+        }
+        rc = sgpu_vcm_server_create_impl(ctx, &cfg, stream, &sv);
+        if (rc) return rc;
+        if (lk.owns_lock()) ctx->vcm_cache = sv;
+    }
     for (uint32_t j = 0; j < n_jobs && rc == SGPU_OK; ++j)
         rc = sgpu_vcm_server_submit_impl(sv, scenes[j], &jobs[j], framebuffers[j], nullptr, stream);
     uint32_t n_active = n_jobs;
     while (rc == SGPU_OK && n_active > 0) rc = sgpu_vcm_server_step_impl(sv, stream, nullptr, 0, nullptr, &n_active);
     if (rc == SGPU_OK && stats) sgpu_vcm_server_stats_impl(sv, stats);
-    sgpu_vcm_server_destroy_impl(sv, stream);
+    if (sv != ctx->vcm_cache || !lk.owns_lock()) {
+        sgpu_vcm_server_destroy_impl(sv, stream);
+    } else {
+        // nothing of this call stays referenced: jobs that failed mid-way are dropped, scenes forgotten
+        sgpu_vcm_server_config keep = sv->cfg;
+        keep.flags = 0;
+        server_reset(sv, keep);
+    }
     return rc;
+}
+
+void sgpu_vcm_release_cached(sgpu_context ctx)
+{
+    std::lock_guard<std::mutex> lk(ctx->vcm_mu);
+    if (ctx->vcm_cache) sgpu_vcm_server_destroy_impl(ctx->vcm_cache, nullptr);
+    ctx->vcm_cache = nullptr;
 }
 
 int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* fb, sgpu_vcm_stats* stats,

@@ -52,6 +52,9 @@ _U32P = C.POINTER(C.c_uint32)
 _SIGNATURES = {
     "sgpu_context_create": [C.c_int, C.POINTER(_P)],
     "sgpu_context_destroy": [_P],
+    "sgpu_set_tuning": [_P, _P],
+    "sgpu_get_tuning": [_P, _P],
+    "sgpu_get_device_info": [_P, _P],
     "sgpu_malloc": [_P, C.c_size_t, C.POINTER(_P)],
     "sgpu_free": [_P, _P],
     "sgpu_memcpy_htod": [_P, _P, C.c_size_t, _P],
@@ -202,11 +205,37 @@ class DeviceBuffer:
             pass
 
 
+class Tuning(C.Structure):
+    _fields_ = [("vcm_wavefront_paths", C.c_uint64), ("trace_blocks_per_cu", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 128), ("compute_units", C.c_uint32), ("reserved", C.c_uint32),
+                ("total_bytes", C.c_uint64), ("free_bytes", C.c_uint64)]
+
+
 class Context:
     def __init__(self, device: int = 0):
         h = _P()
         check(lib.sgpu_context_create(device, C.byref(h)))
         self.handle = h.value
+
+    def device_info(self) -> dict:
+        d = DeviceInfo()
+        check(lib.sgpu_get_device_info(self.handle, C.byref(d)))
+        return {"name": d.name.decode(errors="replace"), "compute_units": d.compute_units,
+                "total_bytes": d.total_bytes, "free_bytes": d.free_bytes}
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        check(lib.sgpu_get_tuning(self.handle, C.byref(t)))
+        return {"vcm_wavefront_paths": t.vcm_wavefront_paths, "trace_blocks_per_cu": t.trace_blocks_per_cu}
+
+    def set_tuning(self, vcm_wavefront_paths=None, trace_blocks_per_cu=None):
+        cur = self.tuning()
+        t = Tuning(cur["vcm_wavefront_paths"] if vcm_wavefront_paths is None else int(vcm_wavefront_paths),
+                   cur["trace_blocks_per_cu"] if trace_blocks_per_cu is None else int(trace_blocks_per_cu), 0)
+        check(lib.sgpu_set_tuning(self.handle, C.byref(t)))
 
     def array(self, a: np.ndarray) -> DeviceBuffer:
         a = np.ascontiguousarray(a)

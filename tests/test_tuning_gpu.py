@@ -1,0 +1,45 @@
+"""Launch-shape tuning (sgpu_set_tuning): values round-trip, out-of-range values are refused, and
+images do not depend on the launch shape (slices are independent of how they are packed)."""
+import numpy as np
+import pytest
+
+from selas_amd import native as N
+from selas_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def test_tuning_roundtrip_and_limits(gpu_ctx):
+    old = gpu_ctx.tuning()
+    try:
+        gpu_ctx.set_tuning(vcm_wavefront_paths=1 << 14, trace_blocks_per_cu=3)
+        assert gpu_ctx.tuning() == {"vcm_wavefront_paths": 1 << 14, "trace_blocks_per_cu": 3}
+        for bad in ({"vcm_wavefront_paths": 1 << 11}, {"vcm_wavefront_paths": (1 << 24) + 1},
+                    {"trace_blocks_per_cu": 0}, {"trace_blocks_per_cu": 33}):
+            with pytest.raises(N.SgpuError):
+                gpu_ctx.set_tuning(**bad)
+        assert gpu_ctx.tuning() == {"vcm_wavefront_paths": 1 << 14, "trace_blocks_per_cu": 3}
+        info = gpu_ctx.device_info()
+        assert info["compute_units"] > 0 and info["total_bytes"] >= info["free_bytes"] > 0 and info["name"]
+    finally:
+        gpu_ctx.set_tuning(**old)
+
+
+def test_images_do_not_depend_on_launch_shape(gpu_ctx):
+    sc = W.vcm_scene(seed=31, n_meshes=8, nu=16, nv=8)
+    scene = N.build_render_scene(gpu_ctx, sc)
+    old = gpu_ctx.tuning()
+    try:
+        imgs = []
+        for paths, blocks in ((1 << 12, 1), (1 << 14, 8), (1 << 16, 32)):
+            gpu_ctx.set_tuning(vcm_wavefront_paths=paths, trace_blocks_per_cu=blocks)
+            img, st = N.render_vcm(scene, sc["camera"], 48, 40, 9, 10, sc["merge_radius"], 0.75, 5, return_stats=True)
+            imgs.append((img, st["wavefronts"]))
+        # 1920 paths per iteration: 2, 8 and all 9 iterations per wavefront
+        assert [w for _, w in imgs] == [5, 2, 1]
+        for img, _ in imgs[1:]:
+            close = np.abs(img - imgs[0][0]) <= 1e-3 * np.abs(imgs[0][0]) + 1e-5
+            assert close.mean() >= 0.999
+    finally:
+        gpu_ctx.set_tuning(**old)
+        scene.release()
