@@ -127,14 +127,19 @@ def bench_vcm(args):
     cam = sc["camera"]
     fb = ctx.empty(3 * w * w * 4)
 
-    # traversal counts over the closest-hit rays of one untimed iteration
+    tuning = apply_tuning(args, ctx, lambda c: N.render_vcm(scene, cam, w, w, steps, L, r, alpha, 1, framebuffer=fb))
+    # traversal counts over the closest-hit rays of one untimed iteration, on the tree in use
     _, cnt = N.render_vcm(scene, cam, w, w, 1, L, r, alpha, 1, iteration_base=0, return_stats=True,
                           flags=N.VCM_COUNT_TRAVERSAL, framebuffer=fb)
+    node_bytes = 128 if ctx.tuning()["trace_tree"] == 1 else 64
     nodes_per_ray = cnt["nodes_visited"] / cnt["closest_rays"]
     tris_per_ray = cnt["tris_tested"] / cnt["closest_rays"]
-    bytes_per_ray = 48 + nodes_per_ray * 64 + tris_per_ray * 48
-    log(f"closest hit in VCM: {nodes_per_ray:.2f} nodes/ray, {tris_per_ray:.2f} tris/ray, {bytes_per_ray:.1f} B/ray")
-    tuning = apply_tuning(args, ctx, lambda c: N.render_vcm(scene, cam, w, w, steps, L, r, alpha, 1, framebuffer=fb))
+    bytes_per_ray = 48 + nodes_per_ray * node_bytes + tris_per_ray * 48
+    log(f"closest hit in VCM: {nodes_per_ray:.2f} nodes/ray ({node_bytes} B), {tris_per_ray:.2f} tris/ray, "
+        f"{bytes_per_ray:.1f} B/ray")
+    # warm-up: the timed job itself, untimed (allocates the context's wavefront buffers at the
+    # size the timed render uses), then `warmup` more iterations
+    N.render_vcm(scene, cam, w, w, steps, L, r, alpha, 1, framebuffer=fb)
     if args.warmup:
         N.render_vcm(scene, cam, w, w, args.warmup, L, r, alpha, 1, framebuffer=fb)
     N.synchronize()
@@ -211,7 +216,9 @@ def bench_vcm(args):
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": load_traffic("traffic_bvh_intersect_vcm_1Mtri.json"),
-            "kernel": "k_trace<false,false,false> (closest hit, all launches of the render)",
+            "kernel": ("k_trace8<false,false,false> (closest hit, 8-wide nodes, all launches of the render)"
+                       if node_bytes == 128 else "k_trace<false,false,false> (closest hit, all launches of the render)"),
+            "node_bytes": node_bytes,
             "launches": st["closest_launches"],
             "avg_launch_ms": round(closest_ms / max(1, st["closest_launches"]), 4),
             "bytes_per_ray": round(bytes_per_ray, 1),
@@ -236,6 +243,8 @@ def apply_tuning(args, ctx, probe):
     found by the autotuner (selas_amd/autotune.py) with `probe` (the timed workload, run once)."""
     from selas_amd import autotune as A
 
+    if args.trace_tree is not None:
+        ctx.set_tuning(trace_tree=args.trace_tree)
     if args.wavefront_paths or args.trace_blocks:
         ctx.set_tuning(vcm_wavefront_paths=args.wavefront_paths, trace_blocks_per_cu=args.trace_blocks)
         return dict(ctx.tuning(), source="command line")
@@ -553,6 +562,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="use the default launch shape instead of autotuning")
     ap.add_argument("--wavefront-paths", type=int, default=None, help="VCM paths per wavefront (skips autotuning)")
     ap.add_argument("--trace-blocks", type=int, default=None, help="traversal blocks per CU (skips autotuning)")
+    ap.add_argument("--trace-tree", type=int, default=None, choices=(0, 1),
+                    help="traversal tree: 0 binary, 1 8-wide compressed (default: the context's)")
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the triangles (1.0 = the workload)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run sgpu_device_copy of BYTES (HBM counter calibration) and exit")
@@ -597,6 +608,11 @@ def bench_traversal(args):
     n = int(rays["org_x"].shape[0])
 
     ctx = N.Context(0)
+    if args.trace_tree is not None:
+        ctx.set_tuning(trace_tree=args.trace_tree)
+    if args.trace_blocks:
+        ctx.set_tuning(trace_blocks_per_cu=args.trace_blocks)
+    wide = ctx.tuning()["trace_tree"] == 1
     stream = None
     scene = N.Scene(ctx)
     scene.add_soup(t9)
@@ -669,7 +685,8 @@ def bench_traversal(args):
             "triangles": int(t9.shape[0]),
             "rays": n,
             "passes": "closest hit + any hit",
-            "bvh": "device LBVH, binary nodes (64 B), 1 triangle per leaf (48 B)",
+            "bvh": ("device LBVH collapsed to 8-wide compressed nodes (128 B), triangles as children (48 B)"
+                    if wide else "device LBVH, binary nodes (64 B), 1 triangle per leaf (48 B)"),
         },
         "closest_hit_mrays_s": round(n / (avg_isect / 1e3) / 1e6, 2),
         "any_hit_mrays_s": round(n / (avg_occ / 1e3) / 1e6, 2),
@@ -683,7 +700,7 @@ def bench_traversal(args):
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": load_traffic(),
-            "kernel": "k_trace<false,false,false> (closest hit)",
+            "kernel": "k_trace8<false,false,false> (closest hit)" if wide else "k_trace<false,false,false> (closest hit)",
             "bytes_per_ray": round(bytes_per_ray, 1),
             "nodes_per_ray": round(nodes_per_ray, 3),
             "tris_per_ray": round(tris_per_ray, 3),

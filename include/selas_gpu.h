@@ -103,7 +103,8 @@ typedef struct sgpu_tuning {
                                      (default 2^20); a larger image than this gets one slice each */
     uint32_t trace_blocks_per_cu; /* traversal grid: at most this many 256-lane blocks per CU, the
                                      rest of the rays taken grid-stride: 1..32 (default 8) */
-    uint32_t reserved;            /* 0 */
+    uint32_t trace_tree;          /* traversal tree: 0 binary nodes (64 B), 1 8-wide compressed
+                                     nodes (128 B, default); both are built at commit */
 } sgpu_tuning;
 int sgpu_set_tuning(sgpu_context ctx, const sgpu_tuning* tuning);
 int sgpu_get_tuning(sgpu_context ctx, sgpu_tuning* tuning);

@@ -74,7 +74,8 @@ def tune(ctx, probe, wavefront_paths=None, blocks_per_cu=None, memory_budget=Non
     size = min(fits, key=lambda s: cost(s, grid))
     best = {"vcm_wavefront_paths": size, "trace_blocks_per_cu": grid}
     ctx.set_tuning(**best)
-    return {"best": best, "seconds": cost(size, grid), "default_seconds": default_seconds, "trials": trials}
+    return {"best": dict(ctx.tuning(), **best), "seconds": cost(size, grid), "default_seconds": default_seconds,
+            "trials": trials}
 
 
 def cache_key(ctx, workload: str) -> str:

@@ -18,8 +18,10 @@
 // closest hits do not depend on the tree's shape.
 #include "sgpu_internal.h"
 
+#include <algorithm>
 #include <cstring>  // rocprim/iterator/texture_cache_iterator.hpp uses memset without including it
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 namespace {
 
@@ -283,6 +285,142 @@ __global__ __launch_bounds__(kBlock) void k_depth(int n, const int* __restrict p
     atomicMax(maxdepth, d);
 }
 
+// ---- 8-wide collapse (breadth-first, one level per pair of launches) ----
+//
+// A wide node starts from its binary node's two children and repeatedly opens the internal child
+// of largest surface area (the first such child on ties) until it has 8 children or only leaves.
+// k_wide_expand writes each task's children (binary refs and boxes) and its count of internal
+// children; an exclusive scan of those counts numbers the next level's nodes in task order, so
+// the numbering (and the tree) is deterministic; k_wide_emit quantizes the boxes and writes the
+// node and the next level's tasks.
+
+__device__ __forceinline__ float half_area(const float* b)
+{
+    float dx = fmaxf(b[3] - b[0], 0.0f), dy = fmaxf(b[4] - b[1], 0.0f), dz = fmaxf(b[5] - b[2], 0.0f);
+    return dx * dy + dy * dz + dz * dx;
+}
+
+__device__ __forceinline__ void child_box(const BvhNode& nd, int c, float* b)
+{
+    b[0] = c ? nd.x.z : nd.x.x;
+    b[1] = c ? nd.y.z : nd.y.x;
+    b[2] = c ? nd.z.z : nd.z.x;
+    b[3] = c ? nd.x.w : nd.x.y;
+    b[4] = c ? nd.y.w : nd.y.y;
+    b[5] = c ? nd.z.w : nd.z.y;
+}
+
+__global__ __launch_bounds__(kBlock) void k_wide_expand(const int* __restrict tasks, uint32_t n_tasks,
+                                                        const BvhNode* __restrict bn, int* __restrict cref,
+                                                        float* __restrict cbox, uint32_t* __restrict nchild,
+                                                        uint32_t* __restrict ninternal)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n_tasks) return;
+    int ref[8];
+    float box[8][6];
+    const BvhNode root = bn[tasks[t]];
+    ref[0] = root.child.x;
+    ref[1] = root.child.y;
+    child_box(root, 0, box[0]);
+    child_box(root, 1, box[1]);
+    uint32_t m = 2;
+    while (m < 8) {
+        int best = -1;
+        float best_a = -1.0f;
+        for (uint32_t k = 0; k < m; ++k)
+            if (ref[k] >= 0) {
+                float a = half_area(box[k]);
+                if (a > best_a) {
+                    best_a = a;
+                    best = (int)k;
+                }
+            }
+        if (best < 0) break;
+        const BvhNode nd = bn[ref[best]];
+        ref[best] = nd.child.x;
+        child_box(nd, 0, box[best]);
+        ref[m] = nd.child.y;
+        child_box(nd, 1, box[m]);
+        ++m;
+    }
+    uint32_t internal = 0;
+    for (uint32_t k = 0; k < m; ++k) {
+        cref[8 * (size_t)t + k] = ref[k];
+        for (int a = 0; a < 6; ++a) cbox[48 * (size_t)t + 6 * k + a] = box[k][a];
+        internal += ref[k] >= 0 ? 1u : 0u;
+    }
+    nchild[t] = m;
+    ninternal[t] = internal;
+}
+
+// Smallest e in [-126, 127] with ext <= 254 * 2^e (ext >= 0).
+__device__ __forceinline__ int plane_exponent(double ext)
+{
+    if (!(ext > 0.0)) return -126;
+    int k;
+    double f = frexp(ext / 254.0, &k);  // ext / 254 = f * 2^k, f in [0.5, 1)
+    int e = f == 0.5 ? k - 1 : k;
+    return e < -126 ? -126 : (e > 127 ? 127 : e);
+}
+
+__global__ __launch_bounds__(kBlock) void k_wide_emit(uint32_t n_tasks, uint32_t level_base, uint32_t next_base,
+                                                      const uint32_t* __restrict offsets, const int* __restrict cref,
+                                                      const float* __restrict cbox, const uint32_t* __restrict nchild,
+                                                      WideNode* __restrict wide, int* __restrict next_tasks,
+                                                      uint32_t* __restrict bad)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n_tasks) return;
+    const uint32_t m = nchild[t];
+    const float* B = cbox + 48 * (size_t)t;
+    float lo[3] = {B[0], B[1], B[2]}, hi[3] = {B[3], B[4], B[5]};
+    for (uint32_t k = 1; k < m; ++k)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fminf(lo[a], B[6 * k + a]);
+            hi[a] = fmaxf(hi[a], B[6 * k + 3 + a]);
+        }
+    WideNode w;
+    w.ox = lo[0];
+    w.oy = lo[1];
+    w.oz = lo[2];
+    uint32_t meta = m << 24;
+    for (int a = 0; a < 3; ++a) {
+        w.qlo[a][0] = w.qlo[a][1] = 0;
+        w.qhi[a][0] = w.qhi[a][1] = 0;
+        const int e = plane_exponent((double)hi[a] - (double)lo[a]);
+        const float sc = __uint_as_float((uint32_t)(e + 127) << 23);
+        const double sd = (double)sc;
+        meta |= (uint32_t)(e + 127) << (8 * a);
+        for (uint32_t k = 0; k < m; ++k) {
+            const float clo = B[6 * k + a], chi = B[6 * k + 3 + a];
+            double ql = floor(((double)clo - (double)lo[a]) / sd);
+            double qh = ceil(((double)chi - (double)lo[a]) / sd);
+            int qlo = ql < 0.0 ? 0 : (ql > 255.0 ? 255 : (int)ql);
+            int qhi = qh < 0.0 ? 0 : (qh > 255.0 ? 255 : (int)qh);
+            // the decoded planes (one FP32 rounding, as the traversal decodes them) must contain the box
+            while (qlo > 0 && fmaf((float)qlo, sc, lo[a]) > clo) --qlo;
+            while (qhi < 255 && fmaf((float)qhi, sc, lo[a]) < chi) ++qhi;
+            if (fmaf((float)qlo, sc, lo[a]) > clo || fmaf((float)qhi, sc, lo[a]) < chi) atomicOr(bad, 1u);
+            w.qlo[a][k >> 2] |= (uint32_t)qlo << (8 * (k & 3));
+            w.qhi[a][k >> 2] |= (uint32_t)qhi << (8 * (k & 3));
+        }
+    }
+    w.meta = meta;
+    uint32_t rank = 0;
+    for (uint32_t k = 0; k < 8; ++k) {
+        int r = k < m ? cref[8 * (size_t)t + k] : -1;
+        if (k < m && r >= 0) {
+            next_tasks[offsets[t] + rank] = r;
+            r = (int)(next_base + offsets[t] + rank);
+            ++rank;
+        }
+        w.child[k] = r;
+    }
+    for (int k = 0; k < 8; ++k) w.pad[k] = 0;
+    wide[level_base + t] = w;
+}
+
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
@@ -292,6 +430,70 @@ struct DevBuf {
         if (p) (void)hipFree(p);
     }
 };
+
+// Collapses the binary tree (nodes, n_int internal nodes, root 0) into 8-wide nodes.
+int build_wide(sgpu_scene scene, const BvhNode* nodes, uint32_t n_int, hipStream_t stream)
+{
+    DevBuf<int> d_tasks, d_next, d_cref;
+    DevBuf<float> d_cbox;
+    DevBuf<uint32_t> d_nch, d_nint, d_off, d_bad;
+    DevBuf<WideNode> d_all;
+    DevBuf<uint8_t> d_tmp;
+    SGPU_HIP_TRY(d_tasks.alloc(n_int));
+    SGPU_HIP_TRY(d_next.alloc(n_int));
+    SGPU_HIP_TRY(d_cref.alloc(8 * (size_t)n_int));
+    SGPU_HIP_TRY(d_cbox.alloc(48 * (size_t)n_int));
+    SGPU_HIP_TRY(d_nch.alloc(n_int));
+    SGPU_HIP_TRY(d_nint.alloc(n_int));
+    SGPU_HIP_TRY(d_off.alloc(n_int));
+    SGPU_HIP_TRY(d_bad.alloc(1));
+    SGPU_HIP_TRY(d_all.alloc(n_int));
+    size_t tmp_bytes = 0;
+    SGPU_HIP_TRY(rocprim::exclusive_scan(nullptr, tmp_bytes, d_nint.p, d_off.p, 0u, (size_t)n_int,
+                                         rocprim::plus<uint32_t>(), stream));
+    SGPU_HIP_TRY(d_tmp.alloc(tmp_bytes));
+    const int root = 0;
+    SGPU_HIP_TRY(hipMemcpyAsync(d_tasks.p, &root, sizeof(int), hipMemcpyHostToDevice, stream));
+    SGPU_HIP_TRY(hipMemsetAsync(d_bad.p, 0, sizeof(uint32_t), stream));
+    uint32_t level_base = 0, n_tasks = 1, depth = 0;
+    while (n_tasks > 0) {
+        const uint32_t grid = (n_tasks + kBlock - 1) / kBlock;
+        hipLaunchKernelGGL(k_wide_expand, dim3(grid), dim3(kBlock), 0, stream, d_tasks.p, n_tasks, nodes, d_cref.p,
+                           d_cbox.p, d_nch.p, d_nint.p);
+        SGPU_HIP_TRY(hipGetLastError());
+        SGPU_HIP_TRY(rocprim::exclusive_scan(d_tmp.p, tmp_bytes, d_nint.p, d_off.p, 0u, (size_t)n_tasks,
+                                             rocprim::plus<uint32_t>(), stream));
+        uint32_t last[2] = {0, 0};
+        SGPU_HIP_TRY(hipMemcpyAsync(&last[0], d_off.p + n_tasks - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        SGPU_HIP_TRY(hipMemcpyAsync(&last[1], d_nint.p + n_tasks - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        const uint32_t next_base = level_base + n_tasks;
+        hipLaunchKernelGGL(k_wide_emit, dim3(grid), dim3(kBlock), 0, stream, n_tasks, level_base, next_base, d_off.p,
+                           d_cref.p, d_cbox.p, d_nch.p, d_all.p, d_next.p, d_bad.p);
+        SGPU_HIP_TRY(hipGetLastError());
+        SGPU_HIP_TRY(hipStreamSynchronize(stream));
+        const uint32_t n_next = last[0] + last[1];
+        if ((uint64_t)next_base + n_next > n_int) return sgpu_fail(SGPU_E_RANGE, "wide tree larger than the binary tree");
+        std::swap(d_tasks.p, d_next.p);
+        level_base = next_base;
+        n_tasks = n_next;
+        ++depth;
+    }
+    uint32_t bad = 0;
+    SGPU_HIP_TRY(hipMemcpy(&bad, d_bad.p, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (bad) return sgpu_fail(SGPU_E_RANGE, "wide node quantization could not contain a child box");
+    WideNode* wide = nullptr;
+    SGPU_HIP_TRY(hipMalloc((void**)&wide, (size_t)level_base * sizeof(WideNode)));
+    hipError_t e = hipMemcpyAsync(wide, d_all.p, (size_t)level_base * sizeof(WideNode), hipMemcpyDeviceToDevice, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) {
+        (void)hipFree(wide);
+        return sgpu_fail(SGPU_E_HIP, std::string("wide tree copy: ") + hipGetErrorString(e));
+    }
+    scene->d_wide = wide;
+    scene->n_wide = level_base;
+    scene->wide_depth = depth;
+    return SGPU_OK;
+}
 
 }  // namespace
 
@@ -383,7 +585,9 @@ int sgpu_bvh_build(sgpu_scene scene, hipStream_t stream)
         scene->d_tris = tris;
         scene->n_nodes = 1;
         scene->max_depth = 1;
-        return SGPU_OK;
+        int rc = build_wide(scene, nodes, 1, stream);
+        scene->stack_cap = std::max(scene->max_depth + 1, kWideMaxPush * scene->wide_depth + 1);
+        return rc;
     }
 
     hipLaunchKernelGGL(k_morton, dim3(grid), dim3(kBlock), 0, stream, d_cent.p, n, d_bounds.p, d_keys.p, d_vals.p);
@@ -417,5 +621,7 @@ int sgpu_bvh_build(sgpu_scene scene, hipStream_t stream)
     scene->d_tris = tris;
     scene->n_nodes = n - 1;
     scene->max_depth = depth;
-    return SGPU_OK;
+    int rc = build_wide(scene, nodes, n - 1, stream);
+    scene->stack_cap = std::max(scene->max_depth + 1, kWideMaxPush * scene->wide_depth + 1);
+    return rc;
 }

@@ -85,9 +85,10 @@ int sgpu_set_tuning(sgpu_context ctx, const sgpu_tuning* t)
         return sgpu_fail(SGPU_E_RANGE, "sgpu_set_tuning: vcm_wavefront_paths outside 2^12..2^24");
     if (t->trace_blocks_per_cu < 1 || t->trace_blocks_per_cu > kMaxTraceBlocksPerCu)
         return sgpu_fail(SGPU_E_RANGE, "sgpu_set_tuning: trace_blocks_per_cu outside 1..32");
-    if (t->reserved != 0) return sgpu_fail(SGPU_E_INVALID, "sgpu_set_tuning: reserved must be 0");
+    if (t->trace_tree > 1) return sgpu_fail(SGPU_E_RANGE, "sgpu_set_tuning: trace_tree must be 0 or 1");
     ctx->vcm_wavefront_paths = t->vcm_wavefront_paths;
     ctx->trace_blocks_per_cu = t->trace_blocks_per_cu;
+    ctx->trace_tree = t->trace_tree;
     return SGPU_OK;
 }
 
@@ -96,7 +97,7 @@ int sgpu_get_tuning(sgpu_context ctx, sgpu_tuning* t)
     if (!ctx || !t) return sgpu_fail(SGPU_E_INVALID, "sgpu_get_tuning: NULL argument");
     t->vcm_wavefront_paths = ctx->vcm_wavefront_paths;
     t->trace_blocks_per_cu = ctx->trace_blocks_per_cu;
-    t->reserved = 0;
+    t->trace_tree = ctx->trace_tree;
     return SGPU_OK;
 }
 
@@ -341,6 +342,7 @@ int sgpu_scene_release(sgpu_scene scene)
     if (!scene) return SGPU_OK;
     if (scene->d_nodes) (void)hipFree(scene->d_nodes);
     if (scene->d_tris) (void)hipFree(scene->d_tris);
+    if (scene->d_wide) (void)hipFree(scene->d_wide);
     if (scene->d_overflow) (void)hipFree(scene->d_overflow);
     if (scene->d_materials) (void)hipFree(scene->d_materials);
     if (scene->d_geom_material) (void)hipFree(scene->d_geom_material);
@@ -641,7 +643,8 @@ int sgpu_scene_device_bytes(sgpu_scene scene, uint64_t* bytes)
 {
     if (!scene || !bytes) return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_device_bytes: NULL argument");
     if (!scene->committed) return sgpu_fail(SGPU_E_STATE, "sgpu_scene_device_bytes: scene not committed");
-    uint64_t b = (uint64_t)scene->n_nodes * sizeof(BvhNode) + (uint64_t)scene->n_tris * sizeof(TriRecord) +
+    uint64_t b = (uint64_t)scene->n_nodes * sizeof(BvhNode) + (uint64_t)scene->n_wide * sizeof(WideNode) +
+                 (uint64_t)scene->n_tris * sizeof(TriRecord) +
                  sizeof(uint32_t) + (uint64_t)scene->n_materials * 22 * sizeof(float) +
                  (uint64_t)scene->n_geom_material * sizeof(uint32_t) + (uint64_t)scene->n_lights * sizeof(sgpu_light) +
                  (uint64_t)scene->n_geom_tex * sizeof(sgpu::PtexTex);
@@ -722,7 +725,7 @@ int sgpu_intersect_stats(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu
     stats->rays = n;
     stats->nodes_visited = h[0];
     stats->tris_tested = h[1];
-    stats->bytes_node = sizeof(BvhNode);
+    stats->bytes_node = scene->ctx->trace_tree == 1 ? sizeof(WideNode) : sizeof(BvhNode);
     stats->bytes_tri = sizeof(TriRecord);
     return SGPU_OK;
 }

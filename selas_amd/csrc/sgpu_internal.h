@@ -37,6 +37,22 @@ struct TriRecord {
 };
 static_assert(sizeof(TriRecord) == 48, "TriRecord must be 48 bytes");
 
+// 8-wide node, 128 B (one cache line), built by collapsing the binary tree. Child boxes are
+// quantized to 8 bits per plane relative to the node's box: plane = fma(q, 2^(e - 127), origin)
+// in FP32 (q * 2^(e - 127) is exact, so the decode rounds once), with q chosen in the builder so
+// that the decoded box contains the binary tree's (inflated) child box. child[c] >= 0: wide node
+// index; < 0: ~(triangle record index); slots c >= nchild are unused.
+struct WideNode {
+    float ox, oy, oz;
+    uint32_t meta;       // ex | ey << 8 | ez << 16 | nchild << 24 (biased exponents of the plane scales)
+    uint32_t qlo[3][2];  // axis a, child c: byte (c & 3) of qlo[a][c >> 2]
+    uint32_t qhi[3][2];
+    int32_t child[8];
+    uint32_t pad[8];
+};
+static_assert(sizeof(WideNode) == 128, "WideNode must be 128 bytes");
+constexpr uint32_t kWideMaxPush = 7;  // stack entries a wide node can push
+
 // Slots of the LDS part of the per-lane traversal stack; deeper entries spill to global memory.
 #define SGPU_STACK_LDS 16
 #define SGPU_TRACE_BLOCK 256
@@ -50,6 +66,7 @@ struct sgpu_context_s {
     hipDeviceProp_t props;
     uint64_t vcm_wavefront_paths = 1u << 20;
     uint32_t trace_blocks_per_cu = 8;
+    uint32_t trace_tree = 1;  // 0: binary nodes, 1: 8-wide compressed nodes
     // the VCM server of the last sgpu_render_vcm(_batch) call, reused while large enough
     std::mutex vcm_mu;
     sgpu_vcm_server vcm_cache = nullptr;
@@ -69,6 +86,10 @@ struct sgpu_scene_s {
     uint32_t max_depth = 0;
     BvhNode* d_nodes = nullptr;
     TriRecord* d_tris = nullptr;
+    WideNode* d_wide = nullptr;  // the same tree collapsed to 8-wide nodes (breadth-first order)
+    uint32_t n_wide = 0;
+    uint32_t wide_depth = 0;
+    uint32_t stack_cap = 0;      // traversal stack entries either tree can need
     // set to non-zero by a traversal whose stack would exceed the tree depth (never expected)
     uint32_t* d_overflow = nullptr;
     // shading data (sgpu_scene_set_materials / sgpu_scene_set_lights)
@@ -97,26 +118,28 @@ int sgpu_fail(int code, const std::string& msg);
     } while (0)
 
 // ---- internal entry points between translation units --------------------------------------------
-// Builds d_nodes/d_tris for the scene's triangles (sgpu_bvh.hip).
+// Builds d_nodes/d_tris (and the 8-wide d_wide) for the scene's triangles (sgpu_bvh.hip).
 int sgpu_bvh_build(sgpu_scene scene, hipStream_t stream);
-// Launches the traversal kernels (sgpu_trace.hip). `spill` (may be NULL: then the call allocates
-// its own, stream-ordered) holds the spilled part of the per-lane stacks: at least
-// sgpu_trace_spill_words(scene) 32-bit words, not shared with a concurrent launch.
+// Launches the traversal kernels (sgpu_trace.hip), on the context's tree kind (trace_tree).
+// `spill` (may be NULL: then the call allocates its own, stream-ordered) holds the spilled part of
+// the per-lane stacks: at least sgpu_trace_spill_words(scene) 32-bit words, not shared with a
+// concurrent launch.
 int sgpu_trace_launch(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits,
                       uint32_t* occluded, uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream,
                       uint32_t* spill = nullptr);
 size_t sgpu_trace_spill_words(sgpu_scene scene);
-size_t sgpu_trace_spill_words(sgpu_context ctx, uint32_t max_depth);
-// A scene's tree as a multi-scene traversal launch reads it (one table entry per scene of a batch).
+size_t sgpu_trace_spill_words(sgpu_context ctx, uint32_t stack_cap);
+// A scene's trees as a multi-scene traversal launch reads them (one table entry per scene).
 struct TraceScene {
     const BvhNode* nodes;
+    const WideNode* wide;
     const TriRecord* tris;
     uint32_t n_tris;
     uint32_t pad;
 };
 // One launch over rays of several scenes: ray i traverses table[ray_scene[i]] (device arrays);
-// max_depth is the largest tree depth in the table.
-int sgpu_trace_launch_multi(sgpu_context ctx, const TraceScene* table, const uint32_t* ray_scene, uint32_t max_depth,
+// stack_cap is the largest stack_cap of the table's scenes.
+int sgpu_trace_launch_multi(sgpu_context ctx, const TraceScene* table, const uint32_t* ray_scene, uint32_t stack_cap,
                             uint32_t* overflow, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits,
                             uint32_t* occluded, uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream,
                             uint32_t* spill = nullptr);

@@ -54,6 +54,46 @@ __device__ __forceinline__ float safe_rcp(float x)
     return __builtin_amdgcn_rcpf(y);
 }
 
+
+// The query's result: occlusion flag, or Embree's hit fields (quad hits reported as the quad and its
+// own (u, v): triangle 2q + h of quad q, second triangle (1 - u, 1 - v)).
+template <bool ANY>
+__device__ __forceinline__ void write_result(uint32_t i, bool hit_any, uint32_t best_tri, float best_t, float best_u,
+                                             float best_v, const TriRecord* __restrict rt, sgpu_hits_soa& hits,
+                                             uint32_t* __restrict occluded)
+{
+    if (ANY) {
+        occluded[i] = hit_any ? 1u : 0u;
+        return;
+    }
+    bool found = best_tri != 0xFFFFFFFFu;
+    uint32_t prim = SGPU_INVALID_ID, geom = SGPU_INVALID_ID;
+    f3 ng = mk3(0.0f, 0.0f, 0.0f);
+    if (found) {
+        const TriRecord r = rt[best_tri];
+        geom = __float_as_uint(r.b.w);
+        prim = __float_as_uint(r.c.w);
+        if (hits.ng_x) ng = cross3(mk3(r.b.x, r.b.y, r.b.z), mk3(r.c.x, r.c.y, r.c.z));
+        if (prim & 0x80000000u) {
+            if (prim & 1u) {
+                best_u = 1.0f - best_u;
+                best_v = 1.0f - best_v;
+            }
+            prim = (prim & 0x7FFFFFFFu) >> 1;
+        }
+    }
+    if (hits.t) hits.t[i] = best_t;
+    if (hits.u) hits.u[i] = found ? best_u : 0.0f;
+    if (hits.v) hits.v[i] = found ? best_v : 0.0f;
+    if (hits.prim_id) hits.prim_id[i] = prim;
+    if (hits.geom_id) hits.geom_id[i] = geom;
+    if (hits.ng_x) {
+        hits.ng_x[i] = ng.x;
+        hits.ng_y[i] = ng.y;
+        hits.ng_z[i] = ng.z;
+    }
+}
+
 template <bool ANY, bool STATS, bool MULTI>
 __global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace(const BvhNode* __restrict nodes,
                                                             const TriRecord* __restrict tris, uint32_t n_tris,
@@ -171,37 +211,202 @@ __global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace(const BvhNode* __res
             if (sp >= SGPU_STACK_LDS) lds_stack[slot][lane] = spill[(size_t)(sp - SGPU_STACK_LDS) * stride + gtid];
         }
 
-        if (ANY) {
-            occluded[i] = hit_any ? 1u : 0u;
-        } else {
-            bool found = best_tri != 0xFFFFFFFFu;
-            uint32_t prim = SGPU_INVALID_ID, geom = SGPU_INVALID_ID;
-            f3 ng = mk3(0.0f, 0.0f, 0.0f);
-            if (found) {
-                const TriRecord r = rt[best_tri];
-                geom = __float_as_uint(r.b.w);
-                prim = __float_as_uint(r.c.w);
-                if (hits.ng_x) ng = cross3(mk3(r.b.x, r.b.y, r.b.z), mk3(r.c.x, r.c.y, r.c.z));
-                if (prim & 0x80000000u) {
-                    // triangle 2q + h of quad q: report the quad and its own (u, v)
-                    if (prim & 1u) {
-                        best_u = 1.0f - best_u;
-                        best_v = 1.0f - best_v;
+        write_result<ANY>(i, hit_any, best_tri, best_t, best_u, best_v, rt, hits, occluded);
+    }
+    if (STATS) {
+        atomicAdd(&stats[0], cnt_nodes);
+        atomicAdd(&stats[1], cnt_tris);
+    }
+}
+
+// Compare-exchange of (key, ref) pairs: ascending keys.
+__device__ __forceinline__ void cswap(float& ka, int& ra, float& kb, int& rb)
+{
+    const bool sw = kb < ka;
+    const float k = sw ? kb : ka;
+    const int r = sw ? rb : ra;
+    kb = sw ? ka : kb;
+    rb = sw ? ra : rb;
+    ka = k;
+    ra = r;
+}
+
+// 8-wide traversal of the collapsed tree (WideNode). Per node the 8 quantized child boxes are
+// decoded and slab-tested in registers. Triangle children whose box is hit are tested at once;
+// internal children whose box is hit are ordered by entry distance with a fixed sorting network
+// (closest hit; the any-hit query keeps slot order), the nearest is entered and the rest are pushed
+// far to near. Hits are the triangle test's, independent of the tree and of the visiting order.
+template <bool ANY, bool STATS, bool MULTI>
+__global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace8(const WideNode* __restrict wnodes,
+                                                             const TriRecord* __restrict tris, uint32_t n_tris,
+                                                             const TraceScene* __restrict table,
+                                                             const uint32_t* __restrict ray_scene,
+                                                             sgpu_rays_soa rays, sgpu_hits_soa hits,
+                                                             uint32_t* __restrict occluded, uint32_t n,
+                                                             uint32_t* __restrict spill, uint32_t stack_cap,
+                                                             unsigned long long* __restrict stats,
+                                                             uint32_t* __restrict overflow)
+{
+    __shared__ uint32_t lds_stack[SGPU_STACK_LDS][SGPU_TRACE_BLOCK];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t gtid = blockIdx.x * SGPU_TRACE_BLOCK + threadIdx.x;
+    const uint32_t stride = gridDim.x * SGPU_TRACE_BLOCK;
+    unsigned long long cnt_nodes = 0, cnt_tris = 0;
+
+    for (uint32_t i = gtid; i < n; i += stride) {
+        f3 o = mk3(rays.org_x[i], rays.org_y[i], rays.org_z[i]);
+        f3 d = mk3(rays.dir_x[i], rays.dir_y[i], rays.dir_z[i]);
+        float tnear = rays.tnear[i];
+        float tfar = rays.tfar[i];
+        f3 inv = mk3(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
+        f3 oinv = mk3(o.x * inv.x, o.y * inv.y, o.z * inv.z);
+        const WideNode* __restrict rw = wnodes;
+        const TriRecord* __restrict rt = tris;
+        uint32_t rnt = n_tris;
+        if (MULTI) {
+            const TraceScene ts = table[ray_scene[i]];
+            rw = ts.wide;
+            rt = ts.tris;
+            rnt = ts.n_tris;
+        }
+
+        float best_t = tfar;
+        uint32_t best_key = 0xFFFFFFFFu;
+        uint32_t best_tri = 0xFFFFFFFFu;
+        float best_u = 0.0f, best_v = 0.0f;
+        bool hit_any = false;
+
+        // the per-lane stack: top SGPU_STACK_LDS entries in LDS (a ring), deeper ones spilled
+        uint32_t sp = 0;
+        bool overflowed = false;
+        auto push = [&](int x) {
+            if (sp >= stack_cap) {
+                overflowed = true;
+                return;
+            }
+            if (sp < SGPU_STACK_LDS) {
+                lds_stack[sp][lane] = (uint32_t)x;
+            } else {
+                const uint32_t slot = sp % SGPU_STACK_LDS;
+                spill[(size_t)(sp - SGPU_STACK_LDS) * stride + gtid] = lds_stack[slot][lane];
+                lds_stack[slot][lane] = (uint32_t)x;
+            }
+            ++sp;
+        };
+        auto test_tri = [&](int ref) -> bool {  // true: any-hit found
+            const uint32_t j = (uint32_t)(~ref);
+            const TriRecord r = rt[j];
+            if (STATS) ++cnt_tris;
+            float t, u, v;
+            if (tri_test(r, o, d, tnear, t, u, v)) {
+                if (ANY) {
+                    if (t <= tfar) return true;
+                } else {
+                    uint32_t key = __float_as_uint(r.a.w);
+                    if (t < best_t || (t == best_t && key < best_key)) {
+                        best_t = t;
+                        best_key = key;
+                        best_tri = j;
+                        best_u = u;
+                        best_v = v;
                     }
-                    prim = (prim & 0x7FFFFFFFu) >> 1;
                 }
             }
-            if (hits.t) hits.t[i] = best_t;
-            if (hits.u) hits.u[i] = found ? best_u : 0.0f;
-            if (hits.v) hits.v[i] = found ? best_v : 0.0f;
-            if (hits.prim_id) hits.prim_id[i] = prim;
-            if (hits.geom_id) hits.geom_id[i] = geom;
-            if (hits.ng_x) {
-                hits.ng_x[i] = ng.x;
-                hits.ng_y[i] = ng.y;
-                hits.ng_z[i] = ng.z;
+            return false;
+        };
+
+        int node = rnt > 0 ? 0 : -1;
+        while (node >= 0 && !overflowed) {
+            const uint4* __restrict np = reinterpret_cast<const uint4*>(rw + node);
+            const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4], w5 = np[5];
+            if (STATS) ++cnt_nodes;
+            const float ox = __uint_as_float(w0.x), oy = __uint_as_float(w0.y), oz = __uint_as_float(w0.z);
+            const uint32_t meta = w0.w;
+            const float sx = __uint_as_float((meta & 0xFFu) << 23);
+            const float sy = __uint_as_float(((meta >> 8) & 0xFFu) << 23);
+            const float sz = __uint_as_float(((meta >> 16) & 0xFFu) << 23);
+            const uint32_t nch = meta >> 24;
+            const uint32_t lox[2] = {w1.x, w1.y}, loy[2] = {w1.z, w1.w}, loz[2] = {w2.x, w2.y};
+            const uint32_t hix[2] = {w2.z, w2.w}, hiy[2] = {w3.x, w3.y}, hiz[2] = {w3.z, w3.w};
+            int ref[8] = {(int)w4.x, (int)w4.y, (int)w4.z, (int)w4.w, (int)w5.x, (int)w5.y, (int)w5.z, (int)w5.w};
+            float key[8];
+            uint32_t leaves = 0;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int sh = 8 * (c & 3), wi = c >> 2;
+                const float bx0 = fmaf((float)((lox[wi] >> sh) & 0xFFu), sx, ox);
+                const float bx1 = fmaf((float)((hix[wi] >> sh) & 0xFFu), sx, ox);
+                const float by0 = fmaf((float)((loy[wi] >> sh) & 0xFFu), sy, oy);
+                const float by1 = fmaf((float)((hiy[wi] >> sh) & 0xFFu), sy, oy);
+                const float bz0 = fmaf((float)((loz[wi] >> sh) & 0xFFu), sz, oz);
+                const float bz1 = fmaf((float)((hiz[wi] >> sh) & 0xFFu), sz, oz);
+                const float l_x = fmaf(bx0, inv.x, -oinv.x), h_x = fmaf(bx1, inv.x, -oinv.x);
+                const float l_y = fmaf(by0, inv.y, -oinv.y), h_y = fmaf(by1, inv.y, -oinv.y);
+                const float l_z = fmaf(bz0, inv.z, -oinv.z), h_z = fmaf(bz1, inv.z, -oinv.z);
+                const float tmin = fmaxf(fmaxf(fminf(l_x, h_x), fminf(l_y, h_y)), fmaxf(fminf(l_z, h_z), tnear));
+                const float tmax = fminf(fminf(fmaxf(l_x, h_x), fmaxf(l_y, h_y)), fminf(fmaxf(l_z, h_z), best_t));
+                const bool hit = (uint32_t)c < nch && tmin <= tmax;
+                // internal children: entry distance as the sort key (+inf: not to visit)
+                key[c] = hit && ref[c] >= 0 ? tmin : __int_as_float(0x7f800000);
+                if (hit && ref[c] < 0) leaves |= 1u << c;
             }
+            // triangle children, in slot order
+            bool found = false;
+            while (leaves) {
+                const int c = __builtin_ctz(leaves);
+                leaves &= leaves - 1;
+                int r = ref[0];
+#pragma unroll
+                for (int k = 1; k < 8; ++k) r = c == k ? ref[k] : r;
+                if (test_tri(r)) {
+                    found = true;
+                    break;
+                }
+            }
+            if (ANY && found) {
+                hit_any = true;
+                break;
+            }
+            const float inf = __int_as_float(0x7f800000);
+            if (!ANY) {
+                // entry distances beyond the closest hit so far cannot hold a closer one
+#pragma unroll
+                for (int c = 0; c < 8; ++c) key[c] = key[c] <= best_t ? key[c] : inf;
+                // ascending (key, ref): Batcher's odd-even merge sort network for 8 inputs
+                cswap(key[0], ref[0], key[1], ref[1]); cswap(key[2], ref[2], key[3], ref[3]);
+                cswap(key[4], ref[4], key[5], ref[5]); cswap(key[6], ref[6], key[7], ref[7]);
+                cswap(key[0], ref[0], key[2], ref[2]); cswap(key[1], ref[1], key[3], ref[3]);
+                cswap(key[4], ref[4], key[6], ref[6]); cswap(key[5], ref[5], key[7], ref[7]);
+                cswap(key[1], ref[1], key[2], ref[2]); cswap(key[5], ref[5], key[6], ref[6]);
+                cswap(key[0], ref[0], key[4], ref[4]); cswap(key[1], ref[1], key[5], ref[5]);
+                cswap(key[2], ref[2], key[6], ref[6]); cswap(key[3], ref[3], key[7], ref[7]);
+                cswap(key[2], ref[2], key[4], ref[4]); cswap(key[3], ref[3], key[5], ref[5]);
+                cswap(key[1], ref[1], key[2], ref[2]); cswap(key[3], ref[3], key[4], ref[4]);
+                cswap(key[5], ref[5], key[6], ref[6]);
+            }
+            // the nearest (first valid) is entered, the others pushed far to near
+            int next = -1;
+#pragma unroll
+            for (int c = 7; c >= 0; --c) {
+                if (key[c] != inf) {
+                    if (next >= 0) push(next);
+                    next = ref[c];
+                }
+            }
+            if (next >= 0) {
+                node = next;
+                continue;
+            }
+            // pop
+            if (sp == 0) break;
+            --sp;
+            const uint32_t slot = sp % SGPU_STACK_LDS;
+            node = (int)lds_stack[slot][lane];
+            if (sp >= SGPU_STACK_LDS) lds_stack[slot][lane] = spill[(size_t)(sp - SGPU_STACK_LDS) * stride + gtid];
         }
+        if (overflowed) atomicOr(overflow, 1u);
+
+        write_result<ANY>(i, hit_any, best_tri, best_t, best_u, best_v, rt, hits, occluded);
     }
     if (STATS) {
         atomicAdd(&stats[0], cnt_nodes);
@@ -215,56 +420,66 @@ __global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace(const BvhNode* __res
 namespace {
 
 template <bool ANY, bool STATS>
-void launch_trace(bool multi, uint32_t blocks, hipStream_t stream, const BvhNode* nodes, const TriRecord* tris,
-                  uint32_t n_tris, const TraceScene* table, const uint32_t* ray_scene, const sgpu_rays_soa& rays,
-                  const sgpu_hits_soa& h, uint32_t* occluded, uint32_t n, uint32_t* spill, uint32_t cap,
-                  unsigned long long* st, uint32_t* overflow)
+void launch_trace(bool wide, bool multi, uint32_t blocks, hipStream_t stream, const BvhNode* nodes,
+                  const WideNode* wnodes, const TriRecord* tris, uint32_t n_tris, const TraceScene* table,
+                  const uint32_t* ray_scene, const sgpu_rays_soa& rays, const sgpu_hits_soa& h, uint32_t* occluded,
+                  uint32_t n, uint32_t* spill, uint32_t cap, unsigned long long* st, uint32_t* overflow)
 {
-    if (multi)
-        hipLaunchKernelGGL((k_trace<ANY, STATS, true>), dim3(blocks), dim3(SGPU_TRACE_BLOCK), 0, stream, nodes, tris,
-                           n_tris, table, ray_scene, rays, h, occluded, n, spill, cap, st, overflow);
-    else
-        hipLaunchKernelGGL((k_trace<ANY, STATS, false>), dim3(blocks), dim3(SGPU_TRACE_BLOCK), 0, stream, nodes,
-                           tris, n_tris, table, ray_scene, rays, h, occluded, n, spill, cap, st, overflow);
+    if (wide) {
+        if (multi)
+            hipLaunchKernelGGL((k_trace8<ANY, STATS, true>), dim3(blocks), dim3(SGPU_TRACE_BLOCK), 0, stream, wnodes,
+                               tris, n_tris, table, ray_scene, rays, h, occluded, n, spill, cap, st, overflow);
+        else
+            hipLaunchKernelGGL((k_trace8<ANY, STATS, false>), dim3(blocks), dim3(SGPU_TRACE_BLOCK), 0, stream, wnodes,
+                               tris, n_tris, table, ray_scene, rays, h, occluded, n, spill, cap, st, overflow);
+    } else {
+        if (multi)
+            hipLaunchKernelGGL((k_trace<ANY, STATS, true>), dim3(blocks), dim3(SGPU_TRACE_BLOCK), 0, stream, nodes,
+                               tris, n_tris, table, ray_scene, rays, h, occluded, n, spill, cap, st, overflow);
+        else
+            hipLaunchKernelGGL((k_trace<ANY, STATS, false>), dim3(blocks), dim3(SGPU_TRACE_BLOCK), 0, stream, nodes,
+                               tris, n_tris, table, ray_scene, rays, h, occluded, n, spill, cap, st, overflow);
+    }
 }
 
-// One traversal launch over n rays: of `scene`, or (table != nullptr) of the scenes of a batch,
-// ray i in table[ray_scene[i]]; max_depth bounds every scene's tree depth.
-int trace_launch(sgpu_context ctx, const BvhNode* nodes, const TriRecord* tris, uint32_t n_tris, uint32_t max_depth,
-                 uint32_t* overflow, const TraceScene* table, const uint32_t* ray_scene, const sgpu_rays_soa* rays,
-                 const sgpu_hits_soa* hits, uint32_t* occluded, uint32_t n, bool any_hit, uint64_t* d_stats,
-                 hipStream_t stream, uint32_t* spill_buf)
+// One traversal launch over n rays: of one scene (its trees `nodes` / `wnodes`), or (table !=
+// nullptr) of the scenes of a batch, ray i in table[ray_scene[i]]; stack_cap bounds every scene's
+// stack. The context's trace_tree picks the tree kind.
+int trace_launch(sgpu_context ctx, const BvhNode* nodes, const WideNode* wnodes, const TriRecord* tris,
+                 uint32_t n_tris, uint32_t stack_cap, uint32_t* overflow, const TraceScene* table,
+                 const uint32_t* ray_scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits, uint32_t* occluded,
+                 uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream, uint32_t* spill_buf)
 {
     if (n == 0) return SGPU_OK;
     const uint32_t cus = (uint32_t)ctx->props.multiProcessorCount;
     uint32_t blocks = (n + SGPU_TRACE_BLOCK - 1) / SGPU_TRACE_BLOCK;
     const uint32_t max_blocks = cus * std::min(ctx->trace_blocks_per_cu, kMaxTraceBlocksPerCu);
     if (blocks > max_blocks) blocks = max_blocks;
-    const uint32_t cap = max_depth + 1;
 
     // stream-ordered scratch for the spilled part of the stacks, private to this call, unless the
     // caller lends a buffer of sgpu_trace_spill_words (sized for the largest grid)
     uint32_t* spill = spill_buf;
-    if (!spill) SGPU_HIP_TRY(hipMallocAsync((void**)&spill, sgpu_trace_spill_words(ctx, max_depth) * sizeof(uint32_t),
+    if (!spill) SGPU_HIP_TRY(hipMallocAsync((void**)&spill, sgpu_trace_spill_words(ctx, stack_cap) * sizeof(uint32_t),
                                             stream));
     sgpu_hits_soa h = {};
     if (hits) h = *hits;
     unsigned long long* st = (unsigned long long*)d_stats;
     const bool multi = table != nullptr;
+    const bool wide = ctx->trace_tree == 1;
     if (any_hit) {
         if (st)
-            launch_trace<true, true>(multi, blocks, stream, nodes, tris, n_tris, table, ray_scene, *rays, h, occluded,
-                                     n, spill, cap, st, overflow);
+            launch_trace<true, true>(wide, multi, blocks, stream, nodes, wnodes, tris, n_tris, table, ray_scene, *rays,
+                                     h, occluded, n, spill, stack_cap, st, overflow);
         else
-            launch_trace<true, false>(multi, blocks, stream, nodes, tris, n_tris, table, ray_scene, *rays, h, occluded,
-                                      n, spill, cap, st, overflow);
+            launch_trace<true, false>(wide, multi, blocks, stream, nodes, wnodes, tris, n_tris, table, ray_scene, *rays,
+                                      h, occluded, n, spill, stack_cap, st, overflow);
     } else {
         if (st)
-            launch_trace<false, true>(multi, blocks, stream, nodes, tris, n_tris, table, ray_scene, *rays, h, occluded,
-                                      n, spill, cap, st, overflow);
+            launch_trace<false, true>(wide, multi, blocks, stream, nodes, wnodes, tris, n_tris, table, ray_scene, *rays,
+                                      h, occluded, n, spill, stack_cap, st, overflow);
         else
-            launch_trace<false, false>(multi, blocks, stream, nodes, tris, n_tris, table, ray_scene, *rays, h,
-                                       occluded, n, spill, cap, st, overflow);
+            launch_trace<false, false>(wide, multi, blocks, stream, nodes, wnodes, tris, n_tris, table, ray_scene,
+                                       *rays, h, occluded, n, spill, stack_cap, st, overflow);
     }
     SGPU_HIP_TRY(hipGetLastError());
     if (!spill_buf) SGPU_HIP_TRY(hipFreeAsync(spill, stream));
@@ -273,29 +488,29 @@ int trace_launch(sgpu_context ctx, const BvhNode* nodes, const TriRecord* tris, 
 
 }  // namespace
 
-size_t sgpu_trace_spill_words(sgpu_context ctx, uint32_t max_depth)
+size_t sgpu_trace_spill_words(sgpu_context ctx, uint32_t stack_cap)
 {
     const uint32_t cus = (uint32_t)ctx->props.multiProcessorCount;
     const uint32_t threads = cus * kMaxTraceBlocksPerCu * SGPU_TRACE_BLOCK;
-    const uint32_t cap = max_depth + 1;
-    const uint32_t spill_depth = cap > SGPU_STACK_LDS ? cap - SGPU_STACK_LDS : 1;
+    const uint32_t spill_depth = stack_cap > SGPU_STACK_LDS ? stack_cap - SGPU_STACK_LDS : 1;
     return (size_t)spill_depth * threads;
 }
 
-size_t sgpu_trace_spill_words(sgpu_scene scene) { return sgpu_trace_spill_words(scene->ctx, scene->max_depth); }
+size_t sgpu_trace_spill_words(sgpu_scene scene) { return sgpu_trace_spill_words(scene->ctx, scene->stack_cap); }
 
 int sgpu_trace_launch(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits, uint32_t* occluded,
                       uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream, uint32_t* spill_buf)
 {
-    return trace_launch(scene->ctx, scene->d_nodes, scene->d_tris, scene->n_tris, scene->max_depth, scene->d_overflow,
-                        nullptr, nullptr, rays, hits, occluded, n, any_hit, d_stats, stream, spill_buf);
+    return trace_launch(scene->ctx, scene->d_nodes, scene->d_wide, scene->d_tris, scene->n_tris, scene->stack_cap,
+                        scene->d_overflow, nullptr, nullptr, rays, hits, occluded, n, any_hit, d_stats, stream,
+                        spill_buf);
 }
 
-int sgpu_trace_launch_multi(sgpu_context ctx, const TraceScene* table, const uint32_t* ray_scene, uint32_t max_depth,
+int sgpu_trace_launch_multi(sgpu_context ctx, const TraceScene* table, const uint32_t* ray_scene, uint32_t stack_cap,
                             uint32_t* overflow, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits,
                             uint32_t* occluded, uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream,
                             uint32_t* spill_buf)
 {
-    return trace_launch(ctx, nullptr, nullptr, 0, max_depth, overflow, table, ray_scene, rays, hits, occluded, n,
-                        any_hit, d_stats, stream, spill_buf);
+    return trace_launch(ctx, nullptr, nullptr, nullptr, 0, stack_cap, overflow, table, ray_scene, rays, hits, occluded,
+                        n, any_hit, d_stats, stream, spill_buf);
 }

@@ -1227,7 +1227,8 @@ struct sgpu_vcm_server_s {
     Slice* d_slices = nullptr;
     CamVertex* cam_vm = nullptr;
     uint32_t* spill = nullptr;
-    uint32_t spill_depth_cap = 0;
+    uint32_t spill_cap = 0;           // stack entries the spill buffer is sized for
+    uint32_t* spill_grown = nullptr;  // a larger spill buffer allocated at submit (freed at destroy)
     CamSlots S;
     void* tmp = nullptr;
     size_t tmp_bytes = 0;
@@ -1241,7 +1242,7 @@ struct sgpu_vcm_server_s {
     TraceScene* d_table = nullptr;
     uint32_t table_cap = 0;
     bool table_dirty = false;
-    uint32_t max_depth = 0;
+    uint32_t max_stack = 0;  // largest traversal stack of the scenes
     std::vector<ServerJob> active;
     uint64_t next_id = 1;
     // totals
@@ -1253,7 +1254,7 @@ struct sgpu_vcm_server_s {
 namespace {
 
 constexpr uint32_t kMaxSlicesPerWave = 4096;
-constexpr uint32_t kMaxSceneDepth = 64;  // stack spill is sized for trees up to this depth
+constexpr uint32_t kDefaultStackCap = 96;  // stack spill sized at creation (grown for deeper trees)
 
 int server_alloc(sgpu_vcm_server sv)
 {
@@ -1327,8 +1328,8 @@ int server_alloc(sgpu_vcm_server sv)
                                          rocprim::plus<uint32_t>(), A.stream));
     sv->tmp_bytes = std::max(sort_bytes, std::max(scan_bytes, scan2_bytes));
     sv->tmp = A.get<uint8_t>(sv->tmp_bytes);
-    sv->spill_depth_cap = kMaxSceneDepth;
-    sv->spill = A.get<uint32_t>(sgpu_trace_spill_words(sv->ctx, kMaxSceneDepth));
+    sv->spill_cap = kDefaultStackCap;
+    sv->spill = A.get<uint32_t>(sgpu_trace_spill_words(sv->ctx, kDefaultStackCap));
     if (A.err != hipSuccess) return sgpu_fail(SGPU_E_NOMEM, "sgpu_vcm_server: device allocation failed");
     if (A.dry) return SGPU_OK;
     const size_t hbytes = 64 + sizeof(Slice) * (size_t)sv->max_slices;
@@ -1345,7 +1346,7 @@ uint32_t server_scene(sgpu_vcm_server sv, sgpu_scene scene)
         if (sv->scenes[k] == scene) return (uint32_t)k;
     sv->scenes.push_back(scene);
     sv->table_dirty = true;
-    sv->max_depth = std::max(sv->max_depth, scene->max_depth);
+    sv->max_stack = std::max(sv->max_stack, scene->stack_cap);
     return (uint32_t)(sv->scenes.size() - 1);
 }
 
@@ -1364,7 +1365,8 @@ int server_upload_table(sgpu_vcm_server sv, hipStream_t stream)
         SGPU_HIP_TRY(hipHostMalloc((void**)&sv->h_table, sizeof(TraceScene) * sv->table_cap, hipHostMallocDefault));
     }
     for (uint32_t k = 0; k < n; ++k)
-        sv->h_table[k] = TraceScene{sv->scenes[k]->d_nodes, sv->scenes[k]->d_tris, sv->scenes[k]->n_tris, 0};
+        sv->h_table[k] = TraceScene{sv->scenes[k]->d_nodes, sv->scenes[k]->d_wide, sv->scenes[k]->d_tris,
+                                    sv->scenes[k]->n_tris, 0};
     SGPU_HIP_TRY(hipMemcpyAsync(sv->d_table, sv->h_table, sizeof(TraceScene) * n, hipMemcpyHostToDevice, stream));
     SGPU_HIP_TRY(hipStreamSynchronize(stream));
     sv->table_dirty = false;
@@ -1380,7 +1382,7 @@ static void server_reset(sgpu_vcm_server sv, const sgpu_vcm_server_config& cfg)
     sv->active.clear();
     sv->scenes.clear();
     sv->table_dirty = true;
-    sv->max_depth = 0;
+    sv->max_stack = 0;
     sv->cfg.flags = cfg.flags;
     sv->cap = (uint32_t)std::min<uint64_t>(cfg.wavefront_paths, sv->TM);
     sv->n_closest = sv->n_shadow = sv->n_light_verts = sv->n_launches = sv->n_samples = sv->n_waves = 0;
@@ -1431,6 +1433,7 @@ int sgpu_vcm_server_destroy_impl(sgpu_vcm_server sv, hipStream_t stream)
     if (sv->hbuf) (void)hipHostFree(sv->hbuf);
     if (sv->h_table) (void)hipHostFree(sv->h_table);
     if (sv->d_table) (void)hipFree(sv->d_table);
+    if (sv->spill_grown) (void)hipFreeAsync(sv->spill_grown, stream);
     delete sv;  // the arena frees its buffers, stream-ordered
     return SGPU_OK;
 }
@@ -1443,7 +1446,16 @@ int sgpu_vcm_server_submit_impl(sgpu_vcm_server sv, sgpu_scene sc, const sgpu_vc
     if ((uint64_t)W * H > sv->cap) return sgpu_fail(SGPU_E_RANGE, "sgpu_vcm_server_submit: image larger than a wavefront");
     if (L > sv->cfg.max_path_length)
         return sgpu_fail(SGPU_E_RANGE, "sgpu_vcm_server_submit: path length above the server's maximum");
-    if (sc->max_depth > kMaxSceneDepth) return sgpu_fail(SGPU_E_RANGE, "sgpu_vcm_server_submit: tree too deep");
+    if (sc->stack_cap > sv->spill_cap) {
+        // a deeper tree than the spill buffer holds stacks for: a larger one, stream-ordered
+        uint32_t* grown = nullptr;
+        SGPU_HIP_TRY(hipMallocAsync((void**)&grown, sgpu_trace_spill_words(sv->ctx, sc->stack_cap) * sizeof(uint32_t),
+                                    stream));
+        if (sv->spill_grown) SGPU_HIP_TRY(hipFreeAsync(sv->spill_grown, stream));
+        sv->spill_grown = grown;
+        sv->spill = grown;
+        sv->spill_cap = sc->stack_cap;
+    }
     ServerJob j;
     j.id = sv->next_id++;
     j.scene = sc;
@@ -1547,7 +1559,7 @@ int sgpu_vcm_server_step_impl(sgpu_vcm_server sv, hipStream_t stream, uint64_t* 
                      uint64_t* d_st) -> int {
         sgpu_rays_soa rs = as_soa(r);
         if (!single)
-            return sgpu_trace_launch_multi(sv->ctx, sv->d_table, r.scene, sv->max_depth, first_scene->d_overflow, &rs,
+            return sgpu_trace_launch_multi(sv->ctx, sv->d_table, r.scene, sv->max_stack, first_scene->d_overflow, &rs,
                                            hits, occ, n, any, d_st, stream, sv->spill);
         return sgpu_trace_launch(first_scene, &rs, hits, occ, n, any, d_st, stream, sv->spill);
     };

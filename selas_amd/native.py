@@ -206,7 +206,7 @@ class DeviceBuffer:
 
 
 class Tuning(C.Structure):
-    _fields_ = [("vcm_wavefront_paths", C.c_uint64), ("trace_blocks_per_cu", C.c_uint32), ("reserved", C.c_uint32)]
+    _fields_ = [("vcm_wavefront_paths", C.c_uint64), ("trace_blocks_per_cu", C.c_uint32), ("trace_tree", C.c_uint32)]
 
 
 class DeviceInfo(C.Structure):
@@ -229,12 +229,14 @@ class Context:
     def tuning(self) -> dict:
         t = Tuning()
         check(lib.sgpu_get_tuning(self.handle, C.byref(t)))
-        return {"vcm_wavefront_paths": t.vcm_wavefront_paths, "trace_blocks_per_cu": t.trace_blocks_per_cu}
+        return {"vcm_wavefront_paths": t.vcm_wavefront_paths, "trace_blocks_per_cu": t.trace_blocks_per_cu,
+                "trace_tree": t.trace_tree}
 
-    def set_tuning(self, vcm_wavefront_paths=None, trace_blocks_per_cu=None):
+    def set_tuning(self, vcm_wavefront_paths=None, trace_blocks_per_cu=None, trace_tree=None):
         cur = self.tuning()
-        t = Tuning(cur["vcm_wavefront_paths"] if vcm_wavefront_paths is None else int(vcm_wavefront_paths),
-                   cur["trace_blocks_per_cu"] if trace_blocks_per_cu is None else int(trace_blocks_per_cu), 0)
+        pick = lambda new, key: cur[key] if new is None else int(new)
+        t = Tuning(pick(vcm_wavefront_paths, "vcm_wavefront_paths"), pick(trace_blocks_per_cu, "trace_blocks_per_cu"),
+                   pick(trace_tree, "trace_tree"))
         check(lib.sgpu_set_tuning(self.handle, C.byref(t)))
 
     def array(self, a: np.ndarray) -> DeviceBuffer:

@@ -193,3 +193,30 @@ def test_full_traversal_workload(gpu_ctx):
            "v": h["v"][sel]}
     assert_same(got, ref, [0, t9.shape[0]])
     scene.release()
+
+
+@pytest.mark.parametrize("seed", [4, 5])
+def test_wide_and_binary_trees_agree(gpu_ctx, seed):
+    """The 8-wide compressed tree (default) and the binary tree it is collapsed from give the same
+    hits, bit for bit (both only cull; hits are the triangle test's), with far fewer node visits."""
+    t9 = W.traversal_scene(scale=1 / 16, seed=seed)  # 131,072 triangles in Gaussian clusters
+    rays = W.traversal_rays(t9, 128, 128, 2, n_bounce=32768, seed=seed)
+    old = gpu_ctx.tuning()
+    try:
+        out, stats = {}, {}
+        for tree in (0, 1):
+            gpu_ctx.set_tuning(trace_tree=tree)
+            out[tree] = gpu_trace(gpu_ctx, t9, rays)
+            scene = N.Scene(gpu_ctx)
+            scene.add_soup(t9)
+            scene.commit()
+            stats[tree] = scene.intersect_stats(N.DeviceRays(gpu_ctx, rays))
+            scene.release()
+        for k in ("prim_id", "geom_id", "occluded"):
+            assert np.array_equal(out[0][k], out[1][k]), k
+        for k in ("t", "u", "v"):
+            assert np.array_equal(out[0][k].view(np.uint32), out[1][k].view(np.uint32)), k
+        assert stats[0]["bytes_node"] == 64 and stats[1]["bytes_node"] == 128
+        assert stats[1]["nodes_visited"] < 0.6 * stats[0]["nodes_visited"]
+    finally:
+        gpu_ctx.set_tuning(**old)

@@ -12,13 +12,14 @@ pytestmark = pytest.mark.gpu
 def test_tuning_roundtrip_and_limits(gpu_ctx):
     old = gpu_ctx.tuning()
     try:
-        gpu_ctx.set_tuning(vcm_wavefront_paths=1 << 14, trace_blocks_per_cu=3)
-        assert gpu_ctx.tuning() == {"vcm_wavefront_paths": 1 << 14, "trace_blocks_per_cu": 3}
+        gpu_ctx.set_tuning(vcm_wavefront_paths=1 << 14, trace_blocks_per_cu=3, trace_tree=0)
+        want = {"vcm_wavefront_paths": 1 << 14, "trace_blocks_per_cu": 3, "trace_tree": 0}
+        assert gpu_ctx.tuning() == want
         for bad in ({"vcm_wavefront_paths": 1 << 11}, {"vcm_wavefront_paths": (1 << 24) + 1},
-                    {"trace_blocks_per_cu": 0}, {"trace_blocks_per_cu": 33}):
+                    {"trace_blocks_per_cu": 0}, {"trace_blocks_per_cu": 33}, {"trace_tree": 2}):
             with pytest.raises(N.SgpuError):
                 gpu_ctx.set_tuning(**bad)
-        assert gpu_ctx.tuning() == {"vcm_wavefront_paths": 1 << 14, "trace_blocks_per_cu": 3}
+        assert gpu_ctx.tuning() == want
         info = gpu_ctx.device_info()
         assert info["compute_units"] > 0 and info["total_bytes"] >= info["free_bytes"] > 0 and info["name"]
     finally:
