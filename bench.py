@@ -430,7 +430,7 @@ def cpu_baseline_batch(mix, iterations, threads):
                       f"restatement (scalar C, oracle SAH tree, not Embree)"}
 
 
-def serve_sweep(N, W, ctx, rates, n_jobs, seed, n_scenes=16, plain=True, budget=200 << 30, wavefront_paths=1 << 20):
+def serve_sweep(N, W, ctx, rates, n_jobs, seed, n_scenes=16, plain=True, budget=200 << 30, wavefront_paths=1 << 22):
     """Poisson request streams at each rate, served by the request loop over resident scenes,
     and by the plain one-job-at-a-time GPU schedule on the same requests."""
     from selas_amd import serving as S
@@ -556,7 +556,7 @@ def main():
     ap.add_argument("--serve-jobs", type=int, default=None,
                     help="requests per rate (default: 24 in the serving leg; 8 s of arrivals, 16..64, in serve)")
     ap.add_argument("--no-plain", action="store_true", help="serve: skip the plain one-job-at-a-time runs")
-    ap.add_argument("--serve-paths", type=int, default=1 << 20, help="serving: paths per wavefront of the server")
+    ap.add_argument("--serve-paths", type=int, default=1 << 22, help="serving: paths per wavefront of the server")
     ap.add_argument("--no-batch", action="store_true", help="vcm: skip the batched leg")
     ap.add_argument("--no-serve", action="store_true", help="vcm: skip the serving leg")
     ap.add_argument("--no-tune", action="store_true", help="use the default launch shape instead of autotuning")

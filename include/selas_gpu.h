@@ -105,6 +105,9 @@ typedef struct sgpu_tuning {
                                      rest of the rays taken grid-stride: 1..32 (default 8) */
     uint32_t trace_tree;          /* traversal tree: 0 binary nodes (64 B), 1 8-wide compressed
                                      nodes (128 B, default); both are built at commit */
+    uint32_t trace_lds_nodes;     /* 8-wide single-scene launches: this many top nodes (breadth-first)
+                                     read from LDS instead of memory: 0..9 (default 0) */
+    uint32_t reserved;            /* 0 */
 } sgpu_tuning;
 int sgpu_set_tuning(sgpu_context ctx, const sgpu_tuning* tuning);
 int sgpu_get_tuning(sgpu_context ctx, sgpu_tuning* tuning);

@@ -73,8 +73,21 @@ def tune(ctx, probe, wavefront_paths=None, blocks_per_cu=None, memory_budget=Non
     grid = min(grids, key=lambda g: cost(size, g))
     size = min(fits, key=lambda s: cost(s, grid))
     best = {"vcm_wavefront_paths": size, "trace_blocks_per_cu": grid}
+    seconds = cost(size, grid)
+    if "trace_lds_nodes" in start:
+        # the top of the 8-wide tree in LDS, tried at the best shape
+        for lds in sorted({0, 9} - {start["trace_lds_nodes"]}):
+            ctx.set_tuning(trace_lds_nodes=lds, **best)
+            t = _timed(probe, ctx, repeats, timer)
+            trials.append(dict(best, trace_lds_nodes=lds, seconds=t))
+            if log:
+                log(f"autotune: {lds} top nodes in LDS: {t * 1e3:.1f} ms")
+            if t < seconds:
+                seconds = t
+                best["trace_lds_nodes"] = lds
+        best.setdefault("trace_lds_nodes", start["trace_lds_nodes"])
     ctx.set_tuning(**best)
-    return {"best": dict(ctx.tuning(), **best), "seconds": cost(size, grid), "default_seconds": default_seconds,
+    return {"best": dict(ctx.tuning(), **best), "seconds": seconds, "default_seconds": default_seconds,
             "trials": trials}
 
 

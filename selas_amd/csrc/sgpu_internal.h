@@ -67,6 +67,7 @@ struct sgpu_context_s {
     uint64_t vcm_wavefront_paths = 1u << 20;
     uint32_t trace_blocks_per_cu = 8;
     uint32_t trace_tree = 1;  // 0: binary nodes, 1: 8-wide compressed nodes
+    uint32_t trace_lds_nodes = 0;  // top wide nodes read from LDS (single-scene launches)
     // the VCM server of the last sgpu_render_vcm(_batch) call, reused while large enough
     std::mutex vcm_mu;
     sgpu_vcm_server vcm_cache = nullptr;

@@ -236,8 +236,13 @@ __device__ __forceinline__ void cswap(float& ka, int& ra, float& kb, int& rb)
 // internal children whose box is hit are ordered by entry distance with a fixed sorting network
 // (closest hit; the any-hit query keeps slot order), the nearest is entered and the rest are pushed
 // far to near. Hits are the triangle test's, independent of the tree and of the visiting order.
+// Wide nodes of a single-scene launch that may be kept in LDS (sgpu_tuning.trace_lds_nodes): the root
+// and the next breadth-first nodes (the root's children when it has 8 internal ones), visited by
+// nearly every ray. Measured on MI355X they hit in L1/L2 anyway, and the LDS branch costs 1-3 %.
+constexpr uint32_t kTopNodes = 9;
+
 template <bool ANY, bool STATS, bool MULTI>
-__global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace8(const WideNode* __restrict wnodes,
+__global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace8(const WideNode* __restrict wnodes, uint32_t n_top,
                                                              const TriRecord* __restrict tris, uint32_t n_tris,
                                                              const TraceScene* __restrict table,
                                                              const uint32_t* __restrict ray_scene,
@@ -248,10 +253,15 @@ __global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace8(const WideNode* __r
                                                              uint32_t* __restrict overflow)
 {
     __shared__ uint32_t lds_stack[SGPU_STACK_LDS][SGPU_TRACE_BLOCK];
+    __shared__ uint4 lds_top[kTopNodes * 6];  // the 96 bytes of each top node the traversal reads
     const uint32_t lane = threadIdx.x;
     const uint32_t gtid = blockIdx.x * SGPU_TRACE_BLOCK + threadIdx.x;
     const uint32_t stride = gridDim.x * SGPU_TRACE_BLOCK;
     unsigned long long cnt_nodes = 0, cnt_tris = 0;
+    if (MULTI) n_top = 0;
+    for (uint32_t k = threadIdx.x; k < 6 * n_top; k += SGPU_TRACE_BLOCK)
+        lds_top[k] = reinterpret_cast<const uint4*>(wnodes + k / 6)[k % 6];
+    __syncthreads();
 
     for (uint32_t i = gtid; i < n; i += stride) {
         f3 o = mk3(rays.org_x[i], rays.org_y[i], rays.org_z[i]);
@@ -317,8 +327,14 @@ __global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace8(const WideNode* __r
 
         int node = rnt > 0 ? 0 : -1;
         while (node >= 0 && !overflowed) {
-            const uint4* __restrict np = reinterpret_cast<const uint4*>(rw + node);
-            const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4], w5 = np[5];
+            uint4 w0, w1, w2, w3, w4, w5;
+            if ((uint32_t)node < n_top) {
+                const uint4* t = lds_top + 6 * node;
+                w0 = t[0], w1 = t[1], w2 = t[2], w3 = t[3], w4 = t[4], w5 = t[5];
+            } else {
+                const uint4* __restrict np = reinterpret_cast<const uint4*>(rw + node);
+                w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4], w5 = np[5];
+            }
             if (STATS) ++cnt_nodes;
             const float ox = __uint_as_float(w0.x), oy = __uint_as_float(w0.y), oz = __uint_as_float(w0.z);
             const uint32_t meta = w0.w;
@@ -421,17 +437,17 @@ namespace {
 
 template <bool ANY, bool STATS>
 void launch_trace(bool wide, bool multi, uint32_t blocks, hipStream_t stream, const BvhNode* nodes,
-                  const WideNode* wnodes, const TriRecord* tris, uint32_t n_tris, const TraceScene* table,
+                  const WideNode* wnodes, uint32_t n_top, const TriRecord* tris, uint32_t n_tris, const TraceScene* table,
                   const uint32_t* ray_scene, const sgpu_rays_soa& rays, const sgpu_hits_soa& h, uint32_t* occluded,
                   uint32_t n, uint32_t* spill, uint32_t cap, unsigned long long* st, uint32_t* overflow)
 {
     if (wide) {
         if (multi)
             hipLaunchKernelGGL((k_trace8<ANY, STATS, true>), dim3(blocks), dim3(SGPU_TRACE_BLOCK), 0, stream, wnodes,
-                               tris, n_tris, table, ray_scene, rays, h, occluded, n, spill, cap, st, overflow);
+                               n_top, tris, n_tris, table, ray_scene, rays, h, occluded, n, spill, cap, st, overflow);
         else
             hipLaunchKernelGGL((k_trace8<ANY, STATS, false>), dim3(blocks), dim3(SGPU_TRACE_BLOCK), 0, stream, wnodes,
-                               tris, n_tris, table, ray_scene, rays, h, occluded, n, spill, cap, st, overflow);
+                               n_top, tris, n_tris, table, ray_scene, rays, h, occluded, n, spill, cap, st, overflow);
     } else {
         if (multi)
             hipLaunchKernelGGL((k_trace<ANY, STATS, true>), dim3(blocks), dim3(SGPU_TRACE_BLOCK), 0, stream, nodes,
@@ -445,7 +461,7 @@ void launch_trace(bool wide, bool multi, uint32_t blocks, hipStream_t stream, co
 // One traversal launch over n rays: of one scene (its trees `nodes` / `wnodes`), or (table !=
 // nullptr) of the scenes of a batch, ray i in table[ray_scene[i]]; stack_cap bounds every scene's
 // stack. The context's trace_tree picks the tree kind.
-int trace_launch(sgpu_context ctx, const BvhNode* nodes, const WideNode* wnodes, const TriRecord* tris,
+int trace_launch(sgpu_context ctx, const BvhNode* nodes, const WideNode* wnodes, uint32_t n_wide, const TriRecord* tris,
                  uint32_t n_tris, uint32_t stack_cap, uint32_t* overflow, const TraceScene* table,
                  const uint32_t* ray_scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits, uint32_t* occluded,
                  uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream, uint32_t* spill_buf)
@@ -466,19 +482,20 @@ int trace_launch(sgpu_context ctx, const BvhNode* nodes, const WideNode* wnodes,
     unsigned long long* st = (unsigned long long*)d_stats;
     const bool multi = table != nullptr;
     const bool wide = ctx->trace_tree == 1;
+    const uint32_t n_top = std::min(n_wide, std::min(ctx->trace_lds_nodes, kTopNodes));
     if (any_hit) {
         if (st)
-            launch_trace<true, true>(wide, multi, blocks, stream, nodes, wnodes, tris, n_tris, table, ray_scene, *rays,
+            launch_trace<true, true>(wide, multi, blocks, stream, nodes, wnodes, n_top, tris, n_tris, table, ray_scene, *rays,
                                      h, occluded, n, spill, stack_cap, st, overflow);
         else
-            launch_trace<true, false>(wide, multi, blocks, stream, nodes, wnodes, tris, n_tris, table, ray_scene, *rays,
+            launch_trace<true, false>(wide, multi, blocks, stream, nodes, wnodes, n_top, tris, n_tris, table, ray_scene, *rays,
                                       h, occluded, n, spill, stack_cap, st, overflow);
     } else {
         if (st)
-            launch_trace<false, true>(wide, multi, blocks, stream, nodes, wnodes, tris, n_tris, table, ray_scene, *rays,
+            launch_trace<false, true>(wide, multi, blocks, stream, nodes, wnodes, n_top, tris, n_tris, table, ray_scene, *rays,
                                       h, occluded, n, spill, stack_cap, st, overflow);
         else
-            launch_trace<false, false>(wide, multi, blocks, stream, nodes, wnodes, tris, n_tris, table, ray_scene,
+            launch_trace<false, false>(wide, multi, blocks, stream, nodes, wnodes, n_top, tris, n_tris, table, ray_scene,
                                        *rays, h, occluded, n, spill, stack_cap, st, overflow);
     }
     SGPU_HIP_TRY(hipGetLastError());
@@ -501,7 +518,8 @@ size_t sgpu_trace_spill_words(sgpu_scene scene) { return sgpu_trace_spill_words(
 int sgpu_trace_launch(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits, uint32_t* occluded,
                       uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream, uint32_t* spill_buf)
 {
-    return trace_launch(scene->ctx, scene->d_nodes, scene->d_wide, scene->d_tris, scene->n_tris, scene->stack_cap,
+    return trace_launch(scene->ctx, scene->d_nodes, scene->d_wide, scene->n_wide, scene->d_tris, scene->n_tris,
+                        scene->stack_cap,
                         scene->d_overflow, nullptr, nullptr, rays, hits, occluded, n, any_hit, d_stats, stream,
                         spill_buf);
 }
@@ -511,6 +529,6 @@ int sgpu_trace_launch_multi(sgpu_context ctx, const TraceScene* table, const uin
                             uint32_t* occluded, uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream,
                             uint32_t* spill_buf)
 {
-    return trace_launch(ctx, nullptr, nullptr, nullptr, 0, stack_cap, overflow, table, ray_scene, rays, hits, occluded,
+    return trace_launch(ctx, nullptr, nullptr, 0, nullptr, 0, stack_cap, overflow, table, ray_scene, rays, hits, occluded,
                         n, any_hit, d_stats, stream, spill_buf);
 }

@@ -22,11 +22,24 @@ class SimCtx:
         return {"name": "sim", "free_bytes": 100 << 30}
 
 
+class SimCtxLds(SimCtx):
+    def __init__(self):
+        super().__init__()
+        self.t["trace_lds_nodes"] = 0
+
+    def set_tuning(self, vcm_wavefront_paths=None, trace_blocks_per_cu=None, trace_lds_nodes=None):
+        self.sets += 1
+        lds = self.t["trace_lds_nodes"] if trace_lds_nodes is None else trace_lds_nodes
+        self.t = {"vcm_wavefront_paths": vcm_wavefront_paths, "trace_blocks_per_cu": trace_blocks_per_cu,
+                  "trace_lds_nodes": lds}
+
+
 def cost(t):
-    # falls with the wavefront size until 2^22, then rises; best grid 16
+    # falls with the wavefront size until 2^22, then rises; best grid 16; LDS top nodes save 1 %
     import math
     k = math.log2(t["vcm_wavefront_paths"])
-    return 1.0 + 0.05 * (k - 22) ** 2 + 0.01 * abs(t["trace_blocks_per_cu"] - 16) / 4
+    c = 1.0 + 0.05 * (k - 22) ** 2 + 0.01 * abs(t["trace_blocks_per_cu"] - 16) / 4
+    return c * (0.99 if t.get("trace_lds_nodes") else 1.0)
 
 
 def run(ctx, budget=None, footprint=lambda c, s, L: s * 1000):
@@ -63,3 +76,11 @@ def test_cache_roundtrip(tmp_path):
     A.save(p, "j", {"vcm_wavefront_paths": 16, "trace_blocks_per_cu": 8})
     assert A.load(p, "k") == {"vcm_wavefront_paths": 8, "trace_blocks_per_cu": 4}
     assert A.load(p, "j")["trace_blocks_per_cu"] == 8
+
+
+def test_lds_nodes_tried_at_the_best_shape():
+    ctx = SimCtxLds()
+    r = run(ctx)
+    assert r["best"] == {"vcm_wavefront_paths": 1 << 22, "trace_blocks_per_cu": 16, "trace_lds_nodes": 9}
+    assert ctx.tuning() == r["best"]
+    assert r["seconds"] == pytest.approx(0.99)

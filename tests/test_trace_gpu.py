@@ -204,18 +204,21 @@ def test_wide_and_binary_trees_agree(gpu_ctx, seed):
     old = gpu_ctx.tuning()
     try:
         out, stats = {}, {}
-        for tree in (0, 1):
-            gpu_ctx.set_tuning(trace_tree=tree)
+        for tree in (0, 1, 2):  # 2: the 8-wide tree with its top nodes read from LDS
+            gpu_ctx.set_tuning(trace_tree=min(tree, 1), trace_lds_nodes=9 if tree == 2 else 0)
             out[tree] = gpu_trace(gpu_ctx, t9, rays)
+            if tree == 2:
+                continue
             scene = N.Scene(gpu_ctx)
             scene.add_soup(t9)
             scene.commit()
             stats[tree] = scene.intersect_stats(N.DeviceRays(gpu_ctx, rays))
             scene.release()
-        for k in ("prim_id", "geom_id", "occluded"):
-            assert np.array_equal(out[0][k], out[1][k]), k
-        for k in ("t", "u", "v"):
-            assert np.array_equal(out[0][k].view(np.uint32), out[1][k].view(np.uint32)), k
+        for other in (1, 2):
+            for k in ("prim_id", "geom_id", "occluded"):
+                assert np.array_equal(out[0][k], out[other][k]), (other, k)
+            for k in ("t", "u", "v"):
+                assert np.array_equal(out[0][k].view(np.uint32), out[other][k].view(np.uint32)), (other, k)
         assert stats[0]["bytes_node"] == 64 and stats[1]["bytes_node"] == 128
         assert stats[1]["nodes_visited"] < 0.6 * stats[0]["nodes_visited"]
     finally:

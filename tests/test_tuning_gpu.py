@@ -12,11 +12,12 @@ pytestmark = pytest.mark.gpu
 def test_tuning_roundtrip_and_limits(gpu_ctx):
     old = gpu_ctx.tuning()
     try:
-        gpu_ctx.set_tuning(vcm_wavefront_paths=1 << 14, trace_blocks_per_cu=3, trace_tree=0)
-        want = {"vcm_wavefront_paths": 1 << 14, "trace_blocks_per_cu": 3, "trace_tree": 0}
+        gpu_ctx.set_tuning(vcm_wavefront_paths=1 << 14, trace_blocks_per_cu=3, trace_tree=0, trace_lds_nodes=9)
+        want = {"vcm_wavefront_paths": 1 << 14, "trace_blocks_per_cu": 3, "trace_tree": 0, "trace_lds_nodes": 9}
         assert gpu_ctx.tuning() == want
         for bad in ({"vcm_wavefront_paths": 1 << 11}, {"vcm_wavefront_paths": (1 << 24) + 1},
-                    {"trace_blocks_per_cu": 0}, {"trace_blocks_per_cu": 33}, {"trace_tree": 2}):
+                    {"trace_blocks_per_cu": 0}, {"trace_blocks_per_cu": 33}, {"trace_tree": 2},
+                    {"trace_lds_nodes": 10}):
             with pytest.raises(N.SgpuError):
                 gpu_ctx.set_tuning(**bad)
         assert gpu_ctx.tuning() == want
