@@ -107,7 +107,9 @@ typedef struct sgpu_tuning {
                                      nodes (128 B, default); both are built at commit */
     uint32_t trace_lds_nodes;     /* 8-wide single-scene launches: this many top nodes (breadth-first)
                                      read from LDS instead of memory: 0..9 (default 0) */
-    uint32_t reserved;            /* 0 */
+    uint32_t vcm_graphs;          /* 1 (default): each VCM wavefront replays its light and camera
+                                     passes as captured hipGraphs (queue counts read on the device);
+                                     0: launches one by one, sized from counts read back per bounce */
 } sgpu_tuning;
 int sgpu_set_tuning(sgpu_context ctx, const sgpu_tuning* tuning);
 int sgpu_get_tuning(sgpu_context ctx, sgpu_tuning* tuning);

@@ -87,11 +87,12 @@ int sgpu_set_tuning(sgpu_context ctx, const sgpu_tuning* t)
         return sgpu_fail(SGPU_E_RANGE, "sgpu_set_tuning: trace_blocks_per_cu outside 1..32");
     if (t->trace_tree > 1) return sgpu_fail(SGPU_E_RANGE, "sgpu_set_tuning: trace_tree must be 0 or 1");
     if (t->trace_lds_nodes > 9) return sgpu_fail(SGPU_E_RANGE, "sgpu_set_tuning: trace_lds_nodes outside 0..9");
-    if (t->reserved != 0) return sgpu_fail(SGPU_E_INVALID, "sgpu_set_tuning: reserved must be 0");
+    if (t->vcm_graphs > 1) return sgpu_fail(SGPU_E_RANGE, "sgpu_set_tuning: vcm_graphs must be 0 or 1");
     ctx->vcm_wavefront_paths = t->vcm_wavefront_paths;
     ctx->trace_blocks_per_cu = t->trace_blocks_per_cu;
     ctx->trace_tree = t->trace_tree;
     ctx->trace_lds_nodes = t->trace_lds_nodes;
+    ctx->vcm_graphs = t->vcm_graphs;
     return SGPU_OK;
 }
 
@@ -102,7 +103,7 @@ int sgpu_get_tuning(sgpu_context ctx, sgpu_tuning* t)
     t->trace_blocks_per_cu = ctx->trace_blocks_per_cu;
     t->trace_tree = ctx->trace_tree;
     t->trace_lds_nodes = ctx->trace_lds_nodes;
-    t->reserved = 0;
+    t->vcm_graphs = ctx->vcm_graphs;
     return SGPU_OK;
 }
 

@@ -68,6 +68,7 @@ struct sgpu_context_s {
     uint32_t trace_blocks_per_cu = 8;
     uint32_t trace_tree = 1;  // 0: binary nodes, 1: 8-wide compressed nodes
     uint32_t trace_lds_nodes = 0;  // top wide nodes read from LDS (single-scene launches)
+    uint32_t vcm_graphs = 1;       // VCM passes replayed as captured graphs
     // the VCM server of the last sgpu_render_vcm(_batch) call, reused while large enough
     std::mutex vcm_mu;
     sgpu_vcm_server vcm_cache = nullptr;
@@ -124,10 +125,11 @@ int sgpu_bvh_build(sgpu_scene scene, hipStream_t stream);
 // Launches the traversal kernels (sgpu_trace.hip), on the context's tree kind (trace_tree).
 // `spill` (may be NULL: then the call allocates its own, stream-ordered) holds the spilled part of
 // the per-lane stacks: at least sgpu_trace_spill_words(scene) 32-bit words, not shared with a
-// concurrent launch.
+// concurrent launch. With n_ptr, the kernels read the ray count from that device word (n then only
+// sizes the grid and must bound it); `overflow` replaces the scene's stack-overflow flag.
 int sgpu_trace_launch(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits,
                       uint32_t* occluded, uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream,
-                      uint32_t* spill = nullptr);
+                      uint32_t* spill = nullptr, const uint32_t* n_ptr = nullptr, uint32_t* overflow = nullptr);
 size_t sgpu_trace_spill_words(sgpu_scene scene);
 size_t sgpu_trace_spill_words(sgpu_context ctx, uint32_t stack_cap);
 // A scene's trees as a multi-scene traversal launch reads them (one table entry per scene).
@@ -143,7 +145,7 @@ struct TraceScene {
 int sgpu_trace_launch_multi(sgpu_context ctx, const TraceScene* table, const uint32_t* ray_scene, uint32_t stack_cap,
                             uint32_t* overflow, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits,
                             uint32_t* occluded, uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream,
-                            uint32_t* spill = nullptr);
+                            uint32_t* spill = nullptr, const uint32_t* n_ptr = nullptr);
 // Wavefront path tracer (sgpu_pt.hip); arguments validated by sgpu_render_pt.
 int sgpu_pt_render_impl(sgpu_scene scene, const sgpu_pt_job* job, float* framebuffer, hipStream_t stream);
 int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* framebuffer, sgpu_vcm_stats* stats,

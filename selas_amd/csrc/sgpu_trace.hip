@@ -101,6 +101,7 @@ __global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace(const BvhNode* __res
                                                             const uint32_t* __restrict ray_scene,
                                                             sgpu_rays_soa rays, sgpu_hits_soa hits,
                                                             uint32_t* __restrict occluded, uint32_t n,
+                                                            const uint32_t* __restrict n_ptr,
                                                             uint32_t* __restrict spill, uint32_t stack_cap,
                                                             unsigned long long* __restrict stats,
                                                             uint32_t* __restrict overflow)
@@ -110,6 +111,7 @@ __global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace(const BvhNode* __res
     const uint32_t gtid = blockIdx.x * SGPU_TRACE_BLOCK + threadIdx.x;
     const uint32_t stride = gridDim.x * SGPU_TRACE_BLOCK;
     unsigned long long cnt_nodes = 0, cnt_tris = 0;
+    if (n_ptr) n = *n_ptr;  // the count written on the device by the kernel that queued the rays
 
     for (uint32_t i = gtid; i < n; i += stride) {
         f3 o = mk3(rays.org_x[i], rays.org_y[i], rays.org_z[i]);
@@ -248,6 +250,7 @@ __global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace8(const WideNode* __r
                                                              const uint32_t* __restrict ray_scene,
                                                              sgpu_rays_soa rays, sgpu_hits_soa hits,
                                                              uint32_t* __restrict occluded, uint32_t n,
+                                                            const uint32_t* __restrict n_ptr,
                                                              uint32_t* __restrict spill, uint32_t stack_cap,
                                                              unsigned long long* __restrict stats,
                                                              uint32_t* __restrict overflow)
@@ -258,6 +261,7 @@ __global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace8(const WideNode* __r
     const uint32_t gtid = blockIdx.x * SGPU_TRACE_BLOCK + threadIdx.x;
     const uint32_t stride = gridDim.x * SGPU_TRACE_BLOCK;
     unsigned long long cnt_nodes = 0, cnt_tris = 0;
+    if (n_ptr) n = *n_ptr;  // the count written on the device by the kernel that queued the rays
     if (MULTI) n_top = 0;
     for (uint32_t k = threadIdx.x; k < 6 * n_top; k += SGPU_TRACE_BLOCK)
         lds_top[k] = reinterpret_cast<const uint4*>(wnodes + k / 6)[k % 6];
@@ -439,22 +443,23 @@ template <bool ANY, bool STATS>
 void launch_trace(bool wide, bool multi, uint32_t blocks, hipStream_t stream, const BvhNode* nodes,
                   const WideNode* wnodes, uint32_t n_top, const TriRecord* tris, uint32_t n_tris, const TraceScene* table,
                   const uint32_t* ray_scene, const sgpu_rays_soa& rays, const sgpu_hits_soa& h, uint32_t* occluded,
-                  uint32_t n, uint32_t* spill, uint32_t cap, unsigned long long* st, uint32_t* overflow)
+                  uint32_t n, const uint32_t* n_ptr, uint32_t* spill, uint32_t cap, unsigned long long* st,
+                  uint32_t* overflow)
 {
     if (wide) {
         if (multi)
             hipLaunchKernelGGL((k_trace8<ANY, STATS, true>), dim3(blocks), dim3(SGPU_TRACE_BLOCK), 0, stream, wnodes,
-                               n_top, tris, n_tris, table, ray_scene, rays, h, occluded, n, spill, cap, st, overflow);
+                               n_top, tris, n_tris, table, ray_scene, rays, h, occluded, n, n_ptr, spill, cap, st, overflow);
         else
             hipLaunchKernelGGL((k_trace8<ANY, STATS, false>), dim3(blocks), dim3(SGPU_TRACE_BLOCK), 0, stream, wnodes,
-                               n_top, tris, n_tris, table, ray_scene, rays, h, occluded, n, spill, cap, st, overflow);
+                               n_top, tris, n_tris, table, ray_scene, rays, h, occluded, n, n_ptr, spill, cap, st, overflow);
     } else {
         if (multi)
             hipLaunchKernelGGL((k_trace<ANY, STATS, true>), dim3(blocks), dim3(SGPU_TRACE_BLOCK), 0, stream, nodes,
-                               tris, n_tris, table, ray_scene, rays, h, occluded, n, spill, cap, st, overflow);
+                               tris, n_tris, table, ray_scene, rays, h, occluded, n, n_ptr, spill, cap, st, overflow);
         else
             hipLaunchKernelGGL((k_trace<ANY, STATS, false>), dim3(blocks), dim3(SGPU_TRACE_BLOCK), 0, stream, nodes,
-                               tris, n_tris, table, ray_scene, rays, h, occluded, n, spill, cap, st, overflow);
+                               tris, n_tris, table, ray_scene, rays, h, occluded, n, n_ptr, spill, cap, st, overflow);
     }
 }
 
@@ -464,7 +469,8 @@ void launch_trace(bool wide, bool multi, uint32_t blocks, hipStream_t stream, co
 int trace_launch(sgpu_context ctx, const BvhNode* nodes, const WideNode* wnodes, uint32_t n_wide, const TriRecord* tris,
                  uint32_t n_tris, uint32_t stack_cap, uint32_t* overflow, const TraceScene* table,
                  const uint32_t* ray_scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits, uint32_t* occluded,
-                 uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream, uint32_t* spill_buf)
+                 uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream, uint32_t* spill_buf,
+                 const uint32_t* n_ptr)
 {
     if (n == 0) return SGPU_OK;
     const uint32_t cus = (uint32_t)ctx->props.multiProcessorCount;
@@ -486,17 +492,17 @@ int trace_launch(sgpu_context ctx, const BvhNode* nodes, const WideNode* wnodes,
     if (any_hit) {
         if (st)
             launch_trace<true, true>(wide, multi, blocks, stream, nodes, wnodes, n_top, tris, n_tris, table, ray_scene, *rays,
-                                     h, occluded, n, spill, stack_cap, st, overflow);
+                                     h, occluded, n, n_ptr, spill, stack_cap, st, overflow);
         else
             launch_trace<true, false>(wide, multi, blocks, stream, nodes, wnodes, n_top, tris, n_tris, table, ray_scene, *rays,
-                                      h, occluded, n, spill, stack_cap, st, overflow);
+                                      h, occluded, n, n_ptr, spill, stack_cap, st, overflow);
     } else {
         if (st)
             launch_trace<false, true>(wide, multi, blocks, stream, nodes, wnodes, n_top, tris, n_tris, table, ray_scene, *rays,
-                                      h, occluded, n, spill, stack_cap, st, overflow);
+                                      h, occluded, n, n_ptr, spill, stack_cap, st, overflow);
         else
             launch_trace<false, false>(wide, multi, blocks, stream, nodes, wnodes, n_top, tris, n_tris, table, ray_scene,
-                                       *rays, h, occluded, n, spill, stack_cap, st, overflow);
+                                       *rays, h, occluded, n, n_ptr, spill, stack_cap, st, overflow);
     }
     SGPU_HIP_TRY(hipGetLastError());
     if (!spill_buf) SGPU_HIP_TRY(hipFreeAsync(spill, stream));
@@ -516,19 +522,19 @@ size_t sgpu_trace_spill_words(sgpu_context ctx, uint32_t stack_cap)
 size_t sgpu_trace_spill_words(sgpu_scene scene) { return sgpu_trace_spill_words(scene->ctx, scene->stack_cap); }
 
 int sgpu_trace_launch(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits, uint32_t* occluded,
-                      uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream, uint32_t* spill_buf)
+                      uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream, uint32_t* spill_buf,
+                      const uint32_t* n_ptr, uint32_t* overflow)
 {
     return trace_launch(scene->ctx, scene->d_nodes, scene->d_wide, scene->n_wide, scene->d_tris, scene->n_tris,
-                        scene->stack_cap,
-                        scene->d_overflow, nullptr, nullptr, rays, hits, occluded, n, any_hit, d_stats, stream,
-                        spill_buf);
+                        scene->stack_cap, overflow ? overflow : scene->d_overflow, nullptr, nullptr, rays, hits,
+                        occluded, n, any_hit, d_stats, stream, spill_buf, n_ptr);
 }
 
 int sgpu_trace_launch_multi(sgpu_context ctx, const TraceScene* table, const uint32_t* ray_scene, uint32_t stack_cap,
                             uint32_t* overflow, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits,
                             uint32_t* occluded, uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream,
-                            uint32_t* spill_buf)
+                            uint32_t* spill_buf, const uint32_t* n_ptr)
 {
     return trace_launch(ctx, nullptr, nullptr, 0, nullptr, 0, stack_cap, overflow, table, ray_scene, rays, hits, occluded,
-                        n, any_hit, d_stats, stream, spill_buf);
+                        n, any_hit, d_stats, stream, spill_buf, n_ptr);
 }

@@ -157,10 +157,9 @@ struct Slice {
 // Wavefront-wide arguments.
 struct Wave {
     const Slice* slices;
-    uint32_t n_slices;
-    uint32_t total;   // paths in the wavefront (sum of the slices' N)
-    uint32_t mlv;     // vertex slots per path (largest slice mlv)
-    uint32_t nslots;  // camera contribution slots per path (mlv + 2)
+    const uint32_t* dims;  // device: [0] slices, [1] paths in the wavefront (sum of the slices' N)
+    uint32_t mlv;          // vertex slots per path (largest slice mlv)
+    uint32_t nslots;       // camera contribution slots per path (mlv + 2)
 };
 
 __device__ __forceinline__ f3 g3(const float* a, uint32_t i) { return mk3(a[3 * i], a[3 * i + 1], a[3 * i + 2]); }
@@ -192,7 +191,7 @@ __device__ __forceinline__ Pcg32 rng_load(const PathState& ps, uint32_t g, const
 // slice of wavefront path g (slices are sorted by path_base)
 __device__ __forceinline__ uint32_t find_slice(const Wave& w, uint32_t g)
 {
-    uint32_t lo = 0, hi = w.n_slices - 1;
+    uint32_t lo = 0, hi = w.dims[0] - 1;
     while (lo < hi) {
         uint32_t mid = (lo + hi + 1) >> 1;
         if (w.slices[mid].path_base <= g)
@@ -256,11 +255,14 @@ __device__ __forceinline__ bool scatter(U& u, const Slice& sl, const Surface& s,
 __global__ __launch_bounds__(kBlock) void k_light_gen(Wave w, PathState ps, RayBuf rays, uint32_t* __restrict queue,
                                                       uint32_t* __restrict counters)
 {
-    uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t total = w.dims[1];
+    // blocks past the end leave at once (captured launches are sized for capacity)
+    if (blockIdx.x * kBlock >= total) return;
+    const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
     bool live = false;
     uint32_t scn = 0;
     f3 o = mk3(0, 0, 0), d = mk3(0, 0, 0);
-    if (g < w.total) {
+    if (g < total) {
         const uint32_t si = find_slice(w, g);
         const Slice& sl = w.slices[si];
         scn = sl.scene;
@@ -319,14 +321,17 @@ __global__ __launch_bounds__(kBlock) void k_light_gen(Wave w, PathState ps, RayB
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_light_shade(Wave w, uint32_t count, const uint32_t* __restrict queue,
+__global__ __launch_bounds__(kBlock) void k_light_shade(Wave w, const uint32_t* __restrict count_ptr, const uint32_t* __restrict queue,
                                                         RayBuf rays, HitBuf hits, PathState ps, LightVertex* __restrict verts,
                                                         uint32_t* __restrict vcount, RayBuf next_rays,
                                                         uint32_t* __restrict next_queue, RayBuf shadow,
                                                         float* __restrict splat_c, uint32_t* __restrict splat_pix,
-                                                        uint32_t* __restrict counters)
+                                                        uint32_t* __restrict next_count, uint32_t* __restrict shadow_count)
 {
-    uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t count = *count_ptr;
+    // blocks past the end leave at once (captured launches are sized for capacity)
+    if (blockIdx.x * kBlock >= count) return;
+    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
     bool cont = false, emit = false;
     uint32_t p = 0, pix = 0, scn = 0;
     f3 next_o = mk3(0, 0, 0), next_d = mk3(0, 0, 0), sh_o = mk3(0, 0, 0), sh_d = mk3(0, 0, 0), c = mk3(0, 0, 0);
@@ -443,29 +448,32 @@ __global__ __launch_bounds__(kBlock) void k_light_shade(Wave w, uint32_t count, 
             ps.rng[p] = u.r.state;
         }
     }
-    uint32_t ks = wave_append(emit, &counters[1]);
+    uint32_t ks = wave_append(emit, shadow_count);
     if (emit) {
         put_ray(shadow, ks, sh_o, sh_d, sh_tnear, sh_tfar, scn);
         s3(splat_c, ks, c);
         splat_pix[ks] = pix;
     }
-    uint32_t kn = wave_append(cont, &counters[0]);
+    uint32_t kn = wave_append(cont, next_count);
     if (cont) {
         put_ray(next_rays, kn, next_o, next_d, kRayEps, kFloatMax, scn);
         next_queue[kn] = p;
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_splat(uint32_t count, const uint32_t* __restrict occluded,
+__global__ __launch_bounds__(kBlock) void k_splat(const uint32_t* __restrict count_ptr, const uint32_t* __restrict occluded,
                                                   const float* __restrict splat_c, const uint32_t* __restrict splat_pix,
                                                   float* __restrict frame)
 {
-    uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= count || occluded[k]) return;
-    uint32_t pix = splat_pix[k];
-    atomicAdd(&frame[3 * pix], splat_c[3 * k]);
-    atomicAdd(&frame[3 * pix + 1], splat_c[3 * k + 1]);
-    atomicAdd(&frame[3 * pix + 2], splat_c[3 * k + 2]);
+    const uint32_t count = *count_ptr;
+    for (uint32_t k0 = blockIdx.x * kBlock; k0 < count; k0 += gridDim.x * kBlock) {
+        const uint32_t k = k0 + threadIdx.x;
+        if (k >= count || occluded[k]) continue;
+        uint32_t pix = splat_pix[k];
+        atomicAdd(&frame[3 * pix], splat_c[3 * k]);
+        atomicAdd(&frame[3 * pix + 1], splat_c[3 * k + 1]);
+        atomicAdd(&frame[3 * pix + 2], splat_c[3 * k + 2]);
+    }
 }
 
 // path-ordered compaction of the light vertices (the order of Selas's lightVertices array)
@@ -474,16 +482,19 @@ __global__ __launch_bounds__(kBlock) void k_compact(Wave w, const LightVertex* _
                                                     const uint32_t* __restrict path_slice, LightVertex* __restrict cverts,
                                                     float* __restrict cpos, uint32_t* __restrict cslice)
 {
-    uint32_t g = blockIdx.x * kBlock + threadIdx.x;
-    if (g >= w.total) return;
-    uint32_t n = vcount[g], base = offset[g], si = path_slice[g];
-    for (uint32_t k = 0; k < n; ++k) {
-        LightVertex v = verts[(size_t)g * w.mlv + k];
-        cverts[base + k] = v;
-        cpos[3 * (base + k)] = v.pos[0];
-        cpos[3 * (base + k) + 1] = v.pos[1];
-        cpos[3 * (base + k) + 2] = v.pos[2];
-        cslice[base + k] = si;
+    const uint32_t total = w.dims[1];
+    for (uint32_t g0 = blockIdx.x * kBlock; g0 < total; g0 += gridDim.x * kBlock) {
+        const uint32_t g = g0 + threadIdx.x;
+        if (g >= total) continue;
+        uint32_t n = vcount[g], base = offset[g], si = path_slice[g];
+        for (uint32_t k = 0; k < n; ++k) {
+            LightVertex v = verts[(size_t)g * w.mlv + k];
+            cverts[base + k] = v;
+            cpos[3 * (base + k)] = v.pos[0];
+            cpos[3 * (base + k) + 1] = v.pos[1];
+            cpos[3 * (base + k) + 2] = v.pos[2];
+            cslice[base + k] = si;
+        }
     }
 }
 
@@ -609,11 +620,14 @@ __global__ __launch_bounds__(kBlock) void k_grid_gather(const uint32_t* __restri
 __global__ __launch_bounds__(kBlock) void k_cam_gen(Wave w, PathState ps, RayBuf rays, uint32_t* __restrict queue,
                                                     uint32_t* __restrict counters)
 {
-    uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t total = w.dims[1];
+    // blocks past the end leave at once (captured launches are sized for capacity)
+    if (blockIdx.x * kBlock >= total) return;
+    const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
     bool live = false;
     uint32_t scn = 0;
     f3 o = mk3(0, 0, 0), d = mk3(0, 0, 0);
-    if (g < w.total) {
+    if (g < total) {
         const uint32_t si = find_slice(w, g);
         const Slice& sl = w.slices[si];
         scn = sl.scene;
@@ -661,14 +675,18 @@ struct CamSlots {
     uint32_t* ray_slot;  // per queued shadow ray: path * nslots + slot
 };
 
-__global__ __launch_bounds__(kBlock) void k_cam_shade(Wave w, uint32_t count, const uint32_t* __restrict queue,
+__global__ __launch_bounds__(kBlock) void k_cam_shade(Wave w, const uint32_t* __restrict count_ptr, const uint32_t* __restrict queue,
                                                       RayBuf rays, HitBuf hits, PathState ps, GridView G, CamSlots S,
                                                       CamVertex* __restrict cam_vm, RayBuf next_rays,
                                                       uint32_t* __restrict next_queue, RayBuf shadow,
-                                                      uint32_t* __restrict counters)
+                                                      uint32_t* __restrict next_count,
+                                                      uint32_t* __restrict shadow_count)
 {
     const uint32_t nslots = w.nslots;
-    uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t count = *count_ptr;
+    // blocks past the end leave at once (captured launches are sized for capacity)
+    if (blockIdx.x * kBlock >= count) return;
+    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
     bool cont = false;
     uint32_t p = 0, si = 0;
     f3 next_o = mk3(0, 0, 0), next_d = mk3(0, 0, 0);
@@ -781,7 +799,7 @@ __global__ __launch_bounds__(kBlock) void k_cam_shade(Wave w, uint32_t count, co
                 }
             }
         }
-        uint32_t ks = wave_append(emit, &counters[1]);
+        uint32_t ks = wave_append(emit, shadow_count);
         if (emit) {
             put_ray(shadow, ks, so, ld, tn, tf, sl.scene);
             S.ray_slot[ks] = p * nslots + 0;
@@ -846,7 +864,7 @@ __global__ __launch_bounds__(kBlock) void k_cam_shade(Wave w, uint32_t count, co
                 vc_live = false;  // the reference stops at the first vertex that is too long (VCM.cpp:631)
             }
         }
-        uint32_t ks = wave_append(emit, &counters[1]);
+        uint32_t ks = wave_append(emit, shadow_count);
         if (emit) {
             put_ray(shadow, ks, so, cd, tn, tf, sl.scene);
             S.ray_slot[ks] = p * nslots + 1 + k;
@@ -898,7 +916,7 @@ __global__ __launch_bounds__(kBlock) void k_cam_shade(Wave w, uint32_t count, co
         }
         ps.rng[p] = u.r.state;
     }
-    uint32_t kn = wave_append(cont, &counters[0]);
+    uint32_t kn = wave_append(cont, next_count);
     if (cont) {
         put_ray(next_rays, kn, next_o, next_d, kRayEps, kFloatMax, sl.scene);
         next_queue[kn] = p;
@@ -921,9 +939,11 @@ __global__ __launch_bounds__(kBlock) void k_cam_shade(Wave w, uint32_t count, co
 // gives the query's sum.
 constexpr uint32_t kMergeWaves = kBlock / 64;
 
-__global__ __launch_bounds__(kBlock) void k_cam_merge(Wave w, uint32_t count, const CamVertex* __restrict cam_vm,
+__global__ __launch_bounds__(kBlock) void k_cam_merge(Wave w, const uint32_t* __restrict count_ptr,
+                                                      const CamVertex* __restrict cam_vm,
                                                       const uint32_t* __restrict path_slice, GridView G, CamSlots S)
 {
+    const uint32_t count = *count_ptr;
     __shared__ uint32_t list_lds[kMergeWaves][2][64];
     const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
     const uint32_t half = lane >> 5, hl = lane & 31u;  // half-wave and lane within it
@@ -1055,24 +1075,30 @@ __global__ __launch_bounds__(kBlock) void k_cam_merge(Wave w, uint32_t count, co
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_cam_resolve(uint32_t count, const uint32_t* __restrict occluded, CamSlots S)
+__global__ __launch_bounds__(kBlock) void k_cam_resolve(const uint32_t* __restrict count_ptr, const uint32_t* __restrict occluded, CamSlots S)
 {
-    uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= count) return;
-    S.valid[S.ray_slot[k]] = occluded[k] ? 3u : 1u;
+    const uint32_t count = *count_ptr;
+    for (uint32_t k0 = blockIdx.x * kBlock; k0 < count; k0 += gridDim.x * kBlock) {
+        const uint32_t k = k0 + threadIdx.x;
+        if (k >= count) continue;
+        S.valid[S.ray_slot[k]] = occluded[k] ? 3u : 1u;
+    }
 }
 
 // color += the path's valid slots, in slot order (NEE, connections, merging), as the reference adds them
-__global__ __launch_bounds__(kBlock) void k_cam_gather(uint32_t count, const uint32_t* __restrict queue, uint32_t nslots,
+__global__ __launch_bounds__(kBlock) void k_cam_gather(const uint32_t* __restrict count_ptr, const uint32_t* __restrict queue, uint32_t nslots,
                                                        CamSlots S, PathState ps)
 {
-    uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-    if (j >= count) return;
-    uint32_t p = queue[j];
-    f3 col = g3(ps.color, p);
-    for (uint32_t k = 0; k < nslots; ++k)
-        if (S.valid[(size_t)p * nslots + k] == 1u) col += g3(S.c, p * nslots + k);
-    s3(ps.color, p, col);
+    const uint32_t count = *count_ptr;
+    for (uint32_t j0 = blockIdx.x * kBlock; j0 < count; j0 += gridDim.x * kBlock) {
+        const uint32_t j = j0 + threadIdx.x;
+        if (j >= count) continue;
+        uint32_t p = queue[j];
+        f3 col = g3(ps.color, p);
+        for (uint32_t k = 0; k < nslots; ++k)
+            if (S.valid[(size_t)p * nslots + k] == 1u) col += g3(S.c, p * nslots + k);
+        s3(ps.color, p, col);
+    }
 }
 
 // acc (a job's running sum, n pixels) += frame + color of each of the job's slices first..first+count-1
@@ -1205,6 +1231,18 @@ struct ServerJob {
 
 // A persistent VCM executor: wavefront buffers sized once, jobs admitted at any time and advanced
 // wavefront by wavefront (see the file comment for what a slice and a wavefront are).
+// What a captured pass depends on besides the server's own buffers.
+struct GraphKey {
+    const void* table = nullptr;
+    const void* spill = nullptr;
+    uint32_t max_stack = 0, tree = 0, blocks = 0, lds = 0;
+    bool operator==(const GraphKey& o) const
+    {
+        return table == o.table && spill == o.spill && max_stack == o.max_stack && tree == o.tree &&
+               blocks == o.blocks && lds == o.lds;
+    }
+};
+
 struct sgpu_vcm_server_s {
     sgpu_context ctx = nullptr;
     sgpu_vcm_server_config cfg;
@@ -1223,7 +1261,21 @@ struct sgpu_vcm_server_s {
     LightVertex *verts = nullptr, *cverts = nullptr, *sverts = nullptr;
     float4* spos = nullptr;
     uint32_t *cslice = nullptr, *keys = nullptr, *keys2 = nullptr, *vals = nullptr, *vals2 = nullptr;
-    uint32_t *cell_counts = nullptr, *cell_ends = nullptr, *box = nullptr, *counters = nullptr, *nv_dev = nullptr;
+    uint32_t *cell_counts = nullptr, *cell_ends = nullptr, *box = nullptr, *nv_dev = nullptr;
+    // queue counts: per pass (light, camera) and bounce b, 4 words: [0] rays of bounce b, [1] its
+    // shadow rays; the last word of the array is the traversal stack-overflow flag
+    uint32_t Lb = 0;  // bounce bound of a pass (the largest path length)
+    uint32_t* cnt = nullptr;
+    uint32_t* ovf = nullptr;
+    size_t cnt_words = 0;
+    uint32_t* dims = nullptr;  // device: slices and paths of the current wavefront
+    uint32_t* h_dims = nullptr;
+    uint32_t* h_cnt = nullptr;
+    // the two passes captured as graphs (valid while the key's pointers and tuning are unchanged)
+    hipGraphExec_t g_pass[2] = {nullptr, nullptr};
+    hipStream_t cap_stream = nullptr;
+    GraphKey gkey;
+    bool graph_failed = false;
     Slice* d_slices = nullptr;
     CamVertex* cam_vm = nullptr;
     uint32_t* spill = nullptr;
@@ -1254,6 +1306,59 @@ struct sgpu_vcm_server_s {
 namespace {
 
 constexpr uint32_t kMaxSlicesPerWave = 4096;
+
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* __restrict p, uint32_t v, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) p[i] = v;
+}
+
+// n words of p = v: a kernel, not a memset node, so that it may sit in a captured graph
+void fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t stream)
+{
+    if (n == 0) return;
+    const uint32_t grid = (uint32_t)std::min<size_t>((n + kBlock - 1) / kBlock, 2048);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid), dim3(kBlock), 0, stream, p, v, n);
+}
+
+// Captures the light and camera passes (pass_fn(pass, capture_stream)) as graphs unless the ones
+// held are still valid. A capture that fails leaves the server on eager launches.
+template <class F>
+int server_graphs(sgpu_vcm_server sv, F&& pass_fn)
+{
+    const sgpu_context ctx = sv->ctx;
+    GraphKey key;
+    key.table = sv->d_table;
+    key.spill = sv->spill;
+    key.max_stack = sv->max_stack;
+    key.tree = ctx->trace_tree;
+    key.blocks = ctx->trace_blocks_per_cu;
+    key.lds = ctx->trace_lds_nodes;
+    if (sv->g_pass[0] && sv->g_pass[1] && key == sv->gkey) return SGPU_OK;
+    for (int k = 0; k < 2; ++k) {
+        if (sv->g_pass[k]) (void)hipGraphExecDestroy(sv->g_pass[k]);
+        sv->g_pass[k] = nullptr;
+    }
+    if (!sv->cap_stream) SGPU_HIP_TRY(hipStreamCreateWithFlags(&sv->cap_stream, hipStreamNonBlocking));
+    for (int pass = 0; pass < 2; ++pass) {
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(sv->cap_stream, hipStreamCaptureModeThreadLocal);
+        int rc = e == hipSuccess ? pass_fn(pass, sv->cap_stream) : SGPU_E_HIP;
+        if (e == hipSuccess) e = hipStreamEndCapture(sv->cap_stream, &g);
+        if (rc == SGPU_OK && e == hipSuccess && g) e = hipGraphInstantiate(&sv->g_pass[pass], g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (rc != SGPU_OK || e != hipSuccess || !sv->g_pass[pass]) {
+            (void)hipGetLastError();
+            for (int k = 0; k < 2; ++k) {
+                if (sv->g_pass[k]) (void)hipGraphExecDestroy(sv->g_pass[k]);
+                sv->g_pass[k] = nullptr;
+            }
+            sv->graph_failed = true;
+            return SGPU_OK;
+        }
+    }
+    sv->gkey = key;
+    return SGPU_OK;
+}
 constexpr uint32_t kDefaultStackCap = 96;  // stack spill sized at creation (grown for deeper trees)
 
 int server_alloc(sgpu_vcm_server sv)
@@ -1306,7 +1411,11 @@ int server_alloc(sgpu_vcm_server sv)
     sv->cell_counts = A.get<uint32_t>(TM);
     sv->cell_ends = A.get<uint32_t>(TM);
     sv->box = A.get<uint32_t>(8 * (size_t)sv->max_slices);
-    sv->counters = A.get<uint32_t>(4);
+    sv->Lb = sv->cfg.max_path_length;
+    sv->cnt_words = 8 * ((size_t)sv->Lb + 2) + 4;
+    sv->cnt = A.get<uint32_t>(sv->cnt_words);
+    sv->ovf = sv->cnt + sv->cnt_words - 1;
+    sv->dims = A.get<uint32_t>(4);
     sv->nv_dev = A.get<uint32_t>(1);
     sv->frame = A.get<float>(3 * (size_t)TM);
     sv->d_slices = A.get<Slice>(sv->max_slices);
@@ -1332,10 +1441,13 @@ int server_alloc(sgpu_vcm_server sv)
     sv->spill = A.get<uint32_t>(sgpu_trace_spill_words(sv->ctx, kDefaultStackCap));
     if (A.err != hipSuccess) return sgpu_fail(SGPU_E_NOMEM, "sgpu_vcm_server: device allocation failed");
     if (A.dry) return SGPU_OK;
-    const size_t hbytes = 64 + sizeof(Slice) * (size_t)sv->max_slices;
+    const size_t slice_bytes = sizeof(Slice) * (size_t)sv->max_slices;
+    const size_t hbytes = 64 + 64 + slice_bytes + sv->cnt_words * sizeof(uint32_t);
     SGPU_HIP_TRY(hipHostMalloc((void**)&sv->hbuf, hbytes, hipHostMallocDefault));
     sv->h = (uint32_t*)sv->hbuf;
-    sv->h_slices = (Slice*)(sv->hbuf + 64);
+    sv->h_dims = (uint32_t*)(sv->hbuf + 64);
+    sv->h_slices = (Slice*)(sv->hbuf + 128);
+    sv->h_cnt = (uint32_t*)(sv->hbuf + 128 + slice_bytes);
     return SGPU_OK;
 }
 
@@ -1434,6 +1546,9 @@ int sgpu_vcm_server_destroy_impl(sgpu_vcm_server sv, hipStream_t stream)
     if (sv->h_table) (void)hipHostFree(sv->h_table);
     if (sv->d_table) (void)hipFree(sv->d_table);
     if (sv->spill_grown) (void)hipFreeAsync(sv->spill_grown, stream);
+    for (int k = 0; k < 2; ++k)
+        if (sv->g_pass[k]) (void)hipGraphExecDestroy(sv->g_pass[k]);
+    if (sv->cap_stream) (void)hipStreamDestroy(sv->cap_stream);
     delete sv;  // the arena frees its buffers, stream-ordered
     return SGPU_OK;
 }
@@ -1554,15 +1669,10 @@ int sgpu_vcm_server_step_impl(sgpu_vcm_server sv, hipStream_t stream, uint64_t* 
 
     const bool kernel_times = flags & SGPU_VCM_KERNEL_TIMES;
     uint64_t* d_trav = (flags & SGPU_VCM_COUNT_TRAVERSAL) ? sv->d_trav : nullptr;
+    // the two passes replay captured graphs unless launches are timed or counted one by one
+    const bool graphs = sv->ctx->vcm_graphs && !kernel_times && !d_trav && !sv->graph_failed;
+    if (graphs) single = false;  // graph launches traverse through the scene table
     PhaseTimer T(kernel_times, stream);
-    auto trace = [&](const RayBuf& r, const sgpu_hits_soa* hits, uint32_t* occ, uint32_t n, bool any,
-                     uint64_t* d_st) -> int {
-        sgpu_rays_soa rs = as_soa(r);
-        if (!single)
-            return sgpu_trace_launch_multi(sv->ctx, sv->d_table, r.scene, sv->max_stack, first_scene->d_overflow, &rs,
-                                           hits, occ, n, any, d_st, stream, sv->spill);
-        return sgpu_trace_launch(first_scene, &rs, hits, occ, n, any, d_st, stream, sv->spill);
-    };
     // per-ray scene indices are only written and read in multi-scene wavefronts
     RayBuf rays[2] = {sv->rays[0], sv->rays[1]}, shadow = sv->shadow;
     if (single) rays[0].scene = rays[1].scene = shadow.scene = nullptr;
@@ -1571,71 +1681,156 @@ int sgpu_vcm_server_step_impl(sgpu_vcm_server sv, hipStream_t stream, uint64_t* 
     CamSlots& S = sv->S;
     uint32_t* h = sv->h;
     uint32_t** queue = sv->queue;
+    const uint32_t TM = sv->TM;
+    const uint32_t Lb = sv->Lb;
+    const uint64_t max_shadow = (uint64_t)TM * (sv->mlv + 1);
+    auto cnt_at = [&](int pass, uint32_t b) { return sv->cnt + 4 * ((size_t)pass * (Lb + 2) + b); };
+    const uint32_t cus = (uint32_t)sv->ctx->props.multiProcessorCount;
+    // eager launches: one thread per item; captured ones (sized for capacity): a grid-stride cap
+    auto grid_for = [&](uint64_t n) {
+        return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(blocks_for(n), 64ull * cus));
+    };
 
+    sv->h_dims[0] = ns;
+    sv->h_dims[1] = total;
     SGPU_HIP_TRY(hipMemcpyAsync(sv->d_slices, sv->h_slices, sizeof(Slice) * ns, hipMemcpyHostToDevice, stream));
+    SGPU_HIP_TRY(hipMemcpyAsync(sv->dims, sv->h_dims, 2 * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
     if (d_trav) SGPU_HIP_TRY(hipMemsetAsync(d_trav, 0, 2 * sizeof(uint64_t), stream));
     Wave wv;
     wv.slices = sv->d_slices;
-    wv.n_slices = ns;
-    wv.total = total;
+    wv.dims = sv->dims;
     wv.mlv = sv->mlv;
     wv.nslots = sv->nslots;
     sv->n_samples += total;
     ++sv->n_waves;
     sgpu_hits_soa hsoa{hb.t, hb.u, hb.v, hb.prim, hb.geom, hb.ngx, hb.ngy, hb.ngz};
+    GridView G;
+    G.box = sv->box;
+    G.cell_ends = sv->cell_ends;
+    G.sverts = sv->sverts;
+    G.spos = sv->spos;
+    G.cverts = sv->cverts;
+    G.path_end = sv->path_end;
+    G.nv_ptr = sv->nv_dev;
 
-    SGPU_HIP_TRY(hipMemsetAsync(sv->frame, 0, 3 * (size_t)total * sizeof(float), stream));
-    SGPU_HIP_TRY(hipMemsetAsync(sv->vcount, 0, ((size_t)total + 1) * sizeof(uint32_t), stream));
-    SGPU_HIP_TRY(hipMemsetAsync(sv->counters, 0, 4 * sizeof(uint32_t), stream));
-
-    // ---- light pass ----
-    SGPU_HIP_TRY(T.begin(kPhaseLight));
-    hipLaunchKernelGGL(k_light_gen, dim3(blocks_for(total)), dim3(kBlock), 0, stream, wv, ps, rays[0], queue[0],
-                       sv->counters);
+    // counts, the frame and the per-path vertex counts start at zero
+    fill_u32(sv->cnt, 0u, sv->cnt_words, stream);
+    fill_u32((uint32_t*)sv->frame, 0u, 3 * (size_t)total, stream);
+    fill_u32(sv->vcount, 0u, (size_t)total + 1, stream);
     SGPU_HIP_TRY(hipGetLastError());
-    SGPU_HIP_TRY(T.end());
-    SGPU_HIP_TRY(hipMemcpyAsync(h, sv->counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-    SGPU_HIP_TRY(hipStreamSynchronize(stream));
-    uint32_t count = h[0];
-    int cur = 0;
-    while (count > 0) {
-        SGPU_HIP_TRY(T.begin(kPhaseClosest));
-        rc = trace(rays[cur], &hsoa, nullptr, count, false, d_trav);
-        if (rc) return rc;
-        SGPU_HIP_TRY(T.end());
-        sv->n_closest += count;
-        ++sv->n_launches;
-        SGPU_HIP_TRY(hipMemsetAsync(sv->counters, 0, 4 * sizeof(uint32_t), stream));
-        SGPU_HIP_TRY(T.begin(kPhaseLight));
-        hipLaunchKernelGGL(k_light_shade, dim3(blocks_for(count)), dim3(kBlock), 0, stream, wv, count, queue[cur],
-                           rays[cur], hb, ps, sv->verts, sv->vcount, rays[1 - cur], queue[1 - cur], shadow, sv->splat_c,
-                           sv->splat_pix, sv->counters);
+
+    // One pass (0: light subpaths, 1: camera subpaths) on stream `st`. Eager: the host reads each
+    // bounce's counts to size its launches and stops at the first empty bounce. Captured: every
+    // bounce up to the path-length bound is launched at capacity and the kernels read the counts.
+    auto run_pass = [&](int pass, hipStream_t st, bool eager) -> int {
+        // one item per thread (the heavy kernels, which exit per block past the count) or grid-stride
+        auto grid1 = [&](uint64_t n) { return std::max<uint32_t>(1, blocks_for(n)); };
+        auto grid = [&](uint64_t n) { return eager ? grid1(n) : grid_for(n); };
+        auto read_counts = [&](const uint32_t* c) -> int {
+            SGPU_HIP_TRY(hipMemcpyAsync(h, c, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+            SGPU_HIP_TRY(hipStreamSynchronize(st));
+            return SGPU_OK;
+        };
+        auto trace = [&](const RayBuf& r, const sgpu_hits_soa* hits, uint32_t* occ, uint32_t n, const uint32_t* n_ptr,
+                         bool any, uint64_t* d_st) -> int {
+            sgpu_rays_soa rs = as_soa(r);
+            if (!single)
+                return sgpu_trace_launch_multi(sv->ctx, sv->d_table, r.scene, sv->max_stack, sv->ovf, &rs, hits, occ, n,
+                                               any, d_st, st, sv->spill, n_ptr);
+            return sgpu_trace_launch(first_scene, &rs, hits, occ, n, any, d_st, st, sv->spill, n_ptr, sv->ovf);
+        };
+        const int ph_pass = pass == 0 ? kPhaseLight : kPhaseCamera;
+        SGPU_HIP_TRY(T.begin(ph_pass));
+        if (pass == 0)
+            hipLaunchKernelGGL(k_light_gen, dim3(grid1(eager ? total : TM)), dim3(kBlock), 0, st, wv, ps, rays[0],
+                               queue[0], cnt_at(0, 0));
+        else
+            hipLaunchKernelGGL(k_cam_gen, dim3(grid1(eager ? total : TM)), dim3(kBlock), 0, st, wv, ps, rays[0],
+                               queue[0], cnt_at(1, 0));
         SGPU_HIP_TRY(hipGetLastError());
         SGPU_HIP_TRY(T.end());
-        SGPU_HIP_TRY(hipMemcpyAsync(h, sv->counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-        SGPU_HIP_TRY(hipStreamSynchronize(stream));
-        uint32_t n_next = h[0], n_sh = h[1];
-        if (n_sh > 0) {
-            SGPU_HIP_TRY(T.begin(kPhaseShadow));
-            rc = trace(shadow, nullptr, sv->occl, n_sh, true, nullptr);
-            if (rc) return rc;
+        for (uint32_t b = 0; b < Lb; ++b) {
+            const int cur = b & 1;
+            uint32_t* c_in = cnt_at(pass, b);
+            uint32_t* c_next = cnt_at(pass, b + 1);
+            uint32_t* c_shadow = c_in + 1;
+            uint32_t n = TM;
+            if (eager) {
+                int rc2 = read_counts(c_in);
+                if (rc2) return rc2;
+                n = h[0];
+                if (n == 0) break;
+            }
+            SGPU_HIP_TRY(T.begin(kPhaseClosest));
+            int rc2 = trace(rays[cur], &hsoa, nullptr, n, c_in, false, d_trav);
+            if (rc2) return rc2;
             SGPU_HIP_TRY(T.end());
-            sv->n_shadow += n_sh;
-            SGPU_HIP_TRY(T.begin(kPhaseLight));
-            hipLaunchKernelGGL(k_splat, dim3(blocks_for(n_sh)), dim3(kBlock), 0, stream, n_sh, sv->occl, sv->splat_c,
-                               sv->splat_pix, sv->frame);
+            SGPU_HIP_TRY(T.begin(ph_pass));
+            if (pass == 0)
+                hipLaunchKernelGGL(k_light_shade, dim3(grid1(n)), dim3(kBlock), 0, st, wv, c_in, queue[cur], rays[cur],
+                                   hb, ps, sv->verts, sv->vcount, rays[1 - cur], queue[1 - cur], shadow, sv->splat_c,
+                                   sv->splat_pix, c_next, c_shadow);
+            else
+                hipLaunchKernelGGL(k_cam_shade, dim3(grid1(n)), dim3(kBlock), 0, st, wv, c_in, queue[cur], rays[cur],
+                                   hb, ps, G, S, sv->cam_vm, rays[1 - cur], queue[1 - cur], shadow, c_next, c_shadow);
             SGPU_HIP_TRY(hipGetLastError());
             SGPU_HIP_TRY(T.end());
+            if (pass == 1) {
+                SGPU_HIP_TRY(T.begin(kPhaseMerge));
+                const uint32_t merge_blocks = std::min<uint32_t>((n + 2 * kMergeWaves - 1) / (2 * kMergeWaves), 16u * cus);
+                hipLaunchKernelGGL(k_cam_merge, dim3(merge_blocks), dim3(kBlock), 0, st, wv, c_in, sv->cam_vm, ps.slice,
+                                   G, S);
+                SGPU_HIP_TRY(hipGetLastError());
+                SGPU_HIP_TRY(T.end());
+            }
+            uint64_t n_sh = max_shadow;
+            if (eager) {
+                rc2 = read_counts(c_in);
+                if (rc2) return rc2;
+                n_sh = h[1];
+            }
+            if (n_sh > 0) {
+                SGPU_HIP_TRY(T.begin(kPhaseShadow));
+                rc2 = trace(shadow, nullptr, sv->occl, (uint32_t)n_sh, c_shadow, true, nullptr);
+                if (rc2) return rc2;
+                SGPU_HIP_TRY(T.end());
+                SGPU_HIP_TRY(T.begin(ph_pass));
+                if (pass == 0)
+                    hipLaunchKernelGGL(k_splat, dim3(grid(n_sh)), dim3(kBlock), 0, st, c_shadow, sv->occl,
+                                       sv->splat_c, sv->splat_pix, sv->frame);
+                else
+                    hipLaunchKernelGGL(k_cam_resolve, dim3(grid(n_sh)), dim3(kBlock), 0, st, c_shadow, sv->occl, S);
+                SGPU_HIP_TRY(hipGetLastError());
+                SGPU_HIP_TRY(T.end());
+            }
+            if (pass == 1) {
+                SGPU_HIP_TRY(T.begin(kPhaseCamera));
+                hipLaunchKernelGGL(k_cam_gather, dim3(grid(n)), dim3(kBlock), 0, st, c_in, queue[cur], sv->nslots, S,
+                                   ps);
+                SGPU_HIP_TRY(hipGetLastError());
+                SGPU_HIP_TRY(T.end());
+            }
         }
-        count = n_next;
-        cur = 1 - cur;
+        return SGPU_OK;
+    };
+    if (graphs) {
+        rc = server_graphs(sv, [&](int pass, hipStream_t cs) { return run_pass(pass, cs, false); });
+        if (rc) return rc;
+    }
+
+    // ---- light pass ----
+    if (graphs && sv->g_pass[0]) {
+        SGPU_HIP_TRY(hipGraphLaunch(sv->g_pass[0], stream));
+    } else {
+        rc = run_pass(0, stream, true);
+        if (rc) return rc;
     }
 
     // ---- light vertices, in path order ----
     SGPU_HIP_TRY(T.begin(kPhaseGrid));
     SGPU_HIP_TRY(rocprim::exclusive_scan(sv->tmp, sv->tmp_bytes, sv->vcount, sv->voffset, 0u, (size_t)total + 1,
                                          rocprim::plus<uint32_t>(), stream));
-    hipLaunchKernelGGL(k_compact, dim3(blocks_for(total)), dim3(kBlock), 0, stream, wv, sv->verts, sv->vcount,
+    hipLaunchKernelGGL(k_compact, dim3(std::max<uint32_t>(1, blocks_for(total))), dim3(kBlock), 0, stream, wv, sv->verts, sv->vcount,
                        sv->voffset, ps.slice, sv->cverts, sv->cpos, sv->cslice);
     SGPU_HIP_TRY(hipGetLastError());
     // path_end[g] = offset[g + 1] (inclusive end); the vertex count is offset[total]
@@ -1672,67 +1867,13 @@ int sgpu_vcm_server_step_impl(sgpu_vcm_server sv, hipStream_t stream, uint64_t* 
     SGPU_HIP_TRY(rocprim::inclusive_scan(sv->tmp, sv->tmp_bytes, sv->cell_counts, sv->cell_ends, (size_t)total,
                                          rocprim::plus<uint32_t>(), stream));
     SGPU_HIP_TRY(T.end());
-    GridView G;
-    G.box = sv->box;
-    G.cell_ends = sv->cell_ends;
-    G.sverts = sv->sverts;
-    G.spos = sv->spos;
-    G.cverts = sv->cverts;
-    G.path_end = sv->path_end;
-    G.nv_ptr = sv->nv_dev;
 
     // ---- camera pass ----
-    SGPU_HIP_TRY(hipMemsetAsync(sv->counters, 0, 4 * sizeof(uint32_t), stream));
-    SGPU_HIP_TRY(T.begin(kPhaseCamera));
-    hipLaunchKernelGGL(k_cam_gen, dim3(blocks_for(total)), dim3(kBlock), 0, stream, wv, ps, rays[0], queue[0],
-                       sv->counters);
-    SGPU_HIP_TRY(hipGetLastError());
-    SGPU_HIP_TRY(T.end());
-    SGPU_HIP_TRY(hipMemcpyAsync(h, sv->counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-    SGPU_HIP_TRY(hipStreamSynchronize(stream));
-    count = h[0];
-    cur = 0;
-    while (count > 0) {
-        SGPU_HIP_TRY(T.begin(kPhaseClosest));
-        rc = trace(rays[cur], &hsoa, nullptr, count, false, d_trav);
+    if (graphs && sv->g_pass[1]) {
+        SGPU_HIP_TRY(hipGraphLaunch(sv->g_pass[1], stream));
+    } else {
+        rc = run_pass(1, stream, true);
         if (rc) return rc;
-        SGPU_HIP_TRY(T.end());
-        sv->n_closest += count;
-        ++sv->n_launches;
-        SGPU_HIP_TRY(hipMemsetAsync(sv->counters, 0, 4 * sizeof(uint32_t), stream));
-        SGPU_HIP_TRY(T.begin(kPhaseCamera));
-        hipLaunchKernelGGL(k_cam_shade, dim3(blocks_for(count)), dim3(kBlock), 0, stream, wv, count, queue[cur],
-                           rays[cur], hb, ps, G, S, sv->cam_vm, rays[1 - cur], queue[1 - cur], shadow, sv->counters);
-        SGPU_HIP_TRY(hipGetLastError());
-        SGPU_HIP_TRY(T.end());
-        SGPU_HIP_TRY(T.begin(kPhaseMerge));
-        const uint32_t merge_blocks = std::min<uint32_t>((count + 2 * kMergeWaves - 1) / (2 * kMergeWaves),
-                                                         16u * (uint32_t)sv->ctx->props.multiProcessorCount);
-        hipLaunchKernelGGL(k_cam_merge, dim3(merge_blocks), dim3(kBlock), 0, stream, wv, count, sv->cam_vm, ps.slice,
-                           G, S);
-        SGPU_HIP_TRY(hipGetLastError());
-        SGPU_HIP_TRY(T.end());
-        SGPU_HIP_TRY(hipMemcpyAsync(h, sv->counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-        SGPU_HIP_TRY(hipStreamSynchronize(stream));
-        uint32_t n_next = h[0], n_sh = h[1];
-        if (n_sh > 0) {
-            SGPU_HIP_TRY(T.begin(kPhaseShadow));
-            rc = trace(shadow, nullptr, sv->occl, n_sh, true, nullptr);
-            if (rc) return rc;
-            SGPU_HIP_TRY(T.end());
-            sv->n_shadow += n_sh;
-            SGPU_HIP_TRY(T.begin(kPhaseCamera));
-            hipLaunchKernelGGL(k_cam_resolve, dim3(blocks_for(n_sh)), dim3(kBlock), 0, stream, n_sh, sv->occl, S);
-            SGPU_HIP_TRY(hipGetLastError());
-            SGPU_HIP_TRY(T.end());
-        }
-        SGPU_HIP_TRY(T.begin(kPhaseCamera));
-        hipLaunchKernelGGL(k_cam_gather, dim3(blocks_for(count)), dim3(kBlock), 0, stream, count, queue[cur],
-                           sv->nslots, S, ps);
-        SGPU_HIP_TRY(hipGetLastError());
-        SGPU_HIP_TRY(T.end());
-        count = n_next;
-        cur = 1 - cur;
     }
     // each job's slices of this wavefront (consecutive, in iteration order) into its sum; a job
     // whose last iteration ran is scaled by 1 / iterations and retired
@@ -1754,7 +1895,19 @@ int sgpu_vcm_server_step_impl(sgpu_vcm_server sv, hipStream_t stream, uint64_t* 
     }
     SGPU_HIP_TRY(T.end());
     SGPU_HIP_TRY(T.collect());
+    // per-bounce ray counts (and the stack-overflow word after them) for the totals
+    SGPU_HIP_TRY(hipMemcpyAsync(sv->h_cnt, sv->cnt, sv->cnt_words * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     SGPU_HIP_TRY(hipStreamSynchronize(stream));
+    for (int pass = 0; pass < 2; ++pass)
+        for (uint32_t b = 0; b < Lb; ++b) {
+            const uint32_t* c = sv->h_cnt + 4 * ((size_t)pass * (Lb + 2) + b);
+            if (c[0] == 0) continue;
+            sv->n_closest += c[0];
+            sv->n_shadow += c[1];
+            ++sv->n_launches;
+        }
+    if (sv->h_cnt[sv->cnt_words - 1] != 0)
+        return sgpu_fail(SGPU_E_RANGE, "sgpu_vcm_server_step: traversal stack overflow (results incomplete)");
     for (int k = kPhaseClosest; k < kPhases; ++k) sv->ms[k] += T.ms[k];
     if (d_trav) {
         uint64_t tr[2] = {0, 0};

@@ -207,7 +207,7 @@ class DeviceBuffer:
 
 class Tuning(C.Structure):
     _fields_ = [("vcm_wavefront_paths", C.c_uint64), ("trace_blocks_per_cu", C.c_uint32), ("trace_tree", C.c_uint32),
-                ("trace_lds_nodes", C.c_uint32), ("reserved", C.c_uint32)]
+                ("trace_lds_nodes", C.c_uint32), ("vcm_graphs", C.c_uint32)]
 
 
 class DeviceInfo(C.Structure):
@@ -231,13 +231,14 @@ class Context:
         t = Tuning()
         check(lib.sgpu_get_tuning(self.handle, C.byref(t)))
         return {"vcm_wavefront_paths": t.vcm_wavefront_paths, "trace_blocks_per_cu": t.trace_blocks_per_cu,
-                "trace_tree": t.trace_tree, "trace_lds_nodes": t.trace_lds_nodes}
+                "trace_tree": t.trace_tree, "trace_lds_nodes": t.trace_lds_nodes, "vcm_graphs": t.vcm_graphs}
 
-    def set_tuning(self, vcm_wavefront_paths=None, trace_blocks_per_cu=None, trace_tree=None, trace_lds_nodes=None):
+    def set_tuning(self, vcm_wavefront_paths=None, trace_blocks_per_cu=None, trace_tree=None, trace_lds_nodes=None,
+                   vcm_graphs=None):
         cur = self.tuning()
         pick = lambda new, key: cur[key] if new is None else int(new)
         t = Tuning(pick(vcm_wavefront_paths, "vcm_wavefront_paths"), pick(trace_blocks_per_cu, "trace_blocks_per_cu"),
-                   pick(trace_tree, "trace_tree"), pick(trace_lds_nodes, "trace_lds_nodes"), 0)
+                   pick(trace_tree, "trace_tree"), pick(trace_lds_nodes, "trace_lds_nodes"), pick(vcm_graphs, "vcm_graphs"))
         check(lib.sgpu_set_tuning(self.handle, C.byref(t)))
 
     def array(self, a: np.ndarray) -> DeviceBuffer:

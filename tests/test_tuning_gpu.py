@@ -12,12 +12,14 @@ pytestmark = pytest.mark.gpu
 def test_tuning_roundtrip_and_limits(gpu_ctx):
     old = gpu_ctx.tuning()
     try:
-        gpu_ctx.set_tuning(vcm_wavefront_paths=1 << 14, trace_blocks_per_cu=3, trace_tree=0, trace_lds_nodes=9)
-        want = {"vcm_wavefront_paths": 1 << 14, "trace_blocks_per_cu": 3, "trace_tree": 0, "trace_lds_nodes": 9}
+        gpu_ctx.set_tuning(vcm_wavefront_paths=1 << 14, trace_blocks_per_cu=3, trace_tree=0, trace_lds_nodes=9,
+                           vcm_graphs=0)
+        want = {"vcm_wavefront_paths": 1 << 14, "trace_blocks_per_cu": 3, "trace_tree": 0, "trace_lds_nodes": 9,
+                "vcm_graphs": 0}
         assert gpu_ctx.tuning() == want
         for bad in ({"vcm_wavefront_paths": 1 << 11}, {"vcm_wavefront_paths": (1 << 24) + 1},
                     {"trace_blocks_per_cu": 0}, {"trace_blocks_per_cu": 33}, {"trace_tree": 2},
-                    {"trace_lds_nodes": 10}):
+                    {"trace_lds_nodes": 10}, {"vcm_graphs": 2}):
             with pytest.raises(N.SgpuError):
                 gpu_ctx.set_tuning(**bad)
         assert gpu_ctx.tuning() == want
@@ -45,3 +47,34 @@ def test_images_do_not_depend_on_launch_shape(gpu_ctx):
     finally:
         gpu_ctx.set_tuning(**old)
         scene.release()
+
+
+def test_graph_replays_match_eager_launches(gpu_ctx):
+    """The light and camera passes replayed from captured graphs (queue counts read on the device,
+    launches at capacity) give the images of eager launches sized from counts read back: a job of
+    5 wavefronts (5 replays of each graph, eager grid builds between them) and a two-scene batch."""
+    scs = [W.vcm_scene(seed=33, n_meshes=8, nu=16, nv=8), W.vcm_scene(seed=34, n_meshes=6, nu=12, nv=6, textures=False)]
+    scenes = [N.build_render_scene(gpu_ctx, sc) for sc in scs]
+    old = gpu_ctx.tuning()
+    try:
+        out = {}
+        for graphs in (0, 1):
+            gpu_ctx.set_tuning(vcm_wavefront_paths=1 << 12, vcm_graphs=graphs)
+            sc = scs[0]
+            img, st = N.render_vcm(scenes[0], sc["camera"], 48, 40, 9, 10, sc["merge_radius"], 0.75, 5,
+                                   return_stats=True)
+            assert st["wavefronts"] == 5
+            jobs = [{"scene": scenes[k], "camera": scs[k]["camera"], "width": 32, "height": 24, "iterations": 3,
+                     "max_path_length": 8, "base_radius": float(scs[k]["merge_radius"]), "radius_alpha": 0.75,
+                     "seed": 11 + k} for k in range(2)]
+            out[graphs] = (img, st, N.render_vcm_batch(jobs))
+        (a, sa, ba), (b, sb, bb) = out[0], out[1]
+        assert sa["samples"] == sb["samples"] and sa["closest_rays"] == sb["closest_rays"]
+        assert sa["shadow_rays"] == sb["shadow_rays"] and sa["light_vertices"] == sb["light_vertices"]
+        for x, y in [(a, b)] + list(zip(ba, bb)):
+            close = np.abs(x - y) <= 1e-4 * np.abs(y) + 1e-6
+            assert close.mean() >= 0.999
+    finally:
+        gpu_ctx.set_tuning(**old)
+        for s in scenes:
+            s.release()
