@@ -36,6 +36,7 @@
 #include "sgpu_light.h"
 #include "sgpu_ptex.h"
 #include "sgpu_surface.h"
+#include "sgpu_vcm_common.h"
 
 #include <cstring>  // rocprim/iterator/texture_cache_iterator.hpp uses memset without including it
 #include <rocprim/device/device_radix_sort.hpp>
@@ -48,207 +49,7 @@
 namespace {
 
 using namespace sgpu;
-
-constexpr int kBlock = 256;
-constexpr float kFloatMax = 3.402823466e+38F;
-constexpr float kRayEps = 0.00001f;  // RayPick's tnear in the VCM code (VCM.cpp:137)
-constexpr uint32_t kNoQuery = 0xFFFFFFFFu;
-
-__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter)
-{
-    const uint64_t mask = __ballot(pred);
-    if (mask == 0) return 0xFFFFFFFFu;
-    const uint32_t lane = __lane_id();
-    const uint32_t below = __popcll(mask & ((1ull << lane) - 1ull));
-    const uint32_t leader = __ffsll((unsigned long long)mask) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader, 64);
-    return pred ? base + below : 0xFFFFFFFFu;
-}
-
-struct RayBuf {
-    float *ox, *oy, *oz, *dx, *dy, *dz, *tnear, *tfar;
-    uint32_t* scene;  // per ray, the scene it traverses (multi-scene wavefronts only, else nullptr)
-};
-struct HitBuf {
-    float *t, *u, *v, *ngx, *ngy, *ngz;
-    uint32_t *prim, *geom;
-};
-
-__device__ __forceinline__ void put_ray(const RayBuf& r, uint32_t i, f3 o, f3 d, float tnear, float tfar,
-                                        uint32_t scene)
-{
-    if (r.scene) r.scene[i] = scene;
-    r.ox[i] = o.x;
-    r.oy[i] = o.y;
-    r.oz[i] = o.z;
-    r.dx[i] = d.x;
-    r.dy[i] = d.y;
-    r.dz[i] = d.z;
-    r.tnear[i] = tnear;
-    r.tfar[i] = tfar;
-}
-
-// Light vertex, 80 B (VCMVertex, VCMCommon.h:36-46, with the surface data a connection needs in
-// place of the hit record: position, normal, incoming direction, material, and the base colour
-// CalculateSurfaceParams derived -- the filtered texel for textured materials).
-struct LightVertex {
-    float pos[3];
-    float thr[3];
-    float n[3];
-    float wfix[3];  // direction toward the previous vertex of the light subpath
-    float dvcm, dvc, dvm;
-    uint32_t geom_plen;  // geometry << 5 | path length (path length < 32, as in PathState)
-    float base[3];       // base colour (^2.2 applied)
-    float cp;            // Russian-roulette continuation probability of the surface
-};
-static_assert(sizeof(LightVertex) == 80, "LightVertex must be 80 bytes");
-
-// Camera vertex handed from k_cam_shade to k_cam_merge (80 B); geom_plen = kNoQuery: no query.
-struct CamVertex {
-    float pos[3];
-    float n[3];
-    float view[3];
-    float thr[3];
-    float dvcm, dvm;
-    uint32_t geom_plen;  // geometry << 5 | path length
-    uint32_t path;
-    float base[3];       // base colour (^2.2 applied)
-    float pad;
-};
-static_assert(sizeof(CamVertex) == 80, "CamVertex must be 80 bytes");
-
-// Subpath state (PathState, VCMCommon.h:48-60), SoA over the paths of a wavefront.
-struct PathState {
-    float* pos;  // 3
-    float* dir;  // 3
-    float* thr;  // 3
-    float* dvcm;
-    float* dvc;
-    float* dvm;
-    uint32_t* plen;
-    uint32_t* slice;
-    uint64_t* rng;
-    float* color;  // 3, camera paths only
-};
-
-// One VCM iteration of one job (see the file comment).
-struct Slice {
-    CameraParams cam;
-    float vipd;  // W / (2 tan(hfov / 2))
-    uint32_t W, H, N, L, mlv;
-    uint32_t path_base;  // first wavefront path of the slice; also its first grid cell
-    uint32_t iteration;  // global iteration index (radius schedule and random streams)
-    uint32_t job;        // index of the job whose image the slice adds to
-    uint32_t scene;      // index of the job's scene in the wavefront's traversal table
-    uint64_t seed;
-    float vm_weight, vc_weight, vm_norm, radius_sq, inv_cell;
-    const float* materials;  // shading records (sgpu::shading_record)
-    const uint32_t* geom_material;
-    uint32_t n_geoms;
-    const Light* lights;
-    uint32_t n_lights;
-    f3 background;
-    const PtexTex* geom_tex;  // per-face texture per geometry (texels == nullptr: none)
-    uint32_t n_geom_tex;
-};
-
-// Wavefront-wide arguments.
-struct Wave {
-    const Slice* slices;
-    const uint32_t* dims;  // device: [0] slices, [1] paths in the wavefront (sum of the slices' N)
-    uint32_t mlv;          // vertex slots per path (largest slice mlv)
-    uint32_t nslots;       // camera contribution slots per path (mlv + 2)
-};
-
-__device__ __forceinline__ f3 g3(const float* a, uint32_t i) { return mk3(a[3 * i], a[3 * i + 1], a[3 * i + 2]); }
-__device__ __forceinline__ void s3(float* a, uint32_t i, f3 v)
-{
-    a[3 * i] = v.x;
-    a[3 * i + 1] = v.y;
-    a[3 * i + 2] = v.z;
-}
-
-__device__ __forceinline__ uint64_t stream_id(uint32_t iteration, uint32_t path, uint32_t pass)
-{
-    return ((uint64_t)iteration << 33) | ((uint64_t)path << 1) | pass;
-}
-
-__device__ __forceinline__ Pcg32 rng_init(uint64_t seed, uint64_t sid)
-{
-    return pcg32_seed(seed ^ (sid * 0x9E3779B97F4A7C15ull), sid);
-}
-
-__device__ __forceinline__ Pcg32 rng_load(const PathState& ps, uint32_t g, const Slice& sl, uint32_t pass)
-{
-    Pcg32 r;
-    r.state = ps.rng[g];
-    r.inc = (stream_id(sl.iteration, g - sl.path_base, pass) << 1u) | 1u;
-    return r;
-}
-
-// slice of wavefront path g (slices are sorted by path_base)
-__device__ __forceinline__ uint32_t find_slice(const Wave& w, uint32_t g)
-{
-    uint32_t lo = 0, hi = w.dims[0] - 1;
-    while (lo < hi) {
-        uint32_t mid = (lo + hi + 1) >> 1;
-        if (w.slices[mid].path_base <= g)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    return lo;
-}
-
-__device__ __forceinline__ float cont_prob(const Surface& s) { return saturate_(max_(max_(s.base_color.x, s.base_color.y), s.base_color.z)); }
-
-// Surface of a stored light vertex (the material transforms of CalculateSurfaceParams).
-__device__ __forceinline__ void vertex_surface(const Slice& sl, const LightVertex& v, Surface& s)
-{
-    const float* mat = sl.materials + 22 * (size_t)sl.geom_material[v.geom_plen >> 5];
-    s.position = mk3(v.pos[0], v.pos[1], v.pos[2]);
-    s.error = 0.0f;
-    surface_material(mat, mk3(v.n[0], v.n[1], v.n[2]), mk3(v.wfix[0], v.wfix[1], v.wfix[2]), nullptr, s);
-    s.base_color = mk3(v.base[0], v.base[1], v.base[2]);
-}
-
-// CalculateSurfaceParams' eUsesPtex branch (SurfaceParameters.cpp:196-203): the filtered texel of
-// the hit's face at its (u, v) when the material asks for it and its geometry has a texture.
-__device__ __forceinline__ bool hit_texel(const Slice& sl, uint32_t geom, const float* mat, uint32_t prim, float u,
-                                          float v, f3& texel)
-{
-    if (!((uint32_t)mat[21] & kMatUsesPtex) || geom >= sl.n_geom_tex) return false;
-    const PtexTex tx = sl.geom_tex[geom];
-    if (!tx.texels) return false;
-    texel = ptex_eval(tx, (int)prim, u, v);
-    return true;
-}
-
-// SampleBsdfScattering (VCM.cpp:412-446) with the Russian-roulette-compensated throughput.
-template <class U>
-__device__ __forceinline__ bool scatter(U& u, const Slice& sl, const Surface& s, f3& dir, f3& thr, float& dvcm,
-                                        float& dvc, float& dvm, uint32_t& plen)
-{
-    BsdfSample bs;
-    if (!bsdf_sample(u, s, -dir, bs)) return false;
-    if (is_zero3(bs.reflectance)) return false;
-    float cp = cont_prob(s);
-    if (u.next() > cp) return false;
-    float fwd = bs.fwd_pdf * cp, rev = bs.rev_pdf * cp;
-    float cos_b = absf_(dot3(bs.wi, s.n));
-    thr = (thr * bs.reflectance) * (1.0f / cp);
-    float k = cos_b / fwd;
-    float nvc = k * (dvc * rev + dvcm + sl.vm_weight);
-    float nvm = k * (dvm * rev + dvcm * sl.vc_weight + 1.0f);
-    dvc = nvc;
-    dvm = nvm;
-    dvcm = 1.0f / fwd;
-    dir = bs.wi;
-    ++plen;
-    return true;
-}
+using namespace sgpu_vcm;
 
 // ---- light pass ----------------------------------------------------------------------------------
 
@@ -500,13 +301,6 @@ __global__ __launch_bounds__(kBlock) void k_compact(Wave w, const LightVertex* _
 
 // ---- photon grid (BuildHashGrid, VCMHashGrid.cpp:48-97) ---------------------------------------------
 
-__device__ __forceinline__ uint32_t f2ord(float f)
-{
-    uint32_t u = __float_as_uint(f);
-    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-__device__ __forceinline__ float ord2f(uint32_t u) { return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u); }
-
 // Per slice s: box[8 s + 0..2] = ord(min), box[8 s + 3..5] = ~ord(max), both reduced with atomicMin.
 // Vertices are slice-contiguous, so a block almost always holds one slice: it then reduces through
 // its waves and LDS and issues one atomic per value; a block that straddles slices falls back to
@@ -573,12 +367,6 @@ __global__ __launch_bounds__(kBlock) void k_grid_bounds(const float* __restrict 
             }
         }
     }
-}
-
-__device__ __forceinline__ uint32_t cell_hash(int32_t x, int32_t y, int32_t z, uint32_t cell_count)
-{
-    uint32_t ux = (uint32_t)x, uy = (uint32_t)y, uz = (uint32_t)z;
-    return ((ux * 73856093u) ^ (uy * 19349663u) ^ (uz * 83492791u)) % cell_count;
 }
 
 // key = the slice's first cell + hashed cell (a slice of N paths owns cells [path_base, path_base + N))
@@ -658,22 +446,6 @@ __global__ __launch_bounds__(kBlock) void k_cam_gen(Wave w, PathState ps, RayBuf
         queue[k] = g;
     }
 }
-
-struct GridView {
-    const uint32_t* box;        // 8 per slice
-    const uint32_t* cell_ends;  // inclusive ends over all cells of the wavefront
-    const LightVertex* sverts;  // vertices in grid order (cell ranges are contiguous)
-    const float4* spos;         // their positions
-    const LightVertex* cverts;  // vertices in path order
-    const uint32_t* path_end;   // inclusive end of each light path's vertices
-    const uint32_t* nv_ptr;
-};
-
-struct CamSlots {
-    float* c;            // nslots x 3 per path: NEE, VC 0..mlv-1, VM
-    uint32_t* valid;     // nslots per path: 0 none, 1 unconditional, 2 pending shadow ray, 3 occluded
-    uint32_t* ray_slot;  // per queued shadow ray: path * nslots + slot
-};
 
 __global__ __launch_bounds__(kBlock) void k_cam_shade(Wave w, const uint32_t* __restrict count_ptr, const uint32_t* __restrict queue,
                                                       RayBuf rays, HitBuf hits, PathState ps, GridView G, CamSlots S,
@@ -928,153 +700,6 @@ __global__ __launch_bounds__(kBlock) void k_cam_shade(Wave w, const uint32_t* __
 // and the group adds its eight partial sums with a fixed butterfly ((c0+c1)+(c2+c3))+((c4+c5)+
 // (c6+c7)). A cell's vertices are contiguous (grid order); positions are tested against the radius
 // before the 64-byte vertex is read. MergeVertices (VCM.cpp:362-409) per vertex found.
-// Vertex merging (SearchHashGrid + MergeVertices, VCMHashGrid.cpp:107-157, VCM.cpp:362-409), one
-// query per half-wave (32 lanes), the queries of a bounce taken grid-stride. The lanes of a half
-// scan the photons of their query's 2x2x2 cells as one list (the cells' ranges concatenated in the
-// reference's visiting order, strided over the lanes), append those within the radius to the half's
-// LDS list (ballot and prefix count, list order = scan order) and evaluate the camera BSDF for up to
-// 32 listed photons at a time, one per lane, both halves in the same pass: an evaluation pass then
-// serves two queries, and the acceptance test does not make whole waves wait on every BSDF
-// evaluation. Each lane sums its photons in list order; a fixed butterfly over the half's lanes
-// gives the query's sum.
-constexpr uint32_t kMergeWaves = kBlock / 64;
-
-__global__ __launch_bounds__(kBlock) void k_cam_merge(Wave w, const uint32_t* __restrict count_ptr,
-                                                      const CamVertex* __restrict cam_vm,
-                                                      const uint32_t* __restrict path_slice, GridView G, CamSlots S)
-{
-    const uint32_t count = *count_ptr;
-    __shared__ uint32_t list_lds[kMergeWaves][2][64];
-    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
-    const uint32_t half = lane >> 5, hl = lane & 31u;  // half-wave and lane within it
-    uint32_t* list = list_lds[wid][half];
-    const unsigned long long below = (1ull << lane) - 1ull;
-    const unsigned long long half_bits = half ? 0xFFFFFFFF00000000ull : 0x00000000FFFFFFFFull;
-    if (*G.nv_ptr == 0) return;
-    const uint32_t stride = 2 * gridDim.x * kMergeWaves;
-    for (uint32_t j0 = 2 * (blockIdx.x * kMergeWaves + wid); j0 < count; j0 += stride) {
-        // ---- this half's query (none past the end, or without a query, or outside the photon box) ----
-        const uint32_t j = j0 + half;
-        CamVertex v;
-        v.geom_plen = kNoQuery;
-        if (j < count) v = cam_vm[j];
-        bool active = v.geom_plen != kNoQuery;
-        uint32_t si = 0;
-        float dmin[3] = {0.0f, 0.0f, 0.0f};
-        if (active) {
-            si = path_slice[v.path];
-            const uint32_t* box = G.box + 8 * si;
-            for (int k = 0; k < 3; ++k) {
-                dmin[k] = v.pos[k] - ord2f(box[k]);
-                const float dmax = ord2f(~box[3 + k]) - v.pos[k];
-                if (dmin[k] < 0.0f || dmax < 0.0f) active = false;
-            }
-        }
-        if (__ballot(active) == 0) continue;
-        const Slice& sl = w.slices[si];
-        // lanes 0..7 of the half: cell k = (x, y, z) bits of k, its photon range, offset in the scan list
-        uint32_t cb = 0, cn = 0;
-        if (active && hl < 8) {
-            int32_t mn[3];
-            for (int k = 0; k < 3; ++k) {
-                float cp = sl.inv_cell * dmin[k];
-                float cf = floorf(cp);
-                float fr = cp - cf;
-                mn[k] = (int32_t)cf;
-                if (fr < 0.5f) --mn[k];
-            }
-            const uint32_t cell = sl.path_base + cell_hash(mn[0] + (int32_t)(hl & 1u), mn[1] + (int32_t)((hl >> 1) & 1u),
-                                                           mn[2] + (int32_t)(hl >> 2), sl.N);
-            cb = cell == 0 ? 0 : G.cell_ends[cell - 1];
-            cn = G.cell_ends[cell] - cb;
-        }
-        uint32_t incl = cn;
-        for (uint32_t d = 1; d < 8; d <<= 1) {
-            const uint32_t y = (uint32_t)__shfl_up((int)incl, d, 32);
-            if (hl >= d) incl += y;
-        }
-        const uint32_t excl = incl - cn;
-        uint32_t ends[8];
-        for (int k = 0; k < 8; ++k) ends[k] = (uint32_t)__shfl((int)incl, k, 32);
-        const uint32_t total = ends[7];  // 0 for a half without a query
-
-        const f3 x = mk3(v.pos[0], v.pos[1], v.pos[2]);
-        const f3 view = mk3(v.view[0], v.view[1], v.view[2]);
-        const uint32_t plen = v.geom_plen & 31u;
-        Surface s;
-        s.position = x;
-        s.error = 0.0f;
-        float ccp = 0.0f;
-        if (active) {
-            surface_material(sl.materials + 22 * (size_t)sl.geom_material[v.geom_plen >> 5],
-                             mk3(v.n[0], v.n[1], v.n[2]), view, nullptr, s);
-            s.base_color = mk3(v.base[0], v.base[1], v.base[2]);
-            ccp = cont_prob(s);
-        }
-        f3 acc = mk3(0, 0, 0);
-        auto merge = [&](uint32_t q) {
-            const LightVertex& lv = G.sverts[q];
-            const uint32_t lgp = lv.geom_plen;
-            if (plen + (lgp & 31u) > sl.L) return;
-            const float* lmat = sl.materials + 22 * (size_t)sl.geom_material[lgp >> 5];
-            // the reference's transparency rule (VCM.cpp:381-383)
-            if ((s.flags & kMatTransparent) != ((uint32_t)lmat[21] & kMatTransparent)) return;
-            f3 lw = mk3(lv.wfix[0], lv.wfix[1], lv.wfix[2]);
-            float fwd, rev;
-            f3 f = bsdf_eval(s, view, lw, fwd, rev);
-            if (is_zero3(f)) return;
-            float cos_l = absf_(dot3(s.n, lw));
-            fwd *= ccp;
-            rev *= lv.cp;
-            float w_light = lv.dvcm * sl.vc_weight + lv.dvm * fwd;
-            float w_cam = v.dvcm * sl.vc_weight + v.dvm * rev;
-            float mis = 1.0f / (w_light + 1.0f + w_cam);
-            acc += (f * mk3(lv.thr[0], lv.thr[1], lv.thr[2])) * (mis / cos_l);
-        };
-        uint32_t listed = 0;  // same in every lane of the half
-        for (uint32_t c0 = 0; __ballot(c0 < total) != 0; c0 += 32) {
-            const uint32_t i = c0 + hl;
-            // every lane takes part in the shuffles (lanes past the end look at their last photon)
-            const uint32_t ic = i < total ? i : (total > 0 ? total - 1 : 0);
-            uint32_t k = 0;
-            for (int m = 0; m < 7; ++m) k += ic >= ends[m] ? 1u : 0u;
-            const uint32_t q = (uint32_t)__shfl((int)cb, (int)k, 32) + (ic - (uint32_t)__shfl((int)excl, (int)k, 32));
-            bool near = false;
-            if (i < total) {
-                const float4 lp = G.spos[q];
-                const f3 dv = x - mk3(lp.x, lp.y, lp.z);
-                near = dot3(dv, dv) <= sl.radius_sq;
-            }
-            const unsigned long long m = __ballot(near) & half_bits;
-            if (near) list[listed + (uint32_t)__popcll(m & below)] = q;
-            listed += (uint32_t)__popcll(m);
-            __builtin_amdgcn_wave_barrier();
-            if (__ballot(listed >= 32) != 0) {
-                // both halves evaluate up to 32 listed photons and keep the rest
-                const uint32_t mine = hl < listed ? list[hl] : 0u;
-                const uint32_t rest = hl + 32 < listed ? list[hl + 32] : 0u;
-                __builtin_amdgcn_wave_barrier();
-                if (hl + 32 < listed) list[hl] = rest;
-                __builtin_amdgcn_wave_barrier();
-                if (hl < listed) merge(mine);
-                listed = listed > 32 ? listed - 32 : 0;
-            }
-        }
-        if (hl < listed) merge(list[hl]);
-        __builtin_amdgcn_wave_barrier();
-        for (int off = 1; off < 32; off <<= 1) {
-            acc.x += __shfl_xor(acc.x, off, 32);
-            acc.y += __shfl_xor(acc.y, off, 32);
-            acc.z += __shfl_xor(acc.z, off, 32);
-        }
-        if (active && hl == 0) {
-            const uint32_t p = v.path;
-            s3(S.c, p * w.nslots + w.nslots - 1, (mk3(v.thr[0], v.thr[1], v.thr[2]) * acc) * sl.vm_norm);
-            S.valid[(size_t)p * w.nslots + w.nslots - 1] = 1;
-        }
-    }
-}
-
 __global__ __launch_bounds__(kBlock) void k_cam_resolve(const uint32_t* __restrict count_ptr, const uint32_t* __restrict occluded, CamSlots S)
 {
     const uint32_t count = *count_ptr;
@@ -1778,8 +1403,7 @@ int sgpu_vcm_server_step_impl(sgpu_vcm_server sv, hipStream_t stream, uint64_t* 
             if (pass == 1) {
                 SGPU_HIP_TRY(T.begin(kPhaseMerge));
                 const uint32_t merge_blocks = std::min<uint32_t>((n + 2 * kMergeWaves - 1) / (2 * kMergeWaves), 16u * cus);
-                hipLaunchKernelGGL(k_cam_merge, dim3(merge_blocks), dim3(kBlock), 0, st, wv, c_in, sv->cam_vm, ps.slice,
-                                   G, S);
+                launch_cam_merge(merge_blocks, st, wv, c_in, sv->cam_vm, ps.slice, G, S);
                 SGPU_HIP_TRY(hipGetLastError());
                 SGPU_HIP_TRY(T.end());
             }

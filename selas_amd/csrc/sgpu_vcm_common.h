@@ -1,0 +1,252 @@
+// Types and device helpers shared by the VCM translation units (sgpu_vcm.hip: the passes, the
+// photon grid and the server; sgpu_vcm_merge.hip: vertex merging, built with its own division
+// and square-root settings). Everything here is the device-side layout of one wavefront.
+#pragma once
+
+#include "sgpu_camera.h"
+#include "sgpu_internal.h"
+#include "sgpu_light.h"
+#include "sgpu_ptex.h"
+#include "sgpu_surface.h"
+
+namespace sgpu_vcm {
+
+using namespace sgpu;
+
+
+constexpr int kBlock = 256;
+constexpr float kFloatMax = 3.402823466e+38F;
+constexpr float kRayEps = 0.00001f;  // RayPick's tnear in the VCM code (VCM.cpp:137)
+constexpr uint32_t kNoQuery = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter)
+{
+    const uint64_t mask = __ballot(pred);
+    if (mask == 0) return 0xFFFFFFFFu;
+    const uint32_t lane = __lane_id();
+    const uint32_t below = __popcll(mask & ((1ull << lane) - 1ull));
+    const uint32_t leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader, 64);
+    return pred ? base + below : 0xFFFFFFFFu;
+}
+
+struct RayBuf {
+    float *ox, *oy, *oz, *dx, *dy, *dz, *tnear, *tfar;
+    uint32_t* scene;  // per ray, the scene it traverses (multi-scene wavefronts only, else nullptr)
+};
+struct HitBuf {
+    float *t, *u, *v, *ngx, *ngy, *ngz;
+    uint32_t *prim, *geom;
+};
+
+__device__ __forceinline__ void put_ray(const RayBuf& r, uint32_t i, f3 o, f3 d, float tnear, float tfar,
+                                        uint32_t scene)
+{
+    if (r.scene) r.scene[i] = scene;
+    r.ox[i] = o.x;
+    r.oy[i] = o.y;
+    r.oz[i] = o.z;
+    r.dx[i] = d.x;
+    r.dy[i] = d.y;
+    r.dz[i] = d.z;
+    r.tnear[i] = tnear;
+    r.tfar[i] = tfar;
+}
+
+// Light vertex, 80 B (VCMVertex, VCMCommon.h:36-46, with the surface data a connection needs in
+// place of the hit record: position, normal, incoming direction, material, and the base colour
+// CalculateSurfaceParams derived -- the filtered texel for textured materials).
+struct LightVertex {
+    float pos[3];
+    float thr[3];
+    float n[3];
+    float wfix[3];  // direction toward the previous vertex of the light subpath
+    float dvcm, dvc, dvm;
+    uint32_t geom_plen;  // geometry << 5 | path length (path length < 32, as in PathState)
+    float base[3];       // base colour (^2.2 applied)
+    float cp;            // Russian-roulette continuation probability of the surface
+};
+static_assert(sizeof(LightVertex) == 80, "LightVertex must be 80 bytes");
+
+// Camera vertex handed from k_cam_shade to k_cam_merge (80 B); geom_plen = kNoQuery: no query.
+struct CamVertex {
+    float pos[3];
+    float n[3];
+    float view[3];
+    float thr[3];
+    float dvcm, dvm;
+    uint32_t geom_plen;  // geometry << 5 | path length
+    uint32_t path;
+    float base[3];       // base colour (^2.2 applied)
+    float pad;
+};
+static_assert(sizeof(CamVertex) == 80, "CamVertex must be 80 bytes");
+
+// Subpath state (PathState, VCMCommon.h:48-60), SoA over the paths of a wavefront.
+struct PathState {
+    float* pos;  // 3
+    float* dir;  // 3
+    float* thr;  // 3
+    float* dvcm;
+    float* dvc;
+    float* dvm;
+    uint32_t* plen;
+    uint32_t* slice;
+    uint64_t* rng;
+    float* color;  // 3, camera paths only
+};
+
+// One VCM iteration of one job (see the file comment).
+struct Slice {
+    CameraParams cam;
+    float vipd;  // W / (2 tan(hfov / 2))
+    uint32_t W, H, N, L, mlv;
+    uint32_t path_base;  // first wavefront path of the slice; also its first grid cell
+    uint32_t iteration;  // global iteration index (radius schedule and random streams)
+    uint32_t job;        // index of the job whose image the slice adds to
+    uint32_t scene;      // index of the job's scene in the wavefront's traversal table
+    uint64_t seed;
+    float vm_weight, vc_weight, vm_norm, radius_sq, inv_cell;
+    const float* materials;  // shading records (sgpu::shading_record)
+    const uint32_t* geom_material;
+    uint32_t n_geoms;
+    const Light* lights;
+    uint32_t n_lights;
+    f3 background;
+    const PtexTex* geom_tex;  // per-face texture per geometry (texels == nullptr: none)
+    uint32_t n_geom_tex;
+};
+
+// Wavefront-wide arguments.
+struct Wave {
+    const Slice* slices;
+    const uint32_t* dims;  // device: [0] slices, [1] paths in the wavefront (sum of the slices' N)
+    uint32_t mlv;          // vertex slots per path (largest slice mlv)
+    uint32_t nslots;       // camera contribution slots per path (mlv + 2)
+};
+
+__device__ __forceinline__ f3 g3(const float* a, uint32_t i) { return mk3(a[3 * i], a[3 * i + 1], a[3 * i + 2]); }
+__device__ __forceinline__ void s3(float* a, uint32_t i, f3 v)
+{
+    a[3 * i] = v.x;
+    a[3 * i + 1] = v.y;
+    a[3 * i + 2] = v.z;
+}
+
+__device__ __forceinline__ uint64_t stream_id(uint32_t iteration, uint32_t path, uint32_t pass)
+{
+    return ((uint64_t)iteration << 33) | ((uint64_t)path << 1) | pass;
+}
+
+__device__ __forceinline__ Pcg32 rng_init(uint64_t seed, uint64_t sid)
+{
+    return pcg32_seed(seed ^ (sid * 0x9E3779B97F4A7C15ull), sid);
+}
+
+__device__ __forceinline__ Pcg32 rng_load(const PathState& ps, uint32_t g, const Slice& sl, uint32_t pass)
+{
+    Pcg32 r;
+    r.state = ps.rng[g];
+    r.inc = (stream_id(sl.iteration, g - sl.path_base, pass) << 1u) | 1u;
+    return r;
+}
+
+// slice of wavefront path g (slices are sorted by path_base)
+__device__ __forceinline__ uint32_t find_slice(const Wave& w, uint32_t g)
+{
+    uint32_t lo = 0, hi = w.dims[0] - 1;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) >> 1;
+        if (w.slices[mid].path_base <= g)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ float cont_prob(const Surface& s) { return saturate_(max_(max_(s.base_color.x, s.base_color.y), s.base_color.z)); }
+
+// Surface of a stored light vertex (the material transforms of CalculateSurfaceParams).
+__device__ __forceinline__ void vertex_surface(const Slice& sl, const LightVertex& v, Surface& s)
+{
+    const float* mat = sl.materials + 22 * (size_t)sl.geom_material[v.geom_plen >> 5];
+    s.position = mk3(v.pos[0], v.pos[1], v.pos[2]);
+    s.error = 0.0f;
+    surface_material(mat, mk3(v.n[0], v.n[1], v.n[2]), mk3(v.wfix[0], v.wfix[1], v.wfix[2]), nullptr, s);
+    s.base_color = mk3(v.base[0], v.base[1], v.base[2]);
+}
+
+// CalculateSurfaceParams' eUsesPtex branch (SurfaceParameters.cpp:196-203): the filtered texel of
+// the hit's face at its (u, v) when the material asks for it and its geometry has a texture.
+__device__ __forceinline__ bool hit_texel(const Slice& sl, uint32_t geom, const float* mat, uint32_t prim, float u,
+                                          float v, f3& texel)
+{
+    if (!((uint32_t)mat[21] & kMatUsesPtex) || geom >= sl.n_geom_tex) return false;
+    const PtexTex tx = sl.geom_tex[geom];
+    if (!tx.texels) return false;
+    texel = ptex_eval(tx, (int)prim, u, v);
+    return true;
+}
+
+// SampleBsdfScattering (VCM.cpp:412-446) with the Russian-roulette-compensated throughput.
+template <class U>
+__device__ __forceinline__ bool scatter(U& u, const Slice& sl, const Surface& s, f3& dir, f3& thr, float& dvcm,
+                                        float& dvc, float& dvm, uint32_t& plen)
+{
+    BsdfSample bs;
+    if (!bsdf_sample(u, s, -dir, bs)) return false;
+    if (is_zero3(bs.reflectance)) return false;
+    float cp = cont_prob(s);
+    if (u.next() > cp) return false;
+    float fwd = bs.fwd_pdf * cp, rev = bs.rev_pdf * cp;
+    float cos_b = absf_(dot3(bs.wi, s.n));
+    thr = (thr * bs.reflectance) * (1.0f / cp);
+    float k = cos_b / fwd;
+    float nvc = k * (dvc * rev + dvcm + sl.vm_weight);
+    float nvm = k * (dvm * rev + dvcm * sl.vc_weight + 1.0f);
+    dvc = nvc;
+    dvm = nvm;
+    dvcm = 1.0f / fwd;
+    dir = bs.wi;
+    ++plen;
+    return true;
+}
+
+__device__ __forceinline__ uint32_t f2ord(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(uint32_t u) { return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u); }
+
+__device__ __forceinline__ uint32_t cell_hash(int32_t x, int32_t y, int32_t z, uint32_t cell_count)
+{
+    uint32_t ux = (uint32_t)x, uy = (uint32_t)y, uz = (uint32_t)z;
+    return ((ux * 73856093u) ^ (uy * 19349663u) ^ (uz * 83492791u)) % cell_count;
+}
+
+struct GridView {
+    const uint32_t* box;        // 8 per slice
+    const uint32_t* cell_ends;  // inclusive ends over all cells of the wavefront
+    const LightVertex* sverts;  // vertices in grid order (cell ranges are contiguous)
+    const float4* spos;         // their positions
+    const LightVertex* cverts;  // vertices in path order
+    const uint32_t* path_end;   // inclusive end of each light path's vertices
+    const uint32_t* nv_ptr;
+};
+
+struct CamSlots {
+    float* c;            // nslots x 3 per path: NEE, VC 0..mlv-1, VM
+    uint32_t* valid;     // nslots per path: 0 none, 1 unconditional, 2 pending shadow ray, 3 occluded
+    uint32_t* ray_slot;  // per queued shadow ray: path * nslots + slot
+};
+
+// Vertex merging of the camera vertices of one bounce (sgpu_vcm_merge.hip).
+constexpr uint32_t kMergeWaves = kBlock / 64;
+void launch_cam_merge(uint32_t blocks, hipStream_t stream, const Wave& w, const uint32_t* count_ptr,
+                      const CamVertex* cam_vm, const uint32_t* path_slice, const GridView& G, const CamSlots& S);
+
+}  // namespace sgpu_vcm
