@@ -122,7 +122,7 @@ __global__ __launch_bounds__(kBlock) void k_light_gen(Wave w, PathState ps, RayB
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_light_shade(Wave w, const uint32_t* __restrict count_ptr, const uint32_t* __restrict queue,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_light_shade(Wave w, const uint32_t* __restrict count_ptr, const uint32_t* __restrict queue,
                                                         RayBuf rays, HitBuf hits, PathState ps, LightVertex* __restrict verts,
                                                         uint32_t* __restrict vcount, RayBuf next_rays,
                                                         uint32_t* __restrict next_queue, RayBuf shadow,
@@ -447,7 +447,7 @@ __global__ __launch_bounds__(kBlock) void k_cam_gen(Wave w, PathState ps, RayBuf
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_cam_shade(Wave w, const uint32_t* __restrict count_ptr, const uint32_t* __restrict queue,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_cam_shade(Wave w, const uint32_t* __restrict count_ptr, const uint32_t* __restrict queue,
                                                       RayBuf rays, HitBuf hits, PathState ps, GridView G, CamSlots S,
                                                       CamVertex* __restrict cam_vm, RayBuf next_rays,
                                                       uint32_t* __restrict next_queue, RayBuf shadow,
