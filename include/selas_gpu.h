@@ -239,6 +239,19 @@ int sgpu_scene_set_materials(sgpu_scene scene, const float* materials, uint32_t 
  * background radiance (SceneResourceData::backgroundIntensity, SceneResource.h:53). */
 int sgpu_scene_set_lights(sgpu_scene scene, const sgpu_light* lights, uint32_t n_lights, const float* background_rgb);
 
+/* Image-based light of a scene: replaces ImageBasedLightResource (ImageBasedLightResource.h:20-40)
+ * with the sampling tables BuildImageBasedLight.cpp:18-103 computes (CDFs over sin(theta) x CIE
+ * luma), built here from the texels. rgb: height x width x 3 floats (host array, copied), an
+ * equirectangular map whose row 0 is at +y; every value finite and >= 0; width, height in
+ * [1, 16384]. rotation_radians is added to phi (normalized to [0, 2 pi)); exposure scales the
+ * radiance (exposureScale). rgb == NULL removes the light. Emission starts on the disc of the
+ * sphere around the triangles' bounding box, so the scene must be committed first. VCM picks it
+ * like one more light (emission, NEE, and camera rays that leave the scene); the path tracer adds
+ * it where camera paths leave the scene (EvaluateBackground, AreaLighting.cpp:415). It replaces
+ * the constant background. */
+int sgpu_scene_set_ibl(sgpu_scene scene, uint32_t width, uint32_t height, const float* rgb, float rotation_radians,
+                       float exposure);
+
 /* Per-face texture of a quad geometry: replaces TextureCache::FetchPtex + PtexTexture::open
  * (TextureCache.cpp:168, SurfaceParameters.cpp:197) for 8-bit RGB Ptex data with one resolution
  * for every face. texels: n_faces x (1 << res_log2)^2 x 3 bytes (face-major, rows along v);
@@ -320,7 +333,9 @@ typedef struct sgpu_vcm_stats {
 
 /* Vertex connection and merging: replaces VCM::GenerateImage and its helpers
  * (Source/Applications/Selas/Source/VCM.h:18, VCM.cpp:669-760, VCMCommon.cpp:37-113,
- * VCMHashGrid.cpp:15-157) for quad and point lights (sgpu_scene_set_lights). framebuffer is a
+ * VCMHashGrid.cpp:15-157) for quad and point lights (sgpu_scene_set_lights) and the image-based
+ * light (sgpu_scene_set_ibl; EmitIblLightSample, DirectIblLightSample, ConnectToSkyLight,
+ * AreaLighting.cpp:246-331, VCM.cpp:171-186), the lights picked uniformly. framebuffer is a
  * device array of height x width x 3 floats, the radiance averaged over the iterations. The merge
  * radius of global iteration k is base_radius / (k + 1)^((1 - radius_alpha) / 2). stats may be
  * NULL. Synchronous on `stream`. */

@@ -234,7 +234,86 @@ class _Scene(C.Structure):
     _fields_ = [("bvh", C.c_void_p), ("tri9", C.c_void_p), ("tri_geom", C.c_void_p), ("materials", C.c_void_p),
                 ("geom_material", C.c_void_p), ("n_geoms", C.c_uint32), ("lights", C.c_void_p),
                 ("n_lights", C.c_uint32), ("background", C.c_float * 3), ("geom_first", C.c_void_p),
-                ("geom_tex", C.c_void_p)]
+                ("geom_tex", C.c_void_p), ("ibl", C.c_void_p)]
+
+
+# ---- image-based light (orc_ibl.c) ---------------------------------------------------------------
+
+class _Ibl(C.Structure):
+    _fields_ = [("w", C.c_uint32), ("h", C.c_uint32), ("rgb", C.c_void_p), ("mdf", C.c_void_p), ("cdf", C.c_void_p),
+                ("rotation", C.c_float), ("exposure", C.c_float), ("center", C.c_float * 3), ("radius", C.c_float)]
+
+
+lib.orc_ibl_tables.argtypes = [C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]
+lib.orc_ibl_rotation.argtypes = [C.c_float]
+lib.orc_ibl_rotation.restype = C.c_float
+lib.orc_bounding_sphere.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
+lib.orc_ibl_sample_distribution.argtypes = [C.c_void_p, C.c_uint32, C.c_float, C.c_void_p]
+lib.orc_ibl_sample_distribution.restype = C.c_uint32
+lib.orc_ibl_sample.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_void_p]
+lib.orc_ibl_lookup.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+lib.orc_ibl_lookup.restype = C.c_float
+lib.orc_ibl_emit.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float, C.c_void_p]
+lib.orc_ibl_direct.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_void_p]
+lib.orc_ibl_position_pdf.argtypes = [C.c_void_p]
+lib.orc_ibl_position_pdf.restype = C.c_float
+
+
+def ibl_tables(rgb):
+    """(marginal CDF (h), conditional CDFs (h x w)) of an h x w x 3 map (BuildImageBasedLight.cpp)."""
+    a = _f(rgb)
+    h, w = a.shape[0], a.shape[1]
+    mdf = np.zeros(h, np.float32)
+    cdf = np.zeros((h, w), np.float32)
+    lib.orc_ibl_tables(w, h, a.ctypes.data, mdf.ctypes.data, cdf.ctypes.data)
+    return mdf, cdf
+
+
+def bounding_sphere(tri9):
+    t = _f(tri9)
+    out = np.zeros(4, np.float32)
+    lib.orc_bounding_sphere(t.ctypes.data, t.size // 9, out.ctypes.data)
+    return out
+
+
+class Ibl:
+    """The oracle's image-based light: tables, rotation and the emission sphere of a triangle set."""
+
+    def __init__(self, rgb, rotation=0.0, exposure=1.0, tri9=None, sphere=None):
+        self.rgb = _f(rgb)
+        self.mdf, self.cdf = ibl_tables(self.rgb)
+        sph = bounding_sphere(tri9) if tri9 is not None else np.asarray(sphere, np.float32)
+        self.s = _Ibl(self.rgb.shape[1], self.rgb.shape[0], self.rgb.ctypes.data, self.mdf.ctypes.data,
+                      self.cdf.ctypes.data, lib.orc_ibl_rotation(rotation), exposure,
+                      (C.c_float * 3)(*[float(x) for x in sph[:3]]), float(sph[3]))
+
+    def sample(self, r0, r1):
+        """Ibl: (theta, phi, x, y, pdf)"""
+        out = np.zeros(6, np.float32)
+        lib.orc_ibl_sample(C.byref(self.s), r0, r1, out.ctypes.data)
+        return out[:5]
+
+    def lookup(self, wi):
+        """SampleIbl(wi): (radiance, pdf)"""
+        w = _f(wi)
+        rgb = np.zeros(3, np.float32)
+        pdf = lib.orc_ibl_lookup(C.byref(self.s), w.ctypes.data, rgb.ctypes.data)
+        return rgb, np.float32(pdf)
+
+    def emit(self, r0, r1, r2, r3):
+        """EmitIblLightSample: (position, direction, radiance, emission pdf, direction pdf)"""
+        out = np.zeros(14, np.float32)
+        lib.orc_ibl_emit(C.byref(self.s), r0, r1, r2, r3, out.ctypes.data)
+        return out[0:3], out[3:6], out[6:9], out[9], out[10]
+
+    def direct(self, r0, r1):
+        """DirectIblLightSample: (direction, radiance, pdf)"""
+        out = np.zeros(8, np.float32)
+        lib.orc_ibl_direct(C.byref(self.s), r0, r1, out.ctypes.data)
+        return out[0:3], out[3:6], out[6]
+
+    def position_pdf(self):
+        return lib.orc_ibl_position_pdf(C.byref(self.s))
 
 
 class _PtJob(C.Structure):
@@ -268,11 +347,15 @@ class RenderScene:
                 self.textures.append(t)
                 self.geom_tex[i] = t.s
         self.bvh = CpuBvh(self.tri9)
+        ibl = sc.get("ibl")
+        self.ibl = None if ibl is None else Ibl(ibl["rgb"], ibl.get("rotation", 0.0), ibl.get("exposure", 1.0),
+                                                tri9=self.tri9)
         self.s = _Scene(self.bvh.handle, self.tri9.ctypes.data, self.tri_geom.ctypes.data, self.materials.ctypes.data,
                         self.geom_material.ctypes.data, self.geom_material.shape[0],
                         self.lights.ctypes.data if self.lights.size else None, self.lights.shape[0],
                         (C.c_float * 3)(*[float(x) for x in sc["background"]]), self.geom_first.ctypes.data,
-                        C.cast(self.geom_tex, C.c_void_p))
+                        C.cast(self.geom_tex, C.c_void_p),
+                        C.cast(C.pointer(self.ibl.s), C.c_void_p) if self.ibl is not None else None)
 
 
 def render_pt(rs: RenderScene, cam: dict, width, height, spp, max_path_length, seed, flags=0, threads=0):

@@ -122,6 +122,31 @@ typedef struct {
     const int32_t* adjedges; /* 4 per face: the neighbour's edge id of the shared edge */
 } orc_ptex;
 
+/* ---- image-based light (orc_ibl.c) -------------------------------------------------------------- */
+typedef struct {
+    uint32_t w, h;
+    const float* rgb;  /* h x w x 3, row 0 at +y */
+    const float* mdf;  /* h: marginal CDF */
+    const float* cdf;  /* h x w: conditional CDFs */
+    float rotation;    /* radians in [0, 2 pi) (orc_ibl_rotation) */
+    float exposure;
+    float center[3], radius; /* sphere around the triangles' bounding box (orc_bounding_sphere) */
+} orc_ibl;
+
+void orc_ibl_tables(uint32_t w, uint32_t h, const float* rgb, float* mdf, float* cdf);
+float orc_ibl_rotation(float radians);
+void orc_bounding_sphere(const float* tri9, uint32_t n_tris, float* out4);
+uint32_t orc_ibl_sample_distribution(const float* d, uint32_t count, float r, float* pdf);
+/* out6: theta, phi, x, y, pdf, 0 */
+void orc_ibl_sample(const orc_ibl* b, float r0, float r1, float* out6);
+/* radiance (rgb_out) from unit direction wi; returns the pdf of sampling wi */
+float orc_ibl_lookup(const orc_ibl* b, const float* wi, float* rgb_out);
+/* out14: position 3, direction 3, radiance 3, emission pdf (area x solid angle), direction pdf, 0 x 3 */
+void orc_ibl_emit(const orc_ibl* b, float r0, float r1, float r2, float r3, float* out14);
+/* out8: direction 3, radiance 3, pdf, 0 */
+void orc_ibl_direct(const orc_ibl* b, float r0, float r1, float* out8);
+float orc_ibl_position_pdf(const orc_ibl* b);
+
 typedef struct {
     const orc_bvh* bvh;
     const float* tri9;          /* world-space triangles, 9 floats each (global index order) */
@@ -135,6 +160,7 @@ typedef struct {
     const uint32_t* geom_first; /* first triangle of each geometry; NULL when no textures */
     const orc_ptex* geom_tex;   /* per geometry (nfaces == 0: none); a textured geometry is a quad
                                    mesh whose triangles 2q and 2q+1 are Embree's split of quad q */
+    const orc_ibl* ibl;         /* image-based light, NULL for none (replaces the background) */
 } orc_scene;
 
 typedef struct {

@@ -20,6 +20,10 @@ extern int orc_libm_single;
 static inline float cos_f(float x) { return orc_libm_single ? cosf(x) : (float)cos((double)x); }
 static inline float sin_f(float x) { return orc_libm_single ? sinf(x) : (float)sin((double)x); }
 static inline float acos_f(float x) { return orc_libm_single ? acosf(x) : (float)acos((double)x); }
+static inline float atan2_f(float y, float x)
+{
+    return orc_libm_single ? atan2f(y, x) : (float)atan2((double)y, (double)x);
+}
 static inline float pow_f(float x, float y) { return orc_libm_single ? powf(x, y) : (float)pow((double)x, (double)y); }
 /* x^5 (Schlick's weight): powf in mode 1, else double multiplications (the device's way) */
 static inline float pow5_f(float x)

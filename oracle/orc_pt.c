@@ -214,7 +214,14 @@ static void evaluate_path(const pt_ctx* c, uint32_t* stack, uint32_t x, uint32_t
         float t, hu, hv;
         uint32_t prim = orc_bvh_closest1(sc->bvh, stack, o, d, 0.0f, FLOAT_MAX, &t, &hu, &hv);
         if (prim == 0xFFFFFFFFu) {
-            l1 = add(l1, mul(ld3(sc->background), thr));
+            /* EvaluateBackground (AreaLighting.cpp:415-424): the image-based light, else the
+             * constant background */
+            float bg[3];
+            if (sc->ibl)
+                (void)orc_ibl_lookup(sc->ibl, d, bg);
+            else
+                memcpy(bg, sc->background, sizeof(bg));
+            l1 = add(l1, mul(ld3(bg), thr));
             break;
         }
         uint32_t geom = sc->tri_geom[prim];

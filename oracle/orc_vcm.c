@@ -216,66 +216,105 @@ static int world_to_image(const orc_camera* c, ov3 p, float* px, float* py)
 typedef struct {
     ov3 position, direction, throughput;
     float dvcm, dvc, dvm;
+    int infinite; /* the subpath starts at the image-based light */
 } emit_state;
 
-/* Light subpath start (GenerateLightSample, VCMCommon.cpp:37-63, for quad and point lights).
- * Uses 5 uniforms: light pick, position (2, quad only), direction (2). */
-static void emit_light(orc_uniforms* u, const orc_light* lights, uint32_t n_lights, float vc_weight, emit_state* s)
+/* Light subpath start (GenerateLightSample, VCMCommon.cpp:37-63): one light picked uniformly among
+ * the quad and point lights and the image-based light (light n_lights). Uses 5 uniforms for quad
+ * and point lights (pick, position 2 -- quad only --, direction 2), 5 for the image-based light
+ * (pick, then EmitIblLightSample's 4, AreaLighting.cpp:246-290). Returns 0 when the sample carries
+ * no light (a zero emission pdf). */
+static int emit_light(orc_uniforms* u, const orc_scene* sc, float vc_weight, emit_state* s)
 {
-    float p_pick = 1.0f / n_lights;
-    uint32_t idx = (uint32_t)(orc_uniform(u) * n_lights);
-    if (idx >= n_lights) idx = n_lights - 1;
-    const orc_light* L = &lights[idx];
+    const uint32_t n_lights = sc->n_lights, n_pick = n_lights + (sc->ibl ? 1u : 0u);
+    float p_pick = 1.0f / n_pick;
+    uint32_t idx = (uint32_t)(orc_uniform(u) * n_pick);
+    if (idx >= n_pick) idx = n_pick - 1;
     float emission_pdf_w, direct_pdf_a;
     ov3 radiance;
-    if (L->type == 1) {
-        float r0 = orc_uniform(u), r1 = orc_uniform(u);
-        /* uniform sphere (CSampler::UniformSphere, Sampler.cpp) */
-        float uu = 2.0f * r1 - 1.0f;
-        float norm = sqrtf(maxf(0.0f, 1.0f - uu * uu));
-        float theta = (kPi * 2.0f) * r0;
-        s->direction = v3(norm * cos_f(theta), uu, norm * sin_f(theta));
-        s->position = ld3(L->position);
-        emission_pdf_w = (1.0f / (4.0f * kPi)) * p_pick;
-        direct_pdf_a = 1.0f * p_pick;
-        radiance = ld3(L->radiance);
+    s->infinite = idx == n_lights;
+    if (s->infinite) {
+        float r0 = orc_uniform(u), r1 = orc_uniform(u), r2 = orc_uniform(u), r3 = orc_uniform(u);
+        float e[14];
+        orc_ibl_emit(sc->ibl, r0, r1, r2, r3, e);
+        s->position = v3(e[0], e[1], e[2]);
+        s->direction = v3(e[3], e[4], e[5]);
+        radiance = v3(e[6], e[7], e[8]);
+        emission_pdf_w = e[9] * p_pick;
+        direct_pdf_a = e[10] * p_pick;
     } else {
-        ov3 ex = ld3(L->x), ez = ld3(L->z);
-        float a = orc_uniform(u), b = orc_uniform(u);
-        s->position = add(add(sub(sub(ld3(L->position), scl(ex, 0.5f)), scl(ez, 0.5f)), scl(ex, a)), scl(ez, b));
-        float r0 = orc_uniform(u), r1 = orc_uniform(u);
-        float r = sqrtf(r0);
-        float th = (kPi * 2.0f) * r1;
-        ov3 local = v3(r * cos_f(th), sqrtf(maxf(0.0f, 1 - r0)), r * sin_f(th));
-        float t[3], bb[3];
-        ov3 n = ld3(L->direction);
-        float nn[3] = {n.x, n.y, n.z};
-        orc_make_frame(nn, t, bb);
-        s->direction = normalize(add(add(scl(ld3(t), local.x), scl(n, local.y)), scl(ld3(bb), local.z)));
-        float cos_l = local.y;
-        float area = length(ex) * length(ez);
-        float pdf_a = 1.0f / area;
-        emission_pdf_w = pdf_a * (cos_l / kPi) * p_pick;
-        direct_pdf_a = pdf_a * p_pick;
-        radiance = scl(ld3(L->radiance), cos_l);
+        const orc_light* L = &sc->lights[idx];
+        if (L->type == 1) {
+            float r0 = orc_uniform(u), r1 = orc_uniform(u);
+            /* uniform sphere (CSampler::UniformSphere, Sampler.cpp) */
+            float uu = 2.0f * r1 - 1.0f;
+            float norm = sqrtf(maxf(0.0f, 1.0f - uu * uu));
+            float theta = (kPi * 2.0f) * r0;
+            s->direction = v3(norm * cos_f(theta), uu, norm * sin_f(theta));
+            s->position = ld3(L->position);
+            emission_pdf_w = (1.0f / (4.0f * kPi)) * p_pick;
+            direct_pdf_a = 1.0f * p_pick;
+            radiance = ld3(L->radiance);
+        } else {
+            ov3 ex = ld3(L->x), ez = ld3(L->z);
+            float a = orc_uniform(u), b = orc_uniform(u);
+            s->position = add(add(sub(sub(ld3(L->position), scl(ex, 0.5f)), scl(ez, 0.5f)), scl(ex, a)), scl(ez, b));
+            float r0 = orc_uniform(u), r1 = orc_uniform(u);
+            float r = sqrtf(r0);
+            float th = (kPi * 2.0f) * r1;
+            ov3 local = v3(r * cos_f(th), sqrtf(maxf(0.0f, 1 - r0)), r * sin_f(th));
+            float t[3], bb[3];
+            ov3 n = ld3(L->direction);
+            float nn[3] = {n.x, n.y, n.z};
+            orc_make_frame(nn, t, bb);
+            s->direction = normalize(add(add(scl(ld3(t), local.x), scl(n, local.y)), scl(ld3(bb), local.z)));
+            float cos_l = local.y;
+            float area = length(ex) * length(ez);
+            float pdf_a = 1.0f / area;
+            emission_pdf_w = pdf_a * (cos_l / kPi) * p_pick;
+            direct_pdf_a = pdf_a * p_pick;
+            radiance = scl(ld3(L->radiance), cos_l);
+        }
     }
-    (void)vc_weight;
+    if (!(emission_pdf_w > 0.0f)) return 0;
     s->throughput = scl(radiance, 1.0f / emission_pdf_w);
     s->dvcm = direct_pdf_a / emission_pdf_w;
-    s->dvc = 0.0f; /* lights are not intersectable: no "hit the light" technique */
-    s->dvm = 0.0f;
+    if (s->infinite) {
+        /* the image-based light can be hit by camera paths: cosThetaLight = 1 */
+        s->dvc = 1.0f / emission_pdf_w;
+        s->dvm = s->dvc * vc_weight;
+    } else {
+        /* quad and point lights are not intersectable: no "hit the light" technique */
+        s->dvc = 0.0f;
+        s->dvm = 0.0f;
+    }
+    return 1;
 }
 
 /* Direct light sample for NEE at x (ConnectCameraPathToLight's light part). Uses 3 uniforms.
- * out: dir, radiance, distance, directPdfW (incl. pick), emissionPdfW (incl. pick), cosAtLight */
-static int direct_light(orc_uniforms* u, const orc_light* lights, uint32_t n_lights, ov3 x, ov3* dir, ov3* radiance,
-                        float* dist, float* direct_pdf_w, float* emission_pdf_w, float* cos_at_light)
+ * out: dir, radiance, distance, directPdfW (incl. pick), emissionPdfW (incl. pick), cosAtLight,
+ * and whether the light is the image-based one (DirectIblLightSample, AreaLighting.cpp:293-314). */
+static int direct_light(orc_uniforms* u, const orc_scene* sc, ov3 x, ov3* dir, ov3* radiance, float* dist,
+                        float* direct_pdf_w, float* emission_pdf_w, float* cos_at_light, int* infinite)
 {
-    float p_pick = 1.0f / n_lights;
-    uint32_t idx = (uint32_t)(orc_uniform(u) * n_lights);
-    if (idx >= n_lights) idx = n_lights - 1;
-    const orc_light* L = &lights[idx];
+    const uint32_t n_lights = sc->n_lights, n_pick = n_lights + (sc->ibl ? 1u : 0u);
+    float p_pick = 1.0f / n_pick;
+    uint32_t idx = (uint32_t)(orc_uniform(u) * n_pick);
+    if (idx >= n_pick) idx = n_pick - 1;
     float a = orc_uniform(u), b = orc_uniform(u);
+    *infinite = idx == n_lights;
+    if (*infinite) {
+        float o[8];
+        orc_ibl_direct(sc->ibl, a, b, o);
+        *dir = v3(o[0], o[1], o[2]);
+        *radiance = v3(o[3], o[4], o[5]);
+        *dist = 1e36f;
+        *direct_pdf_w = o[6] * p_pick;
+        *emission_pdf_w = (orc_ibl_position_pdf(sc->ibl) * o[6]) * p_pick;
+        *cos_at_light = 1.0f;
+        return o[6] > 0.0f;
+    }
+    const orc_light* L = &sc->lights[idx];
     if (L->type == 1) {
         ov3 d = sub(ld3(L->position), x);
         float d2 = dot(d, d);
@@ -447,9 +486,8 @@ static void light_paths(void* ctx, uint32_t b, uint32_t e)
         uint64_t st, inc, sid = stream_id(I->iteration, i, 0);
         orc_pcg32_seed(c.job->seed ^ (sid * 0x9E3779B97F4A7C15ull), sid, &st, &inc);
         orc_uniforms u = {NULL, 0, 0, &st, inc};
-        if (sc->n_lights > 0) {
-            emit_state es;
-            emit_light(&u, sc->lights, sc->n_lights, c.vc_weight, &es);
+        emit_state es;
+        if (sc->n_lights + (sc->ibl ? 1u : 0u) > 0 && emit_light(&u, sc, c.vc_weight, &es)) {
             ov3 pos = es.position, dir = es.direction, thr = es.throughput;
             float dvcm = es.dvcm, dvc = es.dvc, dvm = es.dvm;
             uint32_t plen = 1;
@@ -464,8 +502,9 @@ static void light_paths(void* ctx, uint32_t b, uint32_t e)
                 ov3 view = neg(dir);
                 float d2 = dot(sub(pos, x), sub(pos, x));
                 float cos_v = absf(dot(s.n, view));
-                /* finite lights: the first segment is converted to area measure too */
-                dvcm *= d2;
+                /* the first segment is converted to area measure too, except from a light at
+                 * infinity */
+                if (plen > 1 || !es.infinite) dvcm *= d2;
                 dvcm *= (1.0f / cos_v);
                 dvc *= (1.0f / cos_v);
                 dvm *= (1.0f / cos_v);
@@ -593,7 +632,21 @@ static void camera_paths(void* ctx, uint32_t b, uint32_t e)
             uint32_t geom;
             ++nrays;
             if (!hit_surface(&c, o, d, 0.00001f, &s, hp, &err, &geom)) {
-                if (plen == 1) color = add(color, mul(thr, ld3(sc->background)));
+                if (sc->ibl) {
+                    /* ConnectToSkyLight (VCM.cpp:171-186): unweighted when seen from the camera */
+                    float rad[3];
+                    float pdf = orc_ibl_lookup(sc->ibl, d, rad);
+                    float mis = 1.0f;
+                    if (plen != 1) {
+                        float p_pick = 1.0f / (sc->n_lights + 1);
+                        float direct_pdf_a = pdf * p_pick;
+                        float emission_pdf_w = (orc_ibl_position_pdf(sc->ibl) * pdf) * p_pick;
+                        mis = 1.0f / (1.0f + (direct_pdf_a * dvcm + emission_pdf_w * dvc));
+                    }
+                    color = add(color, mul(thr, scl(ld3(rad), mis)));
+                } else if (plen == 1) {
+                    color = add(color, mul(thr, ld3(sc->background)));
+                }
                 break;
             }
             ov3 x = ld3(hp);
@@ -605,10 +658,11 @@ static void camera_paths(void* ctx, uint32_t b, uint32_t e)
             dvc /= cos_v;
             dvm /= cos_v;
             /* NEE (ConnectCameraPathToLight, VCM.cpp:245-290) */
-            if (plen + 1 < L && sc->n_lights > 0) {
+            if (plen + 1 < L && sc->n_lights + (sc->ibl ? 1u : 0u) > 0) {
                 ov3 ldir, rad;
                 float ldist, dpdf, epdf, cl;
-                if (direct_light(&u, sc->lights, sc->n_lights, x, &ldir, &rad, &ldist, &dpdf, &epdf, &cl)) {
+                int infinite;
+                if (direct_light(&u, sc, x, &ldir, &rad, &ldist, &dpdf, &epdf, &cl, &infinite)) {
                     ov3 f;
                     float fwd, rev;
                     eval_bsdf(&s, view, ldir, &f, &fwd, &rev);
@@ -616,8 +670,10 @@ static void camera_paths(void* ctx, uint32_t b, uint32_t e)
                         float cp = cont_prob(&s);
                         rev *= cp;
                         float cos_s = absf(dot(s.n, ldir));
+                        /* only the image-based light can also be reached by the camera path */
+                        float w_light = infinite ? (fwd * cp) / dpdf : 0.0f;
                         float w_cam = (epdf * cos_s / (dpdf * cl)) * (c.vm_weight + dvcm + dvc * rev);
-                        float mis = 1.0f / (1.0f + w_cam);
+                        float mis = 1.0f / (w_light + 1.0f + w_cam);
                         ov3 contrib = scl(mul(f, rad), mis / dpdf);
                         if (!is_zero(contrib)) {
                             float so[3], ld[3] = {ldir.x, ldir.y, ldir.z};

@@ -1,8 +1,11 @@
 // C ABI of libselas_gpu: contexts, device memory helpers, scenes and ray queries.
 // Declarations and the Selas/Embree interfaces each entry point replaces are in include/selas_gpu.h.
+#include "sgpu_ibl.h"
 #include "sgpu_internal.h"
 #include "sgpu_surface.h"
 
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -338,6 +341,19 @@ int sgpu_scene_commit(sgpu_scene scene, void* stream)
     SGPU_HIP_TRY(hipMemset(scene->d_overflow, 0, sizeof(uint32_t)));
     int rc = sgpu_bvh_build(scene, as_stream(stream));
     if (rc != SGPU_OK) return rc;
+    // the sphere around the triangles' bounding box (the IBL emits from its disc)
+    if (!scene->tri_pos.empty()) {
+        float lo[3], hi[3];
+        for (int k = 0; k < 3; ++k) lo[k] = hi[k] = scene->tri_pos[k];
+        for (size_t i = 0; i < scene->tri_pos.size(); i += 3)
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = std::min(lo[k], scene->tri_pos[i + k]);
+                hi[k] = std::max(hi[k], scene->tri_pos[i + k]);
+            }
+        const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        for (int k = 0; k < 3; ++k) scene->bsphere[k] = (lo[k] + hi[k]) * 0.5f;
+        scene->bsphere[3] = 0.5f * sqrtf(dx * dx + dy * dy + dz * dz);
+    }
     scene->committed = true;
     std::vector<float>().swap(scene->tri_pos);
     return SGPU_OK;
@@ -355,6 +371,7 @@ int sgpu_scene_release(sgpu_scene scene)
     if (scene->d_lights) (void)hipFree(scene->d_lights);
     for (void* a : scene->tex_allocs) (void)hipFree(a);
     if (scene->d_geom_tex) (void)hipFree(scene->d_geom_tex);
+    if (scene->d_ibl) (void)hipFree(scene->d_ibl);
     delete scene;
     return SGPU_OK;
 }
@@ -420,6 +437,92 @@ int sgpu_scene_set_lights(sgpu_scene scene, const sgpu_light* lights, uint32_t n
     scene->d_lights = dl;
     scene->n_lights = n_lights;
     for (int k = 0; k < 3; ++k) scene->background[k] = background_rgb ? background_rgb[k] : 0.0f;
+    return SGPU_OK;
+}
+
+// CalculateIntensityMap + CalculateStrataDistributionFunctions (BuildImageBasedLight.cpp:18-103):
+// mdf (h entries) and cdf (w h entries), summed in the same order, the last entry of each forced
+// to 1.
+static void ibl_build_tables(uint32_t w, uint32_t h, const float* rgb, float* mdf, float* cdf)
+{
+    const float hf = (float)h;
+    std::fill(mdf, mdf + h, 0.0f);
+    std::fill(cdf, cdf + (size_t)w * h, 0.0f);
+    std::vector<float> row(w);
+    float marginal_sum = 0.0f;
+    for (uint32_t y = 0; y < h; ++y) {
+        const float theta = ((float)y + 0.5f) * sgpu::kPi / hf;
+        const float st = sin_f(theta);
+        float conditional_sum = 0.0f;
+        for (uint32_t x = 0; x < w; ++x) {
+            const float* t = rgb + 3 * ((size_t)y * w + x);
+            row[x] = st * (t[0] * 0.299f + t[1] * 0.587f + t[2] * 0.114f);
+            conditional_sum += row[x];
+        }
+        mdf[y] = conditional_sum;
+        marginal_sum += conditional_sum;
+        float* c = cdf + (size_t)y * w;
+        if (conditional_sum > 0.0f) {
+            const float oo = 1.0f / conditional_sum;
+            float integration = 0.0f;
+            for (uint32_t x = 0; x < w; ++x) {
+                integration += oo * row[x];
+                c[x] = integration;
+            }
+        }
+        c[w - 1] = 1.0f;
+    }
+    if (marginal_sum > 0.0f) {
+        const float oo = 1.0f / marginal_sum;
+        float integration = 0.0f;
+        for (uint32_t y = 0; y < h; ++y) {
+            integration += mdf[y] * oo;
+            mdf[y] = integration;
+        }
+    }
+    mdf[h - 1] = 1.0f;
+}
+
+int sgpu_scene_set_ibl(sgpu_scene scene, uint32_t width, uint32_t height, const float* rgb, float rotation_radians,
+                       float exposure)
+{
+    if (!scene) return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_ibl: NULL scene");
+    std::lock_guard<std::mutex> lock(scene->mu);
+    if (!scene->committed) return sgpu_fail(SGPU_E_STATE, "sgpu_scene_set_ibl: scene not committed");
+    SGPU_HIP_TRY(hipSetDevice(scene->ctx->device));
+    if (!rgb) {
+        if (scene->d_ibl) SGPU_HIP_TRY(hipFree(scene->d_ibl));
+        scene->d_ibl = nullptr;
+        scene->ibl_w = scene->ibl_h = 0;
+        return SGPU_OK;
+    }
+    if (width == 0 || height == 0 || width > 16384 || height > 16384)
+        return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_ibl: width and height must be in [1, 16384]");
+    if (!std::isfinite(rotation_radians) || !std::isfinite(exposure) || exposure < 0.0f)
+        return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_ibl: rotation must be finite and exposure finite and >= 0");
+    const size_t n = (size_t)width * height;
+    for (size_t i = 0; i < 3 * n; ++i)
+        if (!(rgb[i] >= 0.0f) || !std::isfinite(rgb[i]))
+            return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_ibl: texels must be finite and >= 0");
+    std::vector<float> host(3 * n + height + n);
+    std::memcpy(host.data(), rgb, 3 * n * sizeof(float));
+    ibl_build_tables(width, height, rgb, host.data() + 3 * n, host.data() + 3 * n + height);
+    float* d = nullptr;
+    SGPU_HIP_TRY(hipMalloc((void**)&d, host.size() * sizeof(float)));
+    hipError_t e = hipMemcpy(d, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        return sgpu_fail(SGPU_E_HIP, std::string("sgpu_scene_set_ibl: ") + hipGetErrorString(e));
+    }
+    if (scene->d_ibl) (void)hipFree(scene->d_ibl);
+    scene->d_ibl = d;
+    scene->ibl_w = width;
+    scene->ibl_h = height;
+    float r = fmodf(rotation_radians, sgpu::kTwoPi);
+    if (r < 0.0f) r += sgpu::kTwoPi;
+    if (r >= sgpu::kTwoPi) r = 0.0f;
+    scene->ibl_rotation = r;
+    scene->ibl_exposure = exposure;
     return SGPU_OK;
 }
 
@@ -653,7 +756,8 @@ int sgpu_scene_device_bytes(sgpu_scene scene, uint64_t* bytes)
                  (uint64_t)scene->n_tris * sizeof(TriRecord) +
                  sizeof(uint32_t) + (uint64_t)scene->n_materials * 22 * sizeof(float) +
                  (uint64_t)scene->n_geom_material * sizeof(uint32_t) + (uint64_t)scene->n_lights * sizeof(sgpu_light) +
-                 (uint64_t)scene->n_geom_tex * sizeof(sgpu::PtexTex);
+                 (uint64_t)scene->n_geom_tex * sizeof(sgpu::PtexTex) +
+                 (uint64_t)scene->ibl_w * scene->ibl_h * 4 * sizeof(float) + (uint64_t)scene->ibl_h * sizeof(float);
     for (const sgpu::PtexTex& t : scene->tex_host)
         if (t.texels) b += (uint64_t)t.nfaces * (3ull << (2 * t.res_log2)) + (uint64_t)t.nfaces * 8 * sizeof(int32_t);
     *bytes = b;

@@ -107,6 +107,13 @@ struct sgpu_scene_s {
     std::vector<void*> tex_allocs;
     sgpu::PtexTex* d_geom_tex = nullptr;
     uint32_t n_geom_tex = 0;
+    // image-based light (sgpu_scene_set_ibl): texels, marginal CDF and conditional CDFs in one
+    // allocation (d_ibl: w h 3 + h + w h floats); none when d_ibl == nullptr
+    float* d_ibl = nullptr;
+    uint32_t ibl_w = 0, ibl_h = 0;
+    float ibl_rotation = 0.0f, ibl_exposure = 1.0f;
+    // sphere around the triangles' bounding box (centre, radius), set at commit
+    float bsphere[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 };
 
 // ---- error plumbing ------------------------------------------------------------------------------

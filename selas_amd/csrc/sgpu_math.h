@@ -89,6 +89,7 @@ SGPU_HD Pcg32 pcg32_seed(uint64_t initstate, uint64_t initseq)
 SGPU_HD float cos_f(float x) { return (float)cos((double)x); }
 SGPU_HD float sin_f(float x) { return (float)sin((double)x); }
 SGPU_HD float acos_f(float x) { return (float)acos((double)x); }
+SGPU_HD float atan2_f(float y, float x) { return (float)atan2((double)y, (double)x); }
 SGPU_HD float pow_f(float x, float y) { return (float)pow((double)x, (double)y); }
 // x^5 (Schlick's weight), by double multiplications: the same bits on host and device.
 SGPU_HD float pow5_f(float x)

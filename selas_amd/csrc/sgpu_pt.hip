@@ -12,6 +12,7 @@
 // and the frame is scaled by 1/spp, as FrameBuffer_Scale does (PathTracer.cpp:346).
 // Layers: 0 = first-bounce reflectance, 1 = radiance (PathTracer.cpp:191,224,241).
 #include "sgpu_camera.h"
+#include "sgpu_ibl.h"
 #include "sgpu_internal.h"
 #include "sgpu_light.h"
 #include "sgpu_surface.h"
@@ -132,6 +133,8 @@ struct ShadeParams {
     const Light* lights;
     uint32_t n_lights;
     f3 background;
+    uint32_t has_ibl;  // 1: misses look up the image-based light instead of the background
+    IblView ibl;
     uint32_t bounce;  // bounces completed before this one
     uint32_t max_len;
     uint32_t flags;
@@ -156,8 +159,12 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(ShadeParams sp, uint32_t co
         f3 thr = get3(ps.thr, p);
         uint32_t geom = hits.geom[j];
         if (geom == SGPU_INVALID_ID || geom >= sp.n_geoms) {
-            // miss: background (EvaluateBackground and EvaluateBackgroundMiss agree without an IBL)
-            add3(ps.ld1, p, sp.background * thr);
+            // miss: EvaluateBackground (AreaLighting.cpp:415-424): the image-based light's radiance
+            // from the ray's direction, else the constant background (EvaluateBackgroundMiss reads
+            // a second image of the same light, which here is the same image)
+            float pdf;
+            const f3 bg = sp.has_ibl ? ibl_lookup(sp.ibl, d, pdf) : sp.background;
+            add3(ps.ld1, p, bg * thr);
         } else {
             PcgUniforms u{load_rng(ps, sp.path_base + p, p)};
             const float* mat = sp.materials + 22 * (size_t)sp.geom_material[geom];
@@ -347,6 +354,7 @@ int sgpu_pt_render_impl(sgpu_scene scene, const sgpu_pt_job* job, float* fb, hip
     sp.lights = (const Light*)scene->d_lights;
     sp.n_lights = scene->n_lights;
     sp.background = mk3(scene->background[0], scene->background[1], scene->background[2]);
+    sp.has_ibl = ibl_view(scene, sp.ibl) ? 1u : 0u;
     sp.geom_tex = scene->d_geom_tex;
     sp.n_geom_tex = scene->n_geom_tex;
     sp.max_len = job->max_path_length;

@@ -70,48 +70,72 @@ __global__ __launch_bounds__(kBlock) void k_light_gen(Wave w, PathState ps, RayB
         ps.slice[g] = si;
         PcgUniforms u{rng_init(sl.seed, stream_id(sl.iteration, g - sl.path_base, 0))};
         ps.plen[g] = 1;
-        if (sl.n_lights > 0) {
-            float p_pick = 1.0f / sl.n_lights;
-            uint32_t idx = (uint32_t)(u.next() * sl.n_lights);
-            if (idx >= sl.n_lights) idx = sl.n_lights - 1;
-            const Light L = sl.lights[idx];
+        // GenerateLightSample (VCMCommon.cpp:37-63): one light picked uniformly, the image-based
+        // light being light n_lights
+        const uint32_t n_pick = sl.n_lights + sl.has_ibl;
+        if (n_pick > 0) {
+            float p_pick = 1.0f / n_pick;
+            uint32_t idx = (uint32_t)(u.next() * n_pick);
+            if (idx >= n_pick) idx = n_pick - 1;
+            const bool infinite = idx == sl.n_lights;
             float emission_pdf_w, direct_pdf_a;
             f3 radiance;
-            if (L.type == 1) {
-                float r0 = u.next(), r1 = u.next();
-                float uu = 2.0f * r1 - 1.0f;
-                float norm = sqrtf(max_(0.0f, 1.0f - uu * uu));
-                float theta = (kPi * 2.0f) * r0;
-                d = mk3(norm * cos_f(theta), uu, norm * sin_f(theta));
-                o = ld3(L.position);
-                emission_pdf_w = (1.0f / (4.0f * kPi)) * p_pick;
-                direct_pdf_a = 1.0f * p_pick;
-                radiance = ld3(L.radiance);
+            if (infinite) {
+                // EmitIblLightSample (AreaLighting.cpp:246-290)
+                float r0 = u.next(), r1 = u.next(), r2 = u.next(), r3 = u.next();
+                float dir_pdf;
+                ibl_emit(sl.ibl, r0, r1, r2, r3, o, d, radiance, emission_pdf_w, dir_pdf);
+                emission_pdf_w *= p_pick;
+                direct_pdf_a = dir_pdf * p_pick;
             } else {
-                f3 ex = ld3(L.x), ez = ld3(L.z);
-                float a = u.next(), b = u.next();
-                o = ld3(L.position) - ex * 0.5f - ez * 0.5f + ex * a + ez * b;
-                float r0 = u.next(), r1 = u.next();
-                float r = sqrtf(r0);
-                float th = (kPi * 2.0f) * r1;
-                f3 local = mk3(r * cos_f(th), sqrtf(max_(0.0f, 1 - r0)), r * sin_f(th));
-                f3 n = ld3(L.direction), t, bb;
-                make_frame(n, t, bb);
-                d = normalize3(t * local.x + n * local.y + bb * local.z);
-                float cos_l = local.y;
-                float area = length3(ex) * length3(ez);
-                float pdf_a = 1.0f / area;
-                emission_pdf_w = pdf_a * (cos_l / kPi) * p_pick;
-                direct_pdf_a = pdf_a * p_pick;
-                radiance = ld3(L.radiance) * cos_l;
+                const Light L = sl.lights[idx];
+                if (L.type == 1) {
+                    float r0 = u.next(), r1 = u.next();
+                    float uu = 2.0f * r1 - 1.0f;
+                    float norm = sqrtf(max_(0.0f, 1.0f - uu * uu));
+                    float theta = (kPi * 2.0f) * r0;
+                    d = mk3(norm * cos_f(theta), uu, norm * sin_f(theta));
+                    o = ld3(L.position);
+                    emission_pdf_w = (1.0f / (4.0f * kPi)) * p_pick;
+                    direct_pdf_a = 1.0f * p_pick;
+                    radiance = ld3(L.radiance);
+                } else {
+                    f3 ex = ld3(L.x), ez = ld3(L.z);
+                    float a = u.next(), b = u.next();
+                    o = ld3(L.position) - ex * 0.5f - ez * 0.5f + ex * a + ez * b;
+                    float r0 = u.next(), r1 = u.next();
+                    float r = sqrtf(r0);
+                    float th = (kPi * 2.0f) * r1;
+                    f3 local = mk3(r * cos_f(th), sqrtf(max_(0.0f, 1 - r0)), r * sin_f(th));
+                    f3 n = ld3(L.direction), t, bb;
+                    make_frame(n, t, bb);
+                    d = normalize3(t * local.x + n * local.y + bb * local.z);
+                    float cos_l = local.y;
+                    float area = length3(ex) * length3(ez);
+                    float pdf_a = 1.0f / area;
+                    emission_pdf_w = pdf_a * (cos_l / kPi) * p_pick;
+                    direct_pdf_a = pdf_a * p_pick;
+                    radiance = ld3(L.radiance) * cos_l;
+                }
             }
-            s3(ps.pos, g, o);
-            s3(ps.dir, g, d);
-            s3(ps.thr, g, radiance * (1.0f / emission_pdf_w));
-            ps.dvcm[g] = direct_pdf_a / emission_pdf_w;
-            ps.dvc[g] = 0.0f;
-            ps.dvm[g] = 0.0f;
-            live = 1 + 2 < sl.L;
+            if (emission_pdf_w > 0.0f) {
+                s3(ps.pos, g, o);
+                s3(ps.dir, g, d);
+                s3(ps.thr, g, radiance * (1.0f / emission_pdf_w));
+                ps.dvcm[g] = direct_pdf_a / emission_pdf_w;
+                if (infinite) {
+                    // the image-based light can be hit by camera paths (cosThetaLight = 1)
+                    const float dvc = 1.0f / emission_pdf_w;
+                    ps.dvc[g] = dvc;
+                    ps.dvm[g] = dvc * sl.vc_weight;
+                    ps.plen[g] = 1u | kInfiniteOrigin;
+                } else {
+                    // quad and point lights cannot be hit: no "camera path hits the light" technique
+                    ps.dvc[g] = 0.0f;
+                    ps.dvm[g] = 0.0f;
+                }
+                live = 1 + 2 < sl.L;
+            }
         }
         ps.rng[g] = u.r.state;
     }
@@ -158,10 +182,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             f3 thr = g3(ps.thr, p);
             float dvcm = ps.dvcm[p], dvc = ps.dvc[p], dvm = ps.dvm[p];
             uint32_t plen = ps.plen[p];
+            const bool from_infinite = (plen & kInfiniteOrigin) != 0u;
+            plen &= ~kInfiniteOrigin;
             f3 dd = pos - x;
             float d2 = dot3(dd, dd);
             float cos_v = absf_(dot3(s.n, view));
-            dvcm *= d2;
+            // the first segment is converted to area measure, except from a light at infinity
+            if (!from_infinite) dvcm *= d2;
             dvcm *= (1.0f / cos_v);
             dvc *= (1.0f / cos_v);
             dvm *= (1.0f / cos_v);
@@ -476,7 +503,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         uint32_t geom = hits.geom[j];
         f3 d = mk3(rays.dx[j], rays.dy[j], rays.dz[j]);
         if (geom == SGPU_INVALID_ID || geom >= sl.n_geoms) {
-            if (ps.plen[p] == 1) {
+            const uint32_t plen_miss = ps.plen[p];
+            if (sl.has_ibl) {
+                // ConnectToSkyLight (VCM.cpp:171-186): unweighted when seen from the camera
+                float pdf;
+                const f3 rad = ibl_lookup(sl.ibl, d, pdf);
+                float mis = 1.0f;
+                if (plen_miss != 1) {
+                    const float p_pick = 1.0f / (sl.n_lights + 1);
+                    const float direct_pdf_a = pdf * p_pick;
+                    const float emission_pdf_w = (ibl_position_pdf(sl.ibl) * pdf) * p_pick;
+                    mis = 1.0f / (1.0f + (direct_pdf_a * ps.dvcm[p] + emission_pdf_w * ps.dvc[p]));
+                }
+                s3(ps.color, p, g3(ps.color, p) + g3(ps.thr, p) * (rad * mis));
+            } else if (plen_miss == 1) {
                 f3 col = g3(ps.color, p) + g3(ps.thr, p) * sl.background;
                 s3(ps.color, p, col);
             }
@@ -519,37 +559,50 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         bool emit = false;
         f3 so = mk3(0, 0, 0), ld = mk3(0, 0, 0);
         float tn = 0, tf = 0;
-        if (hit && plen + 1 < sl.L && sl.n_lights > 0) {
-            float p_pick = 1.0f / sl.n_lights;
-            uint32_t idx = (uint32_t)(u.next() * sl.n_lights);
-            if (idx >= sl.n_lights) idx = sl.n_lights - 1;
-            const Light L = sl.lights[idx];
+        const uint32_t n_pick = sl.n_lights + sl.has_ibl;
+        if (hit && plen + 1 < sl.L && n_pick > 0) {
+            float p_pick = 1.0f / n_pick;
+            uint32_t idx = (uint32_t)(u.next() * n_pick);
+            if (idx >= n_pick) idx = n_pick - 1;
+            const bool infinite = idx == sl.n_lights;
             float a = u.next(), b = u.next();
             bool ok = true;
             f3 rad;
             float dist, dpdf, epdf, cl;
-            if (L.type == 1) {
-                f3 dv = ld3(L.position) - x;
-                float dd2 = dot3(dv, dv);
-                dist = sqrtf(dd2);
-                ld = dv * (1.0f / dist);
-                rad = ld3(L.radiance);
-                dpdf = dd2 * p_pick;
-                epdf = (1.0f / (4.0f * kPi)) * p_pick;
+            if (infinite) {
+                // DirectIblLightSample (AreaLighting.cpp:293-314)
+                float pdf;
+                rad = ibl_direct(sl.ibl, a, b, ld, pdf);
+                ok = pdf > 0.0f;
+                dist = 1e36f;
+                dpdf = pdf * p_pick;
+                epdf = (ibl_position_pdf(sl.ibl) * pdf) * p_pick;
                 cl = 1.0f;
             } else {
-                f3 ex = ld3(L.x), ez = ld3(L.z);
-                f3 y = ld3(L.position) - ex * 0.5f - ez * 0.5f + ex * a + ez * b;
-                f3 dv = y - x;
-                float dd2 = dot3(dv, dv);
-                dist = sqrtf(dd2);
-                ld = dv * (1.0f / dist);
-                cl = dot3(ld3(L.direction), -ld);
-                ok = cl > 0.0f;
-                float pdf_a = 1.0f / (length3(ex) * length3(ez));
-                rad = ld3(L.radiance);
-                dpdf = pdf_a * dd2 / cl * p_pick;
-                epdf = pdf_a * (cl / kPi) * p_pick;
+                const Light L = sl.lights[idx];
+                if (L.type == 1) {
+                    f3 dv = ld3(L.position) - x;
+                    float dd2 = dot3(dv, dv);
+                    dist = sqrtf(dd2);
+                    ld = dv * (1.0f / dist);
+                    rad = ld3(L.radiance);
+                    dpdf = dd2 * p_pick;
+                    epdf = (1.0f / (4.0f * kPi)) * p_pick;
+                    cl = 1.0f;
+                } else {
+                    f3 ex = ld3(L.x), ez = ld3(L.z);
+                    f3 y = ld3(L.position) - ex * 0.5f - ez * 0.5f + ex * a + ez * b;
+                    f3 dv = y - x;
+                    float dd2 = dot3(dv, dv);
+                    dist = sqrtf(dd2);
+                    ld = dv * (1.0f / dist);
+                    cl = dot3(ld3(L.direction), -ld);
+                    ok = cl > 0.0f;
+                    float pdf_a = 1.0f / (length3(ex) * length3(ez));
+                    rad = ld3(L.radiance);
+                    dpdf = pdf_a * dd2 / cl * p_pick;
+                    epdf = pdf_a * (cl / kPi) * p_pick;
+                }
             }
             if (ok) {
                 float fwd, rev;
@@ -558,8 +611,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
                     float cp = cont_prob(s);
                     rev *= cp;
                     float cos_s = absf_(dot3(s.n, ld));
+                    // only the image-based light can also be reached by the camera path itself
+                    float w_light = infinite ? (fwd * cp) / dpdf : 0.0f;
                     float w_cam = (epdf * cos_s / (dpdf * cl)) * (sl.vm_weight + dvcm + dvc * rev);
-                    float mis = 1.0f / (1.0f + w_cam);
+                    float mis = 1.0f / (w_light + 1.0f + w_cam);
                     f3 c = (f * rad) * (mis / dpdf);
                     if (!is_zero3(c)) {
                         emit = true;
@@ -1224,6 +1279,7 @@ int sgpu_vcm_server_submit_impl(sgpu_vcm_server sv, sgpu_scene sc, const sgpu_vc
     b.background = mk3(sc->background[0], sc->background[1], sc->background[2]);
     b.geom_tex = sc->d_geom_tex;
     b.n_geom_tex = sc->n_geom_tex;
+    b.has_ibl = ibl_view(sc, b.ibl) ? 1u : 0u;
     SGPU_HIP_TRY(hipMemsetAsync(fb, 0, 3 * (size_t)b.N * sizeof(float), stream));
     sv->active.push_back(j);
     if (id_out) *id_out = j.id;

@@ -4,6 +4,7 @@
 #pragma once
 
 #include "sgpu_camera.h"
+#include "sgpu_ibl.h"
 #include "sgpu_internal.h"
 #include "sgpu_light.h"
 #include "sgpu_ptex.h"
@@ -16,6 +17,8 @@ using namespace sgpu;
 
 constexpr int kBlock = 256;
 constexpr float kFloatMax = 3.402823466e+38F;
+// PathState::plen bit of a light path that starts at the image-based light, cleared at its first hit
+constexpr uint32_t kInfiniteOrigin = 1u << 31;
 constexpr float kRayEps = 0.00001f;  // RayPick's tnear in the VCM code (VCM.cpp:137)
 constexpr uint32_t kNoQuery = 0xFFFFFFFFu;
 
@@ -117,6 +120,8 @@ struct Slice {
     f3 background;
     const PtexTex* geom_tex;  // per-face texture per geometry (texels == nullptr: none)
     uint32_t n_geom_tex;
+    uint32_t has_ibl;         // 1: the image-based light is light n_lights of the uniform pick
+    IblView ibl;
 };
 
 // Wavefront-wide arguments.

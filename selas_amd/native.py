@@ -84,6 +84,7 @@ _SIGNATURES = {
     "sgpu_intersect_stats": [_P, C.POINTER(RaysSoA), C.POINTER(HitsSoA), _U32, C.c_int, C.POINTER(TraceStats)],
     "sgpu_scene_set_materials": [_P, _FP, _U32, _U32P, _U32],
     "sgpu_scene_set_lights": [_P, _P, _U32, _FP],
+    "sgpu_scene_set_ibl": [_P, _U32, _U32, _FP, _F, _F],
     "sgpu_render_pt": [_P, _P, _P, _P],
     "sgpu_render_vcm": [_P, _P, _P, _P, _P],
     "sgpu_render_vcm_batch": [_P, _P, _P, _U32, _U32, _P, _P],
@@ -321,6 +322,17 @@ class Scene:
         check(lib.sgpu_scene_set_ptex(self.handle, geom, res_log2, af.shape[0], t.ctypes.data, af.ctypes.data,
                                       ae.ctypes.data))
 
+    def set_ibl(self, rgb, rotation: float = 0.0, exposure: float = 1.0):
+        """Image-based light from an equirectangular map (height x width x 3 floats, row 0 at +y)
+        through sgpu_scene_set_ibl; rgb=None removes it. The scene must be committed."""
+        if rgb is None:
+            check(lib.sgpu_scene_set_ibl(self.handle, 0, 0, None, 0.0, 1.0))
+            return
+        a = np.ascontiguousarray(rgb, np.float32)
+        if a.ndim != 3 or a.shape[2] != 3:
+            raise ValueError("set_ibl: rgb must be height x width x 3")
+        check(lib.sgpu_scene_set_ibl(self.handle, a.shape[1], a.shape[0], fptr(a), float(rotation), float(exposure)))
+
     def ptex_eval(self, geom: int, faces, u, v, stream=None) -> np.ndarray:
         """Filtered texels (n x 3) of geometry geom's texture at (face, u, v), through sgpu_ptex_eval."""
         faces = np.ascontiguousarray(faces, np.int32)
@@ -434,6 +446,9 @@ def build_render_scene(ctx: Context, sc: dict) -> Scene:
     bg = np.ascontiguousarray(sc["background"], np.float32)
     check(lib.sgpu_scene_set_lights(scene.handle, lights.ctypes.data if lights.size else None, lights.shape[0],
                                     fptr(bg)))
+    ibl = sc.get("ibl")
+    if ibl is not None:
+        scene.set_ibl(ibl["rgb"], ibl.get("rotation", 0.0), ibl.get("exposure", 1.0))
     return scene
 
 

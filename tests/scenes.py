@@ -15,6 +15,13 @@ the planes, so every technique (NEE, light tracing, vertex connection, vertex me
 multi-bounce paths. With Lambert BSDFs (pdfs consistent between sampling and evaluation) VCM and
 the path tracer estimate the same integral.
 
+cube_scene: a closed Lambert cube on a closed slab (every face wound outward, so every hit is on
+a front face -- Selas's Lambert sampling does not check the side of the outgoing direction, which
+would let light through the back of an open panel) under a point light, a quad light and a smooth
+sky with a broad bright lobe (sky_map). With Lambert BSDFs VCM and the path tracer estimate the same
+integral, and the image-based light adds paths of every technique: emission from the sky's disc,
+NEE toward it, and camera paths that leave the scene.
+
 dim_room: the closed room with albedos low enough (<= 0.1 after the 2.2 power) that paths longer
 than 8 segments carry < 1e-7 of the energy, so the path-length limits of the techniques (which
 differ between the path tracer and VCM) do not change the image.
@@ -119,4 +126,71 @@ def dim_room(seed=1):
     mats = sc["materials"].copy()
     mats[:, 0:3] = np.minimum(mats[:, 0:3], 0.35)
     sc["materials"] = mats
+    return sc
+
+
+def sky_map(h, w):
+    """An equirectangular sky (h x w x 3): brighter toward +y, with a broad bright lobe."""
+    th = (np.arange(h) + 0.5) / h * np.pi
+    ph = (np.arange(w) + 0.5) / w * 2 * np.pi - np.pi
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    d = np.stack([np.sin(T) * np.cos(P), np.cos(T), np.sin(T) * np.sin(P)], -1)
+    sun = np.array([0.5, 0.7, 0.3])
+    sun /= np.linalg.norm(sun)
+    lobe = np.maximum(d @ sun, 0) ** 8
+    base = 0.3 + 0.7 * np.maximum(d[..., 1], 0)
+    rgb = np.stack([base * 0.6 + 4 * lobe, base * 0.8 + 3.5 * lobe, base * 1.0 + 2.5 * lobe], -1)
+    return rgb.astype(np.float32)
+
+
+def closed_box(lo, hi):
+    """The 12 triangles (9 floats each) of an axis-aligned box, wound so every normal points out."""
+    lo = np.asarray(lo, np.float64)
+    hi = np.asarray(hi, np.float64)
+    c = [np.array([hi[0] if i & 1 else lo[0], hi[1] if i & 2 else lo[1], hi[2] if i & 4 else lo[2]])
+         for i in range(8)]
+    centre = 0.5 * (lo + hi)
+    tris = []
+    for q in ((0, 2, 3, 1), (4, 5, 7, 6), (0, 1, 5, 4), (2, 6, 7, 3), (0, 4, 6, 2), (1, 3, 7, 5)):
+        a, b, cc, d = (c[k] for k in q)
+        for t in ((a, b, cc), (a, cc, d)):
+            if np.dot(np.cross(t[1] - t[0], t[2] - t[0]), t[0] - centre) < 0:
+                t = (t[0], t[2], t[1])
+            tris.append(np.concatenate(t))
+    return np.array(tris, np.float32)
+
+
+def cube_scene(albedo=0.7, lights=True, ibl=True, sky=(64, 128)):
+    sc = plane_scene(albedo)
+    sc["geoms"] = [closed_box((-40, -0.2, -40), (40, 0.0, 40)), closed_box((-0.5, 0.0, 0.3), (0.5, 0.9, 1.3))]
+    sc["materials"] = np.stack([W.material(2, (albedo, 0.8 * albedo, 0.6 * albedo)), W.material(2, (0.5, 0.7, 0.6))])
+    sc["geom_material"] = np.array([0, 1], np.uint32)
+    lt = sc["lights"].copy()
+    lt[1]["position"] = (-0.3, 1.6, 0.2)
+    sc["lights"] = lt if lights else lt[:0]
+    if ibl:
+        sc["ibl"] = {"rgb": sky_map(*sky), "rotation": 0.7, "exposure": 0.8}
+    return sc
+
+
+def uniform_sky_plane(albedo=0.6, h=64, w=128, radiance=1.0, rotation=0.3):
+    """plane_scene lit only by a uniform sky of the given radiance: every camera pixel sees the floor,
+    whose radiance is rho x radiance (a Lambert reflector under a uniform hemisphere)."""
+    sc = plane_scene(albedo)
+    sc["lights"] = sc["lights"][:0]
+    sc["ibl"] = {"rgb": np.full((h, w, 3), radiance, np.float32), "rotation": rotation, "exposure": 1.0}
+    return sc
+
+
+def small_sky_plane(half=1.0, radiance=1.0):
+    """A 2 half x 2 half Lambert floor under a uniform sky, seen from close by (every pixel on the
+    floor, radiance rho x radiance). Its bounding sphere is small, so light tracing and merging from
+    the sky's emission disc carry a large share of the image and every MIS term of sky emission
+    (the first segment without d^2, dVC and dVM of an intersectable light) moves it by 9-11 %."""
+    sc = plane_scene()
+    p = [np.array(v, np.float64) for v in ((-half, 0, -half), (-half, 0, half), (half, 0, half), (half, 0, -half))]
+    sc["geoms"] = [np.array([[p[0], p[1], p[2]], [p[0], p[2], p[3]]], np.float32).reshape(-1, 9)]
+    sc["lights"] = sc["lights"][:0]
+    sc["ibl"] = {"rgb": np.full((64, 128, 3), radiance, np.float32), "rotation": 0.3, "exposure": 1.0}
+    sc["camera"] = {"position": (0.0, 1.2, -0.3), "look_at": (0.0, 0.0, 0.1), "up": (0.0, 1.0, 0.0), "fov": 50.0}
     return sc
