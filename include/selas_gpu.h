@@ -64,7 +64,11 @@ typedef struct sgpu_rays_soa {
  *   prim_id primitive index within its geometry (the quad index for sgpu_scene_add_quads
  *           geometries), SGPU_INVALID_ID on a miss
  *   geom_id geometry id returned by sgpu_scene_add_triangles, SGPU_INVALID_ID on a miss
- *   ng_*    unnormalised geometric normal (v1-v0)x(v2-v0)
+ *   ng_*    unnormalised geometric normal (v1-v0)x(v2-v0), in the space of the hit geometry
+ *           (an instanced geometry's own space, as Embree reports it)
+ *   inst_id0, inst_id1  instID[0], instID[1]: the instance of the hit in the queried scene and
+ *           the instance inside that one (sgpu_scene_add_instance); SGPU_INVALID_ID where there is
+ *           no such level and on a miss
  * Passing rays.tfar's storage as hits.t reproduces Embree's in-place tfar update. */
 typedef struct sgpu_hits_soa {
     float* t;
@@ -75,6 +79,8 @@ typedef struct sgpu_hits_soa {
     float* ng_x;
     float* ng_y;
     float* ng_z;
+    uint32_t* inst_id0;
+    uint32_t* inst_id1;
 } sgpu_hits_soa;
 
 /* Per-query traversal counters (filled by sgpu_intersect_stats). */
@@ -165,9 +171,25 @@ int sgpu_scene_add_triangles(sgpu_scene scene, const float* positions, uint32_t 
  * triangle's barycentrics (u', v') become (1 - u', 1 - v')), the coordinates Ptex faces use. */
 int sgpu_scene_add_quads(sgpu_scene scene, const float* positions, uint32_t n_verts, const uint32_t* indices,
                          uint32_t n_quads, const float* colmajor4x4, uint32_t* geom_id_out);
+/* Instance of a committed scene `child`, placed by a column-major 4x4 local-to-world transform
+ * (an affine one: the last row is ignored): replaces rtcNewGeometry(RTC_GEOMETRY_TYPE_INSTANCE) +
+ * rtcSetGeometryInstancedScene + rtcSetGeometryTransform(RTC_FORMAT_FLOAT4X4_COLUMN_MAJOR)
+ * (SubsceneResource.cpp:61-78, a model instanced in a subscene) and Selas's user-geometry
+ * subscene instances (SceneResource.cpp:87-169, 220-249, a subscene instanced in the scene). A
+ * scene holds either geometries or instances; a child holds geometries or instances of scenes
+ * with geometries, so a query crosses at most two instance levels (MaxInstanceLevelCount_ = 2),
+ * reported as inst_id0 and inst_id1 (SceneResource.cpp:136-137). Instance ids count from 0 in
+ * call order. The two levels are composed at commit (in double precision, rounded once) into one
+ * level of instances with world-to-local transforms, over which the device builds the top tree; a
+ * ray is transformed into the geometry's space, where the triangle test runs, and t is the world
+ * ray's parameter. The child must stay alive while an instance refers to it (its release fails
+ * with SGPU_E_STATE). Instanced scenes answer queries (sgpu_intersect, sgpu_occluded,
+ * sgpu_intersect_stats); the renderers take scenes of geometries. */
+int sgpu_scene_add_instance(sgpu_scene scene, sgpu_scene child, const float* colmajor4x4, uint32_t* inst_id_out);
 int sgpu_scene_commit(sgpu_scene scene, void* stream);
 int sgpu_scene_release(sgpu_scene scene);
-/* Number of triangles, BVH nodes and maximum leaf depth of a committed scene. */
+/* Number of triangles, BVH nodes and maximum leaf depth of a committed scene (of an instanced
+ * scene: 0 triangles, and the top tree's nodes and depth over its composed instances). */
 int sgpu_scene_info(sgpu_scene scene, uint32_t* n_tris, uint32_t* n_nodes, uint32_t* max_depth);
 /* Device memory the scene holds (tree, triangles, shading records, lights, textures), in bytes:
  * what keeping it resident costs. No counterpart in Selas (its scenes live in host memory). */

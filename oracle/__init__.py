@@ -468,3 +468,42 @@ class PtexTexture:
         lib.orc_ptex_eval_batch(C.byref(self.s), faces.shape[0], faces.ctypes.data, u.ctypes.data, v.ctypes.data,
                                 out.ctypes.data)
         return out
+
+
+# ---- instanced scenes (orc_trace.c) -------------------------------------------------------------
+
+class _Inst(C.Structure):
+    _fields_ = [("w2l", C.c_float * 12), ("blas", C.c_void_p)]
+
+
+lib.orc_inst_closest.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(_Rays), C.c_uint32, C.POINTER(_Hits), C.c_void_p,
+                                 C.c_int]
+lib.orc_inst_occluded.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(_Rays), C.c_uint32, C.c_void_p, C.c_int]
+
+
+class InstancedScene:
+    """Composed instances: (world-to-local float rows of 4 (3 x 4), CpuBvh of the placed scene)."""
+
+    def __init__(self, insts):
+        self.blas = [b for _, b in insts]
+        self.arr = (_Inst * max(len(insts), 1))()
+        for k, (w2l, b) in enumerate(insts):
+            self.arr[k] = _Inst((C.c_float * 12)(*[float(x) for x in np.asarray(w2l, np.float32).reshape(-1)]),
+                                b.handle)
+        self.n = len(insts)
+
+    def closest(self, rays, threads=0):
+        """Closest hits of a ray dict (RAY_KEYS): t, u, v, prim (in the placed scene), inst."""
+        rs, keep, n = _rays(rays)
+        out = {"t": np.zeros(n, np.float32), "u": np.zeros(n, np.float32), "v": np.zeros(n, np.float32),
+               "prim": np.zeros(n, np.uint32), "inst": np.zeros(n, np.uint32)}
+        h = _Hits(*[out[k].ctypes.data for k in ("t", "prim", "u", "v")])
+        lib.orc_inst_closest(C.cast(self.arr, C.c_void_p), self.n, C.byref(rs), n, C.byref(h), out["inst"].ctypes.data,
+                             _threads(threads))
+        return out
+
+    def occluded(self, rays, threads=0):
+        rs, keep, n = _rays(rays)
+        occ = np.zeros(n, np.uint32)
+        lib.orc_inst_occluded(C.cast(self.arr, C.c_void_p), self.n, C.byref(rs), n, occ.ctypes.data, _threads(threads))
+        return occ

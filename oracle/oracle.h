@@ -43,6 +43,16 @@ uint32_t orc_bvh_closest1(const orc_bvh* b, uint32_t* stack, const float* o, con
                           float* t, float* u, float* v);
 int orc_bvh_occluded1(const orc_bvh* b, uint32_t* stack, const float* o, const float* d, float tnear, float tfar);
 void orc_parallel_for(void (*fn)(void* ctx, uint32_t begin, uint32_t end), void* ctx, uint32_t n, int threads);
+/* instanced scenes (composed instances: world-to-local rows of 4 floats and the placed scene's tree);
+ * closest hits give prim (triangle index in the placed scene) and inst_out (composed instance) */
+typedef struct {
+    float w2l[12];
+    const orc_bvh* blas;
+} orc_inst;
+void orc_inst_closest(const orc_inst* insts, uint32_t n_inst, const orc_rays* rays, uint32_t n, const orc_hits* hits,
+                      uint32_t* inst_out, int threads);
+void orc_inst_occluded(const orc_inst* insts, uint32_t n_inst, const orc_rays* rays, uint32_t n, uint32_t* occluded,
+                       int threads);
 
 /* ---- shading (orc_shade.c) ---------------------------------------------------------------------- */
 typedef struct {

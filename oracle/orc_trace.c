@@ -598,3 +598,90 @@ void orc_bvh_occluded(const orc_bvh* b, const orc_rays* rays, uint32_t n, uint32
 }
 
 uint32_t orc_bvh_node_count(const orc_bvh* b) { return b->n_nodes; }
+
+/* ---- instanced scenes --------------------------------------------------------------------------
+ * An instanced scene restated as its composed instances (instance i of the scene, then its
+ * instance j), each a world-to-local transform (3 rows of 4, float) of a scene of geometries with
+ * its own tree. Every instance is tested (no top tree): the ray is moved into the instance's space
+ * with the fused multiply-add chains the device uses (x' = fma(w0, x, fma(w1, y, fma(w2, z, w3))),
+ * directions without w3), the instance's closest hit found by trav_one, and the smallest t kept,
+ * ties to the earlier instance (then, inside it, the smaller triangle index). What Embree does
+ * for RTC_GEOMETRY_TYPE_INSTANCE (SubsceneResource.cpp:61-78) and Selas's user-geometry instances
+ * (SceneResource.cpp:87-169): t is the world ray's parameter (the direction is not renormalized). */
+
+static void inst_ray(const float* w, const float* o, const float* d, float* lo, float* ld)
+{
+    for (int r = 0; r < 3; ++r) {
+        const float* row = w + 4 * r;
+        lo[r] = fmaf(row[0], o[0], fmaf(row[1], o[1], fmaf(row[2], o[2], row[3])));
+        ld[r] = fmaf(row[0], d[0], fmaf(row[1], d[1], row[2] * d[2]));
+    }
+}
+
+typedef struct {
+    const orc_inst* insts;
+    uint32_t n_inst;
+    orc_rays rays;
+    orc_hits hits;
+    uint32_t* inst_out;
+    uint32_t* occluded;
+    int any;
+} inst_ctx;
+
+static void inst_range(void* cv, uint32_t begin, uint32_t end)
+{
+    inst_ctx* c = (inst_ctx*)cv;
+    uint32_t depth = 0;
+    for (uint32_t k = 0; k < c->n_inst; ++k)
+        if (c->insts[k].blas->depth > depth) depth = c->insts[k].blas->depth;
+    uint32_t* stack = (uint32_t*)malloc(sizeof(uint32_t) * (depth + 4));
+    for (uint32_t i = begin; i < end; ++i) {
+        const float o[3] = {c->rays.ox[i], c->rays.oy[i], c->rays.oz[i]};
+        const float d[3] = {c->rays.dx[i], c->rays.dy[i], c->rays.dz[i]};
+        const float tnear = c->rays.tnear[i], tfar = c->rays.tfar[i];
+        float best_t = tfar, bu = 0.0f, bv = 0.0f;
+        uint32_t best = 0xFFFFFFFFu, best_inst = 0xFFFFFFFFu;
+        int occ = 0;
+        for (uint32_t k = 0; k < c->n_inst && !occ; ++k) {
+            float lo[3], ld[3];
+            inst_ray(c->insts[k].w2l, o, d, lo, ld);
+            if (c->any) {
+                occ = (int)trav_one(c->insts[k].blas, stack, lo, ld, tnear, tfar, 1, NULL, NULL, NULL);
+                continue;
+            }
+            float t, u, v;
+            uint32_t p = trav_one(c->insts[k].blas, stack, lo, ld, tnear, best_t, 0, &t, &u, &v);
+            if (p != 0xFFFFFFFFu && (best == 0xFFFFFFFFu || t < best_t)) {
+                best_t = t;
+                best = p;
+                best_inst = k;
+                bu = u;
+                bv = v;
+            }
+        }
+        if (c->any) {
+            c->occluded[i] = (uint32_t)occ;
+        } else {
+            c->hits.t[i] = best_t;
+            c->hits.u[i] = bu;
+            c->hits.v[i] = bv;
+            c->hits.prim[i] = best;
+            c->inst_out[i] = best_inst;
+        }
+    }
+    free(stack);
+}
+
+void orc_inst_closest(const orc_inst* insts, uint32_t n_inst, const orc_rays* rays, uint32_t n, const orc_hits* hits,
+                      uint32_t* inst_out, int threads)
+{
+    inst_ctx c = {insts, n_inst, *rays, *hits, inst_out, NULL, 0};
+    orc_parallel_for(inst_range, &c, n, threads);
+}
+
+void orc_inst_occluded(const orc_inst* insts, uint32_t n_inst, const orc_rays* rays, uint32_t n, uint32_t* occluded,
+                       int threads)
+{
+    inst_ctx c = {insts, n_inst, *rays, {0}, NULL, occluded, 1};
+    orc_parallel_for(inst_range, &c, n, threads);
+}

@@ -19,6 +19,7 @@
 #include "sgpu_internal.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>  // rocprim/iterator/texture_cache_iterator.hpp uses memset without including it
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
@@ -497,6 +498,102 @@ int build_wide(sgpu_scene scene, const BvhNode* nodes, uint32_t n_int, hipStream
 
 }  // namespace
 
+namespace {
+
+float host_ord2f(uint32_t u)
+{
+    const uint32_t b = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+    float f;
+    std::memcpy(&f, &b, sizeof(f));
+    return f;
+}
+
+uint32_t host_f2ord(float f)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, sizeof(u));
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// The binary tree over n primitive boxes (d_blo, d_bhi, centroids d_cent, bounds as k_prep
+// leaves them): Morton codes, radix sort, Karras hierarchy, bottom-up fit. On success *nodes_out
+// holds max(n - 1, 1) nodes, d_order the primitive of each leaf in leaf order and *depth the
+// deepest leaf.
+int build_binary(uint32_t n, const float4* d_blo, const float4* d_bhi, const float4* d_cent, const uint32_t* d_bounds,
+                 hipStream_t stream, BvhNode** nodes_out, DevBuf<uint32_t>& d_order, uint32_t* depth_out)
+{
+    const uint32_t n_int = n > 1 ? n - 1 : 1;
+    BvhNode* nodes = nullptr;
+    SGPU_HIP_TRY(hipMalloc((void**)&nodes, (size_t)n_int * sizeof(BvhNode)));
+    auto fail_free = [&](int code, const std::string& msg) {
+        (void)hipFree(nodes);
+        return sgpu_fail(code, msg);
+    };
+#define SGPU_BUILD_TRY(expr)                                                                  \
+    do {                                                                                      \
+        hipError_t _e = (expr);                                                               \
+        if (_e != hipSuccess) return fail_free(SGPU_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+    SGPU_BUILD_TRY(d_order.alloc(n));
+    if (n == 1) {
+        hipLaunchKernelGGL(k_single, dim3(1), dim3(1), 0, stream, d_blo, d_bhi, d_bounds, nodes);
+        SGPU_BUILD_TRY(hipGetLastError());
+        SGPU_BUILD_TRY(hipMemsetAsync(d_order.p, 0, sizeof(uint32_t), stream));
+        SGPU_BUILD_TRY(hipStreamSynchronize(stream));
+        *nodes_out = nodes;
+        *depth_out = 1;
+        return SGPU_OK;
+    }
+    DevBuf<uint32_t> d_vals, d_flags, d_depth;
+    DevBuf<float4> d_ilo, d_ihi;
+    DevBuf<uint64_t> d_keys, d_keys2;
+    DevBuf<int2> d_child;
+    DevBuf<int> d_pint, d_pleaf;
+    DevBuf<uint8_t> d_tmp;
+    SGPU_BUILD_TRY(d_keys.alloc(n));
+    SGPU_BUILD_TRY(d_keys2.alloc(n));
+    SGPU_BUILD_TRY(d_vals.alloc(n));
+    SGPU_BUILD_TRY(d_child.alloc(n_int));
+    SGPU_BUILD_TRY(d_pint.alloc(n_int));
+    SGPU_BUILD_TRY(d_pleaf.alloc(n));
+    SGPU_BUILD_TRY(d_flags.alloc(n_int));
+    SGPU_BUILD_TRY(d_ilo.alloc(n_int));
+    SGPU_BUILD_TRY(d_ihi.alloc(n_int));
+    SGPU_BUILD_TRY(d_depth.alloc(1));
+    SGPU_BUILD_TRY(hipMemsetAsync(d_flags.p, 0, (size_t)n_int * sizeof(uint32_t), stream));
+    SGPU_BUILD_TRY(hipMemsetAsync(d_pint.p, 0xFF, (size_t)n_int * sizeof(int), stream));
+    SGPU_BUILD_TRY(hipMemsetAsync(d_pleaf.p, 0xFF, (size_t)n * sizeof(int), stream));
+    SGPU_BUILD_TRY(hipMemsetAsync(d_depth.p, 0, sizeof(uint32_t), stream));
+
+    const uint32_t grid = (n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_morton, dim3(grid), dim3(kBlock), 0, stream, d_cent, n, d_bounds, d_keys.p, d_vals.p);
+    SGPU_BUILD_TRY(hipGetLastError());
+    size_t tmp_bytes = 0;
+    SGPU_BUILD_TRY(rocprim::radix_sort_pairs(nullptr, tmp_bytes, d_keys.p, d_keys2.p, d_vals.p, d_order.p,
+                                             (size_t)n, 0, 63, stream));
+    SGPU_BUILD_TRY(d_tmp.alloc(tmp_bytes));
+    SGPU_BUILD_TRY(rocprim::radix_sort_pairs(d_tmp.p, tmp_bytes, d_keys.p, d_keys2.p, d_vals.p, d_order.p,
+                                             (size_t)n, 0, 63, stream));
+    const uint32_t grid_int = (n - 1 + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_karras, dim3(grid_int), dim3(kBlock), 0, stream, d_keys2.p, (int)n, d_child.p, d_pint.p,
+                       d_pleaf.p);
+    SGPU_BUILD_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_fit, dim3(grid), dim3(kBlock), 0, stream, (int)n, d_child.p, d_pint.p, d_pleaf.p, d_blo, d_bhi,
+                       d_order.p, d_bounds, d_ilo.p, d_ihi.p, d_flags.p, nodes);
+    SGPU_BUILD_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_depth, dim3(grid), dim3(kBlock), 0, stream, (int)n, d_pint.p, d_pleaf.p, d_depth.p);
+    SGPU_BUILD_TRY(hipGetLastError());
+    uint32_t depth = 0;
+    SGPU_BUILD_TRY(hipMemcpyAsync(&depth, d_depth.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    SGPU_BUILD_TRY(hipStreamSynchronize(stream));
+#undef SGPU_BUILD_TRY
+    *nodes_out = nodes;
+    *depth_out = depth;
+    return SGPU_OK;
+}
+
+}  // namespace
+
 int sgpu_bvh_build(sgpu_scene scene, hipStream_t stream)
 {
     const uint32_t n = (uint32_t)scene->tri_geom.size();
@@ -507,15 +604,9 @@ int sgpu_bvh_build(sgpu_scene scene, hipStream_t stream)
     if (n > (1u << 30)) return sgpu_fail(SGPU_E_RANGE, "scene has more than 2^30 triangles");
 
     DevBuf<float> d_pos;
-    DevBuf<uint32_t> d_geom, d_first, d_bounds, d_vals, d_vals2, d_flags, d_depth;
+    DevBuf<uint32_t> d_geom, d_first, d_bounds, d_order;
     DevBuf<TriRecord> d_rec;
-    DevBuf<float4> d_blo, d_bhi, d_cent, d_ilo, d_ihi;
-    DevBuf<uint64_t> d_keys, d_keys2;
-    DevBuf<int2> d_child;
-    DevBuf<int> d_pint, d_pleaf;
-    DevBuf<uint8_t> d_tmp;
-
-    const uint32_t n_int = n > 1 ? n - 1 : 1;
+    DevBuf<float4> d_blo, d_bhi, d_cent;
     SGPU_HIP_TRY(d_pos.alloc((size_t)n * 9));
     SGPU_HIP_TRY(d_geom.alloc(n));
     SGPU_HIP_TRY(d_first.alloc(scene->geom_first.size()));
@@ -524,104 +615,231 @@ int sgpu_bvh_build(sgpu_scene scene, hipStream_t stream)
     SGPU_HIP_TRY(d_blo.alloc(n));
     SGPU_HIP_TRY(d_bhi.alloc(n));
     SGPU_HIP_TRY(d_cent.alloc(n));
-    SGPU_HIP_TRY(d_keys.alloc(n));
-    SGPU_HIP_TRY(d_keys2.alloc(n));
-    SGPU_HIP_TRY(d_vals.alloc(n));
-    SGPU_HIP_TRY(d_vals2.alloc(n));
-    SGPU_HIP_TRY(d_child.alloc(n_int));
-    SGPU_HIP_TRY(d_pint.alloc(n_int));
-    SGPU_HIP_TRY(d_pleaf.alloc(n));
-    SGPU_HIP_TRY(d_flags.alloc(n_int));
-    SGPU_HIP_TRY(d_ilo.alloc(n_int));
-    SGPU_HIP_TRY(d_ihi.alloc(n_int));
-    SGPU_HIP_TRY(d_depth.alloc(1));
+
+    SGPU_HIP_TRY(hipMemcpyAsync(d_pos.p, scene->tri_pos.data(), (size_t)n * 9 * sizeof(float), hipMemcpyHostToDevice,
+                                stream));
+    SGPU_HIP_TRY(hipMemcpyAsync(d_geom.p, scene->tri_geom.data(), (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                stream));
+    // device copy of the geometries' first indices with bit 31 set for quad geometries
+    std::vector<uint32_t> first_flagged(scene->geom_first.size());
+    for (size_t g = 0; g < first_flagged.size(); ++g)
+        first_flagged[g] = scene->geom_first[g] | (scene->geom_quad[g] ? 0x80000000u : 0u);
+    SGPU_HIP_TRY(hipMemcpy(d_first.p, first_flagged.data(), first_flagged.size() * sizeof(uint32_t),
+                           hipMemcpyHostToDevice));
+    SGPU_HIP_TRY(hipMemsetAsync(d_bounds.p, 0xFF, 12 * sizeof(uint32_t), stream));
+    const uint32_t grid = (n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_prep, dim3(grid), dim3(kBlock), 0, stream, d_pos.p, n, d_geom.p, d_first.p, d_rec.p, d_blo.p,
+                       d_bhi.p, d_cent.p, d_bounds.p);
+    SGPU_HIP_TRY(hipGetLastError());
+    uint32_t hb[12];
+    SGPU_HIP_TRY(hipMemcpyAsync(hb, d_bounds.p, sizeof(hb), hipMemcpyDeviceToHost, stream));
 
     BvhNode* nodes = nullptr;
+    uint32_t depth = 0;
+    int rc = build_binary(n, d_blo.p, d_bhi.p, d_cent.p, d_bounds.p, stream, &nodes, d_order, &depth);
+    if (rc != SGPU_OK) return rc;
+    for (int k = 0; k < 3; ++k) {
+        scene->box[k] = host_ord2f(hb[k]);
+        scene->box[3 + k] = host_ord2f(~hb[3 + k]);
+    }
     TriRecord* tris = nullptr;
-    SGPU_HIP_TRY(hipMalloc((void**)&nodes, (size_t)n_int * sizeof(BvhNode)));
     hipError_t e = hipMalloc((void**)&tris, (size_t)n * sizeof(TriRecord));
     if (e != hipSuccess) {
         (void)hipFree(nodes);
         return sgpu_fail(SGPU_E_NOMEM, "hipMalloc of triangle records failed");
     }
-    auto fail_free = [&](int code, const std::string& msg) {
+    hipLaunchKernelGGL(k_reorder, dim3(grid), dim3(kBlock), 0, stream, d_rec.p, d_order.p, n, tris);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) {
         (void)hipFree(nodes);
         (void)hipFree(tris);
-        return sgpu_fail(code, msg);
-    };
-#define SGPU_BUILD_TRY(expr)                                                                  \
-    do {                                                                                      \
-        hipError_t _e = (expr);                                                               \
-        if (_e != hipSuccess) return fail_free(SGPU_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
-    } while (0)
-
-    SGPU_BUILD_TRY(hipMemcpyAsync(d_pos.p, scene->tri_pos.data(), (size_t)n * 9 * sizeof(float),
-                                  hipMemcpyHostToDevice, stream));
-    SGPU_BUILD_TRY(hipMemcpyAsync(d_geom.p, scene->tri_geom.data(), (size_t)n * sizeof(uint32_t),
-                                  hipMemcpyHostToDevice, stream));
-    // device copy of the geometries' first indices with bit 31 set for quad geometries
-    std::vector<uint32_t> first_flagged(scene->geom_first.size());
-    for (size_t g = 0; g < first_flagged.size(); ++g)
-        first_flagged[g] = scene->geom_first[g] | (scene->geom_quad[g] ? 0x80000000u : 0u);
-    SGPU_BUILD_TRY(hipMemcpy(d_first.p, first_flagged.data(), first_flagged.size() * sizeof(uint32_t),
-                             hipMemcpyHostToDevice));
-    SGPU_BUILD_TRY(hipMemsetAsync(d_bounds.p, 0xFF, 12 * sizeof(uint32_t), stream));
-    SGPU_BUILD_TRY(hipMemsetAsync(d_flags.p, 0, (size_t)n_int * sizeof(uint32_t), stream));
-    SGPU_BUILD_TRY(hipMemsetAsync(d_pint.p, 0xFF, (size_t)n_int * sizeof(int), stream));
-    SGPU_BUILD_TRY(hipMemsetAsync(d_pleaf.p, 0xFF, (size_t)n * sizeof(int), stream));
-    SGPU_BUILD_TRY(hipMemsetAsync(d_depth.p, 0, sizeof(uint32_t), stream));
-
-    const uint32_t grid = (n + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_prep, dim3(grid), dim3(kBlock), 0, stream, d_pos.p, n, d_geom.p, d_first.p, d_rec.p,
-                       d_blo.p, d_bhi.p, d_cent.p, d_bounds.p);
-    SGPU_BUILD_TRY(hipGetLastError());
-
-    if (n == 1) {
-        hipLaunchKernelGGL(k_single, dim3(1), dim3(1), 0, stream, d_blo.p, d_bhi.p, d_bounds.p, nodes);
-        SGPU_BUILD_TRY(hipGetLastError());
-        SGPU_BUILD_TRY(hipMemcpyAsync(tris, d_rec.p, sizeof(TriRecord), hipMemcpyDeviceToDevice, stream));
-        SGPU_BUILD_TRY(hipStreamSynchronize(stream));
-        scene->d_nodes = nodes;
-        scene->d_tris = tris;
-        scene->n_nodes = 1;
-        scene->max_depth = 1;
-        int rc = build_wide(scene, nodes, 1, stream);
-        scene->stack_cap = std::max(scene->max_depth + 1, kWideMaxPush * scene->wide_depth + 1);
-        return rc;
+        return sgpu_fail(SGPU_E_HIP, std::string("triangle reorder: ") + hipGetErrorString(e));
     }
-
-    hipLaunchKernelGGL(k_morton, dim3(grid), dim3(kBlock), 0, stream, d_cent.p, n, d_bounds.p, d_keys.p, d_vals.p);
-    SGPU_BUILD_TRY(hipGetLastError());
-
-    size_t tmp_bytes = 0;
-    SGPU_BUILD_TRY(rocprim::radix_sort_pairs(nullptr, tmp_bytes, d_keys.p, d_keys2.p, d_vals.p, d_vals2.p,
-                                             (size_t)n, 0, 63, stream));
-    SGPU_BUILD_TRY(d_tmp.alloc(tmp_bytes));
-    SGPU_BUILD_TRY(rocprim::radix_sort_pairs(d_tmp.p, tmp_bytes, d_keys.p, d_keys2.p, d_vals.p, d_vals2.p,
-                                             (size_t)n, 0, 63, stream));
-
-    const uint32_t grid_int = (n - 1 + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_karras, dim3(grid_int), dim3(kBlock), 0, stream, d_keys2.p, (int)n, d_child.p, d_pint.p,
-                       d_pleaf.p);
-    SGPU_BUILD_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_fit, dim3(grid), dim3(kBlock), 0, stream, (int)n, d_child.p, d_pint.p, d_pleaf.p, d_blo.p,
-                       d_bhi.p, d_vals2.p, d_bounds.p, d_ilo.p, d_ihi.p, d_flags.p, nodes);
-    SGPU_BUILD_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_reorder, dim3(grid), dim3(kBlock), 0, stream, d_rec.p, d_vals2.p, n, tris);
-    SGPU_BUILD_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_depth, dim3(grid), dim3(kBlock), 0, stream, (int)n, d_pint.p, d_pleaf.p, d_depth.p);
-    SGPU_BUILD_TRY(hipGetLastError());
-
-    uint32_t depth = 0;
-    SGPU_BUILD_TRY(hipMemcpyAsync(&depth, d_depth.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-    SGPU_BUILD_TRY(hipStreamSynchronize(stream));
-#undef SGPU_BUILD_TRY
-
     scene->d_nodes = nodes;
     scene->d_tris = tris;
-    scene->n_nodes = n - 1;
+    scene->n_nodes = n > 1 ? n - 1 : 1;
     scene->max_depth = depth;
-    int rc = build_wide(scene, nodes, n - 1, stream);
+    rc = build_wide(scene, nodes, scene->n_nodes, stream);
     scene->stack_cap = std::max(scene->max_depth + 1, kWideMaxPush * scene->wide_depth + 1);
+    return rc;
+}
+
+namespace {
+
+// r = a * b for row-major 3x4 affine transforms (the implicit last row 0 0 0 1)
+void affine_mul(const double* a, const double* b, double* r)
+{
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 4; ++j) {
+            double v = a[4 * i + 0] * b[0 + j] + a[4 * i + 1] * b[4 + j] + a[4 * i + 2] * b[8 + j];
+            if (j == 3) v += a[4 * i + 3];
+            r[4 * i + j] = v;
+        }
+    }
+}
+
+// inverse of a row-major 3x4 affine transform; false when singular
+bool affine_inverse(const double* m, double* r)
+{
+    const double a = m[0], b = m[1], c = m[2], d = m[4], e = m[5], f = m[6], g = m[8], h = m[9], k = m[10];
+    const double A = e * k - f * h, B = -(d * k - f * g), C = d * h - e * g;
+    const double det = a * A + b * B + c * C;
+    if (!(std::fabs(det) > 0.0) || !std::isfinite(det)) return false;
+    const double id = 1.0 / det;
+    const double inv[9] = {A * id, -(b * k - c * h) * id, (b * f - c * e) * id,
+                           B * id, (a * k - c * g) * id,  -(a * f - c * d) * id,
+                           C * id, -(a * h - b * g) * id, (a * e - b * d) * id};
+    for (int i = 0; i < 3; ++i) {
+        r[4 * i + 0] = inv[3 * i + 0];
+        r[4 * i + 1] = inv[3 * i + 1];
+        r[4 * i + 2] = inv[3 * i + 2];
+        r[4 * i + 3] = -(inv[3 * i + 0] * m[3] + inv[3 * i + 1] * m[7] + inv[3 * i + 2] * m[11]);
+    }
+    return true;
+}
+
+}  // namespace
+
+int sgpu_bvh_build_instances(sgpu_scene scene, hipStream_t stream)
+{
+    // compose the levels: (instance i of a scene of geometries) or (instance i of an instanced
+    // scene, its instance j), in that order
+    struct Flat {
+        sgpu_scene blas;
+        double l2w[12];
+        uint32_t id0, id1;
+    };
+    std::vector<Flat> flat;
+    uint32_t levels = 1;
+    for (size_t i = 0; i < scene->instances.size(); ++i) {
+        const auto& in = scene->instances[i];
+        if (in.child->instances.empty()) {
+            Flat f{in.child, {}, (uint32_t)i, SGPU_INVALID_ID};
+            std::memcpy(f.l2w, in.l2w, sizeof(f.l2w));
+            flat.push_back(f);
+        } else {
+            levels = 2;
+            for (size_t j = 0; j < in.child->instances.size(); ++j) {
+                const auto& in2 = in.child->instances[j];
+                Flat f{in2.child, {}, (uint32_t)i, (uint32_t)j};
+                affine_mul(in.l2w, in2.l2w, f.l2w);
+                flat.push_back(f);
+            }
+        }
+    }
+    // scenes without triangles place nothing
+    flat.erase(std::remove_if(flat.begin(), flat.end(), [](const Flat& f) { return f.blas->n_tris == 0; }), flat.end());
+    if (flat.size() > (1u << 30)) return sgpu_fail(SGPU_E_RANGE, "scene has more than 2^30 composed instances");
+    const uint32_t n = (uint32_t)flat.size();
+    scene->n_tris = 0;
+    scene->n_nodes = 0;
+    scene->max_depth = 0;
+    scene->inst_levels = levels;
+    if (n == 0) return SGPU_OK;
+
+    // world boxes (the 8 corners of each placed scene's box through its transform, in double,
+    // rounded outward), centroids and the bounds k_morton and k_fit read
+    std::vector<float4> blo(n), bhi(n), cent(n);
+    std::vector<InstRecord> rec(n);
+    std::vector<uint32_t> ids(2 * (size_t)n);
+    float slo[3] = {INFINITY, INFINITY, INFINITY}, shi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    uint32_t blas_cap = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        const Flat& f = flat[k];
+        double w2l[12];
+        if (!affine_inverse(f.l2w, w2l)) return sgpu_fail(SGPU_E_INVALID, "instance transform is singular");
+        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int c = 0; c < 8; ++c) {
+            const double p[3] = {f.blas->box[(c & 1) ? 3 : 0], f.blas->box[(c & 2) ? 4 : 1], f.blas->box[(c & 4) ? 5 : 2]};
+            for (int r = 0; r < 3; ++r) {
+                const double w = f.l2w[4 * r] * p[0] + f.l2w[4 * r + 1] * p[1] + f.l2w[4 * r + 2] * p[2] + f.l2w[4 * r + 3];
+                lo[r] = std::min(lo[r], w);
+                hi[r] = std::max(hi[r], w);
+            }
+        }
+        float l[3], h[3];
+        for (int r = 0; r < 3; ++r) {
+            l[r] = std::nextafter((float)lo[r], -INFINITY);
+            h[r] = std::nextafter((float)hi[r], INFINITY);
+            if (!std::isfinite(l[r]) || !std::isfinite(h[r]))
+                return sgpu_fail(SGPU_E_RANGE, "instance box is not finite in single precision");
+            slo[r] = std::min(slo[r], l[r]);
+            shi[r] = std::max(shi[r], h[r]);
+            const float cc = 0.5f * (l[r] + h[r]);
+            clo[r] = std::min(clo[r], cc);
+            chi[r] = std::max(chi[r], cc);
+        }
+        blo[k] = make_float4(l[0], l[1], l[2], 0.0f);
+        bhi[k] = make_float4(h[0], h[1], h[2], 0.0f);
+        cent[k] = make_float4(0.5f * (l[0] + h[0]), 0.5f * (l[1] + h[1]), 0.5f * (l[2] + h[2]), 0.0f);
+        for (int r = 0; r < 3; ++r)
+            rec[k].w2l[r] = make_float4((float)w2l[4 * r], (float)w2l[4 * r + 1], (float)w2l[4 * r + 2], (float)w2l[4 * r + 3]);
+        rec[k].wide = f.blas->d_wide;
+        rec[k].tris = f.blas->d_tris;
+        ids[2 * k] = f.id0;
+        ids[2 * k + 1] = f.id1;
+        blas_cap = std::max(blas_cap, f.blas->stack_cap);
+    }
+    uint32_t hb[12];
+    for (int r = 0; r < 3; ++r) {
+        hb[r] = host_f2ord(slo[r]);
+        hb[3 + r] = ~host_f2ord(shi[r]);
+        hb[6 + r] = host_f2ord(clo[r]);
+        hb[9 + r] = ~host_f2ord(chi[r]);
+        scene->box[r] = slo[r];
+        scene->box[3 + r] = shi[r];
+    }
+    DevBuf<float4> d_blo, d_bhi, d_cent;
+    DevBuf<uint32_t> d_bounds, d_order;
+    SGPU_HIP_TRY(d_blo.alloc(n));
+    SGPU_HIP_TRY(d_bhi.alloc(n));
+    SGPU_HIP_TRY(d_cent.alloc(n));
+    SGPU_HIP_TRY(d_bounds.alloc(12));
+    SGPU_HIP_TRY(hipMemcpyAsync(d_blo.p, blo.data(), n * sizeof(float4), hipMemcpyHostToDevice, stream));
+    SGPU_HIP_TRY(hipMemcpyAsync(d_bhi.p, bhi.data(), n * sizeof(float4), hipMemcpyHostToDevice, stream));
+    SGPU_HIP_TRY(hipMemcpyAsync(d_cent.p, cent.data(), n * sizeof(float4), hipMemcpyHostToDevice, stream));
+    SGPU_HIP_TRY(hipMemcpyAsync(d_bounds.p, hb, sizeof(hb), hipMemcpyHostToDevice, stream));
+    BvhNode* nodes = nullptr;
+    uint32_t depth = 0;
+    int rc = build_binary(n, d_blo.p, d_bhi.p, d_cent.p, d_bounds.p, stream, &nodes, d_order, &depth);
+    if (rc != SGPU_OK) return rc;
+    // the top tree's leaves name composed instances by their composition index (instance i, then
+    // its instance j), so the closest-hit tie rule follows (inst_id0, inst_id1)
+    std::vector<uint32_t> order(n);
+    const uint32_t n_int = n > 1 ? n - 1 : 1;
+    std::vector<BvhNode> hn(n_int);
+    hipError_t e = hipMemcpy(order.data(), d_order.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(hn.data(), nodes, n_int * sizeof(BvhNode), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) {
+        for (BvhNode& nd : hn) {
+            if (nd.child.x < 0) nd.child.x = ~(int)order[~nd.child.x];
+            if (nd.child.y < 0) nd.child.y = ~(int)order[~nd.child.y];
+        }
+        e = hipMemcpy(nodes, hn.data(), n_int * sizeof(BvhNode), hipMemcpyHostToDevice);
+    }
+    InstRecord* d_inst = nullptr;
+    uint32_t* d_ids = nullptr;
+    if (e == hipSuccess) e = hipMalloc((void**)&d_inst, n * sizeof(InstRecord));
+    if (e == hipSuccess) e = hipMalloc((void**)&d_ids, 2 * (size_t)n * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemcpy(d_inst, rec.data(), n * sizeof(InstRecord), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_ids, ids.data(), 2 * (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(nodes);
+        if (d_inst) (void)hipFree(d_inst);
+        if (d_ids) (void)hipFree(d_ids);
+        return sgpu_fail(SGPU_E_HIP, std::string("instance records: ") + hipGetErrorString(e));
+    }
+    scene->d_nodes = nodes;
+    scene->d_inst = d_inst;
+    scene->d_inst_ids = d_ids;
+    scene->n_inst = n;
+    scene->n_nodes = n > 1 ? n - 1 : 1;
+    scene->max_depth = depth;
+    scene->blas_stack_cap = blas_cap;
+    rc = build_wide(scene, nodes, scene->n_nodes, stream);
+    // the top tree's stack, the exit entry, and the deepest placed tree's stack
+    scene->stack_cap = std::max(scene->max_depth + 1, kWideMaxPush * scene->wide_depth + 1) + 1 + blas_cap;
     return rc;
 }

@@ -265,6 +265,8 @@ int sgpu_scene_add_triangles(sgpu_scene scene, const float* positions, uint32_t 
         return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_add_triangles: NULL positions or indices");
     std::lock_guard<std::mutex> lock(scene->mu);
     if (scene->committed) return sgpu_fail(SGPU_E_STATE, "sgpu_scene_add_triangles: scene already committed");
+    if (!scene->instances.empty())
+        return sgpu_fail(SGPU_E_STATE, "sgpu_scene_add_triangles: the scene holds instances, not geometries");
     const uint64_t total = (uint64_t)scene->tri_geom.size() + n_tris;
     if (total > (1ull << 30)) return sgpu_fail(SGPU_E_RANGE, "sgpu_scene_add_triangles: more than 2^30 triangles");
     for (uint64_t t = 0; t < (uint64_t)n_tris * 3; ++t)
@@ -331,6 +333,40 @@ int sgpu_scene_add_quads(sgpu_scene scene, const float* positions, uint32_t n_ve
     return SGPU_OK;
 }
 
+int sgpu_scene_add_instance(sgpu_scene scene, sgpu_scene child, const float* colmajor4x4, uint32_t* inst_id_out)
+{
+    if (!scene || !child || !colmajor4x4) return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_add_instance: NULL argument");
+    if (child == scene) return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_add_instance: a scene cannot place itself");
+    if (child->ctx != scene->ctx)
+        return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_add_instance: the child belongs to another context");
+    for (int k = 0; k < 16; ++k)
+        if (!std::isfinite(colmajor4x4[k]))
+            return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_add_instance: transform is not finite");
+    std::lock(scene->mu, child->mu);
+    std::lock_guard<std::mutex> la(scene->mu, std::adopt_lock), lb(child->mu, std::adopt_lock);
+    if (scene->committed) return sgpu_fail(SGPU_E_STATE, "sgpu_scene_add_instance: scene already committed");
+    if (!child->committed) return sgpu_fail(SGPU_E_STATE, "sgpu_scene_add_instance: child scene not committed");
+    if (!scene->tri_geom.empty())
+        return sgpu_fail(SGPU_E_STATE, "sgpu_scene_add_instance: the scene holds geometries, not instances");
+    if (child->inst_levels > 1)
+        return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_add_instance: more than two instance levels");
+    if (scene->instances.size() >= (1u << 30))
+        return sgpu_fail(SGPU_E_RANGE, "sgpu_scene_add_instance: more than 2^30 instances");
+    sgpu_scene_s::Instance in;
+    in.child = child;
+    // column-major 4x4 -> row-major 3x4 (Embree's RTC_FORMAT_FLOAT4X4_COLUMN_MAJOR)
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 4; ++c) in.l2w[4 * r + c] = (double)colmajor4x4[4 * c + r];
+    try {
+        scene->instances.push_back(in);
+    } catch (...) {
+        return sgpu_fail(SGPU_E_NOMEM, "sgpu_scene_add_instance: out of host memory");
+    }
+    ++child->refs;
+    if (inst_id_out) *inst_id_out = (uint32_t)(scene->instances.size() - 1);
+    return SGPU_OK;
+}
+
 int sgpu_scene_commit(sgpu_scene scene, void* stream)
 {
     if (!scene) return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_commit: NULL scene");
@@ -339,6 +375,18 @@ int sgpu_scene_commit(sgpu_scene scene, void* stream)
     SGPU_HIP_TRY(hipSetDevice(scene->ctx->device));
     SGPU_HIP_TRY(hipMalloc((void**)&scene->d_overflow, sizeof(uint32_t)));
     SGPU_HIP_TRY(hipMemset(scene->d_overflow, 0, sizeof(uint32_t)));
+    if (!scene->instances.empty()) {
+        int rc = sgpu_bvh_build_instances(scene, as_stream(stream));
+        if (rc != SGPU_OK) return rc;
+        if (scene->n_inst > 0) {
+            const float* b = scene->box;
+            const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+            for (int k = 0; k < 3; ++k) scene->bsphere[k] = (b[k] + b[3 + k]) * 0.5f;
+            scene->bsphere[3] = 0.5f * sqrtf(dx * dx + dy * dy + dz * dz);
+        }
+        scene->committed = true;
+        return SGPU_OK;
+    }
     int rc = sgpu_bvh_build(scene, as_stream(stream));
     if (rc != SGPU_OK) return rc;
     // the sphere around the triangles' bounding box (the IBL emits from its disc)
@@ -362,6 +410,17 @@ int sgpu_scene_commit(sgpu_scene scene, void* stream)
 int sgpu_scene_release(sgpu_scene scene)
 {
     if (!scene) return SGPU_OK;
+    {
+        std::lock_guard<std::mutex> lock(scene->mu);
+        if (scene->refs > 0)
+            return sgpu_fail(SGPU_E_STATE, "sgpu_scene_release: the scene is placed by an instance of another scene");
+    }
+    for (const auto& in : scene->instances) {
+        std::lock_guard<std::mutex> lock(in.child->mu);
+        --in.child->refs;
+    }
+    if (scene->d_inst) (void)hipFree(scene->d_inst);
+    if (scene->d_inst_ids) (void)hipFree(scene->d_inst_ids);
     if (scene->d_nodes) (void)hipFree(scene->d_nodes);
     if (scene->d_tris) (void)hipFree(scene->d_tris);
     if (scene->d_wide) (void)hipFree(scene->d_wide);
@@ -612,6 +671,8 @@ int sgpu_render_pt(sgpu_scene scene, const sgpu_pt_job* job, float* framebuffer,
 {
     if (!scene || !job || !framebuffer) return sgpu_fail(SGPU_E_INVALID, "sgpu_render_pt: NULL argument");
     if (!scene->committed) return sgpu_fail(SGPU_E_STATE, "sgpu_render_pt: scene not committed");
+    if (!scene->instances.empty())
+        return sgpu_fail(SGPU_E_INVALID, "sgpu_render_pt: instanced scenes answer queries; render scenes of geometries");
     if (!scene->d_materials || scene->n_geom_material < scene->geom_first.size())
         return sgpu_fail(SGPU_E_STATE, "sgpu_render_pt: materials not set for every geometry");
     if (job->camera.width == 0 || job->camera.height == 0 || job->spp == 0 || job->max_path_length == 0)
@@ -628,6 +689,8 @@ static int check_vcm_job(sgpu_scene scene, const sgpu_vcm_job* job, const void* 
 {
     if (!scene || !job || !framebuffer) return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": NULL argument");
     if (!scene->committed) return sgpu_fail(SGPU_E_STATE, std::string(who) + ": scene not committed");
+    if (!scene->instances.empty())
+        return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": instanced scenes answer queries; render scenes of geometries");
     if (!scene->d_materials || scene->n_geom_material < scene->geom_first.size())
         return sgpu_fail(SGPU_E_STATE, std::string(who) + ": materials not set for every geometry");
     if (scene->n_geom_material >= (1u << 27))
@@ -757,7 +820,8 @@ int sgpu_scene_device_bytes(sgpu_scene scene, uint64_t* bytes)
                  sizeof(uint32_t) + (uint64_t)scene->n_materials * 22 * sizeof(float) +
                  (uint64_t)scene->n_geom_material * sizeof(uint32_t) + (uint64_t)scene->n_lights * sizeof(sgpu_light) +
                  (uint64_t)scene->n_geom_tex * sizeof(sgpu::PtexTex) +
-                 (uint64_t)scene->ibl_w * scene->ibl_h * 4 * sizeof(float) + (uint64_t)scene->ibl_h * sizeof(float);
+                 (uint64_t)scene->ibl_w * scene->ibl_h * 4 * sizeof(float) + (uint64_t)scene->ibl_h * sizeof(float) +
+                 (uint64_t)scene->n_inst * (sizeof(InstRecord) + 2 * sizeof(uint32_t));
     for (const sgpu::PtexTex& t : scene->tex_host)
         if (t.texels) b += (uint64_t)t.nfaces * (3ull << (2 * t.res_log2)) + (uint64_t)t.nfaces * 8 * sizeof(int32_t);
     *bytes = b;

@@ -53,6 +53,18 @@ struct WideNode {
 static_assert(sizeof(WideNode) == 128, "WideNode must be 128 bytes");
 constexpr uint32_t kWideMaxPush = 7;  // stack entries a wide node can push
 
+// One composed instance of an instanced scene as the traversal reads it (64 bytes): the rows of
+// its world-to-local affine transform (x' = w.x x + w.y y + w.z z + w.w) and the 8-wide tree and
+// triangle records of the scene of geometries it places.
+struct InstRecord {
+    float4 w2l[3];
+    const WideNode* wide;
+    const TriRecord* tris;
+};
+static_assert(sizeof(InstRecord) == 64, "InstRecord must be 64 bytes");
+// The traversal stack entry that returns from an instance's tree to the top tree.
+constexpr int kInstExit = (int)0x80000000;
+
 // Slots of the LDS part of the per-lane traversal stack; deeper entries spill to global memory.
 #define SGPU_STACK_LDS 16
 #define SGPU_TRACE_BLOCK 256
@@ -114,6 +126,21 @@ struct sgpu_scene_s {
     float ibl_rotation = 0.0f, ibl_exposure = 1.0f;
     // sphere around the triangles' bounding box (centre, radius), set at commit
     float bsphere[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    // bounding box of the committed content (lo xyz, hi xyz; empty: lo > hi)
+    float box[6] = {1.0f, 1.0f, 1.0f, -1.0f, -1.0f, -1.0f};
+    // instances (sgpu_scene_add_instance): children and local-to-world transforms (row-major 3x4)
+    // until commit; then the composed instances on the device, in top-tree leaf order
+    struct Instance {
+        sgpu_scene child;
+        double l2w[12];
+    };
+    std::vector<Instance> instances;
+    uint32_t refs = 0;                 // instances of other scenes that place this one
+    InstRecord* d_inst = nullptr;      // composed instances (traversal records)
+    uint32_t* d_inst_ids = nullptr;    // inst_id0, inst_id1 per composed instance
+    uint32_t n_inst = 0;
+    uint32_t inst_levels = 0;          // 1 or 2 for an instanced scene
+    uint32_t blas_stack_cap = 0;       // the largest stack of the placed scenes' trees
 };
 
 // ---- error plumbing ------------------------------------------------------------------------------
@@ -129,6 +156,9 @@ int sgpu_fail(int code, const std::string& msg);
 // ---- internal entry points between translation units --------------------------------------------
 // Builds d_nodes/d_tris (and the 8-wide d_wide) for the scene's triangles (sgpu_bvh.hip).
 int sgpu_bvh_build(sgpu_scene scene, hipStream_t stream);
+// Composes an instanced scene's instances and builds its top tree (d_nodes, d_wide over the
+// composed instances, d_inst, d_inst_ids) (sgpu_bvh.hip).
+int sgpu_bvh_build_instances(sgpu_scene scene, hipStream_t stream);
 // Launches the traversal kernels (sgpu_trace.hip), on the context's tree kind (trace_tree).
 // `spill` (may be NULL: then the call allocates its own, stream-ordered) holds the spilled part of
 // the per-lane stacks: at least sgpu_trace_spill_words(scene) 32-bit words, not shared with a

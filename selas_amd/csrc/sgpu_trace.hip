@@ -60,7 +60,8 @@ __device__ __forceinline__ float safe_rcp(float x)
 template <bool ANY>
 __device__ __forceinline__ void write_result(uint32_t i, bool hit_any, uint32_t best_tri, float best_t, float best_u,
                                              float best_v, const TriRecord* __restrict rt, sgpu_hits_soa& hits,
-                                             uint32_t* __restrict occluded)
+                                             uint32_t* __restrict occluded, uint32_t id0 = SGPU_INVALID_ID,
+                                             uint32_t id1 = SGPU_INVALID_ID)
 {
     if (ANY) {
         occluded[i] = hit_any ? 1u : 0u;
@@ -92,6 +93,8 @@ __device__ __forceinline__ void write_result(uint32_t i, bool hit_any, uint32_t 
         hits.ng_y[i] = ng.y;
         hits.ng_z[i] = ng.z;
     }
+    if (hits.inst_id0) hits.inst_id0[i] = found ? id0 : SGPU_INVALID_ID;
+    if (hits.inst_id1) hits.inst_id1[i] = found ? id1 : SGPU_INVALID_ID;
 }
 
 template <bool ANY, bool STATS, bool MULTI>
@@ -437,6 +440,235 @@ __global__ __launch_bounds__(SGPU_TRACE_BLOCK) __attribute__((amdgpu_waves_per_e
     }
 }
 
+// A ray into an instance's space: x' = w.x x + w.y y + w.z z (+ w.w for points), summed by fused
+// multiply-adds from the last term (the arithmetic the oracle restates).
+__device__ __forceinline__ f3 inst_point(const float4* w, f3 p)
+{
+    return mk3(fmaf(w[0].x, p.x, fmaf(w[0].y, p.y, fmaf(w[0].z, p.z, w[0].w))),
+               fmaf(w[1].x, p.x, fmaf(w[1].y, p.y, fmaf(w[1].z, p.z, w[1].w))),
+               fmaf(w[2].x, p.x, fmaf(w[2].y, p.y, fmaf(w[2].z, p.z, w[2].w))));
+}
+__device__ __forceinline__ f3 inst_vector(const float4* w, f3 v)
+{
+    return mk3(fmaf(w[0].x, v.x, fmaf(w[0].y, v.y, w[0].z * v.z)), fmaf(w[1].x, v.x, fmaf(w[1].y, v.y, w[1].z * v.z)),
+               fmaf(w[2].x, v.x, fmaf(w[2].y, v.y, w[2].z * v.z)));
+}
+
+// Traversal of an instanced scene: the top 8-wide tree over the composed instances, whose leaves
+// (~k: instance record k) are entered in entry-distance order like internal children; entering
+// one pushes kInstExit, moves the ray into the instance's space (t keeps its meaning: the
+// direction is not renormalized) and walks the instance's own 8-wide tree with triangles tested
+// inline; popping kInstExit restores the world ray. Composed instances are numbered in
+// composition order (instance i of the scene, then its instance j), so the closest hit is the
+// smallest t with ties to the smaller (composed instance, triangle) = (inst_id0, inst_id1, geomID,
+// primID).
+template <bool ANY, bool STATS>
+__global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace8i(const WideNode* __restrict top, uint32_t n_inst,
+                                                              const InstRecord* __restrict insts,
+                                                              const uint32_t* __restrict inst_ids, sgpu_rays_soa rays,
+                                                              sgpu_hits_soa hits, uint32_t* __restrict occluded,
+                                                              uint32_t n, uint32_t* __restrict spill, uint32_t stack_cap,
+                                                              unsigned long long* __restrict stats,
+                                                              uint32_t* __restrict overflow)
+{
+    __shared__ uint32_t lds_stack[SGPU_STACK_LDS][SGPU_TRACE_BLOCK];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t gtid = blockIdx.x * SGPU_TRACE_BLOCK + threadIdx.x;
+    const uint32_t stride = gridDim.x * SGPU_TRACE_BLOCK;
+    unsigned long long cnt_nodes = 0, cnt_tris = 0;
+    const float inf = __int_as_float(0x7f800000);
+
+    for (uint32_t i = gtid; i < n; i += stride) {
+        const f3 wo = mk3(rays.org_x[i], rays.org_y[i], rays.org_z[i]);
+        const f3 wd = mk3(rays.dir_x[i], rays.dir_y[i], rays.dir_z[i]);
+        const float tnear = rays.tnear[i];
+        const float tfar = rays.tfar[i];
+        f3 o = wo, d = wd;
+        f3 inv = mk3(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
+        f3 oinv = mk3(o.x * inv.x, o.y * inv.y, o.z * inv.z);
+        const WideNode* __restrict rw = top;
+        const TriRecord* __restrict rt = nullptr;
+        uint32_t cur = SGPU_INVALID_ID;  // the instance being walked; SGPU_INVALID_ID: the top tree
+
+        float best_t = tfar;
+        uint32_t best_key = 0xFFFFFFFFu, best_tri = 0xFFFFFFFFu, best_inst = 0xFFFFFFFFu;
+        float best_u = 0.0f, best_v = 0.0f;
+        bool hit_any = false;
+
+        uint32_t sp = 0;
+        bool overflowed = false;
+        auto push = [&](int x) {
+            if (sp >= stack_cap) {
+                overflowed = true;
+                return;
+            }
+            if (sp < SGPU_STACK_LDS) {
+                lds_stack[sp][lane] = (uint32_t)x;
+            } else {
+                const uint32_t slot = sp % SGPU_STACK_LDS;
+                spill[(size_t)(sp - SGPU_STACK_LDS) * stride + gtid] = lds_stack[slot][lane];
+                lds_stack[slot][lane] = (uint32_t)x;
+            }
+            ++sp;
+        };
+        auto pop = [&](int& x) -> bool {
+            if (sp == 0) return false;
+            --sp;
+            const uint32_t slot = sp % SGPU_STACK_LDS;
+            x = (int)lds_stack[slot][lane];
+            if (sp >= SGPU_STACK_LDS) lds_stack[slot][lane] = spill[(size_t)(sp - SGPU_STACK_LDS) * stride + gtid];
+            return true;
+        };
+        auto test_tri = [&](int ref) -> bool {  // true: any-hit found
+            const uint32_t j = (uint32_t)(~ref);
+            const TriRecord r = rt[j];
+            if (STATS) ++cnt_tris;
+            float t, u, v;
+            if (tri_test(r, o, d, tnear, t, u, v)) {
+                if (ANY) {
+                    if (t <= tfar) return true;
+                } else {
+                    const uint32_t key = __float_as_uint(r.a.w);
+                    if (t < best_t || (t == best_t && (cur < best_inst || (cur == best_inst && key < best_key)))) {
+                        best_t = t;
+                        best_key = key;
+                        best_tri = j;
+                        best_inst = cur;
+                        best_u = u;
+                        best_v = v;
+                    }
+                }
+            }
+            return false;
+        };
+
+        int node = 0;
+        bool walking = n_inst > 0;
+        while (walking && !overflowed) {
+            if (node == kInstExit) {
+                // back in the top tree
+                o = wo;
+                d = wd;
+                inv = mk3(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
+                oinv = mk3(o.x * inv.x, o.y * inv.y, o.z * inv.z);
+                rw = top;
+                cur = SGPU_INVALID_ID;
+                if (!pop(node)) break;
+                continue;
+            }
+            if (node < 0) {
+                // a leaf of the top tree: enter composed instance ~node
+                cur = (uint32_t)(~node);
+                const InstRecord ir = insts[cur];
+                o = inst_point(ir.w2l, wo);
+                d = inst_vector(ir.w2l, wd);
+                inv = mk3(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
+                oinv = mk3(o.x * inv.x, o.y * inv.y, o.z * inv.z);
+                rw = ir.wide;
+                rt = ir.tris;
+                push(kInstExit);
+                node = 0;
+                continue;
+            }
+            const uint4* __restrict np = reinterpret_cast<const uint4*>(rw + node);
+            const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4], w5 = np[5];
+            if (STATS) ++cnt_nodes;
+            const float ox = __uint_as_float(w0.x), oy = __uint_as_float(w0.y), oz = __uint_as_float(w0.z);
+            const uint32_t meta = w0.w;
+            const float sx = __uint_as_float((meta & 0xFFu) << 23);
+            const float sy = __uint_as_float(((meta >> 8) & 0xFFu) << 23);
+            const float sz = __uint_as_float(((meta >> 16) & 0xFFu) << 23);
+            const uint32_t nch = meta >> 24;
+            const uint32_t lox[2] = {w1.x, w1.y}, loy[2] = {w1.z, w1.w}, loz[2] = {w2.x, w2.y};
+            const uint32_t hix[2] = {w2.z, w2.w}, hiy[2] = {w3.x, w3.y}, hiz[2] = {w3.z, w3.w};
+            int ref[8] = {(int)w4.x, (int)w4.y, (int)w4.z, (int)w4.w, (int)w5.x, (int)w5.y, (int)w5.z, (int)w5.w};
+            const bool in_top = cur == SGPU_INVALID_ID;
+            float key[8];
+            uint32_t leaves = 0;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int sh = 8 * (c & 3), wi = c >> 2;
+                const float bx0 = fmaf((float)((lox[wi] >> sh) & 0xFFu), sx, ox);
+                const float bx1 = fmaf((float)((hix[wi] >> sh) & 0xFFu), sx, ox);
+                const float by0 = fmaf((float)((loy[wi] >> sh) & 0xFFu), sy, oy);
+                const float by1 = fmaf((float)((hiy[wi] >> sh) & 0xFFu), sy, oy);
+                const float bz0 = fmaf((float)((loz[wi] >> sh) & 0xFFu), sz, oz);
+                const float bz1 = fmaf((float)((hiz[wi] >> sh) & 0xFFu), sz, oz);
+                const float l_x = fmaf(bx0, inv.x, -oinv.x), h_x = fmaf(bx1, inv.x, -oinv.x);
+                const float l_y = fmaf(by0, inv.y, -oinv.y), h_y = fmaf(by1, inv.y, -oinv.y);
+                const float l_z = fmaf(bz0, inv.z, -oinv.z), h_z = fmaf(bz1, inv.z, -oinv.z);
+                const float tmin = fmaxf(fmaxf(fminf(l_x, h_x), fminf(l_y, h_y)), fmaxf(fminf(l_z, h_z), tnear));
+                const float tmax = fminf(fminf(fmaxf(l_x, h_x), fmaxf(l_y, h_y)), fminf(fmaxf(l_z, h_z), best_t));
+                const bool hit = (uint32_t)c < nch && tmin <= tmax;
+                // the top tree's instance leaves are visited like internal children
+                const bool visit = hit && (ref[c] >= 0 || in_top);
+                key[c] = visit ? tmin : inf;
+                if (hit && ref[c] < 0 && !in_top) leaves |= 1u << c;
+            }
+            bool found = false;
+            while (leaves) {
+                const int c = __builtin_ctz(leaves);
+                leaves &= leaves - 1;
+                int r = ref[0];
+#pragma unroll
+                for (int k = 1; k < 8; ++k) r = c == k ? ref[k] : r;
+                if (test_tri(r)) {
+                    found = true;
+                    break;
+                }
+            }
+            if (ANY && found) {
+                hit_any = true;
+                break;
+            }
+            if (!ANY) {
+#pragma unroll
+                for (int c = 0; c < 8; ++c) key[c] = key[c] <= best_t ? key[c] : inf;
+                cswap(key[0], ref[0], key[1], ref[1]); cswap(key[2], ref[2], key[3], ref[3]);
+                cswap(key[4], ref[4], key[5], ref[5]); cswap(key[6], ref[6], key[7], ref[7]);
+                cswap(key[0], ref[0], key[2], ref[2]); cswap(key[1], ref[1], key[3], ref[3]);
+                cswap(key[4], ref[4], key[6], ref[6]); cswap(key[5], ref[5], key[7], ref[7]);
+                cswap(key[1], ref[1], key[2], ref[2]); cswap(key[5], ref[5], key[6], ref[6]);
+                cswap(key[0], ref[0], key[4], ref[4]); cswap(key[1], ref[1], key[5], ref[5]);
+                cswap(key[2], ref[2], key[6], ref[6]); cswap(key[3], ref[3], key[7], ref[7]);
+                cswap(key[2], ref[2], key[4], ref[4]); cswap(key[3], ref[3], key[5], ref[5]);
+                cswap(key[1], ref[1], key[2], ref[2]); cswap(key[3], ref[3], key[4], ref[4]);
+                cswap(key[5], ref[5], key[6], ref[6]);
+            }
+            // the nearest is entered, the others pushed far to near
+            bool have = false;
+            int next = 0;
+#pragma unroll
+            for (int c = 7; c >= 0; --c) {
+                if (key[c] != inf) {
+                    if (have) push(next);
+                    next = ref[c];
+                    have = true;
+                }
+            }
+            if (have) {
+                node = next;
+                continue;
+            }
+            if (!pop(node)) break;
+        }
+        if (overflowed) atomicOr(overflow, 1u);
+
+        uint32_t id0 = SGPU_INVALID_ID, id1 = SGPU_INVALID_ID;
+        const TriRecord* __restrict bt = nullptr;
+        if (!ANY && best_tri != 0xFFFFFFFFu) {
+            bt = insts[best_inst].tris;
+            id0 = inst_ids[2 * best_inst];
+            id1 = inst_ids[2 * best_inst + 1];
+        }
+        write_result<ANY>(i, hit_any, best_tri, best_t, best_u, best_v, bt, hits, occluded, id0, id1);
+    }
+    if (STATS) {
+        atomicAdd(&stats[0], cnt_nodes);
+        atomicAdd(&stats[1], cnt_tris);
+    }
+}
+
 }  // namespace
 
 
@@ -524,10 +756,52 @@ size_t sgpu_trace_spill_words(sgpu_context ctx, uint32_t stack_cap)
 
 size_t sgpu_trace_spill_words(sgpu_scene scene) { return sgpu_trace_spill_words(scene->ctx, scene->stack_cap); }
 
+namespace {
+
+// One traversal launch over the rays of an instanced scene (k_trace8i).
+int trace_launch_instanced(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits, uint32_t* occluded,
+                           uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream, uint32_t* overflow)
+{
+    if (n == 0) return SGPU_OK;
+    sgpu_context ctx = scene->ctx;
+    const uint32_t cus = (uint32_t)ctx->props.multiProcessorCount;
+    uint32_t blocks = (n + SGPU_TRACE_BLOCK - 1) / SGPU_TRACE_BLOCK;
+    const uint32_t max_blocks = cus * std::min(ctx->trace_blocks_per_cu, kMaxTraceBlocksPerCu);
+    if (blocks > max_blocks) blocks = max_blocks;
+    uint32_t* spill = nullptr;
+    SGPU_HIP_TRY(hipMallocAsync((void**)&spill, sgpu_trace_spill_words(ctx, scene->stack_cap) * sizeof(uint32_t),
+                                stream));
+    sgpu_hits_soa h = {};
+    if (hits) h = *hits;
+    unsigned long long* st = (unsigned long long*)d_stats;
+    const dim3 g(blocks), b(SGPU_TRACE_BLOCK);
+#define SGPU_LAUNCH_I(A, S)                                                                                        \
+    hipLaunchKernelGGL((k_trace8i<A, S>), g, b, 0, stream, scene->d_wide, scene->n_inst, scene->d_inst,          \
+                       scene->d_inst_ids, *rays, h, occluded, n, spill, scene->stack_cap, st, overflow)
+    if (any_hit) {
+        if (st) SGPU_LAUNCH_I(true, true);
+        else SGPU_LAUNCH_I(true, false);
+    } else {
+        if (st) SGPU_LAUNCH_I(false, true);
+        else SGPU_LAUNCH_I(false, false);
+    }
+#undef SGPU_LAUNCH_I
+    SGPU_HIP_TRY(hipGetLastError());
+    SGPU_HIP_TRY(hipFreeAsync(spill, stream));
+    return SGPU_OK;
+}
+
+}  // namespace
+
 int sgpu_trace_launch(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits, uint32_t* occluded,
                       uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream, uint32_t* spill_buf,
                       const uint32_t* n_ptr, uint32_t* overflow)
 {
+    if (!scene->instances.empty()) {
+        if (n_ptr) return sgpu_fail(SGPU_E_INVALID, "instanced scenes take host ray counts");
+        return trace_launch_instanced(scene, rays, hits, occluded, n, any_hit, d_stats, stream,
+                                      overflow ? overflow : scene->d_overflow);
+    }
     return trace_launch(scene->ctx, scene->d_nodes, scene->d_wide, scene->n_wide, scene->d_tris, scene->n_tris,
                         scene->stack_cap, overflow ? overflow : scene->d_overflow, nullptr, nullptr, rays, hits,
                         occluded, n, any_hit, d_stats, stream, spill_buf, n_ptr);

@@ -34,7 +34,8 @@ class RaysSoA(C.Structure):
 
 
 class HitsSoA(C.Structure):
-    _fields_ = [(n, C.c_void_p) for n in ("t", "u", "v", "prim_id", "geom_id", "ng_x", "ng_y", "ng_z")]
+    _fields_ = [(n, C.c_void_p) for n in ("t", "u", "v", "prim_id", "geom_id", "ng_x", "ng_y", "ng_z", "inst_id0",
+                                          "inst_id1")]
 
 
 class TraceStats(C.Structure):
@@ -74,6 +75,7 @@ _SIGNATURES = {
     "sgpu_scene_add_quads": [_P, _FP, _U32, _U32P, _U32, _FP, _U32P],
     "sgpu_scene_set_ptex": [_P, _U32, _U32, _U32, _P, _P, _P],
     "sgpu_ptex_eval": [_P, _U32, _P, _P, _P, _U32, _P, _P],
+    "sgpu_scene_add_instance": [_P, _P, _FP, _U32P],
     "sgpu_scene_commit": [_P, _P],
     "sgpu_scene_release": [_P],
     "sgpu_scene_info": [_P, _U32P, _U32P, _U32P],
@@ -271,14 +273,15 @@ class DeviceRays:
 
 class DeviceHits:
     FIELDS = (("t", np.float32), ("u", np.float32), ("v", np.float32), ("prim_id", np.uint32),
-              ("geom_id", np.uint32), ("ng_x", np.float32), ("ng_y", np.float32), ("ng_z", np.float32))
+              ("geom_id", np.uint32), ("ng_x", np.float32), ("ng_y", np.float32), ("ng_z", np.float32),
+              ("inst_id0", np.uint32), ("inst_id1", np.uint32))
 
-    def __init__(self, ctx: Context, n: int, with_ng=True):
+    def __init__(self, ctx: Context, n: int, with_ng=True, with_inst=False):
         self.n = n
         self.bufs = {}
         ptrs = []
         for name, dt in self.FIELDS:
-            if name.startswith("ng_") and not with_ng:
+            if (name.startswith("ng_") and not with_ng) or (name.startswith("inst_") and not with_inst):
                 ptrs.append(None)
                 continue
             b = ctx.empty(4 * n)
@@ -349,6 +352,15 @@ class Scene:
         finally:
             for b in bufs:
                 b.free()
+
+    def add_instance(self, child: "Scene", local_to_world=None) -> int:
+        """Instance of the committed scene `child` (sgpu_scene_add_instance); local_to_world is a
+        4 x 4 matrix acting on column vectors (default: identity). Returns the instance id."""
+        m = np.eye(4, dtype=np.float64) if local_to_world is None else np.asarray(local_to_world, np.float64)
+        colmajor = np.ascontiguousarray(m.T.reshape(-1), np.float32)
+        iid = C.c_uint32()
+        check(lib.sgpu_scene_add_instance(self.handle, child.handle, fptr(colmajor), C.byref(iid)))
+        return iid.value
 
     def add_soup(self, tri9: np.ndarray) -> int:
         tri9 = np.ascontiguousarray(tri9, np.float32).reshape(-1, 9)
