@@ -234,7 +234,7 @@ class _Scene(C.Structure):
     _fields_ = [("bvh", C.c_void_p), ("tri9", C.c_void_p), ("tri_geom", C.c_void_p), ("materials", C.c_void_p),
                 ("geom_material", C.c_void_p), ("n_geoms", C.c_uint32), ("lights", C.c_void_p),
                 ("n_lights", C.c_uint32), ("background", C.c_float * 3), ("geom_first", C.c_void_p),
-                ("geom_tex", C.c_void_p), ("ibl", C.c_void_p)]
+                ("geom_tex", C.c_void_p), ("ibl", C.c_void_p), ("inst", C.c_void_p)]
 
 
 # ---- image-based light (orc_ibl.c) ---------------------------------------------------------------
@@ -507,3 +507,166 @@ class InstancedScene:
         occ = np.zeros(n, np.uint32)
         lib.orc_inst_occluded(C.cast(self.arr, C.c_void_p), self.n, C.byref(rs), n, occ.ctypes.data, _threads(threads))
         return occ
+
+
+# ---- instanced render scenes ---------------------------------------------------------------------
+# The composition of sgpu_scene_add_instance's levels as the library performs it: row-major 3 x 4
+# affine transforms from column-major float input, products and inverses in double with the same
+# operations in the same order (Python floats are IEEE doubles), rounded once to float.
+
+def rows_from_colmajor(m_colmajor):
+    """Column-major 4 x 4 floats -> row-major 3 x 4 doubles (l2w[4 r + c] = m[4 c + r])."""
+    m = np.asarray(m_colmajor, np.float32).reshape(-1)
+    return [float(m[4 * c + r]) for r in range(3) for c in range(4)]
+
+
+def affine_mul(a, b):
+    r = [0.0] * 12
+    for i in range(3):
+        for j in range(4):
+            v = a[4 * i + 0] * b[0 + j] + a[4 * i + 1] * b[4 + j] + a[4 * i + 2] * b[8 + j]
+            if j == 3:
+                v += a[4 * i + 3]
+            r[4 * i + j] = v
+    return r
+
+
+def affine_inverse(m):
+    a, b, c, d, e, f, g, h, k = m[0], m[1], m[2], m[4], m[5], m[6], m[8], m[9], m[10]
+    A = e * k - f * h
+    B = -(d * k - f * g)
+    Cc = d * h - e * g
+    det = a * A + b * B + c * Cc
+    idet = 1.0 / det
+    inv = [A * idet, -(b * k - c * h) * idet, (b * f - c * e) * idet,
+           B * idet, (a * k - c * g) * idet, -(a * f - c * d) * idet,
+           Cc * idet, -(a * h - b * g) * idet, (a * e - b * d) * idet]
+    r = [0.0] * 12
+    for i in range(3):
+        r[4 * i + 0] = inv[3 * i + 0]
+        r[4 * i + 1] = inv[3 * i + 1]
+        r[4 * i + 2] = inv[3 * i + 2]
+        r[4 * i + 3] = -(inv[3 * i + 0] * m[3] + inv[3 * i + 1] * m[7] + inv[3 * i + 2] * m[11])
+    return r
+
+
+def colmajor(m4):
+    """A 4 x 4 matrix acting on column vectors -> column-major float32 (what the C ABI takes)."""
+    return np.ascontiguousarray(np.asarray(m4, np.float64).T.reshape(-1), np.float32)
+
+
+def compose_instances(sc):
+    """Composed instances of an instanced scene dict, in composition order: (model index, l2w rows
+    (double, 12), id0, id1)."""
+    out = []
+    for i, (kind, idx, m) in enumerate(sc["instances"]):
+        l2w = rows_from_colmajor(colmajor(m))
+        if kind == "model":
+            out.append((idx, l2w, i, 0xFFFFFFFF))
+        else:
+            for j, (midx, m2) in enumerate(sc["subscenes"][idx]):
+                out.append((midx, affine_mul(l2w, rows_from_colmajor(colmajor(m2))), i, j))
+    return out
+
+
+class _IScene(C.Structure):
+    _fields_ = [("insts", C.c_void_p), ("n_inst", C.c_uint32), ("tri9", C.c_void_p), ("tri_geom", C.c_void_p),
+                ("geom_first", C.c_void_p), ("geom_base", C.c_void_p), ("l2w", C.c_void_p)]
+
+
+class InstancedRenderScene:
+    """The oracle's view of an instanced scene dict: "models" (scene dicts of geometries with their
+    materials and textures), optional "subscenes" (lists of (model index, 4 x 4 local-to-world)),
+    "instances" (("model" | "subscene", index, 4 x 4 local-to-world)), and the scene-level lights,
+    background and optional "ibl". The placed scenes' geometries are numbered as one list in
+    first-appearance order, as the library numbers them."""
+
+    def __init__(self, sc: dict):
+        from selas_amd.workloads import geom_tri9
+
+        comp = compose_instances(sc)
+        self.models = []
+        for mdesc in sc["models"]:
+            tris = [geom_tri9(g) for g in mdesc["geoms"]]
+            tri9 = np.ascontiguousarray(np.concatenate(tris), np.float32)
+            mm = {"tri9": tri9,
+                  "tri_geom": np.concatenate([np.full(t.shape[0], i, np.uint32) for i, t in enumerate(tris)]),
+                  "geom_first": np.cumsum([0] + [t.shape[0] for t in tris[:-1]]).astype(np.uint32),
+                  "materials": np.ascontiguousarray(mdesc["materials"], np.float32),
+                  "geom_material": np.ascontiguousarray(mdesc["geom_material"], np.uint32),
+                  "n_geoms": len(tris), "bvh": CpuBvh(tri9), "geoms": mdesc["geoms"]}
+            pts = tri9.reshape(-1, 3)
+            mm["box"] = (pts.min(0), pts.max(0))
+            self.models.append(mm)
+        comp = [c for c in comp if self.models[c[0]]["tri9"].shape[0] > 0]
+        order = []
+        for c in comp:
+            if c[0] not in order:
+                order.append(c[0])
+        g0, m0, ng, nm = {}, {}, 0, 0
+        for mi in order:
+            g0[mi], m0[mi] = ng, nm
+            ng += self.models[mi]["n_geoms"]
+            nm += self.models[mi]["materials"].shape[0]
+        mats = [self.models[mi]["materials"] for mi in order]
+        self.materials = np.ascontiguousarray(np.concatenate(mats) if mats else np.zeros((1, 22)), np.float32)
+        gm = [self.models[mi]["geom_material"][:self.models[mi]["n_geoms"]] + m0[mi] for mi in order]
+        self.geom_material = np.ascontiguousarray(np.concatenate(gm) if gm else np.zeros(1), np.uint32)
+        self.textures = []
+        self.geom_tex = (_Ptex * max(ng, 1))()
+        for mi in order:
+            for gi, g in enumerate(self.models[mi]["geoms"]):
+                if isinstance(g, dict) and g.get("ptex") is not None:
+                    p = g["ptex"]
+                    t = PtexTexture(p["texels"], p["res_log2"], p["adjfaces"], p["adjedges"])
+                    self.textures.append(t)
+                    self.geom_tex[g0[mi] + gi] = t.s
+        n = len(comp)
+        self.inst_arr = (_Inst * max(n, 1))()
+        self.tri9_ptrs = (C.c_void_p * max(n, 1))()
+        self.tri_geom_ptrs = (C.c_void_p * max(n, 1))()
+        self.geom_first_ptrs = (C.c_void_p * max(n, 1))()
+        self.geom_base = np.zeros(max(n, 1), np.uint32)
+        self.l2w = np.zeros((max(n, 1), 9), np.float32)
+        lo = np.full(3, np.inf, np.float32)
+        hi = np.full(3, -np.inf, np.float32)
+        for k, (mi, l2w, _, _) in enumerate(comp):
+            mm = self.models[mi]
+            w2l = np.array(affine_inverse(l2w), np.float32)
+            self.inst_arr[k] = _Inst((C.c_float * 12)(*[float(x) for x in w2l]), mm["bvh"].handle)
+            self.tri9_ptrs[k] = mm["tri9"].ctypes.data
+            self.tri_geom_ptrs[k] = mm["tri_geom"].ctypes.data
+            self.geom_first_ptrs[k] = mm["geom_first"].ctypes.data
+            self.geom_base[k] = g0[mi]
+            self.l2w[k] = [l2w[4 * r + c] for r in range(3) for c in range(3)]
+            # the instance's world box: the placed scene's box corners through l2w in double,
+            # rounded outward to float (the library's top-tree boxes; they bound the emission sphere)
+            blo, bhi = mm["box"]
+            wlo, whi = [np.inf] * 3, [-np.inf] * 3
+            for cidx in range(8):
+                pnt = [float(bhi[0] if cidx & 1 else blo[0]), float(bhi[1] if cidx & 2 else blo[1]),
+                       float(bhi[2] if cidx & 4 else blo[2])]
+                for r in range(3):
+                    wv = l2w[4 * r] * pnt[0] + l2w[4 * r + 1] * pnt[1] + l2w[4 * r + 2] * pnt[2] + l2w[4 * r + 3]
+                    wlo[r] = min(wlo[r], wv)
+                    whi[r] = max(whi[r], wv)
+            for r in range(3):
+                lo[r] = min(lo[r], np.nextafter(np.float32(wlo[r]), np.float32(-np.inf)))
+                hi[r] = max(hi[r], np.nextafter(np.float32(whi[r]), np.float32(np.inf)))
+        self.iscene = _IScene(C.cast(self.inst_arr, C.c_void_p), n, C.cast(self.tri9_ptrs, C.c_void_p),
+                              C.cast(self.tri_geom_ptrs, C.c_void_p), C.cast(self.geom_first_ptrs, C.c_void_p),
+                              self.geom_base.ctypes.data, self.l2w.ctypes.data)
+        half = np.float32(0.5)
+        d = hi - lo
+        self.sphere = np.array([(lo[0] + hi[0]) * half, (lo[1] + hi[1]) * half, (lo[2] + hi[2]) * half,
+                                half * np.sqrt(np.float32(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]))], np.float32)
+        ibl = sc.get("ibl")
+        self.ibl = None if ibl is None else Ibl(ibl["rgb"], ibl.get("rotation", 0.0), ibl.get("exposure", 1.0),
+                                                sphere=self.sphere)
+        self.lights = np.ascontiguousarray(sc["lights"])
+        self.s = _Scene(None, None, None, self.materials.ctypes.data, self.geom_material.ctypes.data, max(ng, 1),
+                        self.lights.ctypes.data if self.lights.size else None, self.lights.shape[0],
+                        (C.c_float * 3)(*[float(x) for x in sc["background"]]), None,
+                        C.cast(self.geom_tex, C.c_void_p),
+                        C.cast(C.pointer(self.ibl.s), C.c_void_p) if self.ibl is not None else None,
+                        C.cast(C.pointer(self.iscene), C.c_void_p))

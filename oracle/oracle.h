@@ -157,6 +157,20 @@ void orc_ibl_emit(const orc_ibl* b, float r0, float r1, float r2, float r3, floa
 void orc_ibl_direct(const orc_ibl* b, float r0, float r1, float* out8);
 float orc_ibl_position_pdf(const orc_ibl* b);
 
+/* An instanced scene for the integrators (orc_pt.c): the composed instances with, per composed
+ * instance, the placed scene's triangles (its own space), their geometry ids and each geometry's
+ * first triangle, the instance's first geometry in the scene's list of geometries, and the
+ * linear part of its local-to-world transform (row-major 3x3) that carries hit normals to world. */
+typedef struct {
+    const orc_inst* insts;
+    uint32_t n_inst;
+    const float* const* tri9;
+    const uint32_t* const* tri_geom;
+    const uint32_t* const* geom_first;
+    const uint32_t* geom_base;
+    const float* l2w;
+} orc_iscene;
+
 typedef struct {
     const orc_bvh* bvh;
     const float* tri9;          /* world-space triangles, 9 floats each (global index order) */
@@ -171,7 +185,21 @@ typedef struct {
     const orc_ptex* geom_tex;   /* per geometry (nfaces == 0: none); a textured geometry is a quad
                                    mesh whose triangles 2q and 2q+1 are Embree's split of quad q */
     const orc_ibl* ibl;         /* image-based light, NULL for none (replaces the background) */
+    const orc_iscene* inst;     /* instanced scene, NULL for none: then bvh, tri9, tri_geom and
+                                   geom_first are unused and the shading tables above are indexed
+                                   by the scene's list of geometries */
 } orc_scene;
+
+/* Closest hit in a scene of geometries or an instanced scene: 0 on a miss; else t, u, v, the
+ * hit's geometry (scene list), its unnormalised world-space geometric normal and the index of the
+ * triangle within its geometry. */
+int orc_scene_closest1(const orc_scene* sc, uint32_t* stack, const float* o, const float* d, float tnear, float tfar,
+                       float* t, float* u, float* v, uint32_t* geom, float* ng, uint32_t* tri_in_geom);
+int orc_scene_occluded1(const orc_scene* sc, uint32_t* stack, const float* o, const float* d, float tnear, float tfar);
+uint32_t orc_scene_stack_size(const orc_scene* sc);
+/* the texel of a hit on triangle tri_in_geom of geometry geom (NULL: the material's colour) */
+const float* orc_hit_texel_local(const orc_scene* sc, uint32_t tri_in_geom, uint32_t geom, const float* mat, float hu,
+                                 float hv, float* texel3);
 
 typedef struct {
     float cam10[10]; /* position, look_at, up, fov_degrees */

@@ -86,10 +86,16 @@ void orc_surface_from_hit(const float* o, const float* d, float t, const float* 
 const float* orc_hit_texel(const orc_scene* sc, uint32_t tri, uint32_t geom, const float* mat, float hu, float hv,
                            float* texel3)
 {
-    if (!(((uint32_t)mat[21]) & 16u) || !sc->geom_tex || !sc->geom_first) return NULL;
+    if (!sc->geom_first) return NULL;
+    return orc_hit_texel_local(sc, tri - sc->geom_first[geom], geom, mat, hu, hv, texel3);
+}
+
+const float* orc_hit_texel_local(const orc_scene* sc, uint32_t local, uint32_t geom, const float* mat, float hu,
+                                 float hv, float* texel3)
+{
+    if (!(((uint32_t)mat[21]) & 16u) || !sc->geom_tex) return NULL;
     const orc_ptex* tx = &sc->geom_tex[geom];
     if (tx->nfaces == 0) return NULL;
-    uint32_t local = tri - sc->geom_first[geom];
     float u = hu, v = hv;
     if (local & 1u) {
         u = 1.0f - hu;
@@ -193,6 +199,94 @@ static int bsdf_sample_any(orc_uniforms* u, const orc_surface* s, const float* v
     return orc_disney_sample(u, s, v, s->shader == 0, out14);
 }
 
+/* ---- scene queries: a scene of geometries, or the composed instances of an instanced scene ---- */
+
+static void tri_normal(const float* tp, float* ng)
+{
+    float e1[3] = {tp[3] - tp[0], tp[4] - tp[1], tp[5] - tp[2]};
+    float e2[3] = {tp[6] - tp[0], tp[7] - tp[1], tp[8] - tp[2]};
+    ov3 n = cross(ld3(e1), ld3(e2));
+    ng[0] = n.x;
+    ng[1] = n.y;
+    ng[2] = n.z;
+}
+
+/* world = m n for the row-major 3x3 m, by the device's fused multiply-add chain */
+static void xform_linear(const float* m, const float* n, float* out)
+{
+    for (int r = 0; r < 3; ++r) out[r] = fmaf(m[3 * r], n[0], fmaf(m[3 * r + 1], n[1], m[3 * r + 2] * n[2]));
+}
+
+static void inst_ray(const float* w, const float* o, const float* d, float* lo, float* ld)
+{
+    for (int r = 0; r < 3; ++r) {
+        const float* row = w + 4 * r;
+        lo[r] = fmaf(row[0], o[0], fmaf(row[1], o[1], fmaf(row[2], o[2], row[3])));
+        ld[r] = fmaf(row[0], d[0], fmaf(row[1], d[1], row[2] * d[2]));
+    }
+}
+
+int orc_scene_closest1(const orc_scene* sc, uint32_t* stack, const float* o, const float* d, float tnear, float tfar,
+                       float* t, float* u, float* v, uint32_t* geom, float* ng, uint32_t* tri_in_geom)
+{
+    if (!sc->inst) {
+        uint32_t prim = orc_bvh_closest1(sc->bvh, stack, o, d, tnear, tfar, t, u, v);
+        if (prim == 0xFFFFFFFFu) return 0;
+        *geom = sc->tri_geom[prim];
+        tri_normal(sc->tri9 + (size_t)prim * 9, ng);
+        *tri_in_geom = sc->geom_first ? prim - sc->geom_first[*geom] : prim;
+        return 1;
+    }
+    const orc_iscene* is = sc->inst;
+    float best_t = tfar, bu = 0.0f, bv = 0.0f;
+    uint32_t best = 0xFFFFFFFFu, best_k = 0;
+    for (uint32_t k = 0; k < is->n_inst; ++k) {
+        float lo[3], ldir[3], tk, uk, vk;
+        inst_ray(is->insts[k].w2l, o, d, lo, ldir);
+        uint32_t p = orc_bvh_closest1(is->insts[k].blas, stack, lo, ldir, tnear, best_t, &tk, &uk, &vk);
+        if (p != 0xFFFFFFFFu && (best == 0xFFFFFFFFu || tk < best_t)) {
+            best_t = tk;
+            best = p;
+            best_k = k;
+            bu = uk;
+            bv = vk;
+        }
+    }
+    *t = best_t;
+    if (best == 0xFFFFFFFFu) return 0;
+    *u = bu;
+    *v = bv;
+    const uint32_t lg = is->tri_geom[best_k][best];
+    *geom = is->geom_base[best_k] + lg;
+    *tri_in_geom = best - is->geom_first[best_k][lg];
+    float nl[3];
+    tri_normal(is->tri9[best_k] + (size_t)best * 9, nl);
+    xform_linear(is->l2w + 9 * (size_t)best_k, nl, ng);
+    return 1;
+}
+
+int orc_scene_occluded1(const orc_scene* sc, uint32_t* stack, const float* o, const float* d, float tnear, float tfar)
+{
+    if (!sc->inst) return orc_bvh_occluded1(sc->bvh, stack, o, d, tnear, tfar);
+    for (uint32_t k = 0; k < sc->inst->n_inst; ++k) {
+        float lo[3], ldir[3];
+        inst_ray(sc->inst->insts[k].w2l, o, d, lo, ldir);
+        if (orc_bvh_occluded1(sc->inst->insts[k].blas, stack, lo, ldir, tnear, tfar)) return 1;
+    }
+    return 0;
+}
+
+uint32_t orc_scene_stack_size(const orc_scene* sc)
+{
+    if (!sc->inst) return orc_bvh_stack_size(sc->bvh);
+    uint32_t m = 4;
+    for (uint32_t k = 0; k < sc->inst->n_inst; ++k) {
+        const uint32_t s = orc_bvh_stack_size(sc->inst->insts[k].blas);
+        if (s > m) m = s;
+    }
+    return m;
+}
+
 /* EvaluatePath for one (pixel, sample); writes ld[6] = layer0 rgb, layer1 rgb */
 static void evaluate_path(const pt_ctx* c, uint32_t* stack, uint32_t x, uint32_t y, uint64_t path, float* ld)
 {
@@ -211,9 +305,9 @@ static void evaluate_path(const pt_ctx* c, uint32_t* stack, uint32_t x, uint32_t
     orc_camera_dir(&c->cam, vx, vy, d);
     uint32_t bounce = 0;
     while (bounce < job->max_path_length) {
-        float t, hu, hv;
-        uint32_t prim = orc_bvh_closest1(sc->bvh, stack, o, d, 0.0f, FLOAT_MAX, &t, &hu, &hv);
-        if (prim == 0xFFFFFFFFu) {
+        float t, hu, hv, ng[3];
+        uint32_t geom, tri_in_geom;
+        if (!orc_scene_closest1(sc, stack, o, d, 0.0f, FLOAT_MAX, &t, &hu, &hv, &geom, ng, &tri_in_geom)) {
             /* EvaluateBackground (AreaLighting.cpp:415-424): the image-based light, else the
              * constant background */
             float bg[3];
@@ -224,17 +318,12 @@ static void evaluate_path(const pt_ctx* c, uint32_t* stack, uint32_t x, uint32_t
             l1 = add(l1, mul(ld3(bg), thr));
             break;
         }
-        uint32_t geom = sc->tri_geom[prim];
-        const float* tp = sc->tri9 + (size_t)prim * 9;
-        float e1[3] = {tp[3] - tp[0], tp[4] - tp[1], tp[5] - tp[2]};
-        float e2[3] = {tp[6] - tp[0], tp[7] - tp[1], tp[8] - tp[2]};
-        ov3 ngv = cross(ld3(e1), ld3(e2));
-        float ng[3] = {ngv.x, ngv.y, ngv.z};
         const float* mat = sc->materials + 22 * (size_t)sc->geom_material[geom];
         orc_surface s;
         float pos[3], err;
         float texel[3];
-        orc_surface_from_hit(o, d, t, ng, mat, orc_hit_texel(sc, prim, geom, mat, hu, hv, texel), &s, pos, &err);
+        orc_surface_from_hit(o, d, t, ng, mat, orc_hit_texel_local(sc, tri_in_geom, geom, mat, hu, hv, texel), &s, pos,
+                             &err);
         float v[3] = {-d[0], -d[1], -d[2]};
         float nrm[3] = {s.n.x, s.n.y, s.n.z};
         float ls[8];
@@ -245,7 +334,7 @@ static void evaluate_path(const pt_ctx* c, uint32_t* stack, uint32_t x, uint32_t
             if (dot(ld3(f), v3(1, 1, 1)) > 0) {
                 float so[3];
                 orc_offset_origin(&s, pos, err, ls, 0.1f, so, NULL);
-                if (!orc_bvh_occluded1(sc->bvh, stack, so, ls, err, ls[6])) {
+                if (!orc_scene_occluded1(sc, stack, so, ls, err, ls[6])) {
                     ov3 smp = scl(mul(ld3(f), ld3(ls + 3)), 1.0f / ls[7]);
                     l1 = add(l1, mul(smp, thr));
                 }
@@ -281,7 +370,7 @@ static void evaluate_path(const pt_ctx* c, uint32_t* stack, uint32_t x, uint32_t
 static void pt_range(void* cv, uint32_t begin, uint32_t end)
 {
     const pt_ctx* c = (const pt_ctx*)cv;
-    uint32_t* stack = (uint32_t*)malloc(sizeof(uint32_t) * orc_bvh_stack_size(c->sc->bvh));
+    uint32_t* stack = (uint32_t*)malloc(sizeof(uint32_t) * orc_scene_stack_size(c->sc));
     const uint32_t W = c->job->width, spp = c->job->spp;
     for (uint32_t pix = begin; pix < end; ++pix) {
         uint32_t x = pix % W, y = pix / W;

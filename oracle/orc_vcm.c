@@ -365,18 +365,14 @@ typedef struct {
 static int hit_surface(const vcm_ctx* c, const float* o, const float* d, float tnear, orc_surface* s, float* pos,
                        float* err, uint32_t* geom_out)
 {
-    float t, hu, hv;
-    uint32_t prim = orc_bvh_closest1(c->sc->bvh, c->stack, o, d, tnear, 3.402823466e+38F, &t, &hu, &hv);
-    if (prim == 0xFFFFFFFFu) return 0;
-    uint32_t geom = c->sc->tri_geom[prim];
-    const float* tp = c->sc->tri9 + (size_t)prim * 9;
-    float e1[3] = {tp[3] - tp[0], tp[4] - tp[1], tp[5] - tp[2]};
-    float e2[3] = {tp[6] - tp[0], tp[7] - tp[1], tp[8] - tp[2]};
-    ov3 ngv = cross(ld3(e1), ld3(e2));
-    float ng[3] = {ngv.x, ngv.y, ngv.z};
+    float t, hu, hv, ng[3];
+    uint32_t geom, tri_in_geom;
+    if (!orc_scene_closest1(c->sc, c->stack, o, d, tnear, 3.402823466e+38F, &t, &hu, &hv, &geom, ng, &tri_in_geom))
+        return 0;
     const float* mat = c->sc->materials + 22 * (size_t)c->sc->geom_material[geom];
     float texel[3];
-    orc_surface_from_hit(o, d, t, ng, mat, orc_hit_texel(c->sc, prim, geom, mat, hu, hv, texel), s, pos, err);
+    orc_surface_from_hit(o, d, t, ng, mat, orc_hit_texel_local(c->sc, tri_in_geom, geom, mat, hu, hv, texel), s, pos,
+                         err);
     *geom_out = geom;
     return 1;
 }
@@ -479,7 +475,7 @@ static void light_paths(void* ctx, uint32_t b, uint32_t e)
     vcm_ctx c = I->c;
     const orc_scene* sc = c.sc;
     const uint32_t W = I->W, H = I->H, N = I->N, L = I->L;
-    c.stack = (uint32_t*)malloc(sizeof(uint32_t) * orc_bvh_stack_size(sc->bvh));
+    c.stack = (uint32_t*)malloc(sizeof(uint32_t) * orc_scene_stack_size(sc));
     for (uint32_t i = b; i < e; ++i) {
         uint32_t nvert = 0, nsplat = 0, nrays = 0;
         vcm_vertex* mine = I->slot_verts + (size_t)i * I->mlv;
@@ -548,7 +544,7 @@ static void light_paths(void* ctx, uint32_t b, uint32_t e)
                             float tc[3] = {to_cam.x, to_cam.y, to_cam.z};
                             orc_offset_origin(&s, hp, err, tc, 0.1f, so, NULL);
                             ++nrays;
-                            if (!orc_bvh_occluded1(sc->bvh, c.stack, so, tc, err, dist)) {
+                            if (!orc_scene_occluded1(sc, c.stack, so, tc, err, dist)) {
                                 I->splat_c[(size_t)i * I->mlv + nsplat] = contrib;
                                 I->splat_pix[(size_t)i * I->mlv + nsplat] = (uint32_t)iy * W + (uint32_t)ix;
                                 ++nsplat;
@@ -608,7 +604,7 @@ static void camera_paths(void* ctx, uint32_t b, uint32_t e)
     vcm_ctx c = I->c;
     const orc_scene* sc = c.sc;
     const uint32_t W = I->W, N = I->N, L = I->L;
-    c.stack = (uint32_t*)malloc(sizeof(uint32_t) * orc_bvh_stack_size(sc->bvh));
+    c.stack = (uint32_t*)malloc(sizeof(uint32_t) * orc_scene_stack_size(sc));
     for (uint32_t i = b; i < e; ++i) {
         uint32_t nrays = 0;
         uint64_t st, inc, sid = stream_id(I->iteration, i, 1);
@@ -679,7 +675,7 @@ static void camera_paths(void* ctx, uint32_t b, uint32_t e)
                             float so[3], ld[3] = {ldir.x, ldir.y, ldir.z};
                             orc_offset_origin(&s, hp, err, ld, 0.1f, so, NULL);
                             ++nrays;
-                            if (!orc_bvh_occluded1(sc->bvh, c.stack, so, ld, err, ldist))
+                            if (!orc_scene_occluded1(sc, c.stack, so, ld, err, ldist))
                                 color = add(color, mul(thr, contrib));
                         }
                     }
@@ -723,7 +719,7 @@ static void camera_paths(void* ctx, uint32_t b, uint32_t e)
                     float so[3], cd[3] = {cdir.x, cdir.y, cdir.z};
                     orc_offset_origin(&s, hp, err, cd, 0.1f, so, &sb);
                     ++nrays;
-                    if (!orc_bvh_occluded1(sc->bvh, c.stack, so, cd, err, dist - 16.0f * absf(sb)))
+                    if (!orc_scene_occluded1(sc, c.stack, so, cd, err, dist - 16.0f * absf(sb)))
                         color = add(color, mul(mul(thr, lv->throughput), contrib));
                 }
             }

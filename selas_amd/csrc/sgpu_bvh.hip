@@ -740,6 +740,8 @@ int sgpu_bvh_build_instances(sgpu_scene scene, hipStream_t stream)
 
     // world boxes (the 8 corners of each placed scene's box through its transform, in double,
     // rounded outward), centroids and the bounds k_morton and k_fit read
+    scene->comp_blas.clear();
+    scene->comp_l2w.clear();
     std::vector<float4> blo(n), bhi(n), cent(n);
     std::vector<InstRecord> rec(n);
     std::vector<uint32_t> ids(2 * (size_t)n);
@@ -778,6 +780,9 @@ int sgpu_bvh_build_instances(sgpu_scene scene, hipStream_t stream)
             rec[k].w2l[r] = make_float4((float)w2l[4 * r], (float)w2l[4 * r + 1], (float)w2l[4 * r + 2], (float)w2l[4 * r + 3]);
         rec[k].wide = f.blas->d_wide;
         rec[k].tris = f.blas->d_tris;
+        scene->comp_blas.push_back(f.blas);
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) scene->comp_l2w.push_back((float)f.l2w[4 * r + c]);
         ids[2 * k] = f.id0;
         ids[2 * k + 1] = f.id1;
         blas_cap = std::max(blas_cap, f.blas->stack_cap);

@@ -367,6 +367,67 @@ int sgpu_scene_add_instance(sgpu_scene scene, sgpu_scene child, const float* col
     return SGPU_OK;
 }
 
+// The shading data of an instanced scene (see sgpu_scene_s): the placed scenes' geometries as one
+// list, their shading records and textures as they are now. A placed scene without materials for
+// every geometry leaves the instanced scene queryable but not renderable (render_error).
+static int gather_instance_shading(sgpu_scene scene)
+{
+    std::vector<sgpu_scene> models;
+    std::vector<uint32_t> model_of(scene->comp_blas.size());
+    for (size_t k = 0; k < scene->comp_blas.size(); ++k) {
+        sgpu_scene m = scene->comp_blas[k];
+        size_t j = std::find(models.begin(), models.end(), m) - models.begin();
+        if (j == models.size()) models.push_back(m);
+        model_of[k] = (uint32_t)j;
+    }
+    std::vector<uint32_t> g0(models.size()), m0(models.size());
+    uint64_t n_geoms = 0, n_mats = 0;
+    for (size_t j = 0; j < models.size(); ++j) {
+        sgpu_scene m = models[j];
+        if (!m->d_materials || m->n_geom_material < m->geom_first.size()) {
+            scene->render_error = "a placed scene has no materials for every geometry (set them before committing "
+                                  "the scene that places it)";
+            return SGPU_OK;
+        }
+        g0[j] = (uint32_t)n_geoms;
+        m0[j] = (uint32_t)n_mats;
+        n_geoms += m->geom_first.size();
+        n_mats += m->n_materials;
+    }
+    if (n_geoms >= (1u << 27)) {
+        scene->render_error = "the placed scenes hold more than 2^27 geometries";
+        return SGPU_OK;
+    }
+    std::vector<uint32_t> gmat(n_geoms);
+    std::vector<sgpu::PtexTex> gtex(n_geoms, sgpu::PtexTex{nullptr, nullptr, 0, 0});
+    for (size_t j = 0; j < models.size(); ++j) {
+        sgpu_scene m = models[j];
+        const size_t ng = m->geom_first.size();
+        SGPU_HIP_TRY(hipMemcpy(gmat.data() + g0[j], m->d_geom_material, ng * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        for (size_t g = 0; g < ng; ++g) {
+            gmat[g0[j] + g] += m0[j];
+            if (g < m->tex_host.size()) gtex[g0[j] + g] = m->tex_host[g];
+        }
+    }
+    std::vector<uint32_t> base(model_of.size());
+    for (size_t k = 0; k < base.size(); ++k) base[k] = g0[model_of[k]];
+    SGPU_HIP_TRY(hipMalloc((void**)&scene->d_g_materials, 22 * sizeof(float) * (size_t)std::max<uint64_t>(n_mats, 1)));
+    for (size_t j = 0; j < models.size(); ++j)
+        SGPU_HIP_TRY(hipMemcpy(scene->d_g_materials + 22 * (size_t)m0[j], models[j]->d_materials,
+                               22 * sizeof(float) * (size_t)models[j]->n_materials, hipMemcpyDeviceToDevice));
+    SGPU_HIP_TRY(hipMalloc((void**)&scene->d_g_geom_material, sizeof(uint32_t) * (size_t)std::max<uint64_t>(n_geoms, 1)));
+    SGPU_HIP_TRY(hipMemcpy(scene->d_g_geom_material, gmat.data(), sizeof(uint32_t) * n_geoms, hipMemcpyHostToDevice));
+    SGPU_HIP_TRY(hipMalloc((void**)&scene->d_g_geom_tex, sizeof(sgpu::PtexTex) * (size_t)std::max<uint64_t>(n_geoms, 1)));
+    SGPU_HIP_TRY(hipMemcpy(scene->d_g_geom_tex, gtex.data(), sizeof(sgpu::PtexTex) * n_geoms, hipMemcpyHostToDevice));
+    SGPU_HIP_TRY(hipMalloc((void**)&scene->d_inst_geom_base, sizeof(uint32_t) * std::max<size_t>(base.size(), 1)));
+    SGPU_HIP_TRY(hipMemcpy(scene->d_inst_geom_base, base.data(), sizeof(uint32_t) * base.size(), hipMemcpyHostToDevice));
+    SGPU_HIP_TRY(hipMalloc((void**)&scene->d_inst_l2w, sizeof(float) * std::max<size_t>(scene->comp_l2w.size(), 1)));
+    SGPU_HIP_TRY(hipMemcpy(scene->d_inst_l2w, scene->comp_l2w.data(), sizeof(float) * scene->comp_l2w.size(),
+                           hipMemcpyHostToDevice));
+    scene->n_g_geoms = (uint32_t)n_geoms;
+    return SGPU_OK;
+}
+
 int sgpu_scene_commit(sgpu_scene scene, void* stream)
 {
     if (!scene) return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_commit: NULL scene");
@@ -377,6 +438,7 @@ int sgpu_scene_commit(sgpu_scene scene, void* stream)
     SGPU_HIP_TRY(hipMemset(scene->d_overflow, 0, sizeof(uint32_t)));
     if (!scene->instances.empty()) {
         int rc = sgpu_bvh_build_instances(scene, as_stream(stream));
+        if (rc == SGPU_OK) rc = gather_instance_shading(scene);
         if (rc != SGPU_OK) return rc;
         if (scene->n_inst > 0) {
             const float* b = scene->box;
@@ -421,6 +483,11 @@ int sgpu_scene_release(sgpu_scene scene)
     }
     if (scene->d_inst) (void)hipFree(scene->d_inst);
     if (scene->d_inst_ids) (void)hipFree(scene->d_inst_ids);
+    if (scene->d_g_materials) (void)hipFree(scene->d_g_materials);
+    if (scene->d_g_geom_material) (void)hipFree(scene->d_g_geom_material);
+    if (scene->d_g_geom_tex) (void)hipFree(scene->d_g_geom_tex);
+    if (scene->d_inst_geom_base) (void)hipFree(scene->d_inst_geom_base);
+    if (scene->d_inst_l2w) (void)hipFree(scene->d_inst_l2w);
     if (scene->d_nodes) (void)hipFree(scene->d_nodes);
     if (scene->d_tris) (void)hipFree(scene->d_tris);
     if (scene->d_wide) (void)hipFree(scene->d_wide);
@@ -671,10 +738,11 @@ int sgpu_render_pt(sgpu_scene scene, const sgpu_pt_job* job, float* framebuffer,
 {
     if (!scene || !job || !framebuffer) return sgpu_fail(SGPU_E_INVALID, "sgpu_render_pt: NULL argument");
     if (!scene->committed) return sgpu_fail(SGPU_E_STATE, "sgpu_render_pt: scene not committed");
-    if (!scene->instances.empty())
-        return sgpu_fail(SGPU_E_INVALID, "sgpu_render_pt: instanced scenes answer queries; render scenes of geometries");
-    if (!scene->d_materials || scene->n_geom_material < scene->geom_first.size())
+    if (!scene->instances.empty()) {
+        if (!scene->render_error.empty()) return sgpu_fail(SGPU_E_STATE, "sgpu_render_pt: " + scene->render_error);
+    } else if (!scene->d_materials || scene->n_geom_material < scene->geom_first.size()) {
         return sgpu_fail(SGPU_E_STATE, "sgpu_render_pt: materials not set for every geometry");
+    }
     if (job->camera.width == 0 || job->camera.height == 0 || job->spp == 0 || job->max_path_length == 0)
         return sgpu_fail(SGPU_E_INVALID, "sgpu_render_pt: zero width, height, spp or path length");
     if ((uint64_t)job->camera.width * job->camera.height > (1ull << 26))
@@ -689,10 +757,11 @@ static int check_vcm_job(sgpu_scene scene, const sgpu_vcm_job* job, const void* 
 {
     if (!scene || !job || !framebuffer) return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": NULL argument");
     if (!scene->committed) return sgpu_fail(SGPU_E_STATE, std::string(who) + ": scene not committed");
-    if (!scene->instances.empty())
-        return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": instanced scenes answer queries; render scenes of geometries");
-    if (!scene->d_materials || scene->n_geom_material < scene->geom_first.size())
+    if (!scene->instances.empty()) {
+        if (!scene->render_error.empty()) return sgpu_fail(SGPU_E_STATE, std::string(who) + ": " + scene->render_error);
+    } else if (!scene->d_materials || scene->n_geom_material < scene->geom_first.size()) {
         return sgpu_fail(SGPU_E_STATE, std::string(who) + ": materials not set for every geometry");
+    }
     if (scene->n_geom_material >= (1u << 27))
         return sgpu_fail(SGPU_E_RANGE, std::string(who) + ": more than 2^27 geometries");
     if (job->camera.width == 0 || job->camera.height == 0 || job->iterations == 0 || job->max_path_length == 0)

@@ -141,6 +141,20 @@ struct sgpu_scene_s {
     uint32_t n_inst = 0;
     uint32_t inst_levels = 0;          // 1 or 2 for an instanced scene
     uint32_t blas_stack_cap = 0;       // the largest stack of the placed scenes' trees
+    // shading data of an instanced scene, gathered at commit from the placed scenes: their
+    // geometries numbered as one list (geometry g of placed scene m is d_inst_geom_base[k] + g for
+    // every composed instance k of m), with the materials and textures those scenes held then
+    float* d_g_materials = nullptr;       // shading records of all placed scenes
+    uint32_t* d_g_geom_material = nullptr;
+    sgpu::PtexTex* d_g_geom_tex = nullptr;
+    uint32_t n_g_geoms = 0;
+    uint32_t* d_inst_geom_base = nullptr;  // per composed instance
+    float* d_inst_l2w = nullptr;           // per composed instance: local-to-world linear part, 9 floats
+    std::string render_error;              // why the scene cannot be rendered ("": it can)
+    // host: the placed scene and local-to-world linear part (row-major 3x3) of each composed
+    // instance, in composition order (kept for the shading data)
+    std::vector<sgpu_scene> comp_blas;
+    std::vector<float> comp_l2w;
 };
 
 // ---- error plumbing ------------------------------------------------------------------------------
@@ -164,9 +178,11 @@ int sgpu_bvh_build_instances(sgpu_scene scene, hipStream_t stream);
 // the per-lane stacks: at least sgpu_trace_spill_words(scene) 32-bit words, not shared with a
 // concurrent launch. With n_ptr, the kernels read the ray count from that device word (n then only
 // sizes the grid and must bound it); `overflow` replaces the scene's stack-overflow flag.
+// For an instanced scene, comp_out (may be NULL) receives each closest hit's composed instance.
 int sgpu_trace_launch(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits,
                       uint32_t* occluded, uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream,
-                      uint32_t* spill = nullptr, const uint32_t* n_ptr = nullptr, uint32_t* overflow = nullptr);
+                      uint32_t* spill = nullptr, const uint32_t* n_ptr = nullptr, uint32_t* overflow = nullptr,
+                      uint32_t* comp_out = nullptr);
 size_t sgpu_trace_spill_words(sgpu_scene scene);
 size_t sgpu_trace_spill_words(sgpu_context ctx, uint32_t stack_cap);
 // A scene's trees as a multi-scene traversal launch reads them (one table entry per scene).
@@ -175,14 +191,18 @@ struct TraceScene {
     const WideNode* wide;
     const TriRecord* tris;
     uint32_t n_tris;
-    uint32_t pad;
+    uint32_t n_inst;           // > 0: an instanced scene (wide = its top tree over `insts`)
+    const InstRecord* insts;
 };
 // One launch over rays of several scenes: ray i traverses table[ray_scene[i]] (device arrays);
 // stack_cap is the largest stack_cap of the table's scenes.
+// instanced: some entries are instanced scenes (then comp_out, if not NULL, receives each closest
+// hit's composed instance, SGPU_INVALID_ID for hits in scenes of geometries).
 int sgpu_trace_launch_multi(sgpu_context ctx, const TraceScene* table, const uint32_t* ray_scene, uint32_t stack_cap,
                             uint32_t* overflow, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits,
                             uint32_t* occluded, uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream,
-                            uint32_t* spill = nullptr, const uint32_t* n_ptr = nullptr);
+                            uint32_t* spill = nullptr, const uint32_t* n_ptr = nullptr, bool instanced = false,
+                            uint32_t* comp_out = nullptr);
 // Wavefront path tracer (sgpu_pt.hip); arguments validated by sgpu_render_pt.
 int sgpu_pt_render_impl(sgpu_scene scene, const sgpu_pt_job* job, float* framebuffer, hipStream_t stream);
 int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* framebuffer, sgpu_vcm_stats* stats,

@@ -91,6 +91,14 @@ SGPU_HD float sin_f(float x) { return (float)sin((double)x); }
 SGPU_HD float acos_f(float x) { return (float)acos((double)x); }
 SGPU_HD float atan2_f(float y, float x) { return (float)atan2((double)y, (double)x); }
 SGPU_HD float pow_f(float x, float y) { return (float)pow((double)x, (double)y); }
+// A vector through a row-major 3x3 matrix m (the linear part of an instance's local-to-world
+// transform: how Selas carries hit normals to world space, MatrixMultiplyVector(normal,
+// localToWorld), SurfaceParameters.cpp:167), by fused multiply-adds from the last term.
+SGPU_HD f3 xform_linear(const float* m, f3 v)
+{
+    return mk3(fmaf(m[0], v.x, fmaf(m[1], v.y, m[2] * v.z)), fmaf(m[3], v.x, fmaf(m[4], v.y, m[5] * v.z)),
+               fmaf(m[6], v.x, fmaf(m[7], v.y, m[8] * v.z)));
+}
 // x^5 (Schlick's weight), by double multiplications: the same bits on host and device.
 SGPU_HD float pow5_f(float x)
 {

@@ -57,6 +57,7 @@ struct RayBuf {
 struct HitBuf {
     float *t, *u, *v, *ngx, *ngy, *ngz;
     uint32_t *prim, *geom;
+    uint32_t* inst;  // composed instance of the hit (instanced scenes)
 };
 
 struct ShadowBuf {
@@ -135,6 +136,10 @@ struct ShadeParams {
     f3 background;
     uint32_t has_ibl;  // 1: misses look up the image-based light instead of the background
     IblView ibl;
+    // instanced scenes: geometry numbering and normal transform per composed instance
+    uint32_t instanced;
+    const uint32_t* inst_geom_base;
+    const float* inst_l2w;
     uint32_t bounce;  // bounces completed before this one
     uint32_t max_len;
     uint32_t flags;
@@ -158,6 +163,13 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(ShadeParams sp, uint32_t co
         f3 d = mk3(rays.dx[j], rays.dy[j], rays.dz[j]);
         f3 thr = get3(ps.thr, p);
         uint32_t geom = hits.geom[j];
+        f3 ng = mk3(hits.ngx[j], hits.ngy[j], hits.ngz[j]);
+        if (sp.instanced && geom != SGPU_INVALID_ID) {
+            // the placed scene's geometry in the scene's list, its normal in world space
+            const uint32_t k = hits.inst[j];
+            geom += sp.inst_geom_base[k];
+            ng = xform_linear(sp.inst_l2w + 9 * (size_t)k, ng);
+        }
         if (geom == SGPU_INVALID_ID || geom >= sp.n_geoms) {
             // miss: EvaluateBackground (AreaLighting.cpp:415-424): the image-based light's radiance
             // from the ray's direction, else the constant background (EvaluateBackgroundMiss reads
@@ -176,7 +188,7 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(ShadeParams sp, uint32_t co
                 texel = ptex_eval(sp.geom_tex[geom], (int)hits.prim[j], hits.u[j], hits.v[j]);
                 tex = true;
             }
-            surface_from_hit(o, d, hits.t[j], mk3(hits.ngx[j], hits.ngy[j], hits.ngz[j]), mat, tex ? &texel : nullptr,
+            surface_from_hit(o, d, hits.t[j], ng, mat, tex ? &texel : nullptr,
                              s);
             f3 v = -d;
             LightSample ls;
@@ -329,6 +341,7 @@ int sgpu_pt_render_impl(sgpu_scene scene, const sgpu_pt_job* job, float* fb, hip
     hb.ngz = A.get<float>(cap);
     hb.prim = A.get<uint32_t>(cap);
     hb.geom = A.get<uint32_t>(cap);
+    hb.inst = A.get<uint32_t>(cap);
     ShadowBuf sb;
     sb.ray = A.rays(cap);
     sb.path = A.get<uint32_t>(cap);
@@ -351,12 +364,22 @@ int sgpu_pt_render_impl(sgpu_scene scene, const sgpu_pt_job* job, float* fb, hip
     sp.materials = scene->d_materials;
     sp.geom_material = scene->d_geom_material;
     sp.n_geoms = scene->n_geom_material;
+    sp.instanced = scene->instances.empty() ? 0u : 1u;
+    sp.inst_geom_base = scene->d_inst_geom_base;
+    sp.inst_l2w = scene->d_inst_l2w;
     sp.lights = (const Light*)scene->d_lights;
     sp.n_lights = scene->n_lights;
     sp.background = mk3(scene->background[0], scene->background[1], scene->background[2]);
     sp.has_ibl = ibl_view(scene, sp.ibl) ? 1u : 0u;
     sp.geom_tex = scene->d_geom_tex;
     sp.n_geom_tex = scene->n_geom_tex;
+    if (sp.instanced) {
+        sp.materials = scene->d_g_materials;
+        sp.geom_material = scene->d_g_geom_material;
+        sp.n_geoms = scene->n_g_geoms;
+        sp.geom_tex = scene->d_g_geom_tex;
+        sp.n_geom_tex = scene->n_g_geoms;
+    }
     sp.max_len = job->max_path_length;
     sp.flags = job->flags;
 
@@ -372,7 +395,8 @@ int sgpu_pt_render_impl(sgpu_scene scene, const sgpu_pt_job* job, float* fb, hip
         sp.path_base = path_base;
         for (uint32_t bounce = 0; bounce < job->max_path_length && count > 0; ++bounce) {
             sgpu_rays_soa rs = as_soa(rays[cur]);
-            int rc = sgpu_trace_launch(scene, &rs, &hsoa, nullptr, count, false, nullptr, stream, spill);
+            int rc = sgpu_trace_launch(scene, &rs, &hsoa, nullptr, count, false, nullptr, stream, spill, nullptr, nullptr,
+                                       hb.inst);
             if (rc) return rc;
             SGPU_HIP_TRY(hipMemsetAsync(counters, 0, 4 * sizeof(uint32_t), stream));
             sp.bounce = bounce;

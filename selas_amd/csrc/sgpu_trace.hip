@@ -462,14 +462,18 @@ __device__ __forceinline__ f3 inst_vector(const float4* w, f3 v)
 // composition order (instance i of the scene, then its instance j), so the closest hit is the
 // smallest t with ties to the smaller (composed instance, triangle) = (inst_id0, inst_id1, geomID,
 // primID).
-template <bool ANY, bool STATS>
-__global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace8i(const WideNode* __restrict top, uint32_t n_inst,
+template <bool ANY, bool STATS, bool MULTI>
+__global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace8i(const WideNode* __restrict top, const TriRecord* __restrict top_tris,
+                                                              uint32_t top_n_tris, uint32_t n_inst,
                                                               const InstRecord* __restrict insts,
-                                                              const uint32_t* __restrict inst_ids, sgpu_rays_soa rays,
+                                                              const uint32_t* __restrict inst_ids,
+                                                              const TraceScene* __restrict table,
+                                                              const uint32_t* __restrict ray_scene, sgpu_rays_soa rays,
                                                               sgpu_hits_soa hits, uint32_t* __restrict occluded,
-                                                              uint32_t n, uint32_t* __restrict spill, uint32_t stack_cap,
+                                                              uint32_t n, const uint32_t* __restrict n_ptr,
+                                                              uint32_t* __restrict spill, uint32_t stack_cap,
                                                               unsigned long long* __restrict stats,
-                                                              uint32_t* __restrict overflow)
+                                                              uint32_t* __restrict overflow, uint32_t* __restrict comp_out)
 {
     __shared__ uint32_t lds_stack[SGPU_STACK_LDS][SGPU_TRACE_BLOCK];
     const uint32_t lane = threadIdx.x;
@@ -477,6 +481,7 @@ __global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace8i(const WideNode* __
     const uint32_t stride = gridDim.x * SGPU_TRACE_BLOCK;
     unsigned long long cnt_nodes = 0, cnt_tris = 0;
     const float inf = __int_as_float(0x7f800000);
+    if (n_ptr) n = *n_ptr;  // the count written on the device by the kernel that queued the rays
 
     for (uint32_t i = gtid; i < n; i += stride) {
         const f3 wo = mk3(rays.org_x[i], rays.org_y[i], rays.org_z[i]);
@@ -486,8 +491,23 @@ __global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace8i(const WideNode* __
         f3 o = wo, d = wd;
         f3 inv = mk3(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
         f3 oinv = mk3(o.x * inv.x, o.y * inv.y, o.z * inv.z);
-        const WideNode* __restrict rw = top;
-        const TriRecord* __restrict rt = nullptr;
+        // the ray's scene: an instanced one (top tree over instances) or, in a multi-scene launch,
+        // possibly a scene of geometries (top tree over triangles)
+        const WideNode* __restrict tw = top;
+        const TriRecord* __restrict tt = top_tris;
+        uint32_t tn = top_n_tris, ni = n_inst;
+        const InstRecord* __restrict ri = insts;
+        if (MULTI) {
+            const TraceScene ts = table[ray_scene[i]];
+            tw = ts.wide;
+            tt = ts.tris;
+            tn = ts.n_tris;
+            ni = ts.n_inst;
+            ri = ts.insts;
+        }
+        const bool has_inst = ni > 0;
+        const WideNode* __restrict rw = tw;
+        const TriRecord* __restrict rt = tt;
         uint32_t cur = SGPU_INVALID_ID;  // the instance being walked; SGPU_INVALID_ID: the top tree
 
         float best_t = tfar;
@@ -543,7 +563,7 @@ __global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace8i(const WideNode* __
         };
 
         int node = 0;
-        bool walking = n_inst > 0;
+        const bool walking = has_inst || tn > 0;
         while (walking && !overflowed) {
             if (node == kInstExit) {
                 // back in the top tree
@@ -551,7 +571,7 @@ __global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace8i(const WideNode* __
                 d = wd;
                 inv = mk3(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
                 oinv = mk3(o.x * inv.x, o.y * inv.y, o.z * inv.z);
-                rw = top;
+                rw = tw;
                 cur = SGPU_INVALID_ID;
                 if (!pop(node)) break;
                 continue;
@@ -559,7 +579,7 @@ __global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace8i(const WideNode* __
             if (node < 0) {
                 // a leaf of the top tree: enter composed instance ~node
                 cur = (uint32_t)(~node);
-                const InstRecord ir = insts[cur];
+                const InstRecord ir = ri[cur];
                 o = inst_point(ir.w2l, wo);
                 d = inst_vector(ir.w2l, wd);
                 inv = mk3(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
@@ -582,7 +602,7 @@ __global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace8i(const WideNode* __
             const uint32_t lox[2] = {w1.x, w1.y}, loy[2] = {w1.z, w1.w}, loz[2] = {w2.x, w2.y};
             const uint32_t hix[2] = {w2.z, w2.w}, hiy[2] = {w3.x, w3.y}, hiz[2] = {w3.z, w3.w};
             int ref[8] = {(int)w4.x, (int)w4.y, (int)w4.z, (int)w4.w, (int)w5.x, (int)w5.y, (int)w5.z, (int)w5.w};
-            const bool in_top = cur == SGPU_INVALID_ID;
+            const bool in_top = cur == SGPU_INVALID_ID && has_inst;
             float key[8];
             uint32_t leaves = 0;
 #pragma unroll
@@ -655,13 +675,16 @@ __global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace8i(const WideNode* __
         if (overflowed) atomicOr(overflow, 1u);
 
         uint32_t id0 = SGPU_INVALID_ID, id1 = SGPU_INVALID_ID;
-        const TriRecord* __restrict bt = nullptr;
-        if (!ANY && best_tri != 0xFFFFFFFFu) {
-            bt = insts[best_inst].tris;
-            id0 = inst_ids[2 * best_inst];
-            id1 = inst_ids[2 * best_inst + 1];
+        const TriRecord* __restrict bt = tt;
+        if (!ANY && best_tri != 0xFFFFFFFFu && best_inst != SGPU_INVALID_ID) {
+            bt = ri[best_inst].tris;
+            if (inst_ids) {
+                id0 = inst_ids[2 * best_inst];
+                id1 = inst_ids[2 * best_inst + 1];
+            }
         }
         write_result<ANY>(i, hit_any, best_tri, best_t, best_u, best_v, bt, hits, occluded, id0, id1);
+        if (!ANY && comp_out) comp_out[i] = best_tri != 0xFFFFFFFFu ? best_inst : SGPU_INVALID_ID;
     }
     if (STATS) {
         atomicAdd(&stats[0], cnt_nodes);
@@ -705,7 +728,7 @@ int trace_launch(sgpu_context ctx, const BvhNode* nodes, const WideNode* wnodes,
                  uint32_t n_tris, uint32_t stack_cap, uint32_t* overflow, const TraceScene* table,
                  const uint32_t* ray_scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits, uint32_t* occluded,
                  uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream, uint32_t* spill_buf,
-                 const uint32_t* n_ptr)
+                 const uint32_t* n_ptr, bool instanced = false, uint32_t* comp_out = nullptr)
 {
     if (n == 0) return SGPU_OK;
     const uint32_t cus = (uint32_t)ctx->props.multiProcessorCount;
@@ -724,6 +747,24 @@ int trace_launch(sgpu_context ctx, const BvhNode* nodes, const WideNode* wnodes,
     const bool multi = table != nullptr;
     const bool wide = ctx->trace_tree == 1;
     const uint32_t n_top = std::min(n_wide, std::min(ctx->trace_lds_nodes, kTopNodes));
+    if (multi && instanced) {
+        // a table with instanced scenes: k_trace8i walks both kinds of entry
+        const dim3 g(blocks), b(SGPU_TRACE_BLOCK);
+#define SGPU_LAUNCH_IM(A, S)                                                                                       \
+    hipLaunchKernelGGL((k_trace8i<A, S, true>), g, b, 0, stream, nullptr, nullptr, 0u, 0u, nullptr, nullptr, table, \
+                       ray_scene, *rays, h, occluded, n, n_ptr, spill, stack_cap, st, overflow, comp_out)
+        if (any_hit) {
+            if (st) SGPU_LAUNCH_IM(true, true);
+            else SGPU_LAUNCH_IM(true, false);
+        } else {
+            if (st) SGPU_LAUNCH_IM(false, true);
+            else SGPU_LAUNCH_IM(false, false);
+        }
+#undef SGPU_LAUNCH_IM
+        SGPU_HIP_TRY(hipGetLastError());
+        if (!spill_buf) SGPU_HIP_TRY(hipFreeAsync(spill, stream));
+        return SGPU_OK;
+    }
     if (any_hit) {
         if (st)
             launch_trace<true, true>(wide, multi, blocks, stream, nodes, wnodes, n_top, tris, n_tris, table, ray_scene, *rays,
@@ -760,7 +801,8 @@ namespace {
 
 // One traversal launch over the rays of an instanced scene (k_trace8i).
 int trace_launch_instanced(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits, uint32_t* occluded,
-                           uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream, uint32_t* overflow)
+                           uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream, uint32_t* overflow,
+                           uint32_t* spill_buf, const uint32_t* n_ptr, uint32_t* comp_out)
 {
     if (n == 0) return SGPU_OK;
     sgpu_context ctx = scene->ctx;
@@ -768,16 +810,18 @@ int trace_launch_instanced(sgpu_scene scene, const sgpu_rays_soa* rays, const sg
     uint32_t blocks = (n + SGPU_TRACE_BLOCK - 1) / SGPU_TRACE_BLOCK;
     const uint32_t max_blocks = cus * std::min(ctx->trace_blocks_per_cu, kMaxTraceBlocksPerCu);
     if (blocks > max_blocks) blocks = max_blocks;
-    uint32_t* spill = nullptr;
-    SGPU_HIP_TRY(hipMallocAsync((void**)&spill, sgpu_trace_spill_words(ctx, scene->stack_cap) * sizeof(uint32_t),
-                                stream));
+    uint32_t* spill = spill_buf;
+    if (!spill)
+        SGPU_HIP_TRY(hipMallocAsync((void**)&spill, sgpu_trace_spill_words(ctx, scene->stack_cap) * sizeof(uint32_t),
+                                    stream));
     sgpu_hits_soa h = {};
     if (hits) h = *hits;
     unsigned long long* st = (unsigned long long*)d_stats;
     const dim3 g(blocks), b(SGPU_TRACE_BLOCK);
 #define SGPU_LAUNCH_I(A, S)                                                                                        \
-    hipLaunchKernelGGL((k_trace8i<A, S>), g, b, 0, stream, scene->d_wide, scene->n_inst, scene->d_inst,          \
-                       scene->d_inst_ids, *rays, h, occluded, n, spill, scene->stack_cap, st, overflow)
+    hipLaunchKernelGGL((k_trace8i<A, S, false>), g, b, 0, stream, scene->d_wide, nullptr, 0u, scene->n_inst,      \
+                       scene->d_inst, scene->d_inst_ids, nullptr, nullptr, *rays, h, occluded, n, n_ptr, spill,    \
+                       scene->stack_cap, st, overflow, comp_out)
     if (any_hit) {
         if (st) SGPU_LAUNCH_I(true, true);
         else SGPU_LAUNCH_I(true, false);
@@ -787,7 +831,7 @@ int trace_launch_instanced(sgpu_scene scene, const sgpu_rays_soa* rays, const sg
     }
 #undef SGPU_LAUNCH_I
     SGPU_HIP_TRY(hipGetLastError());
-    SGPU_HIP_TRY(hipFreeAsync(spill, stream));
+    if (!spill_buf) SGPU_HIP_TRY(hipFreeAsync(spill, stream));
     return SGPU_OK;
 }
 
@@ -795,13 +839,11 @@ int trace_launch_instanced(sgpu_scene scene, const sgpu_rays_soa* rays, const sg
 
 int sgpu_trace_launch(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits, uint32_t* occluded,
                       uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream, uint32_t* spill_buf,
-                      const uint32_t* n_ptr, uint32_t* overflow)
+                      const uint32_t* n_ptr, uint32_t* overflow, uint32_t* comp_out)
 {
-    if (!scene->instances.empty()) {
-        if (n_ptr) return sgpu_fail(SGPU_E_INVALID, "instanced scenes take host ray counts");
+    if (!scene->instances.empty())
         return trace_launch_instanced(scene, rays, hits, occluded, n, any_hit, d_stats, stream,
-                                      overflow ? overflow : scene->d_overflow);
-    }
+                                      overflow ? overflow : scene->d_overflow, spill_buf, n_ptr, comp_out);
     return trace_launch(scene->ctx, scene->d_nodes, scene->d_wide, scene->n_wide, scene->d_tris, scene->n_tris,
                         scene->stack_cap, overflow ? overflow : scene->d_overflow, nullptr, nullptr, rays, hits,
                         occluded, n, any_hit, d_stats, stream, spill_buf, n_ptr);
@@ -810,8 +852,8 @@ int sgpu_trace_launch(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hi
 int sgpu_trace_launch_multi(sgpu_context ctx, const TraceScene* table, const uint32_t* ray_scene, uint32_t stack_cap,
                             uint32_t* overflow, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits,
                             uint32_t* occluded, uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream,
-                            uint32_t* spill_buf, const uint32_t* n_ptr)
+                            uint32_t* spill_buf, const uint32_t* n_ptr, bool instanced, uint32_t* comp_out)
 {
     return trace_launch(ctx, nullptr, nullptr, 0, nullptr, 0, stack_cap, overflow, table, ray_scene, rays, hits, occluded,
-                        n, any_hit, d_stats, stream, spill_buf, n_ptr);
+                        n, any_hit, d_stats, stream, spill_buf, n_ptr, instanced, comp_out);
 }

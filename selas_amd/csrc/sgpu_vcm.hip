@@ -165,7 +165,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         p = queue[j];
         const Slice& sl = w.slices[ps.slice[p]];
         scn = sl.scene;
-        uint32_t geom = hits.geom[j];
+        uint32_t geom;
+        f3 ng;
+        hit_geometry(sl, hits, j, geom, ng);
         if (geom != SGPU_INVALID_ID && geom < sl.n_geoms) {
             PcgUniforms u{rng_load(ps, p, sl, 0)};
             f3 o = mk3(rays.ox[j], rays.oy[j], rays.oz[j]);
@@ -174,7 +176,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             Surface s;
             f3 texel;
             const bool tex = hit_texel(sl, geom, mat, hits.prim[j], hits.u[j], hits.v[j], texel);
-            surface_from_hit(o, d, hits.t[j], mk3(hits.ngx[j], hits.ngy[j], hits.ngz[j]), mat, tex ? &texel : nullptr,
+            surface_from_hit(o, d, hits.t[j], ng, mat, tex ? &texel : nullptr,
                              s);
             f3 x = s.position;
             f3 view = -d;
@@ -500,7 +502,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         si = ps.slice[p];
         for (uint32_t k = 0; k < nslots; ++k) S.valid[(size_t)p * nslots + k] = 0;
         const Slice& sl = w.slices[si];
-        uint32_t geom = hits.geom[j];
+        uint32_t geom;
+        f3 ng;
+        hit_geometry(sl, hits, j, geom, ng);
         f3 d = mk3(rays.dx[j], rays.dy[j], rays.dz[j]);
         if (geom == SGPU_INVALID_ID || geom >= sl.n_geoms) {
             const uint32_t plen_miss = ps.plen[p];
@@ -527,7 +531,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             const float* mat = sl.materials + 22 * (size_t)sl.geom_material[geom];
             f3 texel;
             const bool tex = hit_texel(sl, geom, mat, hits.prim[j], hits.u[j], hits.v[j], texel);
-            surface_from_hit(o, d, hits.t[j], mk3(hits.ngx[j], hits.ngy[j], hits.ngz[j]), mat, tex ? &texel : nullptr,
+            surface_from_hit(o, d, hits.t[j], ng, mat, tex ? &texel : nullptr,
                              s);
             x = s.position;
             view = -d;
@@ -915,11 +919,11 @@ struct ServerJob {
 struct GraphKey {
     const void* table = nullptr;
     const void* spill = nullptr;
-    uint32_t max_stack = 0, tree = 0, blocks = 0, lds = 0;
+    uint32_t max_stack = 0, tree = 0, blocks = 0, lds = 0, inst = 0;
     bool operator==(const GraphKey& o) const
     {
         return table == o.table && spill == o.spill && max_stack == o.max_stack && tree == o.tree &&
-               blocks == o.blocks && lds == o.lds;
+               blocks == o.blocks && lds == o.lds && inst == o.inst;
     }
 };
 
@@ -969,6 +973,7 @@ struct sgpu_vcm_server_s {
     uint32_t* h = nullptr;
     Slice* h_slices = nullptr;
     TraceScene* h_table = nullptr;
+    bool any_instanced = false;  // an instanced scene is in the table (traversal by k_trace8i)
     // scenes seen by the server (traversal table), jobs in admission order
     std::vector<sgpu_scene> scenes;
     TraceScene* d_table = nullptr;
@@ -1013,6 +1018,7 @@ int server_graphs(sgpu_vcm_server sv, F&& pass_fn)
     key.tree = ctx->trace_tree;
     key.blocks = ctx->trace_blocks_per_cu;
     key.lds = ctx->trace_lds_nodes;
+    key.inst = sv->any_instanced ? 1u : 0u;
     if (sv->g_pass[0] && sv->g_pass[1] && key == sv->gkey) return SGPU_OK;
     for (int k = 0; k < 2; ++k) {
         if (sv->g_pass[k]) (void)hipGraphExecDestroy(sv->g_pass[k]);
@@ -1069,6 +1075,7 @@ int server_alloc(sgpu_vcm_server sv)
     hb.ngz = A.get<float>(TM);
     hb.prim = A.get<uint32_t>(TM);
     hb.geom = A.get<uint32_t>(TM);
+    hb.inst = A.get<uint32_t>(TM);
     const size_t max_shadow = (size_t)TM * (mlv + 1);
     const size_t max_verts = (size_t)TM * mlv;
     sv->shadow = A.rays(max_shadow, true);
@@ -1138,6 +1145,7 @@ uint32_t server_scene(sgpu_vcm_server sv, sgpu_scene scene)
         if (sv->scenes[k] == scene) return (uint32_t)k;
     sv->scenes.push_back(scene);
     sv->table_dirty = true;
+    if (!scene->instances.empty()) sv->any_instanced = true;
     sv->max_stack = std::max(sv->max_stack, scene->stack_cap);
     return (uint32_t)(sv->scenes.size() - 1);
 }
@@ -1158,7 +1166,7 @@ int server_upload_table(sgpu_vcm_server sv, hipStream_t stream)
     }
     for (uint32_t k = 0; k < n; ++k)
         sv->h_table[k] = TraceScene{sv->scenes[k]->d_nodes, sv->scenes[k]->d_wide, sv->scenes[k]->d_tris,
-                                    sv->scenes[k]->n_tris, 0};
+                                    sv->scenes[k]->n_tris, sv->scenes[k]->n_inst, sv->scenes[k]->d_inst};
     SGPU_HIP_TRY(hipMemcpyAsync(sv->d_table, sv->h_table, sizeof(TraceScene) * n, hipMemcpyHostToDevice, stream));
     SGPU_HIP_TRY(hipStreamSynchronize(stream));
     sv->table_dirty = false;
@@ -1175,6 +1183,7 @@ static void server_reset(sgpu_vcm_server sv, const sgpu_vcm_server_config& cfg)
     sv->scenes.clear();
     sv->table_dirty = true;
     sv->max_stack = 0;
+    sv->any_instanced = false;
     sv->cfg.flags = cfg.flags;
     sv->cap = (uint32_t)std::min<uint64_t>(cfg.wavefront_paths, sv->TM);
     sv->n_closest = sv->n_shadow = sv->n_light_verts = sv->n_launches = sv->n_samples = sv->n_waves = 0;
@@ -1280,6 +1289,16 @@ int sgpu_vcm_server_submit_impl(sgpu_vcm_server sv, sgpu_scene sc, const sgpu_vc
     b.geom_tex = sc->d_geom_tex;
     b.n_geom_tex = sc->n_geom_tex;
     b.has_ibl = ibl_view(sc, b.ibl) ? 1u : 0u;
+    if (!sc->instances.empty()) {
+        b.materials = sc->d_g_materials;
+        b.geom_material = sc->d_g_geom_material;
+        b.n_geoms = sc->n_g_geoms;
+        b.geom_tex = sc->d_g_geom_tex;
+        b.n_geom_tex = sc->n_g_geoms;
+        b.instanced = 1;
+        b.inst_geom_base = sc->d_inst_geom_base;
+        b.inst_l2w = sc->d_inst_l2w;
+    }
     SGPU_HIP_TRY(hipMemsetAsync(fb, 0, 3 * (size_t)b.N * sizeof(float), stream));
     sv->active.push_back(j);
     if (id_out) *id_out = j.id;
@@ -1415,10 +1434,11 @@ int sgpu_vcm_server_step_impl(sgpu_vcm_server sv, hipStream_t stream, uint64_t* 
         auto trace = [&](const RayBuf& r, const sgpu_hits_soa* hits, uint32_t* occ, uint32_t n, const uint32_t* n_ptr,
                          bool any, uint64_t* d_st) -> int {
             sgpu_rays_soa rs = as_soa(r);
+            uint32_t* comp = hits ? hb.inst : nullptr;
             if (!single)
                 return sgpu_trace_launch_multi(sv->ctx, sv->d_table, r.scene, sv->max_stack, sv->ovf, &rs, hits, occ, n,
-                                               any, d_st, st, sv->spill, n_ptr);
-            return sgpu_trace_launch(first_scene, &rs, hits, occ, n, any, d_st, st, sv->spill, n_ptr, sv->ovf);
+                                               any, d_st, st, sv->spill, n_ptr, sv->any_instanced, comp);
+            return sgpu_trace_launch(first_scene, &rs, hits, occ, n, any, d_st, st, sv->spill, n_ptr, sv->ovf, comp);
         };
         const int ph_pass = pass == 0 ? kPhaseLight : kPhaseCamera;
         SGPU_HIP_TRY(T.begin(ph_pass));

@@ -42,6 +42,7 @@ struct RayBuf {
 struct HitBuf {
     float *t, *u, *v, *ngx, *ngy, *ngz;
     uint32_t *prim, *geom;
+    uint32_t* inst;  // composed instance of the hit (instanced scenes)
 };
 
 __device__ __forceinline__ void put_ray(const RayBuf& r, uint32_t i, f3 o, f3 d, float tnear, float tfar,
@@ -122,7 +123,25 @@ struct Slice {
     uint32_t n_geom_tex;
     uint32_t has_ibl;         // 1: the image-based light is light n_lights of the uniform pick
     IblView ibl;
+    // instanced scene: a hit's geometry is numbered inst_geom_base[k] + geom in the scene's list
+    // (materials, geom_material and geom_tex above are the scene's gathered ones) and its normal is
+    // carried to world space by inst_l2w[k]
+    uint32_t instanced;
+    const uint32_t* inst_geom_base;
+    const float* inst_l2w;
 };
+
+// The geometry (in the slice's list) and world-space geometric normal of hit j.
+__device__ __forceinline__ void hit_geometry(const Slice& sl, const HitBuf& hits, uint32_t j, uint32_t& geom, f3& ng)
+{
+    geom = hits.geom[j];
+    ng = mk3(hits.ngx[j], hits.ngy[j], hits.ngz[j]);
+    if (sl.instanced && geom != SGPU_INVALID_ID) {
+        const uint32_t k = hits.inst[j];
+        geom += sl.inst_geom_base[k];
+        ng = xform_linear(sl.inst_l2w + 9 * (size_t)k, ng);
+    }
+}
 
 // Wavefront-wide arguments.
 struct Wave {

@@ -398,6 +398,9 @@ class Scene:
         if self.handle:
             check(lib.sgpu_scene_release(self.handle))
             self.handle = None
+        for s in getattr(self, "owned", []):
+            s.release()
+        self.owned = []
 
 
 def synchronize():
@@ -440,7 +443,34 @@ PT_NEE_COSINE_ONCE = 1
 
 
 def build_render_scene(ctx: Context, sc: dict) -> Scene:
-    """A committed device scene with materials and lights from a workloads.*_scene dict."""
+    """A committed device scene with materials and lights from a workloads.*_scene dict, or an
+    instanced scene from a dict with "models" (scene dicts of geometries, materials and textures),
+    optional "subscenes" (lists of (model index, 4 x 4 local-to-world)) and "instances" (("model" |
+    "subscene", index, 4 x 4 local-to-world)); the returned scene releases the scenes it places."""
+    if "models" in sc:
+        models = [build_render_scene(ctx, dict(m, lights=np.zeros(0, sc["lights"].dtype),
+                                               background=np.zeros(3, np.float32)))
+                  for m in sc["models"]]
+        subs = []
+        for desc in sc.get("subscenes", []):
+            sub = Scene(ctx)
+            for midx, m in desc:
+                sub.add_instance(models[midx], m)
+            sub.commit()
+            subs.append(sub)
+        top = Scene(ctx)
+        for kind, idx, m in sc["instances"]:
+            top.add_instance(models[idx] if kind == "model" else subs[idx], m)
+        top.commit()
+        top.owned = subs[::-1] + models[::-1]
+        lights = np.ascontiguousarray(sc["lights"])
+        bg = np.ascontiguousarray(sc["background"], np.float32)
+        check(lib.sgpu_scene_set_lights(top.handle, lights.ctypes.data if lights.size else None, lights.shape[0],
+                                        fptr(bg)))
+        ibl = sc.get("ibl")
+        if ibl is not None:
+            top.set_ibl(ibl["rgb"], ibl.get("rotation", 0.0), ibl.get("exposure", 1.0))
+        return top
     scene = Scene(ctx)
     for g in sc["geoms"]:
         if isinstance(g, dict):

@@ -1,8 +1,7 @@
-"""Instanced scenes for the traversal tests: the composition of sgpu_scene_add_instance's two levels
-as the library performs it (row-major 3 x 4 affine transforms from the column-major float input,
-products and inverses in double with the same operations in the same order, rounded once to
-float), so the oracle's composed instances carry the device's world-to-local transforms bit for
-bit."""
+"""Instanced scenes for the traversal tests: models with their (geometry, primitive) tables and the
+composition of sgpu_scene_add_instance's two levels as the library performs it (the oracle's
+rows_from_colmajor / affine_mul / affine_inverse), so the oracle's composed instances carry the
+device's world-to-local transforms bit for bit."""
 import numpy as np
 
 from selas_amd import native as N
@@ -10,40 +9,9 @@ from selas_amd import native as N
 import oracle as O
 
 
-def rows_from_colmajor(m_colmajor):
-    """Column-major 4 x 4 floats -> row-major 3 x 4 doubles (l2w[4 r + c] = m[4 c + r])."""
-    m = np.asarray(m_colmajor, np.float32).reshape(-1)
-    return [float(m[4 * c + r]) for r in range(3) for c in range(4)]
-
-
-def affine_mul(a, b):
-    r = [0.0] * 12
-    for i in range(3):
-        for j in range(4):
-            v = a[4 * i + 0] * b[0 + j] + a[4 * i + 1] * b[4 + j] + a[4 * i + 2] * b[8 + j]
-            if j == 3:
-                v += a[4 * i + 3]
-            r[4 * i + j] = v
-    return r
-
-
-def affine_inverse(m):
-    a, b, c, d, e, f, g, h, k = m[0], m[1], m[2], m[4], m[5], m[6], m[8], m[9], m[10]
-    A = e * k - f * h
-    B = -(d * k - f * g)
-    C = d * h - e * g
-    det = a * A + b * B + c * C
-    idet = 1.0 / det
-    inv = [A * idet, -(b * k - c * h) * idet, (b * f - c * e) * idet,
-           B * idet, (a * k - c * g) * idet, -(a * f - c * d) * idet,
-           C * idet, -(a * h - b * g) * idet, (a * e - b * d) * idet]
-    r = [0.0] * 12
-    for i in range(3):
-        r[4 * i + 0] = inv[3 * i + 0]
-        r[4 * i + 1] = inv[3 * i + 1]
-        r[4 * i + 2] = inv[3 * i + 2]
-        r[4 * i + 3] = -(inv[3 * i + 0] * m[3] + inv[3 * i + 1] * m[7] + inv[3 * i + 2] * m[11])
-    return r
+rows_from_colmajor = O.rows_from_colmajor
+affine_mul = O.affine_mul
+affine_inverse = O.affine_inverse
 
 
 def random_transform(rng, scale=(0.5, 2.0), spread=4.0):

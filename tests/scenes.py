@@ -194,3 +194,70 @@ def small_sky_plane(half=1.0, radiance=1.0):
     sc["ibl"] = {"rgb": np.full((64, 128, 3), radiance, np.float32), "rotation": 0.3, "exposure": 1.0}
     sc["camera"] = {"position": (0.0, 1.2, -0.3), "look_at": (0.0, 0.0, 0.1), "up": (0.0, 1.0, 0.0), "fov": 50.0}
     return sc
+
+
+def _m4(ry=0.0, scale=1.0, t=(0.0, 0.0, 0.0), rx=0.0):
+    """4 x 4 local-to-world: rotation about x, then about y, uniform scale, translation (acts on
+    column vectors)."""
+    c, s_ = np.cos(ry), np.sin(ry)
+    cx, sx = np.cos(rx), np.sin(rx)
+    r = np.array([[c, 0, s_], [0, 1, 0], [-s_, 0, c]]) @ np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+    m = np.eye(4)
+    m[:3, :3] = r * scale
+    m[:3, 3] = t
+    return m
+
+
+def instanced_scene(ibl=False):
+    """Three placed scenes -- a closed slab (Lambert), a closed crate (Disney) and a textured
+    displaced panel (Disney with a per-face texture) -- arranged with two instance levels: a
+    subscene (crate + panel above it) placed twice, the slab and a third crate placed directly.
+    Lit by a quad and a point light (and, with `ibl`, a sky)."""
+    floor = {"geoms": [closed_box((-6, -0.2, -6), (6, 0.0, 6))],
+             "materials": np.stack([W.material(2, (0.6, 0.6, 0.55))]), "geom_material": np.zeros(1, np.uint32)}
+    crate = {"geoms": [closed_box((-0.5, 0.0, -0.5), (0.5, 1.0, 0.5))],
+             "materials": np.stack([W.material(1, (0.7, 0.4, 0.2), 0.0, 0.5, 1.5)]),
+             "geom_material": np.zeros(1, np.uint32)}
+    # the workload's textured panel kind, flat and facing +y in its own space
+    v = W.vcm_scene(seed=11, n_meshes=1, nu=8, nv=4)
+    g = dict(v["geoms"][0])
+    X, Z = np.meshgrid(np.linspace(0, 1, 9), np.linspace(0, 0.5, 5), indexing="ij")
+    g["positions"] = np.stack([X - 0.5, np.zeros_like(X), Z - 0.25], -1).reshape(-1, 3).astype(np.float32)
+    panel = {"geoms": [g], "materials": v["materials"][:1].copy(), "geom_material": np.zeros(1, np.uint32)}
+    lights = np.zeros(2, W.LIGHT_DTYPE)
+    lights[0] = (0, (0.0, 4.0, 0.0), (0.0, -1.0, 0.0), (2.0, 0.0, 0.0), (0.0, 0.0, 2.0), (6.0, 6.0, 6.0))
+    lights[1] = (1, (2.0, 3.0, -1.0), (0, 0, 0), (0, 0, 0), (0, 0, 0), (10.0, 9.0, 8.0))
+    sc = {
+        "models": [floor, crate, panel],
+        "subscenes": [[(1, _m4(0.3)), (2, _m4(0.2, 1.2, (0.0, 1.5, 0.0), rx=-1.0))]],
+        "instances": [("model", 0, _m4()), ("subscene", 0, _m4(0.0, 1.0, (-1.5, 0.0, 0.5))),
+                      ("subscene", 0, _m4(1.0, 0.9, (1.2, 0.0, -0.3))), ("model", 1, _m4(0.7, 0.7, (0.0, 0.0, 2.0)))],
+        "lights": lights,
+        "background": np.zeros(3, np.float32),
+        "camera": {"position": (0.0, 3.0, -5.0), "look_at": (0.0, 0.6, 0.5), "up": (0.0, 1.0, 0.0), "fov": 60.0},
+    }
+    if ibl:
+        sc["ibl"] = {"rgb": sky_map(32, 64), "rotation": 0.4, "exposure": 0.5}
+    return sc
+
+
+def flatten_instanced(sc):
+    """The scene of geometries an instanced scene dict describes: every composed instance's
+    geometries moved to world space (in double, rounded to float), with their materials."""
+    geoms, mats, gm = [], [], []
+    for midx, l2w, _, _ in O.compose_instances(sc):
+        m = np.array(l2w, np.float64).reshape(3, 4)
+        model = sc["models"][midx]
+        base = sum(x.shape[0] for x in mats)
+        mats.append(np.asarray(model["materials"], np.float32))
+        for gi, g in enumerate(model["geoms"]):
+            gm.append(base + int(model["geom_material"][gi]))
+            if isinstance(g, dict):
+                p = np.asarray(g["positions"], np.float64)
+                geoms.append(dict(g, positions=(p @ m[:, :3].T + m[:, 3]).astype(np.float32)))
+            else:
+                p = np.asarray(g, np.float64).reshape(-1, 3)
+                geoms.append((p @ m[:, :3].T + m[:, 3]).astype(np.float32).reshape(-1, 9))
+    out = {k: v for k, v in sc.items() if k not in ("models", "subscenes", "instances")}
+    out.update(geoms=geoms, materials=np.concatenate(mats), geom_material=np.array(gm, np.uint32))
+    return out

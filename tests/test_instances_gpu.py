@@ -194,3 +194,53 @@ def test_instancing_errors(gpu_ctx, models):
         g2.release()
         g1.release()
         da.release()
+
+
+def _close(gpu, cpu):
+    close = np.abs(gpu - cpu) <= 1e-3 * np.abs(cpu) + 1e-5
+    rmse = np.sqrt(np.mean((gpu - cpu) ** 2))
+    return close.mean(), rmse / max(cpu.mean(), 1e-12)
+
+
+@pytest.mark.parametrize("ibl", [False, True])
+def test_instanced_renders_match_oracle(gpu_ctx, ibl):
+    """The path tracer and VCM on a scene with two instance levels (placed scenes with Lambert,
+    Disney and textured materials): per pixel against the oracle (99 % within 1e-3, VCM RMSE
+    <= 2e-3 of the mean)."""
+    import oracle as O
+    import scenes
+    sc = scenes.instanced_scene(ibl=ibl)
+    scene = N.build_render_scene(gpu_ctx, sc)
+    try:
+        rs = O.InstancedRenderScene(sc)
+        print(f"instanced render ibl={ibl}", flush=True)
+        g = N.render_pt(scene, sc["camera"], 64, 48, 4, 8, 5, N.PT_NEE_COSINE_ONCE)
+        c = O.render_pt(rs, sc["camera"], 64, 48, 4, 8, 5, 1)
+        frac, _ = _close(g, c)
+        assert c[1].mean() > 0.01 and frac >= 0.99, frac
+        g = N.render_vcm(scene, sc["camera"], 64, 48, 3, 8, 0.02, 0.75, 17)
+        c, _ = O.render_vcm(rs, sc["camera"], 64, 48, 3, 8, 0.02, 0.75, 17)
+        frac, rel_rmse = _close(g, c)
+        assert c.mean() > 0.01 and frac >= 0.99 and rel_rmse <= 2e-3, (frac, rel_rmse)
+    finally:
+        scene.release()
+
+
+def test_batch_mixes_instanced_and_flat_scenes(gpu_ctx):
+    import scenes
+    inst = scenes.instanced_scene(ibl=True)
+    flat = scenes.cube_scene(lights=True)
+    si = N.build_render_scene(gpu_ctx, inst)
+    sf = N.build_render_scene(gpu_ctx, flat)
+    try:
+        jobs = [(si, inst["camera"], 40, 32, 2, 21), (sf, flat["camera"], 48, 24, 3, 22), (si, inst["camera"], 24, 24, 1, 23)]
+        alone = [N.render_vcm(s, cam, w, h, its, 10, 0.02, 0.75, seed) for s, cam, w, h, its, seed in jobs]
+        batch = N.render_vcm_batch([dict(scene=s, camera=cam, width=w, height=h, iterations=its, max_path_length=10,
+                                         base_radius=0.02, radius_alpha=0.75, seed=seed)
+                                    for s, cam, w, h, its, seed in jobs])
+        for a, b in zip(alone, batch):
+            frac, rel_rmse = _close(b, a)
+            assert a.mean() > 0.005 and frac >= 0.99 and rel_rmse <= 2e-3, (frac, rel_rmse)
+    finally:
+        si.release()
+        sf.release()

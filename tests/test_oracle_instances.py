@@ -113,3 +113,20 @@ def test_composition_helpers_invert():
         ma = np.vstack([np.array(a).reshape(3, 4), [0, 0, 0, 1]])
         mb = np.vstack([np.array(b).reshape(3, 4), [0, 0, 0, 1]])
         np.testing.assert_allclose(m, ma @ mb, atol=1e-12)
+
+
+def test_instanced_render_equals_the_flattened_scene():
+    """Rendering through two instance levels (rays moved into each placed scene, normals carried
+    back by the instance's linear part, geometries numbered as one list) gives the image of the same
+    geometry moved to world space, up to the float rounding of the two placements (which moves a few
+    hits): 99 % of pixel values within 1e-3, means within 0.2 %."""
+    import scenes
+    sc = scenes.instanced_scene(ibl=True)
+    rs = O.InstancedRenderScene(sc)
+    rf = O.RenderScene(scenes.flatten_instanced(sc))
+    for a, b in [(O.render_pt(rs, sc["camera"], 64, 48, 16, 8, 3, 1)[1], O.render_pt(rf, sc["camera"], 64, 48, 16, 8, 3, 1)[1]),
+                 (O.render_vcm(rs, sc["camera"], 64, 48, 8, 8, 0.02, 0.75, 3)[0],
+                  O.render_vcm(rf, sc["camera"], 64, 48, 8, 8, 0.02, 0.75, 3)[0])]:
+        assert b.mean() > 0.01
+        assert np.mean(np.abs(a - b) <= 1e-3 * np.abs(b) + 1e-5) >= 0.99
+        assert abs(a.mean() / b.mean() - 1) < 2e-3
