@@ -559,7 +559,7 @@ def bench_serve(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=("vcm", "traversal", "batch", "serve"), default="vcm")
+    ap.add_argument("--workload", choices=("vcm", "traversal", "batch", "serve", "instances"), default="vcm")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (vcm: 32 iterations, traversal: 10)")
     ap.add_argument("--warmup", type=int, default=2)
@@ -613,6 +613,10 @@ def main():
         bench_batch(args)
     elif args.workload == "serve":
         bench_serve(args)
+    elif args.workload == "instances":
+        if args.steps is None:
+            args.steps = 10
+        bench_instances(args)
     else:
         if args.steps is None:
             args.steps = 10
@@ -727,6 +731,121 @@ def bench_traversal(args):
             "tris_per_ray": round(tris_per_ray, 3),
         },
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def _rotation(rng):
+    q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+    return q * np.sign(np.diag(r))
+
+
+def bench_instances(args):
+    """Ray casts through two instance levels: one 8192-triangle cluster placed 16 times in a subscene,
+    the subscene placed 16 times in the scene (256 composed instances, 2,097,152 triangles in world
+    space), against the same triangles moved to world space as one scene of geometries, on the
+    traversal workload's rays (8 camera samples per pixel at 1024 x 1024 plus as many cosine
+    bounce rays from the world triangles)."""
+    from selas_amd import native as N
+    from selas_amd import workloads as W
+
+    log("building instanced workload")
+    rng = np.random.default_rng(2024)
+    model = N.gen_cluster_soup(2024, 1, 8192, 0.02, 1e-4, 1e-2)
+    model = (model.reshape(-1, 3) - model.reshape(-1, 3).mean(0)).reshape(-1, 9).astype(np.float32)
+    sub = [np.vstack([np.hstack([_rotation(rng), rng.uniform(-0.15, 0.15, (3, 1))]), [0, 0, 0, 1]]) for _ in range(16)]
+    top = [np.vstack([np.hstack([_rotation(rng), rng.uniform(0.2, 0.8, (3, 1))]), [0, 0, 0, 1]]) for _ in range(16)]
+    world = []
+    p = model.reshape(-1, 3).astype(np.float64)
+    for a in top:
+        for b in sub:
+            m = a @ b
+            world.append((p @ m[:3, :3].T + m[:3, 3]).astype(np.float32).reshape(-1, 9))
+    world = np.concatenate(world)
+    rays = W.traversal_rays(world)
+    n = int(rays["org_x"].shape[0])
+
+    ctx = N.Context(0)
+    if args.trace_blocks:
+        ctx.set_tuning(trace_blocks_per_cu=args.trace_blocks)
+    dm = N.Scene(ctx)
+    dm.add_soup(model)
+    dm.commit()
+    ds = N.Scene(ctx)
+    for b in sub:
+        ds.add_instance(dm, b)
+    ds.commit()
+    inst = N.Scene(ctx)
+    tb = time.perf_counter()
+    for a in top:
+        inst.add_instance(ds, a)
+    inst.commit()
+    build_ms = (time.perf_counter() - tb) * 1e3
+    flat = N.Scene(ctx)
+    flat.add_soup(world)
+    flat.commit()
+    log(f"instanced: {inst.info()}, device bytes {inst.device_bytes() + ds.device_bytes() + dm.device_bytes()}; "
+        f"flat: {flat.info()}, device bytes {flat.device_bytes()}")
+
+    dr = N.DeviceRays(ctx, rays)
+    dh = N.DeviceHits(ctx, n, with_ng=False)
+    res = {}
+    for name, scene in (("instanced", inst), ("flat", flat)):
+        scene.intersect(dr, dh)
+        h = dh.to_host()
+        tfar, _ = W.anyhit_tfar(h["t"], h["prim_id"])
+        rays_any = dict(rays)
+        rays_any["tfar"] = tfar
+        dra = N.DeviceRays(ctx, rays_any)
+        occ = ctx.empty(4 * n)
+        st = scene.intersect_stats(dr)
+        for _ in range(args.warmup):
+            scene.intersect(dr, dh)
+            scene.occluded(dra, occ)
+        N.synchronize()
+        ev = [(N.Event(), N.Event(), N.Event()) for _ in range(args.steps)]
+        for a, b, c in ev:
+            a.record(None)
+            scene.intersect(dr, dh)
+            b.record(None)
+            scene.occluded(dra, occ)
+            c.record(None)
+        N.synchronize()
+        scene.check()
+        mi = sum(a.elapsed_ms(b) for a, b, _ in ev) / args.steps
+        mo = sum(b.elapsed_ms(c) for _, b, c in ev) / args.steps
+        res[name] = {"closest_hit_mrays_s": round(n / (mi / 1e3) / 1e6, 2), "any_hit_mrays_s": round(n / (mo / 1e3) / 1e6, 2),
+                     "ms_per_step": round(mi + mo, 3), "nodes_per_ray": round(st["nodes_visited"] / n, 3),
+                     "tris_per_ray": round(st["tris_tested"] / n, 3), "hit_rate": round(float((h["prim_id"] != N.INVALID_ID).mean()), 4)}
+        log(f"{name}: {res[name]}")
+        occ.free()
+    r = res["instanced"]
+    bytes_per_ray = 48 + r["nodes_per_ray"] * 128 + r["tris_per_ray"] * 48
+    ms_i = 1e3 * n / (r["closest_hit_mrays_s"] * 1e6)
+    achieved = n * bytes_per_ray / (ms_i / 1e3) / 1e9
+    out = {
+        "metric": "instanced closest + any-hit Mrays/s",
+        "value": round(2 * n / (r["ms_per_step"] / 1e3) / 1e6, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "instanced_traversal_16x16x8k_16Mray", "composed_instances": 256, "levels": 2,
+                   "triangles_stored": int(model.shape[0]), "triangles_world": int(world.shape[0]), "rays": n},
+        "instanced": res["instanced"],
+        "flat_same_triangles": res["flat"],
+        "top_tree_build_ms": round(build_ms, 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_trace8i<false,false,false> (closest hit; 64-B instance records not counted)",
+                     "bytes_per_ray": round(bytes_per_ray, 1)},
+        "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
 

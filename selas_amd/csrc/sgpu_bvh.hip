@@ -738,8 +738,37 @@ int sgpu_bvh_build_instances(sgpu_scene scene, hipStream_t stream)
     scene->inst_levels = levels;
     if (n == 0) return SGPU_OK;
 
-    // world boxes (the 8 corners of each placed scene's box through its transform, in double,
-    // rounded outward), centroids and the bounds k_morton and k_fit read
+    // The top tree's box of an instance: the boxes of the placed scene's root children (decoded
+    // from its 8-wide root as the traversal decodes them) through the transform, their 8 corners in
+    // double, united and rounded outward: much tighter than the transformed box of the whole scene
+    // when the transform rotates. The scene's own box (the IBL's emission sphere) takes the 8
+    // corners of each placed scene's box.
+    std::vector<std::pair<sgpu_scene, std::vector<float>>> root_boxes;
+    auto child_boxes = [&](sgpu_scene b, std::vector<float>& out) -> int {
+        for (const auto& rb : root_boxes)
+            if (rb.first == b) {
+                out = rb.second;
+                return SGPU_OK;
+            }
+        WideNode w;
+        SGPU_HIP_TRY(hipMemcpy(&w, b->d_wide, sizeof(WideNode), hipMemcpyDeviceToHost));
+        const uint32_t nch = w.meta >> 24;
+        const float o[3] = {w.ox, w.oy, w.oz};
+        float sc[3];
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t bits = ((w.meta >> (8 * a)) & 0xFFu) << 23;
+            std::memcpy(&sc[a], &bits, sizeof(float));
+        }
+        out.clear();
+        for (uint32_t c = 0; c < nch && c < 8; ++c)
+            for (int side = 0; side < 2; ++side)
+                for (int a = 0; a < 3; ++a) {
+                    const uint32_t word = side ? w.qhi[a][c >> 2] : w.qlo[a][c >> 2];
+                    out.push_back(std::fma((float)((word >> (8 * (c & 3))) & 0xFFu), sc[a], o[a]));
+                }
+        root_boxes.emplace_back(b, out);
+        return SGPU_OK;
+    };
     scene->comp_blas.clear();
     scene->comp_l2w.clear();
     std::vector<float4> blo(n), bhi(n), cent(n);
@@ -752,23 +781,39 @@ int sgpu_bvh_build_instances(sgpu_scene scene, hipStream_t stream)
         const Flat& f = flat[k];
         double w2l[12];
         if (!affine_inverse(f.l2w, w2l)) return sgpu_fail(SGPU_E_INVALID, "instance transform is singular");
-        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-        for (int c = 0; c < 8; ++c) {
-            const double p[3] = {f.blas->box[(c & 1) ? 3 : 0], f.blas->box[(c & 2) ? 4 : 1], f.blas->box[(c & 4) ? 5 : 2]};
-            for (int r = 0; r < 3; ++r) {
-                const double w = f.l2w[4 * r] * p[0] + f.l2w[4 * r + 1] * p[1] + f.l2w[4 * r + 2] * p[2] + f.l2w[4 * r + 3];
-                lo[r] = std::min(lo[r], w);
-                hi[r] = std::max(hi[r], w);
+        // the 8 corners of a local box through the transform, in double
+        auto world_box = [&](const float* b6, double* lo, double* hi) {
+            for (int c = 0; c < 8; ++c) {
+                const double p[3] = {b6[(c & 1) ? 3 : 0], b6[(c & 2) ? 4 : 1], b6[(c & 4) ? 5 : 2]};
+                for (int r = 0; r < 3; ++r) {
+                    const double w =
+                        f.l2w[4 * r] * p[0] + f.l2w[4 * r + 1] * p[1] + f.l2w[4 * r + 2] * p[2] + f.l2w[4 * r + 3];
+                    lo[r] = std::min(lo[r], w);
+                    hi[r] = std::max(hi[r], w);
+                }
             }
-        }
+        };
+        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        world_box(f.blas->box, lo, hi);
+        std::vector<float> cb;
+        int rcb = child_boxes(f.blas, cb);
+        if (rcb != SGPU_OK) return rcb;
+        double tlo[3] = {INFINITY, INFINITY, INFINITY}, thi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (size_t c = 0; c + 6 <= cb.size(); c += 6) world_box(cb.data() + c, tlo, thi);
+        if (cb.empty())
+            for (int r = 0; r < 3; ++r) {
+                tlo[r] = lo[r];
+                thi[r] = hi[r];
+            }
         float l[3], h[3];
         for (int r = 0; r < 3; ++r) {
-            l[r] = std::nextafter((float)lo[r], -INFINITY);
-            h[r] = std::nextafter((float)hi[r], INFINITY);
-            if (!std::isfinite(l[r]) || !std::isfinite(h[r]))
+            const float sl = std::nextafter((float)lo[r], -INFINITY), sh = std::nextafter((float)hi[r], INFINITY);
+            l[r] = std::nextafter((float)std::max(tlo[r], lo[r]), -INFINITY);
+            h[r] = std::nextafter((float)std::min(thi[r], hi[r]), INFINITY);
+            if (!std::isfinite(sl) || !std::isfinite(sh))
                 return sgpu_fail(SGPU_E_RANGE, "instance box is not finite in single precision");
-            slo[r] = std::min(slo[r], l[r]);
-            shi[r] = std::max(shi[r], h[r]);
+            slo[r] = std::min(slo[r], sl);
+            shi[r] = std::max(shi[r], sh);
             const float cc = 0.5f * (l[r] + h[r]);
             clo[r] = std::min(clo[r], cc);
             chi[r] = std::max(chi[r], cc);

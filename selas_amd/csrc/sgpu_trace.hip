@@ -565,6 +565,9 @@ __global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace8i(const WideNode* __
         int node = 0;
         const bool walking = has_inst || tn > 0;
         while (walking && !overflowed) {
+            // Leaving or entering an instance falls through to the node visit of the same trip: the
+            // entry popped after an exit is a top-tree entry (exits are only pushed on entering an
+            // instance from the top tree), and an entered instance starts at its root.
             if (node == kInstExit) {
                 // back in the top tree
                 o = wo;
@@ -574,7 +577,6 @@ __global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace8i(const WideNode* __
                 rw = tw;
                 cur = SGPU_INVALID_ID;
                 if (!pop(node)) break;
-                continue;
             }
             if (node < 0) {
                 // a leaf of the top tree: enter composed instance ~node
@@ -587,8 +589,8 @@ __global__ __launch_bounds__(SGPU_TRACE_BLOCK) void k_trace8i(const WideNode* __
                 rw = ir.wide;
                 rt = ir.tris;
                 push(kInstExit);
+                if (overflowed) break;
                 node = 0;
-                continue;
             }
             const uint4* __restrict np = reinterpret_cast<const uint4*>(rw + node);
             const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4], w5 = np[5];
