@@ -353,6 +353,15 @@ typedef struct sgpu_vcm_stats {
     float merge_ms;            /* ... of the vertex merging (photon range searches) */
 } sgpu_vcm_stats;
 
+/* Writes each layer of a device framebuffer (layers x height x width x 3 floats, the layout the
+ * renderers write) to "<path_prefix>_<layer>.hdr": replaces FrameBuffer_Save
+ * (Source/Core/TextureLib/Framebuffer.cpp:43-59, stb_image_write's HDR writer through
+ * StbImageWrite, StbImage.cpp:64-78). Radiance RGBE, run-length coded scanlines (flat for widths
+ * below 8 or above 32767), top row first. The RGBE conversion runs on the device; synchronous on
+ * `stream`. */
+int sgpu_framebuffer_save_hdr(const char* path_prefix, const float* framebuffer, uint32_t width, uint32_t height,
+                              uint32_t layers, void* stream);
+
 /* Vertex connection and merging: replaces VCM::GenerateImage and its helpers
  * (Source/Applications/Selas/Source/VCM.h:18, VCM.cpp:669-760, VCMCommon.cpp:37-113,
  * VCMHashGrid.cpp:15-157) for quad and point lights (sgpu_scene_set_lights) and the image-based

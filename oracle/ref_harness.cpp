@@ -22,6 +22,13 @@
 
 #include <stdint.h>
 
+// stb_image_write's HDR writer (what StbImageWrite calls for FrameBuffer_Save) and stb_image's
+// reader, from the reference's own copies (Middleware/stb)
+#define STB_IMAGE_WRITE_IMPLEMENTATION
+#include "stb/stb_image_write.h"
+#define STB_IMAGE_IMPLEMENTATION
+#include "stb/stb_image.h"
+
 using namespace Selas;
 
 namespace {
@@ -283,6 +290,21 @@ int ref_ptex_eval(const char* path, uint32_t n, const int32_t* faces, const floa
     filter->release();
     tex->release();
     return 0;
+}
+
+// stbi_write_hdr as StbImageWrite calls it (StbImage.cpp:78): 3 channels, rows top first
+int ref_write_hdr(const char* path, int w, int h, const float* rgb) { return stbi_write_hdr(path, w, h, 3, rgb); }
+
+// stbi_loadf of an .hdr file: 3 channels into out (at most cap floats); returns w * h, or -1
+int ref_read_hdr(const char* path, float* out, int cap)
+{
+    int w = 0, h = 0, comp = 0;
+    float* data = stbi_loadf(path, &w, &h, &comp, 3);
+    if (!data) return -1;
+    const int n = w * h;
+    for (int i = 0; i < 3 * n && i < cap; ++i) out[i] = data[i];
+    stbi_image_free(data);
+    return n;
 }
 
 }  // extern "C"

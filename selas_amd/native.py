@@ -88,6 +88,7 @@ _SIGNATURES = {
     "sgpu_scene_set_lights": [_P, _P, _U32, _FP],
     "sgpu_scene_set_ibl": [_P, _U32, _U32, _FP, _F, _F],
     "sgpu_render_pt": [_P, _P, _P, _P],
+    "sgpu_framebuffer_save_hdr": [C.c_char_p, _P, _U32, _U32, _U32, _P],
     "sgpu_render_vcm": [_P, _P, _P, _P, _P],
     "sgpu_render_vcm_batch": [_P, _P, _P, _U32, _U32, _P, _P],
     "sgpu_vcm_server_create": [_P, _P, _P, _P],
@@ -623,6 +624,24 @@ def render_vcm_batch(jobs, flags=0, stream=None, return_stats=False, keep_on_dev
     if return_stats:
         return out, {k: getattr(st, k) for k, _ in VcmStats._fields_}
     return out
+
+
+def save_hdr(ctx: Context, path_prefix: str, framebuffer, width: int, height: int, layers: int = 1, stream=None):
+    """sgpu_framebuffer_save_hdr: writes <path_prefix>_<layer>.hdr for each layer of a device
+    framebuffer (a DeviceBuffer) or of a host array (layers x height x width x 3, uploaded first)."""
+    buf, owned = framebuffer, None
+    if isinstance(framebuffer, np.ndarray):
+        a = np.ascontiguousarray(framebuffer, np.float32)
+        if a.size != layers * height * width * 3:
+            raise ValueError("save_hdr: the array is not layers x height x width x 3")
+        owned = ctx.empty(a.nbytes)
+        owned.upload(a)
+        buf = owned
+    try:
+        check(lib.sgpu_framebuffer_save_hdr(path_prefix.encode(), buf.ptr, width, height, layers, stream))
+    finally:
+        if owned is not None:
+            owned.free()
 
 
 def render_pt(scene: Scene, cam: dict, width, height, spp, max_path_length, seed, flags=0, stream=None):

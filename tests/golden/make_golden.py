@@ -9,7 +9,8 @@ Inputs are drawn from a seeded numpy generator; every output is computed by the 
 functions (EvaluateDisney, SampleDisney, the GGX/Fresnel helpers, CorrelatedMultiJitter,
 InitializeRayCastCamera + JitteredCameraRay, the spherical-rectangle sampler, RayQuad,
 SampleConcentricDisc, BuildHashGrid + SearchHashGrid, PtexFilter f_bspline over Ptex files written
-by tests/golden/ptx.py). The fixtures are data only. `--only ptex` rewrites just the Ptex cases.
+by tests/golden/ptx.py, stbi_write_hdr + stbi_loadf for the .hdr output). The fixtures are data only.
+`--only ptex` / `--only hdr` rewrite just those cases.
 """
 from __future__ import annotations
 
@@ -296,12 +297,42 @@ def ptex_cases():
     return out
 
 
+def hdr_cases():
+    """stbi_write_hdr (FrameBuffer_Save's writer) on two small images -- 21 x 6 (run-length coded
+    scanlines: runs of equal pixels, zeros, values below 1e-32, large and tiny components) and
+    5 x 3 (flat) -- with the file's bytes and stbi_loadf's reading of it."""
+    import tempfile
+
+    lib = C.CDLL(LIB)
+    rng = np.random.default_rng(424242)
+    out = {}
+    for k, (w, h) in enumerate([(21, 6), (5, 3)]):
+        img = (rng.lognormal(-1.0, 2.0, (h, w, 3)) * (rng.random((h, w, 1)) < 0.9)).astype(np.float32)
+        img[0, 2:9] = [0.25, 0.5, 0.75]          # a run of equal pixels
+        img[1, :] = 0.0                           # a black row
+        img[2, 0] = [1e-35, 1e-36, 0.0]           # below the smallest RGBE value
+        img[2, 1] = [3e4, 1e-3, 7.0]              # components far apart
+        img[h - 1, w - 1] = [1.0, 1.0, 1.0]
+        with tempfile.TemporaryDirectory() as td:
+            path = os.path.join(td, "x.hdr").encode()
+            assert lib.ref_write_hdr(path, w, h, fp(np.ascontiguousarray(img)))
+            data = open(path, "rb").read()
+            dec = np.zeros((h, w, 3), np.float32)
+            assert lib.ref_read_hdr(path, fp(dec), dec.size) == w * h
+        out[f"hdr_{k}"] = dict(image=img, bytes=np.frombuffer(data, np.uint8), decoded=dec)
+    return out
+
+
 if __name__ == "__main__":
     import sys
 
     sys.path.insert(0, HERE)
-    if "--only" in sys.argv and sys.argv[sys.argv.index("--only") + 1] == "ptex":
+    only = sys.argv[sys.argv.index("--only") + 1] if "--only" in sys.argv else None
+    if only == "ptex":
         write(ptex_cases())
+    elif only == "hdr":
+        write(hdr_cases())
     else:
         main()
         write(ptex_cases())
+        write(hdr_cases())
