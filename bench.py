@@ -127,7 +127,8 @@ def bench_vcm(args):
     cam = sc["camera"]
     fb = ctx.empty(3 * w * w * 4)
 
-    tuning = apply_tuning(args, ctx, lambda c: N.render_vcm(scene, cam, w, w, steps, L, r, alpha, 1, framebuffer=fb))
+    tuning = apply_tuning(args, ctx, lambda c: N.render_vcm(scene, cam, w, w, steps, L, r, alpha, 1, framebuffer=fb),
+                          workload=f"vcm_{w}x{w}_{steps}it_L{L}_{n_tris}tri")
     # traversal counts over the closest-hit rays of one untimed iteration, on the tree in use
     _, cnt = N.render_vcm(scene, cam, w, w, 1, L, r, alpha, 1, iteration_base=0, return_stats=True,
                           flags=N.VCM_COUNT_TRAVERSAL, framebuffer=fb)
@@ -257,9 +258,14 @@ def bench_vcm(args):
     print(json.dumps(out), flush=True)
 
 
-def apply_tuning(args, ctx, probe):
-    """Launch shape for the timed runs: given on the command line, the defaults (--no-tune), or
-    found by the autotuner (selas_amd/autotune.py) with `probe` (the timed workload, run once)."""
+TUNING_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "selas_amd", "tuning.json")
+
+
+def apply_tuning(args, ctx, probe, workload=None):
+    """Launch shape for the timed runs: given on the command line, the defaults (--no-tune), the
+    shape the autotuner stored for this GPU architecture and workload (selas_amd/tuning.json, unless
+    --retune), or found by the autotuner (selas_amd/autotune.py) with `probe` (the timed workload,
+    run once)."""
     from selas_amd import autotune as A
 
     if args.trace_tree is not None:
@@ -269,6 +275,12 @@ def apply_tuning(args, ctx, probe):
         return dict(ctx.tuning(), source="command line")
     if args.no_tune:
         return dict(ctx.tuning(), source="defaults")
+    arch = ctx.device_info()["name"].split(":")[0]
+    key = f"{arch}|{workload}"
+    stored = None if (args.retune or workload is None) else A.load(TUNING_FILE, key)
+    if stored:
+        ctx.set_tuning(**{k: v for k, v in stored.items() if k in ctx.tuning()})
+        return dict(ctx.tuning(), source=f"stored autotuner result ({os.path.basename(TUNING_FILE)}: {key})")
     t0 = time.perf_counter()
     r = A.tune(ctx, probe, log=log)
     return dict(r["best"], source="autotuned", tuned_ms=round(r["seconds"] * 1e3, 1),
@@ -579,6 +591,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="vcm: skip the batched leg")
     ap.add_argument("--no-serve", action="store_true", help="vcm: skip the serving leg")
     ap.add_argument("--no-tune", action="store_true", help="use the default launch shape instead of autotuning")
+    ap.add_argument("--retune", action="store_true",
+                    help="run the autotuner even where selas_amd/tuning.json holds a result")
     ap.add_argument("--no-phase-times", action="store_true",
                     help="vcm: time the render without per-launch events (lets it replay captured graphs)")
     ap.add_argument("--wavefront-paths", type=int, default=None, help="VCM paths per wavefront (skips autotuning)")
