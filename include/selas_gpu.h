@@ -316,6 +316,27 @@ typedef struct sgpu_pt_job {
  * averaged over spp. Synchronous on `stream`. */
 int sgpu_render_pt(sgpu_scene scene, const sgpu_pt_job* job, float* framebuffer, void* stream);
 
+#define SGPU_DPT_SORT_HITS 1u /* shade each bounce's hits in (instID[0], instID[1], geomID, primID) order */
+
+typedef struct sgpu_dpt_job {
+    sgpu_camera camera;
+    uint32_t spp_x, spp_y;    /* CMJ strata per pixel (SamplesPerPixelX_ x SamplesPerPixelY_) */
+    uint32_t max_path_length; /* bound on segments per path (Selas ends paths by Russian roulette
+                                 alone, after MaxTrackedBounces_ = 7) */
+    uint64_t seed;            /* PCG32 seed; path i uses stream i */
+    uint32_t flags;           /* SGPU_DPT_* */
+} sgpu_dpt_job;
+
+/* The deferred path tracer: replaces DeferredPathTracer::GenerateImage
+ * (Source/Applications/Selas/Source/DeferredPathTracer.cpp:388-430) with its ShadeHitPosition
+ * estimator (:61-148): CMJ primary rays, NEE toward the quad and point lights, a sky sample (the
+ * image-based light, else the constant background) and the BSDF's continuation weighted against
+ * each other by the balance heuristic, Russian roulette after 7 bounces. SGPU_DPT_SORT_HITS orders
+ * the shading as PathTracingBatcher::GetSortedHits (PathTracingBatcher.cpp:221-258, 799-830) does;
+ * the image does not depend on it. framebuffer: a device array of height x width x 3 floats (one
+ * layer, OutputLayers_ = 1), averaged over spp_x x spp_y. Synchronous on `stream`. */
+int sgpu_render_deferred_pt(sgpu_scene scene, const sgpu_dpt_job* job, float* framebuffer, void* stream);
+
 #define SGPU_VCM_KERNEL_TIMES 1u    /* time each phase with events (sgpu_vcm_stats.*_ms) */
 #define SGPU_VCM_COUNT_TRAVERSAL 2u /* count nodes and triangles visited by closest-hit rays (slower) */
 #define SGPU_VCM_ONE_SLICE_PER_WAVEFRONT 4u /* run each iteration of each job as its own wavefront (the

@@ -366,6 +366,23 @@ def render_pt(rs: RenderScene, cam: dict, width, height, spp, max_path_length, s
     return fb
 
 
+class _DptJob(C.Structure):
+    _fields_ = [("cam10", C.c_float * 10), ("width", C.c_uint32), ("height", C.c_uint32), ("spp_x", C.c_uint32),
+                ("spp_y", C.c_uint32), ("max_path_length", C.c_uint32), ("seed", C.c_uint64)]
+
+
+lib.orc_render_dpt.argtypes = [C.POINTER(_Scene), C.POINTER(_DptJob), C.c_void_p, C.c_int]
+
+
+def render_dpt(rs: RenderScene, cam: dict, width, height, spp_x, spp_y, max_path_length, seed, threads=0):
+    """The deferred path tracer's estimator (orc_pt.c, orc_render_dpt): height x width x 3."""
+    cam10 = list(cam["position"]) + list(cam["look_at"]) + list(cam["up"]) + [cam["fov"]]
+    job = _DptJob((C.c_float * 10)(*cam10), width, height, spp_x, spp_y, max_path_length, seed)
+    fb = np.zeros((height, width, 3), np.float32)
+    lib.orc_render_dpt(C.byref(rs.s), C.byref(job), fb.ctypes.data, _threads(threads))
+    return fb
+
+
 class _VcmJob(C.Structure):
     _fields_ = [("cam10", C.c_float * 10), ("width", C.c_uint32), ("height", C.c_uint32),
                 ("iterations", C.c_uint32), ("iteration_base", C.c_uint32), ("max_path_length", C.c_uint32),

@@ -221,6 +221,14 @@ void orc_nee(orc_uniforms* u, const orc_light* lights, uint32_t n_lights, const 
 /* fb: 2 x height x width x 3 floats (layer 0 first-bounce reflectance, layer 1 radiance) */
 void orc_render_pt(const orc_scene* sc, const orc_pt_job* job, float* fb, int threads);
 
+typedef struct {
+    float cam10[10]; /* position, look_at, up, fov_degrees */
+    uint32_t width, height, spp_x, spp_y, max_path_length;
+    uint64_t seed;
+} orc_dpt_job;
+/* the deferred path tracer's estimator; fb: height x width x 3 floats */
+void orc_render_dpt(const orc_scene* sc, const orc_dpt_job* job, float* fb, int threads);
+
 /* VCM (orc_vcm.c) */
 typedef struct {
     float lo[3], hi[3];

@@ -405,3 +405,168 @@ void orc_render_pt(const orc_scene* sc, const orc_pt_job* job, float* fb, int th
     c.npix = job->width * job->height;
     orc_parallel_for(pt_range, &c, c.npix, threads);
 }
+
+/* ---- the deferred path tracer's estimator --------------------------------------------------------
+ * DeferredPathTracer::ShadeHitPosition / TraceRayBatch / TraceOcclusionBatch / GeneratePrimaryRays
+ * (Source/Applications/Selas/Source/DeferredPathTracer.cpp:61-351). Per path, in this order of
+ * random numbers: at each hit NEE toward the quad and point lights (light pick + 2, weight 1:
+ * lights are not hittable), a sky sample (2: SampleBackground, AreaLighting.cpp:293-314, 334-350)
+ * weighted by the balance heuristic against the BSDF's pdf, a BSDF sample (1 + 2..3), and after 7
+ * tracked bounces Russian roulette (1) on the largest throughput component. Primary rays are
+ * CMJ-jittered (JitteredCameraRay with s of spp_x x spp_y, pattern = pixel index; no random
+ * numbers). A ray leaving the scene adds the sky (EvaluateBackground) times the throughput, weighted
+ * by the balance heuristic of the BSDF pdf it was sampled with against the sky's pdf; unweighted
+ * while every scatter so far was a Dirac event, or when the last one was. Deviations, documented:
+ * the MIS weight of the continuation multiplies the sky it finds, not the path's throughput (Selas
+ * weights the throughput, which darkens all later lighting of the path); the constant background
+ * is sampled with its pdf 1/(4 pi) (BackgroundLightSample sets 1 while BackgroundLightingPdf
+ * returns 1/(4 pi)); max_path_length bounds the segments (Selas ends paths by roulette only);
+ * one PCG32 stream per path as in orc_render_pt. Shadow rays start at tnear 0 from the origin
+ * offset by 0.1 error, as TraceOcclusionBatch sets them. One output layer. */
+typedef struct {
+    const orc_scene* sc;
+    const orc_dpt_job* job;
+    orc_camera cam;
+    float* fb;
+} dpt_ctx;
+
+#define DPT_INV_4PI 0.0795774715459476678844f
+static const float kDptPi = 3.141592653589793f;
+
+/* SampleBackground: direction, radiance, solid-angle pdf */
+static void dpt_sky_sample(const orc_scene* sc, orc_uniforms* u, float* dir, float* rad, float* pdf)
+{
+    const float r0 = orc_uniform(u), r1 = orc_uniform(u);
+    if (sc->ibl) {
+        float s8[8];
+        orc_ibl_direct(sc->ibl, r0, r1, s8);
+        memcpy(dir, s8, 3 * sizeof(float));
+        memcpy(rad, s8 + 3, 3 * sizeof(float));
+        *pdf = s8[6];
+        return;
+    }
+    const float uu = 2.0f * r1 - 1.0f;
+    const float norm = sqrtf(maxf(0.0f, 1.0f - uu * uu));
+    const float theta = (kDptPi * 2.0f) * r0;
+    dir[0] = norm * cos_f(theta);
+    dir[1] = uu;
+    dir[2] = norm * sin_f(theta);
+    memcpy(rad, sc->background, 3 * sizeof(float));
+    *pdf = DPT_INV_4PI;
+}
+
+/* EvaluateBackground + BackgroundLightingPdf */
+static float dpt_sky_lookup(const orc_scene* sc, const float* d, float* rad)
+{
+    if (sc->ibl) return orc_ibl_lookup(sc->ibl, d, rad);
+    memcpy(rad, sc->background, 3 * sizeof(float));
+    return DPT_INV_4PI;
+}
+
+static ov3 dpt_path(const dpt_ctx* c, uint32_t* stack, uint32_t x, uint32_t y, uint32_t s, uint64_t path)
+{
+    const orc_scene* sc = c->sc;
+    const orc_dpt_job* job = c->job;
+    uint64_t st, inc;
+    orc_pcg32_seed(job->seed ^ (path * 0x9E3779B97F4A7C15ull), path, &st, &inc);
+    orc_uniforms u = {NULL, 0, 0, &st, inc};
+    ov3 l = v3(0, 0, 0), thr = v3(1, 1, 1);
+    float o[3] = {c->cam.pos.x, c->cam.pos.y, c->cam.pos.z}, d[3];
+    const int32_t pixel = (int32_t)(y * job->width + x);
+    orc_camera_cmj_dir(&c->cam, (int32_t)x, (int32_t)y, (int32_t)s, (int32_t)job->spp_x, (int32_t)job->spp_y, pixel, d);
+    int dirac_only = 1, last_dirac = 1;
+    float last_pdf = 0.0f;
+    for (uint32_t bounce = 0; bounce < job->max_path_length; ++bounce) {
+        float t, hu, hv, ng[3];
+        uint32_t geom, tri_in_geom;
+        if (!orc_scene_closest1(sc, stack, o, d, 0.0f, FLOAT_MAX, &t, &hu, &hv, &geom, ng, &tri_in_geom)) {
+            float bg[3];
+            const float sky_pdf = dpt_sky_lookup(sc, d, bg);
+            float w = 1.0f;
+            if (!dirac_only && !last_dirac) w = last_pdf + sky_pdf > 0.0f ? last_pdf / (last_pdf + sky_pdf) : 0.0f;
+            l = add(l, mul(scl(ld3(bg), w), thr));
+            break;
+        }
+        const float* mat = sc->materials + 22 * (size_t)sc->geom_material[geom];
+        orc_surface sf;
+        float pos[3], err, texel[3];
+        orc_surface_from_hit(o, d, t, ng, mat, orc_hit_texel_local(sc, tri_in_geom, geom, mat, hu, hv, texel), &sf, pos,
+                             &err);
+        float v[3] = {-d[0], -d[1], -d[2]};
+        float nrm[3] = {sf.n.x, sf.n.y, sf.n.z};
+        /* NEE toward the lights */
+        float ls[8];
+        orc_nee(&u, sc->lights, sc->n_lights, pos, nrm, 1, ls);
+        if (dot(ld3(ls + 3), v3(1, 1, 1)) > 0) {
+            float f[5];
+            bsdf_eval_any(&sf, v, ls, f);
+            const ov3 smp = scl(mul(ld3(f), ld3(ls + 3)), 1.0f / ls[7]);
+            if (dot(smp, v3(1, 1, 1)) > 0) {
+                float so[3];
+                orc_offset_origin(&sf, pos, err, ls, 0.1f, so, NULL);
+                if (!orc_scene_occluded1(sc, stack, so, ls, 0.0f, ls[6])) l = add(l, mul(smp, thr));
+            }
+        }
+        /* a sky sample, MIS-weighted against the BSDF */
+        float sd[3], srad[3], spdf;
+        dpt_sky_sample(sc, &u, sd, srad, &spdf);
+        if (dot(ld3(srad), v3(1, 1, 1)) > 0 && spdf > 0.0f) {
+            float f[5];
+            bsdf_eval_any(&sf, v, sd, f);
+            const float w = spdf / (spdf + f[3]);
+            const ov3 smp = scl(mul(scl(ld3(f), w), ld3(srad)), 1.0f / spdf);
+            if (dot(smp, v3(1, 1, 1)) > 0) {
+                float so[3];
+                orc_offset_origin(&sf, pos, err, sd, 0.1f, so, NULL);
+                if (!orc_scene_occluded1(sc, stack, so, sd, 0.0f, 1e36f)) l = add(l, mul(smp, thr));
+            }
+        }
+        /* the BSDF sample continues the path */
+        float bs[14];
+        if (!bsdf_sample_any(&u, &sf, v, bs)) break;
+        thr = mul(thr, ld3(bs + 2));
+        if (dot(thr, thr) == 0.0f) break;
+        const int dirac = (((uint32_t)bs[1]) & 4u) != 0;
+        dirac_only = dirac_only && dirac;
+        last_dirac = dirac;
+        last_pdf = bs[8];
+        if (bounce >= 7) {
+            const float cp = maxf(maxf(thr.x, thr.y), thr.z);
+            if (orc_uniform(&u) >= cp) break;
+            thr = scl(thr, 1.0f / cp);
+        }
+        float no[3];
+        orc_offset_origin(&sf, pos, err, bs + 5, 1.0f, no, NULL);
+        memcpy(o, no, sizeof(no));
+        memcpy(d, bs + 5, 3 * sizeof(float));
+    }
+    return l;
+}
+
+static void dpt_range(void* cv, uint32_t begin, uint32_t end)
+{
+    const dpt_ctx* c = (const dpt_ctx*)cv;
+    uint32_t* stack = (uint32_t*)malloc(sizeof(uint32_t) * orc_scene_stack_size(c->sc));
+    const uint32_t W = c->job->width, spp = c->job->spp_x * c->job->spp_y;
+    for (uint32_t pix = begin; pix < end; ++pix) {
+        const uint32_t x = pix % W, y = pix / W;
+        ov3 a = v3(0, 0, 0);
+        for (uint32_t s = 0; s < spp; ++s) a = add(a, dpt_path(c, stack, x, y, s, (uint64_t)pix * spp + s));
+        const float inv = 1.0f / (float)spp;
+        float* f = c->fb + 3 * (size_t)pix;
+        f[0] = (0.0f + a.x) * inv;
+        f[1] = (0.0f + a.y) * inv;
+        f[2] = (0.0f + a.z) * inv;
+    }
+    free(stack);
+}
+
+void orc_render_dpt(const orc_scene* sc, const orc_dpt_job* job, float* fb, int threads)
+{
+    dpt_ctx c;
+    c.sc = sc;
+    c.job = job;
+    orc_camera_init(job->cam10, job->width, job->height, &c.cam);
+    c.fb = fb;
+    orc_parallel_for(dpt_range, &c, job->width * job->height, threads);
+}

@@ -753,6 +753,29 @@ int sgpu_render_pt(sgpu_scene scene, const sgpu_pt_job* job, float* framebuffer,
     return sgpu_pt_render_impl(scene, job, framebuffer, as_stream(stream));
 }
 
+int sgpu_render_deferred_pt(sgpu_scene scene, const sgpu_dpt_job* job, float* framebuffer, void* stream)
+{
+    const char* who = "sgpu_render_deferred_pt";
+    if (!scene || !job || !framebuffer) return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": NULL argument");
+    if (!scene->committed) return sgpu_fail(SGPU_E_STATE, std::string(who) + ": scene not committed");
+    if (!scene->instances.empty()) {
+        if (!scene->render_error.empty()) return sgpu_fail(SGPU_E_STATE, std::string(who) + ": " + scene->render_error);
+    } else if (!scene->d_materials || scene->n_geom_material < scene->geom_first.size()) {
+        return sgpu_fail(SGPU_E_STATE, std::string(who) + ": materials not set for every geometry");
+    }
+    if (job->camera.width == 0 || job->camera.height == 0 || job->spp_x == 0 || job->spp_y == 0 ||
+        job->max_path_length == 0)
+        return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": zero width, height, strata or path length");
+    if ((uint64_t)job->camera.width * job->camera.height > (1ull << 26))
+        return sgpu_fail(SGPU_E_RANGE, std::string(who) + ": more than 2^26 pixels");
+    if ((uint64_t)job->spp_x * job->spp_y > (1u << 20))
+        return sgpu_fail(SGPU_E_RANGE, std::string(who) + ": more than 2^20 samples per pixel");
+    if (job->max_path_length > 1024) return sgpu_fail(SGPU_E_RANGE, std::string(who) + ": path length above 1024");
+    if (job->flags & ~SGPU_DPT_SORT_HITS) return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": unknown flags");
+    SGPU_HIP_TRY(hipSetDevice(scene->ctx->device));
+    return sgpu_dpt_render_impl(scene, job, framebuffer, as_stream(stream));
+}
+
 static int check_vcm_job(sgpu_scene scene, const sgpu_vcm_job* job, const void* framebuffer, const char* who)
 {
     if (!scene || !job || !framebuffer) return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": NULL argument");

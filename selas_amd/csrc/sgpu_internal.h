@@ -205,6 +205,7 @@ int sgpu_trace_launch_multi(sgpu_context ctx, const TraceScene* table, const uin
                             uint32_t* comp_out = nullptr);
 // Wavefront path tracer (sgpu_pt.hip); arguments validated by sgpu_render_pt.
 int sgpu_pt_render_impl(sgpu_scene scene, const sgpu_pt_job* job, float* framebuffer, hipStream_t stream);
+int sgpu_dpt_render_impl(sgpu_scene scene, const sgpu_dpt_job* job, float* framebuffer, hipStream_t stream);
 int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* framebuffer, sgpu_vcm_stats* stats,
                          hipStream_t stream);
 int sgpu_vcm_render_batch_impl(const sgpu_scene* scenes, const sgpu_vcm_job* jobs, float* const* framebuffers,

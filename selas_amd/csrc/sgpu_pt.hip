@@ -11,12 +11,23 @@
 // path's result does not depend on scheduling. Samples of a pixel are summed in sample order
 // and the frame is scaled by 1/spp, as FrameBuffer_Scale does (PathTracer.cpp:346).
 // Layers: 0 = first-bounce reflectance, 1 = radiance (PathTracer.cpp:191,224,241).
+//
+// The same wavefront also runs the deferred path tracer (DeferredPathTracer.cpp:61-430): CMJ
+// primary rays, at each hit NEE toward the lights (weight 1) and a sky sample weighted by the
+// balance heuristic against the BSDF, the continuation's sky weighted the other way on a miss,
+// Russian roulette after 7 tracked bounces, one output layer; and, as PathTracingBatcher orders its
+// hit batches (PathTracingBatcher.cpp:221-258, 799-830), hits shaded in (instance, geometry,
+// primitive) order by a device radix sort of the hit queue. A path's random numbers and the order
+// of its contributions do not depend on the shading order, so sorted and unsorted renders are
+// bit-identical.
 #include "sgpu_camera.h"
 #include "sgpu_ibl.h"
 #include "sgpu_internal.h"
 #include "sgpu_light.h"
 #include "sgpu_surface.h"
 #include "sgpu_ptex.h"
+
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 
@@ -47,7 +58,8 @@ struct PathState {
     float* ld0;    // 3 per path, layer 0
     float* ld1;    // 3 per path, layer 1
     uint64_t* rng; // PCG32 state per path (increment = 2 * path_id + 1)
-    uint32_t* flags;  // bit 0: every scatter so far was a Dirac event
+    uint32_t* flags;  // bit 0: every scatter so far was a Dirac event; bit 1: the last one was
+    float* pdf;       // deferred: solid-angle pdf of the last BSDF sample
 };
 
 struct RayBuf {
@@ -125,6 +137,50 @@ __global__ __launch_bounds__(kBlock) void k_pt_generate(CameraParams cam, uint32
     ps.flags[i] = 1u;
 }
 
+// GeneratePrimaryRays (DeferredPathTracer.cpp:338-370): sample s of the pixel's spp_x x spp_y
+// CMJ pattern (JitteredCameraRay, Camera.cpp:77-87, pattern = pixel index); no random numbers.
+__global__ __launch_bounds__(kBlock) void k_dpt_generate(CameraParams cam, uint32_t spp_x, uint32_t spp_y,
+                                                         uint64_t seed, uint64_t path_base, uint32_t n_paths,
+                                                         PathState ps, RayBuf rays, uint32_t* __restrict queue)
+{
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n_paths) return;
+    const uint32_t spp = spp_x * spp_y;
+    uint64_t path = path_base + i;
+    uint32_t pixel = (uint32_t)(path / spp);
+    uint32_t s = (uint32_t)(path - (uint64_t)pixel * spp);
+    uint32_t x = pixel % cam.width, y = pixel / cam.width;
+    f3 d = camera_cmj_dir(cam, (int32_t)x, (int32_t)y, (int32_t)s, (int32_t)spp_x, (int32_t)spp_y, (int32_t)pixel);
+    put_ray(rays, i, cam.position, d, 0.0f, kFloatMax);
+    queue[i] = i;
+    set3(ps.thr, i, mk3(1.0f, 1.0f, 1.0f));
+    set3(ps.ld1, i, mk3(0, 0, 0));
+    ps.rng[i] = pcg32_seed(seed ^ (path * 0x9E3779B97F4A7C15ull), path).state;
+    ps.flags[i] = 3u;
+    ps.pdf[i] = 0.0f;
+}
+
+// Sort keys of the hit queue: (geometry in the scene's list, primitive), misses last. For an
+// instanced scene the list numbers geometries in composition order, which is (instID[0],
+// instID[1]) order, so the key order is PathTracingBatcher's (instance, geometry, primitive).
+__global__ __launch_bounds__(kBlock) void k_dpt_keys(uint32_t count, const uint32_t* __restrict geom,
+                                                     const uint32_t* __restrict prim, const uint32_t* __restrict inst,
+                                                     const uint32_t* __restrict inst_geom_base, uint32_t instanced,
+                                                     uint32_t n_geoms, uint64_t* __restrict keys,
+                                                     uint32_t* __restrict vals)
+{
+    uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= count) return;
+    uint32_t g = geom[j];
+    uint64_t key = (uint64_t)n_geoms << 32;
+    if (g != SGPU_INVALID_ID) {
+        if (instanced) g += inst_geom_base[inst[j]];
+        key = ((uint64_t)min(g, n_geoms) << 32) | prim[j];
+    }
+    keys[j] = key;
+    vals[j] = j;
+}
+
 struct ShadeParams {
     const float* materials;
     const uint32_t* geom_material;
@@ -146,16 +202,24 @@ struct ShadeParams {
     uint64_t path_base;
 };
 
+constexpr float kInv4Pi = 0.0795774715459476678844f;
+
+// DPT: the deferred path tracer's estimator (ShadeHitPosition, DeferredPathTracer.cpp:61-148; miss
+// in TraceRayBatch, :190-200), with its sky sample appended to `sky` and hits taken in the order
+// `order` gives (nullptr: queue order).
+template <bool DPT>
 __global__ __launch_bounds__(kBlock) void k_pt_shade(ShadeParams sp, uint32_t count, const uint32_t* __restrict queue,
-                                                     RayBuf rays, HitBuf hits, PathState ps, RayBuf next_rays,
-                                                     uint32_t* __restrict next_queue, ShadowBuf shadow,
-                                                     uint32_t* __restrict counters)
+                                                     const uint32_t* __restrict order, RayBuf rays, HitBuf hits,
+                                                     PathState ps, RayBuf next_rays, uint32_t* __restrict next_queue,
+                                                     ShadowBuf shadow, ShadowBuf sky, uint32_t* __restrict counters)
 {
-    uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-    bool active = j < count;
-    bool cont = false, emit = false;
+    const uint32_t jj = blockIdx.x * kBlock + threadIdx.x;
+    bool active = jj < count;
+    const uint32_t j = active && order ? order[jj] : jj;
+    bool cont = false, emit = false, emit_sky = false;
     uint32_t p = 0;
     f3 next_o = mk3(0, 0, 0), next_d = mk3(0, 0, 0), sh_o = mk3(0, 0, 0), sh_d = mk3(0, 0, 0), c = mk3(0, 0, 0);
+    f3 sk_o = mk3(0, 0, 0), sk_d = mk3(0, 0, 0), sk_c = mk3(0, 0, 0);
     float sh_tnear = 0.0f, sh_tfar = 0.0f;
     if (active) {
         p = queue[j];
@@ -174,9 +238,20 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(ShadeParams sp, uint32_t co
             // miss: EvaluateBackground (AreaLighting.cpp:415-424): the image-based light's radiance
             // from the ray's direction, else the constant background (EvaluateBackgroundMiss reads
             // a second image of the same light, which here is the same image)
-            float pdf;
+            float pdf = kInv4Pi;
             const f3 bg = sp.has_ibl ? ibl_lookup(sp.ibl, d, pdf) : sp.background;
-            add3(ps.ld1, p, bg * thr);
+            if (DPT) {
+                // BackgroundLightingPdf against the pdf the direction was sampled with, unless
+                // the last scatter (or every one, for camera rays) was a Dirac event
+                float w = 1.0f;
+                if (!(ps.flags[p] & 2u)) {
+                    const float lp = ps.pdf[p];
+                    w = lp + pdf > 0.0f ? lp / (lp + pdf) : 0.0f;
+                }
+                add3(ps.ld1, p, (bg * w) * thr);
+            } else {
+                add3(ps.ld1, p, bg * thr);
+            }
         } else {
             PcgUniforms u{load_rng(ps, sp.path_base + p, p)};
             const float* mat = sp.materials + 22 * (size_t)sp.geom_material[geom];
@@ -192,9 +267,50 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(ShadeParams sp, uint32_t co
                              s);
             f3 v = -d;
             LightSample ls;
-            next_event_estimation(u, sp.lights, sp.n_lights, s.position, s.n, (sp.flags & SGPU_PT_NEE_COSINE_ONCE) != 0,
-                                  ls);
-            if (dot3(ls.radiance, mk3(1, 1, 1)) > 0) {
+            next_event_estimation(u, sp.lights, sp.n_lights, s.position, s.n,
+                                  DPT || (sp.flags & SGPU_PT_NEE_COSINE_ONCE) != 0, ls);
+            if (DPT) {
+                if (dot3(ls.radiance, mk3(1, 1, 1)) > 0) {
+                    float fp, rp;
+                    const f3 f = bsdf_eval(s, v, ls.direction, fp, rp);
+                    const f3 smp = (f * ls.radiance) * (1.0f / ls.pdf_w);
+                    if (dot3(smp, mk3(1, 1, 1)) > 0) {
+                        // TraceOcclusionBatch: tnear 0 from the origin biased by 0.1 error
+                        emit = true;
+                        sh_o = offset_ray_origin(s, ls.direction, 0.1f);
+                        sh_d = ls.direction;
+                        sh_tfar = ls.distance;
+                        c = smp * thr;
+                    }
+                }
+                // SampleBackground: the image-based light, else the constant background sampled
+                // uniformly over the sphere (pdf 1 / (4 pi))
+                const float r0 = u.next(), r1 = u.next();
+                f3 sd, srad;
+                float spdf;
+                if (sp.has_ibl) {
+                    srad = ibl_direct(sp.ibl, r0, r1, sd, spdf);
+                } else {
+                    const float uu = 2.0f * r1 - 1.0f;
+                    const float norm = sqrtf(max_(0.0f, 1.0f - uu * uu));
+                    const float theta = (kPi * 2.0f) * r0;
+                    sd = mk3(norm * cos_f(theta), uu, norm * sin_f(theta));
+                    srad = sp.background;
+                    spdf = kInv4Pi;
+                }
+                if (dot3(srad, mk3(1, 1, 1)) > 0 && spdf > 0.0f) {
+                    float fp, rp;
+                    const f3 f = bsdf_eval(s, v, sd, fp, rp);
+                    const float w = spdf / (spdf + fp);
+                    const f3 smp = ((f * w) * srad) * (1.0f / spdf);
+                    if (dot3(smp, mk3(1, 1, 1)) > 0) {
+                        emit_sky = true;
+                        sk_o = offset_ray_origin(s, sd, 0.1f);
+                        sk_d = sd;
+                        sk_c = smp * thr;
+                    }
+                }
+            } else if (dot3(ls.radiance, mk3(1, 1, 1)) > 0) {
                 float fp, rp;
                 f3 f = bsdf_eval(s, v, ls.direction, fp, rp);
                 if (dot3(f, mk3(1, 1, 1)) > 0) {
@@ -208,7 +324,28 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(ShadeParams sp, uint32_t co
                 }
             }
             BsdfSample bs;
-            if (bsdf_sample(u, s, v, bs)) {
+            if (DPT) {
+                if (bsdf_sample(u, s, v, bs)) {
+                    thr = thr * bs.reflectance;
+                    if (!((thr.x * thr.x + thr.y * thr.y) + thr.z * thr.z == 0.0f)) {
+                        const uint32_t dirac = (bs.flags & kDirac) ? 1u : 0u;
+                        ps.flags[p] = (ps.flags[p] & dirac) | (dirac << 1);
+                        ps.pdf[p] = bs.fwd_pdf;
+                        cont = sp.bounce + 1 < sp.max_len;
+                        if (sp.bounce >= 7) {
+                            // Russian roulette once the tracked bounces saturate (MaxTrackedBounces_ = 7)
+                            const float cp = max_(max_(thr.x, thr.y), thr.z);
+                            if (u.next() >= cp)
+                                cont = false;
+                            else
+                                thr = thr * (1.0f / cp);
+                        }
+                        next_o = offset_ray_origin(s, bs.wi, 1.0f);
+                        next_d = bs.wi;
+                        set3(ps.thr, p, thr);
+                    }
+                }
+            } else if (bsdf_sample(u, s, v, bs)) {
                 uint32_t fl = ps.flags[p];
                 if (!(bs.flags & kDirac)) fl &= ~1u;
                 ps.flags[p] = fl;
@@ -236,6 +373,14 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(ShadeParams sp, uint32_t co
         shadow.path[ks] = p;
         set3(shadow.c, ks, c);
     }
+    if (DPT) {
+        uint32_t kk = wave_append(emit_sky, &counters[2]);
+        if (emit_sky) {
+            put_ray(sky.ray, kk, sk_o, sk_d, 0.0f, 1e36f);
+            sky.path[kk] = p;
+            set3(sky.c, kk, sk_c);
+        }
+    }
     uint32_t kn = wave_append(cont, &counters[0]);
     if (cont) {
         put_ray(next_rays, kn, next_o, next_d, 0.0f, kFloatMax);
@@ -251,19 +396,25 @@ __global__ __launch_bounds__(kBlock) void k_pt_resolve(uint32_t count, ShadowBuf
     add3(ps.ld1, shadow.path[k], get3(shadow.c, k));
 }
 
-// Sum each pixel's samples in sample order into the frame (both layers).
+// Sum each pixel's samples in sample order into the frame: both layers, or (one_layer) the
+// radiance alone.
 __global__ __launch_bounds__(kBlock) void k_pt_accumulate(uint32_t spp, uint32_t pixel_base, uint32_t n_pixels,
-                                                          uint32_t n_frame_pixels, PathState ps, float* __restrict fb)
+                                                          uint32_t n_frame_pixels, uint32_t one_layer, PathState ps,
+                                                          float* __restrict fb)
 {
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n_pixels) return;
     f3 a0 = mk3(0, 0, 0), a1 = mk3(0, 0, 0);
     for (uint32_t s = 0; s < spp; ++s) {
         uint32_t p = i * spp + s;
-        a0 += get3(ps.ld0, p);
+        if (!one_layer) a0 += get3(ps.ld0, p);
         a1 += get3(ps.ld1, p);
     }
     uint32_t pix = pixel_base + i;
+    if (one_layer) {
+        add3(fb, pix, a1);
+        return;
+    }
     add3(fb, pix, a0);
     add3(fb + 3 * (size_t)n_frame_pixels, pix, a1);
 }
@@ -309,27 +460,43 @@ sgpu_rays_soa as_soa(const RayBuf& r) { return sgpu_rays_soa{r.ox, r.oy, r.oz, r
 
 }  // namespace
 
-int sgpu_pt_render_impl(sgpu_scene scene, const sgpu_pt_job* job, float* fb, hipStream_t stream)
+namespace {
+
+// One render of either tracer: the path tracer (two layers) or, with dpt, the deferred path
+// tracer (one layer, CMJ primary rays, optional hit sort).
+struct PathJob {
+    sgpu_camera camera;
+    uint32_t spp_x, spp_y;  // the path tracer: spp_x = paths per pixel, spp_y = 1
+    uint32_t max_path_length;
+    uint64_t seed;
+    uint32_t flags;
+    bool dpt;
+    bool sort_hits;
+};
+
+int render_paths(sgpu_scene scene, const PathJob& job, float* fb, hipStream_t stream)
 {
-    const uint32_t W = job->camera.width, H = job->camera.height, spp = job->spp;
+    const uint32_t W = job.camera.width, H = job.camera.height, spp = job.spp_x * job.spp_y;
     const uint64_t n_pixels = (uint64_t)W * H;
+    const uint32_t layers = job.dpt ? 1u : 2u;
     const uint32_t max_chunk_paths = 1u << 22;
     uint32_t chunk_pixels = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(n_pixels, max_chunk_paths / spp));
     const uint32_t cap = chunk_pixels * spp;
 
-    CameraParams cam = camera_init(mk3(job->camera.position[0], job->camera.position[1], job->camera.position[2]),
-                                   mk3(job->camera.look_at[0], job->camera.look_at[1], job->camera.look_at[2]),
-                                   mk3(job->camera.up[0], job->camera.up[1], job->camera.up[2]),
-                                   job->camera.fov_degrees, W, H);
+    CameraParams cam = camera_init(mk3(job.camera.position[0], job.camera.position[1], job.camera.position[2]),
+                                   mk3(job.camera.look_at[0], job.camera.look_at[1], job.camera.look_at[2]),
+                                   mk3(job.camera.up[0], job.camera.up[1], job.camera.up[2]),
+                                   job.camera.fov_degrees, W, H);
 
     Arena A;
     A.stream = stream;
     PathState ps;
     ps.thr = A.get<float>(3 * (size_t)cap);
-    ps.ld0 = A.get<float>(3 * (size_t)cap);
+    ps.ld0 = job.dpt ? nullptr : A.get<float>(3 * (size_t)cap);
     ps.ld1 = A.get<float>(3 * (size_t)cap);
     ps.rng = A.get<uint64_t>(cap);
     ps.flags = A.get<uint32_t>(cap);
+    ps.pdf = job.dpt ? A.get<float>(cap) : nullptr;
     RayBuf rays[2] = {A.rays(cap), A.rays(cap)};
     uint32_t* queue[2] = {A.get<uint32_t>(cap), A.get<uint32_t>(cap)};
     HitBuf hb;
@@ -342,24 +509,19 @@ int sgpu_pt_render_impl(sgpu_scene scene, const sgpu_pt_job* job, float* fb, hip
     hb.prim = A.get<uint32_t>(cap);
     hb.geom = A.get<uint32_t>(cap);
     hb.inst = A.get<uint32_t>(cap);
-    ShadowBuf sb;
-    sb.ray = A.rays(cap);
-    sb.path = A.get<uint32_t>(cap);
-    sb.c = A.get<float>(3 * (size_t)cap);
-    sb.occluded = A.get<uint32_t>(cap);
+    auto shadow_buf = [&]() {
+        ShadowBuf b;
+        b.ray = A.rays(cap);
+        b.path = A.get<uint32_t>(cap);
+        b.c = A.get<float>(3 * (size_t)cap);
+        b.occluded = A.get<uint32_t>(cap);
+        return b;
+    };
+    ShadowBuf sb = shadow_buf();
+    ShadowBuf sky = job.dpt ? shadow_buf() : ShadowBuf{};
     uint32_t* spill = A.get<uint32_t>(sgpu_trace_spill_words(scene));
     uint32_t* counters = A.get<uint32_t>(4);
-    uint32_t* h_counters = nullptr;
-    if (A.err != hipSuccess) return sgpu_fail(SGPU_E_NOMEM, "sgpu_render_pt: device allocation failed");
-    SGPU_HIP_TRY(hipHostMalloc((void**)&h_counters, 4 * sizeof(uint32_t), hipHostMallocDefault));
-    struct HostFree {
-        uint32_t* p;
-        ~HostFree() { (void)hipHostFree(p); }
-    } hf{h_counters};
 
-    SGPU_HIP_TRY(hipMemsetAsync(fb, 0, 2 * 3 * n_pixels * sizeof(float), stream));
-
-    sgpu_hits_soa hsoa{hb.t, hb.u, hb.v, hb.prim, hb.geom, hb.ngx, hb.ngy, hb.ngz};
     ShadeParams sp;
     sp.materials = scene->d_materials;
     sp.geom_material = scene->d_geom_material;
@@ -380,51 +542,119 @@ int sgpu_pt_render_impl(sgpu_scene scene, const sgpu_pt_job* job, float* fb, hip
         sp.geom_tex = scene->d_g_geom_tex;
         sp.n_geom_tex = scene->n_g_geoms;
     }
-    sp.max_len = job->max_path_length;
-    sp.flags = job->flags;
+    sp.max_len = job.max_path_length;
+    sp.flags = job.flags;
+
+    // hit sort: 64-bit keys (geometry << 32 | primitive), the geometry part sized to the scene
+    uint64_t* keys[2] = {nullptr, nullptr};
+    uint32_t* vals[2] = {nullptr, nullptr};
+    void* sort_tmp = nullptr;
+    size_t sort_bytes = 0;
+    uint32_t end_bit = 32;
+    while (end_bit < 64 && ((uint64_t)sp.n_geoms >> (end_bit - 32)) != 0) ++end_bit;
+    if (job.sort_hits) {
+        keys[0] = A.get<uint64_t>(cap);
+        keys[1] = A.get<uint64_t>(cap);
+        vals[0] = A.get<uint32_t>(cap);
+        vals[1] = A.get<uint32_t>(cap);
+        SGPU_HIP_TRY(rocprim::radix_sort_pairs(nullptr, sort_bytes, keys[0], keys[1], vals[0], vals[1], (size_t)cap, 0,
+                                               end_bit, stream));
+        sort_tmp = A.get<uint8_t>(sort_bytes);
+    }
+    uint32_t* h_counters = nullptr;
+    if (A.err != hipSuccess) return sgpu_fail(SGPU_E_NOMEM, "sgpu_render: device allocation failed");
+    SGPU_HIP_TRY(hipHostMalloc((void**)&h_counters, 4 * sizeof(uint32_t), hipHostMallocDefault));
+    struct HostFree {
+        uint32_t* p;
+        ~HostFree() { (void)hipHostFree(p); }
+    } hf{h_counters};
+
+    SGPU_HIP_TRY(hipMemsetAsync(fb, 0, layers * 3 * n_pixels * sizeof(float), stream));
+    sgpu_hits_soa hsoa{hb.t, hb.u, hb.v, hb.prim, hb.geom, hb.ngx, hb.ngy, hb.ngz};
+
+    auto resolve = [&](ShadowBuf& b, uint32_t n) -> int {
+        if (n == 0) return SGPU_OK;
+        sgpu_rays_soa ss = as_soa(b.ray);
+        int rc = sgpu_trace_launch(scene, &ss, nullptr, b.occluded, n, true, nullptr, stream, spill);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_pt_resolve, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, n, b, ps);
+        SGPU_HIP_TRY(hipGetLastError());
+        return SGPU_OK;
+    };
 
     for (uint64_t pix0 = 0; pix0 < n_pixels; pix0 += chunk_pixels) {
         const uint32_t npix = (uint32_t)std::min<uint64_t>(chunk_pixels, n_pixels - pix0);
         const uint32_t n_paths = npix * spp;
         const uint64_t path_base = pix0 * spp;
-        hipLaunchKernelGGL(k_pt_generate, dim3((n_paths + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, cam, spp,
-                           job->seed, path_base, n_paths, ps, rays[0], queue[0]);
+        if (job.dpt)
+            hipLaunchKernelGGL(k_dpt_generate, dim3((n_paths + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, cam,
+                               job.spp_x, job.spp_y, job.seed, path_base, n_paths, ps, rays[0], queue[0]);
+        else
+            hipLaunchKernelGGL(k_pt_generate, dim3((n_paths + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, cam, spp,
+                               job.seed, path_base, n_paths, ps, rays[0], queue[0]);
         SGPU_HIP_TRY(hipGetLastError());
         uint32_t count = n_paths;
         int cur = 0;
         sp.path_base = path_base;
-        for (uint32_t bounce = 0; bounce < job->max_path_length && count > 0; ++bounce) {
+        for (uint32_t bounce = 0; bounce < job.max_path_length && count > 0; ++bounce) {
             sgpu_rays_soa rs = as_soa(rays[cur]);
             int rc = sgpu_trace_launch(scene, &rs, &hsoa, nullptr, count, false, nullptr, stream, spill, nullptr, nullptr,
                                        hb.inst);
             if (rc) return rc;
+            const uint32_t* order = nullptr;
+            if (job.sort_hits) {
+                hipLaunchKernelGGL(k_dpt_keys, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, count,
+                                   hb.geom, hb.prim, hb.inst, sp.inst_geom_base, sp.instanced, sp.n_geoms, keys[0],
+                                   vals[0]);
+                SGPU_HIP_TRY(hipGetLastError());
+                size_t bytes = sort_bytes;
+                SGPU_HIP_TRY(rocprim::radix_sort_pairs(sort_tmp, bytes, keys[0], keys[1], vals[0], vals[1], (size_t)count,
+                                                       0, end_bit, stream));
+                order = vals[1];
+            }
             SGPU_HIP_TRY(hipMemsetAsync(counters, 0, 4 * sizeof(uint32_t), stream));
             sp.bounce = bounce;
-            hipLaunchKernelGGL(k_pt_shade, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, sp, count,
-                               queue[cur], rays[cur], hb, ps, rays[1 - cur], queue[1 - cur], sb, counters);
+            if (job.dpt)
+                hipLaunchKernelGGL(k_pt_shade<true>, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, sp,
+                                   count, queue[cur], order, rays[cur], hb, ps, rays[1 - cur], queue[1 - cur], sb, sky,
+                                   counters);
+            else
+                hipLaunchKernelGGL(k_pt_shade<false>, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, sp,
+                                   count, queue[cur], order, rays[cur], hb, ps, rays[1 - cur], queue[1 - cur], sb, sky,
+                                   counters);
             SGPU_HIP_TRY(hipGetLastError());
             SGPU_HIP_TRY(hipMemcpyAsync(h_counters, counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
             SGPU_HIP_TRY(hipStreamSynchronize(stream));
-            const uint32_t n_next = h_counters[0], n_shadow = h_counters[1];
-            if (n_shadow > 0) {
-                sgpu_rays_soa ss = as_soa(sb.ray);
-                rc = sgpu_trace_launch(scene, &ss, nullptr, sb.occluded, n_shadow, true, nullptr, stream, spill);
-                if (rc) return rc;
-                hipLaunchKernelGGL(k_pt_resolve, dim3((n_shadow + kBlock - 1) / kBlock), dim3(kBlock), 0, stream,
-                                   n_shadow, sb, ps);
-                SGPU_HIP_TRY(hipGetLastError());
-            }
+            const uint32_t n_next = h_counters[0];
+            // per path: the light sample's contribution, then the sky sample's
+            if ((rc = resolve(sb, h_counters[1]))) return rc;
+            if (job.dpt && (rc = resolve(sky, h_counters[2]))) return rc;
             count = n_next;
             cur = 1 - cur;
         }
         hipLaunchKernelGGL(k_pt_accumulate, dim3((npix + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, spp,
-                           (uint32_t)pix0, npix, (uint32_t)n_pixels, ps, fb);
+                           (uint32_t)pix0, npix, (uint32_t)n_pixels, job.dpt ? 1u : 0u, ps, fb);
         SGPU_HIP_TRY(hipGetLastError());
     }
-    const size_t nf = 2 * 3 * n_pixels;
+    const size_t nf = layers * 3 * n_pixels;
     hipLaunchKernelGGL(k_scale, dim3((uint32_t)((nf + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream, fb, nf,
                        1.0f / (float)spp);
     SGPU_HIP_TRY(hipGetLastError());
     SGPU_HIP_TRY(hipStreamSynchronize(stream));
     return SGPU_OK;
+}
+
+}  // namespace
+
+int sgpu_pt_render_impl(sgpu_scene scene, const sgpu_pt_job* job, float* fb, hipStream_t stream)
+{
+    PathJob pj{job->camera, job->spp, 1u, job->max_path_length, job->seed, job->flags, false, false};
+    return render_paths(scene, pj, fb, stream);
+}
+
+int sgpu_dpt_render_impl(sgpu_scene scene, const sgpu_dpt_job* job, float* fb, hipStream_t stream)
+{
+    PathJob pj{job->camera, job->spp_x, job->spp_y, job->max_path_length, job->seed, job->flags, true,
+               (job->flags & SGPU_DPT_SORT_HITS) != 0};
+    return render_paths(scene, pj, fb, stream);
 }

@@ -88,6 +88,7 @@ _SIGNATURES = {
     "sgpu_scene_set_lights": [_P, _P, _U32, _FP],
     "sgpu_scene_set_ibl": [_P, _U32, _U32, _FP, _F, _F],
     "sgpu_render_pt": [_P, _P, _P, _P],
+    "sgpu_render_deferred_pt": [_P, _P, _P, _P],
     "sgpu_framebuffer_save_hdr": [C.c_char_p, _P, _U32, _U32, _U32, _P],
     "sgpu_render_vcm": [_P, _P, _P, _P, _P],
     "sgpu_render_vcm_batch": [_P, _P, _P, _U32, _U32, _P, _P],
@@ -443,6 +444,14 @@ class PtJob(C.Structure):
 PT_NEE_COSINE_ONCE = 1
 
 
+class DptJob(C.Structure):
+    _fields_ = [("camera", Camera), ("spp_x", C.c_uint32), ("spp_y", C.c_uint32), ("max_path_length", C.c_uint32),
+                ("seed", C.c_uint64), ("flags", C.c_uint32)]
+
+
+DPT_SORT_HITS = 1
+
+
 def build_render_scene(ctx: Context, sc: dict) -> Scene:
     """A committed device scene with materials and lights from a workloads.*_scene dict, or an
     instanced scene from a dict with "models" (scene dicts of geometries, materials and textures),
@@ -642,6 +651,22 @@ def save_hdr(ctx: Context, path_prefix: str, framebuffer, width: int, height: in
     finally:
         if owned is not None:
             owned.free()
+
+
+def render_deferred_pt(scene: Scene, cam: dict, width, height, spp_x, spp_y, max_path_length, seed,
+                       flags=DPT_SORT_HITS, framebuffer=None, stream=None):
+    """sgpu_render_deferred_pt: height x width x 3 (or into `framebuffer`, a DeviceBuffer, and None)."""
+    ctx = scene.ctx
+    fb = framebuffer if framebuffer is not None else ctx.empty(3 * width * height * 4)
+    job = DptJob(Camera.make(cam, width, height), spp_x, spp_y, max_path_length, seed, flags)
+    try:
+        check(lib.sgpu_render_deferred_pt(scene.handle, C.byref(job), fb.ptr, stream))
+        if framebuffer is not None:
+            return None
+        return fb.download(np.float32, 3 * width * height).reshape(height, width, 3)
+    finally:
+        if framebuffer is None:
+            fb.free()
 
 
 def render_pt(scene: Scene, cam: dict, width, height, spp, max_path_length, seed, flags=0, stream=None):
