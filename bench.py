@@ -571,7 +571,7 @@ def bench_serve(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=("vcm", "traversal", "batch", "serve", "instances"), default="vcm")
+    ap.add_argument("--workload", choices=("vcm", "traversal", "batch", "serve", "instances", "deferred"), default="vcm")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (vcm: 32 iterations, traversal: 10)")
     ap.add_argument("--warmup", type=int, default=2)
@@ -627,6 +627,8 @@ def main():
         bench_batch(args)
     elif args.workload == "serve":
         bench_serve(args)
+    elif args.workload == "deferred":
+        bench_deferred(args)
     elif args.workload == "instances":
         if args.steps is None:
             args.steps = 10
@@ -860,6 +862,87 @@ def bench_instances(args):
                      "kernel": "k_trace8i<false,false,false> (closest hit; 64-B instance records not counted)",
                      "bytes_per_ray": round(bytes_per_ray, 1)},
         "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_deferred(args):
+    """The deferred path tracer (sgpu_render_deferred_pt, DeferredPathTracer::GenerateImage) on the
+    VCM workload's 1M-triangle textured scene with an image-based sky (256 x 512): 512 x 512 pixels,
+    4 x 4 CMJ samples per pixel (SamplesPerPixelX_ x SamplesPerPixelY_), paths bounded at 10
+    segments; one step = one whole image (4,194,304 paths). Timed without the hit sort (the
+    library's default, `value`) and with it (Selas's ordering of shading); the two images are
+    compared bit for bit."""
+    from selas_amd import native as N
+    from selas_amd import workloads as W
+
+    w, sx, sy, L = args.width, 4, 4, 10
+    steps = 5 if args.steps is None else args.steps
+    log("building deferred workload scene")
+    sc = W.vcm_scene()
+    sc["ibl"] = {"rgb": W.sky_map(256, 512), "rotation": 0.0, "exposure": 1.0}
+    n_tris = sum(W.geom_tri9(g).shape[0] for g in sc["geoms"])
+    ctx = N.Context(0)
+    if args.trace_blocks:
+        ctx.set_tuning(trace_blocks_per_cu=args.trace_blocks)
+    scene = N.build_render_scene(ctx, sc)
+    cam = sc["camera"]
+    fb = ctx.empty(3 * w * w * 4)
+    res, imgs = {}, {}
+    for name, flags in (("unsorted", 0), ("sorted", N.DPT_SORT_HITS)):
+        for _ in range(args.warmup):
+            N.render_deferred_pt(scene, cam, w, w, sx, sy, L, 1, flags=flags, framebuffer=fb)
+        N.synchronize()
+        e0, e1 = N.Event(), N.Event()
+        e0.record()
+        for _ in range(steps):
+            N.render_deferred_pt(scene, cam, w, w, sx, sy, L, 1, flags=flags, framebuffer=fb)
+        e1.record()
+        N.synchronize()
+        ms = e0.elapsed_ms(e1) / steps
+        imgs[name] = fb.download(np.float32, 3 * w * w)
+        res[name] = {"samples_per_s": round(w * w * sx * sy / (ms / 1e3), 1), "ms_per_image": round(ms, 3)}
+        log(f"deferred {name}: {res[name]}")
+    scene.check()
+    fb.free()
+    identical = bool(np.array_equal(imgs["sorted"].view(np.uint32), imgs["unsorted"].view(np.uint32)))
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        threads = min(16, os.cpu_count() or 1)
+        log("cpu baseline: building the oracle SAH tree")
+        rs = O.RenderScene(sc)
+        t0 = time.perf_counter()
+        img = O.render_dpt(rs, cam, w, w, sx, sy, L, 1, threads=threads)
+        secs = time.perf_counter() - t0
+        cpu = {"value": round(w * w * sx * sy / secs, 1), "unit": "samples/s", "cores": threads, "kind": "port",
+               "sample": f"the whole image ({w}x{w} pixels x {sx}x{sy} samples, same seed; {secs:.2f} s), oracle "
+                         "restatement (scalar C, oracle SAH tree, not Embree)",
+               "image_mean": round(float(img.mean()), 5),
+               "mean_rel_diff_vs_gpu": round(float(abs(img.mean() / imgs["unsorted"].mean() - 1)), 6)}
+    scene.release()
+    out = {
+        "metric": "deferred path tracer samples/s at 1M tris",
+        "value": res["unsorted"]["samples_per_s"],
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["unsorted"]["ms_per_image"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"deferred_pt_{w}x{w}_{sx}x{sy}spp_L{L}_1Mtri_sky", "triangles": int(n_tris),
+                   "width": w, "height": w, "spp": [sx, sy], "max_path_length": L,
+                   "sky": "256 x 512 equirectangular image-based light"},
+        "unsorted": res["unsorted"],
+        "sorted": res["sorted"],
+        "sorted_equals_unsorted": identical,
+        "image_mean": round(float(imgs["unsorted"].mean()), 5),
+        "roofline": None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

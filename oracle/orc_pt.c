@@ -15,7 +15,9 @@
  * initstate = seed ^ (path * 0x9E3779B97F4A7C15)) instead of one mt19937 per thread; samples of a
  * pixel are summed in sample order; media are vacuum (no config has a participating medium);
  * point lights are an addition. With flags & 1 the quad-light estimator drops the extra
- * Dot(normal, l) that Selas multiplies into the light sample (AreaLighting.cpp:140).
+ * Dot(normal, l) that Selas multiplies into the light sample (AreaLighting.cpp:140). A BSDF sample
+ * of pdf 0 ends the path (SampleDisneyDiffuse, Disney.cpp:494-496, takes diffuse transmission when
+ * its uniform is exactly 0 even at diffTrans = 0, returning pdf 0 and an infinite reflectance).
  * Also the timed CPU path of the C1 benchmark ("port" baseline).
  */
 #include "oracle.h"
@@ -341,7 +343,7 @@ static void evaluate_path(const pt_ctx* c, uint32_t* stack, uint32_t x, uint32_t
             }
         }
         float bs[14];
-        if (!bsdf_sample_any(&u, &s, v, bs)) break;
+        if (!bsdf_sample_any(&u, &s, v, bs) || !(bs[8] > 0.0f)) break; /* pdf 0 ends the path */
         thr = mul(scl(thr, 1.0f), ld3(bs + 2));
         float no[3];
         orc_offset_origin(&s, pos, err, bs + 5, 1.0f, no, NULL);
@@ -523,7 +525,7 @@ static ov3 dpt_path(const dpt_ctx* c, uint32_t* stack, uint32_t x, uint32_t y, u
         }
         /* the BSDF sample continues the path */
         float bs[14];
-        if (!bsdf_sample_any(&u, &sf, v, bs)) break;
+        if (!bsdf_sample_any(&u, &sf, v, bs) || !(bs[8] > 0.0f)) break; /* pdf 0 ends the path */
         thr = mul(thr, ld3(bs + 2));
         if (dot(thr, thr) == 0.0f) break;
         const int dirac = (((uint32_t)bs[1]) & 4u) != 0;

@@ -27,6 +27,8 @@
  * and its MIS terms are zero (dVC = dVM = 0 at the light, no light weight in NEE) -- exactly the
  * treatment the report gives delta lights; NEE samples quad lights by area (a point-independent
  * pdf, as the MIS weights require). Random numbers: one PCG32 stream per (iteration, path, pass).
+ * A BSDF sample of pdf 0 ends the path (SampleDisneyDiffuse, Disney.cpp:494-496, can return one
+ * with an infinite reflectance).
  * Also the timed CPU path of the VCM benchmark ("port" baseline).
  */
 #include "oracle.h"
@@ -421,6 +423,7 @@ static int scatter(orc_uniforms* u, const vcm_ctx* c, const orc_surface* s, ov3 
     float v[3] = {-dir->x, -dir->y, -dir->z};
     int ok = s->shader == 2 ? orc_lambert_sample(u, s, v, out) : orc_disney_sample(u, s, v, s->shader == 0, out);
     if (!ok) return 0;
+    if (!(out[8] > 0.0f)) return 0; /* a sample of pdf 0 ends the path */
     ov3 refl = ld3(out + 2);
     if (is_zero(refl)) return 0;
     float cp = cont_prob(s);

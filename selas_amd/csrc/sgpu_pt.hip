@@ -325,7 +325,7 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(ShadeParams sp, uint32_t co
             }
             BsdfSample bs;
             if (DPT) {
-                if (bsdf_sample(u, s, v, bs)) {
+                if (bsdf_sample(u, s, v, bs) && bs.fwd_pdf > 0.0f) {
                     thr = thr * bs.reflectance;
                     if (!((thr.x * thr.x + thr.y * thr.y) + thr.z * thr.z == 0.0f)) {
                         const uint32_t dirac = (bs.flags & kDirac) ? 1u : 0u;
@@ -345,7 +345,7 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(ShadeParams sp, uint32_t co
                         set3(ps.thr, p, thr);
                     }
                 }
-            } else if (bsdf_sample(u, s, v, bs)) {
+            } else if (bsdf_sample(u, s, v, bs) && bs.fwd_pdf > 0.0f) {
                 uint32_t fl = ps.flags[p];
                 if (!(bs.flags & kDirac)) fl &= ~1u;
                 ps.flags[p] = fl;

@@ -222,6 +222,9 @@ __device__ __forceinline__ bool scatter(U& u, const Slice& sl, const Surface& s,
 {
     BsdfSample bs;
     if (!bsdf_sample(u, s, -dir, bs)) return false;
+    // a sample of pdf 0 (SampleDisney's diffuse transmission taken at diffTrans = 0 when the
+    // uniform is exactly 0, with an infinite reflectance) ends the path
+    if (!(bs.fwd_pdf > 0.0f)) return false;
     if (is_zero3(bs.reflectance)) return false;
     float cp = cont_prob(s);
     if (u.next() > cp) return false;

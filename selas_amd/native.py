@@ -654,7 +654,7 @@ def save_hdr(ctx: Context, path_prefix: str, framebuffer, width: int, height: in
 
 
 def render_deferred_pt(scene: Scene, cam: dict, width, height, spp_x, spp_y, max_path_length, seed,
-                       flags=DPT_SORT_HITS, framebuffer=None, stream=None):
+                       flags=0, framebuffer=None, stream=None):
     """sgpu_render_deferred_pt: height x width x 3 (or into `framebuffer`, a DeviceBuffer, and None)."""
     ctx = scene.ctx
     fb = framebuffer if framebuffer is not None else ctx.empty(3 * width * height * 4)

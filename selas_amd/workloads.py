@@ -271,6 +271,21 @@ MAT_TRANSPARENT = 1
 MAT_USES_PTEX = 16
 
 
+def sky_map(h: int, w: int) -> np.ndarray:
+    """An equirectangular sky (h x w x 3 float32 radiance): brighter toward +y, with a broad bright
+    lobe around (0.5, 0.7, 0.3)."""
+    th = (np.arange(h) + 0.5) / h * np.pi
+    ph = (np.arange(w) + 0.5) / w * 2 * np.pi - np.pi
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    d = np.stack([np.sin(T) * np.cos(P), np.cos(T), np.sin(T) * np.sin(P)], -1)
+    sun = np.array([0.5, 0.7, 0.3])
+    sun /= np.linalg.norm(sun)
+    lobe = np.maximum(d @ sun, 0) ** 8
+    base = 0.3 + 0.7 * np.maximum(d[..., 1], 0)
+    rgb = np.stack([base * 0.6 + 4 * lobe, base * 0.8 + 3.5 * lobe, base * 1.0 + 2.5 * lobe], -1)
+    return rgb.astype(np.float32)
+
+
 def vcm_scene(seed: int = 3, n_meshes: int = 64, nu: int = 128, nv: int = 64, textures: bool = True,
               res_log2: int = 4):
     """VCM workload scene: n_meshes displaced-grid quad meshes of nu x nv quads (2 nu nv triangles
