@@ -576,7 +576,8 @@ def main():
     ap.add_argument("--steps", type=int, default=None, help="timed steps (vcm: 32 iterations, traversal: 10)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--width", type=int, default=512, help="vcm: image width and height")
-    ap.add_argument("--cpu-iterations", type=int, default=2, help="vcm: iterations in the CPU baseline sample")
+    ap.add_argument("--cpu-iterations", type=int, default=32,
+                    help="vcm: iterations in the CPU baseline sample (32: the whole job)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-rays", type=int, default=4194304, help="rays in the CPU baseline sample")
     ap.add_argument("--batch-jobs", type=int, default=64, help="batch: jobs in the mix; serve: -")
