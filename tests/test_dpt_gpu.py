@@ -43,26 +43,28 @@ CASES = {
 }
 
 
+@pytest.mark.parametrize("flags", [0, N.DPT_SORT_HITS])
 @pytest.mark.parametrize("case", list(CASES))
-def test_deferred_matches_oracle_per_pixel(gpu_ctx, case):
+def test_deferred_matches_oracle_per_pixel(gpu_ctx, case, flags):
     sc = CASES[case]()
     scene = N.build_render_scene(gpu_ctx, sc)
     try:
         rs = O.InstancedRenderScene(sc) if "instances" in sc else O.RenderScene(sc)
         w, h = 64, 48
-        print(f"deferred {case}", flush=True)
-        gpu = N.render_deferred_pt(scene, sc["camera"], w, h, 2, 2, 8, 77)
+        print(f"deferred {case} flags={flags}", flush=True)
+        gpu = N.render_deferred_pt(scene, sc["camera"], w, h, 2, 2, 8, 77, flags=flags)
         cpu = O.render_dpt(rs, sc["camera"], w, h, 2, 2, 8, 77)
         _agree(gpu, cpu)
     finally:
         scene.release()
 
 
-def test_hit_sort_does_not_change_the_image(gpu_ctx):
-    sc = W.sized_vcm_scene(6, 5e4)
+@pytest.mark.parametrize("case", ["textured", "instanced"])
+def test_hit_sort_does_not_change_the_image(gpu_ctx, case):
+    sc = W.sized_vcm_scene(6, 5e4) if case == "textured" else scenes.instanced_scene(ibl=True)
     scene = N.build_render_scene(gpu_ctx, sc)
     try:
-        print("deferred sort on/off", flush=True)
+        print(f"deferred sort on/off {case}", flush=True)
         a = N.render_deferred_pt(scene, sc["camera"], 96, 64, 3, 3, 10, 5, flags=N.DPT_SORT_HITS)
         b = N.render_deferred_pt(scene, sc["camera"], 96, 64, 3, 3, 10, 5, flags=0)
         assert a.mean() > 0.005
