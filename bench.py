@@ -890,7 +890,9 @@ def bench_deferred(args):
     cam = sc["camera"]
     fb = ctx.empty(3 * w * w * 4)
     res, imgs = {}, {}
-    for name, flags in (("unsorted", 0), ("sorted", N.DPT_SORT_HITS)):
+    modes = (("unsorted", 0), ("sorted", N.DPT_SORT_HITS), ("rays_sorted", N.DPT_SORT_RAYS),
+             ("hits_and_rays_sorted", N.DPT_SORT_HITS | N.DPT_SORT_RAYS))
+    for name, flags in modes:
         for _ in range(args.warmup):
             N.render_deferred_pt(scene, cam, w, w, sx, sy, L, 1, flags=flags, framebuffer=fb)
         N.synchronize()
@@ -906,7 +908,7 @@ def bench_deferred(args):
         log(f"deferred {name}: {res[name]}")
     scene.check()
     fb.free()
-    identical = bool(np.array_equal(imgs["sorted"].view(np.uint32), imgs["unsorted"].view(np.uint32)))
+    identical = all(np.array_equal(imgs[m].view(np.uint32), imgs["unsorted"].view(np.uint32)) for m, _ in modes)
     cpu = None
     if not args.no_cpu:
         import oracle as O
@@ -940,6 +942,8 @@ def bench_deferred(args):
                    "sky": "256 x 512 equirectangular image-based light"},
         "unsorted": res["unsorted"],
         "sorted": res["sorted"],
+        "rays_sorted": res["rays_sorted"],
+        "hits_and_rays_sorted": res["hits_and_rays_sorted"],
         "sorted_equals_unsorted": identical,
         "image_mean": round(float(imgs["unsorted"].mean()), 5),
         "roofline": None,

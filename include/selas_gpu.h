@@ -317,6 +317,7 @@ typedef struct sgpu_pt_job {
 int sgpu_render_pt(sgpu_scene scene, const sgpu_pt_job* job, float* framebuffer, void* stream);
 
 #define SGPU_DPT_SORT_HITS 1u /* shade each bounce's hits in (instID[0], instID[1], geomID, primID) order */
+#define SGPU_DPT_SORT_RAYS 2u /* trace each bounce's rays binned by direction, sorted along it (GetSortedBatch) */
 
 typedef struct sgpu_dpt_job {
     sgpu_camera camera;
@@ -333,7 +334,8 @@ typedef struct sgpu_dpt_job {
  * image-based light, else the constant background) and the BSDF's continuation weighted against
  * each other by the balance heuristic, Russian roulette after 7 bounces. SGPU_DPT_SORT_HITS orders
  * the shading as PathTracingBatcher::GetSortedHits (PathTracingBatcher.cpp:221-258, 799-830) does;
- * the image does not depend on it. framebuffer: a device array of height x width x 3 floats (one
+ * SGPU_DPT_SORT_RAYS orders each bounce's continuation rays as GetSortedBatch (:705-750) does; the
+ * image depends on neither. framebuffer: a device array of height x width x 3 floats (one
  * layer, OutputLayers_ = 1), averaged over spp_x x spp_y. Synchronous on `stream`. */
 int sgpu_render_deferred_pt(sgpu_scene scene, const sgpu_dpt_job* job, float* framebuffer, void* stream);
 

@@ -771,7 +771,7 @@ int sgpu_render_deferred_pt(sgpu_scene scene, const sgpu_dpt_job* job, float* fr
     if ((uint64_t)job->spp_x * job->spp_y > (1u << 20))
         return sgpu_fail(SGPU_E_RANGE, std::string(who) + ": more than 2^20 samples per pixel");
     if (job->max_path_length > 1024) return sgpu_fail(SGPU_E_RANGE, std::string(who) + ": path length above 1024");
-    if (job->flags & ~SGPU_DPT_SORT_HITS) return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": unknown flags");
+    if (job->flags & ~(SGPU_DPT_SORT_HITS | SGPU_DPT_SORT_RAYS)) return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": unknown flags");
     SGPU_HIP_TRY(hipSetDevice(scene->ctx->device));
     return sgpu_dpt_render_impl(scene, job, framebuffer, as_stream(stream));
 }

@@ -181,6 +181,49 @@ __global__ __launch_bounds__(kBlock) void k_dpt_keys(uint32_t count, const uint3
     vals[j] = j;
 }
 
+// PathTracingBatcher::GetSortedBatch (PathTracingBatcher.cpp:529-560, 705-750): rays binned by the
+// axis and sign of their direction's largest component (RayBatchCategory), sorted by origin along
+// that axis (quantised to 29 bits over the scene box).
+__global__ __launch_bounds__(kBlock) void k_dpt_ray_keys(uint32_t count, RayBuf r, f3 lo, f3 inv_ext,
+                                                         uint32_t* __restrict keys, uint32_t* __restrict vals)
+{
+    uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= count) return;
+    const float ax = fabsf(r.dx[j]), ay = fabsf(r.dy[j]), az = fabsf(r.dz[j]);
+    uint32_t axis = 2;
+    float dv = r.dz[j], o = r.oz[j], l = lo.z, s = inv_ext.z;
+    if (ax >= ay && ax >= az) {
+        axis = 0;
+        dv = r.dx[j];
+        o = r.ox[j];
+        l = lo.x;
+        s = inv_ext.x;
+    } else if (ay >= az) {
+        axis = 1;
+        dv = r.dy[j];
+        o = r.oy[j];
+        l = lo.y;
+        s = inv_ext.y;
+    }
+    const float t = fminf(fmaxf((o - l) * s, 0.0f), 1.0f);
+    const uint32_t q = min((uint32_t)(t * 536870911.0f), 536870911u);
+    keys[j] = ((2u * axis + (dv < 0.0f ? 1u : 0u)) << 29) | q;
+    vals[j] = j;
+}
+
+// Rays and their path ids copied into sorted order.
+__global__ __launch_bounds__(kBlock) void k_dpt_ray_gather(uint32_t count, const uint32_t* __restrict order, RayBuf src,
+                                                           const uint32_t* __restrict qsrc, RayBuf dst,
+                                                           uint32_t* __restrict qdst)
+{
+    uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= count) return;
+    const uint32_t j = order[k];
+    put_ray(dst, k, mk3(src.ox[j], src.oy[j], src.oz[j]), mk3(src.dx[j], src.dy[j], src.dz[j]), src.tnear[j],
+            src.tfar[j]);
+    qdst[k] = qsrc[j];
+}
+
 struct ShadeParams {
     const float* materials;
     const uint32_t* geom_material;
@@ -472,6 +515,7 @@ struct PathJob {
     uint32_t flags;
     bool dpt;
     bool sort_hits;
+    bool sort_rays;
 };
 
 int render_paths(sgpu_scene scene, const PathJob& job, float* fb, hipStream_t stream)
@@ -497,8 +541,12 @@ int render_paths(sgpu_scene scene, const PathJob& job, float* fb, hipStream_t st
     ps.rng = A.get<uint64_t>(cap);
     ps.flags = A.get<uint32_t>(cap);
     ps.pdf = job.dpt ? A.get<float>(cap) : nullptr;
-    RayBuf rays[2] = {A.rays(cap), A.rays(cap)};
-    uint32_t* queue[2] = {A.get<uint32_t>(cap), A.get<uint32_t>(cap)};
+    RayBuf rays[3] = {A.rays(cap), A.rays(cap), RayBuf{}};
+    uint32_t* queue[3] = {A.get<uint32_t>(cap), A.get<uint32_t>(cap), nullptr};
+    if (job.sort_rays) {
+        rays[2] = A.rays(cap);
+        queue[2] = A.get<uint32_t>(cap);
+    }
     HitBuf hb;
     hb.t = A.get<float>(cap);
     hb.u = A.get<float>(cap);
@@ -552,6 +600,23 @@ int render_paths(sgpu_scene scene, const PathJob& job, float* fb, hipStream_t st
     size_t sort_bytes = 0;
     uint32_t end_bit = 32;
     while (end_bit < 64 && ((uint64_t)sp.n_geoms >> (end_bit - 32)) != 0) ++end_bit;
+    uint32_t* rkeys[2] = {nullptr, nullptr};
+    uint32_t* rvals[2] = {nullptr, nullptr};
+    void* rsort_tmp = nullptr;
+    size_t rsort_bytes = 0;
+    if (job.sort_rays) {
+        for (int k = 0; k < 2; ++k) {
+            rkeys[k] = A.get<uint32_t>(cap);
+            rvals[k] = A.get<uint32_t>(cap);
+        }
+        SGPU_HIP_TRY(rocprim::radix_sort_pairs(nullptr, rsort_bytes, rkeys[0], rkeys[1], rvals[0], rvals[1], (size_t)cap,
+                                               0, 32, stream));
+        rsort_tmp = A.get<uint8_t>(rsort_bytes);
+    }
+    const f3 box_lo = mk3(scene->box[0], scene->box[1], scene->box[2]);
+    auto inv_extent = [](float lo, float hi) { return hi > lo ? 1.0f / (hi - lo) : 0.0f; };
+    const f3 box_inv = mk3(inv_extent(scene->box[0], scene->box[3]), inv_extent(scene->box[1], scene->box[4]),
+                           inv_extent(scene->box[2], scene->box[5]));
     if (job.sort_hits) {
         keys[0] = A.get<uint64_t>(cap);
         keys[1] = A.get<uint64_t>(cap);
@@ -629,6 +694,21 @@ int render_paths(sgpu_scene scene, const PathJob& job, float* fb, hipStream_t st
             // per path: the light sample's contribution, then the sky sample's
             if ((rc = resolve(sb, h_counters[1]))) return rc;
             if (job.dpt && (rc = resolve(sky, h_counters[2]))) return rc;
+            if (job.sort_rays && n_next > 1) {
+                const int nx = 1 - cur;
+                const dim3 g((n_next + kBlock - 1) / kBlock);
+                hipLaunchKernelGGL(k_dpt_ray_keys, g, dim3(kBlock), 0, stream, n_next, rays[nx], box_lo, box_inv,
+                                   rkeys[0], rvals[0]);
+                SGPU_HIP_TRY(hipGetLastError());
+                size_t bytes = rsort_bytes;
+                SGPU_HIP_TRY(rocprim::radix_sort_pairs(rsort_tmp, bytes, rkeys[0], rkeys[1], rvals[0], rvals[1],
+                                                       (size_t)n_next, 0, 32, stream));
+                hipLaunchKernelGGL(k_dpt_ray_gather, g, dim3(kBlock), 0, stream, n_next, rvals[1], rays[nx], queue[nx],
+                                   rays[2], queue[2]);
+                SGPU_HIP_TRY(hipGetLastError());
+                std::swap(rays[nx], rays[2]);
+                std::swap(queue[nx], queue[2]);
+            }
             count = n_next;
             cur = 1 - cur;
         }
@@ -648,13 +728,13 @@ int render_paths(sgpu_scene scene, const PathJob& job, float* fb, hipStream_t st
 
 int sgpu_pt_render_impl(sgpu_scene scene, const sgpu_pt_job* job, float* fb, hipStream_t stream)
 {
-    PathJob pj{job->camera, job->spp, 1u, job->max_path_length, job->seed, job->flags, false, false};
+    PathJob pj{job->camera, job->spp, 1u, job->max_path_length, job->seed, job->flags, false, false, false};
     return render_paths(scene, pj, fb, stream);
 }
 
 int sgpu_dpt_render_impl(sgpu_scene scene, const sgpu_dpt_job* job, float* fb, hipStream_t stream)
 {
     PathJob pj{job->camera, job->spp_x, job->spp_y, job->max_path_length, job->seed, job->flags, true,
-               (job->flags & SGPU_DPT_SORT_HITS) != 0};
+               (job->flags & SGPU_DPT_SORT_HITS) != 0, (job->flags & SGPU_DPT_SORT_RAYS) != 0};
     return render_paths(scene, pj, fb, stream);
 }

@@ -450,6 +450,7 @@ class DptJob(C.Structure):
 
 
 DPT_SORT_HITS = 1
+DPT_SORT_RAYS = 2
 
 
 def build_render_scene(ctx: Context, sc: dict) -> Scene:

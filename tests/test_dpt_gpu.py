@@ -43,7 +43,7 @@ CASES = {
 }
 
 
-@pytest.mark.parametrize("flags", [0, N.DPT_SORT_HITS])
+@pytest.mark.parametrize("flags", [0, N.DPT_SORT_HITS | N.DPT_SORT_RAYS])
 @pytest.mark.parametrize("case", list(CASES))
 def test_deferred_matches_oracle_per_pixel(gpu_ctx, case, flags):
     sc = CASES[case]()
@@ -65,10 +65,11 @@ def test_hit_sort_does_not_change_the_image(gpu_ctx, case):
     scene = N.build_render_scene(gpu_ctx, sc)
     try:
         print(f"deferred sort on/off {case}", flush=True)
-        a = N.render_deferred_pt(scene, sc["camera"], 96, 64, 3, 3, 10, 5, flags=N.DPT_SORT_HITS)
         b = N.render_deferred_pt(scene, sc["camera"], 96, 64, 3, 3, 10, 5, flags=0)
-        assert a.mean() > 0.005
-        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+        assert b.mean() > 0.005
+        for flags in (N.DPT_SORT_HITS, N.DPT_SORT_RAYS, N.DPT_SORT_HITS | N.DPT_SORT_RAYS):
+            a = N.render_deferred_pt(scene, sc["camera"], 96, 64, 3, 3, 10, 5, flags=flags)
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), flags
     finally:
         scene.release()
 
