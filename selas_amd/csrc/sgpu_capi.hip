@@ -250,9 +250,11 @@ int sgpu_event_elapsed_ms(void* start, void* stop, float* ms)
 int sgpu_scene_create(sgpu_context ctx, sgpu_scene* out)
 {
     if (!ctx || !out) return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_create: NULL argument");
+    static std::atomic<uint64_t> next_uid{1};
     sgpu_scene s = new (std::nothrow) sgpu_scene_s();
     if (!s) return sgpu_fail(SGPU_E_NOMEM, "sgpu_scene_create: out of host memory");
     s->ctx = ctx;
+    s->uid = next_uid.fetch_add(1);
     *out = s;
     return SGPU_OK;
 }
@@ -476,6 +478,8 @@ int sgpu_scene_release(sgpu_scene scene)
         std::lock_guard<std::mutex> lock(scene->mu);
         if (scene->refs > 0)
             return sgpu_fail(SGPU_E_STATE, "sgpu_scene_release: the scene is placed by an instance of another scene");
+        if (scene->server_jobs.load() > 0)
+            return sgpu_fail(SGPU_E_STATE, "sgpu_scene_release: jobs on the scene are admitted to a VCM server");
     }
     for (const auto& in : scene->instances) {
         std::lock_guard<std::mutex> lock(in.child->mu);
@@ -516,6 +520,8 @@ int sgpu_scene_set_materials(sgpu_scene scene, const float* materials, uint32_t 
             return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_materials: shader must be 0, 1 or 2");
     }
     std::lock_guard<std::mutex> lock(scene->mu);
+    if (scene->server_jobs.load() > 0)
+        return sgpu_fail(SGPU_E_STATE, "sgpu_scene_set_materials: jobs on the scene are admitted to a VCM server");
     SGPU_HIP_TRY(hipSetDevice(scene->ctx->device));
     float* dm = nullptr;
     uint32_t* dg = nullptr;
@@ -549,6 +555,8 @@ int sgpu_scene_set_lights(sgpu_scene scene, const sgpu_light* lights, uint32_t n
     for (uint32_t i = 0; i < n_lights; ++i)
         if (lights[i].type > 1) return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_lights: light type must be 0 or 1");
     std::lock_guard<std::mutex> lock(scene->mu);
+    if (scene->server_jobs.load() > 0)
+        return sgpu_fail(SGPU_E_STATE, "sgpu_scene_set_lights: jobs on the scene are admitted to a VCM server");
     SGPU_HIP_TRY(hipSetDevice(scene->ctx->device));
     void* dl = nullptr;
     if (n_lights > 0) {
@@ -614,6 +622,8 @@ int sgpu_scene_set_ibl(sgpu_scene scene, uint32_t width, uint32_t height, const 
 {
     if (!scene) return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_ibl: NULL scene");
     std::lock_guard<std::mutex> lock(scene->mu);
+    if (scene->server_jobs.load() > 0)
+        return sgpu_fail(SGPU_E_STATE, "sgpu_scene_set_ibl: jobs on the scene are admitted to a VCM server");
     if (!scene->committed) return sgpu_fail(SGPU_E_STATE, "sgpu_scene_set_ibl: scene not committed");
     SGPU_HIP_TRY(hipSetDevice(scene->ctx->device));
     if (!rgb) {
@@ -658,6 +668,8 @@ int sgpu_scene_set_ptex(sgpu_scene scene, uint32_t geom_id, uint32_t res_log2, u
     if (!scene || !texels || !adjfaces || !adjedges || n_faces == 0)
         return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_ptex: NULL or empty argument");
     std::lock_guard<std::mutex> lock(scene->mu);
+    if (scene->server_jobs.load() > 0)
+        return sgpu_fail(SGPU_E_STATE, "sgpu_scene_set_ptex: jobs on the scene are admitted to a VCM server");
     const size_t n_geoms = scene->geom_first.size();
     if (geom_id >= n_geoms) return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_ptex: no such geometry");
     if (!scene->geom_quad[geom_id]) return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_ptex: not a quad geometry");
@@ -734,23 +746,42 @@ int sgpu_ptex_eval(sgpu_scene scene, uint32_t geom_id, const int32_t* faces, con
     return SGPU_OK;
 }
 
-int sgpu_render_pt(sgpu_scene scene, const sgpu_pt_job* job, float* framebuffer, void* stream)
+static int check_pt_job(sgpu_scene scene, const sgpu_pt_job* job, const float* framebuffer, const std::string& who)
 {
-    if (!scene || !job || !framebuffer) return sgpu_fail(SGPU_E_INVALID, "sgpu_render_pt: NULL argument");
-    if (!scene->committed) return sgpu_fail(SGPU_E_STATE, "sgpu_render_pt: scene not committed");
+    if (!scene || !job || !framebuffer) return sgpu_fail(SGPU_E_INVALID, who + ": NULL argument");
+    if (!scene->committed) return sgpu_fail(SGPU_E_STATE, who + ": scene not committed");
     if (!scene->instances.empty()) {
-        if (!scene->render_error.empty()) return sgpu_fail(SGPU_E_STATE, "sgpu_render_pt: " + scene->render_error);
+        if (!scene->render_error.empty()) return sgpu_fail(SGPU_E_STATE, who + ": " + scene->render_error);
     } else if (!scene->d_materials || scene->n_geom_material < scene->geom_first.size()) {
-        return sgpu_fail(SGPU_E_STATE, "sgpu_render_pt: materials not set for every geometry");
+        return sgpu_fail(SGPU_E_STATE, who + ": materials not set for every geometry");
     }
     if (job->camera.width == 0 || job->camera.height == 0 || job->spp == 0 || job->max_path_length == 0)
-        return sgpu_fail(SGPU_E_INVALID, "sgpu_render_pt: zero width, height, spp or path length");
+        return sgpu_fail(SGPU_E_INVALID, who + ": zero width, height, spp or path length");
     if ((uint64_t)job->camera.width * job->camera.height > (1ull << 26))
-        return sgpu_fail(SGPU_E_RANGE, "sgpu_render_pt: more than 2^26 pixels");
-    if (job->spp > (1u << 20)) return sgpu_fail(SGPU_E_RANGE, "sgpu_render_pt: spp above 2^20");
-    if (job->max_path_length > 1024) return sgpu_fail(SGPU_E_RANGE, "sgpu_render_pt: path length above 1024");
+        return sgpu_fail(SGPU_E_RANGE, who + ": more than 2^26 pixels");
+    if (job->spp > (1u << 20)) return sgpu_fail(SGPU_E_RANGE, who + ": spp above 2^20");
+    if (job->max_path_length > 1024) return sgpu_fail(SGPU_E_RANGE, who + ": path length above 1024");
+    if (job->flags & ~(SGPU_PT_NEE_COSINE_ONCE | SGPU_PT_COUNT_TRAVERSAL))
+        return sgpu_fail(SGPU_E_INVALID, who + ": unknown flags");
+    return SGPU_OK;
+}
+
+int sgpu_render_pt(sgpu_scene scene, const sgpu_pt_job* job, float* framebuffer, void* stream)
+{
+    int rc = check_pt_job(scene, job, framebuffer, "sgpu_render_pt");
+    if (rc) return rc;
     SGPU_HIP_TRY(hipSetDevice(scene->ctx->device));
     return sgpu_pt_render_impl(scene, job, framebuffer, as_stream(stream));
+}
+
+int sgpu_render_pt_stats(sgpu_scene scene, const sgpu_pt_job* job, float* framebuffer, sgpu_pt_stats* stats,
+                         void* stream)
+{
+    int rc = check_pt_job(scene, job, framebuffer, "sgpu_render_pt_stats");
+    if (rc) return rc;
+    if (!stats) return sgpu_fail(SGPU_E_INVALID, "sgpu_render_pt_stats: NULL stats");
+    SGPU_HIP_TRY(hipSetDevice(scene->ctx->device));
+    return sgpu_pt_render_impl(scene, job, framebuffer, as_stream(stream), stats);
 }
 
 int sgpu_render_deferred_pt(sgpu_scene scene, const sgpu_dpt_job* job, float* framebuffer, void* stream)
@@ -789,7 +820,8 @@ static int check_vcm_job(sgpu_scene scene, const sgpu_vcm_job* job, const void* 
         return sgpu_fail(SGPU_E_RANGE, std::string(who) + ": more than 2^27 geometries");
     if (job->camera.width == 0 || job->camera.height == 0 || job->iterations == 0 || job->max_path_length == 0)
         return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": zero width, height, iterations or path length");
-    if (job->flags & ~(SGPU_VCM_KERNEL_TIMES | SGPU_VCM_COUNT_TRAVERSAL | SGPU_VCM_ONE_SLICE_PER_WAVEFRONT))
+    if (job->flags & ~(SGPU_VCM_KERNEL_TIMES | SGPU_VCM_COUNT_TRAVERSAL | SGPU_VCM_ONE_SLICE_PER_WAVEFRONT |
+                       SGPU_VCM_COUNT_MERGE))
         return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": unknown flags");
     if (!(job->base_radius > 0.0f) || !(job->base_radius < 3.0e38f))
         return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": base_radius must be positive and finite");
@@ -816,7 +848,8 @@ int sgpu_render_vcm_batch(const sgpu_scene* scenes, const sgpu_vcm_job* jobs, fl
 {
     if (!scenes || !jobs || !framebuffers || n_jobs == 0)
         return sgpu_fail(SGPU_E_INVALID, "sgpu_render_vcm_batch: NULL or empty argument");
-    if (flags & ~(SGPU_VCM_KERNEL_TIMES | SGPU_VCM_COUNT_TRAVERSAL | SGPU_VCM_ONE_SLICE_PER_WAVEFRONT))
+    if (flags & ~(SGPU_VCM_KERNEL_TIMES | SGPU_VCM_COUNT_TRAVERSAL | SGPU_VCM_ONE_SLICE_PER_WAVEFRONT |
+                  SGPU_VCM_COUNT_MERGE))
         return sgpu_fail(SGPU_E_INVALID, "sgpu_render_vcm_batch: unknown flags");
     for (uint32_t j = 0; j < n_jobs; ++j) {
         int rc = check_vcm_job(scenes[j], &jobs[j], framebuffers[j], "sgpu_render_vcm_batch");
@@ -839,8 +872,10 @@ static int check_server_config(const sgpu_vcm_server_config* cfg, const char* wh
         return sgpu_fail(SGPU_E_RANGE, std::string(who) + ": wavefront_paths must lie in 1..2^26");
     if (cfg->max_path_length == 0 || cfg->max_path_length > 31)
         return sgpu_fail(SGPU_E_RANGE, std::string(who) + ": max_path_length must lie in 1..31");
+    // camera contribution slots (paths x (max_path_length - 2 + 2)) are numbered in 32 bits
+    static_assert((1ull << 26) * 31 < (1ull << 32), "slot numbers of the largest server exceed 32 bits");
     if (cfg->flags & ~(SGPU_VCM_KERNEL_TIMES | SGPU_VCM_COUNT_TRAVERSAL | SGPU_VCM_ONE_SLICE_PER_WAVEFRONT |
-                       SGPU_VCM_SHORTEST_FIRST))
+                       SGPU_VCM_SHORTEST_FIRST | SGPU_VCM_COUNT_MERGE))
         return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": unknown flags");
     return SGPU_OK;
 }

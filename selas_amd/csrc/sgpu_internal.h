@@ -6,6 +6,7 @@
 #include "sgpu_ptex.h"
 
 #include <hip/hip_runtime.h>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -89,6 +90,10 @@ struct sgpu_context_s {
 struct sgpu_scene_s {
     sgpu_context ctx;
     std::mutex mu;
+    uint64_t uid = 0;  // unique in the process (addresses of released scenes are reused)
+    // jobs on this scene admitted to VCM servers and not yet retired: while non-zero the scene
+    // cannot be released and its shading data cannot be replaced (the jobs hold its device pointers)
+    std::atomic<uint32_t> server_jobs{0};
     // host copy of the world-space triangles (9 floats each) until commit, in global index order
     std::vector<float> tri_pos;
     std::vector<uint32_t> tri_geom;   // geomID per triangle
@@ -157,6 +162,58 @@ struct sgpu_scene_s {
     std::vector<float> comp_l2w;
 };
 
+// ---- per-phase device times ----------------------------------------------------------------------
+// Events recorded on `stream` around each launch of a phase (begin/end), summed per phase by
+// collect(); a disabled timer records nothing.
+struct sgpu_phase_timer {
+    static constexpr int kMaxPhases = 8;
+    bool on;
+    hipStream_t stream;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> phase;
+    size_t used = 0;
+    double ms[kMaxPhases] = {};
+    sgpu_phase_timer(bool enabled, hipStream_t s) : on(enabled), stream(s) {}
+    sgpu_phase_timer(const sgpu_phase_timer&) = delete;
+    sgpu_phase_timer& operator=(const sgpu_phase_timer&) = delete;
+    hipError_t begin(int ph)
+    {
+        if (!on) return hipSuccess;
+        while (ev.size() < 2 * (used + 1)) {
+            hipEvent_t e;
+            hipError_t err = hipEventCreate(&e);
+            if (err != hipSuccess) return err;
+            ev.push_back(e);
+        }
+        if (phase.size() < used + 1) phase.resize(used + 1);
+        phase[used] = ph;
+        return hipEventRecord(ev[2 * used], stream);
+    }
+    hipError_t end()
+    {
+        if (!on) return hipSuccess;
+        hipError_t err = hipEventRecord(ev[2 * used + 1], stream);
+        ++used;
+        return err;
+    }
+    hipError_t collect()
+    {
+        if (!on) return hipSuccess;
+        hipError_t err = hipStreamSynchronize(stream);
+        for (size_t i = 0; i < used && err == hipSuccess; ++i) {
+            float t = 0.0f;
+            err = hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]);
+            ms[phase[i]] += t;
+        }
+        used = 0;
+        return err;
+    }
+    ~sgpu_phase_timer()
+    {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+};
+
 // ---- error plumbing ------------------------------------------------------------------------------
 void sgpu_set_error(const std::string& msg);
 int sgpu_fail(int code, const std::string& msg);
@@ -204,7 +261,8 @@ int sgpu_trace_launch_multi(sgpu_context ctx, const TraceScene* table, const uin
                             uint32_t* spill = nullptr, const uint32_t* n_ptr = nullptr, bool instanced = false,
                             uint32_t* comp_out = nullptr);
 // Wavefront path tracer (sgpu_pt.hip); arguments validated by sgpu_render_pt.
-int sgpu_pt_render_impl(sgpu_scene scene, const sgpu_pt_job* job, float* framebuffer, hipStream_t stream);
+int sgpu_pt_render_impl(sgpu_scene scene, const sgpu_pt_job* job, float* framebuffer, hipStream_t stream,
+                        sgpu_pt_stats* stats = nullptr);
 int sgpu_dpt_render_impl(sgpu_scene scene, const sgpu_dpt_job* job, float* framebuffer, hipStream_t stream);
 int sgpu_vcm_render_impl(sgpu_scene scene, const sgpu_vcm_job* job, float* framebuffer, sgpu_vcm_stats* stats,
                          hipStream_t stream);

@@ -622,7 +622,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
                     f3 c = (f * rad) * (mis / dpdf);
                     if (!is_zero3(c)) {
                         emit = true;
-                        s3(S.c, p * nslots + 0, thr * c);
+                        s3(S.c, (size_t)p * nslots + 0, thr * c);
                         so = offset_ray_origin(s, ld, 0.1f);
                         tn = s.error;
                         tf = dist;
@@ -687,7 +687,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
                             tn = s.error;
                             tf = dist - 16.0f * absf_(sb);
                             emit = true;
-                            s3(S.c, p * nslots + 1 + k, (thr * lthr) * c);
+                            s3(S.c, (size_t)p * nslots + 1 + k, (thr * lthr) * c);
                         }
                     }
                 }
@@ -780,7 +780,7 @@ __global__ __launch_bounds__(kBlock) void k_cam_gather(const uint32_t* __restric
         uint32_t p = queue[j];
         f3 col = g3(ps.color, p);
         for (uint32_t k = 0; k < nslots; ++k)
-            if (S.valid[(size_t)p * nslots + k] == 1u) col += g3(S.c, p * nslots + k);
+            if (S.valid[(size_t)p * nslots + k] == 1u) col += g3(S.c, (size_t)p * nslots + k);
         s3(ps.color, p, col);
     }
 }
@@ -851,51 +851,8 @@ struct Arena {
 // Device time per phase, from event pairs recorded around each phase's launches on the render's
 // stream (only when asked for: SGPU_VCM_KERNEL_TIMES). collect() runs after a stream sync.
 enum { kPhaseClosest, kPhaseShadow, kPhaseLight, kPhaseCamera, kPhaseGrid, kPhaseMerge, kPhases };
-struct PhaseTimer {
-    bool on;
-    hipStream_t stream;
-    std::vector<hipEvent_t> ev;
-    std::vector<int> phase;
-    size_t used = 0;
-    double ms[kPhases] = {};
-    PhaseTimer(bool enabled, hipStream_t s) : on(enabled), stream(s) {}
-    hipError_t begin(int ph)
-    {
-        if (!on) return hipSuccess;
-        while (ev.size() < 2 * (used + 1)) {
-            hipEvent_t e;
-            hipError_t err = hipEventCreate(&e);
-            if (err != hipSuccess) return err;
-            ev.push_back(e);
-        }
-        if (phase.size() < used + 1) phase.resize(used + 1);
-        phase[used] = ph;
-        return hipEventRecord(ev[2 * used], stream);
-    }
-    hipError_t end()
-    {
-        if (!on) return hipSuccess;
-        hipError_t err = hipEventRecord(ev[2 * used + 1], stream);
-        ++used;
-        return err;
-    }
-    hipError_t collect()
-    {
-        if (!on) return hipSuccess;
-        hipError_t err = hipStreamSynchronize(stream);
-        for (size_t i = 0; i < used && err == hipSuccess; ++i) {
-            float t = 0.0f;
-            err = hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]);
-            ms[phase[i]] += t;
-        }
-        used = 0;
-        return err;
-    }
-    ~PhaseTimer()
-    {
-        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
-    }
-};
+using PhaseTimer = sgpu_phase_timer;
+static_assert(kPhases <= sgpu_phase_timer::kMaxPhases, "too many phases for the timer");
 
 sgpu_rays_soa as_soa(const RayBuf& r) { return sgpu_rays_soa{r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, r.tnear, r.tfar}; }
 
@@ -974,8 +931,10 @@ struct sgpu_vcm_server_s {
     Slice* h_slices = nullptr;
     TraceScene* h_table = nullptr;
     bool any_instanced = false;  // an instanced scene is in the table (traversal by k_trace8i)
-    // scenes seen by the server (traversal table), jobs in admission order
+    // the scenes of the admitted jobs (traversal table, rebuilt at every step) and their unique
+    // ids, jobs in admission order
     std::vector<sgpu_scene> scenes;
+    std::vector<uint64_t> scene_uids;
     TraceScene* d_table = nullptr;
     uint32_t table_cap = 0;
     bool table_dirty = false;
@@ -985,6 +944,7 @@ struct sgpu_vcm_server_s {
     // totals
     uint64_t n_closest = 0, n_shadow = 0, n_light_verts = 0, n_launches = 0, n_samples = 0, n_waves = 0;
     uint64_t trav_nodes = 0, trav_tris = 0;
+    uint64_t n_merge_launches = 0, merge_cnt[4] = {0, 0, 0, 0};
     double ms[kPhases] = {};
 };
 
@@ -1107,7 +1067,7 @@ int server_alloc(sgpu_vcm_server sv)
     sv->frame = A.get<float>(3 * (size_t)TM);
     sv->d_slices = A.get<Slice>(sv->max_slices);
     sv->cam_vm = A.get<CamVertex>(TM);
-    sv->d_trav = A.get<uint64_t>(2);
+    sv->d_trav = A.get<uint64_t>(6);  // [0..1] traversal counters, [2..5] merging counters
     sv->S.c = A.get<float>(3 * (size_t)TM * nslots);
     sv->S.valid = A.get<uint32_t>((size_t)TM * nslots);
     sv->S.ray_slot = A.get<uint32_t>(max_shadow);
@@ -1138,16 +1098,43 @@ int server_alloc(sgpu_vcm_server sv)
     return SGPU_OK;
 }
 
-// Scene table entry of `scene` (added on first use).
-uint32_t server_scene(sgpu_vcm_server sv, sgpu_scene scene)
+// The scene table of the next step: the scenes of the admitted jobs in order of first admission,
+// each job's table index in its base slice. Built anew at every step, so a retired job's scene
+// (which may since have been released, and a new scene created at its address) is never reached
+// through a stale entry; the table is re-uploaded when the list of scene ids changed.
+void server_sync_scenes(sgpu_vcm_server sv)
 {
-    for (size_t k = 0; k < sv->scenes.size(); ++k)
-        if (sv->scenes[k] == scene) return (uint32_t)k;
-    sv->scenes.push_back(scene);
-    sv->table_dirty = true;
-    if (!scene->instances.empty()) sv->any_instanced = true;
-    sv->max_stack = std::max(sv->max_stack, scene->stack_cap);
-    return (uint32_t)(sv->scenes.size() - 1);
+    std::vector<sgpu_scene> list;
+    std::vector<uint64_t> uids;
+    bool inst = false;
+    uint32_t stack = 0;
+    for (ServerJob& j : sv->active) {
+        uint32_t k = 0;
+        while (k < list.size() && list[k] != j.scene) ++k;  // admitted scenes are alive: addresses differ
+        if (k == list.size()) {
+            list.push_back(j.scene);
+            uids.push_back(j.scene->uid);
+            inst |= !j.scene->instances.empty();
+            stack = std::max(stack, j.scene->stack_cap);
+        }
+        j.base.scene = k;
+    }
+    if (uids != sv->scene_uids) {
+        sv->scenes.swap(list);
+        sv->scene_uids.swap(uids);
+        sv->table_dirty = true;
+    }
+    // kept from earlier steps (reset with the server): a larger stack bound and the instanced
+    // traversal serve every scene, and the captured graphs, which bake both in, stay valid
+    sv->any_instanced |= inst;
+    sv->max_stack = std::max(sv->max_stack, stack);
+}
+
+// A job leaves the server (retired or dropped): its scene may be changed or released again.
+void server_forget_job(ServerJob& j)
+{
+    if (j.scene) j.scene->server_jobs.fetch_sub(1);
+    j.scene = nullptr;
 }
 
 int server_upload_table(sgpu_vcm_server sv, hipStream_t stream)
@@ -1179,8 +1166,10 @@ int server_upload_table(sgpu_vcm_server sv, hipStream_t stream)
 // size (at most the allocated one) and flags.
 static void server_reset(sgpu_vcm_server sv, const sgpu_vcm_server_config& cfg)
 {
+    for (ServerJob& j : sv->active) server_forget_job(j);
     sv->active.clear();
     sv->scenes.clear();
+    sv->scene_uids.clear();
     sv->table_dirty = true;
     sv->max_stack = 0;
     sv->any_instanced = false;
@@ -1188,6 +1177,8 @@ static void server_reset(sgpu_vcm_server sv, const sgpu_vcm_server_config& cfg)
     sv->cap = (uint32_t)std::min<uint64_t>(cfg.wavefront_paths, sv->TM);
     sv->n_closest = sv->n_shadow = sv->n_light_verts = sv->n_launches = sv->n_samples = sv->n_waves = 0;
     sv->trav_nodes = sv->trav_tris = 0;
+    sv->n_merge_launches = 0;
+    for (int k = 0; k < 4; ++k) sv->merge_cnt[k] = 0;
     for (int k = 0; k < kPhases; ++k) sv->ms[k] = 0.0;
 }
 
@@ -1231,6 +1222,7 @@ int sgpu_vcm_server_destroy_impl(sgpu_vcm_server sv, hipStream_t stream)
 {
     if (!sv) return SGPU_OK;
     sv->A.stream = stream;
+    for (ServerJob& j : sv->active) server_forget_job(j);
     if (sv->hbuf) (void)hipHostFree(sv->hbuf);
     if (sv->h_table) (void)hipHostFree(sv->h_table);
     if (sv->d_table) (void)hipFree(sv->d_table);
@@ -1279,7 +1271,7 @@ int sgpu_vcm_server_submit_impl(sgpu_vcm_server sv, sgpu_scene sc, const sgpu_vc
     b.L = L;
     b.mlv = L > 2 ? L - 2 : 1;
     b.seed = jb->seed;
-    b.scene = server_scene(sv, sc);
+    b.scene = 0;  // the job's scene table index, set at each step (server_sync_scenes)
     b.materials = sc->d_materials;
     b.geom_material = sc->d_geom_material;
     b.n_geoms = sc->n_geom_material;
@@ -1300,6 +1292,7 @@ int sgpu_vcm_server_submit_impl(sgpu_vcm_server sv, sgpu_scene sc, const sgpu_vc
         b.inst_l2w = sc->d_inst_l2w;
     }
     SGPU_HIP_TRY(hipMemsetAsync(fb, 0, 3 * (size_t)b.N * sizeof(float), stream));
+    sc->server_jobs.fetch_add(1);
     sv->active.push_back(j);
     if (id_out) *id_out = j.id;
     return SGPU_OK;
@@ -1316,6 +1309,7 @@ int sgpu_vcm_server_step_impl(sgpu_vcm_server sv, hipStream_t stream, uint64_t* 
     }
     const uint32_t flags = sv->cfg.flags;
     const bool one_slice = flags & SGPU_VCM_ONE_SLICE_PER_WAVEFRONT;
+    server_sync_scenes(sv);
     int rc = server_upload_table(sv, stream);
     if (rc) return rc;
 
@@ -1369,8 +1363,9 @@ int sgpu_vcm_server_step_impl(sgpu_vcm_server sv, hipStream_t stream, uint64_t* 
 
     const bool kernel_times = flags & SGPU_VCM_KERNEL_TIMES;
     uint64_t* d_trav = (flags & SGPU_VCM_COUNT_TRAVERSAL) ? sv->d_trav : nullptr;
+    unsigned long long* d_mstat = (flags & SGPU_VCM_COUNT_MERGE) ? (unsigned long long*)(sv->d_trav + 2) : nullptr;
     // the two passes replay captured graphs unless launches are timed or counted one by one
-    const bool graphs = sv->ctx->vcm_graphs && !kernel_times && !d_trav && !sv->graph_failed;
+    const bool graphs = sv->ctx->vcm_graphs && !kernel_times && !d_trav && !d_mstat && !sv->graph_failed;
     if (graphs) single = false;  // graph launches traverse through the scene table
     PhaseTimer T(kernel_times, stream);
     // per-ray scene indices are only written and read in multi-scene wavefronts
@@ -1395,7 +1390,7 @@ int sgpu_vcm_server_step_impl(sgpu_vcm_server sv, hipStream_t stream, uint64_t* 
     sv->h_dims[1] = total;
     SGPU_HIP_TRY(hipMemcpyAsync(sv->d_slices, sv->h_slices, sizeof(Slice) * ns, hipMemcpyHostToDevice, stream));
     SGPU_HIP_TRY(hipMemcpyAsync(sv->dims, sv->h_dims, 2 * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
-    if (d_trav) SGPU_HIP_TRY(hipMemsetAsync(d_trav, 0, 2 * sizeof(uint64_t), stream));
+    if (d_trav || d_mstat) SGPU_HIP_TRY(hipMemsetAsync(sv->d_trav, 0, 6 * sizeof(uint64_t), stream));
     Wave wv;
     wv.slices = sv->d_slices;
     wv.dims = sv->dims;
@@ -1479,7 +1474,7 @@ int sgpu_vcm_server_step_impl(sgpu_vcm_server sv, hipStream_t stream, uint64_t* 
             if (pass == 1) {
                 SGPU_HIP_TRY(T.begin(kPhaseMerge));
                 const uint32_t merge_blocks = std::min<uint32_t>((n + 2 * kMergeWaves - 1) / (2 * kMergeWaves), 16u * cus);
-                launch_cam_merge(merge_blocks, st, wv, c_in, sv->cam_vm, ps.slice, G, S);
+                launch_cam_merge(merge_blocks, st, wv, c_in, sv->cam_vm, ps.slice, G, S, d_mstat);
                 SGPU_HIP_TRY(hipGetLastError());
                 SGPU_HIP_TRY(T.end());
             }
@@ -1605,15 +1600,17 @@ int sgpu_vcm_server_step_impl(sgpu_vcm_server sv, hipStream_t stream, uint64_t* 
             sv->n_closest += c[0];
             sv->n_shadow += c[1];
             ++sv->n_launches;
+            if (pass == 1) ++sv->n_merge_launches;
         }
     if (sv->h_cnt[sv->cnt_words - 1] != 0)
         return sgpu_fail(SGPU_E_RANGE, "sgpu_vcm_server_step: traversal stack overflow (results incomplete)");
     for (int k = kPhaseClosest; k < kPhases; ++k) sv->ms[k] += T.ms[k];
-    if (d_trav) {
-        uint64_t tr[2] = {0, 0};
-        SGPU_HIP_TRY(hipMemcpy(tr, d_trav, sizeof(tr), hipMemcpyDeviceToHost));
+    if (d_trav || d_mstat) {
+        uint64_t tr[6] = {0, 0, 0, 0, 0, 0};
+        SGPU_HIP_TRY(hipMemcpy(tr, sv->d_trav, sizeof(tr), hipMemcpyDeviceToHost));
         sv->trav_nodes += tr[0];
         sv->trav_tris += tr[1];
+        for (int k = 0; k < 4; ++k) sv->merge_cnt[k] += tr[2 + k];
     }
     std::sort(finished.begin(), finished.end());  // erase from the back
     uint32_t nd = 0;
@@ -1621,6 +1618,7 @@ int sgpu_vcm_server_step_impl(sgpu_vcm_server sv, hipStream_t stream, uint64_t* 
         const uint32_t ji = finished[k];
         if (done_ids && nd < max_done) done_ids[nd] = sv->active[ji].id;
         ++nd;
+        server_forget_job(sv->active[ji]);
         sv->active.erase(sv->active.begin() + ji);
     }
     if (n_done) *n_done = nd;
@@ -1646,6 +1644,11 @@ int sgpu_vcm_server_stats_impl(sgpu_vcm_server sv, sgpu_vcm_stats* st)
     st->camera_ms = (float)sv->ms[kPhaseCamera];
     st->grid_ms = (float)sv->ms[kPhaseGrid];
     st->merge_ms = (float)sv->ms[kPhaseMerge];
+    st->merge_launches = sv->n_merge_launches;
+    st->merge_entries = sv->merge_cnt[0];
+    st->merge_queries = sv->merge_cnt[1];
+    st->merge_visited = sv->merge_cnt[2];
+    st->merge_near = sv->merge_cnt[3];
     return SGPU_OK;
 }
 

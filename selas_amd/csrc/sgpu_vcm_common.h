@@ -151,8 +151,10 @@ struct Wave {
     uint32_t nslots;       // camera contribution slots per path (mlv + 2)
 };
 
-__device__ __forceinline__ f3 g3(const float* a, uint32_t i) { return mk3(a[3 * i], a[3 * i + 1], a[3 * i + 2]); }
-__device__ __forceinline__ void s3(float* a, uint32_t i, f3 v)
+// Element i of an array of float triples; the index is 64-bit because 3 i exceeds 2^32 for the
+// camera contribution slots of the largest wavefronts (2^26 paths x 31 slots).
+__device__ __forceinline__ f3 g3(const float* a, size_t i) { return mk3(a[3 * i], a[3 * i + 1], a[3 * i + 2]); }
+__device__ __forceinline__ void s3(float* a, size_t i, f3 v)
 {
     a[3 * i] = v.x;
     a[3 * i + 1] = v.y;
@@ -273,7 +275,10 @@ struct CamSlots {
 
 // Vertex merging of the camera vertices of one bounce (sgpu_vcm_merge.hip).
 constexpr uint32_t kMergeWaves = kBlock / 64;
+// mstat (NULL: not counted): [0] camera vertices read, [1] range searches, [2] photons visited,
+// [3] photons within the radius, summed with atomics.
 void launch_cam_merge(uint32_t blocks, hipStream_t stream, const Wave& w, const uint32_t* count_ptr,
-                      const CamVertex* cam_vm, const uint32_t* path_slice, const GridView& G, const CamSlots& S);
+                      const CamVertex* cam_vm, const uint32_t* path_slice, const GridView& G, const CamSlots& S,
+                      unsigned long long* mstat);
 
 }  // namespace sgpu_vcm

@@ -72,3 +72,31 @@ def test_admission_bounded_by_footprint_on_gpu(gpu_ctx):
             assert _close(r.image, _alone(eng, r)) >= 0.999
     finally:
         eng.close()
+
+
+def test_eviction_and_reload_match_alone(gpu_ctx):
+    """An HBM budget that holds one scene at a time: scenes are evicted (released while the server
+    lives) and built again for later requests; every result equals the job alone on a fresh build."""
+    eng = S.GpuEngine(gpu_ctx, lambda sid: SPECS[sid](), wavefront_paths=6000, max_path_length=10)
+    try:
+        sizes = {}
+        for sid in SPECS:
+            eng.load_scene(sid)
+            sizes[sid] = eng.scene_bytes(sid)
+            eng.evict_scene(sid)
+        rng = np.random.default_rng(4)
+        order = [0, 1, 2, 1, 0, 2, 0]
+        jobs = [{"scene": s, "width": 32, "height": 24, "iterations": int(rng.integers(1, 4)),
+                 "seed": int(rng.integers(0, 1 << 40)), "max_path_length": 8} for s in order]
+        reqs = S.poisson_requests(5, 1000.0, jobs)
+        loop = S.RequestLoop(eng, hbm_budget=max(sizes.values()) + 4 * 12 * 32 * 24, max_slots=4, keep_images=True)
+        loop.run(reqs)
+        assert all(r.status == "done" for r in reqs)
+        assert loop.evictions >= 3
+        for sid in list(eng.resident_scenes()):
+            eng.evict_scene(sid)
+        for r in reqs:
+            eng.load_scene(r.scene)
+            assert _close(r.image, _alone(eng, r)) >= 0.999, r
+    finally:
+        eng.close()

@@ -84,3 +84,55 @@ def test_server_limits_and_footprint(gpu_ctx, two_scenes):
     finally:
         srv.close()
         fb.free()
+
+
+def test_scene_held_while_jobs_admitted(gpu_ctx):
+    """A scene with admitted jobs can be neither released nor given new lights or an image-based
+    light (the jobs hold its device pointers); once its jobs retire both work again."""
+    spec = W.room_scene(2)
+    sc = N.build_render_scene(gpu_ctx, spec)
+    srv = N.VcmServer(gpu_ctx, wavefront_paths=4096, max_path_length=6)
+    fb = gpu_ctx.empty(12 * 32 * 32)
+    try:
+        srv.submit(sc, _job(spec, sc, 32, 32, 3, 7, L=6), fb)
+        for call in (sc.release, lambda: sc.set_ibl(np.ones((4, 8, 3), np.float32))):
+            with pytest.raises(N.SgpuError) as e:
+                call()
+            assert e.value.code == N.SGPU_E_STATE
+        assert sc.handle is not None
+        active = 1
+        while active:
+            _, active = srv.step()
+        sc.set_ibl(np.ones((4, 8, 3), np.float32))
+        sc.set_ibl(None)
+    finally:
+        srv.close()
+        fb.free()
+        sc.release()
+    assert sc.handle is None
+
+
+def test_released_scene_address_reuse(gpu_ctx):
+    """Scenes released between a live server's jobs and new scenes built in their place (often at
+    the same host address) are traversed through fresh table entries: each job's image equals the
+    job rendered alone on its own scene."""
+    srv = N.VcmServer(gpu_ctx, wavefront_paths=8192, max_path_length=8)
+    try:
+        for rnd, seed in enumerate((31, 32, 33)):
+            spec = W.vcm_scene(seed=seed, n_meshes=4 + 2 * rnd, nu=8 + 4 * rnd, nv=8)
+            sc = N.build_render_scene(gpu_ctx, spec)
+            j = _job(spec, sc, 40, 24, 3, 100 + rnd, L=8)
+            fb = gpu_ctx.empty(12 * 40 * 24)
+            try:
+                srv.submit(sc, j, fb)
+                active = 1
+                while active:
+                    _, active = srv.step()
+                served = fb.download(np.float32, 3 * 40 * 24).reshape(24, 40, 3)
+                alone = N.render_vcm(sc, j["camera"], 40, 24, 3, 8, j["base_radius"], 0.75, j["seed"])
+                assert _close(served, alone) >= 0.999, rnd
+            finally:
+                fb.free()
+                sc.release()
+    finally:
+        srv.close()
