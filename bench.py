@@ -2,26 +2,33 @@
 
 --workload vcm (default): vertex connection and merging on the VCM workload scene
 (selas_amd/workloads.py vcm_scene: 64 displaced-grid quad meshes, 1,048,576 triangles, per-face
-16 x 16 RGB8 textures, one quad and one point light), 512 x 512, max path length 10, merge radius
-0.3 % of the scene diagonal shrinking
-with alpha 0.75. One step = one VCM iteration (a light path and a camera path per pixel); the timed
-region is one sgpu_render_vcm call of `steps` iterations (32 = the workload's job), with the scene
-and the BVH resident in HBM. value = camera samples per second; Mrays/s counts every closest-hit
-and shadow ray. The roofline object describes the closest-hit traversal kernel inside the render:
-algorithmic bytes per ray = 48 B of compulsory ray/hit traffic + nodes visited x 64 B + triangles
-tested x 48 B (counted over all closest-hit rays of an untimed iteration), divided by the kernel's
-device time summed from HIP events recorded around each launch on the render's stream.
+16 x 16 RGB8 textures, one quad and one point light), 512 x 512, 32 iterations, max path length 10,
+merge radius 0.3 % of the scene diagonal shrinking with alpha 0.75. One step = one whole job (the
+512 x 512 x 32-iteration render, from scratch); the timed region is `steps` such sgpu_render_vcm
+calls with the scene and the BVH resident in HBM. value = camera samples per second; Mrays/s
+counts every closest-hit and shadow ray. HIP events recorded around every launch on the render's
+stream time the kernels. The roofline object describes the vertex-merging kernel (k_cam_merge, the
+largest kernel of the render): its algorithmic bytes are counted over an untimed run of the same
+job (camera vertices read, range searches, photons visited, photons within the radius; see
+merge_bytes); roofline_closest_hit does the same for the closest-hit traversal (48 B of ray/hit
+traffic + nodes visited x node bytes + triangles tested x 48 B per ray). cpu_baseline renders the
+same job with the oracle on the host's cores and compares the images pixel by pixel.
 
 The default run adds two legs to the same line (skip them with --no-batch / --no-serve):
 "batched": 64 uneven jobs (square images with side log-uniform in [64, 1024], 1..64 iterations,
 each on a scene of its own of 10K..2M triangles), rendered as one batch and, for comparison, one job
-at a time with one wavefront per iteration (the plain GPU schedule); "serving": a Poisson stream of
-such requests over 16 resident scenes at --serve-rate requests/s through the request loop
-(selas_amd/serving.py), p50/p99 latency from arrival to the image on the host, beside the plain
-schedule on the same stream. The top-level p50_ms/p99_ms and vs_plain_gpu come from these legs.
+at a time with its iterations packed (sgpu_render_vcm, the solo schedule) and one job at a time
+with one wavefront per iteration (the plain GPU schedule); "serving": a Poisson stream of such
+requests over 16 resident scenes at --serve-rate requests/s (more than the GPU serves, so requests
+queue) through the request loop (selas_amd/serving.py), p50/p99 latency from arrival to the image
+on the host, beside the plain schedule on the same stream. The top-level p50_ms/p99_ms,
+vs_plain_gpu and vs_solo_gpu come from these legs.
 
 --workload batch / serve: the batched leg alone (with --solo, also one job at a time with its
 iterations packed), and the serving leg swept over --rates.
+
+--workload pt: unidirectional path tracing of the closed room (24 + 20,480 triangles), 256 x 256,
+16 samples per pixel, path length 8; one step = the whole image.
 
 --workload traversal: the BVH traversal microbenchmark (2,097,152 triangles in 256 Gaussian
 clusters, 16,777,216 rays, half primary and half bounce rays); one step = one closest-hit pass plus
@@ -87,7 +94,18 @@ def cpu_baseline_traversal(t9, rays, sample_rays, threads):
     }
 
 
-def cpu_baseline_vcm(sc, width, iterations, threads):
+def cpu_threads():
+    """Threads for the CPU baselines: the host CPUs this process may run on, at most 16 (the CPU
+    share of one GPU on an 8-GPU host), with what the host reports beside it."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return max(1, min(16, avail)), {"host_cpus": os.cpu_count(), "affinity_cpus": avail,
+                                    "share_rule": "min(16, CPUs this process may run on): one GPU's share of the host"}
+
+
+def cpu_baseline_vcm(sc, width, iterations, threads, seed=1):
     import oracle as O
 
     log("cpu baseline: building the oracle SAH tree")
@@ -95,24 +113,41 @@ def cpu_baseline_vcm(sc, width, iterations, threads):
     log(f"cpu baseline: VCM {width}x{width}, {iterations} iterations on {threads} threads")
     t0 = time.perf_counter()
     fb, st = O.render_vcm(rs, sc["camera"], width, width, iterations, sc["max_path_length"], sc["merge_radius"],
-                          sc["radius_alpha"], 1, threads=threads)
+                          sc["radius_alpha"], seed, threads=threads)
     secs = time.perf_counter() - t0
     return {
         "value": round(width * width * iterations / secs, 1),
         "unit": "samples/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{iterations} VCM iterations at {width}x{width} on the same scene ({secs:.2f} s, "
-                  f"{st['rays'] / secs / 1e6:.2f} Mrays/s), oracle restatement (scalar C, oracle SAH tree, not Embree)",
+        "sample": f"the whole job ({iterations} VCM iterations at {width}x{width}, seed {seed}) on the same scene "
+                  f"({secs:.2f} s, {st['rays'] / secs / 1e6:.2f} Mrays/s), oracle restatement (scalar C, oracle "
+                  f"SAH tree, not Embree)",
         "image_mean": round(float(fb.mean()), 5),
-    }
+    }, fb
+
+
+# The VCM workload's job: 512 x 512, 32 iterations (one step of the vcm workload).
+C3_ITERATIONS = 32
+
+# Algorithmic bytes of the vertex-merging kernel (k_cam_merge): per camera vertex read 80 B (its
+# CamVertex record); per range search 4 B (its path's slice) + 8 cells x 8 B (two cell ends) + 16 B
+# (the contribution slot and its flag); per photon visited 16 B (its position); per photon within
+# the radius 80 B (its light-vertex record).
+MERGE_B_ENTRY, MERGE_B_QUERY, MERGE_B_VISIT, MERGE_B_NEAR = 80, 84, 16, 80
+
+
+def merge_bytes(st):
+    return (st["merge_entries"] * MERGE_B_ENTRY + st["merge_queries"] * MERGE_B_QUERY +
+            st["merge_visited"] * MERGE_B_VISIT + st["merge_near"] * MERGE_B_NEAR)
 
 
 def bench_vcm(args):
     from selas_amd import native as N
     from selas_amd import workloads as W
 
-    steps = 32 if args.steps is None else args.steps
+    steps = 5 if args.steps is None else args.steps
+    its = C3_ITERATIONS
     w = args.width
     log("building VCM workload scene")
     sc = W.vcm_scene()
@@ -127,70 +162,90 @@ def bench_vcm(args):
     cam = sc["camera"]
     fb = ctx.empty(3 * w * w * 4)
 
-    tuning = apply_tuning(args, ctx, lambda c: N.render_vcm(scene, cam, w, w, steps, L, r, alpha, 1, framebuffer=fb),
-                          workload=f"vcm_{w}x{w}_{steps}it_L{L}_{n_tris}tri")
-    # traversal counts over the closest-hit rays of one untimed iteration, on the tree in use
-    _, cnt = N.render_vcm(scene, cam, w, w, 1, L, r, alpha, 1, iteration_base=0, return_stats=True,
-                          flags=N.VCM_COUNT_TRAVERSAL, framebuffer=fb)
+    def job(flags=0, stats=False):
+        return N.render_vcm(scene, cam, w, w, its, L, r, alpha, 1, return_stats=stats, flags=flags, framebuffer=fb)
+
+    tuning = apply_tuning(args, ctx, lambda c: job(), workload=f"vcm_{w}x{w}_{its}it_L{L}_{n_tris}tri")
+    # work counters of the same job (untimed): traversal of every closest-hit ray and the vertex
+    # merging's range searches, on the tree in use
+    _, cnt = job(N.VCM_COUNT_TRAVERSAL | N.VCM_COUNT_MERGE, stats=True)
     node_bytes = 128 if ctx.tuning()["trace_tree"] == 1 else 64
     nodes_per_ray = cnt["nodes_visited"] / cnt["closest_rays"]
     tris_per_ray = cnt["tris_tested"] / cnt["closest_rays"]
     bytes_per_ray = 48 + nodes_per_ray * node_bytes + tris_per_ray * 48
+    m_bytes_job = merge_bytes(cnt)
     log(f"closest hit in VCM: {nodes_per_ray:.2f} nodes/ray ({node_bytes} B), {tris_per_ray:.2f} tris/ray, "
         f"{bytes_per_ray:.1f} B/ray")
-    # warm-up: the timed job itself, untimed (allocates the context's wavefront buffers at the
-    # size the timed render uses), then `warmup` more iterations
-    N.render_vcm(scene, cam, w, w, steps, L, r, alpha, 1, framebuffer=fb)
-    if args.warmup:
-        N.render_vcm(scene, cam, w, w, args.warmup, L, r, alpha, 1, framebuffer=fb)
+    log(f"merging: {cnt['merge_entries']} camera vertices, {cnt['merge_queries']} range searches, "
+        f"{cnt['merge_visited']} photons visited, {cnt['merge_near']} within the radius, "
+        f"{cnt['merge_launches']} launches, {m_bytes_job / 1e6:.1f} MB per job")
+    # warm-up: `warmup` whole jobs, untimed
+    for _ in range(args.warmup):
+        job()
     N.synchronize()
 
+    # timed: `steps` whole jobs (512 x 512, 32 iterations each), each rendered from scratch; events
+    # around every launch time the kernels on the render's stream
+    flags = 0 if args.no_phase_times else N.VCM_KERNEL_TIMES
+    tot = None
     e0, e1 = N.Event(), N.Event()
     t0 = time.perf_counter()
     e0.record()
-    _, st = N.render_vcm(scene, cam, w, w, steps, L, r, alpha, 1, return_stats=True,
-                         flags=0 if args.no_phase_times else N.VCM_KERNEL_TIMES, framebuffer=fb)
+    for _ in range(steps):
+        _, st = job(flags, stats=True)
+        tot = st if tot is None else {k: tot[k] + st[k] for k in tot}
     e1.record()
     N.synchronize()
     wall = time.perf_counter() - t0
     ms = e0.elapsed_ms(e1)
+    st = tot
     scene.check()
-    img = fb.download(np.float32, 3 * w * w)
-    # the same job timed as the product runs it: both passes replayed from captured graphs (no
-    # per-launch events, so not the run the roofline is measured on)
+    img = fb.download(np.float32, 3 * w * w).reshape(w, w, 3)
+    # the same job as captured graphs replay it (no per-launch events)
     replay = None
     if not args.no_phase_times and ctx.tuning()["vcm_graphs"]:
         ga, gb = N.Event(), N.Event()
         ga.record()
-        N.render_vcm(scene, cam, w, w, steps, L, r, alpha, 1, framebuffer=fb)
+        for _ in range(max(1, min(steps, 3))):
+            job()
         gb.record()
         N.synchronize()
-        g_ms = ga.elapsed_ms(gb)
-        replay = {"samples_per_s": round(w * w * steps / (g_ms / 1e3), 1), "ms": round(g_ms, 2),
+        g_ms = ga.elapsed_ms(gb) / max(1, min(steps, 3))
+        replay = {"samples_per_s": round(w * w * its / (g_ms / 1e3), 1), "ms_per_job": round(g_ms, 2),
                   "note": "the light and camera passes of each wavefront replayed as captured hipGraphs"}
     if args.no_phase_times:
-        # per-launch device times from one more (untimed) render of the same job with events
-        _, st_t = N.render_vcm(scene, cam, w, w, steps, L, r, alpha, 1, return_stats=True, flags=N.VCM_KERNEL_TIMES,
-                               framebuffer=fb)
-        for k in ("closest_ms", "shadow_ms", "light_ms", "camera_ms", "merge_ms", "grid_ms"):
-            st[k] = st_t[k]
+        # per-launch device times from one more (untimed) job with events
+        _, st_t = job(N.VCM_KERNEL_TIMES, stats=True)
+        for k in ("closest_ms", "shadow_ms", "light_ms", "camera_ms", "merge_ms", "grid_ms", "merge_launches"):
+            st[k] = st_t[k] * steps
     fb.free()
     secs = ms / 1e3
-    samples = w * w * steps
+    samples = w * w * its * steps
     value = samples / secs
     closest_ms = st["closest_ms"]
-    achieved = st["closest_rays"] * bytes_per_ray / (closest_ms / 1e3) / 1e9
+    achieved_closest = st["closest_rays"] * bytes_per_ray / (closest_ms / 1e3) / 1e9
+    merge_ms = st["merge_ms"]
+    m_launches = st["merge_launches"]
+    achieved_merge = m_bytes_job * steps / (merge_ms / 1e3) / 1e9
+    phases = {k: round(st[k] / steps, 2) for k in ("closest_ms", "shadow_ms", "light_ms", "camera_ms", "merge_ms",
+                                                     "grid_ms")}
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline_vcm(sc, w, args.cpu_iterations, min(16, os.cpu_count() or 1))
+        threads, host = cpu_threads()
+        cpu, cpu_img = cpu_baseline_vcm(sc, w, its, threads)
+        cpu.update(host)
+        cpu_img = cpu_img.reshape(w, w, 3)
+        cpu["same_job_as_gpu"] = True
+        cpu["pixels_within_1e-3"] = round(_frac_close(img, cpu_img), 5)
+        cpu["rmse_gpu_vs_cpu_rel"] = round(float(np.sqrt(((img - cpu_img) ** 2).mean()) / max(1e-12, cpu_img.mean())), 6)
     scene.release()
     del sc
     batch = serving = None
     if not args.no_batch:
-        batch = batch_summary(run_batch(N, W, ctx, args.batch_jobs, args.batch_seed, args.solo))
+        batch = batch_summary(run_batch(N, W, ctx, args.batch_jobs, args.batch_seed, not args.no_solo))
     if not args.no_serve:
-        serving = serving_summary(serve_sweep(N, W, ctx, [args.serve_rate], args.serve_jobs or 24,
+        serving = serving_summary(serve_sweep(N, W, ctx, [args.serve_rate], args.serve_jobs or 32,
                                               args.batch_seed, wavefront_paths=args.serve_paths)[0])
 
     out = {
@@ -207,11 +262,12 @@ def bench_vcm(args):
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"vcm_{w}x{w}_{steps}it_L{L}_1Mtri",
+            "workload": f"vcm_{w}x{w}_{its}it_L{L}_1Mtri",
+            "step": f"one whole job: {w}x{w}, {its} iterations, rendered from scratch",
             "triangles": int(n_tris),
             "width": w,
             "height": w,
-            "iterations": steps,
+            "iterations": its,
             "max_path_length": L,
             "merge_radius": round(float(r), 6),
             "radius_alpha": alpha,
@@ -220,23 +276,39 @@ def bench_vcm(args):
         },
         "mrays_s": round(st["rays"] / secs / 1e6, 2),
         "rays_per_sample": round(st["rays"] / samples, 3),
-        "closest_rays": st["closest_rays"],
-        "shadow_rays": st["shadow_rays"],
-        "light_vertices_per_iteration": round(st["light_vertices"] / steps, 1),
-        "phase_ms": {k: round(st[k], 2) for k in ("closest_ms", "shadow_ms", "light_ms", "camera_ms", "merge_ms",
-                                                  "grid_ms")},
+        "closest_rays_per_job": st["closest_rays"] // steps,
+        "shadow_rays_per_job": st["shadow_rays"] // steps,
+        "light_vertices_per_iteration": round(st["light_vertices"] / (its * steps), 1),
+        "phase_ms_per_job": phases,
         "bvh_build_ms": round(build_ms, 1),
         "wall_s_timed_region": round(wall, 3),
         "image_mean": round(float(img.mean()), 5),
         "roofline": {
             "bound": "hbm",
-            "achieved": round(achieved, 1),
+            "achieved": round(achieved_merge, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "frac": round(achieved_merge / HBM_PEAK_GBS, 4),
+            "traffic": load_traffic("traffic_cam_merge_vcm_512x512_32it_1Mtri.json"),
+            "kernel": "sgpu_vcm::k_cam_merge (vertex merging, all launches of the timed jobs)",
+            "launches": m_launches,
+            "avg_launch_ms": round(merge_ms / max(1, m_launches), 4),
+            "bytes_per_launch": round(m_bytes_job * steps / max(1, m_launches)),
+            "per_job": {"camera_vertices": cnt["merge_entries"], "range_searches": cnt["merge_queries"],
+                        "photons_visited": cnt["merge_visited"], "photons_near": cnt["merge_near"],
+                        "bytes": m_bytes_job},
+            "bytes_rule": f"{MERGE_B_ENTRY} B per camera vertex + {MERGE_B_QUERY} B per range search + "
+                          f"{MERGE_B_VISIT} B per photon visited + {MERGE_B_NEAR} B per photon within the radius",
+        },
+        "roofline_closest_hit": {
+            "bound": "hbm",
+            "achieved": round(achieved_closest, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved_closest / HBM_PEAK_GBS, 4),
             "traffic": load_traffic("traffic_bvh_intersect_vcm_1Mtri.json"),
-            "kernel": ("k_trace8<false,false,false> (closest hit, 8-wide nodes, all launches of the render)"
-                       if node_bytes == 128 else "k_trace<false,false,false> (closest hit, all launches of the render)"),
+            "kernel": ("k_trace8<false,false,false> (closest hit, 8-wide nodes, all launches of the timed jobs)"
+                       if node_bytes == 128 else "k_trace<false,false,false> (closest hit, all launches)"),
             "node_bytes": node_bytes,
             "launches": st["closest_launches"],
             "avg_launch_ms": round(closest_ms / max(1, st["closest_launches"]), 4),
@@ -255,6 +327,7 @@ def bench_vcm(args):
         out["p99_ms"] = serving["served"]["p99_ms"]
     if batch:
         out["vs_plain_gpu"] = batch["vs_plain_gpu"]
+        out["vs_solo_gpu"] = batch.get("vs_solo_gpu")
     print(json.dumps(out), flush=True)
 
 
@@ -395,6 +468,7 @@ def batch_summary(b):
                       "mrays_s": round(b["plain_mrays_s"], 2),
                       "schedule": "one job at a time, one wavefront per iteration"},
         "vs_plain_gpu": round(b["plain_ms"] / b["ms"], 3),
+        "vs_solo_gpu": None if b["solo_ms"] is None else round(b["solo_ms"] / b["ms"], 3),
         "solo_gpu": None if b["solo_ms"] is None else {
             "samples_per_s": round(b["samples"] / (b["solo_ms"] / 1e3), 1), "ms": round(b["solo_ms"], 1),
             "schedule": "one job at a time, its iterations packed into wavefronts (sgpu_render_vcm)"},
@@ -410,7 +484,8 @@ def bench_batch(args):
     b = run_batch(N, W, ctx, args.batch_jobs, args.batch_seed, args.solo, tune_args=args)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline_batch(b["mix"], args.cpu_iterations, min(16, os.cpu_count() or 1))
+        threads, host = cpu_threads()
+        cpu = dict(cpu_baseline_batch(b["mix"], args.cpu_iterations, threads), **host)
     summ = batch_summary(b)
     out = {
         "metric": METRIC,
@@ -571,9 +646,11 @@ def bench_serve(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=("vcm", "traversal", "batch", "serve", "instances", "deferred"), default="vcm")
+    ap.add_argument("--workload", choices=("vcm", "pt", "traversal", "batch", "serve", "instances", "deferred"),
+                    default="vcm")
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None, help="timed steps (vcm: 32 iterations, traversal: 10)")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (vcm: whole 32-iteration jobs, default 5; traversal: passes, default 10)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--width", type=int, default=512, help="vcm: image width and height")
     ap.add_argument("--cpu-iterations", type=int, default=32,
@@ -582,11 +659,13 @@ def main():
     ap.add_argument("--cpu-rays", type=int, default=4194304, help="rays in the CPU baseline sample")
     ap.add_argument("--batch-jobs", type=int, default=64, help="batch: jobs in the mix; serve: -")
     ap.add_argument("--batch-seed", type=int, default=7, help="batch/serve: seed of the job mix and scenes")
-    ap.add_argument("--solo", action="store_true", help="batch: also time the packed one-job-at-a-time schedule")
+    ap.add_argument("--solo", action="store_true", help="batch workload: also time the packed one-job-at-a-time schedule")
+    ap.add_argument("--no-solo", action="store_true", help="vcm: skip the packed one-job-at-a-time schedule")
     ap.add_argument("--rates", default="1,2,5,10,20,50,100,200", help="serve: Poisson arrival rates (jobs/s)")
-    ap.add_argument("--serve-rate", type=float, default=10.0, help="vcm: arrival rate of the serving leg")
+    ap.add_argument("--serve-rate", type=float, default=50.0,
+                    help="vcm: arrival rate of the serving leg (requests/s; 50 offers more than the GPU serves)")
     ap.add_argument("--serve-jobs", type=int, default=None,
-                    help="requests per rate (default: 24 in the serving leg; 8 s of arrivals, 16..64, in serve)")
+                    help="requests per rate (default: 32 in the serving leg; 8 s of arrivals, 16..64, in serve)")
     ap.add_argument("--no-plain", action="store_true", help="serve: skip the plain one-job-at-a-time runs")
     ap.add_argument("--serve-paths", type=int, default=1 << 22, help="serving: paths per wavefront of the server")
     ap.add_argument("--no-batch", action="store_true", help="vcm: skip the batched leg")
@@ -624,6 +703,8 @@ def main():
         sys.exit(2)
     if args.workload == "vcm":
         bench_vcm(args)
+    elif args.workload == "pt":
+        bench_pt(args)
     elif args.workload == "batch":
         bench_batch(args)
     elif args.workload == "serve":
@@ -638,6 +719,117 @@ def main():
         if args.steps is None:
             args.steps = 10
         bench_traversal(args)
+
+
+# Algorithmic bytes of the path tracer's shading kernel (k_pt_shade): per closest-hit ray read 60 B
+# (queue entry 4, ray origin and direction 24, hit geometry, distance and normal 20, throughput 12);
+# per hit 36 B more (random state read and written 16, path flags read and written 8, throughput
+# written 12); per shadow ray written 48 B (ray 32, path 4, contribution 12); per continuation ray
+# written 36 B (ray 32, queue entry 4); per miss 24 B (radiance read and written).
+PT_B_RAY, PT_B_HIT, PT_B_SHADOW, PT_B_CONT, PT_B_MISS = 60, 36, 48, 36, 24
+
+
+def bench_pt(args):
+    from selas_amd import native as N
+    from selas_amd import workloads as W
+
+    steps = 10 if args.steps is None else args.steps
+    w, spp, L, seed = 256, 16, 8, 1
+    sc = W.room_scene(1)
+    n_tris = sum(W.geom_tri9(g).shape[0] for g in sc["geoms"])
+    ctx = N.Context(0)
+    scene = N.build_render_scene(ctx, sc)
+    cam = sc["camera"]
+    fb = ctx.empty(2 * 3 * w * w * 4)
+
+    def image(flags=0, stats=True):
+        return N.render_pt(scene, cam, w, w, spp, L, seed, flags=flags, return_stats=stats, framebuffer=fb)
+
+    _, cnt = image(N.PT_COUNT_TRAVERSAL)
+    node_bytes = 128 if ctx.tuning()["trace_tree"] == 1 else 64
+    nodes_per_ray = cnt["nodes_visited"] / cnt["closest_rays"]
+    tris_per_ray = cnt["tris_tested"] / cnt["closest_rays"]
+    bytes_per_ray = 48 + nodes_per_ray * node_bytes + tris_per_ray * 48
+    cont = cnt["closest_rays"] - w * w * spp
+    shade_bytes = (cnt["closest_rays"] * PT_B_RAY + cnt["hits"] * PT_B_HIT + cnt["shadow_rays"] * PT_B_SHADOW +
+                   cont * PT_B_CONT + (cnt["closest_rays"] - cnt["hits"]) * PT_B_MISS)
+    for _ in range(args.warmup):
+        image(stats=False)
+    N.synchronize()
+    tot = None
+    e0, e1 = N.Event(), N.Event()
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(steps):
+        _, st = image()
+        tot = st if tot is None else {k: tot[k] + st[k] for k in tot}
+    e1.record()
+    N.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_ms(e1)
+    img = fb.download(np.float32, 2 * 3 * w * w).reshape(2, w, w, 3)
+    fb.free()
+    scene.check()
+    samples = w * w * spp * steps
+    value = samples / (ms / 1e3)
+    b = max(1, tot["bounces"])
+    shade = {"bound": "hbm", "achieved": round(shade_bytes * steps / (tot["shade_ms"] / 1e3) / 1e9, 1),
+             "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": load_traffic("traffic_pt_shade_room_256x256_16spp.json"),
+             "kernel": "k_pt_shade<false> (all launches of the timed images)", "launches": tot["bounces"],
+             "avg_launch_ms": round(tot["shade_ms"] / b, 4), "bytes_per_launch": round(shade_bytes * steps / b),
+             "bytes_rule": f"{PT_B_RAY} B per closest-hit ray + {PT_B_HIT} B per hit + {PT_B_SHADOW} B per shadow "
+                           f"ray + {PT_B_CONT} B per continuation ray + {PT_B_MISS} B per miss"}
+    closest = {"bound": "hbm", "achieved": round(tot["closest_rays"] * bytes_per_ray / (tot["closest_ms"] / 1e3) / 1e9, 1),
+               "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None,
+               "kernel": "k_trace8<false,false,false> (closest hit, all launches of the timed images)",
+               "launches": tot["bounces"], "avg_launch_ms": round(tot["closest_ms"] / b, 4),
+               "bytes_per_ray": round(bytes_per_ray, 1), "nodes_per_ray": round(nodes_per_ray, 3),
+               "tris_per_ray": round(tris_per_ray, 3)}
+    for r in (shade, closest):
+        r["frac"] = round(r["achieved"] / HBM_PEAK_GBS, 4)
+    dominant, other = (shade, closest) if tot["shade_ms"] >= tot["closest_ms"] else (closest, shade)
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        threads, host = cpu_threads()
+        rs = O.RenderScene(sc)
+        t1 = time.perf_counter()
+        cimg = O.render_pt(rs, cam, w, w, spp, L, seed, threads=threads)
+        secs = time.perf_counter() - t1
+        cpu = {"value": round(w * w * spp / secs, 1), "unit": "samples/s", "cores": threads, "kind": "port",
+               "sample": f"the whole image ({w}x{w} pixels x {spp} paths, seed {seed}; {secs:.2f} s), oracle "
+                         "restatement of EvaluatePath (scalar C, oracle SAH tree, not Embree)",
+               "pixels_within_1e-3": round(_frac_close(img, cimg), 5),
+               "rmse_gpu_vs_cpu_rel": round(float(np.sqrt(((img[1] - cimg[1]) ** 2).mean()) /
+                                                  max(1e-12, cimg[1].mean())), 6), **host}
+    scene.release()
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms / steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"pt_room_{w}x{w}_{spp}spp_L{L}", "step": "one whole image, rendered from scratch",
+                   "triangles": int(n_tris), "width": w, "height": w, "spp": spp, "max_path_length": L,
+                   "scene": "closed room of 12 quads + icosphere (20,480 tris), one quad light of radiance 15"},
+        "mrays_s": round((tot["closest_rays"] + tot["shadow_rays"]) / (ms / 1e3) / 1e6, 2),
+        "closest_rays_per_image": tot["closest_rays"] // steps,
+        "shadow_rays_per_image": tot["shadow_rays"] // steps,
+        "phase_ms_per_image": {k: round(tot[k] / steps, 3) for k in ("closest_ms", "shadow_ms", "shade_ms", "other_ms")},
+        "wall_s_timed_region": round(wall, 3),
+        "image_mean": round(float(img[1].mean()), 5),
+        "roofline": dominant,
+        "roofline_other": other,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
 
 
 def bench_traversal(args):
@@ -707,7 +899,8 @@ def bench_traversal(args):
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline_traversal(t9, rays, args.cpu_rays, min(16, os.cpu_count() or 1))
+        threads, host = cpu_threads()
+        cpu = dict(cpu_baseline_traversal(t9, rays, args.cpu_rays, threads), **host)
 
     out = {
         "metric": METRIC,
@@ -912,7 +1105,7 @@ def bench_deferred(args):
     cpu = None
     if not args.no_cpu:
         import oracle as O
-        threads = min(16, os.cpu_count() or 1)
+        threads, host = cpu_threads()
         log("cpu baseline: building the oracle SAH tree")
         rs = O.RenderScene(sc)
         t0 = time.perf_counter()
@@ -922,7 +1115,7 @@ def bench_deferred(args):
                "sample": f"the whole image ({w}x{w} pixels x {sx}x{sy} samples, same seed; {secs:.2f} s), oracle "
                          "restatement (scalar C, oracle SAH tree, not Embree)",
                "image_mean": round(float(img.mean()), 5),
-               "mean_rel_diff_vs_gpu": round(float(abs(img.mean() / imgs["unsorted"].mean() - 1)), 6)}
+               "mean_rel_diff_vs_gpu": round(float(abs(img.mean() / imgs["unsorted"].mean() - 1)), 6), **host}
     scene.release()
     out = {
         "metric": "deferred path tracer samples/s at 1M tris",

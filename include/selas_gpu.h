@@ -316,6 +316,27 @@ typedef struct sgpu_pt_job {
  * averaged over spp. Synchronous on `stream`. */
 int sgpu_render_pt(sgpu_scene scene, const sgpu_pt_job* job, float* framebuffer, void* stream);
 
+#define SGPU_PT_COUNT_TRAVERSAL 2u /* sgpu_render_pt_stats: count nodes and triangles of closest-hit rays */
+
+/* Work and device time of one sgpu_render_pt_stats call (no counterpart in Selas). */
+typedef struct sgpu_pt_stats {
+    uint64_t closest_rays;   /* closest-hit rays: camera rays and continuations */
+    uint64_t shadow_rays;    /* occlusion rays (NEE) */
+    uint64_t hits;           /* closest-hit rays that hit a surface (the rest are misses) */
+    uint64_t bounces;        /* closest-hit launches (= shading launches) */
+    uint64_t nodes_visited;  /* SGPU_PT_COUNT_TRAVERSAL: over all closest-hit rays */
+    uint64_t tris_tested;    /* SGPU_PT_COUNT_TRAVERSAL: over all closest-hit rays */
+    float closest_ms;        /* device time of the closest-hit traversals (events around each launch) */
+    float shadow_ms;         /* ... of the occlusion traversals */
+    float shade_ms;          /* ... of the shading kernels (k_pt_shade) */
+    float other_ms;          /* ... of path generation, shadow resolution and accumulation */
+} sgpu_pt_stats;
+
+/* sgpu_render_pt with per-phase device times (HIP events recorded around every launch on `stream`)
+ * and ray counts; job->flags may add SGPU_PT_COUNT_TRAVERSAL. Same image as sgpu_render_pt. */
+int sgpu_render_pt_stats(sgpu_scene scene, const sgpu_pt_job* job, float* framebuffer, sgpu_pt_stats* stats,
+                         void* stream);
+
 #define SGPU_DPT_SORT_HITS 1u /* shade each bounce's hits in (instID[0], instID[1], geomID, primID) order */
 #define SGPU_DPT_SORT_RAYS 2u /* trace each bounce's rays binned by direction, sorted along it (GetSortedBatch) */
 
@@ -346,6 +367,8 @@ int sgpu_render_deferred_pt(sgpu_scene scene, const sgpu_dpt_job* job, float* fr
 #define SGPU_VCM_SHORTEST_FIRST 8u /* servers only: fill each wavefront with the admitted jobs of least
                                       remaining work (paths x iterations left) first, ties in admission
                                       order, instead of in admission order */
+#define SGPU_VCM_COUNT_MERGE 16u /* count the vertex-merging work: camera vertices read, range searches,
+                                    photons visited and photons within the radius (slower) */
 
 typedef struct sgpu_vcm_job {
     sgpu_camera camera;
@@ -374,6 +397,11 @@ typedef struct sgpu_vcm_stats {
     float camera_ms;           /* ... of the camera-subpath kernels (NEE, connections, continuation) */
     float grid_ms;             /* ... of the vertex compaction and photon grid build */
     float merge_ms;            /* ... of the vertex merging (photon range searches) */
+    uint64_t merge_launches;   /* vertex-merging launches (camera bounces with rays) */
+    uint64_t merge_entries;    /* SGPU_VCM_COUNT_MERGE: camera vertices read by the merging kernel */
+    uint64_t merge_queries;    /* ... of them range searches (a merging vertex inside its slice's photon box) */
+    uint64_t merge_visited;    /* ... photons whose positions were tested (the 2x2x2 cells' ranges) */
+    uint64_t merge_near;       /* ... photons within the radius (their vertex records read) */
 } sgpu_vcm_stats;
 
 /* Writes each layer of a device framebuffer (layers x height x width x 3 floats, the layout the

@@ -30,6 +30,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
+#include <cstring>
 
 namespace {
 
@@ -259,7 +260,7 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(ShadeParams sp, uint32_t co
     const uint32_t jj = blockIdx.x * kBlock + threadIdx.x;
     bool active = jj < count;
     const uint32_t j = active && order ? order[jj] : jj;
-    bool cont = false, emit = false, emit_sky = false;
+    bool cont = false, emit = false, emit_sky = false, hit = false;
     uint32_t p = 0;
     f3 next_o = mk3(0, 0, 0), next_d = mk3(0, 0, 0), sh_o = mk3(0, 0, 0), sh_d = mk3(0, 0, 0), c = mk3(0, 0, 0);
     f3 sk_o = mk3(0, 0, 0), sk_d = mk3(0, 0, 0), sk_c = mk3(0, 0, 0);
@@ -296,6 +297,7 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(ShadeParams sp, uint32_t co
                 add3(ps.ld1, p, bg * thr);
             }
         } else {
+            hit = true;
             PcgUniforms u{load_rng(ps, sp.path_base + p, p)};
             const float* mat = sp.materials + 22 * (size_t)sp.geom_material[geom];
             Surface s;
@@ -425,6 +427,7 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(ShadeParams sp, uint32_t co
         }
     }
     uint32_t kn = wave_append(cont, &counters[0]);
+    (void)wave_append(hit, &counters[3]);
     if (cont) {
         put_ray(next_rays, kn, next_o, next_d, 0.0f, kFloatMax);
         next_queue[kn] = p;
@@ -518,7 +521,11 @@ struct PathJob {
     bool sort_rays;
 };
 
-int render_paths(sgpu_scene scene, const PathJob& job, float* fb, hipStream_t stream)
+enum { kPtClosest, kPtShadow, kPtShade, kPtOther, kPtPhases };
+
+// stats (may be NULL): ray counts and per-phase device times of the render (events around every
+// launch), traversal counters with SGPU_PT_COUNT_TRAVERSAL.
+int render_paths(sgpu_scene scene, const PathJob& job, float* fb, hipStream_t stream, sgpu_pt_stats* stats = nullptr)
 {
     const uint32_t W = job.camera.width, H = job.camera.height, spp = job.spp_x * job.spp_y;
     const uint64_t n_pixels = (uint64_t)W * H;
@@ -569,6 +576,10 @@ int render_paths(sgpu_scene scene, const PathJob& job, float* fb, hipStream_t st
     ShadowBuf sky = job.dpt ? shadow_buf() : ShadowBuf{};
     uint32_t* spill = A.get<uint32_t>(sgpu_trace_spill_words(scene));
     uint32_t* counters = A.get<uint32_t>(4);
+    const bool count_trav = stats && (job.flags & SGPU_PT_COUNT_TRAVERSAL);
+    uint64_t* d_trav = count_trav ? A.get<uint64_t>(2) : nullptr;
+    sgpu_phase_timer T(stats != nullptr, stream);
+    uint64_t n_closest = 0, n_shadow = 0, n_hits = 0, n_bounces = 0;
 
     ShadeParams sp;
     sp.materials = scene->d_materials;
@@ -635,15 +646,21 @@ int render_paths(sgpu_scene scene, const PathJob& job, float* fb, hipStream_t st
     } hf{h_counters};
 
     SGPU_HIP_TRY(hipMemsetAsync(fb, 0, layers * 3 * n_pixels * sizeof(float), stream));
+    if (d_trav) SGPU_HIP_TRY(hipMemsetAsync(d_trav, 0, 2 * sizeof(uint64_t), stream));
     sgpu_hits_soa hsoa{hb.t, hb.u, hb.v, hb.prim, hb.geom, hb.ngx, hb.ngy, hb.ngz};
 
     auto resolve = [&](ShadowBuf& b, uint32_t n) -> int {
         if (n == 0) return SGPU_OK;
+        n_shadow += n;
         sgpu_rays_soa ss = as_soa(b.ray);
+        SGPU_HIP_TRY(T.begin(kPtShadow));
         int rc = sgpu_trace_launch(scene, &ss, nullptr, b.occluded, n, true, nullptr, stream, spill);
         if (rc) return rc;
+        SGPU_HIP_TRY(T.end());
+        SGPU_HIP_TRY(T.begin(kPtOther));
         hipLaunchKernelGGL(k_pt_resolve, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, n, b, ps);
         SGPU_HIP_TRY(hipGetLastError());
+        SGPU_HIP_TRY(T.end());
         return SGPU_OK;
     };
 
@@ -651,6 +668,7 @@ int render_paths(sgpu_scene scene, const PathJob& job, float* fb, hipStream_t st
         const uint32_t npix = (uint32_t)std::min<uint64_t>(chunk_pixels, n_pixels - pix0);
         const uint32_t n_paths = npix * spp;
         const uint64_t path_base = pix0 * spp;
+        SGPU_HIP_TRY(T.begin(kPtOther));
         if (job.dpt)
             hipLaunchKernelGGL(k_dpt_generate, dim3((n_paths + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, cam,
                                job.spp_x, job.spp_y, job.seed, path_base, n_paths, ps, rays[0], queue[0]);
@@ -658,14 +676,19 @@ int render_paths(sgpu_scene scene, const PathJob& job, float* fb, hipStream_t st
             hipLaunchKernelGGL(k_pt_generate, dim3((n_paths + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, cam, spp,
                                job.seed, path_base, n_paths, ps, rays[0], queue[0]);
         SGPU_HIP_TRY(hipGetLastError());
+        SGPU_HIP_TRY(T.end());
         uint32_t count = n_paths;
         int cur = 0;
         sp.path_base = path_base;
         for (uint32_t bounce = 0; bounce < job.max_path_length && count > 0; ++bounce) {
             sgpu_rays_soa rs = as_soa(rays[cur]);
-            int rc = sgpu_trace_launch(scene, &rs, &hsoa, nullptr, count, false, nullptr, stream, spill, nullptr, nullptr,
+            n_closest += count;
+            ++n_bounces;
+            SGPU_HIP_TRY(T.begin(kPtClosest));
+            int rc = sgpu_trace_launch(scene, &rs, &hsoa, nullptr, count, false, d_trav, stream, spill, nullptr, nullptr,
                                        hb.inst);
             if (rc) return rc;
+            SGPU_HIP_TRY(T.end());
             const uint32_t* order = nullptr;
             if (job.sort_hits) {
                 hipLaunchKernelGGL(k_dpt_keys, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, count,
@@ -679,6 +702,7 @@ int render_paths(sgpu_scene scene, const PathJob& job, float* fb, hipStream_t st
             }
             SGPU_HIP_TRY(hipMemsetAsync(counters, 0, 4 * sizeof(uint32_t), stream));
             sp.bounce = bounce;
+            SGPU_HIP_TRY(T.begin(kPtShade));
             if (job.dpt)
                 hipLaunchKernelGGL(k_pt_shade<true>, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, sp,
                                    count, queue[cur], order, rays[cur], hb, ps, rays[1 - cur], queue[1 - cur], sb, sky,
@@ -688,9 +712,11 @@ int render_paths(sgpu_scene scene, const PathJob& job, float* fb, hipStream_t st
                                    count, queue[cur], order, rays[cur], hb, ps, rays[1 - cur], queue[1 - cur], sb, sky,
                                    counters);
             SGPU_HIP_TRY(hipGetLastError());
+            SGPU_HIP_TRY(T.end());
             SGPU_HIP_TRY(hipMemcpyAsync(h_counters, counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
             SGPU_HIP_TRY(hipStreamSynchronize(stream));
             const uint32_t n_next = h_counters[0];
+            n_hits += h_counters[3];
             // per path: the light sample's contribution, then the sky sample's
             if ((rc = resolve(sb, h_counters[1]))) return rc;
             if (job.dpt && (rc = resolve(sky, h_counters[2]))) return rc;
@@ -712,24 +738,44 @@ int render_paths(sgpu_scene scene, const PathJob& job, float* fb, hipStream_t st
             count = n_next;
             cur = 1 - cur;
         }
+        SGPU_HIP_TRY(T.begin(kPtOther));
         hipLaunchKernelGGL(k_pt_accumulate, dim3((npix + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, spp,
                            (uint32_t)pix0, npix, (uint32_t)n_pixels, job.dpt ? 1u : 0u, ps, fb);
         SGPU_HIP_TRY(hipGetLastError());
+        SGPU_HIP_TRY(T.end());
     }
     const size_t nf = layers * 3 * n_pixels;
     hipLaunchKernelGGL(k_scale, dim3((uint32_t)((nf + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream, fb, nf,
                        1.0f / (float)spp);
     SGPU_HIP_TRY(hipGetLastError());
     SGPU_HIP_TRY(hipStreamSynchronize(stream));
+    if (stats) {
+        SGPU_HIP_TRY(T.collect());
+        std::memset(stats, 0, sizeof(*stats));
+        stats->closest_rays = n_closest;
+        stats->shadow_rays = n_shadow;
+        stats->hits = n_hits;
+        stats->bounces = n_bounces;
+        if (d_trav) {
+            uint64_t tr[2] = {0, 0};
+            SGPU_HIP_TRY(hipMemcpy(tr, d_trav, sizeof(tr), hipMemcpyDeviceToHost));
+            stats->nodes_visited = tr[0];
+            stats->tris_tested = tr[1];
+        }
+        stats->closest_ms = (float)T.ms[kPtClosest];
+        stats->shadow_ms = (float)T.ms[kPtShadow];
+        stats->shade_ms = (float)T.ms[kPtShade];
+        stats->other_ms = (float)T.ms[kPtOther];
+    }
     return SGPU_OK;
 }
 
 }  // namespace
 
-int sgpu_pt_render_impl(sgpu_scene scene, const sgpu_pt_job* job, float* fb, hipStream_t stream)
+int sgpu_pt_render_impl(sgpu_scene scene, const sgpu_pt_job* job, float* fb, hipStream_t stream, sgpu_pt_stats* stats)
 {
     PathJob pj{job->camera, job->spp, 1u, job->max_path_length, job->seed, job->flags, false, false, false};
-    return render_paths(scene, pj, fb, stream);
+    return render_paths(scene, pj, fb, stream, stats);
 }
 
 int sgpu_dpt_render_impl(sgpu_scene scene, const sgpu_dpt_job* job, float* fb, hipStream_t stream)

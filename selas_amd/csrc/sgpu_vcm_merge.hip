@@ -20,7 +20,8 @@ namespace {
 // gives the query's sum.
 __global__ __launch_bounds__(kBlock) void k_cam_merge(Wave w, const uint32_t* __restrict count_ptr,
                                                       const CamVertex* __restrict cam_vm,
-                                                      const uint32_t* __restrict path_slice, GridView G, CamSlots S)
+                                                      const uint32_t* __restrict path_slice, GridView G, CamSlots S,
+                                                      unsigned long long* __restrict mstat)
 {
     const uint32_t count = *count_ptr;
     __shared__ uint32_t list_lds[kMergeWaves][2][64];
@@ -111,6 +112,7 @@ __global__ __launch_bounds__(kBlock) void k_cam_merge(Wave w, const uint32_t* __
             acc += (f * mk3(lv.thr[0], lv.thr[1], lv.thr[2])) * (mis / cos_l);
         };
         uint32_t listed = 0;  // same in every lane of the half
+        uint32_t n_near = 0;  // photons within the radius (counted only for mstat)
         for (uint32_t c0 = 0; __ballot(c0 < total) != 0; c0 += 32) {
             const uint32_t i = c0 + hl;
             // every lane takes part in the shuffles (lanes past the end look at their last photon)
@@ -127,6 +129,7 @@ __global__ __launch_bounds__(kBlock) void k_cam_merge(Wave w, const uint32_t* __
             const unsigned long long m = __ballot(near) & half_bits;
             if (near) list[listed + (uint32_t)__popcll(m & below)] = q;
             listed += (uint32_t)__popcll(m);
+            n_near += (uint32_t)__popcll(m);
             __builtin_amdgcn_wave_barrier();
             if (__ballot(listed >= 32) != 0) {
                 // both halves evaluate up to 32 listed photons and keep the rest
@@ -146,9 +149,17 @@ __global__ __launch_bounds__(kBlock) void k_cam_merge(Wave w, const uint32_t* __
             acc.y += __shfl_xor(acc.y, off, 32);
             acc.z += __shfl_xor(acc.z, off, 32);
         }
+        if (mstat && hl == 0 && j < count) {
+            atomicAdd(mstat + 0, 1ull);
+            if (active) {
+                atomicAdd(mstat + 1, 1ull);
+                atomicAdd(mstat + 2, (unsigned long long)total);
+                atomicAdd(mstat + 3, (unsigned long long)n_near);
+            }
+        }
         if (active && hl == 0) {
             const uint32_t p = v.path;
-            s3(S.c, p * w.nslots + w.nslots - 1, (mk3(v.thr[0], v.thr[1], v.thr[2]) * acc) * sl.vm_norm);
+            s3(S.c, (size_t)p * w.nslots + w.nslots - 1, (mk3(v.thr[0], v.thr[1], v.thr[2]) * acc) * sl.vm_norm);
             S.valid[(size_t)p * w.nslots + w.nslots - 1] = 1;
         }
     }
@@ -157,9 +168,11 @@ __global__ __launch_bounds__(kBlock) void k_cam_merge(Wave w, const uint32_t* __
 }  // namespace
 
 void launch_cam_merge(uint32_t blocks, hipStream_t stream, const Wave& w, const uint32_t* count_ptr,
-                      const CamVertex* cam_vm, const uint32_t* path_slice, const GridView& G, const CamSlots& S)
+                      const CamVertex* cam_vm, const uint32_t* path_slice, const GridView& G, const CamSlots& S,
+                      unsigned long long* mstat)
 {
-    hipLaunchKernelGGL(k_cam_merge, dim3(blocks), dim3(kBlock), 0, stream, w, count_ptr, cam_vm, path_slice, G, S);
+    hipLaunchKernelGGL(k_cam_merge, dim3(blocks), dim3(kBlock), 0, stream, w, count_ptr, cam_vm, path_slice, G, S,
+                       mstat);
 }
 
 }  // namespace sgpu_vcm

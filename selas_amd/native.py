@@ -88,6 +88,7 @@ _SIGNATURES = {
     "sgpu_scene_set_lights": [_P, _P, _U32, _FP],
     "sgpu_scene_set_ibl": [_P, _U32, _U32, _FP, _F, _F],
     "sgpu_render_pt": [_P, _P, _P, _P],
+    "sgpu_render_pt_stats": [_P, _P, _P, _P, _P],
     "sgpu_render_deferred_pt": [_P, _P, _P, _P],
     "sgpu_framebuffer_save_hdr": [C.c_char_p, _P, _U32, _U32, _U32, _P],
     "sgpu_render_vcm": [_P, _P, _P, _P, _P],
@@ -509,6 +510,7 @@ VCM_KERNEL_TIMES = 1
 VCM_COUNT_TRAVERSAL = 2
 VCM_ONE_SLICE_PER_WAVEFRONT = 4
 VCM_SHORTEST_FIRST = 8  # servers only
+VCM_COUNT_MERGE = 16
 
 
 class VcmJob(C.Structure):
@@ -522,7 +524,9 @@ class VcmStats(C.Structure):
                                           "light_vertices", "samples", "wavefronts", "nodes_visited",
                                           "tris_tested")] + \
                [(k, C.c_float) for k in ("closest_ms", "shadow_ms", "light_ms", "camera_ms", "grid_ms",
-                                         "merge_ms")]
+                                         "merge_ms")] + \
+               [(k, C.c_uint64) for k in ("merge_launches", "merge_entries", "merge_queries", "merge_visited",
+                                          "merge_near")]
 
 
 def render_vcm(scene: Scene, cam: dict, width, height, iterations, max_path_length, base_radius, radius_alpha, seed,
@@ -670,11 +674,34 @@ def render_deferred_pt(scene: Scene, cam: dict, width, height, spp_x, spp_y, max
             fb.free()
 
 
-def render_pt(scene: Scene, cam: dict, width, height, spp, max_path_length, seed, flags=0, stream=None):
+class PtStats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("closest_rays", "shadow_rays", "hits", "bounces", "nodes_visited",
+                                          "tris_tested")] + \
+               [(k, C.c_float) for k in ("closest_ms", "shadow_ms", "shade_ms", "other_ms")]
+
+
+PT_COUNT_TRAVERSAL = 2
+
+
+def render_pt(scene: Scene, cam: dict, width, height, spp, max_path_length, seed, flags=0, stream=None,
+              return_stats=False, framebuffer=None):
+    """Path-traced layers (2 x height x width x 3) through sgpu_render_pt, or with return_stats through
+    sgpu_render_pt_stats, returning (layers, stats dict). With `framebuffer` (a device buffer of
+    24 x width x height bytes) the layers stay on the device and None is returned in their place."""
     ctx = scene.ctx
-    fb = ctx.empty(2 * 3 * width * height * 4)
+    fb = framebuffer if framebuffer is not None else ctx.empty(2 * 3 * width * height * 4)
     job = PtJob(Camera.make(cam, width, height), spp, max_path_length, seed, flags)
-    check(lib.sgpu_render_pt(scene.handle, C.byref(job), fb.ptr, stream))
-    out = fb.download(np.float32, 2 * 3 * width * height).reshape(2, height, width, 3)
-    fb.free()
+    st = PtStats()
+    try:
+        if return_stats:
+            check(lib.sgpu_render_pt_stats(scene.handle, C.byref(job), fb.ptr, C.byref(st), stream))
+        else:
+            check(lib.sgpu_render_pt(scene.handle, C.byref(job), fb.ptr, stream))
+        out = None if framebuffer is not None else \
+            fb.download(np.float32, 2 * 3 * width * height).reshape(2, height, width, 3)
+    finally:
+        if framebuffer is None:
+            fb.free()
+    if return_stats:
+        return out, {k: getattr(st, k) for k, _ in PtStats._fields_}
     return out
