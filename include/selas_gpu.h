@@ -171,6 +171,19 @@ int sgpu_scene_add_triangles(sgpu_scene scene, const float* positions, uint32_t 
  * triangle's barycentrics (u', v') become (1 - u', 1 - v')), the coordinates Ptex faces use. */
 int sgpu_scene_add_quads(sgpu_scene scene, const float* positions, uint32_t n_verts, const uint32_t* indices,
                          uint32_t n_quads, const float* colmajor4x4, uint32_t* geom_id_out);
+/* Per-vertex shading attributes of geometry geom_id (host arrays, copied; before commit): replaces
+ * rtcSetGeometryVertexAttributeCount + rtcSetSharedGeometryBuffer(RTC_BUFFER_TYPE_VERTEX_ATTRIBUTE)
+ * with slot 0 = normals (FLOAT3), slot 1 = tangents (FLOAT4: xyz and the bitangent sign w), slot 2 =
+ * UVs (FLOAT2) (ModelResource.cpp:164-200), which CalculateSurfaceParams interpolates with
+ * rtcInterpolate0 at the hit (SurfaceParameters.cpp:155-190): the renderers then shade with the
+ * normalised interpolated normal and the frame (t, n, cross(n, t) * w) of the interpolated tangent.
+ * n_verts must equal the vertex count the geometry was added with; the geometry's transform
+ * (colmajor4x4) is applied to the normals and the tangents' xyz. Tangents need normals, UVs need
+ * normals and tangents (as ModelResource.cpp asserts); normals == NULL removes the attributes. UVs
+ * are kept for Selas's 2D texture lookups, which this library does not implement (its base colour
+ * textures are per-face Ptex, sgpu_scene_set_ptex): they do not change the image. */
+int sgpu_scene_set_vertex_attributes(sgpu_scene scene, uint32_t geom_id, const float* normals, const float* tangents,
+                                     const float* uvs, uint32_t n_verts);
 /* Instance of a committed scene `child`, placed by a column-major 4x4 local-to-world transform
  * (an affine one: the last row is ignored): replaces rtcNewGeometry(RTC_GEOMETRY_TYPE_INSTANCE) +
  * rtcSetGeometryInstancedScene + rtcSetGeometryTransform(RTC_FORMAT_FLOAT4X4_COLUMN_MAJOR)
@@ -184,7 +197,9 @@ int sgpu_scene_add_quads(sgpu_scene scene, const float* positions, uint32_t n_ve
  * ray is transformed into the geometry's space, where the triangle test runs, and t is the world
  * ray's parameter. The child must stay alive while an instance refers to it (its release fails
  * with SGPU_E_STATE). Instanced scenes answer queries (sgpu_intersect, sgpu_occluded,
- * sgpu_intersect_stats); the renderers take scenes of geometries. */
+ * sgpu_intersect_stats) and are rendered by every renderer: at commit the placed scenes' materials,
+ * textures and vertex attributes are gathered (set them before committing the scene that places
+ * them). */
 int sgpu_scene_add_instance(sgpu_scene scene, sgpu_scene child, const float* colmajor4x4, uint32_t* inst_id_out);
 int sgpu_scene_commit(sgpu_scene scene, void* stream);
 int sgpu_scene_release(sgpu_scene scene);

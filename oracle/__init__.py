@@ -234,7 +234,56 @@ class _Scene(C.Structure):
     _fields_ = [("bvh", C.c_void_p), ("tri9", C.c_void_p), ("tri_geom", C.c_void_p), ("materials", C.c_void_p),
                 ("geom_material", C.c_void_p), ("n_geoms", C.c_uint32), ("lights", C.c_void_p),
                 ("n_lights", C.c_uint32), ("background", C.c_float * 3), ("geom_first", C.c_void_p),
-                ("geom_tex", C.c_void_p), ("ibl", C.c_void_p), ("inst", C.c_void_p)]
+                ("geom_tex", C.c_void_p), ("ibl", C.c_void_p), ("inst", C.c_void_p), ("geom_attr", C.c_void_p)]
+
+
+class _Attr(C.Structure):
+    _fields_ = [("n", C.c_void_p), ("t", C.c_void_p), ("uv", C.c_void_p), ("corners", C.c_uint32)]
+
+
+lib.orc_interp_attr.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_void_p]
+
+
+def geom_corner_attrs(g):
+    """Corner arrays (primitive order, the primitive's own vertex order: 3 per triangle, 4 per quad)
+    of a geometry dict's per-vertex "normals", "tangents" and "uvs" (None where absent), and the
+    corner count; (None, None, None, 0) for a geometry without attributes."""
+    if not isinstance(g, dict) or g.get("normals") is None:
+        return None, None, None, 0
+    idx = np.asarray(g["triangles"] if "triangles" in g else g["quads"], np.int64)
+    out = []
+    for key, k in (("normals", 3), ("tangents", 4), ("uvs", 2)):
+        a = g.get(key)
+        out.append(None if a is None else np.ascontiguousarray(np.asarray(a, np.float32).reshape(-1, k)[idx]
+                                                               .reshape(-1), np.float32))
+    return out[0], out[1], out[2], idx.shape[1]
+
+
+def interp_attr(corner_values, K, corners, prim, u, v):
+    """The oracle's rtcInterpolate0 of a K-float corner array at primitive prim, (u, v)."""
+    a = _f(corner_values)
+    out = np.zeros(K, np.float32)
+    lib.orc_interp_attr(a.ctypes.data, K, corners, prim, float(u), float(v), out.ctypes.data)
+    return out
+
+
+class _AttrTable:
+    """Per-geometry attribute records (a list of geometry dicts or arrays, in the scene's list
+    order) with the corner arrays they point to; `ptr` is NULL when no geometry has attributes."""
+
+    def __init__(self, geoms):
+        self.keep = []
+        self.table = (_Attr * max(len(geoms), 1))()
+        any_attr = False
+        for i, g in enumerate(geoms):
+            n, t, uv, corners = geom_corner_attrs(g)
+            if n is None:
+                continue
+            any_attr = True
+            self.keep += [a for a in (n, t, uv) if a is not None]
+            self.table[i] = _Attr(n.ctypes.data, None if t is None else t.ctypes.data,
+                                  None if uv is None else uv.ctypes.data, corners)
+        self.ptr = C.cast(self.table, C.c_void_p) if any_attr else None
 
 
 # ---- image-based light (orc_ibl.c) ---------------------------------------------------------------
@@ -347,6 +396,7 @@ class RenderScene:
                 self.textures.append(t)
                 self.geom_tex[i] = t.s
         self.bvh = CpuBvh(self.tri9)
+        self.attrs = _AttrTable(sc["geoms"])
         ibl = sc.get("ibl")
         self.ibl = None if ibl is None else Ibl(ibl["rgb"], ibl.get("rotation", 0.0), ibl.get("exposure", 1.0),
                                                 tri9=self.tri9)
@@ -355,7 +405,8 @@ class RenderScene:
                         self.lights.ctypes.data if self.lights.size else None, self.lights.shape[0],
                         (C.c_float * 3)(*[float(x) for x in sc["background"]]), self.geom_first.ctypes.data,
                         C.cast(self.geom_tex, C.c_void_p),
-                        C.cast(C.pointer(self.ibl.s), C.c_void_p) if self.ibl is not None else None)
+                        C.cast(C.pointer(self.ibl.s), C.c_void_p) if self.ibl is not None else None, None,
+                        self.attrs.ptr)
 
 
 def render_pt(rs: RenderScene, cam: dict, width, height, spp, max_path_length, seed, flags=0, threads=0):
@@ -629,6 +680,7 @@ class InstancedRenderScene:
         self.materials = np.ascontiguousarray(np.concatenate(mats) if mats else np.zeros((1, 22)), np.float32)
         gm = [self.models[mi]["geom_material"][:self.models[mi]["n_geoms"]] + m0[mi] for mi in order]
         self.geom_material = np.ascontiguousarray(np.concatenate(gm) if gm else np.zeros(1), np.uint32)
+        self.attrs = _AttrTable([g for mi in order for g in self.models[mi]["geoms"]])
         self.textures = []
         self.geom_tex = (_Ptex * max(ng, 1))()
         for mi in order:
@@ -686,4 +738,4 @@ class InstancedRenderScene:
                         (C.c_float * 3)(*[float(x) for x in sc["background"]]), None,
                         C.cast(self.geom_tex, C.c_void_p),
                         C.cast(C.pointer(self.ibl.s), C.c_void_p) if self.ibl is not None else None,
-                        C.cast(C.pointer(self.iscene), C.c_void_p))
+                        C.cast(C.pointer(self.iscene), C.c_void_p), self.attrs.ptr)

@@ -171,6 +171,20 @@ typedef struct {
     const float* l2w;
 } orc_iscene;
 
+/* Vertex attributes of one geometry, as the device keeps them: per primitive its corners' values
+ * in the primitive's own vertex order (3 corners per triangle, 4 per quad), normals 3, tangents 4
+ * (xyz, w), UVs 2 floats per corner; NULL where the geometry has none. */
+typedef struct {
+    const float* n;
+    const float* t;
+    const float* uv;
+    uint32_t corners;
+} orc_attr;
+
+/* rtcInterpolate0 of a K-float attribute at primitive prim, (u, v) (a quad's own coordinates for
+ * quads), as Embree 3 evaluates it (see orc_pt.c). */
+void orc_interp_attr(const float* a, int K, uint32_t corners, uint32_t prim, float u, float v, float* out);
+
 typedef struct {
     const orc_bvh* bvh;
     const float* tri9;          /* world-space triangles, 9 floats each (global index order) */
@@ -188,13 +202,15 @@ typedef struct {
     const orc_iscene* inst;     /* instanced scene, NULL for none: then bvh, tri9, tri_geom and
                                    geom_first are unused and the shading tables above are indexed
                                    by the scene's list of geometries */
+    const orc_attr* geom_attr;  /* vertex attributes per geometry (scene list), NULL for none */
 } orc_scene;
 
 /* Closest hit in a scene of geometries or an instanced scene: 0 on a miss; else t, u, v, the
  * hit's geometry (scene list), its unnormalised world-space geometric normal and the index of the
  * triangle within its geometry. */
 int orc_scene_closest1(const orc_scene* sc, uint32_t* stack, const float* o, const float* d, float tnear, float tfar,
-                       float* t, float* u, float* v, uint32_t* geom, float* ng, uint32_t* tri_in_geom);
+                       float* t, float* u, float* v, uint32_t* geom, float* ng, uint32_t* tri_in_geom,
+                       const float** l2w);
 int orc_scene_occluded1(const orc_scene* sc, uint32_t* stack, const float* o, const float* d, float tnear, float tfar);
 uint32_t orc_scene_stack_size(const orc_scene* sc);
 /* the texel of a hit on triangle tri_in_geom of geometry geom (NULL: the material's colour) */
@@ -208,8 +224,17 @@ typedef struct {
     uint32_t flags; /* 1: quad-light estimator without the extra Dot(normal, l) */
 } orc_pt_job;
 
-void orc_surface_from_hit(const float* o, const float* d, float t, const float* ng, const float* mat,
-                          const float* texel, orc_surface* s, float* position, float* error);
+/* CalculateSurfaceParams at a hit: position and error bound; the shading frame from the
+ * geometry's vertex attributes ga (NULL: none) interpolated on triangle tri_in_geom of the geometry
+ * (a quad geometry's triangles 2q and 2q + 1 are quad q) at (hu, hv) and carried to world space by
+ * l2w (row-major 3x3, NULL: identity), else from the geometric normal ng (world space); the
+ * material with `texel` (or NULL) as base colour. Returns 1 when the frame came from tangents
+ * (then tan4, if not NULL, receives (t, w)). */
+int orc_surface_from_hit(const float* o, const float* d, float t, const float* ng, const orc_attr* ga,
+                         uint32_t tri_in_geom, float hu, float hv, const float* l2w, const float* mat,
+                         const float* texel, orc_surface* s, float* position, float* error, float* tan4);
+/* The frame of a stored vertex with shading normal s->n and stored tangent (t, w). */
+void orc_tangent_frame(const float* tan4, orc_surface* s);
 /* The base-colour texel of a hit on triangle `tri` (barycentrics hu, hv) when the material uses a
  * per-face texture and its geometry has one; returns NULL otherwise. */
 const float* orc_hit_texel(const orc_scene* sc, uint32_t tri, uint32_t geom, const float* mat, float hu, float hv,

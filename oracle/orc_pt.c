@@ -54,10 +54,43 @@ static inline float maxf(float a, float b) { return (a > b) ? a : b; }
 static inline float saturate(float x) { return x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x); }
 static inline ov3 ld3(const float* p) { return v3(p[0], p[1], p[2]); }
 
-/* CalculateSurfaceParams for a flat-shaded triangle hit; `texel` (or NULL) replaces the
- * material's baseColor before the 2.2 power. */
-void orc_surface_from_hit(const float* o, const float* d, float t, const float* ng, const float* mat,
-                          const float* texel, orc_surface* s, float* position, float* error)
+/* rtcInterpolate0 (SurfaceParameters.cpp:155-190 calls it for slots 0, 1 and 2) as Embree 3
+ * evaluates it: on a triangle (p0, p1, p2) fma(w, p0, fma(u, p1, v p2)) with w = 1 - u - v; on a
+ * quad (p0, p1, p2, p3) in the triangle of Embree's split that holds (u, v): (p0, p1, p3) with
+ * (u, v) when u + v <= 1, else (p2, p3, p1) with (1 - u, 1 - v). */
+void orc_interp_attr(const float* a, int K, uint32_t corners, uint32_t prim, float u, float v, float* out)
+{
+    const size_t base = (size_t)prim * corners;
+    size_t i0 = base, i1 = base + 1, i2 = base + 2;
+    float U = u, V = v;
+    if (corners == 4) {
+        if (u + v <= 1.0f) {
+            i2 = base + 3;
+        } else {
+            i0 = base + 2;
+            i1 = base + 3;
+            i2 = base + 1;
+            U = 1.0f - u;
+            V = 1.0f - v;
+        }
+    }
+    const float W = 1.0f - U - V;
+    for (int k = 0; k < K; ++k) out[k] = fmaf(W, a[i0 * K + k], fmaf(U, a[i1 * K + k], V * a[i2 * K + k]));
+}
+
+static void xform_linear(const float* m, const float* n, float* out);
+
+void orc_tangent_frame(const float* tan4, orc_surface* s)
+{
+    s->t = v3(tan4[0], tan4[1], tan4[2]);
+    s->b = scl(cross(s->n, s->t), tan4[3]);
+}
+
+/* CalculateSurfaceParams for a triangle hit (SurfaceParameters.cpp:144-247); `texel` (or NULL)
+ * replaces the material's baseColor before the 2.2 power. */
+int orc_surface_from_hit(const float* o, const float* d, float t, const float* ng, const orc_attr* ga,
+                         uint32_t tri_in_geom, float hu, float hv, const float* l2w, const float* mat,
+                         const float* texel, orc_surface* s, float* position, float* error, float* tan4)
 {
     ov3 p = v3(o[0] + t * d[0], o[1] + t * d[1], o[2] + t * d[2]);
     const float kErr = 32.0f * 1.19209e-07f;
@@ -65,7 +98,28 @@ void orc_surface_from_hit(const float* o, const float* d, float t, const float* 
     position[0] = p.x;
     position[1] = p.y;
     position[2] = p.z;
-    ov3 n = normalize(v3(ng[0], ng[1], ng[2]));
+    /* the primitive and its own (u, v): for quads the quad and the coordinates the hit reports */
+    uint32_t prim = tri_in_geom;
+    float pu = hu, pv = hv;
+    if (ga && ga->corners == 4) {
+        prim = tri_in_geom >> 1;
+        if (tri_in_geom & 1u) {
+            pu = 1.0f - hu;
+            pv = 1.0f - hv;
+        }
+    }
+    ov3 n;
+    if (ga && ga->n) {
+        float a[3], w[3];
+        orc_interp_attr(ga->n, 3, ga->corners, prim, pu, pv, a);
+        if (l2w) {
+            xform_linear(l2w, a, w);
+            memcpy(a, w, sizeof(a));
+        }
+        n = normalize(v3(a[0], a[1], a[2]));
+    } else {
+        n = normalize(v3(ng[0], ng[1], ng[2]));
+    }
     float nn[3] = {n.x, n.y, n.z};
     float block[22];
     memcpy(block, mat, sizeof(block));
@@ -80,6 +134,18 @@ void orc_surface_from_hit(const float* o, const float* d, float t, const float* 
     uint32_t flags = (uint32_t)mat[21];
     block[15] = ((flags & 1u) && dot(view, n) < 0.0f) ? mat[19] : 1.0f / mat[19];
     orc_surface_from_block(block, nn, s);
+    if (ga && ga->t) {
+        float a[4], w[3];
+        orc_interp_attr(ga->t, 4, ga->corners, prim, pu, pv, a);
+        if (l2w) {
+            xform_linear(l2w, a, w);
+            memcpy(a, w, sizeof(w));
+        }
+        orc_tangent_frame(a, s);
+        if (tan4) memcpy(tan4, a, 4 * sizeof(float));
+        return 1;
+    }
+    return 0;
 }
 
 /* CalculateSurfaceParams' eUsesPtex branch (SurfaceParameters.cpp:196-203): the texture is
@@ -229,8 +295,10 @@ static void inst_ray(const float* w, const float* o, const float* d, float* lo, 
 }
 
 int orc_scene_closest1(const orc_scene* sc, uint32_t* stack, const float* o, const float* d, float tnear, float tfar,
-                       float* t, float* u, float* v, uint32_t* geom, float* ng, uint32_t* tri_in_geom)
+                       float* t, float* u, float* v, uint32_t* geom, float* ng, uint32_t* tri_in_geom,
+                       const float** l2w)
 {
+    if (l2w) *l2w = NULL;
     if (!sc->inst) {
         uint32_t prim = orc_bvh_closest1(sc->bvh, stack, o, d, tnear, tfar, t, u, v);
         if (prim == 0xFFFFFFFFu) return 0;
@@ -264,6 +332,7 @@ int orc_scene_closest1(const orc_scene* sc, uint32_t* stack, const float* o, con
     float nl[3];
     tri_normal(is->tri9[best_k] + (size_t)best * 9, nl);
     xform_linear(is->l2w + 9 * (size_t)best_k, nl, ng);
+    if (l2w) *l2w = is->l2w + 9 * (size_t)best_k;
     return 1;
 }
 
@@ -309,7 +378,8 @@ static void evaluate_path(const pt_ctx* c, uint32_t* stack, uint32_t x, uint32_t
     while (bounce < job->max_path_length) {
         float t, hu, hv, ng[3];
         uint32_t geom, tri_in_geom;
-        if (!orc_scene_closest1(sc, stack, o, d, 0.0f, FLOAT_MAX, &t, &hu, &hv, &geom, ng, &tri_in_geom)) {
+        const float* l2w = NULL;
+        if (!orc_scene_closest1(sc, stack, o, d, 0.0f, FLOAT_MAX, &t, &hu, &hv, &geom, ng, &tri_in_geom, &l2w)) {
             /* EvaluateBackground (AreaLighting.cpp:415-424): the image-based light, else the
              * constant background */
             float bg[3];
@@ -324,8 +394,8 @@ static void evaluate_path(const pt_ctx* c, uint32_t* stack, uint32_t x, uint32_t
         orc_surface s;
         float pos[3], err;
         float texel[3];
-        orc_surface_from_hit(o, d, t, ng, mat, orc_hit_texel_local(sc, tri_in_geom, geom, mat, hu, hv, texel), &s, pos,
-                             &err);
+        orc_surface_from_hit(o, d, t, ng, sc->geom_attr ? &sc->geom_attr[geom] : NULL, tri_in_geom, hu, hv, l2w, mat,
+                             orc_hit_texel_local(sc, tri_in_geom, geom, mat, hu, hv, texel), &s, pos, &err, NULL);
         float v[3] = {-d[0], -d[1], -d[2]};
         float nrm[3] = {s.n.x, s.n.y, s.n.z};
         float ls[8];
@@ -481,7 +551,8 @@ static ov3 dpt_path(const dpt_ctx* c, uint32_t* stack, uint32_t x, uint32_t y, u
     for (uint32_t bounce = 0; bounce < job->max_path_length; ++bounce) {
         float t, hu, hv, ng[3];
         uint32_t geom, tri_in_geom;
-        if (!orc_scene_closest1(sc, stack, o, d, 0.0f, FLOAT_MAX, &t, &hu, &hv, &geom, ng, &tri_in_geom)) {
+        const float* l2w = NULL;
+        if (!orc_scene_closest1(sc, stack, o, d, 0.0f, FLOAT_MAX, &t, &hu, &hv, &geom, ng, &tri_in_geom, &l2w)) {
             float bg[3];
             const float sky_pdf = dpt_sky_lookup(sc, d, bg);
             float w = 1.0f;
@@ -492,8 +563,8 @@ static ov3 dpt_path(const dpt_ctx* c, uint32_t* stack, uint32_t x, uint32_t y, u
         const float* mat = sc->materials + 22 * (size_t)sc->geom_material[geom];
         orc_surface sf;
         float pos[3], err, texel[3];
-        orc_surface_from_hit(o, d, t, ng, mat, orc_hit_texel_local(sc, tri_in_geom, geom, mat, hu, hv, texel), &sf, pos,
-                             &err);
+        orc_surface_from_hit(o, d, t, ng, sc->geom_attr ? &sc->geom_attr[geom] : NULL, tri_in_geom, hu, hv, l2w, mat,
+                             orc_hit_texel_local(sc, tri_in_geom, geom, mat, hu, hv, texel), &sf, pos, &err, NULL);
         float v[3] = {-d[0], -d[1], -d[2]};
         float nrm[3] = {sf.n.x, sf.n.y, sf.n.z};
         /* NEE toward the lights */

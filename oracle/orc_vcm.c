@@ -354,6 +354,8 @@ typedef struct {
     uint32_t path_length;
     uint32_t geom;
     float error;
+    int has_tan;   /* the frame came from the geometry's tangents (t, w) */
+    float tan[4];
 } vcm_vertex;
 
 typedef struct {
@@ -364,17 +366,24 @@ typedef struct {
     uint32_t* stack;
 } vcm_ctx;
 
+/* The surface of the closest hit; *has_tan (may be NULL) tells whether its frame came from the
+ * geometry's tangents, then stored in tan4. */
 static int hit_surface(const vcm_ctx* c, const float* o, const float* d, float tnear, orc_surface* s, float* pos,
-                       float* err, uint32_t* geom_out)
+                       float* err, uint32_t* geom_out, int* has_tan, float* tan4)
 {
     float t, hu, hv, ng[3];
     uint32_t geom, tri_in_geom;
-    if (!orc_scene_closest1(c->sc, c->stack, o, d, tnear, 3.402823466e+38F, &t, &hu, &hv, &geom, ng, &tri_in_geom))
+    const float* l2w = NULL;
+    if (!orc_scene_closest1(c->sc, c->stack, o, d, tnear, 3.402823466e+38F, &t, &hu, &hv, &geom, ng, &tri_in_geom,
+                            &l2w))
         return 0;
     const float* mat = c->sc->materials + 22 * (size_t)c->sc->geom_material[geom];
     float texel[3];
-    orc_surface_from_hit(o, d, t, ng, mat, orc_hit_texel_local(c->sc, tri_in_geom, geom, mat, hu, hv, texel), s, pos,
-                         err);
+    const int tf = orc_surface_from_hit(o, d, t, ng, c->sc->geom_attr ? &c->sc->geom_attr[geom] : NULL, tri_in_geom,
+                                        hu, hv, l2w, mat,
+                                        orc_hit_texel_local(c->sc, tri_in_geom, geom, mat, hu, hv, texel), s, pos, err,
+                                        tan4);
+    if (has_tan) *has_tan = tf;
     *geom_out = geom;
     return 1;
 }
@@ -395,6 +404,7 @@ static void surface_of_vertex(const vcm_ctx* c, const vcm_vertex* v, orc_surface
     block[15] = ((flags & 1u) && dot(v->wfix, v->n) < 0.0f) ? mat[19] : 1.0f / mat[19];
     float nn[3] = {v->n.x, v->n.y, v->n.z};
     orc_surface_from_block(block, nn, s);
+    if (v->has_tan) orc_tangent_frame(v->tan, s);
 }
 
 static void eval_bsdf(const orc_surface* s, ov3 v, ov3 l, ov3* f, float* fwd, float* rev)
@@ -496,7 +506,9 @@ static void light_paths(void* ctx, uint32_t b, uint32_t e)
                 float hp[3], err;
                 uint32_t geom;
                 ++nrays;
-                if (!hit_surface(&c, o, d, 0.00001f, &s, hp, &err, &geom)) break;
+                int has_tan = 0;
+                float tan4[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                if (!hit_surface(&c, o, d, 0.00001f, &s, hp, &err, &geom, &has_tan, tan4)) break;
                 ov3 x = ld3(hp);
                 ov3 view = neg(dir);
                 float d2 = dot(sub(pos, x), sub(pos, x));
@@ -519,6 +531,8 @@ static void light_paths(void* ctx, uint32_t b, uint32_t e)
                 vx->path_length = plen;
                 vx->geom = geom;
                 vx->error = err;
+                vx->has_tan = has_tan;
+                memcpy(vx->tan, tan4, sizeof(tan4));
                 /* ConnectLightPathToCamera (VCM.cpp:190-242) */
                 ov3 to_pos = sub(x, c.cam.pos);
                 float px, py;
@@ -527,9 +541,11 @@ static void light_paths(void* ctx, uint32_t b, uint32_t e)
                     float dist = length(to_pos);
                     ov3 to_cam = scl(to_pos, -1.0f / dist);
                     float cos_cam = dot(c.cam.forward, neg(to_cam));
-                    ov3 f;
+                    ov3 f = v3(0, 0, 0);
                     float fwd, rev;
-                    eval_bsdf(&s, view, to_cam, &f, &fwd, &rev);
+                    /* a connection in the surface's plane carries nothing (its BSDF value holds the
+                     * cosine) and the BSDF is not evaluated there (0/0 in Disney's terms) */
+                    if (dot(s.n, to_cam) != 0.0f) eval_bsdf(&s, view, to_cam, &f, &fwd, &rev);
                     if (!is_zero(f)) {
                         float cp = cont_prob(&s);
                         rev *= cp;
@@ -587,6 +603,8 @@ static void merge_vertex(void* ctx, uint32_t index)
     surface_of_vertex(m->c, lv, &ls);
     /* the reference's transparency rule (VCM.cpp:381-383) */
     if ((m->s->flags & 1u) != (ls.flags & 1u)) return;
+    /* a photon arriving in the surface's plane is not evaluated (measure zero; 0/0 below) */
+    if (dot(m->s->n, lv->wfix) == 0.0f) return;
     ov3 f;
     float fwd, rev;
     eval_bsdf(m->s, m->view, lv->wfix, &f, &fwd, &rev);
@@ -630,7 +648,7 @@ static void camera_paths(void* ctx, uint32_t b, uint32_t e)
             float hp[3], err;
             uint32_t geom;
             ++nrays;
-            if (!hit_surface(&c, o, d, 0.00001f, &s, hp, &err, &geom)) {
+            if (!hit_surface(&c, o, d, 0.00001f, &s, hp, &err, &geom, NULL, NULL)) {
                 if (sc->ibl) {
                     /* ConnectToSkyLight (VCM.cpp:171-186): unweighted when seen from the camera */
                     float rad[3];
@@ -662,9 +680,9 @@ static void camera_paths(void* ctx, uint32_t b, uint32_t e)
                 float ldist, dpdf, epdf, cl;
                 int infinite;
                 if (direct_light(&u, sc, x, &ldir, &rad, &ldist, &dpdf, &epdf, &cl, &infinite)) {
-                    ov3 f;
+                    ov3 f = v3(0, 0, 0);
                     float fwd, rev;
-                    eval_bsdf(&s, view, ldir, &f, &fwd, &rev);
+                    if (dot(s.n, ldir) != 0.0f) eval_bsdf(&s, view, ldir, &f, &fwd, &rev);
                     if (!is_zero(f)) {
                         float cp = cont_prob(&s);
                         rev *= cp;
@@ -694,6 +712,8 @@ static void camera_paths(void* ctx, uint32_t b, uint32_t e)
                     float dist2 = dot(dv, dv);
                     float dist = sqrtf(dist2);
                     ov3 cdir = scl(dv, 1.0f / dist);
+                    /* a connection in either surface's plane carries nothing (see the splats) */
+                    if (dot(cdir, s.n) == 0.0f || dot(neg(cdir), lv->n) == 0.0f) continue;
                     ov3 fc;
                     float cfwd, crev;
                     eval_bsdf(&s, view, cdir, &fc, &cfwd, &crev);

@@ -275,15 +275,24 @@ int sgpu_scene_add_triangles(sgpu_scene scene, const float* positions, uint32_t 
         if (indices[t] >= n_verts)
             return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_add_triangles: index out of range");
     const uint32_t geom = (uint32_t)scene->geom_first.size();
-    scene->geom_first.push_back((uint32_t)scene->tri_geom.size());
-    scene->geom_quad.push_back(0);
     const size_t base = scene->tri_pos.size();
     try {
         scene->tri_pos.resize(base + (size_t)n_tris * 9);
         scene->tri_geom.resize(scene->tri_geom.size() + n_tris, geom);
+        scene->geom_idx.emplace_back(indices, indices + (size_t)n_tris * 3);
+        scene->geom_nverts.push_back(n_verts);
+        scene->geom_xform.emplace_back();
+        if (colmajor4x4) scene->geom_xform.back().assign(colmajor4x4, colmajor4x4 + 16);
     } catch (...) {
+        scene->tri_pos.resize(base);
+        scene->tri_geom.resize(scene->geom_first.empty() ? 0 : scene->tri_geom.size() - n_tris);
+        scene->geom_idx.resize(geom);
+        scene->geom_nverts.resize(geom);
+        scene->geom_xform.resize(geom);
         return sgpu_fail(SGPU_E_NOMEM, "sgpu_scene_add_triangles: out of host memory");
     }
+    scene->geom_first.push_back((uint32_t)scene->tri_geom.size() - n_tris);
+    scene->geom_quad.push_back(0);
     float* dst = scene->tri_pos.data() + base;
     for (uint32_t t = 0; t < n_tris; ++t) {
         for (int k = 0; k < 3; ++k) {
@@ -331,8 +340,73 @@ int sgpu_scene_add_quads(sgpu_scene scene, const float* positions, uint32_t n_ve
     if (rc != SGPU_OK) return rc;
     std::lock_guard<std::mutex> lock(scene->mu);
     scene->geom_quad[geom] = 1;
+    // attribute corners follow the quad's own vertex order
+    scene->geom_idx[geom].assign(indices, indices + (size_t)n_quads * 4);
     if (geom_id_out) *geom_id_out = geom;
     return SGPU_OK;
+}
+
+int sgpu_scene_set_vertex_attributes(sgpu_scene scene, uint32_t geom_id, const float* normals, const float* tangents,
+                                     const float* uvs, uint32_t n_verts)
+{
+    const char* who = "sgpu_scene_set_vertex_attributes";
+    if (!scene) return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": NULL scene");
+    if ((tangents && !normals) || (uvs && !(normals && tangents)))
+        return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": tangents need normals, UVs need normals and tangents");
+    std::lock_guard<std::mutex> lock(scene->mu);
+    if (scene->committed) return sgpu_fail(SGPU_E_STATE, std::string(who) + ": scene already committed");
+    if (geom_id >= scene->geom_first.size()) return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": no such geometry");
+    if (n_verts != scene->geom_nverts[geom_id])
+        return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": vertex count differs from the geometry's");
+    const size_t nv = n_verts;
+    for (size_t i = 0; i < 3 * nv && normals; ++i)
+        if (!std::isfinite(normals[i])) return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": non-finite normal");
+    for (size_t i = 0; i < 4 * nv && tangents; ++i)
+        if (!std::isfinite(tangents[i])) return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": non-finite tangent");
+    for (size_t i = 0; i < 2 * nv && uvs; ++i)
+        if (!std::isfinite(uvs[i])) return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": non-finite UV");
+    SGPU_HIP_TRY(hipSetDevice(scene->ctx->device));
+    if (scene->attr_host.size() < scene->geom_first.size())
+        scene->attr_host.resize(scene->geom_first.size(), sgpu::GeomAttr{nullptr, nullptr, nullptr, 0, 0});
+    sgpu::GeomAttr& ga = scene->attr_host[geom_id];
+    // replaced attributes: their allocation is released
+    const void* olds[3] = {ga.n, ga.t, ga.uv};
+    for (const void* o : olds) {
+        if (!o) continue;
+        auto it = std::find_if(scene->attr_allocs.begin(), scene->attr_allocs.end(),
+                               [&](const std::pair<void*, size_t>& a) { return a.first == o; });
+        if (it != scene->attr_allocs.end()) {
+            (void)hipFree(it->first);
+            scene->attr_allocs.erase(it);
+        }
+    }
+    const std::vector<uint32_t>& idx = scene->geom_idx[geom_id];
+    const uint32_t corners = scene->geom_quad[geom_id] ? 4u : 3u;
+    const std::vector<float>& m = scene->geom_xform[geom_id];
+    ga = sgpu::GeomAttr{nullptr, nullptr, nullptr, corners, 0};
+    // per corner: the vertex's value, normals and tangents' xyz through the placement's linear part
+    auto upload = [&](const float* src, int K, bool vec, const float** out) -> int {
+        std::vector<float> c(idx.size() * (size_t)K);
+        for (size_t i = 0; i < idx.size(); ++i) {
+            const float* a = src + (size_t)idx[i] * K;
+            float* o = c.data() + i * K;
+            for (int k = 0; k < K; ++k) o[k] = a[k];
+            if (vec && !m.empty())
+                for (int r = 0; r < 3; ++r) o[r] = m[0 * 4 + r] * a[0] + m[1 * 4 + r] * a[1] + m[2 * 4 + r] * a[2];
+        }
+        void* d = nullptr;
+        SGPU_HIP_TRY(hipMalloc(&d, std::max<size_t>(c.size(), 1) * sizeof(float)));
+        scene->attr_allocs.emplace_back(d, c.size() * sizeof(float));
+        SGPU_HIP_TRY(hipMemcpy(d, c.data(), c.size() * sizeof(float), hipMemcpyHostToDevice));
+        *out = (const float*)d;
+        return SGPU_OK;
+    };
+    int rc = SGPU_OK;
+    if (normals) rc = upload(normals, 3, true, &ga.n);
+    if (rc == SGPU_OK && tangents) rc = upload(tangents, 4, true, &ga.t);
+    if (rc == SGPU_OK && uvs) rc = upload(uvs, 2, false, &ga.uv);
+    if (rc != SGPU_OK) ga = sgpu::GeomAttr{nullptr, nullptr, nullptr, corners, 0};
+    return rc;
 }
 
 int sgpu_scene_add_instance(sgpu_scene scene, sgpu_scene child, const float* colmajor4x4, uint32_t* inst_id_out)
@@ -402,6 +476,8 @@ static int gather_instance_shading(sgpu_scene scene)
     }
     std::vector<uint32_t> gmat(n_geoms);
     std::vector<sgpu::PtexTex> gtex(n_geoms, sgpu::PtexTex{nullptr, nullptr, 0, 0});
+    std::vector<sgpu::GeomAttr> gattr(n_geoms, sgpu::GeomAttr{nullptr, nullptr, nullptr, 0, 0});
+    bool any_attr = false;
     for (size_t j = 0; j < models.size(); ++j) {
         sgpu_scene m = models[j];
         const size_t ng = m->geom_first.size();
@@ -409,7 +485,17 @@ static int gather_instance_shading(sgpu_scene scene)
         for (size_t g = 0; g < ng; ++g) {
             gmat[g0[j] + g] += m0[j];
             if (g < m->tex_host.size()) gtex[g0[j] + g] = m->tex_host[g];
+            if (m->d_geom_attr && g < m->attr_host.size()) {
+                gattr[g0[j] + g] = m->attr_host[g];
+                any_attr |= m->attr_host[g].n != nullptr;
+                scene->has_tangents |= m->attr_host[g].t != nullptr;
+            }
         }
+    }
+    if (any_attr) {
+        SGPU_HIP_TRY(hipMalloc((void**)&scene->d_g_geom_attr, sizeof(sgpu::GeomAttr) * (size_t)n_geoms));
+        SGPU_HIP_TRY(hipMemcpy(scene->d_g_geom_attr, gattr.data(), sizeof(sgpu::GeomAttr) * n_geoms,
+                               hipMemcpyHostToDevice));
     }
     std::vector<uint32_t> base(model_of.size());
     for (size_t k = 0; k < base.size(); ++k) base[k] = g0[model_of[k]];
@@ -453,6 +539,20 @@ int sgpu_scene_commit(sgpu_scene scene, void* stream)
     }
     int rc = sgpu_bvh_build(scene, as_stream(stream));
     if (rc != SGPU_OK) return rc;
+    // the attribute table (one record per geometry) when any geometry has attributes
+    bool any_attr = false;
+    for (const sgpu::GeomAttr& ga : scene->attr_host) {
+        any_attr |= ga.n != nullptr;
+        scene->has_tangents |= ga.t != nullptr;
+    }
+    if (any_attr) {
+        scene->attr_host.resize(scene->geom_first.size(), sgpu::GeomAttr{nullptr, nullptr, nullptr, 0, 0});
+        SGPU_HIP_TRY(hipMalloc((void**)&scene->d_geom_attr, sizeof(sgpu::GeomAttr) * scene->attr_host.size()));
+        SGPU_HIP_TRY(hipMemcpy(scene->d_geom_attr, scene->attr_host.data(),
+                               sizeof(sgpu::GeomAttr) * scene->attr_host.size(), hipMemcpyHostToDevice));
+    }
+    std::vector<std::vector<uint32_t>>().swap(scene->geom_idx);
+    std::vector<std::vector<float>>().swap(scene->geom_xform);
     // the sphere around the triangles' bounding box (the IBL emits from its disc)
     if (!scene->tri_pos.empty()) {
         float lo[3], hi[3];
@@ -490,6 +590,9 @@ int sgpu_scene_release(sgpu_scene scene)
     if (scene->d_g_materials) (void)hipFree(scene->d_g_materials);
     if (scene->d_g_geom_material) (void)hipFree(scene->d_g_geom_material);
     if (scene->d_g_geom_tex) (void)hipFree(scene->d_g_geom_tex);
+    if (scene->d_g_geom_attr) (void)hipFree(scene->d_g_geom_attr);
+    if (scene->d_geom_attr) (void)hipFree(scene->d_geom_attr);
+    for (const auto& a : scene->attr_allocs) (void)hipFree(a.first);
     if (scene->d_inst_geom_base) (void)hipFree(scene->d_inst_geom_base);
     if (scene->d_inst_l2w) (void)hipFree(scene->d_inst_l2w);
     if (scene->d_nodes) (void)hipFree(scene->d_nodes);
@@ -951,6 +1054,8 @@ int sgpu_scene_device_bytes(sgpu_scene scene, uint64_t* bytes)
                  (uint64_t)scene->n_inst * (sizeof(InstRecord) + 2 * sizeof(uint32_t));
     for (const sgpu::PtexTex& t : scene->tex_host)
         if (t.texels) b += (uint64_t)t.nfaces * (3ull << (2 * t.res_log2)) + (uint64_t)t.nfaces * 8 * sizeof(int32_t);
+    for (const auto& a : scene->attr_allocs) b += a.second;
+    if (scene->d_geom_attr) b += scene->attr_host.size() * sizeof(sgpu::GeomAttr);
     *bytes = b;
     return SGPU_OK;
 }

@@ -4,11 +4,13 @@
 #include "../../include/selas_gpu.h"
 #include "sgpu_math.h"
 #include "sgpu_ptex.h"
+#include "sgpu_surface.h"
 
 #include <hip/hip_runtime.h>
 #include <atomic>
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 // ---- device data layout of a committed scene ---------------------------------------------------
@@ -119,6 +121,18 @@ struct sgpu_scene_s {
     void* d_lights = nullptr;          // sgpu_light records
     uint32_t n_lights = 0;
     float background[3] = {0.0f, 0.0f, 0.0f};
+    // vertex attributes (sgpu_scene_set_vertex_attributes). Until commit, per geometry: its index
+    // buffer in primitive vertex order (3 or 4 per primitive), vertex count and placement transform
+    // (column-major 4x4, empty for none), to expand per-vertex values to corners. attr_host[g] points
+    // into attr_allocs (device corner arrays); d_geom_attr is the device table of attr_host (NULL
+    // when no geometry has attributes)
+    std::vector<std::vector<uint32_t>> geom_idx;
+    std::vector<uint32_t> geom_nverts;
+    std::vector<std::vector<float>> geom_xform;
+    std::vector<sgpu::GeomAttr> attr_host;
+    std::vector<std::pair<void*, size_t>> attr_allocs;  // device allocation, bytes
+    sgpu::GeomAttr* d_geom_attr = nullptr;
+    bool has_tangents = false;  // some geometry (placed geometry, for an instanced scene) has tangents
     // per-face textures (sgpu_scene_set_ptex): one record per geometry, texels == nullptr for none
     std::vector<sgpu::PtexTex> tex_host;
     std::vector<void*> tex_allocs;
@@ -152,6 +166,7 @@ struct sgpu_scene_s {
     float* d_g_materials = nullptr;       // shading records of all placed scenes
     uint32_t* d_g_geom_material = nullptr;
     sgpu::PtexTex* d_g_geom_tex = nullptr;
+    sgpu::GeomAttr* d_g_geom_attr = nullptr;  // NULL when no placed geometry has attributes
     uint32_t n_g_geoms = 0;
     uint32_t* d_inst_geom_base = nullptr;  // per composed instance
     float* d_inst_l2w = nullptr;           // per composed instance: local-to-world linear part, 9 floats

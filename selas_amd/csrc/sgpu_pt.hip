@@ -231,6 +231,7 @@ struct ShadeParams {
     uint32_t n_geoms;
     const PtexTex* geom_tex;  // per-face texture per geometry (texels == nullptr: none)
     uint32_t n_geom_tex;
+    const GeomAttr* geom_attr;  // vertex attributes per geometry (NULL: no geometry has any)
     const Light* lights;
     uint32_t n_lights;
     f3 background;
@@ -272,11 +273,13 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(ShadeParams sp, uint32_t co
         f3 thr = get3(ps.thr, p);
         uint32_t geom = hits.geom[j];
         f3 ng = mk3(hits.ngx[j], hits.ngy[j], hits.ngz[j]);
+        const float* l2w = nullptr;
         if (sp.instanced && geom != SGPU_INVALID_ID) {
             // the placed scene's geometry in the scene's list, its normal in world space
             const uint32_t k = hits.inst[j];
             geom += sp.inst_geom_base[k];
-            ng = xform_linear(sp.inst_l2w + 9 * (size_t)k, ng);
+            l2w = sp.inst_l2w + 9 * (size_t)k;
+            ng = xform_linear(l2w, ng);
         }
         if (geom == SGPU_INVALID_ID || geom >= sp.n_geoms) {
             // miss: EvaluateBackground (AreaLighting.cpp:415-424): the image-based light's radiance
@@ -308,8 +311,10 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(ShadeParams sp, uint32_t co
                 texel = ptex_eval(sp.geom_tex[geom], (int)hits.prim[j], hits.u[j], hits.v[j]);
                 tex = true;
             }
-            surface_from_hit(o, d, hits.t[j], ng, mat, tex ? &texel : nullptr,
-                             s);
+            GeomAttr ga;
+            if (sp.geom_attr) ga = sp.geom_attr[geom];
+            surface_from_hit(o, d, hits.t[j], ng, sp.geom_attr ? &ga : nullptr, hits.prim[j], hits.u[j], hits.v[j], l2w,
+                             mat, tex ? &texel : nullptr, s);
             f3 v = -d;
             LightSample ls;
             next_event_estimation(u, sp.lights, sp.n_lights, s.position, s.n,
@@ -594,12 +599,14 @@ int render_paths(sgpu_scene scene, const PathJob& job, float* fb, hipStream_t st
     sp.has_ibl = ibl_view(scene, sp.ibl) ? 1u : 0u;
     sp.geom_tex = scene->d_geom_tex;
     sp.n_geom_tex = scene->n_geom_tex;
+    sp.geom_attr = scene->d_geom_attr;
     if (sp.instanced) {
         sp.materials = scene->d_g_materials;
         sp.geom_material = scene->d_g_geom_material;
         sp.n_geoms = scene->n_g_geoms;
         sp.geom_tex = scene->d_g_geom_tex;
         sp.n_geom_tex = scene->n_g_geoms;
+        sp.geom_attr = scene->d_g_geom_attr;
     }
     sp.max_len = job.max_path_length;
     sp.flags = job.flags;

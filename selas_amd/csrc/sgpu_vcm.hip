@@ -167,17 +167,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         scn = sl.scene;
         uint32_t geom;
         f3 ng;
-        hit_geometry(sl, hits, j, geom, ng);
+        const float* l2w;
+        hit_geometry(sl, hits, j, geom, ng, l2w);
         if (geom != SGPU_INVALID_ID && geom < sl.n_geoms) {
             PcgUniforms u{rng_load(ps, p, sl, 0)};
             f3 o = mk3(rays.ox[j], rays.oy[j], rays.oz[j]);
             f3 d = mk3(rays.dx[j], rays.dy[j], rays.dz[j]);
             const float* mat = sl.materials + 22 * (size_t)sl.geom_material[geom];
             Surface s;
-            f3 texel;
-            const bool tex = hit_texel(sl, geom, mat, hits.prim[j], hits.u[j], hits.v[j], texel);
-            surface_from_hit(o, d, hits.t[j], ng, mat, tex ? &texel : nullptr,
-                             s);
+            float4 tan;
+            const bool tf = hit_surface(sl, hits, j, geom, ng, l2w, o, d, mat, s, &tan);
             f3 x = s.position;
             f3 view = -d;
             f3 pos = g3(ps.pos, p);
@@ -219,6 +218,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 lv.base[2] = s.base_color.z;
                 lv.cp = cont_prob(s);
                 verts[(size_t)p * w.mlv + k] = lv;
+                if (tf) w.vtan[(size_t)p * w.mlv + k] = tan;
                 vcount[p] = k + 1;
             }
             // ConnectLightPathToCamera (VCM.cpp:190-242)
@@ -236,7 +236,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                         f3 to_cam = to_pos * (-1.0f / dist);
                         float cos_cam = dot3(cam.forward, -to_cam);
                         float fwd, rev;
-                        f3 f = bsdf_eval(s, view, to_cam, fwd, rev);
+                        // a connection in the surface's plane carries nothing (its BSDF value holds
+                        // the cosine) and the BSDF is not evaluated there (0/0 in Disney's terms)
+                        f3 f = mk3(0, 0, 0);
+                        if (dot3(s.n, to_cam) != 0.0f) f = bsdf_eval(s, view, to_cam, fwd, rev);
                         if (!is_zero3(f)) {
                             float cp = cont_prob(s);
                             rev *= cp;
@@ -320,6 +323,7 @@ __global__ __launch_bounds__(kBlock) void k_compact(Wave w, const LightVertex* _
         for (uint32_t k = 0; k < n; ++k) {
             LightVertex v = verts[(size_t)g * w.mlv + k];
             cverts[base + k] = v;
+            if (w.vtan) w.cvtan[base + k] = w.vtan[(size_t)g * w.mlv + k];
             cpos[3 * (base + k)] = v.pos[0];
             cpos[3 * (base + k) + 1] = v.pos[1];
             cpos[3 * (base + k) + 2] = v.pos[2];
@@ -504,7 +508,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         const Slice& sl = w.slices[si];
         uint32_t geom;
         f3 ng;
-        hit_geometry(sl, hits, j, geom, ng);
+        const float* l2w;
+        hit_geometry(sl, hits, j, geom, ng, l2w);
         f3 d = mk3(rays.dx[j], rays.dy[j], rays.dz[j]);
         if (geom == SGPU_INVALID_ID || geom >= sl.n_geoms) {
             const uint32_t plen_miss = ps.plen[p];
@@ -529,10 +534,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             geom_hit = geom;
             f3 o = mk3(rays.ox[j], rays.oy[j], rays.oz[j]);
             const float* mat = sl.materials + 22 * (size_t)sl.geom_material[geom];
-            f3 texel;
-            const bool tex = hit_texel(sl, geom, mat, hits.prim[j], hits.u[j], hits.v[j], texel);
-            surface_from_hit(o, d, hits.t[j], ng, mat, tex ? &texel : nullptr,
-                             s);
+            float4 tan;
+            if (hit_surface(sl, hits, j, geom, ng, l2w, o, d, mat, s, &tan)) w.cam_tan[j] = tan;
             x = s.position;
             view = -d;
         }
@@ -610,7 +613,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             }
             if (ok) {
                 float fwd, rev;
-                f3 f = bsdf_eval(s, view, ld, fwd, rev);
+                f3 f = mk3(0, 0, 0);
+                if (dot3(s.n, ld) != 0.0f) f = bsdf_eval(s, view, ld, fwd, rev);
                 if (!is_zero3(f)) {
                     float cp = cont_prob(s);
                     rev *= cp;
@@ -659,10 +663,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
                 float dist = sqrtf(dist2);
                 cd = dv * (1.0f / dist);
                 float cfwd, crev;
-                f3 fc = bsdf_eval(s, view, cd, cfwd, crev);
+                // a connection in either surface's plane carries nothing (see the splats)
+                f3 fc = mk3(0, 0, 0);
+                if (dot3(cd, s.n) != 0.0f && dot3(-cd, mk3(lv.n[0], lv.n[1], lv.n[2])) != 0.0f)
+                    fc = bsdf_eval(s, view, cd, cfwd, crev);
                 if (!is_zero3(fc)) {
                     Surface ls;
-                    vertex_surface(sl, lv, ls);
+                    vertex_surface(sl, lv, w.cvtan ? &w.cvtan[vi] : nullptr, ls);
                     float lfwd, lrev;
                     f3 lw = mk3(lv.wfix[0], lv.wfix[1], lv.wfix[2]);
                     f3 fl = bsdf_eval(ls, lw, -cd, lfwd, lrev);
@@ -876,10 +883,11 @@ struct ServerJob {
 struct GraphKey {
     const void* table = nullptr;
     const void* spill = nullptr;
+    const void* tan = nullptr;
     uint32_t max_stack = 0, tree = 0, blocks = 0, lds = 0, inst = 0;
     bool operator==(const GraphKey& o) const
     {
-        return table == o.table && spill == o.spill && max_stack == o.max_stack && tree == o.tree &&
+        return table == o.table && spill == o.spill && tan == o.tan && max_stack == o.max_stack && tree == o.tree &&
                blocks == o.blocks && lds == o.lds && inst == o.inst;
     }
 };
@@ -922,6 +930,11 @@ struct sgpu_vcm_server_s {
     uint32_t* spill = nullptr;
     uint32_t spill_cap = 0;           // stack entries the spill buffer is sized for
     uint32_t* spill_grown = nullptr;  // a larger spill buffer allocated at submit (freed at destroy)
+    // tangents of vertices shaded with a tangent frame (Wave::vtan, cvtan, cam_tan): one allocation
+    // made when the first job on a scene with vertex tangents is admitted (freed at destroy)
+    float4* vtan = nullptr;
+    float4* cvtan = nullptr;
+    float4* cam_tan = nullptr;
     CamSlots S;
     void* tmp = nullptr;
     size_t tmp_bytes = 0;
@@ -974,6 +987,7 @@ int server_graphs(sgpu_vcm_server sv, F&& pass_fn)
     GraphKey key;
     key.table = sv->d_table;
     key.spill = sv->spill;
+    key.tan = sv->vtan;
     key.max_stack = sv->max_stack;
     key.tree = ctx->trace_tree;
     key.blocks = ctx->trace_blocks_per_cu;
@@ -1227,6 +1241,7 @@ int sgpu_vcm_server_destroy_impl(sgpu_vcm_server sv, hipStream_t stream)
     if (sv->h_table) (void)hipHostFree(sv->h_table);
     if (sv->d_table) (void)hipFree(sv->d_table);
     if (sv->spill_grown) (void)hipFreeAsync(sv->spill_grown, stream);
+    if (sv->vtan) (void)hipFreeAsync(sv->vtan, stream);
     for (int k = 0; k < 2; ++k)
         if (sv->g_pass[k]) (void)hipGraphExecDestroy(sv->g_pass[k]);
     if (sv->cap_stream) (void)hipStreamDestroy(sv->cap_stream);
@@ -1251,6 +1266,15 @@ int sgpu_vcm_server_submit_impl(sgpu_vcm_server sv, sgpu_scene sc, const sgpu_vc
         sv->spill_grown = grown;
         sv->spill = grown;
         sv->spill_cap = sc->stack_cap;
+    }
+    if (sc->has_tangents && !sv->vtan) {
+        // the tangent arrays: 2 x (paths x vertex slots) for light vertices, paths for camera vertices
+        const size_t nv = (size_t)sv->TM * sv->mlv;
+        float4* t = nullptr;
+        SGPU_HIP_TRY(hipMallocAsync((void**)&t, (2 * nv + sv->TM) * sizeof(float4), stream));
+        sv->vtan = t;
+        sv->cvtan = t + nv;
+        sv->cam_tan = t + 2 * nv;
     }
     ServerJob j;
     j.id = sv->next_id++;
@@ -1280,6 +1304,7 @@ int sgpu_vcm_server_submit_impl(sgpu_vcm_server sv, sgpu_scene sc, const sgpu_vc
     b.background = mk3(sc->background[0], sc->background[1], sc->background[2]);
     b.geom_tex = sc->d_geom_tex;
     b.n_geom_tex = sc->n_geom_tex;
+    b.geom_attr = sc->d_geom_attr;
     b.has_ibl = ibl_view(sc, b.ibl) ? 1u : 0u;
     if (!sc->instances.empty()) {
         b.materials = sc->d_g_materials;
@@ -1287,6 +1312,7 @@ int sgpu_vcm_server_submit_impl(sgpu_vcm_server sv, sgpu_scene sc, const sgpu_vc
         b.n_geoms = sc->n_g_geoms;
         b.geom_tex = sc->d_g_geom_tex;
         b.n_geom_tex = sc->n_g_geoms;
+        b.geom_attr = sc->d_g_geom_attr;
         b.instanced = 1;
         b.inst_geom_base = sc->d_inst_geom_base;
         b.inst_l2w = sc->d_inst_l2w;
@@ -1396,6 +1422,9 @@ int sgpu_vcm_server_step_impl(sgpu_vcm_server sv, hipStream_t stream, uint64_t* 
     wv.dims = sv->dims;
     wv.mlv = sv->mlv;
     wv.nslots = sv->nslots;
+    wv.vtan = sv->vtan;
+    wv.cvtan = sv->cvtan;
+    wv.cam_tan = sv->cam_tan;
     sv->n_samples += total;
     ++sv->n_waves;
     sgpu_hits_soa hsoa{hb.t, hb.u, hb.v, hb.prim, hb.geom, hb.ngx, hb.ngy, hb.ngz};

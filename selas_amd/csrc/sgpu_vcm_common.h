@@ -121,6 +121,7 @@ struct Slice {
     f3 background;
     const PtexTex* geom_tex;  // per-face texture per geometry (texels == nullptr: none)
     uint32_t n_geom_tex;
+    const GeomAttr* geom_attr;  // vertex attributes per geometry (NULL: no geometry has any)
     uint32_t has_ibl;         // 1: the image-based light is light n_lights of the uniform pick
     IblView ibl;
     // instanced scene: a hit's geometry is numbered inst_geom_base[k] + geom in the scene's list
@@ -132,15 +133,51 @@ struct Slice {
 };
 
 // The geometry (in the slice's list) and world-space geometric normal of hit j.
-__device__ __forceinline__ void hit_geometry(const Slice& sl, const HitBuf& hits, uint32_t j, uint32_t& geom, f3& ng)
+__device__ __forceinline__ void hit_geometry(const Slice& sl, const HitBuf& hits, uint32_t j, uint32_t& geom, f3& ng,
+                                             const float*& l2w)
 {
     geom = hits.geom[j];
     ng = mk3(hits.ngx[j], hits.ngy[j], hits.ngz[j]);
+    l2w = nullptr;
     if (sl.instanced && geom != SGPU_INVALID_ID) {
         const uint32_t k = hits.inst[j];
         geom += sl.inst_geom_base[k];
-        ng = xform_linear(sl.inst_l2w + 9 * (size_t)k, ng);
+        l2w = sl.inst_l2w + 9 * (size_t)k;
+        ng = xform_linear(l2w, ng);
     }
+}
+
+// CalculateSurfaceParams' eUsesPtex branch (SurfaceParameters.cpp:196-203): the filtered texel of
+// the hit's face at its (u, v) when the material asks for it and its geometry has a texture.
+__device__ __forceinline__ bool hit_texel(const Slice& sl, uint32_t geom, const float* mat, uint32_t prim, float u,
+                                          float v, f3& texel)
+{
+    if (!((uint32_t)mat[21] & kMatUsesPtex) || geom >= sl.n_geom_tex) return false;
+    const PtexTex tx = sl.geom_tex[geom];
+    if (!tx.texels) return false;
+    texel = ptex_eval(tx, (int)prim, u, v);
+    return true;
+}
+
+// Surface of hit j of geometry geom (hit_geometry's outputs): CalculateSurfaceParams with the
+// geometry's vertex attributes and per-face texture. Returns true when the frame came from the
+// geometry's tangents (then *tangent holds (t, w)).
+__device__ __forceinline__ bool hit_surface(const Slice& sl, const HitBuf& hits, uint32_t j, uint32_t geom, f3 ng,
+                                            const float* l2w, f3 o, f3 d, const float* mat, Surface& s,
+                                            float4* tangent)
+{
+    f3 texel;
+    const bool tex = hit_texel(sl, geom, mat, hits.prim[j], hits.u[j], hits.v[j], texel);
+    GeomAttr ga;
+    if (sl.geom_attr) ga = sl.geom_attr[geom];
+    return surface_from_hit(o, d, hits.t[j], ng, sl.geom_attr ? &ga : nullptr, hits.prim[j], hits.u[j], hits.v[j],
+                            l2w, mat, tex ? &texel : nullptr, s, tangent);
+}
+
+// Whether geometry geom of a slice shades with a tangent frame (vertex tangents).
+__device__ __forceinline__ bool geom_has_tangents(const Slice& sl, uint32_t geom)
+{
+    return sl.geom_attr && sl.geom_attr[geom].t != nullptr;
 }
 
 // Wavefront-wide arguments.
@@ -149,6 +186,12 @@ struct Wave {
     const uint32_t* dims;  // device: [0] slices, [1] paths in the wavefront (sum of the slices' N)
     uint32_t mlv;          // vertex slots per path (largest slice mlv)
     uint32_t nslots;       // camera contribution slots per path (mlv + 2)
+    // tangents (t, w) of the vertices shaded with a tangent frame, beside the light vertices (per
+    // path slot, then compacted in path order) and the camera merging vertices; NULL until a job on
+    // a scene with vertex tangents is admitted
+    float4* vtan;
+    float4* cvtan;
+    float4* cam_tan;
 };
 
 // Element i of an array of float triples; the index is 64-bit because 3 i exceeds 2^32 for the
@@ -195,27 +238,19 @@ __device__ __forceinline__ uint32_t find_slice(const Wave& w, uint32_t g)
 
 __device__ __forceinline__ float cont_prob(const Surface& s) { return saturate_(max_(max_(s.base_color.x, s.base_color.y), s.base_color.z)); }
 
-// Surface of a stored light vertex (the material transforms of CalculateSurfaceParams).
-__device__ __forceinline__ void vertex_surface(const Slice& sl, const LightVertex& v, Surface& s)
+// Surface of a stored light vertex (the material transforms of CalculateSurfaceParams); the frame
+// of its geometry's tangents when it has them (tan: the vertex's stored tangent).
+__device__ __forceinline__ void vertex_surface(const Slice& sl, const LightVertex& v, const float4* tan, Surface& s)
 {
-    const float* mat = sl.materials + 22 * (size_t)sl.geom_material[v.geom_plen >> 5];
+    const uint32_t geom = v.geom_plen >> 5;
+    const float* mat = sl.materials + 22 * (size_t)sl.geom_material[geom];
     s.position = mk3(v.pos[0], v.pos[1], v.pos[2]);
     s.error = 0.0f;
     surface_material(mat, mk3(v.n[0], v.n[1], v.n[2]), mk3(v.wfix[0], v.wfix[1], v.wfix[2]), nullptr, s);
+    if (tan && geom_has_tangents(sl, geom)) tangent_frame(s.n, *tan, s);
     s.base_color = mk3(v.base[0], v.base[1], v.base[2]);
 }
 
-// CalculateSurfaceParams' eUsesPtex branch (SurfaceParameters.cpp:196-203): the filtered texel of
-// the hit's face at its (u, v) when the material asks for it and its geometry has a texture.
-__device__ __forceinline__ bool hit_texel(const Slice& sl, uint32_t geom, const float* mat, uint32_t prim, float u,
-                                          float v, f3& texel)
-{
-    if (!((uint32_t)mat[21] & kMatUsesPtex) || geom >= sl.n_geom_tex) return false;
-    const PtexTex tx = sl.geom_tex[geom];
-    if (!tx.texels) return false;
-    texel = ptex_eval(tx, (int)prim, u, v);
-    return true;
-}
 
 // SampleBsdfScattering (VCM.cpp:412-446) with the Russian-roulette-compensated throughput.
 template <class U>

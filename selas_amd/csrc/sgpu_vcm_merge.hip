@@ -86,8 +86,10 @@ __global__ __launch_bounds__(kBlock) void k_cam_merge(Wave w, const uint32_t* __
         s.error = 0.0f;
         float ccp = 0.0f;
         if (active) {
-            surface_material(sl.materials + 22 * (size_t)sl.geom_material[v.geom_plen >> 5],
-                             mk3(v.n[0], v.n[1], v.n[2]), view, nullptr, s);
+            const uint32_t geom = v.geom_plen >> 5;
+            surface_material(sl.materials + 22 * (size_t)sl.geom_material[geom], mk3(v.n[0], v.n[1], v.n[2]), view,
+                             nullptr, s);
+            if (geom_has_tangents(sl, geom)) tangent_frame(s.n, w.cam_tan[j], s);
             s.base_color = mk3(v.base[0], v.base[1], v.base[2]);
             ccp = cont_prob(s);
         }
@@ -100,6 +102,8 @@ __global__ __launch_bounds__(kBlock) void k_cam_merge(Wave w, const uint32_t* __
             // the reference's transparency rule (VCM.cpp:381-383)
             if ((s.flags & kMatTransparent) != ((uint32_t)lmat[21] & kMatTransparent)) return;
             f3 lw = mk3(lv.wfix[0], lv.wfix[1], lv.wfix[2]);
+            // a photon arriving in the surface's plane is not evaluated (measure zero; 0/0 below)
+            if (dot3(s.n, lw) == 0.0f) return;
             float fwd, rev;
             f3 f = bsdf_eval(s, view, lw, fwd, rev);
             if (is_zero3(f)) return;

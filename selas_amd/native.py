@@ -74,6 +74,7 @@ _SIGNATURES = {
     "sgpu_scene_add_triangles": [_P, _FP, _U32, _U32P, _U32, _FP, _U32P],
     "sgpu_scene_add_quads": [_P, _FP, _U32, _U32P, _U32, _FP, _U32P],
     "sgpu_scene_set_ptex": [_P, _U32, _U32, _U32, _P, _P, _P],
+    "sgpu_scene_set_vertex_attributes": [_P, _U32, _P, _P, _P, _U32],
     "sgpu_ptex_eval": [_P, _U32, _P, _P, _P, _U32, _P, _P],
     "sgpu_scene_add_instance": [_P, _P, _FP, _U32P],
     "sgpu_scene_commit": [_P, _P],
@@ -321,6 +322,16 @@ class Scene:
                                        C.byref(gid)))
         return gid.value
 
+    def set_vertex_attributes(self, geom: int, n_verts: int, normals=None, tangents=None, uvs=None):
+        """Per-vertex normals (n x 3), tangents (n x 4) and UVs (n x 2) of geometry `geom` through
+        sgpu_scene_set_vertex_attributes (before commit)."""
+        arrs = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (normals, tangents, uvs)]
+        for a, k in zip(arrs, (3, 4, 2)):
+            if a is not None and a.shape != (n_verts, k):
+                raise ValueError(f"set_vertex_attributes: expected {n_verts} x {k}, got {a.shape}")
+        check(lib.sgpu_scene_set_vertex_attributes(self.handle, geom, *[None if a is None else a.ctypes.data
+                                                                       for a in arrs], n_verts))
+
     def set_ptex(self, geom: int, res_log2: int, texels: np.ndarray, adjfaces: np.ndarray, adjedges: np.ndarray):
         t = np.ascontiguousarray(texels, np.uint8)
         af = np.ascontiguousarray(adjfaces, np.int32)
@@ -486,10 +497,16 @@ def build_render_scene(ctx: Context, sc: dict) -> Scene:
     scene = Scene(ctx)
     for g in sc["geoms"]:
         if isinstance(g, dict):
-            gid = scene.add_quads(g["positions"], g["quads"])
+            if "triangles" in g:
+                gid = scene.add_triangles(g["positions"], g["triangles"])
+            else:
+                gid = scene.add_quads(g["positions"], g["quads"])
             if g.get("ptex") is not None:
                 p = g["ptex"]
                 scene.set_ptex(gid, p["res_log2"], p["texels"], p["adjfaces"], p["adjedges"])
+            if g.get("normals") is not None:
+                scene.set_vertex_attributes(gid, np.asarray(g["positions"]).reshape(-1, 3).shape[0], g["normals"],
+                                            g.get("tangents"), g.get("uvs"))
         else:
             scene.add_soup(g)
     scene.commit()

@@ -41,9 +41,10 @@ def traversal_rays(tri9: np.ndarray, width: int = 1024, height: int = 1024, spp:
     return {k: np.concatenate([cam[k], bounce[k]]) for k in N.RAY_FIELDS}
 
 
-def icosphere(subdivisions: int, centre, radius):
+def icosphere(subdivisions: int, centre, radius, mesh: bool = False):
     """Icosahedron subdivided `subdivisions` times (20 * 4^n faces), vertices on the sphere,
-    faces wound so that (v1 - v0) x (v2 - v0) points outward. Returns (n, 9) float32."""
+    faces wound so that (v1 - v0) x (v2 - v0) points outward. Returns (n, 9) float32, or with
+    `mesh` the unit-sphere vertices and the faces (vertex indices)."""
     t = (1.0 + 5 ** 0.5) / 2.0
     verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
              (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
@@ -72,9 +73,69 @@ def icosphere(subdivisions: int, centre, radius):
     tri = v[f]  # (n, 3, 3)
     ng = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
     flip = np.sum(ng * tri.mean(axis=1), axis=1) < 0
-    tri[flip] = tri[flip][:, [0, 2, 1]]
-    tri = tri * radius + np.asarray(centre)
+    f[flip] = f[flip][:, [0, 2, 1]]
+    tri = v[f] * radius + np.asarray(centre)
+    if mesh:
+        return v, f
     return tri.reshape(-1, 9).astype(np.float32)
+
+
+def icosphere_mesh(subdivisions: int, centre, radius):
+    """The icosphere as an indexed triangle mesh with per-vertex shading attributes: unit radial
+    normals, tangents (the unit direction of cross(a, n) with a = +y, or +x near the poles; w = +1
+    where x >= 0, else -1) and longitude/latitude UVs."""
+    v, f = icosphere(subdivisions, centre, radius, mesh=True)
+    pos = (v * radius + np.asarray(centre)).astype(np.float32)
+    n = v / np.linalg.norm(v, axis=1, keepdims=True)
+    a = np.where(np.abs(n[:, 1:2]) > 0.99, np.array([[1.0, 0.0, 0.0]]), np.array([[0.0, 1.0, 0.0]]))
+    t = np.cross(a, n)
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    w = np.where(n[:, 0] >= 0.0, 1.0, -1.0)
+    uv = np.stack([np.arctan2(n[:, 2], n[:, 0]) / (2 * np.pi) + 0.5, np.arccos(np.clip(n[:, 1], -1, 1)) / np.pi], 1)
+    return {"positions": pos, "triangles": f.astype(np.uint32), "normals": n.astype(np.float32),
+            "tangents": np.concatenate([t, w[:, None]], 1).astype(np.float32), "uvs": uv.astype(np.float32)}
+
+
+def attributed_panel(nu: int, nv: int, origin, eu, ev, seed: int):
+    """A gently curved quad grid of nu x nv quads spanning origin + [0,1] eu + [0,1] ev (normal
+    eu x ev), with per-vertex normals tilted by a smooth field, tangents along eu (w alternating
+    by column) and UVs: a quad geometry whose shading frame differs from its faces'."""
+    rng = np.random.default_rng(seed)
+    origin, eu, ev = (np.asarray(x, np.float64) for x in (origin, eu, ev))
+    nrm = np.cross(eu, ev)
+    nrm /= np.linalg.norm(nrm)
+    us, vs = np.meshgrid(np.linspace(0, 1, nu + 1), np.linspace(0, 1, nv + 1), indexing="ij")
+    bump = 0.05 * np.sin(np.pi * us) * np.sin(np.pi * vs)
+    pos = origin + us[..., None] * eu + vs[..., None] * ev + bump[..., None] * nrm
+    ph = rng.uniform(0, 2 * np.pi, 2)
+    tilt_u = 0.3 * np.sin(2 * np.pi * us + ph[0])
+    tilt_v = 0.3 * np.cos(2 * np.pi * vs + ph[1])
+    eun = eu / np.linalg.norm(eu)
+    evn = ev / np.linalg.norm(ev)
+    n = nrm + tilt_u[..., None] * eun + tilt_v[..., None] * evn
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    t = np.broadcast_to(eun, n.shape) + 0.2 * tilt_v[..., None] * nrm
+    w = np.where((np.arange(nu + 1)[:, None] + 0 * np.arange(nv + 1)[None, :]) % 2 == 0, 1.0, -1.0)
+    i, j = np.meshgrid(np.arange(nu), np.arange(nv), indexing="ij")
+    i, j = i.reshape(-1), j.reshape(-1)
+    vid = lambda a, b: a * (nv + 1) + b
+    quads = np.stack([vid(i, j), vid(i + 1, j), vid(i + 1, j + 1), vid(i, j + 1)], 1)
+    return {"positions": pos.reshape(-1, 3).astype(np.float32), "quads": quads.astype(np.uint32),
+            "normals": n.reshape(-1, 3).astype(np.float32),
+            "tangents": np.concatenate([t.reshape(-1, 3), w.reshape(-1, 1)], 1).astype(np.float32),
+            "uvs": np.stack([us.reshape(-1), vs.reshape(-1)], 1).astype(np.float32)}
+
+
+def attributed_room_scene(seed: int = 1):
+    """room_scene with shading attributes: the icosphere as a smooth mesh (vertex normals, tangents,
+    UVs; anisotropic metallic Disney) and a curved quad panel on the left wall with tilted vertex
+    normals and tangents (anisotropic Disney, dielectric)."""
+    sc = room_scene(seed)
+    geoms = sc["geoms"][:-1] + [icosphere_mesh(3, (0.5, 0.3, 0.55), 0.25),
+                                attributed_panel(6, 5, (0.02, 0.15, 0.3), (0.0, 0.6, 0.0), (0.0, 0.0, 0.5), seed)]
+    mats = list(sc["materials"][:-1]) + [material(1, (0.8, 0.7, 0.6), 1.0, 0.35, anisotropic=0.7),
+                                         material(0, (0.3, 0.6, 0.8), 0.0, 0.4, anisotropic=0.5, sheen=0.3)]
+    return dict(sc, geoms=geoms, materials=np.stack(mats), geom_material=np.arange(len(geoms), dtype=np.uint32))
 
 
 def material(shader, base_color, metallic=0.3, roughness=0.5, ior=1.5, **kw):
@@ -256,11 +317,15 @@ def grid_ptex(rng, nu, nv, res_log2, tint, lattice=(32, 16)):
 
 
 def geom_tri9(g):
-    """Triangles (n, 9) of a geometry entry: a triangle soup as is, a quad geometry split as
-    Embree splits quads, (v0, v1, v3) then (v2, v3, v1) for each quad in order."""
+    """Triangles (n, 9) of a geometry entry: a triangle soup as is, an indexed triangle mesh
+    ({"positions", "triangles"}) in order, a quad geometry ({"positions", "quads"}) split as Embree
+    splits quads, (v0, v1, v3) then (v2, v3, v1) for each quad in order."""
     if isinstance(g, np.ndarray):
         return g
     p = np.asarray(g["positions"], np.float32)
+    if "triangles" in g:
+        t = np.asarray(g["triangles"], np.int64)
+        return np.ascontiguousarray(p[t].reshape(-1, 9), np.float32)
     q = np.asarray(g["quads"], np.int64)
     ta = np.stack([p[q[:, 0]], p[q[:, 1]], p[q[:, 3]]], 1)
     tb = np.stack([p[q[:, 2]], p[q[:, 3]], p[q[:, 1]]], 1)

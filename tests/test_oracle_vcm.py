@@ -20,6 +20,7 @@ import pytest
 
 import oracle as O
 import scenes
+from selas_amd import workloads as W
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
 
@@ -167,3 +168,17 @@ def test_vcm_without_lights_is_black():
     fb, st = O.render_vcm(rs, sc["camera"], 16, 16, 2, 10, 0.005, 0.75, 1)
     assert not fb.any()
     assert st["light_vertices_per_iteration"] == 0
+
+
+def test_grazing_connection_stays_finite():
+    """Job 3 of the batched mix (seed 7), iteration 7: a camera vertex connects to a light vertex
+    along a direction lying exactly in both surfaces' planes (both cosines 0). Disney's terms are
+    0/0 there; the connection, which carries nothing, is skipped and the image stays finite."""
+    mix = W.job_mix(7, 64)
+    m = mix[3]
+    spec = W.sized_vcm_scene(10003, m["scene_tris"])
+    rs = O.RenderScene(spec)
+    img, _ = O.render_vcm(rs, spec["camera"], m["width"], m["height"], 1, m["max_path_length"], spec["merge_radius"],
+                          spec["radius_alpha"], m["seed"], iteration_base=7)
+    assert np.isfinite(img).all()
+    assert img[37, 36].max() > 0
