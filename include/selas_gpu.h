@@ -116,7 +116,17 @@ typedef struct sgpu_tuning {
     uint32_t vcm_graphs;          /* 1 (default): each VCM wavefront replays its light and camera
                                      passes as captured hipGraphs (queue counts read on the device);
                                      0: launches one by one, sized from counts read back per bounce */
+    uint32_t vcm_order;           /* work order inside a VCM bounce, SGPU_ORDER_* bits (default 0:
+                                     the order the queues were filled in); images do not depend on it */
 } sgpu_tuning;
+/* vcm_order bits. SGPU_ORDER_MERGE_CELLS: the vertex-merging range searches of a bounce run in
+ * spatial order (binned by a 12-bit Morton code of the query point in its photon grid's box, by a
+ * device counting sort), each XCD taking a contiguous run of the bins. SGPU_ORDER_SHADE_MATERIAL:
+ * the hits of a bounce are shaded grouped by material (the same counting sort keyed by the hit
+ * geometry's material), the sort-by-material of PathTracingBatcher::GetSortedHits
+ * (Source/Core/Shading/PathTracingBatcher.cpp:221-258). */
+#define SGPU_ORDER_MERGE_CELLS 1u
+#define SGPU_ORDER_SHADE_MATERIAL 2u
 int sgpu_set_tuning(sgpu_context ctx, const sgpu_tuning* tuning);
 int sgpu_get_tuning(sgpu_context ctx, sgpu_tuning* tuning);
 

@@ -91,11 +91,14 @@ int sgpu_set_tuning(sgpu_context ctx, const sgpu_tuning* t)
     if (t->trace_tree > 1) return sgpu_fail(SGPU_E_RANGE, "sgpu_set_tuning: trace_tree must be 0 or 1");
     if (t->trace_lds_nodes > 9) return sgpu_fail(SGPU_E_RANGE, "sgpu_set_tuning: trace_lds_nodes outside 0..9");
     if (t->vcm_graphs > 1) return sgpu_fail(SGPU_E_RANGE, "sgpu_set_tuning: vcm_graphs must be 0 or 1");
+    if (t->vcm_order & ~(SGPU_ORDER_MERGE_CELLS | SGPU_ORDER_SHADE_MATERIAL))
+        return sgpu_fail(SGPU_E_RANGE, "sgpu_set_tuning: unknown vcm_order bits");
     ctx->vcm_wavefront_paths = t->vcm_wavefront_paths;
     ctx->trace_blocks_per_cu = t->trace_blocks_per_cu;
     ctx->trace_tree = t->trace_tree;
     ctx->trace_lds_nodes = t->trace_lds_nodes;
     ctx->vcm_graphs = t->vcm_graphs;
+    ctx->vcm_order = t->vcm_order;
     return SGPU_OK;
 }
 
@@ -107,6 +110,7 @@ int sgpu_get_tuning(sgpu_context ctx, sgpu_tuning* t)
     t->trace_tree = ctx->trace_tree;
     t->trace_lds_nodes = ctx->trace_lds_nodes;
     t->vcm_graphs = ctx->vcm_graphs;
+    t->vcm_order = ctx->vcm_order;
     return SGPU_OK;
 }
 

@@ -84,6 +84,7 @@ struct sgpu_context_s {
     uint32_t trace_tree = 1;  // 0: binary nodes, 1: 8-wide compressed nodes
     uint32_t trace_lds_nodes = 0;  // top wide nodes read from LDS (single-scene launches)
     uint32_t vcm_graphs = 1;       // VCM passes replayed as captured graphs
+    uint32_t vcm_order = 0;        // SGPU_ORDER_* bits: work order inside a VCM bounce
     // the VCM server of the last sgpu_render_vcm(_batch) call, reused while large enough
     std::mutex vcm_mu;
     sgpu_vcm_server vcm_cache = nullptr;

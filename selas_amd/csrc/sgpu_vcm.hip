@@ -151,17 +151,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                                                         uint32_t* __restrict vcount, RayBuf next_rays,
                                                         uint32_t* __restrict next_queue, RayBuf shadow,
                                                         float* __restrict splat_c, uint32_t* __restrict splat_pix,
-                                                        uint32_t* __restrict next_count, uint32_t* __restrict shadow_count)
+                                                        uint32_t* __restrict next_count, uint32_t* __restrict shadow_count,
+                                                        const uint32_t* __restrict order)
 {
     const uint32_t count = *count_ptr;
     // blocks past the end leave at once (captured launches are sized for capacity)
     if (blockIdx.x * kBlock >= count) return;
-    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    // queue entry of this lane: in queue order, or (order != NULL) in the order of a bin sort
+    const uint32_t jj = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t j = order && jj < count ? order[jj] : jj;
     bool cont = false, emit = false;
     uint32_t p = 0, pix = 0, scn = 0;
     f3 next_o = mk3(0, 0, 0), next_d = mk3(0, 0, 0), sh_o = mk3(0, 0, 0), sh_d = mk3(0, 0, 0), c = mk3(0, 0, 0);
     float sh_tnear = 0.0f, sh_tfar = 0.0f;
-    if (j < count) {
+    if (jj < count) {
         p = queue[j];
         const Slice& sl = w.slices[ps.slice[p]];
         scn = sl.scene;
@@ -485,18 +488,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
                                                       CamVertex* __restrict cam_vm, RayBuf next_rays,
                                                       uint32_t* __restrict next_queue, RayBuf shadow,
                                                       uint32_t* __restrict next_count,
-                                                      uint32_t* __restrict shadow_count)
+                                                      uint32_t* __restrict shadow_count,
+                                                      const uint32_t* __restrict order, uint32_t* __restrict merge_key)
 {
     const uint32_t nslots = w.nslots;
     const uint32_t count = *count_ptr;
     // blocks past the end leave at once (captured launches are sized for capacity)
     if (blockIdx.x * kBlock >= count) return;
-    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    // queue entry of this lane: in queue order, or (order != NULL) in the order of a bin sort
+    const uint32_t jj = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t j = order && jj < count ? order[jj] : jj;
     bool cont = false;
     uint32_t p = 0, si = 0;
     f3 next_o = mk3(0, 0, 0), next_d = mk3(0, 0, 0);
     // shadow rays of this lane, emitted below with wave-level appends (one per slot index)
-    bool active = j < count;
+    bool active = jj < count;
     Surface s;
     f3 x = mk3(0, 0, 0), view = mk3(0, 0, 0);
     bool hit = false;
@@ -736,6 +742,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             v.pad = 0.0f;
         }
         cam_vm[j] = v;
+        // the query's bin for the spatial order of the merging (none for a vertex without a query)
+        if (merge_key) merge_key[j] = hit && *G.nv_ptr != 0 ? merge_bin(x, G.box + 8 * si) : kNoBin;
     }
     // ---- continuation ----
     if (hit) {
@@ -817,6 +825,111 @@ __global__ __launch_bounds__(kBlock) void k_scale(float* __restrict fb, size_t n
     if (i < n) fb[i] = fb[i] * s;
 }
 
+// ---- work order: counting sort of a bounce's queue entries by key (BinBufs) ---------------------------
+
+// Shading key of queue entry j: the material of its hit's geometry, spread over the bins per scene of
+// the wavefront; misses share the last bin.
+__device__ __forceinline__ uint32_t material_bin(const Wave& w, const uint32_t* queue, const HitBuf& hits,
+                                                 const PathState& ps, uint32_t j)
+{
+    const Slice& sl = w.slices[ps.slice[queue[j]]];
+    uint32_t geom;
+    f3 ng;
+    const float* l2w;
+    hit_geometry(sl, hits, j, geom, ng, l2w);
+    if (geom == SGPU_INVALID_ID || geom >= sl.n_geoms) return kBins - 1;
+    return (sl.geom_material[geom] + 97u * sl.scene) % (kBins - 1);
+}
+
+// Bin totals of the entries 0 .. *count_ptr - 1: one LDS histogram per block of kBlock x kBinItems
+// entries, then one atomic per non-empty bin. MATERIAL: the keys are computed here (material_bin)
+// and stored; otherwise they are read.
+template <bool MATERIAL>
+__global__ __launch_bounds__(kBlock) void k_bin_count(Wave w, const uint32_t* __restrict count_ptr,
+                                                      const uint32_t* __restrict queue, HitBuf hits, PathState ps,
+                                                      uint32_t* __restrict key, uint32_t* __restrict total)
+{
+    __shared__ uint32_t hist[kBins];
+    const uint32_t count = *count_ptr;
+    const uint32_t first = blockIdx.x * kBlock * kBinItems;
+    if (first >= count) return;
+    for (uint32_t b = threadIdx.x; b < kBins; b += kBlock) hist[b] = 0;
+    __syncthreads();
+    for (uint32_t k = 0; k < kBinItems; ++k) {
+        const uint32_t j = first + k * kBlock + threadIdx.x;
+        if (j >= count) break;
+        uint32_t kb;
+        if (MATERIAL) {
+            kb = material_bin(w, queue, hits, ps, j);
+            key[j] = kb;
+        } else {
+            kb = key[j];
+        }
+        if (kb != kNoBin) atomicAdd(&hist[kb], 1u);
+    }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < kBins; b += kBlock)
+        if (hist[b]) atomicAdd(&total[b], hist[b]);
+}
+
+// Exclusive scan of the bin totals into the cursors (one block of 1024 lanes, 4 bins each); the
+// totals are cleared for the next sort and the entry count is written to *n.
+constexpr uint32_t kScanLanes = 1024;
+static_assert(kBins == 4 * kScanLanes, "k_bin_scan covers kBins with 4 bins per lane");
+__global__ __launch_bounds__(kScanLanes) void k_bin_scan(uint32_t* __restrict total, uint32_t* __restrict cursor,
+                                                         uint32_t* __restrict n)
+{
+    __shared__ uint32_t wave_sum[kScanLanes / 64];
+    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    uint32_t v[4], sum = 0;
+    for (int k = 0; k < 4; ++k) {
+        v[k] = total[4 * t + k];
+        total[4 * t + k] = 0;
+        sum += v[k];
+    }
+    uint32_t incl = sum;
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)incl, d, 64);
+        if (lane >= d) incl += y;
+    }
+    if (lane == 63) wave_sum[wave] = incl;
+    __syncthreads();
+    uint32_t base = 0;
+    for (uint32_t q = 0; q < wave; ++q) base += wave_sum[q];
+    uint32_t run = base + incl - sum;
+    for (int k = 0; k < 4; ++k) {
+        cursor[4 * t + k] = run;
+        run += v[k];
+    }
+    if (t == kScanLanes - 1) *n = run;
+}
+
+// order[cursor[key] ...] = the entries, each block reserving its run of every bin with one atomic
+// and placing its entries by their rank within the block (LDS atomics).
+__global__ __launch_bounds__(kBlock) void k_bin_scatter(const uint32_t* __restrict count_ptr,
+                                                        const uint32_t* __restrict key, uint32_t* __restrict cursor,
+                                                        uint32_t* __restrict order)
+{
+    __shared__ uint32_t hist[kBins];
+    const uint32_t count = *count_ptr;
+    const uint32_t first = blockIdx.x * kBlock * kBinItems;
+    if (first >= count) return;
+    for (uint32_t b = threadIdx.x; b < kBins; b += kBlock) hist[b] = 0;
+    __syncthreads();
+    uint32_t kb[kBinItems], rank[kBinItems];
+    for (uint32_t k = 0; k < kBinItems; ++k) {
+        const uint32_t j = first + k * kBlock + threadIdx.x;
+        kb[k] = j < count ? key[j] : kNoBin;
+        rank[k] = kb[k] != kNoBin ? atomicAdd(&hist[kb[k]], 1u) : 0u;
+    }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < kBins; b += kBlock)
+        if (hist[b]) hist[b] = atomicAdd(&cursor[b], hist[b]);
+    __syncthreads();
+    for (uint32_t k = 0; k < kBinItems; ++k)
+        if (kb[k] != kNoBin) order[hist[kb[k]] + rank[k]] = first + k * kBlock + threadIdx.x;
+}
+
 // Stream-ordered device allocations freed together; a dry-run arena only adds up their sizes.
 struct Arena {
     std::vector<void*> ptrs;
@@ -884,11 +997,11 @@ struct GraphKey {
     const void* table = nullptr;
     const void* spill = nullptr;
     const void* tan = nullptr;
-    uint32_t max_stack = 0, tree = 0, blocks = 0, lds = 0, inst = 0;
+    uint32_t max_stack = 0, tree = 0, blocks = 0, lds = 0, inst = 0, order = 0;
     bool operator==(const GraphKey& o) const
     {
         return table == o.table && spill == o.spill && tan == o.tan && max_stack == o.max_stack && tree == o.tree &&
-               blocks == o.blocks && lds == o.lds && inst == o.inst;
+               blocks == o.blocks && lds == o.lds && inst == o.inst && order == o.order;
     }
 };
 
@@ -927,6 +1040,7 @@ struct sgpu_vcm_server_s {
     bool graph_failed = false;
     Slice* d_slices = nullptr;
     CamVertex* cam_vm = nullptr;
+    BinBufs bins;  // the work-order sort (tuning vcm_order)
     uint32_t* spill = nullptr;
     uint32_t spill_cap = 0;           // stack entries the spill buffer is sized for
     uint32_t* spill_grown = nullptr;  // a larger spill buffer allocated at submit (freed at destroy)
@@ -993,6 +1107,7 @@ int server_graphs(sgpu_vcm_server sv, F&& pass_fn)
     key.blocks = ctx->trace_blocks_per_cu;
     key.lds = ctx->trace_lds_nodes;
     key.inst = sv->any_instanced ? 1u : 0u;
+    key.order = ctx->vcm_order;
     if (sv->g_pass[0] && sv->g_pass[1] && key == sv->gkey) return SGPU_OK;
     for (int k = 0; k < 2; ++k) {
         if (sv->g_pass[k]) (void)hipGraphExecDestroy(sv->g_pass[k]);
@@ -1081,6 +1196,11 @@ int server_alloc(sgpu_vcm_server sv)
     sv->frame = A.get<float>(3 * (size_t)TM);
     sv->d_slices = A.get<Slice>(sv->max_slices);
     sv->cam_vm = A.get<CamVertex>(TM);
+    sv->bins.key = A.get<uint32_t>(TM);
+    sv->bins.order = A.get<uint32_t>(TM);
+    sv->bins.total = A.get<uint32_t>(kBins);
+    sv->bins.cursor = A.get<uint32_t>(kBins);
+    sv->bins.n = A.get<uint32_t>(4);
     sv->d_trav = A.get<uint64_t>(6);  // [0..1] traversal counters, [2..5] merging counters
     sv->S.c = A.get<float>(3 * (size_t)TM * nslots);
     sv->S.valid = A.get<uint32_t>((size_t)TM * nslots);
@@ -1441,7 +1561,9 @@ int sgpu_vcm_server_step_impl(sgpu_vcm_server sv, hipStream_t stream, uint64_t* 
     fill_u32(sv->cnt, 0u, sv->cnt_words, stream);
     fill_u32((uint32_t*)sv->frame, 0u, 3 * (size_t)total, stream);
     fill_u32(sv->vcount, 0u, (size_t)total + 1, stream);
+    fill_u32(sv->bins.total, 0u, kBins, stream);
     SGPU_HIP_TRY(hipGetLastError());
+    const uint32_t vorder = sv->ctx->vcm_order;
 
     // One pass (0: light subpaths, 1: camera subpaths) on stream `st`. Eager: the host reads each
     // bounce's counts to size its launches and stops at the first empty bounce. Captured: every
@@ -1464,6 +1586,23 @@ int sgpu_vcm_server_step_impl(sgpu_vcm_server sv, hipStream_t stream, uint64_t* 
                                                any, d_st, st, sv->spill, n_ptr, sv->any_instanced, comp);
             return sgpu_trace_launch(first_scene, &rs, hits, occ, n, any, d_st, st, sv->spill, n_ptr, sv->ovf, comp);
         };
+        // the entries 0 .. *c - 1 of this bounce into bins.order by key (material keys computed from
+        // the hits, else read from bins.key); n bounds the count (the capacity for captured launches)
+        auto bin_sort = [&](const uint32_t* c, uint32_t n, bool material, const uint32_t* q) {
+            const uint32_t g = std::max<uint32_t>(1, (uint32_t)(((uint64_t)n + kBlock * kBinItems - 1) /
+                                                                (kBlock * kBinItems)));
+            if (material)
+                hipLaunchKernelGGL(k_bin_count<true>, dim3(g), dim3(kBlock), 0, st, wv, c, q, hb, ps, sv->bins.key,
+                                   sv->bins.total);
+            else
+                hipLaunchKernelGGL(k_bin_count<false>, dim3(g), dim3(kBlock), 0, st, wv, c, q, hb, ps, sv->bins.key,
+                                   sv->bins.total);
+            hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(kScanLanes), 0, st, sv->bins.total, sv->bins.cursor, sv->bins.n);
+            hipLaunchKernelGGL(k_bin_scatter, dim3(g), dim3(kBlock), 0, st, c, sv->bins.key, sv->bins.cursor,
+                               sv->bins.order);
+        };
+        const bool shade_sorted = (vorder & SGPU_ORDER_SHADE_MATERIAL) != 0;
+        const bool merge_sorted = (vorder & SGPU_ORDER_MERGE_CELLS) != 0;
         const int ph_pass = pass == 0 ? kPhaseLight : kPhaseCamera;
         SGPU_HIP_TRY(T.begin(ph_pass));
         if (pass == 0)
@@ -1491,19 +1630,26 @@ int sgpu_vcm_server_step_impl(sgpu_vcm_server sv, hipStream_t stream, uint64_t* 
             if (rc2) return rc2;
             SGPU_HIP_TRY(T.end());
             SGPU_HIP_TRY(T.begin(ph_pass));
+            if (shade_sorted) bin_sort(c_in, n, true, queue[cur]);
+            const uint32_t* shade_order = shade_sorted ? sv->bins.order : nullptr;
             if (pass == 0)
                 hipLaunchKernelGGL(k_light_shade, dim3(grid1(n)), dim3(kBlock), 0, st, wv, c_in, queue[cur], rays[cur],
                                    hb, ps, sv->verts, sv->vcount, rays[1 - cur], queue[1 - cur], shadow, sv->splat_c,
-                                   sv->splat_pix, c_next, c_shadow);
+                                   sv->splat_pix, c_next, c_shadow, shade_order);
             else
                 hipLaunchKernelGGL(k_cam_shade, dim3(grid1(n)), dim3(kBlock), 0, st, wv, c_in, queue[cur], rays[cur],
-                                   hb, ps, G, S, sv->cam_vm, rays[1 - cur], queue[1 - cur], shadow, c_next, c_shadow);
+                                   hb, ps, G, S, sv->cam_vm, rays[1 - cur], queue[1 - cur], shadow, c_next, c_shadow,
+                                   shade_order, merge_sorted ? sv->bins.key : nullptr);
             SGPU_HIP_TRY(hipGetLastError());
             SGPU_HIP_TRY(T.end());
             if (pass == 1) {
                 SGPU_HIP_TRY(T.begin(kPhaseMerge));
-                const uint32_t merge_blocks = std::min<uint32_t>((n + 2 * kMergeWaves - 1) / (2 * kMergeWaves), 16u * cus);
-                launch_cam_merge(merge_blocks, st, wv, c_in, sv->cam_vm, ps.slice, G, S, d_mstat);
+                if (merge_sorted) bin_sort(c_in, n, false, nullptr);
+                // sorted: a multiple of the XCD count, so that the blocks can be numbered xcd-major
+                uint32_t merge_blocks = std::min<uint32_t>((n + 2 * kMergeWaves - 1) / (2 * kMergeWaves), 16u * cus);
+                if (merge_sorted) merge_blocks = (merge_blocks + kXcds - 1) / kXcds * kXcds;
+                launch_cam_merge(merge_blocks, st, wv, merge_sorted ? sv->bins.n : c_in,
+                                 merge_sorted ? sv->bins.order : nullptr, sv->cam_vm, ps.slice, G, S, d_mstat);
                 SGPU_HIP_TRY(hipGetLastError());
                 SGPU_HIP_TRY(T.end());
             }

@@ -308,12 +308,58 @@ struct CamSlots {
     uint32_t* ray_slot;  // per queued shadow ray: path * nslots + slot
 };
 
+// ---- work order: a device counting sort of a bounce's queue entries by a small key ----------------
+// key[j] in [0, kBins) or kNoBin (entry j left out); the sort writes order[0 .. n) = the entries by
+// key (entries of one key in no particular order) and n to its count word. Per bin: a block
+// histogram in LDS, one global atomic per non-empty bin and block, a one-block scan of the bin
+// totals, and the scatter reserving each block's run of every bin with one atomic.
+constexpr uint32_t kBins = 4096;
+constexpr uint32_t kNoBin = 0xFFFFFFFFu;
+constexpr uint32_t kBinItems = 8;  // entries per thread of the binning kernels
+
+struct BinBufs {
+    uint32_t* key;     // per queue entry
+    uint32_t* order;   // queue entries in key order
+    uint32_t* total;   // per bin; zero between sorts (the scan clears it)
+    uint32_t* cursor;  // per bin: next free position of the bin in `order`
+    uint32_t* n;       // entries in `order`
+};
+
+// Bin of a merge query at x: the Morton code of x's cell in a 16 x 16 x 16 division of the photon
+// grid's box (box: the slice's 8 words, ordered-integer min then complemented max), kNoBin for a
+// point outside the box (the range search skips those: VCMHashGrid.cpp:113-117).
+__device__ __forceinline__ uint32_t merge_bin(f3 x, const uint32_t* box)
+{
+    const float p[3] = {x.x, x.y, x.z};
+    uint32_t code = 0;
+    for (int k = 0; k < 3; ++k) {
+        const float lo = ord2f(box[k]), hi = ord2f(~box[3 + k]);
+        if (p[k] - lo < 0.0f || hi - p[k] < 0.0f) return kNoBin;
+        const float ext = hi - lo;
+        float q = ext > 0.0f ? (p[k] - lo) * (16.0f / ext) : 0.0f;
+        q = fminf(fmaxf(q, 0.0f), 15.0f);
+        const uint32_t c = (uint32_t)q;
+        for (int b = 0; b < 4; ++b) code |= ((c >> b) & 1u) << (3 * b + k);
+    }
+    return code;
+}
+
+// XCDs of the chip: consecutive workgroups go to different XCDs in turn, so a launch that wants
+// neighbouring work on one XCD's L2 numbers its blocks xcd-major.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_major_block(uint32_t b, uint32_t grid)
+{
+    if (grid % kXcds) return b;
+    return (b % kXcds) * (grid / kXcds) + b / kXcds;
+}
+
 // Vertex merging of the camera vertices of one bounce (sgpu_vcm_merge.hip).
 constexpr uint32_t kMergeWaves = kBlock / 64;
+// count_ptr: entries of cam_vm to merge, or (order != NULL) entries of order, which names them.
 // mstat (NULL: not counted): [0] camera vertices read, [1] range searches, [2] photons visited,
 // [3] photons within the radius, summed with atomics.
 void launch_cam_merge(uint32_t blocks, hipStream_t stream, const Wave& w, const uint32_t* count_ptr,
-                      const CamVertex* cam_vm, const uint32_t* path_slice, const GridView& G, const CamSlots& S,
-                      unsigned long long* mstat);
+                      const uint32_t* order, const CamVertex* cam_vm, const uint32_t* path_slice, const GridView& G,
+                      const CamSlots& S, unsigned long long* mstat);
 
 }  // namespace sgpu_vcm

@@ -17,8 +17,11 @@ namespace {
 // 32 listed photons at a time, one per lane, both halves in the same pass: an evaluation pass then
 // serves two queries, and the acceptance test does not make whole waves wait on every BSDF
 // evaluation. Each lane sums its photons in list order; a fixed butterfly over the half's lanes
-// gives the query's sum.
+// gives the query's sum. With `order` (the queries of the bounce in spatial order) blocks are
+// numbered xcd-major, so that the queries running at once on one XCD are neighbours in space and
+// read the same photon cells through that XCD's L2.
 __global__ __launch_bounds__(kBlock) void k_cam_merge(Wave w, const uint32_t* __restrict count_ptr,
+                                                      const uint32_t* __restrict order,
                                                       const CamVertex* __restrict cam_vm,
                                                       const uint32_t* __restrict path_slice, GridView G, CamSlots S,
                                                       unsigned long long* __restrict mstat)
@@ -32,12 +35,14 @@ __global__ __launch_bounds__(kBlock) void k_cam_merge(Wave w, const uint32_t* __
     const unsigned long long half_bits = half ? 0xFFFFFFFF00000000ull : 0x00000000FFFFFFFFull;
     if (*G.nv_ptr == 0) return;
     const uint32_t stride = 2 * gridDim.x * kMergeWaves;
-    for (uint32_t j0 = 2 * (blockIdx.x * kMergeWaves + wid); j0 < count; j0 += stride) {
+    const uint32_t block = order ? xcd_major_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    for (uint32_t j0 = 2 * (block * kMergeWaves + wid); j0 < count; j0 += stride) {
         // ---- this half's query (none past the end, or without a query, or outside the photon box) ----
-        const uint32_t j = j0 + half;
+        const uint32_t jj = j0 + half;
+        const uint32_t j = order && jj < count ? order[jj] : jj;
         CamVertex v;
         v.geom_plen = kNoQuery;
-        if (j < count) v = cam_vm[j];
+        if (jj < count) v = cam_vm[j];
         bool active = v.geom_plen != kNoQuery;
         uint32_t si = 0;
         float dmin[3] = {0.0f, 0.0f, 0.0f};
@@ -153,7 +158,7 @@ __global__ __launch_bounds__(kBlock) void k_cam_merge(Wave w, const uint32_t* __
             acc.y += __shfl_xor(acc.y, off, 32);
             acc.z += __shfl_xor(acc.z, off, 32);
         }
-        if (mstat && hl == 0 && j < count) {
+        if (mstat && hl == 0 && jj < count) {
             atomicAdd(mstat + 0, 1ull);
             if (active) {
                 atomicAdd(mstat + 1, 1ull);
@@ -172,11 +177,11 @@ __global__ __launch_bounds__(kBlock) void k_cam_merge(Wave w, const uint32_t* __
 }  // namespace
 
 void launch_cam_merge(uint32_t blocks, hipStream_t stream, const Wave& w, const uint32_t* count_ptr,
-                      const CamVertex* cam_vm, const uint32_t* path_slice, const GridView& G, const CamSlots& S,
-                      unsigned long long* mstat)
+                      const uint32_t* order, const CamVertex* cam_vm, const uint32_t* path_slice, const GridView& G,
+                      const CamSlots& S, unsigned long long* mstat)
 {
-    hipLaunchKernelGGL(k_cam_merge, dim3(blocks), dim3(kBlock), 0, stream, w, count_ptr, cam_vm, path_slice, G, S,
-                       mstat);
+    hipLaunchKernelGGL(k_cam_merge, dim3(blocks), dim3(kBlock), 0, stream, w, count_ptr, order, cam_vm, path_slice, G,
+                       S, mstat);
 }
 
 }  // namespace sgpu_vcm

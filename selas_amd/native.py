@@ -214,7 +214,7 @@ class DeviceBuffer:
 
 class Tuning(C.Structure):
     _fields_ = [("vcm_wavefront_paths", C.c_uint64), ("trace_blocks_per_cu", C.c_uint32), ("trace_tree", C.c_uint32),
-                ("trace_lds_nodes", C.c_uint32), ("vcm_graphs", C.c_uint32)]
+                ("trace_lds_nodes", C.c_uint32), ("vcm_graphs", C.c_uint32), ("vcm_order", C.c_uint32)]
 
 
 class DeviceInfo(C.Structure):
@@ -238,14 +238,16 @@ class Context:
         t = Tuning()
         check(lib.sgpu_get_tuning(self.handle, C.byref(t)))
         return {"vcm_wavefront_paths": t.vcm_wavefront_paths, "trace_blocks_per_cu": t.trace_blocks_per_cu,
-                "trace_tree": t.trace_tree, "trace_lds_nodes": t.trace_lds_nodes, "vcm_graphs": t.vcm_graphs}
+                "trace_tree": t.trace_tree, "trace_lds_nodes": t.trace_lds_nodes, "vcm_graphs": t.vcm_graphs,
+                "vcm_order": t.vcm_order}
 
     def set_tuning(self, vcm_wavefront_paths=None, trace_blocks_per_cu=None, trace_tree=None, trace_lds_nodes=None,
-                   vcm_graphs=None):
+                   vcm_graphs=None, vcm_order=None):
         cur = self.tuning()
         pick = lambda new, key: cur[key] if new is None else int(new)
         t = Tuning(pick(vcm_wavefront_paths, "vcm_wavefront_paths"), pick(trace_blocks_per_cu, "trace_blocks_per_cu"),
-                   pick(trace_tree, "trace_tree"), pick(trace_lds_nodes, "trace_lds_nodes"), pick(vcm_graphs, "vcm_graphs"))
+                   pick(trace_tree, "trace_tree"), pick(trace_lds_nodes, "trace_lds_nodes"), pick(vcm_graphs, "vcm_graphs"),
+                   pick(vcm_order, "vcm_order"))
         check(lib.sgpu_set_tuning(self.handle, C.byref(t)))
 
     def array(self, a: np.ndarray) -> DeviceBuffer:
@@ -528,6 +530,10 @@ VCM_COUNT_TRAVERSAL = 2
 VCM_ONE_SLICE_PER_WAVEFRONT = 4
 VCM_SHORTEST_FIRST = 8  # servers only
 VCM_COUNT_MERGE = 16
+
+# sgpu_tuning.vcm_order bits
+ORDER_MERGE_CELLS = 1
+ORDER_SHADE_MATERIAL = 2
 
 
 class VcmJob(C.Structure):

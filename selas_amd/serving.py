@@ -204,6 +204,7 @@ class GpuEngine:
         # between engines (the caller then releases them)
         self.own_scenes = scenes is None
         self.scenes = {} if scenes is None else scenes
+        self.known_bytes = {}  # sid -> device bytes measured when it was last resident
         self.jobs = {}  # handle -> (framebuffer, request)
 
     def close(self):
@@ -225,13 +226,18 @@ class GpuEngine:
         self.scenes[sid] = (sc, spec["camera"], float(spec["merge_radius"]), float(spec.get("radius_alpha", 0.75)))
 
     def evict_scene(self, sid):
-        self.scenes.pop(sid)[0].release()
+        sc = self.scenes.pop(sid)[0]
+        self.known_bytes[sid] = sc.device_bytes()
+        sc.release()
 
     def scene_bytes(self, sid) -> int:
         if sid in self.scenes:
             return self.scenes[sid][0].device_bytes()
-        # before it is built: two 64-byte nodes and one 48-byte record per triangle, one 16 x 16
-        # RGB8 texture and 32 bytes of adjacency per quad
+        # built before: what it held then (a scene is built the same way every time)
+        if sid in self.known_bytes:
+            return self.known_bytes[sid]
+        # never built: two 64-byte nodes and one 48-byte record per triangle, one 16 x 16 RGB8
+        # texture and 32 bytes of adjacency per quad
         t = float(self.scene_tris(sid)) if self.scene_tris else 1e6
         return int(t * (2 * 64 + 48) + t / 2 * (768 + 32))
 

@@ -59,3 +59,17 @@ def test_invalid_arguments_are_errors_not_crashes():
     assert b"bad argument" in N.lib.sgpu_last_error()
     rc = N.lib.sgpu_gen_camera_rays(None, None, None, 50.0, 4, 4, 1, *([None] * 8))
     assert rc == N.SGPU_E_INVALID
+
+
+def test_c_caller_compiles_against_the_header(tmp_path):
+    """tests/c_caller/capi_threads.c is plain C11 against include/selas_gpu.h (no HIP headers), warnings
+    as errors, linked with libselas_gpu; it is run on the GPU by tests/test_c_caller_gpu.py."""
+    import subprocess
+    src = os.path.join(ROOT, "tests", "c_caller", "capi_threads.c")
+    out = tmp_path / "capi_threads"
+    r = subprocess.run(["gcc", "-std=c11", "-O2", "-Wall", "-Wextra", "-Werror", "-pedantic",
+                        "-I" + os.path.join(ROOT, "include"), "-o", str(out), src,
+                        "-L" + os.path.dirname(N.LIB_PATH), "-lselas_gpu", "-pthread"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert out.exists()
