@@ -233,6 +233,17 @@ int sgpu_intersect(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_
                    uint32_t n, void* stream);
 int sgpu_occluded(sgpu_scene scene, const sgpu_rays_soa* rays, uint32_t* occluded, uint32_t n,
                   void* stream);
+/* sgpu_intersect without any device allocation. No counterpart in Selas or Embree: for callers that
+ * cannot allocate device memory (code under HIP stream capture, tight serving loops, hosts that
+ * forbid hipMalloc). The traversal stacks' spill area, which sgpu_intersect allocates per call, is
+ * a fixed 64 MiB array in device memory instead, one per GPU (looked up at sgpu_context_create and
+ * held until the process ends); the grid is capped so that its spill fits (rays beyond it are taken
+ * grid-stride). Results equal sgpu_intersect's. Fails with SGPU_E_RANGE when even one block's
+ * stacks would not fit (a scene whose stack bound, sgpu_scene_info's depth x 7, exceeds 65,552
+ * entries). Calls that overlap on one GPU (other streams, other threads) share the array and race:
+ * order them. */
+int sgpu_intersect_noalloc(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits, uint32_t n,
+                           void* stream);
 /* Same query as sgpu_intersect with per-ray node/triangle counters, summed into *stats.
  * Synchronous. Used to state the algorithmic bytes per ray. */
 int sgpu_intersect_stats(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits,

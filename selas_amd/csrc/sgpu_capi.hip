@@ -69,6 +69,11 @@ int sgpu_context_create(int device, sgpu_context* out)
         uint64_t keep = UINT64_MAX;
         (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
     }
+    const int rc = sgpu_noalloc_init(ctx);
+    if (rc) {
+        delete ctx;
+        return rc;
+    }
     *out = ctx;
     return SGPU_OK;
 }
@@ -1076,7 +1081,7 @@ int sgpu_scene_check(sgpu_scene scene)
 
 // ---- queries -------------------------------------------------------------------------------------
 
-static int check_query(sgpu_scene scene, const sgpu_rays_soa* rays, uint32_t n, const char* who)
+extern "C++" int sgpu_check_query(sgpu_scene scene, const sgpu_rays_soa* rays, uint32_t n, const char* who)
 {
     if (!scene || !rays) return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": NULL argument");
     if (!scene->committed) return sgpu_fail(SGPU_E_STATE, std::string(who) + ": scene not committed");
@@ -1088,7 +1093,7 @@ static int check_query(sgpu_scene scene, const sgpu_rays_soa* rays, uint32_t n, 
 
 int sgpu_intersect(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits, uint32_t n, void* stream)
 {
-    int rc = check_query(scene, rays, n, "sgpu_intersect");
+    int rc = sgpu_check_query(scene, rays, n, "sgpu_intersect");
     if (rc) return rc;
     if (!hits) return sgpu_fail(SGPU_E_INVALID, "sgpu_intersect: NULL hits");
     if ((hits->ng_x || hits->ng_y || hits->ng_z) && !(hits->ng_x && hits->ng_y && hits->ng_z))
@@ -1098,7 +1103,7 @@ int sgpu_intersect(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_
 
 int sgpu_occluded(sgpu_scene scene, const sgpu_rays_soa* rays, uint32_t* occluded, uint32_t n, void* stream)
 {
-    int rc = check_query(scene, rays, n, "sgpu_occluded");
+    int rc = sgpu_check_query(scene, rays, n, "sgpu_occluded");
     if (rc) return rc;
     if (n > 0 && !occluded) return sgpu_fail(SGPU_E_INVALID, "sgpu_occluded: NULL output");
     return sgpu_trace_launch(scene, rays, nullptr, occluded, n, true, nullptr, as_stream(stream));
@@ -1107,7 +1112,7 @@ int sgpu_occluded(sgpu_scene scene, const sgpu_rays_soa* rays, uint32_t* occlude
 int sgpu_intersect_stats(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits, uint32_t n,
                          int any_hit, sgpu_trace_stats* stats)
 {
-    int rc = check_query(scene, rays, n, "sgpu_intersect_stats");
+    int rc = sgpu_check_query(scene, rays, n, "sgpu_intersect_stats");
     if (rc) return rc;
     if (!stats) return sgpu_fail(SGPU_E_INVALID, "sgpu_intersect_stats: NULL stats");
     uint64_t* d_stats = nullptr;

@@ -85,6 +85,7 @@ struct sgpu_context_s {
     uint32_t trace_lds_nodes = 0;  // top wide nodes read from LDS (single-scene launches)
     uint32_t vcm_graphs = 1;       // VCM passes replayed as captured graphs
     uint32_t vcm_order = 0;        // SGPU_ORDER_* bits: work order inside a VCM bounce
+    uint32_t* noalloc_scratch = nullptr;  // device address of this device's sgpu_intersect_noalloc scratch
     // the VCM server of the last sgpu_render_vcm(_batch) call, reused while large enough
     std::mutex vcm_mu;
     sgpu_vcm_server vcm_cache = nullptr;
@@ -252,10 +253,16 @@ int sgpu_bvh_build_instances(sgpu_scene scene, hipStream_t stream);
 // concurrent launch. With n_ptr, the kernels read the ray count from that device word (n then only
 // sizes the grid and must bound it); `overflow` replaces the scene's stack-overflow flag.
 // For an instanced scene, comp_out (may be NULL) receives each closest hit's composed instance.
+// max_blocks (0: none) caps the grid below the tuned one; a lent spill buffer then needs only
+// (stack_cap - SGPU_STACK_LDS) x max_blocks x SGPU_TRACE_BLOCK words.
 int sgpu_trace_launch(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits,
                       uint32_t* occluded, uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream,
                       uint32_t* spill = nullptr, const uint32_t* n_ptr = nullptr, uint32_t* overflow = nullptr,
-                      uint32_t* comp_out = nullptr);
+                      uint32_t* comp_out = nullptr, uint32_t max_blocks = 0);
+// Argument checks shared by the query entry points (sgpu_capi.hip).
+int sgpu_check_query(sgpu_scene scene, const sgpu_rays_soa* rays, uint32_t n, const char* who);
+// Looks up this device's no-allocation scratch (sgpu_noalloc.hip) with the context's device current.
+int sgpu_noalloc_init(sgpu_context ctx);
 size_t sgpu_trace_spill_words(sgpu_scene scene);
 size_t sgpu_trace_spill_words(sgpu_context ctx, uint32_t stack_cap);
 // A scene's trees as a multi-scene traversal launch reads them (one table entry per scene).

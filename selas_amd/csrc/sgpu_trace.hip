@@ -730,12 +730,13 @@ int trace_launch(sgpu_context ctx, const BvhNode* nodes, const WideNode* wnodes,
                  uint32_t n_tris, uint32_t stack_cap, uint32_t* overflow, const TraceScene* table,
                  const uint32_t* ray_scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits, uint32_t* occluded,
                  uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream, uint32_t* spill_buf,
-                 const uint32_t* n_ptr, bool instanced = false, uint32_t* comp_out = nullptr)
+                 const uint32_t* n_ptr, bool instanced = false, uint32_t* comp_out = nullptr, uint32_t cap_blocks = 0)
 {
     if (n == 0) return SGPU_OK;
     const uint32_t cus = (uint32_t)ctx->props.multiProcessorCount;
     uint32_t blocks = (n + SGPU_TRACE_BLOCK - 1) / SGPU_TRACE_BLOCK;
-    const uint32_t max_blocks = cus * std::min(ctx->trace_blocks_per_cu, kMaxTraceBlocksPerCu);
+    uint32_t max_blocks = cus * std::min(ctx->trace_blocks_per_cu, kMaxTraceBlocksPerCu);
+    if (cap_blocks && max_blocks > cap_blocks) max_blocks = cap_blocks;
     if (blocks > max_blocks) blocks = max_blocks;
 
     // stream-ordered scratch for the spilled part of the stacks, private to this call, unless the
@@ -804,13 +805,14 @@ namespace {
 // One traversal launch over the rays of an instanced scene (k_trace8i).
 int trace_launch_instanced(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits, uint32_t* occluded,
                            uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream, uint32_t* overflow,
-                           uint32_t* spill_buf, const uint32_t* n_ptr, uint32_t* comp_out)
+                           uint32_t* spill_buf, const uint32_t* n_ptr, uint32_t* comp_out, uint32_t cap_blocks)
 {
     if (n == 0) return SGPU_OK;
     sgpu_context ctx = scene->ctx;
     const uint32_t cus = (uint32_t)ctx->props.multiProcessorCount;
     uint32_t blocks = (n + SGPU_TRACE_BLOCK - 1) / SGPU_TRACE_BLOCK;
-    const uint32_t max_blocks = cus * std::min(ctx->trace_blocks_per_cu, kMaxTraceBlocksPerCu);
+    uint32_t max_blocks = cus * std::min(ctx->trace_blocks_per_cu, kMaxTraceBlocksPerCu);
+    if (cap_blocks && max_blocks > cap_blocks) max_blocks = cap_blocks;
     if (blocks > max_blocks) blocks = max_blocks;
     uint32_t* spill = spill_buf;
     if (!spill)
@@ -841,14 +843,14 @@ int trace_launch_instanced(sgpu_scene scene, const sgpu_rays_soa* rays, const sg
 
 int sgpu_trace_launch(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits, uint32_t* occluded,
                       uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream, uint32_t* spill_buf,
-                      const uint32_t* n_ptr, uint32_t* overflow, uint32_t* comp_out)
+                      const uint32_t* n_ptr, uint32_t* overflow, uint32_t* comp_out, uint32_t max_blocks)
 {
     if (!scene->instances.empty())
         return trace_launch_instanced(scene, rays, hits, occluded, n, any_hit, d_stats, stream,
-                                      overflow ? overflow : scene->d_overflow, spill_buf, n_ptr, comp_out);
+                                      overflow ? overflow : scene->d_overflow, spill_buf, n_ptr, comp_out, max_blocks);
     return trace_launch(scene->ctx, scene->d_nodes, scene->d_wide, scene->n_wide, scene->d_tris, scene->n_tris,
                         scene->stack_cap, overflow ? overflow : scene->d_overflow, nullptr, nullptr, rays, hits,
-                        occluded, n, any_hit, d_stats, stream, spill_buf, n_ptr);
+                        occluded, n, any_hit, d_stats, stream, spill_buf, n_ptr, false, nullptr, max_blocks);
 }
 
 int sgpu_trace_launch_multi(sgpu_context ctx, const TraceScene* table, const uint32_t* ray_scene, uint32_t stack_cap,

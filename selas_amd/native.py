@@ -83,6 +83,7 @@ _SIGNATURES = {
     "sgpu_scene_device_bytes": [_P, C.POINTER(C.c_uint64)],
     "sgpu_scene_check": [_P],
     "sgpu_intersect": [_P, C.POINTER(RaysSoA), C.POINTER(HitsSoA), _U32, _P],
+    "sgpu_intersect_noalloc": [_P, C.POINTER(RaysSoA), C.POINTER(HitsSoA), _U32, _P],
     "sgpu_occluded": [_P, C.POINTER(RaysSoA), _P, _U32, _P],
     "sgpu_intersect_stats": [_P, C.POINTER(RaysSoA), C.POINTER(HitsSoA), _U32, C.c_int, C.POINTER(TraceStats)],
     "sgpu_scene_set_materials": [_P, _FP, _U32, _U32P, _U32],
@@ -401,6 +402,9 @@ class Scene:
 
     def intersect(self, rays: DeviceRays, hits: DeviceHits, stream=None):
         check(lib.sgpu_intersect(self.handle, C.byref(rays.soa), C.byref(hits.soa), rays.n, stream))
+
+    def intersect_noalloc(self, rays: DeviceRays, hits: DeviceHits, stream=None):
+        check(lib.sgpu_intersect_noalloc(self.handle, C.byref(rays.soa), C.byref(hits.soa), rays.n, stream))
 
     def occluded(self, rays: DeviceRays, out: DeviceBuffer, stream=None):
         check(lib.sgpu_occluded(self.handle, C.byref(rays.soa), out.ptr, rays.n, stream))
