@@ -343,6 +343,10 @@ def apply_tuning(args, ctx, probe, workload=None):
 
     if args.trace_tree is not None:
         ctx.set_tuning(trace_tree=args.trace_tree)
+    if args.order is not None:
+        ctx.set_tuning(vcm_order=args.order)
+    if args.merge_lanes is not None:
+        ctx.set_tuning(vcm_merge_lanes=args.merge_lanes)
     if args.wavefront_paths or args.trace_blocks:
         ctx.set_tuning(vcm_wavefront_paths=args.wavefront_paths, trace_blocks_per_cu=args.trace_blocks)
         return dict(ctx.tuning(), source="command line")
@@ -353,7 +357,13 @@ def apply_tuning(args, ctx, probe, workload=None):
     stored = None if (args.retune or workload is None) else A.load(TUNING_FILE, key)
     if stored:
         ctx.set_tuning(**{k: v for k, v in stored.items() if k in ctx.tuning()})
-        return dict(ctx.tuning(), source=f"stored autotuner result ({os.path.basename(TUNING_FILE)}: {key})")
+        if args.order is not None:
+            ctx.set_tuning(vcm_order=args.order)
+        if args.merge_lanes is not None:
+            ctx.set_tuning(vcm_merge_lanes=args.merge_lanes)
+        given = [k for k in ("order", "merge_lanes") if getattr(args, k) is not None]
+        return dict(ctx.tuning(), source=f"stored autotuner result ({os.path.basename(TUNING_FILE)}: {key})"
+                    + (f", {' and '.join(given)} from the command line" if given else ""))
     t0 = time.perf_counter()
     r = A.tune(ctx, probe, log=log)
     return dict(r["best"], source="autotuned", tuned_ms=round(r["seconds"] * 1e3, 1),
@@ -677,6 +687,10 @@ def main():
                     help="vcm: time the render without per-launch events (lets it replay captured graphs)")
     ap.add_argument("--wavefront-paths", type=int, default=None, help="VCM paths per wavefront (skips autotuning)")
     ap.add_argument("--trace-blocks", type=int, default=None, help="traversal blocks per CU (skips autotuning)")
+    ap.add_argument("--order", type=int, default=None, choices=(0, 1, 2, 3),
+                    help="VCM work order (sgpu_tuning.vcm_order bits: 1 merge in spatial bins, 2 shade by material)")
+    ap.add_argument("--merge-lanes", type=int, default=None, choices=(8, 16, 32),
+                    help="VCM lanes per vertex-merging range search (sgpu_tuning.vcm_merge_lanes)")
     ap.add_argument("--trace-tree", type=int, default=None, choices=(0, 1),
                     help="traversal tree: 0 binary, 1 8-wide compressed (default: the context's)")
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the triangles (1.0 = the workload)")

@@ -118,6 +118,7 @@ typedef struct sgpu_tuning {
                                      0: launches one by one, sized from counts read back per bounce */
     uint32_t vcm_order;           /* work order inside a VCM bounce, SGPU_ORDER_* bits (default 0:
                                      the order the queues were filled in); images do not depend on it */
+    uint32_t vcm_merge_lanes;     /* lanes per vertex-merging range search: 8, 16 (default) or 32 */
 } sgpu_tuning;
 /* vcm_order bits. SGPU_ORDER_MERGE_CELLS: the vertex-merging range searches of a bounce run in
  * spatial order (binned by a 12-bit Morton code of the query point in its photon grid's box, by a

@@ -98,12 +98,15 @@ int sgpu_set_tuning(sgpu_context ctx, const sgpu_tuning* t)
     if (t->vcm_graphs > 1) return sgpu_fail(SGPU_E_RANGE, "sgpu_set_tuning: vcm_graphs must be 0 or 1");
     if (t->vcm_order & ~(SGPU_ORDER_MERGE_CELLS | SGPU_ORDER_SHADE_MATERIAL))
         return sgpu_fail(SGPU_E_RANGE, "sgpu_set_tuning: unknown vcm_order bits");
+    if (t->vcm_merge_lanes != 8 && t->vcm_merge_lanes != 16 && t->vcm_merge_lanes != 32)
+        return sgpu_fail(SGPU_E_RANGE, "sgpu_set_tuning: vcm_merge_lanes must be 8, 16 or 32");
     ctx->vcm_wavefront_paths = t->vcm_wavefront_paths;
     ctx->trace_blocks_per_cu = t->trace_blocks_per_cu;
     ctx->trace_tree = t->trace_tree;
     ctx->trace_lds_nodes = t->trace_lds_nodes;
     ctx->vcm_graphs = t->vcm_graphs;
     ctx->vcm_order = t->vcm_order;
+    ctx->vcm_merge_lanes = t->vcm_merge_lanes;
     return SGPU_OK;
 }
 
@@ -116,6 +119,7 @@ int sgpu_get_tuning(sgpu_context ctx, sgpu_tuning* t)
     t->trace_lds_nodes = ctx->trace_lds_nodes;
     t->vcm_graphs = ctx->vcm_graphs;
     t->vcm_order = ctx->vcm_order;
+    t->vcm_merge_lanes = ctx->vcm_merge_lanes;
     return SGPU_OK;
 }
 

@@ -85,6 +85,7 @@ struct sgpu_context_s {
     uint32_t trace_lds_nodes = 0;  // top wide nodes read from LDS (single-scene launches)
     uint32_t vcm_graphs = 1;       // VCM passes replayed as captured graphs
     uint32_t vcm_order = 0;        // SGPU_ORDER_* bits: work order inside a VCM bounce
+    uint32_t vcm_merge_lanes = 16;  // lanes per vertex-merging range search (8, 16 or 32)
     uint32_t* noalloc_scratch = nullptr;  // device address of this device's sgpu_intersect_noalloc scratch
     // the VCM server of the last sgpu_render_vcm(_batch) call, reused while large enough
     std::mutex vcm_mu;

@@ -1107,7 +1107,7 @@ int server_graphs(sgpu_vcm_server sv, F&& pass_fn)
     key.blocks = ctx->trace_blocks_per_cu;
     key.lds = ctx->trace_lds_nodes;
     key.inst = sv->any_instanced ? 1u : 0u;
-    key.order = ctx->vcm_order;
+    key.order = ctx->vcm_order | (ctx->vcm_merge_lanes << 8);
     if (sv->g_pass[0] && sv->g_pass[1] && key == sv->gkey) return SGPU_OK;
     for (int k = 0; k < 2; ++k) {
         if (sv->g_pass[k]) (void)hipGraphExecDestroy(sv->g_pass[k]);
@@ -1646,9 +1646,11 @@ int sgpu_vcm_server_step_impl(sgpu_vcm_server sv, hipStream_t stream, uint64_t* 
                 SGPU_HIP_TRY(T.begin(kPhaseMerge));
                 if (merge_sorted) bin_sort(c_in, n, false, nullptr);
                 // sorted: a multiple of the XCD count, so that the blocks can be numbered xcd-major
-                uint32_t merge_blocks = std::min<uint32_t>((n + 2 * kMergeWaves - 1) / (2 * kMergeWaves), 16u * cus);
+                const uint32_t mlanes = sv->ctx->vcm_merge_lanes;
+                const uint32_t per_block = (64 / mlanes) * kMergeWaves;  // queries per block and stride
+                uint32_t merge_blocks = std::min<uint32_t>((n + per_block - 1) / per_block, 16u * cus);
                 if (merge_sorted) merge_blocks = (merge_blocks + kXcds - 1) / kXcds * kXcds;
-                launch_cam_merge(merge_blocks, st, wv, merge_sorted ? sv->bins.n : c_in,
+                launch_cam_merge(mlanes, merge_blocks, st, wv, merge_sorted ? sv->bins.n : c_in,
                                  merge_sorted ? sv->bins.order : nullptr, sv->cam_vm, ps.slice, G, S, d_mstat);
                 SGPU_HIP_TRY(hipGetLastError());
                 SGPU_HIP_TRY(T.end());

@@ -355,10 +355,11 @@ __device__ __forceinline__ uint32_t xcd_major_block(uint32_t b, uint32_t grid)
 
 // Vertex merging of the camera vertices of one bounce (sgpu_vcm_merge.hip).
 constexpr uint32_t kMergeWaves = kBlock / 64;
-// count_ptr: entries of cam_vm to merge, or (order != NULL) entries of order, which names them.
+// lanes: lanes per range search, 8, 16 or 32 (tuning vcm_merge_lanes). count_ptr: entries of cam_vm
+// to merge, or (order != NULL) entries of order, which names them. blocks: a grid-stride cap.
 // mstat (NULL: not counted): [0] camera vertices read, [1] range searches, [2] photons visited,
 // [3] photons within the radius, summed with atomics.
-void launch_cam_merge(uint32_t blocks, hipStream_t stream, const Wave& w, const uint32_t* count_ptr,
+void launch_cam_merge(uint32_t lanes, uint32_t blocks, hipStream_t stream, const Wave& w, const uint32_t* count_ptr,
                       const uint32_t* order, const CamVertex* cam_vm, const uint32_t* path_slice, const GridView& G,
                       const CamSlots& S, unsigned long long* mstat);
 

@@ -13,13 +13,14 @@ def test_tuning_roundtrip_and_limits(gpu_ctx):
     old = gpu_ctx.tuning()
     try:
         gpu_ctx.set_tuning(vcm_wavefront_paths=1 << 14, trace_blocks_per_cu=3, trace_tree=0, trace_lds_nodes=9,
-                           vcm_graphs=0, vcm_order=3)
+                           vcm_graphs=0, vcm_order=3, vcm_merge_lanes=32)
         want = {"vcm_wavefront_paths": 1 << 14, "trace_blocks_per_cu": 3, "trace_tree": 0, "trace_lds_nodes": 9,
-                "vcm_graphs": 0, "vcm_order": 3}
+                "vcm_graphs": 0, "vcm_order": 3, "vcm_merge_lanes": 32}
         assert gpu_ctx.tuning() == want
         for bad in ({"vcm_wavefront_paths": 1 << 11}, {"vcm_wavefront_paths": (1 << 24) + 1},
                     {"trace_blocks_per_cu": 0}, {"trace_blocks_per_cu": 33}, {"trace_tree": 2},
-                    {"trace_lds_nodes": 10}, {"vcm_graphs": 2}, {"vcm_order": 4}):
+                    {"trace_lds_nodes": 10}, {"vcm_graphs": 2}, {"vcm_order": 4}, {"vcm_merge_lanes": 8},
+                    {"vcm_merge_lanes": 64}):
             with pytest.raises(N.SgpuError):
                 gpu_ctx.set_tuning(**bad)
         assert gpu_ctx.tuning() == want
@@ -80,10 +81,10 @@ def test_graph_replays_match_eager_launches(gpu_ctx):
             s.release()
 
 
-@pytest.mark.parametrize("graphs", [0, 1])
-def test_images_do_not_depend_on_work_order(gpu_ctx, graphs):
-    """vcm_order (merge range searches in spatial bins, hits shaded grouped by material) reorders the
-    work of a bounce and nothing else: a textured job of 3 wavefronts and a two-scene batch give the
+@pytest.mark.parametrize("graphs,lanes", [(0, 16), (1, 16), (1, 32)])
+def test_images_do_not_depend_on_work_order(gpu_ctx, graphs, lanes):
+    """vcm_order (merge range searches in spatial bins, hits shaded grouped by material) and the lanes
+    per range search reorder the work of a bounce and nothing else: a textured job of 3 wavefronts and a two-scene batch give the
     images of the queue order (99.9 % of the values within 1e-4: the light-tracing splats are summed
     with atomics, whose order follows the shading order) and the same ray and merge counts."""
     scs = [W.vcm_scene(seed=35, n_meshes=8, nu=16, nv=8), W.vcm_scene(seed=36, n_meshes=6, nu=12, nv=6, textures=False)]
@@ -92,8 +93,8 @@ def test_images_do_not_depend_on_work_order(gpu_ctx, graphs):
     try:
         out = {}
         for order in (0, N.ORDER_MERGE_CELLS, N.ORDER_SHADE_MATERIAL, N.ORDER_MERGE_CELLS | N.ORDER_SHADE_MATERIAL):
-            print(f"vcm_order {order}, graphs {graphs}", flush=True)
-            gpu_ctx.set_tuning(vcm_wavefront_paths=1 << 12, vcm_graphs=graphs, vcm_order=order)
+            print(f"vcm_order {order}, graphs {graphs}, merge lanes {lanes}", flush=True)
+            gpu_ctx.set_tuning(vcm_wavefront_paths=1 << 12, vcm_graphs=graphs, vcm_order=order, vcm_merge_lanes=lanes)
             sc = scs[0]
             img, st = N.render_vcm(scenes[0], sc["camera"], 48, 40, 6, 10, sc["merge_radius"], 0.75, 5,
                                    return_stats=True)
