@@ -305,16 +305,14 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(ShadeParams sp, uint32_t co
             const float* mat = sp.materials + 22 * (size_t)sp.geom_material[geom];
             Surface s;
             // CalculateSurfaceParams' eUsesPtex branch (SurfaceParameters.cpp:196-203)
-            f3 texel;
+            f3 texel = mk3(0, 0, 0);
             bool tex = false;
             if (((uint32_t)mat[21] & kMatUsesPtex) && geom < sp.n_geom_tex && sp.geom_tex[geom].texels) {
                 texel = ptex_eval(sp.geom_tex[geom], (int)hits.prim[j], hits.u[j], hits.v[j]);
                 tex = true;
             }
-            GeomAttr ga;
-            if (sp.geom_attr) ga = sp.geom_attr[geom];
-            surface_from_hit(o, d, hits.t[j], ng, sp.geom_attr ? &ga : nullptr, hits.prim[j], hits.u[j], hits.v[j], l2w,
-                             mat, tex ? &texel : nullptr, s);
+            surface_from_hit(o, d, hits.t[j], ng, sp.geom_attr ? sp.geom_attr + geom : nullptr, hits.prim[j], hits.u[j],
+                             hits.v[j], l2w, mat, tex, texel, s);
             f3 v = -d;
             LightSample ls;
             next_event_estimation(u, sp.lights, sp.n_lights, s.position, s.n,

@@ -28,7 +28,11 @@ struct PtexTex {
 
 namespace ptex_detail {
 
-constexpr int kMax = 10;
+// Widest kernel of a zero-footprint lookup: 6 taps on 4 x 4 faces (PtexWidth4Filter's even-aligned
+// half-resolution branch), 4 at every other resolution. Every array below is indexed only by
+// compile-time indices after unrolling (dynamic positions become selects), so that the filter
+// lives in registers: a dynamically indexed array would go to per-lane scratch memory.
+constexpr int kMax = 6;
 
 struct Kernel {
     int res_log2;
@@ -43,23 +47,76 @@ struct Filter {
     float weight;
 };
 
-SGPU_HD float accumulate(const float* p, int n)
+// The value itself, opaque to the optimizer: a select among pinned values cannot be folded back
+// into a load at a run-time address (which would put the array in scratch memory).
+SGPU_HD float pin(float x)
 {
-    float r = 0;
-    for (int i = 0; i < n; ++i) r += p[i];
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+v"(x));
+#endif
+    return x;
+}
+
+// a[i] for a run-time i in [0, kMax)
+SGPU_HD float get(const float* a, int i)
+{
+    float r = pin(a[0]);
+#pragma unroll
+    for (int j = 1; j < kMax; ++j) r = j == i ? pin(a[j]) : r;
     return r;
 }
-SGPU_HD float kweight(const Kernel& k) { return accumulate(k.ku, k.uw) * accumulate(k.kv, k.vw); }
+SGPU_HD void add_at(float* a, int i, float x)
+{
+#pragma unroll
+    for (int j = 0; j < kMax; ++j) {
+        const float y = pin(a[j]);
+        a[j] = j == i ? y + x : y;
+    }
+}
+// a[lo] + ... + a[lo + n - 1], summed in index order
+SGPU_HD float accumulate(const float* a, int lo, int n)
+{
+    float r = 0;
+#pragma unroll
+    for (int j = 0; j < kMax; ++j)
+        if (j >= lo && j < lo + n) r += a[j];
+    return r;
+}
+SGPU_HD float kweight(const Kernel& k) { return accumulate(k.ku, 0, k.uw) * accumulate(k.kv, 0, k.vw); }
 SGPU_HD int kres(const Kernel& k) { return 1 << k.res_log2; }
+// a[i] = a[i + w] for i + w < n (the rest is left as it was)
 SGPU_HD void drop_front(float* a, int n, int w)
 {
-    for (int i = 0; i + w < n; ++i) a[i] = a[i + w];
+    float b[kMax];
+#pragma unroll
+    for (int i = 0; i < kMax; ++i) b[i] = pin(a[i]);
+#pragma unroll
+    for (int i = 0; i < kMax; ++i) {
+        float x = b[i];
+#pragma unroll
+        for (int s = 1; i + s < kMax; ++s) x = s == w ? b[i + s] : x;
+        a[i] = i + w < n ? x : b[i];
+    }
+}
+// dst[i] = src[i + off] for i + off < kMax
+SGPU_HD void copy_from(float* dst, const float* src, int off)
+{
+    float b[kMax];
+#pragma unroll
+    for (int i = 0; i < kMax; ++i) b[i] = pin(src[i]);
+#pragma unroll
+    for (int i = 0; i < kMax; ++i) {
+        float x = b[i];
+#pragma unroll
+        for (int s = 1; i + s < kMax; ++s) x = s == off ? b[i + s] : x;
+        dst[i] = x;
+    }
 }
 
 SGPU_HD void merge_l(Kernel& k)
 {
     int w = -k.u;
-    k.ku[w] += accumulate(k.ku, w);
+    add_at(k.ku, w, accumulate(k.ku, 0, w));
     drop_front(k.ku, k.uw, w);
     k.uw -= w;
     k.u = 0;
@@ -67,14 +124,14 @@ SGPU_HD void merge_l(Kernel& k)
 SGPU_HD void merge_r(Kernel& k)
 {
     int w = k.uw + k.u - kres(k);
-    float* kp = k.ku + k.uw - w;
-    kp[-1] += accumulate(kp, w);
+    int at = k.uw - w;
+    add_at(k.ku, at - 1, accumulate(k.ku, at, w));
     k.uw -= w;
 }
 SGPU_HD void merge_b(Kernel& k)
 {
     int w = -k.v;
-    k.kv[w] += accumulate(k.kv, w);
+    add_at(k.kv, w, accumulate(k.kv, 0, w));
     drop_front(k.kv, k.vw, w);
     k.vw -= w;
     k.v = 0;
@@ -82,23 +139,28 @@ SGPU_HD void merge_b(Kernel& k)
 SGPU_HD void merge_t(Kernel& k)
 {
     int w = k.vw + k.v - kres(k);
-    float* kp = k.kv + k.vw - w;
-    kp[-1] += accumulate(kp, w);
+    int at = k.vw - w;
+    add_at(k.kv, at - 1, accumulate(k.kv, at, w));
     k.vw -= w;
 }
 
-SGPU_HD void kset(Kernel& d, int res_log2, int u, int v, int uw, int vw, const float* ku, const float* kv)
+// d = a kernel of the given placement whose taps are ku[ku_off ..] and kv[kv_off ..]
+SGPU_HD void kset(Kernel& d, int res_log2, int u, int v, int uw, int vw, const float* ku, int ku_off, const float* kv,
+                  int kv_off)
 {
     float tu[kMax], tv[kMax];
-    for (int i = 0; i < uw; ++i) tu[i] = ku[i];
-    for (int i = 0; i < vw; ++i) tv[i] = kv[i];
+    copy_from(tu, ku, ku_off);
+    copy_from(tv, kv, kv_off);
     d.res_log2 = res_log2;
     d.u = u;
     d.v = v;
     d.uw = uw;
     d.vw = vw;
-    for (int i = 0; i < uw; ++i) d.ku[i] = tu[i];
-    for (int i = 0; i < vw; ++i) d.kv[i] = tv[i];
+#pragma unroll
+    for (int i = 0; i < kMax; ++i) {
+        d.ku[i] = tu[i];
+        d.kv[i] = tv[i];
+    }
     d.rot = 0;
 }
 
@@ -106,7 +168,7 @@ SGPU_HD void split_l(Kernel& k, Kernel& out)
 {
     int w = -k.u;
     if (w < k.uw) {
-        kset(out, k.res_log2, kres(k) - w, k.v, w, k.vw, k.ku, k.kv);
+        kset(out, k.res_log2, kres(k) - w, k.v, w, k.vw, k.ku, 0, k.kv, 0);
         k.u = 0;
         drop_front(k.ku, k.uw, w);
         k.uw -= w;
@@ -121,7 +183,7 @@ SGPU_HD void split_r(Kernel& k, Kernel& out)
 {
     int w = k.u + k.uw - kres(k);
     if (w < k.uw) {
-        kset(out, k.res_log2, 0, k.v, w, k.vw, k.ku + k.uw - w, k.kv);
+        kset(out, k.res_log2, 0, k.v, w, k.vw, k.ku, k.uw - w, k.kv, 0);
         k.uw -= w;
     } else {
         out = k;
@@ -134,7 +196,7 @@ SGPU_HD void split_b(Kernel& k, Kernel& out)
 {
     int w = -k.v;
     if (w < k.vw) {
-        kset(out, k.res_log2, k.u, kres(k) - w, k.uw, w, k.ku, k.kv);
+        kset(out, k.res_log2, k.u, kres(k) - w, k.uw, w, k.ku, 0, k.kv, 0);
         k.v = 0;
         drop_front(k.kv, k.vw, w);
         k.vw -= w;
@@ -149,7 +211,7 @@ SGPU_HD void split_t(Kernel& k, Kernel& out)
 {
     int w = k.v + k.vw - kres(k);
     if (w < k.vw) {
-        kset(out, k.res_log2, k.u, 0, k.uw, w, k.ku, k.kv + k.vw - w);
+        kset(out, k.res_log2, k.u, 0, k.uw, w, k.ku, 0, k.kv, k.vw - w);
         k.vw -= w;
     } else {
         out = k;
@@ -159,12 +221,18 @@ SGPU_HD void split_t(Kernel& k, Kernel& out)
     }
 }
 
+// the first n taps in reverse order
 SGPU_HD void reverse(float* a, int n)
 {
-    for (int i = 0, j = n - 1; i < j; ++i, --j) {
-        float t = a[i];
-        a[i] = a[j];
-        a[j] = t;
+    float b[kMax];
+#pragma unroll
+    for (int i = 0; i < kMax; ++i) b[i] = pin(a[i]);
+#pragma unroll
+    for (int i = 0; i < kMax; ++i) {
+        float x = b[i];
+#pragma unroll
+        for (int j = 0; j < kMax; ++j) x = (i < n && j == n - 1 - i) ? b[j] : x;
+        a[i] = x;
     }
 }
 SGPU_HD void flip_u(Kernel& k)
@@ -185,6 +253,7 @@ SGPU_HD void swap_uv(Kernel& k)
     t = k.uw;
     k.uw = k.vw;
     k.vw = t;
+#pragma unroll
     for (int i = 0; i < kMax; ++i) {
         float f = k.ku[i];
         k.ku[i] = k.kv[i];
@@ -217,19 +286,26 @@ SGPU_HD float make_symmetric(Kernel& k, float initial_weight)
     int n = k.uw < k.vw ? k.uw : k.vw;
     k.uw = k.vw = n;
     float new_weight = 0;
-    for (int i = 0; i < n; ++i) {
-        float sum = k.ku[i] + k.kv[i];
-        k.ku[i] = k.kv[i] = sum;
-        new_weight += sum;
+#pragma unroll
+    for (int i = 0; i < kMax; ++i) {
+        if (i < n) {
+            float sum = k.ku[i] + k.kv[i];
+            k.ku[i] = k.kv[i] = sum;
+            new_weight += sum;
+        }
     }
     new_weight *= new_weight;
     float scale = new_weight == 0 ? 1.f : initial_weight / new_weight;
     if (!(scale >= 1)) {
         if (scale < -1) {
-            for (int i = 0; i < n; ++i) k.ku[i] *= -1;
+#pragma unroll
+            for (int i = 0; i < kMax; ++i)
+                if (i < n) k.ku[i] *= -1;
             new_weight = -new_weight;
         } else {
-            for (int i = 0; i < n; ++i) k.ku[i] *= scale;
+#pragma unroll
+            for (int i = 0; i < kMax; ++i)
+                if (i < n) k.ku[i] *= scale;
             new_weight = initial_weight;
         }
     }
@@ -245,13 +321,18 @@ SGPU_HD void kapply(Filter& F, const Kernel& k, int face)
     if (k.uw <= 0 || k.vw <= 0) return;
     const int res = kres(k);
     const uint8_t* data = F.tx.texels + (size_t)face * res * res * 3;
-    for (int j = 0; j < k.vw; ++j) {
+#pragma unroll
+    for (int j = 0; j < kMax; ++j) {
+        if (j >= k.vw) break;
         const uint8_t* p = data + ((size_t)(k.v + j) * res + k.u) * 3;
         float r0 = (float)p[0] * k.ku[0], r1 = (float)p[1] * k.ku[0], r2 = (float)p[2] * k.ku[0];
-        for (int i = 1; i < k.uw; ++i) {
-            r0 += (float)p[3 * i] * k.ku[i];
-            r1 += (float)p[3 * i + 1] * k.ku[i];
-            r2 += (float)p[3 * i + 2] * k.ku[i];
+#pragma unroll
+        for (int i = 1; i < kMax; ++i) {
+            if (i < k.uw) {
+                r0 += (float)p[3 * i] * k.ku[i];
+                r1 += (float)p[3 * i + 1] * k.ku[i];
+                r2 += (float)p[3 * i + 2] * k.ku[i];
+            }
         }
         F.result[0] += r0 * k.kv[j];
         F.result[1] += r1 * k.kv[j];
@@ -266,34 +347,47 @@ SGPU_HD void apply_across_edge(Filter& F, Kernel& k, int face, int eid)
     kapply(F, k, af);
 }
 
+// One step around a corner: the face across edge e of face f and the edge that follows.
+SGPU_HD void corner_step(const PtexTex& tx, int& f, int& e)
+{
+    const int next = adjface(tx, f, e);
+    const int nexte = (adjedge(tx, f, e) + 1) % 4;
+    f = next;
+    e = nexte;
+}
+
 SGPU_HD void apply_to_corner(Filter& F, Kernel& k, int face, int eid)
 {
+    // the faces around the corner (the reference records them; here the ring is walked again
+    // where they are needed, so that no array is indexed at run time)
     constexpr int kMaxValence = 10;
-    int cface[kMaxValence], cedge[kMaxValence];
-    int af = face, ae = eid, ncorners = 0;
+    int af = face, ae = eid, ncorners = 0, face1 = 0, edge1 = 0;
     for (int i = 0; i < kMaxValence; ++i) {
-        int next = adjface(F.tx, af, ae);
-        int nexte = (adjedge(F.tx, af, ae) + 1) % 4;
-        af = next;
-        ae = nexte;
+        corner_step(F.tx, af, ae);
         if (af < 0 || (af == face && ae == eid)) {
             ncorners = i - 2;
             break;
         }
-        cface[i] = af;
-        cedge[i] = ae;
+        if (i == 1) {
+            face1 = af;
+            edge1 = ae;
+        }
     }
     if (ncorners == 1) {
-        krotate(k, eid - cedge[1] + 2);
-        kapply(F, k, cface[1]);
+        krotate(k, eid - edge1 + 2);
+        kapply(F, k, face1);
     } else if (ncorners > 1) {
         krotate(k, eid + 2);
         float initial = kweight(k);
         float neww = make_symmetric(k, initial);
+        af = face;
+        ae = eid;
+        corner_step(F.tx, af, ae);  // ring position 0
         for (int i = 1; i <= ncorners; ++i) {
+            corner_step(F.tx, af, ae);
             Kernel kc = k;
-            krotate(kc, 2 - cedge[i] + 2);
-            kapply(F, kc, cface[i]);
+            krotate(kc, 2 - ae + 2);
+            kapply(F, kc, af);
         }
         F.weight += neww * (float)ncorners - initial;
     } else {
@@ -402,7 +496,8 @@ SGPU_HD int res_from_width(float w)
     return 127 - ((b >> 23) & 0xff);
 }
 
-// PtexWidth4Filter::buildKernelAxis with lerp off, for a zero footprint
+// PtexWidth4Filter::buildKernelAxis with lerp off, for a zero footprint (uw is at least one texel:
+// the generic branch then has exactly 4 taps, the half-resolution branch at most 6)
 SGPU_HD void build_axis(int& k_reslog2, int& k_u, int& k_uw, float* ku, float u, float uw, int f_reslog2)
 {
     uw = uw > recip_pow2(f_reslog2) ? uw : recip_pow2(f_reslog2);
@@ -419,7 +514,9 @@ SGPU_HD void build_axis(int& k_reslog2, int& k_u, int& k_uw, float* ku, float u,
             k_u = u1;
             k_uw = u2 - u1;
             float x1 = (float)u1 - upix;
-            for (int i = 0; i < k_uw; i += 2) {
+#pragma unroll
+            for (int i = 0; i < kMax; i += 2) {
+                if (i >= k_uw) break;
                 float xa = x1 + (float)i, xb = xa + 1.0f, xc = (xa + xb) * 0.25f;
                 float s = 1.0f / (uw + .75f);
                 float ka = bspline(xa), kb = bspline(xb), kc = blur(xc * s);
@@ -433,7 +530,8 @@ SGPU_HD void build_axis(int& k_reslog2, int& k_u, int& k_uw, float* ku, float u,
             k_u = (int)floorf(u - .5f) * 2;
             k_uw = 4;
             float x1 = (float)k_u - upix;
-            for (int i = 0; i < k_uw; i += 2) {
+#pragma unroll
+            for (int i = 0; i < 4; i += 2) {
                 float xa = x1 + (float)i, xb = xa + 1.0f, xc = (xa + xb) * 0.5f;
                 float s = 1.0f / (uw * 1.5f + .5f);
                 float ka = blur(xa * s), kb = blur(xb * s), kc = blur(xc * s);
@@ -459,7 +557,9 @@ SGPU_HD void build_axis(int& k_reslog2, int& k_u, int& k_uw, float* ku, float u,
     k_u = u1;
     k_uw = u2 - u1;
     float x1 = ((float)u1 - upix) / uwpix, step = 1.0f / uwpix;
-    for (int i = 0; i < k_uw; ++i) ku[i] = bspline(x1 + (float)i * step);
+#pragma unroll
+    for (int i = 0; i < kMax; ++i)
+        if (i < k_uw) ku[i] = bspline(x1 + (float)i * step);
 }
 
 }  // namespace ptex_detail
@@ -478,18 +578,18 @@ SGPU_HD f3 ptex_eval(const PtexTex& tx, int face, float u, float v)
     k.res_log2 = ru;  // square faces: both axes choose the same resolution
     k.rot = 0;
     // stripZeros
-    while (k.ku[0] == 0) {
+    while (get(k.ku, 0) == 0) {
         drop_front(k.ku, k.uw, 1);
         k.u++;
         k.uw--;
     }
-    while (k.ku[k.uw - 1] == 0) k.uw--;
-    while (k.kv[0] == 0) {
+    while (get(k.ku, k.uw - 1) == 0) k.uw--;
+    while (get(k.kv, 0) == 0) {
         drop_front(k.kv, k.vw, 1);
         k.v++;
         k.vw--;
     }
-    while (k.kv[k.vw - 1] == 0) k.vw--;
+    while (get(k.kv, k.vw - 1) == 0) k.vw--;
     Filter F;
     F.tx = tx;
     F.result[0] = F.result[1] = F.result[2] = 0.0f;

@@ -44,15 +44,16 @@ SGPU_HD void shading_record(const float* __restrict raw, float* __restrict out)
 
 // Shading frame (n, t, b) and material of a surface point seen from `view` (the unit direction
 // toward the previous path vertex), from the material's shading record. The base colour is the
-// record's unless `texel` (already filtered, before the 2.2 power) is given.
-SGPU_HD void surface_material_frame(const float* __restrict mat, f3 n, f3 t, f3 b, f3 view, const f3* texel,
+// record's unless a texel (already filtered, before the 2.2 power) is given (has_texel). The texel
+// travels by value: a pointer chosen between a local and NULL would keep the local in scratch memory.
+SGPU_HD void surface_material_frame(const float* __restrict mat, f3 n, f3 t, f3 b, f3 view, bool has_texel, f3 texel,
                                     Surface& s)
 {
     s.n = n;
     s.t = t;
     s.b = b;
-    s.base_color = texel ? mk3(pow_f(texel->x, 2.2f), pow_f(texel->y, 2.2f), pow_f(texel->z, 2.2f))
-                         : mk3(mat[0], mat[1], mat[2]);
+    s.base_color = has_texel ? mk3(pow_f(texel.x, 2.2f), pow_f(texel.y, 2.2f), pow_f(texel.z, 2.2f))
+                             : mk3(mat[0], mat[1], mat[2]);
     s.transmittance_color = mk3(mat[3], mat[4], mat[5]);
     s.sheen = mat[6];
     s.sheen_tint = mat[7];
@@ -73,11 +74,11 @@ SGPU_HD void surface_material_frame(const float* __restrict mat, f3 n, f3 t, f3 
 }
 
 // The same with the frame of make_frame(n) (MakeOrthogonalCoordinateSystem).
-SGPU_HD void surface_material(const float* __restrict mat, f3 n, f3 view, const f3* texel, Surface& s)
+SGPU_HD void surface_material(const float* __restrict mat, f3 n, f3 view, Surface& s)
 {
     f3 t, b;
     make_frame(n, t, b);
-    surface_material_frame(mat, n, t, b, view, texel, s);
+    surface_material_frame(mat, n, t, b, view, false, mk3(0, 0, 0), s);
 }
 
 
@@ -155,7 +156,7 @@ SGPU_HD bool hit_frame(const GeomAttr* ga, uint32_t prim, float u, float v, f3 n
 // true when the frame came from the geometry's tangents; then *tangent (may be NULL) receives
 // (t, w), from which the frame is rebuilt with the same n (tangent_frame).
 SGPU_HD bool surface_from_hit(f3 o, f3 d, float t, f3 ng, const GeomAttr* ga, uint32_t prim, float u, float v,
-                              const float* l2w, const float* __restrict mat, const f3* texel, Surface& s,
+                              const float* l2w, const float* __restrict mat, bool has_texel, f3 texel, Surface& s,
                               float4* tangent = nullptr)
 {
     s.position = mk3(o.x + t * d.x, o.y + t * d.y, o.z + t * d.z);
@@ -164,7 +165,7 @@ SGPU_HD bool surface_from_hit(f3 o, f3 d, float t, f3 ng, const GeomAttr* ga, ui
     float tw = 0.0f;
     const bool tf = hit_frame(ga, prim, u, v, ng, l2w, n, tt, b, tw);
     if (tf && tangent) *tangent = make_float4(tt.x, tt.y, tt.z, tw);
-    surface_material_frame(mat, n, tt, b, -d, texel, s);
+    surface_material_frame(mat, n, tt, b, -d, has_texel, texel, s);
     return tf;
 }
 

@@ -166,12 +166,10 @@ __device__ __forceinline__ bool hit_surface(const Slice& sl, const HitBuf& hits,
                                             const float* l2w, f3 o, f3 d, const float* mat, Surface& s,
                                             float4* tangent)
 {
-    f3 texel;
+    f3 texel = mk3(0, 0, 0);
     const bool tex = hit_texel(sl, geom, mat, hits.prim[j], hits.u[j], hits.v[j], texel);
-    GeomAttr ga;
-    if (sl.geom_attr) ga = sl.geom_attr[geom];
-    return surface_from_hit(o, d, hits.t[j], ng, sl.geom_attr ? &ga : nullptr, hits.prim[j], hits.u[j], hits.v[j],
-                            l2w, mat, tex ? &texel : nullptr, s, tangent);
+    return surface_from_hit(o, d, hits.t[j], ng, sl.geom_attr ? sl.geom_attr + geom : nullptr, hits.prim[j], hits.u[j],
+                            hits.v[j], l2w, mat, tex, texel, s, tangent);
 }
 
 // Whether geometry geom of a slice shades with a tangent frame (vertex tangents).
@@ -246,7 +244,7 @@ __device__ __forceinline__ void vertex_surface(const Slice& sl, const LightVerte
     const float* mat = sl.materials + 22 * (size_t)sl.geom_material[geom];
     s.position = mk3(v.pos[0], v.pos[1], v.pos[2]);
     s.error = 0.0f;
-    surface_material(mat, mk3(v.n[0], v.n[1], v.n[2]), mk3(v.wfix[0], v.wfix[1], v.wfix[2]), nullptr, s);
+    surface_material(mat, mk3(v.n[0], v.n[1], v.n[2]), mk3(v.wfix[0], v.wfix[1], v.wfix[2]), s);
     if (tan && geom_has_tangents(sl, geom)) tangent_frame(s.n, *tan, s);
     s.base_color = mk3(v.base[0], v.base[1], v.base[2]);
 }

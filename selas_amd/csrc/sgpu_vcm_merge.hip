@@ -105,8 +105,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         float ccp = 0.0f;
         if (active) {
             const uint32_t geom = v.geom_plen >> 5;
-            surface_material(sl.materials + 22 * (size_t)sl.geom_material[geom], mk3(v.n[0], v.n[1], v.n[2]), view,
-                             nullptr, s);
+            surface_material(sl.materials + 22 * (size_t)sl.geom_material[geom], mk3(v.n[0], v.n[1], v.n[2]), view, s);
             if (geom_has_tangents(sl, geom)) tangent_frame(s.n, w.cam_tan[j], s);
             s.base_color = mk3(v.base[0], v.base[1], v.base[2]);
             ccp = cont_prob(s);

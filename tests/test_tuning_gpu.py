@@ -19,7 +19,7 @@ def test_tuning_roundtrip_and_limits(gpu_ctx):
         assert gpu_ctx.tuning() == want
         for bad in ({"vcm_wavefront_paths": 1 << 11}, {"vcm_wavefront_paths": (1 << 24) + 1},
                     {"trace_blocks_per_cu": 0}, {"trace_blocks_per_cu": 33}, {"trace_tree": 2},
-                    {"trace_lds_nodes": 10}, {"vcm_graphs": 2}, {"vcm_order": 4}, {"vcm_merge_lanes": 8},
+                    {"trace_lds_nodes": 10}, {"vcm_graphs": 2}, {"vcm_order": 4}, {"vcm_merge_lanes": 4},
                     {"vcm_merge_lanes": 64}):
             with pytest.raises(N.SgpuError):
                 gpu_ctx.set_tuning(**bad)
