@@ -152,7 +152,7 @@ def bench_vcm(args):
     log("building VCM workload scene")
     sc = W.vcm_scene()
     n_tris = sum(W.geom_tri9(g).shape[0] for g in sc["geoms"])
-    ctx = N.Context(0)
+    ctx = new_context(N, args)
     tb = time.perf_counter()
     scene = N.build_render_scene(ctx, sc)
     build_ms = (time.perf_counter() - tb) * 1e3
@@ -334,6 +334,15 @@ def bench_vcm(args):
 TUNING_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "selas_amd", "tuning.json")
 
 
+def new_context(N, args):
+    """A context on device 0, with the tree builder of --bvh-builder (scenes are built with the
+    context's builder, so it is set before any scene)."""
+    ctx = N.Context(0)
+    if getattr(args, "bvh_builder", None) is not None:
+        ctx.set_tuning(bvh_builder=args.bvh_builder)
+    return ctx
+
+
 def apply_tuning(args, ctx, probe, workload=None):
     """Launch shape for the timed runs: given on the command line, the defaults (--no-tune), the
     shape the autotuner stored for this GPU architecture and workload (selas_amd/tuning.json, unless
@@ -490,7 +499,7 @@ def bench_batch(args):
     from selas_amd import native as N
     from selas_amd import workloads as W
 
-    ctx = N.Context(0)
+    ctx = new_context(N, args)
     b = run_batch(N, W, ctx, args.batch_jobs, args.batch_seed, args.solo, tune_args=args)
     cpu = None
     if not args.no_cpu:
@@ -612,7 +621,7 @@ def bench_serve(args):
     from selas_amd import workloads as W
 
     rates = [float(x) for x in args.rates.split(",")]
-    ctx = N.Context(0)
+    ctx = new_context(N, args)
     # requests per rate: about 8 s of arrivals, 16..64
     n_jobs = args.serve_jobs or (lambda r: int(min(64, max(16, round(8 * r)))))
     if args.trace_blocks:
@@ -689,6 +698,8 @@ def main():
     ap.add_argument("--trace-blocks", type=int, default=None, help="traversal blocks per CU (skips autotuning)")
     ap.add_argument("--order", type=int, default=None, choices=(0, 1, 2, 3),
                     help="VCM work order (sgpu_tuning.vcm_order bits: 1 merge in spatial bins, 2 shade by material)")
+    ap.add_argument("--bvh-builder", type=int, default=None, choices=(0, 1),
+                    help="binary tree under the 8-wide one: 0 linear BVH (default), 1 locally-ordered clustering")
     ap.add_argument("--merge-lanes", type=int, default=None, choices=(8, 16, 32),
                     help="VCM lanes per vertex-merging range search (sgpu_tuning.vcm_merge_lanes)")
     ap.add_argument("--trace-tree", type=int, default=None, choices=(0, 1),
@@ -700,7 +711,7 @@ def main():
 
     if args.copy_calibration:
         from selas_amd import native as N
-        ctx = N.Context(0)
+        ctx = new_context(N, args)
         a = ctx.empty(args.copy_calibration)
         b = ctx.empty(args.copy_calibration)
         for _ in range(3):
@@ -751,7 +762,7 @@ def bench_pt(args):
     w, spp, L, seed = 256, 16, 8, 1
     sc = W.room_scene(1)
     n_tris = sum(W.geom_tri9(g).shape[0] for g in sc["geoms"])
-    ctx = N.Context(0)
+    ctx = new_context(N, args)
     scene = N.build_render_scene(ctx, sc)
     cam = sc["camera"]
     fb = ctx.empty(2 * 3 * w * w * 4)
@@ -855,7 +866,7 @@ def bench_traversal(args):
     rays = W.traversal_rays(t9)
     n = int(rays["org_x"].shape[0])
 
-    ctx = N.Context(0)
+    ctx = new_context(N, args)
     if args.trace_tree is not None:
         ctx.set_tuning(trace_tree=args.trace_tree)
     if args.trace_blocks:
@@ -989,7 +1000,7 @@ def bench_instances(args):
     rays = W.traversal_rays(world)
     n = int(rays["org_x"].shape[0])
 
-    ctx = N.Context(0)
+    ctx = new_context(N, args)
     if args.trace_blocks:
         ctx.set_tuning(trace_blocks_per_cu=args.trace_blocks)
     dm = N.Scene(ctx)
@@ -1090,7 +1101,7 @@ def bench_deferred(args):
     sc = W.vcm_scene()
     sc["ibl"] = {"rgb": W.sky_map(256, 512), "rotation": 0.0, "exposure": 1.0}
     n_tris = sum(W.geom_tri9(g).shape[0] for g in sc["geoms"])
-    ctx = N.Context(0)
+    ctx = new_context(N, args)
     if args.trace_blocks:
         ctx.set_tuning(trace_blocks_per_cu=args.trace_blocks)
     scene = N.build_render_scene(ctx, sc)

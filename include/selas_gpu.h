@@ -119,6 +119,11 @@ typedef struct sgpu_tuning {
     uint32_t vcm_order;           /* work order inside a VCM bounce, SGPU_ORDER_* bits (default 0:
                                      the order the queues were filled in); images do not depend on it */
     uint32_t vcm_merge_lanes;     /* lanes per vertex-merging range search: 8, 16 (default) or 32 */
+    uint32_t bvh_builder;         /* binary tree the 8-wide tree is collapsed from, built at
+                                     sgpu_scene_commit: 0 (default) linear BVH (Morton order,
+                                     Karras 2012), 1 locally-ordered clustering (PLOC, Meister and
+                                     Bittner 2018, radius 16) over the same Morton order; queries
+                                     give the same results on either */
 } sgpu_tuning;
 /* vcm_order bits. SGPU_ORDER_MERGE_CELLS: the vertex-merging range searches of a bounce run in
  * spatial order (binned by a 12-bit Morton code of the query point in its photon grid's box, by a

@@ -272,6 +272,132 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const TriRecord* __restrict 
     if (j < n) out[j] = in[order[j]];
 }
 
+// ---- PLOC: the binary tree by parallel locally-ordered clustering ----------------------------------
+//
+// Meister and Bittner, "Parallel Locally-Ordered Clustering for Bounding Volume Hierarchy
+// Construction" (IEEE TVCG 2018). The clusters start as the leaves in Morton order; every round,
+// each cluster finds the neighbour within kPlocRadius positions whose union box has the smallest
+// surface area, mutual nearest neighbours merge into a new internal node (at the lower position),
+// and the surviving clusters are compacted in order. Edges are ordered by (area, lower position,
+// higher position), so the nearest-neighbour relation has no cycles and every round merges at
+// least one pair. Node indices are handed out from n - 2 downwards in position order, so the last
+// merge is the root, node 0, and the tree is deterministic.
+
+constexpr int kPlocRadius = 16;
+
+__device__ __forceinline__ float union_area(float4 alo, float4 ahi, float4 blo, float4 bhi)
+{
+    const float dx = fmaxf(ahi.x, bhi.x) - fminf(alo.x, blo.x);
+    const float dy = fmaxf(ahi.y, bhi.y) - fminf(alo.y, blo.y);
+    const float dz = fmaxf(ahi.z, bhi.z) - fminf(alo.z, blo.z);
+    return dx * dy + dy * dz + dz * dx;
+}
+
+// The leaves as the first clusters: leaf j (Morton order) with its inflated box, height 0.
+__global__ __launch_bounds__(kBlock) void k_ploc_init(uint32_t n, const float4* __restrict blo,
+                                                      const float4* __restrict bhi, const uint32_t* __restrict order,
+                                                      const uint32_t* __restrict bounds, int* __restrict cid,
+                                                      float4* __restrict clo, float4* __restrict chi,
+                                                      uint32_t* __restrict height)
+{
+    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= n) return;
+    const float eps = inflate_eps(bounds);
+    float4 lo, hi;
+    leaf_box(blo, bhi, order, (int)j, eps, lo, hi);
+    cid[j] = ~(int)j;
+    clo[j] = lo;
+    chi[j] = hi;
+    height[j] = 0;
+}
+
+// nn[i] = the neighbour of cluster i (within kPlocRadius positions) of least union area, edges
+// ordered by (area, lower position, higher position). The block's clusters and the radius on both
+// sides are staged in LDS.
+__global__ __launch_bounds__(kBlock) void k_ploc_nn(uint32_t m, const float4* __restrict clo,
+                                                    const float4* __restrict chi, uint32_t* __restrict nn)
+{
+    __shared__ float4 slo[kBlock + 2 * kPlocRadius], shi[kBlock + 2 * kPlocRadius];
+    const int first = (int)(blockIdx.x * kBlock) - kPlocRadius;
+    for (int k = threadIdx.x; k < kBlock + 2 * kPlocRadius; k += kBlock) {
+        const int g = first + k;
+        if (g >= 0 && g < (int)m) {
+            slo[k] = clo[g];
+            shi[k] = chi[g];
+        }
+    }
+    __syncthreads();
+    const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
+    if (i >= (int)m) return;
+    const int li = i - first;
+    const float4 lo = slo[li], hi = shi[li];
+    float best = 0.0f;
+    int bj = -1;
+    const int j0 = i - kPlocRadius < 0 ? 0 : i - kPlocRadius;
+    const int j1 = i + kPlocRadius >= (int)m ? (int)m - 1 : i + kPlocRadius;
+    for (int j = j0; j <= j1; ++j) {
+        if (j == i) continue;
+        const float a = union_area(lo, hi, slo[j - first], shi[j - first]);
+        // (a, min, max) against the best edge so far; with i fixed, a smaller other end wins ties
+        if (bj < 0 || a < best || (a == best && j < bj)) {
+            best = a;
+            bj = j;
+        }
+    }
+    nn[i] = (uint32_t)bj;
+}
+
+// Per position: bit 0 of keep = the cluster survives this round (it is not the higher end of a
+// merging pair), bit 32 = it is the lower end of a merging pair (a new node).
+__global__ __launch_bounds__(kBlock) void k_ploc_flags(uint32_t m, const uint32_t* __restrict nn,
+                                                       uint64_t* __restrict flags)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= m) return;
+    const uint32_t j = nn[i];
+    const bool mutual = nn[j] == i;
+    const uint64_t keep = (mutual && j < i) ? 0ull : 1ull;
+    const uint64_t lead = (mutual && i < j) ? 1ull : 0ull;
+    flags[i] = keep | (lead << 32);
+}
+
+// The round's merges (node base + rank among this round's merges, written as BvhNodes) and the
+// surviving clusters, compacted in position order into the other buffers.
+__global__ __launch_bounds__(kBlock) void k_ploc_merge(uint32_t m, uint32_t node_base, const uint32_t* __restrict nn,
+                                                       const uint64_t* __restrict flags,
+                                                       const uint64_t* __restrict prefix, const int* __restrict cid,
+                                                       const float4* __restrict clo, const float4* __restrict chi,
+                                                       const uint32_t* __restrict height, int* __restrict ncid,
+                                                       float4* __restrict nlo, float4* __restrict nhi,
+                                                       uint32_t* __restrict nheight, BvhNode* __restrict nodes)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= m) return;
+    const uint64_t f = flags[i];
+    if (!(f & 1ull)) return;
+    const uint32_t pos = (uint32_t)(prefix[i] & 0xFFFFFFFFull);
+    if (f >> 32) {
+        const uint32_t j = nn[i];
+        const uint32_t id = node_base + (uint32_t)(prefix[i] >> 32);
+        const float4 lo0 = clo[i], hi0 = chi[i], lo1 = clo[j], hi1 = chi[j];
+        BvhNode nd;
+        nd.x = make_float4(lo0.x, hi0.x, lo1.x, hi1.x);
+        nd.y = make_float4(lo0.y, hi0.y, lo1.y, hi1.y);
+        nd.z = make_float4(lo0.z, hi0.z, lo1.z, hi1.z);
+        nd.child = make_int4(cid[i], cid[j], 0, 0);
+        nodes[id] = nd;
+        ncid[pos] = (int)id;
+        nlo[pos] = make_float4(fminf(lo0.x, lo1.x), fminf(lo0.y, lo1.y), fminf(lo0.z, lo1.z), 0.0f);
+        nhi[pos] = make_float4(fmaxf(hi0.x, hi1.x), fmaxf(hi0.y, hi1.y), fmaxf(hi0.z, hi1.z), 0.0f);
+        nheight[pos] = 1 + max(height[i], height[j]);
+    } else {
+        ncid[pos] = cid[i];
+        nlo[pos] = clo[i];
+        nhi[pos] = chi[i];
+        nheight[pos] = height[i];
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_depth(int n, const int* __restrict parent_int,
                                                   const int* __restrict parent_leaf, uint32_t* maxdepth)
 {
@@ -520,7 +646,8 @@ uint32_t host_f2ord(float f)
 // holds max(n - 1, 1) nodes, d_order the primitive of each leaf in leaf order and *depth the
 // deepest leaf.
 int build_binary(uint32_t n, const float4* d_blo, const float4* d_bhi, const float4* d_cent, const uint32_t* d_bounds,
-                 hipStream_t stream, BvhNode** nodes_out, DevBuf<uint32_t>& d_order, uint32_t* depth_out)
+                 hipStream_t stream, BvhNode** nodes_out, DevBuf<uint32_t>& d_order, uint32_t* depth_out,
+                 uint32_t builder)
 {
     const uint32_t n_int = n > 1 ? n - 1 : 1;
     BvhNode* nodes = nullptr;
@@ -574,6 +701,61 @@ int build_binary(uint32_t n, const float4* d_blo, const float4* d_bhi, const flo
     SGPU_BUILD_TRY(d_tmp.alloc(tmp_bytes));
     SGPU_BUILD_TRY(rocprim::radix_sort_pairs(d_tmp.p, tmp_bytes, d_keys.p, d_keys2.p, d_vals.p, d_order.p,
                                              (size_t)n, 0, 63, stream));
+    if (builder == 1) {
+        // PLOC over the Morton-ordered leaves
+        DevBuf<int> cid[2];
+        DevBuf<float4> clo[2], chi[2];
+        DevBuf<uint32_t> hgt[2], nn;
+        DevBuf<uint64_t> flags, prefix, last;
+        for (int b = 0; b < 2; ++b) {
+            SGPU_BUILD_TRY(cid[b].alloc(n));
+            SGPU_BUILD_TRY(clo[b].alloc(n));
+            SGPU_BUILD_TRY(chi[b].alloc(n));
+            SGPU_BUILD_TRY(hgt[b].alloc(n));
+        }
+        SGPU_BUILD_TRY(nn.alloc(n));
+        SGPU_BUILD_TRY(flags.alloc(n));
+        SGPU_BUILD_TRY(prefix.alloc(n));
+        size_t scan_bytes = 0;
+        SGPU_BUILD_TRY(rocprim::exclusive_scan(nullptr, scan_bytes, flags.p, prefix.p, (uint64_t)0, (size_t)n,
+                                               rocprim::plus<uint64_t>(), stream));
+        DevBuf<uint8_t> d_scan;
+        SGPU_BUILD_TRY(d_scan.alloc(scan_bytes));
+        hipLaunchKernelGGL(k_ploc_init, dim3(grid), dim3(kBlock), 0, stream, n, d_blo, d_bhi, d_order.p, d_bounds,
+                           cid[0].p, clo[0].p, chi[0].p, hgt[0].p);
+        SGPU_BUILD_TRY(hipGetLastError());
+        uint32_t m = n, next_id = n - 1, cur = 0;
+        uint64_t tail[2] = {0, 0};
+        while (m > 1) {
+            const uint32_t g = (m + kBlock - 1) / kBlock;
+            hipLaunchKernelGGL(k_ploc_nn, dim3(g), dim3(kBlock), 0, stream, m, clo[cur].p, chi[cur].p, nn.p);
+            hipLaunchKernelGGL(k_ploc_flags, dim3(g), dim3(kBlock), 0, stream, m, nn.p, flags.p);
+            SGPU_BUILD_TRY(hipGetLastError());
+            SGPU_BUILD_TRY(rocprim::exclusive_scan(d_scan.p, scan_bytes, flags.p, prefix.p, (uint64_t)0, (size_t)m,
+                                                   rocprim::plus<uint64_t>(), stream));
+            SGPU_BUILD_TRY(hipMemcpyAsync(&tail[0], prefix.p + m - 1, sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+            SGPU_BUILD_TRY(hipMemcpyAsync(&tail[1], flags.p + m - 1, sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+            SGPU_BUILD_TRY(hipStreamSynchronize(stream));
+            const uint64_t tot = tail[0] + tail[1];
+            const uint32_t kept = (uint32_t)(tot & 0xFFFFFFFFull), merges = (uint32_t)(tot >> 32);
+            if (merges == 0 || merges > next_id || kept != m - merges)
+                return fail_free(SGPU_E_STATE, "PLOC: a clustering round made no progress");
+            next_id -= merges;
+            hipLaunchKernelGGL(k_ploc_merge, dim3(g), dim3(kBlock), 0, stream, m, next_id, nn.p, flags.p, prefix.p,
+                               cid[cur].p, clo[cur].p, chi[cur].p, hgt[cur].p, cid[1 - cur].p, clo[1 - cur].p,
+                               chi[1 - cur].p, hgt[1 - cur].p, nodes);
+            SGPU_BUILD_TRY(hipGetLastError());
+            cur = 1 - cur;
+            m = kept;
+        }
+        uint32_t h = 0;
+        SGPU_BUILD_TRY(hipMemcpyAsync(&h, hgt[cur].p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        SGPU_BUILD_TRY(hipStreamSynchronize(stream));
+        if (next_id != 0) return fail_free(SGPU_E_STATE, "PLOC: the tree does not end at node 0");
+        *nodes_out = nodes;
+        *depth_out = h;
+        return SGPU_OK;
+    }
     const uint32_t grid_int = (n - 1 + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_karras, dim3(grid_int), dim3(kBlock), 0, stream, d_keys2.p, (int)n, d_child.p, d_pint.p,
                        d_pleaf.p);
@@ -636,7 +818,8 @@ int sgpu_bvh_build(sgpu_scene scene, hipStream_t stream)
 
     BvhNode* nodes = nullptr;
     uint32_t depth = 0;
-    int rc = build_binary(n, d_blo.p, d_bhi.p, d_cent.p, d_bounds.p, stream, &nodes, d_order, &depth);
+    int rc = build_binary(n, d_blo.p, d_bhi.p, d_cent.p, d_bounds.p, stream, &nodes, d_order, &depth,
+                          scene->ctx->bvh_builder);
     if (rc != SGPU_OK) return rc;
     for (int k = 0; k < 3; ++k) {
         scene->box[k] = host_ord2f(hb[k]);
@@ -853,7 +1036,8 @@ int sgpu_bvh_build_instances(sgpu_scene scene, hipStream_t stream)
     SGPU_HIP_TRY(hipMemcpyAsync(d_bounds.p, hb, sizeof(hb), hipMemcpyHostToDevice, stream));
     BvhNode* nodes = nullptr;
     uint32_t depth = 0;
-    int rc = build_binary(n, d_blo.p, d_bhi.p, d_cent.p, d_bounds.p, stream, &nodes, d_order, &depth);
+    int rc = build_binary(n, d_blo.p, d_bhi.p, d_cent.p, d_bounds.p, stream, &nodes, d_order, &depth,
+                          scene->ctx->bvh_builder);
     if (rc != SGPU_OK) return rc;
     // the top tree's leaves name composed instances by their composition index (instance i, then
     // its instance j), so the closest-hit tie rule follows (inst_id0, inst_id1)

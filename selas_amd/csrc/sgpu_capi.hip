@@ -100,6 +100,7 @@ int sgpu_set_tuning(sgpu_context ctx, const sgpu_tuning* t)
         return sgpu_fail(SGPU_E_RANGE, "sgpu_set_tuning: unknown vcm_order bits");
     if (t->vcm_merge_lanes != 8 && t->vcm_merge_lanes != 16 && t->vcm_merge_lanes != 32)
         return sgpu_fail(SGPU_E_RANGE, "sgpu_set_tuning: vcm_merge_lanes must be 8, 16 or 32");
+    if (t->bvh_builder > 1) return sgpu_fail(SGPU_E_RANGE, "sgpu_set_tuning: bvh_builder must be 0 or 1");
     ctx->vcm_wavefront_paths = t->vcm_wavefront_paths;
     ctx->trace_blocks_per_cu = t->trace_blocks_per_cu;
     ctx->trace_tree = t->trace_tree;
@@ -107,6 +108,7 @@ int sgpu_set_tuning(sgpu_context ctx, const sgpu_tuning* t)
     ctx->vcm_graphs = t->vcm_graphs;
     ctx->vcm_order = t->vcm_order;
     ctx->vcm_merge_lanes = t->vcm_merge_lanes;
+    ctx->bvh_builder = t->bvh_builder;
     return SGPU_OK;
 }
 
@@ -120,6 +122,7 @@ int sgpu_get_tuning(sgpu_context ctx, sgpu_tuning* t)
     t->vcm_graphs = ctx->vcm_graphs;
     t->vcm_order = ctx->vcm_order;
     t->vcm_merge_lanes = ctx->vcm_merge_lanes;
+    t->bvh_builder = ctx->bvh_builder;
     return SGPU_OK;
 }
 

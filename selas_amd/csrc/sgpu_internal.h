@@ -86,6 +86,7 @@ struct sgpu_context_s {
     uint32_t vcm_graphs = 1;       // VCM passes replayed as captured graphs
     uint32_t vcm_order = 0;        // SGPU_ORDER_* bits: work order inside a VCM bounce
     uint32_t vcm_merge_lanes = 16;  // lanes per vertex-merging range search (8, 16 or 32)
+    uint32_t bvh_builder = 0;       // binary tree under the 8-wide one: 0 Karras (LBVH), 1 PLOC
     uint32_t* noalloc_scratch = nullptr;  // device address of this device's sgpu_intersect_noalloc scratch
     // the VCM server of the last sgpu_render_vcm(_batch) call, reused while large enough
     std::mutex vcm_mu;

@@ -216,7 +216,7 @@ class DeviceBuffer:
 class Tuning(C.Structure):
     _fields_ = [("vcm_wavefront_paths", C.c_uint64), ("trace_blocks_per_cu", C.c_uint32), ("trace_tree", C.c_uint32),
                 ("trace_lds_nodes", C.c_uint32), ("vcm_graphs", C.c_uint32), ("vcm_order", C.c_uint32),
-                ("vcm_merge_lanes", C.c_uint32)]
+                ("vcm_merge_lanes", C.c_uint32), ("bvh_builder", C.c_uint32)]
 
 
 class DeviceInfo(C.Structure):
@@ -241,15 +241,16 @@ class Context:
         check(lib.sgpu_get_tuning(self.handle, C.byref(t)))
         return {"vcm_wavefront_paths": t.vcm_wavefront_paths, "trace_blocks_per_cu": t.trace_blocks_per_cu,
                 "trace_tree": t.trace_tree, "trace_lds_nodes": t.trace_lds_nodes, "vcm_graphs": t.vcm_graphs,
-                "vcm_order": t.vcm_order, "vcm_merge_lanes": t.vcm_merge_lanes}
+                "vcm_order": t.vcm_order, "vcm_merge_lanes": t.vcm_merge_lanes, "bvh_builder": t.bvh_builder}
 
     def set_tuning(self, vcm_wavefront_paths=None, trace_blocks_per_cu=None, trace_tree=None, trace_lds_nodes=None,
-                   vcm_graphs=None, vcm_order=None, vcm_merge_lanes=None):
+                   vcm_graphs=None, vcm_order=None, vcm_merge_lanes=None, bvh_builder=None):
         cur = self.tuning()
         pick = lambda new, key: cur[key] if new is None else int(new)
         t = Tuning(pick(vcm_wavefront_paths, "vcm_wavefront_paths"), pick(trace_blocks_per_cu, "trace_blocks_per_cu"),
                    pick(trace_tree, "trace_tree"), pick(trace_lds_nodes, "trace_lds_nodes"), pick(vcm_graphs, "vcm_graphs"),
-                   pick(vcm_order, "vcm_order"), pick(vcm_merge_lanes, "vcm_merge_lanes"))
+                   pick(vcm_order, "vcm_order"), pick(vcm_merge_lanes, "vcm_merge_lanes"),
+                   pick(bvh_builder, "bvh_builder"))
         check(lib.sgpu_set_tuning(self.handle, C.byref(t)))
 
     def array(self, a: np.ndarray) -> DeviceBuffer:

@@ -223,3 +223,38 @@ def test_wide_and_binary_trees_agree(gpu_ctx, seed):
         assert stats[1]["nodes_visited"] < 0.6 * stats[0]["nodes_visited"]
     finally:
         gpu_ctx.set_tuning(**old)
+
+
+@pytest.mark.parametrize("builder", [0, 1])
+def test_both_tree_builders_match_brute_force(gpu_ctx, builder):
+    """The binary tree under the 8-wide one built as a linear BVH (0) or by locally-ordered
+    clustering (1): the same bit-exact hits and occlusion flags against FP32 brute force, on a soup
+    with coincident and degenerate triangles (equal centroids, zero-area slivers) and 2 geometries.
+    Prints the nodes visited per closest-hit ray of each tree."""
+    old = gpu_ctx.tuning()
+    try:
+        gpu_ctx.set_tuning(bvh_builder=builder)
+        rng = np.random.default_rng(41)
+        t9 = dense_soup(7, 32, 64)
+        dup = t9[:200].copy()  # coincident copies: equal Morton codes, equal boxes
+        sliver = t9[200:260].copy()
+        sliver[:, 6:9] = sliver[:, 0:3] + 1e-7 * rng.normal(size=(60, 3)).astype(np.float32)  # near-degenerate
+        tri9 = np.concatenate([t9, dup, sliver]).astype(np.float32)
+        bounds = [0, 1000, tri9.shape[0]]
+        cam = N.gen_camera_rays(W.CAMERA_POS, W.CAMERA_LOOK_AT, W.CAMERA_UP, 50.0, 48, 48, 1)
+        bnc = N.gen_bounce_rays(8, tri9, 3000)
+        rays = {k: np.concatenate([cam[k], bnc[k]]) for k in N.RAY_FIELDS}
+        print(f"builder {builder}: {tri9.shape[0]} triangles, {rays['org_x'].size} rays", flush=True)
+        gpu = gpu_trace(gpu_ctx, tri9, rays, bounds)
+        assert_same(gpu, O.closest_brute_f32(tri9, rays), bounds)
+        assert np.array_equal(gpu["occluded"], O.occluded_brute_f32(tri9, rays))
+        scene = N.Scene(gpu_ctx)
+        scene.add_soup(tri9)
+        scene.commit()
+        st = scene.intersect_stats(N.DeviceRays(gpu_ctx, rays))
+        scene.release()
+        print(f"builder {builder}: {st['nodes_visited'] / st['rays']:.2f} nodes/ray, depth {gpu['info']['max_depth']}",
+              flush=True)
+        assert st["nodes_visited"] > 0
+    finally:
+        gpu_ctx.set_tuning(**old)

@@ -13,14 +13,14 @@ def test_tuning_roundtrip_and_limits(gpu_ctx):
     old = gpu_ctx.tuning()
     try:
         gpu_ctx.set_tuning(vcm_wavefront_paths=1 << 14, trace_blocks_per_cu=3, trace_tree=0, trace_lds_nodes=9,
-                           vcm_graphs=0, vcm_order=3, vcm_merge_lanes=32)
+                           vcm_graphs=0, vcm_order=3, vcm_merge_lanes=32, bvh_builder=0)
         want = {"vcm_wavefront_paths": 1 << 14, "trace_blocks_per_cu": 3, "trace_tree": 0, "trace_lds_nodes": 9,
-                "vcm_graphs": 0, "vcm_order": 3, "vcm_merge_lanes": 32}
+                "vcm_graphs": 0, "vcm_order": 3, "vcm_merge_lanes": 32, "bvh_builder": 0}
         assert gpu_ctx.tuning() == want
         for bad in ({"vcm_wavefront_paths": 1 << 11}, {"vcm_wavefront_paths": (1 << 24) + 1},
                     {"trace_blocks_per_cu": 0}, {"trace_blocks_per_cu": 33}, {"trace_tree": 2},
                     {"trace_lds_nodes": 10}, {"vcm_graphs": 2}, {"vcm_order": 4}, {"vcm_merge_lanes": 4},
-                    {"vcm_merge_lanes": 64}):
+                    {"vcm_merge_lanes": 64}, {"bvh_builder": 2}):
             with pytest.raises(N.SgpuError):
                 gpu_ctx.set_tuning(**bad)
         assert gpu_ctx.tuning() == want
