@@ -489,7 +489,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
                                                       uint32_t* __restrict next_queue, RayBuf shadow,
                                                       uint32_t* __restrict next_count,
                                                       uint32_t* __restrict shadow_count,
-                                                      const uint32_t* __restrict order, uint32_t* __restrict merge_key)
+                                                      const uint32_t* __restrict order, uint32_t* __restrict merge_key,
+                                                      float2* __restrict cam_aux, uint32_t* __restrict pairs,
+                                                      uint32_t* __restrict pair_count)
 {
     const uint32_t nslots = w.nslots;
     const uint32_t count = *count_ptr;
@@ -647,73 +649,26 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             S.valid[(size_t)p * nslots + 0] = 2;
         }
     }
-    // ---- vertex connections to light path p (slots 1..mlv) ----
-    uint32_t vb = 0, ve = 0;
-    if (hit) {
-        vb = p == 0 ? 0 : G.path_end[p - 1];
-        ve = G.path_end[p];
-    }
-    bool vc_live = hit;
-    for (uint32_t k = 0; k < w.mlv; ++k) {
-        bool emit = false;
-        f3 so = mk3(0, 0, 0), cd = mk3(0, 0, 0);
-        float tn = 0, tf = 0;
-        const uint32_t vi = vb + k;
-        if (vc_live && vi < ve) {
-            const LightVertex lv = G.cverts[vi];
-            uint32_t lplen = lv.geom_plen & 31u;
-            if (lplen + 1 + plen <= sl.L) {
-                f3 lpos = mk3(lv.pos[0], lv.pos[1], lv.pos[2]);
-                f3 dv = lpos - x;
-                float dist2 = dot3(dv, dv);
-                float dist = sqrtf(dist2);
-                cd = dv * (1.0f / dist);
-                float cfwd, crev;
-                // a connection in either surface's plane carries nothing (see the splats)
-                f3 fc = mk3(0, 0, 0);
-                if (dot3(cd, s.n) != 0.0f && dot3(-cd, mk3(lv.n[0], lv.n[1], lv.n[2])) != 0.0f)
-                    fc = bsdf_eval(s, view, cd, cfwd, crev);
-                if (!is_zero3(fc)) {
-                    Surface ls;
-                    vertex_surface(sl, lv, w.cvtan ? &w.cvtan[vi] : nullptr, ls);
-                    float lfwd, lrev;
-                    f3 lw = mk3(lv.wfix[0], lv.wfix[1], lv.wfix[2]);
-                    f3 fl = bsdf_eval(ls, lw, -cd, lfwd, lrev);
-                    if (!is_zero3(fl)) {
-                        float cos_c = absf_(dot3(cd, s.n));
-                        float cos_l = absf_(dot3(-cd, ls.n));
-                        float ccp = cont_prob(s), lcp = cont_prob(ls);
-                        cfwd *= ccp;
-                        crev *= ccp;
-                        lfwd *= lcp;
-                        lrev *= lcp;
-                        float c_pdf_a = cfwd * cos_l / dist2;
-                        float l_pdf_a = lfwd * cos_c / dist2;
-                        float w_light = c_pdf_a * (sl.vm_weight + lv.dvcm + lv.dvc * lrev);
-                        float w_cam = l_pdf_a * (sl.vm_weight + dvcm + dvc * crev);
-                        float mis = 1.0f / (w_light + 1.0f + w_cam);
-                        f3 c = (fc * fl) * (mis / dist2);
-                        if (!is_zero3(c)) {
-                            f3 lthr = mk3(lv.thr[0], lv.thr[1], lv.thr[2]);
-                            float sb;
-                            so = offset_ray_origin(s, cd, 0.1f, &sb);
-                            tn = s.error;
-                            tf = dist - 16.0f * absf_(sb);
-                            emit = true;
-                            s3(S.c, (size_t)p * nslots + 1 + k, (thr * lthr) * c);
-                        }
-                    }
-                }
-            } else {
-                vc_live = false;  // the reference stops at the first vertex that is too long (VCM.cpp:631)
-            }
+    // ---- vertex connections to light path p (slots 1..mlv): one pair per light vertex, queued
+    // for k_cam_connect (entry j in the low 26 bits, the vertex's index on its path above) ----
+    {
+        uint32_t npair = 0;
+        if (hit) {
+            const uint32_t vb = p == 0 ? 0 : G.path_end[p - 1];
+            npair = min(G.path_end[p] - vb, w.mlv);
         }
-        uint32_t ks = wave_append(emit, shadow_count);
-        if (emit) {
-            put_ray(shadow, ks, so, cd, tn, tf, sl.scene);
-            S.ray_slot[ks] = p * nslots + 1 + k;
-            S.valid[(size_t)p * nslots + 1 + k] = 2;
+        // the wave's pairs: one atomic, then each lane's run at its exclusive prefix
+        uint32_t incl = npair;
+        for (uint32_t d = 1; d < 64; d <<= 1) {
+            const uint32_t y = (uint32_t)__shfl_up((int)incl, d, 64);
+            if (__lane_id() >= d) incl += y;
         }
+        const uint32_t wave_total = (uint32_t)__shfl((int)incl, 63, 64);
+        uint32_t base = 0;
+        if (__lane_id() == 0 && wave_total) base = atomicAdd(pair_count, wave_total);
+        base = (uint32_t)__shfl((int)base, 0, 64) + incl - npair;
+        for (uint32_t k = 0; k < npair; ++k) pairs[base + k] = j | (k << kPairShift);
+        if (hit) cam_aux[j] = make_float2(dvc, s.error);
     }
     // ---- vertex merging (slot nslots - 1): handed to k_cam_merge ----
     if (active) {
@@ -766,6 +721,103 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     if (cont) {
         put_ray(next_rays, kn, next_o, next_d, kRayEps, kFloatMax, sl.scene);
         next_queue[kn] = p;
+    }
+}
+
+// Vertex connections (ConnectVertices, VCM.cpp:631-666), one queued (camera vertex, light vertex)
+// pair per lane, grid-stride: the camera vertex's surface is rebuilt from its merge record (the
+// same inputs CalculateSurfaceParams had, so the same values), the light vertex's from its stored
+// record; a connection that carries light gets its contribution slot and a shadow ray.
+__global__ __launch_bounds__(kBlock) void k_cam_connect(Wave w, const uint32_t* __restrict count_ptr,
+                                                        const uint32_t* __restrict pairs,
+                                                        const CamVertex* __restrict cam_vm,
+                                                        const float2* __restrict cam_aux, PathState ps, GridView G,
+                                                        CamSlots S, RayBuf shadow, uint32_t* __restrict shadow_count)
+{
+    const uint32_t count = *count_ptr;
+    const uint32_t nslots = w.nslots;
+    for (uint32_t i0 = blockIdx.x * kBlock; i0 < count; i0 += gridDim.x * kBlock) {
+        const uint32_t i = i0 + threadIdx.x;
+        bool emit = false;
+        f3 so = mk3(0, 0, 0), cd = mk3(0, 0, 0);
+        float tn = 0, tf = 0;
+        uint32_t slot = 0, scn = 0;
+        if (i < count) {
+            const uint32_t pr = pairs[i];
+            const uint32_t j = pr & ((1u << kPairShift) - 1u), k = pr >> kPairShift;
+            const CamVertex v = cam_vm[j];
+            const uint32_t p = v.path;
+            const Slice& sl = w.slices[ps.slice[p]];
+            scn = sl.scene;
+            const uint32_t vi = (p == 0 ? 0 : G.path_end[p - 1]) + k;
+            const LightVertex lv = G.cverts[vi];
+            const uint32_t plen = v.geom_plen & 31u;
+            const uint32_t lplen = lv.geom_plen & 31u;
+            // the reference stops at the first light vertex that makes the path too long; the
+            // vertices of a light path are stored in order of length, so this test is the same
+            if (lplen + 1 + plen <= sl.L) {
+                const float2 aux = cam_aux[j];
+                const f3 x = mk3(v.pos[0], v.pos[1], v.pos[2]);
+                const f3 view = mk3(v.view[0], v.view[1], v.view[2]);
+                const uint32_t geom = v.geom_plen >> 5;
+                Surface s;
+                s.position = x;
+                s.error = aux.y;
+                surface_material(sl.materials + 22 * (size_t)sl.geom_material[geom], mk3(v.n[0], v.n[1], v.n[2]), view,
+                                 s);
+                if (geom_has_tangents(sl, geom)) tangent_frame(s.n, w.cam_tan[j], s);
+                s.base_color = mk3(v.base[0], v.base[1], v.base[2]);
+                const float dvcm = v.dvcm, dvc = aux.x;
+                f3 lpos = mk3(lv.pos[0], lv.pos[1], lv.pos[2]);
+                f3 dv = lpos - x;
+                float dist2 = dot3(dv, dv);
+                float dist = sqrtf(dist2);
+                cd = dv * (1.0f / dist);
+                float cfwd, crev;
+                // a connection in either surface's plane carries nothing (see the splats)
+                f3 fc = mk3(0, 0, 0);
+                if (dot3(cd, s.n) != 0.0f && dot3(-cd, mk3(lv.n[0], lv.n[1], lv.n[2])) != 0.0f)
+                    fc = bsdf_eval(s, view, cd, cfwd, crev);
+                if (!is_zero3(fc)) {
+                    Surface ls;
+                    vertex_surface(sl, lv, w.cvtan ? &w.cvtan[vi] : nullptr, ls);
+                    float lfwd, lrev;
+                    f3 lw = mk3(lv.wfix[0], lv.wfix[1], lv.wfix[2]);
+                    f3 fl = bsdf_eval(ls, lw, -cd, lfwd, lrev);
+                    if (!is_zero3(fl)) {
+                        float cos_c = absf_(dot3(cd, s.n));
+                        float cos_l = absf_(dot3(-cd, ls.n));
+                        float ccp = cont_prob(s), lcp = cont_prob(ls);
+                        cfwd *= ccp;
+                        crev *= ccp;
+                        lfwd *= lcp;
+                        lrev *= lcp;
+                        float c_pdf_a = cfwd * cos_l / dist2;
+                        float l_pdf_a = lfwd * cos_c / dist2;
+                        float w_light = c_pdf_a * (sl.vm_weight + lv.dvcm + lv.dvc * lrev);
+                        float w_cam = l_pdf_a * (sl.vm_weight + dvcm + dvc * crev);
+                        float mis = 1.0f / (w_light + 1.0f + w_cam);
+                        f3 c = (fc * fl) * (mis / dist2);
+                        if (!is_zero3(c)) {
+                            f3 lthr = mk3(lv.thr[0], lv.thr[1], lv.thr[2]);
+                            float sb;
+                            so = offset_ray_origin(s, cd, 0.1f, &sb);
+                            tn = s.error;
+                            tf = dist - 16.0f * absf_(sb);
+                            emit = true;
+                            slot = p * nslots + 1 + k;
+                            s3(S.c, (size_t)slot, (mk3(v.thr[0], v.thr[1], v.thr[2]) * lthr) * c);
+                        }
+                    }
+                }
+            }
+        }
+        const uint32_t ks = wave_append(emit, shadow_count);
+        if (emit) {
+            put_ray(shadow, ks, so, cd, tn, tf, scn);
+            S.ray_slot[ks] = slot;
+            S.valid[(size_t)slot] = 2;
+        }
     }
 }
 
@@ -1040,6 +1092,8 @@ struct sgpu_vcm_server_s {
     bool graph_failed = false;
     Slice* d_slices = nullptr;
     CamVertex* cam_vm = nullptr;
+    float2* cam_aux = nullptr;  // per camera entry: dvc and the hit's error bound (vertex connections)
+    uint32_t* pairs = nullptr;  // queued vertex connections of a bounce (TM x mlv)
     BinBufs bins;  // the work-order sort (tuning vcm_order)
     uint32_t* spill = nullptr;
     uint32_t spill_cap = 0;           // stack entries the spill buffer is sized for
@@ -1196,6 +1250,8 @@ int server_alloc(sgpu_vcm_server sv)
     sv->frame = A.get<float>(3 * (size_t)TM);
     sv->d_slices = A.get<Slice>(sv->max_slices);
     sv->cam_vm = A.get<CamVertex>(TM);
+    sv->cam_aux = A.get<float2>(TM);
+    sv->pairs = A.get<uint32_t>((size_t)TM * mlv);
     sv->bins.key = A.get<uint32_t>(TM);
     sv->bins.order = A.get<uint32_t>(TM);
     sv->bins.total = A.get<uint32_t>(kBins);
@@ -1636,10 +1692,17 @@ int sgpu_vcm_server_step_impl(sgpu_vcm_server sv, hipStream_t stream, uint64_t* 
                 hipLaunchKernelGGL(k_light_shade, dim3(grid1(n)), dim3(kBlock), 0, st, wv, c_in, queue[cur], rays[cur],
                                    hb, ps, sv->verts, sv->vcount, rays[1 - cur], queue[1 - cur], shadow, sv->splat_c,
                                    sv->splat_pix, c_next, c_shadow, shade_order);
-            else
+            else {
+                uint32_t* c_pairs = c_in + 2;
                 hipLaunchKernelGGL(k_cam_shade, dim3(grid1(n)), dim3(kBlock), 0, st, wv, c_in, queue[cur], rays[cur],
                                    hb, ps, G, S, sv->cam_vm, rays[1 - cur], queue[1 - cur], shadow, c_next, c_shadow,
-                                   shade_order, merge_sorted ? sv->bins.key : nullptr);
+                                   shade_order, merge_sorted ? sv->bins.key : nullptr, sv->cam_aux, sv->pairs,
+                                   c_pairs);
+                SGPU_HIP_TRY(hipGetLastError());
+                // the bounce's vertex connections, one pair per lane (grid-stride over the device count)
+                hipLaunchKernelGGL(k_cam_connect, dim3(grid_for((uint64_t)n * sv->mlv)), dim3(kBlock), 0, st, wv,
+                                   c_pairs, sv->pairs, sv->cam_vm, sv->cam_aux, ps, G, S, shadow, c_shadow);
+            }
             SGPU_HIP_TRY(hipGetLastError());
             SGPU_HIP_TRY(T.end());
             if (pass == 1) {

@@ -74,7 +74,12 @@ struct LightVertex {
 };
 static_assert(sizeof(LightVertex) == 80, "LightVertex must be 80 bytes");
 
-// Camera vertex handed from k_cam_shade to k_cam_merge (80 B); geom_plen = kNoQuery: no query.
+// A queued vertex connection: the camera entry j | (the light vertex's index on its path << kPairShift)
+// (wavefronts hold at most 2^26 paths and paths at most 31 vertices).
+constexpr uint32_t kPairShift = 26;
+
+// Camera vertex handed from k_cam_shade to k_cam_merge and k_cam_connect (80 B); geom_plen =
+// kNoQuery: no hit.
 struct CamVertex {
     float pos[3];
     float n[3];
