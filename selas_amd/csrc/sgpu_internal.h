@@ -80,7 +80,7 @@ struct sgpu_context_s {
     int device;
     hipDeviceProp_t props;
     uint64_t vcm_wavefront_paths = 1u << 20;
-    uint32_t trace_blocks_per_cu = 8;
+    uint32_t trace_blocks_per_cu = 32;
     uint32_t trace_tree = 1;  // 0: binary nodes, 1: 8-wide compressed nodes
     uint32_t trace_lds_nodes = 0;  // top wide nodes read from LDS (single-scene launches)
     uint32_t vcm_graphs = 1;       // VCM passes replayed as captured graphs
