@@ -243,6 +243,65 @@ int sgpu_fail(int code, const std::string& msg);
             return sgpu_fail(SGPU_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));   \
     } while (0)
 
+// Appends to a device queue with one atomic per block of BLOCK threads: returns the slot of a
+// thread with pred (0xFFFFFFFF for the others), slots of a block consecutive in thread order. Every
+// thread of the block calls it, the block's appends in the same order. Same-address atomics
+// serialize in the L2: one per wave costs milliseconds on a launch of millions of threads, one per
+// block a quarter of that.
+template <uint32_t BLOCK>
+__device__ __forceinline__ uint32_t sgpu_block_append(bool pred, uint32_t* counter)
+{
+    __shared__ uint32_t s_wave[BLOCK / 64];
+    __shared__ uint32_t s_base;
+    const uint64_t mask = __ballot(pred);
+    const uint32_t lane = __lane_id(), wid = threadIdx.x >> 6;
+    if (lane == 0) s_wave[wid] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (uint32_t k = 0; k < BLOCK / 64; ++k) {
+            const uint32_t c = s_wave[k];
+            s_wave[k] = run;
+            run += c;
+        }
+        s_base = run ? atomicAdd(counter, run) : 0u;
+    }
+    __syncthreads();
+    const uint32_t r = s_base + s_wave[wid] + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    __syncthreads();  // the next append reuses s_wave and s_base
+    return pred ? r : 0xFFFFFFFFu;
+}
+
+// The first of n consecutive slots for each thread (n may differ per thread), one atomic per
+// block; every thread of the block calls it.
+template <uint32_t BLOCK>
+__device__ __forceinline__ uint32_t sgpu_block_reserve(uint32_t n, uint32_t* counter)
+{
+    __shared__ uint32_t s_wave[BLOCK / 64];
+    __shared__ uint32_t s_base;
+    const uint32_t lane = __lane_id(), wid = threadIdx.x >> 6;
+    uint32_t incl = n;
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)incl, d, 64);
+        if (lane >= d) incl += y;
+    }
+    if (lane == 63) s_wave[wid] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (uint32_t k = 0; k < BLOCK / 64; ++k) {
+            const uint32_t c = s_wave[k];
+            s_wave[k] = run;
+            run += c;
+        }
+        s_base = run ? atomicAdd(counter, run) : 0u;
+    }
+    __syncthreads();
+    const uint32_t r = s_base + s_wave[wid] + incl - n;
+    __syncthreads();  // the next reservation reuses s_wave and s_base
+    return r;
+}
+
 // ---- internal entry points between translation units --------------------------------------------
 // Builds d_nodes/d_tris (and the 8-wide d_wide) for the scene's triangles (sgpu_bvh.hip).
 int sgpu_bvh_build(sgpu_scene scene, hipStream_t stream);

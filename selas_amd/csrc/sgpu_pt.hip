@@ -39,19 +39,10 @@ using namespace sgpu;
 constexpr int kBlock = 256;
 constexpr float kFloatMax = 3.402823466e+38F;
 
-// Append with one atomic per wave: ballot the predicate, count the lanes below, let the first
-// active lane reserve the wave's slots. Returns the slot for lanes with pred, else ~0.
-__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter)
+// Queue appends with one atomic per block (sgpu_internal.h).
+__device__ __forceinline__ uint32_t block_append(bool pred, uint32_t* counter)
 {
-    const uint64_t mask = __ballot(pred);
-    if (mask == 0) return 0xFFFFFFFFu;
-    const uint32_t lane = __lane_id();
-    const uint32_t below = __popcll(mask & ((1ull << lane) - 1ull));
-    const uint32_t leader = __ffsll((unsigned long long)mask) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader, 64);
-    return pred ? base + below : 0xFFFFFFFFu;
+    return sgpu_block_append<kBlock>(pred, counter);
 }
 
 struct PathState {
@@ -415,22 +406,22 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(ShadeParams sp, uint32_t co
             ps.rng[p] = u.r.state;
         }
     }
-    uint32_t ks = wave_append(emit, &counters[1]);
+    uint32_t ks = block_append(emit, &counters[1]);
     if (emit) {
         put_ray(shadow.ray, ks, sh_o, sh_d, sh_tnear, sh_tfar);
         shadow.path[ks] = p;
         set3(shadow.c, ks, c);
     }
     if (DPT) {
-        uint32_t kk = wave_append(emit_sky, &counters[2]);
+        uint32_t kk = block_append(emit_sky, &counters[2]);
         if (emit_sky) {
             put_ray(sky.ray, kk, sk_o, sk_d, 0.0f, 1e36f);
             sky.path[kk] = p;
             set3(sky.c, kk, sk_c);
         }
     }
-    uint32_t kn = wave_append(cont, &counters[0]);
-    (void)wave_append(hit, &counters[3]);
+    uint32_t kn = block_append(cont, &counters[0]);
+    (void)block_append(hit, &counters[3]);
     if (cont) {
         put_ray(next_rays, kn, next_o, next_d, 0.0f, kFloatMax);
         next_queue[kn] = p;

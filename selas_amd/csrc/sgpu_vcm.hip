@@ -139,7 +139,7 @@ __global__ __launch_bounds__(kBlock) void k_light_gen(Wave w, PathState ps, RayB
         }
         ps.rng[g] = u.r.state;
     }
-    uint32_t k = wave_append(live, &counters[0]);
+    uint32_t k = block_append(live, &counters[0]);
     if (live) {
         put_ray(rays, k, o, d, kRayEps, kFloatMax, scn);
         queue[k] = g;
@@ -284,13 +284,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             ps.rng[p] = u.r.state;
         }
     }
-    uint32_t ks = wave_append(emit, shadow_count);
+    uint32_t ks = block_append(emit, shadow_count);
     if (emit) {
         put_ray(shadow, ks, sh_o, sh_d, sh_tnear, sh_tfar, scn);
         s3(splat_c, ks, c);
         splat_pix[ks] = pix;
     }
-    uint32_t kn = wave_append(cont, next_count);
+    uint32_t kn = block_append(cont, next_count);
     if (cont) {
         put_ray(next_rays, kn, next_o, next_d, kRayEps, kFloatMax, scn);
         next_queue[kn] = p;
@@ -476,7 +476,7 @@ __global__ __launch_bounds__(kBlock) void k_cam_gen(Wave w, PathState ps, RayBuf
         ps.rng[g] = u.r.state;
         live = sl.L > 1;  // a camera path of length 1 ends at the lens (VCM.cpp:585)
     }
-    uint32_t k = wave_append(live, &counters[0]);
+    uint32_t k = block_append(live, &counters[0]);
     if (live) {
         put_ray(rays, k, o, d, kRayEps, kFloatMax, scn);
         queue[k] = g;
@@ -642,7 +642,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
                 }
             }
         }
-        uint32_t ks = wave_append(emit, shadow_count);
+        uint32_t ks = block_append(emit, shadow_count);
         if (emit) {
             put_ray(shadow, ks, so, ld, tn, tf, sl.scene);
             S.ray_slot[ks] = p * nslots + 0;
@@ -657,16 +657,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             const uint32_t vb = p == 0 ? 0 : G.path_end[p - 1];
             npair = min(G.path_end[p] - vb, w.mlv);
         }
-        // the wave's pairs: one atomic, then each lane's run at its exclusive prefix
-        uint32_t incl = npair;
-        for (uint32_t d = 1; d < 64; d <<= 1) {
-            const uint32_t y = (uint32_t)__shfl_up((int)incl, d, 64);
-            if (__lane_id() >= d) incl += y;
-        }
-        const uint32_t wave_total = (uint32_t)__shfl((int)incl, 63, 64);
-        uint32_t base = 0;
-        if (__lane_id() == 0 && wave_total) base = atomicAdd(pair_count, wave_total);
-        base = (uint32_t)__shfl((int)base, 0, 64) + incl - npair;
+        // the block's pairs: one atomic, then each lane's run at its exclusive prefix
+        const uint32_t base = block_reserve(npair, pair_count);
         for (uint32_t k = 0; k < npair; ++k) pairs[base + k] = j | (k << kPairShift);
         if (hit) cam_aux[j] = make_float2(dvc, s.error);
     }
@@ -717,7 +709,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         }
         ps.rng[p] = u.r.state;
     }
-    uint32_t kn = wave_append(cont, next_count);
+    uint32_t kn = block_append(cont, next_count);
     if (cont) {
         put_ray(next_rays, kn, next_o, next_d, kRayEps, kFloatMax, sl.scene);
         next_queue[kn] = p;
@@ -812,7 +804,7 @@ __global__ __launch_bounds__(kBlock) void k_cam_connect(Wave w, const uint32_t* 
                 }
             }
         }
-        const uint32_t ks = wave_append(emit, shadow_count);
+        const uint32_t ks = block_append(emit, shadow_count);
         if (emit) {
             put_ray(shadow, ks, so, cd, tn, tf, scn);
             S.ray_slot[ks] = slot;

@@ -35,6 +35,16 @@ __device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter)
     return pred ? base + below : 0xFFFFFFFFu;
 }
 
+// Queue appends with one atomic per block (sgpu_internal.h), for this file's block size.
+__device__ __forceinline__ uint32_t block_append(bool pred, uint32_t* counter)
+{
+    return sgpu_block_append<kBlock>(pred, counter);
+}
+__device__ __forceinline__ uint32_t block_reserve(uint32_t n, uint32_t* counter)
+{
+    return sgpu_block_reserve<kBlock>(n, counter);
+}
+
 struct RayBuf {
     float *ox, *oy, *oz, *dx, *dy, *dz, *tnear, *tfar;
     uint32_t* scene;  // per ray, the scene it traverses (multi-scene wavefronts only, else nullptr)
