@@ -107,9 +107,10 @@ int sgpu_context_destroy(sgpu_context ctx);
 typedef struct sgpu_tuning {
     uint64_t vcm_wavefront_paths; /* paths per wavefront of sgpu_render_vcm(_batch): 2^12..2^24
                                      (default 2^20); a larger image than this gets one slice each */
-    uint32_t trace_blocks_per_cu; /* traversal grid: at most this many 256-lane blocks per CU; each
-                                     wave takes an even share of the rays, a lane the next ray of its
-                                     wave's share when its ray is done: 1..32 (default 32) */
+    uint32_t trace_blocks_per_cu; /* traversal grid: at most this many 256-lane blocks per CU; waves
+                                     take the rays in chunks of 64 (the next chunk from a counter when
+                                     one is used up), a lane the next ray of its wave's chunk when its
+                                     ray is done: 1..32 (default 8) */
     uint32_t trace_tree;          /* traversal tree: 0 binary nodes (64 B), 1 8-wide compressed
                                      nodes (128 B, default); both are built at commit */
     uint32_t trace_lds_nodes;     /* 8-wide single-scene launches: this many top nodes (breadth-first)

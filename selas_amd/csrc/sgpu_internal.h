@@ -80,7 +80,7 @@ struct sgpu_context_s {
     int device;
     hipDeviceProp_t props;
     uint64_t vcm_wavefront_paths = 1u << 20;
-    uint32_t trace_blocks_per_cu = 32;
+    uint32_t trace_blocks_per_cu = 8;
     uint32_t trace_tree = 1;  // 0: binary nodes, 1: 8-wide compressed nodes
     uint32_t trace_lds_nodes = 0;  // top wide nodes read from LDS (single-scene launches)
     uint32_t vcm_graphs = 1;       // VCM passes replayed as captured graphs
@@ -309,8 +309,9 @@ int sgpu_bvh_build(sgpu_scene scene, hipStream_t stream);
 // composed instances, d_inst, d_inst_ids) (sgpu_bvh.hip).
 int sgpu_bvh_build_instances(sgpu_scene scene, hipStream_t stream);
 // Launches the traversal kernels (sgpu_trace.hip), on the context's tree kind (trace_tree).
-// `spill` (may be NULL: then the call allocates its own, stream-ordered) holds the spilled part of
-// the per-lane stacks: at least sgpu_trace_spill_words(scene) 32-bit words, not shared with a
+// `spill` (may be NULL: then the call allocates its own, stream-ordered) holds the launch's ray
+// counters (zeroed by the launch) and the spilled part of the per-lane stacks: at least
+// sgpu_trace_spill_words(scene) 32-bit words, not shared with a
 // concurrent launch. With n_ptr, the kernels read the ray count from that device word (n then only
 // sizes the grid and must bound it); `overflow` replaces the scene's stack-overflow flag.
 // For an instanced scene, comp_out (may be NULL) receives each closest hit's composed instance.
@@ -326,6 +327,10 @@ int sgpu_check_query(sgpu_scene scene, const sgpu_rays_soa* rays, uint32_t n, co
 int sgpu_noalloc_init(sgpu_context ctx);
 size_t sgpu_trace_spill_words(sgpu_scene scene);
 size_t sgpu_trace_spill_words(sgpu_context ctx, uint32_t stack_cap);
+// A spill buffer starts with kTraceWorkWords words of ray counters; a traversal launch clears them
+// on its stream (a small kernel, so that it may be captured) before it starts.
+constexpr uint32_t kTraceWorkWords = 64;
+void sgpu_trace_work_zero(uint32_t* spill, hipStream_t stream);
 // A scene's trees as a multi-scene traversal launch reads them (one table entry per scene).
 struct TraceScene {
     const BvhNode* nodes;

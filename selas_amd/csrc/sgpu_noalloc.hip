@@ -4,7 +4,8 @@
 // the per-lane traversal stacks (entries deeper than the SGPU_STACK_LDS kept in LDS, laid out
 // [depth][thread]); here it is a fixed array in device memory, one per GPU, held until the process
 // ends. The traversal grid is capped so that its spill fits the array (rays beyond the grid are
-// taken grid-stride, as in every traversal launch); the results are those of sgpu_intersect.
+// taken in turn by the grid's waves, as in every traversal launch); the results are those of
+// sgpu_intersect. The array's first words are the traversal's ray counters, cleared by each launch.
 #include <mutex>
 
 #include "sgpu_internal.h"
@@ -51,7 +52,7 @@ extern "C" int sgpu_intersect_noalloc(sgpu_scene scene, const sgpu_rays_soa* ray
     if (!ptr) return sgpu_fail(SGPU_E_STATE, "sgpu_intersect_noalloc: scratch not set up for this device");
     // blocks whose spilled stacks fit the scratch
     const size_t depth = scene->stack_cap > SGPU_STACK_LDS ? scene->stack_cap - SGPU_STACK_LDS : 1;
-    const size_t fit = kScratchWords / (depth * SGPU_TRACE_BLOCK);
+    const size_t fit = (kScratchWords - kTraceWorkWords) / (depth * SGPU_TRACE_BLOCK);
     if (fit == 0)
         return sgpu_fail(SGPU_E_RANGE, "sgpu_intersect_noalloc: the scene's traversal stack (" +
                                            std::to_string(scene->stack_cap) + " entries) needs more than the " +

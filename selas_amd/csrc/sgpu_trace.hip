@@ -241,16 +241,18 @@ __device__ __forceinline__ void cswap(float& ka, int& ra, float& kb, int& rb)
 // internal children whose box is hit are ordered by entry distance with a fixed sorting network
 // (closest hit; the any-hit query keeps slot order), the nearest is entered and the rest are pushed
 // far to near. Hits are the triangle test's, independent of the tree and of the visiting order.
-// Each wave owns a run of the rays and a lane whose ray is done takes the next one at once (one
-// node visit per loop trip), so lanes do not wait for the wave's longest ray. Measured on MI355X:
-// the traversal workload (16M rays, 2M triangles) goes from 1027 to 1553 Mrays/s against the
-// one-ray-per-lane loop, the VCM job's traversal is unchanged (its launches hold about one ray per
-// lane); held to 7 waves per SIMD (72 VGPRs): at 8 the refill state spills and costs more than the
-// eighth wave gives.
+// Waves take rays in chunks of 64 and a lane whose ray is done takes the next one of its wave's
+// chunk at once (one node visit per loop trip), so lanes do not wait for the wave's longest ray;
+// a wave whose chunk is used up takes the next from a counter, so waves do not wait for the
+// launch's longest chunk. Measured on MI355X against the one-ray-per-lane grid-stride loop: the
+// traversal workload (16M rays, 2M triangles) 1027 -> 1824 Mrays/s, the VCM job's closest-hit and
+// shadow traversals 13.2 + 12.2 -> 13.4 + 11.2 ms, both at 8 blocks per CU; held to 7 waves per
+// SIMD (72 VGPRs): at 8 the refill state spills and costs more than the eighth wave gives.
 // Wide nodes of a single-scene launch that may be kept in LDS (sgpu_tuning.trace_lds_nodes): the root
 // and the next breadth-first nodes (the root's children when it has 8 internal ones), visited by
 // nearly every ray. Measured on MI355X they hit in L1/L2 anyway, and the LDS branch costs 1-3 %.
 constexpr uint32_t kTopNodes = 9;
+constexpr uint32_t kTraceChunk = 64;  // rays a wave takes from the launch's counter at a time
 
 template <bool ANY, bool STATS, bool MULTI>
 __global__ __launch_bounds__(SGPU_TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(7))) void k_trace8(const WideNode* __restrict wnodes, uint32_t n_top,
@@ -262,7 +264,7 @@ __global__ __launch_bounds__(SGPU_TRACE_BLOCK) __attribute__((amdgpu_waves_per_e
                                                             const uint32_t* __restrict n_ptr,
                                                              uint32_t* __restrict spill, uint32_t stack_cap,
                                                              unsigned long long* __restrict stats,
-                                                             uint32_t* __restrict overflow)
+                                                             uint32_t* __restrict overflow, uint32_t* __restrict work)
 {
     __shared__ uint32_t lds_stack[SGPU_STACK_LDS][SGPU_TRACE_BLOCK];
     __shared__ uint4 lds_top[kTopNodes * 6];  // the 96 bytes of each top node the traversal reads
@@ -276,14 +278,17 @@ __global__ __launch_bounds__(SGPU_TRACE_BLOCK) __attribute__((amdgpu_waves_per_e
         lds_top[k] = reinterpret_cast<const uint4*>(wnodes + k / 6)[k % 6];
     __syncthreads();
 
-    // The wave's run of rays (n split evenly over the grid's waves). A lane whose ray is done takes
-    // the next ray of the run at once, so that lanes do not idle while the wave's longest ray is
-    // traversed; the lanes' positions in the run come from a ballot, so `next` stays wave-uniform.
-    const uint32_t waves = (gridDim.x * SGPU_TRACE_BLOCK) >> 6;
-    const uint32_t per = (n + waves - 1) / waves;
-    const uint32_t wave_id = gtid >> 6;
-    uint32_t next = min(n, wave_id * per);
-    const uint32_t end = min(n, next + per);
+    // Rays come in chunks of 64: wave w's first chunk is chunk w, the next ones come from the
+    // launch's counter (work[0], zeroed before the launch) past the grid's first chunks. A lane
+    // whose ray is done takes the next ray of its wave's chunk at once, and a wave whose chunk is
+    // used up takes the next chunk, so that lanes do not idle while a wave's longest ray is
+    // traversed and waves do not idle while others hold rays; a launch of at most one chunk per
+    // wave takes no atomic. Positions in a chunk come from a ballot, so `next` and `end` stay
+    // wave-uniform.
+    const uint32_t grid_rays = (gridDim.x * SGPU_TRACE_BLOCK / 64) * kTraceChunk;
+    uint32_t next = min(n, (gtid >> 6) * kTraceChunk);
+    uint32_t end = min(n, next + kTraceChunk);
+    bool drained = false;
     const unsigned long long below = (1ull << (threadIdx.x & 63u)) - 1ull;
 
     bool active = false;
@@ -339,8 +344,23 @@ __global__ __launch_bounds__(SGPU_TRACE_BLOCK) __attribute__((amdgpu_waves_per_e
     };
 
     for (;;) {
-        // lanes without a ray take the next rays of the run
+        // lanes without a ray take the next rays of the chunk (a new chunk when it is used up)
         const unsigned long long idle = __ballot(!active);
+        if (idle && next >= end && !drained) {
+            if (grid_rays >= n) {
+                drained = true;  // the first chunks held every ray
+            } else {
+                uint32_t base = 0;
+                if ((threadIdx.x & 63u) == 0) base = grid_rays + atomicAdd(&work[0], kTraceChunk);
+                base = (uint32_t)__shfl((int)base, 0, 64);
+                if (base >= n) {
+                    drained = true;
+                } else {
+                    next = base;
+                    end = min(n, base + kTraceChunk);
+                }
+            }
+        }
         if (idle && next < end) {
             const uint32_t r = next + (uint32_t)__popcll(idle & below);
             if (!active && r < end) {
@@ -747,15 +767,18 @@ void launch_trace(bool wide, bool multi, uint32_t blocks, hipStream_t stream, co
                   const WideNode* wnodes, uint32_t n_top, const TriRecord* tris, uint32_t n_tris, const TraceScene* table,
                   const uint32_t* ray_scene, const sgpu_rays_soa& rays, const sgpu_hits_soa& h, uint32_t* occluded,
                   uint32_t n, const uint32_t* n_ptr, uint32_t* spill, uint32_t cap, unsigned long long* st,
-                  uint32_t* overflow)
+                  uint32_t* overflow, uint32_t* work)
 {
     if (wide) {
+        sgpu_trace_work_zero(work, stream);
         if (multi)
             hipLaunchKernelGGL((k_trace8<ANY, STATS, true>), dim3(blocks), dim3(SGPU_TRACE_BLOCK), 0, stream, wnodes,
-                               n_top, tris, n_tris, table, ray_scene, rays, h, occluded, n, n_ptr, spill, cap, st, overflow);
+                               n_top, tris, n_tris, table, ray_scene, rays, h, occluded, n, n_ptr, spill, cap, st, overflow,
+                               work);
         else
             hipLaunchKernelGGL((k_trace8<ANY, STATS, false>), dim3(blocks), dim3(SGPU_TRACE_BLOCK), 0, stream, wnodes,
-                               n_top, tris, n_tris, table, ray_scene, rays, h, occluded, n, n_ptr, spill, cap, st, overflow);
+                               n_top, tris, n_tris, table, ray_scene, rays, h, occluded, n, n_ptr, spill, cap, st, overflow,
+                               work);
     } else {
         if (multi)
             hipLaunchKernelGGL((k_trace<ANY, STATS, true>), dim3(blocks), dim3(SGPU_TRACE_BLOCK), 0, stream, nodes,
@@ -783,10 +806,14 @@ int trace_launch(sgpu_context ctx, const BvhNode* nodes, const WideNode* wnodes,
     if (blocks > max_blocks) blocks = max_blocks;
 
     // stream-ordered scratch for the spilled part of the stacks, private to this call, unless the
-    // caller lends a buffer of sgpu_trace_spill_words (sized for the largest grid)
+    // caller lends a buffer of sgpu_trace_spill_words (sized for the largest grid); its first
+    // kTraceWorkWords words are the launch's ray counters (zero between launches)
     uint32_t* spill = spill_buf;
-    if (!spill) SGPU_HIP_TRY(hipMallocAsync((void**)&spill, sgpu_trace_spill_words(ctx, stack_cap) * sizeof(uint32_t),
-                                            stream));
+    if (!spill)
+        SGPU_HIP_TRY(hipMallocAsync((void**)&spill, sgpu_trace_spill_words(ctx, stack_cap) * sizeof(uint32_t), stream));
+    uint32_t* const buf = spill;
+    uint32_t* const work = spill;
+    spill += kTraceWorkWords;
     sgpu_hits_soa h = {};
     if (hits) h = *hits;
     unsigned long long* st = (unsigned long long*)d_stats;
@@ -808,26 +835,26 @@ int trace_launch(sgpu_context ctx, const BvhNode* nodes, const WideNode* wnodes,
         }
 #undef SGPU_LAUNCH_IM
         SGPU_HIP_TRY(hipGetLastError());
-        if (!spill_buf) SGPU_HIP_TRY(hipFreeAsync(spill, stream));
+        if (!spill_buf) SGPU_HIP_TRY(hipFreeAsync(buf, stream));
         return SGPU_OK;
     }
     if (any_hit) {
         if (st)
             launch_trace<true, true>(wide, multi, blocks, stream, nodes, wnodes, n_top, tris, n_tris, table, ray_scene, *rays,
-                                     h, occluded, n, n_ptr, spill, stack_cap, st, overflow);
+                                     h, occluded, n, n_ptr, spill, stack_cap, st, overflow, work);
         else
             launch_trace<true, false>(wide, multi, blocks, stream, nodes, wnodes, n_top, tris, n_tris, table, ray_scene, *rays,
-                                      h, occluded, n, n_ptr, spill, stack_cap, st, overflow);
+                                      h, occluded, n, n_ptr, spill, stack_cap, st, overflow, work);
     } else {
         if (st)
             launch_trace<false, true>(wide, multi, blocks, stream, nodes, wnodes, n_top, tris, n_tris, table, ray_scene, *rays,
-                                      h, occluded, n, n_ptr, spill, stack_cap, st, overflow);
+                                      h, occluded, n, n_ptr, spill, stack_cap, st, overflow, work);
         else
             launch_trace<false, false>(wide, multi, blocks, stream, nodes, wnodes, n_top, tris, n_tris, table, ray_scene,
-                                       *rays, h, occluded, n, n_ptr, spill, stack_cap, st, overflow);
+                                       *rays, h, occluded, n, n_ptr, spill, stack_cap, st, overflow, work);
     }
     SGPU_HIP_TRY(hipGetLastError());
-    if (!spill_buf) SGPU_HIP_TRY(hipFreeAsync(spill, stream));
+    if (!spill_buf) SGPU_HIP_TRY(hipFreeAsync(buf, stream));
     return SGPU_OK;
 }
 
@@ -838,7 +865,19 @@ size_t sgpu_trace_spill_words(sgpu_context ctx, uint32_t stack_cap)
     const uint32_t cus = (uint32_t)ctx->props.multiProcessorCount;
     const uint32_t threads = cus * kMaxTraceBlocksPerCu * SGPU_TRACE_BLOCK;
     const uint32_t spill_depth = stack_cap > SGPU_STACK_LDS ? stack_cap - SGPU_STACK_LDS : 1;
-    return (size_t)spill_depth * threads;
+    return kTraceWorkWords + (size_t)spill_depth * threads;
+}
+
+namespace {
+__global__ void k_zero_work(uint32_t* __restrict w)
+{
+    if (threadIdx.x < kTraceWorkWords) w[threadIdx.x] = 0u;
+}
+}  // namespace
+
+void sgpu_trace_work_zero(uint32_t* spill, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_zero_work, dim3(1), dim3(kTraceWorkWords), 0, stream, spill);
 }
 
 size_t sgpu_trace_spill_words(sgpu_scene scene) { return sgpu_trace_spill_words(scene->ctx, scene->stack_cap); }
@@ -861,6 +900,8 @@ int trace_launch_instanced(sgpu_scene scene, const sgpu_rays_soa* rays, const sg
     if (!spill)
         SGPU_HIP_TRY(hipMallocAsync((void**)&spill, sgpu_trace_spill_words(ctx, scene->stack_cap) * sizeof(uint32_t),
                                     stream));
+    uint32_t* const buf = spill;
+    spill += kTraceWorkWords;  // the counter words (unused by the instanced traversal) precede the spill
     sgpu_hits_soa h = {};
     if (hits) h = *hits;
     unsigned long long* st = (unsigned long long*)d_stats;
@@ -878,7 +919,7 @@ int trace_launch_instanced(sgpu_scene scene, const sgpu_rays_soa* rays, const sg
     }
 #undef SGPU_LAUNCH_I
     SGPU_HIP_TRY(hipGetLastError());
-    if (!spill_buf) SGPU_HIP_TRY(hipFreeAsync(spill, stream));
+    if (!spill_buf) SGPU_HIP_TRY(hipFreeAsync(buf, stream));
     return SGPU_OK;
 }
 
