@@ -382,7 +382,9 @@ int sgpu_render_pt_stats(sgpu_scene scene, const sgpu_pt_job* job, float* frameb
                          void* stream);
 
 #define SGPU_DPT_SORT_HITS 1u /* shade each bounce's hits in (instID[0], instID[1], geomID, primID) order */
-#define SGPU_DPT_SORT_RAYS 2u /* trace each bounce's rays binned by direction, sorted along it (GetSortedBatch) */
+#define SGPU_DPT_SORT_RAYS 2u /* trace each bounce's rays binned by the axis and sign of their direction's largest
+                                 component (DetermineRayCategory's bins), each bin ordered by origin along its axis
+                                 (GetSortedBatch; the reference orders partitions of 64 rays or fewer by direction) */
 
 typedef struct sgpu_dpt_job {
     sgpu_camera camera;

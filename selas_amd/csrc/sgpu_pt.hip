@@ -174,8 +174,11 @@ __global__ __launch_bounds__(kBlock) void k_dpt_keys(uint32_t count, const uint3
 }
 
 // PathTracingBatcher::GetSortedBatch (PathTracingBatcher.cpp:529-560, 705-750): rays binned by the
-// axis and sign of their direction's largest component (RayBatchCategory), sorted by origin along
-// that axis (quantised to 29 bits over the scene box).
+// axis and sign of their direction's largest component as DetermineRayCategory (:170-177) bins them
+// (strict comparisons, x before y before z on ties; a zero component goes to the negative bin),
+// sorted by origin along that axis (quantised to 29 bits over the scene box). The reference's
+// quicksort orders partitions of 64 rays or fewer by the direction component instead (Sample,
+// :183-186); here the whole bin is ordered by origin. The image does not depend on the order.
 __global__ __launch_bounds__(kBlock) void k_dpt_ray_keys(uint32_t count, RayBuf r, f3 lo, f3 inv_ext,
                                                          uint32_t* __restrict keys, uint32_t* __restrict vals)
 {
@@ -184,13 +187,13 @@ __global__ __launch_bounds__(kBlock) void k_dpt_ray_keys(uint32_t count, RayBuf 
     const float ax = fabsf(r.dx[j]), ay = fabsf(r.dy[j]), az = fabsf(r.dz[j]);
     uint32_t axis = 2;
     float dv = r.dz[j], o = r.oz[j], l = lo.z, s = inv_ext.z;
-    if (ax >= ay && ax >= az) {
+    if (ax > ay && ax > az) {
         axis = 0;
         dv = r.dx[j];
         o = r.ox[j];
         l = lo.x;
         s = inv_ext.x;
-    } else if (ay >= az) {
+    } else if (ay > az) {
         axis = 1;
         dv = r.dy[j];
         o = r.oy[j];
@@ -199,7 +202,7 @@ __global__ __launch_bounds__(kBlock) void k_dpt_ray_keys(uint32_t count, RayBuf 
     }
     const float t = fminf(fmaxf((o - l) * s, 0.0f), 1.0f);
     const uint32_t q = min((uint32_t)(t * 536870911.0f), 536870911u);
-    keys[j] = ((2u * axis + (dv < 0.0f ? 1u : 0u)) << 29) | q;
+    keys[j] = ((2u * axis + (dv > 0.0f ? 0u : 1u)) << 29) | q;
     vals[j] = j;
 }
 
