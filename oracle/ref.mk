@@ -10,7 +10,7 @@ CORE := $(REF)/Source/Core
 APP := $(REF)/Source/Applications/Selas/Source
 OUT := _ref
 CXX ?= g++
-CXXFLAGS := -std=c++14 -O2 -fPIC -ffp-contract=off -DIsOsx_=1 -w \
+CXXFLAGS := -std=c++14 -O2 -fPIC -ffp-contract=off -DIsOsx_=1 -w -ffunction-sections -fdata-sections \
             -I$(CORE) -I$(APP) -I$(REF)/Middleware -I$(REF)/Middleware/embree-latest/include \
             -I$(REF)/Middleware/ptex/Include
 
@@ -21,6 +21,15 @@ SRCS := $(CORE)/Shading/Disney.cpp $(CORE)/Shading/Ggx.cpp $(CORE)/Shading/Fresn
         $(CORE)/GeometryLib/RectangulerLightSampler.cpp $(CORE)/GeometryLib/Disc.cpp \
         $(CORE)/GeometryLib/AxisAlignedBox.cpp $(CORE)/SystemLib/Memory.cpp $(CORE)/SystemLib/JsAssert.cpp \
         $(CORE)/IoLib/Serializer.cpp $(APP)/VCMHashGrid.cpp
+# the resource files' serializers (SerializeToBinary / AttachToBinary over each resource's own
+# Serialize function) and the name hash. ModelResource.cpp, SceneResource.cpp and
+# SubsceneResource.cpp also hold the Embree scene set-up; the library is linked with
+# --gc-sections and exports only the harness's ref_* entry points, so the functions that call
+# Embree are dropped as unreferenced (nothing is stubbed: --no-undefined still holds).
+SRCS += $(CORE)/IoLib/BinarySerializers.cpp $(CORE)/IoLib/SizeSerializer.cpp \
+        $(CORE)/SceneLib/ModelResource.cpp $(CORE)/SceneLib/SceneResource.cpp \
+        $(CORE)/SceneLib/SubsceneResource.cpp $(CORE)/SceneLib/ImageBasedLightResource.cpp \
+        $(CORE)/UtilityLib/MurmurHash.cpp
 # the vendored Ptex 2.x reader and filters (PtexFilter::getFilter, PtexTexture::open); zlib is the
 # system's
 PTEX := $(REF)/Middleware/ptex/Include
@@ -43,7 +52,7 @@ $(OUT)/obj/ref_harness.o: ref_harness.cpp
 	@mkdir -p $(dir $@)
 	$(CXX) $(CXXFLAGS) -c $< -o $@
 
-$(OUT)/libselas_ref.so: $(OBJS)
-	$(CXX) -shared -o $@ $(OBJS) -lz -Wl,--no-undefined
+$(OUT)/libselas_ref.so: $(OBJS) ref_exports.map
+	$(CXX) -shared -o $@ $(OBJS) -lz -Wl,--no-undefined -Wl,--gc-sections -Wl,--version-script=ref_exports.map
 
 .PHONY: all

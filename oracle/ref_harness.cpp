@@ -19,8 +19,16 @@
 #include "Shading/SurfaceParameters.h"
 #include "VCMCommon.h"
 #include "VCMHashGrid.h"
+#include "IoLib/BinaryStreamSerializer.h"
+#include "SceneLib/ImageBasedLightResource.h"
+#include "SceneLib/ModelResource.h"
+#include "SceneLib/SceneResource.h"
+#include "SceneLib/SubsceneResource.h"
+#include "UtilityLib/BinarySearch.h"
+#include "UtilityLib/MurmurHash.h"
 
 #include <stdint.h>
+#include <string.h>
 
 // stb_image_write's HDR writer (what StbImageWrite calls for FrameBuffer_Save) and stb_image's
 // reader, from the reference's own copies (Middleware/stb)
@@ -68,6 +76,74 @@ void fill_surface(const float* p, const float* n, SurfaceParameters& s)
     s.shader = p[20] == 0.0f ? eDisneyThin : eDisneySolid;
     s.materialFlags = (uint32)p[21];
     s.lightSetIndex = 0;
+}
+
+// ---- resource files: reference-typed objects filled field by field from flat arrays, then written
+// by the reference's SerializeToBinary (BinaryStreamSerializer.h:14-33) over its own Serialize
+// functions (ModelResource.cpp:36-101, SubsceneResource.cpp:37-46, SceneResource.cpp:37-66,
+// ImageBasedLightResource.cpp:23-43). Fixed strings are copied whole (every byte of the buffer is
+// serialized, the terminator's tail included).
+template <int T>
+void put_str(FixedString<T>& s, const char* src)
+{
+    memcpy(s.str, src, T);
+}
+
+template <class E>
+void sized(CArray<E>& a, uint64_t count, uint64_t capacity)
+{
+    a.Reserve(capacity);
+    a.Resize(count);
+}
+
+float3 f3_at(const float* f) { return float3(f[0], f[1], f[2]); }
+
+// camera: name (128 bytes); f = position[3], fovDegrees, lookAt[3], znear, up[3], zfar
+void put_camera(CameraSettings& c, const char* name, const float* f)
+{
+    put_str(c.name, name);
+    c.position = f3_at(f);
+    c.fovDegrees = f[3];
+    c.lookAt = f3_at(f + 4);
+    c.znear = f[7];
+    c.up = f3_at(f + 8);
+    c.zfar = f[11];
+}
+
+// material: baseColorTexture (512 bytes); u = shader, flags; f = baseColor[3], transmittanceColor[3],
+// the 14 scalar attributes in ScalarMaterialProperties order
+void put_material(MaterialResourceData& m, const char* tex, const uint32_t* u, const float* f)
+{
+    put_str(m.baseColorTexture, tex);
+    m.shader = (ShaderType)u[0];
+    m.flags = u[1];
+    m.baseColor = f3_at(f);
+    m.transmittanceColor = f3_at(f + 3);
+    for (int k = 0; k < eMaterialPropertyCount; ++k) m.scalarAttributeValues[k] = f[6 + k];
+}
+
+void put_instance(Instance& in, uint64_t index, const float* m16)
+{
+    in.index = index;
+    memcpy(&in.localToWorld, m16, sizeof(float4x4));
+}
+
+template <class D>
+uint64_t write_blob(D& obj, uint8_t* out, uint64_t cap)
+{
+    uint8* data = nullptr;
+    Selas::uint size = 0;
+    SerializeToBinary(obj, data, size);
+    if (out && cap >= size) memcpy(out, data, size);
+    FreeAligned_(data);
+    return size;
+}
+
+template <class E>
+void carray_at(const CArray<E>& a, const uint8_t* blob, uint64_t* out)
+{
+    out[0] = a.Count();
+    out[1] = (uint64_t)((const uint8_t*)a.DataPointer() - blob);
 }
 
 }  // namespace
@@ -305,6 +381,255 @@ int ref_read_hdr(const char* path, float* out, int cap)
     for (int i = 0; i < 3 * n && i < cap; ++i) out[i] = data[i];
     stbi_image_free(data);
     return n;
+}
+
+// sizeof of the resource roots and of the records their arrays hold raw: ModelResourceData,
+// ModelGeometryData, SubsceneResourceData, SceneResourceData, ImageBasedLightResourceData,
+// CameraSettings, FilePathString, MaterialResourceData, MeshMetaData, CurveMetaData, Instance,
+// SceneLight, SceneLightSetRange, CArray<uint32>
+void ref_resource_sizes(uint64_t* out)
+{
+    const uint64_t s[] = {sizeof(ModelResourceData), sizeof(ModelGeometryData), sizeof(SubsceneResourceData),
+                          sizeof(SceneResourceData), sizeof(ImageBasedLightResourceData), sizeof(CameraSettings),
+                          sizeof(FilePathString), sizeof(MaterialResourceData), sizeof(MeshMetaData),
+                          sizeof(CurveMetaData), sizeof(Instance), sizeof(SceneLight), sizeof(SceneLightSetRange),
+                          sizeof(CArray<uint32>)};
+    for (size_t i = 0; i < sizeof(s) / sizeof(s[0]); ++i) out[i] = s[i];
+}
+
+// A ModelResource file. box = min[3], max[3]; u4 = totalVertexCount, totalCurveVertexCount,
+// curveModelName, pad; sizes = the eight geometry sizes; counts/caps = count and capacity of
+// cameras, textureResourceNames, materials, materialHashes, meshes, curves; mesh_u = indexCount,
+// indexOffset, vertexCount, vertexOffset, materialHash, indicesPerFace, nameHash per mesh, names 64
+// bytes each; curve_u = indexOffset, indexCount. Returns the file's size (written to out when cap
+// holds it).
+uint64_t ref_model_resource_write(const float* box, const uint32_t* u4, const uint64_t* sizes, const uint64_t* counts,
+                                  const uint64_t* caps, const char* cam_names, const float* cam_f,
+                                  const char* tex_names, const char* mat_tex, const uint32_t* mat_u,
+                                  const float* mat_f, const uint32_t* hashes, const uint32_t* mesh_u,
+                                  const char* mesh_names, const uint32_t* curve_u, uint8_t* out, uint64_t cap)
+{
+    ModelResourceData d;
+    d.aaBox.min = f3_at(box);
+    d.aaBox.max = f3_at(box + 3);
+    d.totalVertexCount = u4[0];
+    d.totalCurveVertexCount = u4[1];
+    d.curveModelName = u4[2];
+    d.pad = u4[3];
+    d.indexSize = sizes[0];
+    d.faceIndexSize = sizes[1];
+    d.positionSize = sizes[2];
+    d.normalsSize = sizes[3];
+    d.tangentsSize = sizes[4];
+    d.uvsSize = sizes[5];
+    d.curveIndexSize = sizes[6];
+    d.curveVertexSize = sizes[7];
+    sized(d.cameras, counts[0], caps[0]);
+    for (uint64_t i = 0; i < counts[0]; ++i) put_camera(d.cameras[i], cam_names + 128 * i, cam_f + 12 * i);
+    sized(d.textureResourceNames, counts[1], caps[1]);
+    for (uint64_t i = 0; i < counts[1]; ++i) put_str(d.textureResourceNames[i], tex_names + 512 * i);
+    sized(d.materials, counts[2], caps[2]);
+    for (uint64_t i = 0; i < counts[2]; ++i)
+        put_material(d.materials[i], mat_tex + 512 * i, mat_u + 2 * i, mat_f + 20 * i);
+    sized(d.materialHashes, counts[3], caps[3]);
+    for (uint64_t i = 0; i < counts[3]; ++i) d.materialHashes[i] = hashes[i];
+    sized(d.meshes, counts[4], caps[4]);
+    for (uint64_t i = 0; i < counts[4]; ++i) {
+        MeshMetaData& m = d.meshes[i];
+        const uint32_t* u = mesh_u + 7 * i;
+        m.indexCount = u[0];
+        m.indexOffset = u[1];
+        m.vertexCount = u[2];
+        m.vertexOffset = u[3];
+        m.materialHash = u[4];
+        m.indicesPerFace = u[5];
+        m.nameHash = u[6];
+        put_str(m.name, mesh_names + 64 * i);
+    }
+    sized(d.curves, counts[5], caps[5]);
+    for (uint64_t i = 0; i < counts[5]; ++i) {
+        d.curves[i].indexOffset = curve_u[2 * i];
+        d.curves[i].indexCount = curve_u[2 * i + 1];
+    }
+    return write_blob(d, out, cap);
+}
+
+// A ModelGeometryResource file: the eight sizes (bytes) and data of indices, faceIndexCounts,
+// positions, normals, tangents, uvs, curveIndices, curveVertices.
+uint64_t ref_model_geometry_write(const uint64_t* sizes, const void* const* data, uint8_t* out, uint64_t cap)
+{
+    ModelGeometryData g;
+    uint64* sz[8] = {&g.indexSize, &g.faceIndexSize, &g.positionSize, &g.normalsSize,
+                     &g.tangentsSize, &g.uvsSize, &g.curveIndexSize, &g.curveVertexSize};
+    void** p[8] = {(void**)&g.indices, (void**)&g.faceIndexCounts, (void**)&g.positions, (void**)&g.normals,
+                   (void**)&g.tangents, (void**)&g.uvs, (void**)&g.curveIndices, (void**)&g.curveVertices};
+    for (int k = 0; k < 8; ++k) {
+        *sz[k] = sizes[k];
+        *p[k] = (void*)data[k];
+    }
+    return write_blob(g, out, cap);
+}
+
+// A SubsceneResource file. counts/caps: modelNames, modelInstances, sceneMaterialNames,
+// sceneMaterials; instances: index and 16 floats (localToWorld as stored) each.
+uint64_t ref_subscene_write(const char* name, uint64_t light_set, const uint64_t* counts, const uint64_t* caps,
+                            const char* model_names, const uint64_t* inst_index, const float* inst_m,
+                            const uint32_t* mat_names, const char* mat_tex, const uint32_t* mat_u, const float* mat_f,
+                            uint8_t* out, uint64_t cap)
+{
+    SubsceneResourceData d;
+    put_str(d.name, name);
+    d.lightSetIndex = light_set;
+    sized(d.modelNames, counts[0], caps[0]);
+    for (uint64_t i = 0; i < counts[0]; ++i) put_str(d.modelNames[i], model_names + 512 * i);
+    sized(d.modelInstances, counts[1], caps[1]);
+    for (uint64_t i = 0; i < counts[1]; ++i) put_instance(d.modelInstances[i], inst_index[i], inst_m + 16 * i);
+    sized(d.sceneMaterialNames, counts[2], caps[2]);
+    for (uint64_t i = 0; i < counts[2]; ++i) d.sceneMaterialNames[i] = mat_names[i];
+    sized(d.sceneMaterials, counts[3], caps[3]);
+    for (uint64_t i = 0; i < counts[3]; ++i)
+        put_material(d.sceneMaterials[i], mat_tex + 512 * i, mat_u + 2 * i, mat_f + 20 * i);
+    return write_blob(d, out, cap);
+}
+
+// A SceneResource file. bg = backgroundIntensity[4]; counts/caps: subsceneNames,
+// subsceneInstances, lights, lightSetRanges, cameras; light_f = position, direction, x, z, radiance
+// (15 floats) per light; ranges = start, count.
+uint64_t ref_scene_write(const char* name, const char* ibl, const float* bg, const uint64_t* counts,
+                         const uint64_t* caps, const char* sub_names, const uint64_t* inst_index, const float* inst_m,
+                         const uint32_t* light_type, const float* light_f, const uint32_t* ranges,
+                         const char* cam_names, const float* cam_f, uint8_t* out, uint64_t cap)
+{
+    SceneResourceData d;
+    put_str(d.name, name);
+    put_str(d.iblName, ibl);
+    d.backgroundIntensity = float4(bg[0], bg[1], bg[2], bg[3]);
+    sized(d.subsceneNames, counts[0], caps[0]);
+    for (uint64_t i = 0; i < counts[0]; ++i) put_str(d.subsceneNames[i], sub_names + 512 * i);
+    sized(d.subsceneInstances, counts[1], caps[1]);
+    for (uint64_t i = 0; i < counts[1]; ++i) put_instance(d.subsceneInstances[i], inst_index[i], inst_m + 16 * i);
+    sized(d.lights, counts[2], caps[2]);
+    for (uint64_t i = 0; i < counts[2]; ++i) {
+        SceneLight& l = d.lights[i];
+        const float* f = light_f + 15 * i;
+        l.type = light_type[i];
+        l.position = f3_at(f);
+        l.direction = f3_at(f + 3);
+        l.x = f3_at(f + 6);
+        l.z = f3_at(f + 9);
+        l.radiance = f3_at(f + 12);
+    }
+    sized(d.lightSetRanges, counts[3], caps[3]);
+    for (uint64_t i = 0; i < counts[3]; ++i) {
+        d.lightSetRanges[i].start = ranges[2 * i];
+        d.lightSetRanges[i].count = ranges[2 * i + 1];
+    }
+    sized(d.cameras, counts[4], caps[4]);
+    for (uint64_t i = 0; i < counts[4]; ++i) put_camera(d.cameras[i], cam_names + 128 * i, cam_f + 12 * i);
+    return write_blob(d, out, cap);
+}
+
+// An IBL file: density functions (height marginal and width x height conditional floats), miss
+// map size, rotation, exposure scale, light and miss texels (3 floats each).
+uint64_t ref_ibl_write(uint64_t width, uint64_t height, const float* marginal, const float* conditional,
+                       uint64_t miss_width, uint64_t miss_height, float rotation, float exposure, const float* light,
+                       const float* miss, uint8_t* out, uint64_t cap)
+{
+    ImageBasedLightResourceData d;
+    d.densityfunctions.width = width;
+    d.densityfunctions.height = height;
+    d.densityfunctions.marginalDensityFunction = (float*)marginal;
+    d.densityfunctions.conditionalDensityFunctions = (float*)conditional;
+    d.missWidth = miss_width;
+    d.missHeight = miss_height;
+    d.rotationRadians = rotation;
+    d.exposureScale = exposure;
+    d.lightData = (float3*)light;
+    d.missData = (float3*)miss;
+    return write_blob(d, out, cap);
+}
+
+// AttachToBinary (BinaryStreamSerializer.h:35-44) of a resource file in place (kind 0 model,
+// 1 geometry, 2 subscene, 3 scene, 4 IBL), then where the reference finds each array: (count,
+// offset of its data from the file's start) per CArray, or the offset of each raw pointer (geometry:
+// the 8 buffers; IBL: marginal, conditional, light, miss). Returns the number of values written.
+int ref_attach_offsets(int kind, uint8_t* blob, uint64_t size, uint64_t* out)
+{
+    int n = 0;
+    if (kind == 0) {
+        ModelResourceData* d = nullptr;
+        AttachToBinary(d, blob, size);
+        carray_at(d->cameras, blob, out + 0);
+        carray_at(d->textureResourceNames, blob, out + 2);
+        carray_at(d->materials, blob, out + 4);
+        carray_at(d->materialHashes, blob, out + 6);
+        carray_at(d->meshes, blob, out + 8);
+        carray_at(d->curves, blob, out + 10);
+        n = 12;
+    }
+    else if (kind == 1) {
+        ModelGeometryData* g = nullptr;
+        AttachToBinary(g, blob, size);
+        const void* p[8] = {g->indices, g->faceIndexCounts, g->positions, g->normals,
+                            g->tangents, g->uvs, g->curveIndices, g->curveVertices};
+        for (int k = 0; k < 8; ++k) out[k] = (uint64_t)((const uint8_t*)p[k] - blob);
+        n = 8;
+    }
+    else if (kind == 2) {
+        SubsceneResourceData* d = nullptr;
+        AttachToBinary(d, blob, size);
+        carray_at(d->modelNames, blob, out + 0);
+        carray_at(d->modelInstances, blob, out + 2);
+        carray_at(d->sceneMaterialNames, blob, out + 4);
+        carray_at(d->sceneMaterials, blob, out + 6);
+        n = 8;
+    }
+    else if (kind == 3) {
+        SceneResourceData* d = nullptr;
+        AttachToBinary(d, blob, size);
+        carray_at(d->subsceneNames, blob, out + 0);
+        carray_at(d->subsceneInstances, blob, out + 2);
+        carray_at(d->lights, blob, out + 4);
+        carray_at(d->lightSetRanges, blob, out + 6);
+        carray_at(d->cameras, blob, out + 8);
+        n = 10;
+    }
+    else if (kind == 4) {
+        ImageBasedLightResourceData* d = nullptr;
+        AttachToBinary(d, blob, size);
+        const void* p[4] = {d->densityfunctions.marginalDensityFunction, d->densityfunctions.conditionalDensityFunctions,
+                            d->lightData, d->missData};
+        for (int k = 0; k < 4; ++k) out[k] = (uint64_t)((const uint8_t*)p[k] - blob);
+        n = 4;
+    }
+    return n;
+}
+
+// SetupSceneCamera (SceneResource.cpp:336-358) on a SceneResource file attached in place: the
+// RayCastCameraSettings basis and image-plane distance as ref_camera_rays reports them, then the
+// camera's position.
+void ref_scene_camera(uint8_t* blob, uint64_t size, uint64_t index, uint32_t width, uint32_t height, float* out)
+{
+    SceneResource scene;
+    AttachToBinary(scene.data, blob, size);
+    RayCastCameraSettings rc;
+    SetupSceneCamera(&scene, index, width, height, rc);
+    scene.data = nullptr;
+    const float v[13] = {rc.cameraX.x, rc.cameraX.y, rc.cameraX.z, rc.cameraY.x, rc.cameraY.y, rc.cameraY.z,
+                         rc.cameraZ.x, rc.cameraZ.y, rc.cameraZ.z, rc.virtualImagePlaneDistance,
+                         rc.position.x, rc.position.y, rc.position.z};
+    for (int k = 0; k < 13; ++k) out[k] = v[k];
+}
+
+// MurmurHash3_x86_32 (MurmurHash.cpp), the hash of names and material names in resource files
+uint32_t ref_murmur3_32(const char* key, int32_t len, uint32_t seed) { return MurmurHash3_x86_32(key, len, seed); }
+
+// BinarySearch (BinarySearch.h:11-35) over uint32 keys, as FindMeshMaterial calls it
+// (ModelResource.cpp:213-229); (uint64)-1 when not found. Callers keep key <= the largest element:
+// beyond it the reference reads data[(uint)-1].
+uint64_t ref_binary_search(const uint32_t* data, uint64_t count, uint32_t key)
+{
+    return BinarySearch((uint32*)data, (Selas::uint)count, (uint32)key);
 }
 
 }  // extern "C"

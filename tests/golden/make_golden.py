@@ -9,8 +9,9 @@ Inputs are drawn from a seeded numpy generator; every output is computed by the 
 functions (EvaluateDisney, SampleDisney, the GGX/Fresnel helpers, CorrelatedMultiJitter,
 InitializeRayCastCamera + JitteredCameraRay, the spherical-rectangle sampler, RayQuad,
 SampleConcentricDisc, BuildHashGrid + SearchHashGrid, PtexFilter f_bspline over Ptex files written
-by tests/golden/ptx.py, stbi_write_hdr + stbi_loadf for the .hdr output). The fixtures are data only.
-`--only ptex` / `--only hdr` rewrite just those cases.
+by tests/golden/ptx.py, stbi_write_hdr + stbi_loadf for the .hdr output, SerializeToBinary /
+AttachToBinary / SetupSceneCamera for the resource files). The fixtures are data only.
+`--only ptex` / `--only hdr` / `--only resources` rewrite just those cases.
 """
 from __future__ import annotations
 
@@ -323,16 +324,56 @@ def hdr_cases():
     return out
 
 
+def resource_cases():
+    """The fixture scene of tests/resource_fixtures.py written by the reference's SerializeToBinary
+    into tests/golden/resources/_Assets (one file per resource, at Selas's asset paths), its floor's
+    Ptex file under tests/golden/resources/Content, and what the reference's AttachToBinary finds in
+    each file and SetupSceneCamera makes of the scene's two cameras (resources/reference.npz)."""
+    import ptx
+    import resource_fixtures as F
+    from selas_amd import resources as R
+
+    lib = F.ref_lib()
+    files, ptex_rel = F.fixture_resources()
+    root = F.FIXTURE_ROOT
+    blobs = {}
+
+    def writer(kind, d):
+        blobs[(kind, d.get("name", ""))] = b = F.ref_write(lib, kind, d)
+        return b
+
+    F.write_tree(root, files, writer)
+    p = F.floor_ptex()
+    path = os.path.join(root, "Content", ptex_rel)
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    ptx.write_ptx(path, p["res_log2"], p["texels"], p["adjfaces"], p["adjedges"])
+    ref = {}
+    for (kind, name) in files:
+        with open(R.asset_path(root, kind, name), "rb") as f:
+            ref[f"attach_{kind}_{name}"] = F.ref_attach(lib, kind, f.read())
+    with open(R.asset_path(root, "scene", "rooms"), "rb") as f:
+        scene = f.read()
+    for index in (0, 1):
+        ref[f"camera_{index}"] = F.ref_scene_camera(lib, scene, index, 64, 48)
+    np.savez_compressed(os.path.join(root, "reference.npz"), **ref)
+    print("wrote resources")
+
+
 if __name__ == "__main__":
     import sys
 
     sys.path.insert(0, HERE)
+    sys.path.insert(0, os.path.dirname(HERE))
+    sys.path.insert(0, ROOT)
     only = sys.argv[sys.argv.index("--only") + 1] if "--only" in sys.argv else None
     if only == "ptex":
         write(ptex_cases())
     elif only == "hdr":
         write(hdr_cases())
+    elif only == "resources":
+        resource_cases()
     else:
         main()
         write(ptex_cases())
         write(hdr_cases())
+        resource_cases()
