@@ -368,6 +368,34 @@ int ref_ptex_eval(const char* path, uint32_t n, const int32_t* faces, const floa
     return 0;
 }
 
+// PtexTexture::open + getFaceInfo + getData(faceid, buffer, 0) (PtexReader.cpp:714-765): face
+// `face`'s texels at its own resolution, pixels interleaved, rows along v, written to out (at most
+// cap bytes); info = meshType, dataType, numChannels, numFaces, alphaChannel, ulog2, vlog2,
+// adjedges packed, flags, adjfaces[4]. Returns the face's byte count, or -1 when the file cannot
+// be opened.
+int ref_ptex_face(const char* path, int32_t face, uint8_t* out, int64_t cap, int32_t* info)
+{
+    Ptex::String err;
+    PtexTexture* tex = PtexTexture::open(path, err);
+    if (!tex) return -1;
+    const Ptex::FaceInfo& fi = tex->getFaceInfo(face);
+    const int psize = Ptex::DataSize(tex->dataType()) * tex->numChannels();
+    const int bytes = psize * fi.res.u() * fi.res.v();
+    info[0] = tex->meshType();
+    info[1] = tex->dataType();
+    info[2] = tex->numChannels();
+    info[3] = tex->numFaces();
+    info[4] = tex->alphaChannel();
+    info[5] = fi.res.ulog2;
+    info[6] = fi.res.vlog2;
+    info[7] = fi.adjedges;
+    info[8] = fi.flags;
+    for (int k = 0; k < 4; ++k) info[9 + k] = fi.adjfaces[k];
+    if (bytes <= cap) tex->getData(face, out, 0);
+    tex->release();
+    return bytes;
+}
+
 // stbi_write_hdr as StbImageWrite calls it (StbImage.cpp:78): 3 channels, rows top first
 int ref_write_hdr(const char* path, int w, int h, const float* rgb) { return stbi_write_hdr(path, w, h, 3, rgb); }
 

@@ -11,7 +11,7 @@ InitializeRayCastCamera + JitteredCameraRay, the spherical-rectangle sampler, Ra
 SampleConcentricDisc, BuildHashGrid + SearchHashGrid, PtexFilter f_bspline over Ptex files written
 by tests/golden/ptx.py, stbi_write_hdr + stbi_loadf for the .hdr output, SerializeToBinary /
 AttachToBinary / SetupSceneCamera for the resource files). The fixtures are data only.
-`--only ptex` / `--only hdr` / `--only resources` rewrite just those cases.
+`--only ptex` / `--only hdr` / `--only resources` / `--only ptexio` rewrite just those cases.
 """
 from __future__ import annotations
 
@@ -324,6 +324,86 @@ def hdr_cases():
     return out
 
 
+def ptexio_cases():
+    """Ptex files written by tests/golden/ptx.py's general writer and read back by the reference's
+    PtexReader (face info and getData per face) and filtered by its f_bspline filter:
+    mixed8 -- quad mesh, 8-bit RGBA, faces at different resolutions, a 1 x 1 face, a constant face,
+    zipped, difference-zipped and tiled faces (tiles zipped, difference-zipped and constant);
+    tri16 -- triangle mesh, 16-bit RGB, difference-zipped; half1 -- quad mesh, half floats, one
+    channel; uniform8 -- quad 4 x 3 grid, 8-bit RGB, one resolution with constant faces, a tiled
+    face and difference coding (what the device renders), with filtered lookups."""
+    import ptx
+
+    lib = C.CDLL(LIB)
+    lib.ref_ptex_face.argtypes = [C.c_char_p, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p]
+    lib.ref_ptex_face.restype = C.c_int
+    lib.ref_ptex_eval.argtypes = [C.c_char_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.ref_ptex_eval.restype = C.c_int
+    rng = np.random.default_rng(77)
+    out = {}
+
+    def rand_face(ul, vl, nc, dt):
+        if np.dtype(dt).kind == "u":
+            return rng.integers(0, np.iinfo(dt).max + 1, (1 << vl, 1 << ul, nc)).astype(dt)
+        return rng.normal(size=(1 << vl, 1 << ul, nc)).astype(dt)
+
+    af, ae = ptx.grid_adjacency(3, 2)
+    cases = {}
+    res = [(3, 3), (2, 3), (0, 0), (2, 2), (5, 5), (3, 2)]
+    faces = [rand_face(u, v, 4, np.uint8) for u, v in res]
+    faces[3][:] = faces[3][0, 0]
+    faces[4][8:16, 0:8] = 7
+    tiles = ["zip", "diff", "const"] + ["zip"] * 13
+    cases["mixed8"] = dict(faces=faces, res_log2=res, adjfaces=af, adjedges=ae, data_type=0, alpha=3,
+                           encodings=["zip", "diff", "zip", "zip", ("tiled", 3, 3, tiles), "diff"],
+                           flags=[0, 0, 0, ptx.FLAG_CONSTANT, 0, 0])
+    tri_af = np.array([[-1, 1, -1, -1], [2, 0, -1, -1], [1, -1, 3, -1], [-1, -1, 2, -1]], np.int32)
+    tri_ae = np.array([[0, 1, 0, 0], [1, 1, 0, 0], [0, 0, 2, 0], [0, 0, 2, 0]], np.int32)
+    cases["tri16"] = dict(faces=[rand_face(2, 2, 3, np.uint16) for _ in range(4)], res_log2=[(2, 2)] * 4,
+                          adjfaces=tri_af, adjedges=tri_ae, mesh_type=ptx.MT_TRIANGLE, data_type=1,
+                          encodings=["diff"] * 4)
+    cases["half1"] = dict(faces=[rand_face(2, 1, 1, np.float16) for _ in range(6)], res_log2=[(2, 1)] * 6,
+                          adjfaces=af, adjedges=ae, data_type=2)
+    gaf, gae = ptx.grid_adjacency(4, 3)
+    uf = [rand_face(3, 3, 3, np.uint8) for _ in range(12)]
+    uflags = np.zeros(12, np.uint8)
+    for f in (2, 7):
+        uflags[f] = ptx.FLAG_CONSTANT
+        uf[f][:] = uf[f][0, 0]
+    uenc = ["diff" if f % 2 else "zip" for f in range(12)]
+    uenc[5] = ("tiled", 2, 2, ["zip", "const", "diff", "zip"])
+    cases["uniform8"] = dict(faces=uf, res_log2=[(3, 3)] * 12, adjfaces=gaf, adjedges=gae, data_type=0,
+                             encodings=uenc, flags=uflags)
+    for name, c in cases.items():
+        path = os.path.join(HERE, f"ptexio_{name}.ptx")
+        ptx.write_ptx_general(path, c["faces"], c["res_log2"], c["adjfaces"], c["adjedges"],
+                              mesh_type=c.get("mesh_type", ptx.MT_QUAD), data_type=c["data_type"],
+                              encodings=c.get("encodings"), flags=c.get("flags"), alpha=c.get("alpha", -1))
+        n = len(c["faces"])
+        infos, blobs = [], []
+        for f in range(n):
+            info = np.zeros(13, np.int32)
+            size = lib.ref_ptex_face(path.encode(), f, None, 0, info.ctypes.data)
+            buf = np.zeros(size, np.uint8)
+            assert lib.ref_ptex_face(path.encode(), f, buf.ctypes.data, size, info.ctypes.data) == size
+            infos.append(info)
+            blobs.append(buf)
+        d = dict(info=np.stack(infos), data=np.concatenate(blobs), sizes=np.array([b.size for b in blobs]))
+        if name == "uniform8":
+            m = 2000
+            fc = rng.integers(0, n, m).astype(np.int32)
+            u = rng.random(m).astype(np.float32)
+            v = rng.random(m).astype(np.float32)
+            u[:200] = 0.5 / 8
+            v[200:400] = 1.0
+            res_out = np.zeros((m, 3), np.float32)
+            assert lib.ref_ptex_eval(path.encode(), m, fc.ctypes.data, u.ctypes.data, v.ctypes.data,
+                                     res_out.ctypes.data) == 0
+            d.update(faces=fc, u=u, v=v, out=res_out)
+        out[f"ptexio_{name}"] = d
+    return out
+
+
 def resource_cases():
     """The fixture scene of tests/resource_fixtures.py written by the reference's SerializeToBinary
     into tests/golden/resources/_Assets (one file per resource, at Selas's asset paths), its floor's
@@ -372,8 +452,11 @@ if __name__ == "__main__":
         write(hdr_cases())
     elif only == "resources":
         resource_cases()
+    elif only == "ptexio":
+        write(ptexio_cases())
     else:
         main()
         write(ptex_cases())
         write(hdr_cases())
         resource_cases()
+        write(ptexio_cases())

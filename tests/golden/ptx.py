@@ -51,6 +51,82 @@ def write_ptx(path, res_log2, texels, adjfaces, adjedges):
         f.write(header + ext + fi_z + const_z + levelinfo + level + struct.pack("<Q", 0))
 
 
+MT_TRIANGLE = 0
+ENC_CONSTANT, ENC_DIFFZIPPED, ENC_TILED = 0, 2, 3
+FLAG_CONSTANT = 1
+_NP = {0: np.uint8, 1: np.uint16, 2: np.float16, 3: np.float32}
+
+
+def _face_block(px, enc, dt):
+    """One face's (or tile's) block: (encoding, bytes). px: (v, u, nchan) of dtype dt."""
+    v, u, nc = px.shape
+    if enc == "const":
+        return ENC_CONSTANT, px[0, 0].astype(dt).tobytes()
+    planes = np.ascontiguousarray(px.transpose(2, 0, 1)).astype(dt)
+    if enc == "diff":
+        a = planes.reshape(-1)
+        d = np.empty_like(a)
+        d[0] = a[0]
+        d[1:] = a[1:] - a[:-1]  # encodeDifference (wrapping for unsigned types)
+        return ENC_DIFFZIPPED, zlib.compress(d.tobytes())
+    return ENC_ZIPPED, zlib.compress(planes.tobytes())
+
+
+def write_ptx_general(path, faces, res_log2, adjfaces, adjedges, mesh_type=MT_QUAD, data_type=DT_UINT8,
+                      encodings=None, flags=None, alpha=-1):
+    """A Ptex file with per-face resolutions, encodings and flags, any channel count and data type.
+    faces: list of (v res, u res, nchan) arrays (constant-flagged faces: any shape, first pixel
+    used); res_log2: (n, 2) (ulog2, vlog2); encodings: per face "zip", "diff", "const" or
+    ("tiled", tile ulog2, tile vlog2, [per-tile encodings]); flags: per-face FaceInfo flags."""
+    n = len(faces)
+    dt = _NP[data_type]
+    nchan = faces[0].shape[2]
+    encodings = encodings or ["zip"] * n
+    flags = np.zeros(n, np.uint8) if flags is None else np.asarray(flags, np.uint8)
+    fi = b"".join(
+        struct.pack("<bbBB4i", int(res_log2[f][0]), int(res_log2[f][1]),
+                    sum((int(adjedges[f][e]) & 3) << (2 * e) for e in range(4)), int(flags[f]),
+                    *[int(a) for a in adjfaces[f]])
+        for f in range(n))
+    fi_z = zlib.compress(fi)
+    const = np.stack([np.asarray(a, np.float64).reshape(-1, nchan).mean(0) for a in faces])
+    const = (np.round(const) if np.dtype(dt).kind == "u" else const).astype(dt)
+    for f in range(n):
+        if flags[f] & FLAG_CONSTANT:
+            const[f] = faces[f][0, 0]
+    const_z = zlib.compress(const.tobytes())
+    blocks, hdrs = [], []
+    for f in range(n):
+        if flags[f] & FLAG_CONSTANT or (res_log2[f][0] == 0 and res_log2[f][1] == 0):
+            enc, b = _face_block(faces[f][:1, :1], "const", dt)
+        elif isinstance(encodings[f], tuple):
+            _, tul, tvl, tile_encs = encodings[f]
+            tu, tv = 1 << tul, 1 << tvl
+            px = faces[f]
+            ntu, ntv = px.shape[1] // tu, px.shape[0] // tv
+            tblocks, thdrs = [], []
+            for i in range(ntv):
+                for j in range(ntu):
+                    te, tb = _face_block(px[i * tv:(i + 1) * tv, j * tu:(j + 1) * tu], tile_encs[i * ntu + j], dt)
+                    tblocks.append(tb)
+                    thdrs.append((len(tb) & 0x3FFFFFFF) | (te << 30))
+            th_z = zlib.compress(struct.pack("<%dI" % len(thdrs), *thdrs))
+            b = struct.pack("<bbI", tul, tvl, len(th_z)) + th_z + b"".join(tblocks)
+            enc = ENC_TILED
+        else:
+            enc, b = _face_block(faces[f], encodings[f], dt)
+        blocks.append(b)
+        hdrs.append((len(b) & 0x3FFFFFFF) | (enc << 30))
+    fdh_z = zlib.compress(struct.pack("<%dI" % n, *hdrs))
+    level = fdh_z + b"".join(blocks)
+    levelinfo = struct.pack("<QII", len(level), len(fdh_z), n)
+    ext = struct.pack("<HHHHIIQQQ", 0, 0, 0, 0, 0, 0, 0, 0, 0)
+    header = struct.pack("<IIIIiHHIIIIIIQII", MAGIC, 1, mesh_type, data_type, alpha, nchan, 1, n, len(ext),
+                         len(fi_z), len(const_z), len(levelinfo), 4, len(level), 0, 0)
+    with open(path, "wb") as fh:
+        fh.write(header + ext + fi_z + const_z + levelinfo + level + struct.pack("<Q", 0))
+
+
 def grid_adjacency(nu, nv):
     """Quad grid, face (i, j) = j * nu + i with u along i and v along j (the face order and uv
     orientation of quad_grid_mesh in selas_amd/workloads.py)."""
