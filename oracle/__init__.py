@@ -227,7 +227,7 @@ def concentric_disc(r0, r1):
 
 class _Ptex(C.Structure):
     _fields_ = [("res_log2", C.c_int), ("nfaces", C.c_uint32), ("texels", C.c_void_p), ("adjfaces", C.c_void_p),
-                ("adjedges", C.c_void_p)]
+                ("adjedges", C.c_void_p), ("tri", C.c_int)]
 
 
 class _Scene(C.Structure):
@@ -392,7 +392,7 @@ class RenderScene:
         for i, g in enumerate(sc["geoms"]):
             if isinstance(g, dict) and g.get("ptex") is not None:
                 p = g["ptex"]
-                t = PtexTexture(p["texels"], p["res_log2"], p["adjfaces"], p["adjedges"])
+                t = PtexTexture(p["texels"], p["res_log2"], p["adjfaces"], p["adjedges"], tri="triangles" in g)
                 self.textures.append(t)
                 self.geom_tex[i] = t.s
         self.bvh = CpuBvh(self.tri9)
@@ -519,14 +519,14 @@ lib.orc_ptex_eval_batch.argtypes = [C.POINTER(_Ptex), C.c_uint32, C.c_void_p, C.
 
 class PtexTexture:
     """Per-face texture data for the oracle's filter: texels (nfaces, res, res, 3) uint8,
-    adjfaces / adjedges (nfaces, 4)."""
+    adjfaces / adjedges (nfaces, 4); tri: a triangle mesh (Ptex's triangle filter)."""
 
-    def __init__(self, texels, res_log2, adjfaces, adjedges):
+    def __init__(self, texels, res_log2, adjfaces, adjedges, tri=False):
         self.texels = np.ascontiguousarray(texels, np.uint8)
         self.adjfaces = np.ascontiguousarray(adjfaces, np.int32)
         self.adjedges = np.ascontiguousarray(adjedges, np.int32)
         self.s = _Ptex(int(res_log2), self.texels.shape[0], self.texels.ctypes.data, self.adjfaces.ctypes.data,
-                       self.adjedges.ctypes.data)
+                       self.adjedges.ctypes.data, int(bool(tri)))
 
     def eval(self, faces, u, v):
         faces = np.ascontiguousarray(faces, np.int32)
@@ -687,7 +687,7 @@ class InstancedRenderScene:
             for gi, g in enumerate(self.models[mi]["geoms"]):
                 if isinstance(g, dict) and g.get("ptex") is not None:
                     p = g["ptex"]
-                    t = PtexTexture(p["texels"], p["res_log2"], p["adjfaces"], p["adjedges"])
+                    t = PtexTexture(p["texels"], p["res_log2"], p["adjfaces"], p["adjedges"], tri="triangles" in g)
                     self.textures.append(t)
                     self.geom_tex[g0[mi] + gi] = t.s
         n = len(comp)

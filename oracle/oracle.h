@@ -130,6 +130,7 @@ typedef struct {
     const uint8_t* texels;  /* nfaces x res x res x 3 (row = v texel) */
     const int32_t* adjfaces; /* 4 per face: bottom, right, top, left; -1 = border */
     const int32_t* adjedges; /* 4 per face: the neighbour's edge id of the shared edge */
+    int tri;                 /* 1: a triangle mesh (3 edges per face: v = 0, w = 0, u = 0) */
 } orc_ptex;
 
 /* ---- image-based light (orc_ibl.c) -------------------------------------------------------------- */
@@ -285,6 +286,7 @@ uint32_t orc_hash_grid_query(const orc_hash_grid* g, const float* points, const 
 void orc_render_vcm(const orc_scene* sc, const orc_vcm_job* job, float* fb_out, float* stats_out, int threads);
 
 void orc_ptex_eval(const orc_ptex* tx, int face, float u, float v, float* out3);
+void orc_ptex_tri_eval(const orc_ptex* tx, int face, float u, float v, float* out3);
 void orc_ptex_eval_batch(const orc_ptex* tx, uint32_t n, const int32_t* faces, const float* u, const float* v,
                          float* out);
 

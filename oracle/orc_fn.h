@@ -24,6 +24,7 @@ static inline float atan2_f(float y, float x)
 {
     return orc_libm_single ? atan2f(y, x) : (float)atan2((double)y, (double)x);
 }
+static inline float exp_f(float x) { return orc_libm_single ? expf(x) : (float)exp((double)x); }
 static inline float pow_f(float x, float y) { return orc_libm_single ? powf(x, y) : (float)pow((double)x, (double)y); }
 /* x^5 (Schlick's weight): powf in mode 1, else double multiplications (the device's way) */
 static inline float pow5_f(float x)

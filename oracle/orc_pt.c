@@ -150,7 +150,8 @@ int orc_surface_from_hit(const float* o, const float* d, float t, const float* n
 
 /* CalculateSurfaceParams' eUsesPtex branch (SurfaceParameters.cpp:196-203): the texture is
  * looked up at (primId, baryCoords); for Embree quad geometry those are the quad and its own
- * coordinates, i.e. (u, v) on the first triangle of the split and (1 - u, 1 - v) on the second. */
+ * coordinates, i.e. (u, v) on the first triangle of the split and (1 - u, 1 - v) on the second;
+ * for triangle geometry, the triangle and its barycentrics. */
 const float* orc_hit_texel(const orc_scene* sc, uint32_t tri, uint32_t geom, const float* mat, float hu, float hv,
                            float* texel3)
 {
@@ -164,6 +165,10 @@ const float* orc_hit_texel_local(const orc_scene* sc, uint32_t local, uint32_t g
     if (!(((uint32_t)mat[21]) & 16u) || !sc->geom_tex) return NULL;
     const orc_ptex* tx = &sc->geom_tex[geom];
     if (tx->nfaces == 0) return NULL;
+    if (tx->tri) {
+        orc_ptex_tri_eval(tx, (int)local, hu, hv, texel3);
+        return texel3;
+    }
     float u = hu, v = hv;
     if (local & 1u) {
         u = 1.0f - hu;

@@ -19,8 +19,10 @@
  * neighbours and valence 3, 4 and 5 corners).
  */
 #include "oracle.h"
+#include "orc_fn.h"
 
 #include <math.h>
+#include <stddef.h>
 #include <string.h>
 
 #define KMAX 10
@@ -487,8 +489,278 @@ void orc_ptex_eval(const orc_ptex* tx, int face, float u, float v, float* out3)
     for (int c = 0; c < 3; ++c) out3[c] = (float)(F.result[c] * scale);
 }
 
+/* ---- triangle meshes ------------------------------------------------------------------------
+ * PtexFilter::getFilter serves triangle-mesh textures with PtexTriangleFilter whatever the filter
+ * type asked for (PtexFilters.cpp:436-440): an elliptical Gaussian over the triangular texel grid.
+ * Restated from PtexTriangleFilter.cpp:45-265 (eval, buildKernel, splitAndApply, applyAcrossEdge,
+ * apply, applyIter), PtexTriangleKernel.h:48-231 (split, rotate, reorient, clampRes, clampExtent,
+ * getIterators) and PtexTriangleKernel.cpp:42-75 (gaussian, Apply for 3 channels of 8-bit data),
+ * for the zero footprint Selas asks for (uw1 = vw1 = uw2 = vw2 = 0, width 1, blur 0), one
+ * resolution for every face, no neighbourhood-constant faces. Triangle edge e runs from corner e
+ * to corner e + 1: edge 0 is v = 0, edge 1 is w = 0, edge 2 is u = 0. The Gaussian is exp of a
+ * float argument: the reference's call resolves to the float overload (expf), orc_set_libm mode 1;
+ * mode 0 rounds the double exp once, as the device does (orc_fn.h). */
+#define TRI_KERNEL_WIDTH 3.5f
+
+typedef struct {
+    int res_log2;
+    float u, v, u1, v1, w1, u2, v2, w2, A, B, C;
+} tkernel;
+
+typedef struct {
+    int rowlen;
+    float u, v;
+    int u1, v1, w1, u2, v2, w2;
+    float A, B, C;
+    int valid;
+    float wscale, weight;
+} titer;
+
+static float sqf(float x) { return x * x; }
+static float fminf_(float a, float b) { return a < b ? a : b; }
+static float fmaxf_(float a, float b) { return a > b ? a : b; }
+
+static void tset8(tkernel* k, float u, float v, float u1, float v1, float w1, float u2, float v2, float w2)
+{
+    k->u = u;
+    k->v = v;
+    k->u1 = u1;
+    k->v1 = v1;
+    k->w1 = w1;
+    k->u2 = u2;
+    k->v2 = v2;
+    k->w2 = w2;
+}
+
+static void tset_abc(tkernel* k, float A, float B, float C)
+{
+    k->A = A;
+    k->B = B;
+    k->C = C;
+}
+
+/* rotate1: u' = w, v' = u, w' = v; rotate2: u' = v, v' = w, w' = u */
+static void trotate1(tkernel* k) { tset_abc(k, k->C, 2.0f * k->C - k->B, k->A + k->C - k->B); }
+static void trotate2(tkernel* k) { tset_abc(k, k->A + k->C - k->B, 2.0f * k->A - k->B, k->A); }
+
+static void treorient(tkernel* k, int eid, int aeid)
+{
+    const float u = k->u, v = k->v, w = 1.0f - k->u - k->v;
+    const float u1 = k->u1, v1 = k->v1, w1 = k->w1, u2 = k->u2, v2 = k->v2, w2 = k->w2;
+    switch (eid * 3 + aeid) {
+    case 0: tset8(k, 1.0f - u, -v, 1.0f - u2, -v2, 1.0f - w2, 1.0f - u1, -v1, 1.0f - w1); break;
+    case 1: tset8(k, 1.0f - w, 1.0f - u, 1.0f - w2, 1.0f - u2, -v2, 1.0f - w1, 1.0f - u1, -v1); trotate1(k); break;
+    case 2: tset8(k, -v, 1.0f - w, -v2, 1.0f - w2, 1.0f - u2, -v1, 1.0f - w1, 1.0f - u1); trotate2(k); break;
+    case 3: tset8(k, 1.0f - v, -w, 1.0f - v2, -w2, 1.0f - u2, 1.0f - v1, -w1, 1.0f - u1); trotate2(k); break;
+    case 4: tset8(k, 1.0f - u, 1.0f - v, 1.0f - u2, 1.0f - v2, -w2, 1.0f - u1, 1.0f - v1, -w1); break;
+    case 5: tset8(k, -w, 1.0f - u, -w2, 1.0f - u2, 1.0f - v2, -w1, 1.0f - u1, 1.0f - v1); trotate1(k); break;
+    case 6: tset8(k, 1.0f - w, -u, 1.0f - w2, -u2, 1.0f - v2, 1.0f - w1, -u1, 1.0f - v1); trotate1(k); break;
+    case 7: tset8(k, 1.0f - v, 1.0f - w, 1.0f - v2, 1.0f - w2, -u2, 1.0f - v1, 1.0f - w1, -u1); trotate2(k); break;
+    case 8: tset8(k, -u, 1.0f - v, -u2, 1.0f - v2, 1.0f - w2, -u1, 1.0f - v1, 1.0f - w1); break;
+    }
+}
+
+static float tri_gaussian(float q)
+{
+    const float scale = -0.5f * (TRI_KERNEL_WIDTH * TRI_KERNEL_WIDTH);
+    return exp_f(scale * q);
+}
+
+static void tri_get_iterators(const tkernel* k, titer* ke, titer* ko)
+{
+    const int resu = 1 << k->res_log2;
+    const float third = (float)(1 / 3.0);
+    float Finv = 1.0f / ((float)resu * (float)resu * (k->A * k->C - 0.25f * k->B * k->B));
+    float Ak = k->A * Finv, Bk = k->B * Finv, Ck = k->C * Finv;
+    ke->rowlen = resu;
+    ke->wscale = 1.0f / ((float)resu * (float)resu);
+    float scale = (float)ke->rowlen;
+    ke->u = k->u * scale - third;
+    ke->v = k->v * scale - third;
+    ke->u1 = (int)ceilf(k->u1 * scale - third);
+    ke->v1 = (int)ceilf(k->v1 * scale - third);
+    ke->w1 = (int)ceilf(k->w1 * scale - third);
+    ke->u2 = (int)ceilf(k->u2 * scale - third);
+    ke->v2 = (int)ceilf(k->v2 * scale - third);
+    ke->w2 = (int)ceilf(k->w2 * scale - third);
+    ke->A = Ak;
+    ke->B = Bk;
+    ke->C = Ck;
+    ke->valid = ke->u2 > ke->u1 && ke->v2 > ke->v1 && ke->w2 > ke->w1;
+    ke->weight = 0;
+    ko->rowlen = ke->rowlen;
+    ko->wscale = ke->wscale;
+    ko->u = (1.0f - k->v) * scale - third;
+    ko->v = (1.0f - k->u) * scale - third;
+    ko->u1 = (int)ceilf((1.0f - k->v2) * scale - third);
+    ko->v1 = (int)ceilf((1.0f - k->u2) * scale - third);
+    ko->w1 = (int)ceilf((-k->w2) * scale - third);
+    ko->u2 = (int)ceilf((1.0f - k->v1) * scale - third);
+    ko->v2 = (int)ceilf((1.0f - k->u1) * scale - third);
+    ko->w2 = (int)ceilf((-k->w1) * scale - third);
+    ko->A = Ck;
+    ko->B = Bk;
+    ko->C = Ak;
+    ko->valid = ko->u2 > ko->u1 && ko->v2 > ko->v1 && ko->w2 > ko->w1;
+    ko->weight = 0;
+}
+
+/* Apply<uint8_t, 3>: the ellipse's Q by finite differences along each row (Heckbert) */
+static void tri_iter_apply(titer* k, float* result, const uint8_t* data)
+{
+    float DDQ = 2.0f * k->A;
+    for (int vi = k->v1; vi != k->v2; vi++) {
+        int xw = k->rowlen - vi;
+        int x1 = k->u1 > xw - k->w2 ? k->u1 : xw - k->w2;
+        int x2 = k->u2 < xw - k->w1 ? k->u2 : xw - k->w1;
+        float U = (float)x1 - k->u;
+        float V = (float)vi - k->v;
+        float DQ = k->A * (2.0f * U + 1.0f) + k->B * V;
+        float Q = k->A * U * U + (k->B * U + k->C * V) * V;
+        const uint8_t* p = data + ((size_t)vi * k->rowlen + x1) * 3;
+        const uint8_t* pEnd = p + (ptrdiff_t)(x2 - x1) * 3;
+        for (; p < pEnd; p += 3) {
+            if (Q < 1.0f) {
+                float weight = tri_gaussian(Q) * k->wscale;
+                k->weight += weight;
+                result[0] += (float)p[0] * weight;
+                result[1] += (float)p[1] * weight;
+                result[2] += (float)p[2] * weight;
+            }
+            Q += DQ;
+            DQ += DDQ;
+        }
+    }
+}
+
+static void tri_apply(pfilter* F, tkernel* k, int face)
+{
+    /* clampRes (one resolution: the face's), clampExtent */
+    if (k->res_log2 > F->tx->res_log2) k->res_log2 = F->tx->res_log2;
+    k->u1 = fmaxf_(k->u1, 0.0f);
+    k->v1 = fmaxf_(k->v1, 0.0f);
+    k->w1 = fmaxf_(k->w1, 0.0f);
+    k->u2 = fminf_(k->u2, 1.0f - (k->v1 + k->w1));
+    k->v2 = fminf_(k->v2, 1.0f - (k->w1 + k->u1));
+    k->w2 = fminf_(k->w2, 1.0f - (k->u1 + k->v1));
+    titer ke, ko;
+    tri_get_iterators(k, &ke, &ko);
+    if (!ke.valid && !ko.valid) return;
+    const int res = 1 << F->tx->res_log2;
+    const uint8_t* data = F->tx->texels + (size_t)face * res * res * 3;
+    if (ke.valid) {
+        tri_iter_apply(&ke, F->result, data);
+        F->weight += ke.weight;
+    }
+    if (ko.valid) {
+        tri_iter_apply(&ko, F->result, data);
+        F->weight += ko.weight;
+    }
+}
+
+static void tri_split_and_apply(pfilter* F, tkernel* k, int face);
+
+static void tri_apply_across_edge(pfilter* F, tkernel* k, int face, int eid)
+{
+    int af = adjface(F->tx, face, eid), ae = adjedge(F->tx, face, eid);
+    treorient(k, eid, ae);
+    tri_split_and_apply(F, k, af);
+}
+
+static void tri_split_and_apply(pfilter* F, tkernel* k, int face)
+{
+    if (k->u1 < 0 && adjface(F->tx, face, 2) >= 0) {
+        tkernel ka = *k;
+        k->u1 = 0;
+        ka.u2 = 0;
+        tri_apply_across_edge(F, &ka, face, 2);
+    }
+    if (k->v1 < 0 && adjface(F->tx, face, 0) >= 0) {
+        tkernel ka = *k;
+        k->v1 = 0;
+        ka.v2 = 0;
+        tri_apply_across_edge(F, &ka, face, 0);
+    }
+    if (k->w1 < 0 && adjface(F->tx, face, 1) >= 0) {
+        tkernel ka = *k;
+        k->w1 = 0;
+        ka.w2 = 0;
+        tri_apply_across_edge(F, &ka, face, 1);
+    }
+    tri_apply(F, k, face);
+}
+
+static int calc_res_from_width(float w)
+{
+    union {
+        float f;
+        int32_t i;
+    } b;
+    b.f = w;
+    return 127 - ((b.i >> 23) & 0xff);
+}
+
+/* buildKernel with uw1 = vw1 = uw2 = vw2 = 0, width 1, blur 0, then splitAndApply and the
+ * normalisation of eval */
+void orc_ptex_tri_eval(const orc_ptex* tx, int face, float u, float v, float* out3)
+{
+    out3[0] = out3[1] = out3[2] = 0.0f;
+    if (face < 0 || (uint32_t)face >= tx->nfaces) return;
+    u = fminf_(fmaxf_(u, 0.0f), 1.0f);
+    v = fminf_(fmaxf_(v, 0.0f), 1.0f);
+    const float sqrt3 = 1.7320508075688772f;
+    const float width = 1.0f, blur = 0.0f, uw1 = 0.0f, vw1 = 0.0f, uw2 = 0.0f, vw2 = 0.0f;
+    float scaleAC = 0.25f * width * width;
+    float scaleB = -2.0f * scaleAC;
+    float A = (vw1 * vw1 + vw2 * vw2) * scaleAC;
+    float B = (uw1 * vw1 + uw2 * vw2) * scaleB;
+    float C = (uw1 * uw1 + uw2 * uw2) * scaleAC;
+    float Ac = 0.75f * A;
+    float Bc = (sqrt3 / 2) * (B - A);
+    float Cc = 0.25f * A - 0.5f * B + C;
+    const float maxEcc = 15.0f;
+    const float eccRatio = (maxEcc * maxEcc + 1.0f) / (maxEcc * maxEcc - 1.0f);
+    float X = sqrtf(sqf(Ac - Cc) + sqf(Bc));
+    float b_e = 0.5f * (eccRatio * X - (Ac + Cc));
+    float b_t = sqf(0.5f / (float)(1 << tx->res_log2));
+    float b_b = 0.25f * blur * blur;
+    float b = fmaxf_(b_b, fmaxf_(b_e, b_t));
+    Ac += b;
+    Cc += b;
+    float m = sqrtf(2.0f * (Ac * Cc - 0.25f * Bc * Bc) / (Ac + Cc + X));
+    int reslog2 = calc_res_from_width(2.0f * m);
+    if (reslog2 < 0) reslog2 = 0;
+    A = (float)(4 / 3.0) * Ac;
+    B = (2 / sqrt3) * Bc + A;
+    C = -0.25f * A + 0.5f * B + Cc;
+    float scale = TRI_KERNEL_WIDTH * TRI_KERNEL_WIDTH;
+    A *= scale;
+    B *= scale;
+    C *= scale;
+    float uw = fminf_(sqrtf(C), 1.0f);
+    float vw = fminf_(sqrtf(A), 1.0f);
+    float ww = fminf_(sqrtf(A - B + C), 1.0f);
+    float w = 1.0f - u - v;
+    tkernel k;
+    k.res_log2 = reslog2;
+    tset8(&k, u, v, u - uw, v - vw, w - ww, u + uw, v + vw, w + ww);
+    tset_abc(&k, A, B, C);
+    pfilter F;
+    F.tx = tx;
+    F.result[0] = F.result[1] = F.result[2] = 0.0f;
+    F.weight = 0.0f;
+    tri_split_and_apply(&F, &k, face);
+    float s = 1.0f / (F.weight * 255.0f);
+    for (int c = 0; c < 3; ++c) out3[c] = (float)(F.result[c] * s);
+}
+
 void orc_ptex_eval_batch(const orc_ptex* tx, uint32_t n, const int32_t* faces, const float* u, const float* v,
                          float* out)
 {
-    for (uint32_t i = 0; i < n; ++i) orc_ptex_eval(tx, faces[i], u[i], v[i], out + 3 * (size_t)i);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (tx->tri)
+            orc_ptex_tri_eval(tx, faces[i], u[i], v[i], out + 3 * (size_t)i);
+        else
+            orc_ptex_eval(tx, faces[i], u[i], v[i], out + 3 * (size_t)i);
+    }
 }

@@ -11,7 +11,8 @@ InitializeRayCastCamera + JitteredCameraRay, the spherical-rectangle sampler, Ra
 SampleConcentricDisc, BuildHashGrid + SearchHashGrid, PtexFilter f_bspline over Ptex files written
 by tests/golden/ptx.py, stbi_write_hdr + stbi_loadf for the .hdr output, SerializeToBinary /
 AttachToBinary / SetupSceneCamera for the resource files). The fixtures are data only.
-`--only ptex` / `--only hdr` / `--only resources` / `--only ptexio` rewrite just those cases.
+`--only ptex` / `--only hdr` / `--only resources` / `--only ptexio` / `--only ptex_tri` rewrite just
+those cases.
 """
 from __future__ import annotations
 
@@ -298,6 +299,53 @@ def ptex_cases():
     return out
 
 
+def ptex_tri_cases():
+    """PtexFilter{f_bspline}.eval on triangle-mesh Ptex files, which getFilter serves with
+    PtexTriangleFilter (PtexFilters.cpp:436-440; PtexTriangleFilter.cpp, PtexTriangleKernel.*): a
+    4 x 3 grid split into 24 triangles at 8 x 8 texels (and the same grid with every triangle's corners
+    rotated, so that shared edges pair every two edge ids), a tetrahedron at 4 x 4 (valence 3), an
+    icosahedron at 16 x 16 (valence 5), an open fan of 7 triangles around a valence-7 vertex at
+    8 x 8, and the grid at 2 x 2 and 1 x 1 (kernels wider than a face). Lookups at random
+    barycentrics, corners, edge points and the centre."""
+    import ptx
+
+    lib = C.CDLL(LIB)
+    lib.ref_ptex_eval.argtypes = [C.c_char_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.ref_ptex_eval.restype = C.c_int
+    rng = np.random.default_rng(4242)
+    out = {}
+    rot = ptx.tri_grid(4, 3)
+    shifts = np.random.default_rng(31).integers(0, 3, rot.shape[0])
+    rot = np.stack([np.roll(t, -int(k)) for t, k in zip(rot, shifts)])
+    meshes = [("tgrid", 3, ptx.tri_grid(4, 3), 3000), ("tetra", 2, ptx.tetra_tris(), 1500),
+              ("tgrid_rot", 3, rot, 3000),
+              ("icosa", 4, ptx.icosa_tris(), 2000), ("fan", 3, ptx.fan_tris(7), 1500),
+              ("tgrid_r1", 1, ptx.tri_grid(4, 3), 1500), ("tgrid_r0", 0, ptx.tri_grid(4, 3), 800)]
+    for name, rl, tris, n in meshes:
+        af, ae = ptx.tri_mesh_adjacency(tris)
+        nf, res = af.shape[0], 1 << rl
+        texels = rng.integers(0, 256, (nf, res, res, 3), dtype=np.uint8)
+        path = os.path.join(HERE, f"ptex_tri_{name}.ptx")
+        ptx.write_ptx_general(path, list(texels), [(rl, rl)] * nf, af, ae, mesh_type=ptx.MT_TRIANGLE)
+        faces = rng.integers(0, nf, n).astype(np.int32)
+        u = rng.random(n).astype(np.float32)
+        v = rng.random(n).astype(np.float32)
+        flip = u + v > 1.0
+        u[flip], v[flip] = 1.0 - u[flip], 1.0 - v[flip]
+        special = np.array([(0, 0), (1, 0), (0, 1), (0.5, 0), (0.5, 0.5), (0, 0.5), (1 / 3, 1 / 3), (1e-4, 0.3),
+                            (0.3, 1e-4), (0.6, 0.4 - 1e-4), (0.5 / res, 0.5 / res), (1 - 1.0 / res, 0.0)], np.float32)
+        k = min(n // 4, 300)
+        pick = rng.integers(0, special.shape[0], k)
+        u[:k], v[:k] = special[pick, 0], special[pick, 1]
+        res_out = np.zeros((n, 3), np.float32)
+        rc = lib.ref_ptex_eval(path.encode(), n, faces.ctypes.data, u.ctypes.data, v.ctypes.data,
+                               res_out.ctypes.data)
+        assert rc == 0
+        out[f"ptex_tri_{name}"] = dict(res_log2=np.int32(rl), texels=texels, adjfaces=af, adjedges=ae, faces=faces,
+                                       u=u, v=v, out=res_out)
+    return out
+
+
 def hdr_cases():
     """stbi_write_hdr (FrameBuffer_Save's writer) on two small images -- 21 x 6 (run-length coded
     scanlines: runs of equal pixels, zeros, values below 1e-32, large and tiny components) and
@@ -454,9 +502,12 @@ if __name__ == "__main__":
         resource_cases()
     elif only == "ptexio":
         write(ptexio_cases())
+    elif only == "ptex_tri":
+        write(ptex_tri_cases())
     else:
         main()
         write(ptex_cases())
         write(hdr_cases())
         resource_cases()
         write(ptexio_cases())
+        write(ptex_tri_cases())

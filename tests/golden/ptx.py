@@ -167,6 +167,52 @@ def quad_mesh_adjacency(quads):
     return af, ae
 
 
+def tri_mesh_adjacency(tris):
+    """Adjacency of a triangle mesh given as (n, 3) vertex indices, counter-clockwise: Ptex's
+    triangle edge e runs from corner e to corner e+1 (edge 0: v = 0, edge 1: w = 0, edge 2: u = 0);
+    the fourth slot is unused (-1, 0)."""
+    tris = np.asarray(tris)
+    n = tris.shape[0]
+    edges = {}
+    for f in range(n):
+        for e in range(3):
+            edges[(int(tris[f, e]), int(tris[f, (e + 1) % 3]))] = (f, e)
+    af = -np.ones((n, 4), np.int32)
+    ae = np.zeros((n, 4), np.int32)
+    for f in range(n):
+        for e in range(3):
+            a, b = int(tris[f, e]), int(tris[f, (e + 1) % 3])
+            if (b, a) in edges:
+                af[f, e], ae[f, e] = edges[(b, a)]
+    return af, ae
+
+
+def tri_grid(nu, nv):
+    """A quad grid split into triangles (a, b, c), (a, c, d) per quad."""
+    tris = []
+    for j in range(nv):
+        for i in range(nu):
+            a = j * (nu + 1) + i
+            b, c, d = a + 1, a + nu + 2, a + nu + 1
+            tris += [[a, b, c], [a, c, d]]
+    return np.array(tris)
+
+
+def tetra_tris():
+    return np.array([[0, 2, 1], [0, 1, 3], [0, 3, 2], [1, 2, 3]])
+
+
+def icosa_tris():
+    return np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4],
+                     [11, 10, 2], [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9],
+                     [4, 9, 5], [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]])
+
+
+def fan_tris(k=7):
+    """k triangles around vertex 0 (an open fan: a border on both sides)."""
+    return np.array([[0, i, i + 1] for i in range(1, k + 1)])
+
+
 def cube_quads():
     """A cube as 6 quads with outward orientation and assorted rotations between neighbours (every
     corner has valence 3)."""
