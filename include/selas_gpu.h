@@ -318,13 +318,17 @@ int sgpu_scene_set_lights(sgpu_scene scene, const sgpu_light* lights, uint32_t n
 int sgpu_scene_set_ibl(sgpu_scene scene, uint32_t width, uint32_t height, const float* rgb, float rotation_radians,
                        float exposure);
 
-/* Per-face texture of a quad geometry: replaces TextureCache::FetchPtex + PtexTexture::open
- * (TextureCache.cpp:168, SurfaceParameters.cpp:197) for 8-bit RGB Ptex data with one resolution
- * for every face. texels: n_faces x (1 << res_log2)^2 x 3 bytes (face-major, rows along v);
- * adjfaces / adjedges: 4 per face in Ptex edge order (bottom, right, top, left), the adjacent face
- * (-1 = border) and that face's edge id of the shared edge (Ptex FaceInfo, Ptexture.h:240-300).
- * n_faces must equal the geometry's quad count; res_log2 <= 8. Host arrays, copied. A material
- * with flag 16 (eUsesPtex) takes its base colour from its geometry's texture. */
+/* Per-face texture of a quad or triangle geometry: replaces TextureCache::FetchPtex +
+ * PtexTexture::open (TextureCache.cpp:168, SurfaceParameters.cpp:197) for 8-bit RGB Ptex data with
+ * one resolution for every face. texels: n_faces x (1 << res_log2)^2 x 3 bytes (face-major, rows
+ * along v); adjfaces / adjedges: 4 per face, the adjacent face (-1 = border) and that face's edge id
+ * of the shared edge (Ptex FaceInfo, Ptexture.h:240-300). Quad geometry: edges in Ptex's quad order
+ * (bottom, right, top, left), n_faces = the quad count, filtered with Ptex's cubic B-spline filter
+ * (PtexSeparableFilter). Triangle geometry: edges 0..2 in Ptex's triangle order (edge e from corner
+ * e to corner e + 1: v = 0, w = 0, u = 0; the fourth entry is ignored), n_faces = the triangle
+ * count, filtered with Ptex's triangle filter (PtexTriangleFilter, the filter getFilter returns for
+ * triangle meshes, PtexFilters.cpp:436-440) at the hit's barycentrics. res_log2 <= 8. Host arrays,
+ * copied. A material with flag 16 (eUsesPtex) takes its base colour from its geometry's texture. */
 int sgpu_scene_set_ptex(sgpu_scene scene, uint32_t geom_id, uint32_t res_log2, uint32_t n_faces,
                         const uint8_t* texels, const int32_t* adjfaces, const int32_t* adjedges);
 

@@ -491,7 +491,7 @@ static int gather_instance_shading(sgpu_scene scene)
         return SGPU_OK;
     }
     std::vector<uint32_t> gmat(n_geoms);
-    std::vector<sgpu::PtexTex> gtex(n_geoms, sgpu::PtexTex{nullptr, nullptr, 0, 0});
+    std::vector<sgpu::PtexTex> gtex(n_geoms, sgpu::PtexTex{nullptr, nullptr, 0, 0, 0});
     std::vector<sgpu::GeomAttr> gattr(n_geoms, sgpu::GeomAttr{nullptr, nullptr, nullptr, 0, 0});
     bool any_attr = false;
     for (size_t j = 0; j < models.size(); ++j) {
@@ -791,25 +791,29 @@ int sgpu_scene_set_ptex(sgpu_scene scene, uint32_t geom_id, uint32_t res_log2, u
         return sgpu_fail(SGPU_E_STATE, "sgpu_scene_set_ptex: jobs on the scene are admitted to a VCM server");
     const size_t n_geoms = scene->geom_first.size();
     if (geom_id >= n_geoms) return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_ptex: no such geometry");
-    if (!scene->geom_quad[geom_id]) return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_ptex: not a quad geometry");
+    const bool tri = !scene->geom_quad[geom_id];
     const uint32_t first = scene->geom_first[geom_id];
     const uint32_t end = geom_id + 1 < n_geoms ? scene->geom_first[geom_id + 1] : (uint32_t)scene->tri_geom.size();
-    if ((end - first) / 2 != n_faces)
-        return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_ptex: face count differs from the geometry's quad count");
+    if ((tri ? end - first : (end - first) / 2) != n_faces)
+        return sgpu_fail(SGPU_E_INVALID, tri ? "sgpu_scene_set_ptex: face count differs from the geometry's triangle count"
+                                             : "sgpu_scene_set_ptex: face count differs from the geometry's quad count");
     if (res_log2 > 8) return sgpu_fail(SGPU_E_RANGE, "sgpu_scene_set_ptex: res_log2 above 8");
-    for (size_t i = 0; i < (size_t)n_faces * 4; ++i) {
-        if (adjfaces[i] < -1 || adjfaces[i] >= (int32_t)n_faces)
-            return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_ptex: adjacent face out of range");
-        if (adjedges[i] < 0 || adjedges[i] > 3)
-            return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_ptex: adjacent edge out of range");
-    }
+    const int edges = tri ? 3 : 4;
+    for (size_t f = 0; f < n_faces; ++f)
+        for (int e = 0; e < edges; ++e) {
+            const size_t i = 4 * f + e;
+            if (adjfaces[i] < -1 || adjfaces[i] >= (int32_t)n_faces)
+                return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_ptex: adjacent face out of range");
+            if (adjedges[i] < 0 || adjedges[i] >= edges)
+                return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_ptex: adjacent edge out of range");
+        }
     const size_t res = (size_t)1 << res_log2;
     const size_t texel_bytes = (size_t)n_faces * res * res * 3;
     std::vector<int32_t> adj((size_t)n_faces * 8);
     for (size_t f = 0; f < n_faces; ++f)
         for (int e = 0; e < 4; ++e) {
-            adj[8 * f + e] = adjfaces[4 * f + e];
-            adj[8 * f + 4 + e] = adjedges[4 * f + e];
+            adj[8 * f + e] = e < edges ? adjfaces[4 * f + e] : -1;
+            adj[8 * f + 4 + e] = e < edges ? adjedges[4 * f + e] : 0;
         }
     SGPU_HIP_TRY(hipSetDevice(scene->ctx->device));
     uint8_t* dt = nullptr;
@@ -820,8 +824,8 @@ int sgpu_scene_set_ptex(sgpu_scene scene, uint32_t geom_id, uint32_t res_log2, u
     scene->tex_allocs.push_back(da);
     SGPU_HIP_TRY(hipMemcpy(dt, texels, texel_bytes, hipMemcpyHostToDevice));
     SGPU_HIP_TRY(hipMemcpy(da, adj.data(), adj.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-    scene->tex_host.resize(n_geoms, sgpu::PtexTex{nullptr, nullptr, 0, 0});
-    scene->tex_host[geom_id] = sgpu::PtexTex{dt, da, n_faces, (int32_t)res_log2};
+    scene->tex_host.resize(n_geoms, sgpu::PtexTex{nullptr, nullptr, 0, 0, 0});
+    scene->tex_host[geom_id] = sgpu::PtexTex{dt, da, n_faces, (int32_t)res_log2, tri ? 1 : 0};
     // the per-geometry table the shading kernels read
     if (scene->d_geom_tex) (void)hipFree(scene->d_geom_tex);
     scene->d_geom_tex = nullptr;

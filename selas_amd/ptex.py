@@ -18,8 +18,8 @@ PtexUtils.cpp:162-195 interleave, 255-271 decodeDifference):
 
 Reduction levels, meta data and edits are not read (edit blocks are refused). read_ptx returns every
 face's texels; device_texture turns a file the device renders -- 8-bit data, at least 3 channels,
-one resolution for all non-constant faces -- into the (res_log2, texels, adjfaces, adjedges) that
-sgpu_scene_set_ptex takes, constant faces expanded to that resolution.
+one resolution for all non-constant faces, quad or triangle meshes -- into the (res_log2, texels,
+adjfaces, adjedges) that sgpu_scene_set_ptex takes, constant faces expanded to that resolution.
 """
 from __future__ import annotations
 
@@ -165,8 +165,6 @@ def device_texture(ptx: dict) -> dict:
         raise PtexError("the device's per-face textures are 8-bit")
     if ptx["nchannels"] < 3:
         raise PtexError("the device's per-face textures have 3 channels (the renderer reads channels 0..2)")
-    if ptx["mesh_type"] != MESH_QUAD:
-        raise PtexError("the device's per-face textures are on quad meshes")
     if np.any(ptx["flags"] & FLAG_SUBFACE):
         raise PtexError("subfaces (faces of a split non-quad) are not supported")
     if np.any(ptx["ulog2"] != ptx["vlog2"]):
@@ -184,7 +182,8 @@ def device_texture(ptx: dict) -> dict:
     texels = np.zeros((n, size, size, 3), np.uint8)
     for f, a in enumerate(ptx["faces"]):
         texels[f] = a[:, :, :3] if varying[f] else np.broadcast_to(ptx["constants"][f][:3], (size, size, 3))
-    return {"res_log2": rl, "texels": texels, "adjfaces": ptx["adjfaces"].copy(), "adjedges": ptx["adjedges"].copy()}
+    return {"res_log2": rl, "texels": texels, "adjfaces": ptx["adjfaces"].copy(), "adjedges": ptx["adjedges"].copy(),
+            "mesh_type": ptx["mesh_type"]}
 
 
 def load_device_texture(path) -> dict:

@@ -533,17 +533,18 @@ def build_model(model: dict, geometry: dict, name: str, scene_material_names, sc
         tex = fixed_str(m["base_color_texture"])
         flags = int(m["flags"])
         if flags & assets.USES_PTEX:
-            if faces.shape[1] != 4:
-                drop(f"mesh {mi}: a per-face texture on a triangle mesh")
+            rel = f"{tex}\\{mname}.ptx".replace("~", os.sep).replace("\\", os.sep)
+            path = os.path.join(content_root or "", "Content", rel)
+            if ptex_loader is None:
+                drop(f"mesh {mi}: per-face texture {path} (no ptex_loader)")
                 rec[21] = float(flags & ~assets.USES_PTEX)
             else:
-                rel = f"{tex}\\{mname}.ptx".replace("~", os.sep).replace("\\", os.sep)
-                path = os.path.join(content_root or "", "Content", rel)
-                if ptex_loader is None:
-                    drop(f"mesh {mi}: per-face texture {path} (no ptex_loader)")
-                    rec[21] = float(flags & ~assets.USES_PTEX)
-                else:
-                    g["ptex"] = ptex_loader(path)
+                t = ptex_loader(path)
+                want = 1 if faces.shape[1] == 4 else 0  # Ptex mesh types: 0 triangle, 1 quad
+                if t.get("mesh_type", want) != want:
+                    raise ResourceError(f"{where}: texture {path} is for a {('triangle', 'quad')[t['mesh_type']]} "
+                                        "mesh")
+                g["ptex"] = t
         elif tex:
             drop(f"mesh {mi}: UV base-colour texture {tex}")
         key = rec.tobytes()

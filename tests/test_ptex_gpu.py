@@ -87,6 +87,88 @@ def test_ptex_file_through_reader_matches_reference_filter(gpu_ctx):
         sc.release()
 
 
+def _tri_strip(nf):
+    """nf separate triangles (a geometry to hang a texture of nf faces on)."""
+    x = np.arange(nf, dtype=np.float32)
+    pos = np.stack([np.stack([x, np.zeros_like(x), np.zeros_like(x)], 1),
+                    np.stack([x + 1, np.zeros_like(x), np.zeros_like(x)], 1),
+                    np.stack([x, np.zeros_like(x), np.ones_like(x)], 1)], 1).reshape(-1, 3)
+    return pos, np.arange(3 * nf, dtype=np.uint32).reshape(nf, 3)
+
+
+@pytest.mark.parametrize("name", ["tgrid", "tgrid_rot", "tetra", "icosa", "fan", "tgrid_r1", "tgrid_r0"])
+def test_tri_ptex_eval_matches_oracle(gpu_ctx, name):
+    """Ptex's triangle filter on the device: bit-exact with the oracle's (exp in double rounded
+    once, as the device evaluates it), and within 2 ulp of the reference's Ptex (whose expf is
+    not always correctly rounded) on the recorded lookups."""
+    g = np.load(os.path.join(GOLD, f"ptex_tri_{name}.npz"))
+    nf = g["texels"].shape[0]
+    sc = N.Scene(gpu_ctx)
+    pos, tris = _tri_strip(nf)
+    gid = sc.add_triangles(pos, tris)
+    sc.set_ptex(gid, int(g["res_log2"]), g["texels"], g["adjfaces"], g["adjedges"])
+    sc.commit()
+    try:
+        print(f"tri ptex {name}", flush=True)
+        rng = np.random.default_rng(17)
+        n = 6000
+        u, v = rng.random(n), rng.random(n)
+        flip = u + v > 1
+        u[flip], v[flip] = 1 - u[flip], 1 - v[flip]
+        faces = np.concatenate([g["faces"], rng.integers(0, nf, n)]).astype(np.int32)
+        u = np.concatenate([g["u"], u]).astype(np.float32)
+        v = np.concatenate([g["v"], v]).astype(np.float32)
+        gpu = sc.ptex_eval(gid, faces, u, v)
+        tex = O.PtexTexture(g["texels"], int(g["res_log2"]), g["adjfaces"], g["adjedges"], tri=True)
+        cpu = tex.eval(faces, u, v)
+        assert np.array_equal(gpu.view(np.uint32), cpu.view(np.uint32))
+        k = g["faces"].shape[0]
+        ulps = np.abs(gpu[:k].view(np.int32).astype(np.int64) - g["out"].view(np.int32).astype(np.int64))
+        assert ulps.max() <= 2 and (ulps == 0).mean() >= 0.995
+    finally:
+        sc.release()
+
+
+def test_tri_textured_renders_match_oracle(gpu_ctx):
+    """A triangle mesh (an icosphere) with a per-face texture, in the room scene: the path tracer
+    and VCM per pixel against the oracle (99 % within 1e-3, VCM RMSE <= 2e-3 of the mean)."""
+    import sys
+
+    sys.path.insert(0, GOLD)
+    import ptx
+    import scenes
+
+    ico = W.icosphere_mesh(2, (-1.2, 0.45, 0.0), 0.45)
+    pos, tris = ico["positions"], ico["triangles"]
+    af, ae = ptx.tri_mesh_adjacency(tris)
+    assert (af[:, :3] >= 0).all()  # closed and consistently wound
+    rng = np.random.default_rng(23)
+    texels = rng.integers(0, 256, (tris.shape[0], 8, 8, 3), dtype=np.uint8)
+    sc = scenes.cube_scene(lights=True, ibl=False)
+    sc = dict(sc, geoms=list(sc["geoms"]) + [{"positions": pos, "triangles": tris, "normals": ico["normals"],
+                                              "ptex": {"res_log2": 3, "texels": texels, "adjfaces": af,
+                                                       "adjedges": ae}}])
+    mat = W.material(1, (1.0, 1.0, 1.0), 0.0, 0.6, 1.5)
+    mat[21] = W.MAT_USES_PTEX
+    sc["materials"] = np.concatenate([sc["materials"], mat[None]])
+    sc["geom_material"] = np.concatenate([sc["geom_material"], [sc["materials"].shape[0] - 1]]).astype(np.uint32)
+    scene = N.build_render_scene(gpu_ctx, sc)
+    try:
+        rs = O.RenderScene(sc)
+        print("tri textured render: path tracer", flush=True)
+        gpu = N.render_pt(scene, sc["camera"], 64, 48, 4, 6, 11, N.PT_NEE_COSINE_ONCE)
+        cpu = O.render_pt(rs, sc["camera"], 64, 48, 4, 6, 11, 1)
+        frac, _ = _close(gpu, cpu)
+        assert cpu.mean() > 0 and frac >= 0.99, frac
+        print("tri textured render: VCM", flush=True)
+        gpu = N.render_vcm(scene, sc["camera"], 64, 48, 2, 8, 0.02, 0.75, 13)
+        cpu, _ = O.render_vcm(rs, sc["camera"], 64, 48, 2, 8, 0.02, 0.75, 13)
+        frac, rel = _close(gpu, cpu)
+        assert cpu.mean() > 0 and frac >= 0.99 and rel <= 2e-3, (frac, rel)
+    finally:
+        scene.release()
+
+
 def test_quad_hits_report_quad_coordinates(gpu_ctx):
     nu, nv = 5, 3
     X, Z = np.meshgrid(np.arange(nu + 1, dtype=np.float32), np.arange(nv + 1, dtype=np.float32), indexing="ij")
@@ -167,7 +249,11 @@ def test_set_ptex_errors(gpu_ctx):
         return N.lib.sgpu_scene_set_ptex(sc.handle, geom, rl, nfaces, tex.ctypes.data, af.ctypes.data,
                                          ae.ctypes.data)
 
-    assert call(tri) == N.SGPU_E_INVALID          # not a quad geometry
+    assert call(tri) == N.SGPU_E_INVALID          # face count differs from the triangle count
+    ae3 = ae.copy()
+    ae3[0, 0] = 3
+    assert call(tri, nfaces=1, af=af[:1].copy(), ae=ae3[:1].copy()) == N.SGPU_E_INVALID  # triangles have edges 0..2
+    assert call(tri, nfaces=1, af=af[:1].copy(), ae=ae[:1].copy()) == N.SGPU_OK
     assert call(q, nfaces=3) == N.SGPU_E_INVALID  # face count differs
     assert call(q, rl=9) == N.SGPU_E_RANGE
     bad = af.copy()

@@ -75,14 +75,13 @@ def test_device_texture_refusals(name, match):
         P.device_texture(P.read_ptx(path))
 
 
-def test_device_texture_refuses_fewer_channels_and_triangles():
+def test_device_texture_refuses_fewer_channels_and_subfaces():
     _, path = _gold("uniform8")
     r = P.read_ptx(path)
     r1 = dict(r, nchannels=2)
     with pytest.raises(P.PtexError, match="3 channels"):
         P.device_texture(r1)
-    with pytest.raises(P.PtexError, match="quad meshes"):
-        P.device_texture(dict(r, mesh_type=P.MESH_TRIANGLE))
+    assert P.device_texture(dict(r, mesh_type=P.MESH_TRIANGLE))["mesh_type"] == P.MESH_TRIANGLE
     with pytest.raises(P.PtexError, match="subfaces"):
         P.device_texture(dict(r, flags=r["flags"] | P.FLAG_SUBFACE))
 
