@@ -189,6 +189,20 @@ int sgpu_scene_add_triangles(sgpu_scene scene, const float* positions, uint32_t 
  * triangle's barycentrics (u', v') become (1 - u', 1 - v')), the coordinates Ptex faces use. */
 int sgpu_scene_add_quads(sgpu_scene scene, const float* positions, uint32_t n_verts, const uint32_t* indices,
                          uint32_t n_quads, const float* colmajor4x4, uint32_t* geom_id_out);
+/* Round B-spline curves: replaces rtcNewGeometry(RTC_GEOMETRY_TYPE_ROUND_BSPLINE_CURVE) +
+ * rtcSetSharedGeometryBuffer(INDEX uint, VERTEX float4) + rtcAttachGeometry (InitializeCurves,
+ * ModelResource.cpp:300-330). vertices: n_verts control points (x, y, z, radius); indices: per segment
+ * the first of its four consecutive control points. A segment is the uniform cubic B-spline through
+ * them and its surface the sphere swept along it (end spheres included); a hit reports prim_id = the
+ * segment, u = the curve parameter in [0, 1], v = 0 and ng = the surface normal (the hit point minus
+ * the curve point at u); only entries are hits (a ray starting inside a curve's tube does not hit that
+ * tube). Curves take no vertex attributes and no per-face texture. The optional placement
+ * (colmajor4x4) must be a similarity (rotation, uniform scale, translation): radii scale with it;
+ * an instance placing a scene with curves must be one too. Queries on a scene with curves need
+ * hits.t (the curve pass after the triangle traversal compares against it); sgpu_intersect_stats
+ * counts the triangle traversal only. Curves count as one geometry id each call, like meshes. */
+int sgpu_scene_add_curves(sgpu_scene scene, const float* vertices, uint32_t n_verts, const uint32_t* indices,
+                          uint32_t n_segments, const float* colmajor4x4, uint32_t* geom_id_out);
 /* Per-vertex shading attributes of geometry geom_id (host arrays, copied; before commit): replaces
  * rtcSetGeometryVertexAttributeCount + rtcSetSharedGeometryBuffer(RTC_BUFFER_TYPE_VERTEX_ATTRIBUTE)
  * with slot 0 = normals (FLOAT3), slot 1 = tangents (FLOAT4: xyz and the bitangent sign w), slot 2 =

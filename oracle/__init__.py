@@ -234,7 +234,34 @@ class _Scene(C.Structure):
     _fields_ = [("bvh", C.c_void_p), ("tri9", C.c_void_p), ("tri_geom", C.c_void_p), ("materials", C.c_void_p),
                 ("geom_material", C.c_void_p), ("n_geoms", C.c_uint32), ("lights", C.c_void_p),
                 ("n_lights", C.c_uint32), ("background", C.c_float * 3), ("geom_first", C.c_void_p),
-                ("geom_tex", C.c_void_p), ("ibl", C.c_void_p), ("inst", C.c_void_p), ("geom_attr", C.c_void_p)]
+                ("geom_tex", C.c_void_p), ("ibl", C.c_void_p), ("inst", C.c_void_p), ("geom_attr", C.c_void_p),
+                ("curve_cp", C.c_void_p), ("curve_box", C.c_void_p), ("curve_geom", C.c_void_p),
+                ("curve_prim", C.c_void_p), ("n_curves", C.c_uint32)]
+
+
+def curve_segments(g):
+    """Segments (n, 16) of a curve geometry entry {"curve_vertices": (m, 4), "curves": first control
+    point of each segment}: the four control points (x, y, z, radius) of each."""
+    v = np.asarray(g["curve_vertices"], np.float32).reshape(-1, 4)
+    idx = np.asarray(g["curves"], np.int64).reshape(-1)
+    return np.ascontiguousarray(np.stack([v[idx + j] for j in range(4)], 1).reshape(-1, 16), np.float32)
+
+
+def _curve_arrays(geoms):
+    """(cp, boxes, geom, prim) over the curve geometries of a geometry list (None if there are none)."""
+    cps, gs, ps = [], [], []
+    for i, g in enumerate(geoms):
+        if isinstance(g, dict) and "curves" in g:
+            cp = curve_segments(g)
+            cps.append(cp)
+            gs.append(np.full(cp.shape[0], i, np.uint32))
+            ps.append(np.arange(cp.shape[0], dtype=np.uint32))
+    if not cps:
+        return None
+    cp = np.ascontiguousarray(np.concatenate(cps), np.float32)
+    box = np.zeros((cp.shape[0], 6), np.float32)
+    lib.orc_curve_boxes(cp.ctypes.data, cp.shape[0], box.ctypes.data)
+    return cp, box, np.ascontiguousarray(np.concatenate(gs)), np.ascontiguousarray(np.concatenate(ps))
 
 
 class _Attr(C.Structure):
@@ -407,6 +434,11 @@ class RenderScene:
                         C.cast(self.geom_tex, C.c_void_p),
                         C.cast(C.pointer(self.ibl.s), C.c_void_p) if self.ibl is not None else None, None,
                         self.attrs.ptr)
+        self.curves = _curve_arrays(sc["geoms"])
+        if self.curves is not None:
+            cp, box, cg, cpr = self.curves
+            self.s.curve_cp, self.s.curve_box = cp.ctypes.data, box.ctypes.data
+            self.s.curve_geom, self.s.curve_prim, self.s.n_curves = cg.ctypes.data, cpr.ctypes.data, cp.shape[0]
 
 
 def render_pt(rs: RenderScene, cam: dict, width, height, spp, max_path_length, seed, flags=0, threads=0):
@@ -639,7 +671,9 @@ def compose_instances(sc):
 
 class _IScene(C.Structure):
     _fields_ = [("insts", C.c_void_p), ("n_inst", C.c_uint32), ("tri9", C.c_void_p), ("tri_geom", C.c_void_p),
-                ("geom_first", C.c_void_p), ("geom_base", C.c_void_p), ("l2w", C.c_void_p)]
+                ("geom_first", C.c_void_p), ("geom_base", C.c_void_p), ("l2w", C.c_void_p), ("n_comp", C.c_uint32),
+                ("curve_cp", C.c_void_p), ("curve_box", C.c_void_p), ("curve_geom", C.c_void_p),
+                ("curve_prim", C.c_void_p), ("n_curves", C.c_void_p)]
 
 
 class InstancedRenderScene:
@@ -662,11 +696,16 @@ class InstancedRenderScene:
                   "geom_first": np.cumsum([0] + [t.shape[0] for t in tris[:-1]]).astype(np.uint32),
                   "materials": np.ascontiguousarray(mdesc["materials"], np.float32),
                   "geom_material": np.ascontiguousarray(mdesc["geom_material"], np.uint32),
-                  "n_geoms": len(tris), "bvh": CpuBvh(tri9), "geoms": mdesc["geoms"]}
+                  "n_geoms": len(tris), "bvh": CpuBvh(tri9) if tri9.shape[0] else None, "geoms": mdesc["geoms"]}
             pts = tri9.reshape(-1, 3)
-            mm["box"] = (pts.min(0), pts.max(0))
+            mm["box"] = (pts.min(0), pts.max(0)) if pts.shape[0] else None
+            mm["curves"] = _curve_arrays(mdesc["geoms"])
             self.models.append(mm)
-        comp = [c for c in comp if self.models[c[0]]["tri9"].shape[0] > 0]
+        # composed instances: those of scenes with triangles (the library's top tree), then those of
+        # scenes of curves only, each in composition order
+        n_tri_comp = len([c for c in comp if self.models[c[0]]["tri9"].shape[0] > 0])
+        comp = ([c for c in comp if self.models[c[0]]["tri9"].shape[0] > 0] +
+                [c for c in comp if self.models[c[0]]["tri9"].shape[0] == 0 and self.models[c[0]]["curves"] is not None])
         order = []
         for c in comp:
             if c[0] not in order:
@@ -691,6 +730,8 @@ class InstancedRenderScene:
                     self.textures.append(t)
                     self.geom_tex[g0[mi] + gi] = t.s
         n = len(comp)
+        self.curve_ptrs = [(C.c_void_p * max(n, 1))() for _ in range(4)]
+        self.n_curves = np.zeros(max(n, 1), np.uint32)
         self.inst_arr = (_Inst * max(n, 1))()
         self.tri9_ptrs = (C.c_void_p * max(n, 1))()
         self.tri_geom_ptrs = (C.c_void_p * max(n, 1))()
@@ -702,12 +743,19 @@ class InstancedRenderScene:
         for k, (mi, l2w, _, _) in enumerate(comp):
             mm = self.models[mi]
             w2l = np.array(affine_inverse(l2w), np.float32)
-            self.inst_arr[k] = _Inst((C.c_float * 12)(*[float(x) for x in w2l]), mm["bvh"].handle)
+            self.inst_arr[k] = _Inst((C.c_float * 12)(*[float(x) for x in w2l]),
+                                     mm["bvh"].handle if mm["bvh"] is not None else None)
+            if mm["curves"] is not None:
+                for a, arr in zip(self.curve_ptrs, mm["curves"]):
+                    a[k] = arr.ctypes.data
+                self.n_curves[k] = mm["curves"][0].shape[0]
             self.tri9_ptrs[k] = mm["tri9"].ctypes.data
             self.tri_geom_ptrs[k] = mm["tri_geom"].ctypes.data
             self.geom_first_ptrs[k] = mm["geom_first"].ctypes.data
             self.geom_base[k] = g0[mi]
             self.l2w[k] = [l2w[4 * r + c] for r in range(3) for c in range(3)]
+            if mm["box"] is None:
+                continue
             # the instance's world box: the placed scene's box corners through l2w in double,
             # rounded outward to float (the library's top-tree boxes; they bound the emission sphere)
             blo, bhi = mm["box"]
@@ -722,9 +770,11 @@ class InstancedRenderScene:
             for r in range(3):
                 lo[r] = min(lo[r], np.nextafter(np.float32(wlo[r]), np.float32(-np.inf)))
                 hi[r] = max(hi[r], np.nextafter(np.float32(whi[r]), np.float32(np.inf)))
-        self.iscene = _IScene(C.cast(self.inst_arr, C.c_void_p), n, C.cast(self.tri9_ptrs, C.c_void_p),
+        self.iscene = _IScene(C.cast(self.inst_arr, C.c_void_p), n_tri_comp, C.cast(self.tri9_ptrs, C.c_void_p),
                               C.cast(self.tri_geom_ptrs, C.c_void_p), C.cast(self.geom_first_ptrs, C.c_void_p),
-                              self.geom_base.ctypes.data, self.l2w.ctypes.data)
+                              self.geom_base.ctypes.data, self.l2w.ctypes.data, n,
+                              *[C.cast(a, C.c_void_p) for a in self.curve_ptrs],
+                              self.n_curves.ctypes.data if self.n_curves.any() else None)
         half = np.float32(0.5)
         d = hi - lo
         self.sphere = np.array([(lo[0] + hi[0]) * half, (lo[1] + hi[1]) * half, (lo[2] + hi[2]) * half,
@@ -739,3 +789,59 @@ class InstancedRenderScene:
                         C.cast(self.geom_tex, C.c_void_p),
                         C.cast(C.pointer(self.ibl.s), C.c_void_p) if self.ibl is not None else None,
                         C.cast(C.pointer(self.iscene), C.c_void_p), self.attrs.ptr)
+
+
+# ---- round B-spline curves (orc_curve.c) ---------------------------------------------------------
+
+_D = C.POINTER(C.c_double)
+lib.orc_curve_hit_dense.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_float, _D, _D, C.c_void_p]
+lib.orc_curve_hit.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_float, C.c_int, _D, _D, C.c_void_p]
+lib.orc_curve_clearance.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+lib.orc_curve_clearance.restype = C.c_double
+lib.orc_curves_closest.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(_Rays), C.c_uint32, C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.c_void_p, C.c_int, C.c_int]
+lib.orc_curves_occluded.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(_Rays), C.c_uint32, C.c_void_p, C.c_int]
+lib.orc_curve_boxes.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
+
+
+def curve_hit(cp, o, d, tnear=0.0, tfar=3.4e38, dense=False, any_hit=False):
+    """One segment (4 x (x, y, z, r) control points): (t, u, ng) in double, or None on a miss."""
+    cp = np.ascontiguousarray(cp, np.float32).reshape(16)
+    o = np.ascontiguousarray(o, np.float32)
+    d = np.ascontiguousarray(d, np.float32)
+    t, u = C.c_double(), C.c_double()
+    ng = np.zeros(3, np.float64)
+    if dense:
+        hit = lib.orc_curve_hit_dense(cp.ctypes.data, o.ctypes.data, d.ctypes.data, tnear, tfar, C.byref(t), C.byref(u),
+                                      ng.ctypes.data)
+    else:
+        hit = lib.orc_curve_hit(cp.ctypes.data, o.ctypes.data, d.ctypes.data, tnear, tfar, int(any_hit), C.byref(t),
+                                C.byref(u), ng.ctypes.data)
+    return (t.value, u.value, ng) if hit else None
+
+
+def curve_clearance(cp, o, d):
+    cp = np.ascontiguousarray(cp, np.float32).reshape(16)
+    o = np.ascontiguousarray(o, np.float32)
+    d = np.ascontiguousarray(d, np.float32)
+    return lib.orc_curve_clearance(cp.ctypes.data, o.ctypes.data, d.ctypes.data)
+
+
+def curves_closest(cps, rays, dense=False, threads=0):
+    """Closest hits of a ray dict (RAY_KEYS) on curve segments cps (nseg, 16): t, u, prim (segment,
+    0xFFFFFFFF on a miss), ng (n, 3)."""
+    cps = np.ascontiguousarray(cps, np.float32).reshape(-1, 16)
+    rs, keep, n = _rays(rays)
+    out = {"t": np.zeros(n, np.float32), "u": np.zeros(n, np.float32), "prim": np.zeros(n, np.uint32),
+           "ng": np.zeros((n, 3), np.float32)}
+    lib.orc_curves_closest(cps.ctypes.data, cps.shape[0], C.byref(rs), n, out["t"].ctypes.data, out["u"].ctypes.data,
+                           out["prim"].ctypes.data, out["ng"].ctypes.data, int(dense), _threads(threads))
+    return out
+
+
+def curves_occluded(cps, rays, threads=0):
+    cps = np.ascontiguousarray(cps, np.float32).reshape(-1, 16)
+    rs, keep, n = _rays(rays)
+    occ = np.zeros(n, np.uint32)
+    lib.orc_curves_occluded(cps.ctypes.data, cps.shape[0], C.byref(rs), n, occ.ctypes.data, _threads(threads))
+    return occ

@@ -170,6 +170,15 @@ typedef struct {
     const uint32_t* const* geom_first;
     const uint32_t* geom_base;
     const float* l2w;
+    /* round B-spline curves (orc_curve.c) per composed instance, NULL for none: the placed scene's
+     * segments in its own space (16 floats each), their boxes, geometries and segment indices; the
+     * composed instances n_inst .. n_comp - 1 place scenes of curves only (no triangle tree) */
+    uint32_t n_comp;
+    const float* const* curve_cp;
+    const float* const* curve_box;
+    const uint32_t* const* curve_geom;
+    const uint32_t* const* curve_prim;
+    const uint32_t* n_curves;
 } orc_iscene;
 
 /* Vertex attributes of one geometry, as the device keeps them: per primitive its corners' values
@@ -204,6 +213,13 @@ typedef struct {
                                    geom_first are unused and the shading tables above are indexed
                                    by the scene's list of geometries */
     const orc_attr* geom_attr;  /* vertex attributes per geometry (scene list), NULL for none */
+    /* round B-spline curves of a scene of geometries (world-space segments, 16 floats each), their
+     * boxes, geometries and segment indices within them; n_curves = 0 for none */
+    const float* curve_cp;
+    const float* curve_box;
+    const uint32_t* curve_geom;
+    const uint32_t* curve_prim;
+    uint32_t n_curves;
 } orc_scene;
 
 /* Closest hit in a scene of geometries or an instanced scene: 0 on a miss; else t, u, v, the
@@ -289,6 +305,22 @@ void orc_ptex_eval(const orc_ptex* tx, int face, float u, float v, float* out3);
 void orc_ptex_tri_eval(const orc_ptex* tx, int face, float u, float v, float* out3);
 void orc_ptex_eval_batch(const orc_ptex* tx, uint32_t n, const int32_t* faces, const float* u, const float* v,
                          float* out);
+
+/* Round B-spline curves (orc_curve.c): segments of 16 floats (4 control points x, y, z, radius) */
+int orc_curve_hit_dense(const float* cp, const float* o, const float* d, float tnear, float tfar, double* t,
+                        double* u, double* ng);
+int orc_curve_hit(const float* cp, const float* o, const float* d, float tnear, float tfar, int any, double* t,
+                  double* u, double* ng);
+double orc_curve_clearance(const float* cp, const float* o, const float* d);
+void orc_curve_boxes(const float* cps, uint32_t nseg, float* boxes);
+int orc_curves_closest1(const float* cps, const float* boxes, uint32_t nseg, const float* o, const float* d,
+                        float tnear, float tfar, int dense, float* t, float* u, uint32_t* prim, float* ng);
+int orc_curves_occluded1(const float* cps, const float* boxes, uint32_t nseg, const float* o, const float* d,
+                         float tnear, float tfar);
+void orc_curves_closest(const float* cps, uint32_t nseg, const orc_rays* rays, uint32_t n, float* t, float* u,
+                        uint32_t* prim, float* ng, int dense, int threads);
+void orc_curves_occluded(const float* cps, uint32_t nseg, const orc_rays* rays, uint32_t n, uint32_t* occ,
+                         int threads);
 
 #ifdef __cplusplus
 }

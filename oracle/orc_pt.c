@@ -306,11 +306,27 @@ int orc_scene_closest1(const orc_scene* sc, uint32_t* stack, const float* o, con
     if (l2w) *l2w = NULL;
     if (!sc->inst) {
         uint32_t prim = orc_bvh_closest1(sc->bvh, stack, o, d, tnear, tfar, t, u, v);
-        if (prim == 0xFFFFFFFFu) return 0;
-        *geom = sc->tri_geom[prim];
-        tri_normal(sc->tri9 + (size_t)prim * 9, ng);
-        *tri_in_geom = sc->geom_first ? prim - sc->geom_first[*geom] : prim;
-        return 1;
+        int hit = prim != 0xFFFFFFFFu;
+        if (hit) {
+            *geom = sc->tri_geom[prim];
+            tri_normal(sc->tri9 + (size_t)prim * 9, ng);
+            *tri_in_geom = sc->geom_first ? prim - sc->geom_first[*geom] : prim;
+        }
+        /* the curves after the triangles: a strictly closer curve hit replaces the triangle's */
+        float tc, uc, nc[3];
+        uint32_t pc;
+        if (sc->n_curves && orc_curves_closest1(sc->curve_cp, sc->curve_box, sc->n_curves, o, d, tnear,
+                                                hit ? *t : tfar, 0, &tc, &uc, &pc, nc) &&
+            (!hit || tc < *t)) {
+            *t = tc;
+            *u = uc;
+            *v = 0.0f;
+            *geom = sc->curve_geom[pc];
+            *tri_in_geom = sc->curve_prim[pc];
+            for (int k = 0; k < 3; ++k) ng[k] = nc[k];
+            hit = 1;
+        }
+        return hit;
     }
     const orc_iscene* is = sc->inst;
     float best_t = tfar, bu = 0.0f, bv = 0.0f;
@@ -327,7 +343,37 @@ int orc_scene_closest1(const orc_scene* sc, uint32_t* stack, const float* o, con
             bv = vk;
         }
     }
+    /* curves of the composed instances, in their own space (t is the world ray's parameter) */
+    uint32_t cbest = 0xFFFFFFFFu, cbest_k = 0;
+    float cng[3] = {0.0f, 0.0f, 0.0f};
+    if (is->n_curves)
+        for (uint32_t k = 0; k < is->n_comp; ++k) {
+            if (!is->n_curves[k]) continue;
+            float lo[3], ldir[3], tk, uk, nk[3];
+            uint32_t pk;
+            inst_ray(is->insts[k].w2l, o, d, lo, ldir);
+            const int found = best != 0xFFFFFFFFu || cbest != 0xFFFFFFFFu;
+            if (orc_curves_closest1(is->curve_cp[k], is->curve_box[k], is->n_curves[k], lo, ldir, tnear, best_t, 0, &tk,
+                                    &uk, &pk, nk) &&
+                (!found || tk < best_t)) {
+                best_t = tk;
+                bu = uk;
+                bv = 0.0f;
+                cbest = pk;
+                cbest_k = k;
+                for (int q = 0; q < 3; ++q) cng[q] = nk[q];
+            }
+        }
     *t = best_t;
+    if (cbest != 0xFFFFFFFFu) {
+        *u = bu;
+        *v = bv;
+        *geom = is->geom_base[cbest_k] + is->curve_geom[cbest_k][cbest];
+        *tri_in_geom = is->curve_prim[cbest_k][cbest];
+        xform_linear(is->l2w + 9 * (size_t)cbest_k, cng, ng);
+        if (l2w) *l2w = is->l2w + 9 * (size_t)cbest_k;
+        return 1;
+    }
     if (best == 0xFFFFFFFFu) return 0;
     *u = bu;
     *v = bv;
@@ -343,12 +389,23 @@ int orc_scene_closest1(const orc_scene* sc, uint32_t* stack, const float* o, con
 
 int orc_scene_occluded1(const orc_scene* sc, uint32_t* stack, const float* o, const float* d, float tnear, float tfar)
 {
-    if (!sc->inst) return orc_bvh_occluded1(sc->bvh, stack, o, d, tnear, tfar);
-    for (uint32_t k = 0; k < sc->inst->n_inst; ++k) {
+    if (!sc->inst)
+        return orc_bvh_occluded1(sc->bvh, stack, o, d, tnear, tfar) ||
+               (sc->n_curves && orc_curves_occluded1(sc->curve_cp, sc->curve_box, sc->n_curves, o, d, tnear, tfar));
+    const orc_iscene* is = sc->inst;
+    for (uint32_t k = 0; k < is->n_inst; ++k) {
         float lo[3], ldir[3];
-        inst_ray(sc->inst->insts[k].w2l, o, d, lo, ldir);
-        if (orc_bvh_occluded1(sc->inst->insts[k].blas, stack, lo, ldir, tnear, tfar)) return 1;
+        inst_ray(is->insts[k].w2l, o, d, lo, ldir);
+        if (orc_bvh_occluded1(is->insts[k].blas, stack, lo, ldir, tnear, tfar)) return 1;
     }
+    if (is->n_curves)
+        for (uint32_t k = 0; k < is->n_comp; ++k) {
+            if (!is->n_curves[k]) continue;
+            float lo[3], ldir[3];
+            inst_ray(is->insts[k].w2l, o, d, lo, ldir);
+            if (orc_curves_occluded1(is->curve_cp[k], is->curve_box[k], is->n_curves[k], lo, ldir, tnear, tfar))
+                return 1;
+        }
     return 0;
 }
 

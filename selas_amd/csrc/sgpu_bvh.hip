@@ -776,6 +776,134 @@ int build_binary(uint32_t n, const float4* d_blo, const float4* d_bhi, const flo
 
 }  // namespace
 
+namespace {
+
+// Box of each curve segment (its control points' box grown by the largest radius: the B-spline
+// basis weights are non-negative and sum to one, so the swept balls stay inside), centroid, and the
+// ordered-uint bounds k_prep computes.
+__global__ __launch_bounds__(kBlock) void k_curve_prep(const CurveRecord* __restrict rec, uint32_t n,
+                                                       float4* __restrict blo, float4* __restrict bhi,
+                                                       float4* __restrict cent, uint32_t* __restrict bounds)
+{
+    __shared__ uint32_t red[12][kBlock / 64];
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t v[12];
+    for (int k = 0; k < 12; ++k) v[k] = 0xFFFFFFFFu;
+    if (i < n) {
+        const CurveRecord r = rec[i];
+        f3 lo = mk3(r.p[0].x, r.p[0].y, r.p[0].z), hi = lo;
+        float rmax = r.p[0].w;
+        for (int j = 1; j < 4; ++j) {
+            lo = mk3(fminf(lo.x, r.p[j].x), fminf(lo.y, r.p[j].y), fminf(lo.z, r.p[j].z));
+            hi = mk3(fmaxf(hi.x, r.p[j].x), fmaxf(hi.y, r.p[j].y), fmaxf(hi.z, r.p[j].z));
+            rmax = fmaxf(rmax, r.p[j].w);
+        }
+        lo = mk3(lo.x - rmax, lo.y - rmax, lo.z - rmax);
+        hi = mk3(hi.x + rmax, hi.y + rmax, hi.z + rmax);
+        const f3 c = mk3(0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z));
+        blo[i] = make_float4(lo.x, lo.y, lo.z, 0.0f);
+        bhi[i] = make_float4(hi.x, hi.y, hi.z, 0.0f);
+        cent[i] = make_float4(c.x, c.y, c.z, 0.0f);
+        v[0] = f2ord(lo.x); v[1] = f2ord(lo.y); v[2] = f2ord(lo.z);
+        v[3] = ~f2ord(hi.x); v[4] = ~f2ord(hi.y); v[5] = ~f2ord(hi.z);
+        v[6] = f2ord(c.x); v[7] = f2ord(c.y); v[8] = f2ord(c.z);
+        v[9] = ~f2ord(c.x); v[10] = ~f2ord(c.y); v[11] = ~f2ord(c.z);
+    }
+    for (int k = 0; k < 12; ++k) {
+        uint32_t x = v[k];
+        for (int off = 32; off > 0; off >>= 1) x = min(x, (uint32_t)__shfl_xor((int)x, off, 64));
+        v[k] = x;
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 12; ++k) red[k][wave] = v[k];
+    __syncthreads();
+    if (threadIdx.x < 12) {
+        uint32_t x = red[threadIdx.x][0];
+        for (int w = 1; w < kBlock / 64; ++w) x = min(x, red[threadIdx.x][w]);
+        atomicMin(&bounds[threadIdx.x], x);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_curve_reorder(const CurveRecord* __restrict in,
+                                                          const uint32_t* __restrict order, uint32_t n,
+                                                          CurveRecord* __restrict out)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) out[i] = in[order[i]];
+}
+
+}  // namespace
+
+int sgpu_curve_build(sgpu_scene scene, const std::vector<float>& cp, const std::vector<uint32_t>& geom,
+                     const std::vector<uint32_t>& prim, const std::vector<uint32_t>& inst, hipStream_t stream)
+{
+    const uint32_t n = (uint32_t)geom.size();
+    scene->n_curves = 0;
+    if (n == 0) return SGPU_OK;
+    if (n > (1u << 28)) return sgpu_fail(SGPU_E_RANGE, "scene has more than 2^28 curve segments");
+    std::vector<CurveRecord> rec(n);
+    for (uint32_t s = 0; s < n; ++s) {
+        const float* c = cp.data() + 16 * (size_t)s;
+        for (int j = 0; j < 4; ++j) rec[s].p[j] = make_float4(c[4 * j], c[4 * j + 1], c[4 * j + 2], c[4 * j + 3]);
+        rec[s].geom = geom[s];
+        rec[s].prim = prim[s];
+        rec[s].inst = inst[s];
+        rec[s].pad = 0;
+    }
+    DevBuf<CurveRecord> d_rec;
+    DevBuf<float4> d_blo, d_bhi, d_cent;
+    DevBuf<uint32_t> d_bounds, d_order;
+    SGPU_HIP_TRY(d_rec.alloc(n));
+    SGPU_HIP_TRY(d_blo.alloc(n));
+    SGPU_HIP_TRY(d_bhi.alloc(n));
+    SGPU_HIP_TRY(d_cent.alloc(n));
+    SGPU_HIP_TRY(d_bounds.alloc(12));
+    SGPU_HIP_TRY(hipMemcpyAsync(d_rec.p, rec.data(), n * sizeof(CurveRecord), hipMemcpyHostToDevice, stream));
+    SGPU_HIP_TRY(hipMemsetAsync(d_bounds.p, 0xFF, 12 * sizeof(uint32_t), stream));
+    const uint32_t grid = (n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_curve_prep, dim3(grid), dim3(kBlock), 0, stream, d_rec.p, n, d_blo.p, d_bhi.p, d_cent.p,
+                       d_bounds.p);
+    SGPU_HIP_TRY(hipGetLastError());
+    uint32_t hb[12];
+    SGPU_HIP_TRY(hipMemcpyAsync(hb, d_bounds.p, sizeof(hb), hipMemcpyDeviceToHost, stream));
+    BvhNode* nodes = nullptr;
+    uint32_t depth = 0;
+    int rc = build_binary(n, d_blo.p, d_bhi.p, d_cent.p, d_bounds.p, stream, &nodes, d_order, &depth, 0);
+    if (rc != SGPU_OK) return rc;
+    if (depth + 1 > kCurveStack) {
+        (void)hipFree(nodes);
+        return sgpu_fail(SGPU_E_RANGE, "curve tree deeper than the curve traversal's stack");
+    }
+    CurveRecord* out = nullptr;
+    hipError_t e = hipMalloc((void**)&out, (size_t)n * sizeof(CurveRecord));
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_curve_reorder, dim3(grid), dim3(kBlock), 0, stream, d_rec.p, d_order.p, n, out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) {
+        (void)hipFree(nodes);
+        if (out) (void)hipFree(out);
+        return sgpu_fail(SGPU_E_HIP, std::string("curve records: ") + hipGetErrorString(e));
+    }
+    scene->d_cnodes = nodes;
+    scene->d_curves = out;
+    scene->n_curves = n;
+    // the scene's box takes in the curves
+    for (int k = 0; k < 3; ++k) {
+        const float lo = host_ord2f(hb[k]), hi = host_ord2f(~hb[3 + k]);
+        if (scene->box[k] > scene->box[3 + k]) {
+            scene->box[k] = lo;
+            scene->box[3 + k] = hi;
+        } else {
+            scene->box[k] = std::min(scene->box[k], lo);
+            scene->box[3 + k] = std::max(scene->box[3 + k], hi);
+        }
+    }
+    return SGPU_OK;
+}
+
 int sgpu_bvh_build(sgpu_scene scene, hipStream_t stream)
 {
     const uint32_t n = (uint32_t)scene->tri_geom.size();
@@ -805,7 +933,7 @@ int sgpu_bvh_build(sgpu_scene scene, hipStream_t stream)
     // device copy of the geometries' first indices with bit 31 set for quad geometries
     std::vector<uint32_t> first_flagged(scene->geom_first.size());
     for (size_t g = 0; g < first_flagged.size(); ++g)
-        first_flagged[g] = scene->geom_first[g] | (scene->geom_quad[g] ? 0x80000000u : 0u);
+        first_flagged[g] = scene->geom_first[g] | (scene->geom_quad[g] == 1 ? 0x80000000u : 0u);
     SGPU_HIP_TRY(hipMemcpy(d_first.p, first_flagged.data(), first_flagged.size() * sizeof(uint32_t),
                            hipMemcpyHostToDevice));
     SGPU_HIP_TRY(hipMemsetAsync(d_bounds.p, 0xFF, 12 * sizeof(uint32_t), stream));
@@ -884,6 +1012,57 @@ bool affine_inverse(const double* m, double* r)
 
 }  // namespace
 
+namespace {
+
+// The curves of the composed instances (placed scene, local-to-world row-major 3 x 4 in double) in
+// world space, each segment tagged with its composed instance, and the instanced scene's curve
+// tree over them. A placement that scales unevenly or shears would turn the swept spheres into
+// ellipsoids, which the world-space curve test does not represent: such a placement is refused.
+int bake_instance_curves(sgpu_scene scene, const std::vector<std::pair<sgpu_scene, const double*>>& comp,
+                         hipStream_t stream)
+{
+    std::vector<float> cp;
+    std::vector<uint32_t> geom, prim, inst;
+    for (uint32_t k = 0; k < comp.size(); ++k) {
+        const sgpu_scene b = comp[k].first;
+        const double* m = comp[k].second;
+        if (b->curve_geom.empty()) continue;
+        double n2[3], s2 = 0.0;
+        for (int c = 0; c < 3; ++c) {
+            n2[c] = m[c] * m[c] + m[4 + c] * m[4 + c] + m[8 + c] * m[8 + c];
+            s2 += n2[c] / 3.0;
+        }
+        bool similar = s2 > 0.0;
+        for (int c = 0; c < 3 && similar; ++c) {
+            similar &= std::fabs(n2[c] - s2) <= 1e-5 * s2;
+            for (int e = c + 1; e < 3; ++e)
+                similar &= std::fabs(m[c] * m[e] + m[4 + c] * m[4 + e] + m[8 + c] * m[8 + e]) <= 1e-5 * s2;
+        }
+        if (!similar)
+            return sgpu_fail(SGPU_E_INVALID, "an instance placing curves is not a similarity (rotation, uniform scale, "
+                                             "translation)");
+        const double sc = std::sqrt(s2);
+        const size_t nseg = b->curve_geom.size(), base = cp.size();
+        cp.resize(base + 16 * nseg);
+        for (size_t i = 0; i < nseg; ++i) {
+            const float* src = b->curve_cp.data() + 16 * i;
+            float* dst = cp.data() + base + 16 * i;
+            for (int j = 0; j < 4; ++j) {
+                const double p[3] = {src[4 * j], src[4 * j + 1], src[4 * j + 2]};
+                for (int r = 0; r < 3; ++r)
+                    dst[4 * j + r] = (float)(m[4 * r] * p[0] + m[4 * r + 1] * p[1] + m[4 * r + 2] * p[2] + m[4 * r + 3]);
+                dst[4 * j + 3] = (float)(src[4 * j + 3] * sc);
+            }
+            geom.push_back(b->curve_geom[i]);
+            prim.push_back(b->curve_prim[i]);
+            inst.push_back(k);
+        }
+    }
+    return sgpu_curve_build(scene, cp, geom, prim, inst, stream);
+}
+
+}  // namespace
+
 int sgpu_bvh_build_instances(sgpu_scene scene, hipStream_t stream)
 {
     // compose the levels: (instance i of a scene of geometries) or (instance i of an instanced
@@ -911,15 +1090,60 @@ int sgpu_bvh_build_instances(sgpu_scene scene, hipStream_t stream)
             }
         }
     }
-    // scenes without triangles place nothing
+    // scenes without triangles are not in the top tree; those with curves are composed after the
+    // tree's instances (the curve pass reaches them through their curves), the others place nothing
+    std::vector<Flat> extra;
+    for (const Flat& f : flat)
+        if (f.blas->n_tris == 0 && !f.blas->curve_geom.empty()) extra.push_back(f);
     flat.erase(std::remove_if(flat.begin(), flat.end(), [](const Flat& f) { return f.blas->n_tris == 0; }), flat.end());
-    if (flat.size() > (1u << 30)) return sgpu_fail(SGPU_E_RANGE, "scene has more than 2^30 composed instances");
+    if (flat.size() + extra.size() > (1u << 30)) return sgpu_fail(SGPU_E_RANGE, "scene has more than 2^30 composed instances");
     const uint32_t n = (uint32_t)flat.size();
     scene->n_tris = 0;
     scene->n_nodes = 0;
     scene->max_depth = 0;
     scene->inst_levels = levels;
-    if (n == 0) return SGPU_OK;
+    scene->comp_blas.clear();
+    scene->comp_l2w.clear();
+    // composed instances past the tree's: records (world-to-local), ids, placed scene and transform
+    std::vector<InstRecord> xrec(extra.size());
+    std::vector<uint32_t> xids(2 * extra.size());
+    for (size_t e = 0; e < extra.size(); ++e) {
+        double w2l[12];
+        if (!affine_inverse(extra[e].l2w, w2l)) return sgpu_fail(SGPU_E_INVALID, "instance transform is singular");
+        for (int r = 0; r < 3; ++r)
+            xrec[e].w2l[r] = make_float4((float)w2l[4 * r], (float)w2l[4 * r + 1], (float)w2l[4 * r + 2], (float)w2l[4 * r + 3]);
+        xrec[e].wide = nullptr;
+        xrec[e].tris = nullptr;
+        xids[2 * e] = extra[e].id0;
+        xids[2 * e + 1] = extra[e].id1;
+    }
+    auto composed = [&]() {
+        std::vector<std::pair<sgpu_scene, const double*>> c;
+        for (const Flat& f : flat) c.emplace_back(f.blas, f.l2w);
+        for (const Flat& f : extra) c.emplace_back(f.blas, f.l2w);
+        return c;
+    };
+    auto push_extra_shading = [&]() {
+        for (const Flat& f : extra) {
+            scene->comp_blas.push_back(f.blas);
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) scene->comp_l2w.push_back((float)f.l2w[4 * r + c]);
+        }
+    };
+    if (n == 0) {
+        if (extra.empty()) return SGPU_OK;
+        // only curves: no top tree; the composed instances' records for the curve pass and shading
+        push_extra_shading();
+        hipError_t e = hipMalloc((void**)&scene->d_inst, xrec.size() * sizeof(InstRecord));
+        if (e == hipSuccess) e = hipMalloc((void**)&scene->d_inst_ids, xids.size() * sizeof(uint32_t));
+        if (e == hipSuccess) e = hipMemcpy(scene->d_inst, xrec.data(), xrec.size() * sizeof(InstRecord), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(scene->d_inst_ids, xids.data(), xids.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return sgpu_fail(SGPU_E_HIP, std::string("instance records: ") + hipGetErrorString(e));
+        scene->n_inst = 0;
+        scene->stack_cap = 1;
+        return bake_instance_curves(scene, composed(), stream);
+    }
 
     // The top tree's box of an instance: the boxes of the placed scene's root children (decoded
     // from its 8-wide root as the traversal decodes them) through the transform, their 8 corners in
@@ -952,8 +1176,6 @@ int sgpu_bvh_build_instances(sgpu_scene scene, hipStream_t stream)
         root_boxes.emplace_back(b, out);
         return SGPU_OK;
     };
-    scene->comp_blas.clear();
-    scene->comp_l2w.clear();
     std::vector<float4> blo(n), bhi(n), cent(n);
     std::vector<InstRecord> rec(n);
     std::vector<uint32_t> ids(2 * (size_t)n);
@@ -1015,6 +1237,9 @@ int sgpu_bvh_build_instances(sgpu_scene scene, hipStream_t stream)
         ids[2 * k + 1] = f.id1;
         blas_cap = std::max(blas_cap, f.blas->stack_cap);
     }
+    push_extra_shading();
+    rec.insert(rec.end(), xrec.begin(), xrec.end());
+    ids.insert(ids.end(), xids.begin(), xids.end());
     uint32_t hb[12];
     for (int r = 0; r < 3; ++r) {
         hb[r] = host_f2ord(slo[r]);
@@ -1055,10 +1280,10 @@ int sgpu_bvh_build_instances(sgpu_scene scene, hipStream_t stream)
     }
     InstRecord* d_inst = nullptr;
     uint32_t* d_ids = nullptr;
-    if (e == hipSuccess) e = hipMalloc((void**)&d_inst, n * sizeof(InstRecord));
-    if (e == hipSuccess) e = hipMalloc((void**)&d_ids, 2 * (size_t)n * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMemcpy(d_inst, rec.data(), n * sizeof(InstRecord), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d_ids, ids.data(), 2 * (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc((void**)&d_inst, rec.size() * sizeof(InstRecord));
+    if (e == hipSuccess) e = hipMalloc((void**)&d_ids, ids.size() * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemcpy(d_inst, rec.data(), rec.size() * sizeof(InstRecord), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_ids, ids.data(), ids.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         (void)hipFree(nodes);
         if (d_inst) (void)hipFree(d_inst);
@@ -1075,5 +1300,6 @@ int sgpu_bvh_build_instances(sgpu_scene scene, hipStream_t stream)
     rc = build_wide(scene, nodes, scene->n_nodes, stream);
     // the top tree's stack, the exit entry, and the deepest placed tree's stack
     scene->stack_cap = std::max(scene->max_depth + 1, kWideMaxPush * scene->wide_depth + 1) + 1 + blas_cap;
-    return rc;
+    if (rc != SGPU_OK) return rc;
+    return bake_instance_curves(scene, composed(), stream);
 }

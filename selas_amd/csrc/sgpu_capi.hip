@@ -362,6 +362,88 @@ int sgpu_scene_add_quads(sgpu_scene scene, const float* positions, uint32_t n_ve
     return SGPU_OK;
 }
 
+int sgpu_scene_add_curves(sgpu_scene scene, const float* vertices, uint32_t n_verts, const uint32_t* indices,
+                          uint32_t n_segments, const float* colmajor4x4, uint32_t* geom_id_out)
+{
+    if (!scene) return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_add_curves: NULL scene");
+    if (n_segments > 0 && (!vertices || !indices))
+        return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_add_curves: NULL vertices or indices");
+    std::lock_guard<std::mutex> lock(scene->mu);
+    if (scene->committed) return sgpu_fail(SGPU_E_STATE, "sgpu_scene_add_curves: scene already committed");
+    if (!scene->instances.empty())
+        return sgpu_fail(SGPU_E_STATE, "sgpu_scene_add_curves: the scene holds instances, not geometries");
+    if ((uint64_t)scene->curve_geom.size() + n_segments > (1ull << 28))
+        return sgpu_fail(SGPU_E_RANGE, "sgpu_scene_add_curves: more than 2^28 curve segments");
+    for (uint32_t s = 0; s < n_segments; ++s)
+        if ((uint64_t)indices[s] + 3 >= n_verts)
+            return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_add_curves: a segment's control points run past the vertices");
+    for (uint64_t v = 0; v < (uint64_t)n_verts; ++v) {
+        const float r = vertices[4 * v + 3];
+        if (!(r >= 0.0f) || !std::isfinite(r))
+            return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_add_curves: a radius is negative or not finite");
+    }
+    // the placement scales radii by its scale factor: a similarity keeps the swept spheres round
+    float scale = 1.0f;
+    if (colmajor4x4) {
+        double c[3][3], n2[3];
+        for (int j = 0; j < 3; ++j) {
+            n2[j] = 0.0;
+            for (int i = 0; i < 3; ++i) {
+                c[j][i] = colmajor4x4[4 * j + i];
+                n2[j] += c[j][i] * c[j][i];
+            }
+        }
+        const double s2 = (n2[0] + n2[1] + n2[2]) / 3.0;
+        bool similar = s2 > 0.0;
+        for (int j = 0; j < 3 && similar; ++j) {
+            similar &= std::fabs(n2[j] - s2) <= 1e-5 * s2;
+            for (int k = j + 1; k < 3; ++k)
+                similar &= std::fabs(c[j][0] * c[k][0] + c[j][1] * c[k][1] + c[j][2] * c[k][2]) <= 1e-5 * s2;
+        }
+        if (!similar)
+            return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_add_curves: the placement is not a similarity (rotation, "
+                                             "uniform scale, translation)");
+        scale = (float)std::sqrt(s2);
+    }
+    const uint32_t geom = (uint32_t)scene->geom_first.size();
+    const size_t base = scene->curve_cp.size();
+    try {
+        scene->curve_cp.resize(base + (size_t)n_segments * 16);
+        scene->curve_geom.resize(scene->curve_geom.size() + n_segments, geom);
+        scene->curve_prim.reserve(scene->curve_prim.size() + n_segments);
+        scene->geom_idx.emplace_back();
+        scene->geom_nverts.push_back(0);
+        scene->geom_xform.emplace_back();
+    } catch (...) {
+        scene->curve_cp.resize(base);
+        scene->curve_geom.resize(scene->curve_prim.size());
+        scene->geom_idx.resize(geom);
+        scene->geom_nverts.resize(geom);
+        scene->geom_xform.resize(geom);
+        return sgpu_fail(SGPU_E_NOMEM, "sgpu_scene_add_curves: out of host memory");
+    }
+    float* dst = scene->curve_cp.data() + base;
+    for (uint32_t s = 0; s < n_segments; ++s) {
+        scene->curve_prim.push_back(s);
+        for (int j = 0; j < 4; ++j) {
+            const float* p = vertices + 4 * ((size_t)indices[s] + j);
+            float* o = dst + 16 * (size_t)s + 4 * j;
+            if (colmajor4x4) xform_point(colmajor4x4, p, o);
+            else {
+                o[0] = p[0];
+                o[1] = p[1];
+                o[2] = p[2];
+            }
+            o[3] = p[3] * scale;
+        }
+    }
+    // a geometry of no triangles: its triangle range is empty
+    scene->geom_first.push_back((uint32_t)scene->tri_geom.size());
+    scene->geom_quad.push_back(2);
+    if (geom_id_out) *geom_id_out = geom;
+    return SGPU_OK;
+}
+
 int sgpu_scene_set_vertex_attributes(sgpu_scene scene, uint32_t geom_id, const float* normals, const float* tangents,
                                      const float* uvs, uint32_t n_verts)
 {
@@ -372,6 +454,8 @@ int sgpu_scene_set_vertex_attributes(sgpu_scene scene, uint32_t geom_id, const f
     std::lock_guard<std::mutex> lock(scene->mu);
     if (scene->committed) return sgpu_fail(SGPU_E_STATE, std::string(who) + ": scene already committed");
     if (geom_id >= scene->geom_first.size()) return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": no such geometry");
+    if (scene->geom_quad[geom_id] == 2)
+        return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": a curve geometry (curves take no vertex attributes)");
     if (n_verts != scene->geom_nverts[geom_id])
         return sgpu_fail(SGPU_E_INVALID, std::string(who) + ": vertex count differs from the geometry's");
     const size_t nv = n_verts;
@@ -397,7 +481,7 @@ int sgpu_scene_set_vertex_attributes(sgpu_scene scene, uint32_t geom_id, const f
         }
     }
     const std::vector<uint32_t>& idx = scene->geom_idx[geom_id];
-    const uint32_t corners = scene->geom_quad[geom_id] ? 4u : 3u;
+    const uint32_t corners = scene->geom_quad[geom_id] == 1 ? 4u : 3u;
     const std::vector<float>& m = scene->geom_xform[geom_id];
     ga = sgpu::GeomAttr{nullptr, nullptr, nullptr, corners, 0};
     // per corner: the vertex's value, normals and tangents' xyz through the placement's linear part
@@ -544,7 +628,7 @@ int sgpu_scene_commit(sgpu_scene scene, void* stream)
         int rc = sgpu_bvh_build_instances(scene, as_stream(stream));
         if (rc == SGPU_OK) rc = gather_instance_shading(scene);
         if (rc != SGPU_OK) return rc;
-        if (scene->n_inst > 0) {
+        if (scene->n_inst > 0 || scene->n_curves > 0) {
             const float* b = scene->box;
             const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
             for (int k = 0; k < 3; ++k) scene->bsphere[k] = (b[k] + b[3 + k]) * 0.5f;
@@ -554,6 +638,9 @@ int sgpu_scene_commit(sgpu_scene scene, void* stream)
         return SGPU_OK;
     }
     int rc = sgpu_bvh_build(scene, as_stream(stream));
+    if (rc == SGPU_OK && !scene->curve_geom.empty())
+        rc = sgpu_curve_build(scene, scene->curve_cp, scene->curve_geom, scene->curve_prim,
+                              std::vector<uint32_t>(scene->curve_geom.size(), SGPU_INVALID_ID), as_stream(stream));
     if (rc != SGPU_OK) return rc;
     // the attribute table (one record per geometry) when any geometry has attributes
     bool any_attr = false;
@@ -569,14 +656,22 @@ int sgpu_scene_commit(sgpu_scene scene, void* stream)
     }
     std::vector<std::vector<uint32_t>>().swap(scene->geom_idx);
     std::vector<std::vector<float>>().swap(scene->geom_xform);
-    // the sphere around the triangles' bounding box (the IBL emits from its disc)
-    if (!scene->tri_pos.empty()) {
+    // the sphere around the triangles' (and curves') bounding box (the IBL emits from its disc)
+    if (!scene->tri_pos.empty() || scene->n_curves > 0) {
         float lo[3], hi[3];
-        for (int k = 0; k < 3; ++k) lo[k] = hi[k] = scene->tri_pos[k];
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = scene->tri_pos.empty() ? scene->box[k] : scene->tri_pos[k];
+            hi[k] = scene->tri_pos.empty() ? scene->box[3 + k] : scene->tri_pos[k];
+        }
         for (size_t i = 0; i < scene->tri_pos.size(); i += 3)
             for (int k = 0; k < 3; ++k) {
                 lo[k] = std::min(lo[k], scene->tri_pos[i + k]);
                 hi[k] = std::max(hi[k], scene->tri_pos[i + k]);
+            }
+        if (scene->n_curves > 0)
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = std::min(lo[k], scene->box[k]);
+                hi[k] = std::max(hi[k], scene->box[3 + k]);
             }
         const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
         for (int k = 0; k < 3; ++k) scene->bsphere[k] = (lo[k] + hi[k]) * 0.5f;
@@ -614,6 +709,8 @@ int sgpu_scene_release(sgpu_scene scene)
     if (scene->d_nodes) (void)hipFree(scene->d_nodes);
     if (scene->d_tris) (void)hipFree(scene->d_tris);
     if (scene->d_wide) (void)hipFree(scene->d_wide);
+    if (scene->d_cnodes) (void)hipFree(scene->d_cnodes);
+    if (scene->d_curves) (void)hipFree(scene->d_curves);
     if (scene->d_overflow) (void)hipFree(scene->d_overflow);
     if (scene->d_materials) (void)hipFree(scene->d_materials);
     if (scene->d_geom_material) (void)hipFree(scene->d_geom_material);
@@ -791,6 +888,7 @@ int sgpu_scene_set_ptex(sgpu_scene scene, uint32_t geom_id, uint32_t res_log2, u
         return sgpu_fail(SGPU_E_STATE, "sgpu_scene_set_ptex: jobs on the scene are admitted to a VCM server");
     const size_t n_geoms = scene->geom_first.size();
     if (geom_id >= n_geoms) return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_ptex: no such geometry");
+    if (scene->geom_quad[geom_id] == 2) return sgpu_fail(SGPU_E_INVALID, "sgpu_scene_set_ptex: a curve geometry");
     const bool tri = !scene->geom_quad[geom_id];
     const uint32_t first = scene->geom_first[geom_id];
     const uint32_t end = geom_id + 1 < n_geoms ? scene->geom_first[geom_id + 1] : (uint32_t)scene->tri_geom.size();
@@ -1071,7 +1169,8 @@ int sgpu_scene_device_bytes(sgpu_scene scene, uint64_t* bytes)
                  (uint64_t)scene->n_geom_material * sizeof(uint32_t) + (uint64_t)scene->n_lights * sizeof(sgpu_light) +
                  (uint64_t)scene->n_geom_tex * sizeof(sgpu::PtexTex) +
                  (uint64_t)scene->ibl_w * scene->ibl_h * 4 * sizeof(float) + (uint64_t)scene->ibl_h * sizeof(float) +
-                 (uint64_t)scene->n_inst * (sizeof(InstRecord) + 2 * sizeof(uint32_t));
+                 (uint64_t)scene->n_inst * (sizeof(InstRecord) + 2 * sizeof(uint32_t)) +
+                 (uint64_t)scene->n_curves * (sizeof(CurveRecord) + sizeof(BvhNode));
     for (const sgpu::PtexTex& t : scene->tex_host)
         if (t.texels) b += (uint64_t)t.nfaces * (3ull << (2 * t.res_log2)) + (uint64_t)t.nfaces * 8 * sizeof(int32_t);
     for (const auto& a : scene->attr_allocs) b += a.second;
@@ -1109,6 +1208,9 @@ int sgpu_intersect(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_
     if (!hits) return sgpu_fail(SGPU_E_INVALID, "sgpu_intersect: NULL hits");
     if ((hits->ng_x || hits->ng_y || hits->ng_z) && !(hits->ng_x && hits->ng_y && hits->ng_z))
         return sgpu_fail(SGPU_E_INVALID, "sgpu_intersect: ng_x/ng_y/ng_z must be all set or all NULL");
+    if (scene->n_curves > 0 && !hits->t)
+        return sgpu_fail(SGPU_E_INVALID, "sgpu_intersect: a scene with curves needs hits.t (the curve pass compares "
+                                         "against the triangle hit)");
     return sgpu_trace_launch(scene, rays, hits, nullptr, n, false, nullptr, as_stream(stream));
 }
 

@@ -65,6 +65,19 @@ struct InstRecord {
     const TriRecord* tris;
 };
 static_assert(sizeof(InstRecord) == 64, "InstRecord must be 64 bytes");
+// Round B-spline curve segment, 80 B, in curve-tree leaf order: the four control points (xyz,
+// radius; in world space, also for the curves of an instanced scene's placed scenes, which are
+// carried into world space at its commit), the geometry id (in the placed scene for an instanced
+// scene), the segment's index within its geometry and the composed instance (SGPU_INVALID_ID in a
+// scene of geometries).
+struct CurveRecord {
+    float4 p[4];
+    uint32_t geom, prim, inst, pad;
+};
+static_assert(sizeof(CurveRecord) == 80, "CurveRecord must be 80 bytes");
+// Stack entries of the curve tree's traversal (a binary tree: its depth must stay below this).
+constexpr uint32_t kCurveStack = 96;
+
 // The traversal stack entry that returns from an instance's tree to the top tree.
 constexpr int kInstExit = (int)0x80000000;
 
@@ -104,7 +117,16 @@ struct sgpu_scene_s {
     std::vector<float> tri_pos;
     std::vector<uint32_t> tri_geom;   // geomID per triangle
     std::vector<uint32_t> geom_first; // first global index of each geometry
-    std::vector<uint8_t> geom_quad;   // 1: quad geometry (triangles 2q and 2q+1 form quad q)
+    std::vector<uint8_t> geom_quad;   // 1: quad geometry (triangles 2q and 2q+1 form quad q), 2: curves
+    // round B-spline curves (sgpu_scene_add_curves): 16 floats per segment (control points, placed),
+    // its geometry and index within it; kept after commit for the scenes that place this one
+    std::vector<float> curve_cp;
+    std::vector<uint32_t> curve_geom, curve_prim;
+    // the curve tree (binary, over the segments, or over the placed scenes' curves carried into world
+    // space for an instanced scene) and its records in leaf order
+    BvhNode* d_cnodes = nullptr;
+    CurveRecord* d_curves = nullptr;
+    uint32_t n_curves = 0;
     bool committed = false;
     uint32_t n_tris = 0;
     uint32_t n_nodes = 0;
@@ -339,16 +361,33 @@ struct TraceScene {
     uint32_t n_tris;
     uint32_t n_inst;           // > 0: an instanced scene (wide = its top tree over `insts`)
     const InstRecord* insts;
+    const BvhNode* cnodes;     // curve tree (n_curves == 0: none)
+    const CurveRecord* curves;
+    uint32_t n_curves;
 };
 // One launch over rays of several scenes: ray i traverses table[ray_scene[i]] (device arrays);
 // stack_cap is the largest stack_cap of the table's scenes.
 // instanced: some entries are instanced scenes (then comp_out, if not NULL, receives each closest
 // hit's composed instance, SGPU_INVALID_ID for hits in scenes of geometries).
+// curves: some entries have curves (the curve pass runs after the triangle pass).
 int sgpu_trace_launch_multi(sgpu_context ctx, const TraceScene* table, const uint32_t* ray_scene, uint32_t stack_cap,
                             uint32_t* overflow, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits,
                             uint32_t* occluded, uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream,
                             uint32_t* spill = nullptr, const uint32_t* n_ptr = nullptr, bool instanced = false,
-                            uint32_t* comp_out = nullptr);
+                            uint32_t* comp_out = nullptr, bool curves = false);
+// The curve pass (sgpu_curve.hip): after a triangle pass over the same rays, the curve tree's hits
+// closer than the ray's current hit (hits->t, which the triangle pass wrote: its hit or tfar)
+// replace it; for any-hit queries, rays not yet occluded are tested. One scene (nodes, curves,
+// n_curves; insts / inst_ids of an instanced scene, or NULL) or a table (then ray_scene).
+int sgpu_curve_launch(const BvhNode* nodes, const CurveRecord* curves, uint32_t n_curves, const InstRecord* insts,
+                      const uint32_t* inst_ids, const TraceScene* table, const uint32_t* ray_scene,
+                      const sgpu_rays_soa* rays, const sgpu_hits_soa* hits, uint32_t* occluded, uint32_t n,
+                      bool any_hit, const uint32_t* n_ptr, uint32_t* comp_out, uint32_t* overflow,
+                      hipStream_t stream);
+// Builds the curve tree of a scene of geometries from its segments, or of an instanced scene from
+// the curves of its composed instances (sgpu_bvh.hip; called by the builds above).
+int sgpu_curve_build(sgpu_scene scene, const std::vector<float>& cp, const std::vector<uint32_t>& geom,
+                     const std::vector<uint32_t>& prim, const std::vector<uint32_t>& inst, hipStream_t stream);
 // Wavefront path tracer (sgpu_pt.hip); arguments validated by sgpu_render_pt.
 int sgpu_pt_render_impl(sgpu_scene scene, const sgpu_pt_job* job, float* framebuffer, hipStream_t stream,
                         sgpu_pt_stats* stats = nullptr);

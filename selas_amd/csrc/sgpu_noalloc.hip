@@ -48,6 +48,8 @@ extern "C" int sgpu_intersect_noalloc(sgpu_scene scene, const sgpu_rays_soa* ray
     if (!hits) return sgpu_fail(SGPU_E_INVALID, "sgpu_intersect_noalloc: NULL hits");
     if ((hits->ng_x || hits->ng_y || hits->ng_z) && !(hits->ng_x && hits->ng_y && hits->ng_z))
         return sgpu_fail(SGPU_E_INVALID, "sgpu_intersect_noalloc: ng_x/ng_y/ng_z must be all set or all NULL");
+    if (scene->n_curves > 0 && !hits->t)
+        return sgpu_fail(SGPU_E_INVALID, "sgpu_intersect_noalloc: a scene with curves needs hits.t");
     uint32_t* ptr = scene->ctx->noalloc_scratch;
     if (!ptr) return sgpu_fail(SGPU_E_STATE, "sgpu_intersect_noalloc: scratch not set up for this device");
     // blocks whose spilled stacks fit the scratch

@@ -929,19 +929,29 @@ int sgpu_trace_launch(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hi
                       uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream, uint32_t* spill_buf,
                       const uint32_t* n_ptr, uint32_t* overflow, uint32_t* comp_out, uint32_t max_blocks)
 {
+    int rc;
     if (!scene->instances.empty())
-        return trace_launch_instanced(scene, rays, hits, occluded, n, any_hit, d_stats, stream,
-                                      overflow ? overflow : scene->d_overflow, spill_buf, n_ptr, comp_out, max_blocks);
-    return trace_launch(scene->ctx, scene->d_nodes, scene->d_wide, scene->n_wide, scene->d_tris, scene->n_tris,
-                        scene->stack_cap, overflow ? overflow : scene->d_overflow, nullptr, nullptr, rays, hits,
-                        occluded, n, any_hit, d_stats, stream, spill_buf, n_ptr, false, nullptr, max_blocks);
+        rc = trace_launch_instanced(scene, rays, hits, occluded, n, any_hit, d_stats, stream,
+                                    overflow ? overflow : scene->d_overflow, spill_buf, n_ptr, comp_out, max_blocks);
+    else
+        rc = trace_launch(scene->ctx, scene->d_nodes, scene->d_wide, scene->n_wide, scene->d_tris, scene->n_tris,
+                          scene->stack_cap, overflow ? overflow : scene->d_overflow, nullptr, nullptr, rays, hits,
+                          occluded, n, any_hit, d_stats, stream, spill_buf, n_ptr, false, nullptr, max_blocks);
+    if (rc != SGPU_OK || scene->n_curves == 0) return rc;
+    return sgpu_curve_launch(scene->d_cnodes, scene->d_curves, scene->n_curves, scene->d_inst, scene->d_inst_ids,
+                             nullptr, nullptr, rays, hits, occluded, n, any_hit, n_ptr, comp_out,
+                             overflow ? overflow : scene->d_overflow, stream);
 }
 
 int sgpu_trace_launch_multi(sgpu_context ctx, const TraceScene* table, const uint32_t* ray_scene, uint32_t stack_cap,
                             uint32_t* overflow, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits,
                             uint32_t* occluded, uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream,
-                            uint32_t* spill_buf, const uint32_t* n_ptr, bool instanced, uint32_t* comp_out)
+                            uint32_t* spill_buf, const uint32_t* n_ptr, bool instanced, uint32_t* comp_out,
+                            bool curves)
 {
-    return trace_launch(ctx, nullptr, nullptr, 0, nullptr, 0, stack_cap, overflow, table, ray_scene, rays, hits, occluded,
-                        n, any_hit, d_stats, stream, spill_buf, n_ptr, instanced, comp_out);
+    int rc = trace_launch(ctx, nullptr, nullptr, 0, nullptr, 0, stack_cap, overflow, table, ray_scene, rays, hits,
+                          occluded, n, any_hit, d_stats, stream, spill_buf, n_ptr, instanced, comp_out);
+    if (rc != SGPU_OK || !curves) return rc;
+    return sgpu_curve_launch(nullptr, nullptr, 0, nullptr, nullptr, table, ray_scene, rays, hits, occluded, n, any_hit,
+                             n_ptr, comp_out, overflow, stream);
 }

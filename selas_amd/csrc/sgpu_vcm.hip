@@ -1104,6 +1104,7 @@ struct sgpu_vcm_server_s {
     Slice* h_slices = nullptr;
     TraceScene* h_table = nullptr;
     bool any_instanced = false;  // an instanced scene is in the table (traversal by k_trace8i)
+    bool any_curves = false;     // a scene of the table has curves (the curve pass follows traversal)
     // the scenes of the admitted jobs (traversal table, rebuilt at every step) and their unique
     // ids, jobs in admission order
     std::vector<sgpu_scene> scenes;
@@ -1152,7 +1153,7 @@ int server_graphs(sgpu_vcm_server sv, F&& pass_fn)
     key.tree = ctx->trace_tree;
     key.blocks = ctx->trace_blocks_per_cu;
     key.lds = ctx->trace_lds_nodes;
-    key.inst = sv->any_instanced ? 1u : 0u;
+    key.inst = (sv->any_instanced ? 1u : 0u) | (sv->any_curves ? 2u : 0u);
     key.order = ctx->vcm_order | (ctx->vcm_merge_lanes << 8);
     if (sv->g_pass[0] && sv->g_pass[1] && key == sv->gkey) return SGPU_OK;
     for (int k = 0; k < 2; ++k) {
@@ -1288,7 +1289,7 @@ void server_sync_scenes(sgpu_vcm_server sv)
 {
     std::vector<sgpu_scene> list;
     std::vector<uint64_t> uids;
-    bool inst = false;
+    bool inst = false, curves = false;
     uint32_t stack = 0;
     for (ServerJob& j : sv->active) {
         uint32_t k = 0;
@@ -1297,6 +1298,7 @@ void server_sync_scenes(sgpu_vcm_server sv)
             list.push_back(j.scene);
             uids.push_back(j.scene->uid);
             inst |= !j.scene->instances.empty();
+            curves |= j.scene->n_curves > 0;
             stack = std::max(stack, j.scene->stack_cap);
         }
         j.base.scene = k;
@@ -1309,6 +1311,7 @@ void server_sync_scenes(sgpu_vcm_server sv)
     // kept from earlier steps (reset with the server): a larger stack bound and the instanced
     // traversal serve every scene, and the captured graphs, which bake both in, stay valid
     sv->any_instanced |= inst;
+    sv->any_curves |= curves;
     sv->max_stack = std::max(sv->max_stack, stack);
 }
 
@@ -1334,8 +1337,9 @@ int server_upload_table(sgpu_vcm_server sv, hipStream_t stream)
         SGPU_HIP_TRY(hipHostMalloc((void**)&sv->h_table, sizeof(TraceScene) * sv->table_cap, hipHostMallocDefault));
     }
     for (uint32_t k = 0; k < n; ++k)
-        sv->h_table[k] = TraceScene{sv->scenes[k]->d_nodes, sv->scenes[k]->d_wide, sv->scenes[k]->d_tris,
-                                    sv->scenes[k]->n_tris, sv->scenes[k]->n_inst, sv->scenes[k]->d_inst};
+        sv->h_table[k] = TraceScene{sv->scenes[k]->d_nodes,  sv->scenes[k]->d_wide,   sv->scenes[k]->d_tris,
+                                    sv->scenes[k]->n_tris,   sv->scenes[k]->n_inst,   sv->scenes[k]->d_inst,
+                                    sv->scenes[k]->d_cnodes, sv->scenes[k]->d_curves, sv->scenes[k]->n_curves};
     SGPU_HIP_TRY(hipMemcpyAsync(sv->d_table, sv->h_table, sizeof(TraceScene) * n, hipMemcpyHostToDevice, stream));
     SGPU_HIP_TRY(hipStreamSynchronize(stream));
     sv->table_dirty = false;
@@ -1355,6 +1359,7 @@ static void server_reset(sgpu_vcm_server sv, const sgpu_vcm_server_config& cfg)
     sv->table_dirty = true;
     sv->max_stack = 0;
     sv->any_instanced = false;
+    sv->any_curves = false;
     sv->cfg.flags = cfg.flags;
     sv->cap = (uint32_t)std::min<uint64_t>(cfg.wavefront_paths, sv->TM);
     sv->n_closest = sv->n_shadow = sv->n_light_verts = sv->n_launches = sv->n_samples = sv->n_waves = 0;
@@ -1631,7 +1636,8 @@ int sgpu_vcm_server_step_impl(sgpu_vcm_server sv, hipStream_t stream, uint64_t* 
             uint32_t* comp = hits ? hb.inst : nullptr;
             if (!single)
                 return sgpu_trace_launch_multi(sv->ctx, sv->d_table, r.scene, sv->max_stack, sv->ovf, &rs, hits, occ, n,
-                                               any, d_st, st, sv->spill, n_ptr, sv->any_instanced, comp);
+                                               any, d_st, st, sv->spill, n_ptr, sv->any_instanced, comp,
+                                               sv->any_curves);
             return sgpu_trace_launch(first_scene, &rs, hits, occ, n, any, d_st, st, sv->spill, n_ptr, sv->ovf, comp);
         };
         // the entries 0 .. *c - 1 of this bounce into bins.order by key (material keys computed from

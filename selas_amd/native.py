@@ -73,6 +73,7 @@ _SIGNATURES = {
     "sgpu_scene_create": [_P, C.POINTER(_P)],
     "sgpu_scene_add_triangles": [_P, _FP, _U32, _U32P, _U32, _FP, _U32P],
     "sgpu_scene_add_quads": [_P, _FP, _U32, _U32P, _U32, _FP, _U32P],
+    "sgpu_scene_add_curves": [_P, _FP, _U32, _U32P, _U32, _FP, _U32P],
     "sgpu_scene_set_ptex": [_P, _U32, _U32, _U32, _P, _P, _P],
     "sgpu_scene_set_vertex_attributes": [_P, _U32, _P, _P, _P, _U32],
     "sgpu_ptex_eval": [_P, _U32, _P, _P, _P, _U32, _P, _P],
@@ -327,6 +328,16 @@ class Scene:
                                        C.byref(gid)))
         return gid.value
 
+    def add_curves(self, vertices: np.ndarray, indices: np.ndarray, transform_colmajor=None) -> int:
+        """Round B-spline curves: control points (n x 4: x, y, z, radius) and per segment the index of
+        its first control point (sgpu_scene_add_curves)."""
+        vtx = np.ascontiguousarray(vertices, np.float32).reshape(-1)
+        idx = np.ascontiguousarray(indices, np.uint32).reshape(-1)
+        gid = C.c_uint32()
+        xf = None if transform_colmajor is None else fptr(np.ascontiguousarray(transform_colmajor, np.float32))
+        check(lib.sgpu_scene_add_curves(self.handle, fptr(vtx), vtx.size // 4, uptr(idx), idx.size, xf, C.byref(gid)))
+        return gid.value
+
     def set_vertex_attributes(self, geom: int, n_verts: int, normals=None, tangents=None, uvs=None):
         """Per-vertex normals (n x 3), tangents (n x 4) and UVs (n x 2) of geometry `geom` through
         sgpu_scene_set_vertex_attributes (before commit)."""
@@ -505,6 +516,9 @@ def build_render_scene(ctx: Context, sc: dict) -> Scene:
     scene = Scene(ctx)
     for g in sc["geoms"]:
         if isinstance(g, dict):
+            if "curves" in g:
+                scene.add_curves(g["curve_vertices"], g["curves"])
+                continue
             if "triangles" in g:
                 gid = scene.add_triangles(g["positions"], g["triangles"])
             else:

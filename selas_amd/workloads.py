@@ -322,6 +322,8 @@ def geom_tri9(g):
     splits quads, (v0, v1, v3) then (v2, v3, v1) for each quad in order."""
     if isinstance(g, np.ndarray):
         return g
+    if "curves" in g:  # a curve geometry holds no triangles
+        return np.zeros((0, 9), np.float32)
     p = np.asarray(g["positions"], np.float32)
     if "triangles" in g:
         t = np.asarray(g["triangles"], np.int64)

@@ -1,0 +1,394 @@
+"""Round B-spline curves on the device (sgpu_scene_add_curves: Selas's
+RTC_GEOMETRY_TYPE_ROUND_BSPLINE_CURVE geometry, ModelResource.cpp:300-330) against the oracle,
+through the C ABI.
+
+Bar (written here): the device tests curves in float, the oracle (oracle/orc_curve.c, pinned by
+closed forms in test_oracle_curves.py) in double, so hits are compared with tolerances: the same
+geometry and primitive, t within 1e-5 relative, u within 1e-3, the normal's direction within 5e-3
+rad, and the same occlusion flags -- except for rays that graze a curve (clearance within 1e-3 of
+its radius) or meet a triangle and a curve at nearly the same distance, which may fall either way
+and are at most 1 % of the rays. Renders: per-pixel as the other renderer tests (99 % of values
+within 1e-3, VCM relative RMSE <= 2e-3)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from selas_amd import native as N
+from selas_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _hair(seed, n_strands, n_cp, centre, spread, radius, step=0.08):
+    """n_strands strands of n_cp control points each (a random walk upwards), n_cp - 3 segments each."""
+    rng = np.random.default_rng(seed)
+    verts, idx = [], []
+    for s in range(n_strands):
+        root = np.asarray(centre, np.float64) + rng.uniform(-spread, spread, 3) * np.array([1.0, 0.2, 1.0])
+        d = np.array([0.0, 1.0, 0.0]) + 0.4 * rng.normal(size=3)
+        pts = [root]
+        for _ in range(n_cp - 1):
+            d = d + 0.5 * rng.normal(size=3) * np.array([1.0, 0.3, 1.0])
+            d /= np.linalg.norm(d)
+            pts.append(pts[-1] + step * d)
+        rad = radius * np.linspace(1.0, 0.4, n_cp)
+        base = len(verts)
+        verts.extend([[*p, r] for p, r in zip(pts, rad)])
+        idx.extend(range(base, base + n_cp - 3))
+    return np.array(verts, np.float32), np.array(idx, np.uint32)
+
+
+def _rays(seed, n, centre, spread, dist=3.0):
+    rng = np.random.default_rng(seed)
+    tgt = np.asarray(centre) + rng.uniform(-spread, spread, (n, 3))
+    o = rng.normal(size=(n, 3))
+    o = tgt + dist * o / np.linalg.norm(o, axis=1, keepdims=True)
+    d = (tgt - o) / dist
+    o, d = o.astype(np.float32), d.astype(np.float32)
+    tfar = np.full(n, 1e30, np.float32)
+    tfar[::4] = rng.uniform(0.5, 1.5, tfar[::4].shape[0]) * dist
+    return {"org_x": o[:, 0], "org_y": o[:, 1], "org_z": o[:, 2], "dir_x": d[:, 0], "dir_y": d[:, 1],
+            "dir_z": d[:, 2], "tnear": np.zeros(n, np.float32), "tfar": tfar}
+
+
+def _gpu(ctx, scene, rays):
+    dr = N.DeviceRays(ctx, rays)
+    dh = N.DeviceHits(ctx, dr.n, with_inst=True)
+    scene.intersect(dr, dh)
+    occ = ctx.empty(4 * dr.n)
+    scene.occluded(dr, occ)
+    N.synchronize()
+    scene.check()
+    out = dh.to_host()
+    out["occluded"] = occ.download(np.uint32, dr.n)
+    return out
+
+
+def _oracle_model(tri9, curve_geoms, first_curve_geom, rays):
+    """Closest hits of a scene of one triangle soup (geometry 0) and curve geometries
+    first_curve_geom, ...: geom, prim, t, u, ng (n, 3) and occlusion; plus, per ray, whether it is
+    excused (grazes a curve or ties a triangle)."""
+    n = rays["org_x"].shape[0]
+    tri = O.CpuBvh(tri9).closest(rays)
+    occ_tri = O.CpuBvh(tri9).occluded(rays)
+    cps = np.concatenate([O.curve_segments(g) for g in curve_geoms])
+    cgeom = np.concatenate([np.full(O.curve_segments(g).shape[0], first_curve_geom + i, np.uint32)
+                            for i, g in enumerate(curve_geoms)])
+    cprim = np.concatenate([np.arange(O.curve_segments(g).shape[0], dtype=np.uint32) for g in curve_geoms])
+    thit = tri["prim"] != N.INVALID_ID
+    rc = dict(rays, tfar=np.where(thit, tri["t"], rays["tfar"]).astype(np.float32))
+    cur = O.curves_closest(cps, rc)
+    occ_cur = O.curves_occluded(cps, rays)
+    chit = (cur["prim"] != 0xFFFFFFFF) & (~thit | (cur["t"] < tri["t"]))
+    ref = {"geom_id": np.where(chit, cgeom[np.minimum(cur["prim"], len(cgeom) - 1)], np.where(thit, 0, N.INVALID_ID)),
+           "prim_id": np.where(chit, cprim[np.minimum(cur["prim"], len(cprim) - 1)], tri["prim"]),
+           "t": np.where(chit, cur["t"], tri["t"]), "u": np.where(chit, cur["u"], tri["u"]),
+           "curve": chit, "occluded": (occ_tri | occ_cur).astype(np.uint32)}
+    ref["geom_id"] = ref["geom_id"].astype(np.uint32)
+    ref["prim_id"] = np.where(chit | thit, ref["prim_id"], N.INVALID_ID).astype(np.uint32)
+    ref["ng"] = cur["ng"]
+    # excused rays: grazing some segment the ray's line passes within a few radii of, or a near tie
+    excused = np.zeros(n, bool)
+    o = np.stack([rays["org_x"], rays["org_y"], rays["org_z"]], 1)
+    d = np.stack([rays["dir_x"], rays["dir_y"], rays["dir_z"]], 1)
+    boxes = np.zeros((cps.shape[0], 6), np.float32)
+    O.lib.orc_curve_boxes(cps.ctypes.data, cps.shape[0], boxes.ctypes.data)
+    for i in range(n):
+        inv = 1.0 / np.where(np.abs(d[i]) > 1e-30, d[i], 1e-30)
+        t0 = (boxes[:, :3] - o[i]) * inv
+        t1 = (boxes[:, 3:] - o[i]) * inv
+        near = np.minimum(t0, t1).max(1) <= np.maximum(t0, t1).min(1)
+        for s in np.nonzero(near)[0]:
+            if abs(O.curve_clearance(cps[s], o[i], d[i])) < 1e-3:
+                excused[i] = True
+                break
+    both = thit & (cur["prim"] != 0xFFFFFFFF)
+    excused |= both & (np.abs(cur["t"] - tri["t"]) <= 1e-5 * np.abs(tri["t"]))
+    return ref, excused
+
+
+def _compare(gpu, ref, excused, normals=True):
+    n = gpu["t"].shape[0]
+    ok = ~excused
+    assert excused.mean() <= 0.01, excused.mean()
+    for k in ("geom_id", "prim_id"):
+        bad = np.nonzero(ok & (gpu[k] != ref[k]))[0]
+        assert bad.size == 0, f"{k}: {bad.size} of {n} differ, first {bad[:5]}"
+    hit = ok & (ref["prim_id"] != N.INVALID_ID)
+    np.testing.assert_allclose(gpu["t"][hit], ref["t"][hit], rtol=1e-5)
+    cur = hit & ref["curve"]
+    assert cur.sum() > 100
+    np.testing.assert_allclose(gpu["u"][cur], ref["u"][cur], atol=1e-3)
+    assert (gpu["v"][cur] == 0).all()
+    if normals:
+        g = np.stack([gpu["ng_x"], gpu["ng_y"], gpu["ng_z"]], 1)[cur]
+        r = ref["ng"][cur]
+        cosang = (g * r).sum(1) / np.linalg.norm(g, axis=1) / np.linalg.norm(r, axis=1)
+        assert (cosang > 1 - 1e-5).all(), cosang.min()
+    bad = np.nonzero(ok & (gpu["occluded"] != ref["occluded"]))[0]
+    assert bad.size == 0, f"occluded: {bad.size} differ"
+
+
+def test_curve_queries_match_oracle(gpu_ctx):
+    rng = np.random.default_rng(5)
+    tri9 = (rng.uniform(-0.6, 0.6, (300, 1, 3)) + rng.normal(scale=0.08, size=(300, 3, 3))).reshape(-1, 9)
+    tri9 = tri9.astype(np.float32)
+    h1 = dict(zip(("curve_vertices", "curves"), _hair(1, 60, 9, (0, -0.3, 0), 0.5, 0.01)))
+    h2 = dict(zip(("curve_vertices", "curves"), _hair(2, 30, 6, (0.2, 0.0, 0.1), 0.4, 0.04)))
+    sc = N.Scene(gpu_ctx)
+    assert sc.add_soup(tri9) == 0
+    assert sc.add_curves(h1["curve_vertices"], h1["curves"]) == 1
+    assert sc.add_curves(h2["curve_vertices"], h2["curves"]) == 2
+    sc.commit()
+    try:
+        rays = _rays(7, 10000, (0, 0.0, 0), 0.5)
+        print("curve queries: 10000 rays", flush=True)
+        gpu = _gpu(gpu_ctx, sc, rays)
+        ref, excused = _oracle_model(tri9, [h1, h2], 1, rays)
+        assert ref["curve"].sum() > 1000
+        _compare(gpu, ref, excused)
+    finally:
+        sc.release()
+
+
+def test_curves_only_scene_and_empty_interval(gpu_ctx):
+    h = _hair(3, 40, 8, (0, 0, 0), 0.3, 0.02)
+    g = {"curve_vertices": h[0], "curves": h[1]}
+    sc = N.Scene(gpu_ctx)
+    assert sc.add_curves(*h) == 0
+    sc.commit()
+    try:
+        rays = _rays(8, 5000, (0, 0.2, 0), 0.35)
+        gpu = _gpu(gpu_ctx, sc, rays)
+        cps = O.curve_segments(g)
+        ref = O.curves_closest(cps, rays)
+        hit = ref["prim"] != 0xFFFFFFFF
+        assert hit.sum() > 300
+        agree = gpu["prim_id"] == np.where(hit, ref["prim"], N.INVALID_ID)
+        assert agree.mean() >= 0.99, agree.mean()
+        assert (gpu["geom_id"][gpu["prim_id"] != N.INVALID_ID] == 0).all()
+        both = hit & (gpu["prim_id"] == ref["prim"])
+        np.testing.assert_allclose(gpu["t"][both], ref["t"][both], rtol=1e-5)
+        # an interval ending just before each hit sees nothing
+        r2 = dict(rays, tfar=np.where(both, gpu["t"] * 0.9999, rays["tfar"]).astype(np.float32))
+        g2 = _gpu(gpu_ctx, sc, r2)
+        assert (g2["occluded"][both] == 0).all() and (g2["prim_id"][both] == N.INVALID_ID).all()
+    finally:
+        sc.release()
+
+
+def _rot_m4(ry, rx, scale, t):
+    c, s_ = np.cos(ry), np.sin(ry)
+    cx, sx = np.cos(rx), np.sin(rx)
+    r = np.array([[c, 0, s_], [0, 1, 0], [-s_, 0, c]]) @ np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+    m = np.eye(4)
+    m[:3, :3] = r * scale
+    m[:3, 3] = t
+    return m
+
+
+def test_instanced_curves_match_oracle(gpu_ctx):
+    """A model of triangles and curves and a model of curves only, placed directly and inside a
+    subscene (two instance levels) by similarity transforms: hits against the oracle's query of each
+    composed instance in its own space (t is the world ray's parameter)."""
+    rng = np.random.default_rng(9)
+    tri9 = (rng.uniform(-0.4, 0.4, (150, 1, 3)) + rng.normal(scale=0.06, size=(150, 3, 3))).reshape(-1, 9)
+    tri9 = tri9.astype(np.float32)
+    ha = _hair(4, 30, 7, (0, 0, 0), 0.35, 0.02)
+    hb = _hair(5, 30, 7, (0, 0, 0), 0.35, 0.03)
+    ma, mb = N.Scene(gpu_ctx), N.Scene(gpu_ctx)
+    ma.add_soup(tri9)
+    ma.add_curves(*ha)
+    ma.commit()
+    mb.add_curves(*hb)
+    mb.commit()
+    sub = N.Scene(gpu_ctx)
+    xs = [_rot_m4(0.3, 0.1, 1.2, (0.8, 0.0, 0.0)), _rot_m4(-0.7, 0.0, 0.8, (0.9, 0.6, 0.2))]
+    sub.add_instance(ma, O.colmajor(xs[0]))
+    sub.add_instance(mb, O.colmajor(xs[1]))
+    sub.commit()
+    top = N.Scene(gpu_ctx)
+    xt = [_rot_m4(1.1, -0.2, 1.5, (-0.9, 0.1, 0.0)), _rot_m4(0.5, 0.4, 0.9, (0.0, -0.6, 0.3)), np.eye(4)]
+    top.add_instance(ma, O.colmajor(xt[0]))      # inst 0: model a
+    top.add_instance(mb, O.colmajor(xt[1]))      # inst 1: model b (curves only)
+    top.add_instance(sub, O.colmajor(xt[2]))     # inst 2: the subscene (a, b)
+    top.commit()
+    try:
+        rays = _rays(10, 12000, (0, 0.1, 0.0), 1.0, dist=4.0)
+        print("instanced curve queries: 12000 rays", flush=True)
+        gpu = _gpu(gpu_ctx, top, rays)
+        # composed instances: (model, l2w 4 x 4, id0, id1)
+        comp = [("a", xt[0], 0, N.INVALID_ID), ("b", xt[1], 1, N.INVALID_ID), ("a", xt[2] @ xs[0], 2, 0),
+                ("b", xt[2] @ xs[1], 2, 1)]
+        ga = {"curve_vertices": ha[0], "curves": ha[1]}
+        gb = {"curve_vertices": hb[0], "curves": hb[1]}
+        n = rays["org_x"].shape[0]
+        best = {"t": np.full(n, np.inf), "geom_id": np.full(n, N.INVALID_ID, np.uint32),
+                "prim_id": np.full(n, N.INVALID_ID, np.uint32), "inst_id0": np.full(n, N.INVALID_ID, np.uint32),
+                "inst_id1": np.full(n, N.INVALID_ID, np.uint32), "curve": np.zeros(n, bool)}
+        occ = np.zeros(n, bool)
+        o = np.stack([rays["org_x"], rays["org_y"], rays["org_z"]], 1).astype(np.float64)
+        d = np.stack([rays["dir_x"], rays["dir_y"], rays["dir_z"]], 1).astype(np.float64)
+        for model, m, id0, id1 in comp:
+            w2l = np.linalg.inv(m)
+            lo = (o @ w2l[:3, :3].T + w2l[:3, 3]).astype(np.float32)
+            ld = (d @ w2l[:3, :3].T).astype(np.float32)
+            lr = dict(rays, org_x=lo[:, 0], org_y=lo[:, 1], org_z=lo[:, 2], dir_x=ld[:, 0], dir_y=ld[:, 1],
+                      dir_z=ld[:, 2])
+            cps = O.curve_segments(ga if model == "a" else gb)
+            c = O.curves_closest(cps, lr)
+            occ |= O.curves_occluded(cps, lr) != 0
+            ch = (c["prim"] != 0xFFFFFFFF) & (c["t"] < best["t"])
+            upd = {"t": c["t"], "geom_id": np.full(n, 1 if model == "a" else 0, np.uint32), "prim_id": c["prim"],
+                   "inst_id0": np.full(n, id0, np.uint32), "inst_id1": np.full(n, id1, np.uint32)}
+            for k, v in upd.items():
+                best[k] = np.where(ch, v, best[k])
+            best["curve"] = np.where(ch, True, best["curve"])
+            if model == "a":
+                tr = O.CpuBvh(tri9).closest(lr)
+                occ |= O.CpuBvh(tri9).occluded(lr) != 0
+                th = (tr["prim"] != N.INVALID_ID) & (tr["t"] < best["t"])
+                upd = {"t": tr["t"], "geom_id": np.zeros(n, np.uint32), "prim_id": tr["prim"],
+                       "inst_id0": np.full(n, id0, np.uint32), "inst_id1": np.full(n, id1, np.uint32)}
+                for k, v in upd.items():
+                    best[k] = np.where(th, v, best[k])
+                best["curve"] = np.where(th, False, best["curve"])
+        hit = best["geom_id"] != N.INVALID_ID
+        assert best["curve"].sum() > 500 and (hit & ~best["curve"]).sum() > 500
+        same = np.ones(n, bool)
+        for k in ("geom_id", "prim_id", "inst_id0", "inst_id1"):
+            same &= gpu[k] == best[k]
+        # rays that differ must graze (or tie): at most 1 %, and t agrees where they agree
+        assert same.mean() >= 0.99, same.mean()
+        both = same & hit
+        np.testing.assert_allclose(gpu["t"][both], best["t"][both], rtol=2e-5)
+        assert (gpu["occluded"] == occ).mean() >= 0.99
+    finally:
+        top.release()
+        sub.release()
+        ma.release()
+        mb.release()
+
+
+def test_curve_errors(gpu_ctx):
+    v, idx = _hair(6, 3, 6, (0, 0, 0), 0.2, 0.02)
+    sc = N.Scene(gpu_ctx)
+    try:
+        call = lambda vv, ii, xf=None: N.lib.sgpu_scene_add_curves(  # noqa: E731
+            sc.handle, N.fptr(vv), vv.shape[0], N.uptr(ii), ii.shape[0], None if xf is None else N.fptr(O.colmajor(xf)),
+            None)
+        bad = idx.copy()
+        bad[-1] = v.shape[0] - 3
+        assert call(v, bad) == N.SGPU_E_INVALID  # control points past the vertices
+        vn = v.copy()
+        vn[2, 3] = -0.1
+        assert call(vn, idx) == N.SGPU_E_INVALID  # negative radius
+        shear = np.eye(4)
+        shear[0, 1] = 0.3
+        assert call(v, idx, shear) == N.SGPU_E_INVALID  # not a similarity
+        assert call(v, idx, _rot_m4(0.4, 0.2, 2.0, (1, 2, 3))) == N.SGPU_OK
+        tri = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+        sc.add_triangles(tri, np.array([[0, 1, 2]], np.uint32))
+        assert N.lib.sgpu_scene_set_vertex_attributes(sc.handle, 0, tri.ctypes.data, None, None, 0) == N.SGPU_E_INVALID
+        tex = np.zeros((1, 1, 1, 3), np.uint8)
+        af = np.full((1, 4), -1, np.int32)
+        ae = np.zeros((1, 4), np.int32)
+        assert N.lib.sgpu_scene_set_ptex(sc.handle, 0, 0, 1, tex.ctypes.data, af.ctypes.data,
+                                         ae.ctypes.data) == N.SGPU_E_INVALID
+        sc.commit()
+        rays = _rays(11, 64, (0, 0, 0), 0.3)
+        dr = N.DeviceRays(gpu_ctx, rays)
+        hs = N.HitsSoA()  # no t: a scene with curves cannot compare against the triangle hit
+        dh = N.DeviceHits(gpu_ctx, dr.n)
+        hs.prim_id = dh.soa.prim_id
+        assert N.lib.sgpu_intersect(sc.handle, C_ref(dr.soa), C_ref(hs), dr.n, None) == N.SGPU_E_INVALID
+    finally:
+        sc.release()
+    # an instance placing curves must be a similarity
+    m = N.Scene(gpu_ctx)
+    m.add_curves(v, idx)
+    m.commit()
+    top = N.Scene(gpu_ctx)
+    st = np.diag([1.0, 2.0, 1.0, 1.0])
+    top.add_instance(m, O.colmajor(st))
+    with pytest.raises(N.SgpuError):
+        top.commit()
+    top.release()
+    m.release()
+
+
+def C_ref(x):
+    import ctypes
+
+    return ctypes.byref(x)
+
+
+def _hair_room():
+    sys.path.insert(0, HERE)
+    import scenes
+
+    sc = scenes.cube_scene(lights=True, ibl=False)
+    v, idx = _hair(12, 120, 8, (0.0, 0.9, 0.8), 0.35, 0.012, step=0.06)
+    sc = dict(sc, geoms=list(sc["geoms"]) + [{"curve_vertices": v, "curves": idx}])
+    mat = W.material(0, (0.45, 0.3, 0.15), 0.0, 0.35, 1.55)
+    sc["materials"] = np.concatenate([sc["materials"], mat[None]])
+    sc["geom_material"] = np.concatenate([sc["geom_material"], [sc["materials"].shape[0] - 1]]).astype(np.uint32)
+    return sc
+
+
+def _close(gpu, cpu):
+    close = np.abs(gpu - cpu) <= 1e-3 * np.abs(cpu) + 1e-5
+    return close.mean(), np.sqrt(np.mean((gpu - cpu) ** 2)) / max(cpu.mean(), 1e-12)
+
+
+def test_curve_renders_match_oracle(gpu_ctx):
+    """The room with a tuft of hair on the box (Disney thin): path tracer and VCM per pixel."""
+    sc = _hair_room()
+    scene = N.build_render_scene(gpu_ctx, sc)
+    try:
+        rs = O.RenderScene(sc)
+        print("curve render: path tracer", flush=True)
+        gpu = N.render_pt(scene, sc["camera"], 64, 48, 4, 6, 11, N.PT_NEE_COSINE_ONCE)
+        cpu = O.render_pt(rs, sc["camera"], 64, 48, 4, 6, 11, 1)
+        frac, _ = _close(gpu, cpu)
+        assert cpu.mean() > 0 and frac >= 0.99, frac
+        print("curve render: VCM", flush=True)
+        gpu = N.render_vcm(scene, sc["camera"], 64, 48, 2, 8, 0.02, 0.75, 13)
+        cpu, _ = O.render_vcm(rs, sc["camera"], 64, 48, 2, 8, 0.02, 0.75, 13)
+        frac, rel = _close(gpu, cpu)
+        assert cpu.mean() > 0 and frac >= 0.99 and rel <= 2e-3, (frac, rel)
+    finally:
+        scene.release()
+
+
+def test_instanced_curve_render_matches_oracle(gpu_ctx):
+    """The hair as a placed scene of curves only, instanced twice beside an instanced crate: the path
+    tracer per pixel against the oracle's instanced scene."""
+    sys.path.insert(0, HERE)
+    import scenes
+
+    base = scenes.cube_scene(lights=True, ibl=False)
+    v, idx = _hair(13, 80, 8, (0.0, 0.0, 0.0), 0.3, 0.015, step=0.06)
+    hair = {"geoms": [{"curve_vertices": v, "curves": idx}],
+            "materials": W.material(0, (0.5, 0.35, 0.2), 0.0, 0.4, 1.55)[None],
+            "geom_material": np.array([0], np.uint32)}
+    room = {"geoms": base["geoms"], "materials": base["materials"], "geom_material": base["geom_material"]}
+    sc = {"models": [room, hair],
+          "instances": [("model", 0, np.eye(4)), ("model", 1, _rot_m4(0.2, 0.0, 1.0, (0.0, 0.9, 0.8))),
+                        ("model", 1, _rot_m4(-0.8, 0.1, 0.7, (-0.6, 0.0, 0.0)))],
+          "lights": base["lights"], "background": base["background"], "camera": base["camera"]}
+    scene = N.build_render_scene(gpu_ctx, sc)
+    try:
+        rs = O.InstancedRenderScene(sc)
+        print("instanced curve render: path tracer", flush=True)
+        gpu = N.render_pt(scene, sc["camera"], 64, 48, 4, 6, 17, N.PT_NEE_COSINE_ONCE)
+        cpu = O.render_pt(rs, sc["camera"], 64, 48, 4, 6, 17, 1)
+        frac, _ = _close(gpu, cpu)
+        assert cpu.mean() > 0 and frac >= 0.99, frac
+    finally:
+        scene.release()
