@@ -425,8 +425,16 @@ class RenderScene:
         self.bvh = CpuBvh(self.tri9)
         self.attrs = _AttrTable(sc["geoms"])
         ibl = sc.get("ibl")
+        self.curves = _curve_arrays(sc["geoms"])
+        box9 = self.tri9
+        if self.curves is not None:
+            # the library's emission sphere bounds the curves' boxes too: two degenerate triangles at
+            # the corners of their union
+            cb = self.curves[1]
+            lo, hi = cb[:, :3].min(0), cb[:, 3:].max(0)
+            box9 = np.concatenate([self.tri9, np.tile(lo, 3)[None], np.tile(hi, 3)[None]]).astype(np.float32)
         self.ibl = None if ibl is None else Ibl(ibl["rgb"], ibl.get("rotation", 0.0), ibl.get("exposure", 1.0),
-                                                tri9=self.tri9)
+                                                tri9=box9)
         self.s = _Scene(self.bvh.handle, self.tri9.ctypes.data, self.tri_geom.ctypes.data, self.materials.ctypes.data,
                         self.geom_material.ctypes.data, self.geom_material.shape[0],
                         self.lights.ctypes.data if self.lights.size else None, self.lights.shape[0],
@@ -434,7 +442,6 @@ class RenderScene:
                         C.cast(self.geom_tex, C.c_void_p),
                         C.cast(C.pointer(self.ibl.s), C.c_void_p) if self.ibl is not None else None, None,
                         self.attrs.ptr)
-        self.curves = _curve_arrays(sc["geoms"])
         if self.curves is not None:
             cp, box, cg, cpr = self.curves
             self.s.curve_cp, self.s.curve_box = cp.ctypes.data, box.ctypes.data
@@ -700,6 +707,11 @@ class InstancedRenderScene:
             pts = tri9.reshape(-1, 3)
             mm["box"] = (pts.min(0), pts.max(0)) if pts.shape[0] else None
             mm["curves"] = _curve_arrays(mdesc["geoms"])
+            if mm["box"] is not None and mm["curves"] is not None:
+                # the placed scene's box takes in its curves' boxes (the library's instance box for the
+                # scene's bounds is this box through the transform)
+                cb = mm["curves"][1]
+                mm["box"] = (np.minimum(mm["box"][0], cb[:, :3].min(0)), np.maximum(mm["box"][1], cb[:, 3:].max(0)))
             self.models.append(mm)
         # composed instances: those of scenes with triangles (the library's top tree), then those of
         # scenes of curves only, each in composition order
@@ -770,6 +782,28 @@ class InstancedRenderScene:
             for r in range(3):
                 lo[r] = min(lo[r], np.nextafter(np.float32(wlo[r]), np.float32(-np.inf)))
                 hi[r] = max(hi[r], np.nextafter(np.float32(whi[r]), np.float32(np.inf)))
+        for k, (mi, l2w, _, _) in enumerate(comp):
+            mm = self.models[mi]
+            if mm["curves"] is None:
+                continue
+            # the curves carried into world space as the library carries them (the composed transform in
+            # double, each coordinate rounded once; radii by the transform's scale), then their boxes
+            n2 = [l2w[c] * l2w[c] + l2w[4 + c] * l2w[4 + c] + l2w[8 + c] * l2w[8 + c] for c in range(3)]
+            s2 = 0.0
+            for c in range(3):
+                s2 += n2[c] / 3.0
+            sc_ = np.sqrt(s2)
+            src = mm["curves"][0].reshape(-1, 4).astype(np.float64)
+            w = np.empty((src.shape[0], 4), np.float32)
+            for r in range(3):
+                w[:, r] = ((l2w[4 * r] * src[:, 0] + l2w[4 * r + 1] * src[:, 1]) + l2w[4 * r + 2] * src[:, 2]) + l2w[4 * r + 3]
+            w[:, 3] = src[:, 3] * sc_
+            w = w.reshape(-1, 4, 4)
+            rmax = w[:, :, 3].max(1)
+            clo = (w[:, :, :3].min(1) - rmax[:, None]).astype(np.float32)
+            chi = (w[:, :, :3].max(1) + rmax[:, None]).astype(np.float32)
+            lo = np.minimum(lo, clo.min(0))
+            hi = np.maximum(hi, chi.max(0))
         self.iscene = _IScene(C.cast(self.inst_arr, C.c_void_p), n_tri_comp, C.cast(self.tri9_ptrs, C.c_void_p),
                               C.cast(self.tri_geom_ptrs, C.c_void_p), C.cast(self.geom_first_ptrs, C.c_void_p),
                               self.geom_base.ctypes.data, self.l2w.ctypes.data, n,

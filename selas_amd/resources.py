@@ -484,8 +484,8 @@ def _mesh_faces(indices: np.ndarray, mesh, n_verts: int, where: str) -> np.ndarr
 
 def build_model(model: dict, geometry: dict, name: str, scene_material_names, scene_materials, content_root=None,
                 ptex_loader=None, strict=True, dropped=None) -> dict:
-    """One model as a scene dict of geometries (meshes in order, so geometry ids are Selas's mesh
-    indices), materials and geom_material. Each mesh keeps the vertices its faces use (indices
+    """One model as a scene dict of geometries (meshes in order, then its round B-spline curves, so
+    geometry ids are Selas's: InitializeMeshes then InitializeCurves), materials and geom_material. Each mesh keeps the vertices its faces use (indices
     rebased), with the model's normals, tangents and UVs over them (SetMeshVertexAttributes,
     ModelResource.cpp:164-200). Materials follow InitializeGeometryUserDatas
     (ModelResource.cpp:335-371): FindMeshMaterial per mesh; a per-face texture
@@ -511,8 +511,6 @@ def build_model(model: dict, geometry: dict, name: str, scene_material_names, sc
             raise ValueError(f"model {name}: {what} (not rendered by the device path; strict=False drops it)")
         dropped.append(f"model {name}: {what}")
 
-    if model["curves"].shape[0]:
-        drop(f"{model['curves'].shape[0]} round B-spline curve geometries")
     geoms, mats, gm, keys = [], [], [], {}
     for mi, mesh in enumerate(model["meshes"]):
         mname = fixed_str(mesh["name"])
@@ -553,6 +551,34 @@ def build_model(model: dict, geometry: dict, name: str, scene_material_names, sc
             mats.append(rec)
         geoms.append(g)
         gm.append(keys[key])
+    # InitializeCurves (ModelResource.cpp:300-330): one round B-spline geometry per CurveMetaData,
+    # attached after the meshes (geometry ids continue), indexing the model's shared curve vertices;
+    # every curve takes the material FindMeshMaterial finds for curveModelName (:372-385) and no
+    # texture (its baseColorTextureHandle is empty: a per-face texture flag would have no texture)
+    if model["curves"].shape[0]:
+        ncv = int(model["total_curve_vertex_count"])
+        cv, cidx = geometry["curve_vertices"], geometry["curve_indices"]
+        if cv.shape[0] < ncv:
+            raise ResourceError(f"model {name}: {cv.shape[0]} curve vertices for totalCurveVertexCount {ncv}")
+        m, _ = find_mesh_material(model, scene_material_names, scene_materials, int(model["curve_model_name"]))
+        rec = material_record(m)
+        rec[21] = float(int(m["flags"]) & ~assets.USES_PTEX)
+        key = rec.tobytes()
+        if key not in keys:
+            keys[key] = len(mats)
+            mats.append(rec)
+        verts = np.ascontiguousarray(cv[:ncv], np.float32)
+        for ci, curve in enumerate(model["curves"]):
+            off, cnt = int(curve["index_offset"]), int(curve["index_count"])
+            if off + cnt > cidx.shape[0]:
+                raise ResourceError(f"model {name} curve {ci}: indices {off}..{off + cnt} run past the curve index "
+                                    f"buffer ({cidx.shape[0]})")
+            idx = np.ascontiguousarray(cidx[off:off + cnt], np.uint32)
+            if cnt and int(idx.max()) + 3 >= ncv:
+                raise ResourceError(f"model {name} curve {ci}: a segment's control points run past the {ncv} curve "
+                                    "vertices")
+            geoms.append({"curve_vertices": verts, "curves": idx})
+            gm.append(keys[key])
     if not mats:
         mats.append(material_record(default_material()))
     return {"geoms": geoms, "materials": np.stack(mats), "geom_material": np.array(gm, np.uint32)}

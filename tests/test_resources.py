@@ -367,15 +367,29 @@ def _variant(tmp_path, edit):
 
 def test_load_scene_refuses_what_the_device_does_not_render(tmp_path):
     def curves(files):
-        d, g = F.model_dict([("m", 1, [[0, 1, 2]])], [(0, 0, 0), (1, 0, 0), (0, 1, 0)], curves=[(0, 4)],
-                            curve_indices=[0, 1, 2, 3], curve_vertices=np.ones((4, 4)))
+        d, g = F.model_dict([("m", 1, [[0, 1, 2]])], [(0, 0, 0), (1, 0, 0), (0, 1, 0)], curves=[(0, 1)],
+                            curve_indices=[0], curve_vertices=np.ones((4, 4)))
         files[("model", "panel")], files[("geometry", "panel")] = d, g
 
+    # curves load (InitializeCurves: after the meshes, the material of curveModelName, no texture)
     root = _variant(tmp_path / "curves", curves)
-    with pytest.raises(ValueError, match="curve"):
+    ls = R.load_scene(root, "rooms", ptex_loader=_fixture_ptex)
+    assert not ls.dropped
+    panel = [m for m in ls.description["models"] if any("curves" in g for g in m["geoms"])]
+    assert len(panel) == 1 and "triangles" in panel[0]["geoms"][0]
+    cg = panel[0]["geoms"][1]
+    np.testing.assert_array_equal(cg["curves"], [0])
+    assert cg["curve_vertices"].shape == (4, 4)
+
+    def bad_curve(files):
+        # two segments over four control points: the second (from index 1) needs a fifth
+        d, g = F.model_dict([("m", 1, [[0, 1, 2]])], [(0, 0, 0), (1, 0, 0), (0, 1, 0)], curves=[(0, 2)],
+                            curve_indices=[0, 1], curve_vertices=np.ones((4, 4)))
+        files[("model", "panel")], files[("geometry", "panel")] = d, g
+
+    root = _variant(tmp_path / "badcurve", bad_curve)
+    with pytest.raises(R.ResourceError, match="curve vertices"):
         R.load_scene(root, "rooms", ptex_loader=_fixture_ptex)
-    ls = R.load_scene(root, "rooms", ptex_loader=_fixture_ptex, strict=False)
-    assert any("curve" in x for x in ls.dropped)
     # no texture loader: the floor's texture is refused, or dropped with its flag cleared
     with pytest.raises(ValueError, match="per-face texture"):
         R.load_scene(F.FIXTURE_ROOT, "rooms")

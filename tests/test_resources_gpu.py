@@ -79,3 +79,42 @@ def test_loaded_scene_hits_match_oracle(gpu_ctx):
         np.testing.assert_array_equal(occ.download(np.uint32, n), ref["occluded"])
     finally:
         scene.release()
+
+
+def test_loaded_scene_with_curves_renders_match_oracle(gpu_ctx, tmp_path):
+    """The fixture scene with hair on its panel model (ModelResource curves: InitializeCurves, the
+    curveModelName material): loaded, committed and rendered by the path tracer and VCM (with the
+    scene's image-based light, whose emission sphere takes in the curves) per pixel against the
+    oracle's instanced scene."""
+    from test_curves_gpu import _close as _close_curves
+    from test_curves_gpu import _hair
+
+    files, _ = F.fixture_resources()
+    tp = np.array([(-1, 0, 0), (1, 0, 0), (1, 1.5, 0), (-1, 1.5, 0), (0, 2.2, 0)], np.float32)
+    v, idx = _hair(21, 60, 8, (0.0, 1.5, 0.0), 0.8, 0.02, step=0.08)
+    brass = R.murmur3_32("brass")
+    old = files[("model", "panel")]
+    d, g = F.model_dict([("panel_main", brass, [[0, 1, 2], [0, 2, 3]]), ("panel_top", R.murmur3_32("nothing"),
+                                                                         [[3, 2, 4]])], tp,
+                        list(old["materials"]), list(old["material_hashes"]), curves=[(0, len(idx))],
+                        curve_indices=idx, curve_vertices=v)
+    files[("model", "panel")], files[("geometry", "panel")] = d, g
+    F.write_tree(str(tmp_path), files, F.py_write)
+    ls = R.load_scene(str(tmp_path), "rooms", camera_index=1, ptex_loader=lambda path: F.floor_ptex())
+    sc = ls.description
+    assert any("curves" in gg for m in sc["models"] for gg in m["geoms"])
+    scene = N.build_render_scene(gpu_ctx, sc)
+    try:
+        rs = O.InstancedRenderScene(sc)
+        print("resources with curves: path tracer", flush=True)
+        gi = N.render_pt(scene, ls.camera, 64, 48, 4, 8, 3, N.PT_NEE_COSINE_ONCE)
+        c = O.render_pt(rs, ls.camera, 64, 48, 4, 8, 3, 1)
+        frac, _ = _close(gi, c)
+        assert c.mean() > 0.01 and frac >= 0.99, frac
+        print("resources with curves: VCM", flush=True)
+        gi = N.render_vcm(scene, ls.camera, 64, 48, 3, 8, 0.02, 0.75, 29)
+        c, _ = O.render_vcm(rs, ls.camera, 64, 48, 3, 8, 0.02, 0.75, 29)
+        frac, rel_rmse = _close_curves(gi, c)
+        assert c.mean() > 0.01 and frac >= 0.99 and rel_rmse <= 2e-3, (frac, rel_rmse)
+    finally:
+        scene.release()
