@@ -665,7 +665,8 @@ def bench_serve(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=("vcm", "pt", "traversal", "batch", "serve", "instances", "deferred"),
+    ap.add_argument("--workload", choices=("vcm", "pt", "traversal", "batch", "serve", "instances", "deferred",
+                                           "curves"),
                     default="vcm")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
@@ -740,6 +741,10 @@ def main():
         if args.steps is None:
             args.steps = 10
         bench_instances(args)
+    elif args.workload == "curves":
+        if args.steps is None:
+            args.steps = 10
+        bench_curves(args)
     else:
         if args.steps is None:
             args.steps = 10
@@ -973,6 +978,114 @@ def bench_traversal(args):
 def _rotation(rng):
     q, r = np.linalg.qr(rng.normal(size=(3, 3)))
     return q * np.sign(np.diag(r))
+
+
+def bench_curves(args):
+    """Ray casts against round B-spline curves: a field of 147,456 grass blades of 10 control points
+    (7 segments each, 1,032,192 segments; radius 3 mm tapering to 1 mm over blades 10-25 cm tall, on
+    a 2 m x 2 m ground of two triangles), rays from a camera above the field (1024 x 1024, 2 samples
+    per pixel) plus as many cosine rays up from the ground: closest hit (triangle pass, then the curve
+    pass) and any hit."""
+    from selas_amd import native as N
+    from selas_amd import workloads as W
+
+    log("building curve workload")
+    rng = np.random.default_rng(77)
+    ns, ncp = 147456, 10
+    root = np.stack([rng.uniform(-1, 1, ns), np.zeros(ns), rng.uniform(-1, 1, ns)], 1)
+    height = rng.uniform(0.10, 0.25, ns)
+    d = np.tile([0.0, 1.0, 0.0], (ns, 1)) + 0.3 * rng.normal(size=(ns, 3)) * [1, 0, 1]
+    pts = [root]
+    for k in range(ncp - 1):
+        d = d + 0.15 * rng.normal(size=(ns, 3)) * [1, 0.2, 1]
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        pts.append(pts[-1] + d * (height / (ncp - 1))[:, None])
+    pts = np.stack(pts, 1)
+    rad = np.linspace(0.003, 0.001, ncp)[None, :].repeat(ns, 0)
+    verts = np.concatenate([pts, rad[:, :, None]], 2).reshape(-1, 4).astype(np.float32)
+    idx = (np.arange(ns)[:, None] * ncp + np.arange(ncp - 3)[None, :]).reshape(-1).astype(np.uint32)
+    ground = np.array([[-1.2, 0, -1.2, 1.2, 0, -1.2, 1.2, 0, 1.2], [-1.2, 0, -1.2, 1.2, 0, 1.2, -1.2, 0, 1.2]],
+                      np.float32)
+    cam = N.gen_camera_rays((0.0, 1.2, -1.9), (0.0, 0.05, 0.0), (0.0, 1.0, 0.0), 50.0, 1024, 1024, 2)
+    nb = cam["org_x"].shape[0]
+    o = np.stack([rng.uniform(-1, 1, nb), np.full(nb, 1e-3), rng.uniform(-1, 1, nb)], 1)
+    u1, u2 = rng.random(nb), rng.random(nb)
+    r_, phi = np.sqrt(u1), 2 * np.pi * u2
+    dd = np.stack([r_ * np.cos(phi), np.sqrt(1 - u1), r_ * np.sin(phi)], 1)
+    bounce = {"org_x": o[:, 0], "org_y": o[:, 1], "org_z": o[:, 2], "dir_x": dd[:, 0], "dir_y": dd[:, 1],
+              "dir_z": dd[:, 2], "tnear": np.zeros(nb), "tfar": np.full(nb, 1e30)}
+    rays = {k: np.concatenate([cam[k], bounce[k]]).astype(np.float32) for k in N.RAY_FIELDS}
+    n = int(rays["org_x"].shape[0])
+
+    ctx = new_context(N, args)
+    scene = N.Scene(ctx)
+    scene.add_soup(ground)
+    tb = time.perf_counter()
+    scene.add_curves(verts, idx)
+    scene.commit()
+    build_ms = (time.perf_counter() - tb) * 1e3
+    dr = N.DeviceRays(ctx, rays)
+    dh = N.DeviceHits(ctx, n, with_ng=True)
+    scene.intersect(dr, dh)
+    h = dh.to_host()
+    curve_hit = float((h["geom_id"] == 1).mean())
+    tfar, _ = W.anyhit_tfar(h["t"], h["prim_id"])
+    rays_any = dict(rays)
+    rays_any["tfar"] = tfar
+    dra = N.DeviceRays(ctx, rays_any)
+    occ = ctx.empty(4 * n)
+    st = scene.intersect_stats(dr)
+    for _ in range(args.warmup):
+        scene.intersect(dr, dh)
+        scene.occluded(dra, occ)
+    N.synchronize()
+    ev = [(N.Event(), N.Event(), N.Event()) for _ in range(args.steps)]
+    for a, b, c in ev:
+        a.record(None)
+        scene.intersect(dr, dh)
+        b.record(None)
+        scene.occluded(dra, occ)
+        c.record(None)
+    N.synchronize()
+    scene.check()
+    mi = sum(a.elapsed_ms(b) for a, b, _ in ev) / args.steps
+    mo = sum(b.elapsed_ms(c) for _, b, c in ev) / args.steps
+    nodes, segs = st["curve_nodes_visited"] / n, st["curve_segments_tested"] / n
+    hit_rate = float((h["prim_id"] != N.INVALID_ID).mean())
+    # per ray: ray 32 B + hit distance 4 B read, nodes x 64 B, segments x 80 B, 32 B hit record written
+    bytes_per_ray = 36 + nodes * 64 + segs * 80 + 32 * curve_hit
+    achieved = n * bytes_per_ray / (mi / 1e3) / 1e9
+    out = {
+        "metric": "curve closest + any-hit Mrays/s",
+        "value": round(2 * n / ((mi + mo) / 1e3) / 1e6, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mi + mo, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "curves_grass_1Mseg_4Mray", "segments": int(idx.shape[0]), "strands": ns,
+                   "control_points": int(verts.shape[0]), "rays": n},
+        "closest_hit_mrays_s": round(n / (mi / 1e3) / 1e6, 2),
+        "any_hit_mrays_s": round(n / (mo / 1e3) / 1e6, 2),
+        "hit_rate": round(hit_rate, 4),
+        "curve_hit_rate": round(curve_hit, 4),
+        "curve_nodes_per_ray": round(nodes, 3),
+        "curve_segments_per_ray": round(segs, 3),
+        "build_ms": round(build_ms, 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "closest hit: k_trace8 over the ground, then k_curves<false,false,false>",
+                     "bytes_per_ray": round(bytes_per_ray, 1)},
+        "cpu_baseline": None,
+    }
+    occ.free()
+    scene.release()
+    print(json.dumps(out), flush=True)
 
 
 def bench_instances(args):

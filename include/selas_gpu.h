@@ -90,6 +90,8 @@ typedef struct sgpu_trace_stats {
     uint64_t tris_tested;    /* triangle tests */
     uint64_t bytes_node;     /* size of one node record */
     uint64_t bytes_tri;      /* size of one triangle record */
+    uint64_t curve_nodes_visited;    /* curve-tree nodes whose child boxes were tested (curve pass) */
+    uint64_t curve_segments_tested;  /* curve segment tests (curve pass) */
 } sgpu_trace_stats;
 
 /* ---- errors ----------------------------------------------------------------------------- */
@@ -200,7 +202,8 @@ int sgpu_scene_add_quads(sgpu_scene scene, const float* positions, uint32_t n_ve
  * (colmajor4x4) must be a similarity (rotation, uniform scale, translation): radii scale with it;
  * an instance placing a scene with curves must be one too. Queries on a scene with curves need
  * hits.t (the curve pass after the triangle traversal compares against it); sgpu_intersect_stats
- * counts the triangle traversal only. Curves count as one geometry id each call, like meshes. */
+ * counts the curve pass apart (curve_nodes_visited, curve_segments_tested; bounded by the rays' tfar
+ * when it is given no hits). Curves count as one geometry id each call, like meshes. */
 int sgpu_scene_add_curves(sgpu_scene scene, const float* vertices, uint32_t n_verts, const uint32_t* indices,
                           uint32_t n_segments, const float* colmajor4x4, uint32_t* geom_id_out);
 /* Per-vertex shading attributes of geometry geom_id (host arrays, copied; before commit): replaces

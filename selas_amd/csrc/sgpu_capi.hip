@@ -1230,8 +1230,8 @@ int sgpu_intersect_stats(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu
     if (!stats) return sgpu_fail(SGPU_E_INVALID, "sgpu_intersect_stats: NULL stats");
     uint64_t* d_stats = nullptr;
     uint32_t* d_occ = nullptr;
-    SGPU_HIP_TRY(hipMalloc((void**)&d_stats, 2 * sizeof(uint64_t)));
-    SGPU_HIP_TRY(hipMemset(d_stats, 0, 2 * sizeof(uint64_t)));
+    SGPU_HIP_TRY(hipMalloc((void**)&d_stats, 4 * sizeof(uint64_t)));
+    SGPU_HIP_TRY(hipMemset(d_stats, 0, 4 * sizeof(uint64_t)));
     if (any_hit) {
         hipError_t e = hipMalloc((void**)&d_occ, (size_t)(n ? n : 1) * sizeof(uint32_t));
         if (e != hipSuccess) {
@@ -1241,8 +1241,8 @@ int sgpu_intersect_stats(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu
     }
     sgpu_hits_soa none = {};
     rc = sgpu_trace_launch(scene, rays, any_hit ? &none : (hits ? hits : &none), d_occ, n, any_hit != 0, d_stats,
-                           nullptr);
-    uint64_t h[2] = {0, 0};
+                           nullptr, nullptr, nullptr, nullptr, nullptr, 0, d_stats + 2);
+    uint64_t h[4] = {0, 0, 0, 0};
     if (rc == SGPU_OK) {
         hipError_t e = hipMemcpy(h, d_stats, sizeof(h), hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = sgpu_fail(SGPU_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
@@ -1255,6 +1255,8 @@ int sgpu_intersect_stats(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu
     stats->tris_tested = h[1];
     stats->bytes_node = scene->ctx->trace_tree == 1 ? sizeof(WideNode) : sizeof(BvhNode);
     stats->bytes_tri = sizeof(TriRecord);
+    stats->curve_nodes_visited = h[2];
+    stats->curve_segments_tested = h[3];
     return SGPU_OK;
 }
 

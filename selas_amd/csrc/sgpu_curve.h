@@ -9,7 +9,8 @@
 // point X, curve_contact), v = 0 and the normal X - P(u) (the swept surface touches ball u there).
 // Only entries are hits: a ray starting inside a curve's tube does not hit that tube.
 //
-// Search: the segment in kPieces pieces, each bounded by a capsule around its chord whose radius is
+// Search: a capsule around the whole segment's chord first (rejects most segments whose box the ray
+// crosses); then the segment in kCurvePieces pieces, each bounded by a capsule around its chord whose radius is
 // the larger end radius plus h^2 / 8 times the second-difference bounds of position and radius (a
 // cubic's distance from its chord over a piece of length h). For each capsule the ray's line meets,
 // the parameter nearest the line over the piece (widened by half a piece each side) by golden-section
@@ -102,6 +103,20 @@ SGPU_HD float curve_ball(const CurveSeg& s, const CurveRay& ray, float u, float&
     return cl;
 }
 
+// Lower bound of the entry of the ray into a capsule around chord AB of radius R: the entry into its
+// bounding sphere (centre mid-chord, radius |AB| / 2 + R); false when the ray's line misses that sphere.
+SGPU_HD bool capsule_entry_bound(const CurveRay& ray, f3 A, f3 B, float R, float& t)
+{
+    const f3 c = (A + B) * 0.5f;
+    const float rs = 0.5f * length3(B - A) + R;
+    const f3 oc = ray.o - c;
+    const float tc = -dot3(oc, ray.d) * ray.inv_a;
+    const f3 perp = oc + ray.d * tc;
+    const float h = rs * rs - dot3(perp, perp);
+    t = tc - sqrtf(h > 0.0f ? h : 0.0f) * ray.inv_len;
+    return h >= 0.0f;
+}
+
 // golden-section search over [lo, hi] of the clearance (entry = false) or of the entry t on a run of
 // crossed balls (entry = true)
 SGPU_HD float curve_golden(const CurveSeg& s, const CurveRay& ray, float lo, float hi, bool entry)
@@ -159,6 +174,25 @@ SGPU_HD bool curve_intersect(const CurveSeg& s, f3 o, f3 d, float tnear, float t
     f3 A;
     float ra;
     curve_eval(s, 0.0f, A, ra);
+    {
+        // the whole segment first: its balls lie in the capsule around the chord P(0) P(1) whose radius
+        // is the larger end radius plus (position and radius second-difference bounds) / 8
+        f3 B;
+        float rb;
+        curve_eval(s, 1.0f, B, rb);
+        const float R = fmaxf(ra, rb) + 0.125f * (m2 + m2r);
+        const f3 e = B - A, w0 = o - A;
+        const float b = dot3(d, e), c = dot3(e, e), dd = dot3(d, w0), ee = dot3(e, w0);
+        const float den = a * c - b * b;
+        float sc = den > 0.0f ? (a * ee - b * dd) / den : 0.0f;
+        sc = fminf(fmaxf(sc, 0.0f), 1.0f);
+        const float tc = (sc * b - dd) * ray.inv_a;
+        const f3 x = (o + d * tc) - (A + e * sc);
+        const float reach = (R + sqrtf(c)) * ray.inv_len;
+        if (!(dot3(x, x) <= R * R * 1.0001f && tc + reach >= tnear && tc - reach <= best)) return false;
+        float tl;
+        if (!capsule_entry_bound(ray, A, B, R * 1.0001f, tl) || tl > best) return false;
+    }
     for (int k = 0; k < kCurvePieces; ++k) {
         f3 B;
         float rb;
@@ -173,7 +207,9 @@ SGPU_HD bool curve_intersect(const CurveSeg& s, f3 o, f3 d, float tnear, float t
         const float tc = (sc * b - dd) * ray.inv_a;
         const f3 x = (o + d * tc) - (A + e * sc);
         const float reach = (R + sqrtf(c)) * ray.inv_len;
-        if (dot3(x, x) <= R * R * 1.0001f && tc + reach >= tnear && tc - reach <= best) {
+        float tl;
+        if (dot3(x, x) <= R * R * 1.0001f && tc + reach >= tnear && tc - reach <= best &&
+            capsule_entry_bound(ray, A, B, R * 1.0001f, tl) && tl <= best) {
             const float lo = fmaxf((float)k * h - 0.5f * h, 0.0f);
             const float hi = fminf((float)(k + 1) * h + 0.5f * h, 1.0f);
             const float ug = curve_golden(s, ray, lo, hi, false);

@@ -31,7 +31,7 @@ __device__ __forceinline__ float safe_inv(float x)
     return 1.0f / (fabsf(x) > 1e-30f ? x : copysignf(1e-30f, x));
 }
 
-template <bool ANY, bool MULTI>
+template <bool ANY, bool MULTI, bool STATS>
 __global__ __launch_bounds__(kCurveBlock) void k_curves(const BvhNode* __restrict nodes,
                                                         const CurveRecord* __restrict curves, uint32_t n_curves,
                                                         const InstRecord* __restrict insts,
@@ -40,9 +40,11 @@ __global__ __launch_bounds__(kCurveBlock) void k_curves(const BvhNode* __restric
                                                         const uint32_t* __restrict ray_scene, sgpu_rays_soa rays,
                                                         sgpu_hits_soa hits, uint32_t* __restrict occluded, uint32_t n,
                                                         const uint32_t* __restrict n_ptr,
-                                                        uint32_t* __restrict comp_out, uint32_t* __restrict overflow)
+                                                        uint32_t* __restrict comp_out, uint32_t* __restrict overflow,
+                                                        unsigned long long* __restrict stats)
 {
     const uint32_t count = n_ptr ? min(*n_ptr, n) : n;
+    unsigned long long cnt_nodes = 0, cnt_segs = 0;
     for (uint32_t i = blockIdx.x * kCurveBlock + threadIdx.x; i < count; i += gridDim.x * kCurveBlock) {
         const BvhNode* nd_base = nodes;
         const CurveRecord* recs = curves;
@@ -71,6 +73,7 @@ __global__ __launch_bounds__(kCurveBlock) void k_curves(const BvhNode* __restric
         uint32_t sp = 0;
         int node = 0;
         auto test_leaf = [&](int leaf) {
+            if (STATS) ++cnt_segs;
             const CurveRecord r = recs[leaf];
             sgpu::CurveSeg s;
             s.p[0] = r.p[0];
@@ -88,6 +91,7 @@ __global__ __launch_bounds__(kCurveBlock) void k_curves(const BvhNode* __restric
             }
         };
         while (true) {
+            if (STATS) ++cnt_nodes;
             const BvhNode nd = nd_base[node];
             float t0, t1;
             bool h0 = child_box(nd, 0, o, inv, tnear, best, t0);
@@ -156,6 +160,10 @@ __global__ __launch_bounds__(kCurveBlock) void k_curves(const BvhNode* __restric
         if (hits.inst_id1) hits.inst_id1[i] = id1;
         if (comp_out) comp_out[i] = r.inst;
     }
+    if (STATS) {
+        atomicAdd(&stats[0], cnt_nodes);
+        atomicAdd(&stats[1], cnt_segs);
+    }
 }
 
 }  // namespace
@@ -163,7 +171,8 @@ __global__ __launch_bounds__(kCurveBlock) void k_curves(const BvhNode* __restric
 int sgpu_curve_launch(const BvhNode* nodes, const CurveRecord* curves, uint32_t n_curves, const InstRecord* insts,
                       const uint32_t* inst_ids, const TraceScene* table, const uint32_t* ray_scene,
                       const sgpu_rays_soa* rays, const sgpu_hits_soa* hits, uint32_t* occluded, uint32_t n,
-                      bool any_hit, const uint32_t* n_ptr, uint32_t* comp_out, uint32_t* overflow, hipStream_t stream)
+                      bool any_hit, const uint32_t* n_ptr, uint32_t* comp_out, uint32_t* overflow, hipStream_t stream,
+                      uint64_t* stats)
 {
     if (n == 0) return SGPU_OK;
     sgpu_hits_soa h = {};
@@ -171,15 +180,19 @@ int sgpu_curve_launch(const BvhNode* nodes, const CurveRecord* curves, uint32_t 
     const uint32_t blocks = std::min<uint32_t>((n + kCurveBlock - 1) / kCurveBlock, kCurveMaxBlocks);
     const dim3 g(blocks), b(kCurveBlock);
     const bool multi = table != nullptr;
-#define SGPU_LAUNCH_C(A, M)                                                                                       \
-    hipLaunchKernelGGL((k_curves<A, M>), g, b, 0, stream, nodes, curves, n_curves, insts, inst_ids, table,      \
-                       ray_scene, *rays, h, occluded, n, n_ptr, comp_out, overflow)
-    if (any_hit) {
-        if (multi) SGPU_LAUNCH_C(true, true);
-        else SGPU_LAUNCH_C(true, false);
+    unsigned long long* st = (unsigned long long*)stats;
+#define SGPU_LAUNCH_C(A, M, S)                                                                                    \
+    hipLaunchKernelGGL((k_curves<A, M, S>), g, b, 0, stream, nodes, curves, n_curves, insts, inst_ids, table,   \
+                       ray_scene, *rays, h, occluded, n, n_ptr, comp_out, overflow, st)
+    if (st) {  // counting: single-scene queries (sgpu_intersect_stats)
+        if (any_hit) SGPU_LAUNCH_C(true, false, true);
+        else SGPU_LAUNCH_C(false, false, true);
+    } else if (any_hit) {
+        if (multi) SGPU_LAUNCH_C(true, true, false);
+        else SGPU_LAUNCH_C(true, false, false);
     } else {
-        if (multi) SGPU_LAUNCH_C(false, true);
-        else SGPU_LAUNCH_C(false, false);
+        if (multi) SGPU_LAUNCH_C(false, true, false);
+        else SGPU_LAUNCH_C(false, false, false);
     }
 #undef SGPU_LAUNCH_C
     SGPU_HIP_TRY(hipGetLastError());

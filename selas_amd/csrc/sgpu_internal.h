@@ -342,7 +342,7 @@ int sgpu_bvh_build_instances(sgpu_scene scene, hipStream_t stream);
 int sgpu_trace_launch(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits,
                       uint32_t* occluded, uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream,
                       uint32_t* spill = nullptr, const uint32_t* n_ptr = nullptr, uint32_t* overflow = nullptr,
-                      uint32_t* comp_out = nullptr, uint32_t max_blocks = 0);
+                      uint32_t* comp_out = nullptr, uint32_t max_blocks = 0, uint64_t* curve_stats = nullptr);
 // Argument checks shared by the query entry points (sgpu_capi.hip).
 int sgpu_check_query(sgpu_scene scene, const sgpu_rays_soa* rays, uint32_t n, const char* who);
 // Looks up this device's no-allocation scratch (sgpu_noalloc.hip) with the context's device current.
@@ -383,7 +383,7 @@ int sgpu_curve_launch(const BvhNode* nodes, const CurveRecord* curves, uint32_t 
                       const uint32_t* inst_ids, const TraceScene* table, const uint32_t* ray_scene,
                       const sgpu_rays_soa* rays, const sgpu_hits_soa* hits, uint32_t* occluded, uint32_t n,
                       bool any_hit, const uint32_t* n_ptr, uint32_t* comp_out, uint32_t* overflow,
-                      hipStream_t stream);
+                      hipStream_t stream, uint64_t* stats = nullptr);
 // Builds the curve tree of a scene of geometries from its segments, or of an instanced scene from
 // the curves of its composed instances (sgpu_bvh.hip; called by the builds above).
 int sgpu_curve_build(sgpu_scene scene, const std::vector<float>& cp, const std::vector<uint32_t>& geom,

@@ -927,7 +927,8 @@ int trace_launch_instanced(sgpu_scene scene, const sgpu_rays_soa* rays, const sg
 
 int sgpu_trace_launch(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hits_soa* hits, uint32_t* occluded,
                       uint32_t n, bool any_hit, uint64_t* d_stats, hipStream_t stream, uint32_t* spill_buf,
-                      const uint32_t* n_ptr, uint32_t* overflow, uint32_t* comp_out, uint32_t max_blocks)
+                      const uint32_t* n_ptr, uint32_t* overflow, uint32_t* comp_out, uint32_t max_blocks,
+                      uint64_t* curve_stats)
 {
     int rc;
     if (!scene->instances.empty())
@@ -940,7 +941,7 @@ int sgpu_trace_launch(sgpu_scene scene, const sgpu_rays_soa* rays, const sgpu_hi
     if (rc != SGPU_OK || scene->n_curves == 0) return rc;
     return sgpu_curve_launch(scene->d_cnodes, scene->d_curves, scene->n_curves, scene->d_inst, scene->d_inst_ids,
                              nullptr, nullptr, rays, hits, occluded, n, any_hit, n_ptr, comp_out,
-                             overflow ? overflow : scene->d_overflow, stream);
+                             overflow ? overflow : scene->d_overflow, stream, curve_stats);
 }
 
 int sgpu_trace_launch_multi(sgpu_context ctx, const TraceScene* table, const uint32_t* ray_scene, uint32_t stack_cap,

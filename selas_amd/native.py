@@ -39,7 +39,8 @@ class HitsSoA(C.Structure):
 
 
 class TraceStats(C.Structure):
-    _fields_ = [(n, C.c_uint64) for n in ("rays", "nodes_visited", "tris_tested", "bytes_node", "bytes_tri")]
+    _fields_ = [(n, C.c_uint64) for n in ("rays", "nodes_visited", "tris_tested", "bytes_node", "bytes_tri",
+                                          "curve_nodes_visited", "curve_segments_tested")]
 
 
 _P = C.c_void_p
