@@ -25,7 +25,7 @@
 namespace sgpu {
 
 constexpr int kCurvePieces = 16;
-constexpr int kCurveIters = 24;
+constexpr int kCurveIters = 20;
 
 struct CurveSeg {
     float4 p[4];  // control points: xyz, radius
