@@ -398,3 +398,39 @@ def test_instanced_curve_render_matches_oracle(gpu_ctx):
         assert cpu.mean() > 0 and frac >= 0.99, frac
     finally:
         scene.release()
+
+
+def test_batched_jobs_with_curves_equal_jobs_alone(gpu_ctx):
+    """sgpu_render_vcm_batch over a scene with curves, a scene without and an instanced scene with
+    curves (one traversal launch over the scene table, then the curve pass over the table): every
+    image equals the same job rendered alone."""
+    sys.path.insert(0, HERE)
+    import scenes
+
+    hair_room = N.build_render_scene(gpu_ctx, _hair_room())
+    plain = N.build_render_scene(gpu_ctx, scenes.cube_scene(lights=True, ibl=False))
+    base = scenes.cube_scene(lights=True, ibl=False)
+    v, idx = _hair(14, 50, 8, (0.0, 0.0, 0.0), 0.3, 0.015, step=0.06)
+    inst = {"models": [{"geoms": base["geoms"], "materials": base["materials"], "geom_material": base["geom_material"]},
+                       {"geoms": [{"curve_vertices": v, "curves": idx}],
+                        "materials": W.material(0, (0.5, 0.35, 0.2), 0.0, 0.4, 1.55)[None],
+                        "geom_material": np.array([0], np.uint32)}],
+            "instances": [("model", 0, np.eye(4)), ("model", 1, _rot_m4(0.2, 0.0, 1.0, (0.0, 0.9, 0.8)))],
+            "lights": base["lights"], "background": base["background"]}
+    inst_scene = N.build_render_scene(gpu_ctx, inst)
+    try:
+        cam = base["camera"]
+        jobs = [{"scene": s, "camera": cam, "width": w, "height": h, "iterations": it, "max_path_length": 8,
+                 "base_radius": 0.02, "radius_alpha": 0.75, "seed": seed}
+                for s, w, h, it, seed in ((hair_room, 48, 40, 2, 3), (plain, 32, 24, 3, 5), (inst_scene, 40, 32, 2, 7))]
+        print("batched jobs with curves", flush=True)
+        batch = N.render_vcm_batch(jobs)
+        for img, j in zip(batch, jobs):
+            alone = N.render_vcm(j["scene"], j["camera"], j["width"], j["height"], j["iterations"],
+                                 j["max_path_length"], j["base_radius"], j["radius_alpha"], j["seed"])
+            assert np.isfinite(img).all() and alone.mean() > 0
+            assert (np.abs(img - alone) <= 1e-3 * np.abs(alone) + 1e-5).mean() >= 0.999
+    finally:
+        inst_scene.release()
+        plain.release()
+        hair_room.release()
